@@ -1,0 +1,1068 @@
+// bcube_oracle.cpp -- CPU restatement of Bcube.jl's Float64 assembly path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle for the HIP
+// assembly library: only tests, the smoke check and the CPU-baseline leg of
+// the benchmark may load it, and only to check or time the reference algorithm.
+// The product (bcube.jl_amd/) never links or calls it.
+//
+// Every routine restates one piece of the reference, in the reference's own
+// operation order where that order is observable (numbering, summation order of
+// duplicate COO entries).  Citations are relative to the Bcube.jl source tree.
+//
+// Pinned by (see tests/test_oracle.py):
+//   * dof numbering      : test/fespace/test_dofhandler.jl integer vectors
+//   * densify            : test/test_utils.jl
+//   * mesh numbering     : mesh_generator.jl formulas + dofhandler fixtures
+//   * quadrature         : test/quadrature/test_quadrature.jl weight sums,
+//                          numpy.polynomial.legendre.leggauss, exactness
+//   * shape functions    : test/interpolation/test_lagrange.jl values/gradients
+//   * element integrals  : test/integration/test_integration.jl closed forms
+//   * assembly           : test/assembler/test_assembler.jl known answers and
+//                          the 3D Q2 heat solve (analytic solution to 2e-14)
+// Parity unpinned: bitwise rounding of ForwardDiff/StaticArrays (irrelevant at
+// the 1e-12 tolerance the assembly is checked to).
+//
+// Build: oracle/Makefile (g++ -O2 -ffp-contract=off, no FMA contraction so the
+// left-to-right Julia evaluation order is kept).
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <set>
+#include <vector>
+#include <algorithm>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+
+typedef int64_t i64;
+
+// ---------------------------------------------------------------------------
+// Entity types (src/mesh/entity.jl).  Node lists are 0-based here.
+// ---------------------------------------------------------------------------
+enum CellType { QUAD4 = 0, QUAD9 = 1, TRI3 = 2, HEXA8 = 3, HEXA27 = 4 };
+enum Shape { SQUARE = 0, TRIANGLE = 1, CUBE = 2 };
+
+static Shape shape_of(int ct) {  // shape.jl:25-32
+  switch (ct) {
+    case QUAD4: case QUAD9: return SQUARE;
+    case TRI3: return TRIANGLE;
+    case HEXA8: case HEXA27: return CUBE;
+  }
+  throw std::runtime_error("unknown cell type");
+}
+static int topodim_of(int ct) { return shape_of(ct) == CUBE ? 3 : 2; }
+static int nnodes_of(int ct) {
+  switch (ct) { case QUAD4: return 4; case QUAD9: return 9; case TRI3: return 3;
+                case HEXA8: return 8; case HEXA27: return 27; }
+  return 0;
+}
+
+typedef std::vector<std::vector<int>> Lists;
+
+// edges2nodes (entity.jl:153-158, 203-232, 255-277, 347-362)
+static Lists edges2nodes(int ct) {
+  switch (ct) {
+    case QUAD4: return {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+    case QUAD9: return {{0, 1, 4}, {1, 2, 5}, {2, 3, 6}, {3, 0, 7}};
+    case TRI3: return {{0, 1}, {1, 2}, {2, 0}};
+    case HEXA8: return {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5},
+                        {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}};
+    case HEXA27: return {{0, 1, 8}, {1, 2, 9}, {2, 3, 10}, {3, 0, 11},
+                         {0, 4, 12}, {1, 5, 13}, {2, 6, 14}, {3, 7, 15},
+                         {4, 5, 16}, {5, 6, 17}, {6, 7, 18}, {7, 4, 19}};
+  }
+  return {};
+}
+// faces2nodes: 2D cells use their edges (entity.jl:208), hexa entity.jl:293-296,378-388
+static Lists faces2nodes(int ct) {
+  switch (ct) {
+    case QUAD4: case QUAD9: case TRI3: return edges2nodes(ct);
+    case HEXA8: return {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                        {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}};
+    case HEXA27: return {{0, 3, 2, 1, 11, 10, 9, 8, 20}, {0, 1, 5, 4, 8, 13, 16, 12, 21},
+                         {1, 2, 6, 5, 9, 14, 17, 13, 22}, {2, 3, 7, 6, 10, 15, 18, 14, 23},
+                         {0, 4, 7, 3, 12, 19, 15, 11, 24}, {4, 5, 6, 7, 16, 17, 18, 19, 25}};
+  }
+  return {};
+}
+
+struct Mesh {
+  int dim = 0;                 // space dimension
+  std::vector<double> coords;  // nnodes * dim
+  std::vector<int> ctype;      // per cell
+  std::vector<i64> c2n_ptr;    // ncells + 1
+  std::vector<i64> c2n;        // 0-based node ids
+  i64 ncells() const { return (i64)ctype.size(); }
+  std::vector<i64> cell(i64 c) const {
+    return std::vector<i64>(c2n.begin() + c2n_ptr[c], c2n.begin() + c2n_ptr[c + 1]);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Mesh generators (src/mesh/mesh_generator.jl)
+// ---------------------------------------------------------------------------
+
+// _rectangle_quad_mesh, order 1 (mesh_generator.jl:253-332)
+static void rectangle_mesh_q1(int nx, int ny, double xmin, double xmax, double ymin,
+                              double ymax, double* coords, i64* c2n) {
+  double P00[2] = {xmin, ymin}, P10[2] = {xmax, ymin}, P11[2] = {xmax, ymax},
+         P01[2] = {xmin, ymax};
+  for (int iy = 1; iy <= ny; ++iy)
+    for (int ix = 1; ix <= nx; ++ix) {
+      double u = (double)(ix - 1) / (double)(nx - 1);
+      double v = (double)(iy - 1) / (double)(ny - 1);
+      i64 n = (i64)(iy - 1) * nx + ix - 1;
+      for (int d = 0; d < 2; ++d) {
+        double a = (1 - u) * (1 - v) * P00[d];
+        double b = u * (1 - v) * P10[d];
+        double c = (1 - u) * v * P01[d];
+        double e = u * v * P11[d];
+        coords[2 * n + d] = ((a + b) + c) + e;
+      }
+    }
+  for (int iy = 1; iy < ny; ++iy)
+    for (int ix = 1; ix < nx; ++ix) {
+      i64 e = (i64)(iy - 1) * (nx - 1) + ix - 1;
+      c2n[4 * e + 0] = (i64)(iy - 1) * nx + ix - 1;
+      c2n[4 * e + 1] = (i64)(iy - 1) * nx + ix;
+      c2n[4 * e + 2] = (i64)iy * nx + ix;
+      c2n[4 * e + 3] = (i64)iy * nx + ix - 1;
+    }
+}
+
+// _rectangle_quad_mesh, order 2 (mesh_generator.jl:336-434)
+static void rectangle_mesh_q2(int nx, int ny, double xmin, double xmax, double ymin,
+                              double ymax, double* coords, i64* c2n) {
+  double P00[2] = {xmin, ymin}, P10[2] = {xmax, ymin}, P11[2] = {xmax, ymax},
+         P01[2] = {xmin, ymax};
+  int NX = nx + (nx - 1), NY = ny + (ny - 1);
+  for (int iy = 1; iy <= NY; ++iy)
+    for (int ix = 1; ix <= NX; ++ix) {
+      double u = 0.5 * (ix - 1) / (double)(nx - 1);
+      double v = 0.5 * (iy - 1) / (double)(ny - 1);
+      i64 n = (i64)(iy - 1) * NX + ix - 1;
+      for (int d = 0; d < 2; ++d) {
+        double a = (1 - u) * (1 - v) * P00[d];
+        double b = u * (1 - v) * P10[d];
+        double c = (1 - u) * v * P01[d];
+        double e = u * v * P11[d];
+        coords[2 * n + d] = ((a + b) + c) + e;
+      }
+    }
+  for (int j = 1; j < ny; ++j)
+    for (int i = 1; i < nx; ++i) {
+      i64 e = (i64)(j - 1) * (nx - 1) + i - 1;
+      i64 ix = (i64)(i - 1) * 2 + 1, iy = (i64)(j - 1) * 2 + 1;
+      auto N = [&](i64 dy, i64 dx) { return (iy + dy - 1) * NX + ix + dx - 1; };
+      i64* c = c2n + 9 * e;
+      c[0] = N(0, 0); c[1] = N(0, 2); c[2] = N(2, 2); c[3] = N(2, 0);
+      c[4] = N(0, 1); c[5] = N(1, 2); c[6] = N(2, 1); c[7] = N(1, 0); c[8] = N(1, 1);
+    }
+}
+
+// hexa_mesh (mesh_generator.jl:453-552)
+static void hexa_mesh_q1(int nx, int ny, int nz, double xmin, double xmax, double ymin,
+                         double ymax, double zmin, double zmax, double* coords, i64* c2n) {
+  double dx = (xmax - xmin) / (nx - 1), dy = (ymax - ymin) / (ny - 1),
+         dz = (zmax - zmin) / (nz - 1);
+  for (int iz = 1; iz <= nz; ++iz)
+    for (int iy = 1; iy <= ny; ++iy)
+      for (int ix = 1; ix <= nx; ++ix) {
+        i64 n = (i64)(iz - 1) * nx * ny + (i64)(iy - 1) * nx + ix - 1;
+        coords[3 * n + 0] = xmin + (ix - 1) * dx;
+        coords[3 * n + 1] = ymin + (iy - 1) * dy;
+        coords[3 * n + 2] = zmin + (iz - 1) * dz;
+      }
+  i64 NXY = (i64)nx * ny;
+  for (int iz = 1; iz < nz; ++iz)
+    for (int iy = 1; iy < ny; ++iy)
+      for (int ix = 1; ix < nx; ++ix) {
+        i64 e = (i64)(iz - 1) * (nx - 1) * (ny - 1) + (i64)(iy - 1) * (nx - 1) + ix - 1;
+        auto N = [&](int a, int b, int c) {
+          return (i64)(iz - 1 + c) * NXY + (i64)(iy - 1 + b) * nx + ix - 1 + a;
+        };
+        i64* c = c2n + 8 * e;
+        c[0] = N(0, 0, 0); c[1] = N(1, 0, 0); c[2] = N(1, 1, 0); c[3] = N(0, 1, 0);
+        c[4] = N(0, 0, 1); c[5] = N(1, 0, 1); c[6] = N(1, 1, 1); c[7] = N(0, 1, 1);
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Quadrature (src/quadrature/quadrature.jl)
+// ---------------------------------------------------------------------------
+enum QuadType { Q_LEGENDRE = 0, Q_UNIFORM = 1 };
+
+// Gauss-Legendre, n points on [-1,1], ascending nodes (FastGaussQuadrature
+// gausslegendre, quadrature.jl:203-208).  Newton iteration on P_n in long
+// double; the result is the correctly rounded double to within 1 ulp.
+static void gauss_legendre(int n, std::vector<double>& w, std::vector<double>& x) {
+  w.assign(n, 0.0);
+  x.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    long double z = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) / (n + 0.5L));
+    long double pp = 0;
+    for (int it = 0; it < 100; ++it) {
+      long double p1 = 1, p2 = 0;
+      for (int j = 1; j <= n; ++j) {
+        long double p3 = p2;
+        p2 = p1;
+        p1 = ((2 * j - 1) * z * p2 - (j - 1) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1);
+      long double z1 = z;
+      z = z1 - p1 / pp;
+      if (fabsl(z - z1) < 1e-19L) break;
+    }
+    {
+      long double p1 = 1, p2 = 0;
+      for (int j = 1; j <= n; ++j) {
+        long double p3 = p2;
+        p2 = p1;
+        p1 = ((2 * j - 1) * z * p2 - (j - 1) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1);
+    }
+    x[n - 1 - i] = (double)z;  // ascending
+    w[n - 1 - i] = (double)(2.0L / ((1 - z * z) * pp * pp));
+  }
+  if (n % 2 == 1) x[n / 2] = 0.0;
+  for (int i = 0; i < n / 2; ++i) {  // exact symmetry
+    x[i] = -x[n - 1 - i];
+    w[i] = w[n - 1 - i];
+  }
+}
+
+// _uniformpoints (quadrature.jl:222-232): LinRange(-1, 1, N), weights 1/N.
+// LinRange element j (0-based) = (1 - t) * a + t * b with t = j / (N - 1).
+static void uniform_points(int n, std::vector<double>& w, std::vector<double>& x) {
+  w.assign(n, 1.0 / n);
+  x.assign(n, 0.0);
+  if (n == 1) { x[0] = 0.0; return; }
+  for (int j = 0; j < n; ++j) {
+    double t = (double)j / (double)(n - 1);
+    x[j] = (1 - t) * (-1.0) + t * 1.0;
+  }
+}
+
+// number of 1D nodes (quadrature.jl:238-257)
+static int line_nodes(int degree, int qtype) {
+  if (qtype == Q_UNIFORM) return degree + 1;
+  return (degree + 1 + 1) / 2;  // ceil((degree+1)/2)
+}
+
+static void line_rule(int degree, int qtype, std::vector<double>& w, std::vector<double>& x) {
+  int n = line_nodes(degree, qtype);
+  if (qtype == Q_UNIFORM) uniform_points(n, w, x);
+  else gauss_legendre(n, w, x);
+}
+
+// Square / Cube tensor rules, first index fastest (quadrature.jl:455-487).
+// A degree-0 quadrature integrates with the degree-1 rule (quadrature.jl:130-135).
+static int tensor_rule(int shape, int degree, int qtype, std::vector<double>& w,
+                       std::vector<double>& x) {
+  if (degree == 0) degree = 1;
+  std::vector<double> w1, x1;
+  line_rule(degree, qtype, w1, x1);
+  int n = (int)w1.size();
+  w.clear(); x.clear();
+  if (shape == SQUARE) {
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        w.push_back(w1[i] * w1[j]);
+        x.push_back(x1[i]); x.push_back(x1[j]);
+      }
+  } else if (shape == CUBE) {
+    for (int k = 0; k < n; ++k)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          w.push_back((w1[i] * w1[j]) * w1[k]);
+          x.push_back(x1[i]); x.push_back(x1[j]); x.push_back(x1[k]);
+        }
+  } else {
+    throw std::runtime_error("oracle: triangle quadrature not restated");
+  }
+  return (int)w.size();
+}
+
+// ---------------------------------------------------------------------------
+// Dual numbers: ForwardDiff's arithmetic, used to restate its derivatives
+// (function_space.jl:161-165, mapping.jl:166-168).
+// ---------------------------------------------------------------------------
+template <int N> struct Dual {
+  double v; double d[N];
+  Dual() : v(0) { for (int i = 0; i < N; ++i) d[i] = 0; }
+  explicit Dual(double x) : v(x) { for (int i = 0; i < N; ++i) d[i] = 0; }
+};
+template <int N> static Dual<N> operator+(const Dual<N>& a, const Dual<N>& b) {
+  Dual<N> r; r.v = a.v + b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i]; return r; }
+template <int N> static Dual<N> operator-(const Dual<N>& a, const Dual<N>& b) {
+  Dual<N> r; r.v = a.v - b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i]; return r; }
+template <int N> static Dual<N> operator-(const Dual<N>& a, double b) {
+  Dual<N> r = a; r.v = a.v - b; return r; }
+template <int N> static Dual<N> operator+(const Dual<N>& a, double b) {
+  Dual<N> r = a; r.v = a.v + b; return r; }
+template <int N> static Dual<N> operator-(double b, const Dual<N>& a) {
+  Dual<N> r; r.v = b - a.v; for (int i = 0; i < N; ++i) r.d[i] = -a.d[i]; return r; }
+template <int N> static Dual<N> operator+(double b, const Dual<N>& a) {
+  Dual<N> r = a; r.v = b + a.v; return r; }
+template <int N> static Dual<N> operator-(const Dual<N>& a) {
+  Dual<N> r; r.v = -a.v; for (int i = 0; i < N; ++i) r.d[i] = -a.d[i]; return r; }
+// ForwardDiff: x*y = Dual(x.v*y.v, x.d*y.v + y.d*x.v)
+template <int N> static Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) {
+  Dual<N> r; r.v = a.v * b.v;
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + b.d[i] * a.v;
+  return r; }
+template <int N> static Dual<N> operator*(double s, const Dual<N>& a) {
+  Dual<N> r; r.v = s * a.v; for (int i = 0; i < N; ++i) r.d[i] = s * a.d[i]; return r; }
+template <int N> static Dual<N> operator*(const Dual<N>& a, double s) {
+  Dual<N> r; r.v = a.v * s; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * s; return r; }
+template <int N> static Dual<N> operator/(const Dual<N>& a, double s) {
+  Dual<N> r; r.v = a.v / s; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] / s; return r; }
+
+// ---------------------------------------------------------------------------
+// Lagrange bases (src/function_space/lagrange/hypercube.jl, triangle.jl)
+// ---------------------------------------------------------------------------
+
+// 1D Lagrange nodes: uniform points of `order` (lagrange.jl:16, hypercube.jl:21-27)
+static std::vector<double> lagrange_nodes(int order) {
+  std::vector<double> w, x;
+  uniform_points(order + 1, w, x);
+  return x;
+}
+
+// _lagrange_poly (hypercube.jl:10-19): coef * foldl(*, ξ - n_k for k≠j),
+// coef = 1.0 / prod(n_j - n_k for k≠j).
+template <class T> static T lagrange_poly(int j, const T& xi, const std::vector<double>& nodes) {
+  double den = 1.0;
+  bool first = true;
+  for (size_t k = 0; k < nodes.size(); ++k) {
+    if ((int)k == j) continue;
+    double f = nodes[j] - nodes[k];
+    den = first ? f : den * f;
+    first = false;
+  }
+  double coef = 1.0 / den;
+  if (nodes.size() == 1) return T(1.0) * coef;  // no terms: prod_expr = 1.0
+  T prod;
+  bool init = false;
+  for (size_t k = 0; k < nodes.size(); ++k) {
+    if ((int)k == j) continue;
+    T term = xi - nodes[k];
+    prod = init ? prod * term : term;
+    init = true;
+  }
+  return coef * prod;
+}
+static double lagrange_poly_d(int j, double xi, const std::vector<double>& nodes) {
+  return lagrange_poly<double>(j, xi, nodes);
+}
+
+static int ndofs_lagrange(int shape, int order) {  // hypercube.jl:122-133, triangle.jl:67-70
+  if (shape == TRIANGLE) {
+    static const int t[] = {1, 3, 6, 10};
+    if (order > 3) throw std::runtime_error("oracle: triangle order > 3 has no reference basis");
+    return t[order];
+  }
+  int n = order + 1;
+  return shape == SQUARE ? n * n : n * n * n;
+}
+
+// Values and reference gradients of the tensor Lagrange basis at ξ.
+// shape_functions_symbolic (hypercube.jl:56-92): value[i + n*j (+ n²*k)] =
+// l1[i]*l2[j] (*l3[k]), first index fastest.  The gradient restates
+// ForwardDiff.jacobian of that expression: ∂1 = (l1'*l2)*l3,
+// ∂2 = (l2'*l1)*l3, ∂3 = l3'*(l1*l2).
+static void tensor_basis(int shape, int order, const double* xi, double* val, double* grad) {
+  int d = shape == SQUARE ? 2 : 3;
+  if (order == 0) {
+    val[0] = 1.0;
+    for (int a = 0; a < d; ++a) grad[a] = 0.0;
+    return;
+  }
+  std::vector<double> nodes = lagrange_nodes(order);
+  int n = order + 1;
+  std::vector<double> l[3], dl[3];
+  for (int a = 0; a < d; ++a) {
+    l[a].resize(n); dl[a].resize(n);
+    for (int j = 0; j < n; ++j) {
+      Dual<1> x(xi[a]); x.d[0] = 1.0;
+      Dual<1> p = lagrange_poly<Dual<1>>(j, x, nodes);
+      l[a][j] = p.v; dl[a][j] = p.d[0];
+    }
+  }
+  if (d == 2) {
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        int k = i + n * j;
+        val[k] = l[0][i] * l[1][j];
+        grad[2 * k + 0] = dl[0][i] * l[1][j];
+        grad[2 * k + 1] = dl[1][j] * l[0][i];
+      }
+  } else {
+    for (int kk = 0; kk < n; ++kk)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          int k = i + n * j + n * n * kk;
+          double l12 = l[0][i] * l[1][j];
+          val[k] = l12 * l[2][kk];
+          grad[3 * k + 0] = (dl[0][i] * l[1][j]) * l[2][kk];
+          grad[3 * k + 1] = (dl[1][j] * l[0][i]) * l[2][kk];
+          grad[3 * k + 2] = dl[2][kk] * l12;
+        }
+  }
+}
+
+// idof_by_vertex / idof_by_edge / idof_by_face (hypercube.jl:136-331,
+// triangle.jl:72-98).  0-based local dof indices.
+static Lists idof_by_vertex(int shape, int order) {
+  if (order == 0) return Lists(shape == CUBE ? 8 : (shape == SQUARE ? 4 : 3));
+  if (shape == TRIANGLE) return {{0}, {1}, {2}};
+  int n = order + 1;
+  auto I2 = [&](int i, int j) { return (i - 1) + n * (j - 1); };
+  auto I3 = [&](int i, int j, int k) { return (i - 1) + n * (j - 1) + n * n * (k - 1); };
+  if (shape == SQUARE) return {{I2(1, 1)}, {I2(n, 1)}, {I2(n, n)}, {I2(1, n)}};
+  return {{I3(1, 1, 1)}, {I3(n, 1, 1)}, {I3(n, n, 1)}, {I3(1, n, 1)},
+          {I3(1, 1, n)}, {I3(n, 1, n)}, {I3(n, n, n)}, {I3(1, n, n)}};
+}
+
+static Lists idof_by_edge(int shape, int order) {
+  if (shape == TRIANGLE) {
+    if (order <= 1) return Lists(3);
+    if (order == 2) return {{3}, {4}, {5}};
+    if (order == 3) return {{3, 4}, {5, 6}, {7, 8}};
+    throw std::runtime_error("oracle: triangle order > 3");
+  }
+  if (order == 0) return Lists(shape == CUBE ? 12 : 4);
+  int n = order + 1;
+  std::vector<int> r, rr;  // 2:n-1 and reverse
+  for (int i = 2; i <= n - 1; ++i) r.push_back(i);
+  rr.assign(r.rbegin(), r.rend());
+  if (shape == SQUARE) {
+    auto I = [&](int i, int j) { return (i - 1) + n * (j - 1); };
+    Lists e(4);
+    for (int i : r) e[0].push_back(I(i, 1));
+    for (int j : r) e[1].push_back(I(n, j));
+    for (int i : rr) e[2].push_back(I(i, n));
+    for (int j : rr) e[3].push_back(I(1, j));
+    return e;
+  }
+  auto I = [&](int i, int j, int k) { return (i - 1) + n * (j - 1) + n * n * (k - 1); };
+  Lists e(12);
+  for (int i : r) e[0].push_back(I(i, 1, 1));
+  for (int j : r) e[1].push_back(I(n, j, 1));
+  for (int i : rr) e[2].push_back(I(i, n, 1));
+  for (int j : rr) e[3].push_back(I(1, j, 1));
+  for (int k : r) e[4].push_back(I(1, 1, k));
+  for (int k : r) e[5].push_back(I(n, 1, k));
+  for (int k : r) e[6].push_back(I(n, n, k));
+  for (int k : r) e[7].push_back(I(1, n, k));
+  for (int i : r) e[8].push_back(I(i, 1, n));
+  for (int j : r) e[9].push_back(I(n, j, n));
+  for (int i : rr) e[10].push_back(I(i, n, n));
+  for (int j : rr) e[11].push_back(I(1, j, n));
+  return e;
+}
+
+// Cube faces in the order (f1..f6) of __idof_by_face; vec() is column-major.
+static Lists idof_by_face(int shape, int order) {
+  if (shape != CUBE) return Lists();
+  if (order == 0) return Lists(6);
+  int n = order + 1;
+  std::vector<int> r;
+  for (int i = 2; i <= n - 1; ++i) r.push_back(i);
+  auto I = [&](int i, int j, int k) { return (i - 1) + n * (j - 1) + n * n * (k - 1); };
+  Lists f(6);
+  for (int b : r) for (int a : r) f[0].push_back(I(a, b, 1));   // z-  (range1, range2, 1)
+  for (int b : r) for (int a : r) f[1].push_back(I(a, 1, b));   // y-  (range1, 1, range3)
+  for (int b : r) for (int a : r) f[2].push_back(I(n, a, b));   // x+  (n, range2, range3)
+  for (int b : r) for (int a : r) f[3].push_back(I(a, n, b));   // y+  (range1, n, range3)
+  for (int b : r) for (int a : r) f[4].push_back(I(1, a, b));   // x-  (1, range2, range3)
+  for (int b : r) for (int a : r) f[5].push_back(I(a, b, n));   // z+  (range1, range2, n)
+  return f;
+}
+
+// ---------------------------------------------------------------------------
+// oriented_cell_side (entity.jl:487-502)
+// ---------------------------------------------------------------------------
+static int oriented_cell_side(int ct, const std::vector<i64>& c2n, const std::vector<i64>& f2n) {
+  Lists fl = faces2nodes(ct);
+  for (size_t s = 0; s < fl.size(); ++s) {
+    bool subset = true;
+    for (int ln : fl[s]) {
+      if (std::find(f2n.begin(), f2n.end(), c2n[ln]) == f2n.end()) { subset = false; break; }
+    }
+    if (!subset) continue;
+    int side = (int)s + 1;
+    int n1 = 0;
+    for (size_t q = 0; q < fl[s].size(); ++q)
+      if (c2n[fl[s][q]] == f2n[0]) { n1 = (int)q + 1; break; }
+    if (f2n.size() == 2) return n1 == 2 ? -side : side;
+    int n2 = (n1 + 1 > (int)fl[s].size()) ? 1 : n1 + 1;
+    if (f2n[1] != c2n[fl[s][n2 - 1]]) side = -side;
+    return side;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// DofHandler (src/fespace/dofhandler.jl:37-388) + densify (src/utils.jl:7-16)
+// ---------------------------------------------------------------------------
+static std::vector<i64> densify(const std::vector<i64>& a) {
+  std::map<i64, i64> remap;
+  std::vector<i64> out(a.size());
+  for (size_t i = 0; i < a.size(); ++i) {
+    auto it = remap.find(a[i]);
+    if (it == remap.end()) { i64 v = (i64)remap.size() + 1; remap[a[i]] = v; out[i] = v; }
+    else out[i] = it->second;
+  }
+  return out;
+}
+
+// Returns ndofs_tot; iglob is 1-based global dof ids, offset 0-based positions.
+static i64 dofhandler(const Mesh& m, int order, bool continuous, std::vector<i64>& iglob,
+                      std::vector<i64>& offset, std::vector<int>& ndofs) {
+  i64 nc = m.ncells();
+  offset.assign(nc, 0);
+  ndofs.assign(nc, 0);
+  i64 tot = 0;
+  for (i64 c = 0; c < nc; ++c) {
+    ndofs[c] = ndofs_lagrange(shape_of(m.ctype[c]), order);
+    offset[c] = tot;
+    tot += ndofs[c];
+  }
+  iglob.resize(tot);
+  for (i64 i = 0; i < tot; ++i) iglob[i] = i + 1;
+  if (continuous) {
+    std::map<i64, std::vector<i64>> dict_n;
+    std::map<std::vector<i64>, std::pair<i64, std::vector<i64>>> dict_e, dict_f;
+    for (i64 c = 0; c < nc; ++c) {
+      int ct = m.ctype[c];
+      Shape s = shape_of(ct);
+      std::vector<i64> nodes = m.cell(c);
+      // vertices (dofhandler.jl:180-220); zip truncates to the shorter list
+      Lists vdofs = idof_by_vertex(s, order);
+      for (size_t v = 0; v < vdofs.size() && v < nodes.size(); ++v) {
+        if (vdofs[v].empty()) continue;
+        auto it = dict_n.find(nodes[v]);
+        if (it != dict_n.end()) {
+          for (size_t d = 0; d < it->second.size(); ++d) iglob[offset[c] + vdofs[v][d]] = it->second[d];
+        } else {
+          std::vector<i64> g;
+          for (int ld : vdofs[v]) g.push_back(iglob[offset[c] + ld]);
+          dict_n[nodes[v]] = g;
+        }
+      }
+      // edges (dofhandler.jl:239-290)
+      if (topodim_of(ct) > 1) {
+        Lists e2n = edges2nodes(ct);
+        Lists edofs = idof_by_edge(s, order);
+        for (size_t e = 0; e < e2n.size() && e < edofs.size(); ++e) {
+          if (edofs[e].empty()) continue;
+          std::vector<i64> eg;
+          for (int ln : e2n[e]) eg.push_back(nodes[ln]);
+          std::vector<i64> key = eg;
+          std::sort(key.begin(), key.end());
+          auto it = dict_e.find(key);
+          if (it != dict_e.end()) {
+            i64 jc = it->second.first;
+            const std::vector<i64>& jd = it->second.second;
+            int jside = oriented_cell_side(m.ctype[jc], m.cell(jc), eg);
+            std::vector<i64> re = jd;
+            if (!(jside > 0)) std::reverse(re.begin(), re.end());
+            for (size_t d = 0; d < jd.size(); ++d) iglob[offset[c] + edofs[e][d]] = re[d];
+          } else {
+            std::vector<i64> g;
+            for (int ld : edofs[e]) g.push_back(iglob[offset[c] + ld]);
+            dict_e[key] = {c, g};
+          }
+        }
+      }
+      // faces (dofhandler.jl:298-388)
+      if (topodim_of(ct) > 2) {
+        Lists f2n = faces2nodes(ct);
+        Lists fdofs = idof_by_face(s, order);
+        int ne = 12;
+        for (size_t f = 0; f < f2n.size() && f < fdofs.size(); ++f) {
+          if (fdofs[f].empty()) continue;
+          std::vector<i64> fg;
+          for (int ln : f2n[f]) fg.push_back(nodes[ln]);
+          std::vector<i64> key = fg;
+          std::sort(key.begin(), key.end());
+          auto it = dict_f.find(key);
+          if (it != dict_f.end()) {
+            const std::vector<i64>& jd = it->second.second;
+            int nd_by_edge = (int)jd.size() / ne;
+            if (nd_by_edge > 0)
+              throw std::runtime_error("oracle: face-dof permutation (order >= 4 hexa) not restated");
+            for (size_t d = 0; d < jd.size(); ++d) iglob[offset[c] + fdofs[f][d]] = jd[d];
+          } else {
+            std::vector<i64> g;
+            for (int ld : fdofs[f]) g.push_back(iglob[offset[c] + ld]);
+            dict_f[key] = {c, g};
+          }
+        }
+      }
+    }
+  }
+  iglob = densify(iglob);
+  std::set<i64> u(iglob.begin(), iglob.end());
+  return (i64)u.size();
+}
+
+// ---------------------------------------------------------------------------
+// Geometry (src/mapping/mapping.jl)
+// ---------------------------------------------------------------------------
+struct Geo {
+  double J[9];     // J[a*d+b] = ∂x_a/∂ξ_b
+  double Jinv[9];  // Jinv[a*d+b]
+  double detabs;
+  double x[3];     // physical point
+};
+
+// Quad4: mapping (460-465), mapping_jacobian_inv closed form (478-499),
+// mapping_det_jacobian closed form (501-518).
+static void geo_quad4(const double* X, const double* xi, Geo& g) {
+  double x1 = X[0], y1 = X[1], x2 = X[2], y2 = X[3], x3 = X[4], y3 = X[5], x4 = X[6], y4 = X[7];
+  double s = xi[0], t = xi[1];
+  double a = (1 - s) * (x4 - x1) + (1 + s) * (x3 - x2);
+  double b = (1 - t) * (y2 - y1) + (1 + t) * (y3 - y4);
+  double c = (1 - t) * (x2 - x1) + (1 + t) * (x3 - x4);
+  double e = (1 - s) * (y4 - y1) + (1 + s) * (y3 - y2);
+  double den = a * b - c * e;
+  double m00 = (1 - s) * (y1 - y4) + (1 + s) * (y2 - y3);
+  double m01 = (1 - s) * (x4 - x1) + (1 + s) * (x3 - x2);
+  double m10 = (1 - t) * (y2 - y1) + (1 + t) * (y3 - y4);
+  double m11 = (1 - t) * (x1 - x2) + (1 + t) * (x4 - x3);
+  g.Jinv[0] = 4 * m00 / den; g.Jinv[1] = 4 * m01 / den;
+  g.Jinv[2] = 4 * m10 / den; g.Jinv[3] = 4 * m11 / den;
+  g.detabs = fabs(-a * b + c * e) / 16.0;
+  for (int d = 0; d < 2; ++d) {
+    double p = (s - 1) * (t - 1) * X[0 + d] - (s + 1) * (t - 1) * X[2 + d] +
+               (s + 1) * (t + 1) * X[4 + d] - (s - 1) * (t + 1) * X[6 + d];
+    g.x[d] = p / 4;
+  }
+  // mapping_jacobian (467-476)
+  for (int d = 0; d < 2; ++d) {
+    double P1 = X[d], P2 = X[2 + d], P3 = X[4 + d], P4 = X[6 + d];
+    double k = P1 - P2 + P3 - P4;
+    g.J[d * 2 + 0] = (-P1 + P2 + P3 - P4 + t * k) / 4;
+    g.J[d * 2 + 1] = (-P1 - P2 + P3 + P4 + s * k) / 4;
+  }
+}
+
+// StaticArrays 3x3 det and inv (used by mapping_det_jacobian /
+// mapping_jacobian_inv defaults, mapping.jl:207-209, 275-286).
+static double det3(const double* A) {
+  return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+         A[2] * (A[3] * A[7] - A[4] * A[6]);
+}
+static void inv3(const double* A, double* R) {
+  // columns x0,x1,x2 of A (row-major storage A[r*3+c])
+  double x0[3] = {A[0], A[3], A[6]}, x1[3] = {A[1], A[4], A[7]}, x2[3] = {A[2], A[5], A[8]};
+  auto cross = [](const double* u, const double* v, double* w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+  };
+  double y0[3], y1[3], y2[3];
+  cross(x1, x2, y0);
+  double d = (x0[0] * y0[0] + x0[1] * y0[1]) + x0[2] * y0[2];
+  for (int i = 0; i < 3; ++i) { x0[i] /= d; y0[i] /= d; }
+  cross(x2, x0, y1);
+  cross(x0, x1, y2);
+  // inverse rows are y0, y1, y2
+  for (int c = 0; c < 3; ++c) { R[0 * 3 + c] = y0[c]; R[1 * 3 + c] = y1[c]; R[2 * 3 + c] = y2[c]; }
+}
+
+// Hexa8: mapping (618-639), mapping_jacobian (641-661); inverse/det via StaticArrays.
+static void geo_hexa8(const double* X, const double* xi, Geo& g) {
+  double s = xi[0], t = xi[1], u = xi[2];
+  const double* M[8];
+  for (int i = 0; i < 8; ++i) M[i] = X + 3 * i;
+  for (int d = 0; d < 3; ++d) {
+    double c0 = ((M[1][d] - M[0][d]) * (1 - t) + (M[2][d] - M[3][d]) * (1 + t)) * (1 - u) +
+                ((M[5][d] - M[4][d]) * (1 - t) + (M[6][d] - M[7][d]) * (1 + t)) * (1 + u);
+    double c1 = ((M[3][d] - M[0][d]) * (1 - s) + (M[2][d] - M[1][d]) * (1 + s)) * (1 - u) +
+                ((M[7][d] - M[4][d]) * (1 - s) + (M[6][d] - M[5][d]) * (1 + s)) * (1 + u);
+    double c2 = ((M[4][d] - M[0][d]) * (1 - s) + (M[5][d] - M[1][d]) * (1 + s)) * (1 - t) +
+                ((M[7][d] - M[3][d]) * (1 - s) + (M[6][d] - M[2][d]) * (1 + s)) * (1 + t);
+    g.J[d * 3 + 0] = c0 / 8.0;
+    g.J[d * 3 + 1] = c1 / 8.0;
+    g.J[d * 3 + 2] = c2 / 8.0;
+    double p = (1 - s) * (1 - t) * (1 - u) * M[0][d] + (1 + s) * (1 - t) * (1 - u) * M[1][d] +
+               (1 + s) * (1 + t) * (1 - u) * M[2][d] + (1 - s) * (1 + t) * (1 - u) * M[3][d] +
+               (1 - s) * (1 - t) * (1 + u) * M[4][d] + (1 + s) * (1 - t) * (1 + u) * M[5][d] +
+               (1 + s) * (1 + t) * (1 + u) * M[6][d] + (1 - s) * (1 + t) * (1 + u) * M[7][d];
+    g.x[d] = p / 8;
+  }
+  g.detabs = fabs(det3(g.J));
+  inv3(g.J, g.Jinv);
+}
+
+// Quad9 mapping (mapping.jl:540-568) differentiated with ForwardDiff; inverse
+// and det via the 2x2 StaticArrays formulas.
+template <class T> static void quad9_map(const double* X, const T* xi, T* out) {
+  const T &a = xi[0], &b = xi[1];
+  T l[9];
+  l[0] = a * b * (1.0 - a) * (1.0 - b) / 4.0;
+  l[1] = -a * b * (1.0 + a) * (1.0 - b) / 4.0;
+  l[2] = a * b * (1.0 + a) * (1.0 + b) / 4.0;
+  l[3] = -a * b * (1.0 - a) * (1.0 + b) / 4.0;
+  l[4] = -(1.0 + a) * (1.0 - a) * b * (1.0 - b) / 2.0;
+  l[5] = a * (1.0 + a) * (1.0 - b) * (1.0 + b) / 2.0;
+  l[6] = (1.0 - a) * (1.0 + a) * b * (1.0 + b) / 2.0;
+  l[7] = -a * (1.0 - a) * (1.0 - b) * (1.0 + b) / 2.0;
+  l[8] = (1.0 - a) * (1.0 + a) * (1.0 - b) * (1.0 + b);
+  for (int d = 0; d < 2; ++d) {
+    T s = l[0] * X[d];
+    for (int i = 1; i < 9; ++i) s = s + l[i] * X[2 * i + d];
+    out[d] = s;
+  }
+}
+static void geo_quad9(const double* X, const double* xi, Geo& g) {
+  Dual<2> x[2];
+  x[0] = Dual<2>(xi[0]); x[0].d[0] = 1;
+  x[1] = Dual<2>(xi[1]); x[1].d[1] = 1;
+  Dual<2> o[2];
+  quad9_map(X, x, o);
+  for (int a = 0; a < 2; ++a) {
+    g.x[a] = o[a].v;
+    for (int b = 0; b < 2; ++b) g.J[a * 2 + b] = o[a].d[b];
+  }
+  double det = g.J[0] * g.J[3] - g.J[1] * g.J[2];
+  g.detabs = fabs(det);
+  g.Jinv[0] = g.J[3] / det; g.Jinv[1] = -g.J[1] / det;
+  g.Jinv[2] = -g.J[2] / det; g.Jinv[3] = g.J[0] / det;
+}
+
+static void geometry(int ct, const double* X, const double* xi, Geo& g) {
+  switch (ct) {
+    case QUAD4: geo_quad4(X, xi, g); return;
+    case QUAD9: geo_quad9(X, xi, g); return;
+    case HEXA8: geo_hexa8(X, xi, g); return;
+  }
+  throw std::runtime_error("oracle: geometry not restated for this cell type");
+}
+
+// ---------------------------------------------------------------------------
+// Assembly (src/assembler/assembler.jl, src/integration/integration.jl)
+// ---------------------------------------------------------------------------
+enum Term { T_STIFF = 0, T_MASS = 1 };
+enum Source { F_ONE = 0, F_SINSIN = 1, F_X = 2, F_CONST = 3 };
+
+struct CSC {
+  i64 m = 0, n = 0;
+  std::vector<i64> colptr, rowval;  // 1-based like SparseMatrixCSC
+  std::vector<double> nzval;
+};
+
+// Cells in foreach_element order: subdomains by cell type in first-appearance
+// order, indices ascending inside each (domain.jl:39-50, 1096-1125).
+static std::vector<i64> element_order(const Mesh& m) {
+  std::vector<int> types;
+  for (int t : m.ctype)
+    if (std::find(types.begin(), types.end(), t) == types.end()) types.push_back(t);
+  std::vector<i64> order;
+  for (int t : types)
+    for (i64 c = 0; c < m.ncells(); ++c)
+      if (m.ctype[c] == t) order.push_back(c);
+  return order;
+}
+
+// SparseArrays.sparse(I, J, V, m, n): duplicates combined with + in COO order,
+// rows sorted within each column, explicit zeros kept.
+static void coo_to_csc(const std::vector<i64>& I, const std::vector<i64>& J,
+                       const std::vector<double>& V, i64 m, i64 n, CSC& A) {
+  i64 nz = (i64)I.size();
+  // CSR by row, preserving input order within a row
+  std::vector<i64> rptr(m + 2, 0);
+  for (i64 k = 0; k < nz; ++k) rptr[I[k] + 1]++;  // I is 1-based: row r -> rptr[r+1]
+  for (i64 r = 1; r <= m + 1; ++r) rptr[r] += rptr[r - 1];
+  std::vector<i64> cj(nz), fill(rptr.begin(), rptr.end());
+  std::vector<double> cv(nz);
+  for (i64 k = 0; k < nz; ++k) {
+    i64 p = fill[I[k]]++;
+    cj[p] = J[k]; cv[p] = V[k];
+  }
+  // combine duplicates per row in order (klasttouch)
+  std::vector<i64> last(n + 1, -1);
+  std::vector<i64> ur, uc;
+  std::vector<double> uv;
+  for (i64 r = 1; r <= m; ++r) {
+    i64 start = (i64)uc.size();
+    for (i64 p = rptr[r]; p < rptr[r + 1]; ++p) {
+      i64 c = cj[p];
+      if (last[c] >= start) uv[last[c]] = uv[last[c]] + cv[p];
+      else { last[c] = (i64)uc.size(); ur.push_back(r); uc.push_back(c); uv.push_back(cv[p]); }
+    }
+  }
+  // transpose to CSC; rows come out sorted because rows are visited in order
+  i64 nu = (i64)uc.size();
+  A.m = m; A.n = n;
+  // colptr[j-1] = 1 + number of entries in columns 1..j-1
+  std::vector<i64> cnt(n + 1, 0);
+  for (i64 k = 0; k < nu; ++k) cnt[uc[k]]++;
+  A.colptr.assign(n + 1, 1);
+  for (i64 j = 1; j <= n; ++j) A.colptr[j] = A.colptr[j - 1] + cnt[j];
+  A.rowval.assign(nu, 0);
+  A.nzval.assign(nu, 0.0);
+  std::vector<i64> pos(A.colptr.begin(), A.colptr.end());
+  for (i64 k = 0; k < nu; ++k) {
+    i64 p = pos[uc[k] - 1]++ - 1;
+    A.rowval[p] = ur[k];
+    A.nzval[p] = uv[k];
+  }
+}
+
+static double eval_source(int f, double c, const double* x) {
+  switch (f) {
+    case F_ONE: return 1.0;
+    case F_SINSIN: return sin(M_PI * x[0]) * sin(M_PI * x[1]);
+    case F_X: return x[0];
+    case F_CONST: return c;
+  }
+  return 0.0;
+}
+
+// Per-qp physical gradients: ∂λξ_∂x = ∇̂λ * J⁻¹ (ref2phys.jl:22-26), left folds.
+static void phys_grad(int d, int nl, const double* rgrad, const double* Jinv, double* pg) {
+  for (int i = 0; i < nl; ++i)
+    for (int b = 0; b < d; ++b) {
+      double s = rgrad[i * d + 0] * Jinv[0 * d + b];
+      for (int a = 1; a < d; ++a) s = s + rgrad[i * d + a] * Jinv[a * d + b];
+      pg[i * d + b] = s;
+    }
+}
+
+// One integral ∫ term dΩ on every cell, appended as COO in reference order
+// (__assemble_bilinear! assembler.jl:280-307, _append_bilinear! :632-648;
+// integrate_on_ref_element integration.jl:18-105).
+static void append_term(const Mesh& m, const std::vector<i64>& iglob, const std::vector<i64>& offset,
+                        int order, int term, const double* eta, int degree,
+                        std::vector<i64>& I, std::vector<i64>& J, std::vector<double>& X) {
+  std::vector<i64> cells = element_order(m);
+  int d = m.dim;
+  for (i64 c : cells) {
+    int ct = m.ctype[c];
+    Shape s = shape_of(ct);
+    int nl = ndofs_lagrange(s, order);
+    std::vector<double> w, xq;
+    int nq = tensor_rule(s, degree, Q_LEGENDRE, w, xq);
+    std::vector<i64> nodes = m.cell(c);
+    std::vector<double> Xc(nodes.size() * d);
+    for (size_t k = 0; k < nodes.size(); ++k)
+      for (int a = 0; a < d; ++a) Xc[k * d + a] = m.coords[nodes[k] * d + a];
+    std::vector<double> acc((size_t)nl * nl, 0.0), val(nl), rg(nl * d), pg(nl * d);
+    for (int q = 0; q < nq; ++q) {
+      Geo g;
+      geometry(ct, Xc.data(), &xq[q * d], g);
+      tensor_basis(s, order, &xq[q * d], val.data(), rg.data());
+      phys_grad(d, nl, rg.data(), g.Jinv, pg.data());
+      for (int j = 0; j < nl; ++j)
+        for (int i = 0; i < nl; ++i) {
+          double gij;
+          if (term == T_STIFF) {
+            double e = eta ? eta[c] : 1.0;
+            double sacc = (e * pg[j * d + 0]) * pg[i * d + 0];
+            for (int a = 1; a < d; ++a) sacc = sacc + (e * pg[j * d + a]) * pg[i * d + a];
+            gij = sacc;
+          } else {
+            gij = val[j] * val[i];
+          }
+          double wu = w[q] * (g.detabs * gij);
+          size_t k = (size_t)j * nl + i;
+          acc[k] = (q == 0) ? wu : acc[k] + wu;
+        }
+    }
+    // COO entry k = i + nV*j : row Vdofs[i], col Udofs[j]  (_cartesian_product :1357-1370)
+    for (int j = 0; j < nl; ++j)
+      for (int i = 0; i < nl; ++i) {
+        I.push_back(iglob[offset[c] + i]);
+        J.push_back(iglob[offset[c] + j]);
+        X.push_back(acc[(size_t)j * nl + i]);
+      }
+  }
+}
+
+// assemble_linear with a source term ∫ f v dΩ (assembler.jl:448-533, 855-865).
+static void assemble_source(const Mesh& m, const std::vector<i64>& iglob, const std::vector<i64>& offset,
+                            int order, int fsrc, double fconst, int degree, std::vector<double>& b) {
+  std::vector<i64> cells = element_order(m);
+  int d = m.dim;
+  for (i64 c : cells) {
+    int ct = m.ctype[c];
+    Shape s = shape_of(ct);
+    int nl = ndofs_lagrange(s, order);
+    std::vector<double> w, xq;
+    int nq = tensor_rule(s, degree, Q_LEGENDRE, w, xq);
+    std::vector<i64> nodes = m.cell(c);
+    std::vector<double> Xc(nodes.size() * d);
+    for (size_t k = 0; k < nodes.size(); ++k)
+      for (int a = 0; a < d; ++a) Xc[k * d + a] = m.coords[nodes[k] * d + a];
+    std::vector<double> acc(nl, 0.0), val(nl), rg(nl * d);
+    for (int q = 0; q < nq; ++q) {
+      Geo g;
+      geometry(ct, Xc.data(), &xq[q * d], g);
+      tensor_basis(s, order, &xq[q * d], val.data(), rg.data());
+      double fv = eval_source(fsrc, fconst, g.x);
+      for (int i = 0; i < nl; ++i) {
+        double wu = w[q] * (g.detabs * (fv * val[i]));
+        acc[i] = (q == 0) ? wu : acc[i] + wu;
+      }
+    }
+    for (int i = 0; i < nl; ++i) b[iglob[offset[c] + i] - 1] += acc[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C ABI used by tests (ctypes)
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static Mesh make_mesh(int dim, i64 nnodes, const double* coords, i64 ncells, const int* ctypes,
+                      const i64* c2n_ptr, const i64* c2n) {
+  Mesh m;
+  m.dim = dim;
+  m.coords.assign(coords, coords + nnodes * dim);
+  m.ctype.assign(ctypes, ctypes + ncells);
+  m.c2n_ptr.assign(c2n_ptr, c2n_ptr + ncells + 1);
+  m.c2n.assign(c2n, c2n + c2n_ptr[ncells]);
+  return m;
+}
+
+struct Result {
+  CSC A;
+  std::vector<double> b;
+  std::vector<i64> iglob, offset;
+  i64 ndofs = 0;
+};
+
+extern "C" {
+
+const char* or_last_error() { return g_err.c_str(); }
+
+void or_rectangle_mesh(int nx, int ny, double xmin, double xmax, double ymin, double ymax,
+                       int order, double* coords, int64_t* c2n) {
+  if (order == 1) rectangle_mesh_q1(nx, ny, xmin, xmax, ymin, ymax, coords, c2n);
+  else rectangle_mesh_q2(nx, ny, xmin, xmax, ymin, ymax, coords, c2n);
+}
+
+void or_hexa_mesh(int nx, int ny, int nz, double xmin, double xmax, double ymin, double ymax,
+                  double zmin, double zmax, double* coords, int64_t* c2n) {
+  hexa_mesh_q1(nx, ny, nz, xmin, xmax, ymin, ymax, zmin, zmax, coords, c2n);
+}
+
+int or_quadrature(int shape, int degree, int qtype, double* w, double* x) {
+  std::vector<double> ww, xx;
+  int n;
+  if (shape < 0) {  // 1D line rule
+    if (degree == 0) degree = 1;
+    line_rule(degree, qtype, ww, xx);
+    n = (int)ww.size();
+  } else {
+    n = tensor_rule(shape, degree, qtype, ww, xx);
+  }
+  if (w) std::copy(ww.begin(), ww.end(), w);
+  if (x) std::copy(xx.begin(), xx.end(), x);
+  return n;
+}
+
+int or_shape_functions(int shape, int order, const double* xi, double* val, double* grad) {
+  try {
+    tensor_basis(shape, order, xi, val, grad);
+    return ndofs_lagrange(shape, order);
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+int or_geometry(int ct, const double* X, const double* xi, double* J, double* Jinv,
+                double* detabs, double* x) {
+  try {
+    Geo g;
+    geometry(ct, X, xi, g);
+    int d = shape_of(ct) == CUBE ? 3 : 2;
+    std::copy(g.J, g.J + d * d, J);
+    std::copy(g.Jinv, g.Jinv + d * d, Jinv);
+    *detabs = g.detabs;
+    std::copy(g.x, g.x + d, x);
+    return 0;
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+int or_densify(int64_t n, const int64_t* a, int64_t* out) {
+  std::vector<i64> v(a, a + n);
+  v = densify(v);
+  std::copy(v.begin(), v.end(), out);
+  return 0;
+}
+
+// DofHandler: returns ndofs_tot (>0) or -1; iglob 1-based (size Σ ndofs), offset per cell.
+int64_t or_dofhandler(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                      const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n, int order,
+                      int continuous, int64_t* iglob, int64_t* offset) {
+  try {
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    std::vector<i64> ig, off;
+    std::vector<int> nd;
+    i64 tot = dofhandler(m, order, continuous != 0, ig, off, nd);
+    std::copy(ig.begin(), ig.end(), iglob);
+    std::copy(off.begin(), off.end(), offset);
+    return tot;
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// assemble_bilinear for a sum of terms (MultiIntegration, assembler.jl:220-238):
+// each term appends its own COO segment in order, then sparse() sums them.
+void* or_assemble_bilinear(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                           const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n,
+                           int order, int continuous, int nterms, const int* terms,
+                           const double* eta, int degree) {
+  try {
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    Result* r = new Result();
+    std::vector<int> nd;
+    r->ndofs = dofhandler(m, order, continuous != 0, r->iglob, r->offset, nd);
+    std::vector<i64> I, J;
+    std::vector<double> X;
+    for (int t = 0; t < nterms; ++t)
+      append_term(m, r->iglob, r->offset, order, terms[t], eta, degree, I, J, X);
+    coo_to_csc(I, J, X, r->ndofs, r->ndofs, r->A);
+    return r;
+  } catch (std::exception& e) { g_err = e.what(); return nullptr; }
+}
+
+void* or_assemble_linear(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                         const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n, int order,
+                         int continuous, int fsrc, double fconst, int degree) {
+  try {
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    Result* r = new Result();
+    std::vector<int> nd;
+    r->ndofs = dofhandler(m, order, continuous != 0, r->iglob, r->offset, nd);
+    r->b.assign(r->ndofs, 0.0);
+    assemble_source(m, r->iglob, r->offset, order, fsrc, fconst, degree, r->b);
+    return r;
+  } catch (std::exception& e) { g_err = e.what(); return nullptr; }
+}
+
+int64_t or_result_ndofs(void* h) { return ((Result*)h)->ndofs; }
+int64_t or_result_nnz(void* h) { return (int64_t)((Result*)h)->A.nzval.size(); }
+void or_result_csc(void* h, int64_t* colptr, int64_t* rowval, double* nzval) {
+  Result* r = (Result*)h;
+  std::copy(r->A.colptr.begin(), r->A.colptr.end(), colptr);
+  std::copy(r->A.rowval.begin(), r->A.rowval.end(), rowval);
+  std::copy(r->A.nzval.begin(), r->A.nzval.end(), nzval);
+}
+void or_result_vector(void* h, double* b) {
+  Result* r = (Result*)h;
+  std::copy(r->b.begin(), r->b.end(), b);
+}
+void or_result_dofs(void* h, int64_t* iglob, int64_t* offset) {
+  Result* r = (Result*)h;
+  std::copy(r->iglob.begin(), r->iglob.end(), iglob);
+  std::copy(r->offset.begin(), r->offset.end(), offset);
+}
+void or_result_free(void* h) { delete (Result*)h; }
+
+}  // extern "C"

@@ -1,0 +1,224 @@
+"""ctypes bindings to the CPU oracle (``liboracle.so`` built from bcube_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` -- never by the product package.  Every
+function restates Bcube.jl's Float64 CPU assembly (file:line citations live in
+bcube_oracle.cpp next to each routine).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+CT = {"Quad4": 0, "Quad9": 1, "Tri3": 2, "Hexa8": 3, "Hexa27": 4}
+SHAPE = {"Square": 0, "Triangle": 1, "Cube": 2, "Line": -1}
+TERM = {"stiffness": 0, "mass": 1}
+SOURCE = {"one": 0, "sinsin": 1, "x": 2, "const": 3}
+
+_d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_l = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_i = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build(force: bool = False) -> str:
+    path = os.path.join(_HERE, "liboracle.so")
+    src = os.path.join(_HERE, "bcube_oracle.cpp")
+    if force or not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(src):
+        subprocess.run(["make", "-C", _HERE, "-B", "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return path
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.or_last_error.restype = C.c_char_p
+        L.or_rectangle_mesh.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_int, _d, _l]
+        L.or_hexa_mesh.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_double] * 6 + [_d, _l]
+        L.or_quadrature.argtypes = [C.c_int, C.c_int, C.c_int, _d, _d]
+        L.or_quadrature.restype = C.c_int
+        L.or_shape_functions.argtypes = [C.c_int, C.c_int, _d, _d, _d]
+        L.or_shape_functions.restype = C.c_int
+        L.or_geometry.argtypes = [C.c_int, _d, _d, _d, _d, _d, _d]
+        L.or_geometry.restype = C.c_int
+        L.or_densify.argtypes = [C.c_int64, _l, _l]
+        mesh_args = [C.c_int, C.c_int64, _d, C.c_int64, _i, _l, _l]
+        L.or_dofhandler.argtypes = mesh_args + [C.c_int, C.c_int, _l, _l]
+        L.or_dofhandler.restype = C.c_int64
+        L.or_assemble_bilinear.argtypes = mesh_args + [C.c_int, C.c_int, C.c_int, _i,
+                                                       C.c_void_p, C.c_int]
+        L.or_assemble_bilinear.restype = C.c_void_p
+        L.or_assemble_linear.argtypes = mesh_args + [C.c_int, C.c_int, C.c_int, C.c_double,
+                                                     C.c_int]
+        L.or_assemble_linear.restype = C.c_void_p
+        for n in ("or_result_ndofs", "or_result_nnz"):
+            getattr(L, n).argtypes = [C.c_void_p]
+            getattr(L, n).restype = C.c_int64
+        L.or_result_csc.argtypes = [C.c_void_p, _l, _l, _d]
+        L.or_result_vector.argtypes = [C.c_void_p, _d]
+        L.or_result_dofs.argtypes = [C.c_void_p, _l, _l]
+        L.or_result_free.argtypes = [C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def _err():
+    return lib().or_last_error().decode()
+
+
+class OMesh:
+    """Plain mesh arrays as the oracle consumes them (0-based c2n CSR)."""
+
+    def __init__(self, dim, coords, celltypes, c2n_ptr, c2n):
+        self.dim = dim
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+        self.celltypes = np.ascontiguousarray(celltypes, dtype=np.int32)
+        self.c2n_ptr = np.ascontiguousarray(c2n_ptr, dtype=np.int64)
+        self.c2n = np.ascontiguousarray(c2n, dtype=np.int64)
+
+    @property
+    def nnodes(self):
+        return self.coords.size // self.dim
+
+    @property
+    def ncells(self):
+        return self.celltypes.size
+
+    def args(self):
+        return (self.dim, self.nnodes, self.coords, self.ncells, self.celltypes,
+                self.c2n_ptr, self.c2n)
+
+    @staticmethod
+    def uniform(dim, coords, ctype, c2n_2d):
+        c2n_2d = np.asarray(c2n_2d, dtype=np.int64)
+        nc, k = c2n_2d.shape
+        return OMesh(dim, coords, np.full(nc, ctype, np.int32),
+                     np.arange(nc + 1, dtype=np.int64) * k, c2n_2d.reshape(-1))
+
+
+def rectangle_mesh(nx, ny, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, order=1):
+    """mesh_generator.jl:218-434 (Quad4 for order 1, Quad9 for order 2)."""
+    if order == 1:
+        nn, k, ct = nx * ny, 4, CT["Quad4"]
+    else:
+        nn = nx * ny + (nx - 1) * ny + (ny - 1) * nx + (nx - 1) * (ny - 1)
+        k, ct = 9, CT["Quad9"]
+    coords = np.zeros(nn * 2)
+    c2n = np.zeros((nx - 1) * (ny - 1) * k, dtype=np.int64)
+    lib().or_rectangle_mesh(nx, ny, xmin, xmax, ymin, ymax, order, coords, c2n)
+    return OMesh.uniform(2, coords, ct, c2n.reshape(-1, k))
+
+
+def hexa_mesh(nx, ny, nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0):
+    """mesh_generator.jl:453-552."""
+    coords = np.zeros(nx * ny * nz * 3)
+    c2n = np.zeros((nx - 1) * (ny - 1) * (nz - 1) * 8, dtype=np.int64)
+    lib().or_hexa_mesh(nx, ny, nz, xmin, xmax, ymin, ymax, zmin, zmax, coords, c2n)
+    return OMesh.uniform(3, coords, CT["Hexa8"], c2n.reshape(-1, 8))
+
+
+def quadrature(shape, degree, qtype=0):
+    s = SHAPE[shape]
+    w = np.zeros(4096)
+    x = np.zeros(4096 * 3)
+    n = lib().or_quadrature(s, degree, qtype, w, x)
+    d = 1 if s < 0 else (2 if s == 0 else 3)
+    return w[:n].copy(), x[: n * d].reshape(n, d).copy()
+
+
+def shape_functions(shape, order, xi):
+    s = SHAPE[shape]
+    d = 2 if s == 0 else 3
+    n = (order + 1) ** d
+    val = np.zeros(n)
+    grad = np.zeros(n * d)
+    r = lib().or_shape_functions(s, order, np.ascontiguousarray(xi, dtype=np.float64), val, grad)
+    if r < 0:
+        raise RuntimeError(_err())
+    return val, grad.reshape(n, d)
+
+
+def geometry(celltype, cnodes, xi):
+    ct = CT[celltype]
+    d = 3 if ct in (3, 4) else 2
+    J = np.zeros(d * d)
+    Ji = np.zeros(d * d)
+    det = np.zeros(1)
+    x = np.zeros(d)
+    r = lib().or_geometry(ct, np.ascontiguousarray(cnodes, dtype=np.float64).reshape(-1),
+                          np.ascontiguousarray(xi, dtype=np.float64), J, Ji, det, x)
+    if r < 0:
+        raise RuntimeError(_err())
+    return J.reshape(d, d), Ji.reshape(d, d), det[0], x
+
+
+def densify(a):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    out = np.zeros_like(a)
+    lib().or_densify(a.size, a, out)
+    return out
+
+
+def dofhandler(mesh: OMesh, order, continuous=True):
+    """Returns (iglob 1-based, offset, ndofs_tot)."""
+    nd = {0: (order + 1) ** 2, 1: (order + 1) ** 2, 2: [1, 3, 6, 10][min(order, 3)],
+          3: (order + 1) ** 3, 4: (order + 1) ** 3}
+    total = int(sum(nd[int(t)] for t in mesh.celltypes))
+    iglob = np.zeros(total, dtype=np.int64)
+    offset = np.zeros(mesh.ncells, dtype=np.int64)
+    n = lib().or_dofhandler(*mesh.args(), order, int(continuous), iglob, offset)
+    if n < 0:
+        raise RuntimeError(_err())
+    return iglob, offset, int(n)
+
+
+def assemble_bilinear(mesh: OMesh, order, terms, eta=None, degree=2, continuous=True):
+    """CSC (1-based colptr/rowval, nzval) of Σ terms, in SparseMatrixCSC layout."""
+    t = np.ascontiguousarray([TERM[x] for x in terms], dtype=np.int32)
+    eta_p = None
+    if eta is not None:
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        eta_p = eta.ctypes.data
+    h = lib().or_assemble_bilinear(*mesh.args(), order, int(continuous), len(t), t, eta_p, degree)
+    if not h:
+        raise RuntimeError(_err())
+    try:
+        n = lib().or_result_ndofs(h)
+        nnz = lib().or_result_nnz(h)
+        colptr = np.zeros(n + 1, dtype=np.int64)
+        rowval = np.zeros(nnz, dtype=np.int64)
+        nzval = np.zeros(nnz)
+        lib().or_result_csc(h, colptr, rowval, nzval)
+    finally:
+        lib().or_result_free(h)
+    return colptr, rowval, nzval, int(n)
+
+
+def assemble_linear(mesh: OMesh, order, source="sinsin", const=0.0, degree=2, continuous=True):
+    h = lib().or_assemble_linear(*mesh.args(), order, int(continuous), SOURCE[source],
+                                 float(const), degree)
+    if not h:
+        raise RuntimeError(_err())
+    try:
+        n = lib().or_result_ndofs(h)
+        b = np.zeros(n)
+        lib().or_result_vector(h, b)
+    finally:
+        lib().or_result_free(h)
+    return b
+
+
+def csc_to_scipy(colptr, rowval, nzval, n):
+    import scipy.sparse as sp
+    return sp.csc_matrix((nzval, rowval - 1, colptr - 1), shape=(n, n))

@@ -1,0 +1,23 @@
+"""bcubehip -- MI355X-native finite-element assembly behind Bcube.jl's interface.
+
+The hot path (element integrals + deterministic CSR scatter) runs in
+libbcubehip.so, hand-written HIP for gfx950; this package is the host-side
+mirror of Bcube.jl's assembly API (src/assembler/assembler.jl).
+"""
+from ._lib import BcubeHipError, LIB_PATH, lib
+from .mesh import (AbstractBcubeBackend, BcubeBackendHIP, BcubeBackendHostOnly, Mesh,
+                   get_bcube_backend, hexa_mesh, jitter_nodes, rectangle_mesh)
+from .fespace import DofHandler, FunctionSpace, TestFESpace, TrialFESpace
+from .forms import (CellDomain, Measure, MeshCellData, PhysicalFunction, dot, grad,
+                    integrate, nabla)
+from .assembler import (BilinearPlan, assemble_bilinear, assemble_linear, assemble_linear_,
+                        quadrature_points)
+
+__all__ = [
+    "BcubeHipError", "LIB_PATH", "lib", "AbstractBcubeBackend", "BcubeBackendHIP",
+    "BcubeBackendHostOnly", "Mesh", "get_bcube_backend", "hexa_mesh", "jitter_nodes",
+    "rectangle_mesh", "DofHandler", "FunctionSpace", "TestFESpace", "TrialFESpace",
+    "CellDomain", "Measure", "MeshCellData", "PhysicalFunction", "dot", "grad", "integrate",
+    "nabla", "BilinearPlan", "assemble_bilinear", "assemble_linear", "assemble_linear_",
+    "quadrature_points",
+]

@@ -1,0 +1,224 @@
+"""Meshes, backends and the synthetic mesh generators.
+
+Mirrors src/backend.jl (AbstractBcubeBackend), src/mesh/mesh.jl (Mesh,
+set_nodes!) and src/mesh/mesh_generator.jl (rectangle_mesh, hexa_mesh) of
+Bcube.jl.  Node and cell numbering follow the reference exactly; the seeded
+node jitter is this package's synthetic-workload generator.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class AbstractBcubeBackend:
+    """Plugin point of the reference (src/backend.jl:4)."""
+
+
+class BcubeBackendHIP(AbstractBcubeBackend):
+    """One HIP context (device + stream) of libbcubehip.  Created lazily."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+        self._ctx = None
+
+    @property
+    def ctx(self):
+        if self._ctx is None:
+            h = C.c_void_p()
+            L.check(L.lib().bh_ctx_create(self.device, C.byref(h)), None, "bh_ctx_create")
+            self._ctx = h
+        return self._ctx
+
+    def set_stream(self, stream_ptr: int | None):
+        """Launch on a caller stream (e.g. torch.cuda.current_stream().cuda_stream)."""
+        L.check(L.lib().bh_ctx_set_stream(self.ctx, stream_ptr), self.ctx, "bh_ctx_set_stream")
+
+    def sync(self):
+        L.check(L.lib().bh_sync(self.ctx), self.ctx, "bh_sync")
+
+    def set_timing(self, enable: bool):
+        L.check(L.lib().bh_ctx_set_timing(self.ctx, int(enable)), self.ctx, "bh_ctx_set_timing")
+
+    def timings(self, cap: int = 4096) -> np.ndarray:
+        buf = np.zeros(cap, dtype=np.float32)
+        n = C.c_int()
+        L.check(L.lib().bh_ctx_timings(self.ctx, buf.ctypes.data, cap, C.byref(n)), self.ctx,
+                "bh_ctx_timings")
+        return buf[: min(n.value, cap)].copy()
+
+    def __del__(self):
+        try:
+            if self._ctx is not None:
+                L.lib().bh_ctx_destroy(self._ctx)
+        except Exception:
+            pass
+
+
+class BcubeBackendHostOnly(AbstractBcubeBackend):
+    """Numbering and sparsity-pattern queries only (no GPU context, no assembly)."""
+
+    ctx = None
+
+
+_DEFAULT = None
+
+
+def get_bcube_backend() -> AbstractBcubeBackend:
+    """Default backend (src/backend.jl:8-12): the HIP backend on device 0."""
+    global _DEFAULT
+    if _DEFAULT is None:
+        _DEFAULT = BcubeBackendHIP(0)
+    return _DEFAULT
+
+
+NNODES = {0: 4, 1: 9, 2: 3, 3: 8, 4: 27}
+DIM = {0: 2, 1: 2, 2: 2, 3: 3, 4: 3}
+
+
+class Mesh:
+    """Mesh(nodes, celltypes, c2n; backend) (src/mesh/mesh.jl:221-262).
+
+    nodes: (nnodes, dim) float64; celltypes: names ("Quad4", "Hexa8", ...) or
+    codes; c2n: (ncells, k) array or list of per-cell node lists, 0-based.
+    """
+
+    def __init__(self, nodes, celltypes, c2n, backend: AbstractBcubeBackend | None = None):
+        nodes = np.asarray(nodes, dtype=np.float64)
+        self.dim = nodes.shape[1]
+        self.coords = np.ascontiguousarray(nodes.reshape(-1))
+        ct = [L.CELLTYPE[t] if isinstance(t, str) else int(t) for t in celltypes]
+        self.celltypes = np.ascontiguousarray(ct, dtype=np.int32)
+        if isinstance(c2n, np.ndarray) and c2n.ndim == 2:
+            k = c2n.shape[1]
+            self.c2n_ptr = np.arange(c2n.shape[0] + 1, dtype=np.int64) * k
+            self.c2n = np.ascontiguousarray(c2n.reshape(-1), dtype=np.int64)
+        else:
+            lens = [len(c) for c in c2n]
+            self.c2n_ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+            self.c2n = np.ascontiguousarray(np.concatenate([np.asarray(c) for c in c2n]),
+                                            dtype=np.int64)
+        for t in self.celltypes:
+            if DIM[int(t)] != self.dim:
+                raise ValueError("cell dimension differs from the node dimension")
+        self.backend = backend if backend is not None else get_bcube_backend()
+        self._handle = None
+
+    @property
+    def nnodes(self):
+        return self.coords.size // self.dim
+
+    @property
+    def ncells(self):
+        return self.celltypes.size
+
+    def nodes(self):
+        return self.coords.reshape(-1, self.dim)
+
+    def cell_nodes(self, icell):
+        return self.c2n[self.c2n_ptr[icell]:self.c2n_ptr[icell + 1]]
+
+    @property
+    def handle(self):
+        if self._handle is None:
+            h = C.c_void_p()
+            ctx = self.backend.ctx
+            L.check(L.lib().bh_mesh_create(ctx, self.dim, self.nnodes, self.coords.ctypes.data,
+                                           self.ncells, self.celltypes.ctypes.data,
+                                           self.c2n_ptr.ctypes.data, self.c2n.ctypes.data,
+                                           C.byref(h)), ctx, "bh_mesh_create")
+            self._handle = h
+        return self._handle
+
+    def set_nodes(self, nodes):
+        """set_nodes! (src/mesh/mesh.jl:280): same topology, new coordinates."""
+        nodes = np.ascontiguousarray(np.asarray(nodes, dtype=np.float64).reshape(-1))
+        if nodes.size != self.coords.size:
+            raise ValueError("node array has the wrong size")
+        self.coords = nodes
+        if self._handle is not None:
+            L.check(L.lib().bh_mesh_set_nodes(self._handle, nodes.ctypes.data, L.HOST),
+                    self.backend.ctx, "bh_mesh_set_nodes")
+
+    def __del__(self):
+        try:
+            if self._handle is not None:
+                L.lib().bh_mesh_destroy(self._handle)
+        except Exception:
+            pass
+
+
+def rectangle_mesh(nx, ny, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, order=1,
+                   backend: AbstractBcubeBackend | None = None) -> Mesh:
+    """rectangle_mesh (mesh_generator.jl:218-434): nx*ny vertices, node
+    (iy-1)*nx+ix, Quad4 cells (or Quad9 for order=2) in x-fastest order."""
+    P00, P10, P11, P01 = (xmin, ymin), (xmax, ymin), (xmax, ymax), (xmin, ymax)
+    if order == 1:
+        NX, NY, scale = nx, ny, 1.0
+    elif order == 2:
+        NX, NY, scale = 2 * nx - 1, 2 * ny - 1, 0.5
+    else:
+        raise ValueError("order must be 1 or 2")
+    ix = np.arange(1, NX + 1, dtype=np.float64)
+    iy = np.arange(1, NY + 1, dtype=np.float64)
+    if order == 1:
+        u = (ix - 1) / (nx - 1)
+        v = (iy - 1) / (ny - 1)
+    else:
+        u = scale * (ix - 1) / (nx - 1)
+        v = scale * (iy - 1) / (ny - 1)
+    U, V = np.meshgrid(u, v, indexing="xy")  # shape (NY, NX), x fastest after ravel
+    U, V = U.ravel(), V.ravel()
+    xy = np.empty((U.size, 2))
+    for d in range(2):
+        a = (1 - U) * (1 - V) * P00[d]
+        b = U * (1 - V) * P10[d]
+        c = (1 - U) * V * P01[d]
+        e = U * V * P11[d]
+        xy[:, d] = ((a + b) + c) + e
+    jx, jy = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1), indexing="xy")
+    jx, jy = jx.ravel(), jy.ravel()
+    if order == 1:
+        n0 = jy * nx + jx
+        c2n = np.stack([n0, n0 + 1, n0 + nx + 1, n0 + nx], axis=1)
+        return Mesh(xy, np.full(c2n.shape[0], L.CELLTYPE["Quad4"]), c2n, backend)
+    bx, by = 2 * jx, 2 * jy
+    N = lambda dy, dx: (by + dy) * NX + bx + dx
+    c2n = np.stack([N(0, 0), N(0, 2), N(2, 2), N(2, 0), N(0, 1), N(1, 2), N(2, 1), N(1, 0),
+                    N(1, 1)], axis=1)
+    return Mesh(xy, np.full(c2n.shape[0], L.CELLTYPE["Quad9"]), c2n, backend)
+
+
+def hexa_mesh(nx, ny, nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0,
+              backend: AbstractBcubeBackend | None = None) -> Mesh:
+    """hexa_mesh (mesh_generator.jl:453-552): node (iz-1)nx*ny+(iy-1)nx+ix,
+    Hexa8 cells in CGNS order, x fastest."""
+    dx = (xmax - xmin) / (nx - 1)
+    dy = (ymax - ymin) / (ny - 1)
+    dz = (zmax - zmin) / (nz - 1)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    xyz = np.stack([xmin + x.ravel() * dx, ymin + y.ravel() * dy, zmin + z.ravel() * dz], axis=1)
+    kz, ky, kx = np.meshgrid(np.arange(nz - 1), np.arange(ny - 1), np.arange(nx - 1), indexing="ij")
+    n0 = (kz.ravel() * ny + ky.ravel()) * nx + kx.ravel()
+    nxy = nx * ny
+    c2n = np.stack([n0, n0 + 1, n0 + nx + 1, n0 + nx,
+                    n0 + nxy, n0 + nxy + 1, n0 + nxy + nx + 1, n0 + nxy + nx], axis=1)
+    return Mesh(xyz, np.full(c2n.shape[0], L.CELLTYPE["Hexa8"]), c2n.astype(np.int64), backend)
+
+
+def jitter_nodes(mesh: Mesh, amplitude: float, h, seed: int = 0) -> np.ndarray:
+    """Seeded uniform jitter of interior nodes by ±amplitude*h per axis.
+    Boundary nodes stay put, so the domain (and Σ mass) is unchanged."""
+    X = mesh.nodes().copy()
+    lo, hi = X.min(axis=0), X.max(axis=0)
+    h = np.broadcast_to(np.asarray(h, dtype=np.float64), (mesh.dim,))
+    tol = 1e-12 * np.maximum(1.0, np.abs(hi - lo))
+    interior = np.all((X > lo + tol) & (X < hi - tol), axis=1)
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(-amplitude, amplitude, size=X.shape) * h
+    X[interior] += d[interior]
+    mesh.set_nodes(X)
+    return X

@@ -1,0 +1,664 @@
+// bh_api.hip -- the C ABI of libbcubehip (declared in include/bcubehip.h):
+// handles, device buffers, and the colour-ordered kernel launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "bh_internal.h"
+#include "bh_kernels.h"
+
+using namespace bh;
+
+struct bh_ctx_s {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  size_t nevents = 0;
+  std::map<std::tuple<int, int, int>, double*> tables;  // (geo, order, nq1) -> device table
+};
+
+struct bh_mesh_s {
+  bh_ctx ctx = nullptr;
+  int dev = -1;
+  HostMesh h;
+  int geo = -1;  // uniform geometric cell type, or -1 for mixed meshes
+  int npc = 0;   // nodes per cell when uniform
+  double* d_coords = nullptr;
+  int32_t* d_c2n = nullptr;
+};
+
+struct Colouring {
+  int ncolors = 0;
+  std::vector<int32_t> color_ptr, cells;
+  int32_t* d_cells = nullptr;
+};
+
+// Handles record their device (-1: host-only) so destroy calls never touch
+// another handle, whatever order the caller frees them in.
+struct bh_space_s {
+  bh_mesh mesh = nullptr;
+  int dev = -1;
+  int order = 1;
+  bool continuous = true;
+  int nl = 0;
+  int64_t ndofs = 0;
+  std::vector<int32_t> dofs;  // ncells * nl, 0-based
+  int32_t* d_dofs = nullptr;
+  Colouring col;              // for vector (linear-form) assembly
+  double* d_b = nullptr;      // scratch output for host-side linear forms
+  double* d_coef = nullptr;   // staged coefficients
+  size_t coef_cap = 0;
+};
+
+struct bh_pattern_s {
+  bh_space U = nullptr, V = nullptr;
+  int dev = -1;
+  HostPattern h;
+  int64_t* d_rowptr = nullptr;
+  int32_t* d_cells = nullptr;
+  void* d_scatter = nullptr;
+  double* d_vals = nullptr;
+  double* d_coef = nullptr;  // staged host coefficients (2 * ncells)
+};
+
+// ---------------------------------------------------------------------------
+namespace {
+
+thread_local std::string g_last_error;  // errors of calls made without a context
+
+int fail(bh_ctx c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  g_last_error = msg;
+  return code;
+}
+
+#define BH_HIP(ctx, call)                                                          \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail((ctx), BH_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  return hipMalloc((void**)p, n * sizeof(T));
+}
+
+int set_device(bh_ctx c) {
+  if (!c) return BH_OK;  // host-only handles
+  BH_HIP(c, hipSetDevice(c->device));
+  return BH_OK;
+}
+
+void time_begin(bh_ctx c) {
+  if (!c->timing) return;
+  if (c->nevents == c->events.size()) {
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    c->events.push_back({a, b});
+  }
+  (void)hipEventRecord(c->events[c->nevents].first, c->stream);
+}
+void time_end(bh_ctx c) {
+  if (!c->timing) return;
+  (void)hipEventRecord(c->events[c->nevents].second, c->stream);
+  c->nevents++;
+}
+
+int get_tables(bh_ctx c, int geo, int order, int quad_degree, double** out) {
+  int nq1 = gauss_points_per_axis(quad_degree);
+  auto key = std::make_tuple(geo, order, nq1);
+  auto it = c->tables.find(key);
+  if (it != c->tables.end()) { *out = it->second; return BH_OK; }
+  int size = packed_table_size(geo, order, nq1);
+  if (size < 0)
+    return fail(c, BH_ERR_UNSUPPORTED,
+                "no kernel for cell type " + std::to_string(geo) + ", order " +
+                    std::to_string(order) + ", " + std::to_string(nq1) + " points per axis");
+  Tables T = make_tables(shape_of(geo), order, quad_degree, geo);
+  std::vector<double> packed(size, 0.0);
+  int d = T.dim, nl = T.nl, nq = T.nq, ng = T.ngeo;
+  for (int q = 0; q < nq; ++q) packed[q] = T.w[q];
+  int R = nq, DG = R + nq * nl * 4, G = DG + nq * ng * d;
+  for (int q = 0; q < nq; ++q)
+    for (int l = 0; l < nl; ++l) {
+      for (int a = 0; a < d; ++a) packed[R + (q * nl + l) * 4 + a] = T.dphi[((size_t)q * nl + l) * d + a];
+      packed[R + (q * nl + l) * 4 + d] = T.phi[(size_t)q * nl + l];
+    }
+  for (int q = 0; q < nq; ++q)
+    for (int k = 0; k < ng; ++k) {
+      for (int a = 0; a < d; ++a) packed[DG + (q * ng + k) * d + a] = T.dgphi[((size_t)q * ng + k) * d + a];
+      packed[G + q * ng + k] = T.gphi[(size_t)q * ng + k];
+    }
+  if (G + nq * ng != size) return fail(c, BH_ERR_ARG, "internal: table layout mismatch");
+  double* dt = nullptr;
+  BH_HIP(c, dalloc(&dt, size));
+  BH_HIP(c, hipMemcpy(dt, packed.data(), size * sizeof(double), hipMemcpyHostToDevice));
+  c->tables[key] = dt;
+  *out = dt;
+  return BH_OK;
+}
+
+int build_colouring(bh_ctx c, const std::vector<int32_t>& dofs, int nl, int64_t ndofs,
+                    int64_t ncells, Colouring& col) {
+  col.ncolors = colour_cells(dofs, nl, ndofs, ncells, col.color_ptr, col.cells);
+  if (col.ncolors < 0) return fail(c, BH_ERR_UNSUPPORTED, "colouring needs more than 255 colours");
+  BH_HIP(c, dalloc(&col.d_cells, col.cells.size()));
+  BH_HIP(c, hipMemcpy(col.d_cells, col.cells.data(), col.cells.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice));
+  return BH_OK;
+}
+
+// Coefficient pointer for a term on the device: device pointers pass through,
+// host arrays are staged into `stage` (at least n doubles).
+const double* coef_ptr(const bh_term& t, int where, double* stage, int64_t n, hipStream_t s) {
+  if (t.coef_kind == BH_COEF_CONST || t.coef == nullptr) return nullptr;
+  if (where == BH_DEVICE) return t.coef;
+  (void)hipMemcpyAsync(stage, t.coef, n * sizeof(double), hipMemcpyHostToDevice, s);
+  return stage;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* bh_last_error(bh_ctx ctx) { return ctx ? ctx->err.c_str() : g_last_error.c_str(); }
+
+int bh_ctx_create(int device, bh_ctx* out) {
+  if (!out) return BH_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return BH_ERR_HIP;
+  auto c = new (std::nothrow) bh_ctx_s();
+  if (!c) return BH_ERR_OOM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->own) != hipSuccess) {
+    delete c;
+    return BH_ERR_HIP;
+  }
+  c->stream = c->own;
+  *out = c;
+  return BH_OK;
+}
+
+int bh_ctx_destroy(bh_ctx c) {
+  if (!c) return BH_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->tables) (void)hipFree(kv.second);
+  for (auto& e : c->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+  return BH_OK;
+}
+
+int bh_ctx_set_stream(bh_ctx c, void* s) {
+  if (!c) return BH_ERR_ARG;
+  c->stream = s ? (hipStream_t)s : c->own;
+  return BH_OK;
+}
+
+int bh_sync(bh_ctx c) {
+  if (!c) return BH_ERR_ARG;
+  BH_HIP(c, hipStreamSynchronize(c->stream));
+  return BH_OK;
+}
+
+int bh_ctx_set_timing(bh_ctx c, int enable) {
+  if (!c) return BH_ERR_ARG;
+  c->timing = enable != 0;
+  c->nevents = 0;
+  return BH_OK;
+}
+
+int bh_ctx_timings(bh_ctx c, float* ms, int cap, int* count) {
+  if (!c || !count) return BH_ERR_ARG;
+  BH_HIP(c, hipStreamSynchronize(c->stream));
+  int n = (int)std::min<size_t>(c->nevents, (size_t)std::max(cap, 0));
+  for (int i = 0; i < n; ++i) {
+    float t = 0;
+    BH_HIP(c, hipEventElapsedTime(&t, c->events[i].first, c->events[i].second));
+    if (ms) ms[i] = t;
+  }
+  *count = (int)c->nevents;
+  c->nevents = 0;
+  return BH_OK;
+}
+
+// ---- mesh ------------------------------------------------------------------
+int bh_mesh_create(bh_ctx c, int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                   const int32_t* celltypes, const int64_t* c2n_ptr, const int64_t* c2n,
+                   bh_mesh* out) {
+  // c == NULL creates a host-only mesh: numbering and pattern queries work, assembly does not.
+  if (!out || !coords || !celltypes || !c2n_ptr || !c2n || (dim != 2 && dim != 3) ||
+      nnodes <= 0 || ncells <= 0)
+    return fail(c, BH_ERR_ARG, "bh_mesh_create: bad arguments");
+  *out = nullptr;
+  if (int r = set_device(c)) return r;
+  auto m = std::make_unique<bh_mesh_s>();
+  m->ctx = c;
+  m->dev = c ? c->device : -1;
+  HostMesh& h = m->h;
+  h.dim = dim; h.nnodes = nnodes; h.ncells = ncells;
+  h.coords.assign(coords, coords + nnodes * dim);
+  h.ctype.assign(celltypes, celltypes + ncells);
+  h.c2n_ptr.assign(c2n_ptr, c2n_ptr + ncells + 1);
+  if (h.c2n_ptr[0] != 0) return fail(c, BH_ERR_ARG, "c2n_ptr[0] must be 0");
+  h.c2n.assign(c2n, c2n + h.c2n_ptr[ncells]);
+  for (int64_t k = 0; k < ncells; ++k) {
+    int ct = h.ctype[k];
+    if (shape_of(ct) < 0) return fail(c, BH_ERR_ARG, "unknown cell type");
+    if (dim_of_shape(shape_of(ct)) != dim) return fail(c, BH_ERR_ARG, "cell dimension != mesh dimension");
+    if (h.c2n_ptr[k + 1] - h.c2n_ptr[k] != nnodes_of(ct))
+      return fail(c, BH_ERR_ARG, "cell " + std::to_string(k) + ": wrong node count");
+  }
+  for (int64_t v : h.c2n)
+    if (v < 0 || v >= nnodes) return fail(c, BH_ERR_RANGE, "c2n: node index out of range");
+  h.celltype = h.ctype[0];
+  h.uniform = std::all_of(h.ctype.begin(), h.ctype.end(), [&](int t) { return t == h.celltype; });
+  if (h.uniform && (h.celltype == BH_QUAD4 || h.celltype == BH_HEXA8)) {
+    m->geo = h.celltype;
+    m->npc = nnodes_of(h.celltype);
+  }
+  if (c && m->geo >= 0) {
+    std::vector<int32_t> c2n32(h.c2n.begin(), h.c2n.end());
+    BH_HIP(c, dalloc(&m->d_coords, h.coords.size()));
+    BH_HIP(c, dalloc(&m->d_c2n, c2n32.size()));
+    BH_HIP(c, hipMemcpy(m->d_coords, h.coords.data(), h.coords.size() * sizeof(double),
+                        hipMemcpyHostToDevice));
+    BH_HIP(c, hipMemcpy(m->d_c2n, c2n32.data(), c2n32.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
+  }
+  *out = m.release();
+  return BH_OK;
+}
+
+int bh_mesh_set_nodes(bh_mesh m, const double* coords, int where) {
+  if (!m || !coords) return BH_ERR_ARG;
+  bh_ctx c = m->ctx;
+  if (int r = set_device(c)) return r;
+  size_t n = m->h.coords.size();
+  if (where == BH_DEVICE) {
+    BH_HIP(c, hipMemcpy(m->h.coords.data(), coords, n * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::copy(coords, coords + n, m->h.coords.begin());
+  }
+  if (where == BH_DEVICE && !c) return fail(c, BH_ERR_ARG, "host-only mesh");
+  if (c && m->d_coords)
+    BH_HIP(c, hipMemcpy(m->d_coords, m->h.coords.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  return BH_OK;
+}
+
+int bh_mesh_destroy(bh_mesh m) {
+  if (!m) return BH_ERR_ARG;
+  if (m->dev >= 0) {
+    (void)hipSetDevice(m->dev);
+    (void)hipFree(m->d_coords);
+    (void)hipFree(m->d_c2n);
+  }
+  delete m;
+  return BH_OK;
+}
+
+// ---- dofs / spaces -----------------------------------------------------------
+int bh_dofhandler(bh_mesh m, int order, int continuous, int64_t* iglob, int64_t* offset,
+                  int64_t* ndofs_tot) {
+  if (!m || !iglob || !offset || !ndofs_tot || order < 0) return BH_ERR_ARG;
+  std::vector<int64_t> ig, off;
+  std::string err;
+  int64_t n = dofhandler(m->h, order, continuous != 0, ig, off, err);
+  if (n < 0) return fail(m->ctx, BH_ERR_UNSUPPORTED, err);
+  std::copy(ig.begin(), ig.end(), iglob);
+  std::copy(off.begin(), off.end(), offset);
+  *ndofs_tot = n;
+  return BH_OK;
+}
+
+static int make_space(bh_mesh m, int order, bool continuous, int64_t ndofs,
+                      const std::vector<int64_t>& iglob, const std::vector<int64_t>& offset,
+                      bh_space* out) {
+  bh_ctx c = m->ctx;
+  if (!m->h.uniform) return fail(c, BH_ERR_UNSUPPORTED, "spaces on mixed meshes are not supported yet");
+  auto s = std::make_unique<bh_space_s>();
+  s->mesh = m;
+  s->dev = m->dev;
+  s->order = order;
+  s->continuous = continuous;
+  s->nl = ndofs_lagrange(shape_of(m->h.celltype), order);
+  s->ndofs = ndofs;
+  int64_t nc = m->h.ncells;
+  s->dofs.resize((size_t)nc * s->nl);
+  for (int64_t k = 0; k < nc; ++k)
+    for (int i = 0; i < s->nl; ++i) {
+      int64_t p = offset[k] + i;
+      if (p < 0 || p >= (int64_t)iglob.size()) return fail(c, BH_ERR_RANGE, "offset out of range");
+      int64_t g = iglob[p] - 1;
+      if (g < 0 || g >= ndofs) return fail(c, BH_ERR_RANGE, "dof id out of range");
+      s->dofs[(size_t)k * s->nl + i] = (int32_t)g;
+    }
+  if (c) {
+    if (int r = set_device(c)) return r;
+    BH_HIP(c, dalloc(&s->d_dofs, s->dofs.size()));
+    BH_HIP(c, hipMemcpy(s->d_dofs, s->dofs.data(), s->dofs.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
+  }
+  *out = s.release();
+  return BH_OK;
+}
+
+int bh_space_create(bh_mesh m, int order, int continuous, bh_space* out) {
+  if (!m || !out || order < 0) return BH_ERR_ARG;
+  *out = nullptr;
+  std::vector<int64_t> ig, off;
+  std::string err;
+  int64_t n = dofhandler(m->h, order, continuous != 0, ig, off, err);
+  if (n < 0) return fail(m->ctx, BH_ERR_UNSUPPORTED, err);
+  return make_space(m, order, continuous != 0, n, ig, off, out);
+}
+
+int bh_space_create_from_dofs(bh_mesh m, int order, int continuous, int64_t ndofs,
+                              const int64_t* iglob, const int64_t* offset, bh_space* out) {
+  if (!m || !out || !iglob || !offset || ndofs <= 0) return BH_ERR_ARG;
+  *out = nullptr;
+  int nl = ndofs_lagrange(shape_of(m->h.celltype), order);
+  int64_t nc = m->h.ncells;
+  std::vector<int64_t> off(offset, offset + nc);
+  int64_t len = 0;
+  for (int64_t k = 0; k < nc; ++k) len = std::max(len, off[k] + nl);
+  std::vector<int64_t> ig(iglob, iglob + len);
+  return make_space(m, order, continuous != 0, ndofs, ig, off, out);
+}
+
+int bh_space_info(bh_space s, int64_t* ndofs, int32_t* nl) {
+  if (!s) return BH_ERR_ARG;
+  if (ndofs) *ndofs = s->ndofs;
+  if (nl) *nl = s->nl;
+  return BH_OK;
+}
+
+int bh_space_dofs(bh_space s, int32_t* dofs) {
+  if (!s || !dofs) return BH_ERR_ARG;
+  std::copy(s->dofs.begin(), s->dofs.end(), dofs);
+  return BH_OK;
+}
+
+int bh_space_destroy(bh_space s) {
+  if (!s) return BH_ERR_ARG;
+  if (s->dev >= 0) {
+    (void)hipSetDevice(s->dev);
+    (void)hipFree(s->d_dofs);
+    (void)hipFree(s->col.d_cells);
+    (void)hipFree(s->d_b);
+    (void)hipFree(s->d_coef);
+  }
+  delete s;
+  return BH_OK;
+}
+
+// ---- pattern -------------------------------------------------------------------
+int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out) {
+  if (!U || !V || !out) return BH_ERR_ARG;
+  *out = nullptr;
+  bh_ctx c = V->mesh->ctx;
+  if (U->mesh != V->mesh) return fail(c, BH_ERR_ARG, "U and V live on different meshes");
+  auto p = std::make_unique<bh_pattern_s>();
+  p->U = U;
+  p->V = V;
+  p->dev = V->dev;
+  std::string err;
+  if (!build_pattern(V->dofs, V->nl, V->ndofs, U->dofs, U->nl, U->ndofs, V->mesh->h.ncells, p->h, err))
+    return fail(c, BH_ERR_ARG, err);
+  HostPattern& h = p->h;
+  if (!c) {  // host-only pattern: keep the scatter map for inspection
+    *out = p.release();
+    return BH_OK;
+  }
+  if (int r = set_device(c)) return r;
+  BH_HIP(c, dalloc(&p->d_rowptr, h.rowptr.size()));
+  BH_HIP(c, hipMemcpy(p->d_rowptr, h.rowptr.data(), h.rowptr.size() * sizeof(int64_t),
+                      hipMemcpyHostToDevice));
+  BH_HIP(c, dalloc(&p->d_cells, h.cells_by_color.size()));
+  BH_HIP(c, hipMemcpy(p->d_cells, h.cells_by_color.data(), h.cells_by_color.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice));
+  BH_HIP(c, dalloc((uint8_t**)&p->d_scatter, h.scatter.size()));
+  BH_HIP(c, hipMemcpy(p->d_scatter, h.scatter.data(), h.scatter.size(), hipMemcpyHostToDevice));
+  BH_HIP(c, dalloc(&p->d_vals, (size_t)h.nnz));
+  BH_HIP(c, dalloc(&p->d_coef, (size_t)2 * V->mesh->h.ncells));
+  h.scatter.clear();
+  h.scatter.shrink_to_fit();
+  *out = p.release();
+  return BH_OK;
+}
+
+int bh_pattern_info(bh_pattern p, int64_t* nrows, int64_t* ncols, int64_t* nnz, int32_t* ncolors) {
+  if (!p) return BH_ERR_ARG;
+  if (nrows) *nrows = p->h.nrows;
+  if (ncols) *ncols = p->h.ncols;
+  if (nnz) *nnz = p->h.nnz;
+  if (ncolors) *ncolors = p->h.ncolors;
+  return BH_OK;
+}
+
+int bh_pattern_export_csc(bh_pattern p, int64_t* colptr, int64_t* rowval, int base) {
+  if (!p || !colptr || !rowval) return BH_ERR_ARG;
+  if (p->U != p->V)
+    return fail(p->V->mesh->ctx, BH_ERR_UNSUPPORTED, "CSC export of a rectangular pattern");
+  const HostPattern& h = p->h;
+  for (size_t k = 0; k < h.rowptr.size(); ++k) colptr[k] = h.rowptr[k] + base;
+  for (int64_t k = 0; k < h.nnz; ++k) rowval[k] = (int64_t)h.colval[k] + base;
+  return BH_OK;
+}
+
+int bh_pattern_values_device(bh_pattern p, double** d) {
+  if (!p || !d) return BH_ERR_ARG;
+  *d = p->d_vals;
+  return BH_OK;
+}
+
+int bh_pattern_destroy(bh_pattern p) {
+  if (!p) return BH_ERR_ARG;
+  if (p->dev >= 0) {
+    (void)hipSetDevice(p->dev);
+    (void)hipFree(p->d_rowptr);
+    (void)hipFree(p->d_cells);
+    (void)hipFree(p->d_scatter);
+    (void)hipFree(p->d_vals);
+    (void)hipFree(p->d_coef);
+  }
+  delete p;
+  return BH_OK;
+}
+
+// ---- assembly ----------------------------------------------------------------------
+int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int quad_degree, int mode,
+                         double* nzval, int where) {
+  if (!p || nterms <= 0 || !terms) return BH_ERR_ARG;
+  bh_space V = p->V;
+  bh_mesh m = V->mesh;
+  bh_ctx c = m->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "assembly needs a mesh created with a context");
+  if (p->U != p->V) return fail(c, BH_ERR_UNSUPPORTED, "assembly with U != V is not supported yet");
+  if (mode != BH_FP64) return fail(c, BH_ERR_UNSUPPORTED, "only BH_FP64 is implemented");
+  if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
+  if (where == BH_HOST && !nzval) return fail(c, BH_ERR_ARG, "nzval is NULL");
+  if (m->geo < 0) return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
+  const bh_term* stiff = nullptr;
+  const bh_term* mass = nullptr;
+  for (int k = 0; k < nterms; ++k) {
+    const bh_term& t = terms[k];
+    if (t.coef_kind != BH_COEF_CONST && t.coef_kind != BH_COEF_CELL)
+      return fail(c, BH_ERR_UNSUPPORTED, "bilinear coefficients must be constant or per cell");
+    if (t.coef_kind == BH_COEF_CELL && !t.coef) return fail(c, BH_ERR_ARG, "per-cell coefficient is NULL");
+    if (t.form == BH_FORM_STIFFNESS && !stiff) stiff = &t;
+    else if (t.form == BH_FORM_MASS && !mass) mass = &t;
+    else return fail(c, BH_ERR_UNSUPPORTED, "unsupported or repeated bilinear term");
+  }
+  int nq1 = gauss_points_per_axis(quad_degree);
+  if (!kernel_supported(m->geo, V->order, nq1))
+    return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this (cell type, order, quadrature)");
+  if (int r = set_device(c)) return r;
+  double* tab = nullptr;
+  if (int r = get_tables(c, m->geo, V->order, quad_degree, &tab)) return r;
+  int64_t nc = m->h.ncells;
+  LaunchArgs A;
+  A.kind = LaunchArgs::BILINEAR;
+  A.stream = c->stream;
+  A.coords = m->d_coords;
+  A.c2n = m->d_c2n;
+  A.rdofs = V->d_dofs;
+  A.tab = tab;
+  A.rowptr = p->d_rowptr;
+  A.scatter = p->d_scatter;
+  A.offset_bytes = p->h.offset_bytes;
+  if (stiff) {
+    A.eta = coef_ptr(*stiff, where, p->d_coef, nc, c->stream);
+    A.eta_scale = stiff->scale;
+  }
+  if (mass) {
+    A.mass = true;
+    A.mcoef = coef_ptr(*mass, where, p->d_coef + nc, nc, c->stream);
+    A.mass_scale = mass->scale;
+  }
+  A.vals = (where == BH_DEVICE && nzval) ? nzval : p->d_vals;
+  const HostPattern& h = p->h;
+  for (int k = 0; k < h.ncolors; ++k) {
+    A.slot0 = h.color_ptr[k];
+    A.ncells_color = h.color_ptr[k + 1] - h.color_ptr[k];
+    A.cells = p->d_cells + A.slot0;
+    if (A.ncells_color == 0) continue;
+    time_begin(c);
+    BH_HIP(c, launch(m->geo, V->order, nq1, A));
+    time_end(c);
+  }
+  if (where == BH_HOST) {
+    BH_HIP(c, hipMemcpyAsync(nzval, p->d_vals, (size_t)h.nnz * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    BH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return BH_OK;
+}
+
+int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_degree, int mode,
+                       double* b, int where) {
+  if (!V || nterms <= 0 || !terms) return BH_ERR_ARG;
+  bh_mesh m = V->mesh;
+  bh_ctx c = m->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "assembly needs a mesh created with a context");
+  if (mode != BH_FP64) return fail(c, BH_ERR_UNSUPPORTED, "only BH_FP64 is implemented");
+  if (!b) return fail(c, BH_ERR_ARG, "b is NULL");
+  if (m->geo < 0) return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
+  int nq1 = gauss_points_per_axis(quad_degree);
+  if (!kernel_supported(m->geo, V->order, nq1))
+    return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this (cell type, order, quadrature)");
+  for (int k = 0; k < nterms; ++k) {
+    if (terms[k].form != BH_FORM_SOURCE) return fail(c, BH_ERR_UNSUPPORTED, "unsupported linear term");
+    if (terms[k].coef_kind != BH_COEF_CONST && !terms[k].coef)
+      return fail(c, BH_ERR_ARG, "coefficient is NULL");
+  }
+  if (int r = set_device(c)) return r;
+  double* tab = nullptr;
+  if (int r = get_tables(c, m->geo, V->order, quad_degree, &tab)) return r;
+  if (V->col.ncolors == 0)
+    if (int r = build_colouring(c, V->dofs, V->nl, V->ndofs, m->h.ncells, V->col)) return r;
+  int64_t nc = m->h.ncells;
+  int nq = nq1 * nq1 * (m->h.dim == 3 ? nq1 : 1);
+  double* db = b;
+  if (where == BH_HOST) {
+    if (!V->d_b) BH_HIP(c, dalloc(&V->d_b, (size_t)V->ndofs));
+    db = V->d_b;
+  }
+  BH_HIP(c, launch_fill(c->stream, db, V->ndofs, 0.0));
+  // assemble_linear runs the integrals of a MultiIntegration last-first (assembler.jl:496-516)
+  for (int k = nterms - 1; k >= 0; --k) {
+    const bh_term& t = terms[k];
+    LaunchArgs A;
+    A.kind = LaunchArgs::SOURCE;
+    A.stream = c->stream;
+    A.coords = m->d_coords;
+    A.c2n = m->d_c2n;
+    A.rdofs = V->d_dofs;
+    A.tab = tab;
+    A.fkind = t.coef_kind;
+    A.fscale = t.scale;
+    A.vals = db;
+    if (t.coef_kind != BH_COEF_CONST) {
+      int64_t n = t.coef_kind == BH_COEF_CELL ? nc : nc * nq;
+      if (where == BH_HOST) {
+        if (V->coef_cap < (size_t)n) {
+          (void)hipFree(V->d_coef);
+          BH_HIP(c, dalloc(&V->d_coef, (size_t)n));
+          V->coef_cap = n;
+        }
+        BH_HIP(c, hipMemcpyAsync(V->d_coef, t.coef, n * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        A.f = V->d_coef;
+      } else {
+        A.f = t.coef;
+      }
+    }
+    for (int k2 = 0; k2 < V->col.ncolors; ++k2) {
+      A.ncells_color = V->col.color_ptr[k2 + 1] - V->col.color_ptr[k2];
+      A.cells = V->col.d_cells + V->col.color_ptr[k2];
+      if (A.ncells_color == 0) continue;
+      time_begin(c);
+      BH_HIP(c, launch(m->geo, V->order, nq1, A));
+      time_end(c);
+    }
+  }
+  if (where == BH_HOST) {
+    BH_HIP(c, hipMemcpyAsync(b, db, (size_t)V->ndofs * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    BH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return BH_OK;
+}
+
+int bh_quadrature_points(bh_mesh m, int quad_degree, double* xq, int where, int32_t* nq_out) {
+  if (!m || !xq) return BH_ERR_ARG;
+  bh_ctx c = m->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "needs a mesh created with a context");
+  if (m->geo < 0) return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
+  int nq1 = gauss_points_per_axis(quad_degree);
+  int order = 1;  // any instantiated order carries the geometric tables for nq1
+  while (order <= 4 && !kernel_supported(m->geo, order, nq1)) ++order;
+  if (order > 4) return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this quadrature");
+  if (int r = set_device(c)) return r;
+  double* tab = nullptr;
+  if (int r = get_tables(c, m->geo, order, quad_degree, &tab)) return r;
+  int nq = nq1 * nq1 * (m->h.dim == 3 ? nq1 : 1);
+  int64_t n = m->h.ncells * nq * m->h.dim;
+  double* d = xq;
+  if (where == BH_HOST) BH_HIP(c, dalloc(&d, (size_t)n));
+  LaunchArgs A;
+  A.kind = LaunchArgs::QPOINTS;
+  A.stream = c->stream;
+  A.coords = m->d_coords;
+  A.c2n = m->d_c2n;
+  A.tab = tab;
+  A.ncells_total = m->h.ncells;
+  A.vals = d;
+  hipError_t e = launch(m->geo, order, nq1, A);
+  if (e == hipSuccess && where == BH_HOST) {
+    e = hipMemcpyAsync(xq, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  if (where == BH_HOST) (void)hipFree(d);
+  if (e != hipSuccess) return fail(c, BH_ERR_HIP, hipGetErrorString(e));
+  if (nq_out) *nq_out = nq;
+  return BH_OK;
+}
+
+}  // extern "C"

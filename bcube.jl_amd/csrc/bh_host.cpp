@@ -1,0 +1,573 @@
+// bh_host.cpp -- host-side setup of the assembly path: reference tables, dof
+// numbering, CSR pattern, cell colouring and the per-cell scatter map.
+//
+// Everything here runs once per mesh/space/pattern; the per-assembly work is
+// on the GPU (bh_kernels.hip).  Reference citations are relative to the
+// Bcube.jl source tree.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <unordered_map>
+
+#include "bh_internal.h"
+
+namespace bh {
+
+// ---------------------------------------------------------------------------
+// Small helpers
+// ---------------------------------------------------------------------------
+void parallel_for(i64 n, const std::function<void(i64, i64)>& body) {
+  if (n <= 0) return;
+  unsigned hw = std::thread::hardware_concurrency();
+  int nt = (int)std::min<i64>(std::max(1u, std::min(hw, 16u)), (n + 4095) / 4096);
+  if (nt <= 1) { body(0, n); return; }
+  std::vector<std::thread> th;
+  i64 chunk = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    i64 b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([&body, b, e] { body(b, e); });
+  }
+  for (auto& t : th) t.join();
+}
+
+int shape_of(int ct) {  // src/mesh/shape.jl:25-32
+  switch (ct) {
+    case BH_QUAD4: case BH_QUAD9: return SQUARE;
+    case BH_TRI3: return TRIANGLE;
+    case BH_HEXA8: case BH_HEXA27: return CUBE;
+  }
+  return -1;
+}
+int dim_of_shape(int s) { return s == CUBE ? 3 : 2; }
+int nnodes_of(int ct) {
+  switch (ct) {
+    case BH_QUAD4: return 4; case BH_QUAD9: return 9; case BH_TRI3: return 3;
+    case BH_HEXA8: return 8; case BH_HEXA27: return 27;
+  }
+  return 0;
+}
+int ndofs_lagrange(int shape, int order) {  // hypercube.jl:122-133, triangle.jl:67-70
+  if (order < 0) return 0;
+  if (shape == TRIANGLE) return (order + 1) * (order + 2) / 2;
+  int n = order + 1;
+  return shape == SQUARE ? n * n : n * n * n;
+}
+
+// ---------------------------------------------------------------------------
+// Tables: Gauss-Legendre rule, tensor Lagrange basis on uniform nodes, and the
+// Quad4/Hexa8 geometric basis.
+// ---------------------------------------------------------------------------
+int gauss_points_per_axis(int quad_degree) {  // quadrature.jl:238-240
+  if (quad_degree <= 0) quad_degree = 1;     // degree 0 uses the degree-1 rule
+  return (quad_degree + 2) / 2;
+}
+
+// Gauss-Legendre nodes/weights by Newton iteration on the three-term
+// recurrence of P_n, in long double.
+static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  for (int k = 0; k < (n + 1) / 2; ++k) {
+    long double z = cosl(3.141592653589793238462643383279502884L * (k + 0.75L) / (n + 0.5L));
+    long double dp = 0;
+    for (int it = 0; it < 64; ++it) {
+      long double p0 = 1, p1 = z;
+      for (int j = 2; j <= n; ++j) {
+        long double p2 = ((2 * j - 1) * z * p1 - (j - 1) * p0) / j;
+        p0 = p1; p1 = p2;
+      }
+      if (n == 1) { p1 = z; p0 = 1; }
+      dp = n * (z * p1 - p0) / (z * z - 1);
+      long double dz = p1 / dp;
+      z -= dz;
+      if (fabsl(dz) < 1e-20L) break;
+    }
+    long double p0 = 1, p1 = z;
+    for (int j = 2; j <= n; ++j) {
+      long double p2 = ((2 * j - 1) * z * p1 - (j - 1) * p0) / j;
+      p0 = p1; p1 = p2;
+    }
+    if (n == 1) { p1 = z; p0 = 1; }
+    dp = n * (z * p1 - p0) / (z * z - 1);
+    double wk = (double)(2.0L / ((1 - z * z) * dp * dp));
+    x[n - 1 - k] = (double)z;
+    x[k] = -(double)z;
+    w[k] = w[n - 1 - k] = wk;
+  }
+  if (n % 2 == 1) x[n / 2] = 0.0;
+}
+
+// 1D Lagrange basis on uniform nodes of [-1,1] (Lagrange{:Uniform},
+// lagrange.jl:16) and its derivative.
+static void lagrange_1d(int order, double t, double* v, double* dv) {
+  int n = order + 1;
+  if (order == 0) { v[0] = 1.0; dv[0] = 0.0; return; }
+  std::vector<double> nodes(n);
+  for (int j = 0; j < n; ++j) nodes[j] = -1.0 + 2.0 * j / order;
+  for (int j = 0; j < n; ++j) {
+    double den = 1.0, num = 1.0, der = 0.0;
+    for (int k = 0; k < n; ++k) {
+      if (k == j) continue;
+      den *= nodes[j] - nodes[k];
+      der = der * (t - nodes[k]) + num;  // product rule, accumulated
+      num *= t - nodes[k];
+    }
+    v[j] = num / den;
+    dv[j] = der / den;
+  }
+}
+
+Tables make_tables(int shape, int order, int quad_degree, int geo_ct) {
+  Tables T;
+  T.dim = dim_of_shape(shape);
+  T.order = order;
+  T.nl = ndofs_lagrange(shape, order);
+  int d = T.dim;
+  int n1 = gauss_points_per_axis(quad_degree);
+  std::vector<double> gx, gw;
+  gauss_legendre(n1, gx, gw);
+  T.nq = d == 2 ? n1 * n1 : n1 * n1 * n1;
+  // tensor rule, first axis fastest (quadrature.jl:455-487)
+  for (int q = 0; q < T.nq; ++q) {
+    int i = q % n1, j = (q / n1) % n1, k = q / (n1 * n1);
+    double w = gw[i] * gw[j];
+    if (d == 3) w *= gw[k];
+    T.w.push_back(w);
+    T.xi.push_back(gx[i]);
+    T.xi.push_back(gx[j]);
+    if (d == 3) T.xi.push_back(gx[k]);
+  }
+  // tensor Lagrange basis, dof index first-axis fastest (hypercube.jl:56-92)
+  int n = order + 1;
+  T.phi.assign((size_t)T.nq * T.nl, 0.0);
+  T.dphi.assign((size_t)T.nq * T.nl * d, 0.0);
+  std::vector<double> v[3], dv[3];
+  for (int a = 0; a < 3; ++a) { v[a].resize(n); dv[a].resize(n); }
+  for (int q = 0; q < T.nq; ++q) {
+    for (int a = 0; a < d; ++a) lagrange_1d(order, T.xi[q * d + a], v[a].data(), dv[a].data());
+    for (int l = 0; l < T.nl; ++l) {
+      int i = l % n, j = (l / n) % n, k = l / (n * n);
+      double* g = &T.dphi[((size_t)q * T.nl + l) * d];
+      if (d == 2) {
+        T.phi[(size_t)q * T.nl + l] = v[0][i] * v[1][j];
+        g[0] = dv[0][i] * v[1][j];
+        g[1] = v[0][i] * dv[1][j];
+      } else {
+        T.phi[(size_t)q * T.nl + l] = v[0][i] * v[1][j] * v[2][k];
+        g[0] = dv[0][i] * v[1][j] * v[2][k];
+        g[1] = v[0][i] * dv[1][j] * v[2][k];
+        g[2] = v[0][i] * v[1][j] * dv[2][k];
+      }
+    }
+  }
+  // geometric basis: bilinear / trilinear in CGNS vertex order (mapping.jl:460-465, 618-639)
+  static const int q4[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+  static const int h8[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                               {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  T.ngeo = geo_ct == BH_HEXA8 ? 8 : (geo_ct == BH_QUAD4 ? 4 : 0);
+  T.gphi.assign((size_t)T.nq * T.ngeo, 0.0);
+  T.dgphi.assign((size_t)T.nq * T.ngeo * d, 0.0);
+  for (int q = 0; q < T.nq; ++q) {
+    const double* x = &T.xi[q * d];
+    for (int k = 0; k < T.ngeo; ++k) {
+      double s[3], f[3];
+      for (int a = 0; a < d; ++a) {
+        s[a] = d == 3 ? h8[k][a] : q4[k][a];
+        f[a] = 0.5 * (1.0 + s[a] * x[a]);
+      }
+      double val = 1.0;
+      for (int a = 0; a < d; ++a) val *= f[a];
+      T.gphi[(size_t)q * T.ngeo + k] = val;
+      for (int a = 0; a < d; ++a) {
+        double g = 0.5 * s[a];
+        for (int b = 0; b < d; ++b)
+          if (b != a) g *= f[b];
+        T.dgphi[((size_t)q * T.ngeo + k) * d + a] = g;
+      }
+    }
+  }
+  return T;
+}
+
+// ---------------------------------------------------------------------------
+// Entity tables (src/mesh/entity.jl, CGNS order) and Lagrange dof locations
+// (hypercube.jl:136-331, triangle.jl:72-98); all 0-based.
+// ---------------------------------------------------------------------------
+typedef std::vector<std::vector<int>> Lists;
+
+static Lists edges_of(int ct) {
+  switch (ct) {
+    case BH_QUAD4: return {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+    case BH_QUAD9: return {{0, 1, 4}, {1, 2, 5}, {2, 3, 6}, {3, 0, 7}};
+    case BH_TRI3: return {{0, 1}, {1, 2}, {2, 0}};
+    case BH_HEXA8: return {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 5},
+                           {2, 6}, {3, 7}, {4, 5}, {5, 6}, {6, 7}, {7, 4}};
+    case BH_HEXA27: return {{0, 1, 8}, {1, 2, 9}, {2, 3, 10}, {3, 0, 11},
+                            {0, 4, 12}, {1, 5, 13}, {2, 6, 14}, {3, 7, 15},
+                            {4, 5, 16}, {5, 6, 17}, {6, 7, 18}, {7, 4, 19}};
+  }
+  return {};
+}
+static Lists faces_of(int ct) {
+  switch (ct) {
+    case BH_QUAD4: case BH_QUAD9: case BH_TRI3: return edges_of(ct);
+    case BH_HEXA8: return {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                           {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}};
+    case BH_HEXA27: return {{0, 3, 2, 1, 11, 10, 9, 8, 20}, {0, 1, 5, 4, 8, 13, 16, 12, 21},
+                            {1, 2, 6, 5, 9, 14, 17, 13, 22}, {2, 3, 7, 6, 10, 15, 18, 14, 23},
+                            {0, 4, 7, 3, 12, 19, 15, 11, 24}, {4, 5, 6, 7, 16, 17, 18, 19, 25}};
+  }
+  return {};
+}
+
+static Lists vertex_dofs(int shape, int order) {
+  int nv = shape == CUBE ? 8 : (shape == SQUARE ? 4 : 3);
+  if (order == 0) return Lists(nv);
+  if (shape == TRIANGLE) return {{0}, {1}, {2}};
+  int n = order + 1, m = n - 1;
+  if (shape == SQUARE) return {{0}, {m}, {m + n * m}, {n * m}};
+  int nn = n * n;
+  return {{0}, {m}, {m + n * m}, {n * m},
+          {nn * m}, {m + nn * m}, {m + n * m + nn * m}, {n * m + nn * m}};
+}
+
+static Lists edge_dofs(int shape, int order) {
+  if (shape == TRIANGLE) {
+    if (order <= 1) return Lists(3);
+    if (order == 2) return {{3}, {4}, {5}};
+    if (order == 3) return {{3, 4}, {5, 6}, {7, 8}};
+    return Lists();  // no reference basis beyond P3
+  }
+  int ne = shape == CUBE ? 12 : 4;
+  if (order == 0) return Lists(ne);
+  int n = order + 1;
+  auto I = [&](int i, int j, int k) { return i + n * j + n * n * k; };  // 0-based axes
+  Lists e(ne);
+  int L = n - 1;
+  for (int t = 1; t <= n - 2; ++t) {
+    int r = t, rr = n - 1 - t;  // forward and reversed interior index
+    if (shape == SQUARE) {
+      e[0].push_back(I(r, 0, 0));
+      e[1].push_back(I(L, r, 0));
+      e[2].push_back(I(rr, L, 0));
+      e[3].push_back(I(0, rr, 0));
+    } else {
+      e[0].push_back(I(r, 0, 0));
+      e[1].push_back(I(L, r, 0));
+      e[2].push_back(I(rr, L, 0));
+      e[3].push_back(I(0, rr, 0));
+      e[4].push_back(I(0, 0, r));
+      e[5].push_back(I(L, 0, r));
+      e[6].push_back(I(L, L, r));
+      e[7].push_back(I(0, L, r));
+      e[8].push_back(I(r, 0, L));
+      e[9].push_back(I(L, r, L));
+      e[10].push_back(I(rr, L, L));
+      e[11].push_back(I(0, rr, L));
+    }
+  }
+  return e;
+}
+
+static Lists face_dofs(int shape, int order) {
+  if (shape != CUBE) return Lists();
+  if (order == 0) return Lists(6);
+  int n = order + 1, L = n - 1;
+  auto I = [&](int i, int j, int k) { return i + n * j + n * n * k; };
+  Lists f(6);
+  for (int b = 1; b <= n - 2; ++b)
+    for (int a = 1; a <= n - 2; ++a) {  // column-major vec(): first range fastest
+      f[0].push_back(I(a, b, 0));
+      f[1].push_back(I(a, 0, b));
+      f[2].push_back(I(L, a, b));
+      f[3].push_back(I(a, L, b));
+      f[4].push_back(I(0, a, b));
+      f[5].push_back(I(a, b, L));
+    }
+  return f;
+}
+
+// oriented_cell_side (entity.jl:487-502): index of the face of cell (ct, nodes)
+// whose node set is contained in `fn`, negative if traversed in the other
+// direction, 0 if none.
+static int oriented_cell_side(int ct, const i64* nodes, const std::vector<i64>& fn) {
+  Lists fl = faces_of(ct);
+  for (size_t s = 0; s < fl.size(); ++s) {
+    bool sub = true;
+    for (int ln : fl[s])
+      if (std::find(fn.begin(), fn.end(), nodes[ln]) == fn.end()) { sub = false; break; }
+    if (!sub) continue;
+    int side = (int)s + 1, n1 = 0;
+    for (size_t q = 0; q < fl[s].size(); ++q)
+      if (nodes[fl[s][q]] == fn[0]) { n1 = (int)q + 1; break; }
+    if (fn.size() == 2) return n1 == 2 ? -side : side;
+    int n2 = n1 + 1 > (int)fl[s].size() ? 1 : n1 + 1;
+    if (fn[1] != nodes[fl[s][n2 - 1]]) side = -side;
+    return side;
+  }
+  return 0;
+}
+
+struct Key9 {
+  std::array<i64, 9> k;
+  bool operator==(const Key9& o) const { return k == o.k; }
+};
+struct Key9Hash {
+  size_t operator()(const Key9& x) const {
+    uint64_t h = 1469598103934665603ull;
+    for (i64 v : x.k) { h ^= (uint64_t)v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2); }
+    return (size_t)h;
+  }
+};
+static Key9 make_key(const std::vector<i64>& nodes) {
+  Key9 key;
+  key.k.fill(-1);
+  std::vector<i64> s = nodes;
+  std::sort(s.begin(), s.end());
+  for (size_t i = 0; i < s.size() && i < 9; ++i) key.k[i] = s[i];
+  return key;
+}
+
+i64 dofhandler(const HostMesh& m, int order, bool continuous, std::vector<i64>& iglob,
+               std::vector<i64>& offset, std::string& err) {
+  i64 nc = m.ncells;
+  offset.assign(nc, 0);
+  i64 tot = 0;
+  for (i64 c = 0; c < nc; ++c) {
+    offset[c] = tot;
+    tot += ndofs_lagrange(shape_of(m.ctype[c]), order);
+  }
+  iglob.resize(tot);
+  for (i64 i = 0; i < tot; ++i) iglob[i] = i + 1;
+  if (continuous) {
+    // vertex: one record per node (first cell's global ids)
+    std::vector<i64> vfirst_ptr(m.nnodes, -1);  // index into vstore
+    std::vector<i64> vstore;
+    std::unordered_map<Key9, std::pair<i64, std::vector<i64>>, Key9Hash> dict_e, dict_f;
+    dict_e.reserve((size_t)nc * 4);
+    if (m.dim == 3) dict_f.reserve((size_t)nc * 4);
+    for (i64 c = 0; c < nc; ++c) {
+      int ct = m.ctype[c];
+      int s = shape_of(ct);
+      const i64* nodes = &m.c2n[m.c2n_ptr[c]];
+      int nn = (int)(m.c2n_ptr[c + 1] - m.c2n_ptr[c]);
+      i64* ig = &iglob[offset[c]];
+      Lists vd = vertex_dofs(s, order);
+      for (size_t v = 0; v < vd.size() && (int)v < nn; ++v) {
+        if (vd[v].empty()) continue;
+        i64 node = nodes[v];
+        if (vfirst_ptr[node] >= 0) {
+          for (size_t d = 0; d < vd[v].size(); ++d) ig[vd[v][d]] = vstore[vfirst_ptr[node] + d];
+        } else {
+          vfirst_ptr[node] = (i64)vstore.size();
+          for (int ld : vd[v]) vstore.push_back(ig[ld]);
+        }
+      }
+      if (dim_of_shape(s) > 1) {
+        Lists el = edges_of(ct);
+        Lists ed = edge_dofs(s, order);
+        if (order > 0 && ed.size() != el.size() && s == TRIANGLE) {
+          err = "triangle Lagrange order > 3 has no reference basis";
+          return -1;
+        }
+        for (size_t e = 0; e < el.size() && e < ed.size(); ++e) {
+          if (ed[e].empty()) continue;
+          std::vector<i64> en;
+          for (int ln : el[e]) en.push_back(nodes[ln]);
+          Key9 key = make_key(en);
+          auto it = dict_e.find(key);
+          if (it != dict_e.end()) {
+            i64 jc = it->second.first;
+            const std::vector<i64>& jd = it->second.second;
+            int jside = oriented_cell_side(m.ctype[jc], &m.c2n[m.c2n_ptr[jc]], en);
+            for (size_t d = 0; d < jd.size(); ++d)
+              ig[ed[e][d]] = jside > 0 ? jd[d] : jd[jd.size() - 1 - d];
+          } else {
+            std::vector<i64> g;
+            for (int ld : ed[e]) g.push_back(ig[ld]);
+            dict_e.emplace(key, std::make_pair(c, std::move(g)));
+          }
+        }
+      }
+      if (dim_of_shape(s) > 2) {
+        Lists fl = faces_of(ct);
+        Lists fd = face_dofs(s, order);
+        for (size_t f = 0; f < fl.size() && f < fd.size(); ++f) {
+          if (fd[f].empty()) continue;
+          std::vector<i64> fn;
+          for (int ln : fl[f]) fn.push_back(nodes[ln]);
+          Key9 key = make_key(fn);
+          auto it = dict_f.find(key);
+          if (it != dict_f.end()) {
+            const std::vector<i64>& jd = it->second.second;
+            if (jd.size() / 12 > 0) {
+              err = "continuous hexa Lagrange order >= 4: face-dof permutation not supported";
+              return -1;
+            }
+            for (size_t d = 0; d < jd.size(); ++d) ig[fd[f][d]] = jd[d];
+          } else {
+            std::vector<i64> g;
+            for (int ld : fd[f]) g.push_back(ig[ld]);
+            dict_f.emplace(key, std::make_pair(c, std::move(g)));
+          }
+        }
+      }
+    }
+  }
+  // densify: first-appearance renumbering (utils.jl:7-16)
+  std::vector<i64> remap(tot + 1, 0);
+  i64 next = 0;
+  for (i64 i = 0; i < tot; ++i) {
+    i64 v = iglob[i];
+    if (remap[v] == 0) remap[v] = ++next;
+    iglob[i] = remap[v];
+  }
+  return next;
+}
+
+// ---------------------------------------------------------------------------
+// Cell colouring: greedy, cells in index order, smallest colour not used by a
+// cell sharing a row dof.  On a structured hexa grid this is the 8-colour
+// parity pattern.
+// ---------------------------------------------------------------------------
+int colour_cells(const std::vector<int32_t>& rdofs, int nl, i64 nrows, i64 ncells,
+                 std::vector<int32_t>& color_ptr, std::vector<int32_t>& cells_by_color) {
+  std::vector<i64> aptr(nrows + 1, 0);
+  for (i64 c = 0; c < ncells; ++c)
+    for (int i = 0; i < nl; ++i) aptr[rdofs[c * nl + i] + 1]++;
+  for (i64 r = 0; r < nrows; ++r) aptr[r + 1] += aptr[r];
+  std::vector<int32_t> adj(aptr[nrows]);
+  std::vector<i64> fill(aptr.begin(), aptr.end() - 1);
+  for (i64 c = 0; c < ncells; ++c)
+    for (int i = 0; i < nl; ++i) adj[fill[rdofs[c * nl + i]]++] = (int32_t)c;
+  std::vector<int32_t> color(ncells, -1);
+  int ncol = 0;
+  std::vector<uint8_t> used(256, 0);
+  for (i64 c = 0; c < ncells; ++c) {
+    std::fill(used.begin(), used.begin() + std::max(ncol, 1), 0);
+    for (int i = 0; i < nl; ++i) {
+      i64 r = rdofs[c * nl + i];
+      for (i64 p = aptr[r]; p < aptr[r + 1]; ++p) {
+        int32_t o = adj[p];
+        if (o >= c) break;  // adjacency lists are in cell order
+        used[color[o]] = 1;
+      }
+    }
+    int k = 0;
+    while (k < 255 && used[k]) ++k;
+    if (k >= 255) return -1;
+    color[c] = k;
+    ncol = std::max(ncol, k + 1);
+  }
+  color_ptr.assign(ncol + 1, 0);
+  for (i64 c = 0; c < ncells; ++c) color_ptr[color[c] + 1]++;
+  for (int k = 0; k < ncol; ++k) color_ptr[k + 1] += color_ptr[k];
+  cells_by_color.resize(ncells);
+  std::vector<int32_t> pos(color_ptr.begin(), color_ptr.end() - 1);
+  for (i64 c = 0; c < ncells; ++c) cells_by_color[pos[color[c]]++] = (int32_t)c;
+  return ncol;
+}
+
+// ---------------------------------------------------------------------------
+// CSR pattern of the element dof-pair graph, rows = test dofs, columns = trial
+// dofs, columns sorted and unique per row (the structure sparse() produces,
+// assembler.jl:185), plus the colour-ordered scatter map.
+// ---------------------------------------------------------------------------
+bool build_pattern(const std::vector<int32_t>& rdofs, int nl_row, i64 nrows,
+                   const std::vector<int32_t>& cdofs, int nl_col, i64 ncols, i64 ncells,
+                   HostPattern& p, std::string& err) {
+  p.nrows = nrows; p.ncols = ncols; p.nl_row = nl_row; p.nl_col = nl_col;
+  for (i64 k = 0; k < ncells * nl_row; ++k)
+    if (rdofs[k] < 0 || rdofs[k] >= nrows) { err = "row dof out of range"; return false; }
+  for (i64 k = 0; k < ncells * nl_col; ++k)
+    if (cdofs[k] < 0 || cdofs[k] >= ncols) { err = "column dof out of range"; return false; }
+  // row -> cells adjacency
+  std::vector<i64> aptr(nrows + 1, 0);
+  for (i64 c = 0; c < ncells; ++c)
+    for (int i = 0; i < nl_row; ++i) aptr[rdofs[c * nl_row + i] + 1]++;
+  for (i64 r = 0; r < nrows; ++r) aptr[r + 1] += aptr[r];
+  std::vector<int32_t> adj(aptr[nrows]);
+  {
+    std::vector<i64> fill(aptr.begin(), aptr.end() - 1);
+    for (i64 c = 0; c < ncells; ++c)
+      for (int i = 0; i < nl_row; ++i) adj[fill[rdofs[c * nl_row + i]]++] = (int32_t)c;
+  }
+  // pass 1: row lengths
+  std::vector<i64> len(nrows, 0);
+  parallel_for(nrows, [&](i64 b, i64 e) {
+    std::vector<int32_t> buf;
+    for (i64 r = b; r < e; ++r) {
+      buf.clear();
+      for (i64 q = aptr[r]; q < aptr[r + 1]; ++q) {
+        i64 c = adj[q];
+        for (int j = 0; j < nl_col; ++j) buf.push_back(cdofs[c * nl_col + j]);
+      }
+      std::sort(buf.begin(), buf.end());
+      len[r] = std::unique(buf.begin(), buf.end()) - buf.begin();
+    }
+  });
+  p.rowptr.assign(nrows + 1, 0);
+  for (i64 r = 0; r < nrows; ++r) p.rowptr[r + 1] = p.rowptr[r] + len[r];
+  p.nnz = p.rowptr[nrows];
+  if (p.nnz >= (i64)INT32_MAX) { err = "nnz exceeds 2^31-1"; return false; }
+  p.colval.resize(p.nnz);
+  p.max_row = 0;
+  for (i64 r = 0; r < nrows; ++r) p.max_row = std::max<int>(p.max_row, (int)len[r]);
+  parallel_for(nrows, [&](i64 b, i64 e) {
+    std::vector<int32_t> buf;
+    for (i64 r = b; r < e; ++r) {
+      buf.clear();
+      for (i64 q = aptr[r]; q < aptr[r + 1]; ++q) {
+        i64 c = adj[q];
+        for (int j = 0; j < nl_col; ++j) buf.push_back(cdofs[c * nl_col + j]);
+      }
+      std::sort(buf.begin(), buf.end());
+      auto end = std::unique(buf.begin(), buf.end());
+      std::copy(buf.begin(), end, p.colval.begin() + p.rowptr[r]);
+    }
+  });
+  // colouring on row dofs
+  p.ncolors = colour_cells(rdofs, nl_row, nrows, ncells, p.color_ptr, p.cells_by_color);
+  if (p.ncolors < 0) { err = "colouring needs more than 255 colours"; return false; }
+  // scatter map with first-touch flags (colours are launched in order)
+  if (p.max_row <= 128) p.offset_bytes = 1;
+  else if (p.max_row <= 32768) p.offset_bytes = 2;
+  else { err = "row longer than 32768 entries"; return false; }
+  i64 per = (i64)nl_row * nl_col;
+  p.scatter.assign((size_t)(ncells * per * p.offset_bytes), 0);
+  std::vector<uint8_t> seen(p.nnz, 0);
+  for (int k = 0; k < p.ncolors; ++k) {
+    i64 s0 = p.color_ptr[k], s1 = p.color_ptr[k + 1];
+    parallel_for(s1 - s0, [&](i64 b, i64 e) {
+      for (i64 s = s0 + b; s < s0 + e; ++s) {
+        i64 c = p.cells_by_color[s];
+        for (int i = 0; i < nl_row; ++i) {
+          i64 r = rdofs[c * nl_row + i];
+          const int32_t* row = &p.colval[p.rowptr[r]];
+          i64 rl = p.rowptr[r + 1] - p.rowptr[r];
+          for (int j = 0; j < nl_col; ++j) {
+            int32_t col = cdofs[c * nl_col + j];
+            i64 off = std::lower_bound(row, row + rl, col) - row;
+            i64 g = p.rowptr[r] + off;
+            bool first = !seen[g];
+            seen[g] = 1;
+            size_t idx = (size_t)(s * per + (i64)j * nl_row + i);
+            if (p.offset_bytes == 1) {
+              p.scatter[idx] = (uint8_t)(off | (first ? 0x80 : 0));
+            } else {
+              uint16_t v = (uint16_t)(off | (first ? 0x8000 : 0));
+              std::memcpy(&p.scatter[idx * 2], &v, 2);
+            }
+          }
+        }
+      }
+    });
+  }
+  return true;
+}
+
+}  // namespace bh

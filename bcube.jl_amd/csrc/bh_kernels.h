@@ -1,0 +1,46 @@
+// bh_kernels.h -- launcher interface between the ABI layer and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/bcubehip.h"
+
+namespace bh {
+
+struct LaunchArgs {
+  enum Kind { BILINEAR = 0, SOURCE = 1, QPOINTS = 2 };
+  int kind = BILINEAR;
+  hipStream_t stream = nullptr;
+  const double* coords = nullptr;
+  const int32_t* c2n = nullptr;    // ncells * nodes-per-cell (geometric nodes)
+  const int32_t* rdofs = nullptr;  // ncells * nl
+  const double* tab = nullptr;     // packed reference tables
+  const int32_t* cells = nullptr;  // cells of this colour
+  int ncells_color = 0;
+  int slot0 = 0;                   // slot of cells[0] in the scatter map
+  int64_t ncells_total = 0;
+  // bilinear
+  const double* eta = nullptr;
+  double eta_scale = 0.0;
+  const double* mcoef = nullptr;
+  double mass_scale = 0.0;
+  bool mass = false;
+  const int64_t* rowptr = nullptr;
+  const void* scatter = nullptr;
+  int offset_bytes = 1;
+  // source
+  int fkind = BH_COEF_CONST;
+  const double* f = nullptr;
+  double fscale = 0.0;
+  // output (CSR values, vector b, or quadrature points)
+  double* vals = nullptr;
+};
+
+bool kernel_supported(int geo_celltype, int order, int nq1);
+int packed_table_size(int geo_celltype, int order, int nq1);
+hipError_t launch(int geo_celltype, int order, int nq1, const LaunchArgs& a);
+hipError_t launch_fill(hipStream_t s, double* p, int64_t n, double v);
+
+}  // namespace bh

@@ -1,0 +1,392 @@
+// bh_kernels.hip -- gfx950 kernels of the assembly path.
+//
+// One launch per cell colour.  A workgroup takes groups of CPB cells of that
+// colour (grid-stride), and for each group:
+//   1. stages the cells' geometric nodes in LDS,
+//   2. evaluates J, |J|, J^-1 at every quadrature point in FP64 and folds the
+//      integrand coefficients into per-point factors
+//         G_q = w_q |J_q| η J_q^-1 J_q^-T   (stiffness, symmetric D x D)
+//         m_q = w_q |J_q| c                 (mass)
+//   3. computes one row i of the element matrix per lane,
+//         K_ij = Σ_q ∇̂φ_qi^T G_q ∇̂φ_qj + m_q φ_qi φ_qj,
+//      with the reference tables (∇̂φ, φ) staged once per workgroup in LDS,
+//   4. adds the row into the CSR values through the colour-ordered scatter
+//      map: no two cells of one colour share a row, and colours run in launch
+//      order, so every entry is summed in a fixed order (no atomics).  The
+//      first colour to touch an entry stores instead of adding, so the values
+//      need no zero fill.
+// This replaces the per-cell lazy-operator loop of the reference
+// (__assemble_bilinear! assembler.jl:280-307, integrate_on_ref_element
+// integration.jl:18-105, ∂λξ_∂x ref2phys.jl:22-26, _append_bilinear!
+// assembler.jl:632-648 + sparse()).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bh_kernels.h"
+
+namespace bh {
+
+constexpr int kBlock = 256;
+
+template <int D>
+__device__ __forceinline__ double invert(const double (&J)[D][D], double (&Ji)[D][D]) {
+  if constexpr (D == 2) {
+    double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    double r = 1.0 / det;
+    Ji[0][0] = J[1][1] * r;
+    Ji[0][1] = -J[0][1] * r;
+    Ji[1][0] = -J[1][0] * r;
+    Ji[1][1] = J[0][0] * r;
+    return det;
+  } else {
+    double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    double r = 1.0 / det;
+    Ji[0][0] = c00 * r;
+    Ji[1][0] = c01 * r;
+    Ji[2][0] = c02 * r;
+    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
+    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
+    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
+    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
+    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
+    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
+    return det;
+  }
+}
+
+// Packed table layout (doubles), built by pack_tables():
+//   [0, NQ)                         w_q
+//   [NQ, NQ + NQ*NL*4)              per (q, l): ∇̂φ (D values) then φ, padded to 4
+//   [.., + NQ*NG*D)                 geometric basis gradients ∂N_k/∂ξ
+//   [.., + NQ*NG)                   geometric basis values N_k
+template <int D, int NL, int NQ, int NG>
+struct TabLayout {
+  static constexpr int W = 0;
+  static constexpr int R = NQ;
+  static constexpr int DG = R + NQ * NL * 4;
+  static constexpr int G = DG + NQ * NG * D;
+  static constexpr int SIZE = G + NQ * NG;
+};
+
+template <int D>
+struct SymN {
+  static constexpr int N = D * (D + 1) / 2;
+};
+
+// -------------------------------------------------------------------------
+// Bilinear FP64 kernel
+// -------------------------------------------------------------------------
+template <int D, int NL, int NQ, int NG, bool MASS, typename OFF>
+__global__ __launch_bounds__(kBlock) void k_bilinear_f64(
+    const double* __restrict__ coords, const int32_t* __restrict__ c2n,
+    const int32_t* __restrict__ rdofs, const double* __restrict__ tab,
+    const int32_t* __restrict__ cells, int ncells_color, int slot0,
+    const double* __restrict__ eta, double eta_scale, const double* __restrict__ mcoef,
+    double mass_scale, const int64_t* __restrict__ rowptr, const OFF* __restrict__ scatter,
+    double* __restrict__ vals) {
+  using L = TabLayout<D, NL, NQ, NG>;
+  constexpr int CPB = kBlock / NL;  // cells per group
+  constexpr int NS = SymN<D>::N;
+  constexpr OFF FLAG = (OFF)(sizeof(OFF) == 1 ? 0x80 : 0x8000);
+  constexpr OFF MASK = (OFF)(FLAG - 1);
+
+  __shared__ double s_tab[L::SIZE];
+  __shared__ double s_x[CPB][NG][D];
+  __shared__ double s_g[CPB][NQ][NS + 1];
+
+  const int tid = threadIdx.x;
+  for (int k = tid; k < L::SIZE; k += kBlock) s_tab[k] = tab[k];
+
+  const int ngroups = (ncells_color + CPB - 1) / CPB;
+  const int cs = tid / NL;  // cell slot of this lane in phase 3/4
+  const int li = tid - cs * NL;
+
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int base = grp * CPB;
+    __syncthreads();  // tables ready / previous group done with s_x, s_g
+    // 1. stage nodes
+    for (int k = tid; k < CPB * NG; k += kBlock) {
+      int c = k / NG, n = k - c * NG;
+      if (base + c < ncells_color) {
+        int cell = cells[base + c];
+        int node = c2n[(int64_t)cell * NG + n];
+#pragma unroll
+        for (int a = 0; a < D; ++a) s_x[c][n][a] = coords[(int64_t)node * D + a];
+      }
+    }
+    __syncthreads();
+    // 2. geometry per (cell, q)
+    for (int k = tid; k < CPB * NQ; k += kBlock) {
+      int c = k / NQ, q = k - c * NQ;
+      if (base + c >= ncells_color) continue;
+      int cell = cells[base + c];
+      double J[D][D];
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) J[a][b] = 0.0;
+#pragma unroll
+      for (int n = 0; n < NG; ++n)
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+          double dn = s_tab[L::DG + (q * NG + n) * D + b];
+#pragma unroll
+          for (int a = 0; a < D; ++a) J[a][b] += s_x[c][n][a] * dn;
+        }
+      double Ji[D][D];
+      double det = invert<D>(J, Ji);
+      double wdet = s_tab[L::W + q] * fabs(det);
+      double e = eta ? eta[cell] * eta_scale : eta_scale;
+      double ge = wdet * e;
+      int s = 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = a; b < D; ++b) {
+          double v = 0.0;
+#pragma unroll
+          for (int t = 0; t < D; ++t) v += Ji[a][t] * Ji[b][t];
+          s_g[c][q][s++] = ge * v;
+        }
+      double mc = mcoef ? mcoef[cell] * mass_scale : mass_scale;
+      s_g[c][q][NS] = MASS ? wdet * mc : 0.0;
+    }
+    __syncthreads();
+    // 3. one element-matrix row per lane
+    const bool active = (cs < CPB) && (base + cs < ncells_color);
+    if (!active) continue;
+    double acc[NL];
+#pragma unroll
+    for (int j = 0; j < NL; ++j) acc[j] = 0.0;
+#pragma unroll 1
+    for (int q = 0; q < NQ; ++q) {
+      const double* g = s_g[cs][q];
+      const double* ri = &s_tab[L::R + (q * NL + li) * 4];
+      double t[D];
+      if constexpr (D == 2) {
+        t[0] = g[0] * ri[0] + g[1] * ri[1];
+        t[1] = g[1] * ri[0] + g[2] * ri[1];
+      } else {
+        t[0] = g[0] * ri[0] + g[1] * ri[1] + g[2] * ri[2];
+        t[1] = g[1] * ri[0] + g[3] * ri[1] + g[4] * ri[2];
+        t[2] = g[2] * ri[0] + g[4] * ri[1] + g[5] * ri[2];
+      }
+      const double mp = MASS ? g[NS] * ri[D] : 0.0;
+      const double* rq = &s_tab[L::R + q * NL * 4];
+#pragma unroll
+      for (int j = 0; j < NL; ++j) {
+        double v = acc[j];
+#pragma unroll
+        for (int a = 0; a < D; ++a) v = fma(rq[j * 4 + a], t[a], v);
+        if (MASS) v = fma(rq[j * 4 + D], mp, v);
+        acc[j] = v;
+      }
+    }
+    // 4. colour-ordered scatter into the CSR row of test dof i
+    const int cell = cells[base + cs];
+    const int row = rdofs[(int64_t)cell * NL + li];
+    const int64_t rs = rowptr[row];
+    const OFF* sc = scatter + ((int64_t)(slot0 + base + cs)) * (NL * NL) + li;
+#pragma unroll
+    for (int j = 0; j < NL; ++j) {
+      OFF b = sc[j * NL];
+      int64_t pos = rs + (int64_t)(b & MASK);
+      if (b & FLAG) vals[pos] = acc[j];
+      else vals[pos] += acc[j];
+    }
+  }
+}
+
+// -------------------------------------------------------------------------
+// Linear form ∫ f v dΩ (FP64): b_i += Σ_q w_q |J_q| f_q φ_qi, colour-ordered.
+// -------------------------------------------------------------------------
+template <int D, int NL, int NQ, int NG>
+__global__ __launch_bounds__(kBlock) void k_source_f64(
+    const double* __restrict__ coords, const int32_t* __restrict__ c2n,
+    const int32_t* __restrict__ rdofs, const double* __restrict__ tab,
+    const int32_t* __restrict__ cells, int ncells_color, int fkind,
+    const double* __restrict__ f, double fscale, double* __restrict__ b) {
+  using L = TabLayout<D, NL, NQ, NG>;
+  constexpr int CPB = kBlock / NL;
+  __shared__ double s_tab[L::SIZE];
+  __shared__ double s_x[CPB][NG][D];
+  __shared__ double s_m[CPB][NQ];
+  const int tid = threadIdx.x;
+  for (int k = tid; k < L::SIZE; k += kBlock) s_tab[k] = tab[k];
+  const int ngroups = (ncells_color + CPB - 1) / CPB;
+  const int cs = tid / NL, li = tid - cs * NL;
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int base = grp * CPB;
+    __syncthreads();
+    for (int k = tid; k < CPB * NG; k += kBlock) {
+      int c = k / NG, n = k - c * NG;
+      if (base + c < ncells_color) {
+        int node = c2n[(int64_t)cells[base + c] * NG + n];
+        for (int a = 0; a < D; ++a) s_x[c][n][a] = coords[(int64_t)node * D + a];
+      }
+    }
+    __syncthreads();
+    for (int k = tid; k < CPB * NQ; k += kBlock) {
+      int c = k / NQ, q = k - c * NQ;
+      if (base + c >= ncells_color) continue;
+      int cell = cells[base + c];
+      double J[D][D];
+      for (int a = 0; a < D; ++a)
+        for (int bb = 0; bb < D; ++bb) J[a][bb] = 0.0;
+      for (int n = 0; n < NG; ++n)
+        for (int bb = 0; bb < D; ++bb) {
+          double dn = s_tab[L::DG + (q * NG + n) * D + bb];
+          for (int a = 0; a < D; ++a) J[a][bb] += s_x[c][n][a] * dn;
+        }
+      double Ji[D][D];
+      double det = invert<D>(J, Ji);
+      double fv = fkind == BH_COEF_CONST ? fscale
+                  : fkind == BH_COEF_CELL ? f[cell] * fscale
+                                          : f[(int64_t)cell * NQ + q] * fscale;
+      s_m[c][q] = s_tab[L::W + q] * fabs(det) * fv;
+    }
+    __syncthreads();
+    if (cs >= CPB || base + cs >= ncells_color) continue;
+    double acc = 0.0;
+    for (int q = 0; q < NQ; ++q) acc = fma(s_m[cs][q], s_tab[L::R + (q * NL + li) * 4 + D], acc);
+    const int cell = cells[base + cs];
+    b[rdofs[(int64_t)cell * NL + li]] += acc;
+  }
+}
+
+// Physical quadrature points x_q = Σ_k N_k(ξ_q) X_k (eval_point.jl:107-111).
+template <int D, int NL, int NQ, int NG>
+__global__ __launch_bounds__(kBlock) void k_qpoints(const double* __restrict__ coords,
+                                                    const int32_t* __restrict__ c2n,
+                                                    const double* __restrict__ tab, int64_t ncells,
+                                                    double* __restrict__ xq) {
+  using L = TabLayout<D, NL, NQ, NG>;
+  int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= ncells * NQ) return;
+  int64_t cell = k / NQ;
+  int q = (int)(k - cell * NQ);
+  double x[D];
+  for (int a = 0; a < D; ++a) x[a] = 0.0;
+  for (int n = 0; n < NG; ++n) {
+    int node = c2n[cell * NG + n];
+    double N = tab[L::G + q * NG + n];
+    for (int a = 0; a < D; ++a) x[a] += N * coords[(int64_t)node * D + a];
+  }
+  for (int a = 0; a < D; ++a) xq[k * D + a] = x[a];
+}
+
+__global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// -------------------------------------------------------------------------
+// Host launchers and the (shape, order, points) dispatch table
+// -------------------------------------------------------------------------
+static int grid_for(int ngroups) {
+  // enough workgroups to fill 256 CUs several times, grid-stride beyond that
+  int g = ngroups < 2048 ? ngroups : 2048;
+  return g < 1 ? 1 : g;
+}
+
+template <int D, int NL, int NQ, int NG>
+static hipError_t launch_all(const LaunchArgs& A) {
+  constexpr int CPB = kBlock / NL;
+  if (A.kind == LaunchArgs::BILINEAR) {
+    int ng = (A.ncells_color + CPB - 1) / CPB;
+    dim3 grid(grid_for(ng));
+    if (A.offset_bytes == 1) {
+      auto sc = (const uint8_t*)A.scatter;
+      if (A.mass)
+        hipLaunchKernelGGL((k_bilinear_f64<D, NL, NQ, NG, true, uint8_t>), grid, dim3(kBlock), 0,
+                           A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.ncells_color,
+                           A.slot0, A.eta, A.eta_scale, A.mcoef, A.mass_scale, A.rowptr, sc,
+                           A.vals);
+      else
+        hipLaunchKernelGGL((k_bilinear_f64<D, NL, NQ, NG, false, uint8_t>), grid, dim3(kBlock), 0,
+                           A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.ncells_color,
+                           A.slot0, A.eta, A.eta_scale, A.mcoef, A.mass_scale, A.rowptr, sc,
+                           A.vals);
+    } else {
+      auto sc = (const uint16_t*)A.scatter;
+      if (A.mass)
+        hipLaunchKernelGGL((k_bilinear_f64<D, NL, NQ, NG, true, uint16_t>), grid, dim3(kBlock), 0,
+                           A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.ncells_color,
+                           A.slot0, A.eta, A.eta_scale, A.mcoef, A.mass_scale, A.rowptr, sc,
+                           A.vals);
+      else
+        hipLaunchKernelGGL((k_bilinear_f64<D, NL, NQ, NG, false, uint16_t>), grid, dim3(kBlock), 0,
+                           A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.ncells_color,
+                           A.slot0, A.eta, A.eta_scale, A.mcoef, A.mass_scale, A.rowptr, sc,
+                           A.vals);
+    }
+  } else if (A.kind == LaunchArgs::SOURCE) {
+    int ng = (A.ncells_color + CPB - 1) / CPB;
+    hipLaunchKernelGGL((k_source_f64<D, NL, NQ, NG>), dim3(grid_for(ng)), dim3(kBlock), 0,
+                       A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.ncells_color,
+                       A.fkind, A.f, A.fscale, A.vals);
+  } else {
+    int64_t n = A.ncells_total * NQ;
+    hipLaunchKernelGGL((k_qpoints<D, NL, NQ, NG>), dim3((unsigned)((n + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, A.stream, A.coords, A.c2n, A.tab, A.ncells_total,
+                       A.vals);
+  }
+  return hipGetLastError();
+}
+
+template <int D, int NL, int NQ, int NG>
+static int table_size() {
+  return TabLayout<D, NL, NQ, NG>::SIZE;
+}
+
+struct Entry {
+  int geo, order, nq1;
+  hipError_t (*fn)(const LaunchArgs&);
+  int (*size)();
+};
+
+#define BH_Q(ORDER, NQ1) ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1)
+#define BH_H(ORDER, NQ1) ((ORDER) + 1) * ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1) * (NQ1)
+#define BH_ENTRY_Q(ORDER, NQ1)                                                   \
+  {BH_QUAD4, ORDER, NQ1, &launch_all<2, BH_Q(ORDER, NQ1), 4>, &table_size<2, BH_Q(ORDER, NQ1), 4>}
+#define BH_ENTRY_H(ORDER, NQ1)                                                   \
+  {BH_HEXA8, ORDER, NQ1, &launch_all<3, BH_H(ORDER, NQ1), 8>, &table_size<3, BH_H(ORDER, NQ1), 8>}
+
+static const Entry kTable[] = {
+    BH_ENTRY_Q(1, 1), BH_ENTRY_Q(1, 2), BH_ENTRY_Q(1, 3), BH_ENTRY_Q(2, 2), BH_ENTRY_Q(2, 3),
+    BH_ENTRY_Q(2, 4), BH_ENTRY_Q(3, 3), BH_ENTRY_Q(3, 4), BH_ENTRY_Q(4, 5),
+    BH_ENTRY_H(1, 1), BH_ENTRY_H(1, 2), BH_ENTRY_H(1, 3), BH_ENTRY_H(2, 2), BH_ENTRY_H(2, 3),
+};
+
+bool kernel_supported(int geo, int order, int nq1) {
+  for (const Entry& e : kTable)
+    if (e.geo == geo && e.order == order && e.nq1 == nq1) return true;
+  return false;
+}
+
+hipError_t launch(int geo, int order, int nq1, const LaunchArgs& A) {
+  for (const Entry& e : kTable)
+    if (e.geo == geo && e.order == order && e.nq1 == nq1) return e.fn(A);
+  return hipErrorInvalidValue;
+}
+
+int packed_table_size(int geo, int order, int nq1) {
+  for (const Entry& e : kTable)
+    if (e.geo == geo && e.order == order && e.nq1 == nq1) return e.size();
+  return -1;
+}
+
+hipError_t launch_fill(hipStream_t s, double* p, int64_t n, double v) {
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace bh

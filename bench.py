@@ -1,0 +1,168 @@
+"""Benchmark: Q2 hexa stiffness + mass assembly on a jittered 64^3 mesh, 1 GPU.
+
+One step = one full assemble_bilinear of ∫η∇u⋅∇v dΩ + ∫u v dΩ into the
+device-resident CSR values (135,005,697 nnz), inputs already in HBM.  Prints
+one JSON line (see the field list in main()).
+
+    python bench.py [--steps K] [--warmup W] [--mode fp64] [--n 64] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "bcube.jl_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16"}
+
+
+def workload(n):
+    """Algorithmic bytes and flops of one assembly of the C2 workload."""
+    ncells = n ** 3
+    nodes = (n + 1) ** 3
+    ndofs = (2 * n + 1) ** 3
+    nnz = (5 * (n + 1) + 3 * n - 4) ** 3  # tensor product of the 1D Q2 row counts
+    b_alg = 8 * nnz + nodes * 3 * 8 + ncells * 8 * 4 + ncells * 27 * 4 + ncells * 8
+    f_alg = ncells * 2 * 27 * 27 * 4 * 27  # dense (∇̂φ, φ)^T (G, m) (∇̂φ, φ) per cell, K + M
+    return ncells, ndofs, nnz, b_alg, f_alg
+
+
+def cpu_baseline(n_sample=32):
+    """Oracle (C++ restatement of the reference's serial CPU assembly), 1 thread."""
+    from oracle import oracle as O
+
+    o = O.hexa_mesh(n_sample + 1, n_sample + 1, n_sample + 1)
+    rng = np.random.default_rng(0)
+    X = o.coords.reshape(-1, 3)
+    interior = np.all((X > 1e-12) & (X < 1 - 1e-12), axis=1)
+    X[interior] += rng.uniform(-0.15, 0.15, X.shape)[interior] / n_sample
+    eta = np.random.default_rng(1).uniform(0.5, 2.0, o.ncells)
+    t0 = time.perf_counter()
+    colptr, rowval, nzval, nd = O.assemble_bilinear(o, 2, ["stiffness", "mass"], eta=eta, degree=5)
+    dt = time.perf_counter() - t0
+    return {"value": nzval.size / dt / 1e9, "unit": "GNNZ/s", "cores": 1, "kind": "port",
+            "sample": f"hexa {n_sample}^3 Q2, stiffness+mass, COO + sparse(), "
+                      f"{nzval.size} nnz in {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import bcubehip as B
+
+    n = args.n
+    t_setup = time.perf_counter()
+    m = B.hexa_mesh(n + 1, n + 1, n + 1)
+    B.jitter_nodes(m, 0.15, 1.0 / n, seed=0)
+    eta = np.random.default_rng(1).uniform(0.5, 2.0, m.ncells)
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", 2), m)
+    V = B.TestFESpace(U)
+    dΩ = B.Measure(B.CellDomain(m), 5)
+    ηc = B.MeshCellData(eta)
+    plan = B.BilinearPlan(lambda u, v: B.integrate(ηc * B.dot(B.grad(u), B.grad(v)), dΩ)
+                          + B.integrate(u * v, dΩ), U, V, mode=args.mode)
+    t_setup = time.perf_counter() - t_setup
+    ncells, ndofs, nnz, b_alg, f_alg = workload(n)
+    assert plan.nnz == nnz, (plan.nnz, nnz)
+
+    dev = torch.device("cuda:0")
+    eta_d = torch.from_numpy(eta).to(dev)
+    stream = torch.cuda.current_stream(dev)
+    backend = m.backend
+    backend.set_stream(stream.cuda_stream)
+    coef = [eta_d.data_ptr(), None]
+
+    for _ in range(args.warmup):
+        plan.assemble_device(None, coef)
+    torch.cuda.synchronize()
+    backend.set_timing(True)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan.assemble_device(None, coef)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_total = e0.elapsed_time(e1)
+    launches = backend.timings(cap=100000)
+    backend.set_timing(False)
+    ms_step = ms_total / args.steps
+
+    # dominant kernel: the element kernel, one launch per colour
+    nlaunch = len(launches)
+    avg_launch_ms = float(np.mean(launches)) if nlaunch else float("nan")
+    per_assembly_launches = nlaunch / args.steps
+    f_launch = f_alg / per_assembly_launches
+    b_launch = b_alg / per_assembly_launches
+    peak_f = PEAK_TFLOPS[args.mode]
+    t_f = f_launch / (peak_f * 1e12)
+    t_b = b_launch / (PEAK_HBM_GBS * 1e9)
+    if t_f >= t_b:
+        achieved = f_launch / (avg_launch_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak_f, "unit": "TFLOP/s",
+                "frac": achieved / peak_f}
+    else:
+        achieved = b_launch / (avg_launch_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS}
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        t = json.load(open(tf)).get(f"c2_{n}_{args.mode}")
+        if t:
+            traffic = t["bytes_per_launch"]
+    roof["traffic"] = traffic
+    roof["kernel"] = "k_bilinear_f64<3,27,27,8,true,uint8_t>" if args.mode == "fp64" else args.mode
+    roof["avg_launch_ms"] = avg_launch_ms
+    roof["launches_per_step"] = per_assembly_launches
+
+    value = nnz / (ms_step * 1e-3) / 1e9
+    line = {
+        "metric": "Q2 hex stiffness assembly GNNZ/s, FP64 vs FP32-MFMA on 1 GPU at rel err ≤1e-6",
+        "value": value,
+        "unit": "GNNZ/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[args.mode],
+        "data": "synthetic",
+        "config": {"workload": f"C2: hexa {n}^3 Q2 continuous Lagrange, jitter ±0.15h seed 0, "
+                               f"η~U[0.5,2] seed 1, ∫η∇u⋅∇v+∫uv, 27-pt Gauss",
+                   "mode": args.mode, "ncells": ncells, "ndofs": ndofs, "nnz": nnz,
+                   "elements_per_s": ncells / (ms_step * 1e-3),
+                   "setup_s": round(t_setup, 2)},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

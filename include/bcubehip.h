@@ -1,0 +1,158 @@
+/* bcubehip.h -- C ABI of the MI355X (gfx950) finite-element assembly library.
+ *
+ * This is the drop-in boundary for Bcube.jl's assembly path.  A Julia backend
+ * (`BcubeBackendHIP <: AbstractBcubeBackend`, src/backend.jl:4) binds these
+ * symbols with `ccall`; the Python package bcube.jl_amd/bcubehip binds them with
+ * ctypes.  Signatures use plain pointers, sizes and opaque handles only.
+ *
+ * Conventions
+ *   - Every call returns an int: BH_OK (0) or a negative BH_ERR_* code; the
+ *     message of the last failure on a context is bh_last_error(ctx).
+ *   - Indices crossing the ABI are 0-based unless a `base` argument says
+ *     otherwise (bh_pattern_export_csc writes Julia's 1-based CSC directly).
+ *   - `where` says which address space the caller's value arrays live in:
+ *     BH_HOST (library copies through its own device buffers) or BH_DEVICE
+ *     (pointers are device memory; nothing is copied, results stay in HBM).
+ *   - Calls are ordered on the context's stream.  Host-`where` calls return
+ *     after results are in host memory; device-`where` calls return after the
+ *     kernels are enqueued (bh_sync waits).
+ *   - Summation is deterministic: global sums are ordered by a cell colouring,
+ *     never by float atomics, so repeated calls are bitwise identical.
+ */
+#ifndef BCUBEHIP_H
+#define BCUBEHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BH_OK 0
+#define BH_ERR_ARG (-1)         /* inconsistent sizes, null handle, bad enum  */
+#define BH_ERR_UNSUPPORTED (-2) /* shape / order / form without a kernel      */
+#define BH_ERR_HIP (-3)         /* HIP runtime failure                        */
+#define BH_ERR_OOM (-4)         /* host or device allocation failed           */
+#define BH_ERR_RANGE (-5)       /* dof or node index out of range             */
+
+/* Cell types, CGNS node order (src/mesh/entity.jl).  */
+enum { BH_QUAD4 = 0, BH_QUAD9 = 1, BH_TRI3 = 2, BH_HEXA8 = 3, BH_HEXA27 = 4 };
+
+/* Arithmetic of the element contractions.  Geometry (J, J^-1, |J|) and the
+ * global scatter-sum are FP64 in every mode. */
+enum { BH_FP64 = 0, BH_FP32 = 1, BH_BF16X3 = 2 };
+
+enum { BH_HOST = 0, BH_DEVICE = 1 };
+
+/* Bilinear integrands recognised by the library (assembler.jl:172 lowers
+ * `a(u,v)` to a list of these terms). */
+enum {
+  BH_FORM_STIFFNESS = 0, /* ∫ η ∇u⋅∇v dΩ   (η: constant or per cell) */
+  BH_FORM_MASS = 1       /* ∫ c u v dΩ     (c: constant or per cell) */
+};
+/* Linear integrands (assembler.jl:448 lowers `l(v)`). */
+enum {
+  BH_FORM_SOURCE = 10 /* ∫ f v dΩ  (f: constant, per cell or per quadrature point) */
+};
+/* How a coefficient is given. */
+enum {
+  BH_COEF_CONST = 0, /* value = scale                                  */
+  BH_COEF_CELL = 1,  /* value = scale * coef[icell]     (MeshCellData)  */
+  BH_COEF_QP = 2     /* value = scale * coef[icell*nq+q] (PhysicalFunction
+                        sampled at bh_quadrature_points)                */
+};
+
+typedef struct bh_term {
+  int32_t form;       /* BH_FORM_*          */
+  int32_t coef_kind;  /* BH_COEF_*          */
+  const double* coef; /* NULL for CONST     */
+  double scale;
+} bh_term;
+
+typedef struct bh_ctx_s* bh_ctx;
+typedef struct bh_mesh_s* bh_mesh;
+typedef struct bh_space_s* bh_space;
+typedef struct bh_pattern_s* bh_pattern;
+
+/* ---- context ------------------------------------------------------------ */
+/* One context per host thread.  Replaces the serial backend object
+ * BcubeBackendCPUSerial (src/backend.jl:4-12). */
+int bh_ctx_create(int device, bh_ctx* out);
+int bh_ctx_destroy(bh_ctx ctx);
+const char* bh_last_error(bh_ctx ctx);
+/* Launch on a caller stream (a hipStream_t, e.g. torch's current stream);
+ * NULL restores the library's own stream. */
+int bh_ctx_set_stream(bh_ctx ctx, void* hip_stream);
+int bh_sync(bh_ctx ctx);
+/* Per-launch kernel timing with HIP events on the launching stream.  When
+ * enabled, every kernel launch is bracketed; bh_ctx_timings copies up to `cap`
+ * durations (ms) of the launches since the last reset and returns the count. */
+int bh_ctx_set_timing(bh_ctx ctx, int enable);
+int bh_ctx_timings(bh_ctx ctx, float* ms, int cap, int* count);
+
+/* ---- mesh --------------------------------------------------------------- */
+/* Mesh(nodes, celltypes, Connectivity) (src/mesh/mesh.jl:221-262).
+ * coords: nnodes*dim FP64, c2n_ptr: ncells+1, c2n: 0-based node ids. */
+int bh_mesh_create(bh_ctx ctx, int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                   const int32_t* celltypes, const int64_t* c2n_ptr, const int64_t* c2n,
+                   bh_mesh* out);
+/* set_nodes! (src/mesh/mesh.jl:280): new coordinates, same topology. */
+int bh_mesh_set_nodes(bh_mesh mesh, const double* coords, int where);
+int bh_mesh_destroy(bh_mesh mesh);
+
+/* ---- dof numbering ------------------------------------------------------ */
+/* DofHandler(mesh, FunctionSpace(:Lagrange, order), 1, continuous)
+ * (src/fespace/dofhandler.jl:37-156 with densify!, src/utils.jl:7-16).
+ * iglob: Σ_c ndofs(c), 1-based global ids; offset: ncells. */
+int bh_dofhandler(bh_mesh mesh, int order, int continuous, int64_t* iglob, int64_t* offset,
+                  int64_t* ndofs_tot);
+/* TrialFESpace / TestFESpace (src/fespace/fespace.jl:186-414): numbering by
+ * bh_dofhandler, kept on the device as a per-cell dof map. */
+int bh_space_create(bh_mesh mesh, int order, int continuous, bh_space* out);
+/* Same, with a numbering computed by the caller (Julia's own DofHandler,
+ * get_dofs at dofhandler.jl:496-498): iglob 1-based, offset per cell. */
+int bh_space_create_from_dofs(bh_mesh mesh, int order, int continuous, int64_t ndofs,
+                              const int64_t* iglob, const int64_t* offset, bh_space* out);
+int bh_space_info(bh_space space, int64_t* ndofs, int32_t* ndofs_per_cell);
+/* Per-cell dof map, 0-based, ncells*ndofs_per_cell int32 (host copy). */
+int bh_space_dofs(bh_space space, int32_t* dofs);
+int bh_space_destroy(bh_space space);
+
+/* ---- sparsity pattern --------------------------------------------------- */
+/* The CSC structure sparse(I,J,X,m,n) produces (assembler.jl:185), built once
+ * per (U, V) pair: union of element dof-pair graphs, rows sorted per column,
+ * explicit zeros kept.  Also builds the cell colouring and the per-cell
+ * scatter map used by the deterministic sum. */
+int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out);
+int bh_pattern_info(bh_pattern p, int64_t* nrows, int64_t* ncols, int64_t* nnz,
+                    int32_t* ncolors);
+/* Julia SparseMatrixCSC{Float64,Int64} structure: colptr (ncols+1), rowval
+ * (nnz), with index base `base` (1 for Julia).  Row-major CSR of the
+ * transposed matrix is the same arrays; for U == V they coincide. */
+int bh_pattern_export_csc(bh_pattern p, int64_t* colptr, int64_t* rowval, int base);
+/* Device pointer to the pattern's CSR values buffer (nnz FP64), for callers
+ * that keep the matrix in HBM. */
+int bh_pattern_values_device(bh_pattern p, double** dptr);
+int bh_pattern_destroy(bh_pattern p);
+
+/* ---- assembly ------------------------------------------------------------ */
+/* assemble_bilinear(a, U, V) (assembler.jl:172-186) for a = Σ terms on
+ * CellDomain with Measure(Ω, quad_degree).  nzval (nnz) receives the CSC
+ * values in export order.  With where == BH_DEVICE, nzval may be NULL to
+ * assemble into the pattern's own buffer (bh_pattern_values_device). */
+int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int quad_degree,
+                         int mode, double* nzval, int where);
+/* assemble_linear(l, V) / assemble_linear!(b, l, V) (assembler.jl:448-473).
+ * b has ndofs entries and is overwritten. */
+int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_degree, int mode,
+                       double* b, int where);
+/* Physical coordinates x = F(ξ_q) of every quadrature point
+ * (eval_point.jl:107-111), ncells*nq*dim FP64, for sampling a
+ * PhysicalFunction on the host; *nq receives the points per cell. */
+int bh_quadrature_points(bh_mesh mesh, int quad_degree, double* xq, int where, int32_t* nq);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BCUBEHIP_H */

@@ -91,17 +91,20 @@ class BilinearPlan:
                                              L.HOST), self.ctx, "bh_assemble_bilinear")
         return out
 
-    def assemble_device(self, nzval_ptr: int | None = None, coef_ptrs=None):
+    def assemble_device(self, nzval_ptr: int | None = None, coef_ptrs=None,
+                        mode: str | None = None):
         """Enqueue assembly into device memory (nzval_ptr, or the pattern's own
         buffer if None).  coef_ptrs: optional device pointers replacing the
-        per-cell coefficient arrays, in term order (None keeps a term as is)."""
+        per-cell coefficient arrays, in term order (None keeps a term as is).
+        mode overrides the plan's arithmetic for this call."""
         terms = self.terms
         if coef_ptrs is not None:
             terms = (L.BhTerm * self.nterms)(*[L.BhTerm(t.form, t.coef_kind,
                                                         p if p is not None else None, t.scale)
                                                for t, p in zip(self.terms, coef_ptrs)])
+        md = self.mode if mode is None else L.MODE[mode]
         L.check(L.lib().bh_assemble_bilinear(self.handle, self.nterms, terms, self.measure.degree,
-                                             self.mode, nzval_ptr, L.DEVICE), self.ctx,
+                                             md, nzval_ptr, L.DEVICE), self.ctx,
                 "bh_assemble_bilinear")
 
     def values_device_ptr(self) -> int:
@@ -156,7 +159,7 @@ def _linear_terms(l, V):
 
 
 def assemble_linear_(b: np.ndarray, l, V, mode: str = "fp64") -> None:
-    """assemble_linear!(b, l, V) (assembler.jl:465-473): b is overwritten."""
+    """assemble_linear!(b, l, V) (assembler.jl:465-473): adds l(v) into b."""
     V = getattr(V, "U", V)
     if b.dtype != np.float64 or not b.flags.c_contiguous or b.size != V.get_ndofs():
         raise ValueError("b must be a contiguous float64 vector of ndofs entries")

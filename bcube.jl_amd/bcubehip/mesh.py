@@ -50,10 +50,51 @@ class BcubeBackendHIP(AbstractBcubeBackend):
                 "bh_ctx_timings")
         return buf[: min(n.value, cap)].copy()
 
+    def capture(self) -> "AssemblyGraph":
+        """Capture device-side assembly calls into a HIP graph:
+
+            with backend.capture() as g:
+                plan.assemble_device(ptr, coefs)
+            g.launch()            # replays the captured launches
+        """
+        return AssemblyGraph(self)
+
     def __del__(self):
         try:
             if self._ctx is not None:
                 L.lib().bh_ctx_destroy(self._ctx)
+        except Exception:
+            pass
+
+
+class AssemblyGraph:
+    """A replayable HIP graph of assembly launches (bh_graph_*)."""
+
+    def __init__(self, backend: BcubeBackendHIP):
+        self.backend = backend
+        self.handle = None
+
+    def __enter__(self):
+        L.check(L.lib().bh_graph_begin(self.backend.ctx), self.backend.ctx, "bh_graph_begin")
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = C.c_void_p()
+        rc = L.lib().bh_graph_end(self.backend.ctx, C.byref(h))
+        if exc_type is None:
+            L.check(rc, self.backend.ctx, "bh_graph_end")
+            self.handle = h
+        elif rc == L.BH_OK:
+            L.lib().bh_graph_destroy(h)
+        return False
+
+    def launch(self):
+        L.check(L.lib().bh_graph_launch(self.handle), self.backend.ctx, "bh_graph_launch")
+
+    def __del__(self):
+        try:
+            if self.handle is not None:
+                L.lib().bh_graph_destroy(self.handle)
         except Exception:
             pass
 
@@ -222,3 +263,37 @@ def jitter_nodes(mesh: Mesh, amplitude: float, h, seed: int = 0) -> np.ndarray:
     X[interior] += d[interior]
     mesh.set_nodes(X)
     return X
+
+
+def batch_meshes(meshes, backend: AbstractBcubeBackend | None = None):
+    """Concatenate independent meshes of one dimension into a single mesh whose
+    connected components are the inputs, in order.
+
+    With the reference's first-appearance numbering (dofhandler.jl:37-156), the
+    dofs of instance k form one contiguous block numbered exactly as instance k
+    alone, shifted by the dofs of instances 0..k-1: the assembled matrix is the
+    block diagonal of the per-instance matrices.  Returns (mesh, cell_offsets,
+    node_offsets), offsets of length len(meshes)+1.
+    """
+    if not meshes:
+        raise ValueError("no meshes")
+    dim = meshes[0].dim
+    if any(m.dim != dim for m in meshes):
+        raise ValueError("all meshes must share one dimension")
+    node_off = np.concatenate([[0], np.cumsum([m.nnodes for m in meshes])]).astype(np.int64)
+    cell_off = np.concatenate([[0], np.cumsum([m.ncells for m in meshes])]).astype(np.int64)
+    nodes = np.concatenate([m.nodes() for m in meshes], axis=0)
+    celltypes = np.concatenate([m.celltypes for m in meshes])
+    c2n = np.concatenate([m.c2n + node_off[k] for k, m in enumerate(meshes)])
+    ptr = np.concatenate([[0]] + [m.c2n_ptr[1:] + meshes_ptr_off
+                                  for m, meshes_ptr_off in
+                                  zip(meshes, np.cumsum([0] + [m.c2n.size for m in meshes[:-1]]))])
+    out = Mesh.__new__(Mesh)
+    out.dim = dim
+    out.coords = np.ascontiguousarray(nodes.reshape(-1))
+    out.celltypes = np.ascontiguousarray(celltypes, dtype=np.int32)
+    out.c2n_ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+    out.c2n = np.ascontiguousarray(c2n, dtype=np.int64)
+    out.backend = backend if backend is not None else get_bcube_backend()
+    out._handle = None
+    return out, cell_off, node_off

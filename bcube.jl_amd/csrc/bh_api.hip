@@ -24,6 +24,13 @@ struct bh_ctx_s {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
   size_t nevents = 0;
   std::map<std::tuple<int, int, int>, double*> tables;  // (geo, order, nq1) -> device table
+  bool capturing = false;
+};
+
+struct bh_graph_s {
+  bh_ctx ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
 };
 
 struct bh_mesh_s {
@@ -102,7 +109,7 @@ int set_device(bh_ctx c) {
 }
 
 void time_begin(bh_ctx c) {
-  if (!c->timing) return;
+  if (!c->timing || c->capturing) return;
   if (c->nevents == c->events.size()) {
     hipEvent_t a, b;
     (void)hipEventCreate(&a);
@@ -112,7 +119,7 @@ void time_begin(bh_ctx c) {
   (void)hipEventRecord(c->events[c->nevents].first, c->stream);
 }
 void time_end(bh_ctx c) {
-  if (!c->timing) return;
+  if (!c->timing || c->capturing) return;
   (void)hipEventRecord(c->events[c->nevents].second, c->stream);
   c->nevents++;
 }
@@ -122,6 +129,8 @@ int get_tables(bh_ctx c, int geo, int order, int quad_degree, double** out) {
   auto key = std::make_tuple(geo, order, nq1);
   auto it = c->tables.find(key);
   if (it != c->tables.end()) { *out = it->second; return BH_OK; }
+  if (c->capturing)
+    return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
   int size = packed_table_size(geo, order, nq1);
   if (size < 0)
     return fail(c, BH_ERR_UNSUPPORTED,
@@ -491,9 +500,12 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
   bh_ctx c = m->ctx;
   if (!c) return fail(c, BH_ERR_ARG, "assembly needs a mesh created with a context");
   if (p->U != p->V) return fail(c, BH_ERR_UNSUPPORTED, "assembly with U != V is not supported yet");
-  if (mode != BH_FP64) return fail(c, BH_ERR_UNSUPPORTED, "only BH_FP64 is implemented");
+  if (mode != BH_FP64 && mode != BH_FP32 && mode != BH_BF16X3)
+    return fail(c, BH_ERR_UNSUPPORTED, "bilinear modes: BH_FP64, BH_FP32, BH_BF16X3");
   if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
   if (where == BH_HOST && !nzval) return fail(c, BH_ERR_ARG, "nzval is NULL");
+  if (where == BH_HOST && c->capturing)
+    return fail(c, BH_ERR_ARG, "host-`where` assembly cannot be captured in a graph");
   if (m->geo < 0) return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
   const bh_term* stiff = nullptr;
   const bh_term* mass = nullptr;
@@ -509,12 +521,15 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
   int nq1 = gauss_points_per_axis(quad_degree);
   if (!kernel_supported(m->geo, V->order, nq1))
     return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this (cell type, order, quadrature)");
+  if (mode != BH_FP64 && !mfma_supported(m->geo, V->order, nq1))
+    return fail(c, BH_ERR_UNSUPPORTED, "no FP32 matrix-core kernel for this (cell type, order, quadrature)");
   if (int r = set_device(c)) return r;
   double* tab = nullptr;
   if (int r = get_tables(c, m->geo, V->order, quad_degree, &tab)) return r;
   int64_t nc = m->h.ncells;
   LaunchArgs A;
   A.kind = LaunchArgs::BILINEAR;
+  A.mode = mode;
   A.stream = c->stream;
   A.coords = m->d_coords;
   A.c2n = m->d_c2n;
@@ -540,7 +555,8 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
     A.cells = p->d_cells + A.slot0;
     if (A.ncells_color == 0) continue;
     time_begin(c);
-    BH_HIP(c, launch(m->geo, V->order, nq1, A));
+    if (mode != BH_FP64) BH_HIP(c, launch_mfma(m->geo, V->order, nq1, A));
+    else BH_HIP(c, launch(m->geo, V->order, nq1, A));
     time_end(c);
   }
   if (where == BH_HOST) {
@@ -559,6 +575,8 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
   if (!c) return fail(c, BH_ERR_ARG, "assembly needs a mesh created with a context");
   if (mode != BH_FP64) return fail(c, BH_ERR_UNSUPPORTED, "only BH_FP64 is implemented");
   if (!b) return fail(c, BH_ERR_ARG, "b is NULL");
+  if (where == BH_HOST && c->capturing)
+    return fail(c, BH_ERR_ARG, "host-`where` assembly cannot be captured in a graph");
   if (m->geo < 0) return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
   int nq1 = gauss_points_per_axis(quad_degree);
   if (!kernel_supported(m->geo, V->order, nq1))
@@ -571,8 +589,11 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
   if (int r = set_device(c)) return r;
   double* tab = nullptr;
   if (int r = get_tables(c, m->geo, V->order, quad_degree, &tab)) return r;
-  if (V->col.ncolors == 0)
+  if (V->col.ncolors == 0) {
+    if (c->capturing)
+      return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
     if (int r = build_colouring(c, V->dofs, V->nl, V->ndofs, m->h.ncells, V->col)) return r;
+  }
   int64_t nc = m->h.ncells;
   int nq = nq1 * nq1 * (m->h.dim == 3 ? nq1 : 1);
   double* db = b;
@@ -580,7 +601,9 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
     if (!V->d_b) BH_HIP(c, dalloc(&V->d_b, (size_t)V->ndofs));
     db = V->d_b;
   }
-  BH_HIP(c, launch_fill(c->stream, db, V->ndofs, 0.0));
+  if (where == BH_HOST)  // b += l(v): start from the caller's values
+    BH_HIP(c, hipMemcpyAsync(db, b, (size_t)V->ndofs * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
   // assemble_linear runs the integrals of a MultiIntegration last-first (assembler.jl:496-516)
   for (int k = nterms - 1; k >= 0; --k) {
     const bh_term& t = terms[k];
@@ -626,10 +649,55 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
   return BH_OK;
 }
 
+// ---- graphs ------------------------------------------------------------------
+int bh_graph_begin(bh_ctx c) {
+  if (!c) return BH_ERR_ARG;
+  if (c->capturing) return fail(c, BH_ERR_ARG, "already capturing");
+  if (int r = set_device(c)) return r;
+  BH_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  return BH_OK;
+}
+
+int bh_graph_end(bh_ctx c, bh_graph* out) {
+  if (!c || !out) return BH_ERR_ARG;
+  *out = nullptr;
+  if (!c->capturing) return fail(c, BH_ERR_ARG, "not capturing");
+  c->capturing = false;
+  auto g = std::make_unique<bh_graph_s>();
+  g->ctx = c;
+  BH_HIP(c, hipStreamEndCapture(c->stream, &g->graph));
+  hipError_t e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g->graph);
+    return fail(c, BH_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  *out = g.release();
+  return BH_OK;
+}
+
+int bh_graph_launch(bh_graph g) {
+  if (!g) return BH_ERR_ARG;
+  bh_ctx c = g->ctx;
+  if (int r = set_device(c)) return r;
+  BH_HIP(c, hipGraphLaunch(g->exec, c->stream));
+  return BH_OK;
+}
+
+int bh_graph_destroy(bh_graph g) {
+  if (!g) return BH_ERR_ARG;
+  (void)hipGraphExecDestroy(g->exec);
+  (void)hipGraphDestroy(g->graph);
+  delete g;
+  return BH_OK;
+}
+
 int bh_quadrature_points(bh_mesh m, int quad_degree, double* xq, int where, int32_t* nq_out) {
   if (!m || !xq) return BH_ERR_ARG;
   bh_ctx c = m->ctx;
   if (!c) return fail(c, BH_ERR_ARG, "needs a mesh created with a context");
+  if (c->capturing && where == BH_HOST)
+    return fail(c, BH_ERR_ARG, "host-`where` call cannot be captured in a graph");
   if (m->geo < 0) return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
   int nq1 = gauss_points_per_axis(quad_degree);
   int order = 1;  // any instantiated order carries the geometric tables for nq1

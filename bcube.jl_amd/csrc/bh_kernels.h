@@ -12,6 +12,7 @@ namespace bh {
 struct LaunchArgs {
   enum Kind { BILINEAR = 0, SOURCE = 1, QPOINTS = 2 };
   int kind = BILINEAR;
+  int mode = BH_FP64;
   hipStream_t stream = nullptr;
   const double* coords = nullptr;
   const int32_t* c2n = nullptr;    // ncells * nodes-per-cell (geometric nodes)
@@ -42,5 +43,8 @@ bool kernel_supported(int geo_celltype, int order, int nq1);
 int packed_table_size(int geo_celltype, int order, int nq1);
 hipError_t launch(int geo_celltype, int order, int nq1, const LaunchArgs& a);
 hipError_t launch_fill(hipStream_t s, double* p, int64_t n, double v);
+// Matrix-core element kernels (bh_mfma.hip): BH_FP32 and BH_BF16X3, bilinear forms only.
+bool mfma_supported(int geo_celltype, int order, int nq1);
+hipError_t launch_mfma(int geo_celltype, int order, int nq1, const LaunchArgs& a);
 
 }  // namespace bh

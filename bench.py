@@ -4,7 +4,11 @@ One step = one full assemble_bilinear of ∫η∇u⋅∇v dΩ + ∫u v dΩ into 
 device-resident CSR values (135,005,697 nnz), inputs already in HBM.  Prints
 one JSON line (see the field list in main()).
 
-    python bench.py [--steps K] [--warmup W] [--mode fp64] [--n 64] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--mode all|fp64|fp32|bf16x3] [--n 64] [--no-cpu]
+
+With --mode all (default) the FP64, FP32-MFMA and BF16x3 forms are timed in
+turn; `value` is the FP32-MFMA form (the headline), every mode is listed under
+"modes" with its error against the FP64 GPU result.
 """
 import argparse
 import json
@@ -22,6 +26,9 @@ import numpy as np  # noqa: E402
 PEAK_HBM_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16"}
+KERNEL = {"fp64": "k_bilinear_f64<3,27,27,8,true,uint8_t>",
+          "fp32": "k_bilinear_mfma32<3,27,27,8,true,uint8_t,0>",
+          "bf16x3": "k_bilinear_mfma32<3,27,27,8,true,uint8_t,1>"}
 
 
 def workload(n):
@@ -33,6 +40,23 @@ def workload(n):
     b_alg = 8 * nnz + nodes * 3 * 8 + ncells * 8 * 4 + ncells * 27 * 4 + ncells * 8
     f_alg = ncells * 2 * 27 * 27 * 4 * 27  # dense (∇̂φ, φ)^T (G, m) (∇̂φ, φ) per cell, K + M
     return ncells, ndofs, nnz, b_alg, f_alg
+
+
+def roofline(mode, launches, steps, f_alg, b_alg):
+    """Roofline of the dominant kernel (the element kernel, one launch per colour)."""
+    avg = float(np.mean(launches))
+    per = len(launches) / steps
+    f_launch, b_launch = f_alg / per, b_alg / per
+    peak_f = PEAK_TFLOPS[mode]
+    if f_launch / (peak_f * 1e12) >= b_launch / (PEAK_HBM_GBS * 1e9):
+        ach = f_launch / (avg * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": ach, "peak": peak_f, "unit": "TFLOP/s",
+                "frac": ach / peak_f, "kernel": KERNEL[mode], "avg_launch_ms": avg,
+                "flops_per_launch": f_launch, "alg_bytes_per_launch": b_launch}
+    ach = b_launch / (avg * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": ach / PEAK_HBM_GBS, "kernel": KERNEL[mode], "avg_launch_ms": avg,
+            "flops_per_launch": f_launch, "alg_bytes_per_launch": b_launch}
 
 
 def cpu_baseline(n_sample=32):
@@ -58,7 +82,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--mode", default="all", choices=["all", "fp64", "fp32", "bf16x3"])
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -80,8 +104,9 @@ def main():
     V = B.TestFESpace(U)
     dΩ = B.Measure(B.CellDomain(m), 5)
     ηc = B.MeshCellData(eta)
-    plan = B.BilinearPlan(lambda u, v: B.integrate(ηc * B.dot(B.grad(u), B.grad(v)), dΩ)
-                          + B.integrate(u * v, dΩ), U, V, mode=args.mode)
+    plan_form = (lambda u, v: B.integrate(ηc * B.dot(B.grad(u), B.grad(v)), dΩ)
+                 + B.integrate(u * v, dΩ))
+    plan = B.BilinearPlan(plan_form, U, V)
     t_setup = time.perf_counter() - t_setup
     ncells, ndofs, nnz, b_alg, f_alg = workload(n)
     assert plan.nnz == nnz, (plan.nnz, nnz)
@@ -92,72 +117,72 @@ def main():
     backend = m.backend
     backend.set_stream(stream.cuda_stream)
     coef = [eta_d.data_ptr(), None]
-
-    for _ in range(args.warmup):
-        plan.assemble_device(None, coef)
-    torch.cuda.synchronize()
-    backend.set_timing(True)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(args.steps):
-        plan.assemble_device(None, coef)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms_total = e0.elapsed_time(e1)
-    launches = backend.timings(cap=100000)
-    backend.set_timing(False)
-    ms_step = ms_total / args.steps
-
-    # dominant kernel: the element kernel, one launch per colour
-    nlaunch = len(launches)
-    avg_launch_ms = float(np.mean(launches)) if nlaunch else float("nan")
-    per_assembly_launches = nlaunch / args.steps
-    f_launch = f_alg / per_assembly_launches
-    b_launch = b_alg / per_assembly_launches
-    peak_f = PEAK_TFLOPS[args.mode]
-    t_f = f_launch / (peak_f * 1e12)
-    t_b = b_launch / (PEAK_HBM_GBS * 1e9)
-    if t_f >= t_b:
-        achieved = f_launch / (avg_launch_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": peak_f, "unit": "TFLOP/s",
-                "frac": achieved / peak_f}
-    else:
-        achieved = b_launch / (avg_launch_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM_GBS}
+    modes = ["fp64", "fp32", "bf16x3"] if args.mode == "all" else [args.mode]
+    results = {}
+    ref = None
+    for mode in modes:
+        out = torch.empty(nnz, dtype=torch.float64, device=dev)
+        for _ in range(args.warmup):
+            plan.assemble_device(out.data_ptr(), coef, mode)
+        torch.cuda.synchronize()
+        backend.set_timing(True)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            plan.assemble_device(out.data_ptr(), coef, mode)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_step = e0.elapsed_time(e1) / args.steps
+        launches = backend.timings(cap=100000)
+        backend.set_timing(False)
+        if mode == "fp64":
+            ref = out
+        err = None
+        if ref is not None and mode != "fp64":
+            err = float((out - ref).abs().max() / ref.abs().max())
+        results[mode] = {"ms_per_step": ms_step, "gnnz_per_s": nnz / (ms_step * 1e-3) / 1e9,
+                         "elements_per_s": ncells / (ms_step * 1e-3),
+                         "avg_launch_ms": float(np.mean(launches)),
+                         "launches_per_step": len(launches) / args.steps,
+                         "rel_err_vs_fp64_gpu": err, "roofline": roofline(mode, launches, args.steps, f_alg, b_alg)}
+        if mode != "fp64":
+            del out
+    head = "fp32" if "fp32" in results else modes[0]
+    r = results[head]
+    roof = dict(r["roofline"])
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
-        t = json.load(open(tf)).get(f"c2_{n}_{args.mode}")
+        t = json.load(open(tf)).get(f"c2_{n}_{head}")
         if t:
             traffic = t["bytes_per_launch"]
     roof["traffic"] = traffic
-    roof["kernel"] = "k_bilinear_f64<3,27,27,8,true,uint8_t>" if args.mode == "fp64" else args.mode
-    roof["avg_launch_ms"] = avg_launch_ms
-    roof["launches_per_step"] = per_assembly_launches
-
-    value = nnz / (ms_step * 1e-3) / 1e9
+    speed = None
+    if "fp64" in results and "fp32" in results:
+        speed = results["fp64"]["ms_per_step"] / results["fp32"]["ms_per_step"]
     line = {
         "metric": "Q2 hex stiffness assembly GNNZ/s, FP64 vs FP32-MFMA on 1 GPU at rel err ≤1e-6",
-        "value": value,
+        "value": r["gnnz_per_s"],
         "unit": "GNNZ/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": DTYPE[args.mode],
+        "dtype": DTYPE[head],
         "data": "synthetic",
         "config": {"workload": f"C2: hexa {n}^3 Q2 continuous Lagrange, jitter ±0.15h seed 0, "
-                               f"η~U[0.5,2] seed 1, ∫η∇u⋅∇v+∫uv, 27-pt Gauss",
-                   "mode": args.mode, "ncells": ncells, "ndofs": ndofs, "nnz": nnz,
-                   "elements_per_s": ncells / (ms_step * 1e-3),
-                   "setup_s": round(t_setup, 2)},
+                               f"η~U[0.5,2] seed 1, ∫η∇u⋅∇v+∫uv, 27-pt Gauss, values into "
+                               f"device-resident CSR",
+                   "mode": head, "ncells": ncells, "ndofs": ndofs, "nnz": nnz,
+                   "elements_per_s": r["elements_per_s"], "setup_s": round(t_setup, 2)},
         "roofline": roof,
+        "fp32_speedup_vs_fp64": speed,
+        "rel_err_vs_fp64_gpu": r["rel_err_vs_fp64_gpu"],
+        "modes": results,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()

@@ -142,14 +142,31 @@ int bh_pattern_destroy(bh_pattern p);
  * assemble into the pattern's own buffer (bh_pattern_values_device). */
 int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int quad_degree,
                          int mode, double* nzval, int where);
-/* assemble_linear(l, V) / assemble_linear!(b, l, V) (assembler.jl:448-473).
- * b has ndofs entries and is overwritten. */
+/* assemble_linear!(b, l, V) (assembler.jl:465-473): b += l(v) for every test
+ * function, like __update_b! (assembler.jl:855-865); b has ndofs entries.
+ * assemble_linear(l, V) (assembler.jl:448-458) is this call on a zeroed b. */
 int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_degree, int mode,
                        double* b, int where);
 /* Physical coordinates x = F(ξ_q) of every quadrature point
  * (eval_point.jl:107-111), ncells*nq*dim FP64, for sampling a
  * PhysicalFunction on the host; *nq receives the points per cell. */
 int bh_quadrature_points(bh_mesh mesh, int quad_degree, double* xq, int where, int32_t* nq);
+
+/* ---- graphs -------------------------------------------------------------- */
+/* Re-assembly inside time-stepping or Newton loops (e.g. benchmark/covo.jl:14-33
+ * calls assemble_linear! every step): capture the device-`where` assembly
+ * calls made between bh_graph_begin and bh_graph_end on the context's stream
+ * into one HIP graph, then replay it with bh_graph_launch.  Pointers (values,
+ * coefficients, mesh nodes) are baked in: update their contents in place
+ * between replays (bh_mesh_set_nodes keeps the device buffer).  Every kernel
+ * of a captured call must have run once outside capture (tables and
+ * colourings are created on first use).  Host-`where` calls fail while
+ * capturing.  No memset nodes are recorded. */
+typedef struct bh_graph_s* bh_graph;
+int bh_graph_begin(bh_ctx ctx);
+int bh_graph_end(bh_ctx ctx, bh_graph* out);
+int bh_graph_launch(bh_graph g);
+int bh_graph_destroy(bh_graph g);
 
 #ifdef __cplusplus
 }

@@ -161,3 +161,28 @@ def test_unsupported_forms_raise():
     assert (f, kind, scale) == ("stiffness", "const", 6.0)
     f, kind, data, scale = lower_bilinear(u * 0.5 * v)
     assert (f, kind, scale) == ("mass", "const", 0.5)
+
+
+def test_batched_meshes_are_block_diagonal_in_numbering_and_pattern():
+    """Concatenated instances keep per-instance numbering (shifted) and the
+    pattern is the block diagonal of the per-instance patterns."""
+    sides = [3, 5, 4, 6]
+    ms = [B.rectangle_mesh(s + 1, s + 1, backend=HOST) for s in sides]
+    for order in (1, 2):
+        big, coff, noff = B.batch_meshes(ms, backend=HOST)
+        dbig = B.DofHandler(big, B.FunctionSpace("Lagrange", order))
+        shift = 0
+        plan_big = _plan(big, order, 2 * order)
+        cb, rb = plan_big.csc_structure()
+        for k, m in enumerate(ms):
+            d = B.DofHandler(m, B.FunctionSpace("Lagrange", order))
+            for c in range(m.ncells):
+                assert np.array_equal(dbig.get_dof(coff[k] + c), d.get_dof(c) + shift)
+            pk = _plan(m, order, 2 * order)
+            ck, rk = pk.csc_structure()
+            cols = slice(shift, shift + d.ndofs_tot)
+            seg = slice(cb[shift], cb[shift + d.ndofs_tot])
+            assert np.array_equal(cb[cols] - cb[shift], ck[:-1])
+            assert np.array_equal(rb[seg] - shift, rk)
+            shift += d.ndofs_tot
+        assert shift == dbig.ndofs_tot

@@ -48,6 +48,13 @@ def lib():
     global _lib
     if _lib is not None:
         return _lib
+    try:
+        # One HIP runtime per process: torch bundles its own libamdhip64 with the
+        # same soname, and whichever loads first serves everyone.  Load torch's
+        # first so that torch (device memory, streams) and this library agree.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise BcubeHipError(
             f"HIP assembly library not built: {LIB_PATH} is missing "

@@ -69,7 +69,7 @@ class BilinearPlan:
         nr, nc, nnz, ncol = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
         L.check(L.lib().bh_pattern_info(h, C.byref(nr), C.byref(nc), C.byref(nnz), C.byref(ncol)),
                 ctx, "bh_pattern_info")
-        self.nrows, self.ncols, self.nnz, self.ncolors = nr.value, nc.value, nnz.value, ncol.value
+        self.nrows, self.ncols, self.nnz, self.max_row = nr.value, nc.value, nnz.value, ncol.value
         self.ctx = ctx
         self._colptr = None
 

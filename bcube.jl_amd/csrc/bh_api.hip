@@ -71,8 +71,10 @@ struct bh_pattern_s {
   int dev = -1;
   HostPattern h;
   int64_t* d_rowptr = nullptr;
-  int32_t* d_cells = nullptr;
-  void* d_scatter = nullptr;
+  int32_t* d_pptr = nullptr;
+  int32_t* d_pairs = nullptr;
+  void* d_goff = nullptr;
+  void* d_ke = nullptr;      // element-row workspace, ncells*nl*KS FP64 (FP32 uses half)
   double* d_vals = nullptr;
   double* d_coef = nullptr;  // staged host coefficients (2 * ncells)
 };
@@ -431,7 +433,7 @@ int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out) {
   if (!build_pattern(V->dofs, V->nl, V->ndofs, U->dofs, U->nl, U->ndofs, V->mesh->h.ncells, p->h, err))
     return fail(c, BH_ERR_ARG, err);
   HostPattern& h = p->h;
-  if (!c) {  // host-only pattern: keep the scatter map for inspection
+  if (!c) {  // host-only pattern: structure queries only
     *out = p.release();
     return BH_OK;
   }
@@ -439,25 +441,30 @@ int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out) {
   BH_HIP(c, dalloc(&p->d_rowptr, h.rowptr.size()));
   BH_HIP(c, hipMemcpy(p->d_rowptr, h.rowptr.data(), h.rowptr.size() * sizeof(int64_t),
                       hipMemcpyHostToDevice));
-  BH_HIP(c, dalloc(&p->d_cells, h.cells_by_color.size()));
-  BH_HIP(c, hipMemcpy(p->d_cells, h.cells_by_color.data(), h.cells_by_color.size() * sizeof(int32_t),
+  BH_HIP(c, dalloc(&p->d_pptr, h.pptr.size()));
+  BH_HIP(c, hipMemcpy(p->d_pptr, h.pptr.data(), h.pptr.size() * sizeof(int32_t),
                       hipMemcpyHostToDevice));
-  BH_HIP(c, dalloc((uint8_t**)&p->d_scatter, h.scatter.size()));
-  BH_HIP(c, hipMemcpy(p->d_scatter, h.scatter.data(), h.scatter.size(), hipMemcpyHostToDevice));
+  BH_HIP(c, dalloc(&p->d_pairs, h.pairs.size()));
+  BH_HIP(c, hipMemcpy(p->d_pairs, h.pairs.data(), h.pairs.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice));
+  BH_HIP(c, dalloc((uint8_t**)&p->d_goff, h.goff.size()));
+  BH_HIP(c, hipMemcpy(p->d_goff, h.goff.data(), h.goff.size(), hipMemcpyHostToDevice));
   BH_HIP(c, dalloc(&p->d_vals, (size_t)h.nnz));
   BH_HIP(c, dalloc(&p->d_coef, (size_t)2 * V->mesh->h.ncells));
-  h.scatter.clear();
-  h.scatter.shrink_to_fit();
+  h.goff.clear();
+  h.goff.shrink_to_fit();
+  h.pairs.clear();
+  h.pairs.shrink_to_fit();
   *out = p.release();
   return BH_OK;
 }
 
-int bh_pattern_info(bh_pattern p, int64_t* nrows, int64_t* ncols, int64_t* nnz, int32_t* ncolors) {
+int bh_pattern_info(bh_pattern p, int64_t* nrows, int64_t* ncols, int64_t* nnz, int32_t* max_row) {
   if (!p) return BH_ERR_ARG;
   if (nrows) *nrows = p->h.nrows;
   if (ncols) *ncols = p->h.ncols;
   if (nnz) *nnz = p->h.nnz;
-  if (ncolors) *ncolors = p->h.ncolors;
+  if (max_row) *max_row = p->h.max_row;
   return BH_OK;
 }
 
@@ -482,8 +489,10 @@ int bh_pattern_destroy(bh_pattern p) {
   if (p->dev >= 0) {
     (void)hipSetDevice(p->dev);
     (void)hipFree(p->d_rowptr);
-    (void)hipFree(p->d_cells);
-    (void)hipFree(p->d_scatter);
+    (void)hipFree(p->d_pptr);
+    (void)hipFree(p->d_pairs);
+    (void)hipFree(p->d_goff);
+    (void)hipFree(p->d_ke);
     (void)hipFree(p->d_vals);
     (void)hipFree(p->d_coef);
   }
@@ -527,17 +536,24 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
   double* tab = nullptr;
   if (int r = get_tables(c, m->geo, V->order, quad_degree, &tab)) return r;
   int64_t nc = m->h.ncells;
+  const HostPattern& h = p->h;
+  if (h.max_row > 512) return fail(c, BH_ERR_UNSUPPORTED, "CSR rows longer than 512 entries");
+  if (!p->d_ke) {
+    if (c->capturing)
+      return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
+    int ks = V->nl <= 32 ? 32 : ((V->nl + 31) / 32) * 32;
+    BH_HIP(c, dalloc((double**)&p->d_ke, (size_t)nc * V->nl * ks));
+  }
   LaunchArgs A;
   A.kind = LaunchArgs::BILINEAR;
   A.mode = mode;
   A.stream = c->stream;
   A.coords = m->d_coords;
   A.c2n = m->d_c2n;
-  A.rdofs = V->d_dofs;
   A.tab = tab;
-  A.rowptr = p->d_rowptr;
-  A.scatter = p->d_scatter;
-  A.offset_bytes = p->h.offset_bytes;
+  A.ncells_total = nc;
+  A.ke = p->d_ke;
+  A.ke_fp64 = mode == BH_FP64;
   if (stiff) {
     A.eta = coef_ptr(*stiff, where, p->d_coef, nc, c->stream);
     A.eta_scale = stiff->scale;
@@ -547,18 +563,24 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
     A.mcoef = coef_ptr(*mass, where, p->d_coef + nc, nc, c->stream);
     A.mass_scale = mass->scale;
   }
+  // 1. element rows
+  time_begin(c);
+  if (mode != BH_FP64) BH_HIP(c, launch_mfma(m->geo, V->order, nq1, A));
+  else BH_HIP(c, launch(m->geo, V->order, nq1, A));
+  time_end(c);
+  // 2. deterministic row reduction into the CSR values
+  A.kind = LaunchArgs::GATHER;
+  A.pptr = p->d_pptr;
+  A.pairs = p->d_pairs;
+  A.goff = p->d_goff;
+  A.offset_bytes = h.offset_bytes;
+  A.rowptr = p->d_rowptr;
+  A.nrows = h.nrows;
+  A.max_row = h.max_row;
   A.vals = (where == BH_DEVICE && nzval) ? nzval : p->d_vals;
-  const HostPattern& h = p->h;
-  for (int k = 0; k < h.ncolors; ++k) {
-    A.slot0 = h.color_ptr[k];
-    A.ncells_color = h.color_ptr[k + 1] - h.color_ptr[k];
-    A.cells = p->d_cells + A.slot0;
-    if (A.ncells_color == 0) continue;
-    time_begin(c);
-    if (mode != BH_FP64) BH_HIP(c, launch_mfma(m->geo, V->order, nq1, A));
-    else BH_HIP(c, launch(m->geo, V->order, nq1, A));
-    time_end(c);
-  }
+  time_begin(c);
+  BH_HIP(c, launch(m->geo, V->order, nq1, A));
+  time_end(c);
   if (where == BH_HOST) {
     BH_HIP(c, hipMemcpyAsync(nzval, p->d_vals, (size_t)h.nnz * sizeof(double),
                              hipMemcpyDeviceToHost, c->stream));

@@ -50,6 +50,13 @@ struct TabLayout {
   static constexpr int SIZE = G + NQ * NG;
 };
 
+// Row stride (elements) of the element-matrix workspace: rows start on
+// 128-byte boundaries for FP32.
+template <int NL>
+struct KeStride {
+  static constexpr int value = NL <= 32 ? 32 : ((NL + 31) / 32) * 32;
+};
+
 template <int D>
 struct SymN {
   static constexpr int N = D * (D + 1) / 2;

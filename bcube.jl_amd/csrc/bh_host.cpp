@@ -1,5 +1,6 @@
 // bh_host.cpp -- host-side setup of the assembly path: reference tables, dof
-// numbering, CSR pattern, cell colouring and the per-cell scatter map.
+// numbering, CSR pattern, cell colouring (vector assembly) and the gather
+// structure of the deterministic row reduction (matrix assembly).
 //
 // Everything here runs once per mesh/space/pattern; the per-assembly work is
 // on the GPU (bh_kernels.hip).  Reference citations are relative to the
@@ -475,7 +476,7 @@ int colour_cells(const std::vector<int32_t>& rdofs, int nl, i64 nrows, i64 ncell
 // ---------------------------------------------------------------------------
 // CSR pattern of the element dof-pair graph, rows = test dofs, columns = trial
 // dofs, columns sorted and unique per row (the structure sparse() produces,
-// assembler.jl:185), plus the colour-ordered scatter map.
+// assembler.jl:185), plus the row -> element-row gather lists.
 // ---------------------------------------------------------------------------
 bool build_pattern(const std::vector<int32_t>& rdofs, int nl_row, i64 nrows,
                    const std::vector<int32_t>& cdofs, int nl_col, i64 ncols, i64 ncells,
@@ -530,43 +531,44 @@ bool build_pattern(const std::vector<int32_t>& rdofs, int nl_row, i64 nrows,
       std::copy(buf.begin(), end, p.colval.begin() + p.rowptr[r]);
     }
   });
-  // colouring on row dofs
-  p.ncolors = colour_cells(rdofs, nl_row, nrows, ncells, p.color_ptr, p.cells_by_color);
-  if (p.ncolors < 0) { err = "colouring needs more than 255 colours"; return false; }
-  // scatter map with first-touch flags (colours are launched in order)
-  if (p.max_row <= 128) p.offset_bytes = 1;
-  else if (p.max_row <= 32768) p.offset_bytes = 2;
-  else { err = "row longer than 32768 entries"; return false; }
+  // gather structure: row -> (cell, i) in cell order, and per-(cell, i) column offsets
+  if (ncells * nl_row >= (i64)INT32_MAX) { err = "ncells * nl exceeds 2^31-1"; return false; }
+  p.pptr.assign(nrows + 1, 0);
+  for (i64 r = 0; r <= nrows; ++r) p.pptr[r] = (int32_t)aptr[r];
+  p.pairs.resize(aptr[nrows]);
+  parallel_for(nrows, [&](i64 b, i64 e) {
+    for (i64 r = b; r < e; ++r)
+      for (i64 q = aptr[r]; q < aptr[r + 1]; ++q) {
+        i64 c = adj[q];
+        int li = -1;
+        for (int i = 0; i < nl_row; ++i)
+          if (rdofs[c * nl_row + i] == r) { li = i; break; }
+        p.pairs[q] = (int32_t)(c * nl_row + li);
+      }
+  });
+  if (p.max_row <= 256) p.offset_bytes = 1;
+  else if (p.max_row <= 65536) p.offset_bytes = 2;
+  else { err = "row longer than 65536 entries"; return false; }
   i64 per = (i64)nl_row * nl_col;
-  p.scatter.assign((size_t)(ncells * per * p.offset_bytes), 0);
-  std::vector<uint8_t> seen(p.nnz, 0);
-  for (int k = 0; k < p.ncolors; ++k) {
-    i64 s0 = p.color_ptr[k], s1 = p.color_ptr[k + 1];
-    parallel_for(s1 - s0, [&](i64 b, i64 e) {
-      for (i64 s = s0 + b; s < s0 + e; ++s) {
-        i64 c = p.cells_by_color[s];
-        for (int i = 0; i < nl_row; ++i) {
-          i64 r = rdofs[c * nl_row + i];
-          const int32_t* row = &p.colval[p.rowptr[r]];
-          i64 rl = p.rowptr[r + 1] - p.rowptr[r];
-          for (int j = 0; j < nl_col; ++j) {
-            int32_t col = cdofs[c * nl_col + j];
-            i64 off = std::lower_bound(row, row + rl, col) - row;
-            i64 g = p.rowptr[r] + off;
-            bool first = !seen[g];
-            seen[g] = 1;
-            size_t idx = (size_t)(s * per + (i64)j * nl_row + i);
-            if (p.offset_bytes == 1) {
-              p.scatter[idx] = (uint8_t)(off | (first ? 0x80 : 0));
-            } else {
-              uint16_t v = (uint16_t)(off | (first ? 0x8000 : 0));
-              std::memcpy(&p.scatter[idx * 2], &v, 2);
-            }
+  p.goff.assign((size_t)(ncells * per * p.offset_bytes), 0);
+  parallel_for(ncells, [&](i64 b, i64 e) {
+    for (i64 c = b; c < e; ++c)
+      for (int i = 0; i < nl_row; ++i) {
+        i64 r = rdofs[c * nl_row + i];
+        const int32_t* row = &p.colval[p.rowptr[r]];
+        i64 rl = p.rowptr[r + 1] - p.rowptr[r];
+        for (int j = 0; j < nl_col; ++j) {
+          i64 off = std::lower_bound(row, row + rl, cdofs[c * nl_col + j]) - row;
+          size_t idx = (size_t)(c * per + (i64)i * nl_col + j);
+          if (p.offset_bytes == 1) {
+            p.goff[idx] = (uint8_t)off;
+          } else {
+            uint16_t v = (uint16_t)off;
+            std::memcpy(&p.goff[idx * 2], &v, 2);
           }
         }
       }
-    });
-  }
+  });
   return true;
 }
 

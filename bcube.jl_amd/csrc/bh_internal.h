@@ -50,20 +50,21 @@ struct HostMesh {
 i64 dofhandler(const HostMesh& m, int order, bool continuous, std::vector<i64>& iglob,
                std::vector<i64>& offset, std::string& err);
 
-// CSR pattern + colouring + scatter map.
+// CSR pattern + the gather structure of the deterministic row reduction.
 struct HostPattern {
   i64 nrows = 0, ncols = 0, nnz = 0;
   int nl_row = 0, nl_col = 0;
   std::vector<i64> rowptr;       // nrows+1
   std::vector<int32_t> colval;   // nnz (sorted per row)
-  int ncolors = 0;
-  std::vector<int32_t> color_ptr;  // ncolors+1 into cells_by_color
-  std::vector<int32_t> cells_by_color;
-  // scatter map, in colour-ordered slot order: slot s holds cell cells_by_color[s];
-  // entry [s*nl_row*nl_col + j*nl_row + i] = (offset within row dofs[i]) | first-touch flag
-  int offset_bytes = 1;          // 1 (uint8, flag 0x80) or 2 (uint16, flag 0x8000)
-  std::vector<uint8_t> scatter;  // raw bytes
   int max_row = 0;
+  // Element rows contributing to each CSR row, in increasing cell order:
+  // pairs[pptr[r] .. pptr[r+1]) = cell*nl_row + i with rdofs[cell][i] == r.
+  std::vector<int32_t> pptr;     // nrows+1
+  std::vector<int32_t> pairs;    // ncells*nl_row
+  // goff[(cell*nl_row + i)*nl_col + j] = position of column cdofs[cell][j]
+  // inside row rdofs[cell][i] (uint8 if max_row <= 256, else uint16).
+  int offset_bytes = 1;
+  std::vector<uint8_t> goff;     // raw bytes
 };
 bool build_pattern(const std::vector<int32_t>& rdofs, int nl_row, i64 nrows,
                    const std::vector<int32_t>& cdofs, int nl_col, i64 ncols, i64 ncells,

@@ -10,7 +10,7 @@
 namespace bh {
 
 struct LaunchArgs {
-  enum Kind { BILINEAR = 0, SOURCE = 1, QPOINTS = 2 };
+  enum Kind { BILINEAR = 0, SOURCE = 1, QPOINTS = 2, GATHER = 3 };
   int kind = BILINEAR;
   int mode = BH_FP64;
   hipStream_t stream = nullptr;
@@ -22,14 +22,20 @@ struct LaunchArgs {
   int ncells_color = 0;
   int slot0 = 0;                   // slot of cells[0] in the scatter map
   int64_t ncells_total = 0;
-  // bilinear
+  // bilinear: element kernel writes ke, gather reduces ke into vals
+  void* ke = nullptr;
+  bool ke_fp64 = true;
+  const int32_t* pptr = nullptr;
+  const int32_t* pairs = nullptr;
+  const void* goff = nullptr;
+  int64_t nrows = 0;
+  int max_row = 0;
   const double* eta = nullptr;
   double eta_scale = 0.0;
   const double* mcoef = nullptr;
   double mass_scale = 0.0;
   bool mass = false;
   const int64_t* rowptr = nullptr;
-  const void* scatter = nullptr;
   int offset_bytes = 1;
   // source
   int fkind = BH_COEF_CONST;

@@ -1,24 +1,27 @@
 // bh_kernels.hip -- gfx950 kernels of the assembly path.
 //
-// One launch per cell colour.  A workgroup takes groups of CPB cells of that
-// colour (grid-stride), and for each group:
-//   1. stages the cells' geometric nodes in LDS,
-//   2. evaluates J, |J|, J^-1 at every quadrature point in FP64 and folds the
-//      integrand coefficients into per-point factors
-//         G_q = w_q |J_q| η J_q^-1 J_q^-T   (stiffness, symmetric D x D)
-//         m_q = w_q |J_q| c                 (mass)
-//   3. computes one row i of the element matrix per lane,
-//         K_ij = Σ_q ∇̂φ_qi^T G_q ∇̂φ_qj + m_q φ_qi φ_qj,
-//      with the reference tables (∇̂φ, φ) staged once per workgroup in LDS,
-//   4. adds the row into the CSR values through the colour-ordered scatter
-//      map: no two cells of one colour share a row, and colours run in launch
-//      order, so every entry is summed in a fixed order (no atomics).  The
-//      first colour to touch an entry stores instead of adding, so the values
-//      need no zero fill.
+// Matrix assembly is two launches:
+//   1. an element kernel over all cells (k_element_f64 here, the matrix-core
+//      variants in bh_mfma.hip).  A workgroup takes groups of CPB consecutive
+//      cells (grid-stride) and for each group
+//        - stages the cells' geometric nodes in LDS,
+//        - evaluates J, |J|, J^-1 at every quadrature point in FP64 and folds
+//          the coefficients into per-point factors
+//            G_q = w_q |J_q| η J_q^-1 J_q^-T (stiffness), m_q = w_q |J_q| c (mass),
+//        - computes one row i of the element matrix per lane,
+//            K_ij = Σ_q ∇̂φ_qi^T G_q ∇̂φ_qj + m_q φ_qi φ_qj,
+//          with the reference tables staged once per workgroup in LDS,
+//        - writes the row contiguously to the element workspace ke.
+//   2. k_gather: half a wave per CSR row sums, in FP64 and in increasing cell
+//      order, the element rows that touch the row (a sorted segmented
+//      reduction), into an LDS copy of the row, then writes the row once.
+//      No colouring, no atomics, no read-modify-write of the matrix: every
+//      entry is written exactly once, in a fixed summation order.
 // This replaces the per-cell lazy-operator loop of the reference
 // (__assemble_bilinear! assembler.jl:280-307, integrate_on_ref_element
-// integration.jl:18-105, ∂λξ_∂x ref2phys.jl:22-26, _append_bilinear!
-// assembler.jl:632-648 + sparse()).
+// integration.jl:18-105, ∂λξ_∂x ref2phys.jl:22-26) and its COO append +
+// sparse() (assembler.jl:632-648, :185), whose duplicate sum also runs in
+// cell order.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -29,21 +32,17 @@
 namespace bh {
 
 // -------------------------------------------------------------------------
-// Bilinear FP64 kernel
+// Element matrices, FP64
 // -------------------------------------------------------------------------
-template <int D, int NL, int NQ, int NG, bool MASS, typename OFF>
-__global__ __launch_bounds__(kBlock) void k_bilinear_f64(
+template <int D, int NL, int NQ, int NG, bool MASS>
+__global__ __launch_bounds__(kBlock) void k_element_f64(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
-    const int32_t* __restrict__ rdofs, const double* __restrict__ tab,
-    const int32_t* __restrict__ cells, int ncells_color, int slot0,
-    const double* __restrict__ eta, double eta_scale, const double* __restrict__ mcoef,
-    double mass_scale, const int64_t* __restrict__ rowptr, const OFF* __restrict__ scatter,
-    double* __restrict__ vals) {
+    const double* __restrict__ tab, int ncells, const double* __restrict__ eta, double eta_scale,
+    const double* __restrict__ mcoef, double mass_scale, double* __restrict__ ke) {
   using L = TabLayout<D, NL, NQ, NG>;
   constexpr int CPB = kBlock / NL;  // cells per group
   constexpr int NS = SymN<D>::N;
-  constexpr OFF FLAG = (OFF)(sizeof(OFF) == 1 ? 0x80 : 0x8000);
-  constexpr OFF MASK = (OFF)(FLAG - 1);
+  constexpr int KS = KeStride<NL>::value;
 
   __shared__ double s_tab[L::SIZE];
   __shared__ double s_x[CPB][NG][D];
@@ -52,7 +51,7 @@ __global__ __launch_bounds__(kBlock) void k_bilinear_f64(
   const int tid = threadIdx.x;
   for (int k = tid; k < L::SIZE; k += kBlock) s_tab[k] = tab[k];
 
-  const int ngroups = (ncells_color + CPB - 1) / CPB;
+  const int ngroups = (ncells + CPB - 1) / CPB;
   const int cs = tid / NL;  // cell slot of this lane in phase 3/4
   const int li = tid - cs * NL;
 
@@ -62,9 +61,8 @@ __global__ __launch_bounds__(kBlock) void k_bilinear_f64(
     // 1. stage nodes
     for (int k = tid; k < CPB * NG; k += kBlock) {
       int c = k / NG, n = k - c * NG;
-      if (base + c < ncells_color) {
-        int cell = cells[base + c];
-        int node = c2n[(int64_t)cell * NG + n];
+      if (base + c < ncells) {
+        int node = c2n[(int64_t)(base + c) * NG + n];
 #pragma unroll
         for (int a = 0; a < D; ++a) s_x[c][n][a] = coords[(int64_t)node * D + a];
       }
@@ -73,8 +71,8 @@ __global__ __launch_bounds__(kBlock) void k_bilinear_f64(
     // 2. geometry per (cell, q)
     for (int k = tid; k < CPB * NQ; k += kBlock) {
       int c = k / NQ, q = k - c * NQ;
-      if (base + c >= ncells_color) continue;
-      int cell = cells[base + c];
+      if (base + c >= ncells) continue;
+      int cell = base + c;
       double J[D][D];
 #pragma unroll
       for (int a = 0; a < D; ++a)
@@ -108,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_bilinear_f64(
     }
     __syncthreads();
     // 3. one element-matrix row per lane
-    const bool active = (cs < CPB) && (base + cs < ncells_color);
+    const bool active = (cs < CPB) && (base + cs < ncells);
     if (!active) continue;
     double acc[NL];
 #pragma unroll
@@ -137,18 +135,41 @@ __global__ __launch_bounds__(kBlock) void k_bilinear_f64(
         acc[j] = v;
       }
     }
-    // 4. colour-ordered scatter into the CSR row of test dof i
-    const int cell = cells[base + cs];
-    const int row = rdofs[(int64_t)cell * NL + li];
-    const int64_t rs = rowptr[row];
-    const OFF* sc = scatter + ((int64_t)(slot0 + base + cs)) * (NL * NL) + li;
+    // 4. row i of the element matrix, contiguous in the workspace
+    double* out = ke + ((int64_t)(base + cs) * NL + li) * KS;
 #pragma unroll
-    for (int j = 0; j < NL; ++j) {
-      OFF b = sc[j * NL];
-      int64_t pos = rs + (int64_t)(b & MASK);
-      if (b & FLAG) vals[pos] = acc[j];
-      else vals[pos] += acc[j];
+    for (int j = 0; j < NL; ++j) out[j] = acc[j];
+  }
+}
+
+// -------------------------------------------------------------------------
+// Row gather: vals[row] = Σ over (cell, i) touching row, in cell order, of
+// element row ke[cell][i][:] placed by goff.  Half a wave per row.
+// -------------------------------------------------------------------------
+template <typename T, typename OFF, int NL, int ROWCAP>
+__global__ __launch_bounds__(kBlock) void k_gather(const T* __restrict__ ke,
+                                                   const int32_t* __restrict__ pptr,
+                                                   const int32_t* __restrict__ pairs,
+                                                   const OFF* __restrict__ goff,
+                                                   const int64_t* __restrict__ rowptr,
+                                                   int64_t nrows, double* __restrict__ vals) {
+  constexpr int KS = KeStride<NL>::value;
+  constexpr int HW = kBlock / 32;
+  __shared__ double buf[HW][ROWCAP];
+  const int h = threadIdx.x >> 5;
+  const int l = threadIdx.x & 31;
+  for (int64_t row = (int64_t)blockIdx.x * HW + h; row < nrows; row += (int64_t)gridDim.x * HW) {
+    const int64_t rs = rowptr[row];
+    const int len = (int)(rowptr[row + 1] - rs);
+    for (int k = l; k < len; k += 32) buf[h][k] = 0.0;
+    const int p1 = pptr[row + 1];
+    for (int p = pptr[row]; p < p1; ++p) {
+      const int64_t pr = pairs[p];
+      const T* kr = ke + pr * KS;
+      const OFF* orow = goff + pr * NL;
+      for (int j = l; j < NL; j += 32) buf[h][orow[j]] += (double)kr[j];
     }
+    for (int k = l; k < len; k += 32) vals[rs + k] = buf[h][k];
   }
 }
 
@@ -249,33 +270,34 @@ template <int D, int NL, int NQ, int NG>
 static hipError_t launch_all(const LaunchArgs& A) {
   constexpr int CPB = kBlock / NL;
   if (A.kind == LaunchArgs::BILINEAR) {
-    int ng = (A.ncells_color + CPB - 1) / CPB;
+    int ng = (int)((A.ncells_total + CPB - 1) / CPB);
     dim3 grid(grid_for(ng));
-    if (A.offset_bytes == 1) {
-      auto sc = (const uint8_t*)A.scatter;
-      if (A.mass)
-        hipLaunchKernelGGL((k_bilinear_f64<D, NL, NQ, NG, true, uint8_t>), grid, dim3(kBlock), 0,
-                           A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.ncells_color,
-                           A.slot0, A.eta, A.eta_scale, A.mcoef, A.mass_scale, A.rowptr, sc,
-                           A.vals);
-      else
-        hipLaunchKernelGGL((k_bilinear_f64<D, NL, NQ, NG, false, uint8_t>), grid, dim3(kBlock), 0,
-                           A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.ncells_color,
-                           A.slot0, A.eta, A.eta_scale, A.mcoef, A.mass_scale, A.rowptr, sc,
-                           A.vals);
+    if (A.mass)
+      hipLaunchKernelGGL((k_element_f64<D, NL, NQ, NG, true>), grid, dim3(kBlock), 0, A.stream,
+                         A.coords, A.c2n, A.tab, (int)A.ncells_total, A.eta, A.eta_scale, A.mcoef,
+                         A.mass_scale, (double*)A.ke);
+    else
+      hipLaunchKernelGGL((k_element_f64<D, NL, NQ, NG, false>), grid, dim3(kBlock), 0, A.stream,
+                         A.coords, A.c2n, A.tab, (int)A.ncells_total, A.eta, A.eta_scale, A.mcoef,
+                         A.mass_scale, (double*)A.ke);
+  } else if (A.kind == LaunchArgs::GATHER) {
+    int64_t blocks = (A.nrows + kBlock / 32 - 1) / (kBlock / 32);
+    dim3 grid((unsigned)(blocks < 16384 ? (blocks < 1 ? 1 : blocks) : 16384));
+#define BH_G(T, OFFT, CAP)                                                                  \
+  hipLaunchKernelGGL((k_gather<T, OFFT, NL, CAP>), grid, dim3(kBlock), 0, A.stream,         \
+                     (const T*)A.ke, A.pptr, A.pairs, (const OFFT*)A.goff, A.rowptr, A.nrows, \
+                     A.vals)
+    const bool f64 = A.ke_fp64;
+    if (A.max_row <= 128 && A.offset_bytes == 1) {
+      if (f64) BH_G(double, uint8_t, 128); else BH_G(float, uint8_t, 128);
+    } else if (A.max_row <= 256 && A.offset_bytes == 1) {
+      if (f64) BH_G(double, uint8_t, 256); else BH_G(float, uint8_t, 256);
+    } else if (A.max_row <= 512) {
+      if (f64) BH_G(double, uint16_t, 512); else BH_G(float, uint16_t, 512);
     } else {
-      auto sc = (const uint16_t*)A.scatter;
-      if (A.mass)
-        hipLaunchKernelGGL((k_bilinear_f64<D, NL, NQ, NG, true, uint16_t>), grid, dim3(kBlock), 0,
-                           A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.ncells_color,
-                           A.slot0, A.eta, A.eta_scale, A.mcoef, A.mass_scale, A.rowptr, sc,
-                           A.vals);
-      else
-        hipLaunchKernelGGL((k_bilinear_f64<D, NL, NQ, NG, false, uint16_t>), grid, dim3(kBlock), 0,
-                           A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.ncells_color,
-                           A.slot0, A.eta, A.eta_scale, A.mcoef, A.mass_scale, A.rowptr, sc,
-                           A.vals);
+      return hipErrorInvalidValue;
     }
+#undef BH_G
   } else if (A.kind == LaunchArgs::SOURCE) {
     int ng = (A.ncells_color + CPB - 1) / CPB;
     hipLaunchKernelGGL((k_source_f64<D, NL, NQ, NG>), dim3(grid_for(ng)), dim3(kBlock), 0,

@@ -9,8 +9,9 @@
 // v_mfma_f32_16x16x4f32, one quadrature point per 4-deep k-step (the four
 // lanes groups of the k dimension are the D gradient components and the mass
 // component).  G_q = w_q|J_q|η J_q⁻¹J_q⁻ᵀ and m_q = w_q|J_q|c are formed in FP64
-// and rounded once to FP32; products accumulate in FP32; the global sum into
-// the CSR values is FP64 and colour-ordered, exactly as in the FP64 kernel.
+// and rounded once to FP32; products accumulate in FP32.  Element rows go to
+// the FP32 workspace; k_gather then sums them into the CSR values in FP64,
+// exactly as for the FP64 element kernel.
 //
 // PREC = 1 (BF16x3) runs the same contraction on the BF16 matrix cores with a
 // two-term split x = hi + lo (hi = bf16(x), lo = bf16(x - hi)) of both
@@ -41,14 +42,12 @@ __device__ __forceinline__ float from_bf16(__bf16 h) {
   return __uint_as_float(((uint32_t)v) << 16);
 }
 
-template <int D, int NL, int NQ, int NG, bool MASS, typename OFF, int PREC>
-__global__ __launch_bounds__(kBlock) void k_bilinear_mfma32(
+template <int D, int NL, int NQ, int NG, bool MASS, int PREC>
+__global__ __launch_bounds__(kBlock) void k_element_mfma(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
-    const int32_t* __restrict__ rdofs, const double* __restrict__ tab,
-    const int32_t* __restrict__ cells, int ncells_color, int slot0,
-    const double* __restrict__ eta, double eta_scale, const double* __restrict__ mcoef,
-    double mass_scale, const int64_t* __restrict__ rowptr, const OFF* __restrict__ scatter,
-    double* __restrict__ vals) {
+    const double* __restrict__ tab, int ncells, const double* __restrict__ eta,
+    double eta_scale, const double* __restrict__ mcoef, double mass_scale,
+    float* __restrict__ ke) {
   static_assert(NL <= 32, "two 16-row tiles cover at most 32 local dofs");
   static_assert(D + 1 <= 4, "k-step holds D gradient components + mass");
   using L = TabLayout<D, NL, NQ, NG>;
@@ -57,8 +56,7 @@ __global__ __launch_bounds__(kBlock) void k_bilinear_mfma32(
   constexpr int NT = (NCOL + 15) / 16;     // 16-wide column tiles
   constexpr int NS = SymN<D>::N;
   constexpr int ZERO = 7;                  // slot of s_g that holds 0
-  constexpr OFF FLAG = (OFF)(sizeof(OFF) == 1 ? 0x80 : 0x8000);
-  constexpr OFF MASK = (OFF)(FLAG - 1);
+  constexpr int KS = KeStride<NL>::value;
 
   __shared__ float s_r[NQ][NL][4];  // (∂φ̂ components, φ̂), FP32
   __shared__ double s_dg[NQ * NG * D];
@@ -117,14 +115,14 @@ __global__ __launch_bounds__(kBlock) void k_bilinear_mfma32(
     }
   }
 
-  const int ngroups = (ncells_color + CPB - 1) / CPB;
+  const int ngroups = (ncells + CPB - 1) / CPB;
   for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
     const int base = grp * CPB;
     __syncthreads();
     for (int k = tid; k < CPB * NG; k += kBlock) {
       int c = k / NG, n = k - c * NG;
-      if (base + c < ncells_color) {
-        int node = c2n[(int64_t)cells[base + c] * NG + n];
+      if (base + c < ncells) {
+        int node = c2n[(int64_t)(base + c) * NG + n];
 #pragma unroll
         for (int a = 0; a < D; ++a) s_x[c][n][a] = coords[(int64_t)node * D + a];
       }
@@ -134,11 +132,11 @@ __global__ __launch_bounds__(kBlock) void k_bilinear_mfma32(
     for (int k = tid; k < CPB * NQ; k += kBlock) {
       int c = k / NQ, q = k - c * NQ;
       float* gq = s_g[c][q];
-      if (base + c >= ncells_color) {
+      if (base + c >= ncells) {
         for (int t = 0; t < 8; ++t) gq[t] = 0.0f;
         continue;
       }
-      int cell = cells[base + c];
+      int cell = base + c;
       double J[D][D];
 #pragma unroll
       for (int a = 0; a < D; ++a)
@@ -177,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_bilinear_mfma32(
       const int n = t * 16 + am;
       const int c = n / NL;
       const int j = n - c * NL;
-      const bool valid = (n < NCOL) && (base + c < ncells_color);
+      const bool valid = (n < NCOL) && (base + c < ncells);
       const int cc = valid ? c : 0;
       const int jj = valid ? j : 0;
       floatx4 acc0 = {0.f, 0.f, 0.f, 0.f};
@@ -233,20 +231,13 @@ __global__ __launch_bounds__(kBlock) void k_bilinear_mfma32(
       }
       if (!valid) continue;
       // lane holds K[i][j] for i = rt*16 + 4*(lane/16) + v, column j of cell c
-      const int cell = cells[base + c];
-      const OFF* sc = scatter + (int64_t)(slot0 + base + c) * (NL * NL) + j * NL;
-      const int32_t* dofs = rdofs + (int64_t)cell * NL;
+      float* out = ke + (int64_t)(base + c) * NL * KS + j;
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int i = rt * 16 + 4 * ak + v;
-          if (i >= NL) continue;
-          const float kv = rt == 0 ? acc0[v] : acc1[v];
-          const OFF b = sc[i];
-          const int64_t pos = rowptr[dofs[i]] + (int64_t)(b & MASK);
-          if (b & FLAG) vals[pos] = (double)kv;
-          else vals[pos] += (double)kv;
+          if (i < NL) out[i * KS] = rt == 0 ? acc0[v] : acc1[v];
         }
     }
   }
@@ -255,29 +246,20 @@ __global__ __launch_bounds__(kBlock) void k_bilinear_mfma32(
 // ---------------------------------------------------------------------------
 template <int D, int NL, int NQ, int NG>
 static hipError_t launch_mfma32(const LaunchArgs& A) {
-  const bool bf = A.mode == BH_BF16X3;
   constexpr int CPB = 16;
-  int ng = (A.ncells_color + CPB - 1) / CPB;
+  const bool bf = A.mode == BH_BF16X3;
+  int ng = (int)((A.ncells_total + CPB - 1) / CPB);
   dim3 grid(ng < 2048 ? (ng < 1 ? 1 : ng) : 2048);
-#define BH_LAUNCH1(MASSV, OFFT, PRECV)                                                         \
-  hipLaunchKernelGGL((k_bilinear_mfma32<D, NL, NQ, NG, MASSV, OFFT, PRECV>), grid, dim3(kBlock), \
-                     0, A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.ncells_color,     \
-                     A.slot0, A.eta, A.eta_scale, A.mcoef, A.mass_scale, A.rowptr,             \
-                     (const OFFT*)A.scatter, A.vals)
-#define BH_LAUNCH(MASSV, OFFT)          \
-  do {                                  \
-    if (bf) BH_LAUNCH1(MASSV, OFFT, 1); \
-    else BH_LAUNCH1(MASSV, OFFT, 0);    \
-  } while (0)
-  if (A.offset_bytes == 1) {
-    if (A.mass) BH_LAUNCH(true, uint8_t);
-    else BH_LAUNCH(false, uint8_t);
+#define BH_LAUNCH(MASSV, PRECV)                                                               \
+  hipLaunchKernelGGL((k_element_mfma<D, NL, NQ, NG, MASSV, PRECV>), grid, dim3(kBlock), 0,    \
+                     A.stream, A.coords, A.c2n, A.tab, (int)A.ncells_total, A.eta, A.eta_scale, \
+                     A.mcoef, A.mass_scale, (float*)A.ke)
+  if (A.mass) {
+    if (bf) BH_LAUNCH(true, 1); else BH_LAUNCH(true, 0);
   } else {
-    if (A.mass) BH_LAUNCH(true, uint16_t);
-    else BH_LAUNCH(false, uint16_t);
+    if (bf) BH_LAUNCH(false, 1); else BH_LAUNCH(false, 0);
   }
 #undef BH_LAUNCH
-#undef BH_LAUNCH1
   return hipGetLastError();
 }
 

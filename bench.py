@@ -26,9 +26,12 @@ import numpy as np  # noqa: E402
 PEAK_HBM_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16"}
-KERNEL = {"fp64": "k_bilinear_f64<3,27,27,8,true,uint8_t>",
-          "fp32": "k_bilinear_mfma32<3,27,27,8,true,uint8_t,0>",
-          "bf16x3": "k_bilinear_mfma32<3,27,27,8,true,uint8_t,1>"}
+KERNEL = {"fp64": "k_element_f64<3,27,27,8,true>",
+          "fp32": "k_element_mfma<3,27,27,8,true,0>",
+          "bf16x3": "k_element_mfma<3,27,27,8,true,1>"}
+GATHER = {"fp64": "k_gather<double,uint8_t,27,128>",
+          "fp32": "k_gather<float,uint8_t,27,128>",
+          "bf16x3": "k_gather<float,uint8_t,27,128>"}
 
 
 def workload(n):
@@ -42,21 +45,25 @@ def workload(n):
     return ncells, ndofs, nnz, b_alg, f_alg
 
 
-def roofline(mode, launches, steps, f_alg, b_alg):
-    """Roofline of the dominant kernel (the element kernel, one launch per colour)."""
-    avg = float(np.mean(launches))
-    per = len(launches) / steps
-    f_launch, b_launch = f_alg / per, b_alg / per
+def kernel_rooflines(mode, launches, steps, f_alg, nnz):
+    """Per-kernel rooflines.  Each step is two launches: the element kernel
+    (algorithmic flops F_alg, bound by the matrix/vector FP peak of the mode)
+    and the row gather (algorithmic bytes: the 8*nnz CSR values it writes,
+    bound by HBM).  Returns (dominant, [element, gather])."""
+    t = np.asarray(launches, dtype=np.float64).reshape(steps, -1)
+    el_ms, ga_ms = float(t[:, 0].mean()), float(t[:, 1].mean())
     peak_f = PEAK_TFLOPS[mode]
-    if f_launch / (peak_f * 1e12) >= b_launch / (PEAK_HBM_GBS * 1e9):
-        ach = f_launch / (avg * 1e-3) / 1e12
-        return {"bound": "mfma", "achieved": ach, "peak": peak_f, "unit": "TFLOP/s",
-                "frac": ach / peak_f, "kernel": KERNEL[mode], "avg_launch_ms": avg,
-                "flops_per_launch": f_launch, "alg_bytes_per_launch": b_launch}
-    ach = b_launch / (avg * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": ach / PEAK_HBM_GBS, "kernel": KERNEL[mode], "avg_launch_ms": avg,
-            "flops_per_launch": f_launch, "alg_bytes_per_launch": b_launch}
+    el_ach = f_alg / (el_ms * 1e-3) / 1e12
+    ga_bytes = 8.0 * nnz
+    ga_ach = ga_bytes / (ga_ms * 1e-3) / 1e9
+    el = {"bound": "mfma", "achieved": el_ach, "peak": peak_f, "unit": "TFLOP/s",
+          "frac": el_ach / peak_f, "kernel": KERNEL[mode], "avg_launch_ms": el_ms,
+          "flops_per_launch": f_alg}
+    ga = {"bound": "hbm", "achieved": ga_ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+          "frac": ga_ach / PEAK_HBM_GBS, "kernel": GATHER[mode], "avg_launch_ms": ga_ms,
+          "alg_bytes_per_launch": ga_bytes}
+    dom = el if el_ms >= ga_ms else ga
+    return dom, [el, ga]
 
 
 def cpu_baseline(n_sample=32):
@@ -85,6 +92,9 @@ def main():
     ap.add_argument("--mode", default="all", choices=["all", "fp64", "fp32", "bf16x3"])
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one device copy of this many MiB first (calibrates the "
+                         "FETCH_SIZE/WRITE_SIZE counters against a known byte count)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -117,6 +127,13 @@ def main():
     backend = m.backend
     backend.set_stream(stream.cuda_stream)
     coef = [eta_d.data_ptr(), None]
+    if args.calib_copy_mib:
+        nb = args.calib_copy_mib * (1 << 20) // 8
+        src = torch.ones(nb, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     modes = ["fp64", "fp32", "bf16x3"] if args.mode == "all" else [args.mode]
     results = {}
     ref = None
@@ -141,11 +158,11 @@ def main():
         err = None
         if ref is not None and mode != "fp64":
             err = float((out - ref).abs().max() / ref.abs().max())
+        dom, kernels = kernel_rooflines(mode, launches, args.steps, f_alg, nnz)
         results[mode] = {"ms_per_step": ms_step, "gnnz_per_s": nnz / (ms_step * 1e-3) / 1e9,
                          "elements_per_s": ncells / (ms_step * 1e-3),
-                         "avg_launch_ms": float(np.mean(launches)),
                          "launches_per_step": len(launches) / args.steps,
-                         "rel_err_vs_fp64_gpu": err, "roofline": roofline(mode, launches, args.steps, f_alg, b_alg)}
+                         "rel_err_vs_fp64_gpu": err, "roofline": dom, "kernels": kernels}
         if mode != "fp64":
             del out
     head = "fp32" if "fp32" in results else modes[0]

@@ -16,8 +16,10 @@
  *   - Calls are ordered on the context's stream.  Host-`where` calls return
  *     after results are in host memory; device-`where` calls return after the
  *     kernels are enqueued (bh_sync waits).
- *   - Summation is deterministic: global sums are ordered by a cell colouring,
- *     never by float atomics, so repeated calls are bitwise identical.
+ *   - Summation is deterministic: matrix entries are sums over cells in cell
+ *     order (a segmented reduction per CSR row), vector entries are ordered by
+ *     a cell colouring; never float atomics, so repeated calls are bitwise
+ *     identical.
  */
 #ifndef BCUBEHIP_H
 #define BCUBEHIP_H
@@ -121,11 +123,13 @@ int bh_space_destroy(bh_space space);
 /* ---- sparsity pattern --------------------------------------------------- */
 /* The CSC structure sparse(I,J,X,m,n) produces (assembler.jl:185), built once
  * per (U, V) pair: union of element dof-pair graphs, rows sorted per column,
- * explicit zeros kept.  Also builds the cell colouring and the per-cell
- * scatter map used by the deterministic sum. */
+ * explicit zeros kept.  Also builds, per row, the list of element rows that
+ * touch it (in cell order) and their column positions: the deterministic
+ * segmented reduction of the assembly. */
 int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out);
+/* max_row: the longest CSR row (entries). */
 int bh_pattern_info(bh_pattern p, int64_t* nrows, int64_t* ncols, int64_t* nnz,
-                    int32_t* ncolors);
+                    int32_t* max_row);
 /* Julia SparseMatrixCSC{Float64,Int64} structure: colptr (ncols+1), rowval
  * (nnz), with index base `base` (1 for Julia).  Row-major CSR of the
  * transposed matrix is the same arrays; for U == V they coincide. */

@@ -105,12 +105,11 @@ def test_csr_pattern_bit_exact(cfg):
 
 
 def test_pattern_sizes_c2_formula():
-    """Q2 on an n^3 hexa grid: nnz = (4n+1)^3 * ... checked on the 1D stencil product."""
+    """Q2 on an n^3 hexa grid: the pattern is the tensor product of the 1D Q2
+    pattern on 2n+1 nodes (5 entries per interior vertex row, 3 per boundary
+    vertex row and per mid-edge row)."""
     n = 4
     plan = _plan(B.hexa_mesh(n + 1, n + 1, n + 1, backend=HOST), 2, 5)
-    # per axis the Q2 1D pattern on 2n+1 nodes has 5 entries per vertex row, 3 per mid row
-    # minus boundary truncation: sum over nodes of row length = (2n+1) + 2*(2n) + 2*(2n-1)
-    # for vertex-vertex couplings; simpler: compare with the tensor product of 1D counts
     nodes1 = 2 * n + 1
     row1 = []
     for i in range(nodes1):
@@ -121,12 +120,7 @@ def test_pattern_sizes_c2_formula():
             hi = min(nodes1 - 1, i + 2)
             row1.append(hi - lo + 1)
     assert plan.nnz == sum(row1) ** 3
-    assert plan.ncolors == 8
-
-
-def test_colouring_two_colours_in_1d_strip():
-    plan = _plan(B.rectangle_mesh(9, 2, backend=HOST), 1, 2)
-    assert plan.ncolors == 2
+    assert plan.max_row == 125
 
 
 def test_error_paths_report_without_corrupting_state():

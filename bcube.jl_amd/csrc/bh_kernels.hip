@@ -155,6 +155,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(const T* __restrict__ ke,
                                                    int64_t nrows, double* __restrict__ vals) {
   constexpr int KS = KeStride<NL>::value;
   constexpr int HW = kBlock / 32;
+  constexpr int PB = 8;                 // element rows loaded before they are added
+  constexpr int NJ = (NL + 31) / 32;    // element-row entries per lane
   __shared__ double buf[HW][ROWCAP];
   const int h = threadIdx.x >> 5;
   const int l = threadIdx.x & 31;
@@ -163,11 +165,33 @@ __global__ __launch_bounds__(kBlock) void k_gather(const T* __restrict__ ke,
     const int len = (int)(rowptr[row + 1] - rs);
     for (int k = l; k < len; k += 32) buf[h][k] = 0.0;
     const int p1 = pptr[row + 1];
-    for (int p = pptr[row]; p < p1; ++p) {
-      const int64_t pr = pairs[p];
-      const T* kr = ke + pr * KS;
-      const OFF* orow = goff + pr * NL;
-      for (int j = l; j < NL; j += 32) buf[h][orow[j]] += (double)kr[j];
+    for (int p = pptr[row]; p < p1; p += PB) {
+      const int np = p1 - p < PB ? p1 - p : PB;
+      T v[PB][NJ];
+      int o[PB][NJ];
+#pragma unroll
+      for (int k = 0; k < PB; ++k) {
+        if (k < np) {
+          const int64_t pr = pairs[p + k];
+#pragma unroll
+          for (int t = 0; t < NJ; ++t) {
+            const int j = l + 32 * t;
+            if (j < NL) {
+              v[k][t] = ke[pr * KS + j];
+              o[k][t] = goff[pr * NL + j];
+            }
+          }
+        }
+      }
+      // adds in pair (cell) order: the summation order is fixed
+#pragma unroll
+      for (int k = 0; k < PB; ++k) {
+        if (k < np) {
+#pragma unroll
+          for (int t = 0; t < NJ; ++t)
+            if (l + 32 * t < NL) buf[h][o[k][t]] += (double)v[k][t];
+        }
+      }
     }
     for (int k = l; k < len; k += 32) vals[rs + k] = buf[h][k];
   }

@@ -13,6 +13,9 @@
 // the FP32 workspace; k_gather then sums them into the CSR values in FP64,
 // exactly as for the FP64 element kernel.
 //
+// PREC = 2 (FP64) is the same contraction on v_mfma_f64_16x16x4f64 with the
+// per-point factors, tables and accumulation all in FP64.
+//
 // PREC = 1 (BF16x3) runs the same contraction on the BF16 matrix cores with a
 // two-term split x = hi + lo (hi = bf16(x), lo = bf16(x - hi)) of both
 // operands and three products hi·hi + hi·lo + lo·hi per 32-deep k-step
@@ -21,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "bh_device.h"
 #include "bh_kernels.h"
@@ -28,6 +32,7 @@
 namespace bh {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 // round-to-nearest-even FP32 -> BF16 (finite inputs)
@@ -47,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
     const double* __restrict__ tab, int ncells, const double* __restrict__ eta,
     double eta_scale, const double* __restrict__ mcoef, double mass_scale,
-    float* __restrict__ ke) {
+    void* __restrict__ ke_out) {
   static_assert(NL <= 32, "two 16-row tiles cover at most 32 local dofs");
   static_assert(D + 1 <= 4, "k-step holds D gradient components + mass");
   using L = TabLayout<D, NL, NQ, NG>;
@@ -57,25 +62,31 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
   constexpr int NS = SymN<D>::N;
   constexpr int ZERO = 7;                  // slot of s_g that holds 0
   constexpr int KS = KeStride<NL>::value;
+  using RT = std::conditional_t<PREC == 2, double, float>;  // tables, factors, accumulators
+  using AccT = std::conditional_t<PREC == 2, doublex4, floatx4>;
+  RT* __restrict__ ke = (RT*)ke_out;
 
-  __shared__ float s_r[NQ][NL][4];  // (∂φ̂ components, φ̂), FP32
+  __shared__ RT s_r[NQ][NL][4];  // (∂φ̂ components, φ̂)
   __shared__ double s_dg[NQ * NG * D];
   __shared__ double s_w[NQ];
   __shared__ double s_x[CPB][NG][D];
-  __shared__ float s_g[CPB][NQ][8];  // G (NS values), m, zeros
+  __shared__ RT s_g[CPB][NQ][8];  // G (NS values), m, zeros
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  for (int k = tid; k < NQ * NL * 4; k += kBlock) (&s_r[0][0][0])[k] = (float)tab[L::R + k];
+  for (int k = tid; k < NQ * NL * 4; k += kBlock) (&s_r[0][0][0])[k] = (RT)tab[L::R + k];
   for (int k = tid; k < NQ * NG * D; k += kBlock) s_dg[k] = tab[L::DG + k];
   for (int k = tid; k < NQ; k += kBlock) s_w[k] = tab[L::W + k];
 
   // A operand: lane supplies A[(ks, ak), i = rt*16 + am]
   const int am = lane & 15;
   const int ak = lane >> 4;
-  constexpr int NQA = PREC == 0 ? NQ : 1;
-  float areg[2][NQA];
+  // FP32 keeps the A operand in registers; FP64 reads it from the LDS copy of
+  // the tables (s_r) at each k-step to stay within the register budget.
+  constexpr bool A_IN_LDS = PREC == 2;
+  constexpr int NQA = (PREC == 0) ? NQ : 1;
+  RT areg[2][NQA];
   constexpr int NSTEP = (NQ + 7) / 8;  // BF16: 32-deep k-steps of 8 quadrature points
   constexpr int NSB = PREC == 1 ? NSTEP : 1;
   bf16x8 ahi[2][NSB], alo[2][NSB];
@@ -85,7 +96,9 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
     if constexpr (PREC == 0) {
 #pragma unroll
       for (int ks = 0; ks < NQ; ++ks)
-        areg[rt][ks] = (i < NL) ? (float)tab[L::R + (ks * NL + i) * 4 + ak] : 0.0f;
+        areg[rt][ks] = (i < NL) ? (RT)tab[L::R + (ks * NL + i) * 4 + ak] : (RT)0;
+    } else if constexpr (PREC == 2) {
+      (void)i;
     } else {
 #pragma unroll
       for (int st = 0; st < NSTEP; ++st)
@@ -131,9 +144,9 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
     // FP64 geometry -> FP32 per-point factors
     for (int k = tid; k < CPB * NQ; k += kBlock) {
       int c = k / NQ, q = k - c * NQ;
-      float* gq = s_g[c][q];
+      RT* gq = s_g[c][q];
       if (base + c >= ncells) {
-        for (int t = 0; t < 8; ++t) gq[t] = 0.0f;
+        for (int t = 0; t < 8; ++t) gq[t] = (RT)0;
         continue;
       }
       int cell = base + c;
@@ -163,11 +176,11 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
           double v = 0.0;
 #pragma unroll
           for (int t = 0; t < D; ++t) v += Ji[a][t] * Ji[b][t];
-          gq[s++] = (float)(ge * v);
+          gq[s++] = (RT)(ge * v);
         }
       double mc = mcoef ? mcoef[cell] * mass_scale : mass_scale;
-      gq[NS] = MASS ? (float)(wdet * mc) : 0.0f;
-      for (int t = NS + 1; t < 8; ++t) gq[t] = 0.0f;
+      gq[NS] = MASS ? (RT)(wdet * mc) : (RT)0;
+      for (int t = NS + 1; t < 8; ++t) gq[t] = (RT)0;
     }
     __syncthreads();
     // matrix-core contraction, one 16-column tile at a time per wave
@@ -178,17 +191,25 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
       const bool valid = (n < NCOL) && (base + c < ncells);
       const int cc = valid ? c : 0;
       const int jj = valid ? j : 0;
-      floatx4 acc0 = {0.f, 0.f, 0.f, 0.f};
-      floatx4 acc1 = {0.f, 0.f, 0.f, 0.f};
-      if constexpr (PREC == 0) {
+      AccT acc0 = {0, 0, 0, 0};
+      AccT acc1 = {0, 0, 0, 0};
+      if constexpr (PREC != 1) {
 #pragma unroll
         for (int ks = 0; ks < NQ; ++ks) {
-          const float* g = s_g[cc][ks];
-          const float* r = s_r[ks][jj];
-          float bval = g[sel[0]] * r[0] + g[sel[1]] * r[1] + g[sel[2]] * r[2] + g[sel[3]] * r[3];
-          bval = valid ? bval : 0.0f;
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[0][ks], bval, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[1][ks], bval, acc1, 0, 0, 0);
+          const RT* g = s_g[cc][ks];
+          const RT* r = s_r[ks][jj];
+          RT bval = g[sel[0]] * r[0] + g[sel[1]] * r[1] + g[sel[2]] * r[2] + g[sel[3]] * r[3];
+          bval = valid ? bval : (RT)0;
+          if constexpr (PREC == 0) {
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[0][ks], bval, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[1][ks], bval, acc1, 0, 0, 0);
+          } else {
+            static_assert(A_IN_LDS, "FP64 reads A from LDS");
+            const RT a0 = (am < NL) ? s_r[ks][am < NL ? am : 0][ak] : (RT)0;
+            const RT a1 = (16 + am < NL) ? s_r[ks][16 + am < NL ? 16 + am : 0][ak] : (RT)0;
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bval, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bval, acc1, 0, 0, 0);
+          }
         }
       } else {
 #pragma unroll
@@ -199,8 +220,8 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
             const int q = 8 * st + 2 * ak + h;
             float bv[4] = {0.f, 0.f, 0.f, 0.f};
             if (valid && q < NQ) {
-              const float* g = s_g[cc][q];
-              const float* r = s_r[q][jj];
+              const RT* g = s_g[cc][q];
+              const RT* r = s_r[q][jj];
               // row a of G·r for every component a, from the symmetric storage
               if constexpr (D == 3) {
                 bv[0] = g[0] * r[0] + g[1] * r[1] + g[2] * r[2];
@@ -231,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
       }
       if (!valid) continue;
       // lane holds K[i][j] for i = rt*16 + 4*(lane/16) + v, column j of cell c
-      float* out = ke + (int64_t)(base + c) * NL * KS + j;
+      RT* out = ke + (int64_t)(base + c) * NL * KS + j;
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
@@ -247,17 +268,21 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
 template <int D, int NL, int NQ, int NG>
 static hipError_t launch_mfma32(const LaunchArgs& A) {
   constexpr int CPB = 16;
-  const bool bf = A.mode == BH_BF16X3;
+  const int prec = A.mode == BH_BF16X3 ? 1 : (A.mode == BH_FP64 ? 2 : 0);
   int ng = (int)((A.ncells_total + CPB - 1) / CPB);
   dim3 grid(ng < 2048 ? (ng < 1 ? 1 : ng) : 2048);
 #define BH_LAUNCH(MASSV, PRECV)                                                               \
   hipLaunchKernelGGL((k_element_mfma<D, NL, NQ, NG, MASSV, PRECV>), grid, dim3(kBlock), 0,    \
                      A.stream, A.coords, A.c2n, A.tab, (int)A.ncells_total, A.eta, A.eta_scale, \
-                     A.mcoef, A.mass_scale, (float*)A.ke)
+                     A.mcoef, A.mass_scale, A.ke)
   if (A.mass) {
-    if (bf) BH_LAUNCH(true, 1); else BH_LAUNCH(true, 0);
+    if (prec == 1) BH_LAUNCH(true, 1);
+    else if (prec == 2) BH_LAUNCH(true, 2);
+    else BH_LAUNCH(true, 0);
   } else {
-    if (bf) BH_LAUNCH(false, 1); else BH_LAUNCH(false, 0);
+    if (prec == 1) BH_LAUNCH(false, 1);
+    else if (prec == 2) BH_LAUNCH(false, 2);
+    else BH_LAUNCH(false, 0);
   }
 #undef BH_LAUNCH
   return hipGetLastError();

@@ -25,7 +25,6 @@ struct bh_ctx_s {
   size_t nevents = 0;
   std::map<std::tuple<int, int, int>, double*> tables;  // (geo, order, nq1) -> device table
   bool capturing = false;
-  bool fp64_vector = false;  // FP64 element rows on the vector ALUs instead of MFMA
 };
 
 struct bh_graph_s {
@@ -564,11 +563,11 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
     A.mcoef = coef_ptr(*mass, where, p->d_coef + nc, nc, c->stream);
     A.mass_scale = mass->scale;
   }
-  // 1. element rows: matrix cores where a kernel exists (FP64 too), else the
-  //    FP64 vector kernel
+  // 1. element rows: matrix cores where an instantiation exists (FP64 too),
+  //    else the FP64 vector-ALU kernel
   const bool on_mfma = mfma_supported(m->geo, V->order, nq1);
   time_begin(c);
-  if (on_mfma && !(mode == BH_FP64 && c->fp64_vector)) BH_HIP(c, launch_mfma(m->geo, V->order, nq1, A));
+  if (on_mfma) BH_HIP(c, launch_mfma(m->geo, V->order, nq1, A));
   else BH_HIP(c, launch(m->geo, V->order, nq1, A));
   time_end(c);
   // 2. deterministic row reduction into the CSR values

@@ -251,13 +251,14 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
         }
       }
       if (!valid) continue;
-      // lane holds K[i][j] for i = rt*16 + 4*(lane/16) + v, column j of cell c
+      // lane holds K[i][j], column j of cell c, for rows i = rt*16 + 4*(lane/16) + v
+      // (FP32/BF16 16x16 accumulators) or i = rt*16 + lane/16 + 4*v (FP64 16x16x4)
       RT* out = ke + (int64_t)(base + c) * NL * KS + j;
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int i = rt * 16 + 4 * ak + v;
+          const int i = PREC == 2 ? rt * 16 + ak + 4 * v : rt * 16 + 4 * ak + v;
           if (i < NL) out[i * KS] = rt == 0 ? acc0[v] : acc1[v];
         }
     }

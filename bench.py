@@ -26,7 +26,7 @@ import numpy as np  # noqa: E402
 PEAK_HBM_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16"}
-KERNEL = {"fp64": "k_element_f64<3,27,27,8,true>",
+KERNEL = {"fp64": "k_element_mfma<3,27,27,8,true,2>",
           "fp32": "k_element_mfma<3,27,27,8,true,0>",
           "bf16x3": "k_element_mfma<3,27,27,8,true,1>"}
 GATHER = {"fp64": "k_gather<double,uint8_t,27,128>",

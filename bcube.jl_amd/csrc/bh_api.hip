@@ -444,9 +444,9 @@ int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out) {
   BH_HIP(c, dalloc(&p->d_pptr, h.pptr.size()));
   BH_HIP(c, hipMemcpy(p->d_pptr, h.pptr.data(), h.pptr.size() * sizeof(int32_t),
                       hipMemcpyHostToDevice));
-  BH_HIP(c, dalloc(&p->d_pairs, h.pairs.size()));
-  BH_HIP(c, hipMemcpy(p->d_pairs, h.pairs.data(), h.pairs.size() * sizeof(int32_t),
-                      hipMemcpyHostToDevice));
+  const std::vector<int32_t>& pl = h.pad ? h.pairs_padded : h.pairs;
+  BH_HIP(c, dalloc(&p->d_pairs, pl.size()));
+  BH_HIP(c, hipMemcpy(p->d_pairs, pl.data(), pl.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   BH_HIP(c, dalloc((uint8_t**)&p->d_goff, h.goff.size()));
   BH_HIP(c, hipMemcpy(p->d_goff, h.goff.data(), h.goff.size(), hipMemcpyHostToDevice));
   BH_HIP(c, dalloc(&p->d_vals, (size_t)h.nnz));
@@ -455,6 +455,8 @@ int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out) {
   h.goff.shrink_to_fit();
   h.pairs.clear();
   h.pairs.shrink_to_fit();
+  h.pairs_padded.clear();
+  h.pairs_padded.shrink_to_fit();
   *out = p.release();
   return BH_OK;
 }
@@ -574,6 +576,8 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
   A.kind = LaunchArgs::GATHER;
   A.pptr = p->d_pptr;
   A.pairs = p->d_pairs;
+  A.padded_pairs = h.pad > 0;
+  A.max_adj = h.pad;
   A.goff = p->d_goff;
   A.offset_bytes = h.offset_bytes;
   A.rowptr = p->d_rowptr;

@@ -546,6 +546,14 @@ bool build_pattern(const std::vector<int32_t>& rdofs, int nl_row, i64 nrows,
         p.pairs[q] = (int32_t)(c * nl_row + li);
       }
   });
+  p.max_adj = 0;
+  for (i64 r = 0; r < nrows; ++r) p.max_adj = std::max<int>(p.max_adj, p.pptr[r + 1] - p.pptr[r]);
+  if (p.max_adj <= 16) {
+    p.pad = p.max_adj <= 4 ? 4 : (p.max_adj <= 8 ? 8 : 16);
+    p.pairs_padded.assign((size_t)nrows * p.pad, -1);
+    for (i64 r = 0; r < nrows; ++r)
+      for (int q = p.pptr[r]; q < p.pptr[r + 1]; ++q) p.pairs_padded[r * p.pad + (q - p.pptr[r])] = p.pairs[q];
+  }
   if (p.max_row <= 256) p.offset_bytes = 1;
   else if (p.max_row <= 65536) p.offset_bytes = 2;
   else { err = "row longer than 65536 entries"; return false; }

@@ -61,6 +61,10 @@ struct HostPattern {
   // pairs[pptr[r] .. pptr[r+1]) = cell*nl_row + i with rdofs[cell][i] == r.
   std::vector<int32_t> pptr;     // nrows+1
   std::vector<int32_t> pairs;    // ncells*nl_row
+  int max_adj = 0;               // most element rows touching one CSR row
+  // the same lists padded to pad slots per row (-1 = empty) when max_adj <= 16
+  int pad = 0;
+  std::vector<int32_t> pairs_padded;
   // goff[(cell*nl_row + i)*nl_col + j] = position of column cdofs[cell][j]
   // inside row rdofs[cell][i] (uint8 if max_row <= 256, else uint16).
   int offset_bytes = 1;

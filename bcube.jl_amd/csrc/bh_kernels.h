@@ -26,7 +26,9 @@ struct LaunchArgs {
   void* ke = nullptr;
   bool ke_fp64 = true;
   const int32_t* pptr = nullptr;
-  const int32_t* pairs = nullptr;
+  const int32_t* pairs = nullptr;  // variable-length lists, or padded to max_adj slots
+  bool padded_pairs = false;
+  int max_adj = 0;
   const void* goff = nullptr;
   int64_t nrows = 0;
   int max_row = 0;

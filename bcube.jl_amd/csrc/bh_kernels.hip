@@ -145,7 +145,66 @@ __global__ __launch_bounds__(kBlock) void k_element_f64(
 // -------------------------------------------------------------------------
 // Row gather: vals[row] = Σ over (cell, i) touching row, in cell order, of
 // element row ke[cell][i][:] placed by goff.  Half a wave per row.
+//   k_gather_padded: pair lists padded to MAXP slots per row (-1 = empty);
+//     each half-wave handles RPH rows at once so that 2*RPH*MAXP element-row
+//     loads are in flight before the LDS adds.
+//   k_gather: variable-length lists (rows touched by more than 16 cells).
 // -------------------------------------------------------------------------
+template <typename T, typename OFF, int NL, int ROWCAP, int MAXP, int RPH>
+__global__ __launch_bounds__(kBlock) void k_gather_padded(
+    const T* __restrict__ ke, const int32_t* __restrict__ pairs, const OFF* __restrict__ goff,
+    const int64_t* __restrict__ rowptr, int64_t nrows, double* __restrict__ vals) {
+  constexpr int KS = KeStride<NL>::value;
+  constexpr int HW = kBlock / 32;
+  constexpr int NJ = (NL + 31) / 32;
+  __shared__ double buf[HW][RPH][ROWCAP];
+  const int h = threadIdx.x >> 5;
+  const int l = threadIdx.x & 31;
+  const int64_t step = (int64_t)gridDim.x * HW * RPH;
+  for (int64_t row0 = ((int64_t)blockIdx.x * HW + h) * RPH; row0 < nrows; row0 += step) {
+    int64_t rs[RPH];
+    int len[RPH];
+    int32_t pr[RPH][MAXP];
+#pragma unroll
+    for (int r = 0; r < RPH; ++r) {
+      const int64_t row = row0 + r;
+      const bool ok = row < nrows;
+      rs[r] = ok ? rowptr[row] : 0;
+      len[r] = ok ? (int)(rowptr[row + 1] - rs[r]) : 0;
+#pragma unroll
+      for (int k = 0; k < MAXP; ++k) pr[r][k] = ok ? pairs[row * MAXP + k] : -1;
+    }
+    T v[RPH][MAXP][NJ];
+    int o[RPH][MAXP][NJ];
+#pragma unroll
+    for (int r = 0; r < RPH; ++r)
+#pragma unroll
+      for (int k = 0; k < MAXP; ++k)
+#pragma unroll
+        for (int t = 0; t < NJ; ++t) {
+          const int j = l + 32 * t;
+          if (pr[r][k] >= 0 && j < NL) {
+            v[r][k][t] = ke[(int64_t)pr[r][k] * KS + j];
+            o[r][k][t] = goff[(int64_t)pr[r][k] * NL + j];
+          }
+        }
+#pragma unroll
+    for (int r = 0; r < RPH; ++r)
+      for (int k = l; k < len[r]; k += 32) buf[h][r][k] = 0.0;
+    // adds in pair (cell) order: the summation order is fixed
+#pragma unroll
+    for (int r = 0; r < RPH; ++r)
+#pragma unroll
+      for (int k = 0; k < MAXP; ++k)
+#pragma unroll
+        for (int t = 0; t < NJ; ++t)
+          if (pr[r][k] >= 0 && l + 32 * t < NL) buf[h][r][o[r][k][t]] += (double)v[r][k][t];
+#pragma unroll
+    for (int r = 0; r < RPH; ++r)
+      for (int k = l; k < len[r]; k += 32) vals[rs[r] + k] = buf[h][r][k];
+  }
+}
+
 template <typename T, typename OFF, int NL, int ROWCAP>
 __global__ __launch_bounds__(kBlock) void k_gather(const T* __restrict__ ke,
                                                    const int32_t* __restrict__ pptr,
@@ -183,7 +242,6 @@ __global__ __launch_bounds__(kBlock) void k_gather(const T* __restrict__ ke,
           }
         }
       }
-      // adds in pair (cell) order: the summation order is fixed
 #pragma unroll
       for (int k = 0; k < PB; ++k) {
         if (k < np) {
@@ -305,23 +363,49 @@ static hipError_t launch_all(const LaunchArgs& A) {
                          A.coords, A.c2n, A.tab, (int)A.ncells_total, A.eta, A.eta_scale, A.mcoef,
                          A.mass_scale, (double*)A.ke);
   } else if (A.kind == LaunchArgs::GATHER) {
-    int64_t blocks = (A.nrows + kBlock / 32 - 1) / (kBlock / 32);
-    dim3 grid((unsigned)(blocks < 16384 ? (blocks < 1 ? 1 : blocks) : 16384));
+    const bool f64 = A.ke_fp64;
+    if (A.padded_pairs) {
+      constexpr int RPH = 2;
+      int64_t blocks = (A.nrows + (kBlock / 32) * RPH - 1) / ((kBlock / 32) * RPH);
+      dim3 grid((unsigned)(blocks < 16384 ? (blocks < 1 ? 1 : blocks) : 16384));
+#define BH_GP(T, OFFT, CAP, MP)                                                                   \
+  hipLaunchKernelGGL((k_gather_padded<T, OFFT, NL, CAP, MP, RPH>), grid, dim3(kBlock), 0, A.stream, \
+                     (const T*)A.ke, A.pairs, (const OFFT*)A.goff, A.rowptr, A.nrows, A.vals)
+#define BH_GP_MP(T, OFFT, CAP)                         \
+  do {                                                 \
+    if (A.max_adj <= 4) BH_GP(T, OFFT, CAP, 4);        \
+    else if (A.max_adj <= 8) BH_GP(T, OFFT, CAP, 8);   \
+    else BH_GP(T, OFFT, CAP, 16);                      \
+  } while (0)
+      if (A.max_row <= 128 && A.offset_bytes == 1) {
+        if (f64) BH_GP_MP(double, uint8_t, 128); else BH_GP_MP(float, uint8_t, 128);
+      } else if (A.max_row <= 256 && A.offset_bytes == 1) {
+        if (f64) BH_GP_MP(double, uint8_t, 256); else BH_GP_MP(float, uint8_t, 256);
+      } else if (A.max_row <= 512) {
+        if (f64) BH_GP_MP(double, uint16_t, 512); else BH_GP_MP(float, uint16_t, 512);
+      } else {
+        return hipErrorInvalidValue;
+      }
+#undef BH_GP_MP
+#undef BH_GP
+    } else {
+      int64_t blocks = (A.nrows + kBlock / 32 - 1) / (kBlock / 32);
+      dim3 grid((unsigned)(blocks < 16384 ? (blocks < 1 ? 1 : blocks) : 16384));
 #define BH_G(T, OFFT, CAP)                                                                  \
   hipLaunchKernelGGL((k_gather<T, OFFT, NL, CAP>), grid, dim3(kBlock), 0, A.stream,         \
                      (const T*)A.ke, A.pptr, A.pairs, (const OFFT*)A.goff, A.rowptr, A.nrows, \
                      A.vals)
-    const bool f64 = A.ke_fp64;
-    if (A.max_row <= 128 && A.offset_bytes == 1) {
-      if (f64) BH_G(double, uint8_t, 128); else BH_G(float, uint8_t, 128);
-    } else if (A.max_row <= 256 && A.offset_bytes == 1) {
-      if (f64) BH_G(double, uint8_t, 256); else BH_G(float, uint8_t, 256);
-    } else if (A.max_row <= 512) {
-      if (f64) BH_G(double, uint16_t, 512); else BH_G(float, uint16_t, 512);
-    } else {
-      return hipErrorInvalidValue;
-    }
+      if (A.max_row <= 128 && A.offset_bytes == 1) {
+        if (f64) BH_G(double, uint8_t, 128); else BH_G(float, uint8_t, 128);
+      } else if (A.max_row <= 256 && A.offset_bytes == 1) {
+        if (f64) BH_G(double, uint8_t, 256); else BH_G(float, uint8_t, 256);
+      } else if (A.max_row <= 512) {
+        if (f64) BH_G(double, uint16_t, 512); else BH_G(float, uint16_t, 512);
+      } else {
+        return hipErrorInvalidValue;
+      }
 #undef BH_G
+    }
   } else if (A.kind == LaunchArgs::SOURCE) {
     int ng = (A.ncells_color + CPB - 1) / CPB;
     hipLaunchKernelGGL((k_source_f64<D, NL, NQ, NG>), dim3(grid_for(ng)), dim3(kBlock), 0,

@@ -82,10 +82,10 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
   // A operand: lane supplies A[(ks, ak), i = rt*16 + am]
   const int am = lane & 15;
   const int ak = lane >> 4;
-  // FP32 keeps the A operand in registers; FP64 reads it from the LDS copy of
-  // the tables (s_r) at each k-step to stay within the register budget.
-  constexpr bool A_IN_LDS = PREC == 2;
-  constexpr int NQA = (PREC == 0) ? NQ : 1;
+  // FP32/FP64 read the A operand from the LDS copy of the tables (s_r) at
+  // each k-step (it would take 54/108 VGPRs held in registers).
+  constexpr bool A_IN_LDS = PREC == 2 || PREC == 0;
+  constexpr int NQA = 1;
   RT areg[2][NQA];
   constexpr int NSTEP = (NQ + 7) / 8;  // BF16: 32-deep k-steps of 8 quadrature points
   constexpr int NSB = PREC == 1 ? NSTEP : 1;
@@ -93,11 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
 #pragma unroll
   for (int rt = 0; rt < 2; ++rt) {
     const int i = rt * 16 + am;
-    if constexpr (PREC == 0) {
-#pragma unroll
-      for (int ks = 0; ks < NQ; ++ks)
-        areg[rt][ks] = (i < NL) ? (RT)tab[L::R + (ks * NL + i) * 4 + ak] : (RT)0;
-    } else if constexpr (PREC == 2) {
+    if constexpr (PREC == 0 || PREC == 2) {
       (void)i;
     } else {
 #pragma unroll
@@ -200,13 +196,13 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
           const RT* r = s_r[ks][jj];
           RT bval = g[sel[0]] * r[0] + g[sel[1]] * r[1] + g[sel[2]] * r[2] + g[sel[3]] * r[3];
           bval = valid ? bval : (RT)0;
+          static_assert(A_IN_LDS, "A operand from LDS");
+          const RT a0 = (am < NL) ? s_r[ks][am < NL ? am : 0][ak] : (RT)0;
+          const RT a1 = (16 + am < NL) ? s_r[ks][16 + am < NL ? 16 + am : 0][ak] : (RT)0;
           if constexpr (PREC == 0) {
-            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[0][ks], bval, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(areg[1][ks], bval, acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bval, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bval, acc1, 0, 0, 0);
           } else {
-            static_assert(A_IN_LDS, "FP64 reads A from LDS");
-            const RT a0 = (am < NL) ? s_r[ks][am < NL ? am : 0][ak] : (RT)0;
-            const RT a1 = (16 + am < NL) ? s_r[ks][16 + am < NL ? 16 + am : 0][ak] : (RT)0;
             acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bval, acc0, 0, 0, 0);
             acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bval, acc1, 0, 0, 0);
           }

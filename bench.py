@@ -29,9 +29,9 @@ DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16"}
 KERNEL = {"fp64": "k_element_mfma<3,27,27,8,true,2>",
           "fp32": "k_element_mfma<3,27,27,8,true,0>",
           "bf16x3": "k_element_mfma<3,27,27,8,true,1>"}
-GATHER = {"fp64": "k_gather<double,uint8_t,27,128>",
-          "fp32": "k_gather<float,uint8_t,27,128>",
-          "bf16x3": "k_gather<float,uint8_t,27,128>"}
+GATHER = {"fp64": "k_gather_padded<double,unsigned char,27,128,8,2>",
+          "fp32": "k_gather_padded<float,unsigned char,27,128,8,2>",
+          "bf16x3": "k_gather_padded<float,unsigned char,27,128,8,2>"}
 
 
 def workload(n):
@@ -170,10 +170,12 @@ def main():
     roof = dict(r["roofline"])
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tf):
-        t = json.load(open(tf)).get(f"c2_{n}_{head}")
-        if t:
-            traffic = t["bytes_per_launch"]
+    if os.path.exists(tf) and n == 64:
+        measured = {k.replace(" ", ""): v for k, v in json.load(open(tf))["kernels"].items()}
+        traffic = measured.get(roof["kernel"].replace(" ", ""))
+        for mode_r in results.values():
+            for kr in mode_r["kernels"]:
+                kr["traffic"] = measured.get(kr["kernel"].replace(" ", ""))
     roof["traffic"] = traffic
     speed = None
     if "fp64" in results and "fp32" in results:

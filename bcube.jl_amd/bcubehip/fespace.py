@@ -50,10 +50,9 @@ class DofHandler:
         self.iglob = np.zeros(total, dtype=np.int64)
         self.offset = np.zeros(mesh.ncells, dtype=np.int64)
         n = C.c_int64()
-        ctx = mesh.backend.ctx if mesh.backend is not None else None
         rc = L.lib().bh_dofhandler(_host_mesh(mesh), fs.degree, int(self.continuous),
                                    self.iglob.ctypes.data, self.offset.ctypes.data, C.byref(n))
-        L.check(rc, None if ctx is None else ctx, "bh_dofhandler")
+        L.check(rc, None, "bh_dofhandler")  # host-only handle: no GPU context needed
         self.ndofs_tot = int(n.value)
         self._ndofs_cell = np.diff(np.append(self.offset, total))
 
@@ -68,22 +67,18 @@ class DofHandler:
         return self.iglob[o:o + self._ndofs_cell[icell]]
 
 
-_HOST_MESHES = {}
-
-
 def _host_mesh(mesh: Mesh):
-    """A context-free library mesh for numbering queries (no GPU needed)."""
-    key = id(mesh)
-    h = _HOST_MESHES.get(key)
-    if h is None or h[1] is not mesh:
-        hv = C.c_void_p()
+    """A context-free library mesh for numbering queries (no GPU needed),
+    cached on the mesh and freed with it."""
+    h = getattr(mesh, "_host_handle", None)
+    if h is None:
+        h = C.c_void_p()
         L.check(L.lib().bh_mesh_create(None, mesh.dim, mesh.nnodes, mesh.coords.ctypes.data,
                                        mesh.ncells, mesh.celltypes.ctypes.data,
                                        mesh.c2n_ptr.ctypes.data, mesh.c2n.ctypes.data,
-                                       C.byref(hv)), None, "bh_mesh_create")
-        h = (hv, mesh)
-        _HOST_MESHES[key] = h
-    return h[0]
+                                       C.byref(h)), None, "bh_mesh_create")
+        mesh._host_handle = h
+    return h
 
 
 class SingleFESpace:

@@ -188,6 +188,9 @@ class Mesh:
         try:
             if self._handle is not None:
                 L.lib().bh_mesh_destroy(self._handle)
+            h = getattr(self, "_host_handle", None)
+            if h is not None:
+                L.lib().bh_mesh_destroy(h)
         except Exception:
             pass
 

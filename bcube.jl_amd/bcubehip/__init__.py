@@ -6,7 +6,8 @@ mirror of Bcube.jl's assembly API (src/assembler/assembler.jl).
 """
 from ._lib import BcubeHipError, LIB_PATH, lib
 from .mesh import (AbstractBcubeBackend, BcubeBackendHIP, BcubeBackendHostOnly, Mesh,
-                   batch_meshes, get_bcube_backend, hexa_mesh, jitter_nodes, rectangle_mesh)
+                   batch_meshes, get_bcube_backend, hexa27_mesh, hexa8_to_hexa27, hexa_mesh,
+                   jitter_nodes, rectangle_mesh, sinusoidal_map)
 from .fespace import DofHandler, FunctionSpace, TestFESpace, TrialFESpace
 from .forms import (CellDomain, Measure, MeshCellData, PhysicalFunction, dot, grad,
                     integrate, nabla)
@@ -15,7 +16,8 @@ from .assembler import (BilinearPlan, assemble_bilinear, assemble_linear, assemb
 
 __all__ = [
     "BcubeHipError", "LIB_PATH", "lib", "AbstractBcubeBackend", "BcubeBackendHIP",
-    "BcubeBackendHostOnly", "Mesh", "batch_meshes", "get_bcube_backend", "hexa_mesh", "jitter_nodes",
+    "BcubeBackendHostOnly", "Mesh", "batch_meshes", "get_bcube_backend", "hexa27_mesh",
+    "hexa8_to_hexa27", "hexa_mesh", "jitter_nodes", "sinusoidal_map",
     "rectangle_mesh", "DofHandler", "FunctionSpace", "TestFESpace", "TrialFESpace",
     "CellDomain", "Measure", "MeshCellData", "PhysicalFunction", "dot", "grad", "integrate",
     "nabla", "BilinearPlan", "assemble_bilinear", "assemble_linear", "assemble_linear_",

@@ -253,6 +253,92 @@ def hexa_mesh(nx, ny, nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax
     return Mesh(xyz, np.full(c2n.shape[0], L.CELLTYPE["Hexa8"]), c2n.astype(np.int64), backend)
 
 
+# Reference coordinates of the 27 Hexa27 nodes in CGNS order (src/mesh/entity.jl:318-396):
+# vertices, the 12 edge midpoints of edges2nodes, the 6 face centres of
+# faces2nodes, then the cell centre.
+_H8_VERTS = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                      [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]])
+_H27_EDGES = [(0, 1), (1, 2), (2, 3), (3, 0), (0, 4), (1, 5), (2, 6), (3, 7),
+              (4, 5), (5, 6), (6, 7), (7, 4)]
+_H27_REF = np.concatenate([
+    _H8_VERTS,
+    np.array([(_H8_VERTS[a] + _H8_VERTS[b]) // 2 for a, b in _H27_EDGES]),
+    np.array([[0, 0, -1], [0, -1, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, 0, 1], [0, 0, 0]]),
+]).astype(np.int64)
+
+
+def hexa27_mesh(nx, ny, nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0,
+                backend: AbstractBcubeBackend | None = None) -> Mesh:
+    """(nx-1)(ny-1)(nz-1) Hexa27 cells (quadratic geometry) on the node lattice of
+    (2nx-1)(2ny-1)(2nz-1) points, x fastest, cells in hexa_mesh's order.
+
+    hexa_mesh (mesh_generator.jl:453-552) only builds order-1 meshes; this is
+    the same box with each cell's 27 nodes in CGNS order (entity.jl:318-396).
+    """
+    NX, NY, NZ = 2 * nx - 1, 2 * ny - 1, 2 * nz - 1
+    z, y, x = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
+    xyz = np.stack([xmin + x.ravel() * (xmax - xmin) / (NX - 1),
+                    ymin + y.ravel() * (ymax - ymin) / (NY - 1),
+                    zmin + z.ravel() * (zmax - zmin) / (NZ - 1)], axis=1)
+    kz, ky, kx = np.meshgrid(np.arange(nz - 1), np.arange(ny - 1), np.arange(nx - 1),
+                             indexing="ij")
+    cx = 2 * kx.ravel()[:, None] + 1 + _H27_REF[None, :, 0]
+    cy = 2 * ky.ravel()[:, None] + 1 + _H27_REF[None, :, 1]
+    cz = 2 * kz.ravel()[:, None] + 1 + _H27_REF[None, :, 2]
+    c2n = (cz * NY + cy) * NX + cx
+    return Mesh(xyz, np.full(c2n.shape[0], L.CELLTYPE["Hexa27"]), c2n.astype(np.int64),
+                backend)
+
+
+def sinusoidal_map(mesh: Mesh, amplitude: float = 0.05, seed: int = 0) -> np.ndarray:
+    """Curve a mesh by x_d += A sin(2 pi x + p_d0) sin(2 pi y + p_d1) sin(2 pi z + p_d2)
+    for each component d, phases p ~ U[0, 2 pi) from the seed."""
+    X = mesh.nodes().copy()
+    ph = np.random.default_rng(seed).uniform(0.0, 2.0 * np.pi, size=(mesh.dim, mesh.dim))
+    Y = X.copy()
+    for d in range(mesh.dim):
+        f = np.ones(X.shape[0])
+        for e in range(mesh.dim):
+            f = f * np.sin(2.0 * np.pi * X[:, e] + ph[d, e])
+        Y[:, d] = X[:, d] + amplitude * f
+    mesh.set_nodes(Y)
+    return Y
+
+
+def hexa8_to_hexa27(mesh: Mesh, backend: AbstractBcubeBackend | None = None) -> Mesh:
+    """The Hexa27 mesh whose cells are the trilinear Hexa8 cells of `mesh`:
+    mid-edge, mid-face and centre nodes placed at the trilinear map's values,
+    so both meshes describe the same geometry."""
+    if not np.all(mesh.celltypes == L.CELLTYPE["Hexa8"]):
+        raise ValueError("hexa8_to_hexa27 needs a Hexa8 mesh")
+    X = mesh.nodes()
+    c2n8 = mesh.c2n.reshape(-1, 8)
+    nodes = {}
+    out = [tuple(r) for r in X]
+    for i in range(len(out)):
+        nodes[("v", i)] = i
+    c2n = np.zeros((c2n8.shape[0], 27), dtype=np.int64)
+    ref = _H27_REF.astype(np.float64)
+    for c, cn in enumerate(c2n8):
+        P = X[cn]
+        for k in range(27):
+            if k < 8:
+                c2n[c, k] = cn[k]
+                continue
+            # the node's key: the vertex set whose mean it is
+            sel = [v for v in range(8) if np.all((ref[k] == 0) | (_H8_VERTS[v] == ref[k]))]
+            key = tuple(sorted(int(cn[v]) for v in sel))
+            if key not in nodes:
+                a, b, e = ref[k]
+                w = np.array([(1 + a * vx) * (1 + b * vy) * (1 + e * vz) / 8.0
+                              for vx, vy, vz in _H8_VERTS])
+                nodes[key] = len(out)
+                out.append(tuple(w @ P))
+            c2n[c, k] = nodes[key]
+    return Mesh(np.array(out), np.full(c2n.shape[0], L.CELLTYPE["Hexa27"]), c2n,
+                backend if backend is not None else mesh.backend)
+
+
 def jitter_nodes(mesh: Mesh, amplitude: float, h, seed: int = 0) -> np.ndarray:
     """Seeded uniform jitter of interior nodes by ±amplitude*h per axis.
     Boundary nodes stay put, so the domain (and Σ mass) is unchanged."""

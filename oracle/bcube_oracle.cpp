@@ -32,6 +32,7 @@
 #include <set>
 #include <vector>
 #include <algorithm>
+#include <array>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -737,11 +738,14 @@ static void geo_quad9(const double* X, const double* xi, Geo& g) {
   g.Jinv[2] = -g.J[2] / det; g.Jinv[3] = g.J[0] / det;
 }
 
+static void geo_hexa27(const double* X, const double* xi, Geo& g);
+
 static void geometry(int ct, const double* X, const double* xi, Geo& g) {
   switch (ct) {
     case QUAD4: geo_quad4(X, xi, g); return;
     case QUAD9: geo_quad9(X, xi, g); return;
     case HEXA8: geo_hexa8(X, xi, g); return;
+    case HEXA27: geo_hexa27(X, xi, g); return;
   }
   throw std::runtime_error("oracle: geometry not restated for this cell type");
 }
@@ -917,6 +921,299 @@ static void assemble_source(const Mesh& m, const std::vector<i64>& iglob, const 
 }
 
 // ---------------------------------------------------------------------------
+// Hexa27 geometry (mapping.jl:664-697).  mapping_jacobian for Hexa27 is the
+// ForwardDiff fallback (mapping.jl:166-168): restated with Dual<3>.
+// ---------------------------------------------------------------------------
+template <class T> static void hexa27_map(const double* X, const T* xi, T* out) {
+  const T &a = xi[0], &b = xi[1], &c = xi[2];
+  T w[27];
+  w[0] = a * b * c * (a - 1.0) * (b - 1.0) * (c - 1.0) / 8.0;
+  w[1] = a * b * c * (a + 1.0) * (b - 1.0) * (c - 1.0) / 8.0;
+  w[2] = a * b * c * (a + 1.0) * (b + 1.0) * (c - 1.0) / 8.0;
+  w[3] = a * b * c * (a - 1.0) * (b + 1.0) * (c - 1.0) / 8.0;
+  w[4] = a * b * c * (a - 1.0) * (b - 1.0) * (c + 1.0) / 8.0;
+  w[5] = a * b * c * (a + 1.0) * (b - 1.0) * (c + 1.0) / 8.0;
+  w[6] = a * b * c * (a + 1.0) * (b + 1.0) * (c + 1.0) / 8.0;
+  w[7] = a * b * c * (a - 1.0) * (b + 1.0) * (c + 1.0) / 8.0;
+  w[8] = -b * c * (a * a - 1.0) * (b - 1.0) * (c - 1.0) / 4.0;
+  w[9] = -a * c * (a + 1.0) * (b * b - 1.0) * (c - 1.0) / 4.0;
+  w[10] = -b * c * (a * a - 1.0) * (b + 1.0) * (c - 1.0) / 4.0;
+  w[11] = -a * c * (a - 1.0) * (b * b - 1.0) * (c - 1.0) / 4.0;
+  w[12] = -a * b * (a - 1.0) * (b - 1.0) * (c * c - 1.0) / 4.0;
+  w[13] = -a * b * (a + 1.0) * (b - 1.0) * (c * c - 1.0) / 4.0;
+  w[14] = -a * b * (a + 1.0) * (b + 1.0) * (c * c - 1.0) / 4.0;
+  w[15] = -a * b * (a - 1.0) * (b + 1.0) * (c * c - 1.0) / 4.0;
+  w[16] = -b * c * (a * a - 1.0) * (b - 1.0) * (c + 1.0) / 4.0;
+  w[17] = -a * c * (a + 1.0) * (b * b - 1.0) * (c + 1.0) / 4.0;
+  w[18] = -b * c * (a * a - 1.0) * (b + 1.0) * (c + 1.0) / 4.0;
+  w[19] = -a * c * (a - 1.0) * (b * b - 1.0) * (c + 1.0) / 4.0;
+  w[20] = c * (a * a - 1.0) * (b * b - 1.0) * (c - 1.0) / 2.0;
+  w[21] = b * (a * a - 1.0) * (b - 1.0) * (c * c - 1.0) / 2.0;
+  w[22] = a * (a + 1.0) * (b * b - 1.0) * (c * c - 1.0) / 2.0;
+  w[23] = b * (a * a - 1.0) * (b + 1.0) * (c * c - 1.0) / 2.0;
+  w[24] = a * (a - 1.0) * (b * b - 1.0) * (c * c - 1.0) / 2.0;
+  w[25] = c * (a * a - 1.0) * (b * b - 1.0) * (c + 1.0) / 2.0;
+  w[26] = -(a * a - 1.0) * (b * b - 1.0) * (c * c - 1.0);
+  for (int d = 0; d < 3; ++d) {
+    T s = w[0] * X[d];
+    for (int k = 1; k < 27; ++k) s = s + w[k] * X[3 * k + d];
+    out[d] = s;
+  }
+}
+
+static void geo_hexa27(const double* X, const double* xi, Geo& g) {
+  Dual<3> x[3];
+  for (int a = 0; a < 3; ++a) { x[a] = Dual<3>(xi[a]); x[a].d[a] = 1.0; }
+  Dual<3> o[3];
+  hexa27_map(X, x, o);
+  for (int a = 0; a < 3; ++a) {
+    g.x[a] = o[a].v;
+    for (int b = 0; b < 3; ++b) g.J[a * 3 + b] = o[a].d[b];
+  }
+  g.detabs = fabs(det3(g.J));
+  inv3(g.J, g.Jinv);
+}
+
+// Quad9 mapping of a face with 3D nodes (mapping.jl:540-568), its ForwardDiff
+// Jacobian (3x2) and the surfacic metric |det [J ν]| = |J1 x J2|
+// (mapping.jl:279-311, cell_normal :878-885).
+template <class T> static void quad9_map3(const double* X, const T* xi, T* out) {
+  const T &a = xi[0], &b = xi[1];
+  T l[9];
+  l[0] = a * b * (1.0 - a) * (1.0 - b) / 4.0;
+  l[1] = -a * b * (1.0 + a) * (1.0 - b) / 4.0;
+  l[2] = a * b * (1.0 + a) * (1.0 + b) / 4.0;
+  l[3] = -a * b * (1.0 - a) * (1.0 + b) / 4.0;
+  l[4] = -(1.0 + a) * (1.0 - a) * b * (1.0 - b) / 2.0;
+  l[5] = a * (1.0 + a) * (1.0 - b) * (1.0 + b) / 2.0;
+  l[6] = (1.0 - a) * (1.0 + a) * b * (1.0 + b) / 2.0;
+  l[7] = -a * (1.0 - a) * (1.0 - b) * (1.0 + b) / 2.0;
+  l[8] = (1.0 - a) * (1.0 + a) * (1.0 - b) * (1.0 + b);
+  for (int d = 0; d < 3; ++d) {
+    T s = l[0] * X[d];
+    for (int i = 1; i < 9; ++i) s = s + l[i] * X[3 * i + d];
+    out[d] = s;
+  }
+}
+static double face_metric_quad9(const double* X, const double* xi) {
+  Dual<2> x[2];
+  x[0] = Dual<2>(xi[0]); x[0].d[0] = 1;
+  x[1] = Dual<2>(xi[1]); x[1].d[1] = 1;
+  Dual<2> o[3];
+  quad9_map3(X, x, o);
+  double J1[3], J2[3];
+  for (int a = 0; a < 3; ++a) { J1[a] = o[a].d[0]; J2[a] = o[a].d[1]; }
+  double n[3] = {J1[1] * J2[2] - J1[2] * J2[1], J1[2] * J2[0] - J1[0] * J2[2],
+                 J1[0] * J2[1] - J1[1] * J2[0]};
+  double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  double nu[3] = {n[0] / nn, n[1] / nn, n[2] / nn};
+  double M[9] = {J1[0], J2[0], nu[0], J1[1], J2[1], nu[1], J1[2], J2[2], nu[2]};
+  return fabs(det3(M));
+}
+// Quad4 face with 3D nodes (mapping_jacobian Quad4 :467-476 is dimension-generic)
+static double face_metric_quad4(const double* X, const double* xi) {
+  double s = xi[0], t = xi[1];
+  double J1[3], J2[3];
+  for (int d = 0; d < 3; ++d) {
+    double P1 = X[d], P2 = X[3 + d], P3 = X[6 + d], P4 = X[9 + d];
+    double k = P1 - P2 + P3 - P4;
+    J1[d] = (-P1 + P2 + P3 - P4 + t * k) / 4;
+    J2[d] = (-P1 - P2 + P3 + P4 + s * k) / 4;
+  }
+  double n[3] = {J1[1] * J2[2] - J1[2] * J2[1], J1[2] * J2[0] - J1[0] * J2[2],
+                 J1[0] * J2[1] - J1[1] * J2[0]};
+  double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  double nu[3] = {n[0] / nn, n[1] / nn, n[2] / nn};
+  double M[9] = {J1[0], J2[0], nu[0], J1[1], J2[1], nu[1], J1[2], J2[2], nu[2]};
+  return fabs(det3(M));
+}
+
+static void geometry3(int ct, const double* X, const double* xi, Geo& g) {
+  if (ct == HEXA27) geo_hexa27(X, xi, g);
+  else geometry(ct, X, xi, g);
+}
+
+// Reference cube (shape.jl:212-287): vertex coordinates, faces2nodes, normals.
+static const double kCubeX[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                    {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+static const int kCubeF[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                                 {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}};
+static const double kCubeN[6][3] = {{0, 0, -1}, {0, -1, 0}, {1, 0, 0},
+                                    {0, 1, 0},  {-1, 0, 0}, {0, 0, 1}};
+
+// mapping_face(cshape, side[, permutation]) (mapping.jl:324-342): the Quad4
+// map (mapping.jl:460-465) of the face's reference vertices, taken in the order
+// faces2nodes(cshape, side)[permutation].
+static void mapping_face_cube(int side, const int* perm, const double* xf, double* xc) {
+  int v[4];
+  for (int k = 0; k < 4; ++k) v[k] = kCubeF[side][perm ? perm[k] : k];
+  double s = xf[0], t = xf[1];
+  for (int d = 0; d < 3; ++d) {
+    double p = (s - 1) * (t - 1) * kCubeX[v[0]][d] - (s + 1) * (t - 1) * kCubeX[v[1]][d] +
+               (s + 1) * (t + 1) * kCubeX[v[2]][d] - (s - 1) * (t + 1) * kCubeX[v[3]][d];
+    xc[d] = p / 4;
+  }
+}
+
+// cell_side (entity.jl:478-483): first face of the cell whose nodes are all in f2n.
+static int cell_side(int ct, const std::vector<i64>& c2n, const std::vector<i64>& f2n) {
+  Lists fl = faces2nodes(ct);
+  for (size_t s = 0; s < fl.size(); ++s) {
+    bool sub = true;
+    for (int ln : fl[s])
+      if (std::find(f2n.begin(), f2n.end(), c2n[ln]) == f2n.end()) { sub = false; break; }
+    if (sub) return (int)s;
+  }
+  return -1;
+}
+
+struct Faces {
+  std::vector<std::vector<i64>> f2n;  // face nodes in creator order
+  std::vector<std::array<i64, 2>> f2c;  // creator, neighbour (-1)
+  std::vector<int> ftype;
+};
+
+// _build_faces! (mesh.jl:328-374): faces in order of first appearance.
+static Faces build_faces(const Mesh& m) {
+  Faces F;
+  std::map<std::vector<i64>, i64> dict;
+  for (i64 c = 0; c < m.ncells(); ++c) {
+    std::vector<i64> nodes = m.cell(c);
+    Lists fl = faces2nodes(m.ctype[c]);
+    for (size_t j = 0; j < fl.size(); ++j) {
+      std::vector<i64> fn;
+      for (int ln : fl[j]) fn.push_back(nodes[ln]);
+      std::vector<i64> key = fn;
+      std::sort(key.begin(), key.end());
+      auto it = dict.find(key);
+      if (it == dict.end()) {
+        dict[key] = (i64)F.f2n.size();
+        F.f2n.push_back(fn);
+        F.f2c.push_back({c, -1});
+        int ct = m.ctype[c];
+        F.ftype.push_back(ct == HEXA27 ? QUAD9 : (ct == HEXA8 ? QUAD4 : -1));
+      } else {
+        F.f2c[it->second][1] = c;
+      }
+    }
+  }
+  return F;
+}
+
+// DG linear transport residual, the upwind form the reference would write as
+//   l(v) = ∫((c*u)⋅∇(v))dΩ - ∫(flux ∘ (side⁻(u), side⁺(u), jump(v), side⁻(nΓ)))dΓ
+//   flux(ui, uj, δv, n) = (c⋅n > 0 ? (c⋅n) ui : (c⋅n) uj) δv
+// on CellDomain and InteriorFaceDomain (assemble_linear assembler.jl:448-533,
+// face update :768-800, side⁻/side⁺ eval_point.jl:164-206, normal
+// mapping.jl:843-855, face metric :279-311).  u, r: ndofs x nfields, field fastest.
+static void dg_transport(const Mesh& m, int order, const double* cvec, int vol_degree,
+                         int face_degree, int nfields, const double* u, double* r) {
+  std::vector<i64> iglob, offset;
+  std::vector<int> nd;
+  i64 ndofs = dofhandler(m, order, false, iglob, offset, nd);
+  int nl = ndofs_lagrange(CUBE, order);
+  std::fill(r, r + ndofs * nfields, 0.0);
+  auto cellX = [&](i64 c) {
+    std::vector<i64> nodes = m.cell(c);
+    std::vector<double> X(nodes.size() * 3);
+    for (size_t k = 0; k < nodes.size(); ++k)
+      for (int a = 0; a < 3; ++a) X[k * 3 + a] = m.coords[nodes[k] * 3 + a];
+    return X;
+  };
+  // faces first: assemble_linear runs a MultiIntegration last-first
+  Faces F = build_faces(m);
+  std::vector<double> wf, xf;
+  tensor_rule(SQUARE, face_degree, Q_LEGENDRE, wf, xf);
+  int nqf = (int)wf.size();
+  std::vector<double> val_n(nl), val_p(nl), grd(nl * 3);
+  for (size_t f = 0; f < F.f2n.size(); ++f) {
+    if (F.f2c[f][1] < 0) continue;  // inner_faces (mesh.jl:552)
+    i64 cn = F.f2c[f][0], cp = F.f2c[f][1];
+    std::vector<i64> c2n_n = m.cell(cn), c2n_p = m.cell(cp);
+    int sn = cell_side(m.ctype[cn], c2n_n, F.f2n[f]);
+    int sp = cell_side(m.ctype[cp], c2n_p, F.f2n[f]);
+    // permutation of side⁺ (eval_point.jl:174-206) on the cube vertices
+    int perm[4];
+    for (int k = 0; k < 4; ++k) {
+      i64 gp = c2n_p[kCubeF[sp][k]];
+      perm[k] = -1;
+      for (int q = 0; q < 4; ++q)
+        if (c2n_n[kCubeF[sn][q]] == gp) perm[k] = q;
+    }
+    std::vector<double> Xn = cellX(cn), Xp = cellX(cp);
+    std::vector<double> Xf(F.f2n[f].size() * 3);
+    for (size_t k = 0; k < F.f2n[f].size(); ++k)
+      for (int a = 0; a < 3; ++a) Xf[k * 3 + a] = m.coords[F.f2n[f][k] * 3 + a];
+    std::vector<double> acc_n((size_t)nl * nfields, 0.0), acc_p((size_t)nl * nfields, 0.0);
+    for (int q = 0; q < nqf; ++q) {
+      double metric = F.ftype[f] == QUAD9 ? face_metric_quad9(Xf.data(), &xf[2 * q])
+                                          : face_metric_quad4(Xf.data(), &xf[2 * q]);
+      double xin[3], xip[3];
+      mapping_face_cube(sn, nullptr, &xf[2 * q], xin);
+      mapping_face_cube(sp, perm, &xf[2 * q], xip);
+      Geo gn;
+      geometry3(m.ctype[cn], Xn.data(), xin, gn);
+      // normal: normalize(Jinv^T n_ref)
+      double nv[3];
+      for (int a = 0; a < 3; ++a)
+        nv[a] = gn.Jinv[0 * 3 + a] * kCubeN[sn][0] + gn.Jinv[1 * 3 + a] * kCubeN[sn][1] +
+                gn.Jinv[2 * 3 + a] * kCubeN[sn][2];
+      double nn = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+      for (int a = 0; a < 3; ++a) nv[a] /= nn;
+      double cdn = (cvec[0] * nv[0] + cvec[1] * nv[1]) + cvec[2] * nv[2];
+      tensor_basis(CUBE, order, xin, val_n.data(), grd.data());
+      tensor_basis(CUBE, order, xip, val_p.data(), grd.data());
+      for (int fi = 0; fi < nfields; ++fi) {
+        double un = 0, up = 0;
+        for (int j = 0; j < nl; ++j) {
+          un += u[(iglob[offset[cn] + j] - 1) * nfields + fi] * val_n[j];
+          up += u[(iglob[offset[cp] + j] - 1) * nfields + fi] * val_p[j];
+        }
+        double flux = cdn > 0 ? cdn * un : cdn * up;
+        for (int i = 0; i < nl; ++i) {
+          // -∫ flux jump(v): n side -flux φ⁻, p side +flux φ⁺
+          double gn_i = -(flux * val_n[i]);
+          double gp_i = -(flux * (-val_p[i]));
+          acc_n[(size_t)i * nfields + fi] += wf[q] * (metric * gn_i);
+          acc_p[(size_t)i * nfields + fi] += wf[q] * (metric * gp_i);
+        }
+      }
+    }
+    for (int i = 0; i < nl; ++i)
+      for (int fi = 0; fi < nfields; ++fi) {
+        r[(iglob[offset[cn] + i] - 1) * nfields + fi] += acc_n[(size_t)i * nfields + fi];
+        r[(iglob[offset[cp] + i] - 1) * nfields + fi] += acc_p[(size_t)i * nfields + fi];
+      }
+  }
+  // volume: ∫((c*u)⋅∇(v))dΩ
+  std::vector<double> wv, xv;
+  int nqv = tensor_rule(CUBE, vol_degree, Q_LEGENDRE, wv, xv);
+  std::vector<double> val(nl), rg(nl * 3), pg(nl * 3);
+  for (i64 c : element_order(m)) {
+    std::vector<double> Xc = cellX(c);
+    std::vector<double> acc((size_t)nl * nfields, 0.0);
+    for (int q = 0; q < nqv; ++q) {
+      Geo g;
+      geometry3(m.ctype[c], Xc.data(), &xv[3 * q], g);
+      tensor_basis(CUBE, order, &xv[3 * q], val.data(), rg.data());
+      phys_grad(3, nl, rg.data(), g.Jinv, pg.data());
+      for (int fi = 0; fi < nfields; ++fi) {
+        double uq = 0;
+        for (int j = 0; j < nl; ++j) uq += u[(iglob[offset[c] + j] - 1) * nfields + fi] * val[j];
+        for (int i = 0; i < nl; ++i) {
+          double v = ((cvec[0] * uq) * pg[i * 3 + 0] + (cvec[1] * uq) * pg[i * 3 + 1]) +
+                     (cvec[2] * uq) * pg[i * 3 + 2];
+          acc[(size_t)i * nfields + fi] += wv[q] * (g.detabs * v);
+        }
+      }
+    }
+    for (int i = 0; i < nl; ++i)
+      for (int fi = 0; fi < nfields; ++fi)
+        r[(iglob[offset[c] + i] - 1) * nfields + fi] += acc[(size_t)i * nfields + fi];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // C ABI used by tests (ctypes)
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
@@ -1044,6 +1341,31 @@ void* or_assemble_linear(int dim, int64_t nnodes, const double* coords, int64_t 
     assemble_source(m, r->iglob, r->offset, order, fsrc, fconst, degree, r->b);
     return r;
   } catch (std::exception& e) { g_err = e.what(); return nullptr; }
+}
+
+// DG transport residual (see dg_transport); returns ndofs or -1.
+int64_t or_dg_transport(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                        const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n, int order,
+                        const double* c, int vol_degree, int face_degree, int nfields,
+                        const double* u, double* r) {
+  try {
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    dg_transport(m, order, c, vol_degree, face_degree, nfields, u, r);
+    std::vector<i64> ig, off;
+    std::vector<int> nd;
+    return dofhandler(m, order, false, ig, off, nd);
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+int or_geometry27(const double* X, const double* xi, double* J, double* Jinv, double* detabs,
+                  double* x) {
+  Geo g;
+  geo_hexa27(X, xi, g);
+  std::copy(g.J, g.J + 9, J);
+  std::copy(g.Jinv, g.Jinv + 9, Jinv);
+  *detabs = g.detabs;
+  std::copy(g.x, g.x + 3, x);
+  return 0;
 }
 
 int64_t or_result_ndofs(void* h) { return ((Result*)h)->ndofs; }

@@ -69,6 +69,10 @@ def lib():
         L.or_result_vector.argtypes = [C.c_void_p, _d]
         L.or_result_dofs.argtypes = [C.c_void_p, _l, _l]
         L.or_result_free.argtypes = [C.c_void_p]
+        L.or_dg_transport.argtypes = mesh_args + [C.c_int, _d, C.c_int, C.c_int, C.c_int, _d, _d]
+        L.or_dg_transport.restype = C.c_int64
+        L.or_geometry27.argtypes = [_d, _d, _d, _d, _d, _d]
+        L.or_geometry27.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -217,6 +221,35 @@ def assemble_linear(mesh: OMesh, order, source="sinsin", const=0.0, degree=2, co
     finally:
         lib().or_result_free(h)
     return b
+
+
+def dg_transport(mesh: OMesh, order, c, u, vol_degree, face_degree=None):
+    """DG upwind linear-transport residual, r = l(v) for
+    l(v) = int((c*u).grad(v))dOmega - int(upwind flux * jump(v))dGamma over the interior
+    faces.  u: (ndofs, nfields) nodal values of the discontinuous Lagrange space
+    (cell-by-cell numbering); returns r with the same shape."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    if u.ndim == 1:
+        u = u[:, None]
+    nf = u.shape[1]
+    r = np.zeros_like(u)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    fd = vol_degree if face_degree is None else face_degree
+    n = lib().or_dg_transport(*mesh.args(), order, c, vol_degree, fd, nf, u, r)
+    if n < 0:
+        raise RuntimeError(_err())
+    if n != u.shape[0]:
+        raise ValueError(f"u has {u.shape[0]} rows, the space has {n} dofs")
+    return r
+
+
+def geometry27(cnodes, xi):
+    """Hexa27 map x(xi), Jacobian J, J^-1 and |det J| (ForwardDiff-style)."""
+    X = np.ascontiguousarray(cnodes, dtype=np.float64).reshape(27, 3)
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    J = np.zeros(9); Ji = np.zeros(9); d = np.zeros(1); x = np.zeros(3)
+    lib().or_geometry27(X, xi, J, Ji, d, x)
+    return x, J.reshape(3, 3), Ji.reshape(3, 3), float(d[0])
 
 
 def csc_to_scipy(colptr, rowval, nzval, n):

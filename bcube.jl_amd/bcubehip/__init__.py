@@ -9,8 +9,10 @@ from .mesh import (AbstractBcubeBackend, BcubeBackendHIP, BcubeBackendHostOnly, 
                    batch_meshes, get_bcube_backend, hexa27_mesh, hexa8_to_hexa27, hexa_mesh,
                    jitter_nodes, rectangle_mesh, sinusoidal_map)
 from .fespace import DofHandler, FunctionSpace, TestFESpace, TrialFESpace
-from .forms import (CellDomain, Measure, MeshCellData, PhysicalFunction, dot, grad,
-                    integrate, nabla)
+from .forms import (CellDomain, FEFunction, InteriorFaceDomain, Measure, MeshCellData,
+                    PhysicalFunction, UpwindFlux, compose, dot, get_face_normals, grad,
+                    integrate, jump, nabla, side_minus, side_n, side_p, side_plus)
+from .dg import DGTransport, dg_dof_coordinates
 from .assembler import (BilinearPlan, assemble_bilinear, assemble_linear, assemble_linear_,
                         quadrature_points)
 
@@ -19,7 +21,9 @@ __all__ = [
     "BcubeBackendHostOnly", "Mesh", "batch_meshes", "get_bcube_backend", "hexa27_mesh",
     "hexa8_to_hexa27", "hexa_mesh", "jitter_nodes", "sinusoidal_map",
     "rectangle_mesh", "DofHandler", "FunctionSpace", "TestFESpace", "TrialFESpace",
-    "CellDomain", "Measure", "MeshCellData", "PhysicalFunction", "dot", "grad", "integrate",
-    "nabla", "BilinearPlan", "assemble_bilinear", "assemble_linear", "assemble_linear_",
+    "CellDomain", "FEFunction", "InteriorFaceDomain", "Measure", "MeshCellData",
+    "PhysicalFunction", "UpwindFlux", "compose", "dot", "get_face_normals", "grad", "integrate",
+    "jump", "nabla", "side_minus", "side_n", "side_p", "side_plus", "DGTransport",
+    "dg_dof_coordinates", "BilinearPlan", "assemble_bilinear", "assemble_linear", "assemble_linear_",
     "quadrature_points",
 ]

@@ -26,8 +26,9 @@ EXPORTED = [
     "bh_mesh_destroy", "bh_dofhandler", "bh_space_create", "bh_space_create_from_dofs",
     "bh_space_info", "bh_space_dofs", "bh_space_destroy", "bh_pattern_build", "bh_pattern_info",
     "bh_pattern_export_csc", "bh_pattern_values_device", "bh_pattern_destroy",
-    "bh_assemble_bilinear", "bh_assemble_linear", "bh_quadrature_points", "bh_graph_begin",
-    "bh_graph_end", "bh_graph_launch", "bh_graph_destroy",
+    "bh_assemble_bilinear", "bh_assemble_linear", "bh_quadrature_points", "bh_dg_create",
+    "bh_dg_info", "bh_dg_update_geometry", "bh_dg_transport_residual", "bh_dg_minv_axpy",
+    "bh_dg_destroy", "bh_graph_begin", "bh_graph_end", "bh_graph_launch", "bh_graph_destroy",
 ]
 
 
@@ -87,6 +88,12 @@ def lib():
         "bh_assemble_bilinear": (C.c_int, [vp, C.c_int, P(BhTerm), C.c_int, C.c_int, vp, C.c_int]),
         "bh_assemble_linear": (C.c_int, [vp, C.c_int, P(BhTerm), C.c_int, C.c_int, vp, C.c_int]),
         "bh_quadrature_points": (C.c_int, [vp, C.c_int, vp, C.c_int, P(i32)]),
+        "bh_dg_create": (C.c_int, [vp, C.c_int, vp, C.c_int, P(vp)]),
+        "bh_dg_info": (C.c_int, [vp, P(i64), P(i64), P(i32)]),
+        "bh_dg_update_geometry": (C.c_int, [vp]),
+        "bh_dg_transport_residual": (C.c_int, [vp, i32, vp, vp, C.c_int]),
+        "bh_dg_minv_axpy": (C.c_int, [vp, i32, vp, vp, dbl, vp, vp, dbl, vp]),
+        "bh_dg_destroy": (C.c_int, [vp]),
         "bh_graph_begin": (C.c_int, [vp]),
         "bh_graph_end": (C.c_int, [vp, P(vp)]),
         "bh_graph_launch": (C.c_int, [vp]),

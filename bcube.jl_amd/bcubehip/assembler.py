@@ -14,7 +14,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import _lib as L
-from .forms import Placeholder, lower_bilinear, lower_linear
+from .forms import Placeholder, lower_bilinear, lower_dg_transport, lower_linear
 
 _COEF = {"const": L.COEF_CONST, "cell": L.COEF_CELL, "qp": L.COEF_QP}
 _FORM = {"stiffness": L.FORM_STIFFNESS, "mass": L.FORM_MASS, "source": L.FORM_SOURCE}
@@ -169,9 +169,27 @@ def assemble_linear_(b: np.ndarray, l, V, mode: str = "fp64") -> None:
                                        b.ctypes.data, L.HOST), ctx, "bh_assemble_linear")
 
 
+def _dg_plan(V, c, degree, mode):
+    from .dg import DGTransport
+    cache = V.__dict__.setdefault("_dg_plans", {})
+    key = (tuple(float(x) for x in c), int(degree), mode)
+    if key not in cache:
+        cache[key] = DGTransport(V, c, degree, mode)
+    return cache[key]
+
+
 def assemble_linear(l, V, T=np.float64, mode: str = "fp64") -> np.ndarray:
-    """assemble_linear(l, V; T=Float64) (assembler.jl:448-458)."""
+    """assemble_linear(l, V; T=Float64) (assembler.jl:448-458).
+
+    The upwind DG transport form (see bcubehip.dg) goes to the DG kernels; its
+    FEFunction may carry several fields, (nfields, ndofs), and the result has
+    the same shape (FP32 in mode "fp32")."""
     V = getattr(V, "U", V)
+    items = _integrals(l(Placeholder("test", V)))
+    dg = lower_dg_transport(items)
+    if dg is not None:
+        c, u, degree = dg
+        return _dg_plan(V, c, degree, mode).residual(u.values)
     b = np.zeros(V.get_ndofs(), dtype=np.float64)
     assemble_linear_(b, l, V, mode)
     return b

@@ -22,6 +22,8 @@ import numpy as np
 
 
 class Expr:
+    __array_ufunc__ = None  # numpy arrays (a velocity c) defer to Expr.__rmul__
+
     def __mul__(self, o):
         return Mul(self, _wrap(o))
 
@@ -40,12 +42,94 @@ def _wrap(x):
         return x
     if isinstance(x, numbers.Real):
         return Const(float(x))
+    if isinstance(x, (tuple, list, np.ndarray)):
+        return VecConst(x)
     raise TypeError(f"cannot use {type(x).__name__} in a form")
 
 
 class Const(Expr):
     def __init__(self, v):
         self.v = v
+
+
+class VecConst(Expr):
+    """A constant vector, e.g. the velocity c in (c*u)."""
+
+    def __init__(self, v):
+        self.v = np.asarray(v, dtype=np.float64).reshape(-1)
+
+
+class FEFunction(Expr):
+    """FEFunction(U, values) (src/fespace/fefunction.jl): a discrete field of U.
+    `values` has shape (ndofs,) or, for several fields evaluated together,
+    (nfields, ndofs)."""
+
+    def __init__(self, U, values):
+        self.space = getattr(U, "U", U)
+        self.values = values
+
+
+class FaceNormal(Expr):
+    """get_face_normals(dΓ) (domain.jl:677-690, FaceNormal cellfunction.jl:379-426)."""
+
+    def __init__(self, measure):
+        self.measure = measure
+
+
+class Side(Expr):
+    """side⁻(a) / side⁺(a) (cellfunction.jl:283-306)."""
+
+    def __init__(self, which, a):
+        self.which, self.a = which, a
+
+
+class Jump(Expr):
+    """jump(a) = side⁻(a) - side⁺(a) (cellfunction.jl:308)."""
+
+    def __init__(self, a):
+        self.a = a
+
+
+class UpwindFlux:
+    """The upwind transport flux (ui, uj, δv, n) -> (c⋅n > 0 ? (c⋅n)*ui : (c⋅n)*uj) * δv
+    as a value for compose()."""
+
+    def __init__(self, c):
+        self.c = np.asarray(c, dtype=np.float64).reshape(-1)
+
+    def __call__(self, ui, uj, dv, n):
+        cn = float(np.dot(self.c, n))
+        return (cn * ui if cn > 0 else cn * uj) * dv
+
+
+class Compose(Expr):
+    """f ∘ (a, b, ...) (Julia composition applied to a tuple of operators)."""
+
+    def __init__(self, f, args):
+        self.f, self.args = f, [_wrap(a) for a in args]
+
+
+def side_n(a):
+    return Side("n", _wrap(a) if not isinstance(a, FaceNormal) else a)
+
+
+def side_p(a):
+    return Side("p", _wrap(a) if not isinstance(a, FaceNormal) else a)
+
+
+side_minus, side_plus = side_n, side_p
+
+
+def jump(a):
+    return Jump(_wrap(a))
+
+
+def compose(f, *args):
+    return Compose(f, args)
+
+
+def get_face_normals(measure):
+    return FaceNormal(measure)
 
 
 class MeshCellData(Expr):
@@ -109,12 +193,19 @@ class CellDomain:
         self.mesh = mesh
 
 
+class InteriorFaceDomain:
+    """InteriorFaceDomain(mesh) (domain.jl:186-198): faces with two cells."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+
+
 class Measure:
     """Measure(domain, degree): Gauss-Legendre of that degree (measure.jl:30-55)."""
 
     def __init__(self, domain, degree: int):
-        if not isinstance(domain, CellDomain):
-            raise NotImplementedError("only CellDomain measures are supported")
+        if not isinstance(domain, (CellDomain, InteriorFaceDomain)):
+            raise NotImplementedError("only CellDomain and InteriorFaceDomain measures are supported")
         self.domain = domain
         self.degree = int(degree)
 
@@ -126,6 +217,12 @@ class Integral:
 
     def __add__(self, o):
         return MultiIntegration([self]) + o
+
+    def __sub__(self, o):
+        return MultiIntegration([self]) - o
+
+    def __neg__(self):
+        return Integral(Mul(Const(-1.0), self.expr), self.measure)
 
     def terms(self):
         return [self]
@@ -143,6 +240,13 @@ class MultiIntegration:
         if isinstance(o, MultiIntegration):
             return MultiIntegration(self.items + o.items)
         raise TypeError("can only add integrals")
+
+    def __sub__(self, o):
+        if isinstance(o, Integral):
+            return self + (-o)
+        if isinstance(o, MultiIntegration):
+            return MultiIntegration(self.items + [-it for it in o.items])
+        raise TypeError("can only subtract integrals")
 
     def terms(self):
         return list(self.items)
@@ -227,3 +331,57 @@ def lower_linear(expr):
         kind, data, scale = _coef([f for f in fs if f is not vs[0]])
         return "source", kind, data, scale
     raise NotImplementedError("linear integrand is not of the form f v")
+
+
+def lower_dg_transport(items):
+    """Recognise the upwind DG transport form
+
+        ∫((c*u)⋅∇(v))dΩ - ∫(flux ∘ (side⁻(u), side⁺(u), jump(v), side⁻(nΓ)))dΓ
+
+    (in either order).  Returns (c, u, degree) or None when the form is not it."""
+    vol = face = None
+    for it in items:
+        if isinstance(it.measure.domain, CellDomain):
+            vol = it if vol is None else False
+        elif isinstance(it.measure.domain, InteriorFaceDomain):
+            face = it if face is None else False
+    if not vol or not face or len(items) != 2:
+        return None
+    fs = _factors(vol.expr, [])
+    if len(fs) != 1 or not isinstance(fs[0], Dot):
+        return None
+    d = fs[0]
+    sides = [_factors(d.a, []), _factors(d.b, [])]
+    if isinstance(sides[0][0], Grad):
+        sides.reverse()
+    cu, g = sides
+    if not (len(g) == 1 and isinstance(g[0], Grad) and _is(g[0].a, Placeholder, "test")):
+        return None
+    cs = [f for f in cu if isinstance(f, VecConst)]
+    us = [f for f in cu if isinstance(f, FEFunction)]
+    if len(cs) != 1 or len(us) != 1 or len(cu) != 2:
+        return None
+    c, u = cs[0].v, us[0]
+    ff = _factors(face.expr, [])
+    comp = [f for f in ff if isinstance(f, Compose)]
+    sign = 1.0
+    for f in ff:
+        if isinstance(f, Const):
+            sign *= f.v
+    if len(comp) != 1 or len(ff) != 1 + sum(isinstance(f, Const) for f in ff) or sign != -1.0:
+        return None
+    cp = comp[0]
+    if not isinstance(cp.f, UpwindFlux) or not np.array_equal(cp.f.c, c) or len(cp.args) != 4:
+        return None
+    a0, a1, a2, a3 = cp.args
+    ok = (isinstance(a0, Side) and a0.which == "n" and a0.a is u and
+          isinstance(a1, Side) and a1.which == "p" and a1.a is u and
+          isinstance(a2, Jump) and _is(a2.a, Placeholder, "test") and
+          isinstance(a3, Side) and a3.which == "n" and isinstance(a3.a, FaceNormal))
+    if not ok or c.size != 3:
+        return None
+    if vol.measure.degree != face.measure.degree:
+        raise NotImplementedError("the DG transport kernels use one Gauss rule on cells and faces")
+    if vol.measure.domain.mesh is not face.measure.domain.mesh:
+        raise NotImplementedError("cell and face measures must be on one mesh")
+    return c, u, vol.measure.degree

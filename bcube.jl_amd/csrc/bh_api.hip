@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -10,6 +11,7 @@
 #include <tuple>
 #include <vector>
 
+#include "bh_dg.h"
 #include "bh_internal.h"
 #include "bh_kernels.h"
 
@@ -77,6 +79,23 @@ struct bh_pattern_s {
   void* d_ke = nullptr;      // element-row workspace, ncells*nl*KS FP64 (FP32 uses half)
   double* d_vals = nullptr;
   double* d_coef = nullptr;  // staged host coefficients (2 * ncells)
+};
+
+struct bh_dg_s {
+  bh_space V = nullptr;
+  int dev = -1;
+  int n = 0;        // nodes (= Gauss points) per axis
+  int mode = BH_FP64;
+  int64_t ncells = 0, ndofs = 0, nfaces = 0;
+  double c[3] = {0, 0, 0};
+  DGTables tab;
+  void* d_vfac = nullptr;
+  void* d_ffac = nullptr;
+  int32_t* d_cs = nullptr;
+  int32_t* d_fcell = nullptr;
+  int8_t* d_fside = nullptr;
+  void* d_stage = nullptr;  // host-`where` staging: u then r
+  size_t stage_bytes = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -280,7 +299,7 @@ int bh_mesh_create(bh_ctx c, int dim, int64_t nnodes, const double* coords, int6
     if (v < 0 || v >= nnodes) return fail(c, BH_ERR_RANGE, "c2n: node index out of range");
   h.celltype = h.ctype[0];
   h.uniform = std::all_of(h.ctype.begin(), h.ctype.end(), [&](int t) { return t == h.celltype; });
-  if (h.uniform && (h.celltype == BH_QUAD4 || h.celltype == BH_HEXA8)) {
+  if (h.uniform && (h.celltype == BH_QUAD4 || h.celltype == BH_HEXA8 || h.celltype == BH_HEXA27)) {
     m->geo = h.celltype;
     m->npc = nnodes_of(h.celltype);
   }
@@ -517,7 +536,8 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
   if (where == BH_HOST && !nzval) return fail(c, BH_ERR_ARG, "nzval is NULL");
   if (where == BH_HOST && c->capturing)
     return fail(c, BH_ERR_ARG, "host-`where` assembly cannot be captured in a graph");
-  if (m->geo < 0) return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
+  if (m->geo < 0 || m->geo == BH_HEXA27)
+    return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
   const bh_term* stiff = nullptr;
   const bh_term* mass = nullptr;
   for (int k = 0; k < nterms; ++k) {
@@ -605,7 +625,8 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
   if (!b) return fail(c, BH_ERR_ARG, "b is NULL");
   if (where == BH_HOST && c->capturing)
     return fail(c, BH_ERR_ARG, "host-`where` assembly cannot be captured in a graph");
-  if (m->geo < 0) return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
+  if (m->geo < 0 || m->geo == BH_HEXA27)
+    return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
   int nq1 = gauss_points_per_axis(quad_degree);
   if (!kernel_supported(m->geo, V->order, nq1))
     return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this (cell type, order, quadrature)");
@@ -677,6 +698,253 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
   return BH_OK;
 }
 
+}  // extern "C"
+
+// ---- discontinuous Galerkin transport -------------------------------------
+namespace {
+
+// Gauss rule and 1D operators for N nodes / N points per axis; Φ⁻¹ by
+// Gauss-Jordan elimination with partial pivoting.
+bool make_dg_tables(int N, DGTables& t) {
+  std::memset(&t, 0, sizeof(t));
+  std::vector<double> gx, gw;
+  gauss_legendre(N, gx, gw);
+  double v[8], dv[8];
+  for (int q = 0; q < N; ++q) {
+    t.gx[q] = gx[q];
+    t.gw[q] = gw[q];
+    lagrange_1d(N - 1, gx[q], v, dv);
+    for (int i = 0; i < N; ++i) {
+      t.phi[q * 4 + i] = v[i];
+      t.dphi[q * 4 + i] = dv[i];
+    }
+  }
+  double A[4][8];
+  for (int q = 0; q < N; ++q)
+    for (int i = 0; i < 2 * N; ++i) A[q][i] = i < N ? t.phi[q * 4 + i] : (i - N == q ? 1.0 : 0.0);
+  for (int col = 0; col < N; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < N; ++r)
+      if (std::fabs(A[r][col]) > std::fabs(A[piv][col])) piv = r;
+    if (std::fabs(A[piv][col]) < 1e-14) return false;
+    for (int i = 0; i < 2 * N; ++i) std::swap(A[col][i], A[piv][i]);
+    double d = A[col][col];
+    for (int i = 0; i < 2 * N; ++i) A[col][i] /= d;
+    for (int r = 0; r < N; ++r) {
+      if (r == col) continue;
+      double m = A[r][col];
+      for (int i = 0; i < 2 * N; ++i) A[r][i] -= m * A[col][i];
+    }
+  }
+  // A now holds [I | Φ⁻¹] with Φ[q][i] rows q: (Φ⁻¹)[i][q] = A[i][N + q]
+  for (int i = 0; i < N; ++i)
+    for (int q = 0; q < N; ++q) t.pinv[i * 4 + q] = A[i][N + q];
+  return true;
+}
+
+int dg_geometry(bh_dg g) {
+  bh_mesh m = g->V->mesh;
+  bh_ctx c = m->ctx;
+  DGGeomArgs a;
+  a.stream = c->stream;
+  a.geo = m->geo;
+  a.n = g->n;
+  a.coords = m->d_coords;
+  a.c2n = m->d_c2n;
+  a.ncells = g->ncells;
+  a.face_cell = g->d_fcell;
+  a.face_side = g->d_fside;
+  a.nfaces = g->nfaces;
+  for (int k = 0; k < 3; ++k) a.c[k] = g->c[k];
+  a.fp32 = g->mode == BH_FP32;
+  a.vfac = g->d_vfac;
+  a.ffac = g->d_ffac;
+  BH_HIP(c, launch_dg_geometry(a, g->tab));
+  return BH_OK;
+}
+
+void dg_free(bh_dg g) {
+  if (g->dev >= 0) {
+    (void)hipSetDevice(g->dev);
+    (void)hipFree(g->d_vfac);
+    (void)hipFree(g->d_ffac);
+    (void)hipFree(g->d_cs);
+    (void)hipFree(g->d_fcell);
+    (void)hipFree(g->d_fside);
+    (void)hipFree(g->d_stage);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int bh_dg_create(bh_space V, int quad_degree, const double* velocity, int mode, bh_dg* out) {
+  if (!V || !velocity || !out) return BH_ERR_ARG;
+  *out = nullptr;
+  bh_mesh m = V->mesh;
+  bh_ctx c = m->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "DG transport needs a mesh created with a context");
+  if (c->capturing) return fail(c, BH_ERR_ARG, "bh_dg_create cannot be captured in a graph");
+  if (mode != BH_FP64 && mode != BH_FP32)
+    return fail(c, BH_ERR_UNSUPPORTED, "DG transport: mode must be BH_FP64 or BH_FP32");
+  if (V->continuous) return fail(c, BH_ERR_ARG, "DG transport needs a discontinuous space");
+  if (m->geo != BH_HEXA8 && m->geo != BH_HEXA27)
+    return fail(c, BH_ERR_UNSUPPORTED, "DG transport needs a uniform Hexa8 or Hexa27 mesh");
+  if (V->order < 1 || V->order > 3)
+    return fail(c, BH_ERR_UNSUPPORTED, "DG transport kernels cover orders 1..3");
+  const int N = V->order + 1;
+  if (gauss_points_per_axis(quad_degree) != N)
+    return fail(c, BH_ERR_UNSUPPORTED,
+                "DG transport: quad_degree must give order+1 Gauss points per axis");
+  const int64_t nc = m->h.ncells;
+  const int nl = N * N * N;
+  if (V->nl != nl || V->ndofs != nc * nl)
+    return fail(c, BH_ERR_ARG, "DG transport: space size does not match the mesh");
+  for (int64_t k = 0; k < nc * nl; ++k)
+    if (V->dofs[k] != k)
+      return fail(c, BH_ERR_UNSUPPORTED, "DG transport needs the cell-by-cell dof numbering");
+  if (int r = set_device(c)) return r;
+  auto g = std::make_unique<bh_dg_s>();
+  g->V = V;
+  g->dev = c->device;
+  g->n = N;
+  g->mode = mode;
+  g->ncells = nc;
+  g->ndofs = V->ndofs;
+  for (int k = 0; k < 3; ++k) g->c[k] = velocity[k];
+  if (!make_dg_tables(N, g->tab)) return fail(c, BH_ERR_ARG, "singular 1D interpolation matrix");
+  HostDGFaces F;
+  std::string err;
+  if (!build_dg_faces(m->h, F, err)) return fail(c, BH_ERR_UNSUPPORTED, err);
+  g->nfaces = F.nfaces;
+  const size_t es = mode == BH_FP32 ? 4 : 8;
+  bh_dg raw = g.get();
+  auto fail_free = [&](int code, const std::string& msg) {
+    dg_free(raw);
+    return fail(c, code, msg);
+  };
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = hipMalloc(&g->d_vfac, std::max<size_t>(1, (size_t)nc * 4 * nl * es));
+  if (e == hipSuccess) e = hipMalloc(&g->d_ffac, std::max<size_t>(1, (size_t)F.nfaces * N * N * es));
+  if (e == hipSuccess) e = dalloc(&g->d_cs, F.cs.size());
+  if (e == hipSuccess) e = dalloc(&g->d_fcell, F.cell_n.size());
+  if (e == hipSuccess) e = dalloc(&g->d_fside, F.side_n.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(g->d_cs, F.cs.data(), F.cs.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess && F.nfaces > 0)
+    e = hipMemcpy(g->d_fcell, F.cell_n.data(), F.cell_n.size() * sizeof(int32_t),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess && F.nfaces > 0)
+    e = hipMemcpy(g->d_fside, F.side_n.data(), F.side_n.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail_free(BH_ERR_HIP, std::string("bh_dg_create: ") + hipGetErrorString(e));
+  if (int r = dg_geometry(raw)) {
+    dg_free(raw);
+    return r;
+  }
+  e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail_free(BH_ERR_HIP, std::string("bh_dg_create: ") + hipGetErrorString(e));
+  *out = g.release();
+  return BH_OK;
+}
+
+int bh_dg_info(bh_dg g, int64_t* ndofs, int64_t* nfaces, int32_t* nl) {
+  if (!g) return BH_ERR_ARG;
+  if (ndofs) *ndofs = g->ndofs;
+  if (nfaces) *nfaces = g->nfaces;
+  if (nl) *nl = g->n * g->n * g->n;
+  return BH_OK;
+}
+
+int bh_dg_update_geometry(bh_dg g) {
+  if (!g) return BH_ERR_ARG;
+  bh_ctx c = g->V->mesh->ctx;
+  if (int r = set_device(c)) return r;
+  return dg_geometry(g);
+}
+
+int bh_dg_transport_residual(bh_dg g, int32_t nfields, const void* u, void* r, int where) {
+  if (!g) return BH_ERR_ARG;
+  bh_ctx c = g->V->mesh->ctx;
+  if (nfields <= 0 || !u || !r) return fail(c, BH_ERR_ARG, "bh_dg_transport_residual: bad arguments");
+  if (where == BH_HOST && c->capturing)
+    return fail(c, BH_ERR_ARG, "host-`where` calls cannot be captured in a graph");
+  if (int e = set_device(c)) return e;
+  const size_t es = g->mode == BH_FP32 ? 4 : 8;
+  const size_t bytes = (size_t)nfields * g->ndofs * es;
+  DGArgs a;
+  a.stream = c->stream;
+  a.n = g->n;
+  a.fp32 = g->mode == BH_FP32;
+  a.ndofs = g->ndofs;
+  a.ncells = g->ncells;
+  a.nfields = nfields;
+  a.vfac = g->d_vfac;
+  a.ffac = g->d_ffac;
+  a.cs = g->d_cs;
+  a.u = u;
+  a.r = r;
+  if (where == BH_HOST) {
+    if (g->stage_bytes < 2 * bytes) {
+      (void)hipFree(g->d_stage);
+      g->d_stage = nullptr;
+      g->stage_bytes = 0;
+      BH_HIP(c, hipMalloc(&g->d_stage, 2 * bytes));
+      g->stage_bytes = 2 * bytes;
+    }
+    a.u = g->d_stage;
+    a.r = (char*)g->d_stage + bytes;
+    BH_HIP(c, hipMemcpyAsync((void*)a.u, u, bytes, hipMemcpyHostToDevice, c->stream));
+  }
+  time_begin(c);
+  BH_HIP(c, launch_dg_residual(a, g->tab));
+  time_end(c);
+  if (where == BH_HOST) {
+    BH_HIP(c, hipMemcpyAsync(r, a.r, bytes, hipMemcpyDeviceToHost, c->stream));
+    BH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return BH_OK;
+}
+
+int bh_dg_minv_axpy(bh_dg g, int32_t nfields, const void* r, const void* x1, double a1, void* out1,
+                    const void* x2, double a2, void* out2) {
+  if (!g) return BH_ERR_ARG;
+  bh_ctx c = g->V->mesh->ctx;
+  if (nfields <= 0 || !r || !x1 || !out1 || (out2 && !x2))
+    return fail(c, BH_ERR_ARG, "bh_dg_minv_axpy: bad arguments");
+  if (int e = set_device(c)) return e;
+  DGArgs a;
+  a.stream = c->stream;
+  a.n = g->n;
+  a.fp32 = g->mode == BH_FP32;
+  a.ndofs = g->ndofs;
+  a.ncells = g->ncells;
+  a.nfields = nfields;
+  a.vfac = g->d_vfac;
+  a.r = const_cast<void*>(r);
+  a.x1 = x1;
+  a.a1 = a1;
+  a.out1 = out1;
+  a.x2 = x2;
+  a.a2 = a2;
+  a.out2 = out2;
+  time_begin(c);
+  BH_HIP(c, launch_dg_minv_axpy(a, g->tab));
+  time_end(c);
+  return BH_OK;
+}
+
+int bh_dg_destroy(bh_dg g) {
+  if (!g) return BH_ERR_ARG;
+  dg_free(g);
+  delete g;
+  return BH_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
 // ---- graphs ------------------------------------------------------------------
 int bh_graph_begin(bh_ctx c) {
   if (!c) return BH_ERR_ARG;
@@ -726,7 +994,8 @@ int bh_quadrature_points(bh_mesh m, int quad_degree, double* xq, int where, int3
   if (!c) return fail(c, BH_ERR_ARG, "needs a mesh created with a context");
   if (c->capturing && where == BH_HOST)
     return fail(c, BH_ERR_ARG, "host-`where` call cannot be captured in a graph");
-  if (m->geo < 0) return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
+  if (m->geo < 0 || m->geo == BH_HEXA27)
+    return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
   int nq1 = gauss_points_per_axis(quad_degree);
   int order = 1;  // any instantiated order carries the geometric tables for nq1
   while (order <= 4 && !kernel_supported(m->geo, order, nq1)) ++order;

@@ -69,7 +69,7 @@ int gauss_points_per_axis(int quad_degree) {  // quadrature.jl:238-240
 
 // Gauss-Legendre nodes/weights by Newton iteration on the three-term
 // recurrence of P_n, in long double.
-static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
   x.assign(n, 0.0);
   w.assign(n, 0.0);
   for (int k = 0; k < (n + 1) / 2; ++k) {
@@ -104,7 +104,7 @@ static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w
 
 // 1D Lagrange basis on uniform nodes of [-1,1] (Lagrange{:Uniform},
 // lagrange.jl:16) and its derivative.
-static void lagrange_1d(int order, double t, double* v, double* dv) {
+void lagrange_1d(int order, double t, double* v, double* dv) {
   int n = order + 1;
   if (order == 0) { v[0] = 1.0; dv[0] = 0.0; return; }
   std::vector<double> nodes(n);
@@ -577,6 +577,128 @@ bool build_pattern(const std::vector<int32_t>& rdofs, int nl_row, i64 nrows,
         }
       }
   });
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// DG faces (_build_faces! mesh.jl:328-374, inner_faces mesh.jl:552-555,
+// FaceInfo/cell_side domain.jl:555-593 + entity.jl:478-483, side-p
+// permutation eval_point.jl:174-206, mapping_face mapping.jl:324-342).
+// ---------------------------------------------------------------------------
+namespace {
+
+// Reference cube: vertex coordinates and the vertex lists of its faces
+// (shape.jl:212-287).
+const int kCubeX[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                          {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+const int kCubeF[6][4] = {{0, 3, 2, 1}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                          {2, 3, 7, 6}, {0, 4, 7, 3}, {4, 5, 6, 7}};
+const int kSideDim[6] = {2, 1, 0, 1, 0, 2};
+
+// Code of the affine face map (s,t) -> cell reference point built from the
+// Quad4 map of vertices kCubeF[side][perm[k]]; -1 if it is not a symmetry of
+// the square.
+int face_map_code(int side, const int perm[4]) {
+  int X[4][3];
+  for (int k = 0; k < 4; ++k)
+    for (int a = 0; a < 3; ++a) X[k][a] = kCubeX[kCubeF[side][perm[k]]][a];
+  int ds[3], dt[3], sk[3], ce[3];
+  for (int a = 0; a < 3; ++a) {
+    ds[a] = -X[0][a] + X[1][a] + X[2][a] - X[3][a];  // 4 * d xi / ds
+    dt[a] = -X[0][a] - X[1][a] + X[2][a] + X[3][a];  // 4 * d xi / dt
+    sk[a] = X[0][a] - X[1][a] + X[2][a] - X[3][a];   // bilinear part
+    ce[a] = X[0][a] + X[1][a] + X[2][a] + X[3][a];
+  }
+  int d = kSideDim[side];
+  int fa = d == 0 ? 1 : 0, fb = d == 2 ? 1 : 2;
+  for (int a = 0; a < 3; ++a)
+    if (sk[a] != 0) return -1;
+  if (ds[d] != 0 || dt[d] != 0 || ce[fa] != 0 || ce[fb] != 0) return -1;
+  if (std::abs(ds[fa]) == 4 && std::abs(dt[fb]) == 4)
+    return 0 | (ds[fa] < 0 ? 2 : 0) | (dt[fb] < 0 ? 4 : 0);
+  if (std::abs(dt[fa]) == 4 && std::abs(ds[fb]) == 4)
+    return 1 | (dt[fa] < 0 ? 2 : 0) | (ds[fb] < 0 ? 4 : 0);
+  return -1;
+}
+
+struct Key4 {
+  i64 v[4];
+  bool operator==(const Key4& o) const {
+    return v[0] == o.v[0] && v[1] == o.v[1] && v[2] == o.v[2] && v[3] == o.v[3];
+  }
+};
+struct Key4Hash {
+  size_t operator()(const Key4& k) const {
+    uint64_t h = 1469598103934665603ull;
+    for (int i = 0; i < 4; ++i) h = (h ^ (uint64_t)k.v[i]) * 1099511628211ull;
+    return (size_t)h;
+  }
+};
+
+}  // namespace
+
+bool build_dg_faces(const HostMesh& m, HostDGFaces& F, std::string& err) {
+  if (!m.uniform || (m.celltype != BH_HEXA8 && m.celltype != BH_HEXA27)) {
+    err = "DG faces need a uniform Hexa8 or Hexa27 mesh";
+    return false;
+  }
+  i64 nc = m.ncells;
+  if (nc > INT32_MAX / 6) { err = "too many cells"; return false; }
+  // all faces in creation order: (creator, side), then the second cell
+  std::unordered_map<Key4, int32_t, Key4Hash> dict;
+  dict.reserve((size_t)nc * 4);
+  std::vector<int32_t> fc_n, fc_p;
+  std::vector<int8_t> fs_n, fs_p;
+  for (i64 c = 0; c < nc; ++c) {
+    const i64* cn = &m.c2n[m.c2n_ptr[c]];
+    for (int s = 0; s < 6; ++s) {
+      Key4 k;
+      for (int j = 0; j < 4; ++j) k.v[j] = cn[kCubeF[s][j]];
+      std::sort(k.v, k.v + 4);
+      auto it = dict.find(k);
+      if (it == dict.end()) {
+        dict.emplace(k, (int32_t)fc_n.size());
+        fc_n.push_back((int32_t)c);
+        fs_n.push_back((int8_t)s);
+        fc_p.push_back(-1);
+        fs_p.push_back(-1);
+      } else {
+        int32_t f = it->second;
+        if (fc_p[f] >= 0) { err = "a face is shared by more than two cells"; return false; }
+        fc_p[f] = (int32_t)c;
+        fs_p[f] = (int8_t)s;
+      }
+    }
+  }
+  F.cell_n.clear(); F.cell_p.clear(); F.side_n.clear(); F.side_p.clear();
+  F.cs.assign((size_t)nc * 6 * 4, 0);
+  for (i64 k = 0; k < nc * 6; ++k) F.cs[k * 4] = -1;
+  const int ident[4] = {0, 1, 2, 3};
+  for (size_t f = 0; f < fc_n.size(); ++f) {
+    if (fc_p[f] < 0) continue;  // boundary face: not in the interior-face domain
+    int32_t cn = fc_n[f], cp = fc_p[f];
+    int sn = fs_n[f], sp = fs_p[f];
+    const i64* nn = &m.c2n[m.c2n_ptr[cn]];
+    const i64* pn = &m.c2n[m.c2n_ptr[cp]];
+    // permut[k] = position in side n's vertex list of side p's k-th vertex
+    int perm[4];
+    for (int k = 0; k < 4; ++k) {
+      perm[k] = -1;
+      for (int q = 0; q < 4; ++q)
+        if (nn[kCubeF[sn][q]] == pn[kCubeF[sp][k]]) perm[k] = q;
+      if (perm[k] < 0) { err = "non-conforming face"; return false; }
+    }
+    int code_n = face_map_code(sn, ident), code_p = face_map_code(sp, perm);
+    if (code_n < 0 || code_p < 0) { err = "face orientation is not a square symmetry"; return false; }
+    int32_t id = (int32_t)F.cell_n.size();
+    F.cell_n.push_back(cn); F.cell_p.push_back(cp);
+    F.side_n.push_back((int8_t)sn); F.side_p.push_back((int8_t)sp);
+    int32_t* rn = &F.cs[((size_t)cn * 6 + sn) * 4];
+    int32_t* rp = &F.cs[((size_t)cp * 6 + sp) * 4];
+    rn[0] = id; rn[1] = cp; rn[2] = 0 | (code_n << 1) | (sp << 4) | (code_p << 7);
+    rp[0] = id; rp[1] = cn; rp[2] = 1 | (code_p << 1) | (sn << 4) | (code_n << 7);
+  }
+  F.nfaces = (i64)F.cell_n.size();
   return true;
 }
 

@@ -33,6 +33,10 @@ struct Tables {
 // quad_degree follows Measure(Ω, degree): n = ceil((degree+1)/2) Gauss points per axis.
 Tables make_tables(int shape, int order, int quad_degree, int geo_celltype);
 int gauss_points_per_axis(int quad_degree);
+// 1D Gauss-Legendre rule on [-1,1] (x[k] = -x[n-1-k] exactly) and the 1D
+// uniform-node Lagrange basis with its derivative at t.
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w);
+void lagrange_1d(int order, double t, double* v, double* dv);
 
 // Host mesh mirror.
 struct HostMesh {
@@ -77,6 +81,26 @@ bool build_pattern(const std::vector<int32_t>& rdofs, int nl_row, i64 nrows,
 // Colouring of cells so that no two cells of one colour share a row dof.
 int colour_cells(const std::vector<int32_t>& rdofs, int nl, i64 nrows, i64 ncells,
                  std::vector<int32_t>& color_ptr, std::vector<int32_t>& cells_by_color);
+
+// Interior-face connectivity of a uniform Hexa8/Hexa27 mesh for the DG face
+// terms.  Faces are numbered in creation order (each cell's faces in
+// faces2nodes order, a face belongs to the first cell that lists it -- side n --
+// and the second is side p); only faces with two cells are kept.
+// Per cell side (ncells*6 records of 4 int32):
+//   [0] interior face index or -1, [1] neighbour cell,
+//   [2] bit 0: this cell is side p; bits 1-3: this side's face-point map;
+//       bits 4-6: neighbour's side; bits 7-9: neighbour's face-point map.
+// A face-point map code m (swap | flip_a << 1 | flip_b << 2) sends face
+// quadrature point (qs, qt) to the Gauss indices (ka, kb) along the cell's two
+// in-face axes a < b: ka = swap ? qt : qs (reversed if flip_a), kb = swap ? qs : qt
+// (reversed if flip_b).
+struct HostDGFaces {
+  i64 nfaces = 0;
+  std::vector<int32_t> cell_n, cell_p;
+  std::vector<int8_t> side_n, side_p;
+  std::vector<int32_t> cs;
+};
+bool build_dg_faces(const HostMesh& m, HostDGFaces& F, std::string& err);
 
 // Simple host thread pool loop.
 void parallel_for(i64 n, const std::function<void(i64, i64)>& body);

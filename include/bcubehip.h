@@ -156,6 +156,42 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
  * PhysicalFunction on the host; *nq receives the points per cell. */
 int bh_quadrature_points(bh_mesh mesh, int quad_degree, double* xq, int where, int32_t* nq);
 
+/* ---- discontinuous Galerkin transport ------------------------------------- */
+/* The linear form of an upwind DG discretisation of ∂u/∂t + ∇⋅(c u) = 0,
+ *
+ *   l(v) = ∫((c*u)⋅∇(v))dΩ - ∫(flux ∘ (side⁻(u), side⁺(u), jump(v), side⁻(nΓ)))dΓ,
+ *   flux(ui, uj, δv, n) = (c⋅n > 0 ? (c⋅n)*ui : (c⋅n)*uj) * δv,
+ *
+ * with Ω = CellDomain(mesh), Γ = InteriorFaceDomain(mesh) (domain.jl:186-198),
+ * assembled by assemble_linear(l, V) (assembler.jl:448-533; faces through
+ * _update_b!(::FaceInfo) :768-800, side⁻/side⁺ eval_point.jl:164-206, normal
+ * mapping.jl:843-855, face metric :279-311, jump cellfunction.jl:308).
+ * V: discontinuous Lagrange Q1..Q3 space (bh_space_create with continuous = 0)
+ * on a uniform Hexa8 or Hexa27 mesh; quad_degree must give order+1 Gauss points
+ * per axis (degree 2*order or 2*order+1), on cells and faces alike.
+ * bh_dg_create computes the face connectivity (_build_faces! mesh.jl:328-374,
+ * faces kept in creation order, side n = the creating cell) and, on the device
+ * in FP64, the per-point factors w|J|J⁻¹c, w|J| and w|∂x/∂s × ∂x/∂t|(c⋅n⁻).
+ * mode: BH_FP64 (fields, factors and arithmetic in FP64) or BH_FP32 (FP32
+ * fields and arithmetic, factors computed in FP64 then rounded). */
+typedef struct bh_dg_s* bh_dg;
+int bh_dg_create(bh_space V, int quad_degree, const double velocity[3], int mode, bh_dg* out);
+int bh_dg_info(bh_dg dg, int64_t* ndofs, int64_t* ninterior_faces, int32_t* ndofs_per_cell);
+/* Recompute the geometric factors after bh_mesh_set_nodes. */
+int bh_dg_update_geometry(bh_dg dg);
+/* r = l(v) for `nfields` fields at once: u and r hold nfields vectors of ndofs
+ * values each, one after the other (field f at offset f*ndofs), FP64 or FP32
+ * as the mode says; r is overwritten (assemble_linear, not assemble_linear!). */
+int bh_dg_transport_residual(bh_dg dg, int32_t nfields, const void* u, void* r, int where);
+/* Time stepping with the block-diagonal DG mass matrix M of the same Gauss
+ * rule (∫(u*v)dΩ on each cell): out1 = x1 + a1 M⁻¹r and, if out2 is not NULL,
+ * out2 = x2 + a2 M⁻¹r, elementwise over nfields*ndofs values; out1 may be x1.
+ * Device pointers only.  No counterpart in the reference's assembly API: it is
+ * the update step of explicit time loops that re-assemble l(v) every stage. */
+int bh_dg_minv_axpy(bh_dg dg, int32_t nfields, const void* r, const void* x1, double a1,
+                    void* out1, const void* x2, double a2, void* out2);
+int bh_dg_destroy(bh_dg dg);
+
 /* ---- graphs -------------------------------------------------------------- */
 /* Re-assembly inside time-stepping or Newton loops (e.g. benchmark/covo.jl:14-33
  * calls assemble_linear! every step): capture the device-`where` assembly

@@ -164,6 +164,140 @@ def bench_c5(args, torch, B):
     return out
 
 
+def cpu_baseline_c4(n_sample=26):
+    """Oracle DG transport residual (serial C++ restatement), 1 thread, on a
+    bounded sample of the C4 workload: n_sample^3 curved Hexa27 cells, Q3, 8 fields."""
+    import bcubehip as B
+    from oracle import oracle as O
+
+    m = B.hexa27_mesh(n_sample + 1, n_sample + 1, n_sample + 1, backend=B.BcubeBackendHostOnly())
+    B.sinusoidal_map(m, 0.05, seed=0)
+    om = O.OMesh(3, m.coords, m.celltypes, m.c2n_ptr, m.c2n)
+    u = np.random.default_rng(2).standard_normal((m.ncells * 64, 8))
+    t0 = time.perf_counter()
+    O.dg_transport(om, 3, np.array([1.0, 0.5, 0.25]), u, 7)
+    dt = time.perf_counter() - t0
+    return {"value": u.size / dt / 1e9, "unit": "Gentries/s", "cores": 1, "kind": "port",
+            "sample": f"{n_sample}^3 curved Hexa27, DG Q3, 8 fields, vol+upwind faces: "
+                      f"{u.size} residual entries in {dt:.2f} s"}
+
+
+def bench_c4(args, torch, B):
+    """DG linear-transport residual: 32^3 Hexa27 cells curved by
+    x_d += 0.05 sin(2πx+φ) sin(2πy+φ) sin(2πz+φ) (phases seed 0), discontinuous
+    Q3 (64 dofs/cell), c = (1, 0.5, 0.25), ∫(c u)⋅∇v dΩ - ∫ upwind flux jump(v) dΓ
+    with 4^3 / 4^2 Gauss points, 8 fields of N(0,1) nodal values (seed 2).  One
+    step = one residual of all 8 fields, inputs in HBM.  Also times the 50-step
+    RK4 loop (200 residuals + 200 M⁻¹ updates) replayed from one HIP graph."""
+    n = args.n if args.n != 64 else 32
+    t_setup = time.perf_counter()
+    m = B.hexa27_mesh(n + 1, n + 1, n + 1)
+    B.sinusoidal_map(m, 0.05, seed=0)
+    V = B.TrialFESpace(B.FunctionSpace("Lagrange", 3), m, "discontinuous")
+    c = np.array([1.0, 0.5, 0.25])
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    backend = m.backend
+    backend.set_stream(stream.cuda_stream)
+    plans = {mode: B.DGTransport(V, c, 7, mode) for mode in ("fp64", "fp32")}
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    p32 = plans["fp32"]
+    nd, nf = p32.ndofs, 8
+    nfaces = p32.ninterior_faces
+    u64 = torch.from_numpy(np.random.default_rng(2).standard_normal((nf, nd))).to(dev)
+    entries = nf * nd
+    nodes = (2 * n + 1) ** 3
+    res = {}
+    outs = {}
+    for mode, plan in plans.items():
+        es = 4 if mode == "fp32" else 8
+        u = u64.float() if mode == "fp32" else u64
+        r = torch.empty_like(u)
+        for _ in range(args.warmup):
+            plan.residual_device(nf, u.data_ptr(), r.data_ptr())
+        torch.cuda.synchronize()
+        backend.set_timing(True)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            plan.residual_device(nf, u.data_ptr(), r.data_ptr())
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        launches = np.asarray(backend.timings(cap=100000), dtype=np.float64)
+        backend.set_timing(False)
+        k_ms = float(launches.mean())
+        b_alg = 2.0 * entries * es + nodes * 3 * 8
+        ach = b_alg / (k_ms * 1e-3) / 1e9
+        outs[mode] = r
+        res[mode] = {"ms_per_step": ms, "gentries_per_s": entries / (ms * 1e-3) / 1e9,
+                     "cells_per_s": m.ncells / (ms * 1e-3),
+                     "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS,
+                                  "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+                                  "kernel": f"k_dg_residual<{'float' if es == 4 else 'double'},4>",
+                                  "avg_launch_ms": k_ms, "alg_bytes_per_launch": b_alg,
+                                  "factor_bytes_per_launch":
+                                      (m.ncells * 4 * 64 + nfaces * 16) * es}}
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    measured = {}
+    if os.path.exists(tf) and n == 32:
+        measured = {k.replace(" ", ""): v for k, v in json.load(open(tf))["kernels"].items()}
+    for mode_r in res.values():
+        mode_r["roofline"]["traffic"] = measured.get(mode_r["roofline"]["kernel"].replace(" ", ""))
+    err = float((outs["fp32"].double() - outs["fp64"]).abs().max() / outs["fp64"].abs().max())
+    # 50 RK4 steps of M du/dt = l(u): eager launches vs one graph of one step replayed
+    rk = {}
+    dt = 0.1 / (4 * n)  # inside the RK4 stability limit of DG Q3 (~h / (|c| (p+1)^2))
+    for mode in ("fp32",):
+        plan = plans[mode]
+        u = u64.float().clone()
+        work = plan.rk4_step(u, dt)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(50):
+            plan.rk4_step(u, dt, work)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        eager_ms = e0.elapsed_time(e1)
+        with backend.capture() as g:
+            plan.rk4_step(u, dt, work)
+        g.launch()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(50):
+            g.launch()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        graph_ms = e0.elapsed_time(e1)
+        rk[mode] = {"steps": 50, "eager_ms": eager_ms, "graph_ms": graph_ms,
+                    "residuals_per_s_graph": 200 / (graph_ms * 1e-3),
+                    "finite": bool(torch.isfinite(u).all().item())}
+    r = res["fp32"]
+    line = {
+        "metric": "DG transport residual entries/s (C4), FP32 with FP64 geometry",
+        "value": r["gentries_per_s"], "unit": "Gentries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"C4: {n}^3 Hexa27 cells curved by a 0.05 sinusoidal map (seed 0), "
+                               f"DG Q3, 8 fields N(0,1) seed 2, c=(1,0.5,0.25), volume + upwind "
+                               f"interior faces, 4^3/4^2 Gauss",
+                   "ncells": m.ncells, "ndofs": nd, "interior_faces": nfaces, "fields": nf,
+                   "setup_s": round(t_setup, 2)},
+        "roofline": r["roofline"],
+        "rel_err_fp32_vs_fp64_gpu": err,
+        "fp32_speedup_vs_fp64": res["fp64"]["ms_per_step"] / r["ms_per_step"],
+        "modes": res,
+        "rk4": rk,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_c4()
+    backend.set_stream(None)
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,8 +306,9 @@ def main():
     ap.add_argument("--mode", default="all", choices=["all", "fp64", "fp32", "bf16x3"])
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c2", "c5"],
-                    help="c2: the headline 64^3 Q2 hexa workload; c5: the many-instance batch")
+    ap.add_argument("--config", default="c2", choices=["c2", "c4", "c5"],
+                    help="c2: the headline 64^3 Q2 hexa workload; c4: the DG transport "
+                         "residual; c5: the many-instance batch")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters against a known byte count)")
@@ -189,6 +324,9 @@ def main():
 
     if args.config == "c5":
         print(json.dumps(bench_c5(args, torch, B)), flush=True)
+        return
+    if args.config == "c4":
+        print(json.dumps(bench_c4(args, torch, B)), flush=True)
         return
     n = args.n
     t_setup = time.perf_counter()

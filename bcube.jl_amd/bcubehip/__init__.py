@@ -7,7 +7,8 @@ mirror of Bcube.jl's assembly API (src/assembler/assembler.jl).
 from ._lib import BcubeHipError, LIB_PATH, lib
 from .mesh import (AbstractBcubeBackend, BcubeBackendHIP, BcubeBackendHostOnly, Mesh,
                    batch_meshes, get_bcube_backend, hexa27_mesh, hexa8_to_hexa27, hexa_mesh,
-                   jitter_nodes, rectangle_mesh, sinusoidal_map)
+                   jitter_nodes, mixed_tri_quad_mesh, rectangle_mesh, sinusoidal_map,
+                   sub_mesh)
 from .fespace import DofHandler, FunctionSpace, TestFESpace, TrialFESpace
 from .forms import (CellDomain, FEFunction, InteriorFaceDomain, Measure, MeshCellData,
                     PhysicalFunction, UpwindFlux, compose, dot, get_face_normals, grad,
@@ -19,7 +20,8 @@ from .assembler import (BilinearPlan, assemble_bilinear, assemble_linear, assemb
 __all__ = [
     "BcubeHipError", "LIB_PATH", "lib", "AbstractBcubeBackend", "BcubeBackendHIP",
     "BcubeBackendHostOnly", "Mesh", "batch_meshes", "get_bcube_backend", "hexa27_mesh",
-    "hexa8_to_hexa27", "hexa_mesh", "jitter_nodes", "sinusoidal_map",
+    "hexa8_to_hexa27", "hexa_mesh", "jitter_nodes", "mixed_tri_quad_mesh", "sinusoidal_map",
+    "sub_mesh",
     "rectangle_mesh", "DofHandler", "FunctionSpace", "TestFESpace", "TrialFESpace",
     "CellDomain", "FEFunction", "InteriorFaceDomain", "Measure", "MeshCellData",
     "PhysicalFunction", "UpwindFlux", "compose", "dot", "get_face_normals", "grad", "integrate",

@@ -339,6 +339,78 @@ def hexa8_to_hexa27(mesh: Mesh, backend: AbstractBcubeBackend | None = None) -> 
                 backend if backend is not None else mesh.backend)
 
 
+def mixed_tri_quad_mesh(n, block: int = 16, p_split: float = 1.0 / 3.0, seed: int = 0,
+                        backend: AbstractBcubeBackend | None = None):
+    """n x n quads on the unit square (rectangle_mesh(n+1, n+1) numbering and
+    cell order); the quads of seeded block x block blocks, each chosen with
+    probability p_split, are split into two triangles (v0,v1,v2), (v0,v2,v3)
+    in place.  Returns (mesh, block_of_cell)."""
+    rng = np.random.default_rng(seed)
+    nb = (n + block - 1) // block
+    split = rng.random((nb, nb)) < p_split
+    y, x = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    xy = np.stack([x.ravel() / n, y.ravel() / n], axis=1)
+    ky, kx = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    kx, ky = kx.ravel(), ky.ravel()
+    n0 = ky * (n + 1) + kx
+    quads = np.stack([n0, n0 + 1, n0 + n + 2, n0 + n + 1], axis=1)
+    bid = (ky // block) * nb + (kx // block)
+    is_split = split[ky // block, kx // block]
+    ncell = len(quads) + int(is_split.sum())
+    types = np.empty(ncell, dtype=np.int32)
+    sizes = np.where(is_split, 6, 4)
+    ptr = np.concatenate([[0], np.cumsum(sizes)])
+    flat = np.empty(ptr[-1], dtype=np.int64)
+    cell_block = np.empty(ncell, dtype=np.int64)
+    # build cell lists: split quads contribute two triangles in place
+    cpos = np.concatenate([[0], np.cumsum(np.where(is_split, 2, 1))])[:-1]
+    q = quads
+    for k in range(len(quads)):
+        c = cpos[k]
+        if is_split[k]:
+            types[c] = types[c + 1] = L.CELLTYPE["Tri3"]
+            flat[ptr[k]:ptr[k] + 6] = (q[k, 0], q[k, 1], q[k, 2], q[k, 0], q[k, 2], q[k, 3])
+            cell_block[c] = cell_block[c + 1] = bid[k]
+        else:
+            types[c] = L.CELLTYPE["Quad4"]
+            flat[ptr[k]:ptr[k] + 4] = q[k]
+            cell_block[c] = bid[k]
+    lens = np.where(types == L.CELLTYPE["Tri3"], 3, 4)
+    cptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    m = Mesh.__new__(Mesh)
+    m.dim = 2
+    m.coords = np.ascontiguousarray(xy.reshape(-1))
+    m.celltypes = types
+    m.c2n_ptr = cptr
+    m.c2n = flat
+    m.backend = backend if backend is not None else get_bcube_backend()
+    m._handle = None
+    return m, cell_block
+
+
+def sub_mesh(mesh: Mesh, cells, backend: AbstractBcubeBackend | None = None) -> Mesh:
+    """The cells `cells` (in that order) of `mesh`, on the same node array."""
+    cells = np.asarray(cells, dtype=np.int64)
+    lens = mesh.c2n_ptr[cells + 1] - mesh.c2n_ptr[cells]
+    idx = np.concatenate([np.arange(mesh.c2n_ptr[c], mesh.c2n_ptr[c + 1]) for c in cells]) \
+        if cells.size < 4096 else _gather_ranges(mesh.c2n_ptr[cells], lens)
+    m = Mesh.__new__(Mesh)
+    m.dim = mesh.dim
+    m.coords = mesh.coords
+    m.celltypes = np.ascontiguousarray(mesh.celltypes[cells])
+    m.c2n_ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    m.c2n = np.ascontiguousarray(mesh.c2n[idx])
+    m.backend = backend if backend is not None else mesh.backend
+    m._handle = None
+    return m
+
+
+def _gather_ranges(starts, lens):
+    tot = int(lens.sum())
+    out = np.repeat(starts - np.concatenate([[0], np.cumsum(lens)[:-1]]), lens)
+    return out + np.arange(tot)
+
+
 def jitter_nodes(mesh: Mesh, amplitude: float, h, seed: int = 0) -> np.ndarray:
     """Seeded uniform jitter of interior nodes by ±amplitude*h per axis.
     Boundary nodes stay put, so the domain (and Σ mass) is unchanged."""

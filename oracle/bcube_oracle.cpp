@@ -259,6 +259,69 @@ static void line_rule(int degree, int qtype, std::vector<double>& w, std::vector
   else gauss_legendre(n, w, x);
 }
 
+// Triangle rules of Measure degree 4..8 (quadrature.jl:289-377: GLTRI6, GLTRI7,
+// GLTRI12 and GLTRI16 of Witherden & Vincent, rescaled to the reference triangle
+// (0,0),(1,0),(0,1)).  Degrees 1..3 come from FEMQuad's GLTRI1/3/4 tables
+// (quadrature.jl:280-288), a dependency that is not vendored: not restated.
+static void triangle_rule(int degree, std::vector<double>& w, std::vector<double>& x) {
+  w.clear(); x.clear();
+  auto pt = [&](double a, double b) { x.push_back(a); x.push_back(b); };
+  if (degree == 4) {
+    double P1 = 0.22338158967801146569500700843312 / 2;
+    double P2 = 0.10995174365532186763832632490021 / 2;
+    double A = 0.44594849091596488631832925388305;
+    double B = 0.09157621350977074345957146340220;
+    w = {P2, P2, P2, P1, P1, P1};
+    pt(B, B); pt(1.0 - 2.0 * B, B); pt(B, 1.0 - 2.0 * B);
+    pt(A, 1.0 - 2 * A); pt(A, A); pt(1.0 - 2.0 * A, A);
+  } else if (degree == 5) {
+    double A = 0.47014206410511508977044120951345;
+    double B = 0.10128650732345633880098736191512;
+    double P1 = 0.13239415278850618073764938783315 / 2;
+    double P2 = 0.12593918054482715259568394550018 / 2;
+    w = {9.0 / 80, P1, P1, P1, P2, P2, P2};
+    pt(1.0 / 3, 1.0 / 3); pt(A, A); pt(1.0 - 2.0 * A, A); pt(A, 1.0 - 2.0 * A);
+    pt(B, B); pt(1.0 - 2.0 * B, B); pt(B, 1.0 - 2.0 * B);
+  } else if (degree == 6) {
+    double A = (-0.87382197101699554331933679425836168532 + 1) / 2;
+    double B = (-0.50142650965817915741672289378596184782 + 1) / 2;
+    double C = (-0.37929509793243118916678453208689569359 + 1) / 2;
+    double D = (-0.89370990031036610529350065673720370601 + 1) / 2;
+    double P1 = 0.10168981274041363384187361821373796809 / 4;
+    double P2 = 0.23357255145275873205057922277115888265 / 4;
+    double P3 = 0.16570215123674715038710691284088490796 / 4;
+    w = {P1, P1, P1, P2, P2, P2, P3, P3, P3, P3, P3, P3};
+    pt(A, A); pt(1.0 - 2.0 * A, A); pt(A, 1.0 - 2.0 * A);
+    pt(B, B); pt(1.0 - 2.0 * B, B); pt(B, 1.0 - 2.0 * B);
+    pt(C, D); pt(D, C); pt(1.0 - C - D, C); pt(1.0 - C - D, D); pt(C, 1.0 - C - D); pt(D, 1.0 - C - D);
+  } else if (degree == 7 || degree == 8) {
+    static const double T[16][3] = {
+        {-0.33333333333333333333333333333333333333, -0.33333333333333333333333333333333333333, 0.2886312153555743365021822209781292496},
+        {-0.081414823414553687942368971011661355879, -0.83717035317089262411526205797667728824, 0.1901832685345692495877922087771686332},
+        {-0.83717035317089262411526205797667728824, -0.081414823414553687942368971011661355879, 0.1901832685345692495877922087771686332},
+        {-0.081414823414553687942368971011661355879, -0.081414823414553687942368971011661355879, 0.1901832685345692495877922087771686332},
+        {-0.65886138449647958675541299701707099796, 0.31772276899295917351082599403414199593, 0.20643474106943650056358310058425806003},
+        {0.31772276899295917351082599403414199593, -0.65886138449647958675541299701707099796, 0.20643474106943650056358310058425806003},
+        {-0.65886138449647958675541299701707099796, -0.65886138449647958675541299701707099796, 0.20643474106943650056358310058425806003},
+        {-0.89890554336593804908315289880680210631, 0.79781108673187609816630579761360421262, 0.064916995246396160621851856683561193593},
+        {0.79781108673187609816630579761360421262, -0.89890554336593804908315289880680210631, 0.064916995246396160621851856683561193593},
+        {-0.89890554336593804908315289880680210631, -0.89890554336593804908315289880680210631, 0.064916995246396160621851856683561193593},
+        {-0.98321044518008478932557233092141110162, 0.45698478591080856248200075835212392604, 0.05446062834886998852968938014781784832},
+        {0.45698478591080856248200075835212392604, -0.98321044518008478932557233092141110162, 0.05446062834886998852968938014781784832},
+        {-0.47377434073072377315642842743071282442, 0.45698478591080856248200075835212392604, 0.05446062834886998852968938014781784832},
+        {0.45698478591080856248200075835212392604, -0.47377434073072377315642842743071282442, 0.05446062834886998852968938014781784832},
+        {-0.47377434073072377315642842743071282442, -0.98321044518008478932557233092141110162, 0.05446062834886998852968938014781784832},
+        {-0.98321044518008478932557233092141110162, -0.47377434073072377315642842743071282442, 0.05446062834886998852968938014781784832}};
+    for (int k = 0; k < 16; ++k) {
+      pt(0.5 * (T[k][0] + 1), 0.5 * (T[k][1] + 1));
+      w.push_back(T[k][2] / 4);
+    }
+  } else {
+    throw std::runtime_error("oracle: triangle rule of degree " + std::to_string(degree) +
+                             " is not restated (degrees 1-3 come from FEMQuad)");
+  }
+}
+
 // Square / Cube tensor rules, first index fastest (quadrature.jl:455-487).
 // A degree-0 quadrature integrates with the degree-1 rule (quadrature.jl:130-135).
 static int tensor_rule(int shape, int degree, int qtype, std::vector<double>& w,
@@ -282,7 +345,7 @@ static int tensor_rule(int shape, int degree, int qtype, std::vector<double>& w,
           x.push_back(x1[i]); x.push_back(x1[j]); x.push_back(x1[k]);
         }
   } else {
-    throw std::runtime_error("oracle: triangle quadrature not restated");
+    triangle_rule(degree, w, x);
   }
   return (int)w.size();
 }
@@ -362,12 +425,88 @@ static double lagrange_poly_d(int j, double xi, const std::vector<double>& nodes
 
 static int ndofs_lagrange(int shape, int order) {  // hypercube.jl:122-133, triangle.jl:67-70
   if (shape == TRIANGLE) {
-    static const int t[] = {1, 3, 6, 10};
-    if (order > 3) throw std::runtime_error("oracle: triangle order > 3 has no reference basis");
+    // P0..P3 from triangle.jl:67-70; P4 extends the same rule (15 dofs)
+    static const int t[] = {1, 3, 6, 10, 15};
+    if (order > 4) throw std::runtime_error("oracle: triangle order > 4");
     return t[order];
   }
   int n = order + 1;
   return shape == SQUARE ? n * n : n * n * n;
+}
+
+// Triangle Lagrange bases on (0,0),(1,0),(0,1) (triangle.jl:1-64).  P1/P2
+// gradients are the closed forms of ∂λξ_∂ξ (triangle.jl:47-64); P3 (and the P4
+// extension) go through ForwardDiff (function_space.jl:161-165), restated with
+// Dual<2>.  P4 extends P3's layout: vertices, three dofs per edge from its first
+// vertex to its second, then the interior nodes (2,1,1)/4, (1,2,1)/4, (1,1,2)/4
+// in barycentric coordinates -- no reference counterpart (parity pinned by
+// mathematics only).
+template <class T> static void tri_p3(const T& x, const T& y, T* v) {
+  T l1 = 1.0 - x - y, l2 = x, l3 = y;
+  v[0] = 0.5 * (3.0 * l1 - 1.0) * (3.0 * l1 - 2.0) * l1;
+  v[1] = 0.5 * (3.0 * l2 - 1.0) * (3.0 * l2 - 2.0) * l2;
+  v[2] = 0.5 * (3.0 * l3 - 1.0) * (3.0 * l3 - 2.0) * l3;
+  v[3] = 4.5 * l1 * l2 * (3.0 * l1 - 1.0);
+  v[4] = 4.5 * l1 * l2 * (3.0 * l2 - 1.0);
+  v[5] = 4.5 * l2 * l3 * (3.0 * l2 - 1.0);
+  v[6] = 4.5 * l2 * l3 * (3.0 * l3 - 1.0);
+  v[7] = 4.5 * l3 * l1 * (3.0 * l3 - 1.0);
+  v[8] = 4.5 * l3 * l1 * (3.0 * l1 - 1.0);
+  v[9] = 27.0 * l1 * l2 * l3;
+}
+// Barycentric product formula: node (i,j,k), i+j+k = p:
+// Π_{m<i} (pλ1 - m)/(m+1) · Π_{m<j} (pλ2 - m)/(m+1) · Π_{m<k} (pλ3 - m)/(m+1)
+template <class T> static T bary_node(int p, int i, int j, int k, const T& l1, const T& l2, const T& l3) {
+  T r = T(1.0);
+  for (int m = 0; m < i; ++m) r = r * ((p * l1 - (double)m) / (double)(m + 1));
+  for (int m = 0; m < j; ++m) r = r * ((p * l2 - (double)m) / (double)(m + 1));
+  for (int m = 0; m < k; ++m) r = r * ((p * l3 - (double)m) / (double)(m + 1));
+  return r;
+}
+static const int kTriP4[15][3] = {{4, 0, 0}, {0, 4, 0}, {0, 0, 4}, {3, 1, 0}, {2, 2, 0},
+                                  {1, 3, 0}, {0, 3, 1}, {0, 2, 2}, {0, 1, 3}, {1, 0, 3},
+                                  {2, 0, 2}, {3, 0, 1}, {2, 1, 1}, {1, 2, 1}, {1, 1, 2}};
+template <class T> static void tri_p4(const T& x, const T& y, T* v) {
+  T l1 = 1.0 - x - y, l2 = x, l3 = y;
+  for (int n = 0; n < 15; ++n) v[n] = bary_node(4, kTriP4[n][0], kTriP4[n][1], kTriP4[n][2], l1, l2, l3);
+}
+
+static void triangle_basis(int order, const double* xi, double* val, double* grad) {
+  double x = xi[0], y = xi[1];
+  if (order == 0) {
+    val[0] = 1.0; grad[0] = grad[1] = 0.0;
+    return;
+  }
+  if (order == 1) {
+    val[0] = 1 - x - y; val[1] = x; val[2] = y;
+    const double g[6] = {-1.0, -1.0, 1.0, 0.0, 0.0, 1.0};
+    for (int k = 0; k < 6; ++k) grad[k] = g[k];
+    return;
+  }
+  if (order == 2) {
+    val[0] = (1 - x - y) * (1 - 2 * x - 2 * y);
+    val[1] = x * (2 * x - 1);
+    val[2] = y * (2 * y - 1);
+    val[3] = 4 * x * (1 - x - y);
+    val[4] = 4 * x * y;
+    val[5] = 4 * y * (1 - x - y);
+    const double g[12] = {-3 + 4 * (x + y), -3 + 4 * (x + y), -1 + 4 * x, 0.0, 0.0, -1 + 4 * y,
+                          4 * (1 - 2 * x - y), -4 * x, 4 * y, 4 * x, -4 * y, 4 * (1 - 2 * y - x)};
+    for (int k = 0; k < 12; ++k) grad[k] = g[k];
+    return;
+  }
+  Dual<2> dx(x), dy(y);
+  dx.d[0] = 1.0; dy.d[1] = 1.0;
+  Dual<2> v[15];
+  int nl;
+  if (order == 3) { tri_p3(dx, dy, v); nl = 10; }
+  else if (order == 4) { tri_p4(dx, dy, v); nl = 15; }
+  else throw std::runtime_error("oracle: triangle order > 4");
+  for (int k = 0; k < nl; ++k) {
+    val[k] = v[k].v;
+    grad[2 * k] = v[k].d[0];
+    grad[2 * k + 1] = v[k].d[1];
+  }
 }
 
 // Values and reference gradients of the tensor Lagrange basis at ξ.
@@ -376,6 +515,10 @@ static int ndofs_lagrange(int shape, int order) {  // hypercube.jl:122-133, tria
 // ForwardDiff.jacobian of that expression: ∂1 = (l1'*l2)*l3,
 // ∂2 = (l2'*l1)*l3, ∂3 = l3'*(l1*l2).
 static void tensor_basis(int shape, int order, const double* xi, double* val, double* grad) {
+  if (shape == TRIANGLE) {
+    triangle_basis(order, xi, val, grad);
+    return;
+  }
   int d = shape == SQUARE ? 2 : 3;
   if (order == 0) {
     val[0] = 1.0;
@@ -433,7 +576,9 @@ static Lists idof_by_edge(int shape, int order) {
     if (order <= 1) return Lists(3);
     if (order == 2) return {{3}, {4}, {5}};
     if (order == 3) return {{3, 4}, {5, 6}, {7, 8}};
-    throw std::runtime_error("oracle: triangle order > 3");
+    // P4: edge dofs from the edge's first vertex to its second, like P3
+    if (order == 4) return {{3, 4, 5}, {6, 7, 8}, {9, 10, 11}};
+    throw std::runtime_error("oracle: triangle order > 4");
   }
   if (order == 0) return Lists(shape == CUBE ? 12 : 4);
   int n = order + 1;
@@ -653,6 +798,21 @@ static void geo_quad4(const double* X, const double* xi, Geo& g) {
   }
 }
 
+// Tri3: mapping (395-397), mapping_jacobian (414-416), mapping_jacobian_inv
+// (418-430), mapping_det_jacobian (446-455).
+static void geo_tri3(const double* X, const double* xi, Geo& g) {
+  double x1 = X[0], y1 = X[1], x2 = X[2], y2 = X[3], x3 = X[4], y3 = X[5];
+  for (int d = 0; d < 2; ++d) {
+    g.x[d] = (1 - xi[0] - xi[1]) * X[d] + xi[0] * X[2 + d] + xi[1] * X[4 + d];
+    g.J[d * 2 + 0] = X[2 + d] - X[d];
+    g.J[d * 2 + 1] = X[4 + d] - X[d];
+  }
+  double den = (x1 - x2) * (y1 - y3) - (x1 - x3) * (y1 - y2);
+  g.Jinv[0] = (-y1 + y3) / den; g.Jinv[1] = (x1 - x3) / den;
+  g.Jinv[2] = (y1 - y2) / den;  g.Jinv[3] = (-x1 + x2) / den;
+  g.detabs = fabs((x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1));
+}
+
 // StaticArrays 3x3 det and inv (used by mapping_det_jacobian /
 // mapping_jacobian_inv defaults, mapping.jl:207-209, 275-286).
 static double det3(const double* A) {
@@ -744,6 +904,7 @@ static void geometry(int ct, const double* X, const double* xi, Geo& g) {
   switch (ct) {
     case QUAD4: geo_quad4(X, xi, g); return;
     case QUAD9: geo_quad9(X, xi, g); return;
+    case TRI3: geo_tri3(X, xi, g); return;
     case HEXA8: geo_hexa8(X, xi, g); return;
     case HEXA27: geo_hexa27(X, xi, g); return;
   }
@@ -1252,18 +1413,20 @@ void or_hexa_mesh(int nx, int ny, int nz, double xmin, double xmax, double ymin,
 }
 
 int or_quadrature(int shape, int degree, int qtype, double* w, double* x) {
-  std::vector<double> ww, xx;
-  int n;
-  if (shape < 0) {  // 1D line rule
-    if (degree == 0) degree = 1;
-    line_rule(degree, qtype, ww, xx);
-    n = (int)ww.size();
-  } else {
-    n = tensor_rule(shape, degree, qtype, ww, xx);
-  }
-  if (w) std::copy(ww.begin(), ww.end(), w);
-  if (x) std::copy(xx.begin(), xx.end(), x);
-  return n;
+  try {
+    std::vector<double> ww, xx;
+    int n;
+    if (shape < 0) {  // 1D line rule
+      if (degree == 0) degree = 1;
+      line_rule(degree, qtype, ww, xx);
+      n = (int)ww.size();
+    } else {
+      n = tensor_rule(shape, degree, qtype, ww, xx);
+    }
+    if (w) std::copy(ww.begin(), ww.end(), w);
+    if (x) std::copy(xx.begin(), xx.end(), x);
+    return n;
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
 
 int or_shape_functions(int shape, int order, const double* xi, double* val, double* grad) {

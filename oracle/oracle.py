@@ -137,14 +137,16 @@ def quadrature(shape, degree, qtype=0):
     w = np.zeros(4096)
     x = np.zeros(4096 * 3)
     n = lib().or_quadrature(s, degree, qtype, w, x)
-    d = 1 if s < 0 else (2 if s == 0 else 3)
+    if n < 0:
+        raise RuntimeError(_err())
+    d = 1 if s < 0 else (3 if s == 2 else 2)
     return w[:n].copy(), x[: n * d].reshape(n, d).copy()
 
 
 def shape_functions(shape, order, xi):
     s = SHAPE[shape]
-    d = 2 if s == 0 else 3
-    n = (order + 1) ** d
+    d = 3 if s == 2 else 2
+    n = (order + 1) * (order + 2) // 2 if s == 1 else (order + 1) ** d
     val = np.zeros(n)
     grad = np.zeros(n * d)
     r = lib().or_shape_functions(s, order, np.ascontiguousarray(xi, dtype=np.float64), val, grad)
@@ -176,7 +178,7 @@ def densify(a):
 
 def dofhandler(mesh: OMesh, order, continuous=True):
     """Returns (iglob 1-based, offset, ndofs_tot)."""
-    nd = {0: (order + 1) ** 2, 1: (order + 1) ** 2, 2: [1, 3, 6, 10][min(order, 3)],
+    nd = {0: (order + 1) ** 2, 1: (order + 1) ** 2, 2: (order + 1) * (order + 2) // 2,
           3: (order + 1) ** 3, 4: (order + 1) ** 3}
     total = int(sum(nd[int(t)] for t in mesh.celltypes))
     iglob = np.zeros(total, dtype=np.int64)

@@ -83,11 +83,12 @@ class BilinearPlan:
             self._colptr, self._rowval = colptr, rowval
         return self._colptr, self._rowval
 
-    def assemble_host(self, out: np.ndarray | None = None) -> np.ndarray:
+    def assemble_host(self, out: np.ndarray | None = None, mode: str | None = None) -> np.ndarray:
         if out is None:
             out = np.empty(self.nnz, dtype=np.float64)
+        md = self.mode if mode is None else L.MODE[mode]
         L.check(L.lib().bh_assemble_bilinear(self.handle, self.nterms, self.terms,
-                                             self.measure.degree, self.mode, out.ctypes.data,
+                                             self.measure.degree, md, out.ctypes.data,
                                              L.HOST), self.ctx, "bh_assemble_bilinear")
         return out
 

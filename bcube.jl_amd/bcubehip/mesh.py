@@ -356,27 +356,21 @@ def mixed_tri_quad_mesh(n, block: int = 16, p_split: float = 1.0 / 3.0, seed: in
     quads = np.stack([n0, n0 + 1, n0 + n + 2, n0 + n + 1], axis=1)
     bid = (ky // block) * nb + (kx // block)
     is_split = split[ky // block, kx // block]
-    ncell = len(quads) + int(is_split.sum())
-    types = np.empty(ncell, dtype=np.int32)
-    sizes = np.where(is_split, 6, 4)
-    ptr = np.concatenate([[0], np.cumsum(sizes)])
-    flat = np.empty(ptr[-1], dtype=np.int64)
-    cell_block = np.empty(ncell, dtype=np.int64)
-    # build cell lists: split quads contribute two triangles in place
-    cpos = np.concatenate([[0], np.cumsum(np.where(is_split, 2, 1))])[:-1]
-    q = quads
-    for k in range(len(quads)):
-        c = cpos[k]
-        if is_split[k]:
-            types[c] = types[c + 1] = L.CELLTYPE["Tri3"]
-            flat[ptr[k]:ptr[k] + 6] = (q[k, 0], q[k, 1], q[k, 2], q[k, 0], q[k, 2], q[k, 3])
-            cell_block[c] = cell_block[c + 1] = bid[k]
-        else:
-            types[c] = L.CELLTYPE["Quad4"]
-            flat[ptr[k]:ptr[k] + 4] = q[k]
-            cell_block[c] = bid[k]
+    cnt = np.where(is_split, 2, 1)
+    cpos = np.concatenate([[0], np.cumsum(cnt)])[:-1]
+    ncell = int(cnt.sum())
+    types = np.full(ncell, L.CELLTYPE["Quad4"], dtype=np.int32)
+    sk = np.flatnonzero(is_split)
+    uq = np.flatnonzero(~is_split)
+    types[cpos[sk]] = L.CELLTYPE["Tri3"]
+    types[cpos[sk] + 1] = L.CELLTYPE["Tri3"]
     lens = np.where(types == L.CELLTYPE["Tri3"], 3, 4)
     cptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    flat = np.empty(int(cptr[-1]), dtype=np.int64)
+    flat[cptr[cpos[uq]][:, None] + np.arange(4)] = quads[uq]
+    flat[cptr[cpos[sk]][:, None] + np.arange(3)] = quads[sk][:, [0, 1, 2]]
+    flat[cptr[cpos[sk] + 1][:, None] + np.arange(3)] = quads[sk][:, [0, 2, 3]]
+    cell_block = np.repeat(bid, cnt)
     m = Mesh.__new__(Mesh)
     m.dim = 2
     m.coords = np.ascontiguousarray(xy.reshape(-1))

@@ -35,15 +35,26 @@ struct bh_graph_s {
   hipGraphExec_t exec = nullptr;
 };
 
+// Cells of one geometric type, in mesh order (the subdomains of
+// build_subdomains_by_celltypes, domain.jl:39-50, in first-appearance order).
+struct GeoBucket {
+  int geo = -1;
+  int64_t n = 0;
+  int32_t* d_cells = nullptr;  // global cell ids (NULL: all cells, uniform mesh)
+  int32_t* d_c2n = nullptr;    // n * nodes-per-cell
+};
+
 struct bh_mesh_s {
   bh_ctx ctx = nullptr;
   int dev = -1;
   HostMesh h;
-  int geo = -1;  // uniform geometric cell type, or -1 for mixed meshes
+  int geo = -1;  // uniform geometric cell type; kMixed for Quad4 + Tri3 meshes; -1 otherwise
   int npc = 0;   // nodes per cell when uniform
   double* d_coords = nullptr;
   int32_t* d_c2n = nullptr;
+  std::vector<GeoBucket> buckets;
 };
+constexpr int kMixed = -2;
 
 struct Colouring {
   int ncolors = 0;
@@ -60,7 +71,7 @@ struct bh_space_s {
   bool continuous = true;
   int nl = 0;
   int64_t ndofs = 0;
-  std::vector<int32_t> dofs;  // ncells * nl, 0-based
+  std::vector<int32_t> dofs;  // ncells * nl, 0-based; -1 pads smaller cells of mixed meshes
   int32_t* d_dofs = nullptr;
   Colouring col;              // for vector (linear-form) assembly
   double* d_b = nullptr;      // scratch output for host-side linear forms
@@ -145,8 +156,15 @@ void time_end(bh_ctx c) {
   c->nevents++;
 }
 
+// Kernel-table key of a Measure degree: Gauss points per axis on quads and
+// hexahedra, points of the triangle rule on triangles.
+int rule_key(int geo, int quad_degree) {
+  if (geo == BH_TRI3) return triangle_rule_points(quad_degree);
+  return gauss_points_per_axis(quad_degree);
+}
+
 int get_tables(bh_ctx c, int geo, int order, int quad_degree, double** out) {
-  int nq1 = gauss_points_per_axis(quad_degree);
+  int nq1 = rule_key(geo, quad_degree);
   auto key = std::make_tuple(geo, order, nq1);
   auto it = c->tables.find(key);
   if (it != c->tables.end()) { *out = it->second; return BH_OK; }
@@ -299,18 +317,53 @@ int bh_mesh_create(bh_ctx c, int dim, int64_t nnodes, const double* coords, int6
     if (v < 0 || v >= nnodes) return fail(c, BH_ERR_RANGE, "c2n: node index out of range");
   h.celltype = h.ctype[0];
   h.uniform = std::all_of(h.ctype.begin(), h.ctype.end(), [&](int t) { return t == h.celltype; });
-  if (h.uniform && (h.celltype == BH_QUAD4 || h.celltype == BH_HEXA8 || h.celltype == BH_HEXA27)) {
+  if (h.uniform && (h.celltype == BH_QUAD4 || h.celltype == BH_HEXA8 || h.celltype == BH_HEXA27 ||
+                   h.celltype == BH_TRI3)) {
     m->geo = h.celltype;
     m->npc = nnodes_of(h.celltype);
+  } else if (!h.uniform && std::all_of(h.ctype.begin(), h.ctype.end(), [](int t) {
+               return t == BH_QUAD4 || t == BH_TRI3;
+             })) {
+    m->geo = kMixed;
   }
-  if (c && m->geo >= 0) {
-    std::vector<int32_t> c2n32(h.c2n.begin(), h.c2n.end());
+  if (c && m->geo != -1) {
     BH_HIP(c, dalloc(&m->d_coords, h.coords.size()));
-    BH_HIP(c, dalloc(&m->d_c2n, c2n32.size()));
     BH_HIP(c, hipMemcpy(m->d_coords, h.coords.data(), h.coords.size() * sizeof(double),
                         hipMemcpyHostToDevice));
-    BH_HIP(c, hipMemcpy(m->d_c2n, c2n32.data(), c2n32.size() * sizeof(int32_t),
-                        hipMemcpyHostToDevice));
+    if (m->geo >= 0) {
+      std::vector<int32_t> c2n32(h.c2n.begin(), h.c2n.end());
+      BH_HIP(c, dalloc(&m->d_c2n, c2n32.size()));
+      BH_HIP(c, hipMemcpy(m->d_c2n, c2n32.data(), c2n32.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice));
+      GeoBucket b;
+      b.geo = m->geo;
+      b.n = ncells;
+      b.d_c2n = m->d_c2n;
+      m->buckets.push_back(b);
+    } else {
+      std::vector<int> types;
+      for (int t : h.ctype)
+        if (std::find(types.begin(), types.end(), t) == types.end()) types.push_back(t);
+      for (int t : types) {
+        std::vector<int32_t> cells, c2n32;
+        for (int64_t k = 0; k < ncells; ++k)
+          if (h.ctype[k] == t) {
+            cells.push_back((int32_t)k);
+            for (int64_t q = h.c2n_ptr[k]; q < h.c2n_ptr[k + 1]; ++q) c2n32.push_back((int32_t)h.c2n[q]);
+          }
+        GeoBucket b;
+        b.geo = t;
+        b.n = (int64_t)cells.size();
+        m->buckets.push_back(b);
+        GeoBucket& bb = m->buckets.back();
+        BH_HIP(c, dalloc(&bb.d_cells, cells.size()));
+        BH_HIP(c, dalloc(&bb.d_c2n, c2n32.size()));
+        BH_HIP(c, hipMemcpy(bb.d_cells, cells.data(), cells.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice));
+        BH_HIP(c, hipMemcpy(bb.d_c2n, c2n32.data(), c2n32.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice));
+      }
+    }
   }
   *out = m.release();
   return BH_OK;
@@ -338,6 +391,10 @@ int bh_mesh_destroy(bh_mesh m) {
     (void)hipSetDevice(m->dev);
     (void)hipFree(m->d_coords);
     (void)hipFree(m->d_c2n);
+    for (GeoBucket& b : m->buckets) {
+      (void)hipFree(b.d_cells);
+      if (b.d_c2n != m->d_c2n) (void)hipFree(b.d_c2n);
+    }
   }
   delete m;
   return BH_OK;
@@ -361,18 +418,20 @@ static int make_space(bh_mesh m, int order, bool continuous, int64_t ndofs,
                       const std::vector<int64_t>& iglob, const std::vector<int64_t>& offset,
                       bh_space* out) {
   bh_ctx c = m->ctx;
-  if (!m->h.uniform) return fail(c, BH_ERR_UNSUPPORTED, "spaces on mixed meshes are not supported yet");
   auto s = std::make_unique<bh_space_s>();
   s->mesh = m;
   s->dev = m->dev;
   s->order = order;
   s->continuous = continuous;
-  s->nl = ndofs_lagrange(shape_of(m->h.celltype), order);
-  s->ndofs = ndofs;
   int64_t nc = m->h.ncells;
-  s->dofs.resize((size_t)nc * s->nl);
+  // per-cell dof lists padded to the largest cell (-1 in the empty slots)
+  s->nl = 0;
   for (int64_t k = 0; k < nc; ++k)
-    for (int i = 0; i < s->nl; ++i) {
+    s->nl = std::max(s->nl, ndofs_lagrange(shape_of(m->h.ctype[k]), order));
+  s->ndofs = ndofs;
+  s->dofs.assign((size_t)nc * s->nl, -1);
+  for (int64_t k = 0; k < nc; ++k)
+    for (int i = 0, nlk = ndofs_lagrange(shape_of(m->h.ctype[k]), order); i < nlk; ++i) {
       int64_t p = offset[k] + i;
       if (p < 0 || p >= (int64_t)iglob.size()) return fail(c, BH_ERR_RANGE, "offset out of range");
       int64_t g = iglob[p] - 1;
@@ -403,11 +462,11 @@ int bh_space_create_from_dofs(bh_mesh m, int order, int continuous, int64_t ndof
                               const int64_t* iglob, const int64_t* offset, bh_space* out) {
   if (!m || !out || !iglob || !offset || ndofs <= 0) return BH_ERR_ARG;
   *out = nullptr;
-  int nl = ndofs_lagrange(shape_of(m->h.celltype), order);
   int64_t nc = m->h.ncells;
   std::vector<int64_t> off(offset, offset + nc);
   int64_t len = 0;
-  for (int64_t k = 0; k < nc; ++k) len = std::max(len, off[k] + nl);
+  for (int64_t k = 0; k < nc; ++k)
+    len = std::max(len, off[k] + ndofs_lagrange(shape_of(m->h.ctype[k]), order));
   std::vector<int64_t> ig(iglob, iglob + len);
   return make_space(m, order, continuous != 0, ndofs, ig, off, out);
 }
@@ -536,8 +595,9 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
   if (where == BH_HOST && !nzval) return fail(c, BH_ERR_ARG, "nzval is NULL");
   if (where == BH_HOST && c->capturing)
     return fail(c, BH_ERR_ARG, "host-`where` assembly cannot be captured in a graph");
-  if (m->geo < 0 || m->geo == BH_HEXA27)
-    return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
+  if (m->geo == -1 || m->geo == BH_HEXA27 || m->buckets.empty())
+    return fail(c, BH_ERR_UNSUPPORTED,
+                "bilinear kernels need Quad4, Tri3 (or a mix of both) or Hexa8 cells");
   const bh_term* stiff = nullptr;
   const bh_term* mass = nullptr;
   for (int k = 0; k < nterms; ++k) {
@@ -549,14 +609,17 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
     else if (t.form == BH_FORM_MASS && !mass) mass = &t;
     else return fail(c, BH_ERR_UNSUPPORTED, "unsupported or repeated bilinear term");
   }
-  int nq1 = gauss_points_per_axis(quad_degree);
-  if (!kernel_supported(m->geo, V->order, nq1))
-    return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this (cell type, order, quadrature)");
-  if (mode != BH_FP64 && !mfma_supported(m->geo, V->order, nq1))
-    return fail(c, BH_ERR_UNSUPPORTED, "no FP32 matrix-core kernel for this (cell type, order, quadrature)");
+  for (const GeoBucket& gb : m->buckets) {
+    const int key = rule_key(gb.geo, quad_degree);
+    if (!kernel_supported(gb.geo, V->order, key))
+      return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this (cell type, order, quadrature)");
+    if (mode != BH_FP64 && !mfma_supported(gb.geo, V->order, key))
+      return fail(c, BH_ERR_UNSUPPORTED,
+                  "no FP32 matrix-core kernel for this (cell type, order, quadrature)");
+  }
+  if (!gather_supported(V->nl))
+    return fail(c, BH_ERR_UNSUPPORTED, "no row-gather kernel for this element size");
   if (int r = set_device(c)) return r;
-  double* tab = nullptr;
-  if (int r = get_tables(c, m->geo, V->order, quad_degree, &tab)) return r;
   int64_t nc = m->h.ncells;
   const HostPattern& h = p->h;
   if (h.max_row > 512) return fail(c, BH_ERR_UNSUPPORTED, "CSR rows longer than 512 entries");
@@ -571,11 +634,9 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
   A.mode = mode;
   A.stream = c->stream;
   A.coords = m->d_coords;
-  A.c2n = m->d_c2n;
-  A.tab = tab;
-  A.ncells_total = nc;
   A.ke = p->d_ke;
   A.ke_fp64 = mode == BH_FP64;
+  A.erow = V->nl;
   if (stiff) {
     A.eta = coef_ptr(*stiff, where, p->d_coef, nc, c->stream);
     A.eta_scale = stiff->scale;
@@ -585,13 +646,23 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
     A.mcoef = coef_ptr(*mass, where, p->d_coef + nc, nc, c->stream);
     A.mass_scale = mass->scale;
   }
-  // 1. element rows: matrix cores where an instantiation exists (FP64 too),
-  //    else the FP64 vector-ALU kernel
-  const bool on_mfma = mfma_supported(m->geo, V->order, nq1);
-  time_begin(c);
-  if (on_mfma) BH_HIP(c, launch_mfma(m->geo, V->order, nq1, A));
-  else BH_HIP(c, launch(m->geo, V->order, nq1, A));
-  time_end(c);
+  // 1. element rows, one launch per cell type (a single one on uniform
+  //    meshes): matrix cores where an instantiation exists (FP64 too), else
+  //    the FP64 vector-ALU kernel.  Rows go to ke[(cell * nl_max + i) * 32 ...].
+  for (const GeoBucket& gb : m->buckets) {
+    const int key = rule_key(gb.geo, quad_degree);
+    double* tab = nullptr;
+    if (int r = get_tables(c, gb.geo, V->order, quad_degree, &tab)) return r;
+    A.kind = LaunchArgs::BILINEAR;
+    A.c2n = gb.d_c2n;
+    A.cells = gb.d_cells;
+    A.tab = tab;
+    A.ncells_total = gb.n;
+    time_begin(c);
+    if (mfma_supported(gb.geo, V->order, key)) BH_HIP(c, launch_mfma(gb.geo, V->order, key, A));
+    else BH_HIP(c, launch(gb.geo, V->order, key, A));
+    time_end(c);
+  }
   // 2. deterministic row reduction into the CSR values
   A.kind = LaunchArgs::GATHER;
   A.pptr = p->d_pptr;
@@ -605,7 +676,7 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
   A.max_row = h.max_row;
   A.vals = (where == BH_DEVICE && nzval) ? nzval : p->d_vals;
   time_begin(c);
-  BH_HIP(c, launch(m->geo, V->order, nq1, A));
+  BH_HIP(c, launch_gather(V->nl, A));
   time_end(c);
   if (where == BH_HOST) {
     BH_HIP(c, hipMemcpyAsync(nzval, p->d_vals, (size_t)h.nnz * sizeof(double),
@@ -626,8 +697,8 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
   if (where == BH_HOST && c->capturing)
     return fail(c, BH_ERR_ARG, "host-`where` assembly cannot be captured in a graph");
   if (m->geo < 0 || m->geo == BH_HEXA27)
-    return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
-  int nq1 = gauss_points_per_axis(quad_degree);
+    return fail(c, BH_ERR_UNSUPPORTED, "linear-form kernels need a uniform Quad4, Tri3 or Hexa8 mesh");
+  int nq1 = rule_key(m->geo, quad_degree);
   if (!kernel_supported(m->geo, V->order, nq1))
     return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this (cell type, order, quadrature)");
   for (int k = 0; k < nterms; ++k) {
@@ -995,15 +1066,15 @@ int bh_quadrature_points(bh_mesh m, int quad_degree, double* xq, int where, int3
   if (c->capturing && where == BH_HOST)
     return fail(c, BH_ERR_ARG, "host-`where` call cannot be captured in a graph");
   if (m->geo < 0 || m->geo == BH_HEXA27)
-    return fail(c, BH_ERR_UNSUPPORTED, "geometry kernels need a uniform Quad4/Hexa8 mesh");
-  int nq1 = gauss_points_per_axis(quad_degree);
+    return fail(c, BH_ERR_UNSUPPORTED, "quadrature points need a uniform Quad4, Tri3 or Hexa8 mesh");
+  int nq1 = rule_key(m->geo, quad_degree);
   int order = 1;  // any instantiated order carries the geometric tables for nq1
   while (order <= 4 && !kernel_supported(m->geo, order, nq1)) ++order;
   if (order > 4) return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this quadrature");
   if (int r = set_device(c)) return r;
   double* tab = nullptr;
   if (int r = get_tables(c, m->geo, order, quad_degree, &tab)) return r;
-  int nq = nq1 * nq1 * (m->h.dim == 3 ? nq1 : 1);
+  int nq = m->geo == BH_TRI3 ? nq1 : nq1 * nq1 * (m->h.dim == 3 ? nq1 : 1);
   int64_t n = m->h.ncells * nq * m->h.dim;
   double* d = xq;
   if (where == BH_HOST) BH_HIP(c, dalloc(&d, (size_t)n));

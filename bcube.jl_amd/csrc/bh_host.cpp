@@ -122,7 +122,136 @@ void lagrange_1d(int order, double t, double* v, double* dv) {
   }
 }
 
+// Triangle rules of Measure degree 4..8: the GLTRI6/7/12/16 tables of the
+// reference (quadrature.jl:289-377, Witherden & Vincent), reference triangle
+// (0,0),(1,0),(0,1).  Degrees 1..3 use FEMQuad tables the reference does not
+// vendor: no rule (empty).
+static void triangle_rule(int degree, std::vector<double>& w, std::vector<double>& x) {
+  w.clear();
+  x.clear();
+  auto pt = [&](double a, double b) { x.push_back(a); x.push_back(b); };
+  if (degree == 4) {
+    const double P1 = 0.22338158967801146569500700843312 / 2;
+    const double P2 = 0.10995174365532186763832632490021 / 2;
+    const double A = 0.44594849091596488631832925388305;
+    const double B = 0.09157621350977074345957146340220;
+    w = {P2, P2, P2, P1, P1, P1};
+    pt(B, B); pt(1.0 - 2.0 * B, B); pt(B, 1.0 - 2.0 * B);
+    pt(A, 1.0 - 2 * A); pt(A, A); pt(1.0 - 2.0 * A, A);
+  } else if (degree == 5) {
+    const double A = 0.47014206410511508977044120951345;
+    const double B = 0.10128650732345633880098736191512;
+    const double P1 = 0.13239415278850618073764938783315 / 2;
+    const double P2 = 0.12593918054482715259568394550018 / 2;
+    w = {9.0 / 80, P1, P1, P1, P2, P2, P2};
+    pt(1.0 / 3, 1.0 / 3); pt(A, A); pt(1.0 - 2.0 * A, A); pt(A, 1.0 - 2.0 * A);
+    pt(B, B); pt(1.0 - 2.0 * B, B); pt(B, 1.0 - 2.0 * B);
+  } else if (degree == 6) {
+    const double A = (-0.87382197101699554331933679425836168532 + 1) / 2;
+    const double B = (-0.50142650965817915741672289378596184782 + 1) / 2;
+    const double C = (-0.37929509793243118916678453208689569359 + 1) / 2;
+    const double D = (-0.89370990031036610529350065673720370601 + 1) / 2;
+    const double P1 = 0.10168981274041363384187361821373796809 / 4;
+    const double P2 = 0.23357255145275873205057922277115888265 / 4;
+    const double P3 = 0.16570215123674715038710691284088490796 / 4;
+    w = {P1, P1, P1, P2, P2, P2, P3, P3, P3, P3, P3, P3};
+    pt(A, A); pt(1.0 - 2.0 * A, A); pt(A, 1.0 - 2.0 * A);
+    pt(B, B); pt(1.0 - 2.0 * B, B); pt(B, 1.0 - 2.0 * B);
+    pt(C, D); pt(D, C); pt(1.0 - C - D, C); pt(1.0 - C - D, D); pt(C, 1.0 - C - D);
+    pt(D, 1.0 - C - D);
+  } else if (degree == 7 || degree == 8) {
+    // (x, y, w) on [-1,1]^2, rescaled by (x+1)/2, (y+1)/2, w/4
+    static const double T[16][3] = {
+        {-0.33333333333333333333333333333333333333, -0.33333333333333333333333333333333333333, 0.2886312153555743365021822209781292496},
+        {-0.081414823414553687942368971011661355879, -0.83717035317089262411526205797667728824, 0.1901832685345692495877922087771686332},
+        {-0.83717035317089262411526205797667728824, -0.081414823414553687942368971011661355879, 0.1901832685345692495877922087771686332},
+        {-0.081414823414553687942368971011661355879, -0.081414823414553687942368971011661355879, 0.1901832685345692495877922087771686332},
+        {-0.65886138449647958675541299701707099796, 0.31772276899295917351082599403414199593, 0.20643474106943650056358310058425806003},
+        {0.31772276899295917351082599403414199593, -0.65886138449647958675541299701707099796, 0.20643474106943650056358310058425806003},
+        {-0.65886138449647958675541299701707099796, -0.65886138449647958675541299701707099796, 0.20643474106943650056358310058425806003},
+        {-0.89890554336593804908315289880680210631, 0.79781108673187609816630579761360421262, 0.064916995246396160621851856683561193593},
+        {0.79781108673187609816630579761360421262, -0.89890554336593804908315289880680210631, 0.064916995246396160621851856683561193593},
+        {-0.89890554336593804908315289880680210631, -0.89890554336593804908315289880680210631, 0.064916995246396160621851856683561193593},
+        {-0.98321044518008478932557233092141110162, 0.45698478591080856248200075835212392604, 0.05446062834886998852968938014781784832},
+        {0.45698478591080856248200075835212392604, -0.98321044518008478932557233092141110162, 0.05446062834886998852968938014781784832},
+        {-0.47377434073072377315642842743071282442, 0.45698478591080856248200075835212392604, 0.05446062834886998852968938014781784832},
+        {0.45698478591080856248200075835212392604, -0.47377434073072377315642842743071282442, 0.05446062834886998852968938014781784832},
+        {-0.47377434073072377315642842743071282442, -0.98321044518008478932557233092141110162, 0.05446062834886998852968938014781784832},
+        {-0.98321044518008478932557233092141110162, -0.47377434073072377315642842743071282442, 0.05446062834886998852968938014781784832}};
+    for (int k = 0; k < 16; ++k) {
+      pt(0.5 * (T[k][0] + 1), 0.5 * (T[k][1] + 1));
+      w.push_back(T[k][2] / 4);
+    }
+  }
+}
+
+int triangle_rule_points(int quad_degree) {
+  std::vector<double> w, x;
+  triangle_rule(quad_degree, w, x);
+  return (int)w.size();
+}
+
+// Triangle Lagrange nodes as barycentric lattice points (i, j, k), i+j+k = p,
+// in the reference's dof order: vertices, edge dofs from each edge's first
+// vertex to its second, interior (triangle.jl:1-45, 120-145; P4 extends P3's
+// layout with interior nodes (2,1,1), (1,2,1), (1,1,2)).
+static const int kTriLattice[5][15][3] = {
+    {{0, 0, 0}},
+    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
+    {{2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}},
+    {{3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {2, 1, 0}, {1, 2, 0}, {0, 2, 1}, {0, 1, 2}, {1, 0, 2},
+     {2, 0, 1}, {1, 1, 1}},
+    {{4, 0, 0}, {0, 4, 0}, {0, 0, 4}, {3, 1, 0}, {2, 2, 0}, {1, 3, 0}, {0, 3, 1}, {0, 2, 2},
+     {0, 1, 3}, {1, 0, 3}, {2, 0, 2}, {3, 0, 1}, {2, 1, 1}, {1, 2, 1}, {1, 1, 2}}};
+
+// Π_{m<n} (p λ - m)/(m+1) and its derivative in λ.
+static void lattice_factor(int p, int n, double lam, double& f, double& df) {
+  f = 1.0;
+  df = 0.0;
+  for (int m = 0; m < n; ++m) {
+    double t = (p * lam - m) / (m + 1);
+    df = df * t + f * (double)p / (m + 1);
+    f *= t;
+  }
+}
+
+static void triangle_basis(int p, double x, double y, double* val, double* grad) {
+  const double l[3] = {1.0 - x - y, x, y};
+  const int nl = (p + 1) * (p + 2) / 2;
+  for (int n = 0; n < nl; ++n) {
+    double f[3], df[3];
+    for (int a = 0; a < 3; ++a) lattice_factor(p, kTriLattice[p][n][a], l[a], f[a], df[a]);
+    val[n] = f[0] * f[1] * f[2];
+    const double d1 = df[0] * f[1] * f[2], d2 = f[0] * df[1] * f[2], d3 = f[0] * f[1] * df[2];
+    grad[2 * n] = d2 - d1;      // ∂/∂ξ1: λ2 = ξ1, λ1 = 1 - ξ1 - ξ2
+    grad[2 * n + 1] = d3 - d1;  // ∂/∂ξ2: λ3 = ξ2
+  }
+}
+
+static Tables make_triangle_tables(int order, int quad_degree) {
+  Tables T;
+  T.dim = 2;
+  T.order = order;
+  T.nl = (order + 1) * (order + 2) / 2;
+  triangle_rule(quad_degree, T.w, T.xi);
+  T.nq = (int)T.w.size();
+  T.phi.assign((size_t)T.nq * T.nl, 0.0);
+  T.dphi.assign((size_t)T.nq * T.nl * 2, 0.0);
+  for (int q = 0; q < T.nq; ++q)
+    triangle_basis(order, T.xi[2 * q], T.xi[2 * q + 1], &T.phi[(size_t)q * T.nl],
+                   &T.dphi[(size_t)q * T.nl * 2]);
+  // Tri3 geometry (mapping.jl:395-416): N = (1-ξ1-ξ2, ξ1, ξ2)
+  T.ngeo = 3;
+  for (int q = 0; q < T.nq; ++q) {
+    const double x = T.xi[2 * q], y = T.xi[2 * q + 1];
+    T.gphi.insert(T.gphi.end(), {1.0 - x - y, x, y});
+    T.dgphi.insert(T.dgphi.end(), {-1.0, -1.0, 1.0, 0.0, 0.0, 1.0});
+  }
+  return T;
+}
+
 Tables make_tables(int shape, int order, int quad_degree, int geo_ct) {
+  if (shape == TRIANGLE) return make_triangle_tables(order, quad_degree);
   Tables T;
   T.dim = dim_of_shape(shape);
   T.order = order;
@@ -241,7 +370,9 @@ static Lists edge_dofs(int shape, int order) {
     if (order <= 1) return Lists(3);
     if (order == 2) return {{3}, {4}, {5}};
     if (order == 3) return {{3, 4}, {5, 6}, {7, 8}};
-    return Lists();  // no reference basis beyond P3
+    // P4 (beyond the reference's P3): three dofs per edge, first vertex to second
+    if (order == 4) return {{3, 4, 5}, {6, 7, 8}, {9, 10, 11}};
+    return Lists();
   }
   int ne = shape == CUBE ? 12 : 4;
   if (order == 0) return Lists(ne);
@@ -372,7 +503,7 @@ i64 dofhandler(const HostMesh& m, int order, bool continuous, std::vector<i64>& 
         Lists el = edges_of(ct);
         Lists ed = edge_dofs(s, order);
         if (order > 0 && ed.size() != el.size() && s == TRIANGLE) {
-          err = "triangle Lagrange order > 3 has no reference basis";
+          err = "triangle Lagrange order > 4 is not supported";
           return -1;
         }
         for (size_t e = 0; e < el.size() && e < ed.size(); ++e) {
@@ -439,12 +570,14 @@ int colour_cells(const std::vector<int32_t>& rdofs, int nl, i64 nrows, i64 ncell
                  std::vector<int32_t>& color_ptr, std::vector<int32_t>& cells_by_color) {
   std::vector<i64> aptr(nrows + 1, 0);
   for (i64 c = 0; c < ncells; ++c)
-    for (int i = 0; i < nl; ++i) aptr[rdofs[c * nl + i] + 1]++;
+    for (int i = 0; i < nl; ++i)
+      if (rdofs[c * nl + i] >= 0) aptr[rdofs[c * nl + i] + 1]++;
   for (i64 r = 0; r < nrows; ++r) aptr[r + 1] += aptr[r];
   std::vector<int32_t> adj(aptr[nrows]);
   std::vector<i64> fill(aptr.begin(), aptr.end() - 1);
   for (i64 c = 0; c < ncells; ++c)
-    for (int i = 0; i < nl; ++i) adj[fill[rdofs[c * nl + i]]++] = (int32_t)c;
+    for (int i = 0; i < nl; ++i)
+      if (rdofs[c * nl + i] >= 0) adj[fill[rdofs[c * nl + i]]++] = (int32_t)c;
   std::vector<int32_t> color(ncells, -1);
   int ncol = 0;
   std::vector<uint8_t> used(256, 0);
@@ -452,6 +585,7 @@ int colour_cells(const std::vector<int32_t>& rdofs, int nl, i64 nrows, i64 ncell
     std::fill(used.begin(), used.begin() + std::max(ncol, 1), 0);
     for (int i = 0; i < nl; ++i) {
       i64 r = rdofs[c * nl + i];
+      if (r < 0) continue;
       for (i64 p = aptr[r]; p < aptr[r + 1]; ++p) {
         int32_t o = adj[p];
         if (o >= c) break;  // adjacency lists are in cell order
@@ -481,35 +615,41 @@ int colour_cells(const std::vector<int32_t>& rdofs, int nl, i64 nrows, i64 ncell
 bool build_pattern(const std::vector<int32_t>& rdofs, int nl_row, i64 nrows,
                    const std::vector<int32_t>& cdofs, int nl_col, i64 ncols, i64 ncells,
                    HostPattern& p, std::string& err) {
+  // dof lists are nl per cell; -1 marks a padding slot (cells with fewer dofs
+  // than the stride on mixed meshes)
   p.nrows = nrows; p.ncols = ncols; p.nl_row = nl_row; p.nl_col = nl_col;
   for (i64 k = 0; k < ncells * nl_row; ++k)
-    if (rdofs[k] < 0 || rdofs[k] >= nrows) { err = "row dof out of range"; return false; }
+    if (rdofs[k] < -1 || rdofs[k] >= nrows) { err = "row dof out of range"; return false; }
   for (i64 k = 0; k < ncells * nl_col; ++k)
-    if (cdofs[k] < 0 || cdofs[k] >= ncols) { err = "column dof out of range"; return false; }
+    if (cdofs[k] < -1 || cdofs[k] >= ncols) { err = "column dof out of range"; return false; }
   // row -> cells adjacency
   std::vector<i64> aptr(nrows + 1, 0);
   for (i64 c = 0; c < ncells; ++c)
-    for (int i = 0; i < nl_row; ++i) aptr[rdofs[c * nl_row + i] + 1]++;
+    for (int i = 0; i < nl_row; ++i)
+      if (rdofs[c * nl_row + i] >= 0) aptr[rdofs[c * nl_row + i] + 1]++;
   for (i64 r = 0; r < nrows; ++r) aptr[r + 1] += aptr[r];
   std::vector<int32_t> adj(aptr[nrows]);
   {
     std::vector<i64> fill(aptr.begin(), aptr.end() - 1);
     for (i64 c = 0; c < ncells; ++c)
-      for (int i = 0; i < nl_row; ++i) adj[fill[rdofs[c * nl_row + i]]++] = (int32_t)c;
+      for (int i = 0; i < nl_row; ++i)
+        if (rdofs[c * nl_row + i] >= 0) adj[fill[rdofs[c * nl_row + i]]++] = (int32_t)c;
   }
+  auto row_columns = [&](i64 r, std::vector<int32_t>& buf) {
+    buf.clear();
+    for (i64 q = aptr[r]; q < aptr[r + 1]; ++q) {
+      i64 c = adj[q];
+      for (int j = 0; j < nl_col; ++j)
+        if (cdofs[c * nl_col + j] >= 0) buf.push_back(cdofs[c * nl_col + j]);
+    }
+    std::sort(buf.begin(), buf.end());
+    return (i64)(std::unique(buf.begin(), buf.end()) - buf.begin());
+  };
   // pass 1: row lengths
   std::vector<i64> len(nrows, 0);
   parallel_for(nrows, [&](i64 b, i64 e) {
     std::vector<int32_t> buf;
-    for (i64 r = b; r < e; ++r) {
-      buf.clear();
-      for (i64 q = aptr[r]; q < aptr[r + 1]; ++q) {
-        i64 c = adj[q];
-        for (int j = 0; j < nl_col; ++j) buf.push_back(cdofs[c * nl_col + j]);
-      }
-      std::sort(buf.begin(), buf.end());
-      len[r] = std::unique(buf.begin(), buf.end()) - buf.begin();
-    }
+    for (i64 r = b; r < e; ++r) len[r] = row_columns(r, buf);
   });
   p.rowptr.assign(nrows + 1, 0);
   for (i64 r = 0; r < nrows; ++r) p.rowptr[r + 1] = p.rowptr[r] + len[r];
@@ -521,14 +661,8 @@ bool build_pattern(const std::vector<int32_t>& rdofs, int nl_row, i64 nrows,
   parallel_for(nrows, [&](i64 b, i64 e) {
     std::vector<int32_t> buf;
     for (i64 r = b; r < e; ++r) {
-      buf.clear();
-      for (i64 q = aptr[r]; q < aptr[r + 1]; ++q) {
-        i64 c = adj[q];
-        for (int j = 0; j < nl_col; ++j) buf.push_back(cdofs[c * nl_col + j]);
-      }
-      std::sort(buf.begin(), buf.end());
-      auto end = std::unique(buf.begin(), buf.end());
-      std::copy(buf.begin(), end, p.colval.begin() + p.rowptr[r]);
+      i64 n = row_columns(r, buf);
+      std::copy(buf.begin(), buf.begin() + n, p.colval.begin() + p.rowptr[r]);
     }
   });
   // gather structure: row -> (cell, i) in cell order, and per-(cell, i) column offsets
@@ -554,20 +688,28 @@ bool build_pattern(const std::vector<int32_t>& rdofs, int nl_row, i64 nrows,
     for (i64 r = 0; r < nrows; ++r)
       for (int q = p.pptr[r]; q < p.pptr[r + 1]; ++q) p.pairs_padded[r * p.pad + (q - p.pptr[r])] = p.pairs[q];
   }
-  if (p.max_row <= 256) p.offset_bytes = 1;
-  else if (p.max_row <= 65536) p.offset_bytes = 2;
-  else { err = "row longer than 65536 entries"; return false; }
+  // offsets: the largest value of the type marks a padding column (never a
+  // valid offset: rows are shorter than that)
+  if (p.max_row <= 255) p.offset_bytes = 1;
+  else if (p.max_row <= 65535) p.offset_bytes = 2;
+  else { err = "row longer than 65535 entries"; return false; }
   i64 per = (i64)nl_row * nl_col;
   p.goff.assign((size_t)(ncells * per * p.offset_bytes), 0);
   parallel_for(ncells, [&](i64 b, i64 e) {
     for (i64 c = b; c < e; ++c)
       for (int i = 0; i < nl_row; ++i) {
         i64 r = rdofs[c * nl_row + i];
-        const int32_t* row = &p.colval[p.rowptr[r]];
-        i64 rl = p.rowptr[r + 1] - p.rowptr[r];
         for (int j = 0; j < nl_col; ++j) {
-          i64 off = std::lower_bound(row, row + rl, cdofs[c * nl_col + j]) - row;
           size_t idx = (size_t)(c * per + (i64)i * nl_col + j);
+          i64 cd = cdofs[c * nl_col + j];
+          i64 off;
+          if (r < 0 || cd < 0) {
+            off = p.offset_bytes == 1 ? 0xFF : 0xFFFF;
+          } else {
+            const int32_t* row = &p.colval[p.rowptr[r]];
+            i64 rl = p.rowptr[r + 1] - p.rowptr[r];
+            off = std::lower_bound(row, row + rl, (int32_t)cd) - row;
+          }
           if (p.offset_bytes == 1) {
             p.goff[idx] = (uint8_t)off;
           } else {

@@ -33,6 +33,8 @@ struct Tables {
 // quad_degree follows Measure(Ω, degree): n = ceil((degree+1)/2) Gauss points per axis.
 Tables make_tables(int shape, int order, int quad_degree, int geo_celltype);
 int gauss_points_per_axis(int quad_degree);
+// Points of the triangle rule of a Measure degree (0 if there is none).
+int triangle_rule_points(int quad_degree);
 // 1D Gauss-Legendre rule on [-1,1] (x[k] = -x[n-1-k] exactly) and the 1D
 // uniform-node Lagrange basis with its derivative at t.
 void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w);
@@ -55,6 +57,9 @@ i64 dofhandler(const HostMesh& m, int order, bool continuous, std::vector<i64>& 
                std::vector<i64>& offset, std::string& err);
 
 // CSR pattern + the gather structure of the deterministic row reduction.
+// Dof lists have nl_row / nl_col slots per cell; -1 marks an empty slot (mixed
+// meshes pad every cell to the largest cell's dof count), whose offsets in goff
+// are the sentinel 0xFF / 0xFFFF.
 struct HostPattern {
   i64 nrows = 0, ncols = 0, nnz = 0;
   int nl_row = 0, nl_col = 0;

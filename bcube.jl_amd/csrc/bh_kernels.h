@@ -18,13 +18,14 @@ struct LaunchArgs {
   const int32_t* c2n = nullptr;    // ncells * nodes-per-cell (geometric nodes)
   const int32_t* rdofs = nullptr;  // ncells * nl
   const double* tab = nullptr;     // packed reference tables
-  const int32_t* cells = nullptr;  // cells of this colour
+  const int32_t* cells = nullptr;  // cells of this colour (source) / of this cell type (bilinear)
   int ncells_color = 0;
   int slot0 = 0;                   // slot of cells[0] in the scatter map
   int64_t ncells_total = 0;
   // bilinear: element kernel writes ke, gather reduces ke into vals
   void* ke = nullptr;
   bool ke_fp64 = true;
+  int erow = 0;                    // element rows per cell in ke (the largest cell's dof count)
   const int32_t* pptr = nullptr;
   const int32_t* pairs = nullptr;  // variable-length lists, or padded to max_adj slots
   bool padded_pairs = false;
@@ -51,6 +52,9 @@ bool kernel_supported(int geo_celltype, int order, int nq1);
 int packed_table_size(int geo_celltype, int order, int nq1);
 hipError_t launch(int geo_celltype, int order, int nq1, const LaunchArgs& a);
 hipError_t launch_fill(hipStream_t s, double* p, int64_t n, double v);
+// Row gather for element rows of `nl` entries (the space's padded dof count).
+bool gather_supported(int nl);
+hipError_t launch_gather(int nl, const LaunchArgs& a);
 // Matrix-core element kernels (bh_mfma.hip): BH_FP32 and BH_BF16X3, bilinear forms only.
 bool mfma_supported(int geo_celltype, int order, int nq1);
 hipError_t launch_mfma(int geo_celltype, int order, int nq1, const LaunchArgs& a);

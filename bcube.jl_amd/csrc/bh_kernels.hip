@@ -37,8 +37,11 @@ namespace bh {
 template <int D, int NL, int NQ, int NG, bool MASS>
 __global__ __launch_bounds__(kBlock) void k_element_f64(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
-    const double* __restrict__ tab, int ncells, const double* __restrict__ eta, double eta_scale,
-    const double* __restrict__ mcoef, double mass_scale, double* __restrict__ ke) {
+    const double* __restrict__ tab, int ncells, const int32_t* __restrict__ cells, int erow,
+    const double* __restrict__ eta, double eta_scale, const double* __restrict__ mcoef,
+    double mass_scale, double* __restrict__ ke) {
+  // cells[k]: global id of the k-th cell of this launch (NULL: k itself);
+  // coefficients are per global cell, element rows go to ke[(gcell*erow + i)*KS].
   using L = TabLayout<D, NL, NQ, NG>;
   constexpr int CPB = kBlock / NL;  // cells per group
   constexpr int NS = SymN<D>::N;
@@ -72,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void k_element_f64(
     for (int k = tid; k < CPB * NQ; k += kBlock) {
       int c = k / NQ, q = k - c * NQ;
       if (base + c >= ncells) continue;
-      int cell = base + c;
+      const int cell = cells ? cells[base + c] : base + c;
       double J[D][D];
 #pragma unroll
       for (int a = 0; a < D; ++a)
@@ -136,7 +139,8 @@ __global__ __launch_bounds__(kBlock) void k_element_f64(
       }
     }
     // 4. row i of the element matrix, contiguous in the workspace
-    double* out = ke + ((int64_t)(base + cs) * NL + li) * KS;
+    const int64_t gcell = cells ? cells[base + cs] : base + cs;
+    double* out = ke + (gcell * erow + li) * KS;
 #pragma unroll
     for (int j = 0; j < NL; ++j) out[j] = acc[j];
   }
@@ -157,6 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_padded(
   constexpr int KS = KeStride<NL>::value;
   constexpr int HW = kBlock / 32;
   constexpr int NJ = (NL + 31) / 32;
+  constexpr int kSent = (int)(OFF)~(OFF)0;  // padding column (mixed meshes)
   __shared__ double buf[HW][RPH][ROWCAP];
   const int h = threadIdx.x >> 5;
   const int l = threadIdx.x & 31;
@@ -183,6 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_padded(
 #pragma unroll
         for (int t = 0; t < NJ; ++t) {
           const int j = l + 32 * t;
+          o[r][k][t] = kSent;
           if (pr[r][k] >= 0 && j < NL) {
             v[r][k][t] = ke[(int64_t)pr[r][k] * KS + j];
             o[r][k][t] = goff[(int64_t)pr[r][k] * NL + j];
@@ -198,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_padded(
       for (int k = 0; k < MAXP; ++k)
 #pragma unroll
         for (int t = 0; t < NJ; ++t)
-          if (pr[r][k] >= 0 && l + 32 * t < NL) buf[h][r][o[r][k][t]] += (double)v[r][k][t];
+          if (o[r][k][t] != kSent) buf[h][r][o[r][k][t]] += (double)v[r][k][t];
 #pragma unroll
     for (int r = 0; r < RPH; ++r)
       for (int k = l; k < len[r]; k += 32) vals[rs[r] + k] = buf[h][r][k];
@@ -216,6 +222,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const T* __restrict__ ke,
   constexpr int HW = kBlock / 32;
   constexpr int PB = 8;                 // element rows loaded before they are added
   constexpr int NJ = (NL + 31) / 32;    // element-row entries per lane
+  constexpr int kSent = (int)(OFF)~(OFF)0;  // padding column (mixed meshes)
   __shared__ double buf[HW][ROWCAP];
   const int h = threadIdx.x >> 5;
   const int l = threadIdx.x & 31;
@@ -235,6 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const T* __restrict__ ke,
 #pragma unroll
           for (int t = 0; t < NJ; ++t) {
             const int j = l + 32 * t;
+            o[k][t] = kSent;
             if (j < NL) {
               v[k][t] = ke[pr * KS + j];
               o[k][t] = goff[pr * NL + j];
@@ -247,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const T* __restrict__ ke,
         if (k < np) {
 #pragma unroll
           for (int t = 0; t < NJ; ++t)
-            if (l + 32 * t < NL) buf[h][o[k][t]] += (double)v[k][t];
+            if (o[k][t] != kSent) buf[h][o[k][t]] += (double)v[k][t];
         }
       }
     }
@@ -348,65 +356,73 @@ static int grid_for(int ngroups) {
   return g < 1 ? 1 : g;
 }
 
+template <int NL>
+static hipError_t gather_nl(const LaunchArgs& A) {
+  const bool f64 = A.ke_fp64;
+  if (A.padded_pairs) {
+    constexpr int RPH = 2;
+    int64_t blocks = (A.nrows + (kBlock / 32) * RPH - 1) / ((kBlock / 32) * RPH);
+    dim3 grid((unsigned)(blocks < 16384 ? (blocks < 1 ? 1 : blocks) : 16384));
+#define BH_GP(T, OFFT, CAP, MP)                                                                   \
+  hipLaunchKernelGGL((k_gather_padded<T, OFFT, NL, CAP, MP, RPH>), grid, dim3(kBlock), 0, A.stream, \
+                   (const T*)A.ke, A.pairs, (const OFFT*)A.goff, A.rowptr, A.nrows, A.vals)
+#define BH_GP_MP(T, OFFT, CAP)                         \
+  do {                                                 \
+  if (A.max_adj <= 4) BH_GP(T, OFFT, CAP, 4);        \
+  else if (A.max_adj <= 8) BH_GP(T, OFFT, CAP, 8);   \
+  else BH_GP(T, OFFT, CAP, 16);                      \
+  } while (0)
+    if (A.max_row <= 128 && A.offset_bytes == 1) {
+      if (f64) BH_GP_MP(double, uint8_t, 128); else BH_GP_MP(float, uint8_t, 128);
+    } else if (A.max_row <= 256 && A.offset_bytes == 1) {
+      if (f64) BH_GP_MP(double, uint8_t, 256); else BH_GP_MP(float, uint8_t, 256);
+    } else if (A.max_row <= 512) {
+      if (f64) BH_GP_MP(double, uint16_t, 512); else BH_GP_MP(float, uint16_t, 512);
+    } else {
+      return hipErrorInvalidValue;
+    }
+#undef BH_GP_MP
+#undef BH_GP
+  } else {
+    int64_t blocks = (A.nrows + kBlock / 32 - 1) / (kBlock / 32);
+    dim3 grid((unsigned)(blocks < 16384 ? (blocks < 1 ? 1 : blocks) : 16384));
+#define BH_G(T, OFFT, CAP)                                                                  \
+  hipLaunchKernelGGL((k_gather<T, OFFT, NL, CAP>), grid, dim3(kBlock), 0, A.stream,         \
+                   (const T*)A.ke, A.pptr, A.pairs, (const OFFT*)A.goff, A.rowptr, A.nrows, \
+                   A.vals)
+    if (A.max_row <= 128 && A.offset_bytes == 1) {
+      if (f64) BH_G(double, uint8_t, 128); else BH_G(float, uint8_t, 128);
+    } else if (A.max_row <= 256 && A.offset_bytes == 1) {
+      if (f64) BH_G(double, uint8_t, 256); else BH_G(float, uint8_t, 256);
+    } else if (A.max_row <= 512) {
+      if (f64) BH_G(double, uint16_t, 512); else BH_G(float, uint16_t, 512);
+    } else {
+      return hipErrorInvalidValue;
+    }
+#undef BH_G
+  }
+  return hipGetLastError();
+}
+
 template <int D, int NL, int NQ, int NG>
 static hipError_t launch_all(const LaunchArgs& A) {
   constexpr int CPB = kBlock / NL;
   if (A.kind == LaunchArgs::BILINEAR) {
     int ng = (int)((A.ncells_total + CPB - 1) / CPB);
     dim3 grid(grid_for(ng));
+    const int erow = A.erow > 0 ? A.erow : NL;
     if (A.mass)
       hipLaunchKernelGGL((k_element_f64<D, NL, NQ, NG, true>), grid, dim3(kBlock), 0, A.stream,
-                         A.coords, A.c2n, A.tab, (int)A.ncells_total, A.eta, A.eta_scale, A.mcoef,
-                         A.mass_scale, (double*)A.ke);
+                         A.coords, A.c2n, A.tab, (int)A.ncells_total, A.cells, erow, A.eta,
+                         A.eta_scale, A.mcoef, A.mass_scale, (double*)A.ke);
     else
       hipLaunchKernelGGL((k_element_f64<D, NL, NQ, NG, false>), grid, dim3(kBlock), 0, A.stream,
-                         A.coords, A.c2n, A.tab, (int)A.ncells_total, A.eta, A.eta_scale, A.mcoef,
-                         A.mass_scale, (double*)A.ke);
+                         A.coords, A.c2n, A.tab, (int)A.ncells_total, A.cells, erow, A.eta,
+                         A.eta_scale, A.mcoef, A.mass_scale, (double*)A.ke);
   } else if (A.kind == LaunchArgs::GATHER) {
-    const bool f64 = A.ke_fp64;
-    if (A.padded_pairs) {
-      constexpr int RPH = 2;
-      int64_t blocks = (A.nrows + (kBlock / 32) * RPH - 1) / ((kBlock / 32) * RPH);
-      dim3 grid((unsigned)(blocks < 16384 ? (blocks < 1 ? 1 : blocks) : 16384));
-#define BH_GP(T, OFFT, CAP, MP)                                                                   \
-  hipLaunchKernelGGL((k_gather_padded<T, OFFT, NL, CAP, MP, RPH>), grid, dim3(kBlock), 0, A.stream, \
-                     (const T*)A.ke, A.pairs, (const OFFT*)A.goff, A.rowptr, A.nrows, A.vals)
-#define BH_GP_MP(T, OFFT, CAP)                         \
-  do {                                                 \
-    if (A.max_adj <= 4) BH_GP(T, OFFT, CAP, 4);        \
-    else if (A.max_adj <= 8) BH_GP(T, OFFT, CAP, 8);   \
-    else BH_GP(T, OFFT, CAP, 16);                      \
-  } while (0)
-      if (A.max_row <= 128 && A.offset_bytes == 1) {
-        if (f64) BH_GP_MP(double, uint8_t, 128); else BH_GP_MP(float, uint8_t, 128);
-      } else if (A.max_row <= 256 && A.offset_bytes == 1) {
-        if (f64) BH_GP_MP(double, uint8_t, 256); else BH_GP_MP(float, uint8_t, 256);
-      } else if (A.max_row <= 512) {
-        if (f64) BH_GP_MP(double, uint16_t, 512); else BH_GP_MP(float, uint16_t, 512);
-      } else {
-        return hipErrorInvalidValue;
-      }
-#undef BH_GP_MP
-#undef BH_GP
-    } else {
-      int64_t blocks = (A.nrows + kBlock / 32 - 1) / (kBlock / 32);
-      dim3 grid((unsigned)(blocks < 16384 ? (blocks < 1 ? 1 : blocks) : 16384));
-#define BH_G(T, OFFT, CAP)                                                                  \
-  hipLaunchKernelGGL((k_gather<T, OFFT, NL, CAP>), grid, dim3(kBlock), 0, A.stream,         \
-                     (const T*)A.ke, A.pptr, A.pairs, (const OFFT*)A.goff, A.rowptr, A.nrows, \
-                     A.vals)
-      if (A.max_row <= 128 && A.offset_bytes == 1) {
-        if (f64) BH_G(double, uint8_t, 128); else BH_G(float, uint8_t, 128);
-      } else if (A.max_row <= 256 && A.offset_bytes == 1) {
-        if (f64) BH_G(double, uint8_t, 256); else BH_G(float, uint8_t, 256);
-      } else if (A.max_row <= 512) {
-        if (f64) BH_G(double, uint16_t, 512); else BH_G(float, uint16_t, 512);
-      } else {
-        return hipErrorInvalidValue;
-      }
-#undef BH_G
-    }
+    return launch_gather(NL, A);
   } else if (A.kind == LaunchArgs::SOURCE) {
+
     int ng = (A.ncells_color + CPB - 1) / CPB;
     hipLaunchKernelGGL((k_source_f64<D, NL, NQ, NG>), dim3(grid_for(ng)), dim3(kBlock), 0,
                        A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.ncells_color,
@@ -437,11 +453,16 @@ struct Entry {
   {BH_QUAD4, ORDER, NQ1, &launch_all<2, BH_Q(ORDER, NQ1), 4>, &table_size<2, BH_Q(ORDER, NQ1), 4>}
 #define BH_ENTRY_H(ORDER, NQ1)                                                   \
   {BH_HEXA8, ORDER, NQ1, &launch_all<3, BH_H(ORDER, NQ1), 8>, &table_size<3, BH_H(ORDER, NQ1), 8>}
+// triangles: key = points of the triangle rule
+#define BH_ENTRY_T(ORDER, NPTS)                                                             \
+  {BH_TRI3, ORDER, NPTS, &launch_all<2, ((ORDER) + 1) * ((ORDER) + 2) / 2, NPTS, 3>,          \
+   &table_size<2, ((ORDER) + 1) * ((ORDER) + 2) / 2, NPTS, 3>}
 
 static const Entry kTable[] = {
     BH_ENTRY_Q(1, 1), BH_ENTRY_Q(1, 2), BH_ENTRY_Q(1, 3), BH_ENTRY_Q(2, 2), BH_ENTRY_Q(2, 3),
     BH_ENTRY_Q(2, 4), BH_ENTRY_Q(3, 3), BH_ENTRY_Q(3, 4), BH_ENTRY_Q(4, 5),
     BH_ENTRY_H(1, 1), BH_ENTRY_H(1, 2), BH_ENTRY_H(1, 3), BH_ENTRY_H(2, 2), BH_ENTRY_H(2, 3),
+    BH_ENTRY_T(1, 6), BH_ENTRY_T(2, 6), BH_ENTRY_T(2, 7), BH_ENTRY_T(3, 12), BH_ENTRY_T(4, 16),
 };
 
 bool kernel_supported(int geo, int order, int nq1) {
@@ -460,6 +481,32 @@ int packed_table_size(int geo, int order, int nq1) {
   for (const Entry& e : kTable)
     if (e.geo == geo && e.order == order && e.nq1 == nq1) return e.size();
   return -1;
+}
+
+// Element-row lengths with a gather instantiation: quads Q1..Q4, hexahedra
+// Q1/Q2, triangles P1..P4 (mixed Quad4/Tri3 meshes use the quad length).
+bool gather_supported(int nl) {
+  switch (nl) {
+    case 3: case 4: case 6: case 8: case 9: case 10: case 15: case 16: case 25: case 27:
+      return true;
+  }
+  return false;
+}
+
+hipError_t launch_gather(int nl, const LaunchArgs& A) {
+  switch (nl) {
+    case 3: return gather_nl<3>(A);
+    case 4: return gather_nl<4>(A);
+    case 6: return gather_nl<6>(A);
+    case 8: return gather_nl<8>(A);
+    case 9: return gather_nl<9>(A);
+    case 10: return gather_nl<10>(A);
+    case 15: return gather_nl<15>(A);
+    case 16: return gather_nl<16>(A);
+    case 25: return gather_nl<25>(A);
+    case 27: return gather_nl<27>(A);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_fill(hipStream_t s, double* p, int64_t n, double v) {

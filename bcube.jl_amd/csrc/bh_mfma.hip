@@ -50,9 +50,11 @@ __device__ __forceinline__ float from_bf16(__bf16 h) {
 template <int D, int NL, int NQ, int NG, bool MASS, int PREC>
 __global__ __launch_bounds__(kBlock) void k_element_mfma(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
-    const double* __restrict__ tab, int ncells, const double* __restrict__ eta,
-    double eta_scale, const double* __restrict__ mcoef, double mass_scale,
-    void* __restrict__ ke_out) {
+    const double* __restrict__ tab, int ncells, const int32_t* __restrict__ cells, int erow,
+    const double* __restrict__ eta, double eta_scale, const double* __restrict__ mcoef,
+    double mass_scale, void* __restrict__ ke_out) {
+  // cells[k]: global id of the k-th cell of this launch (NULL: k itself);
+  // coefficients are per global cell, element rows go to ke[(gcell*erow + i)*KS].
   static_assert(NL <= 32, "two 16-row tiles cover at most 32 local dofs");
   static_assert(D + 1 <= 4, "k-step holds D gradient components + mass");
   using L = TabLayout<D, NL, NQ, NG>;
@@ -145,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
         for (int t = 0; t < 8; ++t) gq[t] = (RT)0;
         continue;
       }
-      int cell = base + c;
+      const int cell = cells ? cells[base + c] : base + c;
       double J[D][D];
 #pragma unroll
       for (int a = 0; a < D; ++a)
@@ -249,7 +251,8 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
       if (!valid) continue;
       // lane holds K[i][j], column j of cell c, for rows i = rt*16 + 4*(lane/16) + v
       // (FP32/BF16 16x16 accumulators) or i = rt*16 + lane/16 + 4*v (FP64 16x16x4)
-      RT* out = ke + (int64_t)(base + c) * NL * KS + j;
+      const int64_t gcell = cells ? cells[base + c] : base + c;
+      RT* out = ke + gcell * erow * KS + j;
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
@@ -270,8 +273,8 @@ static hipError_t launch_mfma32(const LaunchArgs& A) {
   dim3 grid(ng < 2048 ? (ng < 1 ? 1 : ng) : 2048);
 #define BH_LAUNCH(MASSV, PRECV)                                                               \
   hipLaunchKernelGGL((k_element_mfma<D, NL, NQ, NG, MASSV, PRECV>), grid, dim3(kBlock), 0,    \
-                     A.stream, A.coords, A.c2n, A.tab, (int)A.ncells_total, A.eta, A.eta_scale, \
-                     A.mcoef, A.mass_scale, A.ke)
+                     A.stream, A.coords, A.c2n, A.tab, (int)A.ncells_total, A.cells,            \
+                     A.erow > 0 ? A.erow : NL, A.eta, A.eta_scale, A.mcoef, A.mass_scale, A.ke)
   if (A.mass) {
     if (prec == 1) BH_LAUNCH(true, 1);
     else if (prec == 2) BH_LAUNCH(true, 2);
@@ -296,8 +299,12 @@ struct MfmaEntry {
   {BH_HEXA8, ORDER, NQ1,                                                                      \
    &launch_mfma32<3, ((ORDER) + 1) * ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1) * (NQ1), 8>}
 
+#define BH_MT(ORDER, NPTS) \
+  {BH_TRI3, ORDER, NPTS, &launch_mfma32<2, ((ORDER) + 1) * ((ORDER) + 2) / 2, NPTS, 3>}
+
 static const MfmaEntry kMfma[] = {
-    BH_MQ(1, 2), BH_MQ(2, 2), BH_MQ(2, 3), BH_MQ(3, 3), BH_MQ(3, 4), BH_MQ(4, 5),
+    BH_MQ(1, 2), BH_MQ(1, 3), BH_MQ(2, 2), BH_MQ(2, 3), BH_MQ(3, 3), BH_MQ(3, 4), BH_MQ(4, 5),
+    BH_MT(1, 6), BH_MT(2, 6), BH_MT(2, 7), BH_MT(3, 12), BH_MT(4, 16),
     BH_MH(1, 2), BH_MH(2, 2), BH_MH(2, 3),
 };
 

@@ -164,6 +164,132 @@ def bench_c5(args, torch, B):
     return out
 
 
+def c3_mesh(B, n=866, seed=0, backend=None):
+    """866^2 quads, 16x16 blocks split into triangles with probability 1/3
+    (seed), block orders p ~ {1,2,3,4} with probabilities {0.4,0.3,0.2,0.1}."""
+    m, blocks = B.mixed_tri_quad_mesh(n, block=16, p_split=1.0 / 3.0, seed=seed, backend=backend)
+    nb = int(blocks.max()) + 1
+    border = np.random.default_rng(seed + 1).choice([1, 2, 3, 4], size=nb, p=[0.4, 0.3, 0.2, 0.1])
+    return m, border[blocks]
+
+
+def cpu_baseline_c3(n_sample=400):
+    """Oracle (serial restatement), 1 thread: a bounded sample of the C3 workload,
+    the P2/Q2 space of an n_sample^2 mixed mesh with the same block layout."""
+    import bcubehip as B
+    from oracle import oracle as O
+
+    m, _ = c3_mesh(B, n_sample, backend=B.BcubeBackendHostOnly())
+    o = O.OMesh(2, m.coords, m.celltypes, m.c2n_ptr, m.c2n)
+    t0 = time.perf_counter()
+    r = O.assemble_bilinear(o, 2, ["stiffness", "mass"], degree=4)
+    dt = time.perf_counter() - t0
+    return {"value": r[2].size / dt / 1e9, "unit": "GNNZ/s", "cores": 1, "kind": "port",
+            "sample": f"{n_sample}^2 mixed tri/quad mesh ({m.ncells} cells), P2/Q2, stiffness+mass, "
+                      f"COO + sparse(): {r[2].size} nnz in {dt:.2f} s"}
+
+
+def bench_c3(args, torch, B):
+    """Mixed triangle/quad assembly per order bucket: the cells of all blocks of
+    order p form one mesh (triangles and quads), continuous Lagrange P_p/Q_p,
+    ∫∇u⋅∇v + ∫uv with Measure degree max(2p, 4) (the reference's inline triangle
+    rules).  One step = the assembly of every bucket (one element launch per
+    cell type per bucket + one row gather per bucket) into device CSR values."""
+    n = args.n if args.n != 64 else 866
+    t_setup = time.perf_counter()
+    m, order = c3_mesh(B, n)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    backend = m.backend
+    backend.set_stream(stream.cuda_stream)
+    buckets = {}
+    for p in (1, 2, 3, 4):
+        cells = np.flatnonzero(order == p)
+        if cells.size == 0:
+            continue
+        sub = B.sub_mesh(m, cells)
+        U = B.TrialFESpace(B.FunctionSpace("Lagrange", p), sub)
+        deg = max(2 * p, 4)
+        dΩ = B.Measure(B.CellDomain(sub), deg)
+        form = lambda u, v, dΩ=dΩ: B.integrate(B.dot(B.grad(u), B.grad(v)), dΩ) + B.integrate(u * v, dΩ)
+        plan = B.BilinearPlan(form, U, B.TestFESpace(U))
+        ntri = int(np.sum(sub.celltypes == 2))
+        nq_t = {4: 6, 6: 12, 8: 16}[deg]
+        nq_q = ((deg + 2) // 2) ** 2
+        nl_t, nl_q = (p + 1) * (p + 2) // 2, (p + 1) ** 2
+        f_alg = (ntri * 2 * nl_t * nl_t * 3 * nq_t + (cells.size - ntri) * 2 * nl_q * nl_q * 3 * nq_q)
+        buckets[p] = {"plan": plan, "cells": int(cells.size), "triangles": ntri, "degree": deg,
+                      "f_alg": f_alg, "ndofs": plan.nrows}
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    res = {}
+    for mode in ("fp64", "fp32"):
+        tot_ms, tot_nnz, tot_cells = 0.0, 0, 0
+        per = {}
+        for p, bk in buckets.items():
+            plan = bk["plan"]
+            out = torch.empty(plan.nnz, dtype=torch.float64, device=dev)
+            for _ in range(args.warmup):
+                plan.assemble_device(out.data_ptr(), None, mode)
+            torch.cuda.synchronize()
+            backend.set_timing(True)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(args.steps):
+                plan.assemble_device(out.data_ptr(), None, mode)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / args.steps
+            lt = np.asarray(backend.timings(cap=100000), dtype=np.float64).reshape(args.steps, -1)
+            backend.set_timing(False)
+            el_ms = float(lt[:, :-1].sum(axis=1).mean())
+            ga_ms = float(lt[:, -1].mean())
+            peak = PEAK_TFLOPS[mode]
+            el_ach = bk["f_alg"] / (el_ms * 1e-3) / 1e12
+            ga_ach = 8.0 * plan.nnz / (ga_ms * 1e-3) / 1e9
+            per[f"P{p}"] = {"cells": bk["cells"], "triangles": bk["triangles"], "degree": bk["degree"],
+                           "ndofs": bk["ndofs"], "nnz": plan.nnz, "ms": ms,
+                           "gnnz_per_s": plan.nnz / (ms * 1e-3) / 1e9,
+                           "elements_per_s": bk["cells"] / (ms * 1e-3),
+                           "element_kernels": {"bound": "mfma", "achieved": el_ach, "peak": peak,
+                                               "unit": "TFLOP/s", "frac": el_ach / peak,
+                                               "ms": el_ms, "flops_per_step": bk["f_alg"]},
+                           "gather": {"bound": "hbm", "achieved": ga_ach, "peak": PEAK_HBM_GBS,
+                                      "unit": "GB/s", "frac": ga_ach / PEAK_HBM_GBS, "ms": ga_ms,
+                                      "alg_bytes": 8.0 * plan.nnz}}
+            tot_ms += ms
+            tot_nnz += plan.nnz
+            tot_cells += bk["cells"]
+            del out
+        res[mode] = {"buckets": per, "ms_per_step": tot_ms,
+                     "gnnz_per_s": tot_nnz / (tot_ms * 1e-3) / 1e9,
+                     "elements_per_s": tot_cells / (tot_ms * 1e-3)}
+    r = res["fp32"]
+    dom_p = max(r["buckets"], key=lambda k: r["buckets"][k]["ms"])
+    db = r["buckets"][dom_p]
+    roof = dict(db["gather"] if db["gather"]["ms"] >= db["element_kernels"]["ms"] else db["element_kernels"])
+    roof["kernel"] = f"{'row gather' if roof['bound'] == 'hbm' else 'element kernels'} of bucket {dom_p}"
+    roof["traffic"] = None
+    line = {
+        "metric": "mixed tri/quad assembly (C3), aggregate GNNZ/s over order buckets, FP32-MFMA",
+        "value": r["gnnz_per_s"], "unit": "GNNZ/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"C3: {n}^2 quads, 16x16 blocks split into triangles w.p. 1/3 "
+                               f"(seed 0), block orders p~{{1:.4,2:.3,3:.2,4:.1}} (seed 1), "
+                               f"per-bucket continuous P_p/Q_p, ∫∇u⋅∇v+∫uv, degree max(2p,4)",
+                   "cells": int(m.ncells), "triangles": int(np.sum(m.celltypes == 2)),
+                   "setup_s": round(t_setup, 2)},
+        "roofline": roof,
+        "fp32_speedup_vs_fp64": res["fp64"]["ms_per_step"] / r["ms_per_step"],
+        "modes": res,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_c3()
+    backend.set_stream(None)
+    return line
+
+
 def cpu_baseline_c4(n_sample=26):
     """Oracle DG transport residual (serial C++ restatement), 1 thread, on a
     bounded sample of the C4 workload: n_sample^3 curved Hexa27 cells, Q3, 8 fields."""
@@ -306,9 +432,9 @@ def main():
     ap.add_argument("--mode", default="all", choices=["all", "fp64", "fp32", "bf16x3"])
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c2", "c4", "c5"],
-                    help="c2: the headline 64^3 Q2 hexa workload; c4: the DG transport "
-                         "residual; c5: the many-instance batch")
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2: the headline 64^3 Q2 hexa workload; c3: mixed tri/quad order "
+                         "buckets; c4: the DG transport residual; c5: the many-instance batch")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters against a known byte count)")
@@ -327,6 +453,9 @@ def main():
         return
     if args.config == "c4":
         print(json.dumps(bench_c4(args, torch, B)), flush=True)
+        return
+    if args.config == "c3":
+        print(json.dumps(bench_c3(args, torch, B)), flush=True)
         return
     n = args.n
     t_setup = time.perf_counter()

@@ -102,6 +102,7 @@ struct bh_dg_s {
   DGTables tab;
   void* d_vfac = nullptr;
   void* d_ffac = nullptr;
+  void* d_sfac = nullptr;
   int32_t* d_cs = nullptr;
   int32_t* d_fcell = nullptr;
   int8_t* d_fside = nullptr;
@@ -810,6 +811,29 @@ bool make_dg_tables(int N, DGTables& t) {
   // A now holds [I | Φ⁻¹] with Φ[q][i] rows q: (Φ⁻¹)[i][q] = A[i][N + q]
   for (int i = 0; i < N; ++i)
     for (int q = 0; q < N; ++q) t.pinv[i * 4 + q] = A[i][N + q];
+  // Lagrange basis on the Gauss points: ℓ_k(x) = Π_{m≠k} (x - g_m) / (g_k - g_m)
+  auto ell = [&](int k, double x) {
+    double v = 1.0;
+    for (int m = 0; m < N; ++m)
+      if (m != k) v *= (x - gx[m]) / (gx[k] - gx[m]);
+    return v;
+  };
+  for (int k = 0; k < N; ++k) {
+    t.em[k] = ell(k, -1.0);
+    t.ep[k] = ell(k, 1.0);
+    for (int q = 0; q < N; ++q) {
+      double d = 0.0;
+      if (q == k) {
+        for (int m = 0; m < N; ++m)
+          if (m != k) d += 1.0 / (gx[k] - gx[m]);
+      } else {
+        d = 1.0 / (gx[k] - gx[q]);
+        for (int m = 0; m < N; ++m)
+          if (m != k && m != q) d *= (gx[q] - gx[m]) / (gx[k] - gx[m]);
+      }
+      t.dg[q * 4 + k] = d;
+    }
+  }
   return true;
 }
 
@@ -830,6 +854,8 @@ int dg_geometry(bh_dg g) {
   a.fp32 = g->mode == BH_FP32;
   a.vfac = g->d_vfac;
   a.ffac = g->d_ffac;
+  a.cs = g->d_cs;
+  a.sfac = g->d_sfac;
   BH_HIP(c, launch_dg_geometry(a, g->tab));
   return BH_OK;
 }
@@ -839,6 +865,7 @@ void dg_free(bh_dg g) {
     (void)hipSetDevice(g->dev);
     (void)hipFree(g->d_vfac);
     (void)hipFree(g->d_ffac);
+    (void)hipFree(g->d_sfac);
     (void)hipFree(g->d_cs);
     (void)hipFree(g->d_fcell);
     (void)hipFree(g->d_fside);
@@ -898,6 +925,7 @@ int bh_dg_create(bh_space V, int quad_degree, const double* velocity, int mode, 
   hipError_t e = hipSuccess;
   if (e == hipSuccess) e = hipMalloc(&g->d_vfac, std::max<size_t>(1, (size_t)nc * 4 * nl * es));
   if (e == hipSuccess) e = hipMalloc(&g->d_ffac, std::max<size_t>(1, (size_t)F.nfaces * N * N * es));
+  if (e == hipSuccess) e = hipMalloc(&g->d_sfac, std::max<size_t>(1, (size_t)nc * 6 * N * N * es));
   if (e == hipSuccess) e = dalloc(&g->d_cs, F.cs.size());
   if (e == hipSuccess) e = dalloc(&g->d_fcell, F.cell_n.size());
   if (e == hipSuccess) e = dalloc(&g->d_fside, F.side_n.size());
@@ -952,6 +980,7 @@ int bh_dg_transport_residual(bh_dg g, int32_t nfields, const void* u, void* r, i
   a.nfields = nfields;
   a.vfac = g->d_vfac;
   a.ffac = g->d_ffac;
+  a.sfac = g->d_sfac;
   a.cs = g->d_cs;
   a.u = u;
   a.r = r;

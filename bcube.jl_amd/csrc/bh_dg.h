@@ -8,12 +8,14 @@
 namespace bh {
 
 // 1D operators of the tensor basis on N uniform Lagrange nodes at the N Gauss
-// points of one axis, entry [q * 4 + i] (row = Gauss point q, column = node i):
-//   phi  = φ_i(g_q),  dphi = φ_i'(g_q),  pinv = (Φ^-1)[q][i]  (Φ = phi as N x N).
-// Gauss rule: gx/gw (N points).
+// points of one axis: phi[q*4+i] = φ_i(g_q), dphi[q*4+i] = φ_i'(g_q), and
+// pinv[i*4+q] = (Φ⁻¹)[i][q] for Φ = phi as an N x N matrix.  Gauss rule: gx/gw.
 struct DGTables {
   double phi[16], dphi[16], pinv[16];
   double gx[4], gw[4];
+  // Lagrange basis ℓ_k on the Gauss points themselves: dg[q*4+k] = ℓ_k'(g_q),
+  // em[k] = ℓ_k(-1), ep[k] = ℓ_k(+1)
+  double dg[16], em[4], ep[4];
 };
 
 struct DGGeomArgs {
@@ -29,7 +31,9 @@ struct DGGeomArgs {
   double c[3] = {0, 0, 0};
   bool fp32 = false;
   void* vfac = nullptr;  // [ncells][4][n^3]: w|J| (J^-1 c) per axis, then w|J|
-  void* ffac = nullptr;  // [nfaces][n^2]:  w |∂x/∂s x ∂x/∂t| (c . n)
+  void* ffac = nullptr;  // [nfaces][n^2]:  w |∂x/∂s x ∂x/∂t| (c . n⁻), canonical face points
+  const int32_t* cs = nullptr;  // cell-side records (see DGArgs)
+  void* sfac = nullptr;  // [ncells][6][n^2]: -w |∂x/∂s x ∂x/∂t| (c . n_out), in each side's frame
 };
 
 struct DGArgs {
@@ -40,6 +44,7 @@ struct DGArgs {
   int nfields = 0;
   const void* vfac = nullptr;
   const void* ffac = nullptr;
+  const void* sfac = nullptr;
   const int32_t* cs = nullptr;  // ncells * 6 * 4 cell-side records
   // residual: r = l(v) of u
   const void* u = nullptr;

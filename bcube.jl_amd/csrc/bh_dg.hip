@@ -168,6 +168,33 @@ __global__ __launch_bounds__(kBlock) void k_dg_facefac(
   ffac[f * N2 + q] = (T)(w * metric * cn);
 }
 
+// Per cell side, in the side's own frame: F_side = -w|∂x/∂s × ∂x/∂t|(c⋅n_out),
+// i.e. the face factor F = w|..|(c⋅n⁻) of side n negated, and F itself on side
+// p.  The upwind face term of the side is then G = (F_side < 0 ? u_own :
+// u_neighbour) * F_side at every point (0 on boundary sides).
+template <int N, typename T>
+__global__ __launch_bounds__(kBlock) void k_dg_sidefac(const int4* __restrict__ cs,
+                                                       const T* __restrict__ ffac, int64_t ncells,
+                                                       T* __restrict__ sfac) {
+  constexpr int N2 = N * N;
+  const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (gid >= ncells * 6 * N2) return;
+  const int64_t cside = gid / N2;
+  const int p = (int)(gid % N2);
+  const int4 rec = cs[cside];
+  T v = 0;
+  if (rec.x >= 0) {
+    const int role = rec.z & 1, m = (rec.z >> 1) & 7;
+    // this frame (ka, kb) -> canonical face point
+    const int ka = p % N, kb = p / N;
+    const int a = (m & 2) ? N - 1 - ka : ka, b = (m & 4) ? N - 1 - kb : kb;
+    const int qs = (m & 1) ? b : a, qt = (m & 1) ? a : b;
+    const T F = ffac[(int64_t)rec.x * N2 + qs + N * qt];
+    v = role ? F : -F;
+  }
+  sfac[gid] = v;
+}
+
 // 1D operators in the residual's arithmetic type (kernel argument, read
 // through the scalar path for compile-time indices).
 template <typename T>
@@ -429,6 +456,315 @@ __global__ __launch_bounds__(kBlock) void k_dg_residual(
   }
 }
 
+// Conditional reversal along the first / second face axis and transposition of
+// an N x N face array a[ia + N*ib] (v_cndmask pairs; c is per lane).
+template <typename T, int N>
+__device__ __forceinline__ void cflip_a(T (&x)[N * N], bool c) {
+#pragma unroll
+  for (int b = 0; b < N; ++b)
+#pragma unroll
+    for (int a = 0; a < N / 2; ++a) {
+      const T p = x[a + N * b], q = x[N - 1 - a + N * b];
+      x[a + N * b] = c ? q : p;
+      x[N - 1 - a + N * b] = c ? p : q;
+    }
+}
+template <typename T, int N>
+__device__ __forceinline__ void cflip_b(T (&x)[N * N], bool c) {
+#pragma unroll
+  for (int b = 0; b < N / 2; ++b)
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+      const T p = x[a + N * b], q = x[a + N * (N - 1 - b)];
+      x[a + N * b] = c ? q : p;
+      x[a + N * (N - 1 - b)] = c ? p : q;
+    }
+}
+template <typename T, int N>
+__device__ __forceinline__ void ctranspose(T (&x)[N * N], bool c) {
+#pragma unroll
+  for (int b = 0; b < N; ++b)
+#pragma unroll
+    for (int a = 0; a < b; ++a) {
+      const T p = x[a + N * b], q = x[b + N * a];
+      x[a + N * b] = c ? q : p;
+      x[b + N * a] = c ? p : q;
+    }
+}
+// canonical face-point array c[qs + N*qt] -> this side's frame (code m):
+// out[ka + N*kb] = c at the point face_point(m) sends to (ka, kb)
+template <typename T, int N>
+__device__ __forceinline__ void canon_to_frame(T (&x)[N * N], int m) {
+  ctranspose<T, N>(x, m & 1);
+  cflip_a<T, N>(x, m & 2);
+  cflip_b<T, N>(x, m & 4);
+}
+// frame array (code m) -> canonical face-point array
+template <typename T, int N>
+__device__ __forceinline__ void frame_to_canon(T (&x)[N * N], int m) {
+  cflip_a<T, N>(x, m & 2);
+  cflip_b<T, N>(x, m & 4);
+  ctranspose<T, N>(x, m & 1);
+}
+
+// Neighbour-frame array (code nb) -> this side's frame (code my): the
+// composition frame_to_canon(nb) then canon_to_frame(my), reduced to one
+// conditional transposition and two conditional reversals.
+template <typename T, int N>
+__device__ __forceinline__ void nb_to_frame(T (&x)[N * N], int nb, int my) {
+  const bool s = ((nb ^ my) & 1) != 0;
+  const bool fan = nb & 2, fbn = nb & 4;
+  ctranspose<T, N>(x, s);
+  cflip_a<T, N>(x, (s ? fbn : fan) != ((my & 2) != 0));
+  cflip_b<T, N>(x, (s ? fan : fbn) != ((my & 4) != 0));
+}
+
+// The N x N dofs of side `s` of a cell block, in the side's own frame
+// (ia along the first in-face axis), with 4-wide loads where rows are
+// contiguous.
+template <typename T, int N>
+__device__ __forceinline__ void load_side(const T* __restrict__ blk, int s, T (&x)[N * N]) {
+  constexpr int N2 = N * N;
+  if constexpr (N == 4) {
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    if (s == 0 || s == 5 || s == 1 || s == 3) {
+      const int o0 = s == 5 ? N2 * (N - 1) : (s == 3 ? N * (N - 1) : 0);
+      const int rs = (s == 0 || s == 5) ? N : N2;  // stride between rows of the face
+#pragma unroll
+      for (int b = 0; b < N; ++b) {
+        const T4 v = *(const T4*)(blk + o0 + rs * b);
+        x[0 + N * b] = v.x;
+        x[1 + N * b] = v.y;
+        x[2 + N * b] = v.z;
+        x[3 + N * b] = v.w;
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < N; ++b)
+#pragma unroll
+    for (int a = 0; a < N; ++a) x[a + N * b] = blk[face_dof<N>(s, a, b)];
+}
+
+// ---------------------------------------------------------------------------
+// Register form: one lane per (cell, field).  The cell's N³ values live in
+// registers for the whole residual, so there is no LDS traffic and no
+// synchronisation:
+//   uq  = (Φ ⊗ Φ ⊗ Φ) u               values at the N³ Gauss points
+//   t   = Σ_a Dgᵀ_a (B_a ∘ uq)         volume term in the Gauss-point Lagrange
+//                                      basis ℓ (exact: N points represent
+//                                      degree N-1 exactly)
+//   per side: trace of uq on the side (extrapolation by ℓ(±1) along the normal
+//   axis), neighbour trace from its N x N face dofs (loaded already permuted
+//   into this cell's side frame), upwind flux G, lift t += ℓ(±1) ⊗ G
+//   r   = (Φᵀ ⊗ Φᵀ ⊗ Φᵀ) t
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Ops2 {
+  T phi[16], dg[16], em[4], ep[4];
+};
+
+template <int N>
+__device__ __forceinline__ constexpr int side_axis(int side) {
+  return side == 2 || side == 4 ? 0 : (side == 1 || side == 3 ? 1 : 2);
+}
+
+// Blocks of the wave's 64 (cell, field) slots, staged through LDS so that the
+// global loads and stores are contiguous runs (16 lanes per block) instead of
+// one scattered block per lane.
+template <int N3>
+struct SlotLds {
+  static constexpr int stride = N3 + 4;  // 16-byte aligned rows, no bank aliasing within 16 lanes
+};
+
+template <typename T, int N, bool STORE>
+__device__ __forceinline__ void wave_blocks(T* __restrict__ g, T* W, int64_t g0, int64_t total,
+                                            int nfields, int64_t ndofs, int lane) {
+  constexpr int N3 = N * N * N, LST = SlotLds<N3>::stride;
+  if constexpr (N3 % 4 == 0) {
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    constexpr int Q = N3 / 4;            // 4-wide chunks per block
+    constexpr int SPI = 64 / Q;          // blocks per instruction
+#pragma unroll
+    for (int i = 0; i < 64 / SPI; ++i) {
+      const int sl = i * SPI + lane / Q, k = lane % Q;
+      const int64_t gs = g0 + sl;
+      if (gs < total) {
+        T* gp = g + (int64_t)(gs % nfields) * ndofs + (gs / nfields) * N3 + 4 * k;
+        T* lp = W + sl * LST + 4 * k;
+        if constexpr (STORE) *(T4*)gp = *(const T4*)lp;
+        else *(T4*)lp = *(const T4*)gp;
+      }
+    }
+  } else {
+    for (int i = lane; i < 64 * N3; i += 64) {
+      const int sl = i / N3, k = i % N3;
+      const int64_t gs = g0 + sl;
+      if (gs < total) {
+        T* gp = g + (int64_t)(gs % nfields) * ndofs + (gs / nfields) * N3 + k;
+        if constexpr (STORE) *gp = W[sl * LST + k];
+        else W[sl * LST + k] = *gp;
+      }
+    }
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(kBlock) void k_dg_residual_reg(
+    const T* __restrict__ u, T* __restrict__ r, int64_t ndofs, int nfields, int64_t ncells,
+    const T* __restrict__ vfac, const T* __restrict__ sfac, const int4* __restrict__ cs,
+    Ops2<T> op) {
+  constexpr int N2 = N * N, N3 = N * N * N, LST = SlotLds<N3>::stride;
+  __shared__ T lds[kBlock / 64][64 * LST];
+  const int lane = threadIdx.x & 63;
+  T* W = lds[threadIdx.x >> 6];
+  const int64_t g0 = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63);
+  const int64_t total = ncells * nfields;
+  wave_blocks<T, N, false>(const_cast<T*>(u), W, g0, total, nfields, ndofs, lane);
+  wave_sync();
+  const int64_t g = g0 + lane;
+  const bool valid = g < total;
+  const int64_t cell = valid ? g / nfields : 0;
+  const int f = valid ? (int)(g % nfields) : 0;
+  const T* uf = u + (int64_t)f * ndofs;
+  T uq[N3], t[N3];
+#pragma unroll
+  for (int i = 0; i < N3; ++i) uq[i] = W[lane * LST + i];
+  // interpolate to the Gauss points, axis by axis, in place
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const int st = ax == 0 ? 1 : (ax == 1 ? N : N2);
+#pragma unroll
+    for (int p = 0; p < N2; ++p) {
+      const int base = ax == 0 ? p * N : (ax == 1 ? (p % N) + N2 * (p / N) : p);
+      T v[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] = uq[base + st * i];
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        T s = 0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) s = madd(op.phi[q * 4 + i], v[i], s);
+        uq[base + st * q] = s;
+      }
+    }
+  }
+  // volume: t = Σ_a Dgᵀ along a of (B_a ∘ uq)
+#pragma unroll
+  for (int i = 0; i < N3; ++i) t[i] = 0;
+  const T* vf = vfac + cell * 4 * N3;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const int st = ax == 0 ? 1 : (ax == 1 ? N : N2);
+#pragma unroll
+    for (int p = 0; p < N2; ++p) {
+      const int base = ax == 0 ? p * N : (ax == 1 ? (p % N) + N2 * (p / N) : p);
+      T s[N];
+#pragma unroll
+      for (int q = 0; q < N; ++q) s[q] = vf[ax * N3 + base + st * q] * uq[base + st * q];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        T a = t[base + st * k];
+#pragma unroll
+        for (int q = 0; q < N; ++q) a = madd(op.dg[q * 4 + k], s[q], a);
+        t[base + st * k] = a;
+      }
+    }
+  }
+  // interior faces (boundary sides have zero factors: they add nothing)
+  const int4* rec_c = cs + cell * 6;
+#pragma unroll
+  for (int side = 0; side < 6; ++side) {
+    const int4 rec = rec_c[side];
+    if (!valid || rec.x < 0) continue;
+    const int mycode = (rec.z >> 1) & 7;
+    const int nbside = (rec.z >> 4) & 7, nbcode = (rec.z >> 7) & 7;
+    const int d = side_axis<N>(side);
+    const int fa = d == 0 ? 1 : 0, fb = d == 2 ? 1 : 2;
+    const int sd = d == 0 ? 1 : (d == 1 ? N : N2);
+    const int sa = fa == 0 ? 1 : N;
+    const int sb = fb == 1 ? N : N2;
+    const bool plus = side == 2 || side == 3 || side == 5;
+    // neighbour's face dofs (loaded in its frame, then brought to this one:
+    // the dof lattice and the Gauss points have the same square symmetries)
+    // and the side's factors (already in this frame)
+    T nbv[N2], Fq[N2];
+    // a neighbour staged by this wave is read from LDS, others from memory
+    // (one generic pointer, one code path)
+    const int64_t gnb = (int64_t)rec.y * nfields + f - g0;
+    const T* src = (gnb >= 0 && gnb < 64) ? (const T*)(W + gnb * LST) : uf + (int64_t)rec.y * N3;
+    load_side<T, N>(src, nbside, nbv);
+    nb_to_frame<T, N>(nbv, nbcode, mycode);
+    {
+      const T* Fs = sfac + (cell * 6 + side) * N2;
+#pragma unroll
+      for (int q = 0; q < N2; ++q) Fq[q] = Fs[q];
+    }
+    // neighbour trace: Φ along a, then along b
+    T tn[N2];
+#pragma unroll
+    for (int kb = 0; kb < N; ++kb)
+#pragma unroll
+      for (int ka = 0; ka < N; ++ka) {
+        T s = 0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) s = madd(op.phi[ka * 4 + i], nbv[i + N * kb], s);
+        tn[ka + N * kb] = s;
+      }
+#pragma unroll
+    for (int ka = 0; ka < N; ++ka) {
+      T w[N];
+#pragma unroll
+      for (int kb = 0; kb < N; ++kb) w[kb] = tn[ka + N * kb];
+#pragma unroll
+      for (int kb = 0; kb < N; ++kb) {
+        T s = 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) s = madd(op.phi[kb * 4 + j], w[j], s);
+        tn[ka + N * kb] = s;
+      }
+    }
+    // own trace, upwind face term G = (F_side < 0 ? own : neighbour) * F_side, lift
+#pragma unroll
+    for (int kb = 0; kb < N; ++kb)
+#pragma unroll
+      for (int ka = 0; ka < N; ++ka) {
+        const int p0 = sa * ka + sb * kb;
+        T tm = 0;
+#pragma unroll
+        for (int k = 0; k < N; ++k) tm = madd(plus ? op.ep[k] : op.em[k], uq[p0 + sd * k], tm);
+        const T fq = Fq[ka + N * kb];
+        const T gq = fq * (fq < 0 ? tm : tn[ka + N * kb]);
+#pragma unroll
+        for (int k = 0; k < N; ++k) t[p0 + sd * k] = madd(plus ? op.ep[k] : op.em[k], gq, t[p0 + sd * k]);
+      }
+  }
+  // r = (Φᵀ ⊗ Φᵀ ⊗ Φᵀ) t
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const int st = ax == 0 ? 1 : (ax == 1 ? N : N2);
+#pragma unroll
+    for (int p = 0; p < N2; ++p) {
+      const int base = ax == 0 ? p * N : (ax == 1 ? (p % N) + N2 * (p / N) : p);
+      T v[N];
+#pragma unroll
+      for (int q = 0; q < N; ++q) v[q] = t[base + st * q];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        T s = 0;
+#pragma unroll
+        for (int q = 0; q < N; ++q) s = madd(op.phi[q * 4 + i], v[q], s);
+        t[base + st * i] = s;
+      }
+    }
+  }
+  if (!valid) return;
+  T* rc = r + (int64_t)f * ndofs + cell * N3;
+#pragma unroll
+  for (int i = 0; i < N3; ++i) rc[i] = t[i];
+}
+
 // out1 = x1 + a1 M⁻¹ r, out2 = x2 + a2 M⁻¹ r for the block-diagonal DG mass
 // matrix M_K = Φ₃ᵀ diag(W_q) Φ₃ of the same Gauss rule (N points, N nodes per
 // axis): M_K⁻¹ = Φ₃⁻¹ diag(1/W_q) Φ₃⁻ᵀ with Φ₃⁻¹ = Φ⁻¹ ⊗ Φ⁻¹ ⊗ Φ⁻¹.
@@ -539,6 +875,25 @@ Ops<T> ops_of(const DGTables& t) {
   return o;
 }
 
+// Which residual kernel a (type, order) uses: the register form where its
+// per-lane arrays fit the register file without spilling.
+template <typename T, int N>
+constexpr bool kResidualInRegisters = sizeof(T) == 4 || N <= 3;
+
+template <typename T>
+Ops2<T> ops2_of(const DGTables& t) {
+  Ops2<T> o;
+  for (int i = 0; i < 16; ++i) {
+    o.phi[i] = (T)t.phi[i];
+    o.dg[i] = (T)t.dg[i];
+  }
+  for (int i = 0; i < 4; ++i) {
+    o.em[i] = (T)t.em[i];
+    o.ep[i] = (T)t.ep[i];
+  }
+  return o;
+}
+
 unsigned blocks_for(int64_t n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
 
 template <int GEO, int N, typename T>
@@ -552,6 +907,10 @@ hipError_t geometry_impl(const DGGeomArgs& a, const DGTables& t) {
     hipLaunchKernelGGL((k_dg_facefac<GEO, N, T>), dim3(blocks_for(a.nfaces * N2, kBlock)),
                        dim3(kBlock), 0, a.stream, a.coords, a.c2n, a.face_cell, a.face_side,
                        a.nfaces, t, a.c[0], a.c[1], a.c[2], (T*)a.ffac);
+  if (a.ncells > 0)
+    hipLaunchKernelGGL((k_dg_sidefac<N, T>), dim3(blocks_for(a.ncells * 6 * N2, kBlock)),
+                       dim3(kBlock), 0, a.stream, (const int4*)a.cs, (const T*)a.ffac, a.ncells,
+                       (T*)a.sfac);
   return hipGetLastError();
 }
 
@@ -574,9 +933,15 @@ template <typename T, int N>
 hipError_t residual_impl(const DGArgs& a, const DGTables& t) {
   const int64_t nslots = a.ncells * a.nfields;
   if (nslots == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_dg_residual<T, N>), dim3(blocks_for(nslots, kSlots)), dim3(kBlock), 0,
-                     a.stream, (const T*)a.u, (T*)a.r, a.ndofs, a.nfields, a.ncells,
-                     (const T*)a.vfac, (const T*)a.ffac, (const int4*)a.cs, ops_of<T>(t));
+  if constexpr (kResidualInRegisters<T, N>) {
+    hipLaunchKernelGGL((k_dg_residual_reg<T, N>), dim3(blocks_for(nslots, kBlock)), dim3(kBlock),
+                       0, a.stream, (const T*)a.u, (T*)a.r, a.ndofs, a.nfields, a.ncells,
+                       (const T*)a.vfac, (const T*)a.sfac, (const int4*)a.cs, ops2_of<T>(t));
+  } else {
+    hipLaunchKernelGGL((k_dg_residual<T, N>), dim3(blocks_for(nslots, kSlots)), dim3(kBlock), 0,
+                       a.stream, (const T*)a.u, (T*)a.r, a.ndofs, a.nfields, a.ncells,
+                       (const T*)a.vfac, (const T*)a.ffac, (const int4*)a.cs, ops_of<T>(t));
+  }
   return hipGetLastError();
 }
 

@@ -44,11 +44,18 @@ class BcubeBackendHIP(AbstractBcubeBackend):
         L.check(L.lib().bh_ctx_set_timing(self.ctx, int(enable)), self.ctx, "bh_ctx_set_timing")
 
     def timings(self, cap: int = 4096) -> np.ndarray:
+        return self.timings_ex(cap)[0]
+
+    def timings_ex(self, cap: int = 4096):
+        """(ms, kinds) of the launches timed since the last reset; kind 0 =
+        element kernel, 1 = row gather, 2 = other."""
         buf = np.zeros(cap, dtype=np.float32)
+        kinds = np.zeros(cap, dtype=np.int32)
         n = C.c_int()
-        L.check(L.lib().bh_ctx_timings(self.ctx, buf.ctypes.data, cap, C.byref(n)), self.ctx,
-                "bh_ctx_timings")
-        return buf[: min(n.value, cap)].copy()
+        L.check(L.lib().bh_ctx_timings_ex(self.ctx, buf.ctypes.data, kinds.ctypes.data, cap,
+                                          C.byref(n)), self.ctx, "bh_ctx_timings_ex")
+        k = min(n.value, cap)
+        return buf[:k].copy(), kinds[:k].copy()
 
     def capture(self) -> "AssemblyGraph":
         """Capture device-side assembly calls into a HIP graph:

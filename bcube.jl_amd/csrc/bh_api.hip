@@ -24,6 +24,7 @@ struct bh_ctx_s {
   std::string err;
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  std::vector<int32_t> kinds;  // per timed launch: 0 element, 1 row gather, 2 other
   size_t nevents = 0;
   std::map<std::tuple<int, int, int>, double*> tables;  // (geo, order, nq1) -> device table
   bool capturing = false;
@@ -141,14 +142,16 @@ int set_device(bh_ctx c) {
   return BH_OK;
 }
 
-void time_begin(bh_ctx c) {
+void time_begin(bh_ctx c, int kind = 2) {
   if (!c->timing || c->capturing) return;
   if (c->nevents == c->events.size()) {
     hipEvent_t a, b;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     c->events.push_back({a, b});
+    c->kinds.push_back(2);
   }
+  c->kinds[c->nevents] = kind;
   (void)hipEventRecord(c->events[c->nevents].first, c->stream);
 }
 void time_end(bh_ctx c) {
@@ -273,18 +276,23 @@ int bh_ctx_set_timing(bh_ctx c, int enable) {
   return BH_OK;
 }
 
-int bh_ctx_timings(bh_ctx c, float* ms, int cap, int* count) {
+int bh_ctx_timings_ex(bh_ctx c, float* ms, int32_t* kinds, int cap, int* count) {
   if (!c || !count) return BH_ERR_ARG;
   BH_HIP(c, hipStreamSynchronize(c->stream));
   int n = (int)std::min<size_t>(c->nevents, (size_t)std::max(cap, 0));
   for (int i = 0; i < n; ++i) {
-    float t = 0;
+    float t = 0.f;
     BH_HIP(c, hipEventElapsedTime(&t, c->events[i].first, c->events[i].second));
     if (ms) ms[i] = t;
+    if (kinds) kinds[i] = c->kinds[i];
   }
   *count = (int)c->nevents;
   c->nevents = 0;
   return BH_OK;
+}
+
+int bh_ctx_timings(bh_ctx c, float* ms, int cap, int* count) {
+  return bh_ctx_timings_ex(c, ms, nullptr, cap, count);
 }
 
 // ---- mesh ------------------------------------------------------------------
@@ -659,7 +667,7 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
     A.cells = gb.d_cells;
     A.tab = tab;
     A.ncells_total = gb.n;
-    time_begin(c);
+    time_begin(c, 0);
     if (mfma_supported(gb.geo, V->order, key)) BH_HIP(c, launch_mfma(gb.geo, V->order, key, A));
     else BH_HIP(c, launch(gb.geo, V->order, key, A));
     time_end(c);
@@ -676,7 +684,7 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
   A.nrows = h.nrows;
   A.max_row = h.max_row;
   A.vals = (where == BH_DEVICE && nzval) ? nzval : p->d_vals;
-  time_begin(c);
+  time_begin(c, 1);
   BH_HIP(c, launch_gather(V->nl, A));
   time_end(c);
   if (where == BH_HOST) {

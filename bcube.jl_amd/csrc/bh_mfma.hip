@@ -72,7 +72,12 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
   __shared__ double s_dg[NQ * NG * D];
   __shared__ double s_w[NQ];
   __shared__ double s_x[CPB][NG][D];
-  __shared__ RT s_g[CPB][NQ][8];  // G (NS values), m, zeros
+  constexpr int G_C = PREC == 0 ? 1 : CPB, G_Q = PREC == 0 ? 1 : NQ;
+  __shared__ RT s_g[G_C][G_Q][8];  // G (NS values), m, zeros (BF16x3, FP64)
+  // FP32: the same factors expanded per k-row, row ak = the coefficients that
+  // multiply (∂φ̂, φ̂) for B row ak, so one 16-byte read forms a B element
+  constexpr int GE_C = PREC == 0 ? CPB : 1, GE_Q = PREC == 0 ? NQ : 1;
+  __shared__ RT s_ge[GE_C][GE_Q][4][4];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -142,9 +147,12 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
     // FP64 geometry -> FP32 per-point factors
     for (int k = tid; k < CPB * NQ; k += kBlock) {
       int c = k / NQ, q = k - c * NQ;
-      RT* gq = s_g[c][q];
+      RT gl[8];
+      RT* gq = PREC == 0 ? gl : s_g[PREC == 0 ? 0 : c][PREC == 0 ? 0 : q];
       if (base + c >= ncells) {
         for (int t = 0; t < 8; ++t) gq[t] = (RT)0;
+        if constexpr (PREC == 0)
+          for (int t = 0; t < 16; ++t) (&s_ge[c][q][0][0])[t] = (RT)0;
         continue;
       }
       const int cell = cells ? cells[base + c] : base + c;
@@ -179,6 +187,17 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
       double mc = mcoef ? mcoef[cell] * mass_scale : mass_scale;
       gq[NS] = MASS ? (RT)(wdet * mc) : (RT)0;
       for (int t = NS + 1; t < 8; ++t) gq[t] = (RT)0;
+      if constexpr (PREC == 0) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            RT v = (RT)0;
+            if (a < D && b < D) v = gq[a <= b ? a * D - a * (a - 1) / 2 + (b - a) : b * D - b * (b - 1) / 2 + (a - b)];
+            else if (a == D && b == D) v = gq[NS];
+            s_ge[c][q][a][b] = v;
+          }
+      }
     }
     __syncthreads();
     // matrix-core contraction, one 16-column tile at a time per wave
@@ -194,9 +213,15 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
       if constexpr (PREC != 1) {
 #pragma unroll
         for (int ks = 0; ks < NQ; ++ks) {
-          const RT* g = s_g[cc][ks];
           const RT* r = s_r[ks][jj];
-          RT bval = g[sel[0]] * r[0] + g[sel[1]] * r[1] + g[sel[2]] * r[2] + g[sel[3]] * r[3];
+          RT bval;
+          if constexpr (PREC == 0) {
+            const RT* ge = s_ge[cc][ks][ak];
+            bval = ge[0] * r[0] + ge[1] * r[1] + ge[2] * r[2] + ge[3] * r[3];
+          } else {
+            const RT* g = s_g[PREC == 0 ? 0 : cc][PREC == 0 ? 0 : ks];
+            bval = g[sel[0]] * r[0] + g[sel[1]] * r[1] + g[sel[2]] * r[2] + g[sel[3]] * r[3];
+          }
           bval = valid ? bval : (RT)0;
           static_assert(A_IN_LDS, "A operand from LDS");
           const RT a0 = (am < NL) ? s_r[ks][am < NL ? am : 0][ak] : (RT)0;
@@ -218,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
             const int q = 8 * st + 2 * ak + h;
             float bv[4] = {0.f, 0.f, 0.f, 0.f};
             if (valid && q < NQ) {
-              const RT* g = s_g[cc][q];
+              const RT* g = s_g[PREC == 0 ? 0 : cc][PREC == 0 ? 0 : q];
               const RT* r = s_r[q][jj];
               // row a of G·r for every component a, from the symmetric storage
               if constexpr (D == 3) {

@@ -91,6 +91,9 @@ int bh_sync(bh_ctx ctx);
  * durations (ms) of the launches since the last reset and returns the count. */
 int bh_ctx_set_timing(bh_ctx ctx, int enable);
 int bh_ctx_timings(bh_ctx ctx, float* ms, int cap, int* count);
+/* Same, with the kind of each timed launch: 0 element kernel, 1 row gather,
+ * 2 other (DG, linear forms, quadrature points). */
+int bh_ctx_timings_ex(bh_ctx ctx, float* ms, int32_t* kinds, int cap, int* count);
 
 /* ---- mesh --------------------------------------------------------------- */
 /* Mesh(nodes, celltypes, Connectivity) (src/mesh/mesh.jl:221-262).

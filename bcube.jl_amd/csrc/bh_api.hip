@@ -27,6 +27,7 @@ struct bh_ctx_s {
   std::vector<int32_t> kinds;  // per timed launch: 0 element, 1 row gather, 2 other
   size_t nevents = 0;
   std::map<std::tuple<int, int, int>, double*> tables;  // (geo, order, nq1) -> device table
+  std::map<std::tuple<int, int, int, int>, float*> gp_tables;  // (geo, order, nq1, mass) -> P
   bool capturing = false;
 };
 
@@ -88,7 +89,7 @@ struct bh_pattern_s {
   int32_t* d_pptr = nullptr;
   int32_t* d_pairs = nullptr;
   void* d_goff = nullptr;
-  void* d_ke = nullptr;      // element-row workspace, ncells*nl*KS FP64 (FP32 uses half)
+  void* d_ke = nullptr;      // element-row workspace, 16*ceil(ncells/16)*nl*KS FP64 (FP32 uses half)
   double* d_vals = nullptr;
   double* d_coef = nullptr;  // staged host coefficients (2 * ncells)
 };
@@ -203,6 +204,52 @@ int get_tables(bh_ctx c, int geo, int order, int quad_degree, double** out) {
   return BH_OK;
 }
 
+// Constant operand of the GEMM element kernel (bh_gp.hip): the reference
+// products P[k][p], k = q*NC + f over the points and the factors (the
+// D(D+1)/2 entries of G in row-major upper order, then the mass factor),
+// p over the pairs i <= j in row-major upper order.  Diagonal G entries
+// multiply ∂_aφ̂_i ∂_aφ̂_j, off-diagonal ones ∂_aφ̂_i ∂_bφ̂_j + ∂_bφ̂_i ∂_aφ̂_j, the
+// mass factor φ̂_i φ̂_j.  Products in FP64, rounded once to FP32.
+int get_gp_table(bh_ctx c, int geo, int order, int quad_degree, bool mass, float** out) {
+  const int nq1 = rule_key(geo, quad_degree);
+  auto key = std::make_tuple(geo, order, nq1, mass ? 1 : 0);
+  auto it = c->gp_tables.find(key);
+  if (it != c->gp_tables.end()) { *out = it->second; return BH_OK; }
+  if (c->capturing)
+    return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
+  Tables T = make_tables(shape_of(geo), order, quad_degree, geo);
+  const int d = T.dim, nl = T.nl, nq = T.nq;
+  const int ns = d * (d + 1) / 2, nc = ns + (mass ? 1 : 0);
+  const int kg = gp_kgroup(d, nq, mass), ncol = 16 * gp_ntiles(nl);
+  std::vector<float> P((size_t)4 * kg * ncol, 0.f);
+  auto dphi = [&](int q, int i, int a) { return T.dphi[((size_t)q * nl + i) * d + a]; };
+  for (int q = 0; q < nq; ++q) {
+    int f = 0;
+    for (int a = 0; a <= d; ++a)
+      for (int b = a; b <= d; ++b) {
+        const bool is_mass = a == d && b == d;
+        if ((a == d || b == d) && !(is_mass && mass)) continue;
+        const size_t k = (size_t)q * nc + f++;
+        int pcol = 0;
+        for (int i = 0; i < nl; ++i)
+          for (int j = i; j < nl; ++j, ++pcol) {
+            double v;
+            if (is_mass) v = T.phi[(size_t)q * nl + i] * T.phi[(size_t)q * nl + j];
+            else if (a == b) v = dphi(q, i, a) * dphi(q, j, a);
+            else v = dphi(q, i, a) * dphi(q, j, b) + dphi(q, i, b) * dphi(q, j, a);
+            P[k * ncol + pcol] = (float)v;
+          }
+      }
+    if (f != nc) return fail(c, BH_ERR_ARG, "internal: factor count mismatch");
+  }
+  float* dp = nullptr;
+  BH_HIP(c, hipMalloc((void**)&dp, P.size() * sizeof(float)));
+  BH_HIP(c, hipMemcpy(dp, P.data(), P.size() * sizeof(float), hipMemcpyHostToDevice));
+  c->gp_tables[key] = dp;
+  *out = dp;
+  return BH_OK;
+}
+
 int build_colouring(bh_ctx c, const std::vector<int32_t>& dofs, int nl, int64_t ndofs,
                     int64_t ncells, Colouring& col) {
   col.ncolors = colour_cells(dofs, nl, ndofs, ncells, col.color_ptr, col.cells);
@@ -251,6 +298,7 @@ int bh_ctx_destroy(bh_ctx c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->tables) (void)hipFree(kv.second);
+  for (auto& kv : c->gp_tables) (void)hipFree(kv.second);
   for (auto& e : c->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
@@ -636,7 +684,8 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
     if (c->capturing)
       return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
     int ks = V->nl <= 32 ? 32 : ((V->nl + 31) / 32) * 32;
-    BH_HIP(c, dalloc((double**)&p->d_ke, (size_t)nc * V->nl * ks));
+    // whole groups of 16 cells: the FP32 GEMM element kernel writes them all
+    BH_HIP(c, dalloc((double**)&p->d_ke, (size_t)((nc + 15) / 16 * 16) * V->nl * ks));
   }
   LaunchArgs A;
   A.kind = LaunchArgs::BILINEAR;
@@ -667,8 +716,16 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
     A.cells = gb.d_cells;
     A.tab = tab;
     A.ncells_total = gb.n;
+    A.ptab = nullptr;
+    const bool gp = mode == BH_FP32 && !gb.d_cells && gp_supported(gb.geo, V->order, key);
+    if (gp) {
+      float* pt = nullptr;
+      if (int r = get_gp_table(c, gb.geo, V->order, quad_degree, A.mass, &pt)) return r;
+      A.ptab = pt;
+    }
     time_begin(c, 0);
-    if (mfma_supported(gb.geo, V->order, key)) BH_HIP(c, launch_mfma(gb.geo, V->order, key, A));
+    if (gp) BH_HIP(c, launch_gp(gb.geo, V->order, key, A));
+    else if (mfma_supported(gb.geo, V->order, key)) BH_HIP(c, launch_mfma(gb.geo, V->order, key, A));
     else BH_HIP(c, launch(gb.geo, V->order, key, A));
     time_end(c);
   }

@@ -1,0 +1,314 @@
+// bh_gp.hip -- FP32 element matrices as one GEMM over cells on the matrix cores.
+//
+// For the recognised bilinear form ∫η∇u⋅∇v + ∫c u v the element matrix is
+//     K_c[i][j] = Σ_q Σ_{a,b} G_{c,q}[a][b] ∂_aφ̂_i(ξ_q) ∂_bφ̂_j(ξ_q) + m_{c,q} φ̂_i(ξ_q) φ̂_j(ξ_q)
+// with G_{c,q} = w_q|J|η J⁻¹J⁻ᵀ (symmetric) and m_{c,q} = w_q|J|c.  Flattening
+// k = (q, factor) over the D(D+1)/2 entries of G plus m, and p = (i <= j) over
+// the upper triangle of K (K is symmetric), this is a product
+//     K[c][p] = Σ_k Gf[c][k] P[k][p]
+// with the per-cell factors Gf as the A operand (16 cells per tile) and the
+// reference products P (off-diagonal G entries carry both ∂_aφ̂_i∂_bφ̂_j and
+// ∂_bφ̂_i∂_aφ̂_j) as the B operand.  P is the same for every cell: each wave
+// keeps the P columns of its output tiles in registers for the whole kernel,
+// so the contraction is a stream of v_mfma_f32_16x16x4f32 fed by one 16-byte
+// LDS read of Gf per four k-steps and no other per-step work.
+//
+// Per group of 16 cells: nodes and coefficients (prefetched a group ahead)
+// go to LDS; J, J⁻¹ and |J| per point in FP64 give Gf, rounded once to FP32;
+// the matrix cores form the 16 upper triangles; they are mirrored into full
+// element matrices in LDS and written as whole 128-byte ke rows
+// ke[(cell*erow + i)*KS + j] (partial-line stores run at half the HBM rate),
+// which the row gather sums into the CSR values in FP64, exactly as for the
+// other element kernels.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bh_device.h"
+#include "bh_kernels.h"
+
+namespace bh {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+template <int D, int NL, int NQ, int NG, bool MASS>
+struct GPShape {
+  static constexpr int CPB = 16;                        // cells per tile (MFMA M)
+  static constexpr int NS = D * (D + 1) / 2;            // entries of G
+  static constexpr int NC = NS + (MASS ? 1 : 0);        // factors per point
+  static constexpr int KT = NQ * NC;                    // contraction length
+  static constexpr int KG = gp_kgroup(D, NQ, MASS);     // k-steps; lane group g holds k in [g*KG, (g+1)*KG)
+  static constexpr int NP = NL * (NL + 1) / 2;          // pairs i <= j
+  static constexpr int NT = gp_ntiles(NL);              // 16-wide output tiles
+  static constexpr int TPW = (NT + 7) / 8;              // tiles per wave
+  static constexpr int W = (NT + TPW - 1) / TPW;        // waves per block
+  static constexpr int NTH = W * 64;
+  static constexpr int GS = 4 * KG + 4;                 // LDS row stride of Gf (floats)
+  static constexpr int KS = KeStride<NL>::value;        // ke / staged row stride (floats)
+  static constexpr int NGEO = (CPB * NQ + NTH - 1) / NTH;             // geometry passes
+  static constexpr int NOUT = (CPB * NL * (KS / 4) + NTH - 1) / NTH;  // float4 row stores per thread
+  static_assert(KT <= 4 * KG, "k groups cover the contraction");
+  static_assert(CPB * NG <= NTH, "one thread per prefetched node");
+  static_assert(KS % 32 == 0, "row swizzle permutes 4-float chunks of 32-float rows");
+};
+
+// A value the compiler must treat as changed here: addresses derived from it
+// are recomputed per step instead of held in registers for the whole loop.
+__device__ __forceinline__ int gp_opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// Staged element rows use a 32-float stride with their 4-float chunks permuted
+// by row (chunk ^ (row & 7)), so the mirrored column writes of one MFMA tile
+// fall into different LDS banks while rows are still read as float4.
+__device__ __forceinline__ int gp_swz(int row, int j) { return j ^ ((row & 7) << 2); }
+
+template <int D, int NL, int NQ, int NG, bool MASS>
+__global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS>::NTH)) void k_element_gp(
+    const double* __restrict__ coords, const int32_t* __restrict__ c2n,
+    const double* __restrict__ tab, const float* __restrict__ ptab, int ncells, int erow,
+    const double* __restrict__ eta, double eta_scale, const double* __restrict__ mcoef,
+    double mass_scale, float* __restrict__ ke) {
+  // ke must hold whole groups: cells up to 16 * ceil(ncells / 16) are written
+  // (rows of cells past ncells get unused values).
+  using S = GPShape<D, NL, NQ, NG, MASS>;
+  using L = TabLayout<D, NL, NQ, NG>;
+  constexpr int CPB = S::CPB, KG = S::KG, TPW = S::TPW, NTH = S::NTH, GS = S::GS;
+  constexpr int NC = S::NC, NS = S::NS, NT = S::NT, NP = S::NP, KS = S::KS;
+
+  __shared__ double s_dg[NQ * NG * D];
+  __shared__ double s_w[NQ];
+  // nodes and per-cell coefficients; the last slot takes the stores of
+  // threads with nothing to stage, so staging needs no branch
+  __shared__ double s_x[CPB * NG + 1][D];
+  __shared__ double s_c[CPB + 1][2];  // η and mass coefficient of each cell
+  __shared__ __attribute__((aligned(16))) float s_G[2][CPB * GS];
+  __shared__ __attribute__((aligned(16))) float s_K[CPB * NL * KS];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int ln = lane & 15;  // MFMA row (A) / column (B, C) of this lane
+  const int lg = lane >> 4;  // k group (A, B) / row quad (C)
+  for (int k = tid; k < NQ * NG * D; k += NTH) s_dg[k] = tab[L::DG + k];
+  for (int k = tid; k < NQ; k += NTH) s_w[k] = tab[L::W + k];
+  for (int k = tid; k < 2 * CPB * GS; k += NTH) (&s_G[0][0])[k] = 0.f;  // k >= KT stays 0
+
+  // B operand: P[k][p] for k = lg*KG + s and the columns p of this wave's tiles
+  float bp[TPW][KG];
+  int pi[TPW], pj[TPW];
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int t = wave * TPW + tt;
+    const int col = t * 16 + ln;
+    const int tc = t < NT ? t : NT - 1;  // waves past the last tile hold zeros
+#pragma unroll
+    for (int s = 0; s < KG; ++s) {
+      const float v = ptab[(size_t)(lg * KG + s) * (NT * 16) + tc * 16 + ln];
+      bp[tt][s] = t < NT ? v : 0.f;
+    }
+    // pair (i, j), i <= j, of column col in row-major upper-triangle order
+    int p = col, i = 0;
+    while (i < NL && p >= NL - i) { p -= NL - i; ++i; }
+    pi[tt] = (t < NT && col < NP) ? i : -1;
+    pj[tt] = i + p;
+  }
+
+  // This block's groups are g(t) = blockIdx.x + t*gridDim.x, t < T.  Step t
+  // first writes out group t-1 (staged in s_K), stages the nodes of group t+1
+  // and contracts group t from s_G[t&1]: one stretch of code with no
+  // branches, so the row stores issue between the matrix-core instructions.
+  // Then it mirrors group t into s_K and forms the factors of group t+1 in
+  // s_G[(t+1)&1].  Node coordinates are loaded two groups ahead, ids three.
+  const int ngroups = (ncells + CPB - 1) / CPB;
+  const int G = (int)gridDim.x;
+  const int T = ((int)blockIdx.x < ngroups) ? (ngroups - 1 - (int)blockIdx.x) / G + 1 : 0;
+  const int pc = tid / NG, pn = tid - (tid / NG) * NG;
+  const bool loader = tid < CPB * NG;
+  if (T == 0) return;
+  auto group = [&](int t) { return (int)blockIdx.x + t * G; };
+  auto node_of = [&](int t) -> int {
+    const int cell = group(t) * CPB + pc;
+    return (loader && t < T && cell < ncells) ? c2n[(int64_t)cell * NG + pn] : -1;
+  };
+  auto load_x = [&](int node, double (&x)[D]) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = node >= 0 ? coords[(int64_t)node * D + a] : 0.0;
+  };
+  auto load_c = [&](int t, double& e, double& mc) {
+    const int cell = group(t) * CPB + tid;
+    const bool ok = tid < CPB && t < T && cell < ncells;
+    e = (ok && eta) ? eta[cell] * eta_scale : eta_scale;
+    mc = (ok && mcoef) ? mcoef[cell] * mass_scale : mass_scale;
+  };
+  const int xslot = loader ? tid : CPB * NG;
+  const int cslot = tid < CPB ? tid : CPB;
+  auto stage_nodes = [&](const double (&x)[D], double e, double mc) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) s_x[xslot][a] = x[a];
+    s_c[cslot][0] = e;
+    s_c[cslot][1] = mc;
+  };
+  // FP64 geometry of the staged group -> FP32 factors Gf[c][q*NC + f] in g.
+  // Threads past CPB*NQ repeat the last point (same values, same address).
+  auto geometry = [&](float* __restrict__ gbuf) {
+#pragma unroll
+    for (int it = 0; it < S::NGEO; ++it) {
+      const int k = min(tid + it * NTH, CPB * NQ - 1);
+      const int c = k / NQ, q = k - (k / NQ) * NQ;
+      double J[D][D];
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) J[a][b] = 0.0;
+#pragma unroll
+      for (int n = 0; n < NG; ++n) {
+        double xn[D], dn[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) xn[a] = s_x[c * NG + n][a];
+#pragma unroll
+        for (int b = 0; b < D; ++b) dn[b] = s_dg[(q * NG + n) * D + b];
+#pragma unroll
+        for (int b = 0; b < D; ++b)
+#pragma unroll
+          for (int a = 0; a < D; ++a) J[a][b] += xn[a] * dn[b];
+      }
+      double Ji[D][D];
+      const double det = invert<D>(J, Ji);
+      const double wdet = s_w[q] * fabs(det);
+      const double ge = wdet * s_c[c][0];
+      float* g = gbuf + c * GS + q * NC;
+      int f = 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = a; b < D; ++b) {
+          double v = 0.0;
+#pragma unroll
+          for (int t = 0; t < D; ++t) v += Ji[a][t] * Ji[b][t];
+          g[f++] = (float)(ge * v);
+        }
+      if constexpr (MASS) g[NS] = (float)(wdet * s_c[c][1]);
+    }
+  };
+
+  // prologue: factors of group 0; coordinates of group 1 in registers
+  double xr[D], er, mr;
+  load_x(node_of(0), xr);
+  load_c(0, er, mr);
+  stage_nodes(xr, er, mr);
+  load_x(node_of(1), xr);
+  load_c(1, er, mr);
+  int nid = node_of(2);
+  __syncthreads();
+  geometry(s_G[0]);
+  __syncthreads();
+
+  for (int t = 0; t <= T; ++t) {
+    // 1. rows of group t-1 (step 0 writes the unstaged buffer over group 0's
+    //    rows, which step 1 then overwrites from the same threads), nodes of
+    //    group t+1, contraction of group t
+    {
+      const int64_t ob = (int64_t)group(t > 0 ? t - 1 : 0) * CPB * erow * KS;
+      const int tl = gp_opaque(tid);
+#pragma unroll
+      for (int it = 0; it < S::NOUT; ++it) {
+        const int k = min(tl + it * NTH, CPB * NL * (KS / 4) - 1);
+        const int row = k / (KS / 4), j4 = (k - row * (KS / 4)) * 4;
+        const int c = row / NL;
+        const floatx4 v = *(const floatx4*)&s_K[row * KS + gp_swz(row, j4)];
+        *(floatx4*)&ke[ob + ((int64_t)c * erow + (row - c * NL)) * KS + j4] = v;
+      }
+    }
+    stage_nodes(xr, er, mr);
+    floatx4 acc[TPW];
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt) acc[tt] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const float* ga = &s_G[t & 1][ln * GS + lg * KG];
+#pragma unroll
+    for (int s = 0; s < KG; s += 4) {
+      const floatx4 a = *(const floatx4*)(ga + s);
+#pragma unroll
+      for (int tt = 0; tt < TPW; ++tt) {
+        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], bp[tt][s + 0], acc[tt], 0, 0, 0);
+        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], bp[tt][s + 1], acc[tt], 0, 0, 0);
+        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], bp[tt][s + 2], acc[tt], 0, 0, 0);
+        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], bp[tt][s + 3], acc[tt], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+    // 2. group t's upper triangles mirrored into full rows of s_K; factors of
+    //    group t+1; loads for groups t+2 (coordinates) and t+3 (node ids)
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt) {
+      if (pi[tt] < 0) continue;
+      const int ii = gp_opaque(pi[tt]), jj = gp_opaque(pj[tt]), c0 = 4 * gp_opaque(lg);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int ri = (c0 + v) * NL + ii, rj = (c0 + v) * NL + jj;
+        s_K[ri * KS + gp_swz(ri, jj)] = acc[tt][v];
+        s_K[rj * KS + gp_swz(rj, ii)] = acc[tt][v];
+      }
+    }
+    load_x(nid, xr);
+    load_c(t + 2, er, mr);
+    nid = node_of(t + 3);
+    geometry(s_G[(t + 1) & 1]);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <int D, int NL, int NQ, int NG>
+static hipError_t launch_gp_t(const LaunchArgs& A) {
+  if (A.cells || !A.ptab || A.mode != BH_FP32) return hipErrorInvalidValue;
+  constexpr int CPB = 16;
+  const int ng = (int)((A.ncells_total + CPB - 1) / CPB);
+  if (ng < 1) return hipSuccess;
+  auto go = [&](auto kern, int nth) -> hipError_t {
+    static int resident = 0;  // blocks per device: CUs x blocks per CU
+    if (resident == 0) {
+      int dev = 0, ncu = 0, per = 0;
+      hipError_t e = hipGetDevice(&dev);
+      if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+      if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, nth, 0);
+      if (e != hipSuccess) return e;
+      resident = ncu * (per > 0 ? per : 1);
+    }
+    dim3 grid(ng < resident ? ng : resident);
+    hipLaunchKernelGGL(kern, grid, dim3(nth), 0, A.stream, A.coords, A.c2n, A.tab, A.ptab,
+                       (int)A.ncells_total, A.erow > 0 ? A.erow : NL, A.eta, A.eta_scale, A.mcoef,
+                       A.mass_scale, (float*)A.ke);
+    return hipGetLastError();
+  };
+  if (A.mass) return go(k_element_gp<D, NL, NQ, NG, true>, GPShape<D, NL, NQ, NG, true>::NTH);
+  return go(k_element_gp<D, NL, NQ, NG, false>, GPShape<D, NL, NQ, NG, false>::NTH);
+}
+
+struct GPEntry {
+  int geo, order, nq1;
+  hipError_t (*fn)(const LaunchArgs&);
+};
+
+#define BH_GH(ORDER, NQ1)                                                                     \
+  {BH_HEXA8, ORDER, NQ1,                                                                      \
+   &launch_gp_t<3, ((ORDER) + 1) * ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1) * (NQ1), 8>}
+
+static const GPEntry kGP[] = {
+    BH_GH(1, 2), BH_GH(2, 3),
+};
+
+bool gp_supported(int geo, int order, int nq1) {
+  for (const GPEntry& e : kGP)
+    if (e.geo == geo && e.order == order && e.nq1 == nq1) return true;
+  return false;
+}
+
+hipError_t launch_gp(int geo, int order, int nq1, const LaunchArgs& A) {
+  for (const GPEntry& e : kGP)
+    if (e.geo == geo && e.order == order && e.nq1 == nq1) return e.fn(A);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace bh

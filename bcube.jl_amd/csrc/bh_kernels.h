@@ -18,6 +18,7 @@ struct LaunchArgs {
   const int32_t* c2n = nullptr;    // ncells * nodes-per-cell (geometric nodes)
   const int32_t* rdofs = nullptr;  // ncells * nl
   const double* tab = nullptr;     // packed reference tables
+  const float* ptab = nullptr;     // reference products P of the GEMM element kernel (bh_gp.hip)
   const int32_t* cells = nullptr;  // cells of this colour (source) / of this cell type (bilinear)
   int ncells_color = 0;
   int slot0 = 0;                   // slot of cells[0] in the scatter map
@@ -58,5 +59,17 @@ hipError_t launch_gather(int nl, const LaunchArgs& a);
 // Matrix-core element kernels (bh_mfma.hip): BH_FP32 and BH_BF16X3, bilinear forms only.
 bool mfma_supported(int geo_celltype, int order, int nq1);
 hipError_t launch_mfma(int geo_celltype, int order, int nq1, const LaunchArgs& a);
+
+// FP32 element matrices as one GEMM over cells (bh_gp.hip), uniform meshes.
+// Its constant operand P is float [4*kgroup][16*ntiles], row k = q*NC + f
+// (f over the D(D+1)/2 entries of G in row-major upper order, then the mass
+// factor when present), column p over the pairs i <= j in row-major upper
+// order, zero-padded; see gp_table() in bh_api.hip.
+constexpr int gp_kgroup(int dim, int nq, bool mass) {
+  return ((nq * (dim * (dim + 1) / 2 + (mass ? 1 : 0)) + 15) / 16) * 4;
+}
+constexpr int gp_ntiles(int nl) { return (nl * (nl + 1) / 2 + 15) / 16; }
+bool gp_supported(int geo_celltype, int order, int nq1);
+hipError_t launch_gp(int geo_celltype, int order, int nq1, const LaunchArgs& a);
 
 }  // namespace bh

@@ -89,7 +89,7 @@ struct bh_pattern_s {
   int32_t* d_pptr = nullptr;
   int32_t* d_pairs = nullptr;
   void* d_goff = nullptr;
-  void* d_ke = nullptr;      // element-row workspace, 16*ceil(ncells/16)*nl*KS FP64 (FP32 uses half)
+  void* d_ke = nullptr;      // element rows, 16*ceil(ncells/16)*nl*ke_stride(nl) FP64 (FP32 uses half)
   double* d_vals = nullptr;
   double* d_coef = nullptr;  // staged host coefficients (2 * ncells)
 };
@@ -683,7 +683,7 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
   if (!p->d_ke) {
     if (c->capturing)
       return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
-    int ks = V->nl <= 32 ? 32 : ((V->nl + 31) / 32) * 32;
+    const int ks = ke_stride(V->nl);
     // whole groups of 16 cells: the FP32 GEMM element kernel writes them all
     BH_HIP(c, dalloc((double**)&p->d_ke, (size_t)((nc + 15) / 16 * 16) * V->nl * ks));
   }

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "bh_kernels.h"
+
 namespace bh {
 
 constexpr int kBlock = 256;
@@ -50,11 +52,11 @@ struct TabLayout {
   static constexpr int SIZE = G + NQ * NG;
 };
 
-// Row stride (elements) of the element-matrix workspace: rows start on
-// 128-byte boundaries for FP32.
+// Row stride of the element-row workspace for a space of NL local dofs
+// (ke_stride in bh_kernels.h).
 template <int NL>
 struct KeStride {
-  static constexpr int value = NL <= 32 ? 32 : ((NL + 31) / 32) * 32;
+  static constexpr int value = ke_stride(NL);
 };
 
 template <int D>

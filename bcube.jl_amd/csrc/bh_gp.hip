@@ -44,12 +44,13 @@ struct GPShape {
   static constexpr int W = (NT + TPW - 1) / TPW;        // waves per block
   static constexpr int NTH = W * 64;
   static constexpr int GS = 4 * KG + 4;                 // LDS row stride of Gf (floats)
-  static constexpr int KS = KeStride<NL>::value;        // ke / staged row stride (floats)
+  static constexpr int KS = KeStride<NL>::value;        // ke row stride (floats)
+  static constexpr int KSL = 32;                        // staged row stride in LDS (floats)
   static constexpr int NGEO = (CPB * NQ + NTH - 1) / NTH;             // geometry passes
   static constexpr int NOUT = (CPB * NL * (KS / 4) + NTH - 1) / NTH;  // float4 row stores per thread
   static_assert(KT <= 4 * KG, "k groups cover the contraction");
   static_assert(CPB * NG <= NTH, "one thread per prefetched node");
-  static_assert(KS % 32 == 0, "row swizzle permutes 4-float chunks of 32-float rows");
+  static_assert(KS % 4 == 0 && KS <= KSL, "float4 rows inside the swizzled 32-float LDS rows");
 };
 
 // A value the compiler must treat as changed here: addresses derived from it
@@ -75,7 +76,7 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS>::NTH)) void k_element
   using S = GPShape<D, NL, NQ, NG, MASS>;
   using L = TabLayout<D, NL, NQ, NG>;
   constexpr int CPB = S::CPB, KG = S::KG, TPW = S::TPW, NTH = S::NTH, GS = S::GS;
-  constexpr int NC = S::NC, NS = S::NS, NT = S::NT, NP = S::NP, KS = S::KS;
+  constexpr int NC = S::NC, NS = S::NS, NT = S::NT, NP = S::NP, KS = S::KS, KSL = S::KSL;
 
   __shared__ double s_dg[NQ * NG * D];
   __shared__ double s_w[NQ];
@@ -84,7 +85,7 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS>::NTH)) void k_element
   __shared__ double s_x[CPB * NG + 1][D];
   __shared__ double s_c[CPB + 1][2];  // η and mass coefficient of each cell
   __shared__ __attribute__((aligned(16))) float s_G[2][CPB * GS];
-  __shared__ __attribute__((aligned(16))) float s_K[CPB * NL * KS];
+  __shared__ __attribute__((aligned(16))) float s_K[CPB * NL * KSL];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -217,7 +218,7 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS>::NTH)) void k_element
         const int k = min(tl + it * NTH, CPB * NL * (KS / 4) - 1);
         const int row = k / (KS / 4), j4 = (k - row * (KS / 4)) * 4;
         const int c = row / NL;
-        const floatx4 v = *(const floatx4*)&s_K[row * KS + gp_swz(row, j4)];
+        const floatx4 v = *(const floatx4*)&s_K[row * KSL + gp_swz(row, j4)];
         *(floatx4*)&ke[ob + ((int64_t)c * erow + (row - c * NL)) * KS + j4] = v;
       }
     }
@@ -247,8 +248,8 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS>::NTH)) void k_element
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int ri = (c0 + v) * NL + ii, rj = (c0 + v) * NL + jj;
-        s_K[ri * KS + gp_swz(ri, jj)] = acc[tt][v];
-        s_K[rj * KS + gp_swz(rj, ii)] = acc[tt][v];
+        s_K[ri * KSL + gp_swz(ri, jj)] = acc[tt][v];
+        s_K[rj * KSL + gp_swz(rj, ii)] = acc[tt][v];
       }
     }
     load_x(nid, xr);

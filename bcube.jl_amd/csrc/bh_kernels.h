@@ -49,6 +49,15 @@ struct LaunchArgs {
   double* vals = nullptr;
 };
 
+// Row stride (entries) of the element-row workspace ke for elements of nl
+// local dofs (the space's largest cell): 4, 8, 16 or a multiple of 32, so rows
+// tile 128-byte lines without straddling them and small elements do not drag
+// 128-byte rows through HBM.  Element kernels and the row gather of one
+// assembly use the same stride.
+constexpr int ke_stride(int nl) {
+  return nl <= 4 ? 4 : nl <= 8 ? 8 : nl <= 16 ? 16 : ((nl + 31) / 32) * 32;
+}
+
 bool kernel_supported(int geo_celltype, int order, int nq1);
 int packed_table_size(int geo_celltype, int order, int nq1);
 hipError_t launch(int geo_celltype, int order, int nq1, const LaunchArgs& a);

@@ -41,11 +41,12 @@ __global__ __launch_bounds__(kBlock) void k_element_f64(
     const double* __restrict__ eta, double eta_scale, const double* __restrict__ mcoef,
     double mass_scale, double* __restrict__ ke) {
   // cells[k]: global id of the k-th cell of this launch (NULL: k itself);
-  // coefficients are per global cell, element rows go to ke[(gcell*erow + i)*KS].
+  // coefficients are per global cell, element rows go to ke[(gcell*erow + i)*ks]
+  // with erow the space's padded dof count and ks = ke_stride(erow).
   using L = TabLayout<D, NL, NQ, NG>;
   constexpr int CPB = kBlock / NL;  // cells per group
   constexpr int NS = SymN<D>::N;
-  constexpr int KS = KeStride<NL>::value;
+  const int ks = ke_stride(erow);
 
   __shared__ double s_tab[L::SIZE];
   __shared__ double s_x[CPB][NG][D];
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_element_f64(
     }
     // 4. row i of the element matrix, contiguous in the workspace
     const int64_t gcell = cells ? cells[base + cs] : base + cs;
-    double* out = ke + (gcell * erow + li) * KS;
+    double* out = ke + (gcell * erow + li) * ks;
 #pragma unroll
     for (int j = 0; j < NL; ++j) out[j] = acc[j];
   }

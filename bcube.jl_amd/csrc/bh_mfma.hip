@@ -54,7 +54,8 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
     const double* __restrict__ eta, double eta_scale, const double* __restrict__ mcoef,
     double mass_scale, void* __restrict__ ke_out) {
   // cells[k]: global id of the k-th cell of this launch (NULL: k itself);
-  // coefficients are per global cell, element rows go to ke[(gcell*erow + i)*KS].
+  // coefficients are per global cell, element rows go to ke[(gcell*erow + i)*ks]
+  // with erow the space's padded dof count and ks = ke_stride(erow).
   static_assert(NL <= 32, "two 16-row tiles cover at most 32 local dofs");
   static_assert(D + 1 <= 4, "k-step holds D gradient components + mass");
   using L = TabLayout<D, NL, NQ, NG>;
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
   constexpr int NT = (NCOL + 15) / 16;     // 16-wide column tiles
   constexpr int NS = SymN<D>::N;
   constexpr int ZERO = 7;                  // slot of s_g that holds 0
-  constexpr int KS = KeStride<NL>::value;
+  const int ks = ke_stride(erow);
   using RT = std::conditional_t<PREC == 2, double, float>;  // tables, factors, accumulators
   using AccT = std::conditional_t<PREC == 2, doublex4, floatx4>;
   RT* __restrict__ ke = (RT*)ke_out;
@@ -277,13 +278,13 @@ __global__ __launch_bounds__(kBlock) void k_element_mfma(
       // lane holds K[i][j], column j of cell c, for rows i = rt*16 + 4*(lane/16) + v
       // (FP32/BF16 16x16 accumulators) or i = rt*16 + lane/16 + 4*v (FP64 16x16x4)
       const int64_t gcell = cells ? cells[base + c] : base + c;
-      RT* out = ke + gcell * erow * KS + j;
+      RT* out = ke + gcell * erow * ks + j;
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int i = PREC == 2 ? rt * 16 + ak + 4 * v : rt * 16 + 4 * ak + v;
-          if (i < NL) out[i * KS] = rt == 0 ? acc0[v] : acc1[v];
+          if (i < NL) out[i * ks] = rt == 0 ? acc0[v] : acc1[v];
         }
     }
   }

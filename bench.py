@@ -27,7 +27,7 @@ PEAK_HBM_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16"}
 KERNEL = {"fp64": "k_element_mfma<3,27,27,8,true,2>",
-          "fp32": "k_element_mfma<3,27,27,8,true,0>",
+          "fp32": "k_element_gp<3,27,27,8,true>",
           "bf16x3": "k_element_mfma<3,27,27,8,true,1>"}
 GATHER = {"fp64": "k_gather_padded<double,unsigned char,27,128,8,2>",
           "fp32": "k_gather_padded<float,unsigned char,27,128,8,2>",
@@ -49,9 +49,7 @@ def kernel_rooflines(mode, launches, kinds, steps, f_alg, nnz):
     """Per-kernel rooflines.  Each step is the element kernel (algorithmic
     flops F_alg, bound by the matrix/vector FP peak of the mode) and the row
     gather (algorithmic bytes: the 8*nnz CSR values it writes, bound by HBM);
-    large meshes run both in cell chunks (the gather of one chunk overlapping
-    the next chunk's element kernel), whose times are summed per kernel.
-    Returns (dominant, [element, gather])."""
+    launch times are summed per kernel kind.  Returns (dominant, [element, gather])."""
     t = np.asarray(launches, dtype=np.float64)
     k = np.asarray(kinds)
     el_ms = float(t[k == 0].sum() / steps)
@@ -366,7 +364,8 @@ def bench_c4(args, torch, B):
                      "cells_per_s": m.ncells / (ms * 1e-3),
                      "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS,
                                   "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
-                                  "kernel": f"k_dg_residual<{'float' if es == 4 else 'double'},4>",
+                                  "kernel": ("k_dg_residual_reg<float,4>" if es == 4
+                                             else "k_dg_residual<double,4>"),
                                   "avg_launch_ms": k_ms, "alg_bytes_per_launch": b_alg,
                                   "factor_bytes_per_launch":
                                       (m.ncells * 4 * 64 + nfaces * 16) * es}}

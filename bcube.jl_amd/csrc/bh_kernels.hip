@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "bh_device.h"
 #include "bh_kernels.h"
@@ -149,23 +150,24 @@ __global__ __launch_bounds__(kBlock) void k_element_f64(
 
 // -------------------------------------------------------------------------
 // Row gather: vals[row] = Σ over (cell, i) touching row, in cell order, of
-// element row ke[cell][i][:] placed by goff.  Half a wave per row.
+// element row ke[cell][i][:] placed by goff.
 //   k_gather_padded: pair lists padded to MAXP slots per row (-1 = empty);
-//     each half-wave handles RPH rows at once so that 2*RPH*MAXP element-row
-//     loads are in flight before the LDS adds.
+//     a group of SG lanes (4 to 32, sized to the element) takes RPH rows at
+//     once, so one load instruction covers the element rows of 64/SG*RPH rows
+//     and RPH*MAXP loads per lane are in flight before the LDS adds.
 //   k_gather: variable-length lists (rows touched by more than 16 cells).
 // -------------------------------------------------------------------------
-template <typename T, typename OFF, int NL, int ROWCAP, int MAXP, int RPH>
+template <typename T, typename OFF, int NL, int ROWCAP, int MAXP, int RPH, int SG>
 __global__ __launch_bounds__(kBlock) void k_gather_padded(
     const T* __restrict__ ke, const int32_t* __restrict__ pairs, const OFF* __restrict__ goff,
     const int64_t* __restrict__ rowptr, int64_t nrows, double* __restrict__ vals) {
   constexpr int KS = KeStride<NL>::value;
-  constexpr int HW = kBlock / 32;
-  constexpr int NJ = (NL + 31) / 32;
+  constexpr int HW = kBlock / SG;        // lane groups per block
+  constexpr int NJ = (NL + SG - 1) / SG;
   constexpr int kSent = (int)(OFF)~(OFF)0;  // padding column (mixed meshes)
   __shared__ double buf[HW][RPH][ROWCAP];
-  const int h = threadIdx.x >> 5;
-  const int l = threadIdx.x & 31;
+  const int h = threadIdx.x / SG;
+  const int l = threadIdx.x % SG;
   const int64_t step = (int64_t)gridDim.x * HW * RPH;
   for (int64_t row0 = ((int64_t)blockIdx.x * HW + h) * RPH; row0 < nrows; row0 += step) {
     int64_t rs[RPH];
@@ -188,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_padded(
       for (int k = 0; k < MAXP; ++k)
 #pragma unroll
         for (int t = 0; t < NJ; ++t) {
-          const int j = l + 32 * t;
+          const int j = l + SG * t;
           o[r][k][t] = kSent;
           if (pr[r][k] >= 0 && j < NL) {
             v[r][k][t] = ke[(int64_t)pr[r][k] * KS + j];
@@ -197,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_padded(
         }
 #pragma unroll
     for (int r = 0; r < RPH; ++r)
-      for (int k = l; k < len[r]; k += 32) buf[h][r][k] = 0.0;
+      for (int k = l; k < len[r]; k += SG) buf[h][r][k] = 0.0;
     // adds in pair (cell) order: the summation order is fixed
 #pragma unroll
     for (int r = 0; r < RPH; ++r)
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_padded(
           if (o[r][k][t] != kSent) buf[h][r][o[r][k][t]] += (double)v[r][k][t];
 #pragma unroll
     for (int r = 0; r < RPH; ++r)
-      for (int k = l; k < len[r]; k += 32) vals[rs[r] + k] = buf[h][r][k];
+      for (int k = l; k < len[r]; k += SG) vals[rs[r] + k] = buf[h][r][k];
   }
 }
 
@@ -357,34 +359,51 @@ static int grid_for(int ngroups) {
   return g < 1 ? 1 : g;
 }
 
+// Padded row gather with SG lanes per row and LDS rows of CAP entries; two
+// rows per lane group when the block's row buffers stay within 32 KB.
+template <int NL, int SG, typename T, typename OFFT, int CAP>
+static hipError_t gather_padded_go(const LaunchArgs& A) {
+  constexpr int HW = kBlock / SG;
+  constexpr int RPH = HW * CAP * 8 * 2 <= 32768 ? 2 : 1;
+  int64_t blocks = (A.nrows + HW * RPH - 1) / (HW * RPH);
+  dim3 grid((unsigned)(blocks < 16384 ? (blocks < 1 ? 1 : blocks) : 16384));
+#define BH_GP(MP)                                                                              \
+  hipLaunchKernelGGL((k_gather_padded<T, OFFT, NL, CAP, MP, RPH, SG>), grid, dim3(kBlock), 0,   \
+                     A.stream, (const T*)A.ke, A.pairs, (const OFFT*)A.goff, A.rowptr, A.nrows, \
+                     A.vals)
+  if (A.max_adj <= 4) BH_GP(4);
+  else if (A.max_adj <= 8) BH_GP(8);
+  else BH_GP(16);
+#undef BH_GP
+  return hipGetLastError();
+}
+
+// Lanes per row: the element's size (4, 8, 16 or 32) while its rows fit the
+// small row buffers, else 16, else 32 lanes.
+template <int NL, typename T>
+static hipError_t gather_padded_t(const LaunchArgs& A) {
+  constexpr int SGS = NL <= 4 ? 4 : NL <= 8 ? 8 : NL <= 16 ? 16 : 32;
+  const bool b8 = A.offset_bytes == 1;
+  if constexpr (SGS <= 8) {
+    if (A.max_row <= 32 && b8) return gather_padded_go<NL, SGS, T, uint8_t, 32>(A);
+    if (A.max_row <= 64 && b8) return gather_padded_go<NL, SGS, T, uint8_t, 64>(A);
+  }
+  if constexpr (SGS <= 16) {
+    if (A.max_row <= 64 && b8) return gather_padded_go<NL, 16, T, uint8_t, 64>(A);
+    if (A.max_row <= 128 && b8) return gather_padded_go<NL, 16, T, uint8_t, 128>(A);
+    if (A.max_row <= 256 && b8) return gather_padded_go<NL, 16, T, uint8_t, 256>(A);
+  }
+  if (A.max_row <= 128 && b8) return gather_padded_go<NL, 32, T, uint8_t, 128>(A);
+  if (A.max_row <= 256 && b8) return gather_padded_go<NL, 32, T, uint8_t, 256>(A);
+  if (A.max_row <= 512) return gather_padded_go<NL, 32, T, uint16_t, 512>(A);
+  return hipErrorInvalidValue;
+}
+
 template <int NL>
 static hipError_t gather_nl(const LaunchArgs& A) {
   const bool f64 = A.ke_fp64;
-  if (A.padded_pairs) {
-    constexpr int RPH = 2;
-    int64_t blocks = (A.nrows + (kBlock / 32) * RPH - 1) / ((kBlock / 32) * RPH);
-    dim3 grid((unsigned)(blocks < 16384 ? (blocks < 1 ? 1 : blocks) : 16384));
-#define BH_GP(T, OFFT, CAP, MP)                                                                   \
-  hipLaunchKernelGGL((k_gather_padded<T, OFFT, NL, CAP, MP, RPH>), grid, dim3(kBlock), 0, A.stream, \
-                   (const T*)A.ke, A.pairs, (const OFFT*)A.goff, A.rowptr, A.nrows, A.vals)
-#define BH_GP_MP(T, OFFT, CAP)                         \
-  do {                                                 \
-  if (A.max_adj <= 4) BH_GP(T, OFFT, CAP, 4);        \
-  else if (A.max_adj <= 8) BH_GP(T, OFFT, CAP, 8);   \
-  else BH_GP(T, OFFT, CAP, 16);                      \
-  } while (0)
-    if (A.max_row <= 128 && A.offset_bytes == 1) {
-      if (f64) BH_GP_MP(double, uint8_t, 128); else BH_GP_MP(float, uint8_t, 128);
-    } else if (A.max_row <= 256 && A.offset_bytes == 1) {
-      if (f64) BH_GP_MP(double, uint8_t, 256); else BH_GP_MP(float, uint8_t, 256);
-    } else if (A.max_row <= 512) {
-      if (f64) BH_GP_MP(double, uint16_t, 512); else BH_GP_MP(float, uint16_t, 512);
-    } else {
-      return hipErrorInvalidValue;
-    }
-#undef BH_GP_MP
-#undef BH_GP
-  } else {
+  if (A.padded_pairs) return f64 ? gather_padded_t<NL, double>(A) : gather_padded_t<NL, float>(A);
+  {
     int64_t blocks = (A.nrows + kBlock / 32 - 1) / (kBlock / 32);
     dim3 grid((unsigned)(blocks < 16384 ? (blocks < 1 ? 1 : blocks) : 16384));
 #define BH_G(T, OFFT, CAP)                                                                  \

@@ -89,7 +89,7 @@ struct bh_pattern_s {
   int32_t* d_pptr = nullptr;
   int32_t* d_pairs = nullptr;
   void* d_goff = nullptr;
-  void* d_ke = nullptr;      // element rows, 16*ceil(ncells/16)*nl*ke_stride(nl) FP64 (FP32 uses half)
+  void* d_ke = nullptr;      // element rows, (ncells+16)*nl*ke_stride(nl) FP64 (FP32 uses half)
   double* d_vals = nullptr;
   double* d_coef = nullptr;  // staged host coefficients (2 * ncells)
 };
@@ -684,8 +684,9 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
     if (c->capturing)
       return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
     const int ks = ke_stride(V->nl);
-    // whole groups of 16 cells: the FP32 GEMM element kernel writes them all
-    BH_HIP(c, dalloc((double**)&p->d_ke, (size_t)((nc + 15) / 16 * 16) * V->nl * ks));
+    // 16 spare cells past the last: the FP32 GEMM element kernel writes whole
+    // groups of 16 cells, a bucket's last partial group into the spare rows
+    BH_HIP(c, dalloc((double**)&p->d_ke, (size_t)(nc + 16) * V->nl * ks));
   }
   LaunchArgs A;
   A.kind = LaunchArgs::BILINEAR;
@@ -717,7 +718,8 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
     A.tab = tab;
     A.ncells_total = gb.n;
     A.ptab = nullptr;
-    const bool gp = mode == BH_FP32 && !gb.d_cells && gp_supported(gb.geo, V->order, key);
+    const bool gp = mode == BH_FP32 && gp_supported(gb.geo, V->order, key);
+    A.spare_cell = nc;
     if (gp) {
       float* pt = nullptr;
       if (int r = get_gp_table(c, gb.geo, V->order, quad_degree, A.mass, &pt)) return r;

@@ -31,7 +31,7 @@ namespace bh {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-template <int D, int NL, int NQ, int NG, bool MASS>
+template <int D, int NL, int NQ, int NG, bool MASS, int KSR>
 struct GPShape {
   static constexpr int CPB = 16;                        // cells per tile (MFMA M)
   static constexpr int NS = D * (D + 1) / 2;            // entries of G
@@ -44,7 +44,7 @@ struct GPShape {
   static constexpr int W = (NT + TPW - 1) / TPW;        // waves per block
   static constexpr int NTH = W * 64;
   static constexpr int GS = 4 * KG + 4;                 // LDS row stride of Gf (floats)
-  static constexpr int KS = KeStride<NL>::value;        // ke row stride (floats)
+  static constexpr int KS = KSR;                        // ke row stride (floats)
   static constexpr int KSL = 32;                        // staged row stride in LDS (floats)
   static constexpr int NGEO = (CPB * NQ + NTH - 1) / NTH;             // geometry passes
   static constexpr int NOUT = (CPB * NL * (KS / 4) + NTH - 1) / NTH;  // float4 row stores per thread
@@ -65,15 +65,19 @@ __device__ __forceinline__ int gp_opaque(int v) {
 // fall into different LDS banks while rows are still read as float4.
 __device__ __forceinline__ int gp_swz(int row, int j) { return j ^ ((row & 7) << 2); }
 
-template <int D, int NL, int NQ, int NG, bool MASS>
-__global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS>::NTH)) void k_element_gp(
+template <int D, int NL, int NQ, int NG, bool MASS, int KSR, bool IND>
+__global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS, KSR>::NTH)) void k_element_gp(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
-    const double* __restrict__ tab, const float* __restrict__ ptab, int ncells, int erow,
-    const double* __restrict__ eta, double eta_scale, const double* __restrict__ mcoef,
-    double mass_scale, float* __restrict__ ke) {
-  // ke must hold whole groups: cells up to 16 * ceil(ncells / 16) are written
-  // (rows of cells past ncells get unused values).
-  using S = GPShape<D, NL, NQ, NG, MASS>;
+    const double* __restrict__ tab, const float* __restrict__ ptab, int ncells,
+    const int32_t* __restrict__ cells, int spare, int erow, const double* __restrict__ eta,
+    double eta_scale, const double* __restrict__ mcoef, double mass_scale,
+    float* __restrict__ ke) {
+  // cells[k]: global id of the k-th cell of this launch (IND; else k itself),
+  // c2n in launch order; coefficients are per global cell and element rows go to
+  // ke[(gcell*erow + i)*KSR + j] for i < NL, whole KSR-float rows.  The
+  // missing cells of a last, partial group write rows of the spare cells
+  // spare .. spare+15, which ke must hold.
+  using S = GPShape<D, NL, NQ, NG, MASS, KSR>;
   using L = TabLayout<D, NL, NQ, NG>;
   constexpr int CPB = S::CPB, KG = S::KG, TPW = S::TPW, NTH = S::NTH, GS = S::GS;
   constexpr int NC = S::NC, NS = S::NS, NT = S::NT, NP = S::NP, KS = S::KS, KSL = S::KSL;
@@ -84,6 +88,7 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS>::NTH)) void k_element
   // threads with nothing to stage, so staging needs no branch
   __shared__ double s_x[CPB * NG + 1][D];
   __shared__ double s_c[CPB + 1][2];  // η and mass coefficient of each cell
+  __shared__ int s_gc[4][CPB + 1];    // global cell ids of the groups in flight (-1: none)
   __shared__ __attribute__((aligned(16))) float s_G[2][CPB * GS];
   __shared__ __attribute__((aligned(16))) float s_K[CPB * NL * KSL];
 
@@ -137,19 +142,24 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS>::NTH)) void k_element
 #pragma unroll
     for (int a = 0; a < D; ++a) x[a] = node >= 0 ? coords[(int64_t)node * D + a] : 0.0;
   };
-  auto load_c = [&](int t, double& e, double& mc) {
+  auto gid_of = [&](int t) -> int {
     const int cell = group(t) * CPB + tid;
-    const bool ok = tid < CPB && t < T && cell < ncells;
-    e = (ok && eta) ? eta[cell] * eta_scale : eta_scale;
-    mc = (ok && mcoef) ? mcoef[cell] * mass_scale : mass_scale;
+    if (!(tid < CPB && t < T && cell < ncells)) return -1;
+    if constexpr (IND) return cells[cell];
+    return cell;
+  };
+  auto load_c = [&](int gid, double& e, double& mc) {
+    e = (gid >= 0 && eta) ? eta[gid] * eta_scale : eta_scale;
+    mc = (gid >= 0 && mcoef) ? mcoef[gid] * mass_scale : mass_scale;
   };
   const int xslot = loader ? tid : CPB * NG;
   const int cslot = tid < CPB ? tid : CPB;
-  auto stage_nodes = [&](const double (&x)[D], double e, double mc) {
+  auto stage_nodes = [&](int t, const double (&x)[D], double e, double mc, int gid) {
 #pragma unroll
     for (int a = 0; a < D; ++a) s_x[xslot][a] = x[a];
     s_c[cslot][0] = e;
     s_c[cslot][1] = mc;
+    if constexpr (IND) s_gc[t & 3][cslot] = gid;
   };
   // FP64 geometry of the staged group -> FP32 factors Gf[c][q*NC + f] in g.
   // Threads past CPB*NQ repeat the last point (same values, same address).
@@ -196,12 +206,15 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS>::NTH)) void k_element
 
   // prologue: factors of group 0; coordinates of group 1 in registers
   double xr[D], er, mr;
+  int gid = gid_of(0);
   load_x(node_of(0), xr);
-  load_c(0, er, mr);
-  stage_nodes(xr, er, mr);
+  load_c(gid, er, mr);
+  stage_nodes(0, xr, er, mr, gid);
   load_x(node_of(1), xr);
-  load_c(1, er, mr);
+  gid = gid_of(1);
+  load_c(gid, er, mr);
   int nid = node_of(2);
+  int gidn = gid_of(2);
   __syncthreads();
   geometry(s_G[0]);
   __syncthreads();
@@ -211,6 +224,8 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS>::NTH)) void k_element
     //    rows, which step 1 then overwrites from the same threads), nodes of
     //    group t+1, contraction of group t
     {
+      const int slot = t > 0 ? (t - 1) & 3 : 0;
+      // direct cell numbering: rows of group t-1 from one base (past ncells: spare rows)
       const int64_t ob = (int64_t)group(t > 0 ? t - 1 : 0) * CPB * erow * KS;
       const int tl = gp_opaque(tid);
 #pragma unroll
@@ -219,10 +234,16 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS>::NTH)) void k_element
         const int row = k / (KS / 4), j4 = (k - row * (KS / 4)) * 4;
         const int c = row / NL;
         const floatx4 v = *(const floatx4*)&s_K[row * KSL + gp_swz(row, j4)];
-        *(floatx4*)&ke[ob + ((int64_t)c * erow + (row - c * NL)) * KS + j4] = v;
+        if constexpr (IND) {
+          const int gc = s_gc[slot][c];
+          const int64_t cell = gc >= 0 ? gc : spare + c;
+          *(floatx4*)&ke[(cell * erow + (row - c * NL)) * KS + j4] = v;
+        } else {
+          *(floatx4*)&ke[ob + ((int64_t)c * erow + (row - c * NL)) * KS + j4] = v;
+        }
       }
     }
-    stage_nodes(xr, er, mr);
+    stage_nodes(t + 1, xr, er, mr, gid);
     floatx4 acc[TPW];
 #pragma unroll
     for (int tt = 0; tt < TPW; ++tt) acc[tt] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -253,17 +274,18 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS>::NTH)) void k_element
       }
     }
     load_x(nid, xr);
-    load_c(t + 2, er, mr);
+    gid = gidn;
+    load_c(gid, er, mr);
     nid = node_of(t + 3);
+    gidn = gid_of(t + 3);
     geometry(s_G[(t + 1) & 1]);
     __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
-template <int D, int NL, int NQ, int NG>
-static hipError_t launch_gp_t(const LaunchArgs& A) {
-  if (A.cells || !A.ptab || A.mode != BH_FP32) return hipErrorInvalidValue;
+template <int D, int NL, int NQ, int NG, int KSR>
+static hipError_t launch_gp_ks(const LaunchArgs& A) {
   constexpr int CPB = 16;
   const int ng = (int)((A.ncells_total + CPB - 1) / CPB);
   if (ng < 1) return hipSuccess;
@@ -279,12 +301,35 @@ static hipError_t launch_gp_t(const LaunchArgs& A) {
     }
     dim3 grid(ng < resident ? ng : resident);
     hipLaunchKernelGGL(kern, grid, dim3(nth), 0, A.stream, A.coords, A.c2n, A.tab, A.ptab,
-                       (int)A.ncells_total, A.erow > 0 ? A.erow : NL, A.eta, A.eta_scale, A.mcoef,
-                       A.mass_scale, (float*)A.ke);
+                       (int)A.ncells_total, A.cells, (int)A.spare_cell, A.erow > 0 ? A.erow : NL,
+                       A.eta, A.eta_scale, A.mcoef, A.mass_scale, (float*)A.ke);
     return hipGetLastError();
   };
-  if (A.mass) return go(k_element_gp<D, NL, NQ, NG, true>, GPShape<D, NL, NQ, NG, true>::NTH);
-  return go(k_element_gp<D, NL, NQ, NG, false>, GPShape<D, NL, NQ, NG, false>::NTH);
+  constexpr int NT1 = GPShape<D, NL, NQ, NG, true, KSR>::NTH;
+  constexpr int NT0 = GPShape<D, NL, NQ, NG, false, KSR>::NTH;
+  if (A.cells) {
+    // cell buckets of mixed meshes (Quad4 + Tri3)
+    if constexpr (D == 2) {
+      if (A.mass) return go(k_element_gp<D, NL, NQ, NG, true, KSR, true>, NT1);
+      return go(k_element_gp<D, NL, NQ, NG, false, KSR, true>, NT0);
+    }
+    return hipErrorInvalidValue;
+  }
+  if (A.mass) return go(k_element_gp<D, NL, NQ, NG, true, KSR, false>, NT1);
+  return go(k_element_gp<D, NL, NQ, NG, false, KSR, false>, NT0);
+}
+
+// The row stride follows the space's padded dof count erow >= NL.
+template <int D, int NL, int NQ, int NG>
+static hipError_t launch_gp_t(const LaunchArgs& A) {
+  if (!A.ptab || A.mode != BH_FP32 || A.spare_cell < 0) return hipErrorInvalidValue;
+  const int ks = ke_stride(A.erow > 0 ? A.erow : NL);
+  constexpr int K0 = KeStride<NL>::value;
+  if (ks == K0) return launch_gp_ks<D, NL, NQ, NG, K0>(A);
+  if constexpr (K0 < 8) if (ks == 8) return launch_gp_ks<D, NL, NQ, NG, 8>(A);
+  if constexpr (K0 < 16) if (ks == 16) return launch_gp_ks<D, NL, NQ, NG, 16>(A);
+  if constexpr (K0 < 32) if (ks == 32) return launch_gp_ks<D, NL, NQ, NG, 32>(A);
+  return hipErrorInvalidValue;
 }
 
 struct GPEntry {
@@ -296,8 +341,15 @@ struct GPEntry {
   {BH_HEXA8, ORDER, NQ1,                                                                      \
    &launch_gp_t<3, ((ORDER) + 1) * ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1) * (NQ1), 8>}
 
+#define BH_GQ(ORDER, NQ1) \
+  {BH_QUAD4, ORDER, NQ1, &launch_gp_t<2, ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1), 4>}
+#define BH_GT(ORDER, NPTS) \
+  {BH_TRI3, ORDER, NPTS, &launch_gp_t<2, ((ORDER) + 1) * ((ORDER) + 2) / 2, NPTS, 3>}
+
 static const GPEntry kGP[] = {
-    BH_GH(1, 2), BH_GH(2, 3),
+    BH_GQ(1, 2), BH_GQ(1, 3), BH_GQ(2, 2), BH_GQ(2, 3), BH_GQ(3, 3), BH_GQ(3, 4), BH_GQ(4, 5),
+    BH_GT(1, 6), BH_GT(2, 6), BH_GT(2, 7), BH_GT(3, 12), BH_GT(4, 16),
+    BH_GH(1, 2), BH_GH(2, 2), BH_GH(2, 3),
 };
 
 bool gp_supported(int geo, int order, int nq1) {

@@ -27,6 +27,7 @@ struct LaunchArgs {
   void* ke = nullptr;
   bool ke_fp64 = true;
   int erow = 0;                    // element rows per cell in ke (the largest cell's dof count)
+  int64_t spare_cell = -1;         // first of 16 spare cells in ke (GEMM element kernel)
   const int32_t* pptr = nullptr;
   const int32_t* pairs = nullptr;  // variable-length lists, or padded to max_adj slots
   bool padded_pairs = false;

@@ -74,3 +74,38 @@ def test_mixed_determinism_and_symmetry():
     assert np.array_equal(a, b)
     A = plan.to_csc(a)
     assert abs(A - A.T).max() <= 1e-14 * abs(A).max()
+
+
+# Every (cell type, order, rule) of the FP32 GEMM element kernel, on a uniform
+# mesh (cells in mesh order) and, for quads and triangles, inside a mixed mesh
+# (cells picked through their global ids, rows at the larger element's
+# stride) where both cell types have a kernel for the order and degree.
+@pytest.mark.parametrize("shape,p,deg", [
+    ("quad", 1, 2), ("quad", 1, 4), ("quad", 2, 2), ("quad", 2, 4), ("quad", 3, 4),
+    ("quad", 3, 6), ("quad", 4, 8),
+    ("tri", 1, 4), ("tri", 2, 4), ("tri", 2, 5), ("tri", 3, 6), ("tri", 4, 8),
+    ("hexa", 1, 2), ("hexa", 2, 2), ("hexa", 2, 5),
+])
+def test_gemm_element_kernel_every_rule_fp32(shape, p, deg):
+    rng = np.random.default_rng(100 + p * 10 + deg)
+    if shape == "hexa":
+        m = B.hexa_mesh(6, 5, 4)
+        B.jitter_nodes(m, 0.15, 0.2, seed=deg)
+    elif shape == "quad":
+        m = B.rectangle_mesh(11, 9)
+        B.jitter_nodes(m, 0.2, 0.1, seed=deg)
+    else:
+        m, _ = B.mixed_tri_quad_mesh(8, block=8, p_split=1.0, seed=p)
+        B.jitter_nodes(m, 0.2, 1.0 / 8, seed=deg)
+    kappa = rng.uniform(0.5, 2.0, m.ncells)
+    plan = _plan(m, p, deg, kappa, mode="fp32")
+    e32 = _check(plan, _ref(m, p, deg, kappa), RTOL_FP32, mode="fp32")
+    msg = f"{shape} order {p} degree {deg}: uniform FP32 {e32:.2e}"
+    # mixed meshes where both cell types have a kernel for (p, degree)
+    if shape != "hexa" and (p, deg) in {(1, 4), (2, 4), (2, 5), (3, 6), (4, 8)}:
+        mm, _ = B.mixed_tri_quad_mesh(10, block=2, p_split=0.5, seed=deg)
+        B.jitter_nodes(mm, 0.2, 0.1, seed=p)
+        k2 = rng.uniform(0.5, 2.0, mm.ncells)
+        e2 = _check(_plan(mm, p, deg, k2, mode="fp32"), _ref(mm, p, deg, k2), RTOL_FP32, mode="fp32")
+        msg += f", mixed FP32 {e2:.2e}"
+    print(msg)

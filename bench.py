@@ -27,11 +27,11 @@ PEAK_HBM_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16"}
 KERNEL = {"fp64": "k_element_mfma<3,27,27,8,true,2>",
-          "fp32": "k_element_gp<3,27,27,8,true>",
+          "fp32": "k_element_gp<3,27,27,8,true,32,false>",
           "bf16x3": "k_element_mfma<3,27,27,8,true,1>"}
-GATHER = {"fp64": "k_gather_padded<double,unsigned char,27,128,8,2>",
-          "fp32": "k_gather_padded<float,unsigned char,27,128,8,2>",
-          "bf16x3": "k_gather_padded<float,unsigned char,27,128,8,2>"}
+GATHER = {"fp64": "k_gather_padded<double,unsigned char,27,128,8,2,32>",
+          "fp32": "k_gather_padded<float,unsigned char,27,128,8,2,32>",
+          "bf16x3": "k_gather_padded<float,unsigned char,27,128,8,2,32>"}
 
 
 def workload(n):

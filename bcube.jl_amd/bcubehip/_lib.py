@@ -22,9 +22,9 @@ COEF_CONST, COEF_CELL, COEF_QP = 0, 1, 2
 
 EXPORTED = [
     "bh_ctx_create", "bh_ctx_destroy", "bh_last_error", "bh_ctx_set_stream", "bh_sync",
-    "bh_ctx_set_timing", "bh_ctx_timings", "bh_ctx_timings_ex", "bh_mesh_create", "bh_mesh_set_nodes",
+    "bh_ctx_set_timing", "bh_ctx_timings", "bh_ctx_timings_ex", "bh_ctx_set_autotune", "bh_mesh_create", "bh_mesh_set_nodes",
     "bh_mesh_destroy", "bh_dofhandler", "bh_space_create", "bh_space_create_from_dofs",
-    "bh_space_info", "bh_space_dofs", "bh_space_destroy", "bh_pattern_build", "bh_pattern_info",
+    "bh_space_info", "bh_space_dofs", "bh_space_destroy", "bh_pattern_build", "bh_pattern_info", "bh_pattern_tuning", "bh_pattern_set_tuning",
     "bh_pattern_export_csc", "bh_pattern_values_device", "bh_pattern_destroy",
     "bh_assemble_bilinear", "bh_assemble_linear", "bh_quadrature_points", "bh_dg_create",
     "bh_dg_info", "bh_dg_update_geometry", "bh_dg_transport_residual", "bh_dg_minv_axpy",
@@ -72,6 +72,9 @@ def lib():
         "bh_ctx_set_timing": (C.c_int, [vp, C.c_int]),
         "bh_ctx_timings": (C.c_int, [vp, vp, C.c_int, P(C.c_int)]),
         "bh_ctx_timings_ex": (C.c_int, [vp, vp, vp, C.c_int, P(C.c_int)]),
+        "bh_ctx_set_autotune": (C.c_int, [vp, C.c_int]),
+        "bh_pattern_tuning": (C.c_int, [vp, C.c_int, vp, vp]),
+        "bh_pattern_set_tuning": (C.c_int, [vp, C.c_int, vp]),
         "bh_mesh_create": (C.c_int, [vp, C.c_int, i64, vp, i64, vp, vp, vp, P(vp)]),
         "bh_mesh_set_nodes": (C.c_int, [vp, vp, C.c_int]),
         "bh_mesh_destroy": (C.c_int, [vp]),

@@ -108,6 +108,30 @@ class BilinearPlan:
                                              md, nzval_ptr, L.DEVICE), self.ctx,
                 "bh_assemble_bilinear")
 
+    def tuning(self, mode: str | None = None):
+        """Launch shapes the autotuner chose for this plan and mode, or None
+        before tuning: element-kernel waves per block target, gather lanes per
+        row and rows per lane group (0 = default), and their times in ms."""
+        md = self.mode if mode is None else L.MODE[mode]
+        shape = np.zeros(3, dtype=np.int32)
+        ms = np.zeros(2, dtype=np.float32)
+        rc = L.lib().bh_pattern_tuning(self.handle, md, shape.ctypes.data, ms.ctypes.data)
+        if rc < 0:
+            L.check(rc, self.ctx, "bh_pattern_tuning")
+        if rc == 0:
+            return None
+        return {"elem_waves": int(shape[0]), "gather_lanes": int(shape[1]),
+                "gather_rows": int(shape[2]), "elem_ms": float(ms[0]), "gather_ms": float(ms[1])}
+
+    def set_tuning(self, elem_waves: int = 0, gather_lanes: int = 0, gather_rows: int = 0,
+                   mode: str | None = None):
+        """Fix the launch shapes for a mode (0 = default); raises BcubeHipError
+        for a gather shape that is not a candidate of this pattern."""
+        md = self.mode if mode is None else L.MODE[mode]
+        shape = np.array([elem_waves, gather_lanes, gather_rows], dtype=np.int32)
+        L.check(L.lib().bh_pattern_set_tuning(self.handle, md, shape.ctypes.data), self.ctx,
+                "bh_pattern_set_tuning")
+
     def values_device_ptr(self) -> int:
         p = C.c_void_p()
         L.check(L.lib().bh_pattern_values_device(self.handle, C.byref(p)), self.ctx,

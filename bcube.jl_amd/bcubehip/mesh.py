@@ -43,6 +43,11 @@ class BcubeBackendHIP(AbstractBcubeBackend):
     def set_timing(self, enable: bool):
         L.check(L.lib().bh_ctx_set_timing(self.ctx, int(enable)), self.ctx, "bh_ctx_set_timing")
 
+    def set_autotune(self, enable: bool):
+        """Time the candidate launch shapes on the first assembly of each plan
+        and mode, and keep the fastest (bitwise identical values)."""
+        L.check(L.lib().bh_ctx_set_autotune(self.ctx, int(enable)), self.ctx, "bh_ctx_set_autotune")
+
     def timings(self, cap: int = 4096) -> np.ndarray:
         return self.timings_ex(cap)[0]
 

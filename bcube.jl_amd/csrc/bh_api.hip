@@ -29,6 +29,7 @@ struct bh_ctx_s {
   std::map<std::tuple<int, int, int>, double*> tables;  // (geo, order, nq1) -> device table
   std::map<std::tuple<int, int, int, int>, float*> gp_tables;  // (geo, order, nq1, mass) -> P
   bool capturing = false;
+  bool autotune = false;
 };
 
 struct bh_graph_s {
@@ -81,8 +82,16 @@ struct bh_space_s {
   size_t coef_cap = 0;
 };
 
+// Launch shapes chosen by the autotuner for one pattern and mode (0: default).
+struct Tuning {
+  bool done = false;
+  int elem_waves = 0, gather_lanes = 0, gather_rows = 0;
+  float elem_ms = 0.f, gather_ms = 0.f;
+};
+
 struct bh_pattern_s {
   bh_space U = nullptr, V = nullptr;
+  Tuning tune[3];  // BH_FP64, BH_FP32, BH_BF16X3
   int dev = -1;
   HostPattern h;
   int64_t* d_rowptr = nullptr;
@@ -321,6 +330,12 @@ int bh_ctx_set_timing(bh_ctx c, int enable) {
   if (!c) return BH_ERR_ARG;
   c->timing = enable != 0;
   c->nevents = 0;
+  return BH_OK;
+}
+
+int bh_ctx_set_autotune(bh_ctx c, int enable) {
+  if (!c) return BH_ERR_ARG;
+  c->autotune = enable != 0;
   return BH_OK;
 }
 
@@ -596,6 +611,43 @@ int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out) {
   return BH_OK;
 }
 
+int bh_pattern_tuning(bh_pattern p, int mode, int32_t* shape, float* ms) {
+  if (!p || (mode != BH_FP64 && mode != BH_FP32 && mode != BH_BF16X3)) return BH_ERR_ARG;
+  const Tuning& tu = p->tune[mode == BH_FP64 ? 0 : (mode == BH_FP32 ? 1 : 2)];
+  if (shape) {
+    shape[0] = tu.elem_waves;
+    shape[1] = tu.gather_lanes;
+    shape[2] = tu.gather_rows;
+  }
+  if (ms) {
+    ms[0] = tu.elem_ms;
+    ms[1] = tu.gather_ms;
+  }
+  return tu.done ? 1 : 0;
+}
+
+int bh_pattern_set_tuning(bh_pattern p, int mode, const int32_t* shape) {
+  if (!p || !shape || (mode != BH_FP64 && mode != BH_FP32 && mode != BH_BF16X3)) return BH_ERR_ARG;
+  bh_ctx c = p->V->mesh->ctx;
+  if (shape[0] != 0 && shape[0] != 8 && shape[0] != 12)
+    return fail(c, BH_ERR_ARG, "element-kernel waves per block: 0, 8 or 12");
+  const bool dflt = shape[1] == 0 && shape[2] == 0;
+  bool ok = dflt;
+  if (!dflt && p->h.pad > 0) {
+    int lanes[4], rows[4];
+    const int n = gather_shapes(p->V->nl, p->h.max_row, p->h.offset_bytes, lanes, rows);
+    for (int k = 0; k < n; ++k) ok = ok || (lanes[k] == shape[1] && rows[k] == shape[2]);
+  }
+  if (!ok) return fail(c, BH_ERR_ARG, "gather shape is not a candidate for this pattern");
+  Tuning& tu = p->tune[mode == BH_FP64 ? 0 : (mode == BH_FP32 ? 1 : 2)];
+  tu = Tuning();
+  tu.done = true;
+  tu.elem_waves = shape[0];
+  tu.gather_lanes = shape[1];
+  tu.gather_rows = shape[2];
+  return BH_OK;
+}
+
 int bh_pattern_info(bh_pattern p, int64_t* nrows, int64_t* ncols, int64_t* nnz, int32_t* max_row) {
   if (!p) return BH_ERR_ARG;
   if (nrows) *nrows = p->h.nrows;
@@ -706,46 +758,113 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
     A.mass_scale = mass->scale;
   }
   // 1. element rows, one launch per cell type (a single one on uniform
-  //    meshes): matrix cores where an instantiation exists (FP64 too), else
-  //    the FP64 vector-ALU kernel.  Rows go to ke[(cell * nl_max + i) * 32 ...].
-  for (const GeoBucket& gb : m->buckets) {
-    const int key = rule_key(gb.geo, quad_degree);
-    double* tab = nullptr;
-    if (int r = get_tables(c, gb.geo, V->order, quad_degree, &tab)) return r;
-    A.kind = LaunchArgs::BILINEAR;
-    A.c2n = gb.d_c2n;
-    A.cells = gb.d_cells;
-    A.tab = tab;
-    A.ncells_total = gb.n;
-    A.ptab = nullptr;
-    const bool gp = mode == BH_FP32 && gp_supported(gb.geo, V->order, key);
-    A.spare_cell = nc;
-    if (gp) {
-      float* pt = nullptr;
-      if (int r = get_gp_table(c, gb.geo, V->order, quad_degree, A.mass, &pt)) return r;
-      A.ptab = pt;
+  //    meshes): the FP32 GEMM kernel, else matrix cores where an instantiation
+  //    exists (FP64, BF16x3), else the FP64 vector-ALU kernel.  Rows go to
+  //    ke[(cell * nl_max + i) * ke_stride(nl_max) ...].
+  bool any12 = false;
+  auto elements = [&](int waves, bool timed) -> int {
+    for (const GeoBucket& gb : m->buckets) {
+      const int key = rule_key(gb.geo, quad_degree);
+      double* tab = nullptr;
+      if (int r = get_tables(c, gb.geo, V->order, quad_degree, &tab)) return r;
+      LaunchArgs e = A;
+      e.kind = LaunchArgs::BILINEAR;
+      e.c2n = gb.d_c2n;
+      e.cells = gb.d_cells;
+      e.tab = tab;
+      e.ncells_total = gb.n;
+      e.spare_cell = nc;
+      e.elem_waves = waves;
+      const bool gp = mode == BH_FP32 && gp_supported(gb.geo, V->order, key);
+      if (gp) {
+        float* pt = nullptr;
+        if (int r = get_gp_table(c, gb.geo, V->order, quad_degree, A.mass, &pt)) return r;
+        e.ptab = pt;
+        int tg[2];
+        any12 = any12 || gp_wave_targets(gb.geo, V->order, key, tg) == 2;
+      }
+      if (timed) time_begin(c, 0);
+      if (gp) BH_HIP(c, launch_gp(gb.geo, V->order, key, e));
+      else if (mfma_supported(gb.geo, V->order, key)) BH_HIP(c, launch_mfma(gb.geo, V->order, key, e));
+      else BH_HIP(c, launch(gb.geo, V->order, key, e));
+      if (timed) time_end(c);
     }
-    time_begin(c, 0);
-    if (gp) BH_HIP(c, launch_gp(gb.geo, V->order, key, A));
-    else if (mfma_supported(gb.geo, V->order, key)) BH_HIP(c, launch_mfma(gb.geo, V->order, key, A));
-    else BH_HIP(c, launch(gb.geo, V->order, key, A));
-    time_end(c);
-  }
+    return BH_OK;
+  };
   // 2. deterministic row reduction into the CSR values
-  A.kind = LaunchArgs::GATHER;
-  A.pptr = p->d_pptr;
-  A.pairs = p->d_pairs;
-  A.padded_pairs = h.pad > 0;
-  A.max_adj = h.pad;
-  A.goff = p->d_goff;
-  A.offset_bytes = h.offset_bytes;
-  A.rowptr = p->d_rowptr;
-  A.nrows = h.nrows;
-  A.max_row = h.max_row;
-  A.vals = (where == BH_DEVICE && nzval) ? nzval : p->d_vals;
-  time_begin(c, 1);
-  BH_HIP(c, launch_gather(V->nl, A));
-  time_end(c);
+  LaunchArgs G = A;
+  G.kind = LaunchArgs::GATHER;
+  G.pptr = p->d_pptr;
+  G.pairs = p->d_pairs;
+  G.padded_pairs = h.pad > 0;
+  G.max_adj = h.pad;
+  G.goff = p->d_goff;
+  G.offset_bytes = h.offset_bytes;
+  G.rowptr = p->d_rowptr;
+  G.nrows = h.nrows;
+  G.max_row = h.max_row;
+  G.vals = (where == BH_DEVICE && nzval) ? nzval : p->d_vals;
+  auto gather = [&](int lanes, int rows, bool timed) -> int {
+    LaunchArgs g = G;
+    g.gather_lanes = lanes;
+    g.gather_rows = rows;
+    if (timed) time_begin(c, 1);
+    BH_HIP(c, launch_gather(V->nl, g));
+    if (timed) time_end(c);
+    return BH_OK;
+  };
+  Tuning& tu = p->tune[mode == BH_FP64 ? 0 : (mode == BH_FP32 ? 1 : 2)];
+  if (c->autotune && !tu.done && !c->capturing) {
+    // time every candidate shape on this stream (one warm launch, then the
+    // fastest of three); all of them give bitwise identical values
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    BH_HIP(c, hipEventCreate(&e0));
+    BH_HIP(c, hipEventCreate(&e1));
+    auto timed_ms = [&](auto&& run, float* ms) -> int {
+      if (int r = run()) return r;
+      *ms = 1e30f;
+      for (int rep = 0; rep < 3; ++rep) {
+        BH_HIP(c, hipEventRecord(e0, c->stream));
+        if (int r = run()) return r;
+        BH_HIP(c, hipEventRecord(e1, c->stream));
+        BH_HIP(c, hipEventSynchronize(e1));
+        float t = 0.f;
+        BH_HIP(c, hipEventElapsedTime(&t, e0, e1));
+        *ms = std::min(*ms, t);
+      }
+      return BH_OK;
+    };
+    int r = BH_OK;
+    Tuning best;
+    best.elem_ms = 1e30f;
+    for (int waves : {8, 12}) {
+      if (waves == 12 && !any12) break;
+      float ms = 0.f;
+      if ((r = timed_ms([&] { return elements(waves, false); }, &ms))) break;
+      if (ms < best.elem_ms) { best.elem_ms = ms; best.elem_waves = waves; }
+    }
+    if (!r && h.pad > 0) {
+      int lanes[4], rows[4];
+      const int n = gather_shapes(V->nl, h.max_row, h.offset_bytes, lanes, rows);
+      best.gather_ms = 1e30f;
+      for (int k = 0; k < n && !r; ++k) {
+        float ms = 0.f;
+        if ((r = timed_ms([&] { return gather(lanes[k], rows[k], false); }, &ms))) break;
+        if (ms < best.gather_ms) {
+          best.gather_ms = ms;
+          best.gather_lanes = lanes[k];
+          best.gather_rows = rows[k];
+        }
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r) return r;
+    best.done = true;
+    tu = best;
+  }
+  if (int r = elements(tu.elem_waves, true)) return r;
+  if (int r = gather(tu.gather_lanes, tu.gather_rows, true)) return r;
   if (where == BH_HOST) {
     BH_HIP(c, hipMemcpyAsync(nzval, p->d_vals, (size_t)h.nnz * sizeof(double),
                              hipMemcpyDeviceToHost, c->stream));

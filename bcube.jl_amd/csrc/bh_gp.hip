@@ -31,7 +31,7 @@ namespace bh {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-template <int D, int NL, int NQ, int NG, bool MASS, int KSR>
+template <int D, int NL, int NQ, int NG, bool MASS, int KSR, int WT = 8>
 struct GPShape {
   static constexpr int CPB = 16;                        // cells per tile (MFMA M)
   static constexpr int NS = D * (D + 1) / 2;            // entries of G
@@ -40,7 +40,7 @@ struct GPShape {
   static constexpr int KG = gp_kgroup(D, NQ, MASS);     // k-steps; lane group g holds k in [g*KG, (g+1)*KG)
   static constexpr int NP = NL * (NL + 1) / 2;          // pairs i <= j
   static constexpr int NT = gp_ntiles(NL);              // 16-wide output tiles
-  static constexpr int TPW = (NT + 7) / 8;              // tiles per wave
+  static constexpr int TPW = (NT + WT - 1) / WT;        // tiles per wave (about WT waves)
   static constexpr int W = (NT + TPW - 1) / TPW;        // waves per block
   static constexpr int NTH = W * 64;
   static constexpr int GS = 4 * KG + 4;                 // LDS row stride of Gf (floats)
@@ -65,8 +65,8 @@ __device__ __forceinline__ int gp_opaque(int v) {
 // fall into different LDS banks while rows are still read as float4.
 __device__ __forceinline__ int gp_swz(int row, int j) { return j ^ ((row & 7) << 2); }
 
-template <int D, int NL, int NQ, int NG, bool MASS, int KSR, bool IND>
-__global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS, KSR>::NTH)) void k_element_gp(
+template <int D, int NL, int NQ, int NG, bool MASS, int KSR, bool IND, int WT>
+__global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS, KSR, WT>::NTH)) void k_element_gp(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
     const double* __restrict__ tab, const float* __restrict__ ptab, int ncells,
     const int32_t* __restrict__ cells, int spare, int erow, const double* __restrict__ eta,
@@ -77,7 +77,7 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS, KSR>::NTH)) void k_el
   // ke[(gcell*erow + i)*KSR + j] for i < NL, whole KSR-float rows.  The
   // missing cells of a last, partial group write rows of the spare cells
   // spare .. spare+15, which ke must hold.
-  using S = GPShape<D, NL, NQ, NG, MASS, KSR>;
+  using S = GPShape<D, NL, NQ, NG, MASS, KSR, WT>;
   using L = TabLayout<D, NL, NQ, NG>;
   constexpr int CPB = S::CPB, KG = S::KG, TPW = S::TPW, NTH = S::NTH, GS = S::GS;
   constexpr int NC = S::NC, NS = S::NS, NT = S::NT, NP = S::NP, KS = S::KS, KSL = S::KSL;
@@ -284,8 +284,8 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS, KSR>::NTH)) void k_el
 }
 
 // ---------------------------------------------------------------------------
-template <int D, int NL, int NQ, int NG, int KSR>
-static hipError_t launch_gp_ks(const LaunchArgs& A) {
+template <int D, int NL, int NQ, int NG, int KSR, int WT>
+static hipError_t launch_gp_kw(const LaunchArgs& A) {
   constexpr int CPB = 16;
   const int ng = (int)((A.ncells_total + CPB - 1) / CPB);
   if (ng < 1) return hipSuccess;
@@ -305,18 +305,34 @@ static hipError_t launch_gp_ks(const LaunchArgs& A) {
                        A.eta, A.eta_scale, A.mcoef, A.mass_scale, (float*)A.ke);
     return hipGetLastError();
   };
-  constexpr int NT1 = GPShape<D, NL, NQ, NG, true, KSR>::NTH;
-  constexpr int NT0 = GPShape<D, NL, NQ, NG, false, KSR>::NTH;
+  constexpr int NT1 = GPShape<D, NL, NQ, NG, true, KSR, WT>::NTH;
+  constexpr int NT0 = GPShape<D, NL, NQ, NG, false, KSR, WT>::NTH;
   if (A.cells) {
     // cell buckets of mixed meshes (Quad4 + Tri3)
     if constexpr (D == 2) {
-      if (A.mass) return go(k_element_gp<D, NL, NQ, NG, true, KSR, true>, NT1);
-      return go(k_element_gp<D, NL, NQ, NG, false, KSR, true>, NT0);
+      if (A.mass) return go(k_element_gp<D, NL, NQ, NG, true, KSR, true, WT>, NT1);
+      return go(k_element_gp<D, NL, NQ, NG, false, KSR, true, WT>, NT0);
     }
     return hipErrorInvalidValue;
   }
-  if (A.mass) return go(k_element_gp<D, NL, NQ, NG, true, KSR, false>, NT1);
-  return go(k_element_gp<D, NL, NQ, NG, false, KSR, false>, NT0);
+  if (A.mass) return go(k_element_gp<D, NL, NQ, NG, true, KSR, false, WT>, NT1);
+  return go(k_element_gp<D, NL, NQ, NG, false, KSR, false, WT>, NT0);
+}
+
+// Waves-per-block target: 8, or 12 where that changes the block shape.
+template <int D, int NL, int NQ, int NG>
+constexpr bool gp_has12() {
+  constexpr int NT = gp_ntiles(NL);
+  return (NT + 7) / 8 != (NT + 11) / 12;
+}
+
+template <int D, int NL, int NQ, int NG, int KSR>
+static hipError_t launch_gp_ks(const LaunchArgs& A) {
+  // a 12-wave request on an element with no distinct 12-wave shape runs the 8-wave one
+  if constexpr (gp_has12<D, NL, NQ, NG>()) {
+    if (A.elem_waves == 12) return launch_gp_kw<D, NL, NQ, NG, KSR, 12>(A);
+  }
+  return launch_gp_kw<D, NL, NQ, NG, KSR, 8>(A);
 }
 
 // The row stride follows the space's padded dof count erow >= NL.
@@ -335,16 +351,20 @@ static hipError_t launch_gp_t(const LaunchArgs& A) {
 struct GPEntry {
   int geo, order, nq1;
   hipError_t (*fn)(const LaunchArgs&);
+  bool has12;
 };
 
 #define BH_GH(ORDER, NQ1)                                                                     \
   {BH_HEXA8, ORDER, NQ1,                                                                      \
-   &launch_gp_t<3, ((ORDER) + 1) * ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1) * (NQ1), 8>}
+   &launch_gp_t<3, ((ORDER) + 1) * ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1) * (NQ1), 8>,  \
+   gp_has12<3, ((ORDER) + 1) * ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1) * (NQ1), 8>()}
 
-#define BH_GQ(ORDER, NQ1) \
-  {BH_QUAD4, ORDER, NQ1, &launch_gp_t<2, ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1), 4>}
-#define BH_GT(ORDER, NPTS) \
-  {BH_TRI3, ORDER, NPTS, &launch_gp_t<2, ((ORDER) + 1) * ((ORDER) + 2) / 2, NPTS, 3>}
+#define BH_GQ(ORDER, NQ1)                                                            \
+  {BH_QUAD4, ORDER, NQ1, &launch_gp_t<2, ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1), 4>, \
+   gp_has12<2, ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1), 4>()}
+#define BH_GT(ORDER, NPTS)                                                             \
+  {BH_TRI3, ORDER, NPTS, &launch_gp_t<2, ((ORDER) + 1) * ((ORDER) + 2) / 2, NPTS, 3>, \
+   gp_has12<2, ((ORDER) + 1) * ((ORDER) + 2) / 2, NPTS, 3>()}
 
 static const GPEntry kGP[] = {
     BH_GQ(1, 2), BH_GQ(1, 3), BH_GQ(2, 2), BH_GQ(2, 3), BH_GQ(3, 3), BH_GQ(3, 4), BH_GQ(4, 5),
@@ -356,6 +376,16 @@ bool gp_supported(int geo, int order, int nq1) {
   for (const GPEntry& e : kGP)
     if (e.geo == geo && e.order == order && e.nq1 == nq1) return true;
   return false;
+}
+
+int gp_wave_targets(int geo, int order, int nq1, int targets[2]) {
+  for (const GPEntry& e : kGP)
+    if (e.geo == geo && e.order == order && e.nq1 == nq1) {
+      targets[0] = 8;
+      if (e.has12) targets[1] = 12;
+      return e.has12 ? 2 : 1;
+    }
+  return 0;
 }
 
 hipError_t launch_gp(int geo, int order, int nq1, const LaunchArgs& A) {

@@ -28,6 +28,12 @@ struct LaunchArgs {
   bool ke_fp64 = true;
   int erow = 0;                    // element rows per cell in ke (the largest cell's dof count)
   int64_t spare_cell = -1;         // first of 16 spare cells in ke (GEMM element kernel)
+  // launch shapes (0: the default choice): GEMM element kernel waves per
+  // block target (8 or 12), row gather lanes per row (4..32) and rows per
+  // lane group (1 or 2); every shape gives bitwise identical values
+  int elem_waves = 0;
+  int gather_lanes = 0;
+  int gather_rows = 0;
   const int32_t* pptr = nullptr;
   const int32_t* pairs = nullptr;  // variable-length lists, or padded to max_adj slots
   bool padded_pairs = false;
@@ -66,6 +72,9 @@ hipError_t launch_fill(hipStream_t s, double* p, int64_t n, double v);
 // Row gather for element rows of `nl` entries (the space's padded dof count).
 bool gather_supported(int nl);
 hipError_t launch_gather(int nl, const LaunchArgs& a);
+// Candidate (lanes per row, rows per lane group) shapes of the padded row
+// gather for elements of nl dofs; returns their count (at most 4).
+int gather_shapes(int nl, int max_row, int offset_bytes, int lanes[4], int rows[4]);
 // Matrix-core element kernels (bh_mfma.hip): BH_FP32 and BH_BF16X3, bilinear forms only.
 bool mfma_supported(int geo_celltype, int order, int nq1);
 hipError_t launch_mfma(int geo_celltype, int order, int nq1, const LaunchArgs& a);
@@ -80,6 +89,9 @@ constexpr int gp_kgroup(int dim, int nq, bool mass) {
 }
 constexpr int gp_ntiles(int nl) { return (nl * (nl + 1) / 2 + 15) / 16; }
 bool gp_supported(int geo_celltype, int order, int nq1);
+// Waves-per-block targets of the GEMM element kernel that give a different
+// block shape for this element (1 or 2 entries: 8, then 12).
+int gp_wave_targets(int geo_celltype, int order, int nq1, int targets[2]);
 hipError_t launch_gp(int geo_celltype, int order, int nq1, const LaunchArgs& a);
 
 }  // namespace bh

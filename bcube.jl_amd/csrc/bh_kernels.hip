@@ -359,12 +359,12 @@ static int grid_for(int ngroups) {
   return g < 1 ? 1 : g;
 }
 
-// Padded row gather with SG lanes per row and LDS rows of CAP entries; two
-// rows per lane group when the block's row buffers stay within 32 KB.
-template <int NL, int SG, typename T, typename OFFT, int CAP>
+// Padded row gather with SG lanes per row, LDS rows of CAP entries and RPH
+// rows per lane group.
+template <int NL, int SG, typename T, typename OFFT, int CAP, int RPH>
 static hipError_t gather_padded_go(const LaunchArgs& A) {
   constexpr int HW = kBlock / SG;
-  constexpr int RPH = HW * CAP * 8 * 2 <= 32768 ? 2 : 1;
+  static_assert(HW * RPH * CAP * 8 <= 65536, "row buffers fit in 64 KB of LDS");
   int64_t blocks = (A.nrows + HW * RPH - 1) / (HW * RPH);
   dim3 grid((unsigned)(blocks < 16384 ? (blocks < 1 ? 1 : blocks) : 16384));
 #define BH_GP(MP)                                                                              \
@@ -378,24 +378,46 @@ static hipError_t gather_padded_go(const LaunchArgs& A) {
   return hipGetLastError();
 }
 
-// Lanes per row: the element's size (4, 8, 16 or 32) while its rows fit the
-// small row buffers, else 16, else 32 lanes.
+// Rows per lane group: the requested 1 or 2, else 2 when the block's row
+// buffers stay within 32 KB.
+template <int NL, int SG, typename T, typename OFFT, int CAP>
+static hipError_t gather_padded_rph(const LaunchArgs& A) {
+  constexpr int HW = kBlock / SG;
+  constexpr int DEF = HW * CAP * 8 * 2 <= 32768 ? 2 : 1;
+  const int rph = A.gather_rows > 0 ? A.gather_rows : DEF;
+  if constexpr (HW * CAP * 8 * 2 <= 65536) {
+    if (rph == 2) return gather_padded_go<NL, SG, T, OFFT, CAP, 2>(A);
+  }
+  if (rph == 1) return gather_padded_go<NL, SG, T, OFFT, CAP, 1>(A);
+  return hipErrorInvalidValue;
+}
+
+// Lanes per row: by default the element's size (4, 8, 16 or 32) while its
+// rows fit the small row buffers, else 16, else 32; a requested 32 takes the
+// half-wave form for any element.
 template <int NL, typename T>
 static hipError_t gather_padded_t(const LaunchArgs& A) {
   constexpr int SGS = NL <= 4 ? 4 : NL <= 8 ? 8 : NL <= 16 ? 16 : 32;
   const bool b8 = A.offset_bytes == 1;
+  const bool any = A.gather_lanes == 0;
   if constexpr (SGS <= 8) {
-    if (A.max_row <= 32 && b8) return gather_padded_go<NL, SGS, T, uint8_t, 32>(A);
-    if (A.max_row <= 64 && b8) return gather_padded_go<NL, SGS, T, uint8_t, 64>(A);
+    if (any || A.gather_lanes == SGS) {
+      if (A.max_row <= 32 && b8) return gather_padded_rph<NL, SGS, T, uint8_t, 32>(A);
+      if (A.max_row <= 64 && b8) return gather_padded_rph<NL, SGS, T, uint8_t, 64>(A);
+    }
   }
   if constexpr (SGS <= 16) {
-    if (A.max_row <= 64 && b8) return gather_padded_go<NL, 16, T, uint8_t, 64>(A);
-    if (A.max_row <= 128 && b8) return gather_padded_go<NL, 16, T, uint8_t, 128>(A);
-    if (A.max_row <= 256 && b8) return gather_padded_go<NL, 16, T, uint8_t, 256>(A);
+    if (any || A.gather_lanes == 16) {
+      if (A.max_row <= 64 && b8) return gather_padded_rph<NL, 16, T, uint8_t, 64>(A);
+      if (A.max_row <= 128 && b8) return gather_padded_rph<NL, 16, T, uint8_t, 128>(A);
+      if (A.max_row <= 256 && b8) return gather_padded_rph<NL, 16, T, uint8_t, 256>(A);
+    }
   }
-  if (A.max_row <= 128 && b8) return gather_padded_go<NL, 32, T, uint8_t, 128>(A);
-  if (A.max_row <= 256 && b8) return gather_padded_go<NL, 32, T, uint8_t, 256>(A);
-  if (A.max_row <= 512) return gather_padded_go<NL, 32, T, uint16_t, 512>(A);
+  if (any || A.gather_lanes == 32) {
+    if (A.max_row <= 128 && b8) return gather_padded_rph<NL, 32, T, uint8_t, 128>(A);
+    if (A.max_row <= 256 && b8) return gather_padded_rph<NL, 32, T, uint8_t, 256>(A);
+    if (A.max_row <= 512) return gather_padded_rph<NL, 32, T, uint16_t, 512>(A);
+  }
   return hipErrorInvalidValue;
 }
 
@@ -505,6 +527,21 @@ int packed_table_size(int geo, int order, int nq1) {
 
 // Element-row lengths with a gather instantiation: quads Q1..Q4, hexahedra
 // Q1/Q2, triangles P1..P4 (mixed Quad4/Tri3 meshes use the quad length).
+int gather_shapes(int nl, int max_row, int offset_bytes, int lanes[4], int rows[4]) {
+  const int sgs = nl <= 4 ? 4 : nl <= 8 ? 8 : nl <= 16 ? 16 : 32;
+  // the default lanes per row for these rows (as gather_padded_t picks them)
+  int sg = 32;
+  if (offset_bytes == 1) {
+    if (sgs <= 8 && max_row <= 64) sg = sgs;
+    else if (sgs <= 16 && max_row <= 256) sg = 16;
+  }
+  int n = 0;
+  lanes[n] = sg; rows[n++] = 1;
+  lanes[n] = sg; rows[n++] = 2;
+  if (sg < 32) { lanes[n] = 32; rows[n++] = 1; lanes[n] = 32; rows[n++] = 2; }
+  return n;
+}
+
 bool gather_supported(int nl) {
   switch (nl) {
     case 3: case 4: case 6: case 8: case 9: case 10: case 15: case 16: case 25: case 27:

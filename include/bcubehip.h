@@ -94,6 +94,13 @@ int bh_ctx_timings(bh_ctx ctx, float* ms, int cap, int* count);
 /* Same, with the kind of each timed launch: 0 element kernel, 1 row gather,
  * 2 other (DG, linear forms, quadrature points). */
 int bh_ctx_timings_ex(bh_ctx ctx, float* ms, int32_t* kinds, int cap, int* count);
+/* Autotuning of launch shapes (off by default).  When enabled, the first
+ * bh_assemble_bilinear of a pattern in each mode outside graph capture times
+ * the candidate shapes of its kernels on the context stream (waves per block
+ * of the FP32 GEMM element kernel; lanes per CSR row and rows per lane group
+ * of the row gather) and keeps the fastest for that pattern and mode.  All
+ * candidates give bitwise identical values.  No counterpart in the reference. */
+int bh_ctx_set_autotune(bh_ctx ctx, int enable);
 
 /* ---- mesh --------------------------------------------------------------- */
 /* Mesh(nodes, celltypes, Connectivity) (src/mesh/mesh.jl:221-262).
@@ -131,6 +138,16 @@ int bh_space_destroy(bh_space space);
  * segmented reduction of the assembly. */
 int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out);
 /* max_row: the longest CSR row (entries). */
+/* Launch shapes the autotuner chose for `mode`: shape[0] element-kernel waves
+ * per block target, shape[1] gather lanes per row, shape[2] gather rows per
+ * lane group (0: the default); ms[0], ms[1] their measured times.  Returns 1
+ * once tuned, 0 before, negative on error.  No counterpart in the reference. */
+int bh_pattern_tuning(bh_pattern p, int mode, int32_t* shape, float* ms);
+/* Fix the launch shapes for `mode` (same layout as bh_pattern_tuning's shape;
+ * zeros: the defaults), e.g. from an earlier tuning; the autotuner then leaves
+ * them.  A gather shape must be one of the pattern's candidates (BH_ERR_ARG
+ * otherwise).  No counterpart in the reference. */
+int bh_pattern_set_tuning(bh_pattern p, int mode, const int32_t* shape);
 int bh_pattern_info(bh_pattern p, int64_t* nrows, int64_t* ncols, int64_t* nnz,
                     int32_t* max_row);
 /* Julia SparseMatrixCSC{Float64,Int64} structure: colptr (ncols+1), rowval

@@ -619,7 +619,12 @@ __global__ __launch_bounds__(kBlock) void k_dg_residual_reg(
   __shared__ T lds[kBlock / 64][64 * LST];
   const int lane = threadIdx.x & 63;
   T* W = lds[threadIdx.x >> 6];
-  const int64_t g0 = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63);
+  // Workgroups go round-robin to the 8 XCDs (each with its own L2); give each
+  // XCD a contiguous run of blocks, so that the neighbour cells of a block
+  // (the next rows of cells) are mostly handled, and cached, on the same XCD.
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int blk = nb % 8 == 0 ? (b % 8) * (nb / 8) + b / 8 : b;
+  const int64_t g0 = (int64_t)blk * kBlock + (threadIdx.x & ~63);
   const int64_t total = ncells * nfields;
   wave_blocks<T, N, false>(const_cast<T*>(u), W, g0, total, nfields, ndofs, lane);
   wave_sync();

@@ -177,6 +177,106 @@ def c3_mesh(B, n=866, seed=0, backend=None):
     return m, border[blocks]
 
 
+def c1_problem(B, backend=None):
+    """C1: unit square as 64x64 quads, interior nodes jittered by U(-0.2h, 0.2h)
+    (seed 0), continuous Q1, Measure degree 2 (2x2 Gauss)."""
+    n = 64
+    m = B.rectangle_mesh(n + 1, n + 1, backend=backend) if backend else B.rectangle_mesh(n + 1, n + 1)
+    B.jitter_nodes(m, 0.2, 1.0 / n, seed=0)
+    return m
+
+
+def cpu_baseline_c1(reps=200):
+    """Oracle (serial restatement), 1 thread, on the whole C1 workload: the
+    stiffness matrix (COO + sparse()) and the right-hand side, repeated."""
+    import bcubehip as B
+    from oracle import oracle as O
+
+    m = c1_problem(B, backend=B.BcubeBackendHostOnly())
+    o = O.rectangle_mesh(65, 65)
+    o.coords = m.coords.copy()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = O.assemble_bilinear(o, 1, ["stiffness"], degree=2)
+        O.assemble_linear(o, 1, "sinsin", degree=2)
+    dt = (time.perf_counter() - t0) / reps
+    return {"value": r[2].size / dt / 1e9, "unit": "GNNZ/s", "cores": 1, "kind": "port",
+            "sample": f"the whole C1 step (64^2 Q1 stiffness, COO + sparse(), and the sin*sin "
+                      f"right-hand side; each oracle call also numbers the dofs), {reps} "
+                      f"repetitions, {dt * 1e3:.3f} ms each",
+            "ms_per_step": dt * 1e3}
+
+
+def bench_c1(args, torch, B):
+    """C1, the reference's CPU-runnable case: 2D Poisson, 64x64 jittered Q1
+    quads, degree 2.  One step = ∫∇u⋅∇v into device-resident CSR values plus
+    ∫f v with f = sin(πx)sin(πy) sampled at the quadrature points into a
+    zeroed device vector, FP64 (the reference's precision)."""
+    from bcubehip import _lib as L
+
+    m = c1_problem(B)
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", 1), m)
+    V = B.TestFESpace(U)
+    dΩ = B.Measure(B.CellDomain(m), 2)
+    plan = B.BilinearPlan(lambda u, v: B.integrate(B.dot(B.grad(u), B.grad(v)), dΩ), U, V)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    backend = m.backend
+    backend.set_stream(stream.cuda_stream)
+    backend.set_autotune(True)
+    xq = B.quadrature_points(m, 2).reshape(-1, 2)
+    fq = torch.from_numpy(np.sin(np.pi * xq[:, 0]) * np.sin(np.pi * xq[:, 1])).to(dev)
+    b = torch.zeros(U.get_ndofs(), dtype=torch.float64, device=dev)
+    vals = torch.empty(plan.nnz, dtype=torch.float64, device=dev)
+    term = (L.BhTerm * 1)(L.BhTerm(L.FORM_SOURCE, L.COEF_QP, fq.data_ptr(), 1.0))
+
+    def step():
+        plan.assemble_device(vals.data_ptr(), None, "fp64")
+        b.zero_()
+        L.check(L.lib().bh_assemble_linear(U.handle, 1, term, 2, L.MODE["fp64"], b.data_ptr(),
+                                           L.DEVICE), backend.ctx, "bh_assemble_linear")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    # one step captured in a HIP graph and replayed (launch-bound size)
+    with backend.capture() as g:
+        plan.assemble_device(vals.data_ptr(), None, "fp64")
+        L.check(L.lib().bh_assemble_linear(U.handle, 1, term, 2, L.MODE["fp64"], b.data_ptr(),
+                                           L.DEVICE), backend.ctx, "bh_assemble_linear")
+    g.launch()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        g.launch()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_graph = e0.elapsed_time(e1) / args.steps
+    line = {"metric": "C1 2D Poisson Q1 64^2 assembly (matrix + right-hand side), GNNZ/s, FP64",
+            "value": plan.nnz / (ms * 1e-3) / 1e9, "unit": "GNNZ/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "C1: unit square, 64x64 Q1 quads, interior nodes jittered "
+                                   "±0.2h (seed 0), Gauss degree 2, ∫∇u⋅∇v + ∫ sin(πx)sin(πy) v",
+                       "ndofs": int(U.get_ndofs()), "nnz": int(plan.nnz), "cells": int(m.ncells)},
+            "ms_per_step_graph_replay": ms_graph,
+            "note": "launch-latency bound (4,096 cells). The graph figure replays one capture "
+                    "of the matrix and right-hand-side launches (the vector then accumulates "
+                    "over replays, as assemble_linear! does); the eager step also zeroes it."}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_c1()
+    backend.set_stream(None)
+    return line
+
+
 def cpu_baseline_c3(n_sample=400):
     """Oracle (serial restatement), 1 thread: a bounded sample of the C3 workload,
     the P2/Q2 space of an n_sample^2 mixed mesh with the same block layout."""
@@ -440,9 +540,10 @@ def main():
     ap.add_argument("--mode", default="all", choices=["all", "fp64", "fp32", "bf16x3"])
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
-                    help="c2: the headline 64^3 Q2 hexa workload; c3: mixed tri/quad order "
-                         "buckets; c4: the DG transport residual; c5: the many-instance batch")
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+                    help="c2: the headline 64^3 Q2 hexa workload; c1: the 64^2 Q1 Poisson "
+                         "matrix + right-hand side; c3: mixed tri/quad order buckets; c4: the DG "
+                         "transport residual; c5: the many-instance batch")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters against a known byte count)")
@@ -456,6 +557,9 @@ def main():
 
     import bcubehip as B
 
+    if args.config == "c1":
+        print(json.dumps(bench_c1(args, torch, B)), flush=True)
+        return
     if args.config == "c5":
         print(json.dumps(bench_c5(args, torch, B)), flush=True)
         return

@@ -14,8 +14,12 @@ from .forms import (CellDomain, FEFunction, InteriorFaceDomain, Measure, MeshCel
                     PhysicalFunction, UpwindFlux, compose, dot, get_face_normals, grad,
                     integrate, jump, nabla, side_minus, side_n, side_p, side_plus)
 from .dg import DGTransport, dg_dof_coordinates
-from .assembler import (BilinearPlan, assemble_bilinear, assemble_linear, assemble_linear_,
-                        quadrature_points)
+from .assembler import (BilinearPlan, Pattern, assemble_bilinear, assemble_linear,
+                        assemble_linear_, pattern_of, quadrature_points)
+from .dirichlet import (DirichletPlan, apply_dirichlet_to_matrix_, apply_dirichlet_to_vector_,
+                        apply_homogeneous_dirichlet_, apply_homogeneous_dirichlet_to_matrix_,
+                        apply_homogeneous_dirichlet_to_vector_, assemble_dirichlet_vector,
+                        boundary_faces, dirichlet_dofs_values, dof_coordinates)
 
 __all__ = [
     "BcubeHipError", "LIB_PATH", "lib", "AbstractBcubeBackend", "BcubeBackendHIP",
@@ -27,5 +31,8 @@ __all__ = [
     "PhysicalFunction", "UpwindFlux", "compose", "dot", "get_face_normals", "grad", "integrate",
     "jump", "nabla", "side_minus", "side_n", "side_p", "side_plus", "DGTransport",
     "dg_dof_coordinates", "BilinearPlan", "assemble_bilinear", "assemble_linear", "assemble_linear_",
-    "quadrature_points",
+    "quadrature_points", "Pattern", "pattern_of", "DirichletPlan", "apply_dirichlet_to_matrix_",
+    "apply_dirichlet_to_vector_", "apply_homogeneous_dirichlet_",
+    "apply_homogeneous_dirichlet_to_matrix_", "apply_homogeneous_dirichlet_to_vector_",
+    "assemble_dirichlet_vector", "boundary_faces", "dirichlet_dofs_values", "dof_coordinates",
 ]

@@ -25,6 +25,8 @@ EXPORTED = [
     "bh_ctx_set_timing", "bh_ctx_timings", "bh_ctx_timings_ex", "bh_ctx_set_autotune", "bh_mesh_create", "bh_mesh_set_nodes",
     "bh_mesh_destroy", "bh_dofhandler", "bh_space_create", "bh_space_create_from_dofs",
     "bh_space_info", "bh_space_dofs", "bh_space_destroy", "bh_pattern_build", "bh_pattern_info", "bh_pattern_tuning", "bh_pattern_set_tuning",
+    "bh_dirichlet_create", "bh_dirichlet_info", "bh_dirichlet_positions", "bh_dirichlet_apply_matrix",
+    "bh_dirichlet_apply_vector", "bh_dirichlet_destroy",
     "bh_pattern_export_csc", "bh_pattern_values_device", "bh_pattern_destroy",
     "bh_assemble_bilinear", "bh_assemble_linear", "bh_quadrature_points", "bh_dg_create",
     "bh_dg_info", "bh_dg_update_geometry", "bh_dg_transport_residual", "bh_dg_minv_axpy",
@@ -75,6 +77,12 @@ def lib():
         "bh_ctx_set_autotune": (C.c_int, [vp, C.c_int]),
         "bh_pattern_tuning": (C.c_int, [vp, C.c_int, vp, vp]),
         "bh_pattern_set_tuning": (C.c_int, [vp, C.c_int, vp]),
+        "bh_dirichlet_create": (C.c_int, [vp, i64, vp, P(vp)]),
+        "bh_dirichlet_info": (C.c_int, [vp, P(i64), P(i64)]),
+        "bh_dirichlet_positions": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+        "bh_dirichlet_apply_matrix": (C.c_int, [vp, C.c_double, C.c_int, vp, C.c_int]),
+        "bh_dirichlet_apply_vector": (C.c_int, [vp, vp, vp, C.c_int]),
+        "bh_dirichlet_destroy": (C.c_int, [vp]),
         "bh_mesh_create": (C.c_int, [vp, C.c_int, i64, vp, i64, vp, vp, vp, P(vp)]),
         "bh_mesh_set_nodes": (C.c_int, [vp, vp, C.c_int]),
         "bh_mesh_destroy": (C.c_int, [vp]),

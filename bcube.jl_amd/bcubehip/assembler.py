@@ -151,6 +151,34 @@ class BilinearPlan:
             pass
 
 
+class Pattern:
+    """The CSR pattern of (U, U) alone (bh_pattern_build), for edits of
+    assembled matrices such as Dirichlet conditions."""
+
+    def __init__(self, U):
+        self.U = getattr(U, "U", U)
+        self.ctx = self.U.mesh.backend.ctx
+        h = C.c_void_p()
+        L.check(L.lib().bh_pattern_build(self.U.handle, self.U.handle, C.byref(h)), self.ctx,
+                "bh_pattern_build")
+        self.handle = h
+
+    def __del__(self):
+        try:
+            L.lib().bh_pattern_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def pattern_of(U) -> Pattern:
+    """The space's (U, U) pattern, built once."""
+    U = getattr(U, "U", U)
+    p = U.__dict__.get("_pattern")
+    if p is None:
+        p = U.__dict__["_pattern"] = Pattern(U)
+    return p
+
+
 def assemble_bilinear(a, U, V, T=np.float64, mode: str = "fp64") -> sp.csc_matrix:
     """assemble_bilinear(a, U, V; T=Float64) (assembler.jl:172-186)."""
     if T is not np.float64:

@@ -119,12 +119,18 @@ class SingleFESpace:
 
 
 class TrialFESpace(SingleFESpace):
-    """TrialFESpace(fs, mesh[, :discontinuous]) (fespace.jl:274)."""
+    """TrialFESpace(fs, mesh[, :discontinuous][, dirichlet]) (fespace.jl:186-274):
+    `dirichlet` maps boundary tags to values (a number, a PhysicalFunction, an
+    FEFunction or a function of time), applied by the functions of
+    bcubehip.dirichlet."""
 
-    def __init__(self, fs: FunctionSpace, mesh: Mesh, kind: str = "continuous", *,
+    def __init__(self, fs: FunctionSpace, mesh: Mesh, kind="continuous", dirichlet=None, *,
                  isContinuous: bool | None = None):
+        if isinstance(kind, dict):  # TrialFESpace(fs, mesh, Dict(...))
+            kind, dirichlet = "continuous", kind
         cont = (kind != "discontinuous") if isContinuous is None else bool(isContinuous)
         super().__init__(fs, mesh, cont)
+        self.dirichlet = dict(dirichlet or {})
 
 
 class TestFESpace:

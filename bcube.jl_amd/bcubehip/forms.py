@@ -64,8 +64,15 @@ class FEFunction(Expr):
     `values` has shape (ndofs,) or, for several fields evaluated together,
     (nfields, ndofs)."""
 
-    def __init__(self, U, values):
+    def __init__(self, U, values, f=None):
+        # FEFunction(U, mesh, f) (fefunction.jl): the nodal interpolant of a
+        # PhysicalFunction on a continuous Lagrange space
+        if f is not None:
+            values = f
         self.space = getattr(U, "U", U)
+        if isinstance(values, PhysicalFunction):
+            from .dirichlet import dof_coordinates
+            values = np.asarray(values.f(dof_coordinates(self.space)), dtype=np.float64)
         self.values = values
 
 

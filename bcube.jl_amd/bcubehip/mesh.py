@@ -159,6 +159,9 @@ class Mesh:
                 raise ValueError("cell dimension differs from the node dimension")
         self.backend = backend if backend is not None else get_bcube_backend()
         self._handle = None
+        # boundary tags: name -> 0-based node ids, in tag order (bc_names and
+        # bc_nodes of Mesh, mesh.jl:138-148; set by the generators)
+        self.bc_nodes: dict[str, np.ndarray] = {}
 
     @property
     def nnodes(self):
@@ -237,15 +240,24 @@ def rectangle_mesh(nx, ny, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, order=1,
         xy[:, d] = ((a + b) + c) + e
     jx, jy = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1), indexing="xy")
     jx, jy = jx.ravel(), jy.ravel()
+    # boundary nodes (mesh_generator.jl:297-301, 389-392): first and last index per axis
+    gx, gy = np.meshgrid(np.arange(NX), np.arange(NY), indexing="xy")
+    gx, gy = gx.ravel(), gy.ravel()
+    bc = {"xmin": np.flatnonzero(gx == 0), "xmax": np.flatnonzero(gx == NX - 1),
+          "ymin": np.flatnonzero(gy == 0), "ymax": np.flatnonzero(gy == NY - 1)}
     if order == 1:
         n0 = jy * nx + jx
         c2n = np.stack([n0, n0 + 1, n0 + nx + 1, n0 + nx], axis=1)
-        return Mesh(xy, np.full(c2n.shape[0], L.CELLTYPE["Quad4"]), c2n, backend)
+        m = Mesh(xy, np.full(c2n.shape[0], L.CELLTYPE["Quad4"]), c2n, backend)
+        m.bc_nodes = bc
+        return m
     bx, by = 2 * jx, 2 * jy
     N = lambda dy, dx: (by + dy) * NX + bx + dx
     c2n = np.stack([N(0, 0), N(0, 2), N(2, 2), N(2, 0), N(0, 1), N(1, 2), N(2, 1), N(1, 0),
                     N(1, 1)], axis=1)
-    return Mesh(xy, np.full(c2n.shape[0], L.CELLTYPE["Quad9"]), c2n, backend)
+    m = Mesh(xy, np.full(c2n.shape[0], L.CELLTYPE["Quad9"]), c2n, backend)
+    m.bc_nodes = bc
+    return m
 
 
 def hexa_mesh(nx, ny, nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax=1.0,
@@ -262,7 +274,13 @@ def hexa_mesh(nx, ny, nz, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, zmin=0.0, zmax
     nxy = nx * ny
     c2n = np.stack([n0, n0 + 1, n0 + nx + 1, n0 + nx,
                     n0 + nxy, n0 + nxy + 1, n0 + nxy + nx + 1, n0 + nxy + nx], axis=1)
-    return Mesh(xyz, np.full(c2n.shape[0], L.CELLTYPE["Hexa8"]), c2n.astype(np.int64), backend)
+    m = Mesh(xyz, np.full(c2n.shape[0], L.CELLTYPE["Hexa8"]), c2n.astype(np.int64), backend)
+    # boundary nodes (mesh_generator.jl:490-505)
+    xr, yr, zr = x.ravel(), y.ravel(), z.ravel()
+    m.bc_nodes = {"xmin": np.flatnonzero(xr == 0), "xmax": np.flatnonzero(xr == nx - 1),
+                  "ymin": np.flatnonzero(yr == 0), "ymax": np.flatnonzero(yr == ny - 1),
+                  "zmin": np.flatnonzero(zr == 0), "zmax": np.flatnonzero(zr == nz - 1)}
+    return m
 
 
 # Reference coordinates of the 27 Hexa27 nodes in CGNS order (src/mesh/entity.jl:318-396):

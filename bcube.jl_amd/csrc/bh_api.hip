@@ -103,6 +103,23 @@ struct bh_pattern_s {
   double* d_coef = nullptr;  // staged host coefficients (2 * ncells)
 };
 
+// Dirichlet dofs of a pattern and the CSC value positions they touch
+// (apply_dirichlet_to_matrix! / _vector!, dirichlet_condition.jl:29-197).
+struct bh_dirichlet_s {
+  bh_pattern p = nullptr;
+  int dev = -1;
+  std::vector<int64_t> dofs;      // distinct, in order of first appearance
+  std::vector<int32_t> src;       // index of each dof's last occurrence in the caller's list
+  std::vector<int64_t> row_pos;   // entries (d, j) of every Dirichlet row d
+  std::vector<int64_t> col_pos;   // entries (i, d) of every Dirichlet column d
+  std::vector<int64_t> diag_pos;  // entry (d, d)
+  int64_t nsrc = 0;               // length of the caller's list (values per call)
+  int64_t *d_dofs = nullptr, *d_row = nullptr, *d_col = nullptr, *d_diag = nullptr;
+  int32_t* d_src = nullptr;
+  double* d_values = nullptr;  // host-`where` staging: the caller's values
+  double* d_b = nullptr;       // host-`where` staging: the vector
+};
+
 struct bh_dg_s {
   bh_space V = nullptr;
   int dev = -1;
@@ -645,6 +662,155 @@ int bh_pattern_set_tuning(bh_pattern p, int mode, const int32_t* shape) {
   tu.elem_waves = shape[0];
   tu.gather_lanes = shape[1];
   tu.gather_rows = shape[2];
+  return BH_OK;
+}
+
+// ---- Dirichlet conditions -------------------------------------------------
+int bh_dirichlet_create(bh_pattern p, int64_t n, const int64_t* dofs, bh_dirichlet* out) {
+  if (!p || !out || n < 0 || (n > 0 && !dofs)) return BH_ERR_ARG;
+  *out = nullptr;
+  bh_ctx c = p->V->mesh->ctx;
+  if (p->U != p->V) return fail(c, BH_ERR_UNSUPPORTED, "Dirichlet conditions need U == V");
+  if (n > INT32_MAX) return fail(c, BH_ERR_RANGE, "too many Dirichlet dofs");
+  const HostPattern& h = p->h;
+  auto d = std::make_unique<bh_dirichlet_s>();
+  d->p = p;
+  d->dev = p->dev;
+  d->nsrc = n;
+  // distinct dofs; a dof listed twice takes its last value, as the
+  // reference's sequential loop over the boundary faces does
+  std::vector<int32_t> last(h.nrows, -1);
+  for (int64_t k = 0; k < n; ++k) {
+    if (dofs[k] < 0 || dofs[k] >= h.nrows) return fail(c, BH_ERR_RANGE, "Dirichlet dof out of range");
+    if (last[dofs[k]] < 0) d->dofs.push_back(dofs[k]);
+    last[dofs[k]] = (int32_t)k;
+  }
+  auto find = [&](int64_t row, int64_t col) -> int64_t {
+    const int32_t* b = h.colval.data() + h.rowptr[row];
+    const int32_t* e = h.colval.data() + h.rowptr[row + 1];
+    const int32_t* it = std::lower_bound(b, e, (int32_t)col);
+    return (it != e && *it == col) ? (int64_t)(it - h.colval.data()) : -1;
+  };
+  for (int64_t dof : d->dofs) {
+    d->src.push_back(last[dof]);
+    // the caller sees CSC values: column j's entries at colptr[j] = rowptr[j]
+    // (the pattern is symmetric), so row d is entry d of every column j of
+    // row d's pattern, and column d is the contiguous run of row d
+    for (int64_t q = h.rowptr[dof]; q < h.rowptr[dof + 1]; ++q) {
+      const int64_t pos = find(h.colval[q], dof);
+      if (pos < 0) return fail(c, BH_ERR_ARG, "internal: pattern is not symmetric");
+      d->row_pos.push_back(pos);
+      d->col_pos.push_back(q);
+    }
+    d->diag_pos.push_back(find(dof, dof));
+  }
+  if (c) {
+    if (int r = set_device(c)) return r;
+    auto up = [&](auto** dst, const auto& v) -> int {
+      BH_HIP(c, dalloc(dst, v.size()));
+      if (!v.empty())
+        BH_HIP(c, hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
+      return BH_OK;
+    };
+    if (int r = up(&d->d_dofs, d->dofs)) return r;
+    if (int r = up(&d->d_src, d->src)) return r;
+    if (int r = up(&d->d_row, d->row_pos)) return r;
+    if (int r = up(&d->d_col, d->col_pos)) return r;
+    if (int r = up(&d->d_diag, d->diag_pos)) return r;
+  }
+  *out = d.release();
+  return BH_OK;
+}
+
+int bh_dirichlet_info(bh_dirichlet d, int64_t* ndofs, int64_t* npos) {
+  if (!d) return BH_ERR_ARG;
+  if (ndofs) *ndofs = (int64_t)d->dofs.size();
+  if (npos) *npos = (int64_t)d->row_pos.size();
+  return BH_OK;
+}
+
+int bh_dirichlet_positions(bh_dirichlet d, int64_t* dofs, int32_t* src, int64_t* row_pos,
+                           int64_t* col_pos, int64_t* diag_pos) {
+  if (!d) return BH_ERR_ARG;
+  auto out = [](auto* dst, const auto& v) {
+    if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+  };
+  out(dofs, d->dofs);
+  out(src, d->src);
+  out(row_pos, d->row_pos);
+  out(col_pos, d->col_pos);
+  out(diag_pos, d->diag_pos);
+  return BH_OK;
+}
+
+int bh_dirichlet_apply_matrix(bh_dirichlet d, double diag, int homogeneous, double* nzval,
+                              int where) {
+  if (!d || !nzval) return BH_ERR_ARG;
+  bh_pattern p = d->p;
+  bh_ctx c = p->V->mesh->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "applying Dirichlet conditions needs a pattern with a context");
+  if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
+  if (where == BH_HOST && c->capturing)
+    return fail(c, BH_ERR_ARG, "host-`where` calls cannot be captured in a graph");
+  if (int r = set_device(c)) return r;
+  const size_t bytes = (size_t)p->h.nnz * sizeof(double);
+  double* v = nzval;
+  if (where == BH_HOST) {
+    v = p->d_vals;
+    BH_HIP(c, hipMemcpyAsync(v, nzval, bytes, hipMemcpyHostToDevice, c->stream));
+  }
+  BH_HIP(c, launch_set_at(c->stream, v, d->d_row, (int64_t)d->row_pos.size(), 0.0));
+  if (homogeneous) BH_HIP(c, launch_set_at(c->stream, v, d->d_col, (int64_t)d->col_pos.size(), 0.0));
+  BH_HIP(c, launch_set_at(c->stream, v, d->d_diag, (int64_t)d->diag_pos.size(), diag));
+  if (where == BH_HOST) {
+    BH_HIP(c, hipMemcpyAsync(nzval, v, bytes, hipMemcpyDeviceToHost, c->stream));
+    BH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return BH_OK;
+}
+
+int bh_dirichlet_apply_vector(bh_dirichlet d, const double* values, double* b, int where) {
+  if (!d || !b) return BH_ERR_ARG;
+  bh_pattern p = d->p;
+  bh_ctx c = p->V->mesh->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "applying Dirichlet conditions needs a pattern with a context");
+  if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
+  if (where == BH_HOST && c->capturing)
+    return fail(c, BH_ERR_ARG, "host-`where` calls cannot be captured in a graph");
+  if (int r = set_device(c)) return r;
+  const int64_t nd = p->h.nrows;
+  double* db = b;
+  const double* dv = values;
+  if (where == BH_HOST) {
+    if (!d->d_b) {
+      BH_HIP(c, dalloc(&d->d_b, (size_t)nd));
+      BH_HIP(c, dalloc(&d->d_values, (size_t)std::max<int64_t>(d->nsrc, 1)));
+    }
+    db = d->d_b;
+    BH_HIP(c, hipMemcpyAsync(db, b, (size_t)nd * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (values) {
+      BH_HIP(c, hipMemcpyAsync(d->d_values, values, (size_t)d->nsrc * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
+      dv = d->d_values;
+    }
+  }
+  BH_HIP(c, launch_set_dofs(c->stream, db, d->d_dofs, d->d_src, dv, (int64_t)d->dofs.size()));
+  if (where == BH_HOST) {
+    BH_HIP(c, hipMemcpyAsync(b, db, (size_t)nd * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    BH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return BH_OK;
+}
+
+int bh_dirichlet_destroy(bh_dirichlet d) {
+  if (!d) return BH_ERR_ARG;
+  if (d->dev >= 0) {
+    (void)hipSetDevice(d->dev);
+    for (void* q : {(void*)d->d_dofs, (void*)d->d_row, (void*)d->d_col, (void*)d->d_diag,
+                    (void*)d->d_src, (void*)d->d_values, (void*)d->d_b})
+      (void)hipFree(q);
+  }
+  delete d;
   return BH_OK;
 }
 

@@ -69,6 +69,11 @@ bool kernel_supported(int geo_celltype, int order, int nq1);
 int packed_table_size(int geo_celltype, int order, int nq1);
 hipError_t launch(int geo_celltype, int order, int nq1, const LaunchArgs& a);
 hipError_t launch_fill(hipStream_t s, double* p, int64_t n, double v);
+// Dirichlet edits (bh_dirichlet_*): v[pos[i]] = val, and b[dofs[i]] =
+// values ? values[src[i]] : 0.
+hipError_t launch_set_at(hipStream_t s, double* v, const int64_t* pos, int64_t n, double val);
+hipError_t launch_set_dofs(hipStream_t s, double* b, const int64_t* dofs, const int32_t* src,
+                           const double* values, int64_t n);
 // Row gather for element rows of `nl` entries (the space's padded dof count).
 bool gather_supported(int nl);
 hipError_t launch_gather(int nl, const LaunchArgs& a);

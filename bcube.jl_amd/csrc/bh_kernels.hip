@@ -566,6 +566,40 @@ hipError_t launch_gather(int nl, const LaunchArgs& A) {
   return hipErrorInvalidValue;
 }
 
+// Dirichlet edits: v[pos[i]] = val; b[dofs[i]] = values ? values[src[i]] : 0.
+// Positions and dofs within one call are distinct, so the writes do not race.
+__global__ void k_set_at(double* __restrict__ v, const int64_t* __restrict__ pos, int64_t n,
+                         double val) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v[pos[i]] = val;
+}
+__global__ void k_set_dofs(double* __restrict__ b, const int64_t* __restrict__ dofs,
+                           const int32_t* __restrict__ src, const double* __restrict__ values,
+                           int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[dofs[i]] = values ? values[src[i]] : 0.0;
+}
+
+static dim3 grid_1d(int64_t n) {
+  int64_t blocks = (n + 255) / 256;
+  return dim3((unsigned)(blocks > 4096 ? 4096 : (blocks < 1 ? 1 : blocks)));
+}
+
+hipError_t launch_set_at(hipStream_t s, double* v, const int64_t* pos, int64_t n, double val) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_set_at, grid_1d(n), dim3(256), 0, s, v, pos, n, val);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_dofs(hipStream_t s, double* b, const int64_t* dofs, const int32_t* src,
+                           const double* values, int64_t n) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_set_dofs, grid_1d(n), dim3(256), 0, s, b, dofs, src, values, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill(hipStream_t s, double* p, int64_t n, double v) {
   int64_t blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;

@@ -75,6 +75,7 @@ typedef struct bh_ctx_s* bh_ctx;
 typedef struct bh_mesh_s* bh_mesh;
 typedef struct bh_space_s* bh_space;
 typedef struct bh_pattern_s* bh_pattern;
+typedef struct bh_dirichlet_s* bh_dirichlet;
 
 /* ---- context ------------------------------------------------------------ */
 /* One context per host thread.  Replaces the serial backend object
@@ -175,6 +176,36 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
  * (eval_point.jl:107-111), ncells*nq*dim FP64, for sampling a
  * PhysicalFunction on the host; *nq receives the points per cell. */
 int bh_quadrature_points(bh_mesh mesh, int quad_degree, double* xq, int where, int32_t* nq);
+
+/* ---- Dirichlet conditions ------------------------------------------------ */
+/* The device side of apply_dirichlet_to_matrix! / apply_dirichlet_to_vector! /
+ * apply_homogeneous_dirichlet_to_matrix! / _vector! (src/assembler/
+ * dirichlet_condition.jl:29-197).  The caller (as the reference's
+ * _apply_dirichlet!, :199-330) finds the dofs on the tagged boundary faces and
+ * the condition's values there; this computes, once, the CSC value positions
+ * of those dofs' rows, columns and diagonal entries in pattern p (U == V) and
+ * applies the edits in place.  `dofs` are 0-based, in any order; a dof listed
+ * twice takes its last value.  The pattern must outlive the handle.  With a
+ * host-only pattern (no context) only the position queries work. */
+int bh_dirichlet_create(bh_pattern p, int64_t n, const int64_t* dofs, bh_dirichlet* out);
+/* Distinct Dirichlet dofs, and the number of row (= column) positions. */
+int bh_dirichlet_info(bh_dirichlet d, int64_t* ndofs, int64_t* npos);
+/* dofs[ndofs] (first-appearance order), src[ndofs] (index in the caller's
+ * list whose value each dof takes), row_pos[npos], col_pos[npos] (CSC value
+ * indices of entries (d, j) and (i, d)), diag_pos[ndofs]; NULL skips. */
+int bh_dirichlet_positions(bh_dirichlet d, int64_t* dofs, int32_t* src, int64_t* row_pos,
+                           int64_t* col_pos, int64_t* diag_pos);
+/* apply_dirichlet_to_matrix! (:161-197): every stored entry of each Dirichlet
+ * row set to zero, the diagonal to `diag` (columns kept); homogeneous != 0:
+ * apply_homogeneous_dirichlet_to_matrix! (:53-89), rows and columns zeroed.
+ * nzval: the CSC values of p, host or device by `where`. */
+int bh_dirichlet_apply_matrix(bh_dirichlet d, double diag, int homogeneous, double* nzval,
+                              int where);
+/* apply_dirichlet_to_vector! (:125-145): b[dof] = values[k], values indexed
+ * like the dofs given to bh_dirichlet_create; values == NULL:
+ * apply_homogeneous_dirichlet_to_vector! (:29-40), b[dof] = 0. */
+int bh_dirichlet_apply_vector(bh_dirichlet d, const double* values, double* b, int where);
+int bh_dirichlet_destroy(bh_dirichlet d);
 
 /* ---- discontinuous Galerkin transport ------------------------------------- */
 /* The linear form of an upwind DG discretisation of ∂u/∂t + ∇⋅(c u) = 0,

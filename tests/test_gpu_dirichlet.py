@@ -119,3 +119,32 @@ def test_homogeneous_system_symmetric_with_unit_rows():
     assert np.abs(D - D.T).max() <= 1e-14 * np.abs(D).max()
     assert np.all(D[d][:, d] == np.eye(d.size)) and np.all(b[d] == 0.0)
     assert np.all(D[d].sum(axis=1) == 1.0)
+
+
+def test_reference_heat_solver_3d_q2():
+    """The reference's 'Heat solver 3d', Q2 part (test_assembler.jl:457-509):
+    hexa_mesh(5,5,5) on [-0.5,0.5]^3, ∫η∇u⋅∇v = ∫q v with η = 100, q = 1500,
+    T = 260 on xmin, solved as AffineFESystem.solve! does
+    (affine_fe_system.jl:100-119: d = Dirichlet vector, b0 = b - A d,
+    homogeneous conditions on (A, b0), x + d); the cell-centre values against
+    260 + (q/η) s (1 - s/2), s = x + 1/2, within the reference's 2e-14."""
+    m = B.hexa_mesh(5, 5, 5, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5)
+    lam, q = 100.0, 1500.0
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", 2), m, {"xmin": 260.0})
+    V = B.TestFESpace(U)
+    dΩ = B.Measure(B.CellDomain(m), 5)
+    A = B.assemble_bilinear(lambda u, v: B.integrate(lam * B.dot(B.grad(u), B.grad(v)), dΩ), U, V)
+    b = B.assemble_linear(lambda v: B.integrate(q * v, dΩ), V)
+    d = B.assemble_dirichlet_vector(U, V, m)
+    b0 = b - A @ d
+    B.apply_homogeneous_dirichlet_(A, b0, U, V, m)
+    x = spla.spsolve(A.tocsc(), b0) + d
+    # var_on_centers: the Q2 value at ξ = 0 is the cell-centre dof's value
+    centre = [U.get_dofs(c)[13] - 1 for c in range(m.ncells)]
+    Tcn = x[centre]
+    xc = np.array([m.nodes()[m.cell_nodes(c)].mean(axis=0) for c in range(m.ncells)])
+    s = xc[:, 0] + 0.5
+    Tca = 260.0 + (q / lam) * s * (1.0 - 0.5 * s)
+    err = np.abs(Tcn - Tca).max() / np.abs(Tca).max()
+    print("heat solver 3d Q2 rel err", err)
+    assert err < 2.0e-14

@@ -20,6 +20,8 @@ from .dirichlet import (DirichletPlan, apply_dirichlet_to_matrix_, apply_dirichl
                         apply_homogeneous_dirichlet_, apply_homogeneous_dirichlet_to_matrix_,
                         apply_homogeneous_dirichlet_to_vector_, assemble_dirichlet_vector,
                         boundary_faces, dirichlet_dofs_values, dof_coordinates)
+from .io import nodes_gmsh2cgns, read_mesh
+from .projection import projection_l2_
 
 __all__ = [
     "BcubeHipError", "LIB_PATH", "lib", "AbstractBcubeBackend", "BcubeBackendHIP",
@@ -35,4 +37,5 @@ __all__ = [
     "apply_dirichlet_to_vector_", "apply_homogeneous_dirichlet_",
     "apply_homogeneous_dirichlet_to_matrix_", "apply_homogeneous_dirichlet_to_vector_",
     "assemble_dirichlet_vector", "boundary_faces", "dirichlet_dofs_values", "dof_coordinates",
+    "nodes_gmsh2cgns", "read_mesh", "projection_l2_",
 ]

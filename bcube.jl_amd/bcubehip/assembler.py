@@ -250,11 +250,12 @@ def assemble_linear(l, V, T=np.float64, mode: str = "fp64") -> np.ndarray:
 
 def quadrature_points(mesh, degree: int) -> np.ndarray:
     """Physical quadrature points, (ncells, nq, dim), computed on the GPU."""
-    from .forms import _wrap  # noqa: F401  (keeps import order simple)
     nq = C.c_int32()
-    n1 = (max(degree, 1) + 2) // 2
-    nq_guess = n1 ** mesh.dim
-    out = np.zeros(mesh.ncells * nq_guess * mesh.dim, dtype=np.float64)
+    ctx = mesh.backend.ctx
+    L.check(L.lib().bh_quadrature_points(mesh.handle, degree, None, L.HOST, C.byref(nq)), ctx,
+            "bh_quadrature_points")
+    out = np.zeros(mesh.ncells * nq.value * mesh.dim, dtype=np.float64)
+    got = C.c_int32()
     L.check(L.lib().bh_quadrature_points(mesh.handle, degree, out.ctypes.data, L.HOST,
-                                         C.byref(nq)), mesh.backend.ctx, "bh_quadrature_points")
-    return out.reshape(mesh.ncells, nq.value, mesh.dim)
+                                         C.byref(got)), ctx, "bh_quadrature_points")
+    return out.reshape(mesh.ncells, got.value, mesh.dim)

@@ -64,16 +64,29 @@ class FEFunction(Expr):
     `values` has shape (ndofs,) or, for several fields evaluated together,
     (nfields, ndofs)."""
 
-    def __init__(self, U, values, f=None):
+    def __init__(self, U, values=None, f=None):
         # FEFunction(U, mesh, f) (fefunction.jl): the nodal interpolant of a
-        # PhysicalFunction on a continuous Lagrange space
+        # PhysicalFunction on a continuous Lagrange space; FEFunction(U) is zero
         if f is not None:
             values = f
         self.space = getattr(U, "U", U)
+        if values is None:
+            values = np.zeros(self.space.get_ndofs(), dtype=np.float64)
         if isinstance(values, PhysicalFunction):
             from .dirichlet import dof_coordinates
             values = np.asarray(values.f(dof_coordinates(self.space)), dtype=np.float64)
         self.values = values
+
+    def get_dof_values(self):
+        """get_dof_values(u) (fefunction.jl)."""
+        return self.values
+
+    def set_dof_values_(self, values):
+        """set_dof_values!(u, values) (fefunction.jl): copies into u's values."""
+        values = np.asarray(values, dtype=np.float64)
+        if values.shape != np.shape(self.values):
+            raise ValueError("dof values have the wrong shape")
+        self.values = values.copy()
 
 
 class FaceNormal(Expr):

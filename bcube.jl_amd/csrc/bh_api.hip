@@ -1441,7 +1441,7 @@ int bh_graph_destroy(bh_graph g) {
 }
 
 int bh_quadrature_points(bh_mesh m, int quad_degree, double* xq, int where, int32_t* nq_out) {
-  if (!m || !xq) return BH_ERR_ARG;
+  if (!m) return BH_ERR_ARG;
   bh_ctx c = m->ctx;
   if (!c) return fail(c, BH_ERR_ARG, "needs a mesh created with a context");
   if (c->capturing && where == BH_HOST)
@@ -1452,10 +1452,15 @@ int bh_quadrature_points(bh_mesh m, int quad_degree, double* xq, int where, int3
   int order = 1;  // any instantiated order carries the geometric tables for nq1
   while (order <= 4 && !kernel_supported(m->geo, order, nq1)) ++order;
   if (order > 4) return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this quadrature");
+  const int nq = m->geo == BH_TRI3 ? nq1 : nq1 * nq1 * (m->h.dim == 3 ? nq1 : 1);
+  if (!xq) {  // size query
+    if (!nq_out) return BH_ERR_ARG;
+    *nq_out = nq;
+    return BH_OK;
+  }
   if (int r = set_device(c)) return r;
   double* tab = nullptr;
   if (int r = get_tables(c, m->geo, order, quad_degree, &tab)) return r;
-  int nq = m->geo == BH_TRI3 ? nq1 : nq1 * nq1 * (m->h.dim == 3 ? nq1 : 1);
   int64_t n = m->h.ncells * nq * m->h.dim;
   double* d = xq;
   if (where == BH_HOST) BH_HIP(c, dalloc(&d, (size_t)n));

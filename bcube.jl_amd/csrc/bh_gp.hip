@@ -368,7 +368,8 @@ struct GPEntry {
 
 static const GPEntry kGP[] = {
     BH_GQ(1, 2), BH_GQ(1, 3), BH_GQ(2, 2), BH_GQ(2, 3), BH_GQ(3, 3), BH_GQ(3, 4), BH_GQ(4, 5),
-    BH_GT(1, 6), BH_GT(2, 6), BH_GT(2, 7), BH_GT(3, 12), BH_GT(4, 16),
+    BH_GT(1, 1), BH_GT(1, 3), BH_GT(1, 4), BH_GT(1, 6),
+    BH_GT(2, 3), BH_GT(2, 4), BH_GT(2, 6), BH_GT(2, 7), BH_GT(3, 12), BH_GT(4, 16),
     BH_GH(1, 2), BH_GH(2, 2), BH_GH(2, 3),
 };
 

@@ -122,15 +122,26 @@ void lagrange_1d(int order, double t, double* v, double* dv) {
   }
 }
 
-// Triangle rules of Measure degree 4..8: the GLTRI6/7/12/16 tables of the
-// reference (quadrature.jl:289-377, Witherden & Vincent), reference triangle
-// (0,0),(1,0),(0,1).  Degrees 1..3 use FEMQuad tables the reference does not
-// vendor: no rule (empty).
+// Triangle rules on the reference triangle (0,0),(1,0),(0,1).  Degrees 1..3:
+// the GLTRI1/GLTRI3/GLTRI4 rules of the FEMQuad 0.3 dependency
+// (quadrature.jl:280-288): the centroid, the three interior points
+// (2/3,1/6),(1/6,2/3),(1/6,1/6), and Strang & Fix's four-point rule with its
+// negative centroid weight.  Degrees 4..8: the GLTRI6/7/12/16 tables of the
+// reference (quadrature.jl:289-377, Witherden & Vincent).
 static void triangle_rule(int degree, std::vector<double>& w, std::vector<double>& x) {
   w.clear();
   x.clear();
   auto pt = [&](double a, double b) { x.push_back(a); x.push_back(b); };
-  if (degree == 4) {
+  if (degree == 1) {
+    w = {0.5};
+    pt(1.0 / 3, 1.0 / 3);
+  } else if (degree == 2) {
+    w = {1.0 / 6, 1.0 / 6, 1.0 / 6};
+    pt(2.0 / 3, 1.0 / 6); pt(1.0 / 6, 2.0 / 3); pt(1.0 / 6, 1.0 / 6);
+  } else if (degree == 3) {
+    w = {-27.0 / 96, 25.0 / 96, 25.0 / 96, 25.0 / 96};
+    pt(1.0 / 3, 1.0 / 3); pt(0.6, 0.2); pt(0.2, 0.6); pt(0.2, 0.2);
+  } else if (degree == 4) {
     const double P1 = 0.22338158967801146569500700843312 / 2;
     const double P2 = 0.10995174365532186763832632490021 / 2;
     const double A = 0.44594849091596488631832925388305;

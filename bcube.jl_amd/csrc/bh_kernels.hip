@@ -504,7 +504,9 @@ static const Entry kTable[] = {
     BH_ENTRY_Q(1, 1), BH_ENTRY_Q(1, 2), BH_ENTRY_Q(1, 3), BH_ENTRY_Q(2, 2), BH_ENTRY_Q(2, 3),
     BH_ENTRY_Q(2, 4), BH_ENTRY_Q(3, 3), BH_ENTRY_Q(3, 4), BH_ENTRY_Q(4, 5),
     BH_ENTRY_H(1, 1), BH_ENTRY_H(1, 2), BH_ENTRY_H(1, 3), BH_ENTRY_H(2, 2), BH_ENTRY_H(2, 3),
-    BH_ENTRY_T(1, 6), BH_ENTRY_T(2, 6), BH_ENTRY_T(2, 7), BH_ENTRY_T(3, 12), BH_ENTRY_T(4, 16),
+    BH_ENTRY_T(1, 1), BH_ENTRY_T(1, 3), BH_ENTRY_T(1, 4), BH_ENTRY_T(1, 6),
+    BH_ENTRY_T(2, 3), BH_ENTRY_T(2, 4), BH_ENTRY_T(2, 6), BH_ENTRY_T(2, 7), BH_ENTRY_T(3, 12),
+    BH_ENTRY_T(4, 16),
 };
 
 bool kernel_supported(int geo, int order, int nq1) {

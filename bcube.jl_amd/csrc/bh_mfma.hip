@@ -330,7 +330,8 @@ struct MfmaEntry {
 
 static const MfmaEntry kMfma[] = {
     BH_MQ(1, 2), BH_MQ(1, 3), BH_MQ(2, 2), BH_MQ(2, 3), BH_MQ(3, 3), BH_MQ(3, 4), BH_MQ(4, 5),
-    BH_MT(1, 6), BH_MT(2, 6), BH_MT(2, 7), BH_MT(3, 12), BH_MT(4, 16),
+    BH_MT(1, 1), BH_MT(1, 3), BH_MT(1, 4), BH_MT(1, 6),
+    BH_MT(2, 3), BH_MT(2, 4), BH_MT(2, 6), BH_MT(2, 7), BH_MT(3, 12), BH_MT(4, 16),
     BH_MH(1, 2), BH_MH(2, 2), BH_MH(2, 3),
 };
 

@@ -174,7 +174,8 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
                        double* b, int where);
 /* Physical coordinates x = F(ξ_q) of every quadrature point
  * (eval_point.jl:107-111), ncells*nq*dim FP64, for sampling a
- * PhysicalFunction on the host; *nq receives the points per cell. */
+ * PhysicalFunction on the host; *nq receives the points per cell.  With
+ * xq NULL only *nq is set (the size query). */
 int bh_quadrature_points(bh_mesh mesh, int quad_degree, double* xq, int where, int32_t* nq);
 
 /* ---- Dirichlet conditions ------------------------------------------------ */

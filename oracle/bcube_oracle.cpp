@@ -261,12 +261,25 @@ static void line_rule(int degree, int qtype, std::vector<double>& w, std::vector
 
 // Triangle rules of Measure degree 4..8 (quadrature.jl:289-377: GLTRI6, GLTRI7,
 // GLTRI12 and GLTRI16 of Witherden & Vincent, rescaled to the reference triangle
-// (0,0),(1,0),(0,1)).  Degrees 1..3 come from FEMQuad's GLTRI1/3/4 tables
-// (quadrature.jl:280-288), a dependency that is not vendored: not restated.
+// (0,0),(1,0),(0,1)).  Degrees 1..3 come from the GLTRI1/3/4 tables of FEMQuad
+// 0.3 (quadrature.jl:280-288; the dependency is not vendored): restated from its
+// published rules -- centroid (weight 1/2); three interior points (2/3,1/6),
+// (1/6,2/3), (1/6,1/6), weights 1/6; Strang & Fix's 4-point rule, -27/96 at the
+// centroid and 25/96 at (3/5,1/5), (1/5,3/5), (1/5,1/5).  Pinned by exactness
+// and by moments that tell these points from the other rules of those degrees.
 static void triangle_rule(int degree, std::vector<double>& w, std::vector<double>& x) {
   w.clear(); x.clear();
   auto pt = [&](double a, double b) { x.push_back(a); x.push_back(b); };
-  if (degree == 4) {
+  if (degree == 1) {
+    w = {0.5};
+    pt(1.0 / 3, 1.0 / 3);
+  } else if (degree == 2) {
+    w = {1.0 / 6, 1.0 / 6, 1.0 / 6};
+    pt(2.0 / 3, 1.0 / 6); pt(1.0 / 6, 2.0 / 3); pt(1.0 / 6, 1.0 / 6);
+  } else if (degree == 3) {
+    w = {-27.0 / 96, 25.0 / 96, 25.0 / 96, 25.0 / 96};
+    pt(1.0 / 3, 1.0 / 3); pt(0.6, 0.2); pt(0.2, 0.6); pt(0.2, 0.2);
+  } else if (degree == 4) {
     double P1 = 0.22338158967801146569500700843312 / 2;
     double P2 = 0.10995174365532186763832632490021 / 2;
     double A = 0.44594849091596488631832925388305;
@@ -317,8 +330,7 @@ static void triangle_rule(int degree, std::vector<double>& w, std::vector<double
       w.push_back(T[k][2] / 4);
     }
   } else {
-    throw std::runtime_error("oracle: triangle rule of degree " + std::to_string(degree) +
-                             " is not restated (degrees 1-3 come from FEMQuad)");
+    throw std::runtime_error("oracle: no triangle rule of degree " + std::to_string(degree));
   }
 }
 

@@ -54,7 +54,21 @@ def test_triangle_mesh_vs_oracle(p, deg):
     print(f"Tri P{p} deg {deg}: rel err {_check(plan, _ref(m, p, deg, kappa), RTOL):.2e}")
 
 
-@pytest.mark.parametrize("p,deg", [(1, 4), (2, 4), (3, 6), (4, 8)])
+@pytest.mark.parametrize("p,deg", [(1, 1), (1, 2), (1, 3), (2, 2), (2, 3)])
+def test_triangle_low_degree_rules_fp64_and_fp32(p, deg):
+    """Measure degrees 1-3 on triangles (FEMQuad's 1-, 3- and 4-point rules,
+    one with a negative weight) through the FP64 and FP32 element kernels."""
+    m, _ = B.mixed_tri_quad_mesh(8, block=8, p_split=1.0, seed=1)
+    B.jitter_nodes(m, 0.2, 1.0 / 8, seed=p + 10)
+    kappa = np.random.default_rng(p).uniform(0.5, 2.0, m.ncells)
+    plan = _plan(m, p, deg, kappa)
+    ref = _ref(m, p, deg, kappa)
+    e64 = _check(plan, ref, RTOL)
+    e32 = _check(plan, ref, RTOL_FP32, mode="fp32")
+    print(f"Tri P{p} deg {deg}: FP64 {e64:.2e}, FP32 {e32:.2e}")
+
+
+@pytest.mark.parametrize("p,deg", [(1, 2), (1, 4), (2, 3), (2, 4), (3, 6), (4, 8)])
 def test_mixed_mesh_vs_oracle_fp64_and_fp32(p, deg):
     m, _ = B.mixed_tri_quad_mesh(12, block=3, p_split=0.5, seed=p)
     B.jitter_nodes(m, 0.2, 1.0 / 12, seed=p + 5)

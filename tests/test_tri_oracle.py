@@ -20,7 +20,8 @@ from oracle import oracle as O
 HOST = B.BcubeBackendHostOnly()
 
 
-@pytest.mark.parametrize("degree,npts", [(4, 6), (5, 7), (6, 12), (7, 16), (8, 16)])
+@pytest.mark.parametrize("degree,npts", [(1, 1), (2, 3), (3, 4), (4, 6), (5, 7), (6, 12),
+                                         (7, 16), (8, 16)])
 def test_triangle_rules_exact(degree, npts):
     w, x = O.quadrature("Triangle", degree)
     assert w.size == npts
@@ -32,9 +33,24 @@ def test_triangle_rules_exact(degree, npts):
             assert got == pytest.approx(want, rel=1e-13, abs=1e-16), (a, b)
 
 
-def test_femquad_degrees_are_refused():
+def test_femquad_rules_are_the_published_ones():
+    """FEMQuad 0.3's GLTRI1/3/4 (quadrature.jl:280-288): exactness alone also
+    admits other rules of these degrees (the edge-midpoint rule of degree 2,
+    a 4-point rule with positive weights), so the first moment beyond each
+    rule's degree tells them apart: centroid (1/3)^2/2 for x^2; interior
+    points (8/27 + 2/216)/6 = 11/216 for x^3 (the edge midpoints give 1/24);
+    Strang & Fix (-27/81 + 25*(0.6^4 + 2*0.2^4))/96 for x^4 with the
+    negative centroid weight."""
+    w, x = O.quadrature("Triangle", 1)
+    assert np.sum(w * x[:, 0] ** 2) == pytest.approx(1 / 18, rel=1e-14)
+    w, x = O.quadrature("Triangle", 2)
+    assert np.all(x > 0) and np.sum(w * x[:, 0] ** 3) == pytest.approx(11 / 216, rel=1e-14)
+    w, x = O.quadrature("Triangle", 3)
+    assert w.min() < 0
+    want = (-27 / 81 + 25 * (0.6 ** 4 + 2 * 0.2 ** 4)) / 96
+    assert np.sum(w * x[:, 0] ** 4) == pytest.approx(want, rel=1e-14)
     with pytest.raises(RuntimeError):
-        O.quadrature("Triangle", 2)
+        O.quadrature("Triangle", 9)
 
 
 def _tri_nodes(p):

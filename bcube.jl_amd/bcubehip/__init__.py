@@ -9,12 +9,14 @@ from .mesh import (AbstractBcubeBackend, BcubeBackendHIP, BcubeBackendHostOnly, 
                    batch_meshes, get_bcube_backend, hexa27_mesh, hexa8_to_hexa27, hexa_mesh,
                    jitter_nodes, mixed_tri_quad_mesh, rectangle_mesh, sinusoidal_map,
                    sub_mesh)
-from .fespace import DofHandler, FunctionSpace, TestFESpace, TrialFESpace
+from .fespace import (DofHandler, FunctionSpace, MultiFESpace, TestFESpace, TrialFESpace,
+                      component_numbering)
 from .forms import (CellDomain, FEFunction, InteriorFaceDomain, Measure, MeshCellData,
-                    PhysicalFunction, UpwindFlux, compose, dot, get_face_normals, grad,
+                    PhysicalFunction, UpwindFlux, compose, ddot, dot, get_face_normals, grad,
                     integrate, jump, nabla, side_minus, side_n, side_p, side_plus)
 from .dg import DGTransport, dg_dof_coordinates
-from .assembler import (BilinearPlan, Pattern, assemble_bilinear, assemble_linear,
+from .assembler import (BilinearPlan, MultiBilinearPlan, Pattern, assemble_bilinear,
+                        assemble_linear,
                         assemble_linear_, pattern_of, quadrature_points)
 from .dirichlet import (DirichletPlan, apply_dirichlet_to_matrix_, apply_dirichlet_to_vector_,
                         apply_homogeneous_dirichlet_, apply_homogeneous_dirichlet_to_matrix_,
@@ -37,5 +39,6 @@ __all__ = [
     "apply_dirichlet_to_vector_", "apply_homogeneous_dirichlet_",
     "apply_homogeneous_dirichlet_to_matrix_", "apply_homogeneous_dirichlet_to_vector_",
     "assemble_dirichlet_vector", "boundary_faces", "dirichlet_dofs_values", "dof_coordinates",
-    "nodes_gmsh2cgns", "read_mesh", "projection_l2_",
+    "nodes_gmsh2cgns", "read_mesh", "projection_l2_", "MultiFESpace", "component_numbering",
+    "ddot", "MultiBilinearPlan",
 ]

@@ -27,6 +27,8 @@ EXPORTED = [
     "bh_space_info", "bh_space_dofs", "bh_space_destroy", "bh_pattern_build", "bh_pattern_info", "bh_pattern_tuning", "bh_pattern_set_tuning",
     "bh_dirichlet_create", "bh_dirichlet_info", "bh_dirichlet_positions", "bh_dirichlet_apply_matrix",
     "bh_dirichlet_apply_vector", "bh_dirichlet_destroy",
+    "bh_blocks_create", "bh_blocks_info", "bh_blocks_export_csc", "bh_blocks_block_values",
+    "bh_blocks_assemble", "bh_blocks_assemble_block", "bh_blocks_add_field", "bh_blocks_destroy",
     "bh_pattern_export_csc", "bh_pattern_values_device", "bh_pattern_destroy",
     "bh_assemble_bilinear", "bh_assemble_linear", "bh_quadrature_points", "bh_dg_create",
     "bh_dg_info", "bh_dg_update_geometry", "bh_dg_transport_residual", "bh_dg_minv_axpy",
@@ -97,6 +99,15 @@ def lib():
         "bh_pattern_export_csc": (C.c_int, [vp, vp, vp, C.c_int]),
         "bh_pattern_values_device": (C.c_int, [vp, P(vp)]),
         "bh_pattern_destroy": (C.c_int, [vp]),
+        "bh_blocks_create": (C.c_int, [vp, C.c_int, vp, i64, C.c_int, vp, P(vp)]),
+        "bh_blocks_info": (C.c_int, [vp, P(i64), P(i64)]),
+        "bh_blocks_export_csc": (C.c_int, [vp, vp, vp, C.c_int]),
+        "bh_blocks_block_values": (C.c_int, [vp, C.c_int, P(vp)]),
+        "bh_blocks_assemble": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int]),
+        "bh_blocks_assemble_block": (C.c_int, [vp, C.c_int, C.c_int, P(BhTerm), C.c_int, C.c_int,
+                                               C.c_int]),
+        "bh_blocks_add_field": (C.c_int, [vp, C.c_int, vp, vp, C.c_int]),
+        "bh_blocks_destroy": (C.c_int, [vp]),
         "bh_assemble_bilinear": (C.c_int, [vp, C.c_int, P(BhTerm), C.c_int, C.c_int, vp, C.c_int]),
         "bh_assemble_linear": (C.c_int, [vp, C.c_int, P(BhTerm), C.c_int, C.c_int, vp, C.c_int]),
         "bh_quadrature_points": (C.c_int, [vp, C.c_int, vp, C.c_int, P(i32)]),

@@ -14,7 +14,8 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import _lib as L
-from .forms import Placeholder, lower_bilinear, lower_dg_transport, lower_linear
+from .forms import (Placeholder, _split_sum, lower_bilinear, lower_dg_transport, lower_linear,
+                    lower_multi_bilinear, lower_multi_linear)
 
 _COEF = {"const": L.COEF_CONST, "cell": L.COEF_CELL, "qp": L.COEF_QP}
 _FORM = {"stiffness": L.FORM_STIFFNESS, "mass": L.FORM_MASS, "source": L.FORM_SOURCE}
@@ -30,6 +31,193 @@ def _single_measure(items):
         if it.measure.degree != m.degree or it.measure.domain.mesh is not m.domain.mesh:
             raise NotImplementedError("all integrals of a form must share one Measure")
     return m
+
+
+def _bilinear_bh_terms(lowered, mesh):
+    keep, terms = [], []
+    for form, kind, data, scale in lowered:
+        t = L.BhTerm(_FORM[form], _COEF[kind], None, scale)
+        if kind == "cell":
+            arr = np.ascontiguousarray(data, dtype=np.float64)
+            if arr.size != mesh.ncells:
+                raise ValueError("MeshCellData size differs from the number of cells")
+            keep.append(arr)
+            t.coef = arr.ctypes.data
+        terms.append(t)
+    return keep, (L.BhTerm * len(terms))(*terms), len(terms)
+
+
+def _is_multi(U) -> bool:
+    from .fespace import MultiFESpace
+    return isinstance(U, MultiFESpace) or getattr(getattr(U, "U", U), "size", 1) > 1
+
+
+class Fields:
+    """The fields of a vector space or MultiFESpace: every component of every
+    space is a copy of one scalar space's dofs, numbered into the global
+    system by maps[f] (0-based) -- the vector numbering of dofhandler.jl:37-156
+    composed with get_mapping (fespace.jl:610-647)."""
+
+    def __init__(self, U):
+        from .fespace import MultiFESpace, _single
+        if isinstance(U, MultiFESpace):
+            self.spaces = [_single(s) for s in U.spaces]
+            mappings = [U.get_mapping(i) - 1 for i in range(len(self.spaces))]
+            self.ndofs = U.get_ndofs()
+        else:
+            self.spaces = [_single(U)]
+            mappings = [np.arange(self.spaces[0].get_ndofs(), dtype=np.int64)]
+            self.ndofs = self.spaces[0].get_ndofs()
+        s0 = self.spaces[0]
+        for sp in self.spaces[1:]:
+            if (sp.mesh is not s0.mesh or sp.fs.degree != s0.fs.degree
+                    or sp.continuous != s0.continuous):
+                raise NotImplementedError(
+                    "fields of different meshes, orders or continuity (rectangular blocks)")
+        self.scalar = s0
+        self.sizes = [sp.size for sp in self.spaces]
+        self.first = np.concatenate([[0], np.cumsum(self.sizes)])[:-1].tolist()
+        maps = []
+        for sp, mp in zip(self.spaces, mappings):
+            cm = sp.component_map
+            for k in range(sp.size):
+                maps.append(mp[cm[k]])
+        self.maps = np.ascontiguousarray(np.stack(maps), dtype=np.int64)
+
+    def field(self, i: int, k: int) -> int:
+        return self.first[i] + k
+
+    def placeholders(self, U, role):
+        from .fespace import MultiFESpace
+        if isinstance(U, MultiFESpace):
+            return tuple(Placeholder(role, s, field=i) for i, s in enumerate(U.spaces))
+        return Placeholder(role, U)
+
+
+class MultiBilinearPlan:
+    """assemble_bilinear over a vector space or MultiFESpace (assembler.jl:
+    327-420): one scalar block per distinct coupled form, assembled by the
+    element kernels into library-owned device buffers over the scalar pattern,
+    then spread into the global CSC by bh_blocks_assemble."""
+
+    def __init__(self, a, U, V, mode: str = "fp64"):
+        self.mode = L.MODE[mode]
+        fu, fv = Fields(U), Fields(V)
+        if fu.sizes != fv.sizes or not np.array_equal(fu.maps, fv.maps):
+            raise NotImplementedError("trial and test fields must be numbered alike")
+        self.fields = fu
+        sizes = fu.sizes
+        items = _integrals(a(fu.placeholders(U, "trial"), fv.placeholders(V, "test")))
+        self.measure = _single_measure(items)
+        mesh = fu.scalar.mesh
+        if self.measure.domain.mesh is not mesh:
+            raise ValueError("the measure's mesh differs from the space's mesh")
+        by_pair = lower_multi_bilinear(items, sizes)
+        self._keep, self.blocks = [], []
+        pairs, blk = [], []
+        for (i, j), lowered in by_pair.items():
+            keep, terms, n = _bilinear_bh_terms(lowered, mesh)
+            self._keep.append(keep)
+            b = len(self.blocks)
+            self.blocks.append((terms, n))
+            for ki in range(sizes[i]):
+                for kj in range(sizes[j]):
+                    pairs += [fu.field(i, ki), fu.field(j, kj)]
+                    blk.append(b if (sizes[i] == 1 or ki == kj) else -1)
+        if len(self.blocks) > 32:
+            raise NotImplementedError("more than 32 coupled forms")
+        self.block_of_pair = np.ascontiguousarray(blk, dtype=np.int8)
+        self.pattern = pattern_of(fu.scalar)
+        self.ctx = mesh.backend.ctx
+        h = C.c_void_p()
+        pr = np.ascontiguousarray(pairs, dtype=np.int32)
+        L.check(L.lib().bh_blocks_create(self.pattern.handle, len(fu.maps), fu.maps.ctypes.data,
+                                         fu.ndofs, len(blk), pr.ctypes.data, C.byref(h)),
+                self.ctx, "bh_blocks_create")
+        self.handle = h
+        n, nnz = C.c_int64(), C.c_int64()
+        L.check(L.lib().bh_blocks_info(h, C.byref(n), C.byref(nnz)), self.ctx, "bh_blocks_info")
+        self.nrows = self.ncols = n.value
+        self.nnz = nnz.value
+        self._colptr = None
+
+    def csc_structure(self):
+        if self._colptr is None:
+            colptr = np.zeros(self.ncols + 1, dtype=np.int64)
+            rowval = np.zeros(self.nnz, dtype=np.int64)
+            L.check(L.lib().bh_blocks_export_csc(self.handle, colptr.ctypes.data,
+                                                 rowval.ctypes.data, 0), self.ctx,
+                    "bh_blocks_export_csc")
+            self._colptr, self._rowval = colptr, rowval
+        return self._colptr, self._rowval
+
+    def _blocks(self, mode):
+        # each scalar block into the library's buffer k; the per-cell
+        # coefficient arrays are host arrays, staged by the library
+        md = self.mode if mode is None else L.MODE[mode]
+        for k, (terms, n) in enumerate(self.blocks):
+            L.check(L.lib().bh_blocks_assemble_block(self.handle, k, n, terms,
+                                                     self.measure.degree, md, L.HOST),
+                    self.ctx, "bh_blocks_assemble_block")
+
+    def assemble_host(self, out: np.ndarray | None = None, mode: str | None = None) -> np.ndarray:
+        if out is None:
+            out = np.empty(self.nnz, dtype=np.float64)
+        self._blocks(mode)
+        L.check(L.lib().bh_blocks_assemble(self.handle, len(self.blocks), None,
+                                           self.block_of_pair.ctypes.data, out.ctypes.data,
+                                           L.HOST), self.ctx, "bh_blocks_assemble")
+        return out
+
+    def assemble_device(self, nzval_ptr: int, mode: str | None = None):
+        self._blocks(mode)
+        L.check(L.lib().bh_blocks_assemble(self.handle, len(self.blocks), None,
+                                           self.block_of_pair.ctypes.data, nzval_ptr, L.DEVICE),
+                self.ctx, "bh_blocks_assemble")
+
+    def to_csc(self, nzval: np.ndarray) -> sp.csc_matrix:
+        colptr, rowval = self.csc_structure()
+        A = sp.csc_matrix((nzval, rowval, colptr), shape=(self.nrows, self.ncols))
+        A.has_sorted_indices = True
+        return A
+
+    def __del__(self):
+        try:
+            L.lib().bh_blocks_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def _fields_blocks(V):
+    """Fields of V and a pair-free bh_blocks over its scalar pattern (for
+    linear forms), built once per space object."""
+    cache = V.__dict__.setdefault("_fields_blocks", {})
+    if "b" not in cache:
+        f = Fields(V)
+        pat = pattern_of(f.scalar)
+        h = C.c_void_p()
+        ctx = f.scalar.mesh.backend.ctx
+        L.check(L.lib().bh_blocks_create(pat.handle, len(f.maps), f.maps.ctypes.data, f.ndofs, 0,
+                                         None, C.byref(h)), ctx, "bh_blocks_create")
+        cache.update(b=h, f=f, pat=pat)
+    return cache["f"], cache["b"]
+
+
+def _assemble_linear_multi(b: np.ndarray, l, V, mode: str) -> None:
+    f, h = _fields_blocks(V)
+    items = _integrals(l(f.placeholders(V, "test")))
+    measure = _single_measure(items)
+    mesh = f.scalar.mesh
+    ctx = mesh.backend.ctx
+    tmp = np.empty(f.scalar.get_scalar_ndofs(), dtype=np.float64)
+    for (i, k), lowered in lower_multi_linear(items, f.sizes).items():
+        keep, terms, n = _linear_bh_terms(lowered, measure, mesh)
+        tmp[:] = 0.0
+        L.check(L.lib().bh_assemble_linear(f.scalar.handle, n, terms, measure.degree,
+                                           L.MODE[mode], tmp.ctypes.data, L.HOST), ctx,
+                "bh_assemble_linear")
+        L.check(L.lib().bh_blocks_add_field(h, f.field(i, k), tmp.ctypes.data, b.ctypes.data,
+                                            L.HOST), ctx, "bh_blocks_add_field")
 
 
 class BilinearPlan:
@@ -48,20 +236,8 @@ class BilinearPlan:
         self.measure = _single_measure(items)
         if self.measure.domain.mesh is not V.mesh:
             raise ValueError("the measure's mesh differs from the space's mesh")
-        self._keep = []
-        terms = []
-        for it in items:
-            form, kind, data, scale = lower_bilinear(it.expr)
-            t = L.BhTerm(_FORM[form], _COEF[kind], None, scale)
-            if kind == "cell":
-                arr = np.ascontiguousarray(data, dtype=np.float64)
-                if arr.size != V.mesh.ncells:
-                    raise ValueError("MeshCellData size differs from the number of cells")
-                self._keep.append(arr)
-                t.coef = arr.ctypes.data
-            terms.append(t)
-        self.terms = (L.BhTerm * len(terms))(*terms)
-        self.nterms = len(terms)
+        lowered = [lower_bilinear(t) for it in items for t in _split_sum(it.expr)]
+        self._keep, self.terms, self.nterms = _bilinear_bh_terms(lowered, V.mesh)
         ctx = V.mesh.backend.ctx
         h = C.c_void_p()
         L.check(L.lib().bh_pattern_build(U.handle, V.handle, C.byref(h)), ctx, "bh_pattern_build")
@@ -180,21 +356,26 @@ def pattern_of(U) -> Pattern:
 
 
 def assemble_bilinear(a, U, V, T=np.float64, mode: str = "fp64") -> sp.csc_matrix:
-    """assemble_bilinear(a, U, V; T=Float64) (assembler.jl:172-186)."""
+    """assemble_bilinear(a, U, V; T=Float64) (assembler.jl:172-186); over vector
+    spaces and MultiFESpaces (assembler.jl:327-420) a, U and V are those of the
+    reference, a receiving tuples for MultiFESpaces."""
     if T is not np.float64:
         raise NotImplementedError("only Float64 output is supported")
-    plan = BilinearPlan(a, U, V, mode)
+    plan = (MultiBilinearPlan if _is_multi(U) or _is_multi(V) else BilinearPlan)(a, U, V, mode)
     return plan.to_csc(plan.assemble_host())
 
 
 def _linear_terms(l, V):
     items = _integrals(l(Placeholder("test", V)))
     measure = _single_measure(items)
-    mesh = V.mesh
+    return (measure,) + _linear_bh_terms([lower_linear(it.expr) for it in items], measure,
+                                         V.mesh)
+
+
+def _linear_bh_terms(lowered, measure, mesh):
     keep, terms = [], []
     xq = None
-    for it in items:
-        form, kind, data, scale = lower_linear(it.expr)
+    for form, kind, data, scale in lowered:
         t = L.BhTerm(_FORM[form], _COEF[kind], None, scale)
         if kind == "cell":
             arr = np.ascontiguousarray(data, dtype=np.float64)
@@ -208,14 +389,18 @@ def _linear_terms(l, V):
             keep.append(vals)
             t.coef = vals.ctypes.data
         terms.append(t)
-    return measure, keep, (L.BhTerm * len(terms))(*terms), len(terms)
+    return keep, (L.BhTerm * len(terms))(*terms), len(terms)
 
 
 def assemble_linear_(b: np.ndarray, l, V, mode: str = "fp64") -> None:
     """assemble_linear!(b, l, V) (assembler.jl:465-473): adds l(v) into b."""
-    V = getattr(V, "U", V)
+    if not _is_multi(V):
+        V = getattr(V, "U", V)
     if b.dtype != np.float64 or not b.flags.c_contiguous or b.size != V.get_ndofs():
         raise ValueError("b must be a contiguous float64 vector of ndofs entries")
+    if _is_multi(V):
+        _assemble_linear_multi(b, l, V, mode)
+        return
     measure, keep, terms, n = _linear_terms(l, V)
     ctx = V.mesh.backend.ctx
     L.check(L.lib().bh_assemble_linear(V.handle, n, terms, measure.degree, L.MODE[mode],
@@ -237,6 +422,10 @@ def assemble_linear(l, V, T=np.float64, mode: str = "fp64") -> np.ndarray:
     The upwind DG transport form (see bcubehip.dg) goes to the DG kernels; its
     FEFunction may carry several fields, (nfields, ndofs), and the result has
     the same shape (FP32 in mode "fp32")."""
+    if _is_multi(V):
+        b = np.zeros(V.get_ndofs(), dtype=np.float64)
+        assemble_linear_(b, l, V, mode)
+        return b
     V = getattr(V, "U", V)
     items = _integrals(l(Placeholder("test", V)))
     dg = lower_dg_transport(items)

@@ -30,16 +30,42 @@ class FunctionSpace:
         return f"FunctionSpace(:{self.kind}, {self.degree})"
 
 
-class DofHandler:
-    """Global dof numbering of a scalar Lagrange space on a mesh.
+def component_numbering(iglob, offset, ncomp: int) -> np.ndarray:
+    """Global ids, 0-based, of component k's copy of scalar dof g, as an
+    (ncomp, ndofs) array: the numbering DofHandler(mesh, fs, ncomp, ...) gives
+    (dofhandler.jl:37-156).  There the cells' (component, local dof) slots are
+    numbered in order of first appearance, the blocks of a cell's components
+    following each other, and a component's dof on a shared vertex, edge or
+    face copies the first cell's id of the same component.  So the dofs that
+    first appear in cell c -- n_c of them, following the base(c) dofs of the
+    cells before, with scalar ids base(c) + r in local order -- take
+    ncomp*base(c) + k*n_c + r."""
+    iglob = np.asarray(iglob, dtype=np.int64) - 1
+    n = int(iglob.max()) + 1 if iglob.size else 0
+    first = np.full(n, iglob.size, dtype=np.int64)
+    np.minimum.at(first, iglob, np.arange(iglob.size, dtype=np.int64))
+    cell = np.searchsorted(np.asarray(offset, dtype=np.int64), first, side="right") - 1
+    ncell = np.bincount(cell, minlength=len(offset))
+    base = np.concatenate([[0], np.cumsum(ncell)])[:-1]
+    g = np.arange(n, dtype=np.int64)
+    r = g - base[cell]
+    k = np.arange(ncomp, dtype=np.int64)[:, None]
+    return ncomp * base[cell][None, :] + k * ncell[cell][None, :] + r[None, :]
 
-    iglob: 1-based global ids, concatenated per cell; offset[c] locates cell c.
+
+class DofHandler:
+    """Global dof numbering of a Lagrange space of `ncomponents` components.
+
+    iglob: 1-based global ids, concatenated per cell (the components' blocks
+    following each other inside a cell); offset[c] locates cell c, and
+    offset[c] + k * ndofs(c) its component k.
     """
 
     def __init__(self, mesh: Mesh, fs: FunctionSpace, ncomponents: int = 1,
                  isContinuous: bool = True):
-        if ncomponents != 1:
-            raise NotImplementedError("vector-valued spaces (size > 1) are not supported yet")
+        if ncomponents < 1:
+            raise ValueError("ncomponents must be >= 1")
+        self.ncomponents = int(ncomponents)
         self.mesh = mesh
         self.fs = fs
         self.continuous = bool(isContinuous)
@@ -55,16 +81,33 @@ class DofHandler:
         L.check(rc, None, "bh_dofhandler")  # host-only handle: no GPU context needed
         self.ndofs_tot = int(n.value)
         self._ndofs_cell = np.diff(np.append(self.offset, total))
+        if self.ncomponents > 1:
+            # the scalar numbering above, expanded to the components
+            nc = self.ncomponents
+            self.scalar_iglob, self.scalar_offset = self.iglob, self.offset
+            self.component_map = component_numbering(self.iglob, self.offset, nc)
+            cell = np.repeat(np.arange(mesh.ncells), self._ndofs_cell)
+            loc = np.arange(total) - self.offset[cell]
+            vec = self.component_map[:, self.iglob - 1] + 1  # (nc, total)
+            self.offset = self.offset * nc
+            pos = self.offset[cell][None, :] + np.arange(nc)[:, None] * self._ndofs_cell[cell] + loc
+            self.iglob = np.empty(total * nc, dtype=np.int64)
+            self.iglob[pos.reshape(-1)] = vec.reshape(-1)
+            self.ndofs_tot *= nc
 
     def get_ndofs(self, icell: int | None = None) -> int:
         if icell is None:
             return self.ndofs_tot
-        return int(self._ndofs_cell[icell])
+        return int(self._ndofs_cell[icell]) * self.ncomponents
 
-    def get_dof(self, icell: int) -> np.ndarray:
-        """1-based global dofs of cell icell (0-based cell index)."""
+    def get_dof(self, icell: int, icomp: int | None = None) -> np.ndarray:
+        """1-based global dofs of cell icell (0-based cell index), all
+        components or component icomp (0-based) (get_dof, dofhandler.jl:489-505)."""
         o = self.offset[icell]
-        return self.iglob[o:o + self._ndofs_cell[icell]]
+        nd = self._ndofs_cell[icell]
+        if icomp is None:
+            return self.iglob[o:o + nd * self.ncomponents]
+        return self.iglob[o + icomp * nd:o + (icomp + 1) * nd]
 
 
 def _host_mesh(mesh: Mesh):
@@ -82,30 +125,46 @@ def _host_mesh(mesh: Mesh):
 
 
 class SingleFESpace:
-    """A scalar Lagrange FE space with a device-resident per-cell dof map."""
+    """A Lagrange FE space of `size` components with a device-resident per-cell
+    dof map.  The library works on the scalar numbering (`handle`); a vector
+    space (size > 1) adds the map of each component's copy of the scalar dofs
+    into its own numbering (`component_map`)."""
 
-    def __init__(self, fs: FunctionSpace, mesh: Mesh, isContinuous: bool = True):
+    def __init__(self, fs: FunctionSpace, mesh: Mesh, isContinuous: bool = True, size: int = 1):
         self.fs = fs
         self.mesh = mesh
         self.continuous = bool(isContinuous)
-        self.dhl = DofHandler(mesh, fs, 1, self.continuous)
+        self.size = int(size)
+        self.dhl = DofHandler(mesh, fs, self.size, self.continuous)
+        self._scalar = DofHandler(mesh, fs, 1, self.continuous) if self.size > 1 else self.dhl
         self._handle = None
 
     def get_ndofs(self) -> int:
         return self.dhl.ndofs_tot
 
+    def get_scalar_ndofs(self) -> int:
+        return self._scalar.ndofs_tot
+
     def get_dofs(self, icell: int) -> np.ndarray:
         return self.dhl.get_dof(icell)
 
     @property
+    def component_map(self) -> np.ndarray:
+        """(size, scalar ndofs) 0-based ids of each component's dofs."""
+        if self.size == 1:
+            return np.arange(self._scalar.ndofs_tot, dtype=np.int64)[None, :]
+        return self.dhl.component_map
+
+    @property
     def handle(self):
-        """bh_space on the mesh's backend, numbered from this space's DofHandler."""
+        """bh_space on the mesh's backend, numbered from the scalar DofHandler."""
         if self._handle is None:
             h = C.c_void_p()
             m = self.mesh.handle
+            sd = self._scalar
             L.check(L.lib().bh_space_create_from_dofs(
-                m, self.fs.degree, int(self.continuous), self.dhl.ndofs_tot,
-                self.dhl.iglob.ctypes.data, self.dhl.offset.ctypes.data, C.byref(h)),
+                m, self.fs.degree, int(self.continuous), sd.ndofs_tot,
+                sd.iglob.ctypes.data, sd.offset.ctypes.data, C.byref(h)),
                 self.mesh.backend.ctx, "bh_space_create_from_dofs")
             self._handle = h
         return self._handle
@@ -125,21 +184,91 @@ class TrialFESpace(SingleFESpace):
     bcubehip.dirichlet."""
 
     def __init__(self, fs: FunctionSpace, mesh: Mesh, kind="continuous", dirichlet=None, *,
-                 isContinuous: bool | None = None):
+                 isContinuous: bool | None = None, size: int = 1):
         if isinstance(kind, dict):  # TrialFESpace(fs, mesh, Dict(...))
             kind, dirichlet = "continuous", kind
         cont = (kind != "discontinuous") if isContinuous is None else bool(isContinuous)
-        super().__init__(fs, mesh, cont)
+        super().__init__(fs, mesh, cont, size)
         self.dirichlet = dict(dirichlet or {})
+        if self.dirichlet and self.size > 1:
+            raise NotImplementedError("Dirichlet conditions on vector spaces")
 
 
 class TestFESpace:
-    """TestFESpace(U) (fespace.jl:414): shares U's numbering."""
+    """TestFESpace(U) (fespace.jl:414): shares U's numbering.  Also
+    TestFESpace(fs, mesh[, kind]; size), a space of its own."""
 
     __test__ = False  # not a pytest class
 
-    def __init__(self, U: SingleFESpace):
+    def __init__(self, U, mesh: Mesh | None = None, kind="continuous", *, size: int = 1):
+        if isinstance(U, FunctionSpace):
+            U = TrialFESpace(U, mesh, kind, size=size)
         self.U = U
 
     def __getattr__(self, name):
         return getattr(self.U, name)
+
+
+def _single(space):
+    return getattr(space, "U", space)
+
+
+class MultiFESpace:
+    """MultiFESpace(U1, U2, ...; arrayOfStruct=True) (fespace.jl:477-647): one
+    global numbering over several FE spaces.  get_mapping(i)[g] is the global
+    id (1-based) of space i's dof g (0-based), numbered by first appearance
+    over cells then spaces (AoS, _build_mapping_AoS) or spaces then cells
+    (SoA, _build_mapping_SoA)."""
+
+    __test__ = False
+
+    def __init__(self, *spaces, arrayOfStruct: bool = True):
+        if len(spaces) == 1 and isinstance(spaces[0], (tuple, list)):
+            spaces = tuple(spaces[0])
+        self.spaces = tuple(spaces)
+        self.arrayOfStruct = bool(arrayOfStruct)
+        singles = [_single(s) for s in self.spaces]
+        ncells = singles[0].mesh.ncells
+        if any(s.mesh.ncells != ncells for s in singles):
+            raise ValueError("the spaces of a MultiFESpace share one mesh")
+        # every (cell, space, local dof) slot in traversal order; a dof takes
+        # the rank of its first slot
+        keyoff = np.concatenate([[0], np.cumsum([s.get_ndofs() for s in singles])])
+        cols = {"cell": [], "space": [], "loc": [], "key": []}
+        for i, sp in enumerate(singles):
+            dh = sp.dhl
+            nd = dh._ndofs_cell * dh.ncomponents
+            cell = np.repeat(np.arange(ncells), nd)
+            cols["cell"].append(cell)
+            cols["space"].append(np.full(cell.size, i))
+            cols["loc"].append(np.arange(cell.size) - np.repeat(dh.offset, nd))
+            cols["key"].append(keyoff[i] + dh.iglob - 1)
+        cell, space, loc, key = (np.concatenate(cols[k]) for k in ("cell", "space", "loc", "key"))
+        order = (np.lexsort((loc, space, cell)) if self.arrayOfStruct
+                 else np.lexsort((loc, cell, space)))
+        uniq, first = np.unique(key[order], return_index=True)
+        ids = np.zeros(int(keyoff[-1]), dtype=np.int64)
+        ids[uniq[np.argsort(first)]] = np.arange(1, uniq.size + 1)
+        if uniq.size != keyoff[-1]:
+            raise ValueError("invalid mapping")
+        self.mapping = [ids[keyoff[i]:keyoff[i + 1]] for i in range(len(singles))]
+        self.ndofs = int(uniq.size)
+
+    def __iter__(self):
+        return iter(self.spaces)
+
+    def __len__(self):
+        return len(self.spaces)
+
+    def __getitem__(self, i):
+        return self.spaces[i]
+
+    def get_ndofs(self) -> int:
+        return self.ndofs
+
+    def get_mapping(self, i: int) -> np.ndarray:
+        """get_mapping(U, i) (fespace.jl:610-647), for 0-based i."""
+        return self.mapping[i]
+
+    def get_fespace(self):
+        return self.spaces

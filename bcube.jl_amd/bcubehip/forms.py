@@ -168,9 +168,10 @@ class PhysicalFunction(Expr):
 
 
 class Placeholder(Expr):
-    def __init__(self, role, space):
+    def __init__(self, role, space, field=None):
         self.role = role  # "trial" or "test"
         self.space = space
+        self.field = field  # index inside a MultiFESpace, None for a single space
 
 
 class Grad(Expr):
@@ -193,6 +194,13 @@ class Add(Expr):
         self.a, self.b = a, b
 
 
+class DDot(Expr):
+    """⊡, the double contraction of two tensors (algebra.jl)."""
+
+    def __init__(self, a, b):
+        self.a, self.b = a, b
+
+
 def grad(a):
     """∇ (gradient.jl:28)."""
     return Grad(a)
@@ -204,6 +212,11 @@ nabla = grad
 def dot(a, b):
     """⋅ between two vector-valued operands."""
     return Dot(_wrap(a), _wrap(b))
+
+
+def ddot(a, b):
+    """⊡ between two tensor-valued operands, e.g. ∇(u) ⊡ ∇(v) on a vector space."""
+    return DDot(_wrap(a), _wrap(b))
 
 
 class CellDomain:
@@ -341,6 +354,107 @@ def lower_bilinear(expr):
                 raise NotImplementedError("mass coefficient must be constant or per cell")
             return "mass", kind, data, scale
     raise NotImplementedError("bilinear integrand is not one of: η ∇u⋅∇v, c u v")
+
+
+def _split_sum(e):
+    """The product terms of an integrand: sums inside products distributed."""
+    if isinstance(e, Add):
+        return _split_sum(e.a) + _split_sum(e.b)
+    if isinstance(e, Mul):
+        return [Mul(x, y) for x in _split_sum(e.a) for y in _split_sum(e.b)]
+    return [e]
+
+
+def _placeholders(e, out):
+    if isinstance(e, Placeholder):
+        out.append(e)
+    for k in ("a", "b"):
+        if hasattr(e, k) and isinstance(getattr(e, k), Expr):
+            _placeholders(getattr(e, k), out)
+    return out
+
+
+def _to_scalar_form(e, vector: bool):
+    """A term over two vector spaces as the scalar form of each component:
+    u⋅v -> u v, ∇u ⊡ ∇v -> ∇u⋅∇v (both component-diagonal).  Over scalar
+    spaces ⊡ and ⋅ of two gradients agree, and u⋅v is u v."""
+    if isinstance(e, Mul):
+        return Mul(_to_scalar_form(e.a, vector), _to_scalar_form(e.b, vector))
+    if isinstance(e, DDot):
+        return Dot(e.a, e.b)
+    if isinstance(e, Dot):
+        pa, pb = _factors(e.a, []), _factors(e.b, [])
+        grads = [f for f in pa + pb if isinstance(f, Grad)]
+        if not grads:
+            return Mul(e.a, e.b)
+        if vector:
+            raise NotImplementedError("∇u⋅∇v on a vector space: use ddot (⊡)")
+    return e
+
+
+def lower_multi_bilinear(items, sizes):
+    """Lower a bilinear form over several fields (assembler.jl:327-420).
+
+    sizes[i]: components of space i.  Returns {(test space i, trial space j):
+    [(form, kind, data, scale), ...]} for the pairs the form couples; a pair of
+    vector spaces gets the scalar form of each component (component-diagonal,
+    the only vector integrands lowered: c u⋅v and η ∇u ⊡ ∇v)."""
+    out: dict[tuple[int, int], list] = {}
+    for it in items:
+        for term in _split_sum(it.expr):
+            ph = _placeholders(term, [])
+            us = [p for p in ph if p.role == "trial"]
+            vs = [p for p in ph if p.role == "test"]
+            if len(us) != 1 or len(vs) != 1:
+                raise NotImplementedError("each term must hold one trial and one test function")
+            i, j = vs[0].field or 0, us[0].field or 0
+            if (sizes[i] > 1) != (sizes[j] > 1) or (sizes[i] > 1 and sizes[i] != sizes[j]):
+                raise NotImplementedError("terms coupling a scalar and a vector space")
+            out.setdefault((i, j), []).append(
+                lower_bilinear(_to_scalar_form(term, sizes[i] > 1)))
+    return out
+
+
+def lower_multi_linear(items, sizes):
+    """Lower l(v) over several fields: {(test space i, component k): [(form,
+    kind, data, scale), ...]}; over a vector space the term is f⋅v with f a
+    vector PhysicalFunction (component k of f.f's value per point) or a
+    constant vector."""
+    out: dict[tuple[int, int], list] = {}
+    for it in items:
+        for term in _split_sum(it.expr):
+            vs = [p for p in _placeholders(term, []) if p.role == "test"]
+            if len(vs) != 1:
+                raise NotImplementedError("each term must hold one test function")
+            i = vs[0].field or 0
+            if sizes[i] == 1:
+                t = _to_scalar_form(term, False)
+                out.setdefault((i, 0), []).append(lower_linear(t))
+                continue
+            fs = _factors(term, [])
+            dots = [f for f in fs if isinstance(f, Dot)]
+            if len(dots) != 1:
+                raise NotImplementedError("vector linear term must be f⋅v")
+            d = dots[0]
+            side = [d.a, d.b]
+            if _is(side[0], Placeholder, "test"):
+                side.reverse()
+            f, v = side
+            if not _is(v, Placeholder, "test"):
+                raise NotImplementedError("vector linear term must be f⋅v")
+            rest = [x for x in fs if x is not d]
+            for k in range(sizes[i]):
+                if isinstance(f, VecConst):
+                    fk = Const(float(f.v[k]))
+                elif isinstance(f, PhysicalFunction):
+                    fk = PhysicalFunction(lambda x, f=f.f, k=k: np.asarray(f(x))[:, k])
+                else:
+                    raise NotImplementedError("f in f⋅v must be a PhysicalFunction or constant")
+                e = v
+                for x in rest + [fk]:
+                    e = Mul(x, e)
+                out.setdefault((i, k), []).append(lower_linear(e))
+    return out
 
 
 def lower_linear(expr):

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -118,6 +119,26 @@ struct bh_dirichlet_s {
   int32_t* d_src = nullptr;
   double* d_values = nullptr;  // host-`where` staging: the caller's values
   double* d_b = nullptr;       // host-`where` staging: the vector
+};
+
+// Several fields over one scalar pattern (vector spaces, MultiFESpace): the
+// global CSC and, per entry, its scalar-block source (assembler.jl:327-420,
+// fespace.jl:610-647).
+struct bh_blocks_s {
+  bh_pattern p = nullptr;
+  int dev = -1;
+  int nfields = 0, npairs = 0;
+  int64_t ns = 0;                 // scalar dofs
+  int64_t n = 0, nnz = 0;         // global size and entries
+  std::vector<int32_t> pairs;     // (row field, column field) per pair
+  std::vector<int64_t> colptr;    // n + 1
+  std::vector<int32_t> rowval;    // nnz, sorted per column
+  int32_t* d_src = nullptr;       // scalar entry of each global entry
+  int16_t* d_pair = nullptr;      // its pair
+  int32_t* d_maps = nullptr;      // nfields * ns global ids
+  double* d_block[kMaxBlocks] = {};  // library-owned scalar block values
+  double* d_out = nullptr;        // host-`where` staging of the values
+  double *d_b = nullptr, *d_sb = nullptr;  // host-`where` staging of vectors
 };
 
 struct bh_dg_s {
@@ -288,6 +309,22 @@ int build_colouring(bh_ctx c, const std::vector<int32_t>& dofs, int nl, int64_t 
 
 // Coefficient pointer for a term on the device: device pointers pass through,
 // host arrays are staged into `stage` (at least n doubles).
+// Per-cell coefficients of a device-`where` call are read by the kernels, so
+// they must be device memory; a host array there would fault the GPU.
+int check_device_coefs(bh_ctx c, int nterms, const bh_term* terms) {
+  if (!c || c->capturing) return BH_OK;
+  for (int k = 0; k < nterms; ++k) {
+    if (terms[k].coef_kind == BH_COEF_CONST || !terms[k].coef) continue;
+    hipPointerAttribute_t a;
+    const hipError_t e = hipPointerGetAttributes(&a, terms[k].coef);
+    if (e != hipSuccess) (void)hipGetLastError();
+    if (e != hipSuccess || (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged))
+      return fail(c, BH_ERR_ARG,
+                  "a device-`where` call takes device coefficient arrays (this one is host memory)");
+  }
+  return BH_OK;
+}
+
 const double* coef_ptr(const bh_term& t, int where, double* stage, int64_t n, hipStream_t s) {
   if (t.coef_kind == BH_COEF_CONST || t.coef == nullptr) return nullptr;
   if (where == BH_DEVICE) return t.coef;
@@ -814,6 +851,209 @@ int bh_dirichlet_destroy(bh_dirichlet d) {
   return BH_OK;
 }
 
+// ---- several fields over one scalar pattern -----------------------------------------
+int bh_blocks_create(bh_pattern p, int nfields, const int64_t* maps, int64_t n, int npairs,
+                     const int32_t* pairs, bh_blocks* out) {
+  if (!p || !out || nfields < 1 || !maps || n < 1 || npairs < 0 || (npairs > 0 && !pairs))
+    return BH_ERR_ARG;
+  *out = nullptr;
+  bh_ctx c = p->V->mesh->ctx;
+  if (p->U != p->V) return fail(c, BH_ERR_UNSUPPORTED, "fields need a square scalar pattern");
+  if (npairs > kMaxPairs) return fail(c, BH_ERR_RANGE, "more than 256 field pairs");
+  if (n > INT32_MAX) return fail(c, BH_ERR_RANGE, "more than 2^31 - 1 global dofs");
+  const HostPattern& h = p->h;
+  if (h.nnz > INT32_MAX) return fail(c, BH_ERR_RANGE, "scalar pattern above 2^31 - 1 entries");
+  const int64_t ns = h.nrows;
+  auto b = std::make_unique<bh_blocks_s>();
+  b->p = p;
+  b->dev = p->dev;
+  b->nfields = nfields;
+  b->npairs = npairs;
+  b->ns = ns;
+  b->n = n;
+  b->pairs.assign(pairs, pairs + 2 * npairs);
+  std::vector<std::vector<int>> by_col(nfields);
+  std::set<std::pair<int, int>> seen;
+  for (int q = 0; q < npairs; ++q) {
+    const int fi = pairs[2 * q], fj = pairs[2 * q + 1];
+    if (fi < 0 || fi >= nfields || fj < 0 || fj >= nfields)
+      return fail(c, BH_ERR_ARG, "pair names a field out of range");
+    if (!seen.insert({fi, fj}).second) return fail(c, BH_ERR_ARG, "field pair listed twice");
+    by_col[fj].push_back(q);
+  }
+  // owner[C] = f*ns + g of the field dof numbered C (maps must not overlap)
+  std::vector<int64_t> owner(n, -1);
+  for (int64_t k = 0; k < (int64_t)nfields * ns; ++k) {
+    const int64_t v = maps[k];
+    if (v < 0 || v >= n) return fail(c, BH_ERR_RANGE, "field map value out of range");
+    if (owner[v] >= 0) return fail(c, BH_ERR_ARG, "two field dofs map to one global dof");
+    owner[v] = k;
+  }
+  b->colptr.assign(n + 1, 0);
+  for (int64_t C = 0; C < n; ++C) {
+    int64_t cnt = 0;
+    if (owner[C] >= 0) {
+      const int fj = (int)(owner[C] / ns);
+      const int64_t hh = owner[C] % ns;
+      cnt = (int64_t)by_col[fj].size() * (h.rowptr[hh + 1] - h.rowptr[hh]);
+    }
+    b->colptr[C + 1] = b->colptr[C] + cnt;
+  }
+  b->nnz = b->colptr[n];
+  b->rowval.resize(b->nnz);
+  std::vector<int32_t> src(b->nnz);
+  std::vector<int16_t> pr(b->nnz);
+  // column C = (field fj, scalar dof h): for every pair (fi, fj), the rows
+  // maps[fi][g] of the scalar column h (= scalar row h, the pattern is
+  // symmetric), whose values sit at the scalar CSC positions of row h's run
+  parallel_for(n, [&](int64_t c0, int64_t c1) {
+    struct Ent { int32_t r; int32_t e; int16_t q; };
+    std::vector<Ent> col;
+    for (int64_t C = c0; C < c1; ++C) {
+      if (owner[C] < 0) continue;
+      const int fj = (int)(owner[C] / ns);
+      const int64_t hh = owner[C] % ns;
+      col.clear();
+      for (int q : by_col[fj]) {
+        const int fi = pairs[2 * q];
+        for (int64_t e = h.rowptr[hh]; e < h.rowptr[hh + 1]; ++e)
+          col.push_back({(int32_t)maps[(int64_t)fi * ns + h.colval[e]], (int32_t)e, (int16_t)q});
+      }
+      std::sort(col.begin(), col.end(), [](const Ent& a, const Ent& z) { return a.r < z.r; });
+      int64_t o = b->colptr[C];
+      for (const Ent& x : col) {
+        b->rowval[o] = x.r;
+        src[o] = x.e;
+        pr[o] = x.q;
+        ++o;
+      }
+    }
+  });
+  if (c) {
+    if (int r = set_device(c)) return r;
+    std::vector<int32_t> m32((size_t)nfields * ns);
+    for (size_t k = 0; k < m32.size(); ++k) m32[k] = (int32_t)maps[k];
+    auto up = [&](auto** dst, const auto& v) -> int {
+      BH_HIP(c, dalloc(dst, std::max<size_t>(v.size(), 1)));
+      if (!v.empty())
+        BH_HIP(c, hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
+      return BH_OK;
+    };
+    if (int r = up(&b->d_src, src)) return r;
+    if (int r = up(&b->d_pair, pr)) return r;
+    if (int r = up(&b->d_maps, m32)) return r;
+  }
+  *out = b.release();
+  return BH_OK;
+}
+
+int bh_blocks_info(bh_blocks b, int64_t* n, int64_t* nnz) {
+  if (!b) return BH_ERR_ARG;
+  if (n) *n = b->n;
+  if (nnz) *nnz = b->nnz;
+  return BH_OK;
+}
+
+int bh_blocks_export_csc(bh_blocks b, int64_t* colptr, int64_t* rowval, int base) {
+  if (!b || !colptr || !rowval) return BH_ERR_ARG;
+  for (int64_t k = 0; k <= b->n; ++k) colptr[k] = b->colptr[k] + base;
+  for (int64_t k = 0; k < b->nnz; ++k) rowval[k] = (int64_t)b->rowval[k] + base;
+  return BH_OK;
+}
+
+int bh_blocks_block_values(bh_blocks b, int k, double** d) {
+  if (!b || !d || k < 0 || k >= kMaxBlocks) return BH_ERR_ARG;
+  bh_ctx c = b->p->V->mesh->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "block buffers need a pattern with a context");
+  if (!b->d_block[k]) {
+    if (c->capturing) return fail(c, BH_ERR_ARG, "allocate block buffers before capturing");
+    if (int r = set_device(c)) return r;
+    BH_HIP(c, dalloc(&b->d_block[k], (size_t)std::max<int64_t>(b->p->h.nnz, 1)));
+  }
+  *d = b->d_block[k];
+  return BH_OK;
+}
+
+int bh_blocks_assemble(bh_blocks b, int nblocks, const double* const* values,
+                       const int8_t* block_of_pair, double* nzval, int where) {
+  if (!b || !nzval || nblocks < 0 || nblocks > kMaxBlocks || (b->npairs > 0 && !block_of_pair))
+    return BH_ERR_ARG;
+  bh_ctx c = b->p->V->mesh->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "assembly needs a pattern with a context");
+  if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
+  if (where == BH_HOST && c->capturing)
+    return fail(c, BH_ERR_ARG, "host-`where` calls cannot be captured in a graph");
+  BlockVals bv = {};
+  for (int k = 0; k < nblocks; ++k) {
+    bv.v[k] = values ? values[k] : b->d_block[k];
+    if (!bv.v[k]) return fail(c, BH_ERR_ARG, "block values missing");
+  }
+  for (int q = 0; q < kMaxPairs; ++q) bv.blk[q] = -1;
+  for (int q = 0; q < b->npairs; ++q) {
+    if (block_of_pair[q] >= nblocks) return fail(c, BH_ERR_ARG, "pair names a block out of range");
+    bv.blk[q] = block_of_pair[q] < 0 ? -1 : block_of_pair[q];
+  }
+  if (int r = set_device(c)) return r;
+  double* out = nzval;
+  if (where == BH_HOST) {
+    if (!b->d_out) BH_HIP(c, dalloc(&b->d_out, (size_t)std::max<int64_t>(b->nnz, 1)));
+    out = b->d_out;
+  }
+  time_begin(c);
+  BH_HIP(c, launch_blocks_expand(c->stream, b->d_src, b->d_pair, b->nnz, bv, out));
+  time_end(c);
+  if (where == BH_HOST) {
+    BH_HIP(c, hipMemcpyAsync(nzval, out, (size_t)b->nnz * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    BH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return BH_OK;
+}
+
+int bh_blocks_add_field(bh_blocks b, int field, const double* v, double* out, int where) {
+  if (!b || !v || !out || field < 0 || field >= b->nfields) return BH_ERR_ARG;
+  bh_ctx c = b->p->V->mesh->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "needs a pattern with a context");
+  if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
+  if (where == BH_HOST && c->capturing)
+    return fail(c, BH_ERR_ARG, "host-`where` calls cannot be captured in a graph");
+  if (int r = set_device(c)) return r;
+  const double* dv = v;
+  double* dout = out;
+  if (where == BH_HOST) {
+    if (!b->d_b) {
+      BH_HIP(c, dalloc(&b->d_b, (size_t)b->n));
+      BH_HIP(c, dalloc(&b->d_sb, (size_t)std::max<int64_t>(b->ns, 1)));
+    }
+    BH_HIP(c, hipMemcpyAsync(b->d_b, out, (size_t)b->n * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    BH_HIP(c, hipMemcpyAsync(b->d_sb, v, (size_t)b->ns * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    dv = b->d_sb;
+    dout = b->d_b;
+  }
+  BH_HIP(c, launch_add_field(c->stream, dout, b->d_maps + (int64_t)field * b->ns, dv, b->ns));
+  if (where == BH_HOST) {
+    BH_HIP(c, hipMemcpyAsync(out, dout, (size_t)b->n * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    BH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return BH_OK;
+}
+
+int bh_blocks_destroy(bh_blocks b) {
+  if (!b) return BH_ERR_ARG;
+  if (b->dev >= 0) {
+    (void)hipSetDevice(b->dev);
+    for (void* q : {(void*)b->d_src, (void*)b->d_pair, (void*)b->d_maps, (void*)b->d_out,
+                    (void*)b->d_b, (void*)b->d_sb})
+      (void)hipFree(q);
+    for (double* q : b->d_block) (void)hipFree(q);
+  }
+  delete b;
+  return BH_OK;
+}
+
 int bh_pattern_info(bh_pattern p, int64_t* nrows, int64_t* ncols, int64_t* nnz, int32_t* max_row) {
   if (!p) return BH_ERR_ARG;
   if (nrows) *nrows = p->h.nrows;
@@ -856,8 +1096,12 @@ int bh_pattern_destroy(bh_pattern p) {
 }
 
 // ---- assembly ----------------------------------------------------------------------
-int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int quad_degree, int mode,
-                         double* nzval, int where) {
+// The assembly itself: values to nzval (host or device by `where`; a NULL
+// device nzval means the pattern's own buffer), per-cell coefficient arrays
+// read from host or device memory by coef_where.
+static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms,
+                                  int quad_degree, int mode, double* nzval, int where,
+                                  int coef_where) {
   if (!p || nterms <= 0 || !terms) return BH_ERR_ARG;
   bh_space V = p->V;
   bh_mesh m = V->mesh;
@@ -868,7 +1112,7 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
     return fail(c, BH_ERR_UNSUPPORTED, "bilinear modes: BH_FP64, BH_FP32, BH_BF16X3");
   if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
   if (where == BH_HOST && !nzval) return fail(c, BH_ERR_ARG, "nzval is NULL");
-  if (where == BH_HOST && c->capturing)
+  if ((where == BH_HOST || coef_where == BH_HOST) && c->capturing)
     return fail(c, BH_ERR_ARG, "host-`where` assembly cannot be captured in a graph");
   if (m->geo == -1 || m->geo == BH_HEXA27 || m->buckets.empty())
     return fail(c, BH_ERR_UNSUPPORTED,
@@ -915,12 +1159,12 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
   A.ke_fp64 = mode == BH_FP64;
   A.erow = V->nl;
   if (stiff) {
-    A.eta = coef_ptr(*stiff, where, p->d_coef, nc, c->stream);
+    A.eta = coef_ptr(*stiff, coef_where, p->d_coef, nc, c->stream);
     A.eta_scale = stiff->scale;
   }
   if (mass) {
     A.mass = true;
-    A.mcoef = coef_ptr(*mass, where, p->d_coef + nc, nc, c->stream);
+    A.mcoef = coef_ptr(*mass, coef_where, p->d_coef + nc, nc, c->stream);
     A.mass_scale = mass->scale;
   }
   // 1. element rows, one launch per cell type (a single one on uniform
@@ -1037,6 +1281,25 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
     BH_HIP(c, hipStreamSynchronize(c->stream));
   }
   return BH_OK;
+}
+
+int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int quad_degree, int mode,
+                         double* nzval, int where) {
+  if (p && terms && where == BH_DEVICE)
+    if (int r = check_device_coefs(p->V->mesh->ctx, nterms, terms)) return r;
+  return assemble_bilinear_impl(p, nterms, terms, quad_degree, mode, nzval, where, where);
+}
+
+int bh_blocks_assemble_block(bh_blocks b, int k, int nterms, const bh_term* terms,
+                             int quad_degree, int mode, int coef_where) {
+  if (!b || k < 0 || k >= kMaxBlocks || !terms) return BH_ERR_ARG;
+  bh_ctx c = b->p->V->mesh->ctx;
+  if (coef_where != BH_HOST && coef_where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
+  if (coef_where == BH_DEVICE)
+    if (int r = check_device_coefs(c, nterms, terms)) return r;
+  double* d = nullptr;
+  if (int r = bh_blocks_block_values(b, k, &d)) return r;
+  return assemble_bilinear_impl(b->p, nterms, terms, quad_degree, mode, d, BH_DEVICE, coef_where);
 }
 
 int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_degree, int mode,

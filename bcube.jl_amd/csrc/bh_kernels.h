@@ -72,6 +72,21 @@ hipError_t launch_fill(hipStream_t s, double* p, int64_t n, double v);
 // Dirichlet edits (bh_dirichlet_*): v[pos[i]] = val, and b[dofs[i]] =
 // values ? values[src[i]] : 0.
 hipError_t launch_set_at(hipStream_t s, double* v, const int64_t* pos, int64_t n, double val);
+
+// Multi-field matrices: every entry of the global CSC takes the value of one
+// entry of one scalar block, out[e] = v[blk[pair[e]]][src[e]], or 0 where the
+// block index is negative; the tables travel as kernel arguments.
+constexpr int kMaxBlocks = 32;
+constexpr int kMaxPairs = 256;
+struct BlockVals {
+  const double* v[kMaxBlocks];
+  int8_t blk[kMaxPairs];
+};
+hipError_t launch_blocks_expand(hipStream_t s, const int32_t* src, const int16_t* pair, int64_t n,
+                                const BlockVals& bv, double* out);
+// out[map[g]] += v[g] for g < n (the targets of one field are distinct).
+hipError_t launch_add_field(hipStream_t s, double* out, const int32_t* map, const double* v,
+                            int64_t n);
 hipError_t launch_set_dofs(hipStream_t s, double* b, const int64_t* dofs, const int32_t* src,
                            const double* values, int64_t n);
 // Row gather for element rows of `nl` entries (the space's padded dof count).

@@ -602,6 +602,72 @@ hipError_t launch_set_dofs(hipStream_t s, double* b, const int64_t* dofs, const 
   return hipGetLastError();
 }
 
+// Global CSC values of a multi-field matrix from its scalar blocks.  VEC:
+// four entries per thread (one 16-byte load of src, 8 bytes of pair indices,
+// two 16-byte stores; out 16-byte aligned); the few entries past the last
+// group of four go to the first threads.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_blocks_expand(const int32_t* __restrict__ src,
+                                                       const int16_t* __restrict__ pair,
+                                                       int64_t n, BlockVals bv,
+                                                       double* __restrict__ out) {
+  __shared__ const double* s_v[kMaxBlocks];
+  __shared__ int8_t s_b[kMaxPairs];
+  static_assert(kMaxPairs == 256 && kMaxBlocks <= 256, "one table entry per thread");
+  if (threadIdx.x < kMaxBlocks) s_v[threadIdx.x] = bv.v[threadIdx.x];
+  s_b[threadIdx.x] = bv.blk[threadIdx.x];
+  __syncthreads();
+  auto value = [&](int32_t e, int16_t p) -> double {
+    const int b = s_b[p];
+    return b < 0 ? 0.0 : s_v[b][e];
+  };
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t done = 0;
+  if constexpr (VEC) {
+    const int64_t n4 = n / 4;
+    const int4* s4 = reinterpret_cast<const int4*>(src);
+    const short4* p4 = reinterpret_cast<const short4*>(pair);
+    double2* o2 = reinterpret_cast<double2*>(out);
+    for (int64_t q = t0; q < n4; q += stride) {
+      const int4 e = s4[q];
+      const short4 pr = p4[q];
+      o2[2 * q] = make_double2(value(e.x, pr.x), value(e.y, pr.y));
+      o2[2 * q + 1] = make_double2(value(e.z, pr.z), value(e.w, pr.w));
+    }
+    done = 4 * n4;
+  }
+  for (int64_t i = done + t0; i < n; i += stride) out[i] = value(src[i], pair[i]);
+}
+
+__global__ void k_add_field(double* __restrict__ out, const int32_t* __restrict__ map,
+                            const double* __restrict__ v, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[map[i]] += v[i];
+}
+
+hipError_t launch_blocks_expand(hipStream_t s, const int32_t* src, const int16_t* pair, int64_t n,
+                                const BlockVals& bv, double* out) {
+  if (n <= 0) return hipSuccess;
+  const bool vec = ((uintptr_t)out % 16 == 0) && ((uintptr_t)src % 16 == 0) &&
+                   ((uintptr_t)pair % 8 == 0);
+  if (vec)
+    hipLaunchKernelGGL(k_blocks_expand<true>, grid_1d((n + 3) / 4), dim3(256), 0, s, src, pair,
+                       n, bv, out);
+  else
+    hipLaunchKernelGGL(k_blocks_expand<false>, grid_1d(n), dim3(256), 0, s, src, pair, n, bv,
+                       out);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_field(hipStream_t s, double* out, const int32_t* map, const double* v,
+                            int64_t n) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_add_field, grid_1d(n), dim3(256), 0, s, out, map, v, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill(hipStream_t s, double* p, int64_t n, double v) {
   int64_t blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;

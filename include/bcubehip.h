@@ -76,6 +76,7 @@ typedef struct bh_mesh_s* bh_mesh;
 typedef struct bh_space_s* bh_space;
 typedef struct bh_pattern_s* bh_pattern;
 typedef struct bh_dirichlet_s* bh_dirichlet;
+typedef struct bh_blocks_s* bh_blocks;
 
 /* ---- context ------------------------------------------------------------ */
 /* One context per host thread.  Replaces the serial backend object
@@ -207,6 +208,49 @@ int bh_dirichlet_apply_matrix(bh_dirichlet d, double diag, int homogeneous, doub
  * apply_homogeneous_dirichlet_to_vector! (:29-40), b[dof] = 0. */
 int bh_dirichlet_apply_vector(bh_dirichlet d, const double* values, double* b, int where);
 int bh_dirichlet_destroy(bh_dirichlet d);
+
+/* ---- several fields: vector spaces and MultiFESpace ------------------------ */
+/* A TrialFESpace(fs, mesh; size = n) (fespace.jl:186-274, numbering
+ * dofhandler.jl:37-156 with ncomponents = n) and a MultiFESpace
+ * (fespace.jl:477-647, get_mapping) number several copies ("fields") of one
+ * scalar space's dofs into one global numbering; assemble_bilinear over such
+ * spaces (assembler.jl:327-420) assembles one scalar form per coupled pair of
+ * fields and maps its entries through the numbering.  Here, over the square
+ * scalar pattern p:
+ *   maps[f*ns + g] = global id (0-based, < n) of field f's copy of scalar dof
+ *     g (ns = rows of p); the maps of all fields must not overlap;
+ *   pairs[2q], pairs[2q+1] = (row field, column field) of the q-th coupled
+ *     pair (at most 256; each pair at most once).
+ * The global pattern holds, for every pair, the mapped entries of p (the
+ * reference's element loop appends every component pair of a vector space,
+ * zero or not, so a vector space lists all n*n component pairs).  Building
+ * needs no GPU; bh_blocks_assemble and bh_blocks_add_field need a context. */
+int bh_blocks_create(bh_pattern p, int nfields, const int64_t* maps, int64_t n, int npairs,
+                     const int32_t* pairs, bh_blocks* out);
+int bh_blocks_info(bh_blocks b, int64_t* n, int64_t* nnz);
+/* The global SparseMatrixCSC structure: colptr[n+1], rowval[nnz], `base` 0/1. */
+int bh_blocks_export_csc(bh_blocks b, int64_t* colptr, int64_t* rowval, int base);
+/* Device buffer k (< 32) of p's nnz doubles, owned by b, for the values of one
+ * scalar block (bh_assemble_bilinear on p with where = BH_DEVICE). */
+int bh_blocks_block_values(bh_blocks b, int k, double** d);
+/* nzval[e] = values[block_of_pair[q]][s] for the pair q and scalar entry s of
+ * global entry e, or 0 where block_of_pair[q] < 0 (a pair whose form is zero,
+ * e.g. two different components under ∫(u⋅v)).  values: nblocks device
+ * arrays of p's CSC values, or NULL for b's own buffers 0..nblocks-1.
+ * nzval: host or device by `where`. */
+int bh_blocks_assemble(bh_blocks b, int nblocks, const double* const* values,
+                       const int8_t* block_of_pair, double* nzval, int where);
+/* bh_assemble_bilinear of one scalar block on b's pattern into b's own
+ * buffer k, with per-cell coefficient arrays in host or device memory by
+ * coef_where (bh_assemble_bilinear with where = BH_DEVICE reads them from the
+ * device and rejects host arrays). */
+int bh_blocks_assemble_block(bh_blocks b, int k, int nterms, const bh_term* terms,
+                             int quad_degree, int mode, int coef_where);
+/* out[maps[field][g]] += v[g]: a scalar linear form (bh_assemble_linear on
+ * p's space) added into the global vector (assembler.jl:476-533 for several
+ * fields); v and out are host or device by `where`. */
+int bh_blocks_add_field(bh_blocks b, int field, const double* v, double* out, int where);
+int bh_blocks_destroy(bh_blocks b);
 
 /* ---- discontinuous Galerkin transport ------------------------------------- */
 /* The linear form of an upwind DG discretisation of ∂u/∂t + ∇⋅(c u) = 0,

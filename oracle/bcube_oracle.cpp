@@ -678,38 +678,46 @@ static std::vector<i64> densify(const std::vector<i64>& a) {
   return out;
 }
 
-// Returns ndofs_tot; iglob is 1-based global dof ids, offset 0-based positions.
+// Returns ndofs_tot; iglob is 1-based global dof ids, offset 0-based positions
+// per (cell, component), offset[c*ncomp + k] (dofhandler.jl:37-72: the blocks
+// of the components follow each other inside a cell).  The dictionaries are
+// keyed by (component, vertex / edge / face) as the reference's (kvar, ...)
+// keys.
 static i64 dofhandler(const Mesh& m, int order, bool continuous, std::vector<i64>& iglob,
-                      std::vector<i64>& offset, std::vector<int>& ndofs) {
+                      std::vector<i64>& offset, std::vector<int>& ndofs, int ncomp = 1) {
   i64 nc = m.ncells();
-  offset.assign(nc, 0);
+  offset.assign(nc * ncomp, 0);
   ndofs.assign(nc, 0);
   i64 tot = 0;
   for (i64 c = 0; c < nc; ++c) {
     ndofs[c] = ndofs_lagrange(shape_of(m.ctype[c]), order);
-    offset[c] = tot;
-    tot += ndofs[c];
+    for (int k = 0; k < ncomp; ++k) {
+      offset[c * ncomp + k] = tot;
+      tot += ndofs[c];
+    }
   }
   iglob.resize(tot);
   for (i64 i = 0; i < tot; ++i) iglob[i] = i + 1;
   if (continuous) {
-    std::map<i64, std::vector<i64>> dict_n;
-    std::map<std::vector<i64>, std::pair<i64, std::vector<i64>>> dict_e, dict_f;
+    std::map<std::pair<int, i64>, std::vector<i64>> dict_n;
+    std::map<std::pair<int, std::vector<i64>>, std::pair<i64, std::vector<i64>>> dict_e, dict_f;
     for (i64 c = 0; c < nc; ++c) {
       int ct = m.ctype[c];
       Shape s = shape_of(ct);
       std::vector<i64> nodes = m.cell(c);
+      for (int k = 0; k < ncomp; ++k) {
+      const i64 off = offset[c * ncomp + k];
       // vertices (dofhandler.jl:180-220); zip truncates to the shorter list
       Lists vdofs = idof_by_vertex(s, order);
       for (size_t v = 0; v < vdofs.size() && v < nodes.size(); ++v) {
         if (vdofs[v].empty()) continue;
-        auto it = dict_n.find(nodes[v]);
+        auto it = dict_n.find({k, nodes[v]});
         if (it != dict_n.end()) {
-          for (size_t d = 0; d < it->second.size(); ++d) iglob[offset[c] + vdofs[v][d]] = it->second[d];
+          for (size_t d = 0; d < it->second.size(); ++d) iglob[off + vdofs[v][d]] = it->second[d];
         } else {
           std::vector<i64> g;
-          for (int ld : vdofs[v]) g.push_back(iglob[offset[c] + ld]);
-          dict_n[nodes[v]] = g;
+          for (int ld : vdofs[v]) g.push_back(iglob[off + ld]);
+          dict_n[{k, nodes[v]}] = g;
         }
       }
       // edges (dofhandler.jl:239-290)
@@ -720,8 +728,9 @@ static i64 dofhandler(const Mesh& m, int order, bool continuous, std::vector<i64
           if (edofs[e].empty()) continue;
           std::vector<i64> eg;
           for (int ln : e2n[e]) eg.push_back(nodes[ln]);
-          std::vector<i64> key = eg;
-          std::sort(key.begin(), key.end());
+          std::vector<i64> sorted = eg;
+          std::sort(sorted.begin(), sorted.end());
+          const auto key = std::make_pair(k, sorted);
           auto it = dict_e.find(key);
           if (it != dict_e.end()) {
             i64 jc = it->second.first;
@@ -729,10 +738,10 @@ static i64 dofhandler(const Mesh& m, int order, bool continuous, std::vector<i64
             int jside = oriented_cell_side(m.ctype[jc], m.cell(jc), eg);
             std::vector<i64> re = jd;
             if (!(jside > 0)) std::reverse(re.begin(), re.end());
-            for (size_t d = 0; d < jd.size(); ++d) iglob[offset[c] + edofs[e][d]] = re[d];
+            for (size_t d = 0; d < jd.size(); ++d) iglob[off + edofs[e][d]] = re[d];
           } else {
             std::vector<i64> g;
-            for (int ld : edofs[e]) g.push_back(iglob[offset[c] + ld]);
+            for (int ld : edofs[e]) g.push_back(iglob[off + ld]);
             dict_e[key] = {c, g};
           }
         }
@@ -746,22 +755,24 @@ static i64 dofhandler(const Mesh& m, int order, bool continuous, std::vector<i64
           if (fdofs[f].empty()) continue;
           std::vector<i64> fg;
           for (int ln : f2n[f]) fg.push_back(nodes[ln]);
-          std::vector<i64> key = fg;
-          std::sort(key.begin(), key.end());
+          std::vector<i64> sorted = fg;
+          std::sort(sorted.begin(), sorted.end());
+          const auto key = std::make_pair(k, sorted);
           auto it = dict_f.find(key);
           if (it != dict_f.end()) {
             const std::vector<i64>& jd = it->second.second;
             int nd_by_edge = (int)jd.size() / ne;
             if (nd_by_edge > 0)
               throw std::runtime_error("oracle: face-dof permutation (order >= 4 hexa) not restated");
-            for (size_t d = 0; d < jd.size(); ++d) iglob[offset[c] + fdofs[f][d]] = jd[d];
+            for (size_t d = 0; d < jd.size(); ++d) iglob[off + fdofs[f][d]] = jd[d];
           } else {
             std::vector<i64> g;
-            for (int ld : fdofs[f]) g.push_back(iglob[offset[c] + ld]);
+            for (int ld : fdofs[f]) g.push_back(iglob[off + ld]);
             dict_f[key] = {c, g};
           }
         }
       }
+      }  // components
     }
   }
   iglob = densify(iglob);
@@ -1478,6 +1489,23 @@ int64_t or_dofhandler(int dim, int64_t nnodes, const double* coords, int64_t nce
     std::vector<i64> ig, off;
     std::vector<int> nd;
     i64 tot = dofhandler(m, order, continuous != 0, ig, off, nd);
+    std::copy(ig.begin(), ig.end(), iglob);
+    std::copy(off.begin(), off.end(), offset);
+    return tot;
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// DofHandler(mesh, fs, ncomp, continuous) of a vector space (size = ncomp):
+// iglob 1-based (size ncomp * Σ ndofs), offset per (cell, component).
+int64_t or_dofhandler_n(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                        const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n, int order,
+                        int continuous, int ncomp, int64_t* iglob, int64_t* offset) {
+  try {
+    if (ncomp < 1) throw std::runtime_error("oracle: ncomp < 1");
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    std::vector<i64> ig, off;
+    std::vector<int> nd;
+    i64 tot = dofhandler(m, order, continuous != 0, ig, off, nd, ncomp);
     std::copy(ig.begin(), ig.end(), iglob);
     std::copy(off.begin(), off.end(), offset);
     return tot;

@@ -56,6 +56,8 @@ def lib():
         mesh_args = [C.c_int, C.c_int64, _d, C.c_int64, _i, _l, _l]
         L.or_dofhandler.argtypes = mesh_args + [C.c_int, C.c_int, _l, _l]
         L.or_dofhandler.restype = C.c_int64
+        L.or_dofhandler_n.argtypes = mesh_args + [C.c_int, C.c_int, C.c_int, _l, _l]
+        L.or_dofhandler_n.restype = C.c_int64
         L.or_assemble_bilinear.argtypes = mesh_args + [C.c_int, C.c_int, C.c_int, _i,
                                                        C.c_void_p, C.c_int]
         L.or_assemble_bilinear.restype = C.c_void_p
@@ -187,6 +189,94 @@ def dofhandler(mesh: OMesh, order, continuous=True):
     if n < 0:
         raise RuntimeError(_err())
     return iglob, offset, int(n)
+
+
+def dofhandler_n(mesh: OMesh, order, ncomp, continuous=True):
+    """DofHandler(mesh, fs, ncomp, continuous) (dofhandler.jl:37-156): iglob
+    (1-based), offset (ncells, ncomp) 0-based, ndofs_tot."""
+    nd = {0: (order + 1) ** 2, 1: (order + 1) ** 2, 2: (order + 1) * (order + 2) // 2,
+          3: (order + 1) ** 3, 4: (order + 1) ** 3}
+    total = int(sum(nd[int(t)] for t in mesh.celltypes)) * ncomp
+    iglob = np.zeros(total, dtype=np.int64)
+    offset = np.zeros(len(mesh.celltypes) * ncomp, dtype=np.int64)
+    n = lib().or_dofhandler_n(*mesh.args(), order, int(continuous), ncomp, iglob, offset)
+    if n < 0:
+        raise RuntimeError(_err())
+    return iglob, offset.reshape(-1, ncomp), int(n)
+
+
+def cell_dofs(iglob, offset, ncomp, ndofs_cell, icell):
+    """get_dofs(U, icell) (dofhandler.jl:489-505): 1-based, components in turn."""
+    o = offset[icell, 0]
+    return iglob[o:o + ncomp * ndofs_cell[icell]]
+
+
+def multi_mapping(cell_dofs_of, ndofs, ncells, aos=True):
+    """_build_mapping_AoS / _build_mapping_SoA (fespace.jl:604-640), as loops:
+    cell_dofs_of[i](icell) -> 1-based dofs of space i; returns (ndofs_tot,
+    mappings with mapping[i][d-1] = global 1-based id)."""
+    mapping = [np.zeros(n, dtype=np.int64) for n in ndofs]
+    tot = 0
+    order = ([(c, i) for c in range(ncells) for i in range(len(ndofs))] if aos
+             else [(c, i) for i in range(len(ndofs)) for c in range(ncells)])
+    for c, i in order:
+        for d in cell_dofs_of[i](c):
+            if mapping[i][d - 1] == 0:
+                tot += 1
+                mapping[i][d - 1] = tot
+    assert all(np.all(m != 0) for m in mapping), "invalid mapping"
+    return tot, mapping
+
+
+def assemble_blocks(mesh: OMesh, order, sizes, blocks, aos=True, degree=2, continuous=True):
+    """assemble_bilinear over a MultiFESpace of spaces of sizes[i] components
+    (size 1: scalar) on one mesh and order (assembler.jl:327-420):
+    blocks[(i, j)] = (terms, eta) couples test space i with trial space j; a
+    pair of vector spaces carries the component-diagonal forms ∫c u⋅v,
+    ∫η ∇u ⊡ ∇v.  The COO of each pair (every component pair of its cells'
+    element matrices, zero or not) is mapped through the spaces' numberings
+    and summed by `sparse`.  Returns (colptr, rowval 1-based, nzval, n)."""
+    om = mesh
+    ncells = len(om.celltypes)
+    sig, soff, _ = dofhandler(om, order, continuous)
+    nd = np.diff(np.append(soff, sig.size))
+    vec, dofs_of, ndofs = [], [], []
+    for ncomp in sizes:
+        ig, off, n = dofhandler_n(om, order, ncomp, continuous)
+        # component k's copy of each scalar dof, read off the cells
+        table = np.full((ncomp, int(sig.max())), -1, dtype=np.int64)
+        for c in range(ncells):
+            g = sig[soff[c]:soff[c] + nd[c]] - 1
+            for k in range(ncomp):
+                v = ig[off[c, k]:off[c, k] + nd[c]] - 1
+                assert np.all((table[k, g] == -1) | (table[k, g] == v)), "inconsistent numbering"
+                table[k, g] = v
+        vec.append(table)
+        dofs_of.append(lambda c, ig=ig, off=off, ncomp=ncomp: cell_dofs(ig, off, ncomp, nd, c))
+        ndofs.append(n)
+    ntot, mapping = multi_mapping(dofs_of, ndofs, ncells, aos)
+    rows, cols, vals = [], [], []
+    for (i, j), (terms, eta) in blocks.items():
+        cp, rv, nz, _ = assemble_bilinear(om, order, terms, eta=eta, degree=degree,
+                                          continuous=continuous)
+        hcol = np.repeat(np.arange(cp.size - 1), np.diff(cp))  # trial (column) scalar dof
+        grow = rv - 1                                           # test (row) scalar dof
+        for ki in range(sizes[i]):
+            for kj in range(sizes[j]):
+                diag = sizes[i] == 1 or ki == kj
+                rows.append(mapping[i][vec[i][ki, grow]] - 1)
+                cols.append(mapping[j][vec[j][kj, hcol]] - 1)
+                vals.append(nz if diag else np.zeros_like(nz))
+    r, c, v = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    o = np.lexsort((r, c))
+    r, c, v = r[o], c[o], v[o]
+    new = np.ones(r.size, dtype=bool)
+    new[1:] = (r[1:] != r[:-1]) | (c[1:] != c[:-1])
+    start = np.flatnonzero(new)
+    nzval = np.add.reduceat(v, start) if v.size else v
+    r, c = r[start], c[start]
+    colptr = np.concatenate([[0], np.cumsum(np.bincount(c, minlength=ntot))]) + 1
+    return colptr, r + 1, nzval, ntot
 
 
 def assemble_bilinear(mesh: OMesh, order, terms, eta=None, degree=2, continuous=True):

@@ -115,11 +115,15 @@ class MultiBilinearPlan:
         by_pair = lower_multi_bilinear(items, sizes)
         self._keep, self.blocks = [], []
         pairs, blk = [], []
+        seen = {}  # identical scalar forms share one block
         for (i, j), lowered in by_pair.items():
-            keep, terms, n = _bilinear_bh_terms(lowered, mesh)
-            self._keep.append(keep)
-            b = len(self.blocks)
-            self.blocks.append((terms, n))
+            key = tuple((f, k, id(d) if d is not None else None, sc) for f, k, d, sc in lowered)
+            if key not in seen:
+                keep, terms, n = _bilinear_bh_terms(lowered, mesh)
+                self._keep.append((keep, lowered))
+                seen[key] = len(self.blocks)
+                self.blocks.append((terms, n))
+            b = seen[key]
             for ki in range(sizes[i]):
                 for kj in range(sizes[j]):
                     pairs += [fu.field(i, ki), fu.field(j, kj)]
@@ -151,13 +155,19 @@ class MultiBilinearPlan:
             self._colptr, self._rowval = colptr, rowval
         return self._colptr, self._rowval
 
-    def _blocks(self, mode):
-        # each scalar block into the library's buffer k; the per-cell
-        # coefficient arrays are host arrays, staged by the library
+    def _blocks(self, mode, coef_ptrs=None):
+        # each scalar block into the library's buffer k; per-cell coefficient
+        # arrays are the plan's host arrays (staged by the library) unless
+        # coef_ptrs[k] gives device pointers for block k, in term order
         md = self.mode if mode is None else L.MODE[mode]
         for k, (terms, n) in enumerate(self.blocks):
+            where = L.HOST
+            if coef_ptrs is not None and coef_ptrs[k] is not None:
+                terms = (L.BhTerm * n)(*[L.BhTerm(t.form, t.coef_kind, p, t.scale)
+                                         for t, p in zip(terms, coef_ptrs[k])])
+                where = L.DEVICE
             L.check(L.lib().bh_blocks_assemble_block(self.handle, k, n, terms,
-                                                     self.measure.degree, md, L.HOST),
+                                                     self.measure.degree, md, where),
                     self.ctx, "bh_blocks_assemble_block")
 
     def assemble_host(self, out: np.ndarray | None = None, mode: str | None = None) -> np.ndarray:
@@ -169,8 +179,11 @@ class MultiBilinearPlan:
                                            L.HOST), self.ctx, "bh_blocks_assemble")
         return out
 
-    def assemble_device(self, nzval_ptr: int, mode: str | None = None):
-        self._blocks(mode)
+    def assemble_device(self, nzval_ptr: int, mode: str | None = None, coef_ptrs=None):
+        """Enqueue the assembly into device memory; coef_ptrs: per block (in
+        the order of `blocks`), None or the device pointers of its terms'
+        coefficient arrays (None for a constant term)."""
+        self._blocks(mode, coef_ptrs)
         L.check(L.lib().bh_blocks_assemble(self.handle, len(self.blocks), None,
                                            self.block_of_pair.ctypes.data, nzval_ptr, L.DEVICE),
                 self.ctx, "bh_blocks_assemble")

@@ -134,7 +134,7 @@ struct bh_blocks_s {
   std::vector<int64_t> colptr;    // n + 1
   std::vector<int32_t> rowval;    // nnz, sorted per column
   int32_t* d_src = nullptr;       // scalar entry of each global entry
-  int16_t* d_pair = nullptr;      // its pair
+  uint8_t* d_pair = nullptr;      // its pair
   int32_t* d_maps = nullptr;      // nfields * ns global ids
   double* d_block[kMaxBlocks] = {};  // library-owned scalar block values
   double* d_out = nullptr;        // host-`where` staging of the values
@@ -902,12 +902,12 @@ int bh_blocks_create(bh_pattern p, int nfields, const int64_t* maps, int64_t n, 
   b->nnz = b->colptr[n];
   b->rowval.resize(b->nnz);
   std::vector<int32_t> src(b->nnz);
-  std::vector<int16_t> pr(b->nnz);
+  std::vector<uint8_t> pr(b->nnz);
   // column C = (field fj, scalar dof h): for every pair (fi, fj), the rows
   // maps[fi][g] of the scalar column h (= scalar row h, the pattern is
   // symmetric), whose values sit at the scalar CSC positions of row h's run
   parallel_for(n, [&](int64_t c0, int64_t c1) {
-    struct Ent { int32_t r; int32_t e; int16_t q; };
+    struct Ent { int32_t r; int32_t e; uint8_t q; };
     std::vector<Ent> col;
     for (int64_t C = c0; C < c1; ++C) {
       if (owner[C] < 0) continue;
@@ -917,7 +917,7 @@ int bh_blocks_create(bh_pattern p, int nfields, const int64_t* maps, int64_t n, 
       for (int q : by_col[fj]) {
         const int fi = pairs[2 * q];
         for (int64_t e = h.rowptr[hh]; e < h.rowptr[hh + 1]; ++e)
-          col.push_back({(int32_t)maps[(int64_t)fi * ns + h.colval[e]], (int32_t)e, (int16_t)q});
+          col.push_back({(int32_t)maps[(int64_t)fi * ns + h.colval[e]], (int32_t)e, (uint8_t)q});
       }
       std::sort(col.begin(), col.end(), [](const Ent& a, const Ent& z) { return a.r < z.r; });
       int64_t o = b->colptr[C];

@@ -82,7 +82,7 @@ struct BlockVals {
   const double* v[kMaxBlocks];
   int8_t blk[kMaxPairs];
 };
-hipError_t launch_blocks_expand(hipStream_t s, const int32_t* src, const int16_t* pair, int64_t n,
+hipError_t launch_blocks_expand(hipStream_t s, const int32_t* src, const uint8_t* pair, int64_t n,
                                 const BlockVals& bv, double* out);
 // out[map[g]] += v[g] for g < n (the targets of one field are distinct).
 hipError_t launch_add_field(hipStream_t s, double* out, const int32_t* map, const double* v,

@@ -24,6 +24,7 @@
 // cell order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 
@@ -603,12 +604,18 @@ hipError_t launch_set_dofs(hipStream_t s, double* b, const int64_t* dofs, const 
 }
 
 // Global CSC values of a multi-field matrix from its scalar blocks.  VEC:
-// four entries per thread (one 16-byte load of src, 8 bytes of pair indices,
-// two 16-byte stores; out 16-byte aligned); the few entries past the last
-// group of four go to the first threads.
+// each wave takes 512 consecutive entries as four chunks of 128, lane l the
+// entries 2l, 2l+1 of each chunk, so every load and store instruction covers
+// one contiguous span of the wave (8-byte src pairs, 2-byte pair indices,
+// 16-byte stores) and each lane has eight independent value loads in flight;
+// the entries past the last group of 512 go to the first threads.  src 8-byte,
+// out 16-byte and pair 2-byte aligned.
+typedef int32_t i32x2_t __attribute__((ext_vector_type(2)));
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_blocks_expand(const int32_t* __restrict__ src,
-                                                       const int16_t* __restrict__ pair,
+                                                       const uint8_t* __restrict__ pair,
                                                        int64_t n, BlockVals bv,
                                                        double* __restrict__ out) {
   __shared__ const double* s_v[kMaxBlocks];
@@ -617,27 +624,39 @@ __global__ __launch_bounds__(256) void k_blocks_expand(const int32_t* __restrict
   if (threadIdx.x < kMaxBlocks) s_v[threadIdx.x] = bv.v[threadIdx.x];
   s_b[threadIdx.x] = bv.blk[threadIdx.x];
   __syncthreads();
-  auto value = [&](int32_t e, int16_t p) -> double {
+  auto value = [&](int32_t e, uint32_t p) -> double {
     const int b = s_b[p];
     return b < 0 ? 0.0 : s_v[b][e];
   };
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t done = 0;
   if constexpr (VEC) {
-    const int64_t n4 = n / 4;
-    const int4* s4 = reinterpret_cast<const int4*>(src);
-    const short4* p4 = reinterpret_cast<const short4*>(pair);
-    double2* o2 = reinterpret_cast<double2*>(out);
-    for (int64_t q = t0; q < n4; q += stride) {
-      const int4 e = s4[q];
-      const short4 pr = p4[q];
-      o2[2 * q] = make_double2(value(e.x, pr.x), value(e.y, pr.y));
-      o2[2 * q + 1] = make_double2(value(e.z, pr.z), value(e.w, pr.w));
+    const int lane = threadIdx.x & 63;
+    const int64_t ngrp = n / 512;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t g = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); g < ngrp;
+         g += nw) {
+      const int64_t e0 = g * 512 + 2 * lane;
+      i32x2_t e[4];
+      uint16_t pp[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        e[k] = *reinterpret_cast<const i32x2_t*>(src + e0 + 128 * k);
+        pp[k] = *reinterpret_cast<const uint16_t*>(pair + e0 + 128 * k);
+      }
+      f64x2_t v[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        v[k].x = value(e[k].x, pp[k] & 0xffu);
+        v[k].y = value(e[k].y, pp[k] >> 8);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) *reinterpret_cast<f64x2_t*>(out + e0 + 128 * k) = v[k];
     }
-    done = 4 * n4;
+    done = 512 * ngrp;
   }
-  for (int64_t i = done + t0; i < n; i += stride) out[i] = value(src[i], pair[i]);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = done + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = value(src[i], pair[i]);
 }
 
 __global__ void k_add_field(double* __restrict__ out, const int32_t* __restrict__ map,
@@ -647,17 +666,20 @@ __global__ void k_add_field(double* __restrict__ out, const int32_t* __restrict_
     out[map[i]] += v[i];
 }
 
-hipError_t launch_blocks_expand(hipStream_t s, const int32_t* src, const int16_t* pair, int64_t n,
+hipError_t launch_blocks_expand(hipStream_t s, const int32_t* src, const uint8_t* pair, int64_t n,
                                 const BlockVals& bv, double* out) {
   if (n <= 0) return hipSuccess;
-  const bool vec = ((uintptr_t)out % 16 == 0) && ((uintptr_t)src % 16 == 0) &&
-                   ((uintptr_t)pair % 8 == 0);
-  if (vec)
-    hipLaunchKernelGGL(k_blocks_expand<true>, grid_1d((n + 3) / 4), dim3(256), 0, s, src, pair,
+  const bool vec = ((uintptr_t)out % 16 == 0) && ((uintptr_t)src % 8 == 0) &&
+                   ((uintptr_t)pair % 2 == 0);
+  if (vec) {
+    // four waves (groups of 512 entries) per block
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n / 512 + 3) / 4, 16384));
+    hipLaunchKernelGGL(k_blocks_expand<true>, dim3((unsigned)blocks), dim3(256), 0, s, src, pair,
                        n, bv, out);
-  else
+  } else {
     hipLaunchKernelGGL(k_blocks_expand<false>, grid_1d(n), dim3(256), 0, s, src, pair, n, bv,
                        out);
+  }
   return hipGetLastError();
 }
 

@@ -532,6 +532,112 @@ def bench_c4(args, torch, B):
     return line
 
 
+def cpu_baseline_vec(n_sample=22):
+    """Oracle block assembly (C++ scalar assembly + the numpy restatement of
+    the per-field COO mapping and `sparse`), 1 thread, on a smaller mesh."""
+    import bcubehip as B
+    from oracle import oracle as O
+
+    m = B.hexa_mesh(n_sample + 1, n_sample + 1, n_sample + 1, backend=B.BcubeBackendHostOnly())
+    B.jitter_nodes(m, 0.15, 1.0 / n_sample, seed=0)
+    om = O.OMesh(m.dim, m.coords, m.celltypes, m.c2n_ptr, m.c2n)
+    eta = np.random.default_rng(1).uniform(0.5, 2.0, m.ncells)
+    t0 = time.perf_counter()
+    _, _, nzval, _ = O.assemble_blocks(om, 2, [3], {(0, 0): (["stiffness", "mass"], eta)},
+                                       degree=5)
+    dt = time.perf_counter() - t0
+    return {"value": nzval.size / dt / 1e9, "unit": "GNNZ/s", "cores": 1, "kind": "port",
+            "sample": f"hexa {n_sample}^3 Q2 x3 components, ∫η∇u⊡∇v+∫u⋅v, COO over all "
+                      f"component pairs + sparse(), {nzval.size} nnz in {dt:.2f} s"}
+
+
+def bench_vec(args, torch, B):
+    """Vector space (size 3) on hexa n^3 Q2: ∫η∇u⊡∇v dΩ + ∫u⋅v dΩ into the
+    global CSC (every component pair of every coupled dof pair, 9 x the scalar
+    nnz).  One step = the scalar block (element kernel + row gather) and the
+    block expansion (k_blocks_expand) into device-resident values."""
+    n = args.n if args.n != 64 else 48
+    t_setup = time.perf_counter()
+    m = B.hexa_mesh(n + 1, n + 1, n + 1)
+    B.jitter_nodes(m, 0.15, 1.0 / n, seed=0)
+    eta = np.random.default_rng(1).uniform(0.5, 2.0, m.ncells)
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", 2), m, size=3)
+    dΩ = B.Measure(B.CellDomain(m), 5)
+    ηc = B.MeshCellData(eta)
+    plan = B.MultiBilinearPlan(lambda u, v: B.integrate(ηc * B.ddot(B.grad(u), B.grad(v)), dΩ)
+                               + B.integrate(B.dot(u, v), dΩ), U, B.TestFESpace(U))
+    t_setup = time.perf_counter() - t_setup
+    ncells, _, nnz_s, _, f_alg = workload(n)
+    nnz = plan.nnz
+    assert nnz == 9 * nnz_s, (nnz, nnz_s)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    backend = m.backend
+    backend.set_stream(stream.cuda_stream)
+    backend.set_autotune(True)
+    eta_d = torch.from_numpy(eta).to(dev)
+    coef = [[eta_d.data_ptr(), None]]
+    modes = ["fp64", "fp32"] if args.mode == "all" else [args.mode]
+    results, ref = {}, None
+    for mode in modes:
+        out = torch.empty(nnz, dtype=torch.float64, device=dev)
+        for _ in range(args.warmup):
+            plan.assemble_device(out.data_ptr(), mode, coef)
+        torch.cuda.synchronize()
+        backend.set_timing(True)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            plan.assemble_device(out.data_ptr(), mode, coef)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        launches, kinds = backend.timings_ex(cap=100000)
+        backend.set_timing(False)
+        t, k = np.asarray(launches, dtype=np.float64), np.asarray(kinds)
+        ex_ms = float(t[k == 2].sum() / args.steps)
+        # algorithmic bytes of the expansion: every global value written once
+        # and each scalar block value read once; the 5-byte per-entry source
+        # (int32 scalar entry + uint8 pair) is listed apart
+        ex_bytes = 8.0 * nnz + 8.0 * nnz_s
+        ex = {"bound": "hbm", "achieved": ex_bytes / (ex_ms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+              "unit": "GB/s", "kernel": "k_blocks_expand<true>",
+              "avg_launch_ms": ex_ms, "alg_bytes_per_launch": ex_bytes,
+              "index_bytes_per_launch": 5.0 * nnz}
+        ex["frac"] = ex["achieved"] / PEAK_HBM_GBS
+        _, scalar_kernels = kernel_rooflines(mode, launches, kinds, args.steps, f_alg, nnz_s)
+        kernels = scalar_kernels + [ex]
+        dom = max(kernels, key=lambda x: x["avg_launch_ms"])
+        err = None
+        if mode == "fp64":
+            ref = out
+        elif ref is not None:
+            err = float((out - ref).abs().max() / ref.abs().max())
+        results[mode] = {"ms_per_step": ms, "gnnz_per_s": nnz / (ms * 1e-3) / 1e9,
+                         "rel_err_vs_fp64_gpu": err, "roofline": dom, "kernels": kernels}
+        if mode != "fp64":
+            del out
+    head = "fp32" if "fp32" in results else modes[0]
+    r = results[head]
+    roof = dict(r["roofline"])
+    roof["traffic"] = None
+    line = {
+        "metric": "vector (3-component) Q2 hex assembly GNNZ/s on 1 GPU",
+        "value": r["gnnz_per_s"], "unit": "GNNZ/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[head], "data": "synthetic",
+        "config": {"workload": f"vector hexa {n}^3 Q2 x3 (TrialFESpace size=3), jitter ±0.15h "
+                               f"seed 0, η~U[0.5,2] seed 1, ∫η∇u⊡∇v+∫u⋅v, 27-pt Gauss, "
+                               f"values into device-resident CSC",
+                   "mode": head, "ncells": ncells, "ndofs": plan.nrows, "nnz": nnz,
+                   "scalar_nnz": nnz_s, "setup_s": round(t_setup, 2)},
+        "roofline": roof, "modes": results,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_vec()
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -540,10 +646,11 @@ def main():
     ap.add_argument("--mode", default="all", choices=["all", "fp64", "fp32", "bf16x3"])
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"],
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "vec"],
                     help="c2: the headline 64^3 Q2 hexa workload; c1: the 64^2 Q1 Poisson "
                          "matrix + right-hand side; c3: mixed tri/quad order buckets; c4: the DG "
-                         "transport residual; c5: the many-instance batch")
+                         "transport residual; c5: the many-instance batch; vec: a 3-component "
+                         "Q2 hexa space (48^3 unless --n)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters against a known byte count)")
@@ -559,6 +666,9 @@ def main():
 
     if args.config == "c1":
         print(json.dumps(bench_c1(args, torch, B)), flush=True)
+        return
+    if args.config == "vec":
+        print(json.dumps(bench_vec(args, torch, B)), flush=True)
         return
     if args.config == "c5":
         print(json.dumps(bench_c5(args, torch, B)), flush=True)

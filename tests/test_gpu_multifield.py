@@ -80,6 +80,50 @@ def test_multifespace_vs_oracle(aos):
     assert np.array_equal(dev.cpu().numpy(), plan.assemble_host())
 
 
+@pytest.mark.parametrize("aos", [True, False])
+def test_fully_coupled_multifespace_vs_oracle(aos):
+    """All four pairs of two scalar fields coupled (off-diagonal blocks carry
+    values), against the oracle."""
+    m, _ = B.mixed_tri_quad_mesh(8, block=2, p_split=0.5, seed=8)
+    B.jitter_nodes(m, 0.2, 0.125, seed=9)
+    sp = [B.TrialFESpace(LAG(2), m), B.TrialFESpace(LAG(2), m)]
+    U = B.MultiFESpace(*sp, arrayOfStruct=aos)
+    V = B.MultiFESpace(*[B.TestFESpace(s) for s in sp], arrayOfStruct=aos)
+    dΩ = B.Measure(B.CellDomain(m), 4)
+
+    def a(u, v):
+        return B.integrate(u[0] * v[0] + B.dot(B.grad(u[1]), B.grad(v[1])) + u[0] * v[1]
+                           + 3.0 * B.dot(B.grad(u[1]), B.grad(v[0])), dΩ)
+    plan = B.MultiBilinearPlan(a, U, V)
+    blocks = {(0, 0): (["mass"], None), (1, 1): (["stiffness"], None), (1, 0): (["mass"], None),
+              (0, 1): (["stiffness"], np.full(m.ncells, 3.0))}
+    e64, e32 = _check(plan, O.assemble_blocks(_om(m), 2, [1, 1], blocks, aos=aos, degree=4))
+    print(f"fully coupled aos={aos}: FP64 {e64:.2e}, FP32 {e32:.2e}")
+
+
+def test_many_fields_vs_oracle():
+    """17 fields, 33 coupled pairs sharing one scalar block, against the oracle."""
+    m, _ = B.mixed_tri_quad_mesh(4, block=2, p_split=0.5, seed=2)
+    nf = 17
+    sp = [B.TrialFESpace(LAG(1), m) for _ in range(nf)]
+    U = B.MultiFESpace(*sp)
+    V = B.MultiFESpace(*[B.TestFESpace(s) for s in sp])
+    dΩ = B.Measure(B.CellDomain(m), 2)
+
+    def a(u, v):
+        e = B.integrate(u[0] * v[0], dΩ)
+        for i in range(1, nf):
+            e = e + B.integrate(u[i] * v[i] + u[i - 1] * v[i], dΩ)
+        return e
+    plan = B.MultiBilinearPlan(a, U, V)
+    assert len(plan.blocks) == 1
+    blocks = {(0, 0): (["mass"], None)}
+    for i in range(1, nf):
+        blocks[(i, i)] = (["mass"], None)
+        blocks[(i, i - 1)] = (["mass"], None)
+    _check(plan, O.assemble_blocks(_om(m), 1, [1] * nf, blocks, degree=2))
+
+
 def test_vector_linear_form_matches_scalar_path():
     # linear-single-vector-fe (test_assembler.jl:95-129) on quads: f = (x, x/2)
     m = B.rectangle_mesh(6, 5)

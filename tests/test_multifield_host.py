@@ -207,3 +207,53 @@ def test_oracle_block_values_invariants():
                                       aos=False, degree=4)
     A = sp.csc_matrix((nz, rv - 1, cp - 1), shape=(n, n))
     assert n == 2 * ns and (A - sp.block_diag([S, S])).count_nonzero() == 0
+
+
+def _full_coupling(m, aos):
+    sp = [B.TrialFESpace(LAG(2), m), B.TrialFESpace(LAG(2), m)]
+    U = B.MultiFESpace(*sp, arrayOfStruct=aos)
+    V = B.MultiFESpace(*[B.TestFESpace(s) for s in sp], arrayOfStruct=aos)
+    dΩ = B.Measure(B.CellDomain(m), 4)
+
+    def a(u, v):
+        return B.integrate(u[0] * v[0] + B.dot(B.grad(u[1]), B.grad(v[1])) + u[0] * v[1]
+                           + 3.0 * B.dot(B.grad(u[1]), B.grad(v[0])), dΩ)
+    blocks = {(0, 0): (["mass"], None), (1, 1): (["stiffness"], None), (1, 0): (["mass"], None),
+              (0, 1): (["stiffness"], np.full(m.ncells, 3.0))}
+    return U, V, a, blocks
+
+
+def test_full_coupling_and_many_fields_patterns():
+    """Fully coupled MultiFESpaces (AoS, SoA) and 17 fields whose 33 coupled
+    pairs share one scalar block: patterns bit-exact with the oracle."""
+    m = _mesh("mixed")
+    for aos in (True, False):
+        U, V, a, blocks = _full_coupling(m, aos)
+        plan = B.MultiBilinearPlan(a, U, V)
+        ocp, orv, _, _ = O.assemble_blocks(_om(m), 2, [1, 1], blocks, aos=aos, degree=4)
+        cp, rv = plan.csc_structure()
+        assert np.array_equal(cp + 1, ocp) and np.array_equal(rv + 1, orv)
+    U, V, a, blocks = _many_fields(m, 17)
+    plan = B.MultiBilinearPlan(a, U, V)
+    assert len(plan.blocks) == 1
+    ocp, orv, _, _ = O.assemble_blocks(_om(m), 1, [1] * 17, blocks, degree=2)
+    cp, rv = plan.csc_structure()
+    assert np.array_equal(cp + 1, ocp) and np.array_equal(rv + 1, orv)
+
+
+def _many_fields(m, nf):
+    sp = [B.TrialFESpace(LAG(1), m) for _ in range(nf)]
+    U = B.MultiFESpace(*sp)
+    V = B.MultiFESpace(*[B.TestFESpace(s) for s in sp])
+    dΩ = B.Measure(B.CellDomain(m), 2)
+
+    def a(u, v):
+        e = B.integrate(u[0] * v[0], dΩ)
+        for i in range(1, nf):
+            e = e + B.integrate(u[i] * v[i] + u[i - 1] * v[i], dΩ)
+        return e
+    blocks = {(0, 0): (["mass"], None)}
+    for i in range(1, nf):
+        blocks[(i, i)] = (["mass"], None)
+        blocks[(i, i - 1)] = (["mass"], None)
+    return U, V, a, blocks

@@ -62,7 +62,9 @@ class DofHandler:
     """
 
     def __init__(self, mesh: Mesh, fs: FunctionSpace, ncomponents: int = 1,
-                 isContinuous: bool = True):
+                 isContinuous: bool = True, device: bool = False):
+        """device=True numbers on the GPU (bh_dofhandler_device, the same
+        ids), for meshes that change; it needs a mesh on a HIP backend."""
         if ncomponents < 1:
             raise ValueError("ncomponents must be >= 1")
         self.ncomponents = int(ncomponents)
@@ -76,9 +78,17 @@ class DofHandler:
         self.iglob = np.zeros(total, dtype=np.int64)
         self.offset = np.zeros(mesh.ncells, dtype=np.int64)
         n = C.c_int64()
-        rc = L.lib().bh_dofhandler(_host_mesh(mesh), fs.degree, int(self.continuous),
-                                   self.iglob.ctypes.data, self.offset.ctypes.data, C.byref(n))
-        L.check(rc, None, "bh_dofhandler")  # host-only handle: no GPU context needed
+        if device:
+            ctx = mesh.backend.ctx
+            rc = L.lib().bh_dofhandler_device(mesh.handle, fs.degree, int(self.continuous),
+                                              self.iglob.ctypes.data, self.offset.ctypes.data,
+                                              C.byref(n))
+            L.check(rc, ctx, "bh_dofhandler_device")
+        else:
+            rc = L.lib().bh_dofhandler(_host_mesh(mesh), fs.degree, int(self.continuous),
+                                       self.iglob.ctypes.data, self.offset.ctypes.data,
+                                       C.byref(n))
+            L.check(rc, None, "bh_dofhandler")  # host-only handle: no GPU context needed
         self.ndofs_tot = int(n.value)
         self._ndofs_cell = np.diff(np.append(self.offset, total))
         if self.ncomponents > 1:

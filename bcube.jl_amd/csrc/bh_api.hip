@@ -540,6 +540,23 @@ int bh_dofhandler(bh_mesh m, int order, int continuous, int64_t* iglob, int64_t*
   return BH_OK;
 }
 
+int bh_dofhandler_device(bh_mesh m, int order, int continuous, int64_t* iglob, int64_t* offset,
+                         int64_t* ndofs_tot) {
+  if (!m || !iglob || !offset || !ndofs_tot || order < 0) return BH_ERR_ARG;
+  bh_ctx c = m->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "the device numbering needs a mesh created with a context");
+  if (c->capturing) return fail(c, BH_ERR_ARG, "the device numbering cannot be captured");
+  if (int r = set_device(c)) return r;
+  std::vector<int64_t> ig, off;
+  std::string err;
+  const int64_t n = dofhandler_device(c->stream, m->h, order, continuous != 0, ig, off, err);
+  if (n < 0) return fail(c, BH_ERR_UNSUPPORTED, err);
+  std::copy(ig.begin(), ig.end(), iglob);
+  std::copy(off.begin(), off.end(), offset);
+  *ndofs_tot = n;
+  return BH_OK;
+}
+
 static int make_space(bh_mesh m, int order, bool continuous, int64_t ndofs,
                       const std::vector<int64_t>& iglob, const std::vector<int64_t>& offset,
                       bh_space* out) {

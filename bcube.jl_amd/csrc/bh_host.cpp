@@ -475,6 +475,53 @@ static Key9 make_key(const std::vector<i64>& nodes) {
   return key;
 }
 
+bool dof_table(int ct, int order, DofTab& t) {
+  t = DofTab();
+  const int s = shape_of(ct);
+  if (s < 0 || order < 0 || order > 4) return false;
+  auto fill = [](auto& a) { for (auto& row : a) for (auto& v : row) v = -1; };
+  for (int& v : t.vdof) v = -1;
+  fill(t.enode); fill(t.edof); fill(t.fnode); fill(t.fdof); fill(t.snode);
+  t.nd = ndofs_lagrange(s, order);
+  t.dim = dim_of_shape(s);
+  const Lists vd = vertex_dofs(s, order);
+  t.nv = std::min((int)vd.size(), nnodes_of(ct));
+  for (int v = 0; v < t.nv; ++v) {
+    if (vd[v].size() > 1) return false;
+    t.vdof[v] = vd[v].empty() ? -1 : vd[v][0];
+  }
+  if (t.dim > 1) {
+    const Lists el = edges_of(ct), ed = edge_dofs(s, order);
+    if (order > 0 && s == TRIANGLE && ed.size() != el.size()) return false;
+    t.ne = (int)std::min(el.size(), ed.size());
+    for (int e = 0; e < t.ne; ++e) {
+      t.elen = (int)el[e].size();
+      for (int k = 0; k < t.elen; ++k) t.enode[e][k] = el[e][k];
+      t.ned = (int)ed[e].size();
+      if (t.ned > 3) return false;
+      for (int k = 0; k < t.ned; ++k) t.edof[e][k] = ed[e][k];
+    }
+  }
+  if (t.dim > 2) {
+    const Lists fl = faces_of(ct), fd = face_dofs(s, order);
+    t.nf = (int)std::min(fl.size(), fd.size());
+    for (int f = 0; f < t.nf; ++f) {
+      t.flen = (int)fl[f].size();
+      for (int k = 0; k < t.flen; ++k) t.fnode[f][k] = fl[f][k];
+      t.nfd = (int)fd[f].size();
+      if (t.nfd > 9) return false;
+      for (int k = 0; k < t.nfd; ++k) t.fdof[f][k] = fd[f][k];
+    }
+  }
+  const Lists sl = faces_of(ct);
+  t.nside = (int)sl.size();
+  for (int q = 0; q < t.nside; ++q) {
+    t.slen = (int)sl[q].size();
+    for (int k = 0; k < t.slen; ++k) t.snode[q][k] = sl[q][k];
+  }
+  return true;
+}
+
 i64 dofhandler(const HostMesh& m, int order, bool continuous, std::vector<i64>& iglob,
                std::vector<i64>& offset, std::string& err) {
   i64 nc = m.ncells;

@@ -1,6 +1,8 @@
 // bh_internal.h -- shared declarations of the bcubehip library (not part of the ABI).
 #pragma once
 
+#include <hip/hip_runtime_api.h>
+
 #include <cstdint>
 #include <functional>
 #include <string>
@@ -106,6 +108,36 @@ struct HostDGFaces {
   std::vector<int32_t> cs;
 };
 bool build_dg_faces(const HostMesh& m, HostDGFaces& F, std::string& err);
+
+// Local dof tables of one cell type and Lagrange order, as the host
+// DofHandler walks them (dofhandler.jl:180-388): vertex dofs, edges (node
+// lists in edges2nodes order and their dofs, first vertex to second), faces
+// (faces2nodes order) and the sides oriented_cell_side searches (the faces;
+// the edges in 2D).  Node indices are cell-local.  Fixed sizes for a device
+// copy; unused entries are -1.
+struct DofTab {
+  int nd = 0;               // dofs per cell
+  int dim = 0;
+  int nv = 0;               // vertices with a dof list (zip of lists and nodes)
+  int vdof[8];              // the vertex's dof, -1 if none (order 0)
+  int ne = 0, elen = 0;     // edges, nodes per edge (2, or 3 on quadratic cells)
+  int enode[12][3];
+  int ned = 0;              // dofs per edge
+  int edof[12][3];
+  int nf = 0, flen = 0;     // faces, nodes per face (4 or 9)
+  int fnode[6][9];
+  int nfd = 0;              // dofs per face
+  int fdof[6][9];
+  int nside = 0, slen = 0;  // sides (faces; edges in 2D), nodes per side
+  int snode[6][9];
+};
+// false if (ct, order) has no table (order > 4, or the reference's own limits).
+bool dof_table(int ct, int order, DofTab& t);
+
+// The DofHandler numbering computed by kernels on `stream` (bh_build.hip);
+// same iglob/offset as dofhandler().  Returns ndofs or -1 with err set.
+int64_t dofhandler_device(hipStream_t stream, const HostMesh& m, int order, bool continuous,
+                          std::vector<i64>& iglob, std::vector<i64>& offset, std::string& err);
 
 // Simple host thread pool loop.
 void parallel_for(i64 n, const std::function<void(i64, i64)>& body);

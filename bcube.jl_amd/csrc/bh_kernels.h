@@ -69,6 +69,8 @@ bool kernel_supported(int geo_celltype, int order, int nq1);
 int packed_table_size(int geo_celltype, int order, int nq1);
 hipError_t launch(int geo_celltype, int order, int nq1, const LaunchArgs& a);
 hipError_t launch_fill(hipStream_t s, double* p, int64_t n, double v);
+hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
+hipError_t launch_fill_u64(hipStream_t s, uint64_t* p, int64_t n, uint64_t v);
 // Dirichlet edits (bh_dirichlet_*): v[pos[i]] = val, and b[dofs[i]] =
 // values ? values[src[i]] : 0.
 hipError_t launch_set_at(hipStream_t s, double* v, const int64_t* pos, int64_t n, double val);

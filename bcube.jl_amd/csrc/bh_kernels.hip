@@ -690,6 +690,25 @@ hipError_t launch_add_field(hipStream_t s, double* out, const int32_t* map, cons
   return hipGetLastError();
 }
 
+template <class T>
+__global__ void k_fill_t(T* __restrict__ p, int64_t n, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_t<int32_t>, grid_1d(n), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(hipStream_t s, uint64_t* p, int64_t n, uint64_t v) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_t<uint64_t>, grid_1d(n), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill(hipStream_t s, double* p, int64_t n, double v) {
   int64_t blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;

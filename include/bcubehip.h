@@ -120,6 +120,14 @@ int bh_mesh_destroy(bh_mesh mesh);
  * iglob: Σ_c ndofs(c), 1-based global ids; offset: ncells. */
 int bh_dofhandler(bh_mesh mesh, int order, int continuous, int64_t* iglob, int64_t* offset,
                   int64_t* ndofs_tot);
+/* The same numbering computed by kernels on the context's stream, for meshes
+ * that change (no counterpart in the reference, whose DofHandler walks the
+ * cells on the CPU): vertex, edge and face owners by atomicMin (edges and faces
+ * in a device hash table), owner slots with the edge orientation rule of
+ * oriented_cell_side, ids by a scan.  Same iglob/offset bit for bit; needs a
+ * mesh with a context; the outputs are host arrays. */
+int bh_dofhandler_device(bh_mesh mesh, int order, int continuous, int64_t* iglob,
+                         int64_t* offset, int64_t* ndofs_tot);
 /* TrialFESpace / TestFESpace (src/fespace/fespace.jl:186-414): numbering by
  * bh_dofhandler, kept on the device as a per-cell dof map. */
 int bh_space_create(bh_mesh mesh, int order, int continuous, bh_space* out);

@@ -238,7 +238,9 @@ class BilinearPlan:
     (time stepping, Newton).  The pattern is built once; `assemble` writes the
     CSC values into HBM (device pointer) or into a host array."""
 
-    def __init__(self, a, U, V, mode: str = "fp64"):
+    def __init__(self, a, U, V, mode: str = "fp64", build: str = "host"):
+        """build="device" builds the pattern by kernels (bh_pattern_build_device,
+        the same structure), for meshes that change."""
         U = getattr(U, "U", U)
         V = getattr(V, "U", V)
         if U is not V:
@@ -253,7 +255,8 @@ class BilinearPlan:
         self._keep, self.terms, self.nterms = _bilinear_bh_terms(lowered, V.mesh)
         ctx = V.mesh.backend.ctx
         h = C.c_void_p()
-        L.check(L.lib().bh_pattern_build(U.handle, V.handle, C.byref(h)), ctx, "bh_pattern_build")
+        fn = {"host": "bh_pattern_build", "device": "bh_pattern_build_device"}[build]
+        L.check(getattr(L.lib(), fn)(U.handle, V.handle, C.byref(h)), ctx, fn)
         self.handle = h
         nr, nc, nnz, ncol = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
         L.check(L.lib().bh_pattern_info(h, C.byref(nr), C.byref(nc), C.byref(nnz), C.byref(ncol)),

@@ -682,6 +682,28 @@ int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out) {
   return BH_OK;
 }
 
+int bh_pattern_build_device(bh_space U, bh_space V, bh_pattern* out) {
+  if (!U || !V || !out) return BH_ERR_ARG;
+  *out = nullptr;
+  bh_ctx c = V->mesh->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "the device pattern build needs a mesh with a context");
+  if (U != V) return fail(c, BH_ERR_UNSUPPORTED, "the device pattern build needs U == V");
+  if (c->capturing) return fail(c, BH_ERR_ARG, "the device pattern build cannot be captured");
+  if (int r = set_device(c)) return r;
+  auto p = std::make_unique<bh_pattern_s>();
+  p->U = U;
+  p->V = V;
+  p->dev = V->dev;
+  std::string err;
+  if (!pattern_device(c->stream, V->d_dofs, V->nl, V->ndofs, V->mesh->h.ncells, p->h,
+                      &p->d_rowptr, &p->d_pptr, &p->d_pairs, &p->d_goff, err))
+    return fail(c, BH_ERR_UNSUPPORTED, err);
+  BH_HIP(c, dalloc(&p->d_vals, (size_t)std::max<int64_t>(p->h.nnz, 1)));
+  BH_HIP(c, dalloc(&p->d_coef, (size_t)2 * V->mesh->h.ncells));
+  *out = p.release();
+  return BH_OK;
+}
+
 int bh_pattern_tuning(bh_pattern p, int mode, int32_t* shape, float* ms) {
   if (!p || (mode != BH_FP64 && mode != BH_FP32 && mode != BH_BF16X3)) return BH_ERR_ARG;
   const Tuning& tu = p->tune[mode == BH_FP64 ? 0 : (mode == BH_FP32 ? 1 : 2)];

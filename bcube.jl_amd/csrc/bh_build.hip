@@ -354,7 +354,297 @@ hipError_t alloc(T** p, size_t n) {
   return hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T));
 }
 
+// ---- pattern ---------------------------------------------------------------------------
+constexpr int kRowCap = 2048;  // candidate columns of one row (adjacent cells x nl) sorted in LDS
+constexpr int kWaves = 4;
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void k_pat_count(const int32_t* __restrict__ dofs, int64_t nslots,
+                            int32_t* __restrict__ cnt) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nslots;
+       k += (int64_t)gridDim.x * blockDim.x)
+    if (dofs[k] >= 0) atomicAdd(&cnt[dofs[k]], 1);
+}
+
+// pptr[0] = 0, pptr[r+1] = inclusive count; the largest row adjacency
+__global__ void k_pat_pptr(const int32_t* __restrict__ incl, const int32_t* __restrict__ cnt,
+                           int64_t nrows, int32_t* __restrict__ pptr, int32_t* __restrict__ stat) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    pptr[r + 1] = incl[r];
+    if (r == 0) pptr[0] = 0;
+    atomicMax(&stat[0], cnt[r]);
+  }
+}
+
+// element rows of each CSR row, any order (sorted next)
+__global__ void k_pat_scatter(const int32_t* __restrict__ dofs, int64_t nslots,
+                              const int32_t* __restrict__ pptr, int32_t* __restrict__ fillc,
+                              int32_t* __restrict__ pairs) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nslots;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t r = dofs[k];
+    if (r >= 0) pairs[pptr[r] + atomicAdd(&fillc[r], 1)] = (int32_t)k;
+  }
+}
+
+// cell order: each row's element rows c*nl + i ascending (one per cell)
+__global__ void k_pat_sort_pairs(const int32_t* __restrict__ pptr, int64_t nrows,
+                                 int32_t* __restrict__ pairs) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    int32_t* a = pairs + pptr[r];
+    const int n = pptr[r + 1] - pptr[r];
+    for (int i = 1; i < n; ++i)
+      for (int j = i; j > 0 && a[j - 1] > a[j]; --j) {
+        const int32_t t = a[j];
+        a[j] = a[j - 1];
+        a[j - 1] = t;
+      }
+  }
+}
+
+// One wave per row: gather the dofs of the row's cells, bitonic-sort them in
+// LDS, keep the distinct ones.  PASS 0 writes the row length (and the largest
+// into stat[1]); PASS 1 writes the columns at rowptr[r].
+template <int PASS>
+__global__ __launch_bounds__(64 * kWaves) void k_pat_rows(
+    const int32_t* __restrict__ dofs, int nl, const int32_t* __restrict__ pptr,
+    const int32_t* __restrict__ pairs, int64_t nrows, int32_t* __restrict__ len,
+    const int64_t* __restrict__ rowptr, int32_t* __restrict__ colval,
+    int32_t* __restrict__ stat) {
+  __shared__ int32_t sbuf[kWaves][kRowCap];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int32_t* buf = sbuf[w];
+  const int64_t nw = (int64_t)gridDim.x * kWaves;
+  for (int64_t r = (int64_t)blockIdx.x * kWaves + w; r < nrows; r += nw) {
+    const int32_t q0 = pptr[r], na = pptr[r + 1] - q0;
+    const int n = na * nl;
+    int S = 1;
+    while (S < n) S <<= 1;
+    for (int t = lane; t < S; t += 64) {
+      int32_t v = INT32_MAX;
+      if (t < n) {
+        const int32_t cell = pairs[q0 + t / nl] / nl;
+        const int32_t d = dofs[(int64_t)cell * nl + t % nl];
+        v = d >= 0 ? d : INT32_MAX;
+      }
+      buf[t] = v;
+    }
+    wave_sync();
+    for (int k = 2; k <= S; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int t = lane; t < S; t += 64) {
+          const int u = t ^ j;
+          if (u > t) {
+            const int32_t a = buf[t], b = buf[u];
+            if ((a > b) == ((t & k) == 0)) {
+              buf[t] = b;
+              buf[u] = a;
+            }
+          }
+        }
+        wave_sync();
+      }
+    int count = 0;
+    for (int t0 = 0; t0 < S; t0 += 64) {
+      const int t = t0 + lane;
+      const bool keep = t < S && buf[t] != INT32_MAX && (t == 0 || buf[t] != buf[t - 1]);
+      const unsigned long long m = __ballot(keep);
+      if (PASS == 1 && keep)
+        colval[rowptr[r] + count + __popcll(m & ((1ull << lane) - 1))] = buf[t];
+      count += __popcll(m);
+    }
+    wave_sync();
+    if (PASS == 0 && lane == 0) {
+      len[r] = count;
+      atomicMax(&stat[1], count);
+    }
+  }
+}
+
+__global__ void k_pat_rowptr(const int32_t* __restrict__ incl, int64_t nrows,
+                             int64_t* __restrict__ rowptr) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    rowptr[r + 1] = incl[r];
+    if (r == 0) rowptr[0] = 0;
+  }
+}
+
+// position of each element-row entry in its CSR row (the type's largest value
+// marks a padding slot)
+template <class OFF>
+__global__ void k_pat_goff(const int32_t* __restrict__ dofs, int nl, int64_t nslots,
+                           const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colval,
+                           OFF* __restrict__ goff) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nslots;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = k / nl;
+    const int32_t r = dofs[k];
+    for (int j = 0; j < nl; ++j) {
+      const int32_t cd = dofs[c * nl + j];
+      OFF off = (OFF)~(OFF)0;
+      if (r >= 0 && cd >= 0) {
+        int64_t lo = rowptr[r], hi = rowptr[r + 1];
+        const int64_t b0 = lo;
+        while (lo < hi) {
+          const int64_t mid = (lo + hi) >> 1;
+          if (colval[mid] < cd) lo = mid + 1;
+          else hi = mid;
+        }
+        off = (OFF)(lo - b0);
+      }
+      goff[k * nl + j] = off;
+    }
+  }
+}
+
+__global__ void k_pat_pad(const int32_t* __restrict__ pptr, const int32_t* __restrict__ pairs,
+                          int64_t nrows, int pad, int32_t* __restrict__ padded) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t q0 = pptr[r], n = pptr[r + 1] - q0;
+    for (int k = 0; k < pad; ++k) padded[r * pad + k] = k < n ? pairs[q0 + k] : -1;
+  }
+}
+
+dim3 grid_stride(int64_t n) {
+  return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1 << 14)));
+}
+
 }  // namespace
+
+// CSR pattern of a square space (rdofs == cdofs, nl slots per cell, -1 pads)
+// built on the device: fills h like build_pattern (rowptr, colval and pptr
+// copied back; goff and pairs left on the device) and returns the device
+// arrays.  false with err set on failure (nothing left allocated).
+bool pattern_device(hipStream_t stream, const int32_t* d_dofs, int nl, int64_t nrows,
+                    int64_t ncells, HostPattern& h, int64_t** d_rowptr_out,
+                    int32_t** d_pptr_out, int32_t** d_pairs_out, void** d_goff_out,
+                    std::string& err) {
+  const int64_t nslots = ncells * nl;
+  if (nslots >= INT32_MAX || nrows >= INT32_MAX) { err = "mesh too large for int32 slots"; return false; }
+  h = HostPattern();
+  h.nrows = h.ncols = nrows;
+  h.nl_row = h.nl_col = nl;
+  int32_t *cnt = nullptr, *incl = nullptr, *tmp = nullptr, *pptr = nullptr, *fillc = nullptr,
+          *pairs = nullptr, *len = nullptr, *colval = nullptr, *stat = nullptr, *padded = nullptr;
+  int64_t* rowptr = nullptr;
+  void* goff = nullptr;
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  auto cleanup = [&](bool keep) {
+    for (void* q : {(void*)cnt, (void*)incl, (void*)tmp, (void*)fillc, (void*)len, (void*)colval,
+                    (void*)stat})
+      (void)hipFree(q);
+    if (!keep) {
+      for (void* q : {(void*)pptr, (void*)pairs, (void*)padded, (void*)rowptr, goff})
+        (void)hipFree(q);
+    }
+  };
+  chk(alloc(&cnt, nrows));
+  chk(alloc(&incl, nrows));
+  chk(alloc(&tmp, nrows / kScanTile + 2));
+  chk(alloc(&pptr, nrows + 1));
+  chk(alloc(&fillc, nrows));
+  chk(alloc(&pairs, nslots));
+  chk(alloc(&len, nrows));
+  chk(alloc(&rowptr, nrows + 1));
+  chk(alloc(&stat, 2));
+  int32_t hstat[2] = {0, 0};
+  if (e == hipSuccess) {
+    chk(launch_fill_i32(stream, cnt, nrows, 0));
+    chk(launch_fill_i32(stream, fillc, nrows, 0));
+    chk(launch_fill_i32(stream, stat, 2, 0));
+    hipLaunchKernelGGL(k_pat_count, grid_stride(nslots), dim3(256), 0, stream, d_dofs, nslots, cnt);
+    chk(scan_incl(stream, cnt, nrows, incl, tmp));
+    hipLaunchKernelGGL(k_pat_pptr, grid_stride(nrows), dim3(256), 0, stream, incl, cnt, nrows,
+                       pptr, stat);
+    hipLaunchKernelGGL(k_pat_scatter, grid_stride(nslots), dim3(256), 0, stream, d_dofs, nslots,
+                       pptr, fillc, pairs);
+    hipLaunchKernelGGL(k_pat_sort_pairs, grid_stride(nrows), dim3(256), 0, stream, pptr, nrows,
+                       pairs);
+    chk(hipGetLastError());
+    chk(hipMemcpyAsync(hstat, stat, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    chk(hipStreamSynchronize(stream));
+  }
+  if (e == hipSuccess && (int64_t)hstat[0] * nl > kRowCap) {
+    err = "rows gather more than 2048 candidate columns: use the host build";
+    cleanup(false);
+    return false;
+  }
+  h.max_adj = hstat[0];
+  const int64_t rgrid = std::max<int64_t>(1, std::min<int64_t>((nrows + kWaves - 1) / kWaves, 1 << 16));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_pat_rows<0>, dim3((unsigned)rgrid), dim3(64 * kWaves), 0, stream, d_dofs,
+                       nl, pptr, pairs, nrows, len, rowptr, colval, stat);
+    chk(scan_incl(stream, len, nrows, incl, tmp));
+    hipLaunchKernelGGL(k_pat_rowptr, grid_stride(nrows), dim3(256), 0, stream, incl, nrows, rowptr);
+    chk(hipGetLastError());
+    chk(hipMemcpyAsync(&hstat[1], stat + 1, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    h.rowptr.resize(nrows + 1);
+    chk(hipMemcpyAsync(h.rowptr.data(), rowptr, (nrows + 1) * sizeof(int64_t),
+                       hipMemcpyDeviceToHost, stream));
+    chk(hipStreamSynchronize(stream));
+  }
+  if (e == hipSuccess) {
+    h.nnz = h.rowptr[nrows];
+    h.max_row = hstat[1];
+    if (h.nnz >= (int64_t)INT32_MAX) { err = "nnz exceeds 2^31-1"; cleanup(false); return false; }
+    if (h.max_row <= 255) h.offset_bytes = 1;
+    else if (h.max_row <= 65535) h.offset_bytes = 2;
+    else { err = "row longer than 65535 entries"; cleanup(false); return false; }
+    chk(alloc(&colval, h.nnz));
+    chk(hipMalloc(&goff, std::max<size_t>((size_t)nslots * nl * h.offset_bytes, 1)));
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_pat_rows<1>, dim3((unsigned)rgrid), dim3(64 * kWaves), 0, stream, d_dofs,
+                       nl, pptr, pairs, nrows, len, rowptr, colval, stat);
+    if (h.offset_bytes == 1)
+      hipLaunchKernelGGL(k_pat_goff<uint8_t>, grid_stride(nslots), dim3(256), 0, stream, d_dofs,
+                         nl, nslots, rowptr, colval, (uint8_t*)goff);
+    else
+      hipLaunchKernelGGL(k_pat_goff<uint16_t>, grid_stride(nslots), dim3(256), 0, stream, d_dofs,
+                         nl, nslots, rowptr, colval, (uint16_t*)goff);
+    chk(hipGetLastError());
+    if (h.max_adj <= 16) {
+      h.pad = h.max_adj <= 4 ? 4 : (h.max_adj <= 8 ? 8 : 16);
+      chk(alloc(&padded, (size_t)nrows * h.pad));
+      if (e == hipSuccess)
+        hipLaunchKernelGGL(k_pat_pad, grid_stride(nrows), dim3(256), 0, stream, pptr, pairs, nrows,
+                           h.pad, padded);
+    }
+    h.colval.resize(h.nnz);
+    h.pptr.resize(nrows + 1);
+    chk(hipMemcpyAsync(h.colval.data(), colval, h.nnz * sizeof(int32_t), hipMemcpyDeviceToHost,
+                       stream));
+    chk(hipMemcpyAsync(h.pptr.data(), pptr, (nrows + 1) * sizeof(int32_t), hipMemcpyDeviceToHost,
+                       stream));
+    chk(hipStreamSynchronize(stream));
+  }
+  if (e != hipSuccess) {
+    err = std::string("pattern_device: ") + hipGetErrorString(e);
+    cleanup(false);
+    return false;
+  }
+  if (padded) {
+    (void)hipFree(pairs);
+    pairs = padded;
+    padded = nullptr;
+  }
+  *d_rowptr_out = rowptr;
+  *d_pptr_out = pptr;
+  *d_pairs_out = pairs;
+  *d_goff_out = goff;
+  cleanup(true);
+  return true;
+}
 
 // Numbering on the device; iglob (1-based, tot slots) and offset (per cell) to
 // the host arrays.  Returns ndofs, or -1 with err set.

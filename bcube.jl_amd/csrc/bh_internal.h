@@ -139,6 +139,14 @@ bool dof_table(int ct, int order, DofTab& t);
 int64_t dofhandler_device(hipStream_t stream, const HostMesh& m, int order, bool continuous,
                           std::vector<i64>& iglob, std::vector<i64>& offset, std::string& err);
 
+// The CSR pattern of a square space built by kernels (bh_build.hip): h as
+// build_pattern fills it (rowptr, colval, pptr and the sizes on the host),
+// the gather arrays (rowptr, pptr, pairs -- padded when h.pad -- and goff)
+// left on the device for the caller to own.
+bool pattern_device(hipStream_t stream, const int32_t* d_dofs, int nl, int64_t nrows,
+                    int64_t ncells, HostPattern& h, int64_t** d_rowptr, int32_t** d_pptr,
+                    int32_t** d_pairs, void** d_goff, std::string& err);
+
 // Simple host thread pool loop.
 void parallel_for(i64 n, const std::function<void(i64, i64)>& body);
 

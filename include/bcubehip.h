@@ -147,6 +147,13 @@ int bh_space_destroy(bh_space space);
  * touch it (in cell order) and their column positions: the deterministic
  * segmented reduction of the assembly. */
 int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out);
+/* The same pattern built by kernels on the context's stream, for meshes that
+ * change (no counterpart in the reference, which builds the COO on the CPU):
+ * each CSR row sorted and deduplicated by one wave in LDS, the gather lists
+ * from a scatter and per-row sort.  U == V; rows of more than 2048 candidate
+ * columns (adjacent cells x local dofs) are refused.  Same structure as
+ * bh_pattern_build bit for bit. */
+int bh_pattern_build_device(bh_space U, bh_space V, bh_pattern* out);
 /* max_row: the longest CSR row (entries). */
 /* Launch shapes the autotuner chose for `mode`: shape[0] element-kernel waves
  * per block target, shape[1] gather lanes per row, shape[2] gather rows per

@@ -371,15 +371,23 @@ __global__ void k_pat_count(const int32_t* __restrict__ dofs, int64_t nslots,
     if (dofs[k] >= 0) atomicAdd(&cnt[dofs[k]], 1);
 }
 
+// the largest value over a wave, then one atomic per wave
+__device__ __forceinline__ void wave_max_atomic(int32_t* dst, int32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(dst, v);
+}
+
 // pptr[0] = 0, pptr[r+1] = inclusive count; the largest row adjacency
 __global__ void k_pat_pptr(const int32_t* __restrict__ incl, const int32_t* __restrict__ cnt,
                            int64_t nrows, int32_t* __restrict__ pptr, int32_t* __restrict__ stat) {
+  int32_t mx = 0;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows;
        r += (int64_t)gridDim.x * blockDim.x) {
     pptr[r + 1] = incl[r];
     if (r == 0) pptr[0] = 0;
-    atomicMax(&stat[0], cnt[r]);
+    mx = max(mx, cnt[r]);
   }
+  wave_max_atomic(&stat[0], mx);
 }
 
 // element rows of each CSR row, any order (sorted next)
@@ -422,6 +430,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_pat_rows(
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int32_t* buf = sbuf[w];
   const int64_t nw = (int64_t)gridDim.x * kWaves;
+  int32_t longest = 0;
   for (int64_t r = (int64_t)blockIdx.x * kWaves + w; r < nrows; r += nw) {
     const int32_t q0 = pptr[r], na = pptr[r + 1] - q0;
     const int n = na * nl;
@@ -461,11 +470,10 @@ __global__ __launch_bounds__(64 * kWaves) void k_pat_rows(
       count += __popcll(m);
     }
     wave_sync();
-    if (PASS == 0 && lane == 0) {
-      len[r] = count;
-      atomicMax(&stat[1], count);
-    }
+    if (PASS == 0 && lane == 0) len[r] = count;
+    longest = max(longest, count);
   }
+  if (PASS == 0 && lane == 0) atomicMax(&stat[1], longest);
 }
 
 __global__ void k_pat_rowptr(const int32_t* __restrict__ incl, int64_t nrows,

@@ -638,6 +638,90 @@ def bench_vec(args, torch, B):
     return line
 
 
+def bench_build(args, torch, B):
+    """Setup on the device for meshes that change: the DofHandler numbering
+    (bh_dofhandler_device) and the CSR pattern with its gather lists
+    (bh_pattern_build_device) of hexa n^3 Q2, against the product's host
+    build on the same box; the oracle's numbering loop (the reference's
+    dictionary walk restated, 1 thread) is the CPU baseline."""
+    import ctypes as C
+
+    from bcubehip import _lib as L
+
+    n = args.n
+    m = B.hexa_mesh(n + 1, n + 1, n + 1)
+    fs = B.FunctionSpace("Lagrange", 2)
+    ctx = m.backend.ctx
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    m.backend.set_stream(stream.cuda_stream)
+    ncells = m.ncells
+    tot = ncells * 27
+    iglob = np.empty(tot, dtype=np.int64)
+    off = np.empty(ncells, dtype=np.int64)
+    nd = C.c_int64()
+
+    def numbering():
+        L.check(L.lib().bh_dofhandler_device(m.handle, 2, 1, iglob.ctypes.data, off.ctypes.data,
+                                             C.byref(nd)), ctx, "bh_dofhandler_device")
+
+    U = B.TrialFESpace(fs, m)
+    sh = U.handle
+
+    def pattern():
+        h = C.c_void_p()
+        L.check(L.lib().bh_pattern_build_device(sh, sh, C.byref(h)), ctx, "bh_pattern_build_device")
+        L.lib().bh_pattern_destroy(h)
+
+    def host_pattern():
+        h = C.c_void_p()
+        L.check(L.lib().bh_pattern_build(sh, sh, C.byref(h)), ctx, "bh_pattern_build")
+        L.lib().bh_pattern_destroy(h)
+
+    def timed(fn, reps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    reps = max(1, min(args.steps, 10))
+    num_ms = timed(numbering, reps)
+    pat_ms = timed(pattern, reps)
+    t0 = time.perf_counter()
+    B.DofHandler(m, fs)
+    host_num_ms = (time.perf_counter() - t0) * 1e3
+    host_pat_ms = timed(host_pattern, 1)
+    line = {
+        "metric": "DofHandler numbering + CSR pattern build time on 1 GPU (hexa Q2)",
+        "value": num_ms + pat_ms, "unit": "ms", "n_gpus": 1, "steps": reps,
+        "warmup": args.warmup, "ms_per_step": num_ms + pat_ms, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"hexa {n}^3 Q2 continuous: bh_dofhandler_device + "
+                               f"bh_pattern_build_device (results copied to the host arrays "
+                               f"the library keeps)", "ncells": ncells, "ndofs": int(nd.value)},
+        "numbering_ms": num_ms, "pattern_ms": pat_ms,
+        "host_build": {"numbering_ms": host_num_ms, "pattern_ms": host_pat_ms,
+                       "threads": "numbering 1, pattern up to 16"},
+        "roofline": None,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ns = 48
+        o = O.hexa_mesh(ns + 1, ns + 1, ns + 1)
+        t0 = time.perf_counter()
+        O.dofhandler(o, 2)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": dt * 1e3 * (n / ns) ** 3, "unit": "ms", "cores": 1,
+                                "kind": "port",
+                                "sample": f"oracle DofHandler loop on hexa {ns}^3 Q2 ({dt:.2f} s), "
+                                          f"scaled by cells to {n}^3"}
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -646,11 +730,13 @@ def main():
     ap.add_argument("--mode", default="all", choices=["all", "fp64", "fp32", "bf16x3"])
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "vec"],
+    ap.add_argument("--config", default="c2",
+                    choices=["c1", "c2", "c3", "c4", "c5", "vec", "build"],
                     help="c2: the headline 64^3 Q2 hexa workload; c1: the 64^2 Q1 Poisson "
                          "matrix + right-hand side; c3: mixed tri/quad order buckets; c4: the DG "
                          "transport residual; c5: the many-instance batch; vec: a 3-component "
-                         "Q2 hexa space (48^3 unless --n)")
+                         "Q2 hexa space (48^3 unless --n); build: the device numbering and "
+                         "pattern build")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters against a known byte count)")
@@ -669,6 +755,9 @@ def main():
         return
     if args.config == "vec":
         print(json.dumps(bench_vec(args, torch, B)), flush=True)
+        return
+    if args.config == "build":
+        print(json.dumps(bench_build(args, torch, B)), flush=True)
         return
     if args.config == "c5":
         print(json.dumps(bench_c5(args, torch, B)), flush=True)

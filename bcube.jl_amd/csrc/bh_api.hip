@@ -64,6 +64,11 @@ struct Colouring {
   int ncolors = 0;
   std::vector<int32_t> color_ptr, cells;
   int32_t* d_cells = nullptr;
+  // mixed meshes: each colour's cells split by cell-type bucket, ranges
+  // sub_ptr[k*nbuckets + b] .. +1 into d_cells (global ids, reordered) and
+  // d_lcells (the cells' rows in the bucket's c2n)
+  std::vector<int32_t> sub_ptr;
+  int32_t* d_lcells = nullptr;
 };
 
 // Handles record their device (-1: host-only) so destroy calls never touch
@@ -298,9 +303,34 @@ int get_gp_table(bh_ctx c, int geo, int order, int quad_degree, bool mass, float
 }
 
 int build_colouring(bh_ctx c, const std::vector<int32_t>& dofs, int nl, int64_t ndofs,
-                    int64_t ncells, Colouring& col) {
+                    int64_t ncells, Colouring& col, const HostMesh* mixed = nullptr,
+                    const std::vector<GeoBucket>* buckets = nullptr) {
   col.ncolors = colour_cells(dofs, nl, ndofs, ncells, col.color_ptr, col.cells);
   if (col.ncolors < 0) return fail(c, BH_ERR_UNSUPPORTED, "colouring needs more than 255 colours");
+  if (mixed) {
+    // bucket of each cell type and each cell's row in its bucket (mesh order)
+    const int nb = (int)buckets->size();
+    std::vector<int> bucket_of(8, -1);
+    for (int b = 0; b < nb; ++b) bucket_of[(*buckets)[b].geo] = b;
+    std::vector<int32_t> local(ncells), seen(nb, 0);
+    for (int64_t k = 0; k < ncells; ++k) local[k] = seen[bucket_of[mixed->ctype[k]]]++;
+    std::vector<int32_t> cells, lcells;
+    col.sub_ptr.assign((size_t)col.ncolors * nb + 1, 0);
+    for (int k = 0; k < col.ncolors; ++k)
+      for (int b = 0; b < nb; ++b) {
+        for (int32_t q = col.color_ptr[k]; q < col.color_ptr[k + 1]; ++q) {
+          const int32_t cell = col.cells[q];
+          if (bucket_of[mixed->ctype[cell]] != b) continue;
+          cells.push_back(cell);
+          lcells.push_back(local[cell]);
+        }
+        col.sub_ptr[(size_t)k * nb + b + 1] = (int32_t)cells.size();
+      }
+    col.cells = cells;
+    BH_HIP(c, dalloc(&col.d_lcells, lcells.size()));
+    BH_HIP(c, hipMemcpy(col.d_lcells, lcells.data(), lcells.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
+  }
   BH_HIP(c, dalloc(&col.d_cells, col.cells.size()));
   BH_HIP(c, hipMemcpy(col.d_cells, col.cells.data(), col.cells.size() * sizeof(int32_t),
                       hipMemcpyHostToDevice));
@@ -633,6 +663,7 @@ int bh_space_destroy(bh_space s) {
     (void)hipSetDevice(s->dev);
     (void)hipFree(s->d_dofs);
     (void)hipFree(s->col.d_cells);
+    (void)hipFree(s->col.d_lcells);
     (void)hipFree(s->d_b);
     (void)hipFree(s->d_coef);
   }
@@ -1351,26 +1382,38 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
   if (!b) return fail(c, BH_ERR_ARG, "b is NULL");
   if (where == BH_HOST && c->capturing)
     return fail(c, BH_ERR_ARG, "host-`where` assembly cannot be captured in a graph");
-  if (m->geo < 0 || m->geo == BH_HEXA27)
-    return fail(c, BH_ERR_UNSUPPORTED, "linear-form kernels need a uniform Quad4, Tri3 or Hexa8 mesh");
-  int nq1 = rule_key(m->geo, quad_degree);
-  if (!kernel_supported(m->geo, V->order, nq1))
-    return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this (cell type, order, quadrature)");
+  if (m->geo == -1 || m->geo == BH_HEXA27 || m->buckets.empty())
+    return fail(c, BH_ERR_UNSUPPORTED,
+                "linear-form kernels need Quad4, Tri3 (or a mix of both) or Hexa8 cells");
+  const bool mixed = m->geo == kMixed;
+  const size_t nb = m->buckets.size();
+  std::vector<int> bkey(nb);
+  int nq = 0;  // quadrature points per cell in a BH_COEF_QP coefficient (the largest rule)
+  for (size_t bk = 0; bk < nb; ++bk) {
+    const int geo = m->buckets[bk].geo;
+    bkey[bk] = rule_key(geo, quad_degree);
+    if (!kernel_supported(geo, V->order, bkey[bk]))
+      return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this (cell type, order, quadrature)");
+    nq = std::max(nq, geo == BH_TRI3 ? bkey[bk]
+                                     : bkey[bk] * bkey[bk] * (m->h.dim == 3 ? bkey[bk] : 1));
+  }
   for (int k = 0; k < nterms; ++k) {
     if (terms[k].form != BH_FORM_SOURCE) return fail(c, BH_ERR_UNSUPPORTED, "unsupported linear term");
     if (terms[k].coef_kind != BH_COEF_CONST && !terms[k].coef)
       return fail(c, BH_ERR_ARG, "coefficient is NULL");
   }
   if (int r = set_device(c)) return r;
-  double* tab = nullptr;
-  if (int r = get_tables(c, m->geo, V->order, quad_degree, &tab)) return r;
+  std::vector<double*> btab(nb, nullptr);
+  for (size_t bk = 0; bk < nb; ++bk)
+    if (int r = get_tables(c, m->buckets[bk].geo, V->order, quad_degree, &btab[bk])) return r;
   if (V->col.ncolors == 0) {
     if (c->capturing)
       return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
-    if (int r = build_colouring(c, V->dofs, V->nl, V->ndofs, m->h.ncells, V->col)) return r;
+    if (int r = build_colouring(c, V->dofs, V->nl, V->ndofs, m->h.ncells, V->col,
+                                mixed ? &m->h : nullptr, &m->buckets))
+      return r;
   }
   int64_t nc = m->h.ncells;
-  int nq = nq1 * nq1 * (m->h.dim == 3 ? nq1 : 1);
   double* db = b;
   if (where == BH_HOST) {
     if (!V->d_b) BH_HIP(c, dalloc(&V->d_b, (size_t)V->ndofs));
@@ -1386,9 +1429,9 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
     A.kind = LaunchArgs::SOURCE;
     A.stream = c->stream;
     A.coords = m->d_coords;
-    A.c2n = m->d_c2n;
     A.rdofs = V->d_dofs;
-    A.tab = tab;
+    A.erow = V->nl;
+    A.fstride = nq;
     A.fkind = t.coef_kind;
     A.fscale = t.scale;
     A.vals = db;
@@ -1407,14 +1450,30 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
         A.f = t.coef;
       }
     }
-    for (int k2 = 0; k2 < V->col.ncolors; ++k2) {
-      A.ncells_color = V->col.color_ptr[k2 + 1] - V->col.color_ptr[k2];
-      A.cells = V->col.d_cells + V->col.color_ptr[k2];
-      if (A.ncells_color == 0) continue;
-      time_begin(c);
-      BH_HIP(c, launch(m->geo, V->order, nq1, A));
-      time_end(c);
-    }
+    // one colour at a time (its cells share no dof row), on a mixed mesh one
+    // launch per cell type inside it
+    for (int k2 = 0; k2 < V->col.ncolors; ++k2)
+      for (size_t bk = 0; bk < nb; ++bk) {
+        const GeoBucket& gb = m->buckets[bk];
+        int32_t q0, q1;
+        if (mixed) {
+          q0 = V->col.sub_ptr[(size_t)k2 * nb + bk];
+          q1 = V->col.sub_ptr[(size_t)k2 * nb + bk + 1];
+          A.lcells = V->col.d_lcells + q0;
+        } else {
+          q0 = V->col.color_ptr[k2];
+          q1 = V->col.color_ptr[k2 + 1];
+          A.lcells = nullptr;
+        }
+        A.ncells_color = q1 - q0;
+        A.cells = V->col.d_cells + q0;
+        A.c2n = gb.d_c2n;
+        A.tab = btab[bk];
+        if (A.ncells_color == 0) continue;
+        time_begin(c);
+        BH_HIP(c, launch(gb.geo, V->order, bkey[bk], A));
+        time_end(c);
+      }
   }
   if (where == BH_HOST) {
     BH_HIP(c, hipMemcpyAsync(b, db, (size_t)V->ndofs * sizeof(double), hipMemcpyDeviceToHost,
@@ -1748,33 +1807,51 @@ int bh_quadrature_points(bh_mesh m, int quad_degree, double* xq, int where, int3
   if (!c) return fail(c, BH_ERR_ARG, "needs a mesh created with a context");
   if (c->capturing && where == BH_HOST)
     return fail(c, BH_ERR_ARG, "host-`where` call cannot be captured in a graph");
-  if (m->geo < 0 || m->geo == BH_HEXA27)
-    return fail(c, BH_ERR_UNSUPPORTED, "quadrature points need a uniform Quad4, Tri3 or Hexa8 mesh");
-  int nq1 = rule_key(m->geo, quad_degree);
-  int order = 1;  // any instantiated order carries the geometric tables for nq1
-  while (order <= 4 && !kernel_supported(m->geo, order, nq1)) ++order;
-  if (order > 4) return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this quadrature");
-  const int nq = m->geo == BH_TRI3 ? nq1 : nq1 * nq1 * (m->h.dim == 3 ? nq1 : 1);
+  if (m->geo == -1 || m->geo == BH_HEXA27 || m->buckets.empty())
+    return fail(c, BH_ERR_UNSUPPORTED,
+                "quadrature points need Quad4, Tri3 (or a mix of both) or Hexa8 cells");
+  // per cell type: a kernel order carrying the geometric tables, the points;
+  // every cell gets nq = the largest count (mixed meshes pad smaller rules)
+  std::vector<int> border(m->buckets.size()), bkey(m->buckets.size());
+  int nq = 0;
+  for (size_t b = 0; b < m->buckets.size(); ++b) {
+    const int geo = m->buckets[b].geo;
+    bkey[b] = rule_key(geo, quad_degree);
+    int order = 1;  // any instantiated order carries the geometric tables for the rule
+    while (order <= 4 && !kernel_supported(geo, order, bkey[b])) ++order;
+    if (order > 4) return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this quadrature");
+    border[b] = order;
+    const int nqb = geo == BH_TRI3 ? bkey[b] : bkey[b] * bkey[b] * (m->h.dim == 3 ? bkey[b] : 1);
+    nq = std::max(nq, nqb);
+  }
   if (!xq) {  // size query
     if (!nq_out) return BH_ERR_ARG;
     *nq_out = nq;
     return BH_OK;
   }
   if (int r = set_device(c)) return r;
-  double* tab = nullptr;
-  if (int r = get_tables(c, m->geo, order, quad_degree, &tab)) return r;
+  std::vector<double*> btab(m->buckets.size(), nullptr);
+  for (size_t b = 0; b < m->buckets.size(); ++b)
+    if (int r = get_tables(c, m->buckets[b].geo, border[b], quad_degree, &btab[b])) return r;
   int64_t n = m->h.ncells * nq * m->h.dim;
   double* d = xq;
   if (where == BH_HOST) BH_HIP(c, dalloc(&d, (size_t)n));
-  LaunchArgs A;
-  A.kind = LaunchArgs::QPOINTS;
-  A.stream = c->stream;
-  A.coords = m->d_coords;
-  A.c2n = m->d_c2n;
-  A.tab = tab;
-  A.ncells_total = m->h.ncells;
-  A.vals = d;
-  hipError_t e = launch(m->geo, order, nq1, A);
+  hipError_t e = hipSuccess;
+  for (size_t b = 0; b < m->buckets.size() && e == hipSuccess; ++b) {
+    const GeoBucket& gb = m->buckets[b];
+    double* tab = btab[b];
+    LaunchArgs A;
+    A.kind = LaunchArgs::QPOINTS;
+    A.stream = c->stream;
+    A.coords = m->d_coords;
+    A.c2n = gb.d_c2n;
+    A.cells = gb.d_cells;  // NULL on a uniform mesh
+    A.tab = tab;
+    A.ncells_total = gb.n;
+    A.fstride = nq;
+    A.vals = d;
+    e = launch(gb.geo, border[b], bkey[b], A);
+  }
   if (e == hipSuccess && where == BH_HOST) {
     e = hipMemcpyAsync(xq, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

@@ -20,6 +20,8 @@ struct LaunchArgs {
   const double* tab = nullptr;     // packed reference tables
   const float* ptab = nullptr;     // reference products P of the GEMM element kernel (bh_gp.hip)
   const int32_t* cells = nullptr;  // cells of this colour (source) / of this cell type (bilinear)
+  const int32_t* lcells = nullptr; // source on a mixed mesh: the cells' rows in this type's c2n
+  int fstride = 0;                 // source / points: quadrature points per cell in f / xq (0: NQ)
   int ncells_color = 0;
   int slot0 = 0;                   // slot of cells[0] in the scatter map
   int64_t ncells_total = 0;

@@ -274,8 +274,12 @@ template <int D, int NL, int NQ, int NG>
 __global__ __launch_bounds__(kBlock) void k_source_f64(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
     const int32_t* __restrict__ rdofs, const double* __restrict__ tab,
-    const int32_t* __restrict__ cells, int ncells_color, int fkind,
-    const double* __restrict__ f, double fscale, double* __restrict__ b) {
+    const int32_t* __restrict__ cells, const int32_t* __restrict__ lcells, int ncells_color,
+    int rstride, int fstride, int fkind, const double* __restrict__ f, double fscale,
+    double* __restrict__ b) {
+  // cells: global ids (dof rows rdofs[cell*rstride], coefficients f[cell] or
+  // f[cell*fstride + q]); lcells: the same cells' rows in c2n (a mixed mesh's
+  // per-type connectivity), or NULL when c2n is indexed by global id.
   using L = TabLayout<D, NL, NQ, NG>;
   constexpr int CPB = kBlock / NL;
   __shared__ double s_tab[L::SIZE];
@@ -291,7 +295,8 @@ __global__ __launch_bounds__(kBlock) void k_source_f64(
     for (int k = tid; k < CPB * NG; k += kBlock) {
       int c = k / NG, n = k - c * NG;
       if (base + c < ncells_color) {
-        int node = c2n[(int64_t)cells[base + c] * NG + n];
+        const int32_t crow = lcells ? lcells[base + c] : cells[base + c];
+        int node = c2n[(int64_t)crow * NG + n];
         for (int a = 0; a < D; ++a) s_x[c][n][a] = coords[(int64_t)node * D + a];
       }
     }
@@ -312,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void k_source_f64(
       double det = invert<D>(J, Ji);
       double fv = fkind == BH_COEF_CONST ? fscale
                   : fkind == BH_COEF_CELL ? f[cell] * fscale
-                                          : f[(int64_t)cell * NQ + q] * fscale;
+                                          : f[(int64_t)cell * fstride + q] * fscale;
       s_m[c][q] = s_tab[L::W + q] * fabs(det) * fv;
     }
     __syncthreads();
@@ -320,29 +325,35 @@ __global__ __launch_bounds__(kBlock) void k_source_f64(
     double acc = 0.0;
     for (int q = 0; q < NQ; ++q) acc = fma(s_m[cs][q], s_tab[L::R + (q * NL + li) * 4 + D], acc);
     const int cell = cells[base + cs];
-    b[rdofs[(int64_t)cell * NL + li]] += acc;
+    b[rdofs[(int64_t)cell * rstride + li]] += acc;
   }
 }
 
-// Physical quadrature points x_q = Σ_k N_k(ξ_q) X_k (eval_point.jl:107-111).
+// Physical quadrature points x_q = Σ_k N_k(ξ_q) X_k (eval_point.jl:107-111),
+// qstride points per cell in xq: cells of c2n row k go to xq row gcells[k]
+// (NULL: k); the slots q >= NQ of a cell (mixed meshes pad every cell to the
+// largest rule) repeat its first point.
 template <int D, int NL, int NQ, int NG>
 __global__ __launch_bounds__(kBlock) void k_qpoints(const double* __restrict__ coords,
                                                     const int32_t* __restrict__ c2n,
                                                     const double* __restrict__ tab, int64_t ncells,
-                                                    double* __restrict__ xq) {
+                                                    const int32_t* __restrict__ gcells,
+                                                    int qstride, double* __restrict__ xq) {
   using L = TabLayout<D, NL, NQ, NG>;
   int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= ncells * NQ) return;
-  int64_t cell = k / NQ;
-  int q = (int)(k - cell * NQ);
+  if (k >= ncells * qstride) return;
+  int64_t cell = k / qstride;
+  int q = (int)(k - cell * qstride);
+  const int qe = q < NQ ? q : 0;
   double x[D];
   for (int a = 0; a < D; ++a) x[a] = 0.0;
   for (int n = 0; n < NG; ++n) {
     int node = c2n[cell * NG + n];
-    double N = tab[L::G + q * NG + n];
+    double N = tab[L::G + qe * NG + n];
     for (int a = 0; a < D; ++a) x[a] += N * coords[(int64_t)node * D + a];
   }
-  for (int a = 0; a < D; ++a) xq[k * D + a] = x[a];
+  const int64_t row = (gcells ? (int64_t)gcells[cell] : cell) * qstride + q;
+  for (int a = 0; a < D; ++a) xq[row * D + a] = x[a];
 }
 
 __global__ void k_fill(double* __restrict__ p, int64_t n, double v) {
@@ -468,13 +479,15 @@ static hipError_t launch_all(const LaunchArgs& A) {
 
     int ng = (A.ncells_color + CPB - 1) / CPB;
     hipLaunchKernelGGL((k_source_f64<D, NL, NQ, NG>), dim3(grid_for(ng)), dim3(kBlock), 0,
-                       A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.ncells_color,
+                       A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.lcells,
+                       A.ncells_color, A.erow > 0 ? A.erow : NL, A.fstride > 0 ? A.fstride : NQ,
                        A.fkind, A.f, A.fscale, A.vals);
   } else {
-    int64_t n = A.ncells_total * NQ;
+    const int qs = A.fstride > 0 ? A.fstride : NQ;
+    int64_t n = A.ncells_total * qs;
     hipLaunchKernelGGL((k_qpoints<D, NL, NQ, NG>), dim3((unsigned)((n + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, A.stream, A.coords, A.c2n, A.tab, A.ncells_total,
-                       A.vals);
+                       A.cells, qs, A.vals);
   }
   return hipGetLastError();
 }

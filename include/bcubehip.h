@@ -191,7 +191,9 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
 /* Physical coordinates x = F(ξ_q) of every quadrature point
  * (eval_point.jl:107-111), ncells*nq*dim FP64, for sampling a
  * PhysicalFunction on the host; *nq receives the points per cell.  With
- * xq NULL only *nq is set (the size query). */
+ * xq NULL only *nq is set (the size query).  On a mixed Quad4 + Tri3 mesh
+ * every cell gets the larger rule's count, its unused slots repeating its
+ * first point; BH_COEF_QP coefficients use the same layout. */
 int bh_quadrature_points(bh_mesh mesh, int quad_degree, double* xq, int where, int32_t* nq);
 
 /* ---- Dirichlet conditions ------------------------------------------------ */

@@ -80,3 +80,34 @@ def test_laplace_on_gmsh_mixed_mesh_reproduces_linear(p):
     err = np.abs(x - lin(X)).max()
     print(f"P{p} Laplace on the gmsh mixed mesh: max err {err:.2e}")
     assert err <= 1e-12
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_projection_on_gmsh_mixed_mesh(p):
+    m = B.read_mesh(MIXED, warn=False)
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", p), m)
+    u = B.FEFunction(U)
+    lin = lambda x: 1.0 - 2.0 * x[:, 0] + 0.5 * x[:, 1]  # noqa: E731
+    B.projection_l2_(u, B.PhysicalFunction(lin), B.Measure(B.CellDomain(m), 2 * p + 1))
+    err = np.abs(u.get_dof_values() - lin(B.dof_coordinates(U))).max()
+    print(f"P{p} projection on the gmsh mixed mesh: max err {err:.2e}")
+    assert err <= 1e-12
+
+
+def test_poisson_on_gmsh_mixed_mesh_reproduces_quadratic():
+    """-Δu = -4 with u = x² + y² on the boundary: P2 triangles and Q2 quads
+    hold the solution, so the Galerkin solution is it."""
+    m = B.read_mesh(MIXED, warn=False)
+    quad = lambda x: x[:, 0] ** 2 + x[:, 1] ** 2  # noqa: E731
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", 2), m, {t: B.PhysicalFunction(quad)
+                                                          for t in TAGS})
+    V = B.TestFESpace(U)
+    dΩ = B.Measure(B.CellDomain(m), 4)
+    A = B.assemble_bilinear(lambda u, v: B.integrate(B.dot(B.grad(u), B.grad(v)), dΩ), U, V)
+    b = B.assemble_linear(lambda v: B.integrate(-4.0 * v, dΩ), V)
+    B.apply_dirichlet_to_matrix_(A, U, V, m)
+    B.apply_dirichlet_to_vector_(b, U, V, m)
+    x = spla.spsolve(A.tocsc(), b)
+    err = np.abs(x - quad(B.dof_coordinates(U))).max()
+    print(f"P2 Poisson on the gmsh mixed mesh: max err {err:.2e}")
+    assert err <= 1e-11

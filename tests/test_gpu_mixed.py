@@ -123,3 +123,22 @@ def test_gemm_element_kernel_every_rule_fp32(shape, p, deg):
         e2 = _check(_plan(mm, p, deg, k2, mode="fp32"), _ref(mm, p, deg, k2), RTOL_FP32, mode="fp32")
         msg += f", mixed FP32 {e2:.2e}"
     print(msg)
+
+
+@pytest.mark.parametrize("p,deg,src", [(1, 2, "one"), (1, 4, "x"), (2, 4, "sinsin"),
+                                       (3, 6, "sinsin"), (4, 8, "x")])
+def test_mixed_mesh_linear_form_vs_oracle(p, deg, src):
+    """∫f v dΩ on a mixed mesh: one launch per colour and cell type, the
+    PhysicalFunction sampled at points padded to the larger rule."""
+    m, _ = B.mixed_tri_quad_mesh(12, block=3, p_split=0.5, seed=p + 20)
+    B.jitter_nodes(m, 0.2, 1.0 / 12, seed=p + 21)
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", p), m)
+    dΩ = B.Measure(B.CellDomain(m), deg)
+    f = {"one": 1.0,
+         "x": B.PhysicalFunction(lambda x: x[:, 0]),
+         "sinsin": B.PhysicalFunction(lambda x: np.sin(np.pi * x[:, 0]) * np.sin(np.pi * x[:, 1]))}
+    b = B.assemble_linear(lambda v: B.integrate(f[src] * v, dΩ), B.TestFESpace(U))
+    ref = O.assemble_linear(_om(m), p, src, degree=deg)
+    err = np.abs(b - ref).max() / np.abs(ref).max()
+    print(f"mixed P{p} deg {deg} f={src}: rel err {err:.2e}")
+    assert err <= 1e-12

@@ -376,7 +376,10 @@ def _placeholders(e, out):
 
 def _to_scalar_form(e, vector: bool):
     """A term over two vector spaces as the scalar form of each component:
-    u⋅v -> u v, ∇u ⊡ ∇v -> ∇u⋅∇v (both component-diagonal).  Over scalar
+    u⋅v -> u v and ∇u ⊡ ∇v -> ∇u⋅∇v, both component-diagonal.  On a vector
+    space ∇u⋅∇v is the same Frobenius product as ∇u ⊡ ∇v: Bcube's ⋅ is
+    LinearAlgebra.dot, which contracts all indices of two matrices
+    (LazyOperators/LazyOperators.jl:26-27, algebra/algebra.jl:137).  Over scalar
     spaces ⊡ and ⋅ of two gradients agree, and u⋅v is u v."""
     if isinstance(e, Mul):
         return Mul(_to_scalar_form(e.a, vector), _to_scalar_form(e.b, vector))
@@ -384,11 +387,8 @@ def _to_scalar_form(e, vector: bool):
         return Dot(e.a, e.b)
     if isinstance(e, Dot):
         pa, pb = _factors(e.a, []), _factors(e.b, [])
-        grads = [f for f in pa + pb if isinstance(f, Grad)]
-        if not grads:
+        if not any(isinstance(f, Grad) for f in pa + pb):
             return Mul(e.a, e.b)
-        if vector:
-            raise NotImplementedError("∇u⋅∇v on a vector space: use ddot (⊡)")
     return e
 
 

@@ -86,6 +86,36 @@ def cpu_baseline(n_sample=32):
                       f"{nzval.size} nnz in {dt:.2f} s"}
 
 
+def downstream_check(B, n=10):
+    """Error on the solution of the assembled system: the C2 workload (Q2,
+    jitter, η, ∫η∇u⋅∇v + ∫uv) at n^3 with b = ∫x v, solved by a direct sparse
+    solve with the oracle's matrix and with the GPU's FP64 and FP32-MFMA
+    matrices; relative 2-norm errors of the GPU solutions against the oracle's."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    from oracle import oracle as O
+
+    m = B.hexa_mesh(n + 1, n + 1, n + 1)
+    B.jitter_nodes(m, 0.15, 1.0 / n, seed=0)
+    eta = np.random.default_rng(1).uniform(0.5, 2.0, m.ncells)
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", 2), m)
+    V = B.TestFESpace(U)
+    dΩ = B.Measure(B.CellDomain(m), 5)
+    ηc = B.MeshCellData(eta)
+    plan = B.BilinearPlan(lambda u, v: B.integrate(ηc * B.dot(B.grad(u), B.grad(v)), dΩ)
+                          + B.integrate(u * v, dΩ), U, V)
+    b = B.assemble_linear(lambda v: B.integrate(B.PhysicalFunction(lambda x: x[:, 0]) * v, dΩ), V)
+    om = O.OMesh(m.dim, m.coords, m.celltypes, m.c2n_ptr, m.c2n)
+    cp, rv, nz, nd = O.assemble_bilinear(om, 2, ["stiffness", "mass"], eta=eta, degree=5)
+    x_ref = spla.spsolve(sp.csc_matrix((nz, rv - 1, cp - 1), shape=(nd, nd)), b)
+    out = {"workload": f"C2 at {n}^3 ({nd} dofs), b = ∫x v, direct solve"}
+    for mode in ("fp64", "fp32"):
+        x = spla.spsolve(plan.to_csc(plan.assemble_host(mode=mode)).tocsc(), b)
+        out[mode] = float(np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref))
+    return out
+
+
 def bench_c5(args, torch, B):
     """Many-instance batch: 2048 unit-square quad meshes, side ~ U{8..96} (seed 3),
     even instances Q1, odd Q2, κ ~ LogNormal(0, 0.5) per cell, ∫κ∇u⋅∇v + ∫uv.
@@ -638,6 +668,71 @@ def bench_vec(args, torch, B):
     return line
 
 
+def bench_basic(args, torch, B):
+    """The reference's own assembly benchmark (benchmark/bench_assemble.jl:
+    10-28): rectangle_mesh(251, 11; xmax=1, ymax=0.1), Q1, TrialFESpace of size
+    2, Measure degree 3, the mass matrix ∫(u ⋅ v)dΩ and the stiffness matrix
+    ∫(∇(u) ⋅ ∇(v))dΩ, each one assemble_bilinear into device-resident CSC values
+    (launch-bound: 2,500 cells).  One step = both matrices."""
+    m = B.rectangle_mesh(251, 11, xmax=1.0, ymax=0.1)
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", 1), m, size=2)
+    V = B.TestFESpace(U)
+    dΩ = B.Measure(B.CellDomain(m), 3)
+    plans = [B.MultiBilinearPlan(lambda u, v: B.integrate(B.dot(u, v), dΩ), U, V),
+             B.MultiBilinearPlan(lambda u, v: B.integrate(B.dot(B.grad(u), B.grad(v)), dΩ),
+                                 U, V)]
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    m.backend.set_stream(stream.cuda_stream)
+    outs = [torch.empty(p.nnz, dtype=torch.float64, device=dev) for p in plans]
+
+    def step():
+        for p, o in zip(plans, outs):
+            p.assemble_device(o.data_ptr(), "fp64")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    nnz = sum(p.nnz for p in plans)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for p in plans:
+            p.to_csc(p.assemble_host())
+    host_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    line = {
+        "metric": "basic bilinear (bench_assemble.jl:10-28): mass + stiffness matrices per second",
+        "value": 1e3 / ms, "unit": "assemblies/s (both matrices)", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "rectangle_mesh(251, 11; xmax=1, ymax=0.1), Q1, size=2, "
+                               "degree 3, ∫u⋅v and ∫∇u⋅∇v into device-resident CSC",
+                   "ncells": m.ncells, "ndofs": plans[0].nrows, "nnz_both": nnz,
+                   "host_csc_ms": host_ms},
+        "roofline": None,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        om = O.OMesh(m.dim, m.coords, m.celltypes, m.c2n_ptr, m.c2n)
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            O.assemble_blocks(om, 1, [2], {(0, 0): (["mass"], None)}, degree=3)
+            O.assemble_blocks(om, 1, [2], {(0, 0): (["stiffness"], None)}, degree=3)
+        dt = (time.perf_counter() - t0) / reps
+        line["cpu_baseline"] = {"value": 1.0 / dt, "unit": "assemblies/s (both matrices)",
+                                "cores": 1, "kind": "port",
+                                "sample": f"the same workload, oracle block assembly, {reps} "
+                                          f"repetitions, {dt * 1e3:.1f} ms each"}
+    return line
+
+
 def bench_build(args, torch, B):
     """Setup on the device for meshes that change: the DofHandler numbering
     (bh_dofhandler_device) and the CSR pattern with its gather lists
@@ -731,12 +826,12 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="c2",
-                    choices=["c1", "c2", "c3", "c4", "c5", "vec", "build"],
+                    choices=["c1", "c2", "c3", "c4", "c5", "vec", "build", "basic"],
                     help="c2: the headline 64^3 Q2 hexa workload; c1: the 64^2 Q1 Poisson "
                          "matrix + right-hand side; c3: mixed tri/quad order buckets; c4: the DG "
                          "transport residual; c5: the many-instance batch; vec: a 3-component "
                          "Q2 hexa space (48^3 unless --n); build: the device numbering and "
-                         "pattern build")
+                         "pattern build; basic: the reference's basic_bilinear benchmark")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters against a known byte count)")
@@ -758,6 +853,9 @@ def main():
         return
     if args.config == "build":
         print(json.dumps(bench_build(args, torch, B)), flush=True)
+        return
+    if args.config == "basic":
+        print(json.dumps(bench_basic(args, torch, B)), flush=True)
         return
     if args.config == "c5":
         print(json.dumps(bench_c5(args, torch, B)), flush=True)
@@ -868,6 +966,7 @@ def main():
         "rel_err_vs_fp64_gpu": r["rel_err_vs_fp64_gpu"],
         "modes": results,
     }
+    line["solution_rel_err_vs_oracle"] = downstream_check(B)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

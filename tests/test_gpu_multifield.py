@@ -163,3 +163,27 @@ def test_vector_projection_reproduces_linear_field():
     err = max(np.abs(x[cm[0]] - want[:, 0]).max(), np.abs(x[cm[1]] - want[:, 1]).max())
     print("vector projection max err", err)
     assert err <= 1e-12
+
+
+@pytest.mark.parametrize("form", ["mass", "stiffness"])
+def test_reference_basic_bilinear_benchmark(form):
+    """The reference's own assembly benchmark, bench_assemble.jl:10-28:
+    rectangle_mesh(251, 11; xmax=1, ymax=0.1), Q1, TrialFESpace size 2,
+    Measure degree 3, a = ∫(u ⋅ v)dΩ and m = ∫(∇(u) ⋅ ∇(v))dΩ -- against the
+    oracle's block assembly."""
+    m = B.rectangle_mesh(251, 11, xmax=1.0, ymax=0.1)
+    U = B.TrialFESpace(LAG(1), m, size=2)
+    V = B.TestFESpace(U)
+    dΩ = B.Measure(B.CellDomain(m), 3)
+    if form == "mass":
+        a = lambda u, v: B.integrate(B.dot(u, v), dΩ)  # noqa: E731
+    else:
+        a = lambda u, v: B.integrate(B.dot(B.grad(u), B.grad(v)), dΩ)  # noqa: E731
+    A = B.assemble_bilinear(a, U, V)
+    colptr, rowval, nzval, n = O.assemble_blocks(_om(m), 1, [2], {(0, 0): ([form], None)},
+                                                 degree=3)
+    assert A.shape == (n, n) and n == 2 * 251 * 11
+    assert np.array_equal(A.indptr + 1, colptr) and np.array_equal(A.indices + 1, rowval)
+    err = np.abs(A.data - nzval).max() / np.abs(nzval).max()
+    print(f"basic bilinear {form}: nnz {A.nnz}, rel err {err:.2e}")
+    assert err <= 1e-12

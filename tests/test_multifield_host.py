@@ -173,9 +173,24 @@ def test_unsupported_couplings_raise():
     with pytest.raises(NotImplementedError):  # two orders: rectangular blocks
         B.MultiBilinearPlan(lambda u, v: B.integrate(u[0] * v[0], dΩ), W, W)
     Uv = B.TrialFESpace(LAG(1), m, size=2)
-    with pytest.raises(NotImplementedError):  # ∇u⋅∇v of a vector field is not a scalar
-        B.MultiBilinearPlan(lambda u, v: B.integrate(B.dot(B.grad(u), B.grad(v)), dΩ), Uv,
+    with pytest.raises(NotImplementedError):  # ∇u⋅v: a matrix against a vector
+        B.MultiBilinearPlan(lambda u, v: B.integrate(B.dot(B.grad(u), v), dΩ), Uv,
                             B.TestFESpace(Uv))
+
+
+def test_dot_of_vector_gradients_is_the_frobenius_product():
+    """bench_assemble.jl:10-28 assembles ∫(∇(u) ⋅ ∇(v))dΩ on a size-2 space:
+    Julia's dot of two matrices contracts both indices, the form of ∇u ⊡ ∇v."""
+    m = _mesh("quad")
+    Uv = B.TrialFESpace(LAG(1), m, size=2)
+    dΩ = B.Measure(B.CellDomain(m), 3)
+    a = B.MultiBilinearPlan(lambda u, v: B.integrate(B.dot(B.grad(u), B.grad(v)), dΩ), Uv,
+                            B.TestFESpace(Uv))
+    b = B.MultiBilinearPlan(lambda u, v: B.integrate(B.ddot(B.grad(u), B.grad(v)), dΩ), Uv,
+                            B.TestFESpace(Uv))
+    assert [(t.form, t.coef_kind, t.scale) for t in a.blocks[0][0]] == \
+        [(t.form, t.coef_kind, t.scale) for t in b.blocks[0][0]]
+    assert np.array_equal(a.csc_structure()[1], b.csc_structure()[1])
 
 
 def test_oracle_block_values_invariants():

@@ -11,9 +11,11 @@ from .mesh import (AbstractBcubeBackend, BcubeBackendHIP, BcubeBackendHostOnly, 
                    sub_mesh)
 from .fespace import (DofHandler, FunctionSpace, MultiFESpace, TestFESpace, TrialFESpace,
                       component_numbering)
-from .forms import (CellDomain, FEFunction, InteriorFaceDomain, Measure, MeshCellData,
-                    PhysicalFunction, UpwindFlux, compose, ddot, dot, get_face_normals, grad,
-                    integrate, jump, nabla, side_minus, side_n, side_p, side_plus)
+from .forms import (BoundaryFaceDomain, CellDomain, FEFunction, InteriorFaceDomain, Measure,
+                    MeshCellData, PhysicalFunction, UpwindFlux, avg, compose, ddot, dot,
+                    get_face_normals, grad, integrate, jump, nabla, side_minus, side_n, side_p,
+                    side_plus)
+from .faces import BoundaryFaces, FaceBilinearPlan
 from .dg import DGTransport, dg_dof_coordinates
 from .assembler import (BilinearPlan, MultiBilinearPlan, Pattern, assemble_bilinear,
                         assemble_linear,
@@ -40,5 +42,6 @@ __all__ = [
     "apply_homogeneous_dirichlet_to_matrix_", "apply_homogeneous_dirichlet_to_vector_",
     "assemble_dirichlet_vector", "boundary_faces", "dirichlet_dofs_values", "dof_coordinates",
     "nodes_gmsh2cgns", "read_mesh", "projection_l2_", "MultiFESpace", "component_numbering",
-    "ddot", "MultiBilinearPlan",
+    "ddot", "MultiBilinearPlan", "BoundaryFaceDomain", "avg", "BoundaryFaces",
+    "FaceBilinearPlan",
 ]

@@ -32,7 +32,9 @@ EXPORTED = [
     "bh_pattern_export_csc", "bh_pattern_values_device", "bh_pattern_destroy",
     "bh_assemble_bilinear", "bh_assemble_linear", "bh_quadrature_points", "bh_dg_create",
     "bh_dg_info", "bh_dg_update_geometry", "bh_dg_transport_residual", "bh_dg_minv_axpy",
-    "bh_dg_destroy", "bh_graph_begin", "bh_graph_end", "bh_graph_launch", "bh_graph_destroy",
+    "bh_dg_destroy", "bh_dgb_create", "bh_dgb_info", "bh_dgb_export_csc", "bh_dgb_assemble",
+    "bh_dgb_boundary_points", "bh_dgb_boundary_linear", "bh_dgb_destroy", "bh_graph_begin",
+    "bh_graph_end", "bh_graph_launch", "bh_graph_destroy",
 ]
 
 
@@ -119,6 +121,13 @@ def lib():
         "bh_dg_transport_residual": (C.c_int, [vp, i32, vp, vp, C.c_int]),
         "bh_dg_minv_axpy": (C.c_int, [vp, i32, vp, vp, dbl, vp, vp, dbl, vp]),
         "bh_dg_destroy": (C.c_int, [vp]),
+        "bh_dgb_create": (C.c_int, [vp, C.c_int, C.c_int, P(BhTerm), C.c_int, vp, vp, P(vp)]),
+        "bh_dgb_info": (C.c_int, [vp, P(i64), P(i64), P(i64), P(i64), P(i32)]),
+        "bh_dgb_export_csc": (C.c_int, [vp, vp, vp, C.c_int]),
+        "bh_dgb_assemble": (C.c_int, [vp, vp, C.c_int]),
+        "bh_dgb_boundary_points": (C.c_int, [vp, vp, C.c_int]),
+        "bh_dgb_boundary_linear": (C.c_int, [vp, dbl, dbl, vp, vp, C.c_int]),
+        "bh_dgb_destroy": (C.c_int, [vp]),
         "bh_graph_begin": (C.c_int, [vp]),
         "bh_graph_end": (C.c_int, [vp, P(vp)]),
         "bh_graph_launch": (C.c_int, [vp]),

@@ -14,6 +14,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from . import _lib as L
+from .faces import FaceBilinearPlan, assemble_linear_faces_, has_faces
 from .forms import (Placeholder, _split_sum, lower_bilinear, lower_dg_transport, lower_linear,
                     lower_multi_bilinear, lower_multi_linear)
 
@@ -377,12 +378,20 @@ def assemble_bilinear(a, U, V, T=np.float64, mode: str = "fp64") -> sp.csc_matri
     reference, a receiving tuples for MultiFESpaces."""
     if T is not np.float64:
         raise NotImplementedError("only Float64 output is supported")
+    if not (_is_multi(U) or _is_multi(V)):
+        Us, Vs = getattr(U, "U", U), getattr(V, "U", V)
+        if has_faces(_integrals(a(Placeholder("trial", Us), Placeholder("test", Vs)))):
+            if mode != "fp64":
+                raise NotImplementedError("face terms are assembled in FP64")
+            plan = FaceBilinearPlan(a, Us, Vs)
+            return plan.to_csc(plan.assemble_host())
     plan = (MultiBilinearPlan if _is_multi(U) or _is_multi(V) else BilinearPlan)(a, U, V, mode)
     return plan.to_csc(plan.assemble_host())
 
 
-def _linear_terms(l, V):
-    items = _integrals(l(Placeholder("test", V)))
+def _linear_terms(l, V, items=None):
+    if items is None:
+        items = _integrals(l(Placeholder("test", V)))
     measure = _single_measure(items)
     return (measure,) + _linear_bh_terms([lower_linear(it.expr) for it in items], measure,
                                          V.mesh)
@@ -417,7 +426,13 @@ def assemble_linear_(b: np.ndarray, l, V, mode: str = "fp64") -> None:
     if _is_multi(V):
         _assemble_linear_multi(b, l, V, mode)
         return
-    measure, keep, terms, n = _linear_terms(l, V)
+    items = _integrals(l(Placeholder("test", V)))
+    if has_faces(items):
+        # boundary-face integrals by the face kernels, the cell integrals below
+        items = assemble_linear_faces_(b, items, V)
+        if not items:
+            return
+    measure, keep, terms, n = _linear_terms(l, V, items)
     ctx = V.mesh.backend.ctx
     L.check(L.lib().bh_assemble_linear(V.handle, n, terms, measure.degree, L.MODE[mode],
                                        b.ctypes.data, L.HOST), ctx, "bh_assemble_linear")

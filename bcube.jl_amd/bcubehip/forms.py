@@ -36,6 +36,15 @@ class Expr:
     def __radd__(self, o):
         return Add(_wrap(o), self)
 
+    def __neg__(self):
+        return Mul(Const(-1.0), self)
+
+    def __sub__(self, o):
+        return Add(self, Mul(Const(-1.0), _wrap(o)))
+
+    def __rsub__(self, o):
+        return Add(_wrap(o), Mul(Const(-1.0), self))
+
 
 def _wrap(x):
     if isinstance(x, Expr):
@@ -140,8 +149,17 @@ def side_p(a):
 side_minus, side_plus = side_n, side_p
 
 
-def jump(a):
-    return Jump(_wrap(a))
+def jump(a, n=None):
+    """jump(a) = side⁻(a) - side⁺(a); jump(a, n) = side⁺(a) side⁺(n) + side⁻(a) side⁻(n)
+    (cellfunction.jl:308-310)."""
+    if n is None:
+        return Jump(_wrap(a))
+    return Add(Mul(side_p(a), side_p(n)), Mul(side_n(a), side_n(n)))
+
+
+def avg(a):
+    """avg(a) = (side⁺(a) + side⁻(a))/2, as bench_assemble.jl:31 defines it."""
+    return Mul(Const(0.5), Add(side_p(a), side_n(a)))
 
 
 def compose(f, *args):
@@ -233,12 +251,21 @@ class InteriorFaceDomain:
         self.mesh = mesh
 
 
+class BoundaryFaceDomain:
+    """BoundaryFaceDomain(mesh) (domain.jl:399-442): the faces with one cell, all
+    boundary labels together."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+
+
 class Measure:
     """Measure(domain, degree): Gauss-Legendre of that degree (measure.jl:30-55)."""
 
     def __init__(self, domain, degree: int):
-        if not isinstance(domain, (CellDomain, InteriorFaceDomain)):
-            raise NotImplementedError("only CellDomain and InteriorFaceDomain measures are supported")
+        if not isinstance(domain, (CellDomain, InteriorFaceDomain, BoundaryFaceDomain)):
+            raise NotImplementedError(
+                "only CellDomain, InteriorFaceDomain and BoundaryFaceDomain measures are supported")
         self.domain = domain
         self.degree = int(degree)
 
@@ -519,3 +546,183 @@ def lower_dg_transport(items):
     if vol.measure.domain.mesh is not face.measure.domain.mesh:
         raise NotImplementedError("cell and face measures must be on one mesh")
     return c, u, vol.measure.degree
+
+
+# ---------------------------------------------------------------------------
+# Face integrands of the interior-penalty kind, lowered symbolically
+# ---------------------------------------------------------------------------
+# A face integrand is expanded into a sum of products of atoms: the test
+# function's value "v{s}" or gradient "gv{s}" on side s (0 = side⁻, the cell
+# that created the face; 1 = side⁺), the same for the trial function ("u",
+# "gu"), the outward normals "n{s}", and a PhysicalFunction "g:{id}".  A
+# product keeps its scalar atoms (dot products "d:x|y" among them) and at most
+# one vector atom.  The normals of the two sides of a straight face are
+# opposite, n1 = -n0, so every normal is σ_s n_0 with σ_0 = 1, σ_1 = -1.
+_SIGMA = {0: 1.0, 1: -1.0}
+
+
+class _Sym:
+    """Σ coef · Π scalars · vector; terms[(scalars (sorted tuple), vector or None)] = coef."""
+
+    def __init__(self, terms=None):
+        self.terms = dict(terms or {})
+
+    @staticmethod
+    def atom(name, vector=False):
+        return _Sym({((), name) if vector else ((name,), None): 1.0})
+
+    def rank(self):
+        ranks = {vec is not None for (_, vec) in self.terms}
+        if len(ranks) > 1:
+            raise NotImplementedError("a face integrand adds a scalar and a vector")
+        return ranks.pop() if ranks else False
+
+    def add(self, o):
+        t = dict(self.terms)
+        for k, c in o.terms.items():
+            t[k] = t.get(k, 0.0) + c
+        out = _Sym(t)
+        out.rank()
+        return out
+
+    def mul(self, o):
+        t = {}
+        for (sa, va), ca in self.terms.items():
+            for (sb, vb), cb in o.terms.items():
+                if va is not None and vb is not None:
+                    raise NotImplementedError("product of two vectors in a face integrand "
+                                              "(use dot)")
+                k = (tuple(sorted(sa + sb)), va if va is not None else vb)
+                t[k] = t.get(k, 0.0) + ca * cb
+        return _Sym(t)
+
+    def dot(self, o):
+        t = {}
+        for (sa, va), ca in self.terms.items():
+            for (sb, vb), cb in o.terms.items():
+                if va is None or vb is None:
+                    raise NotImplementedError("dot of a scalar in a face integrand")
+                d = "d:" + "|".join(sorted((va, vb)))
+                k = (tuple(sorted(sa + sb + (d,))), None)
+                t[k] = t.get(k, 0.0) + ca * cb
+        return _Sym(t)
+
+
+def _face_sym(e, side, gfuns):
+    """Expand a face integrand; `side` is the side an enclosing side⁻/side⁺ picked."""
+    if isinstance(e, Const):
+        return _Sym({((), None): e.v})
+    if isinstance(e, Add):
+        return _face_sym(e.a, side, gfuns).add(_face_sym(e.b, side, gfuns))
+    if isinstance(e, Mul):
+        return _face_sym(e.a, side, gfuns).mul(_face_sym(e.b, side, gfuns))
+    if isinstance(e, Dot):
+        return _face_sym(e.a, side, gfuns).dot(_face_sym(e.b, side, gfuns))
+    if isinstance(e, Side):
+        return _face_sym(e.a, 0 if e.which == "n" else 1, gfuns)
+    if isinstance(e, Jump):
+        return _face_sym(e.a, 0, gfuns).add(_face_sym(Mul(Const(-1.0), e.a), 1, gfuns))
+    if isinstance(e, Compose):
+        if isinstance(e.f, UpwindFlux):
+            raise NotImplementedError("the upwind flux is not an interior-penalty integrand")
+        return _face_sym(_wrap(e.f(*e.args)), side, gfuns)
+    if isinstance(e, PhysicalFunction):
+        gfuns.setdefault(id(e), e)
+        return _Sym.atom(f"g:{id(e)}")
+    if side is None:
+        raise NotImplementedError("a face integrand needs side⁻ / side⁺ on every function")
+    if isinstance(e, Placeholder):
+        return _Sym.atom(("v" if e.role == "test" else "u") + str(side))
+    if isinstance(e, Grad) and isinstance(e.a, Placeholder):
+        return _Sym.atom(("gv" if e.a.role == "test" else "gu") + str(side), vector=True)
+    if isinstance(e, FaceNormal):
+        return _Sym.atom("n" + str(side), vector=True)
+    raise NotImplementedError(f"unsupported operand {type(e).__name__} in a face integrand")
+
+
+def _face_monomials(e, boundary):
+    """(coef, test kind, test side, trial kind or None, trial side, g) per product
+    term; kinds "v" (value) or "dn" (n_0⋅∇), every other normal folded into coef."""
+    gfuns = {}
+    sym = _face_sym(e, 0 if boundary else None, gfuns)
+    if sym.rank():
+        raise NotImplementedError("a face integrand must be a scalar")
+    out = []
+    for (scalars, _), c in sym.terms.items():
+        if c == 0.0:
+            continue
+        test = trial = g = None
+        for a in scalars:
+            if a.startswith("g:"):
+                if g is not None:
+                    raise NotImplementedError("product of two functions in a face integrand")
+                g = gfuns[int(a[2:])]
+                continue
+            if a.startswith("d:"):
+                x, y = a[2:].split("|")
+                if x[0] == "n" and y[0] == "n":
+                    c *= _SIGMA[int(x[1:])] * _SIGMA[int(y[1:])]
+                    continue
+                nrm, grd = (x, y) if x[0] == "n" else (y, x)
+                if nrm[0] != "n":
+                    raise NotImplementedError("∇u⋅∇v on faces")
+                c *= _SIGMA[int(nrm[1:])]
+                kind, s = "dn", int(grd[-1])
+                role = grd[1]
+            else:
+                kind, s, role = "v", int(a[-1]), a[0]
+            slot = (kind, s)
+            if role == "v":
+                if test is not None:
+                    raise NotImplementedError("two test functions in one face term")
+                test = slot
+            else:
+                if trial is not None:
+                    raise NotImplementedError("two trial functions in one face term")
+                trial = slot
+        if test is None:
+            raise NotImplementedError("a face term without the test function")
+        out.append((c, test, trial, g))
+    return out
+
+
+def lower_face_bilinear(expr, boundary: bool):
+    """Coefficients of a bilinear face integrand in the monomials the face
+    kernels evaluate: per (test side t, trial side r) the 4 values
+    (A, B, C, E) of  A (n_t⋅n_r) v_t u_r + B v_t (n_t⋅∇u_r) + C (n_r⋅∇v_t) u_r
+    + E v_t u_r,  tr = t*2 + r; 16 values on interior faces, 4 on boundary
+    faces (t = r = 0).  Integrands are sums of products of side⁻/side⁺ values
+    and gradients of u and v, normals and constants (jump, avg and Python
+    functions applied with compose expand to these)."""
+    coef = np.zeros(16 if not boundary else 4)
+    for c, test, trial, g in _face_monomials(expr, boundary):
+        if trial is None or g is not None:
+            raise NotImplementedError("a bilinear face term needs u and v and no function of x")
+        (tk, t), (rk, r) = test, trial
+        base = 4 * (2 * t + r) if not boundary else 0
+        if tk == "v" and rk == "v":
+            coef[base + 3] += c
+        elif tk == "v" and rk == "dn":
+            coef[base + 1] += c * _SIGMA[t]      # n_0 = σ_t n_t
+        elif tk == "dn" and rk == "v":
+            coef[base + 2] += c * _SIGMA[r]      # n_0 = σ_r n_r
+        else:
+            raise NotImplementedError("(n⋅∇v)(n⋅∇u) face terms")
+    return coef
+
+
+def lower_face_linear(expr):
+    """A boundary linear face integrand as [(c0, c1, g)]: Σ (c0 v + c1 n⋅∇v) g,
+    one entry per PhysicalFunction g (None: g = 1)."""
+    groups = {}
+    for c, test, trial, g in _face_monomials(expr, True):
+        if trial is not None:
+            raise NotImplementedError("a linear face term holds no trial function")
+        kind, _ = test
+        c0, c1, _g = groups.get(id(g), (0.0, 0.0, g))
+        if kind == "v":
+            c0 += c
+        else:
+            c1 += c
+        groups[id(g)] = (c0, c1, g)
+    return list(groups.values())

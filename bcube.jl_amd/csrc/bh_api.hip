@@ -10,6 +10,8 @@
 #include <set>
 #include <string>
 #include <tuple>
+#include <unordered_map>
+#include <array>
 #include <vector>
 
 #include "bh_dg.h"
@@ -144,6 +146,28 @@ struct bh_blocks_s {
   double* d_block[kMaxBlocks] = {};  // library-owned scalar block values
   double* d_out = nullptr;        // host-`where` staging of the values
   double *d_b = nullptr, *d_sb = nullptr;  // host-`where` staging of vectors
+};
+
+// Face terms of bilinear/linear forms on a Quad4 mesh (bh_dgb_*): the
+// faces, their point tables, and the pattern over "virtual cells" -- each
+// cell (volume terms), each interior face's four (test side, trial side)
+// blocks and each boundary face -- with rows indexed by trial dofs, so the
+// row gather produces the CSC values.
+struct bh_dgb_s {
+  bh_space V = nullptr;
+  int dev = -1;
+  int nl = 0, nq = 0, vol_degree = 0, nvol = 0;
+  bh_term vol[2];
+  bool has_vol = false, has_int = false, has_bnd = false;
+  double coef[20] = {};             // interior (t, r) x (A, B, C, E), then boundary (A, B, C, E)
+  int64_t nfi = 0, nfb = 0, nvirt = 0, vbase_int = 0, vbase_bnd = 0;
+  HostPattern h;
+  int32_t *d_fi = nullptr, *d_fb = nullptr, *d_pptr = nullptr, *d_pairs = nullptr;
+  int32_t* d_c2n = nullptr;         // Quad4 nodes, int32
+  int64_t* d_rowptr = nullptr;
+  void* d_goff = nullptr;
+  double *d_vals = nullptr, *d_ke = nullptr, *d_tab = nullptr, *d_coefs = nullptr,
+         *d_vcoef = nullptr, *d_g = nullptr, *d_b = nullptr;
 };
 
 struct bh_dg_s {
@@ -1121,6 +1145,350 @@ int bh_blocks_destroy(bh_blocks b) {
     for (double* q : b->d_block) (void)hipFree(q);
   }
   delete b;
+  return BH_OK;
+}
+
+// ---- face terms on Quad4 meshes -----------------------------------------------------
+int bh_dgb_create(bh_space V, int quad_degree, int nvol, const bh_term* vol, int vol_degree,
+                  const double* interior, const double* boundary, bh_dgb* out) {
+  if (!V || !out || nvol < 0 || nvol > 2 || (nvol > 0 && !vol)) return BH_ERR_ARG;
+  *out = nullptr;
+  bh_mesh m = V->mesh;
+  bh_ctx c = m->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "face terms need a mesh created with a context");
+  if (m->geo != BH_QUAD4) return fail(c, BH_ERR_UNSUPPORTED, "face terms need a uniform Quad4 mesh");
+  const int nl = V->nl;
+  if (nl != 4 && nl != 9 && nl != 16 && nl != 25)
+    return fail(c, BH_ERR_UNSUPPORTED, "face terms: Lagrange orders 1-4");
+  const int nq = gauss_points_per_axis(quad_degree);
+  if (nq > 8) return fail(c, BH_ERR_UNSUPPORTED, "face terms: at most 8 Gauss points per face");
+  auto d = std::make_unique<bh_dgb_s>();
+  d->V = V;
+  d->dev = V->dev;
+  d->nl = nl;
+  d->nq = nq;
+  d->nvol = nvol;
+  d->vol_degree = vol_degree;
+  for (int k = 0; k < nvol; ++k) d->vol[k] = vol[k];
+  d->has_vol = nvol > 0;
+  d->has_int = interior != nullptr;
+  d->has_bnd = boundary != nullptr;
+  if (interior) std::copy(interior, interior + 16, d->coef);
+  if (boundary) std::copy(boundary, boundary + 4, d->coef + 16);
+  const HostMesh& hm = m->h;
+  const int64_t nc = hm.ncells;
+  // faces in creation order (_build_faces!, mesh.jl:328-374): each cell's
+  // sides in faces2nodes order (s, s+1 mod 4); a face seen again gets its
+  // second cell (side p), traversed forward or against side n
+  std::vector<int32_t> fi, fb;
+  {
+    std::unordered_map<uint64_t, int64_t> index;
+    std::vector<std::array<int32_t, 5>> faces;
+    for (int64_t cc = 0; cc < nc; ++cc) {
+      const int64_t* nd = &hm.c2n[hm.c2n_ptr[cc]];
+      for (int sd = 0; sd < 4; ++sd) {
+        const int64_t a = nd[sd], b = nd[(sd + 1) & 3];
+        const uint64_t key = ((uint64_t)std::min(a, b) << 32) | (uint64_t)std::max(a, b);
+        auto it = index.find(key);
+        if (it == index.end()) {
+          index.emplace(key, (int64_t)faces.size());
+          faces.push_back({(int32_t)cc, sd, -1, -1, 0});
+        } else {
+          auto& F = faces[it->second];
+          const int64_t* np = &hm.c2n[hm.c2n_ptr[F[0]]];
+          F[2] = (int32_t)cc;
+          F[3] = sd;
+          F[4] = a == np[F[1]] ? 0 : 1;  // indexin(f2n_p, f2n_n): same first node or not
+        }
+      }
+    }
+    for (const auto& F : faces) {
+      if (F[2] >= 0) fi.insert(fi.end(), F.begin(), F.end());
+      else { fb.push_back(F[0]); fb.push_back(F[1]); }
+    }
+  }
+  d->nfi = (int64_t)fi.size() / 5;
+  d->nfb = (int64_t)fb.size() / 2;
+  // virtual cells: rows = trial dofs (cdofs of the element rows), entries over test dofs
+  std::vector<int32_t> rd, cd;
+  auto dofs_of = [&](int64_t cc) { return &V->dofs[(size_t)cc * nl]; };
+  auto add = [&](int64_t trial_cell, int64_t test_cell) {
+    const int32_t* a = dofs_of(trial_cell);
+    const int32_t* b = dofs_of(test_cell);
+    rd.insert(rd.end(), a, a + nl);
+    cd.insert(cd.end(), b, b + nl);
+  };
+  int64_t nv = 0;
+  if (d->has_vol) { for (int64_t cc = 0; cc < nc; ++cc) add(cc, cc); nv += nc; }
+  d->vbase_int = nv;
+  if (d->has_int) {
+    for (int64_t f = 0; f < d->nfi; ++f) {
+      const int32_t cells[2] = {fi[5 * f], fi[5 * f + 2]};
+      for (int tr = 0; tr < 4; ++tr) add(cells[tr & 1], cells[tr >> 1]);
+    }
+    nv += 4 * d->nfi;
+  }
+  d->vbase_bnd = nv;
+  if (d->has_bnd) { for (int64_t f = 0; f < d->nfb; ++f) add(fb[2 * f], fb[2 * f]); nv += d->nfb; }
+  d->nvirt = nv;
+  std::string err;
+  if (nv > 0 && !build_pattern(rd, nl, V->ndofs, cd, nl, V->ndofs, nv, d->h, err))
+    return fail(c, BH_ERR_ARG, err);
+  if (nv > 0 && d->h.max_row > 512) return fail(c, BH_ERR_UNSUPPORTED, "CSR rows longer than 512 entries");
+  // face-point tables: per (side, orientation, point) the reference point of
+  // the square on that side (mapping_face, mapping.jl:324-336)
+  std::vector<double> gx, gw;
+  gauss_legendre(nq, gx, gw);
+  const int order = V->order;
+  std::vector<double> tab((size_t)dgb_table_size(nl, nq), 0.0);
+  double* tdN = tab.data();
+  double* tN = tdN + 4 * 2 * nq * 4 * 2;
+  double* tphi = tN + 4 * 2 * nq * 4;
+  double* tdphi = tphi + 4 * 2 * nq * nl;
+  double* tw = tdphi + 4 * 2 * nq * nl * 2;
+  const double V4[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+  for (int sd = 0; sd < 4; ++sd)
+    for (int o = 0; o < 2; ++o)
+      for (int q = 0; q < nq; ++q) {
+        const int a = o ? (sd + 1) & 3 : sd, b = o ? sd : (sd + 1) & 3;
+        const double t = gx[q];
+        const double xi = 0.5 * (1 - t) * V4[a][0] + 0.5 * (1 + t) * V4[b][0];
+        const double et = 0.5 * (1 - t) * V4[a][1] + 0.5 * (1 + t) * V4[b][1];
+        const int base = (sd * 2 + o) * nq + q;
+        const double Nv[4] = {(1 - xi) * (1 - et) / 4, (1 + xi) * (1 - et) / 4,
+                              (1 + xi) * (1 + et) / 4, (1 - xi) * (1 + et) / 4};
+        const double dNx[4] = {-(1 - et) / 4, (1 - et) / 4, (1 + et) / 4, -(1 + et) / 4};
+        const double dNy[4] = {-(1 - xi) / 4, -(1 + xi) / 4, (1 + xi) / 4, (1 - xi) / 4};
+        for (int k = 0; k < 4; ++k) {
+          tN[base * 4 + k] = Nv[k];
+          tdN[(base * 4 + k) * 2] = dNx[k];
+          tdN[(base * 4 + k) * 2 + 1] = dNy[k];
+        }
+        double lx[8], dlx[8], ly[8], dly[8];
+        lagrange_1d(order, xi, lx, dlx);
+        lagrange_1d(order, et, ly, dly);
+        for (int j = 0; j <= order; ++j)
+          for (int i = 0; i <= order; ++i) {
+            const int l = i + (order + 1) * j;
+            tphi[base * nl + l] = lx[i] * ly[j];
+            tdphi[(base * nl + l) * 2] = dlx[i] * ly[j];
+            tdphi[(base * nl + l) * 2 + 1] = lx[i] * dly[j];
+          }
+      }
+  for (int q = 0; q < nq; ++q) tw[q] = gw[q];
+  if (int r = set_device(c)) return r;
+  auto up = [&](auto** dst, const auto& v) -> int {
+    BH_HIP(c, dalloc(dst, std::max<size_t>(v.size(), 1)));
+    if (!v.empty())
+      BH_HIP(c, hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
+    return BH_OK;
+  };
+  std::vector<int32_t> c2n32(hm.c2n.begin(), hm.c2n.end());
+  std::vector<double> coefs(d->coef, d->coef + 20);
+  if (int r = up(&d->d_fi, fi)) return r;
+  if (int r = up(&d->d_fb, fb)) return r;
+  if (int r = up(&d->d_c2n, c2n32)) return r;
+  if (int r = up(&d->d_tab, tab)) return r;
+  if (int r = up(&d->d_coefs, coefs)) return r;
+  if (nv > 0) {
+    const HostPattern& h = d->h;
+    if (int r = up(&d->d_rowptr, h.rowptr)) return r;
+    if (int r = up(&d->d_pptr, h.pptr)) return r;
+    if (int r = up(&d->d_pairs, h.pad ? h.pairs_padded : h.pairs)) return r;
+    if (int r = up((uint8_t**)&d->d_goff, h.goff)) return r;
+    BH_HIP(c, dalloc(&d->d_vals, (size_t)std::max<int64_t>(h.nnz, 1)));
+    BH_HIP(c, dalloc(&d->d_ke, (size_t)(nv + 16) * nl * ke_stride(nl)));
+    BH_HIP(c, dalloc(&d->d_vcoef, (size_t)2 * nc));
+    d->h.goff.clear();
+    d->h.pairs.clear();
+    d->h.pairs_padded.clear();
+  }
+  *out = d.release();
+  return BH_OK;
+}
+
+int bh_dgb_info(bh_dgb d, int64_t* n, int64_t* nnz, int64_t* ninterior, int64_t* nboundary,
+                int32_t* nq) {
+  if (!d) return BH_ERR_ARG;
+  if (n) *n = d->V->ndofs;
+  if (nnz) *nnz = d->h.nnz;
+  if (ninterior) *ninterior = d->nfi;
+  if (nboundary) *nboundary = d->nfb;
+  if (nq) *nq = d->nq;
+  return BH_OK;
+}
+
+int bh_dgb_export_csc(bh_dgb d, int64_t* colptr, int64_t* rowval, int base) {
+  if (!d || !colptr || !rowval) return BH_ERR_ARG;
+  if (d->nvirt == 0) return fail(d->V->mesh->ctx, BH_ERR_ARG, "no bilinear terms");
+  const HostPattern& h = d->h;
+  for (size_t k = 0; k < h.rowptr.size(); ++k) colptr[k] = h.rowptr[k] + base;
+  for (int64_t k = 0; k < h.nnz; ++k) rowval[k] = (int64_t)h.colval[k] + base;
+  return BH_OK;
+}
+
+int bh_dgb_assemble(bh_dgb d, double* nzval, int where) {
+  if (!d || !nzval) return BH_ERR_ARG;
+  bh_space V = d->V;
+  bh_mesh m = V->mesh;
+  bh_ctx c = m->ctx;
+  if (d->nvirt == 0) return fail(c, BH_ERR_ARG, "no bilinear terms");
+  if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
+  if (where == BH_HOST && c->capturing)
+    return fail(c, BH_ERR_ARG, "host-`where` calls cannot be captured in a graph");
+  if (int r = set_device(c)) return r;
+  const int nl = d->nl, ks = ke_stride(nl);
+  const int64_t nc = m->h.ncells;
+  if (d->has_vol) {
+    const bh_term* stiff = nullptr;
+    const bh_term* mass = nullptr;
+    for (int k = 0; k < d->nvol; ++k) {
+      const bh_term& t = d->vol[k];
+      if (t.coef_kind != BH_COEF_CONST && t.coef_kind != BH_COEF_CELL)
+        return fail(c, BH_ERR_UNSUPPORTED, "volume coefficients must be constant or per cell");
+      if (t.form == BH_FORM_STIFFNESS && !stiff) stiff = &t;
+      else if (t.form == BH_FORM_MASS && !mass) mass = &t;
+      else return fail(c, BH_ERR_UNSUPPORTED, "unsupported or repeated volume term");
+    }
+    const GeoBucket& gb = m->buckets[0];
+    const int key = rule_key(gb.geo, d->vol_degree);
+    if (!kernel_supported(gb.geo, V->order, key))
+      return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this (cell type, order, quadrature)");
+    double* tab = nullptr;
+    if (int r = get_tables(c, gb.geo, V->order, d->vol_degree, &tab)) return r;
+    LaunchArgs A;
+    A.kind = LaunchArgs::BILINEAR;
+    A.mode = BH_FP64;
+    A.stream = c->stream;
+    A.coords = m->d_coords;
+    A.ke = d->d_ke;
+    A.ke_fp64 = true;
+    A.erow = nl;
+    if (stiff) {
+      A.eta = coef_ptr(*stiff, BH_HOST, d->d_vcoef, nc, c->stream);
+      A.eta_scale = stiff->scale;
+    }
+    if (mass) {
+      A.mass = true;
+      A.mcoef = coef_ptr(*mass, BH_HOST, d->d_vcoef + nc, nc, c->stream);
+      A.mass_scale = mass->scale;
+    }
+    A.c2n = gb.d_c2n;
+    A.cells = gb.d_cells;
+    A.tab = tab;
+    A.ncells_total = gb.n;
+    A.spare_cell = d->nvirt;  // past the face blocks: ke holds nvirt + 16 rows of cells
+    time_begin(c, 0);
+    if (mfma_supported(gb.geo, V->order, key)) BH_HIP(c, launch_mfma(gb.geo, V->order, key, A));
+    else BH_HIP(c, launch(gb.geo, V->order, key, A));
+    time_end(c);
+  }
+  if (d->has_int) {
+    time_begin(c, 0);
+    BH_HIP(c, launch_dgb_faces(c->stream, nl, true, m->d_coords, d->d_c2n, d->d_fi, d->nfi,
+                               d->d_tab, d->nq, d->d_coefs, d->vbase_int, ks, d->d_ke));
+    time_end(c);
+  }
+  if (d->has_bnd) {
+    time_begin(c, 0);
+    BH_HIP(c, launch_dgb_faces(c->stream, nl, false, m->d_coords, d->d_c2n, d->d_fb, d->nfb,
+                               d->d_tab, d->nq, d->d_coefs + 16, d->vbase_bnd, ks, d->d_ke));
+    time_end(c);
+  }
+  const HostPattern& h = d->h;
+  LaunchArgs G;
+  G.kind = LaunchArgs::GATHER;
+  G.mode = BH_FP64;
+  G.stream = c->stream;
+  G.ke = d->d_ke;
+  G.ke_fp64 = true;
+  G.erow = nl;
+  G.pptr = d->d_pptr;
+  G.pairs = d->d_pairs;
+  G.padded_pairs = h.pad > 0;
+  G.max_adj = h.pad;
+  G.goff = d->d_goff;
+  G.offset_bytes = h.offset_bytes;
+  G.rowptr = d->d_rowptr;
+  G.nrows = h.nrows;
+  G.max_row = h.max_row;
+  G.vals = where == BH_DEVICE ? nzval : d->d_vals;
+  time_begin(c, 1);
+  BH_HIP(c, launch_gather(nl, G));
+  time_end(c);
+  if (where == BH_HOST) {
+    BH_HIP(c, hipMemcpyAsync(nzval, d->d_vals, (size_t)h.nnz * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    BH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return BH_OK;
+}
+
+int bh_dgb_boundary_points(bh_dgb d, double* xq, int where) {
+  if (!d || !xq) return BH_ERR_ARG;
+  bh_ctx c = d->V->mesh->ctx;
+  if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
+  if (int r = set_device(c)) return r;
+  const size_t n = (size_t)d->nfb * d->nq * 2;
+  double* x = xq;
+  if (where == BH_HOST) BH_HIP(c, dalloc(&x, std::max<size_t>(n, 1)));
+  hipError_t e = launch_dgb_points(c->stream, d->V->mesh->d_coords, d->d_c2n, d->d_fb, d->nfb,
+                                   d->d_tab, d->nl, d->nq, x);
+  if (e == hipSuccess && where == BH_HOST && n > 0) {
+    e = hipMemcpyAsync(xq, x, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  if (where == BH_HOST) (void)hipFree(x);
+  if (e != hipSuccess) return fail(c, BH_ERR_HIP, hipGetErrorString(e));
+  return BH_OK;
+}
+
+int bh_dgb_boundary_linear(bh_dgb d, double c0, double c1, const double* g, double* b, int where) {
+  if (!d || !g || !b) return BH_ERR_ARG;
+  bh_space V = d->V;
+  bh_ctx c = V->mesh->ctx;
+  if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
+  if (where == BH_HOST && c->capturing)
+    return fail(c, BH_ERR_ARG, "host-`where` calls cannot be captured in a graph");
+  if (V->continuous)
+    return fail(c, BH_ERR_UNSUPPORTED, "boundary linear face terms need a discontinuous space");
+  if (int r = set_device(c)) return r;
+  const double* dg = g;
+  double* db = b;
+  const size_t ng = (size_t)d->nfb * d->nq;
+  if (where == BH_HOST) {
+    if (!d->d_g) {
+      BH_HIP(c, dalloc(&d->d_g, std::max<size_t>(ng, 1)));
+      BH_HIP(c, dalloc(&d->d_b, (size_t)std::max<int64_t>(V->ndofs, 1)));
+    }
+    if (ng > 0)
+      BH_HIP(c, hipMemcpyAsync(d->d_g, g, ng * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    BH_HIP(c, hipMemcpyAsync(d->d_b, b, (size_t)V->ndofs * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    dg = d->d_g;
+    db = d->d_b;
+  }
+  BH_HIP(c, launch_dgb_bnd_linear(c->stream, d->nl, V->mesh->d_coords, d->d_c2n, d->d_fb, d->nfb,
+                                  d->d_tab, d->nq, c0, c1, dg, V->d_dofs, db));
+  if (where == BH_HOST) {
+    BH_HIP(c, hipMemcpyAsync(b, db, (size_t)V->ndofs * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    BH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return BH_OK;
+}
+
+int bh_dgb_destroy(bh_dgb d) {
+  if (!d) return BH_ERR_ARG;
+  if (d->dev >= 0) {
+    (void)hipSetDevice(d->dev);
+    for (void* q : {(void*)d->d_fi, (void*)d->d_fb, (void*)d->d_pptr, (void*)d->d_pairs,
+                    (void*)d->d_c2n, (void*)d->d_rowptr, d->d_goff, (void*)d->d_vals,
+                    (void*)d->d_ke, (void*)d->d_tab, (void*)d->d_coefs, (void*)d->d_vcoef,
+                    (void*)d->d_g, (void*)d->d_b})
+      (void)hipFree(q);
+  }
+  delete d;
   return BH_OK;
 }
 

@@ -88,6 +88,22 @@ struct BlockVals {
 };
 hipError_t launch_blocks_expand(hipStream_t s, const int32_t* src, const uint8_t* pair, int64_t n,
                                 const BlockVals& bv, double* out);
+// Face terms on a Quad4 mesh (bh_dgb.hip).  tabs: the face-point tables,
+// dgb_table_size(nl, nq) doubles: [4 sides][2 orientations][nq] geometric
+// ξ-derivatives (4x2), geometric values (4), Lagrange values (nl), Lagrange
+// ξ-gradients (nl x 2), then the nq Gauss weights.
+int dgb_table_size(int nl, int nq);
+hipError_t launch_dgb_faces(hipStream_t s, int nl, bool interior, const double* coords,
+                            const int32_t* c2n, const int32_t* faces, int64_t nfaces,
+                            const double* tabs, int nq, const double* coef, int64_t vbase, int ks,
+                            double* ke);
+hipError_t launch_dgb_bnd_linear(hipStream_t s, int nl, const double* coords, const int32_t* c2n,
+                                 const int32_t* faces, int64_t nfaces, const double* tabs, int nq,
+                                 double c0, double c1, const double* g, const int32_t* dofs,
+                                 double* b);
+hipError_t launch_dgb_points(hipStream_t s, const double* coords, const int32_t* c2n,
+                             const int32_t* faces, int64_t nfaces, const double* tabs, int nl,
+                             int nq, double* xq);
 // out[map[g]] += v[g] for g < n (the targets of one field are distinct).
 hipError_t launch_add_field(hipStream_t s, double* out, const int32_t* map, const double* v,
                             int64_t n);

@@ -305,6 +305,41 @@ int bh_dg_minv_axpy(bh_dg dg, int32_t nfields, const void* r, const void* x1, do
                     void* out1, const void* x2, double a2, void* out2);
 int bh_dg_destroy(bh_dg dg);
 
+/* ---- face terms of bilinear and linear forms (Quad4 meshes) --------------- */
+/* assemble_bilinear / assemble_linear over InteriorFaceDomain and
+ * BoundaryFaceDomain measures (assembler.jl:172-310, 448-520, domain.jl:192-197,
+ * 399-442) for the symmetric interior penalty forms of
+ * benchmark/bench_assemble.jl:33-96: on each interior face the integrand is a
+ * sum of monomials in the test function v of side t and the trial function u
+ * of side r,
+ *     A·(n_t⋅n_r) v u + B·v (n_t⋅∇u) + C·(n_r⋅∇v) u + E·v u,
+ * with n_s the outward unit normal of side s (on boundary faces t = r and
+ * n_t⋅n_r = 1).  `interior` holds the 16
+ * coefficients (A,B,C,E) per (t, r) block in the order t*2 + r (side 0 = side⁻,
+ * the cell that created the face; side 1 = side⁺); jump(v, n) = v⁻n⁻ + v⁺n⁺
+ * and avg(∇v) = (∇v⁻ + ∇v⁺)/2 expand into such coefficients.  `boundary`
+ * holds (A,B,C,E) of the same monomials on boundary faces.  Either may be
+ * NULL (no such term).  `vol` adds stiffness / mass cell terms (constant or
+ * per-cell coefficients) integrated with rule degree vol_degree.  Faces are
+ * numbered as _build_faces! (mesh.jl:328-374) numbers them; the matrix is one
+ * CSC over the union of the cell and face couplings, values summed in a fixed
+ * order.  `V` is a Lagrange space of order 1-4 on a uniform Quad4 mesh
+ * (the boundary linear form needs a discontinuous one). */
+typedef struct bh_dgb_s* bh_dgb;
+int bh_dgb_create(bh_space V, int quad_degree, int nvol, const bh_term* vol, int vol_degree,
+                  const double interior[16], const double boundary[4], bh_dgb* out);
+int bh_dgb_info(bh_dgb d, int64_t* n, int64_t* nnz, int64_t* ninterior, int64_t* nboundary,
+                int32_t* nq);
+int bh_dgb_export_csc(bh_dgb d, int64_t* colptr, int64_t* rowval, int base);
+int bh_dgb_assemble(bh_dgb d, double* nzval, int where);
+/* Physical coordinates of the boundary-face quadrature points,
+ * [nboundary][nq][2]; the values g of a boundary linear form are given there. */
+int bh_dgb_boundary_points(bh_dgb d, double* xq, int where);
+/* b[dof] += Σ_faces Σ_q w |J_f| (c0·v + c1·(∇v⋅n)) g[f][q]: the boundary linear
+ * form  ∫(fl ∘ map(side⁻, (v, ∇v, n, g)))dΓb  with fl linear in (v, ∇v⋅n). */
+int bh_dgb_boundary_linear(bh_dgb d, double c0, double c1, const double* g, double* b, int where);
+int bh_dgb_destroy(bh_dgb d);
+
 /* ---- graphs -------------------------------------------------------------- */
 /* Re-assembly inside time-stepping or Newton loops (e.g. benchmark/covo.jl:14-33
  * calls assemble_linear! every step): capture the device-`where` assembly

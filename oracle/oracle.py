@@ -347,3 +347,147 @@ def geometry27(cnodes, xi):
 def csc_to_scipy(colptr, rowval, nzval, n):
     import scipy.sparse as sp
     return sp.csc_matrix((nzval, rowval - 1, colptr - 1), shape=(n, n))
+
+
+# ---------------------------------------------------------------------------
+# Symmetric interior-penalty DG on Quad4 meshes: the reference's Poisson DG
+# benchmark (benchmark/bench_assemble.jl:33-96).  Plain loops over faces and
+# points, for small meshes.
+# ---------------------------------------------------------------------------
+_SQ_COORDS = np.array([[-1.0, -1.0], [1.0, -1.0], [1.0, 1.0], [-1.0, 1.0]])  # shape.jl:208
+_SQ_F2N = [(0, 1), (1, 2), (2, 3), (3, 0)]                                 # shape.jl:254
+_SQ_NORMALS = np.array([[0.0, -1.0], [1.0, 0.0], [0.0, 1.0], [-1.0, 0.0]])  # shape.jl:278
+
+
+def quad_faces(mesh: OMesh):
+    """_build_faces! (mesh.jl:328-374) on a Quad4 mesh: faces in creation
+    order (each cell's sides in faces2nodes order), [cell_n, side_n, cell_p,
+    side_p] with cell_p = -1 on the boundary."""
+    faces, index = [], {}
+    for c in range(len(mesh.celltypes)):
+        nodes = mesh.c2n[mesh.c2n_ptr[c]:mesh.c2n_ptr[c + 1]]
+        for s, (a, b) in enumerate(_SQ_F2N):
+            key = tuple(sorted((int(nodes[a]), int(nodes[b]))))
+            if key in index:
+                faces[index[key]][2:] = [c, s]
+            else:
+                index[key] = len(faces)
+                faces.append([c, s, -1, -1])
+    return faces
+
+
+def _face_to_cell(local_pair, xi):
+    """mapping_face (mapping.jl:324-336): the Bar2 map of the face's two
+    reference-square corners."""
+    a, b = local_pair
+    return 0.5 * (1.0 - xi) * _SQ_COORDS[a] + 0.5 * (1.0 + xi) * _SQ_COORDS[b]
+
+
+def _side_eval(mesh, c, order, xi_cell, side):
+    nodes = mesh.c2n[mesh.c2n_ptr[c]:mesh.c2n_ptr[c + 1]]
+    X = mesh.coords.reshape(-1, 2)[nodes]
+    J, Ji, det, x = geometry("Quad4", X, xi_cell)
+    val, gref = shape_functions("Square", order, xi_cell)
+    grad = gref @ Ji                                   # ∇φ = J⁻ᵀ ∇̂φ
+    n = _SQ_NORMALS[side] @ Ji                         # normal: J⁻ᵀ n_ref, normalised
+    return val, grad, n / np.linalg.norm(n), x
+
+
+def sipg_quad(mesh: OMesh, order, degree, penalty, f=None, g=None, terms=("vol", "int", "bnd")):
+    """A (scipy CSC) and b of a_Ω + a_Γ + a_Γb and l_Ω + l_Γb, with the
+    discontinuous Lagrange numbering (cell by cell), restating:
+      a_Ω(u,v)  = ∫ ∇v⋅∇u dΩ
+      a_Γ(u,v)  = ∫ -jump(v,n)⋅avg(∇u) - avg(∇v)⋅jump(u,n) + penalty jump(v,n)⋅jump(u,n) dΓ
+      a_Γb(u,v) = ∫ -v (∇u⋅n) - (∇v⋅n) u + penalty v u dΓb
+      l_Ω(v)    = ∫ v f dΩ,   l_Γb(v) = ∫ -(∇v⋅n) g + penalty v g dΓb
+    with jump(a, n) = a⁺ n⁺ + a⁻ n⁻ (cellfunction.jl:310), avg(a) = (a⁺ + a⁻)/2
+    (bench_assemble.jl:31), each side's normal J⁻ᵀ n_ref of its own cell
+    (mapping.jl:843-855), side⁺ points through faces2nodes(side_p)[indexin(f2n_p,
+    f2n_n)] (eval_point.jl:164-206), and the face metric |x_b - x_a|/2 of the
+    Bar2 face.  f, g: functions of one point (2,) -> float."""
+    import scipy.sparse as sp
+
+    nc = len(mesh.celltypes)
+    nl = (order + 1) ** 2
+    n = nc * nl
+    I, J_, X = [], [], []
+    b = np.zeros(n)
+    if "vol" in terms:
+        cp, rv, nz, _ = assemble_bilinear(mesh, order, ["stiffness"], degree=degree, continuous=False)
+        col = np.repeat(np.arange(n), np.diff(cp))
+        I.append(rv - 1); J_.append(col); X.append(nz)
+    wq, xq = quadrature("Line", degree)
+    xq = xq.reshape(-1)
+    faces = quad_faces(mesh)
+    coords = mesh.coords.reshape(-1, 2)
+    for cn, sn, cpp, spp in faces:
+        nodes_n = mesh.c2n[mesh.c2n_ptr[cn]:mesh.c2n_ptr[cn + 1]]
+        fa, fb = _SQ_F2N[sn]
+        dS = np.linalg.norm(coords[nodes_n[fb]] - coords[nodes_n[fa]]) / 2
+        if cpp >= 0 and "int" in terms:
+            nodes_p = mesh.c2n[mesh.c2n_ptr[cpp]:mesh.c2n_ptr[cpp + 1]]
+            gn = [int(nodes_n[k]) for k in _SQ_F2N[sn]]
+            gp = [int(nodes_p[k]) for k in _SQ_F2N[spp]]
+            perm = [gn.index(v) for v in gp]                   # indexin(f2n_p, f2n_n)
+            loc_p = tuple(_SQ_F2N[spp][k] for k in perm)
+            blocks = {}
+            for w, xi in zip(wq, xq):
+                sides = {0: (cn, _side_eval(mesh, cn, order, _face_to_cell(_SQ_F2N[sn], xi), sn)),
+                         1: (cpp, _side_eval(mesh, cpp, order, _face_to_cell(loc_p, xi), spp))}
+                for t in (0, 1):
+                    vt, gt, nt, _ = sides[t][1]
+                    for r in (0, 1):
+                        vr, gr, nr, _ = sides[r][1]
+                        m = (-0.5 * np.outer(vt, gr @ nt) - 0.5 * np.outer(gt @ nr, vr)
+                             + penalty * (nt @ nr) * np.outer(vt, vr))
+                        blocks[(t, r)] = blocks.get((t, r), 0.0) + w * dS * m
+            for (t, r), m in blocks.items():
+                ct, cr = (cn, cpp)[t], (cn, cpp)[r]
+                I.append(np.repeat(ct * nl + np.arange(nl), nl))
+                J_.append(np.tile(cr * nl + np.arange(nl), nl))
+                X.append(m.reshape(-1))
+        if cpp < 0 and "bnd" in terms:
+            m = 0.0
+            bl = 0.0
+            for w, xi in zip(wq, xq):
+                v, gv, nn, x = _side_eval(mesh, cn, order, _face_to_cell(_SQ_F2N[sn], xi), sn)
+                dn = gv @ nn
+                m = m + w * dS * (-np.outer(v, dn) - np.outer(dn, v) + penalty * np.outer(v, v))
+                if g is not None:
+                    gx = g(x)
+                    bl = bl + w * dS * (-dn * gx + penalty * v * gx)
+            I.append(np.repeat(cn * nl + np.arange(nl), nl))
+            J_.append(np.tile(cn * nl + np.arange(nl), nl))
+            X.append(np.asarray(m).reshape(-1))
+            if g is not None:
+                b[cn * nl:(cn + 1) * nl] += bl
+    if f is not None:
+        wv, xv = quadrature("Square", degree)
+        for c in range(nc):
+            nodes = mesh.c2n[mesh.c2n_ptr[c]:mesh.c2n_ptr[c + 1]]
+            Xc = coords[nodes]
+            acc = np.zeros(nl)
+            for w, xi in zip(wv, xv):
+                _, _, det, x = geometry("Quad4", Xc, xi)
+                val, _ = shape_functions("Square", order, xi)
+                acc += w * det * f(x) * val
+            b[c * nl:(c + 1) * nl] += acc
+    A = sp.coo_matrix((np.concatenate(X), (np.concatenate(I), np.concatenate(J_))),
+                      shape=(n, n)).tocsc()
+    A.sum_duplicates()
+    A.sort_indices()
+    return A, b
+
+
+def dg_node_coords(mesh: OMesh, order):
+    """Physical coordinates of the Lagrange nodes of each cell (uniform nodes,
+    first index fastest), cell by cell: the discontinuous dofs' points."""
+    t = np.linspace(-1.0, 1.0, order + 1)
+    pts = np.array([[t[i], t[j]] for j in range(order + 1) for i in range(order + 1)])
+    out = []
+    coords = mesh.coords.reshape(-1, 2)
+    for c in range(len(mesh.celltypes)):
+        nodes = mesh.c2n[mesh.c2n_ptr[c]:mesh.c2n_ptr[c + 1]]
+        for p in pts:
+            out.append(geometry("Quad4", coords[nodes], p)[3])
+    return np.array(out)

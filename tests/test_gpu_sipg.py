@@ -1,0 +1,197 @@
+"""GPU parity of the symmetric interior-penalty Poisson forms
+(benchmark/bench_assemble.jl:33-96): cell, interior-face and boundary-face
+bilinear terms and the boundary-face linear term, assembled by the face
+kernels + row gather, against the oracle's restatement (FP64 within
+1e-12 * max|A|, CSC structure bit-exact), and the benchmark problem itself
+(u = 3x + y² + 2x³ in Q3) solved exactly."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import bcubehip as B
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-12
+
+
+def _rect(n, lo=-0.5, hi=0.5, jitter=0.0, seed=0, rotate=False):
+    """n x n Quad4 cells on [lo, hi]², optional interior jitter and cyclic
+    rotation of some cells' node lists (other local sides meet on a face)."""
+    t = np.linspace(lo, hi, n + 1)
+    X = np.array([[x, y] for y in t for x in t])
+    if jitter:
+        rng = np.random.default_rng(seed)
+        inner = np.all((np.abs(X - lo) > 1e-12) & (np.abs(X - hi) > 1e-12), axis=1)
+        X[inner] += rng.uniform(-jitter, jitter, (inner.sum(), 2)) * (hi - lo) / n
+    cells = []
+    for j in range(n):
+        for i in range(n):
+            a = j * (n + 1) + i
+            c = [a, a + 1, a + n + 2, a + n + 1]
+            if rotate and (i + j) % 2:
+                k = (i + 2 * j) % 3 + 1
+                c = c[k:] + c[:k]
+            cells.append(c)
+    return B.Mesh(X, ["Quad4"] * (n * n), np.array(cells, dtype=np.int64))
+
+
+def _om(m):
+    return O.OMesh(m.dim, m.coords, m.celltypes, m.c2n_ptr, m.c2n)
+
+
+def _ua(x):
+    return 3 * x[..., 0] + x[..., 1] ** 2 + 2 * x[..., 0] ** 3
+
+
+def _f(x):
+    return -2.0 - 12.0 * x[..., 0]
+
+
+def _sipg(m, order, degree, pen, terms=("vol", "int", "bnd")):
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", order), m, "discontinuous")
+    V = B.TestFESpace(U)
+    dΩ = B.Measure(B.CellDomain(m), degree)
+    dΓ = B.Measure(B.InteriorFaceDomain(m), degree)
+    dΓb = B.Measure(B.BoundaryFaceDomain(m), degree)
+    nΓ, nΓb = B.get_face_normals(dΓ), B.get_face_normals(dΓb)
+
+    def a(u, v):
+        parts = []
+        if "vol" in terms:
+            parts.append(B.integrate(B.dot(B.grad(v), B.grad(u)), dΩ))
+        if "int" in terms:
+            parts.append(B.integrate(
+                -B.dot(B.jump(v, nΓ), B.avg(B.grad(u))) - B.dot(B.avg(B.grad(v)), B.jump(u, nΓ))
+                + pen * B.dot(B.jump(v, nΓ), B.jump(u, nΓ)), dΓ))
+        if "bnd" in terms:
+            fa = lambda u, gu, v, gv, n: -v * B.dot(gu, n) - B.dot(gv, n) * u + pen * v * u  # noqa
+            parts.append(B.integrate(
+                B.compose(fa, *map(B.side_n, (u, B.grad(u), v, B.grad(v), nΓb))), dΓb))
+        out = parts[0]
+        for p in parts[1:]:
+            out = out + p
+        return out
+
+    g = B.PhysicalFunction(_ua)
+    f = B.PhysicalFunction(_f)
+    fl = lambda v, gv, n, g: -B.dot(gv, n) * g + pen * v * g  # noqa: E731
+
+    def l(v):
+        return (B.integrate(v * f, dΩ)
+                + B.integrate(B.compose(fl, *map(B.side_n, (v, B.grad(v), nΓb, g))), dΓb))
+
+    return U, V, a, l
+
+
+def _check_matrix(A, R):
+    R = R.tocsc()
+    R.sum_duplicates()
+    R.sort_indices()
+    assert np.array_equal(A.indptr, R.indptr) and np.array_equal(A.indices, R.indices)
+    err = np.abs(A.data - R.data).max() / np.abs(R.data).max()
+    assert err <= RTOL, err
+    return err
+
+
+CASES = [(1, 4, 0.0, False), (2, 4, 0.15, True), (3, 4, 0.0, False), (3, 5, 0.2, True),
+         (4, 3, 0.1, True)]
+
+
+@pytest.mark.parametrize("order,n,jitter,rotate", CASES)
+def test_sipg_matrix_vs_oracle(order, n, jitter, rotate):
+    degree = 2 * order + 1
+    pen = order * (order + 1) * n
+    m = _rect(n, jitter=jitter, seed=order, rotate=rotate)
+    U, V, a, _ = _sipg(m, order, degree, pen)
+    print(f"SIPG Q{order} n={n} jitter={jitter} rotate={rotate}", flush=True)
+    # the oracle numbers discontinuous dofs cell by cell, as the DofHandler does
+    assert np.array_equal(_dofs(U), np.arange(m.ncells * (order + 1) ** 2))
+    A = B.assemble_bilinear(a, U, V)
+    R, _ = O.sipg_quad(_om(m), order, degree, pen)
+    print(f"  rel err {_check_matrix(A, R):.2e}")
+
+
+def _dofs(U):
+    import ctypes as C
+    from bcubehip import _lib as L
+    nl = C.c_int32()
+    nd = C.c_int64()
+    L.check(L.lib().bh_space_info(U.handle, C.byref(nd), C.byref(nl)))
+    out = np.zeros(U.mesh.ncells * nl.value, dtype=np.int32)
+    L.check(L.lib().bh_space_dofs(U.handle, out.ctypes.data))
+    return out
+
+
+@pytest.mark.parametrize("terms", [("int",), ("bnd",), ("vol", "bnd"), ("vol", "int")])
+def test_sipg_terms_separately(terms):
+    order, n = 2, 4
+    m = _rect(n, jitter=0.1, seed=7, rotate=True)
+    U, V, a, _ = _sipg(m, order, 5, 12.0, terms)
+    A = B.assemble_bilinear(a, U, V)
+    R, _ = O.sipg_quad(_om(m), order, 5, 12.0, terms=terms)
+    print(f"terms {terms}: rel err {_check_matrix(A, R):.2e}")
+
+
+@pytest.mark.parametrize("order,n,jitter", [(1, 4, 0.0), (3, 4, 0.0), (2, 5, 0.2)])
+def test_sipg_linear_vs_oracle(order, n, jitter):
+    degree = 2 * order + 1
+    pen = order * (order + 1) * n
+    m = _rect(n, jitter=jitter, seed=1, rotate=jitter > 0)
+    U, V, _, l = _sipg(m, order, degree, pen)
+    b = B.assemble_linear(l, V)
+    _, rb = O.sipg_quad(_om(m), order, degree, pen, f=_f, g=_ua)
+    err = np.abs(b - rb).max() / np.abs(rb).max()
+    print(f"linear Q{order} n={n}: rel err {err:.2e}")
+    assert err <= RTOL
+
+
+def _dg_points(m, order):
+    t = np.linspace(-1.0, 1.0, order + 1)
+    xi = np.array([[t[i], t[j]] for j in range(order + 1) for i in range(order + 1)])
+    N = 0.25 * np.stack([(1 - xi[:, 0]) * (1 - xi[:, 1]), (1 + xi[:, 0]) * (1 - xi[:, 1]),
+                         (1 + xi[:, 0]) * (1 + xi[:, 1]), (1 - xi[:, 0]) * (1 + xi[:, 1])], 1)
+    X = m.coords.reshape(-1, 2)[m.c2n.reshape(-1, 4)]        # (ncells, 4, 2)
+    return np.einsum("qk,ckd->cqd", N, X).reshape(-1, 2)
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.2])
+def test_sipg_benchmark_problem_exact(jitter):
+    """bench_assemble.jl:33-96 with n = 4, Q3, degree 7, γ = 12, h = 1/4: the
+    cubic solution lies in the space and the discrete system reproduces it."""
+    order, n = 3, 4
+    pen = order * (order + 1) / (1.0 / n)
+    m = _rect(n, jitter=jitter, seed=5, rotate=jitter > 0)
+    U, V, a, l = _sipg(m, order, 2 * order + 1, pen)
+    A = B.assemble_bilinear(a, U, V)
+    b = B.assemble_linear(l, V)
+    x = spla.spsolve(A.tocsc(), b)
+    err = np.abs(x - _ua(_dg_points(m, order))).max()
+    print(f"benchmark problem jitter={jitter}: max nodal error {err:.2e}")
+    assert err < 1e-10
+
+
+def test_sipg_device_assembly_bitwise_repeatable():
+    import torch
+    order, n = 3, 6
+    m = _rect(n, jitter=0.1, seed=2, rotate=True)
+    U, V, a, _ = _sipg(m, order, 7, 72.0)
+    plan = B.FaceBilinearPlan(a, U, V)
+    host = plan.assemble_host()
+    d = torch.empty(plan.nnz, dtype=torch.float64, device="cuda")
+    for _ in range(2):
+        d.fill_(np.nan)
+        torch.cuda.synchronize()
+        plan.assemble_device(d.data_ptr())
+        B.lib().bh_sync(m.backend.ctx)
+        assert np.array_equal(d.cpu().numpy(), host)
+    assert np.array_equal(plan.assemble_host(), host)
+
+
+def test_sipg_boundary_linear_needs_discontinuous_space():
+    m = _rect(3)
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", 1), m)
+    V = B.TestFESpace(U)
+    dΓb = B.Measure(B.BoundaryFaceDomain(m), 3)
+    with pytest.raises(B.BcubeHipError):
+        B.assemble_linear(lambda v: B.integrate(B.side_n(v) * 1.0, dΓb), V)

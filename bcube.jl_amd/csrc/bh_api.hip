@@ -1384,13 +1384,13 @@ int bh_dgb_assemble(bh_dgb d, double* nzval, int where) {
     time_end(c);
   }
   if (d->has_int) {
-    time_begin(c, 0);
+    time_begin(c, 2);
     BH_HIP(c, launch_dgb_faces(c->stream, nl, true, m->d_coords, d->d_c2n, d->d_fi, d->nfi,
                                d->d_tab, d->nq, d->d_coefs, d->vbase_int, ks, d->d_ke));
     time_end(c);
   }
   if (d->has_bnd) {
-    time_begin(c, 0);
+    time_begin(c, 2);
     BH_HIP(c, launch_dgb_faces(c->stream, nl, false, m->d_coords, d->d_c2n, d->d_fb, d->nfb,
                                d->d_tab, d->nq, d->d_coefs + 16, d->vbase_bnd, ks, d->d_ke));
     time_end(c);

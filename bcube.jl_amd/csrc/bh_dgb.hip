@@ -56,72 +56,136 @@ __device__ __forceinline__ void side_geometry(const double* __restrict__ coords,
   n[1] = b * inv;
 }
 
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// waves per block of the interior-face kernel: 4, or fewer where their LDS
+// would pass 64 KB
+inline int face_waves(int lds_doubles) {
+  int w = 4;
+  while (w > 1 && w * lds_doubles * (int)sizeof(double) > 65536) w >>= 1;
+  return w;
+}
+
+// LDS doubles per wave of the interior-face kernel: normals (4 nq), J⁻¹ (8 nq),
+// weights (nq), values (2 nq NL), normal derivatives (4 nq NL), two staged
+// blocks (2 NL (NL + 1)); about 7.8 KB at NL 16, nq 4
+__host__ __device__ constexpr int face_wave_lds(int nl, int nq) {
+  return 13 * nq + 6 * nq * nl + 2 * nl * (nl + 1);
+}
+
 // faces: [nfaces][5] = cell_n, side_n, cell_p, side_p, orientation of side p
-// (1: traversed against side n); coef[16] = A,B,C,E for (t, r) = (0,0),(0,1),(1,0),(1,1)
+// (1: traversed against side n); coef[16] = A,B,C,E for (t, r) = (0,0),(0,1),(1,0),(1,1).
+// One wave per face, its own LDS region (dynamic, sized by nq), wave-level
+// barriers only:
+//   1. J⁻¹, normals and weights of the 2 x nq face points (one lane each);
+//   2. per (side, point, basis function): value and the normal derivatives
+//      n_0⋅∇φ, n_1⋅∇φ;
+//   3. column j of block tr: out[i] = Σ_q φ_t,i(q) P_j(q) + (n_r⋅∇φ_t,i)(q) Q_j(q)
+//      with P = w((A n_t⋅n_r + E) φ_r,j + B n_t⋅∇φ_r,j), Q = w C φ_r,j, two
+//      blocks at a time, a column's i range split over ISPLIT lanes so that
+//      the wave is full; staged in LDS;
+//   4. those blocks' 2 NL rows of ks doubles (contiguous in ke) stored as
+//      16-byte vectors, one contiguous KB per store instruction.
 template <int NL>
 __global__ __launch_bounds__(256) void k_dgb_interior(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
     const int32_t* __restrict__ faces, int64_t nfaces, FacePointTab T, int nq,
     const double* __restrict__ coef, int64_t vbase, int ks, double* __restrict__ ke) {
-  __shared__ double s_val[2][kMaxFaceQ][NL];
-  __shared__ double s_gn[2][2][kMaxFaceQ][NL];  // [side of n][side of φ][q][i]: n_s ⋅ ∇φ
-  __shared__ double s_nn[2][2][kMaxFaceQ];      // n_t ⋅ n_r
-  __shared__ double s_w[kMaxFaceQ];             // w_q |face|/2
-  __shared__ double s_n[2][kMaxFaceQ][2];
-  __shared__ double s_g[2][kMaxFaceQ][NL][2];
-  for (int64_t f = blockIdx.x; f < nfaces; f += gridDim.x) {
+  constexpr int RS = NL + 1;  // LDS row stride of the staged blocks
+  constexpr int ISPLIT = 2 * NL * 2 <= 64 ? 2 : 1;
+  constexpr int NI = (NL + ISPLIT - 1) / ISPLIT;
+  extern __shared__ double dyn_lds[];
+  const int wpb = blockDim.x >> 6;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* Ln = dyn_lds + wv * face_wave_lds(NL, nq);  // [2][nq][2]
+  double* LJ = Ln + 4 * nq;                            // [2][nq][2][2]
+  double* Lw = LJ + 8 * nq;                            // [nq]
+  double* Lval = Lw + nq;                              // [2][nq][NL]
+  double* Lgn = Lval + 2 * nq * NL;                    // [2 (n side)][2 (φ side)][nq][NL]
+  double* Lout = Lgn + 4 * nq * NL;                    // [2 NL][RS]
+  auto val = [&](int sd, int q, int i) -> double& { return Lval[(sd * nq + q) * NL + i]; };
+  auto gn = [&](int ns, int ps, int q, int i) -> double& {
+    return Lgn[((ns * 2 + ps) * nq + q) * NL + i];
+  };
+  for (int64_t f = (int64_t)blockIdx.x * wpb + wv; f < nfaces; f += (int64_t)gridDim.x * wpb) {
     const int32_t* F = faces + 5 * f;
     const int32_t cell[2] = {F[0], F[2]}, side[2] = {F[1], F[3]}, orient[2] = {0, F[4]};
-    __syncthreads();
-    // geometry and physical gradients per side and point
-    for (int k = threadIdx.x; k < 2 * nq; k += blockDim.x) {
-      const int sd = k / nq, q = k - sd * nq;
+    // 1. J⁻¹, normals, weights
+    if (lane < 2 * nq) {
+      const int sd = lane / nq, q = lane - sd * nq;
       const int32_t* nodes = c2n + 4 * (int64_t)cell[sd];
       double Ji[2][2], n[2];
       side_geometry(coords, nodes, side[sd], orient[sd], q, nq, T, Ji, n);
-      s_n[sd][q][0] = n[0];
-      s_n[sd][q][1] = n[1];
-      const int base = ((side[sd] * 2 + orient[sd]) * nq + q);
-      for (int i = 0; i < NL; ++i) {
-        const double gx = T.dphi[(base * NL + i) * 2], gy = T.dphi[(base * NL + i) * 2 + 1];
-        s_g[sd][q][i][0] = gx * Ji[0][0] + gy * Ji[1][0];
-        s_g[sd][q][i][1] = gx * Ji[0][1] + gy * Ji[1][1];
-        s_val[sd][q][i] = T.phi[base * NL + i];
-      }
+      Ln[lane * 2] = n[0];
+      Ln[lane * 2 + 1] = n[1];
+      LJ[lane * 4] = Ji[0][0];
+      LJ[lane * 4 + 1] = Ji[0][1];
+      LJ[lane * 4 + 2] = Ji[1][0];
+      LJ[lane * 4 + 3] = Ji[1][1];
       if (sd == 0) {
         // face metric: half the length of the side-n edge (a Bar2 face)
         const int a = side[0], b = (side[0] + 1) & 3;
         const double dx = coords[2 * (int64_t)nodes[b]] - coords[2 * (int64_t)nodes[a]];
         const double dy = coords[2 * (int64_t)nodes[b] + 1] - coords[2 * (int64_t)nodes[a] + 1];
-        s_w[q] = T.w[q] * (0.5 * sqrt(dx * dx + dy * dy));
+        Lw[q] = T.w[q] * (0.5 * sqrt(dx * dx + dy * dy));
       }
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < 4 * nq * NL + 4 * nq; k += blockDim.x) {
-      if (k < 4 * nq * NL) {
-        const int ns = k / (2 * nq * NL), rest = k - ns * 2 * nq * NL;
-        const int ps = rest / (nq * NL), rest2 = rest - ps * nq * NL;
-        const int q = rest2 / NL, i = rest2 - q * NL;
-        s_gn[ns][ps][q][i] = s_n[ns][q][0] * s_g[ps][q][i][0] + s_n[ns][q][1] * s_g[ps][q][i][1];
-      } else {
-        const int m = k - 4 * nq * NL;
-        const int t = m / (2 * nq), r = (m / nq) & 1, q = m % nq;
-        s_nn[t][r][q] = s_n[t][q][0] * s_n[r][q][0] + s_n[t][q][1] * s_n[r][q][1];
-      }
+    wave_sync();
+    // 2. values and normal derivatives of the basis functions
+    for (int k = lane; k < 2 * nq * NL; k += 64) {
+      const int sq = k / NL, i = k - sq * NL;  // sq = sd * nq + q
+      const int sd = sq / nq, q = sq - sd * nq;
+      const int b = (side[sd] * 2 + orient[sd]) * nq + q;
+      const double gx = T.dphi[(b * NL + i) * 2], gy = T.dphi[(b * NL + i) * 2 + 1];
+      const double* J = LJ + sq * 4;
+      const double g0 = gx * J[0] + gy * J[2], g1 = gx * J[1] + gy * J[3];
+      val(sd, q, i) = T.phi[b * NL + i];
+      gn(0, sd, q, i) = Ln[q * 2] * g0 + Ln[q * 2 + 1] * g1;
+      gn(1, sd, q, i) = Ln[(nq + q) * 2] * g0 + Ln[(nq + q) * 2 + 1] * g1;
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < 4 * NL * NL; k += blockDim.x) {
-      const int tr = k / (NL * NL), ij = k - tr * NL * NL;
-      const int t = tr >> 1, r = tr & 1, j = ij / NL, i = ij - j * NL;
-      const double A = coef[4 * tr], Bc = coef[4 * tr + 1], Cc = coef[4 * tr + 2],
-                   E = coef[4 * tr + 3];
-      double acc = 0.0;
-      for (int q = 0; q < nq; ++q) {
-        const double vt = s_val[t][q][i], vr = s_val[r][q][j];
-        acc += s_w[q] * (A * s_nn[t][r][q] * vt * vr + Bc * vt * s_gn[t][r][q][j] +
-                         Cc * s_gn[r][t][q][i] * vr + E * vt * vr);
+    wave_sync();
+    double* dst = ke + (vbase + 4 * f) * NL * (int64_t)ks;
+    for (int h = 0; h < 2; ++h) {
+      // 3. columns of blocks tr = 2h, 2h + 1
+      for (int k = lane; k < 2 * NL * ISPLIT; k += 64) {
+        const int cl = k / ISPLIT, i0 = (k - cl * ISPLIT) * NI;
+        const int c = 2 * NL * h + cl;
+        const int tr = c / NL, j = c - tr * NL;
+        const int t = tr >> 1, r = tr & 1;
+        const double A = coef[4 * tr], Bc = coef[4 * tr + 1], Cc = coef[4 * tr + 2],
+                     E = coef[4 * tr + 3];
+        double acc[NI];
+#pragma unroll
+        for (int ii = 0; ii < NI; ++ii) acc[ii] = 0.0;
+        for (int q = 0; q < nq; ++q) {
+          const double nn = Ln[(t * nq + q) * 2] * Ln[(r * nq + q) * 2] +
+                            Ln[(t * nq + q) * 2 + 1] * Ln[(r * nq + q) * 2 + 1];
+          const double vr = val(r, q, j);
+          const double P = Lw[q] * ((A * nn + E) * vr + Bc * gn(t, r, q, j));
+          const double Q = Lw[q] * Cc * vr;
+#pragma unroll
+          for (int ii = 0; ii < NI; ++ii)
+            if (i0 + ii < NL) acc[ii] += val(t, q, i0 + ii) * P + gn(r, t, q, i0 + ii) * Q;
+        }
+#pragma unroll
+        for (int ii = 0; ii < NI; ++ii)
+          if (i0 + ii < NL) Lout[cl * RS + i0 + ii] = acc[ii];
       }
-      ke[((vbase + 4 * f + tr) * NL + j) * (int64_t)ks + i] = acc;
+      wave_sync();
+      // 4. rows 2NL h .. 2NL h + 2NL - 1 of the face (contiguous in ke)
+      double* dh = dst + (int64_t)2 * NL * h * ks;
+      for (int u = lane; u < NL * ks; u += 64) {  // 16-byte units (ks is even)
+        const int d = 2 * u, row = d / ks, col = d - row * ks;
+        double2 v;
+        v.x = col < NL ? Lout[row * RS + col] : 0.0;
+        v.y = col + 1 < NL ? Lout[row * RS + col + 1] : 0.0;
+        *reinterpret_cast<double2*>(dh + d) = v;
+      }
+      wave_sync();
     }
   }
 }
@@ -240,11 +304,14 @@ hipError_t launch_dgb_faces(hipStream_t s, int nl, bool interior, const double* 
   T.phi = T.N + 4 * 2 * nq * 4;
   T.dphi = T.phi + 4 * 2 * nq * nl;
   T.w = T.dphi + 4 * 2 * nq * nl * 2;
+  int wpb = 1;
 #define BH_DGB(NL)                                                                              \
   case NL:                                                                                      \
+    wpb = face_waves(face_wave_lds(NL, nq));                                                    \
     if (interior)                                                                               \
-      hipLaunchKernelGGL(k_dgb_interior<NL>, blocks_for(nfaces), dim3(256), 0, s, coords, c2n,  \
-                         faces, nfaces, T, nq, coef, vbase, ks, ke);                            \
+      hipLaunchKernelGGL(k_dgb_interior<NL>, blocks_for((nfaces + wpb - 1) / wpb),             \
+                         dim3(64 * wpb), wpb * face_wave_lds(NL, nq) * sizeof(double), s, coords, \
+                         c2n, faces, nfaces, T, nq, coef, vbase, ks, ke);                        \
     else                                                                                        \
       hipLaunchKernelGGL(k_dgb_boundary<NL>, blocks_for(nfaces), dim3(256), 0, s, coords, c2n,  \
                          faces, nfaces, T, nq, coef, vbase, ks, ke);                            \

@@ -733,6 +733,164 @@ def bench_basic(args, torch, B):
     return line
 
 
+def sipg_forms(B, m, order, pen, degree):
+    """The forms of benchmark/bench_assemble.jl:33-96 on mesh m: a_Ω, a_Γ, a_Γb,
+    l_Ω, l_Γb with u_a = 3x + y² + 2x³, f = -2 - 12x, g = u_a."""
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", order), m, "discontinuous")
+    V = B.TestFESpace(U)
+    dΩ = B.Measure(B.CellDomain(m), degree)
+    dΓ = B.Measure(B.InteriorFaceDomain(m), degree)
+    dΓb = B.Measure(B.BoundaryFaceDomain(m), degree)
+    nΓ, nΓb = B.get_face_normals(dΓ), B.get_face_normals(dΓb)
+    ua = B.PhysicalFunction(lambda x: 3 * x[:, 0] + x[:, 1] ** 2 + 2 * x[:, 0] ** 3)
+    f = B.PhysicalFunction(lambda x: -2.0 - 12.0 * x[:, 0])
+    fa = lambda u, gu, v, gv, n: -v * B.dot(gu, n) - B.dot(gv, n) * u + pen * v * u  # noqa: E731
+    fl = lambda v, gv, n, g: -B.dot(gv, n) * g + pen * v * g  # noqa: E731
+    F = {
+        "a_Ω": lambda u, v: B.integrate(B.dot(B.grad(v), B.grad(u)), dΩ),
+        "a_Γ": lambda u, v: B.integrate(
+            -B.dot(B.jump(v, nΓ), B.avg(B.grad(u))) - B.dot(B.avg(B.grad(v)), B.jump(u, nΓ))
+            + pen * B.dot(B.jump(v, nΓ), B.jump(u, nΓ)), dΓ),
+        "a_Γb": lambda u, v: B.integrate(
+            B.compose(fa, *map(B.side_n, (u, B.grad(u), v, B.grad(v), nΓb))), dΓb),
+        "l_Ω": lambda v: B.integrate(v * f, dΩ),
+        "l_Γb": lambda v: B.integrate(B.compose(fl, *map(B.side_n, (v, B.grad(v), nΓb, ua))),
+                                      dΓb),
+    }
+    F["a"] = lambda u, v: F["a_Ω"](u, v) + F["a_Γ"](u, v) + F["a_Γb"](u, v)
+    F["l"] = lambda v: F["l_Ω"](v) + F["l_Γb"](v)
+    return U, V, F
+
+
+def bench_sipg(args, torch, B):
+    """The reference's Poisson DG benchmark (benchmark/bench_assemble.jl:33-96):
+    DG Q3, degree 7, γ = 12, h = 1/n on [-1/2, 1/2]², cell + interior-face +
+    boundary-face forms.  The reference's own 4x4 suite is timed call by call
+    (assemble_bilinear / assemble_linear end to end, plans built inside the
+    call as the reference builds its sparse matrix inside the call); the
+    throughput line is one re-assembly of a = a_Ω + a_Γ + a_Γb into
+    device-resident CSC values on an n x n mesh (n = 256 unless --n)."""
+    import scipy.sparse.linalg as spla
+
+    order, degree = 3, 7
+    gamma = order * (order + 1)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    # 1. the reference's suite at 4 x 4
+    m4 = B.rectangle_mesh(5, 5, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5)
+    U4, V4, F4 = sipg_forms(B, m4, order, gamma * 4, degree)
+    suite = {}
+    for name in ("a_Ω", "a_Γ", "a_Γb", "l_Ω", "l_Γb"):
+        call = ((lambda nm=name: B.assemble_bilinear(F4[nm], U4, V4)) if name[0] == "a"
+                else (lambda nm=name: B.assemble_linear(F4[nm], V4)))
+        for _ in range(3):
+            call()
+        reps = 50
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        suite[name] = (time.perf_counter() - t0) / reps * 1e3
+    t0 = time.perf_counter()
+    for _ in range(10):
+        A4 = B.assemble_bilinear(F4["a"], U4, V4)
+        b4 = B.assemble_linear(F4["l"], V4)
+    suite["AffineFESystem (a and l)"] = (time.perf_counter() - t0) / 10 * 1e3
+    x4 = spla.spsolve(A4.tocsc(), b4)
+    X = m4.coords.reshape(-1, 2)[m4.c2n.reshape(-1, 4)]
+    t = np.linspace(-1.0, 1.0, order + 1)
+    xi = np.array([[t[i], t[j]] for j in range(order + 1) for i in range(order + 1)])
+    N = 0.25 * np.stack([(1 - xi[:, 0]) * (1 - xi[:, 1]), (1 + xi[:, 0]) * (1 - xi[:, 1]),
+                         (1 + xi[:, 0]) * (1 + xi[:, 1]), (1 - xi[:, 0]) * (1 + xi[:, 1])], 1)
+    P = np.einsum("qk,ckd->cqd", N, X).reshape(-1, 2)
+    sol_err = float(np.abs(x4 - (3 * P[:, 0] + P[:, 1] ** 2 + 2 * P[:, 0] ** 3)).max())
+    # 2. throughput: re-assembly of a into HBM on n x n
+    n = args.n if args.n != 64 else 256
+    m = B.rectangle_mesh(n + 1, n + 1, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5)
+    U, V, F = sipg_forms(B, m, order, gamma * n, degree)
+    t_setup = time.perf_counter()
+    plan = B.FaceBilinearPlan(F["a"], U, V)
+    t_setup = time.perf_counter() - t_setup
+    m.backend.set_stream(stream.cuda_stream)
+    out = torch.empty(plan.nnz, dtype=torch.float64, device=dev)
+    for _ in range(args.warmup):
+        plan.assemble_device(out.data_ptr())
+    torch.cuda.synchronize()
+    m.backend.set_timing(True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan.assemble_device(out.data_ptr())
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    launches, kinds = m.backend.timings_ex(cap=100000)
+    m.backend.set_timing(False)
+    launches = np.asarray(launches, dtype=np.float64)
+    kinds = np.asarray(kinds)
+    nl = (order + 1) ** 2
+    nq = order + 1
+    ncells, nfi, nfb = m.ncells, plan.ninterior, plan.nboundary
+    # per step: element kernel (kind 0), interior then boundary face kernel
+    # (kind 2, in that order), row gather (kind 1)
+    face_t = launches[kinds == 2].reshape(args.steps, -1)
+    per = {"volume": float(launches[kinds == 0].sum() / args.steps),
+           "interior_faces": float(face_t[:, 0].mean()),
+           "boundary_faces": float(face_t[:, 1].mean()),
+           "gather": float(launches[kinds == 1].sum() / args.steps)}
+    # algorithmic bytes: the interior-face kernel writes 4 blocks of nl x nl
+    # FP64 per face and reads 8 node coordinates; the gather writes 8 B per
+    # CSC value
+    alg = {"interior_faces": nfi * (4 * nl * nl * 8 + 8 * 2 * 8 + 5 * 4),
+           "gather": 8 * plan.nnz}
+    dom = max(("interior_faces", "gather", "volume"), key=lambda k: per[k])
+    kname = {"interior_faces": f"k_dgb_interior<{nl}>",
+             "gather": f"k_gather_padded<double,unsigned char,{nl},...>",
+             "volume": "k_element_mfma (Quad4 Q3, 16 points)"}
+    roof = None
+    if dom in alg:
+        ach = alg[dom] / (per[dom] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": kname[dom],
+                "avg_launch_ms": per[dom], "alg_bytes_per_launch": alg[dom]}
+    kernels = {k: {"avg_launch_ms": v, "kernel": kname.get(k, "k_dgb_boundary<16>"),
+                   "achieved_GBs": (alg[k] / (v * 1e-3) / 1e9) if k in alg else None}
+               for k, v in per.items()}
+    line = {
+        "metric": "Poisson DG (bench_assemble.jl:33-96) bilinear assembly GNNZ/s, FP64",
+        "value": plan.nnz / (ms * 1e-3) / 1e9, "unit": "GNNZ/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"rectangle_mesh({n + 1}, {n + 1}) on [-1/2,1/2]², DG Q3, degree "
+                               f"7, γ/h = {gamma * n}, a_Ω + a_Γ + a_Γb into device-resident CSC",
+                   "ncells": ncells, "ndofs": plan.n, "nnz": plan.nnz, "interior_faces": nfi,
+                   "boundary_faces": nfb, "face_points": nq, "setup_s": round(t_setup, 2)},
+        "roofline": roof,
+        "kernels": kernels,
+        "reference_suite_4x4_ms": suite,
+        "benchmark_solution_max_err": sol_err,
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ns = 40
+        ms_ = B.rectangle_mesh(ns + 1, ns + 1, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5)
+        om = O.OMesh(ms_.dim, ms_.coords, ms_.celltypes, ms_.c2n_ptr, ms_.c2n)
+        t0 = time.perf_counter()
+        R, _ = O.sipg_quad(om, order, degree, gamma * ns)
+        dt = time.perf_counter() - t0
+        R.sum_duplicates()
+        om4 = O.OMesh(m4.dim, m4.coords, m4.celltypes, m4.c2n_ptr, m4.c2n)
+        t0 = time.perf_counter()
+        O.sipg_quad(om4, order, degree, gamma * 4)
+        dt4 = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": R.nnz / dt / 1e9, "unit": "GNNZ/s", "cores": 1,
+                                "kind": "port",
+                                "sample": f"oracle restatement (Python loops over faces and "
+                                          f"points), a on a {ns}x{ns} mesh, {R.nnz} nnz in "
+                                          f"{dt:.2f} s; the 4x4 a in {dt4 * 1e3:.0f} ms"}
+    return line
+
+
 def bench_build(args, torch, B):
     """Setup on the device for meshes that change: the DofHandler numbering
     (bh_dofhandler_device) and the CSR pattern with its gather lists
@@ -826,12 +984,13 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="c2",
-                    choices=["c1", "c2", "c3", "c4", "c5", "vec", "build", "basic"],
+                    choices=["c1", "c2", "c3", "c4", "c5", "vec", "build", "basic", "sipg"],
                     help="c2: the headline 64^3 Q2 hexa workload; c1: the 64^2 Q1 Poisson "
                          "matrix + right-hand side; c3: mixed tri/quad order buckets; c4: the DG "
                          "transport residual; c5: the many-instance batch; vec: a 3-component "
                          "Q2 hexa space (48^3 unless --n); build: the device numbering and "
-                         "pattern build; basic: the reference's basic_bilinear benchmark")
+                         "pattern build; basic: the reference's basic_bilinear benchmark; "
+                         "sipg: the reference's Poisson DG benchmark (256^2 unless --n)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters against a known byte count)")
@@ -856,6 +1015,9 @@ def main():
         return
     if args.config == "basic":
         print(json.dumps(bench_basic(args, torch, B)), flush=True)
+        return
+    if args.config == "sipg":
+        print(json.dumps(bench_sipg(args, torch, B)), flush=True)
         return
     if args.config == "c5":
         print(json.dumps(bench_c5(args, torch, B)), flush=True)

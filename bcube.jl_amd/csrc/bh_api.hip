@@ -162,6 +162,18 @@ struct bh_dgb_s {
   double coef[20] = {};             // interior (t, r) x (A, B, C, E), then boundary (A, B, C, E)
   int64_t nfi = 0, nfb = 0, nvirt = 0, vbase_int = 0, vbase_bnd = 0;
   HostPattern h;
+  // Direct path (discontinuous spaces numbered cell by cell, no two cells
+  // sharing two faces): the interior-face kernel stores its two off-diagonal
+  // blocks straight into the CSC values (each of their entries has that one
+  // contribution); the diagonal blocks -- volume, each face's (s, s) block,
+  // boundary faces -- are summed per cell in the order the row gather would
+  // use, so both paths give the same values bit for bit.
+  bool direct = false;
+  int64_t* d_fpos = nullptr;    // [nfi][2]: CSC position of block (0,1), (1,0) entry (j=0, i=0)
+  int32_t* d_fstride = nullptr; // [nfi][2]: their column lengths
+  int32_t* d_dsrc = nullptr;    // [ncells][kDgbSources]: diagonal sources (ke blocks), -1 = none
+  int64_t* d_dpos = nullptr;    // [ncells]: CSC position of the diagonal block, -1 = none
+  int32_t* d_dstride = nullptr; // [ncells]
   int32_t *d_fi = nullptr, *d_fb = nullptr, *d_pptr = nullptr, *d_pairs = nullptr;
   int32_t* d_c2n = nullptr;         // Quad4 nodes, int32
   int64_t* d_rowptr = nullptr;
@@ -1235,6 +1247,73 @@ int bh_dgb_create(bh_space V, int quad_degree, int nvol, const bh_term* vol, int
   if (nv > 0 && !build_pattern(rd, nl, V->ndofs, cd, nl, V->ndofs, nv, d->h, err))
     return fail(c, BH_ERR_ARG, err);
   if (nv > 0 && d->h.max_row > 512) return fail(c, BH_ERR_UNSUPPORTED, "CSR rows longer than 512 entries");
+  // direct path: cell-major discontinuous numbering, distinct cell pairs per face
+  std::vector<int64_t> fpos, dpos;
+  std::vector<int32_t> fstride, dsrc, dstride;
+  if (nv > 0 && !V->continuous) {
+    bool ok = true;
+    for (int64_t k = 0; k < nc * nl && ok; ++k) ok = V->dofs[k] == (int32_t)k;
+    if (ok && d->has_int) {
+      std::unordered_map<uint64_t, int> seen;
+      for (int64_t f = 0; f < d->nfi && ok; ++f) {
+        const uint64_t a = (uint32_t)fi[5 * f], b = (uint32_t)fi[5 * f + 2];
+        ok = seen.emplace((std::min(a, b) << 32) | std::max(a, b), 1).second;
+      }
+    }
+    d->direct = ok;
+  }
+  if (d->direct) {
+    const HostPattern& h = d->h;
+    // the block of cell t inside the columns of cell r: binary search of row
+    // t*nl among the rows of column r*nl (rows sorted, whole cells)
+    auto block_pos = [&](int64_t r, int64_t t) -> int64_t {
+      const int64_t b = h.rowptr[r * nl], e = h.rowptr[r * nl + 1];
+      const int32_t* lo = h.colval.data() + b;
+      const int32_t* it = std::lower_bound(lo, h.colval.data() + e, (int32_t)(t * nl));
+      if (it == h.colval.data() + e || *it != (int32_t)(t * nl)) return -1;
+      return b + (it - lo);
+    };
+    const int64_t nfd = d->has_int ? d->nfi : 0;  // faces whose blocks are in the pattern
+    fpos.resize(2 * nfd);
+    fstride.resize(2 * nfd);
+    for (int64_t f = 0; f < nfd; ++f) {
+      const int64_t cn = fi[5 * f], cp = fi[5 * f + 2];
+      fpos[2 * f] = block_pos(cp, cn);      // block (t=0, r=1): rows of cell n, columns of cell p
+      fpos[2 * f + 1] = block_pos(cn, cp);  // block (t=1, r=0)
+      fstride[2 * f] = (int32_t)(h.rowptr[cp * nl + 1] - h.rowptr[cp * nl]);
+      fstride[2 * f + 1] = (int32_t)(h.rowptr[cn * nl + 1] - h.rowptr[cn * nl]);
+      if (fpos[2 * f] < 0 || fpos[2 * f + 1] < 0)
+        return fail(c, BH_ERR_ARG, "face block missing from the pattern");
+    }
+    // diagonal sources per cell in virtual-cell order: volume, interior faces
+    // (block nc + 2f + side), boundary faces (block nc + 2 nfi + f)
+    dsrc.assign((size_t)nc * kDgbSources, -1);
+    std::vector<int> cnt(nc, 0);
+    auto push = [&](int64_t cell, int64_t blk) -> bool {
+      if (cnt[cell] >= kDgbSources) return false;
+      dsrc[cell * kDgbSources + cnt[cell]++] = (int32_t)blk;
+      return true;
+    };
+    bool ok = true;
+    if (d->has_vol)
+      for (int64_t cc = 0; cc < nc; ++cc) ok = ok && push(cc, cc);
+    if (d->has_int)
+      for (int64_t f = 0; f < d->nfi; ++f) {
+        ok = ok && push(fi[5 * f], nc + 2 * f);
+        ok = ok && push(fi[5 * f + 2], nc + 2 * f + 1);
+      }
+    if (d->has_bnd)
+      for (int64_t f = 0; f < d->nfb; ++f) ok = ok && push(fb[2 * f], nc + 2 * d->nfi + f);
+    if (!ok) return fail(c, BH_ERR_UNSUPPORTED, "a cell with more than 5 diagonal sources");
+    dpos.assign(nc, -1);
+    dstride.assign(nc, 0);
+    for (int64_t cc = 0; cc < nc; ++cc)
+      if (cnt[cc] > 0) {
+        dpos[cc] = block_pos(cc, cc);
+        dstride[cc] = (int32_t)(h.rowptr[cc * nl + 1] - h.rowptr[cc * nl]);
+        if (dpos[cc] < 0) return fail(c, BH_ERR_ARG, "diagonal block missing from the pattern");
+      }
+  }
   // face-point tables: per (side, orientation, point) the reference point of
   // the square on that side (mapping_face, mapping.jl:324-336)
   std::vector<double> gx, gw;
@@ -1290,7 +1369,20 @@ int bh_dgb_create(bh_space V, int quad_degree, int nvol, const bh_term* vol, int
   if (int r = up(&d->d_c2n, c2n32)) return r;
   if (int r = up(&d->d_tab, tab)) return r;
   if (int r = up(&d->d_coefs, coefs)) return r;
-  if (nv > 0) {
+  if (nv > 0 && d->direct) {
+    if (int r = up(&d->d_fpos, fpos)) return r;
+    if (int r = up(&d->d_fstride, fstride)) return r;
+    if (int r = up(&d->d_dsrc, dsrc)) return r;
+    if (int r = up(&d->d_dpos, dpos)) return r;
+    if (int r = up(&d->d_dstride, dstride)) return r;
+    BH_HIP(c, dalloc(&d->d_vals, (size_t)std::max<int64_t>(d->h.nnz, 1)));
+    // blocks: volume (nc), face diagonals (2 nfi), boundary faces (nfb)
+    BH_HIP(c, dalloc(&d->d_ke, (size_t)(nc + 2 * d->nfi + d->nfb + 16) * nl * ke_stride(nl)));
+    BH_HIP(c, dalloc(&d->d_vcoef, (size_t)2 * nc));
+    d->h.goff.clear();
+    d->h.pairs.clear();
+    d->h.pairs_padded.clear();
+  } else if (nv > 0) {
     const HostPattern& h = d->h;
     if (int r = up(&d->d_rowptr, h.rowptr)) return r;
     if (int r = up(&d->d_pptr, h.pptr)) return r;
@@ -1326,6 +1418,8 @@ int bh_dgb_export_csc(bh_dgb d, int64_t* colptr, int64_t* rowval, int base) {
   for (int64_t k = 0; k < h.nnz; ++k) rowval[k] = (int64_t)h.colval[k] + base;
   return BH_OK;
 }
+
+static int gather_values(bh_dgb d, double* vals);
 
 int bh_dgb_assemble(bh_dgb d, double* nzval, int where) {
   if (!d || !nzval) return BH_ERR_ARG;
@@ -1377,24 +1471,62 @@ int bh_dgb_assemble(bh_dgb d, double* nzval, int where) {
     A.cells = gb.d_cells;
     A.tab = tab;
     A.ncells_total = gb.n;
-    A.spare_cell = d->nvirt;  // past the face blocks: ke holds nvirt + 16 rows of cells
+    // past the face blocks: ke holds 16 spare cells after them
+    A.spare_cell = d->direct ? nc + 2 * d->nfi + d->nfb : d->nvirt;
     time_begin(c, 0);
     if (mfma_supported(gb.geo, V->order, key)) BH_HIP(c, launch_mfma(gb.geo, V->order, key, A));
     else BH_HIP(c, launch(gb.geo, V->order, key, A));
     time_end(c);
   }
-  if (d->has_int) {
-    time_begin(c, 2);
-    BH_HIP(c, launch_dgb_faces(c->stream, nl, true, m->d_coords, d->d_c2n, d->d_fi, d->nfi,
-                               d->d_tab, d->nq, d->d_coefs, d->vbase_int, ks, d->d_ke));
+  const HostPattern& h = d->h;
+  double* vals = where == BH_DEVICE ? nzval : d->d_vals;
+  if (d->direct) {
+    // off-diagonal face blocks straight into vals, diagonal blocks summed per cell
+    if (d->has_int) {
+      time_begin(c, 2);
+      BH_HIP(c, launch_dgb_interior_direct(c->stream, nl, m->d_coords, d->d_c2n, d->d_fi, d->nfi,
+                                           d->d_tab, d->nq, d->d_coefs, nc, ks, d->d_ke,
+                                           d->d_fpos, d->d_fstride, vals));
+      time_end(c);
+    }
+    if (d->has_bnd) {
+      time_begin(c, 2);
+      BH_HIP(c, launch_dgb_faces(c->stream, nl, false, m->d_coords, d->d_c2n, d->d_fb, d->nfb,
+                                 d->d_tab, d->nq, d->d_coefs + 16, nc + 2 * d->nfi, ks,
+                                 d->d_ke));
+      time_end(c);
+    }
+    time_begin(c, 1);
+    BH_HIP(c, launch_dgb_diag(c->stream, nl, d->d_ke, ks, d->d_dsrc, d->d_dpos, d->d_dstride, nc,
+                              vals));
     time_end(c);
+  } else {
+    if (d->has_int) {
+      time_begin(c, 2);
+      BH_HIP(c, launch_dgb_faces(c->stream, nl, true, m->d_coords, d->d_c2n, d->d_fi, d->nfi,
+                                 d->d_tab, d->nq, d->d_coefs, d->vbase_int, ks, d->d_ke));
+      time_end(c);
+    }
+    if (d->has_bnd) {
+      time_begin(c, 2);
+      BH_HIP(c, launch_dgb_faces(c->stream, nl, false, m->d_coords, d->d_c2n, d->d_fb, d->nfb,
+                                 d->d_tab, d->nq, d->d_coefs + 16, d->vbase_bnd, ks, d->d_ke));
+      time_end(c);
+    }
+    if (int r = gather_values(d, vals)) return r;
   }
-  if (d->has_bnd) {
-    time_begin(c, 2);
-    BH_HIP(c, launch_dgb_faces(c->stream, nl, false, m->d_coords, d->d_c2n, d->d_fb, d->nfb,
-                               d->d_tab, d->nq, d->d_coefs + 16, d->vbase_bnd, ks, d->d_ke));
-    time_end(c);
+  if (where == BH_HOST) {
+    BH_HIP(c, hipMemcpyAsync(nzval, d->d_vals, (size_t)h.nnz * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    BH_HIP(c, hipStreamSynchronize(c->stream));
   }
+  return BH_OK;
+}
+
+// the row gather of the virtual cells into the CSC values
+static int gather_values(bh_dgb d, double* vals) {
+  bh_ctx c = d->V->mesh->ctx;
+  const int nl = d->nl;
   const HostPattern& h = d->h;
   LaunchArgs G;
   G.kind = LaunchArgs::GATHER;
@@ -1412,15 +1544,10 @@ int bh_dgb_assemble(bh_dgb d, double* nzval, int where) {
   G.rowptr = d->d_rowptr;
   G.nrows = h.nrows;
   G.max_row = h.max_row;
-  G.vals = where == BH_DEVICE ? nzval : d->d_vals;
+  G.vals = vals;
   time_begin(c, 1);
   BH_HIP(c, launch_gather(nl, G));
   time_end(c);
-  if (where == BH_HOST) {
-    BH_HIP(c, hipMemcpyAsync(nzval, d->d_vals, (size_t)h.nnz * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream));
-    BH_HIP(c, hipStreamSynchronize(c->stream));
-  }
   return BH_OK;
 }
 
@@ -1485,7 +1612,8 @@ int bh_dgb_destroy(bh_dgb d) {
     for (void* q : {(void*)d->d_fi, (void*)d->d_fb, (void*)d->d_pptr, (void*)d->d_pairs,
                     (void*)d->d_c2n, (void*)d->d_rowptr, d->d_goff, (void*)d->d_vals,
                     (void*)d->d_ke, (void*)d->d_tab, (void*)d->d_coefs, (void*)d->d_vcoef,
-                    (void*)d->d_g, (void*)d->d_b})
+                    (void*)d->d_g, (void*)d->d_b, (void*)d->d_fpos, (void*)d->d_fstride,
+                    (void*)d->d_dsrc, (void*)d->d_dpos, (void*)d->d_dstride})
       (void)hipFree(q);
   }
   delete d;

@@ -90,11 +90,13 @@ __host__ __device__ constexpr int face_wave_lds(int nl, int nq) {
 //      the wave is full; staged in LDS;
 //   4. those blocks' 2 NL rows of ks doubles (contiguous in ke) stored as
 //      16-byte vectors, one contiguous KB per store instruction.
-template <int NL>
+template <int NL, bool DIRECT>
 __global__ __launch_bounds__(256) void k_dgb_interior(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
     const int32_t* __restrict__ faces, int64_t nfaces, FacePointTab T, int nq,
-    const double* __restrict__ coef, int64_t vbase, int ks, double* __restrict__ ke) {
+    const double* __restrict__ coef, int64_t vbase, int ks, double* __restrict__ ke,
+    const int64_t* __restrict__ fpos, const int32_t* __restrict__ fstride,
+    double* __restrict__ vals) {
   constexpr int RS = NL + 1;  // LDS row stride of the staged blocks
   constexpr int ISPLIT = 2 * NL * 2 <= 64 ? 2 : 1;
   constexpr int NI = (NL + ISPLIT - 1) / ISPLIT;
@@ -176,16 +178,59 @@ __global__ __launch_bounds__(256) void k_dgb_interior(
           if (i0 + ii < NL) Lout[cl * RS + i0 + ii] = acc[ii];
       }
       wave_sync();
-      // 4. rows 2NL h .. 2NL h + 2NL - 1 of the face (contiguous in ke)
-      double* dh = dst + (int64_t)2 * NL * h * ks;
-      for (int u = lane; u < NL * ks; u += 64) {  // 16-byte units (ks is even)
-        const int d = 2 * u, row = d / ks, col = d - row * ks;
-        double2 v;
-        v.x = col < NL ? Lout[row * RS + col] : 0.0;
-        v.y = col + 1 < NL ? Lout[row * RS + col + 1] : 0.0;
-        *reinterpret_cast<double2*>(dh + d) = v;
+      if constexpr (!DIRECT) {
+        // 4. rows 2NL h .. 2NL h + 2NL - 1 of the face (contiguous in ke)
+        double* dh = dst + (int64_t)2 * NL * h * ks;
+        for (int u = lane; u < NL * ks; u += 64) {  // 16-byte units (ks is even)
+          const int d = 2 * u, row = d / ks, col = d - row * ks;
+          double2 v;
+          v.x = col < NL ? Lout[row * RS + col] : 0.0;
+          v.y = col + 1 < NL ? Lout[row * RS + col + 1] : 0.0;
+          *reinterpret_cast<double2*>(dh + d) = v;
+        }
+      } else {
+        // 4. direct: the diagonal block of side h to ke block vbase + 2f + h,
+        //    the off-diagonal block to its CSC columns (runs of NL values)
+        const int bd = h == 0 ? 0 : 1;       // staged block holding (h, h)
+        double* dd = ke + (vbase + 2 * f + h) * NL * (int64_t)ks;
+        double* dv = vals + fpos[2 * f + h];
+        const int64_t st = fstride[2 * f + h];
+        for (int u = lane; u < NL * NL; u += 64) {
+          const int j = u / NL, i = u - j * NL;
+          dd[(int64_t)j * ks + i] = Lout[(bd * NL + j) * RS + i];
+          dv[j * st + i] = Lout[((1 - bd) * NL + j) * RS + i];
+        }
       }
       wave_sync();
+    }
+  }
+}
+
+// Diagonal blocks of the direct path: vals[dpos[c] + j*dstride[c] + i] =
+// Σ_k ke[dsrc[c][k]][j][i] over the cell's sources in their listed order (the
+// order of the row gather), one wave per cell.
+template <int NL>
+__global__ __launch_bounds__(256) void k_dgb_diag(const double* __restrict__ ke, int ks,
+                                                  const int32_t* __restrict__ dsrc,
+                                                  const int64_t* __restrict__ dpos,
+                                                  const int32_t* __restrict__ dstride,
+                                                  int64_t ncells, double* __restrict__ vals) {
+  const int wpb = blockDim.x >> 6;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int64_t c = (int64_t)blockIdx.x * wpb + wv; c < ncells; c += (int64_t)gridDim.x * wpb) {
+    const int64_t pos = dpos[c];
+    if (pos < 0) continue;
+    int32_t src[kDgbSources];
+#pragma unroll
+    for (int k = 0; k < kDgbSources; ++k) src[k] = dsrc[c * kDgbSources + k];
+    const int64_t st = dstride[c];
+    for (int u = lane; u < NL * NL; u += 64) {
+      const int j = u / NL, i = u - j * NL;
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < kDgbSources; ++k)
+        if (src[k] >= 0) acc += ke[((int64_t)src[k] * NL + j) * ks + i];
+      vals[pos + j * st + i] = acc;
     }
   }
 }
@@ -309,9 +354,10 @@ hipError_t launch_dgb_faces(hipStream_t s, int nl, bool interior, const double* 
   case NL:                                                                                      \
     wpb = face_waves(face_wave_lds(NL, nq));                                                    \
     if (interior)                                                                               \
-      hipLaunchKernelGGL(k_dgb_interior<NL>, blocks_for((nfaces + wpb - 1) / wpb),             \
+      hipLaunchKernelGGL((k_dgb_interior<NL, false>), blocks_for((nfaces + wpb - 1) / wpb),    \
                          dim3(64 * wpb), wpb * face_wave_lds(NL, nq) * sizeof(double), s, coords, \
-                         c2n, faces, nfaces, T, nq, coef, vbase, ks, ke);                        \
+                         c2n, faces, nfaces, T, nq, coef, vbase, ks, ke, nullptr, nullptr,      \
+                         nullptr);                                                               \
     else                                                                                        \
       hipLaunchKernelGGL(k_dgb_boundary<NL>, blocks_for(nfaces), dim3(256), 0, s, coords, c2n,  \
                          faces, nfaces, T, nq, coef, vbase, ks, ke);                            \
@@ -321,6 +367,53 @@ hipError_t launch_dgb_faces(hipStream_t s, int nl, bool interior, const double* 
     default: return hipErrorInvalidValue;
   }
 #undef BH_DGB
+  return hipGetLastError();
+}
+
+hipError_t launch_dgb_interior_direct(hipStream_t s, int nl, const double* coords,
+                                      const int32_t* c2n, const int32_t* faces, int64_t nfaces,
+                                      const double* tabs, int nq, const double* coef,
+                                      int64_t vbase, int ks, double* ke, const int64_t* fpos,
+                                      const int32_t* fstride, double* vals) {
+  if (nfaces <= 0) return hipSuccess;
+  if (nq > kMaxFaceQ) return hipErrorInvalidValue;
+  FacePointTab T;
+  T.dN = tabs;
+  T.N = T.dN + 4 * 2 * nq * 4 * 2;
+  T.phi = T.N + 4 * 2 * nq * 4;
+  T.dphi = T.phi + 4 * 2 * nq * nl;
+  T.w = T.dphi + 4 * 2 * nq * nl * 2;
+  int wpb = 1;
+#define BH_DGBD(NL)                                                                             \
+  case NL:                                                                                      \
+    wpb = face_waves(face_wave_lds(NL, nq));                                                    \
+    hipLaunchKernelGGL((k_dgb_interior<NL, true>), blocks_for((nfaces + wpb - 1) / wpb),        \
+                       dim3(64 * wpb), wpb * face_wave_lds(NL, nq) * sizeof(double), s, coords,   \
+                       c2n, faces, nfaces, T, nq, coef, vbase, ks, ke, fpos, fstride, vals);     \
+    break;
+  switch (nl) {
+    BH_DGBD(4) BH_DGBD(9) BH_DGBD(16) BH_DGBD(25)
+    default: return hipErrorInvalidValue;
+  }
+#undef BH_DGBD
+  return hipGetLastError();
+}
+
+hipError_t launch_dgb_diag(hipStream_t s, int nl, const double* ke, int ks, const int32_t* dsrc,
+                           const int64_t* dpos, const int32_t* dstride, int64_t ncells,
+                           double* vals) {
+  if (ncells <= 0) return hipSuccess;
+  const dim3 grid = blocks_for((ncells + 3) / 4);
+#define BH_DGBG(NL)                                                                             \
+  case NL:                                                                                      \
+    hipLaunchKernelGGL(k_dgb_diag<NL>, grid, dim3(256), 0, s, ke, ks, dsrc, dpos, dstride,     \
+                       ncells, vals);                                                           \
+    break;
+  switch (nl) {
+    BH_DGBG(4) BH_DGBG(9) BH_DGBG(16) BH_DGBG(25)
+    default: return hipErrorInvalidValue;
+  }
+#undef BH_DGBG
   return hipGetLastError();
 }
 

@@ -97,6 +97,20 @@ hipError_t launch_dgb_faces(hipStream_t s, int nl, bool interior, const double* 
                             const int32_t* c2n, const int32_t* faces, int64_t nfaces,
                             const double* tabs, int nq, const double* coef, int64_t vbase, int ks,
                             double* ke);
+// Direct path of discontinuous spaces numbered cell by cell: the interior-face
+// kernel stores blocks (0,1) and (1,0) of face f at vals[fpos[2f + b] + j *
+// fstride[2f + b] + i] and the (s,s) blocks to ke block vbase + 2f + s;
+// launch_dgb_diag sums each cell's diagonal sources (kDgbSources ke blocks,
+// -1 = none) into vals[dpos[c] + j * dstride[c] + i].
+constexpr int kDgbSources = 5;
+hipError_t launch_dgb_interior_direct(hipStream_t s, int nl, const double* coords,
+                                      const int32_t* c2n, const int32_t* faces, int64_t nfaces,
+                                      const double* tabs, int nq, const double* coef,
+                                      int64_t vbase, int ks, double* ke, const int64_t* fpos,
+                                      const int32_t* fstride, double* vals);
+hipError_t launch_dgb_diag(hipStream_t s, int nl, const double* ke, int ks, const int32_t* dsrc,
+                           const int64_t* dpos, const int32_t* dstride, int64_t ncells,
+                           double* vals);
 hipError_t launch_dgb_bnd_linear(hipStream_t s, int nl, const double* coords, const int32_t* c2n,
                                  const int32_t* faces, int64_t nfaces, const double* tabs, int nq,
                                  double c0, double c1, const double* g, const int32_t* dofs,

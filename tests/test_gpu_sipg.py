@@ -195,3 +195,48 @@ def test_sipg_boundary_linear_needs_discontinuous_space():
     dΓb = B.Measure(B.BoundaryFaceDomain(m), 3)
     with pytest.raises(B.BcubeHipError):
         B.assemble_linear(lambda v: B.integrate(B.side_n(v) * 1.0, dΓb), V)
+
+
+def test_sipg_direct_path_equals_gather_path():
+    """Cell-major DG numbering takes the direct path (off-diagonal face blocks
+    stored in place, diagonal blocks summed per cell); a permuted numbering
+    takes the virtual-cell row gather.  Same sums in the same order: the two
+    matrices agree entry for entry after the permutation."""
+    order, n = 3, 5
+    m = _rect(n, jitter=0.15, seed=4, rotate=True)
+    U, V, a, _ = _sipg(m, order, 7, 60.0)
+    A = B.assemble_bilinear(a, U, V)
+    U2, V2, a2, _ = _sipg(m, order, 7, 60.0)
+    nd = U2.get_ndofs()
+    perm = np.random.default_rng(0).permutation(nd)
+    U2._scalar.iglob = (perm[U2._scalar.iglob - 1] + 1).astype(np.int64)
+    assert not np.array_equal(_dofs(U2), np.arange(nd))
+    A2 = B.assemble_bilinear(a2, U2, V2)
+    back = A2[perm][:, perm].tocsc()
+    back.sort_indices()
+    assert np.array_equal(back.indptr, A.indptr) and np.array_equal(back.indices, A.indices)
+    assert np.array_equal(back.data, A.data)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_sipg_forms_on_a_continuous_space(order):
+    """Face terms over a continuous space go through the row gather.  Its
+    matrix is Pᵀ A_dg P with P the injection of continuous dofs into the
+    cell-by-cell discontinuous ones (each cell's local basis is the same)."""
+    import scipy.sparse as sp
+    n = 4
+    m = _rect(n, jitter=0.1, seed=8, rotate=True)
+    Uc = B.TrialFESpace(B.FunctionSpace("Lagrange", order), m)
+    Vc = B.TestFESpace(Uc)
+    _, _, a, _ = _sipg(m, order, 2 * order + 1, 10.0)
+    a_c = lambda u, v: a(u, v)  # noqa: E731  (the same forms, placeholders of Uc)
+    A = B.assemble_bilinear(a_c, Uc, Vc)
+    R, _ = O.sipg_quad(_om(m), order, 2 * order + 1, 10.0)
+    dofs = _dofs(Uc)
+    nl = (order + 1) ** 2
+    P = sp.csr_matrix((np.ones(dofs.size), (np.arange(dofs.size), dofs)),
+                      shape=(m.ncells * nl, Uc.get_ndofs()))
+    ref = (P.T @ R @ P).toarray()
+    err = np.abs(A.toarray() - ref).max() / np.abs(ref).max()
+    print(f"continuous Q{order}: rel err {err:.2e}")
+    assert err <= RTOL

@@ -180,6 +180,7 @@ struct bh_dgb_s {
   void* d_goff = nullptr;
   double *d_vals = nullptr, *d_ke = nullptr, *d_tab = nullptr, *d_coefs = nullptr,
          *d_vcoef = nullptr, *d_g = nullptr, *d_b = nullptr;
+  double* d_geo = nullptr;      // [nfi][2][nq][8]: interior face-point geometry
 };
 
 struct bh_dg_s {
@@ -1369,6 +1370,8 @@ int bh_dgb_create(bh_space V, int quad_degree, int nvol, const bh_term* vol, int
   if (int r = up(&d->d_c2n, c2n32)) return r;
   if (int r = up(&d->d_tab, tab)) return r;
   if (int r = up(&d->d_coefs, coefs)) return r;
+  if (nv > 0 && d->has_int)
+    BH_HIP(c, dalloc(&d->d_geo, (size_t)std::max<int64_t>(d->nfi, 1) * 2 * nq * 8));
   if (nv > 0 && d->direct) {
     if (int r = up(&d->d_fpos, fpos)) return r;
     if (int r = up(&d->d_fstride, fstride)) return r;
@@ -1486,14 +1489,14 @@ int bh_dgb_assemble(bh_dgb d, double* nzval, int where) {
       time_begin(c, 2);
       BH_HIP(c, launch_dgb_interior_direct(c->stream, nl, m->d_coords, d->d_c2n, d->d_fi, d->nfi,
                                            d->d_tab, d->nq, d->d_coefs, nc, ks, d->d_ke,
-                                           d->d_fpos, d->d_fstride, vals));
+                                           d->d_fpos, d->d_fstride, vals, d->d_geo));
       time_end(c);
     }
     if (d->has_bnd) {
       time_begin(c, 2);
       BH_HIP(c, launch_dgb_faces(c->stream, nl, false, m->d_coords, d->d_c2n, d->d_fb, d->nfb,
                                  d->d_tab, d->nq, d->d_coefs + 16, nc + 2 * d->nfi, ks,
-                                 d->d_ke));
+                                 d->d_ke, nullptr));
       time_end(c);
     }
     time_begin(c, 1);
@@ -1504,13 +1507,15 @@ int bh_dgb_assemble(bh_dgb d, double* nzval, int where) {
     if (d->has_int) {
       time_begin(c, 2);
       BH_HIP(c, launch_dgb_faces(c->stream, nl, true, m->d_coords, d->d_c2n, d->d_fi, d->nfi,
-                                 d->d_tab, d->nq, d->d_coefs, d->vbase_int, ks, d->d_ke));
+                                 d->d_tab, d->nq, d->d_coefs, d->vbase_int, ks, d->d_ke,
+                                 d->d_geo));
       time_end(c);
     }
     if (d->has_bnd) {
       time_begin(c, 2);
       BH_HIP(c, launch_dgb_faces(c->stream, nl, false, m->d_coords, d->d_c2n, d->d_fb, d->nfb,
-                                 d->d_tab, d->nq, d->d_coefs + 16, d->vbase_bnd, ks, d->d_ke));
+                                 d->d_tab, d->nq, d->d_coefs + 16, d->vbase_bnd, ks, d->d_ke,
+                                 nullptr));
       time_end(c);
     }
     if (int r = gather_values(d, vals)) return r;
@@ -1613,7 +1618,7 @@ int bh_dgb_destroy(bh_dgb d) {
                     (void*)d->d_c2n, (void*)d->d_rowptr, d->d_goff, (void*)d->d_vals,
                     (void*)d->d_ke, (void*)d->d_tab, (void*)d->d_coefs, (void*)d->d_vcoef,
                     (void*)d->d_g, (void*)d->d_b, (void*)d->d_fpos, (void*)d->d_fstride,
-                    (void*)d->d_dsrc, (void*)d->d_dpos, (void*)d->d_dstride})
+                    (void*)d->d_dsrc, (void*)d->d_dpos, (void*)d->d_dstride, (void*)d->d_geo})
       (void)hipFree(q);
   }
   delete d;

@@ -13,6 +13,7 @@
 // the row gather sums them into the CSC values the caller receives.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "bh_device.h"
@@ -71,10 +72,43 @@ inline int face_waves(int lds_doubles) {
 }
 
 // LDS doubles per wave of the interior-face kernel: normals (4 nq), J⁻¹ (8 nq),
-// weights (nq), values (2 nq NL), normal derivatives (4 nq NL), two staged
-// blocks (2 NL (NL + 1)); about 7.8 KB at NL 16, nq 4
+// weights (nq), values (2 nq NL), normal derivatives (4 nq NL), and for NL > 16
+// two staged blocks (2 NL (NL + 1)); 3.5 KB at NL 16, nq 4
 __host__ __device__ constexpr int face_wave_lds(int nl, int nq) {
-  return 13 * nq + 6 * nq * nl + 2 * nl * (nl + 1);
+  return 13 * nq + 6 * nq * nl + (nl > 16 ? 2 * nl * (nl + 1) : 0);
+}
+
+// Face-point geometry of the interior faces, one thread per (face, side,
+// point): geo[((f*2 + side)*nq + q)*8 ..] = J⁻¹ (row-major 2x2), the unit
+// outward normal, w_q |face|/2 (side 0; 0 on side 1), 0.  Computed once per
+// assembly so the face kernel's waves start from one contiguous load instead
+// of the faces -> nodes -> coordinates chain.
+__global__ void k_dgb_face_geo(const double* __restrict__ coords, const int32_t* __restrict__ c2n,
+                               const int32_t* __restrict__ faces, int64_t nfaces, FacePointTab T,
+                               int nq, double* __restrict__ geo) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nfaces * 2 * nq;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t f = k / (2 * nq);
+    const int sq = (int)(k - f * 2 * nq), sd = sq / nq, q = sq - sd * nq;
+    const int32_t* F = faces + 5 * f;
+    const int32_t cell = sd ? F[2] : F[0], side = sd ? F[3] : F[1], orient = sd ? F[4] : 0;
+    const int32_t* nodes = c2n + 4 * (int64_t)cell;
+    double Ji[2][2], n[2];
+    side_geometry(coords, nodes, side, orient, q, nq, T, Ji, n);
+    double w = 0.0;
+    if (sd == 0) {
+      // face metric: half the length of the side-n edge (a Bar2 face)
+      const int a = side, b = (side + 1) & 3;
+      const double dx = coords[2 * (int64_t)nodes[b]] - coords[2 * (int64_t)nodes[a]];
+      const double dy = coords[2 * (int64_t)nodes[b] + 1] - coords[2 * (int64_t)nodes[a] + 1];
+      w = T.w[q] * (0.5 * sqrt(dx * dx + dy * dy));
+    }
+    double2* g = reinterpret_cast<double2*>(geo + 8 * k);
+    g[0] = make_double2(Ji[0][0], Ji[0][1]);
+    g[1] = make_double2(Ji[1][0], Ji[1][1]);
+    g[2] = make_double2(n[0], n[1]);
+    g[3] = make_double2(w, 0.0);
+  }
 }
 
 // faces: [nfaces][5] = cell_n, side_n, cell_p, side_p, orientation of side p
@@ -96,7 +130,7 @@ __global__ __launch_bounds__(256) void k_dgb_interior(
     const int32_t* __restrict__ faces, int64_t nfaces, FacePointTab T, int nq,
     const double* __restrict__ coef, int64_t vbase, int ks, double* __restrict__ ke,
     const int64_t* __restrict__ fpos, const int32_t* __restrict__ fstride,
-    double* __restrict__ vals) {
+    double* __restrict__ vals, const double* __restrict__ geo) {
   constexpr int RS = NL + 1;  // LDS row stride of the staged blocks
   constexpr int ISPLIT = 2 * NL * 2 <= 64 ? 2 : 1;
   constexpr int NI = (NL + ISPLIT - 1) / ISPLIT;
@@ -108,7 +142,7 @@ __global__ __launch_bounds__(256) void k_dgb_interior(
   double* Lw = LJ + 8 * nq;                            // [nq]
   double* Lval = Lw + nq;                              // [2][nq][NL]
   double* Lgn = Lval + 2 * nq * NL;                    // [2 (n side)][2 (φ side)][nq][NL]
-  double* Lout = Lgn + 4 * nq * NL;                    // [2 NL][RS]
+  double* Lout = Lgn + 4 * nq * NL;                    // [2 NL][RS], NL > 16 only
   auto val = [&](int sd, int q, int i) -> double& { return Lval[(sd * nq + q) * NL + i]; };
   auto gn = [&](int ns, int ps, int q, int i) -> double& {
     return Lgn[((ns * 2 + ps) * nq + q) * NL + i];
@@ -116,25 +150,20 @@ __global__ __launch_bounds__(256) void k_dgb_interior(
   for (int64_t f = (int64_t)blockIdx.x * wpb + wv; f < nfaces; f += (int64_t)gridDim.x * wpb) {
     const int32_t* F = faces + 5 * f;
     const int32_t cell[2] = {F[0], F[2]}, side[2] = {F[1], F[3]}, orient[2] = {0, F[4]};
-    // 1. J⁻¹, normals, weights
+    // 1. J⁻¹, normals, weights of the face's points (k_dgb_face_geo)
     if (lane < 2 * nq) {
-      const int sd = lane / nq, q = lane - sd * nq;
-      const int32_t* nodes = c2n + 4 * (int64_t)cell[sd];
-      double Ji[2][2], n[2];
-      side_geometry(coords, nodes, side[sd], orient[sd], q, nq, T, Ji, n);
-      Ln[lane * 2] = n[0];
-      Ln[lane * 2 + 1] = n[1];
-      LJ[lane * 4] = Ji[0][0];
-      LJ[lane * 4 + 1] = Ji[0][1];
-      LJ[lane * 4 + 2] = Ji[1][0];
-      LJ[lane * 4 + 3] = Ji[1][1];
-      if (sd == 0) {
-        // face metric: half the length of the side-n edge (a Bar2 face)
-        const int a = side[0], b = (side[0] + 1) & 3;
-        const double dx = coords[2 * (int64_t)nodes[b]] - coords[2 * (int64_t)nodes[a]];
-        const double dy = coords[2 * (int64_t)nodes[b] + 1] - coords[2 * (int64_t)nodes[a] + 1];
-        Lw[q] = T.w[q] * (0.5 * sqrt(dx * dx + dy * dy));
-      }
+      const double* g = geo + ((int64_t)f * 2 * nq + lane) * 8;
+      const double2 g01 = *reinterpret_cast<const double2*>(g);
+      const double2 g23 = *reinterpret_cast<const double2*>(g + 2);
+      const double2 g45 = *reinterpret_cast<const double2*>(g + 4);
+      const double2 g67 = *reinterpret_cast<const double2*>(g + 6);
+      LJ[lane * 4] = g01.x;
+      LJ[lane * 4 + 1] = g01.y;
+      LJ[lane * 4 + 2] = g23.x;
+      LJ[lane * 4 + 3] = g23.y;
+      Ln[lane * 2] = g45.x;
+      Ln[lane * 2 + 1] = g45.y;
+      if (lane < nq) Lw[lane] = g67.x;
     }
     wave_sync();
     // 2. values and normal derivatives of the basis functions
@@ -151,58 +180,115 @@ __global__ __launch_bounds__(256) void k_dgb_interior(
     }
     wave_sync();
     double* dst = ke + (vbase + 4 * f) * NL * (int64_t)ks;
-    for (int h = 0; h < 2; ++h) {
-      // 3. columns of blocks tr = 2h, 2h + 1
-      for (int k = lane; k < 2 * NL * ISPLIT; k += 64) {
-        const int cl = k / ISPLIT, i0 = (k - cl * ISPLIT) * NI;
-        const int c = 2 * NL * h + cl;
-        const int tr = c / NL, j = c - tr * NL;
+    if constexpr (NL <= 16) {
+      // 3+4. on the FP64 matrix cores, one block (t, r) at a time: block^T =
+      // R^T (NL x 2nq) · L^T (2nq x NL), k = 2q + s, R[k][j] = P_j(q) | Q_j(q),
+      // L[i][k] = φ_t,i(q) | n_r⋅∇φ_t,i(q), zero-padded to 16 x 16 x 4 steps.
+      // Lane (am, ak) feeds A[am][ak] = R[ak][j=am], B[ak][am] = L[i=am][ak] and
+      // holds D[ak + 4v][am] = block[i=am][j=ak+4v]: for each v, 16 lanes own
+      // the NL consecutive values of one column j -- stored straight to ke
+      // (row j of the block, stride ks) or, on the direct path, to the CSC
+      // column (off-diagonal blocks) and ke block vbase + 2f + s (diagonal).
+      const int am = lane & 15, ak = lane >> 4;
+#pragma unroll
+      for (int tr = 0; tr < 4; ++tr) {
         const int t = tr >> 1, r = tr & 1;
         const double A = coef[4 * tr], Bc = coef[4 * tr + 1], Cc = coef[4 * tr + 2],
                      E = coef[4 * tr + 3];
-        double acc[NI];
+        typedef double AccT __attribute__((ext_vector_type(4)));
+        AccT acc = {0.0, 0.0, 0.0, 0.0};
+        for (int st = 0; 4 * st < 2 * nq; ++st) {
+          const int k = 4 * st + ak, q = k >> 1, sel = k & 1;
+          double lv = 0.0, rv = 0.0;
+          if (q < nq && am < NL) {
+            lv = sel == 0 ? val(t, q, am) : gn(r, t, q, am);
+            const double vr = val(r, q, am);
+            if (sel == 0) {
+              const double nn = Ln[(t * nq + q) * 2] * Ln[(r * nq + q) * 2] +
+                                Ln[(t * nq + q) * 2 + 1] * Ln[(r * nq + q) * 2 + 1];
+              rv = Lw[q] * ((A * nn + E) * vr + Bc * gn(t, r, q, am));
+            } else {
+              rv = Lw[q] * Cc * vr;
+            }
+          }
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(rv, lv, acc, 0, 0, 0);
+        }
+        double* out;
+        int64_t st;
+        if constexpr (DIRECT) {
+          if (t == r) {
+            out = ke + (vbase + 2 * f + t) * NL * (int64_t)ks;
+            st = ks;
+          } else {
+            out = vals + fpos[2 * f + t];  // (0,1) -> fpos[2f], (1,0) -> fpos[2f + 1]
+            st = fstride[2 * f + t];
+          }
+        } else {
+          out = dst + (int64_t)tr * NL * ks;
+          st = ks;
+        }
+        if (am < NL)
 #pragma unroll
-        for (int ii = 0; ii < NI; ++ii) acc[ii] = 0.0;
-        for (int q = 0; q < nq; ++q) {
-          const double nn = Ln[(t * nq + q) * 2] * Ln[(r * nq + q) * 2] +
-                            Ln[(t * nq + q) * 2 + 1] * Ln[(r * nq + q) * 2 + 1];
-          const double vr = val(r, q, j);
-          const double P = Lw[q] * ((A * nn + E) * vr + Bc * gn(t, r, q, j));
-          const double Q = Lw[q] * Cc * vr;
+          for (int v = 0; v < 4; ++v) {
+            const int j = ak + 4 * v;
+            if (j < NL) out[j * st + am] = acc[v];
+          }
+      }
+    } else {
+      for (int h = 0; h < 2; ++h) {
+        // 3. columns of blocks tr = 2h, 2h + 1, staged in LDS
+        for (int k = lane; k < 2 * NL * ISPLIT; k += 64) {
+          const int cl = k / ISPLIT, i0 = (k - cl * ISPLIT) * NI;
+          const int c = 2 * NL * h + cl;
+          const int tr = c / NL, j = c - tr * NL;
+          const int t = tr >> 1, r = tr & 1;
+          const double A = coef[4 * tr], Bc = coef[4 * tr + 1], Cc = coef[4 * tr + 2],
+                       E = coef[4 * tr + 3];
+          double acc[NI];
+#pragma unroll
+          for (int ii = 0; ii < NI; ++ii) acc[ii] = 0.0;
+          for (int q = 0; q < nq; ++q) {
+            const double nn = Ln[(t * nq + q) * 2] * Ln[(r * nq + q) * 2] +
+                              Ln[(t * nq + q) * 2 + 1] * Ln[(r * nq + q) * 2 + 1];
+            const double vr = val(r, q, j);
+            const double P = Lw[q] * ((A * nn + E) * vr + Bc * gn(t, r, q, j));
+            const double Q = Lw[q] * Cc * vr;
+#pragma unroll
+            for (int ii = 0; ii < NI; ++ii)
+              if (i0 + ii < NL) acc[ii] += val(t, q, i0 + ii) * P + gn(r, t, q, i0 + ii) * Q;
+          }
 #pragma unroll
           for (int ii = 0; ii < NI; ++ii)
-            if (i0 + ii < NL) acc[ii] += val(t, q, i0 + ii) * P + gn(r, t, q, i0 + ii) * Q;
+            if (i0 + ii < NL) Lout[cl * RS + i0 + ii] = acc[ii];
         }
-#pragma unroll
-        for (int ii = 0; ii < NI; ++ii)
-          if (i0 + ii < NL) Lout[cl * RS + i0 + ii] = acc[ii];
+        wave_sync();
+        if constexpr (!DIRECT) {
+          // 4. rows 2NL h .. 2NL h + 2NL - 1 of the face (contiguous in ke)
+          double* dh = dst + (int64_t)2 * NL * h * ks;
+          for (int u = lane; u < NL * ks; u += 64) {  // 16-byte units (ks is even)
+            const int d = 2 * u, row = d / ks, col = d - row * ks;
+            double2 v;
+            v.x = col < NL ? Lout[row * RS + col] : 0.0;
+            v.y = col + 1 < NL ? Lout[row * RS + col + 1] : 0.0;
+            *reinterpret_cast<double2*>(dh + d) = v;
+          }
+        } else {
+          // 4. direct: the diagonal block of side h to ke block vbase + 2f + h,
+          //    the off-diagonal block to its CSC columns (runs of NL values)
+          const int bd = h == 0 ? 0 : 1;       // staged block holding (h, h)
+          double* dd = ke + (vbase + 2 * f + h) * NL * (int64_t)ks;
+          double* dv = vals + fpos[2 * f + h];
+          const int64_t st = fstride[2 * f + h];
+          for (int u = lane; u < NL * NL; u += 64) {
+            const int j = u / NL, i = u - j * NL;
+            dd[(int64_t)j * ks + i] = Lout[(bd * NL + j) * RS + i];
+            dv[j * st + i] = Lout[((1 - bd) * NL + j) * RS + i];
+          }
+        }
+        wave_sync();
       }
-      wave_sync();
-      if constexpr (!DIRECT) {
-        // 4. rows 2NL h .. 2NL h + 2NL - 1 of the face (contiguous in ke)
-        double* dh = dst + (int64_t)2 * NL * h * ks;
-        for (int u = lane; u < NL * ks; u += 64) {  // 16-byte units (ks is even)
-          const int d = 2 * u, row = d / ks, col = d - row * ks;
-          double2 v;
-          v.x = col < NL ? Lout[row * RS + col] : 0.0;
-          v.y = col + 1 < NL ? Lout[row * RS + col + 1] : 0.0;
-          *reinterpret_cast<double2*>(dh + d) = v;
-        }
-      } else {
-        // 4. direct: the diagonal block of side h to ke block vbase + 2f + h,
-        //    the off-diagonal block to its CSC columns (runs of NL values)
-        const int bd = h == 0 ? 0 : 1;       // staged block holding (h, h)
-        double* dd = ke + (vbase + 2 * f + h) * NL * (int64_t)ks;
-        double* dv = vals + fpos[2 * f + h];
-        const int64_t st = fstride[2 * f + h];
-        for (int u = lane; u < NL * NL; u += 64) {
-          const int j = u / NL, i = u - j * NL;
-          dd[(int64_t)j * ks + i] = Lout[(bd * NL + j) * RS + i];
-          dv[j * st + i] = Lout[((1 - bd) * NL + j) * RS + i];
-        }
-      }
-      wave_sync();
     }
+    wave_sync();
   }
 }
 
@@ -331,16 +417,34 @@ __global__ void k_dgb_points(const double* __restrict__ coords, const int32_t* _
   }
 }
 
+// interior-face grid: waves loop over faces, about 8 each once there are
+// more than 2048 blocks' worth, so a wave's set-up is spread over several
+dim3 face_blocks(int64_t nfaces, int wpb) {
+  const int64_t nb = (nfaces + wpb - 1) / wpb;
+  const int64_t cap = std::max<int64_t>(2048, (nb + 7) / 8);
+  return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(std::min(nb, cap), 65536)));
+}
+
 dim3 blocks_for(int64_t n) {
   return dim3((unsigned)(n < 1 ? 1 : (n > 65536 ? 65536 : n)));
 }
 
 }  // namespace
 
+static hipError_t face_geo(hipStream_t s, const double* coords, const int32_t* c2n,
+                           const int32_t* faces, int64_t nfaces, const FacePointTab& T, int nq,
+                           double* geo) {
+  const int64_t n = nfaces * 2 * nq;
+  const dim3 grid((unsigned)((n + 255) / 256 > 65536 ? 65536 : (n + 255) / 256));
+  hipLaunchKernelGGL(k_dgb_face_geo, grid, dim3(256), 0, s, coords, c2n, faces, nfaces, T, nq,
+                     geo);
+  return hipGetLastError();
+}
+
 hipError_t launch_dgb_faces(hipStream_t s, int nl, bool interior, const double* coords,
                             const int32_t* c2n, const int32_t* faces, int64_t nfaces,
                             const double* tabs, int nq, const double* coef, int64_t vbase, int ks,
-                            double* ke) {
+                            double* ke, double* geo) {
   if (nfaces <= 0) return hipSuccess;
   if (nq > kMaxFaceQ) return hipErrorInvalidValue;
   FacePointTab T;
@@ -349,15 +453,18 @@ hipError_t launch_dgb_faces(hipStream_t s, int nl, bool interior, const double* 
   T.phi = T.N + 4 * 2 * nq * 4;
   T.dphi = T.phi + 4 * 2 * nq * nl;
   T.w = T.dphi + 4 * 2 * nq * nl * 2;
+  if (interior) {
+    if (hipError_t e = face_geo(s, coords, c2n, faces, nfaces, T, nq, geo)) return e;
+  }
   int wpb = 1;
 #define BH_DGB(NL)                                                                              \
   case NL:                                                                                      \
     wpb = face_waves(face_wave_lds(NL, nq));                                                    \
     if (interior)                                                                               \
-      hipLaunchKernelGGL((k_dgb_interior<NL, false>), blocks_for((nfaces + wpb - 1) / wpb),    \
+      hipLaunchKernelGGL((k_dgb_interior<NL, false>), face_blocks(nfaces, wpb),               \
                          dim3(64 * wpb), wpb * face_wave_lds(NL, nq) * sizeof(double), s, coords, \
                          c2n, faces, nfaces, T, nq, coef, vbase, ks, ke, nullptr, nullptr,      \
-                         nullptr);                                                               \
+                         nullptr, geo);                                                          \
     else                                                                                        \
       hipLaunchKernelGGL(k_dgb_boundary<NL>, blocks_for(nfaces), dim3(256), 0, s, coords, c2n,  \
                          faces, nfaces, T, nq, coef, vbase, ks, ke);                            \
@@ -374,7 +481,7 @@ hipError_t launch_dgb_interior_direct(hipStream_t s, int nl, const double* coord
                                       const int32_t* c2n, const int32_t* faces, int64_t nfaces,
                                       const double* tabs, int nq, const double* coef,
                                       int64_t vbase, int ks, double* ke, const int64_t* fpos,
-                                      const int32_t* fstride, double* vals) {
+                                      const int32_t* fstride, double* vals, double* geo) {
   if (nfaces <= 0) return hipSuccess;
   if (nq > kMaxFaceQ) return hipErrorInvalidValue;
   FacePointTab T;
@@ -383,13 +490,15 @@ hipError_t launch_dgb_interior_direct(hipStream_t s, int nl, const double* coord
   T.phi = T.N + 4 * 2 * nq * 4;
   T.dphi = T.phi + 4 * 2 * nq * nl;
   T.w = T.dphi + 4 * 2 * nq * nl * 2;
+  if (hipError_t e = face_geo(s, coords, c2n, faces, nfaces, T, nq, geo)) return e;
   int wpb = 1;
 #define BH_DGBD(NL)                                                                             \
   case NL:                                                                                      \
     wpb = face_waves(face_wave_lds(NL, nq));                                                    \
-    hipLaunchKernelGGL((k_dgb_interior<NL, true>), blocks_for((nfaces + wpb - 1) / wpb),        \
+    hipLaunchKernelGGL((k_dgb_interior<NL, true>), face_blocks(nfaces, wpb),                    \
                        dim3(64 * wpb), wpb * face_wave_lds(NL, nq) * sizeof(double), s, coords,   \
-                       c2n, faces, nfaces, T, nq, coef, vbase, ks, ke, fpos, fstride, vals);     \
+                       c2n, faces, nfaces, T, nq, coef, vbase, ks, ke, fpos, fstride, vals,     \
+                       geo);                                                                    \
     break;
   switch (nl) {
     BH_DGBD(4) BH_DGBD(9) BH_DGBD(16) BH_DGBD(25)

@@ -88,7 +88,8 @@ struct BlockVals {
 };
 hipError_t launch_blocks_expand(hipStream_t s, const int32_t* src, const uint8_t* pair, int64_t n,
                                 const BlockVals& bv, double* out);
-// Face terms on a Quad4 mesh (bh_dgb.hip).  tabs: the face-point tables,
+// Face terms on a Quad4 mesh (bh_dgb.hip).  geo: scratch of nfaces * 2 * nq * 8
+// doubles for the interior faces' point geometry.  tabs: the face-point tables,
 // dgb_table_size(nl, nq) doubles: [4 sides][2 orientations][nq] geometric
 // ξ-derivatives (4x2), geometric values (4), Lagrange values (nl), Lagrange
 // ξ-gradients (nl x 2), then the nq Gauss weights.
@@ -96,7 +97,7 @@ int dgb_table_size(int nl, int nq);
 hipError_t launch_dgb_faces(hipStream_t s, int nl, bool interior, const double* coords,
                             const int32_t* c2n, const int32_t* faces, int64_t nfaces,
                             const double* tabs, int nq, const double* coef, int64_t vbase, int ks,
-                            double* ke);
+                            double* ke, double* geo);
 // Direct path of discontinuous spaces numbered cell by cell: the interior-face
 // kernel stores blocks (0,1) and (1,0) of face f at vals[fpos[2f + b] + j *
 // fstride[2f + b] + i] and the (s,s) blocks to ke block vbase + 2f + s;
@@ -107,7 +108,7 @@ hipError_t launch_dgb_interior_direct(hipStream_t s, int nl, const double* coord
                                       const int32_t* c2n, const int32_t* faces, int64_t nfaces,
                                       const double* tabs, int nq, const double* coef,
                                       int64_t vbase, int ks, double* ke, const int64_t* fpos,
-                                      const int32_t* fstride, double* vals);
+                                      const int32_t* fstride, double* vals, double* geo);
 hipError_t launch_dgb_diag(hipStream_t s, int nl, const double* ke, int ks, const int32_t* dsrc,
                            const int64_t* dpos, const int32_t* dstride, int64_t ncells,
                            double* vals);

@@ -812,6 +812,12 @@ def bench_sipg(args, torch, B):
     t_setup = time.perf_counter() - t_setup
     m.backend.set_stream(stream.cuda_stream)
     out = torch.empty(plan.nnz, dtype=torch.float64, device=dev)
+    if args.calib_copy_mib:
+        nb = args.calib_copy_mib * (1 << 20) // 8
+        src = torch.ones(nb, dtype=torch.float64, device=dev)
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
     for _ in range(args.warmup):
         plan.assemble_device(out.data_ptr())
     torch.cuda.synchronize()
@@ -830,30 +836,42 @@ def bench_sipg(args, torch, B):
     nl = (order + 1) ** 2
     nq = order + 1
     ncells, nfi, nfb = m.ncells, plan.ninterior, plan.nboundary
-    # per step: element kernel (kind 0), interior then boundary face kernel
-    # (kind 2, in that order), row gather (kind 1)
+    # per step: element kernel (kind 0); face-point geometry + interior faces,
+    # then boundary faces (kind 2, in that order); the per-cell diagonal sum
+    # (kind 1; discontinuous cell-major spaces take this direct path)
     face_t = launches[kinds == 2].reshape(args.steps, -1)
     per = {"volume": float(launches[kinds == 0].sum() / args.steps),
            "interior_faces": float(face_t[:, 0].mean()),
            "boundary_faces": float(face_t[:, 1].mean()),
-           "gather": float(launches[kinds == 1].sum() / args.steps)}
-    # algorithmic bytes: the interior-face kernel writes 4 blocks of nl x nl
-    # FP64 per face and reads 8 node coordinates; the gather writes 8 B per
-    # CSC value
-    alg = {"interior_faces": nfi * (4 * nl * nl * 8 + 8 * 2 * 8 + 5 * 4),
-           "gather": 8 * plan.nnz}
-    dom = max(("interior_faces", "gather", "volume"), key=lambda k: per[k])
-    kname = {"interior_faces": f"k_dgb_interior<{nl}>",
-             "gather": f"k_gather_padded<double,unsigned char,{nl},...>",
-             "volume": "k_element_mfma (Quad4 Q3, 16 points)"}
+           "diagonal_sum": float(launches[kinds == 1].sum() / args.steps)}
+    # algorithmic bytes per launch: the interior-face kernel writes 4 blocks
+    # of nl x nl FP64 per face (2 into the CSC values, 2 diagonal blocks) and
+    # reads the face record and 8 node coordinates; the diagonal sum reads
+    # each cell's sources (volume block, one diagonal block per face side) and
+    # writes its diagonal block
+    blk = nl * nl * 8
+    alg = {"interior_faces": nfi * (4 * blk + 8 * 2 * 8 + 5 * 4),
+           "diagonal_sum": (ncells + 2 * nfi + nfb) * blk + ncells * blk}
+    dom = max(("interior_faces", "diagonal_sum", "volume"), key=lambda k: per[k])
+    kname = {"interior_faces": f"k_dgb_face_geo + k_dgb_interior<{nl}, true>",
+             "diagonal_sum": f"k_dgb_diag<{nl}>", "boundary_faces": f"k_dgb_boundary<{nl}>",
+             "volume": "k_element_mfma<2, 16, 16, 4, false, 2>"}
+    # HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    # (calibrated on a device copy of known size) at 256^2
+    measured = {}
+    tf = os.path.join(ROOT, "profiles", "sipg_traffic.json")
+    if os.path.exists(tf) and n == 256:
+        measured = json.load(open(tf))["kernels"]
     roof = None
     if dom in alg:
         ach = alg[dom] / (per[dom] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": kname[dom],
-                "avg_launch_ms": per[dom], "alg_bytes_per_launch": alg[dom]}
-    kernels = {k: {"avg_launch_ms": v, "kernel": kname.get(k, "k_dgb_boundary<16>"),
-                   "achieved_GBs": (alg[k] / (v * 1e-3) / 1e9) if k in alg else None}
+                "frac": ach / PEAK_HBM_GBS, "traffic": measured.get(dom),
+                "kernel": kname[dom], "avg_launch_ms": per[dom],
+                "alg_bytes_per_launch": alg[dom]}
+    kernels = {k: {"avg_launch_ms": v, "kernel": kname[k],
+                   "achieved_GBs": (alg[k] / (v * 1e-3) / 1e9) if k in alg else None,
+                   "traffic": measured.get(k)}
                for k, v in per.items()}
     line = {
         "metric": "Poisson DG (bench_assemble.jl:33-96) bilinear assembly GNNZ/s, FP64",

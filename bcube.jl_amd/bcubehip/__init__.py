@@ -11,10 +11,11 @@ from .mesh import (AbstractBcubeBackend, BcubeBackendHIP, BcubeBackendHostOnly, 
                    sub_mesh)
 from .fespace import (DofHandler, FunctionSpace, MultiFESpace, TestFESpace, TrialFESpace,
                       component_numbering)
-from .forms import (BoundaryFaceDomain, CellDomain, FEFunction, InteriorFaceDomain, Measure,
-                    MeshCellData, PhysicalFunction, UpwindFlux, avg, compose, ddot, dot,
-                    get_face_normals, grad, integrate, jump, nabla, side_minus, side_n, side_p,
-                    side_plus)
+from .forms import (BoundaryFaceDomain, CellDomain, EulerFlux, FEFunction, InteriorFaceDomain,
+                    Measure, MeshCellData, PeriodicBCType, PhysicalFunction, RoeFlux,
+                    Translation, UpwindFlux, avg, compose, ddot, dot, get_face_normals, grad,
+                    integrate, jump, nabla, side_minus, side_n, side_p, side_plus)
+from .euler import EulerDG, periodic_face_pairs
 from .faces import BoundaryFaces, FaceBilinearPlan
 from .dg import DGTransport, dg_dof_coordinates
 from .assembler import (BilinearPlan, MultiBilinearPlan, Pattern, assemble_bilinear,
@@ -43,5 +44,6 @@ __all__ = [
     "assemble_dirichlet_vector", "boundary_faces", "dirichlet_dofs_values", "dof_coordinates",
     "nodes_gmsh2cgns", "read_mesh", "projection_l2_", "MultiFESpace", "component_numbering",
     "ddot", "MultiBilinearPlan", "BoundaryFaceDomain", "avg", "BoundaryFaces",
-    "FaceBilinearPlan",
+    "FaceBilinearPlan", "EulerFlux", "RoeFlux", "PeriodicBCType", "Translation", "EulerDG",
+    "periodic_face_pairs",
 ]

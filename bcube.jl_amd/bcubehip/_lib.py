@@ -33,7 +33,8 @@ EXPORTED = [
     "bh_assemble_bilinear", "bh_assemble_linear", "bh_quadrature_points", "bh_dg_create",
     "bh_dg_info", "bh_dg_update_geometry", "bh_dg_transport_residual", "bh_dg_minv_axpy",
     "bh_dg_destroy", "bh_dgb_create", "bh_dgb_info", "bh_dgb_export_csc", "bh_dgb_assemble",
-    "bh_dgb_boundary_points", "bh_dgb_boundary_linear", "bh_dgb_destroy", "bh_graph_begin",
+    "bh_dgb_boundary_points", "bh_dgb_boundary_linear", "bh_dgb_destroy", "bh_euler_create",
+    "bh_euler_residual", "bh_euler_step", "bh_euler_destroy", "bh_graph_begin",
     "bh_graph_end", "bh_graph_launch", "bh_graph_destroy",
 ]
 
@@ -128,6 +129,10 @@ def lib():
         "bh_dgb_boundary_points": (C.c_int, [vp, vp, C.c_int]),
         "bh_dgb_boundary_linear": (C.c_int, [vp, dbl, dbl, vp, vp, C.c_int]),
         "bh_dgb_destroy": (C.c_int, [vp]),
+        "bh_euler_create": (C.c_int, [vp, C.c_int, dbl, i64, vp, vp, i64, P(vp)]),
+        "bh_euler_residual": (C.c_int, [vp, C.c_int, dbl, vp, vp, C.c_int]),
+        "bh_euler_step": (C.c_int, [vp, dbl, vp, vp, C.c_int]),
+        "bh_euler_destroy": (C.c_int, [vp]),
         "bh_graph_begin": (C.c_int, [vp]),
         "bh_graph_end": (C.c_int, [vp, P(vp)]),
         "bh_graph_launch": (C.c_int, [vp]),

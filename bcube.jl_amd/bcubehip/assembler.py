@@ -424,6 +424,18 @@ def assemble_linear_(b: np.ndarray, l, V, mode: str = "fp64") -> None:
     if b.dtype != np.float64 or not b.flags.c_contiguous or b.size != V.get_ndofs():
         raise ValueError("b must be a contiguous float64 vector of ndofs entries")
     if _is_multi(V):
+        from .euler import euler_plan, lower_euler_dg
+        from .fespace import MultiFESpace
+        if isinstance(V, MultiFESpace):
+            f, _ = _fields_blocks(V)
+            eu = lower_euler_dg(_integrals(l(f.placeholders(V, "test"))))
+            if eu is not None:
+                # the COVO Euler DG residual (benchmark/covo.jl): one kernel
+                gamma, u, degree, periodic, terms, fcoef = eu
+                if mode != "fp64":
+                    raise NotImplementedError("the Euler residual is assembled in FP64")
+                euler_plan(V, gamma, degree, periodic).residual(u.values, b, terms, fcoef)
+                return
         _assemble_linear_multi(b, l, V, mode)
         return
     items = _integrals(l(Placeholder("test", V)))

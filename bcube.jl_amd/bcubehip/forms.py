@@ -132,17 +132,43 @@ class UpwindFlux:
 
 
 class Compose(Expr):
-    """f ∘ (a, b, ...) (Julia composition applied to a tuple of operators)."""
+    """f ∘ (a, b, ...) (Julia composition applied to a tuple of operators).  A
+    tuple argument (the fields of a MultiFESpace, `map(∇, v)`) stays a tuple."""
 
     def __init__(self, f, args):
-        self.f, self.args = f, [_wrap(a) for a in args]
+        self.f = f
+        self.args = [tuple(_wrap(x) for x in a) if isinstance(a, tuple) else _wrap(a)
+                     for a in args]
+
+
+class EulerFlux:
+    """_flux_Ω of the COVO benchmark (benchmark/covo.jl:44-58) as a value for
+    compose(EulerFlux(γ), u, map(∇, v)): the Euler fluxes of (ρ, ρu, ρE, ϕ)
+    against the test gradients."""
+
+    def __init__(self, gamma):
+        self.gamma = float(gamma)
+
+
+class RoeFlux:
+    """flux_roe of the COVO benchmark (benchmark/covo.jl:69-150) as a value for
+    compose(RoeFlux(γ), side⁻(u), side⁺(u), jump(v), side⁻(n))."""
+
+    def __init__(self, gamma):
+        self.gamma = float(gamma)
 
 
 def side_n(a):
+    """side⁻ (cellfunction.jl:298-306); on a tuple, of each entry."""
+    if isinstance(a, tuple):
+        return tuple(side_n(x) for x in a)
     return Side("n", _wrap(a) if not isinstance(a, FaceNormal) else a)
 
 
 def side_p(a):
+    """side⁺ (cellfunction.jl:298-306); on a tuple, of each entry."""
+    if isinstance(a, tuple):
+        return tuple(side_p(x) for x in a)
     return Side("p", _wrap(a) if not isinstance(a, FaceNormal) else a)
 
 
@@ -153,6 +179,8 @@ def jump(a, n=None):
     """jump(a) = side⁻(a) - side⁺(a); jump(a, n) = side⁺(a) side⁺(n) + side⁻(a) side⁻(n)
     (cellfunction.jl:308-310)."""
     if n is None:
+        if isinstance(a, tuple):   # jump(a::Tuple) = map(jump, a)
+            return tuple(Jump(_wrap(x)) for x in a)
         return Jump(_wrap(a))
     return Add(Mul(side_p(a), side_p(n)), Mul(side_n(a), side_n(n)))
 
@@ -251,12 +279,46 @@ class InteriorFaceDomain:
         self.mesh = mesh
 
 
+class Translation:
+    """Translation(vec) (periodic transformations, x -> x + vec)."""
+
+    def __init__(self, vec):
+        self.vec = np.asarray(vec, dtype=np.float64).reshape(-1)
+
+    def __call__(self, x):
+        return np.asarray(x, dtype=np.float64) + self.vec
+
+
+class PeriodicBCType:
+    """PeriodicBCType(transformation, labels_master, labels_slave; tol)
+    (boundary_condition.jl): slave faces are the images of master faces."""
+
+    def __init__(self, transformation, labels_master, labels_slave, tol: float = 1e-9):
+        self.transformation = transformation
+        self.labels_master = tuple(labels_master)
+        self.labels_slave = tuple(labels_slave)
+        self.tol = float(tol)
+
+
 class BoundaryFaceDomain:
     """BoundaryFaceDomain(mesh) (domain.jl:399-442): the faces with one cell, all
-    boundary labels together."""
+    boundary labels together; BoundaryFaceDomain(mesh, PeriodicBCType(...))
+    (domain.jl:232-397): the slave faces, each with its master face's cell as
+    side⁺."""
 
-    def __init__(self, mesh):
+    def __init__(self, mesh, bc=None):
         self.mesh = mesh
+        self.bc = bc
+        self._pairs = None
+
+    def periodic_faces(self):
+        """[cell⁻, side⁻, cell⁺, side⁺, flip] per slave face (int32, (n, 5))."""
+        if self.bc is None:
+            raise ValueError("not a periodic face domain")
+        if self._pairs is None:
+            from .euler import periodic_face_pairs
+            self._pairs = periodic_face_pairs(self.mesh, self.bc)
+        return self._pairs
 
 
 class Measure:

@@ -211,9 +211,12 @@ class Mesh:
 
 
 def rectangle_mesh(nx, ny, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, order=1,
-                   backend: AbstractBcubeBackend | None = None) -> Mesh:
+                   backend: AbstractBcubeBackend | None = None,
+                   bnd_names=("xmin", "xmax", "ymin", "ymax")) -> Mesh:
     """rectangle_mesh (mesh_generator.jl:218-434): nx*ny vertices, node
-    (iy-1)*nx+ix, Quad4 cells (or Quad9 for order=2) in x-fastest order."""
+    (iy-1)*nx+ix, Quad4 cells (or Quad9 for order=2) in x-fastest order; the
+    boundary tags of the x = xmin, x = xmax, y = ymin, y = ymax sides are
+    named by bnd_names."""
     P00, P10, P11, P01 = (xmin, ymin), (xmax, ymin), (xmax, ymax), (xmin, ymax)
     if order == 1:
         NX, NY, scale = nx, ny, 1.0
@@ -243,8 +246,8 @@ def rectangle_mesh(nx, ny, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, order=1,
     # boundary nodes (mesh_generator.jl:297-301, 389-392): first and last index per axis
     gx, gy = np.meshgrid(np.arange(NX), np.arange(NY), indexing="xy")
     gx, gy = gx.ravel(), gy.ravel()
-    bc = {"xmin": np.flatnonzero(gx == 0), "xmax": np.flatnonzero(gx == NX - 1),
-          "ymin": np.flatnonzero(gy == 0), "ymax": np.flatnonzero(gy == NY - 1)}
+    bc = {bnd_names[0]: np.flatnonzero(gx == 0), bnd_names[1]: np.flatnonzero(gx == NX - 1),
+          bnd_names[2]: np.flatnonzero(gy == 0), bnd_names[3]: np.flatnonzero(gy == NY - 1)}
     if order == 1:
         n0 = jy * nx + jx
         c2n = np.stack([n0, n0 + 1, n0 + nx + 1, n0 + nx], axis=1)

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -181,6 +182,18 @@ struct bh_dgb_s {
   double *d_vals = nullptr, *d_ke = nullptr, *d_tab = nullptr, *d_coefs = nullptr,
          *d_vcoef = nullptr, *d_g = nullptr, *d_b = nullptr;
   double* d_geo = nullptr;      // [nfi][2][nq][8]: interior face-point geometry
+};
+
+// The COVO Euler DG residual (bh_euler_*): side table, component maps, rule.
+struct bh_euler_s {
+  bh_space V = nullptr;
+  int dev = -1;
+  int nq1 = 0;
+  double gamma = 1.4;
+  int64_t ncells = 0, ntot = 0;
+  int32_t *d_sides = nullptr, *d_maps = nullptr, *d_c2n = nullptr;
+  double* d_gq = nullptr;
+  double *d_u = nullptr, *d_out = nullptr;  // staging of host-`where` calls
 };
 
 struct bh_dg_s {
@@ -1622,6 +1635,156 @@ int bh_dgb_destroy(bh_dgb d) {
       (void)hipFree(q);
   }
   delete d;
+  return BH_OK;
+}
+
+// ---- COVO Euler DG residual ---------------------------------------------------------
+int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic,
+                    const int32_t* periodic, const int64_t* maps, int64_t ndofs_total,
+                    bh_euler* out) {
+  if (!V || !out || !maps || nperiodic < 0 || (nperiodic > 0 && !periodic) || ndofs_total <= 0)
+    return BH_ERR_ARG;
+  *out = nullptr;
+  bh_mesh m = V->mesh;
+  bh_ctx c = m->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "the Euler residual needs a mesh created with a context");
+  if (m->geo != BH_QUAD4 || V->order != 1 || V->continuous)
+    return fail(c, BH_ERR_UNSUPPORTED, "the Euler residual needs a discontinuous Q1 space on Quad4 cells");
+  const int nq1 = gauss_points_per_axis(quad_degree);
+  if (nq1 > 4) return fail(c, BH_ERR_UNSUPPORTED, "the Euler residual: at most 4 Gauss points per axis");
+  if (ndofs_total > INT32_MAX) return fail(c, BH_ERR_UNSUPPORTED, "more than 2^31 dofs");
+  const HostMesh& hm = m->h;
+  const int64_t nc = hm.ncells, ns = V->ndofs;
+  // sides: interior faces (_build_faces!), then the periodic ones
+  std::vector<int32_t> sides((size_t)nc * 16, 0);
+  auto put = [&](int64_t cm, int sm, int64_t cp, int sp, int flip) -> bool {
+    int32_t* A = &sides[(cm * 4 + sm) * 4];
+    int32_t* B = &sides[(cp * 4 + sp) * 4];
+    if (A[2] != 0 || B[2] != 0 || cm == cp) return false;
+    A[0] = (int32_t)cp; A[1] = sp; A[2] = 1; A[3] = flip;
+    B[0] = (int32_t)cm; B[1] = sm; B[2] = 2; B[3] = flip;
+    return true;
+  };
+  {
+    std::unordered_map<uint64_t, std::array<int64_t, 2>> index;
+    for (int64_t cc = 0; cc < nc; ++cc) {
+      const int64_t* nd = &hm.c2n[hm.c2n_ptr[cc]];
+      for (int sd = 0; sd < 4; ++sd) {
+        const int64_t a = nd[sd], b = nd[(sd + 1) & 3];
+        const uint64_t key = ((uint64_t)std::min(a, b) << 32) | (uint64_t)std::max(a, b);
+        auto it = index.find(key);
+        if (it == index.end()) {
+          index.emplace(key, std::array<int64_t, 2>{cc, sd});
+        } else {
+          const int64_t cn = it->second[0];
+          const int sn = (int)it->second[1];
+          const int flip = a == hm.c2n[hm.c2n_ptr[cn] + sn] ? 0 : 1;
+          if (!put(cn, sn, cc, sd, flip)) return fail(c, BH_ERR_ARG, "a face shared by more than two cells");
+        }
+      }
+    }
+  }
+  for (int64_t f = 0; f < nperiodic; ++f) {
+    const int32_t* P = periodic + 5 * f;
+    if (P[0] < 0 || P[0] >= nc || P[2] < 0 || P[2] >= nc || P[1] < 0 || P[1] > 3 || P[3] < 0 ||
+        P[3] > 3 || (P[4] != 0 && P[4] != 1))
+      return fail(c, BH_ERR_RANGE, "periodic face out of range");
+    if (!put(P[0], P[1], P[2], P[3], P[4]))
+      return fail(c, BH_ERR_ARG, "a periodic face on a side that already has a face");
+  }
+  // maps[k][g] (global ids of component k's scalar dof g) -> per (cell, local dof)
+  std::vector<int32_t> m2((size_t)5 * nc * 4);
+  std::vector<uint8_t> seen((size_t)ndofs_total, 0);
+  for (int k = 0; k < 5; ++k)
+    for (int64_t q = 0; q < nc * 4; ++q) {
+      const int32_t g = V->dofs[q];
+      if (g < 0 || g >= ns) return fail(c, BH_ERR_RANGE, "dof out of range");
+      const int64_t id = maps[(int64_t)k * ns + g];
+      if (id < 0 || id >= ndofs_total) return fail(c, BH_ERR_RANGE, "global id out of range");
+      if (seen[id]++) return fail(c, BH_ERR_ARG, "the component maps are not one-to-one");
+      m2[(size_t)k * nc * 4 + q] = (int32_t)id;
+    }
+  std::vector<double> gx, gw;
+  gauss_legendre(nq1, gx, gw);
+  std::vector<double> gq(gx);
+  gq.insert(gq.end(), gw.begin(), gw.end());
+  std::vector<int32_t> c2n32(hm.c2n.begin(), hm.c2n.end());
+  auto e = std::make_unique<bh_euler_s>();
+  e->V = V;
+  e->dev = V->dev;
+  e->nq1 = nq1;
+  e->gamma = gamma;
+  e->ncells = nc;
+  e->ntot = ndofs_total;
+  if (int r = set_device(c)) return r;
+  auto up = [&](auto** dst, const auto& v) -> int {
+    BH_HIP(c, dalloc(dst, std::max<size_t>(v.size(), 1)));
+    if (!v.empty())
+      BH_HIP(c, hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
+    return BH_OK;
+  };
+  if (int r = up(&e->d_sides, sides)) return r;
+  if (int r = up(&e->d_maps, m2)) return r;
+  if (int r = up(&e->d_c2n, c2n32)) return r;
+  if (int r = up(&e->d_gq, gq)) return r;
+  *out = e.release();
+  return BH_OK;
+}
+
+static int euler_run(bh_euler e, int terms, double fcoef, int step, double dt, const double* u,
+                     double* o, int where) {
+  if (!e || !u || !o) return BH_ERR_ARG;
+  bh_ctx c = e->V->mesh->ctx;
+  if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
+  if (where == BH_HOST && c->capturing)
+    return fail(c, BH_ERR_ARG, "host-`where` calls cannot be captured in a graph");
+  if (terms < 1 || terms > 3) return fail(c, BH_ERR_ARG, "terms: 1 volume, 2 faces, 3 both");
+  if (int r = set_device(c)) return r;
+  const size_t bytes = (size_t)e->ntot * sizeof(double);
+  const double* du = u;
+  double* dout = o;
+  if (where == BH_HOST) {
+    if (!e->d_u) {
+      BH_HIP(c, dalloc(&e->d_u, (size_t)e->ntot));
+      BH_HIP(c, dalloc(&e->d_out, (size_t)e->ntot));
+    }
+    BH_HIP(c, hipMemcpyAsync(e->d_u, u, bytes, hipMemcpyHostToDevice, c->stream));
+    if (!step) BH_HIP(c, hipMemcpyAsync(e->d_out, o, bytes, hipMemcpyHostToDevice, c->stream));
+    du = e->d_u;
+    dout = e->d_out;
+  }
+  time_begin(c, 2);
+  BH_HIP(c, launch_euler_dg(c->stream, e->V->mesh->d_coords, e->d_c2n, e->ncells, e->d_sides,
+                            e->d_maps, e->d_gq, e->nq1, e->gamma, terms, fcoef, step, dt, du,
+                            dout));
+  time_end(c);
+  if (where == BH_HOST) {
+    BH_HIP(c, hipMemcpyAsync(o, dout, bytes, hipMemcpyDeviceToHost, c->stream));
+    BH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return BH_OK;
+}
+
+int bh_euler_residual(bh_euler e, int terms, double face_coef, const double* u, double* b,
+                      int where) {
+  if (e && u == b) return fail(e->V->mesh->ctx, BH_ERR_ARG, "the residual needs b != u");
+  return euler_run(e, terms, face_coef, 0, 0.0, u, b, where);
+}
+
+int bh_euler_step(bh_euler e, double dt, const double* u, double* out, int where) {
+  if (e && u == out) return fail(e->V->mesh->ctx, BH_ERR_ARG, "the step needs out != u");
+  return euler_run(e, 3, -1.0, 1, dt, u, out, where);
+}
+
+int bh_euler_destroy(bh_euler e) {
+  if (!e) return BH_ERR_ARG;
+  if (e->dev >= 0) {
+    (void)hipSetDevice(e->dev);
+    for (void* q : {(void*)e->d_sides, (void*)e->d_maps, (void*)e->d_c2n, (void*)e->d_gq,
+                    (void*)e->d_u, (void*)e->d_out})
+      (void)hipFree(q);
+  }
+  delete e;
   return BH_OK;
 }
 

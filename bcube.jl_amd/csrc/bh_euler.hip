@@ -1,0 +1,322 @@
+// bh_euler.hip -- the residual of the reference's COVO Euler DG benchmark
+// (benchmark/covo.jl:14-150): for a DG Q1 MultiFESpace (ρ, ρu, ρE, ϕ) on Quad4
+// cells,
+//   b = ∫ flux_Ω(u, v) dΩ - Σ_domains ∫ flux_roe ∘ (u⁻, u⁺, jump(v), n⁻) dΓ
+// over interior faces and periodic boundary faces, and the explicit step
+// u + Δt M⁻¹ b (covo.jl:31, forward_euler covo.jl:152).
+//
+// Eight lanes per cell: lanes 0-3 take the volume points (w|J| ∇φ_l ⋅ F_k at
+// point q), lanes 4-7 the cell's four sides (the Roe flux at the side's face
+// points, with the sign of the cell's role on that face).  A face's flux is
+// computed by both of its cells from the same inputs in the same order, so the
+// two see bitwise the same value; each cell's 20 sums (5 components x 4 basis
+// functions) are reduced in a fixed order (volume points, then sides 0-3) and
+// written by the cell that owns those dofs: no atomics, repeatable bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bh_kernels.h"
+
+namespace bh {
+namespace {
+
+constexpr int kEulerCellsPerBlock = 32;  // 8 lanes each, 256 threads
+constexpr int kEulerMaxQ1 = 4;           // Gauss points per axis
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Q1 tensor Lagrange on [-1,1]², first index fastest (l = i + 2j):
+// values and reference gradients (hypercube.jl, uniform nodes)
+__device__ __forceinline__ void q1_basis(double xi, double eta, double phi[4],
+                                         double dphi[4][2]) {
+  const double a0 = 0.5 * (1.0 - xi), a1 = 0.5 * (1.0 + xi);
+  const double b0 = 0.5 * (1.0 - eta), b1 = 0.5 * (1.0 + eta);
+  phi[0] = a0 * b0;
+  phi[1] = a1 * b0;
+  phi[2] = a0 * b1;
+  phi[3] = a1 * b1;
+  dphi[0][0] = -0.5 * b0; dphi[0][1] = -0.5 * a0;
+  dphi[1][0] = 0.5 * b0;  dphi[1][1] = -0.5 * a1;
+  dphi[2][0] = -0.5 * b1; dphi[2][1] = 0.5 * a0;
+  dphi[3][0] = 0.5 * b1;  dphi[3][1] = 0.5 * a1;
+}
+
+// Quad4 map (CGNS node order): Jacobian J[a][b] = ∂x_a/∂ξ_b
+__device__ __forceinline__ void quad4_jacobian(const double X[4][2], double xi, double eta,
+                                               double J[2][2]) {
+  const double dx[4] = {-(1.0 - eta) * 0.25, (1.0 - eta) * 0.25, (1.0 + eta) * 0.25,
+                        -(1.0 + eta) * 0.25};
+  const double de[4] = {-(1.0 - xi) * 0.25, -(1.0 + xi) * 0.25, (1.0 + xi) * 0.25,
+                        (1.0 - xi) * 0.25};
+  J[0][0] = J[0][1] = J[1][0] = J[1][1] = 0.0;
+  for (int k = 0; k < 4; ++k) {
+    J[0][0] += X[k][0] * dx[k];
+    J[0][1] += X[k][0] * de[k];
+    J[1][0] += X[k][1] * dx[k];
+    J[1][1] += X[k][1] * de[k];
+  }
+}
+
+// reference point of side s at Gauss abscissa t (mapping_face, mapping.jl:324-336),
+// traversed from corner s to s+1, or backwards when flipped
+__device__ __forceinline__ void side_point(int s, int flip, double t, double& xi, double& eta) {
+  const double V[4][2] = {{-1.0, -1.0}, {1.0, -1.0}, {1.0, 1.0}, {-1.0, 1.0}};
+  const int a = flip ? (s + 1) & 3 : s, b = flip ? s : (s + 1) & 3;
+  xi = 0.5 * (1.0 - t) * V[a][0] + 0.5 * (1.0 + t) * V[b][0];
+  eta = 0.5 * (1.0 - t) * V[a][1] + 0.5 * (1.0 + t) * V[b][1];
+}
+
+__device__ __forceinline__ double pressure(double rho, double rux, double ruy, double rE,
+                                           double gamma) {
+  // (γ-1)(ρE - tr(ρu ⊗ vel)/2), covo.jl:185-190
+  const double vx = rux / rho, vy = ruy / rho;
+  return (gamma - 1.0) * (rE - (rux * vx + ruy * vy) / 2.0);
+}
+
+// flux_roe (covo.jl:69-150), term by term
+__device__ void roe_flux(const double ui[5], const double uj[5], double nx, double ny,
+                         double gamma, double F[5]) {
+  const double rho1 = ui[0], rux1 = ui[1], ruy1 = ui[2], rE1 = ui[3], phi1 = ui[4];
+  const double rho2 = uj[0], rux2 = uj[1], ruy2 = uj[2], rE2 = uj[3], phi2 = uj[4];
+  const double u1 = rux1 / rho1, v1 = ruy1 / rho1;
+  const double u2 = rux2 / rho2, v2 = ruy2 / rho2;
+  const double p1 = pressure(rho1, rux1, ruy1, rE1, gamma);
+  const double p2 = pressure(rho2, rux2, ruy2, rE2, gamma);
+  const double H2 = (gamma / (gamma - 1.0)) * p2 / rho2 + (u2 * u2 + v2 * v2) / 2.0;
+  const double H1 = (gamma / (gamma - 1.0)) * p1 / rho1 + (u1 * u1 + v1 * v1) / 2.0;
+  const double R = sqrt(rho1 / rho2);
+  const double invR1 = 1.0 / (R + 1.0);
+  const double uAv = (R * u1 + u2) * invR1;
+  const double vAv = (R * v1 + v2) * invR1;
+  const double Hav = (R * H1 + H2) * invR1;
+  const double cAv = sqrt(fabs((gamma - 1.0) * (Hav - (uAv * uAv + vAv * vAv) / 2.0)));
+  const double ecAv = (uAv * uAv + vAv * vAv) / 2.0;
+  const double l1 = nx * uAv + ny * vAv;
+  const double l3 = l1 + cAv;
+  const double l4 = l1 - cAv;
+  const double d1 = rho1 - rho2;
+  const double d2 = rho1 * u1 - rho2 * u2;
+  const double d3 = rho1 * v1 - rho2 * v2;
+  const double d4 = rE1 - rE2;
+  const double flu11 = nx * rho2 * u2 + ny * rho2 * v2;
+  const double flu21 = nx * p2 + flu11 * u2;
+  const double flu31 = ny * p2 + flu11 * v2;
+  const double flu41 = H2 * flu11;
+  const double rc1 = (gamma - 1.0) / cAv;
+  const double rc2 = (gamma - 1.0) / cAv / cAv;
+  const double uq41 = ecAv / cAv + cAv / (gamma - 1.0);
+  const double uq42 = nx * uAv + ny * vAv;
+  const double fdc1 = fmax(l1, 0.0) * (d1 + rc2 * (-ecAv * d1 + uAv * d2 + vAv * d3 - d4));
+  const double fdc2 = fmax(l1, 0.0) * ((nx * vAv - ny * uAv) * d1 + ny * d2 - nx * d3);
+  const double fdc3 = fmax(l3, 0.0) * ((-uq42 * d1 + nx * d2 + ny * d3) / 2.0 +
+                                       rc1 * (ecAv * d1 - uAv * d2 - vAv * d3 + d4) / 2.0);
+  const double fdc4 = fmax(l4, 0.0) * ((uq42 * d1 - nx * d2 - ny * d3) / 2.0 +
+                                       rc1 * (ecAv * d1 - uAv * d2 - vAv * d3 + d4) / 2.0);
+  const double duv1 = fdc1 + (fdc3 + fdc4) / cAv;
+  const double duv2 = uAv * fdc1 + ny * fdc2 + (uAv / cAv + nx) * fdc3 + (uAv / cAv - nx) * fdc4;
+  const double duv3 = vAv * fdc1 - nx * fdc2 + (vAv / cAv + ny) * fdc3 + (vAv / cAv - ny) * fdc4;
+  const double duv4 = ecAv * fdc1 + (ny * uAv - nx * vAv) * fdc2 + (uq41 + uq42) * fdc3 +
+                      (uq41 - uq42) * fdc4;
+  const double v12 = 0.5 * ((u1 + u2) * nx + (v1 + v2) * ny);
+  F[0] = flu11 + duv1;
+  F[1] = flu21 + duv2;
+  F[2] = flu31 + duv3;
+  F[3] = flu41 + duv4;
+  F[4] = fmax(0.0, v12) * phi1 + fmin(0.0, v12) * phi2;
+}
+
+__device__ __forceinline__ void load_cell(const double* __restrict__ coords,
+                                          const int32_t* __restrict__ c2n, int64_t c,
+                                          double X[4][2]) {
+  for (int k = 0; k < 4; ++k) {
+    const int64_t n = c2n[4 * c + k];
+    X[k][0] = coords[2 * n];
+    X[k][1] = coords[2 * n + 1];
+  }
+}
+
+__device__ __forceinline__ void load_state(const double* __restrict__ u,
+                                           const int32_t* __restrict__ maps, int64_t stride,
+                                           int64_t c, double U[5][4]) {
+  for (int k = 0; k < 5; ++k)
+    for (int l = 0; l < 4; ++l) U[k][l] = u[maps[k * stride + 4 * c + l]];
+}
+
+// side table: [ncells][4][4] = (neighbour cell, neighbour side, role, flip);
+// role 0: no face term on this side, 1: this cell is side⁻, 2: side⁺
+__global__ __launch_bounds__(256) void k_euler_dg(
+    const double* __restrict__ coords, const int32_t* __restrict__ c2n, int64_t ncells,
+    const int32_t* __restrict__ sides, const int32_t* __restrict__ maps,
+    const double* __restrict__ gq, int nq1, double gamma, int terms, double fcoef, int step,
+    double dt, const double* __restrict__ u, double* __restrict__ out) {
+  __shared__ double part[kEulerCellsPerBlock][8][20];
+  __shared__ double tot[kEulerCellsPerBlock][20];
+  const int lc = threadIdx.x >> 3, g = threadIdx.x & 7;
+  const int64_t c = (int64_t)blockIdx.x * kEulerCellsPerBlock + lc;
+  const bool valid = c < ncells;
+  const double* gx = gq;
+  const double* gw = gq + nq1;
+  double acc[5][4];
+  for (int k = 0; k < 5; ++k)
+    for (int l = 0; l < 4; ++l) acc[k][l] = 0.0;
+  if (valid) {
+    double X[4][2], U[5][4];
+    if (g < 4 && (terms & 1)) {
+      // volume points q = g, g + 4, ... of the nq1 x nq1 tensor rule
+      load_cell(coords, c2n, c, X);
+      load_state(u, maps, 4 * ncells, c, U);
+      for (int q = g; q < nq1 * nq1; q += 4) {
+        const int qi = q % nq1, qj = q / nq1;
+        const double xi = gx[qi], eta = gx[qj], w = gw[qi] * gw[qj];
+        double J[2][2], phi[4], dphi[4][2];
+        quad4_jacobian(X, xi, eta, J);
+        q1_basis(xi, eta, phi, dphi);
+        const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        const double r = 1.0 / det;
+        const double Ji[2][2] = {{J[1][1] * r, -J[0][1] * r}, {-J[1][0] * r, J[0][0] * r}};
+        double w5[5];
+        for (int k = 0; k < 5; ++k)
+          w5[k] = phi[0] * U[k][0] + phi[1] * U[k][1] + phi[2] * U[k][2] + phi[3] * U[k][3];
+        // _flux_Ω (covo.jl:44-58)
+        const double rho = w5[0], rux = w5[1], ruy = w5[2], rE = w5[3], ph = w5[4];
+        const double vx = rux / rho, vy = ruy / rho;
+        const double p = pressure(rho, rux, ruy, rE, gamma);
+        const double F[5][2] = {{rux, ruy},
+                                {rux * vx + p, rux * vy},
+                                {ruy * vx, ruy * vy + p},
+                                {(rE + p) * vx, (rE + p) * vy},
+                                {ph * vx, ph * vy}};
+        const double wd = w * fabs(det);
+        for (int l = 0; l < 4; ++l) {
+          // ∇φ = J⁻ᵀ ∇̂φ
+          const double gxl = dphi[l][0] * Ji[0][0] + dphi[l][1] * Ji[1][0];
+          const double gyl = dphi[l][0] * Ji[0][1] + dphi[l][1] * Ji[1][1];
+          for (int k = 0; k < 5; ++k) acc[k][l] += wd * (gxl * F[k][0] + gyl * F[k][1]);
+        }
+      }
+    } else if (g >= 4 && (terms & 2)) {
+      const int s = g - 4;
+      const int32_t* S = sides + (4 * c + s) * 4;
+      const int role = S[2];
+      if (role != 0) {
+        const int64_t nb = S[0];
+        const int ns = S[1], flip = S[3];
+        // the face's side⁻ (cm, sm) and side⁺ (cp, sp)
+        const int64_t cm = role == 1 ? c : nb, cp = role == 1 ? nb : c;
+        const int sm = role == 1 ? s : ns, sp = role == 1 ? ns : s;
+        double Xm[4][2], Um[5][4], Up[5][4];
+        load_cell(coords, c2n, cm, Xm);
+        load_state(u, maps, 4 * ncells, cm, Um);
+        load_state(u, maps, 4 * ncells, cp, Up);
+        const double ex = Xm[(sm + 1) & 3][0] - Xm[sm][0], ey = Xm[(sm + 1) & 3][1] - Xm[sm][1];
+        const double dS = sqrt(ex * ex + ey * ey) / 2.0;
+        const double nref[4][2] = {{0.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}, {-1.0, 0.0}};
+        for (int q = 0; q < nq1; ++q) {
+          double xm, em, xp, ep;
+          side_point(sm, 0, gx[q], xm, em);
+          side_point(sp, flip, gx[q], xp, ep);
+          double J[2][2], phm[4], php[4], dd[4][2];
+          quad4_jacobian(Xm, xm, em, J);
+          const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+          const double r = 1.0 / det;
+          // J⁻ᵀ n_ref, normalised (mapping.jl:843-855)
+          const double a = nref[sm][0] * (J[1][1] * r) + nref[sm][1] * (-J[1][0] * r);
+          const double b = nref[sm][0] * (-J[0][1] * r) + nref[sm][1] * (J[0][0] * r);
+          const double inv = 1.0 / sqrt(a * a + b * b);
+          const double nx = a * inv, ny = b * inv;
+          q1_basis(xm, em, phm, dd);
+          q1_basis(xp, ep, php, dd);
+          double wm[5], wp[5], F[5];
+          for (int k = 0; k < 5; ++k) {
+            wm[k] = phm[0] * Um[k][0] + phm[1] * Um[k][1] + phm[2] * Um[k][2] + phm[3] * Um[k][3];
+            wp[k] = php[0] * Up[k][0] + php[1] * Up[k][1] + php[2] * Up[k][2] + php[3] * Up[k][3];
+          }
+          roe_flux(wm, wp, nx, ny, gamma, F);
+          // fcoef ∫ F ⋅ jump(v), jump(v) = v⁻ - v⁺ (fcoef = -1 in covo.jl's l)
+          const double wf = gw[q] * dS;
+          const double* ph = role == 1 ? phm : php;
+          const double sg = role == 1 ? fcoef : -fcoef;
+          for (int k = 0; k < 5; ++k)
+            for (int l = 0; l < 4; ++l) acc[k][l] += sg * (wf * F[k] * ph[l]);
+        }
+      }
+    }
+  }
+  for (int k = 0; k < 5; ++k)
+    for (int l = 0; l < 4; ++l) part[lc][g][k * 4 + l] = acc[k][l];
+  wave_sync();
+  // fixed-order sums: entry e = k*4 + l over the 8 parts
+  for (int e = g; e < 20; e += 8) {
+    double s = 0.0;
+    for (int p = 0; p < 8; ++p) s += part[lc][p][e];
+    tot[lc][e] = s;
+  }
+  wave_sync();
+  if (!valid) return;
+  if (!step) {
+    // b += l(v) (assemble_linear!)
+    for (int e = g; e < 20; e += 8) {
+      const int k = e >> 2, l = e & 3;
+      const int32_t id = maps[k * 4 * ncells + 4 * c + l];
+      out[id] += tot[lc][e];
+    }
+    return;
+  }
+  // u + Δt M⁻¹ b: the cell's Q1 mass matrix (same rule), one component per lane
+  if (g < 5) {
+    double X[4][2];
+    load_cell(coords, c2n, c, X);
+    double M[4][4] = {}, rhs[4];
+    for (int q = 0; q < nq1 * nq1; ++q) {
+      const int qi = q % nq1, qj = q / nq1;
+      const double xi = gx[qi], eta = gx[qj], w = gw[qi] * gw[qj];
+      double J[2][2], phi[4], dphi[4][2];
+      quad4_jacobian(X, xi, eta, J);
+      q1_basis(xi, eta, phi, dphi);
+      const double wd = w * fabs(J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) M[i][j] += wd * phi[i] * phi[j];
+    }
+    for (int l = 0; l < 4; ++l) rhs[l] = tot[lc][g * 4 + l];
+    // Gaussian elimination (M is symmetric positive definite)
+    for (int i = 0; i < 4; ++i) {
+      const double piv = 1.0 / M[i][i];
+      for (int r = i + 1; r < 4; ++r) {
+        const double f = M[r][i] * piv;
+        for (int j = i; j < 4; ++j) M[r][j] -= f * M[i][j];
+        rhs[r] -= f * rhs[i];
+      }
+    }
+    for (int i = 3; i >= 0; --i) {
+      double v = rhs[i];
+      for (int j = i + 1; j < 4; ++j) v -= M[i][j] * rhs[j];
+      rhs[i] = v / M[i][i];
+    }
+    for (int l = 0; l < 4; ++l) {
+      const int32_t id = maps[g * 4 * ncells + 4 * c + l];
+      out[id] = u[id] + dt * rhs[l];
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_euler_dg(hipStream_t s, const double* coords, const int32_t* c2n,
+                           int64_t ncells, const int32_t* sides, const int32_t* maps,
+                           const double* gq, int nq1, double gamma, int terms, double fcoef,
+                           int step, double dt, const double* u, double* out) {
+  if (ncells <= 0) return hipSuccess;
+  if (nq1 < 1 || nq1 > kEulerMaxQ1) return hipErrorInvalidValue;
+  const int64_t nb = (ncells + kEulerCellsPerBlock - 1) / kEulerCellsPerBlock;
+  if (nb > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_euler_dg, dim3((unsigned)nb), dim3(8 * kEulerCellsPerBlock), 0, s, coords,
+                     c2n, ncells, sides, maps, gq, nq1, gamma, terms, fcoef, step, dt, u, out);
+  return hipGetLastError();
+}
+
+}  // namespace bh

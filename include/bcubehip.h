@@ -340,6 +340,35 @@ int bh_dgb_boundary_points(bh_dgb d, double* xq, int where);
 int bh_dgb_boundary_linear(bh_dgb d, double c0, double c1, const double* g, double* b, int where);
 int bh_dgb_destroy(bh_dgb d);
 
+/* ---- the COVO Euler DG residual ------------------------------------------- */
+/* assemble_linear(l, V) (assembler.jl:448-520) of the reference's COVO
+ * benchmark (benchmark/covo.jl:14-150, driver_bench_covo.jl): V a
+ * MultiFESpace (ρ, ρu, ρE, ϕ) of DG Q1 spaces on Quad4 cells, and
+ *   l(v) = ∫(flux_Ω(u, v))dΩ - ∫(flux_Γ(u, v, nΓ))dΓ - Σ ∫(flux_Γ(u, v, n))dΓ_perio
+ * with flux_Ω the Euler volume flux against ∇v (_flux_Ω, covo.jl:44-58) and
+ * flux_Γ = flux_roe ∘ (side⁻(u), side⁺(u), jump(v), side⁻(n)) (covo.jl:69-150).
+ * `V` is the scalar DG Q1 space every field component copies; maps [5][ndofs
+ * of V] are the global ids (0-based) of the components ρ, ρu_x, ρu_y, ρE, ϕ
+ * (the MultiFESpace's get_mapping composed with the vector numbering).
+ * Interior faces come from the mesh (_build_faces!); `periodic` lists
+ * [cell⁻, side⁻, cell⁺, side⁺, flip] per periodic boundary face
+ * (BoundaryFaceDomain(mesh, PeriodicBCType), domain.jl:232-397, 845-890:
+ * side⁻ the slave cell; flip = 1 when side⁺ runs against side⁻). */
+typedef struct bh_euler_s* bh_euler;
+int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic,
+                    const int32_t* periodic, const int64_t* maps, int64_t ndofs_total,
+                    bh_euler* out);
+/* b += the volume (terms & 1) and face (terms & 2) integrals for the state u
+ * (assemble_linear!(b, l, V)), the face integrals with coefficient face_coef
+ * (-1 in covo.jl's residual, +1 for driver_bench_covo.jl's ∫(flux_Γ)dΓ alone);
+ * u and b hold ndofs_total values. */
+int bh_euler_residual(bh_euler e, int terms, double face_coef, const double* u, double* b,
+                      int where);
+/* out = u + dt M⁻¹ b(u), M the block-diagonal DG mass matrix: the explicit
+ * step forward_euler(u, compute_residual, t, Δt) (covo.jl:14-33, 152); out != u. */
+int bh_euler_step(bh_euler e, double dt, const double* u, double* out, int where);
+int bh_euler_destroy(bh_euler e);
+
 /* ---- graphs -------------------------------------------------------------- */
 /* Re-assembly inside time-stepping or Newton loops (e.g. benchmark/covo.jl:14-33
  * calls assemble_linear! every step): capture the device-`where` assembly

@@ -491,3 +491,199 @@ def dg_node_coords(mesh: OMesh, order):
         for p in pts:
             out.append(geometry("Quad4", coords[nodes], p)[3])
     return np.array(out)
+
+
+# ---------------------------------------------------------------------------
+# Euler DG residual of the reference's COVO benchmark (benchmark/covo.jl):
+# volume flux (_flux_Ω, covo.jl:44-58), Roe flux on interior and periodic
+# faces (flux_roe, covo.jl:69-150), pressure (covo.jl:185-190).  Fields are
+# the five scalar components (ρ, ρu_x, ρu_y, ρE, ϕ), each a DG Q1 function
+# numbered cell by cell: U[k, c, l].  Plain loops, small meshes.
+# ---------------------------------------------------------------------------
+def euler_pressure(rho, rhou, rhoE, gamma):
+    """pressure (covo.jl:185-190): (γ-1)(ρE - tr(ρu ⊗ vel)/2)."""
+    vel = rhou / rho
+    return (gamma - 1) * (rhoE - (rhou[0] * vel[0] + rhou[1] * vel[1]) / 2)
+
+
+def euler_volume_flux(w, gamma):
+    """The fluxes of _flux_Ω (covo.jl:44-58): rows for ρ, ρu_x, ρu_y, ρE, ϕ,
+    columns x, y (ρuu + pI for the momentum rows)."""
+    rho, rhou, rhoE, phi = w[0], np.array([w[1], w[2]]), w[3], w[4]
+    vel = rhou / rho
+    p = euler_pressure(rho, rhou, rhoE, gamma)
+    F = np.zeros((5, 2))
+    F[0] = rhou
+    F[1:3] = np.outer(rhou, vel) + p * np.eye(2)
+    F[3] = (rhoE + p) * vel
+    F[4] = phi * vel
+    return F
+
+
+def roe_flux(ui, uj, n, gamma):
+    """flux_roe (covo.jl:69-150), restated line by line: the flux (ρ, ρu_x,
+    ρu_y, ρE, ϕ) through a face of normal n (side⁻'s) from state ui (side⁻)
+    and uj (side⁺)."""
+    nx, ny = n
+    rho1, rux1, ruy1, rE1, phi1 = ui
+    rho2, rux2, ruy2, rE2, phi2 = uj
+    u1, v1 = rux1 / rho1, ruy1 / rho1
+    u2, v2 = rux2 / rho2, ruy2 / rho2
+    p1 = euler_pressure(rho1, np.array([rux1, ruy1]), rE1, gamma)
+    p2 = euler_pressure(rho2, np.array([rux2, ruy2]), rE2, gamma)
+    H2 = (gamma / (gamma - 1)) * p2 / rho2 + (u2 * u2 + v2 * v2) / 2.0
+    H1 = (gamma / (gamma - 1)) * p1 / rho1 + (u1 * u1 + v1 * v1) / 2.0
+    R = np.sqrt(rho1 / rho2)
+    invR1 = 1.0 / (R + 1)
+    uAv = (R * u1 + u2) * invR1
+    vAv = (R * v1 + v2) * invR1
+    Hav = (R * H1 + H2) * invR1
+    cAv = np.sqrt(abs((gamma - 1) * (Hav - (uAv * uAv + vAv * vAv) / 2.0)))
+    ecAv = (uAv * uAv + vAv * vAv) / 2.0
+    l1 = nx * uAv + ny * vAv
+    l3 = l1 + cAv
+    l4 = l1 - cAv
+    d1 = rho1 - rho2
+    d2 = rho1 * u1 - rho2 * u2
+    d3 = rho1 * v1 - rho2 * v2
+    d4 = rE1 - rE2
+    flu11 = nx * rho2 * u2 + ny * rho2 * v2
+    flu21 = nx * p2 + flu11 * u2
+    flu31 = ny * p2 + flu11 * v2
+    flu41 = H2 * flu11
+    rc1 = (gamma - 1) / cAv
+    rc2 = (gamma - 1) / cAv / cAv
+    uq41 = ecAv / cAv + cAv / (gamma - 1)
+    uq42 = nx * uAv + ny * vAv
+    fdc1 = max(l1, 0.0) * (d1 + rc2 * (-ecAv * d1 + uAv * d2 + vAv * d3 - d4))
+    fdc2 = max(l1, 0.0) * ((nx * vAv - ny * uAv) * d1 + ny * d2 - nx * d3)
+    fdc3 = max(l3, 0.0) * ((-uq42 * d1 + nx * d2 + ny * d3) / 2.0
+                           + rc1 * (ecAv * d1 - uAv * d2 - vAv * d3 + d4) / 2.0)
+    fdc4 = max(l4, 0.0) * ((uq42 * d1 - nx * d2 - ny * d3) / 2.0
+                           + rc1 * (ecAv * d1 - uAv * d2 - vAv * d3 + d4) / 2.0)
+    duv1 = fdc1 + (fdc3 + fdc4) / cAv
+    duv2 = uAv * fdc1 + ny * fdc2 + (uAv / cAv + nx) * fdc3 + (uAv / cAv - nx) * fdc4
+    duv3 = vAv * fdc1 - nx * fdc2 + (vAv / cAv + ny) * fdc3 + (vAv / cAv - ny) * fdc4
+    duv4 = (ecAv * fdc1 + (ny * uAv - nx * vAv) * fdc2 + (uq41 + uq42) * fdc3
+            + (uq41 - uq42) * fdc4)
+    v12 = 0.5 * ((u1 + u2) * nx + (v1 + v2) * ny)
+    fphi = max(0.0, v12) * phi1 + min(0.0, v12) * phi2
+    return np.array([flu11 + duv1, flu21 + duv2, flu31 + duv3, flu41 + duv4, fphi])
+
+
+def tag_boundary_faces(mesh: OMesh, nodes_by_tag):
+    """Boundary faces per tag, (cell, side) in face order: a face with one
+    cell whose two nodes both lie in the tag's node set, first matching tag
+    (_build_boundary_faces!, mesh.jl:383-423)."""
+    out = {name: [] for name in nodes_by_tag}
+    sets = {name: set(int(v) for v in nodes) for name, nodes in nodes_by_tag.items()}
+    for cn, sn, cp, _ in quad_faces(mesh):
+        if cp >= 0:
+            continue
+        nodes = mesh.c2n[mesh.c2n_ptr[cn]:mesh.c2n_ptr[cn + 1]]
+        a, b = (int(nodes[k]) for k in _SQ_F2N[sn])
+        for name, s in sets.items():
+            if a in s and b in s:
+                out[name].append((cn, sn))
+                break
+    return out
+
+
+def periodic_faces(mesh: OMesh, nodes_by_tag, shift, master, slave, tol=1e-9):
+    """_compute_periodicity (domain.jl:269-397) and the periodic FaceInfo
+    (domain.jl:845-890): for each slave face (in face order) the master face
+    whose centre translated by `shift` is the slave face's centre; side⁻ is
+    the slave cell, side⁺ the master cell.  Returns [cell⁻, side⁻, cell⁺,
+    side⁺, flip] with flip = 1 when the master side's first node maps onto the
+    slave side's second node (side⁺ traversed against side⁻)."""
+    X = mesh.coords.reshape(-1, 2)
+    tags = tag_boundary_faces(mesh, nodes_by_tag)
+    f1 = [f for lab in master for f in tags[lab]]
+    f2 = [f for lab in slave for f in tags[lab]]
+
+    def face_nodes(c, s):
+        nodes = mesh.c2n[mesh.c2n_ptr[c]:mesh.c2n_ptr[c + 1]]
+        return [int(nodes[k]) for k in _SQ_F2N[s]]
+
+    out = []
+    for ci, si in f2:
+        ni = face_nodes(ci, si)
+        Mi = X[ni].mean(axis=0)
+        cnodes = mesh.c2n[mesh.c2n_ptr[ci]:mesh.c2n_ptr[ci + 1]]
+        dx = np.linalg.norm(X[cnodes].mean(axis=0) - Mi) * np.sqrt(2.0)
+        for cj, sj in f1:
+            nj = face_nodes(cj, sj)
+            Mj = X[nj].mean(axis=0) + shift
+            if np.linalg.norm(Mi - Mj) <= tol * dx:
+                flip = 0 if np.linalg.norm(X[nj[0]] + shift - X[ni[0]]) <= tol * dx else 1
+                out.append([ci, si, cj, sj, flip])
+                break
+        else:
+            raise ValueError("no periodic counterpart")
+    return out
+
+
+def euler_dg_residual(mesh: OMesh, U, gamma, degree, periodic=(), terms=("vol", "face")):
+    """b = ∫ flux_Ω(u, v) dΩ - Σ_domains ∫ flux_Γ(u, v, n) dΓ of covo.jl:14-33
+    for DG Q1 on Quad4 cells: U and the result are (5, ncells, 4).  Interior
+    faces come from _build_faces!, `periodic` lists [cell⁻, side⁻, cell⁺,
+    side⁺, flip] faces (periodic_faces)."""
+    nc = len(mesh.celltypes)
+    X = mesh.coords.reshape(-1, 2)
+    b = np.zeros((5, nc, 4))
+    if "vol" in terms:
+        wv, xv = quadrature("Square", degree)
+        xv = xv.reshape(-1, 2)[:len(wv)]
+        for c in range(nc):
+            nodes = mesh.c2n[mesh.c2n_ptr[c]:mesh.c2n_ptr[c + 1]]
+            for w, xi in zip(wv, xv):
+                J, Ji, det, _ = geometry("Quad4", X[nodes], xi)
+                val, gref = shape_functions("Square", 1, xi)
+                grad = gref.reshape(4, 2) @ Ji
+                wq = U[:, c, :] @ val
+                F = euler_volume_flux(wq, gamma)
+                b[:, c, :] += w * abs(det) * (F @ grad.T)
+    if "face" in terms:
+        wq_, xq_ = quadrature("Line", degree)
+        xq_ = xq_.reshape(-1)[:len(wq_)]
+        faces = [[cn, sn, cp, sp, None] for cn, sn, cp, sp in quad_faces(mesh) if cp >= 0]
+        faces += [list(f) for f in periodic]
+        for cn, sn, cp, sp, flip in faces:
+            nodes_n = mesh.c2n[mesh.c2n_ptr[cn]:mesh.c2n_ptr[cn + 1]]
+            fa, fb = _SQ_F2N[sn]
+            dS = np.linalg.norm(X[nodes_n[fb]] - X[nodes_n[fa]]) / 2
+            if flip is None:   # interior face: indexin(f2n_p, f2n_n)
+                nodes_p = mesh.c2n[mesh.c2n_ptr[cp]:mesh.c2n_ptr[cp + 1]]
+                gn = [int(nodes_n[k]) for k in _SQ_F2N[sn]]
+                gp = [int(nodes_p[k]) for k in _SQ_F2N[sp]]
+                perm = [gn.index(v) for v in gp]
+                loc_p = tuple(_SQ_F2N[sp][k] for k in perm)
+            else:
+                a, bb = _SQ_F2N[sp]
+                loc_p = (bb, a) if flip else (a, bb)
+            for w, xi in zip(wq_, xq_):
+                vn, _, nn, _ = _side_eval(mesh, cn, 1, _face_to_cell(_SQ_F2N[sn], xi), sn)
+                vp = shape_functions("Square", 1, _face_to_cell(loc_p, xi))[0]
+                F = roe_flux(U[:, cn, :] @ vn, U[:, cp, :] @ vp, nn, gamma)
+                b[:, cn, :] -= w * dS * np.outer(F, vn)
+                b[:, cp, :] += w * dS * np.outer(F, vp)
+    return b
+
+
+def euler_dg_mass_inverse_apply(mesh: OMesh, R, degree):
+    """M⁻¹ R with M the DG Q1 mass matrix (block-diagonal, one 4x4 block per
+    cell and component): the `cache.mass \\ rhs` of covo.jl:31."""
+    nc = len(mesh.celltypes)
+    X = mesh.coords.reshape(-1, 2)
+    wv, xv = quadrature("Square", degree)
+    xv = xv.reshape(-1, 2)[:len(wv)]
+    out = np.zeros_like(R)
+    for c in range(nc):
+        nodes = mesh.c2n[mesh.c2n_ptr[c]:mesh.c2n_ptr[c + 1]]
+        M = np.zeros((4, 4))
+        for w, xi in zip(wv, xv):
+            _, _, det, _ = geometry("Quad4", X[nodes], xi)
+            val, _ = shape_functions("Square", 1, xi)
+            M += w * abs(det) * np.outer(val, val)
+        out[:, c, :] = np.linalg.solve(M, R[:, c, :].T).T
+    return out

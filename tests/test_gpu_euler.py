@@ -1,0 +1,188 @@
+"""GPU parity of the COVO Euler DG residual (benchmark/covo.jl,
+driver_bench_covo.jl): the volume flux, the Roe flux on interior and periodic
+faces and the explicit step, assembled by bh_euler_* through assemble_linear
+on the reference's MultiFESpace(ρ, ρu, ρE, ϕ) of DG Q1 spaces, against the
+oracle's restatement (FP64 within 1e-12 of max|b|), plus free stream and
+conservation at larger sizes and bitwise repeatability."""
+import numpy as np
+import pytest
+
+import bcubehip as B
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GAMMA = 1.4
+RTOL = 1e-12
+L_HALF = 0.05
+TAGS = ("West", "East", "South", "North")
+
+
+def _meshes(n, jitter=0.0, seed=0, rotate=False):
+    """The same n x n Quad4 mesh of [-l, l]² as a product Mesh (with boundary
+    tags) and an oracle mesh; optional interior jitter and rotated node lists."""
+    t = np.linspace(-L_HALF, L_HALF, n + 1)
+    X = np.array([[x, y] for y in t for x in t])
+    if jitter:
+        rng = np.random.default_rng(seed)
+        inner = np.all((np.abs(X + L_HALF) > 1e-12) & (np.abs(X - L_HALF) > 1e-12), axis=1)
+        X[inner] += rng.uniform(-jitter, jitter, (inner.sum(), 2)) * 2 * L_HALF / n
+    cells = []
+    for j in range(n):
+        for i in range(n):
+            a = j * (n + 1) + i
+            c = [a, a + 1, a + n + 2, a + n + 1]
+            if rotate and (i + j) % 2:
+                k = (i + 2 * j) % 3 + 1
+                c = c[k:] + c[:k]
+            cells.append(c)
+    cells = np.array(cells, dtype=np.int64)
+    m = B.Mesh(X, ["Quad4"] * (n * n), cells)
+    gx, gy = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="xy")
+    gx, gy = gx.ravel(), gy.ravel()
+    m.bc_nodes = {"West": np.flatnonzero(gx == 0), "East": np.flatnonzero(gx == n),
+                  "South": np.flatnonzero(gy == 0), "North": np.flatnonzero(gy == n)}
+    om = O.OMesh(2, X.reshape(-1), np.zeros(n * n, dtype=np.int32),
+                 np.arange(n * n + 1, dtype=np.int64) * 4, cells.reshape(-1))
+    return m, om
+
+
+def _setup(m):
+    fs = B.FunctionSpace("Lagrange", 1)
+    U_sca = B.TrialFESpace(fs, m, "discontinuous", size=1)
+    U_vec = B.TrialFESpace(fs, m, "discontinuous", size=2)
+    V_sca, V_vec = B.TestFESpace(U_sca), B.TestFESpace(U_vec)
+    U = B.MultiFESpace(U_sca, U_vec, U_sca, U_sca)
+    V = B.MultiFESpace(V_sca, V_vec, V_sca, V_sca)
+    return U, V
+
+
+def _forms(m, U, values, fcoef=-1.0):
+    u = B.FEFunction(U, values)
+    dΩ = B.Measure(B.CellDomain(m), 3)
+    dΓ = B.Measure(B.InteriorFaceDomain(m), 3)
+    px = B.PeriodicBCType(B.Translation([-2 * L_HALF, 0.0]), ("East",), ("West",))
+    py = B.PeriodicBCType(B.Translation([0.0, 2 * L_HALF]), ("South",), ("North",))
+    dΓx = B.Measure(B.BoundaryFaceDomain(m, px), 3)
+    dΓy = B.Measure(B.BoundaryFaceDomain(m, py), 3)
+    nΓ, nx, ny = (B.get_face_normals(d) for d in (dΓ, dΓx, dΓy))
+
+    def flux_Ω(u, v):
+        return B.compose(B.EulerFlux(GAMMA), u, tuple(map(B.grad, v)))
+
+    def flux_Γ(u, v, n):
+        return B.compose(B.RoeFlux(GAMMA), B.side_n(u), B.side_p(u), B.jump(v), B.side_n(n))
+
+    forms = {
+        "l": lambda v: (B.integrate(flux_Ω(u, v), dΩ) - B.integrate(flux_Γ(u, v, nΓ), dΓ)
+                        - B.integrate(flux_Γ(u, v, nx), dΓx) - B.integrate(flux_Γ(u, v, ny), dΓy)),
+        "l_vol": lambda v: B.integrate(flux_Ω(u, v), dΩ),
+        "l_Γ": lambda v: B.integrate(flux_Γ(u, v, nΓ), dΓ),
+    }
+    return forms, (dΓx, dΓy)
+
+
+def _state(nc, seed, smooth=None):
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.8, 1.2, (nc, 4))
+    vel = rng.uniform(-0.4, 0.4, (2, nc, 4))
+    p = rng.uniform(0.7, 1.3, (nc, 4))
+    rE = p / (GAMMA - 1) + 0.5 * rho * (vel ** 2).sum(0)
+    phi = rng.uniform(0.0, 1.0, (nc, 4))
+    return np.stack([rho, rho * vel[0], rho * vel[1], rE, phi])
+
+
+def _to_global(U, Uo):
+    f = B.assembler.Fields(U)
+    u = np.zeros(f.ndofs)
+    u[f.maps] = Uo.reshape(5, -1)
+    return u, f.maps
+
+
+@pytest.mark.parametrize("n,jitter,rotate", [(4, 0.0, False), (6, 0.2, True)])
+def test_euler_residual_vs_oracle(n, jitter, rotate):
+    m, om = _meshes(n, jitter, seed=n, rotate=rotate)
+    U, V = _setup(m)
+    Uo = _state(n * n, seed=n)
+    u, maps = _to_global(U, Uo)
+    forms, (dΓx, dΓy) = _forms(m, U, u)
+    per = np.concatenate([dΓx.domain.periodic_faces(), dΓy.domain.periodic_faces()])
+    ref_per = (O.periodic_faces(om, m.bc_nodes, np.array([-2 * L_HALF, 0.0]), ("East",), ("West",))
+               + O.periodic_faces(om, m.bc_nodes, np.array([0.0, 2 * L_HALF]), ("South",),
+                                  ("North",)))
+    assert np.array_equal(per, np.array(ref_per, dtype=np.int32))
+    print(f"Euler n={n} jitter={jitter} rotate={rotate}", flush=True)
+    for name, terms, fcoef in (("l", ("vol", "face"), -1.0), ("l_vol", ("vol",), 0.0),
+                               ("l_Γ", ("face",), 1.0)):
+        b = B.assemble_linear(forms[name], V)
+        if name == "l_Γ":
+            ref = -O.euler_dg_residual(om, Uo, GAMMA, 3, (), terms=("face",))
+        else:
+            ref = O.euler_dg_residual(om, Uo, GAMMA, 3, ref_per if name == "l" else (),
+                                      terms=terms)
+        got = b[maps].reshape(5, n * n, 4)
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        print(f"  {name}: rel err {err:.2e}")
+        assert err <= RTOL
+
+
+def test_euler_explicit_step_vs_oracle():
+    n = 5
+    m, om = _meshes(n, 0.15, seed=9, rotate=True)
+    U, V = _setup(m)
+    Uo = _state(n * n, seed=1)
+    u, maps = _to_global(U, Uo)
+    _, (dΓx, dΓy) = _forms(m, U, u)
+    plan = B.EulerDG(V, GAMMA, 3, (dΓx.domain, dΓy.domain))
+    dt = 1e-4
+    out = plan.step(u, dt)
+    per = [list(r) for r in np.concatenate([dΓx.domain.periodic_faces(),
+                                            dΓy.domain.periodic_faces()])]
+    R = O.euler_dg_residual(om, Uo, GAMMA, 3, per)
+    ref = Uo + dt * O.euler_dg_mass_inverse_apply(om, R, 3)
+    got = out[maps].reshape(5, n * n, 4)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    inc = np.abs((got - Uo) - (ref - Uo)).max() / np.abs(ref - Uo).max()
+    print(f"step: rel err {err:.2e}, increment rel err {inc:.2e}")
+    assert inc <= 1e-11
+
+
+def test_euler_free_stream_and_conservation_at_64():
+    n = 64
+    m, _ = _meshes(n, 0.2, seed=3, rotate=True)
+    U, V = _setup(m)
+    w = np.array([1.1, 0.3, -0.2, 2.9, 0.4])
+    u, maps = _to_global(U, np.broadcast_to(w[:, None, None], (5, n * n, 4)))
+    forms, _ = _forms(m, U, u)
+    b = B.assemble_linear(forms["l"], V)
+    vol = B.assemble_linear(forms["l_vol"], V)
+    print(f"free stream 64²: max|b| {np.abs(b).max():.2e} vs volume {np.abs(vol).max():.2e}")
+    assert np.abs(b).max() <= 1e-12 * np.abs(vol).max()
+    u2, _ = _to_global(U, _state(n * n, seed=5))
+    forms2, _ = _forms(m, U, u2)
+    b2 = B.assemble_linear(forms2["l"], V)
+    sums = b2[maps].sum(axis=1)
+    assert np.abs(sums).max() <= 1e-11 * np.abs(b2).max()
+
+
+def test_euler_device_path_bitwise_repeatable():
+    import torch
+    n = 16
+    m, _ = _meshes(n, 0.1, seed=4)
+    U, V = _setup(m)
+    u, _ = _to_global(U, _state(n * n, seed=2))
+    _, (dΓx, dΓy) = _forms(m, U, u)
+    plan = B.EulerDG(V, GAMMA, 3, (dΓx.domain, dΓy.domain))
+    host = plan.residual(u)
+    du = torch.from_numpy(u).cuda()
+    db = torch.zeros_like(du)
+    for _ in range(2):
+        db.zero_()
+        torch.cuda.synchronize()
+        plan.residual_device(du.data_ptr(), db.data_ptr())
+        B.lib().bh_sync(m.backend.ctx)
+        assert np.array_equal(db.cpu().numpy(), host)
+    step_host = plan.step(u, 1e-5)
+    do = torch.empty_like(du)
+    plan.step_device(du.data_ptr(), do.data_ptr(), 1e-5)
+    B.lib().bh_sync(m.backend.ctx)
+    assert np.array_equal(do.cpu().numpy(), step_host)

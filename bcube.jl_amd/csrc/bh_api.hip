@@ -191,8 +191,9 @@ struct bh_euler_s {
   int nq1 = 0;
   double gamma = 1.4;
   int64_t ncells = 0, ntot = 0;
-  int32_t *d_sides = nullptr, *d_maps = nullptr, *d_c2n = nullptr;
-  double* d_gq = nullptr;
+  int64_t nfaces = 0;
+  int32_t *d_sides = nullptr, *d_faces = nullptr, *d_maps = nullptr, *d_c2n = nullptr;
+  double *d_gq = nullptr, *d_G = nullptr;
   double *d_u = nullptr, *d_out = nullptr;  // staging of host-`where` calls
 };
 
@@ -1655,14 +1656,17 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
   if (ndofs_total > INT32_MAX) return fail(c, BH_ERR_UNSUPPORTED, "more than 2^31 dofs");
   const HostMesh& hm = m->h;
   const int64_t nc = hm.ncells, ns = V->ndofs;
-  // sides: interior faces (_build_faces!), then the periodic ones
-  std::vector<int32_t> sides((size_t)nc * 16, 0);
+  // faces: interior faces (_build_faces!), then the periodic ones; each
+  // cell side records its face and role
+  std::vector<int32_t> sides((size_t)nc * 16, 0), faces;
   auto put = [&](int64_t cm, int sm, int64_t cp, int sp, int flip) -> bool {
     int32_t* A = &sides[(cm * 4 + sm) * 4];
     int32_t* B = &sides[(cp * 4 + sp) * 4];
     if (A[2] != 0 || B[2] != 0 || cm == cp) return false;
-    A[0] = (int32_t)cp; A[1] = sp; A[2] = 1; A[3] = flip;
-    B[0] = (int32_t)cm; B[1] = sm; B[2] = 2; B[3] = flip;
+    const int32_t f = (int32_t)(faces.size() / 5);
+    faces.insert(faces.end(), {(int32_t)cm, sm, (int32_t)cp, sp, flip});
+    A[0] = f; A[1] = 0; A[2] = 1; A[3] = flip;
+    B[0] = f; B[1] = 0; B[2] = 2; B[3] = flip;
     return true;
   };
   {
@@ -1716,6 +1720,7 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
   e->gamma = gamma;
   e->ncells = nc;
   e->ntot = ndofs_total;
+  e->nfaces = (int64_t)faces.size() / 5;
   if (int r = set_device(c)) return r;
   auto up = [&](auto** dst, const auto& v) -> int {
     BH_HIP(c, dalloc(dst, std::max<size_t>(v.size(), 1)));
@@ -1727,6 +1732,8 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
   if (int r = up(&e->d_maps, m2)) return r;
   if (int r = up(&e->d_c2n, c2n32)) return r;
   if (int r = up(&e->d_gq, gq)) return r;
+  if (int r = up(&e->d_faces, faces)) return r;
+  BH_HIP(c, dalloc(&e->d_G, (size_t)std::max<int64_t>(e->nfaces, 1) * nq1 * 5));
   *out = e.release();
   return BH_OK;
 }
@@ -1755,8 +1762,8 @@ static int euler_run(bh_euler e, int terms, double fcoef, int step, double dt, c
   }
   time_begin(c, 2);
   BH_HIP(c, launch_euler_dg(c->stream, e->V->mesh->d_coords, e->d_c2n, e->ncells, e->d_sides,
-                            e->d_maps, e->d_gq, e->nq1, e->gamma, terms, fcoef, step, dt, du,
-                            dout));
+                            e->d_faces, e->nfaces, e->d_maps, e->d_gq, e->nq1, e->gamma, terms,
+                            fcoef, step, dt, du, e->d_G, dout));
   time_end(c);
   if (where == BH_HOST) {
     BH_HIP(c, hipMemcpyAsync(o, dout, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -1781,7 +1788,7 @@ int bh_euler_destroy(bh_euler e) {
   if (e->dev >= 0) {
     (void)hipSetDevice(e->dev);
     for (void* q : {(void*)e->d_sides, (void*)e->d_maps, (void*)e->d_c2n, (void*)e->d_gq,
-                    (void*)e->d_u, (void*)e->d_out})
+                    (void*)e->d_u, (void*)e->d_out, (void*)e->d_faces, (void*)e->d_G})
       (void)hipFree(q);
   }
   delete e;

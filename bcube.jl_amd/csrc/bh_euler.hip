@@ -5,13 +5,15 @@
 // over interior faces and periodic boundary faces, and the explicit step
 // u + Δt M⁻¹ b (covo.jl:31, forward_euler covo.jl:152).
 //
-// Eight lanes per cell: lanes 0-3 take the volume points (w|J| ∇φ_l ⋅ F_k at
-// point q), lanes 4-7 the cell's four sides (the Roe flux at the side's face
-// points, with the sign of the cell's role on that face).  A face's flux is
-// computed by both of its cells from the same inputs in the same order, so the
-// two see bitwise the same value; each cell's 20 sums (5 components x 4 basis
-// functions) are reduced in a fixed order (volume points, then sides 0-3) and
-// written by the cell that owns those dofs: no atomics, repeatable bit for bit.
+// k_euler_faces computes the Roe flux at every face point once (one thread per
+// point).  k_euler_dg then takes eight lanes per cell: lane g adds the face
+// points q ≡ g (mod 2) of side g / 2 (that flux against the cell's own basis,
+// with the sign of its role on the face) and the volume points q = g, g + 8,
+// ... (w|J| ∇φ_l ⋅ F_k).  Both cells of a face read the same stored flux; each
+// cell's 20 sums (5
+// components x 4 basis functions) are reduced over the lanes in a fixed order
+// and written by the cell that owns those dofs: no atomics, repeatable bit for
+// bit.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -23,12 +25,6 @@ namespace {
 
 constexpr int kEulerCellsPerBlock = 32;  // 8 lanes each, 256 threads
 constexpr int kEulerMaxQ1 = 4;           // Gauss points per axis
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Q1 tensor Lagrange on [-1,1]², first index fastest (l = i + 2j):
 // values and reference gradients (hypercube.jl, uniform nodes)
@@ -140,131 +136,157 @@ __device__ __forceinline__ void load_cell(const double* __restrict__ coords,
   }
 }
 
-__device__ __forceinline__ void load_state(const double* __restrict__ u,
-                                           const int32_t* __restrict__ maps, int64_t stride,
-                                           int64_t c, double U[5][4]) {
-  for (int k = 0; k < 5; ++k)
-    for (int l = 0; l < 4; ++l) U[k][l] = u[maps[k * stride + 4 * c + l]];
+// the five components at one point: w[k] = Σ_l φ_l u[maps[k][4c + l]]
+__device__ __forceinline__ void point_state(const double* __restrict__ u,
+                                            const int32_t* __restrict__ maps, int64_t stride,
+                                            int64_t c, const double phi[4], double w[5]) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const int32_t* mk = maps + k * stride + 4 * c;
+    w[k] = phi[0] * u[mk[0]] + phi[1] * u[mk[1]] + phi[2] * u[mk[2]] + phi[3] * u[mk[3]];
+  }
 }
 
-// side table: [ncells][4][4] = (neighbour cell, neighbour side, role, flip);
-// role 0: no face term on this side, 1: this cell is side⁻, 2: side⁺
+// The Roe flux of every face point once: faces [nfaces][5] = (cell⁻, side⁻,
+// cell⁺, side⁺, flip), G[(f*nq1 + q)*5 + k] = w_q |face|/2 F_k.
+__global__ __launch_bounds__(256) void k_euler_faces(
+    const double* __restrict__ coords, const int32_t* __restrict__ c2n, int64_t ncells,
+    const int32_t* __restrict__ faces, int64_t nfaces, const int32_t* __restrict__ maps,
+    const double* __restrict__ gq, int nq1, double gamma, const double* __restrict__ u,
+    double* __restrict__ G) {
+  const int64_t stride = 4 * ncells;
+  const double* gx = gq;
+  const double* gw = gq + nq1;
+  const double nref[4][2] = {{0.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}, {-1.0, 0.0}};
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nfaces * nq1;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t f = t / nq1;
+    const int q = (int)(t - f * nq1);
+    const int32_t* Fc = faces + 5 * f;
+    const int64_t cm = Fc[0], cp = Fc[2];
+    const int sm = Fc[1], sp = Fc[3], flip = Fc[4];
+    double Xm[4][2];
+    load_cell(coords, c2n, cm, Xm);
+    const double ex = Xm[(sm + 1) & 3][0] - Xm[sm][0], ey = Xm[(sm + 1) & 3][1] - Xm[sm][1];
+    const double dS = sqrt(ex * ex + ey * ey) / 2.0;
+    double xm, em, xp, ep;
+    side_point(sm, 0, gx[q], xm, em);
+    side_point(sp, flip, gx[q], xp, ep);
+    double J[2][2], phm[4], php[4], dd[4][2];
+    quad4_jacobian(Xm, xm, em, J);
+    const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double r = 1.0 / det;
+    // J⁻ᵀ n_ref, normalised (mapping.jl:843-855)
+    const double a = nref[sm][0] * (J[1][1] * r) + nref[sm][1] * (-J[1][0] * r);
+    const double b = nref[sm][0] * (-J[0][1] * r) + nref[sm][1] * (J[0][0] * r);
+    const double inv = 1.0 / sqrt(a * a + b * b);
+    q1_basis(xm, em, phm, dd);
+    q1_basis(xp, ep, php, dd);
+    double wm[5], wp[5], F[5];
+    point_state(u, maps, stride, cm, phm, wm);
+    point_state(u, maps, stride, cp, php, wp);
+    roe_flux(wm, wp, a * inv, b * inv, gamma, F);
+    const double wf = gw[q] * dS;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) G[t * 5 + k] = wf * F[k];
+  }
+}
+
+// side table: [ncells][4][4] = (face, unused, role, flip); role 0: no face
+// term on this side, 1: this cell is the face's side⁻, 2: its side⁺
 __global__ __launch_bounds__(256) void k_euler_dg(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n, int64_t ncells,
     const int32_t* __restrict__ sides, const int32_t* __restrict__ maps,
     const double* __restrict__ gq, int nq1, double gamma, int terms, double fcoef, int step,
-    double dt, const double* __restrict__ u, double* __restrict__ out) {
-  __shared__ double part[kEulerCellsPerBlock][8][20];
-  __shared__ double tot[kEulerCellsPerBlock][20];
+    double dt, const double* __restrict__ u, const double* __restrict__ G,
+    double* __restrict__ out) {
   const int lc = threadIdx.x >> 3, g = threadIdx.x & 7;
   const int64_t c = (int64_t)blockIdx.x * kEulerCellsPerBlock + lc;
   const bool valid = c < ncells;
+  const int64_t stride = 4 * ncells;
   const double* gx = gq;
   const double* gw = gq + nq1;
   double acc[5][4];
+#pragma unroll
   for (int k = 0; k < 5; ++k)
+#pragma unroll
     for (int l = 0; l < 4; ++l) acc[k][l] = 0.0;
-  if (valid) {
-    double X[4][2], U[5][4];
-    if (g < 4 && (terms & 1)) {
-      // volume points q = g, g + 4, ... of the nq1 x nq1 tensor rule
+  if (valid && (terms & 1)) {
+    // volume points q = g, g + 8, ... of the nq1 x nq1 tensor rule
+    for (int q = g; q < nq1 * nq1; q += 8) {
+      double X[4][2];
       load_cell(coords, c2n, c, X);
-      load_state(u, maps, 4 * ncells, c, U);
-      for (int q = g; q < nq1 * nq1; q += 4) {
-        const int qi = q % nq1, qj = q / nq1;
-        const double xi = gx[qi], eta = gx[qj], w = gw[qi] * gw[qj];
-        double J[2][2], phi[4], dphi[4][2];
-        quad4_jacobian(X, xi, eta, J);
-        q1_basis(xi, eta, phi, dphi);
-        const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-        const double r = 1.0 / det;
-        const double Ji[2][2] = {{J[1][1] * r, -J[0][1] * r}, {-J[1][0] * r, J[0][0] * r}};
-        double w5[5];
-        for (int k = 0; k < 5; ++k)
-          w5[k] = phi[0] * U[k][0] + phi[1] * U[k][1] + phi[2] * U[k][2] + phi[3] * U[k][3];
-        // _flux_Ω (covo.jl:44-58)
-        const double rho = w5[0], rux = w5[1], ruy = w5[2], rE = w5[3], ph = w5[4];
-        const double vx = rux / rho, vy = ruy / rho;
-        const double p = pressure(rho, rux, ruy, rE, gamma);
-        const double F[5][2] = {{rux, ruy},
-                                {rux * vx + p, rux * vy},
-                                {ruy * vx, ruy * vy + p},
-                                {(rE + p) * vx, (rE + p) * vy},
-                                {ph * vx, ph * vy}};
-        const double wd = w * fabs(det);
-        for (int l = 0; l < 4; ++l) {
-          // ∇φ = J⁻ᵀ ∇̂φ
-          const double gxl = dphi[l][0] * Ji[0][0] + dphi[l][1] * Ji[1][0];
-          const double gyl = dphi[l][0] * Ji[0][1] + dphi[l][1] * Ji[1][1];
-          for (int k = 0; k < 5; ++k) acc[k][l] += wd * (gxl * F[k][0] + gyl * F[k][1]);
-        }
+      const int qi = q % nq1, qj = q / nq1;
+      const double xi = gx[qi], eta = gx[qj], w = gw[qi] * gw[qj];
+      double J[2][2], phi[4], dphi[4][2];
+      quad4_jacobian(X, xi, eta, J);
+      q1_basis(xi, eta, phi, dphi);
+      const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      const double r = 1.0 / det;
+      const double Ji[2][2] = {{J[1][1] * r, -J[0][1] * r}, {-J[1][0] * r, J[0][0] * r}};
+      double w5[5];
+      point_state(u, maps, stride, c, phi, w5);
+      // _flux_Ω (covo.jl:44-58)
+      const double rho = w5[0], rux = w5[1], ruy = w5[2], rE = w5[3], ph = w5[4];
+      const double vx = rux / rho, vy = ruy / rho;
+      const double p = pressure(rho, rux, ruy, rE, gamma);
+      const double F[5][2] = {{rux, ruy},
+                              {rux * vx + p, rux * vy},
+                              {ruy * vx, ruy * vy + p},
+                              {(rE + p) * vx, (rE + p) * vy},
+                              {ph * vx, ph * vy}};
+      const double wd = w * fabs(det);
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        // ∇φ = J⁻ᵀ ∇̂φ
+        const double gxl = dphi[l][0] * Ji[0][0] + dphi[l][1] * Ji[1][0];
+        const double gyl = dphi[l][0] * Ji[0][1] + dphi[l][1] * Ji[1][1];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc[k][l] += wd * (gxl * F[k][0] + gyl * F[k][1]);
       }
-    } else if (g >= 4 && (terms & 2)) {
-      const int s = g - 4;
-      const int32_t* S = sides + (4 * c + s) * 4;
-      const int role = S[2];
-      if (role != 0) {
-        const int64_t nb = S[0];
-        const int ns = S[1], flip = S[3];
-        // the face's side⁻ (cm, sm) and side⁺ (cp, sp)
-        const int64_t cm = role == 1 ? c : nb, cp = role == 1 ? nb : c;
-        const int sm = role == 1 ? s : ns, sp = role == 1 ? ns : s;
-        double Xm[4][2], Um[5][4], Up[5][4];
-        load_cell(coords, c2n, cm, Xm);
-        load_state(u, maps, 4 * ncells, cm, Um);
-        load_state(u, maps, 4 * ncells, cp, Up);
-        const double ex = Xm[(sm + 1) & 3][0] - Xm[sm][0], ey = Xm[(sm + 1) & 3][1] - Xm[sm][1];
-        const double dS = sqrt(ex * ex + ey * ey) / 2.0;
-        const double nref[4][2] = {{0.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}, {-1.0, 0.0}};
-        for (int q = 0; q < nq1; ++q) {
-          double xm, em, xp, ep;
-          side_point(sm, 0, gx[q], xm, em);
-          side_point(sp, flip, gx[q], xp, ep);
-          double J[2][2], phm[4], php[4], dd[4][2];
-          quad4_jacobian(Xm, xm, em, J);
-          const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-          const double r = 1.0 / det;
-          // J⁻ᵀ n_ref, normalised (mapping.jl:843-855)
-          const double a = nref[sm][0] * (J[1][1] * r) + nref[sm][1] * (-J[1][0] * r);
-          const double b = nref[sm][0] * (-J[0][1] * r) + nref[sm][1] * (J[0][0] * r);
-          const double inv = 1.0 / sqrt(a * a + b * b);
-          const double nx = a * inv, ny = b * inv;
-          q1_basis(xm, em, phm, dd);
-          q1_basis(xp, ep, php, dd);
-          double wm[5], wp[5], F[5];
-          for (int k = 0; k < 5; ++k) {
-            wm[k] = phm[0] * Um[k][0] + phm[1] * Um[k][1] + phm[2] * Um[k][2] + phm[3] * Um[k][3];
-            wp[k] = php[0] * Up[k][0] + php[1] * Up[k][1] + php[2] * Up[k][2] + php[3] * Up[k][3];
-          }
-          roe_flux(wm, wp, nx, ny, gamma, F);
-          // fcoef ∫ F ⋅ jump(v), jump(v) = v⁻ - v⁺ (fcoef = -1 in covo.jl's l)
-          const double wf = gw[q] * dS;
-          const double* ph = role == 1 ? phm : php;
-          const double sg = role == 1 ? fcoef : -fcoef;
-          for (int k = 0; k < 5; ++k)
-            for (int l = 0; l < 4; ++l) acc[k][l] += sg * (wf * F[k] * ph[l]);
+    }
+  }
+  if (valid && (terms & 2)) {
+    // side s = g / 2, its face points q = g % 2, g % 2 + 2, ...: the face's
+    // flux times this cell's basis at the point, with the sign of its role
+    const int s = g >> 1;
+    const int32_t* S = sides + (4 * c + s) * 4;
+    const int role = S[2];
+    if (role != 0) {
+      const int64_t f = S[0];
+      const int flip = role == 1 ? 0 : S[3];
+      // fcoef ∫ F ⋅ jump(v), jump(v) = v⁻ - v⁺ (fcoef = -1 in covo.jl's l)
+      const double sg = role == 1 ? fcoef : -fcoef;
+      for (int q = g & 1; q < nq1; q += 2) {
+        double xi, eta, ph[4], dd[4][2];
+        side_point(s, flip, gx[q], xi, eta);
+        q1_basis(xi, eta, ph, dd);
+        const double* Gq = G + (f * nq1 + q) * 5;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          const double Fk = Gq[k];
+#pragma unroll
+          for (int l = 0; l < 4; ++l) acc[k][l] += sg * (Fk * ph[l]);
         }
       }
     }
   }
-  for (int k = 0; k < 5; ++k)
-    for (int l = 0; l < 4; ++l) part[lc][g][k * 4 + l] = acc[k][l];
-  wave_sync();
-  // fixed-order sums: entry e = k*4 + l over the 8 parts
-  for (int e = g; e < 20; e += 8) {
-    double s = 0.0;
-    for (int p = 0; p < 8; ++p) s += part[lc][p][e];
-    tot[lc][e] = s;
-  }
-  wave_sync();
+  // sums over the cell's 8 lanes, butterfly in a fixed order (a + b = b + a:
+  // all 8 lanes hold the same totals)
+#pragma unroll
+  for (int off = 1; off < 8; off <<= 1)
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+#pragma unroll
+      for (int l = 0; l < 4; ++l) acc[k][l] += __shfl_xor(acc[k][l], off);
   if (!valid) return;
   if (!step) {
-    // b += l(v) (assemble_linear!)
-    for (int e = g; e < 20; e += 8) {
-      const int k = e >> 2, l = e & 3;
-      const int32_t id = maps[k * 4 * ncells + 4 * c + l];
-      out[id] += tot[lc][e];
-    }
+    // b += l(v) (assemble_linear!): lane g writes entries g, g + 8, g + 16
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+#pragma unroll
+      for (int l = 0; l < 4; ++l)
+        if (((k * 4 + l) & 7) == g) out[maps[k * stride + 4 * c + l]] += acc[k][l];
     return;
   }
   // u + Δt M⁻¹ b: the cell's Q1 mass matrix (same rule), one component per lane
@@ -282,7 +304,11 @@ __global__ __launch_bounds__(256) void k_euler_dg(
       for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j) M[i][j] += wd * phi[i] * phi[j];
     }
-    for (int l = 0; l < 4; ++l) rhs[l] = tot[lc][g * 4 + l];
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      if (k == g)
+#pragma unroll
+        for (int l = 0; l < 4; ++l) rhs[l] = acc[k][l];
     // Gaussian elimination (M is symmetric positive definite)
     for (int i = 0; i < 4; ++i) {
       const double piv = 1.0 / M[i][i];
@@ -298,7 +324,7 @@ __global__ __launch_bounds__(256) void k_euler_dg(
       rhs[i] = v / M[i][i];
     }
     for (int l = 0; l < 4; ++l) {
-      const int32_t id = maps[g * 4 * ncells + 4 * c + l];
+      const int32_t id = maps[g * stride + 4 * c + l];
       out[id] = u[id] + dt * rhs[l];
     }
   }
@@ -307,15 +333,22 @@ __global__ __launch_bounds__(256) void k_euler_dg(
 }  // namespace
 
 hipError_t launch_euler_dg(hipStream_t s, const double* coords, const int32_t* c2n,
-                           int64_t ncells, const int32_t* sides, const int32_t* maps,
-                           const double* gq, int nq1, double gamma, int terms, double fcoef,
-                           int step, double dt, const double* u, double* out) {
+                           int64_t ncells, const int32_t* sides, const int32_t* faces,
+                           int64_t nfaces, const int32_t* maps, const double* gq, int nq1,
+                           double gamma, int terms, double fcoef, int step, double dt,
+                           const double* u, double* G, double* out) {
   if (ncells <= 0) return hipSuccess;
   if (nq1 < 1 || nq1 > kEulerMaxQ1) return hipErrorInvalidValue;
   const int64_t nb = (ncells + kEulerCellsPerBlock - 1) / kEulerCellsPerBlock;
   if (nb > 0x7fffffff) return hipErrorInvalidValue;
+  if ((terms & 2) && nfaces > 0) {
+    const int64_t n = nfaces * nq1;
+    const dim3 grid((unsigned)((n + 255) / 256 > 65536 ? 65536 : (n + 255) / 256));
+    hipLaunchKernelGGL(k_euler_faces, grid, dim3(256), 0, s, coords, c2n, ncells, faces, nfaces,
+                       maps, gq, nq1, gamma, u, G);
+  }
   hipLaunchKernelGGL(k_euler_dg, dim3((unsigned)nb), dim3(8 * kEulerCellsPerBlock), 0, s, coords,
-                     c2n, ncells, sides, maps, gq, nq1, gamma, terms, fcoef, step, dt, u, out);
+                     c2n, ncells, sides, maps, gq, nq1, gamma, terms, fcoef, step, dt, u, G, out);
   return hipGetLastError();
 }
 

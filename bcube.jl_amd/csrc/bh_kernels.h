@@ -120,14 +120,17 @@ hipError_t launch_dgb_points(hipStream_t s, const double* coords, const int32_t*
                              const int32_t* faces, int64_t nfaces, const double* tabs, int nl,
                              int nq, double* xq);
 // COVO Euler DG residual / explicit step on DG Q1 Quad4 cells (bh_euler.hip).
-// sides [ncells][4][4] = (neighbour, neighbour side, role 0/1/2, flip); maps
-// [5][ncells*4] global ids of (component, cell, local dof); gq = nq1 Gauss
-// points then weights; terms bit 0 volume, bit 1 faces with coefficient fcoef
-// (-1 in covo.jl's residual); step: out = u + dt M⁻¹ b, else out += b.
+// faces [nfaces][5] = (cell⁻, side⁻, cell⁺, side⁺, flip); sides
+// [ncells][4][4] = (face, 0, role 0/1/2, flip); maps [5][ncells*4] global ids
+// of (component, cell, local dof); gq = nq1 Gauss points then weights; G
+// scratch of nfaces*nq1*5 doubles; terms bit 0 volume, bit 1 faces with
+// coefficient fcoef (-1 in covo.jl's residual); step: out = u + dt M⁻¹ b, else
+// out += b.
 hipError_t launch_euler_dg(hipStream_t s, const double* coords, const int32_t* c2n,
-                           int64_t ncells, const int32_t* sides, const int32_t* maps,
-                           const double* gq, int nq1, double gamma, int terms, double fcoef,
-                           int step, double dt, const double* u, double* out);
+                           int64_t ncells, const int32_t* sides, const int32_t* faces,
+                           int64_t nfaces, const int32_t* maps, const double* gq, int nq1,
+                           double gamma, int terms, double fcoef, int step, double dt,
+                           const double* u, double* G, double* out);
 // out[map[g]] += v[g] for g < n (the targets of one field are distinct).
 hipError_t launch_add_field(hipStream_t s, double* out, const int32_t* map, const double* v,
                             int64_t n);

@@ -909,6 +909,185 @@ def bench_sipg(args, torch, B):
     return line
 
 
+def covo_state(B, m, U, gamma=1.4):
+    """The COVO vortex of benchmark/covo.jl:192-222 (β = 0.2, Rc = 0.005,
+    M₀ = 0.5, T₀ = 300, ρ₀ = 1, r = 287.15), sampled at the DG Q1 nodes of each
+    cell (nodal values; the reference L2-projects the same fields)."""
+    from bcubehip.assembler import Fields
+    beta, Rc, r, T0, M0, rho0 = 0.2, 0.005, 287.15, 300.0, 0.5, 1.0
+    c0 = np.sqrt(gamma * r * T0)
+    U0, V0 = M0 * c0, 0.0
+    Cp = gamma * r / (gamma - 1)
+    X = m.coords.reshape(-1, 2)[m.c2n.reshape(-1, 4)]          # (nc, 4, 2), CGNS order
+    P = X[:, [0, 1, 3, 2], :].reshape(-1, 2)                    # Q1 dof order (first index fastest)
+    x, y = P[:, 0], P[:, 1]
+    r2 = (x ** 2 + y ** 2) / Rc ** 2
+    T = T0 - beta ** 2 * U0 ** 2 / (2 * Cp) * np.exp(-r2)
+    ux = U0 - beta * U0 / Rc * y * np.exp(-r2 / 2)
+    uy = V0 + beta * U0 / Rc * x * np.exp(-r2 / 2)
+    rho = rho0 * (T / T0) ** (1.0 / (gamma - 1))
+    rhoE = rho * ((Cp / gamma) * T + (ux ** 2 + uy ** 2) / 2)
+    phi = np.where(Rc ** 2 * r2 < 0.01, np.exp(-r2 / 2), 0.0)
+    f = Fields(U)
+    u = np.zeros(f.ndofs)
+    u[f.maps] = np.stack([rho, rho * ux, rho * uy, rhoE, phi])
+    return u, f.maps
+
+
+def bench_covo(args, torch, B):
+    """The reference's COVO benchmark (benchmark/driver_bench_covo.jl,
+    covo.jl): Euler equations, DG Q1 MultiFESpace(ρ, ρu, ρE, ϕ) on
+    rectangle_mesh(129, 129) of [-0.05, 0.05]², degree 3, periodic in x and y.
+    Its suite -- integral_volume, integral_surface, explicit_step -- is timed
+    call by call through the host API at 128²; the throughput line is the
+    explicit step u + Δt M⁻¹ b(u) with everything resident in HBM on n x n
+    cells (n = 1024 unless --n)."""
+    gamma, l, degree = 1.4, 0.05, 3
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+
+    def problem(n):
+        m = B.rectangle_mesh(n + 1, n + 1, xmin=-l, xmax=l, ymin=-l, ymax=l,
+                             bnd_names=("West", "East", "South", "North"))
+        fs = B.FunctionSpace("Lagrange", 1)
+        U_sca = B.TrialFESpace(fs, m, "discontinuous", size=1)
+        U_vec = B.TrialFESpace(fs, m, "discontinuous", size=2)
+        V_sca, V_vec = B.TestFESpace(U_sca), B.TestFESpace(U_vec)
+        U = B.MultiFESpace(U_sca, U_vec, U_sca, U_sca)
+        V = B.MultiFESpace(V_sca, V_vec, V_sca, V_sca)
+        u_vals, maps = covo_state(B, m, U, gamma)
+        u = B.FEFunction(U, u_vals)
+        dΩ = B.Measure(B.CellDomain(m), degree)
+        dΓ = B.Measure(B.InteriorFaceDomain(m), degree)
+        px = B.PeriodicBCType(B.Translation([-2 * l, 0.0]), ("East",), ("West",))
+        py = B.PeriodicBCType(B.Translation([0.0, 2 * l]), ("South",), ("North",))
+        dΓx = B.Measure(B.BoundaryFaceDomain(m, px), degree)
+        dΓy = B.Measure(B.BoundaryFaceDomain(m, py), degree)
+        nΓ = B.get_face_normals(dΓ)
+        fΩ = lambda u, v: B.compose(B.EulerFlux(gamma), u, tuple(map(B.grad, v)))  # noqa: E731
+        fΓ = lambda u, v, n: B.compose(B.RoeFlux(gamma), B.side_n(u), B.side_p(u),  # noqa: E731
+                                       B.jump(v), B.side_n(n))
+        forms = {"l_vol": lambda v: B.integrate(fΩ(u, v), dΩ),
+                 "l_Γ": lambda v: B.integrate(fΓ(u, v, nΓ), dΓ)}
+        return m, U, V, u_vals, maps, forms, (dΓx.domain, dΓy.domain)
+
+    # 1. the reference's suite at 128² (host arrays in and out, as the reference's)
+    m, U, V, u_vals, maps, forms, per = problem(128)
+    n_entries = u_vals.size
+    dx = 2 * l / 128
+    c0 = np.sqrt(gamma * 287.15 * 300.0)
+    dt = 0.1 * dx / ((1 + 0.2) * 0.5 * c0 + c0) / 3  # covo.jl:245: CFL 2l/(nx-1)/((1+β)U₀+c₀)/(2p+1)
+    suite = {}
+    b = np.zeros(n_entries)
+    for name in ("l_vol", "l_Γ"):
+        for _ in range(3):
+            B.assemble_linear_(b, forms[name], V)
+        reps = 50
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            B.assemble_linear_(b, forms[name], V)
+        suite["integral_volume" if name == "l_vol" else "integral_surface"] = (
+            (time.perf_counter() - t0) / reps * 1e3)
+    plan128 = B.EulerDG(V, gamma, degree, per)
+    for _ in range(3):
+        plan128.step(u_vals, dt)
+    t0 = time.perf_counter()
+    for _ in range(50):
+        plan128.step(u_vals, dt)
+    suite["explicit_step"] = (time.perf_counter() - t0) / 50 * 1e3
+    # the same three on device-resident arrays
+    du = torch.from_numpy(u_vals).to(dev)
+    db = torch.zeros_like(du)
+    dev_ms = {}
+    m.backend.set_stream(stream.cuda_stream)
+    for name, run in (("integral_volume", lambda: plan128.residual_device(du.data_ptr(), db.data_ptr(), 1, -1.0)),
+                      ("integral_surface", lambda: plan128.residual_device(du.data_ptr(), db.data_ptr(), 2, 1.0)),
+                      ("explicit_step", lambda: plan128.step_device(du.data_ptr(), db.data_ptr(), dt))):
+        for _ in range(5):
+            run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(200):
+            run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        dev_ms[name] = e0.elapsed_time(e1) / 200
+    # 2. throughput: explicit steps on n x n, state resident in HBM
+    n = args.n if args.n != 64 else 1024
+    mb, Ub, Vb, ub_vals, mapsb, _, perb = problem(n)
+    t_setup = time.perf_counter()
+    plan = B.EulerDG(Vb, gamma, degree, perb)
+    t_setup = time.perf_counter() - t_setup
+    mb.backend.set_stream(stream.cuda_stream)
+    ua = torch.from_numpy(ub_vals).to(dev)
+    ub = torch.empty_like(ua)
+    dtn = 0.1 * (2 * l / n) / ((1 + 0.2) * 0.5 * c0 + c0) / 3
+    for _ in range(args.warmup):
+        plan.step_device(ua.data_ptr(), ub.data_ptr(), dtn)
+        ua, ub = ub, ua
+    torch.cuda.synchronize()
+    mb.backend.set_timing(True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        plan.step_device(ua.data_ptr(), ub.data_ptr(), dtn)
+        ua, ub = ub, ua
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    launches, _ = mb.backend.timings_ex(cap=100000)
+    mb.backend.set_timing(False)
+    kms = float(np.mean(launches))
+    ncells = mb.ncells
+    ndofs = ub_vals.size
+    # algorithmic bytes per step: the state read once and written once (20
+    # values per cell), the component maps (20 int32), the side table (16 int32),
+    # the cell's node ids (4 int32) and its nodes' coordinates (one node per cell)
+    alg = ncells * (20 * 8 * 2 + 20 * 4 + 16 * 4 + 4 * 4 + 16)
+    ach = alg / (kms * 1e-3) / 1e9
+    finite = bool(torch.isfinite(ua).all().item())
+    line = {
+        "metric": "COVO Euler DG (driver_bench_covo.jl) explicit step, residual entries per second, FP64",
+        "value": ndofs / (ms * 1e-3) / 1e9, "unit": "Gentries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (the COVO vortex sampled at the DG nodes)",
+        "config": {"workload": f"rectangle_mesh({n + 1}, {n + 1}) of [-0.05, 0.05]², DG Q1 "
+                               f"MultiFESpace(ρ, ρu, ρE, ϕ), degree 3, periodic x and y, "
+                               f"u + Δt M⁻¹ b(u) in HBM",
+                   "ncells": ncells, "ndofs": ndofs, "setup_s": round(t_setup, 2),
+                   "state_finite_after_steps": finite},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": "k_euler_dg",
+                     "avg_launch_ms": kms, "alg_bytes_per_launch": alg},
+        "reference_suite_128_ms": {"host_api": suite, "device_resident": dev_ms},
+    }
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ns = 48
+        ms_ = B.rectangle_mesh(ns + 1, ns + 1, xmin=-l, xmax=l, ymin=-l, ymax=l,
+                               bnd_names=("West", "East", "South", "North"),
+                               backend=B.BcubeBackendHostOnly())
+        om = O.OMesh(ms_.dim, ms_.coords, ms_.celltypes, ms_.c2n_ptr, ms_.c2n)
+        per_o = (O.periodic_faces(om, ms_.bc_nodes, np.array([-2 * l, 0.0]), ("East",), ("West",))
+                 + O.periodic_faces(om, ms_.bc_nodes, np.array([0.0, 2 * l]), ("South",),
+                                    ("North",)))
+        rng = np.random.default_rng(0)
+        Uo = np.stack([rng.uniform(0.9, 1.1, (ns * ns, 4)), rng.uniform(-0.1, 0.1, (ns * ns, 4)),
+                       rng.uniform(-0.1, 0.1, (ns * ns, 4)), rng.uniform(2.4, 2.6, (ns * ns, 4)),
+                       rng.uniform(0, 1, (ns * ns, 4))])
+        t0 = time.perf_counter()
+        R = O.euler_dg_residual(om, Uo, gamma, degree, per_o)
+        O.euler_dg_mass_inverse_apply(om, R, degree)
+        dt_cpu = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": Uo.size / dt_cpu / 1e9, "unit": "Gentries/s", "cores": 1,
+                                "kind": "port",
+                                "sample": f"oracle restatement (Python loops), one explicit step "
+                                          f"on {ns}x{ns} cells ({Uo.size} entries) in {dt_cpu:.2f} s"}
+    return line
+
+
 def bench_build(args, torch, B):
     """Setup on the device for meshes that change: the DofHandler numbering
     (bh_dofhandler_device) and the CSR pattern with its gather lists
@@ -1002,13 +1181,15 @@ def main():
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="c2",
-                    choices=["c1", "c2", "c3", "c4", "c5", "vec", "build", "basic", "sipg"],
+                    choices=["c1", "c2", "c3", "c4", "c5", "vec", "build", "basic", "sipg",
+                             "covo"],
                     help="c2: the headline 64^3 Q2 hexa workload; c1: the 64^2 Q1 Poisson "
                          "matrix + right-hand side; c3: mixed tri/quad order buckets; c4: the DG "
                          "transport residual; c5: the many-instance batch; vec: a 3-component "
                          "Q2 hexa space (48^3 unless --n); build: the device numbering and "
                          "pattern build; basic: the reference's basic_bilinear benchmark; "
-                         "sipg: the reference's Poisson DG benchmark (256^2 unless --n)")
+                         "sipg: the reference's Poisson DG benchmark (256^2 unless --n); "
+                         "covo: the reference's COVO Euler DG benchmark (1024^2 unless --n)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (calibrates the "
                          "FETCH_SIZE/WRITE_SIZE counters against a known byte count)")
@@ -1036,6 +1217,9 @@ def main():
         return
     if args.config == "sipg":
         print(json.dumps(bench_sipg(args, torch, B)), flush=True)
+        return
+    if args.config == "covo":
+        print(json.dumps(bench_covo(args, torch, B)), flush=True)
         return
     if args.config == "c5":
         print(json.dumps(bench_c5(args, torch, B)), flush=True)

@@ -1712,6 +1712,33 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
   gauss_legendre(nq1, gx, gw);
   std::vector<double> gq(gx);
   gq.insert(gq.end(), gw.begin(), gw.end());
+  if (nq1 == 2) {
+    // Φ⁻¹ of Φ[q][l] = φ_l(ξ_q) at the 2x2 points (q = qi + 2 qj; Q1 tensor
+    // basis, l = i + 2j), Gauss-Jordan with partial pivoting
+    double A[4][8] = {};
+    for (int q = 0; q < 4; ++q) {
+      const double xi = gx[q & 1], eta = gx[q >> 1];
+      const double a[2] = {0.5 * (1.0 - xi), 0.5 * (1.0 + xi)};
+      const double b[2] = {0.5 * (1.0 - eta), 0.5 * (1.0 + eta)};
+      for (int l = 0; l < 4; ++l) A[q][l] = a[l & 1] * b[l >> 1];
+      A[q][4 + q] = 1.0;
+    }
+    for (int i = 0; i < 4; ++i) {
+      int pr = i;
+      for (int r = i + 1; r < 4; ++r)
+        if (std::fabs(A[r][i]) > std::fabs(A[pr][i])) pr = r;
+      for (int j = 0; j < 8; ++j) std::swap(A[i][j], A[pr][j]);
+      const double piv = A[i][i];
+      for (int j = 0; j < 8; ++j) A[i][j] /= piv;
+      for (int r = 0; r < 4; ++r)
+        if (r != i) {
+          const double f = A[r][i];
+          for (int j = 0; j < 8; ++j) A[r][j] -= f * A[i][j];
+        }
+    }
+    for (int l = 0; l < 4; ++l)
+      for (int q = 0; q < 4; ++q) gq.push_back(A[l][4 + q]);
+  }
   std::vector<int32_t> c2n32(hm.c2n.begin(), hm.c2n.end());
   auto e = std::make_unique<bh_euler_s>();
   e->V = V;

@@ -289,8 +289,38 @@ __global__ __launch_bounds__(256) void k_euler_dg(
         if (((k * 4 + l) & 7) == g) out[maps[k * stride + 4 * c + l]] += acc[k][l];
     return;
   }
-  // u + Δt M⁻¹ b: the cell's Q1 mass matrix (same rule), one component per lane
-  if (g < 5) {
+  // u + Δt M⁻¹ b, one component per lane
+  if (g < 5 && nq1 == 2) {
+    // 2x2 Gauss points, as many as basis functions: M = Φᵀ D Φ with Φ[q][l] =
+    // φ_l(ξ_q) square and D = diag(w_q |J(ξ_q)|), so M⁻¹ = Φ⁻¹ D⁻¹ Φ⁻ᵀ (Φ⁻¹
+    // row-major after the points and weights in gq)
+    const double* Pi = gq + 2 * nq1;
+    double X[4][2];
+    load_cell(coords, c2n, c, X);
+    double rhs[4], y[4];
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      if (k == g)
+#pragma unroll
+        for (int l = 0; l < 4; ++l) rhs[l] = acc[k][l];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int qi = q & 1, qj = q >> 1;
+      double J[2][2];
+      quad4_jacobian(X, gx[qi], gx[qj], J);
+      const double wd = gw[qi] * gw[qj] * fabs(J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+      // (Φ⁻ᵀ r)_q = Σ_l Φ⁻¹[l][q] r_l
+      y[q] = (Pi[q] * rhs[0] + Pi[4 + q] * rhs[1] + Pi[8 + q] * rhs[2] + Pi[12 + q] * rhs[3]) / wd;
+    }
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const double x = Pi[4 * l] * y[0] + Pi[4 * l + 1] * y[1] + Pi[4 * l + 2] * y[2] +
+                       Pi[4 * l + 3] * y[3];
+      const int32_t id = maps[g * stride + 4 * c + l];
+      out[id] = u[id] + dt * x;
+    }
+  } else if (g < 5) {
+    // the cell's Q1 mass matrix (same rule), Gaussian elimination
     double X[4][2];
     load_cell(coords, c2n, c, X);
     double M[4][4] = {}, rhs[4];

@@ -1023,6 +1023,12 @@ def bench_covo(args, torch, B):
     ua = torch.from_numpy(ub_vals).to(dev)
     ub = torch.empty_like(ua)
     dtn = 0.1 * (2 * l / n) / ((1 + 0.2) * 0.5 * c0 + c0) / 3
+    if args.calib_copy_mib:
+        nbytes = args.calib_copy_mib * (1 << 20) // 8
+        src = torch.ones(nbytes, dtype=torch.float64, device=dev)
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
     for _ in range(args.warmup):
         plan.step_device(ua.data_ptr(), ub.data_ptr(), dtn)
         ua, ub = ub, ua
@@ -1047,6 +1053,12 @@ def bench_covo(args, torch, B):
     alg = ncells * (20 * 8 * 2 + 20 * 4 + 16 * 4 + 4 * 4 + 16)
     ach = alg / (kms * 1e-3) / 1e9
     finite = bool(torch.isfinite(ua).all().item())
+    # HBM bytes per step measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    # (both kernels, calibrated on a device copy of known size) at 1024²
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "covo_traffic.json")
+    if os.path.exists(tf) and n == 1024:
+        traffic = json.load(open(tf))["step_bytes"]
     line = {
         "metric": "COVO Euler DG (driver_bench_covo.jl) explicit step, residual entries per second, FP64",
         "value": ndofs / (ms * 1e-3) / 1e9, "unit": "Gentries/s", "n_gpus": 1,
@@ -1059,7 +1071,8 @@ def bench_covo(args, torch, B):
                    "ncells": ncells, "ndofs": ndofs, "setup_s": round(t_setup, 2),
                    "state_finite_after_steps": finite},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": "k_euler_dg",
+                     "frac": ach / PEAK_HBM_GBS, "traffic": traffic,
+                     "kernel": "k_euler_faces + k_euler_dg",
                      "avg_launch_ms": kms, "alg_bytes_per_launch": alg},
         "reference_suite_128_ms": {"host_api": suite, "device_resident": dev_ms},
     }

@@ -1013,6 +1013,27 @@ def bench_covo(args, torch, B):
         e1.record(stream)
         torch.cuda.synchronize()
         dev_ms[name] = e0.elapsed_time(e1) / 200
+    # covo.jl's time loop as a HIP graph: 20 steps (u -> v -> u ...) captured
+    # once and replayed
+    gstream = torch.cuda.Stream()
+    m.backend.set_stream(gstream.cuda_stream)
+    ga = du.clone()
+    gb = torch.empty_like(ga)
+    torch.cuda.synchronize()
+    with m.backend.capture() as graph:
+        for _ in range(10):
+            plan128.step_device(ga.data_ptr(), gb.data_ptr(), dt)
+            plan128.step_device(gb.data_ptr(), ga.data_ptr(), dt)
+    graph.launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(gstream)
+    for _ in range(20):
+        graph.launch()
+    e1.record(gstream)
+    torch.cuda.synchronize()
+    dev_ms["explicit_step_in_graph"] = e0.elapsed_time(e1) / (20 * 20)
+    m.backend.set_stream(stream.cuda_stream)
     # 2. throughput: explicit steps on n x n, state resident in HBM
     n = args.n if args.n != 64 else 1024
     mb, Ub, Vb, ub_vals, mapsb, _, perb = problem(n)

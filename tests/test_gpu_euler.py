@@ -186,3 +186,36 @@ def test_euler_device_path_bitwise_repeatable():
     plan.step_device(du.data_ptr(), do.data_ptr(), 1e-5)
     B.lib().bh_sync(m.backend.ctx)
     assert np.array_equal(do.cpu().numpy(), step_host)
+
+
+def test_euler_steps_in_a_hip_graph_replay_bitwise():
+    """covo.jl's time loop as a captured graph: two explicit steps (u -> v ->
+    u) captured once and replayed; every replay equals the eager steps bit for
+    bit (eager work and synchronisations between replays)."""
+    import torch
+    n = 32
+    m, _ = _meshes(n, 0.1, seed=6)
+    U, V = _setup(m)
+    u0, _ = _to_global(U, _state(n * n, seed=8))
+    _, (dΓx, dΓy) = _forms(m, U, u0)
+    plan = B.EulerDG(V, GAMMA, 3, (dΓx.domain, dΓy.domain))
+    dt = 2e-5
+    # eager reference trajectory on the host path
+    ref = [u0]
+    for _ in range(6):
+        ref.append(plan.step(ref[-1], dt))
+    stream = torch.cuda.Stream()
+    m.backend.set_stream(stream.cuda_stream)
+    a = torch.from_numpy(u0).cuda()
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    with m.backend.capture() as g:
+        plan.step_device(a.data_ptr(), b.data_ptr(), dt)
+        plan.step_device(b.data_ptr(), a.data_ptr(), dt)
+    for r in range(3):
+        g.launch()
+        B.lib().bh_sync(m.backend.ctx)
+        got = a.cpu().numpy()
+        assert np.array_equal(got, ref[2 * (r + 1)]), r
+        _ = (a * 2.0).sum().item()   # eager work between replays
+    m.backend.set_stream(None)

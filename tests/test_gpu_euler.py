@@ -219,3 +219,28 @@ def test_euler_steps_in_a_hip_graph_replay_bitwise():
         assert np.array_equal(got, ref[2 * (r + 1)]), r
         _ = (a * 2.0).sum().item()   # eager work between replays
     m.backend.set_stream(None)
+
+
+def test_euler_residual_soa_numbering():
+    """MultiFESpace(...; arrayOfStruct=false): a cell's dofs spread over the
+    whole vector (space by space); the kernels read them through the maps."""
+    n = 5
+    m, om = _meshes(n, 0.1, seed=11, rotate=True)
+    fs = B.FunctionSpace("Lagrange", 1)
+    U_sca = B.TrialFESpace(fs, m, "discontinuous", size=1)
+    U_vec = B.TrialFESpace(fs, m, "discontinuous", size=2)
+    U = B.MultiFESpace(U_sca, U_vec, U_sca, U_sca, arrayOfStruct=False)
+    V = B.MultiFESpace(B.TestFESpace(U_sca), B.TestFESpace(U_vec), B.TestFESpace(U_sca),
+                       B.TestFESpace(U_sca), arrayOfStruct=False)
+    Uo = _state(n * n, seed=12)
+    u, maps = _to_global(U, Uo)
+    assert not all(np.array_equal(np.sort(maps[:, 4 * c:4 * c + 4].ravel()),
+                                  np.arange(20 * c, 20 * c + 20)) for c in range(n * n))
+    forms, (dΓx, dΓy) = _forms(m, U, u)
+    b = B.assemble_linear(forms["l"], V)
+    per = [list(r) for r in np.concatenate([dΓx.domain.periodic_faces(),
+                                            dΓy.domain.periodic_faces()])]
+    ref = O.euler_dg_residual(om, Uo, GAMMA, 3, per)
+    err = np.abs(b[maps].reshape(5, n * n, 4) - ref).max() / np.abs(ref).max()
+    print(f"SoA numbering: rel err {err:.2e}")
+    assert err <= RTOL

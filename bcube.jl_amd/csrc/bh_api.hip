@@ -182,6 +182,17 @@ struct bh_dgb_s {
   double *d_vals = nullptr, *d_ke = nullptr, *d_tab = nullptr, *d_coefs = nullptr,
          *d_vcoef = nullptr, *d_g = nullptr, *d_b = nullptr;
   double* d_geo = nullptr;      // [nfi][2][nq][8]: interior face-point geometry
+  // device buffers go with the handle, also when creation fails half way
+  ~bh_dgb_s() {
+    if (dev < 0) return;
+    (void)hipSetDevice(dev);
+    for (void* q : {(void*)d_fi, (void*)d_fb, (void*)d_pptr, (void*)d_pairs, (void*)d_c2n,
+                    (void*)d_rowptr, d_goff, (void*)d_vals, (void*)d_ke, (void*)d_tab,
+                    (void*)d_coefs, (void*)d_vcoef, (void*)d_g, (void*)d_b, (void*)d_fpos,
+                    (void*)d_fstride, (void*)d_dsrc, (void*)d_dpos, (void*)d_dstride,
+                    (void*)d_geo})
+      (void)hipFree(q);
+  }
 };
 
 // The COVO Euler DG residual (bh_euler_*): side table, component maps, rule.
@@ -195,6 +206,13 @@ struct bh_euler_s {
   int32_t *d_sides = nullptr, *d_faces = nullptr, *d_maps = nullptr, *d_c2n = nullptr;
   double *d_gq = nullptr, *d_G = nullptr;
   double *d_u = nullptr, *d_out = nullptr;  // staging of host-`where` calls
+  ~bh_euler_s() {
+    if (dev < 0) return;
+    (void)hipSetDevice(dev);
+    for (void* q : {(void*)d_sides, (void*)d_maps, (void*)d_c2n, (void*)d_gq, (void*)d_u,
+                    (void*)d_out, (void*)d_faces, (void*)d_G})
+      (void)hipFree(q);
+  }
 };
 
 struct bh_dg_s {
@@ -1626,15 +1644,6 @@ int bh_dgb_boundary_linear(bh_dgb d, double c0, double c1, const double* g, doub
 
 int bh_dgb_destroy(bh_dgb d) {
   if (!d) return BH_ERR_ARG;
-  if (d->dev >= 0) {
-    (void)hipSetDevice(d->dev);
-    for (void* q : {(void*)d->d_fi, (void*)d->d_fb, (void*)d->d_pptr, (void*)d->d_pairs,
-                    (void*)d->d_c2n, (void*)d->d_rowptr, d->d_goff, (void*)d->d_vals,
-                    (void*)d->d_ke, (void*)d->d_tab, (void*)d->d_coefs, (void*)d->d_vcoef,
-                    (void*)d->d_g, (void*)d->d_b, (void*)d->d_fpos, (void*)d->d_fstride,
-                    (void*)d->d_dsrc, (void*)d->d_dpos, (void*)d->d_dstride, (void*)d->d_geo})
-      (void)hipFree(q);
-  }
   delete d;
   return BH_OK;
 }
@@ -1812,12 +1821,6 @@ int bh_euler_step(bh_euler e, double dt, const double* u, double* out, int where
 
 int bh_euler_destroy(bh_euler e) {
   if (!e) return BH_ERR_ARG;
-  if (e->dev >= 0) {
-    (void)hipSetDevice(e->dev);
-    for (void* q : {(void*)e->d_sides, (void*)e->d_maps, (void*)e->d_c2n, (void*)e->d_gq,
-                    (void*)e->d_u, (void*)e->d_out, (void*)e->d_faces, (void*)e->d_G})
-      (void)hipFree(q);
-  }
   delete e;
   return BH_OK;
 }

@@ -100,6 +100,8 @@ class EulerDG:
             raise ValueError("u has the wrong size")
         if b is None:
             b = np.zeros(self.ndofs, dtype=np.float64)
+        if b.dtype != np.float64 or not b.flags.c_contiguous or b.size != self.ndofs:
+            raise ValueError("b must be a contiguous float64 vector of ndofs entries")
         L.check(L.lib().bh_euler_residual(self.handle, terms, float(face_coef), u.ctypes.data,
                                           b.ctypes.data, L.HOST), self.ctx, "bh_euler_residual")
         return b

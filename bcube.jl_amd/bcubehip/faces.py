@@ -103,6 +103,8 @@ class FaceBilinearPlan:
     def assemble_host(self, out: np.ndarray | None = None) -> np.ndarray:
         if out is None:
             out = np.empty(self.nnz, dtype=np.float64)
+        if out.dtype != np.float64 or not out.flags.c_contiguous or out.size != self.nnz:
+            raise ValueError("out must be a contiguous float64 vector of nnz entries")
         L.check(L.lib().bh_dgb_assemble(self.handle, out.ctypes.data, L.HOST), self.ctx,
                 "bh_dgb_assemble")
         return out
@@ -151,6 +153,8 @@ class BoundaryFaces:
 
     def add_linear(self, b: np.ndarray, c0: float, c1: float, g=None) -> None:
         """b += Σ_faces ∫ (c0 v + c1 n⋅∇v) g dΓ, g a PhysicalFunction or None (g = 1)."""
+        if b.dtype != np.float64 or not b.flags.c_contiguous or b.size != self.V.get_ndofs():
+            raise ValueError("b must be a contiguous float64 vector of ndofs entries")
         if g is None:
             vals = np.ones(self.nboundary * self.nq, dtype=np.float64)
         else:

@@ -26,6 +26,7 @@ from .dirichlet import (DirichletPlan, apply_dirichlet_to_matrix_, apply_dirichl
                         apply_homogeneous_dirichlet_to_vector_, assemble_dirichlet_vector,
                         boundary_faces, dirichlet_dofs_values, dof_coordinates)
 from .io import nodes_gmsh2cgns, read_mesh
+from .affine import AffineFESystem
 from .projection import projection_l2_
 
 __all__ = [
@@ -45,5 +46,5 @@ __all__ = [
     "nodes_gmsh2cgns", "read_mesh", "projection_l2_", "MultiFESpace", "component_numbering",
     "ddot", "MultiBilinearPlan", "BoundaryFaceDomain", "avg", "BoundaryFaces",
     "FaceBilinearPlan", "EulerFlux", "RoeFlux", "PeriodicBCType", "Translation", "EulerDG",
-    "periodic_face_pairs",
+    "periodic_face_pairs", "AffineFESystem",
 ]

@@ -355,6 +355,10 @@ class Pattern:
         L.check(L.lib().bh_pattern_build(self.U.handle, self.U.handle, C.byref(h)), self.ctx,
                 "bh_pattern_build")
         self.handle = h
+        nr, nc, nnz, mr = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+        L.check(L.lib().bh_pattern_info(h, C.byref(nr), C.byref(nc), C.byref(nnz), C.byref(mr)),
+                self.ctx, "bh_pattern_info")
+        self.nnz = nnz.value
 
     def __del__(self):
         try:

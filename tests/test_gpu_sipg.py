@@ -240,3 +240,34 @@ def test_sipg_forms_on_a_continuous_space(order):
     err = np.abs(A.toarray() - ref).max() / np.abs(ref).max()
     print(f"continuous Q{order}: rel err {err:.2e}")
     assert err <= RTOL
+
+
+def test_affine_fe_system_solves_the_benchmark_problem():
+    """AffineFESystem(a, l, U, V) and solve (affine_fe_system.jl:45-119) on the
+    Poisson DG benchmark: no Dirichlet dofs (Nitsche terms), the cubic
+    solution reproduced."""
+    order, n = 3, 4
+    m = _rect(n)
+    U, V, a, l = _sipg(m, order, 7, order * (order + 1) * n)
+    sol = B.AffineFESystem(a, l, U, V).solve()
+    err = np.abs(sol.get_dof_values() - _ua(_dg_points(m, order))).max()
+    assert err < 1e-10
+
+
+def test_affine_fe_system_with_dirichlet_lifting():
+    """The reference's 3D heat solve (test_assembler.jl:457-509) through
+    AffineFESystem: T = 260 on xmin, the cell-centre values of
+    260 + (q/η) s (1 - s/2) within 2e-14."""
+    m = B.hexa_mesh(5, 5, 5, xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, zmin=-0.5, zmax=0.5)
+    lam, q = 100.0, 1500.0
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", 2), m, {"xmin": 260.0})
+    V = B.TestFESpace(U)
+    dΩ = B.Measure(B.CellDomain(m), 5)
+    sys_ = B.AffineFESystem(lambda u, v: B.integrate(lam * B.dot(B.grad(u), B.grad(v)), dΩ),
+                            lambda v: B.integrate(q * v, dΩ), U, V)
+    x = sys_.solve().get_dof_values()
+    centre = [U.get_dofs(c)[13] - 1 for c in range(m.ncells)]
+    xc = np.array([m.nodes()[m.cell_nodes(c)].mean(axis=0) for c in range(m.ncells)])
+    s = xc[:, 0] + 0.5
+    Tca = 260.0 + (q / lam) * s * (1.0 - 0.5 * s)
+    assert np.abs(x[centre] - Tca).max() / np.abs(Tca).max() < 2.0e-14

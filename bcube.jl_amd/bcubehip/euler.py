@@ -27,24 +27,27 @@ _SQ_F2N = [(0, 1), (1, 2), (2, 3), (3, 0)]  # faces2nodes of the square (shape.j
 
 def periodic_face_pairs(mesh, bc) -> np.ndarray:
     """_compute_periodicity (domain.jl:269-397) and the periodic FaceInfo
-    (domain.jl:845-890) on a Quad4 mesh: for each slave face (face order) the
-    master face whose centre maps onto it; side⁻ is the slave cell, side⁺ the
-    master cell.  Rows [cell⁻, side⁻, cell⁺, side⁺, flip], flip = 1 when the
-    master side's first node maps onto the slave side's second node."""
-    from .dirichlet import boundary_faces
-    if np.any(mesh.celltypes != L.CELLTYPE["Quad4"]):
-        raise NotImplementedError("periodic faces on Quad4 meshes")
+    (domain.jl:845-890) on a Quad4/Tri3 mesh: for each slave face (face
+    order) the master face whose centre maps onto it; side⁻ is the slave cell,
+    side⁺ the master cell.  Rows [cell⁻, side⁻, cell⁺, side⁺, flip], flip = 1
+    when the master side's first node maps onto the slave side's second node."""
+    from .dirichlet import _SIDES, boundary_faces
+    if np.any((mesh.celltypes != L.CELLTYPE["Quad4"]) & (mesh.celltypes != L.CELLTYPE["Tri3"])):
+        raise NotImplementedError("periodic faces on Quad4 and Tri3 meshes")
     tags = boundary_faces(mesh)
     for lab in bc.labels_master + bc.labels_slave:
         if lab not in tags:
             raise ValueError(f"unknown boundary tag {lab!r}")
     X = mesh.nodes()
-    c2n = mesh.c2n.reshape(-1, 4)
     master = [f for lab in bc.labels_master for f in tags[lab]]
     slave = [f for lab in bc.labels_slave for f in tags[lab]]
 
+    def cnodes(c):
+        return mesh.c2n[mesh.c2n_ptr[c]:mesh.c2n_ptr[c + 1]]
+
     def fnodes(c, s):
-        return [int(c2n[c, k]) for k in _SQ_F2N[s]]
+        nodes = cnodes(c)
+        return [int(nodes[k]) for k in _SIDES[int(mesh.celltypes[c])][s]]
 
     mc = np.array([X[fnodes(c, s)].mean(axis=0) for c, s in master]).reshape(-1, 2)
     mimg = np.array([bc.transformation(x) for x in mc]).reshape(-1, 2)
@@ -52,7 +55,7 @@ def periodic_face_pairs(mesh, bc) -> np.ndarray:
     for ci, si in slave:
         ni = fnodes(ci, si)
         Mi = X[ni].mean(axis=0)
-        dx = np.linalg.norm(X[c2n[ci]].mean(axis=0) - Mi) * np.sqrt(mesh.dim)
+        dx = np.linalg.norm(X[cnodes(ci)].mean(axis=0) - Mi) * np.sqrt(mesh.dim)
         d = np.linalg.norm(mimg - Mi, axis=1) if len(master) else np.array([])
         hit = np.flatnonzero(d <= bc.tol * dx)
         if hit.size == 0:
@@ -77,7 +80,7 @@ class EulerDG:
             raise NotImplementedError("the Euler residual needs MultiFESpace(ρ, ρu (size 2), ρE, ϕ)")
         sc = f.scalar
         if sc.continuous or sc.fs.degree != 1:
-            raise NotImplementedError("the Euler residual needs discontinuous Q1 spaces")
+            raise NotImplementedError("the Euler residual needs discontinuous order-1 spaces")
         self.fields = f
         self.ndofs = f.ndofs
         pairs = [d.periodic_faces() for d in periodic_domains]

@@ -205,12 +205,14 @@ struct bh_euler_s {
   int64_t nfaces = 0;
   int32_t *d_sides = nullptr, *d_faces = nullptr, *d_maps = nullptr, *d_c2n = nullptr;
   double *d_gq = nullptr, *d_G = nullptr;
+  int8_t* d_ctype = nullptr;
+  EulerTabs tabs;
   double *d_u = nullptr, *d_out = nullptr;  // staging of host-`where` calls
   ~bh_euler_s() {
     if (dev < 0) return;
     (void)hipSetDevice(dev);
     for (void* q : {(void*)d_sides, (void*)d_maps, (void*)d_c2n, (void*)d_gq, (void*)d_u,
-                    (void*)d_out, (void*)d_faces, (void*)d_G})
+                    (void*)d_out, (void*)d_faces, (void*)d_G, (void*)d_ctype})
       (void)hipFree(q);
   }
 };
@@ -1658,13 +1660,25 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
   bh_mesh m = V->mesh;
   bh_ctx c = m->ctx;
   if (!c) return fail(c, BH_ERR_ARG, "the Euler residual needs a mesh created with a context");
-  if (m->geo != BH_QUAD4 || V->order != 1 || V->continuous)
-    return fail(c, BH_ERR_UNSUPPORTED, "the Euler residual needs a discontinuous Q1 space on Quad4 cells");
+  const HostMesh& hm = m->h;
+  const int64_t nc = hm.ncells, ns = V->ndofs, nlV = V->nl;
+  bool cells_ok = V->order == 1 && !V->continuous;
+  for (int64_t cc = 0; cc < nc && cells_ok; ++cc)
+    cells_ok = hm.ctype[cc] == BH_QUAD4 || hm.ctype[cc] == BH_TRI3;
+  if (!cells_ok)
+    return fail(c, BH_ERR_UNSUPPORTED,
+                "the Euler residual needs a discontinuous order-1 space on Quad4/Tri3 cells");
   const int nq1 = gauss_points_per_axis(quad_degree);
   if (nq1 > 4) return fail(c, BH_ERR_UNSUPPORTED, "the Euler residual: at most 4 Gauss points per axis");
   if (ndofs_total > INT32_MAX) return fail(c, BH_ERR_UNSUPPORTED, "more than 2^31 dofs");
-  const HostMesh& hm = m->h;
-  const int64_t nc = hm.ncells, ns = V->ndofs;
+  std::vector<int8_t> ctype(nc);
+  std::vector<int32_t> c2n4((size_t)nc * 4, -1);
+  for (int64_t cc = 0; cc < nc; ++cc) {
+    ctype[cc] = hm.ctype[cc] == BH_TRI3 ? 1 : 0;
+    const int nn = ctype[cc] ? 3 : 4;
+    for (int k = 0; k < nn; ++k) c2n4[cc * 4 + k] = (int32_t)hm.c2n[hm.c2n_ptr[cc] + k];
+  }
+  auto nsides = [&](int64_t cc) { return ctype[cc] ? 3 : 4; };
   // faces: interior faces (_build_faces!), then the periodic ones; each
   // cell side records its face and role
   std::vector<int32_t> sides((size_t)nc * 16, 0), faces;
@@ -1681,9 +1695,10 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
   {
     std::unordered_map<uint64_t, std::array<int64_t, 2>> index;
     for (int64_t cc = 0; cc < nc; ++cc) {
-      const int64_t* nd = &hm.c2n[hm.c2n_ptr[cc]];
-      for (int sd = 0; sd < 4; ++sd) {
-        const int64_t a = nd[sd], b = nd[(sd + 1) & 3];
+      const int32_t* nd = &c2n4[cc * 4];
+      const int nsd = nsides(cc);
+      for (int sd = 0; sd < nsd; ++sd) {
+        const int64_t a = nd[sd], b = nd[sd + 1 == nsd ? 0 : sd + 1];
         const uint64_t key = ((uint64_t)std::min(a, b) << 32) | (uint64_t)std::max(a, b);
         auto it = index.find(key);
         if (it == index.end()) {
@@ -1691,7 +1706,7 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
         } else {
           const int64_t cn = it->second[0];
           const int sn = (int)it->second[1];
-          const int flip = a == hm.c2n[hm.c2n_ptr[cn] + sn] ? 0 : 1;
+          const int flip = a == c2n4[cn * 4 + sn] ? 0 : 1;
           if (!put(cn, sn, cc, sd, flip)) return fail(c, BH_ERR_ARG, "a face shared by more than two cells");
         }
       }
@@ -1699,56 +1714,78 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
   }
   for (int64_t f = 0; f < nperiodic; ++f) {
     const int32_t* P = periodic + 5 * f;
-    if (P[0] < 0 || P[0] >= nc || P[2] < 0 || P[2] >= nc || P[1] < 0 || P[1] > 3 || P[3] < 0 ||
-        P[3] > 3 || (P[4] != 0 && P[4] != 1))
+    if (P[0] < 0 || P[0] >= nc || P[2] < 0 || P[2] >= nc || P[1] < 0 || P[1] >= nsides(P[0]) ||
+        P[3] < 0 || P[3] >= nsides(P[2]) || (P[4] != 0 && P[4] != 1))
       return fail(c, BH_ERR_RANGE, "periodic face out of range");
     if (!put(P[0], P[1], P[2], P[3], P[4]))
       return fail(c, BH_ERR_ARG, "a periodic face on a side that already has a face");
   }
   // maps[k][g] (global ids of component k's scalar dof g) -> per (cell, local dof)
-  std::vector<int32_t> m2((size_t)5 * nc * 4);
+  std::vector<int32_t> m2((size_t)5 * nc * 4, -1);
   std::vector<uint8_t> seen((size_t)ndofs_total, 0);
   for (int k = 0; k < 5; ++k)
-    for (int64_t q = 0; q < nc * 4; ++q) {
-      const int32_t g = V->dofs[q];
-      if (g < 0 || g >= ns) return fail(c, BH_ERR_RANGE, "dof out of range");
-      const int64_t id = maps[(int64_t)k * ns + g];
-      if (id < 0 || id >= ndofs_total) return fail(c, BH_ERR_RANGE, "global id out of range");
-      if (seen[id]++) return fail(c, BH_ERR_ARG, "the component maps are not one-to-one");
-      m2[(size_t)k * nc * 4 + q] = (int32_t)id;
-    }
-  std::vector<double> gx, gw;
+    for (int64_t cc = 0; cc < nc; ++cc)
+      for (int l = 0; l < (ctype[cc] ? 3 : 4); ++l) {
+        const int32_t g = l < nlV ? V->dofs[cc * nlV + l] : -1;
+        if (g < 0 || g >= ns) return fail(c, BH_ERR_RANGE, "dof out of range");
+        const int64_t id = maps[(int64_t)k * ns + g];
+        if (id < 0 || id >= ndofs_total) return fail(c, BH_ERR_RANGE, "global id out of range");
+        if (seen[id]++) return fail(c, BH_ERR_ARG, "the component maps are not one-to-one");
+        m2[(size_t)k * nc * 4 + cc * 4 + l] = (int32_t)id;
+      }
+  // tables: the face rule, then per shape the volume rule (basis padded to 4)
+  EulerTabs T;
+  T.nq1 = nq1;
+  std::vector<double> tabs, gx, gw;
   gauss_legendre(nq1, gx, gw);
-  std::vector<double> gq(gx);
-  gq.insert(gq.end(), gw.begin(), gw.end());
-  if (nq1 == 2) {
-    // Φ⁻¹ of Φ[q][l] = φ_l(ξ_q) at the 2x2 points (q = qi + 2 qj; Q1 tensor
-    // basis, l = i + 2j), Gauss-Jordan with partial pivoting
-    double A[4][8] = {};
-    for (int q = 0; q < 4; ++q) {
-      const double xi = gx[q & 1], eta = gx[q >> 1];
-      const double a[2] = {0.5 * (1.0 - xi), 0.5 * (1.0 + xi)};
-      const double b[2] = {0.5 * (1.0 - eta), 0.5 * (1.0 + eta)};
-      for (int l = 0; l < 4; ++l) A[q][l] = a[l & 1] * b[l >> 1];
-      A[q][4 + q] = 1.0;
+  T.off_fx = 0;
+  tabs.insert(tabs.end(), gx.begin(), gx.end());
+  T.off_fw = (int)tabs.size();
+  tabs.insert(tabs.end(), gw.begin(), gw.end());
+  for (int sh = 0; sh < 2; ++sh) {
+    const int geo = sh ? BH_TRI3 : BH_QUAD4;
+    Tables Tb = make_tables(sh ? TRIANGLE : SQUARE, 1, quad_degree, geo);
+    if (Tb.nq <= 0) return fail(c, BH_ERR_UNSUPPORTED, "no volume rule of this degree");
+    const int np = Tb.nq, nl = Tb.nl, ng = Tb.ngeo;
+    T.npts[sh] = np;
+    T.off_w[sh] = (int)tabs.size();
+    tabs.insert(tabs.end(), Tb.w.begin(), Tb.w.end());
+    T.off_phi[sh] = (int)tabs.size();
+    for (int q = 0; q < np; ++q)
+      for (int l = 0; l < 4; ++l) tabs.push_back(l < nl ? Tb.phi[q * nl + l] : 0.0);
+    T.off_dphi[sh] = (int)tabs.size();
+    for (int q = 0; q < np; ++q)
+      for (int l = 0; l < 4; ++l)
+        for (int d = 0; d < 2; ++d) tabs.push_back(l < nl ? Tb.dphi[(q * nl + l) * 2 + d] : 0.0);
+    T.off_dg[sh] = (int)tabs.size();
+    for (int q = 0; q < np; ++q)
+      for (int k = 0; k < 4; ++k)
+        for (int d = 0; d < 2; ++d) tabs.push_back(k < ng ? Tb.dgphi[(q * ng + k) * 2 + d] : 0.0);
+    if (sh == 0 && np == 4 && nl == 4) {
+      // Φ⁻¹ of Φ[q][l] = φ_l(ξ_q), Gauss-Jordan with partial pivoting
+      double A[4][8] = {};
+      for (int q = 0; q < 4; ++q) {
+        for (int l = 0; l < 4; ++l) A[q][l] = Tb.phi[q * 4 + l];
+        A[q][4 + q] = 1.0;
+      }
+      for (int i = 0; i < 4; ++i) {
+        int pr = i;
+        for (int r = i + 1; r < 4; ++r)
+          if (std::fabs(A[r][i]) > std::fabs(A[pr][i])) pr = r;
+        for (int j = 0; j < 8; ++j) std::swap(A[i][j], A[pr][j]);
+        const double piv = A[i][i];
+        for (int j = 0; j < 8; ++j) A[i][j] /= piv;
+        for (int r = 0; r < 4; ++r)
+          if (r != i) {
+            const double fct = A[r][i];
+            for (int j = 0; j < 8; ++j) A[r][j] -= fct * A[i][j];
+          }
+      }
+      T.off_pinv = (int)tabs.size();
+      for (int l = 0; l < 4; ++l)
+        for (int q = 0; q < 4; ++q) tabs.push_back(A[l][4 + q]);
     }
-    for (int i = 0; i < 4; ++i) {
-      int pr = i;
-      for (int r = i + 1; r < 4; ++r)
-        if (std::fabs(A[r][i]) > std::fabs(A[pr][i])) pr = r;
-      for (int j = 0; j < 8; ++j) std::swap(A[i][j], A[pr][j]);
-      const double piv = A[i][i];
-      for (int j = 0; j < 8; ++j) A[i][j] /= piv;
-      for (int r = 0; r < 4; ++r)
-        if (r != i) {
-          const double f = A[r][i];
-          for (int j = 0; j < 8; ++j) A[r][j] -= f * A[i][j];
-        }
-    }
-    for (int l = 0; l < 4; ++l)
-      for (int q = 0; q < 4; ++q) gq.push_back(A[l][4 + q]);
   }
-  std::vector<int32_t> c2n32(hm.c2n.begin(), hm.c2n.end());
   auto e = std::make_unique<bh_euler_s>();
   e->V = V;
   e->dev = V->dev;
@@ -1757,6 +1794,7 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
   e->ncells = nc;
   e->ntot = ndofs_total;
   e->nfaces = (int64_t)faces.size() / 5;
+  e->tabs = T;
   if (int r = set_device(c)) return r;
   auto up = [&](auto** dst, const auto& v) -> int {
     BH_HIP(c, dalloc(dst, std::max<size_t>(v.size(), 1)));
@@ -1766,9 +1804,10 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
   };
   if (int r = up(&e->d_sides, sides)) return r;
   if (int r = up(&e->d_maps, m2)) return r;
-  if (int r = up(&e->d_c2n, c2n32)) return r;
-  if (int r = up(&e->d_gq, gq)) return r;
+  if (int r = up(&e->d_c2n, c2n4)) return r;
+  if (int r = up(&e->d_gq, tabs)) return r;
   if (int r = up(&e->d_faces, faces)) return r;
+  if (int r = up(&e->d_ctype, ctype)) return r;
   BH_HIP(c, dalloc(&e->d_G, (size_t)std::max<int64_t>(e->nfaces, 1) * nq1 * 5));
   *out = e.release();
   return BH_OK;
@@ -1797,9 +1836,9 @@ static int euler_run(bh_euler e, int terms, double fcoef, int step, double dt, c
     dout = e->d_out;
   }
   time_begin(c, 2);
-  BH_HIP(c, launch_euler_dg(c->stream, e->V->mesh->d_coords, e->d_c2n, e->ncells, e->d_sides,
-                            e->d_faces, e->nfaces, e->d_maps, e->d_gq, e->nq1, e->gamma, terms,
-                            fcoef, step, dt, du, e->d_G, dout));
+  BH_HIP(c, launch_euler_dg(c->stream, e->V->mesh->d_coords, e->d_c2n, e->d_ctype, e->ncells,
+                            e->d_sides, e->d_faces, e->nfaces, e->d_maps, e->tabs, e->d_gq,
+                            e->gamma, terms, fcoef, step, dt, du, e->d_G, dout));
   time_end(c);
   if (where == BH_HOST) {
     BH_HIP(c, hipMemcpyAsync(o, dout, bytes, hipMemcpyDeviceToHost, c->stream));

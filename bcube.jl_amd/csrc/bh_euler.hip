@@ -26,25 +26,39 @@ namespace {
 constexpr int kEulerCellsPerBlock = 32;  // 8 lanes each, 256 threads
 constexpr int kEulerMaxQ1 = 4;           // Gauss points per axis
 
-// Q1 tensor Lagrange on [-1,1]², first index fastest (l = i + 2j):
-// values and reference gradients (hypercube.jl, uniform nodes)
-__device__ __forceinline__ void q1_basis(double xi, double eta, double phi[4],
-                                         double dphi[4][2]) {
-  const double a0 = 0.5 * (1.0 - xi), a1 = 0.5 * (1.0 + xi);
-  const double b0 = 0.5 * (1.0 - eta), b1 = 0.5 * (1.0 + eta);
-  phi[0] = a0 * b0;
-  phi[1] = a1 * b0;
-  phi[2] = a0 * b1;
-  phi[3] = a1 * b1;
-  dphi[0][0] = -0.5 * b0; dphi[0][1] = -0.5 * a0;
-  dphi[1][0] = 0.5 * b0;  dphi[1][1] = -0.5 * a1;
-  dphi[2][0] = -0.5 * b1; dphi[2][1] = 0.5 * a0;
-  dphi[3][0] = 0.5 * b1;  dphi[3][1] = 0.5 * a1;
+// cell shapes: 0 = Quad4 (Q1, CGNS nodes), 1 = Tri3 (P1)
+__device__ __forceinline__ int nsides_of(int ct) { return ct ? 3 : 4; }
+
+// basis values at a reference point (4 slots; the 4th is 0 on triangles):
+// Q1 tensor Lagrange on [-1,1]², first index fastest (hypercube.jl), or P1
+// on the unit triangle λ = (1-ξ-η, ξ, η) (triangle.jl)
+__device__ __forceinline__ void basis_at(int ct, double xi, double eta, double phi[4]) {
+  if (ct) {
+    phi[0] = 1.0 - xi - eta;
+    phi[1] = xi;
+    phi[2] = eta;
+    phi[3] = 0.0;
+  } else {
+    const double a0 = 0.5 * (1.0 - xi), a1 = 0.5 * (1.0 + xi);
+    const double b0 = 0.5 * (1.0 - eta), b1 = 0.5 * (1.0 + eta);
+    phi[0] = a0 * b0;
+    phi[1] = a1 * b0;
+    phi[2] = a0 * b1;
+    phi[3] = a1 * b1;
+  }
 }
 
-// Quad4 map (CGNS node order): Jacobian J[a][b] = ∂x_a/∂ξ_b
-__device__ __forceinline__ void quad4_jacobian(const double X[4][2], double xi, double eta,
-                                               double J[2][2]) {
+// cell map Jacobian J[a][b] = ∂x_a/∂ξ_b at a reference point (Quad4 bilinear,
+// Tri3 affine)
+__device__ __forceinline__ void jacobian_at(int ct, const double X[4][2], double xi, double eta,
+                                            double J[2][2]) {
+  if (ct) {
+    J[0][0] = X[1][0] - X[0][0];
+    J[0][1] = X[2][0] - X[0][0];
+    J[1][0] = X[1][1] - X[0][1];
+    J[1][1] = X[2][1] - X[0][1];
+    return;
+  }
   const double dx[4] = {-(1.0 - eta) * 0.25, (1.0 - eta) * 0.25, (1.0 + eta) * 0.25,
                         -(1.0 + eta) * 0.25};
   const double de[4] = {-(1.0 - xi) * 0.25, -(1.0 + xi) * 0.25, (1.0 + xi) * 0.25,
@@ -58,13 +72,29 @@ __device__ __forceinline__ void quad4_jacobian(const double X[4][2], double xi, 
   }
 }
 
-// reference point of side s at Gauss abscissa t (mapping_face, mapping.jl:324-336),
-// traversed from corner s to s+1, or backwards when flipped
-__device__ __forceinline__ void side_point(int s, int flip, double t, double& xi, double& eta) {
-  const double V[4][2] = {{-1.0, -1.0}, {1.0, -1.0}, {1.0, 1.0}, {-1.0, 1.0}};
-  const int a = flip ? (s + 1) & 3 : s, b = flip ? s : (s + 1) & 3;
-  xi = 0.5 * (1.0 - t) * V[a][0] + 0.5 * (1.0 + t) * V[b][0];
-  eta = 0.5 * (1.0 - t) * V[a][1] + 0.5 * (1.0 + t) * V[b][1];
+// reference point of side s at Gauss abscissa t (mapping_face, mapping.jl:324-336):
+// the side's corners (faces2nodes, shape.jl:253-254) from the first to the
+// second, or backwards when flipped
+__device__ __forceinline__ void side_point(int ct, int s, int flip, double t, double& xi,
+                                           double& eta) {
+  const double Vq[4][2] = {{-1.0, -1.0}, {1.0, -1.0}, {1.0, 1.0}, {-1.0, 1.0}};
+  const double Vt[3][2] = {{0.0, 0.0}, {1.0, 0.0}, {0.0, 1.0}};
+  const int ns = nsides_of(ct);
+  const int c0 = s, c1 = s + 1 == ns ? 0 : s + 1;
+  const int a = flip ? c1 : c0, b = flip ? c0 : c1;
+  const double* A = ct ? Vt[a] : Vq[a];
+  const double* B = ct ? Vt[b] : Vq[b];
+  xi = 0.5 * (1.0 - t) * A[0] + 0.5 * (1.0 + t) * B[0];
+  eta = 0.5 * (1.0 - t) * A[1] + 0.5 * (1.0 + t) * B[1];
+}
+
+// reference outward normal of side s (shape.jl:277-278)
+__device__ __forceinline__ void ref_normal(int ct, int s, double& nx, double& ny) {
+  const double nq[4][2] = {{0.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}, {-1.0, 0.0}};
+  const double r = 0.70710678118654752440;  // 1/√2
+  const double nt[3][2] = {{0.0, -1.0}, {r, r}, {-1.0, 0.0}};
+  nx = ct ? nt[s][0] : nq[s][0];
+  ny = ct ? nt[s][1] : nq[s][1];
 }
 
 __device__ __forceinline__ double pressure(double rho, double rux, double ruy, double rE,
@@ -127,37 +157,40 @@ __device__ void roe_flux(const double ui[5], const double uj[5], double nx, doub
 }
 
 __device__ __forceinline__ void load_cell(const double* __restrict__ coords,
-                                          const int32_t* __restrict__ c2n, int64_t c,
+                                          const int32_t* __restrict__ c2n, int64_t c, int nn,
                                           double X[4][2]) {
   for (int k = 0; k < 4; ++k) {
-    const int64_t n = c2n[4 * c + k];
+    const int64_t n = c2n[4 * c + (k < nn ? k : 0)];
     X[k][0] = coords[2 * n];
     X[k][1] = coords[2 * n + 1];
   }
 }
 
-// the five components at one point: w[k] = Σ_l φ_l u[maps[k][4c + l]]
+// the five components at one point: w[k] = Σ_{l < nl} φ_l u[maps[k][4c + l]]
 __device__ __forceinline__ void point_state(const double* __restrict__ u,
                                             const int32_t* __restrict__ maps, int64_t stride,
-                                            int64_t c, const double phi[4], double w[5]) {
+                                            int64_t c, int nl, const double phi[4], double w[5]) {
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     const int32_t* mk = maps + k * stride + 4 * c;
-    w[k] = phi[0] * u[mk[0]] + phi[1] * u[mk[1]] + phi[2] * u[mk[2]] + phi[3] * u[mk[3]];
+    double v = phi[0] * u[mk[0]] + phi[1] * u[mk[1]] + phi[2] * u[mk[2]];
+    if (nl == 4) v += phi[3] * u[mk[3]];
+    w[k] = v;
   }
 }
 
 // The Roe flux of every face point once: faces [nfaces][5] = (cell⁻, side⁻,
 // cell⁺, side⁺, flip), G[(f*nq1 + q)*5 + k] = w_q |face|/2 F_k.
 __global__ __launch_bounds__(256) void k_euler_faces(
-    const double* __restrict__ coords, const int32_t* __restrict__ c2n, int64_t ncells,
-    const int32_t* __restrict__ faces, int64_t nfaces, const int32_t* __restrict__ maps,
-    const double* __restrict__ gq, int nq1, double gamma, const double* __restrict__ u,
+    const double* __restrict__ coords, const int32_t* __restrict__ c2n,
+    const int8_t* __restrict__ ctype, int64_t ncells, const int32_t* __restrict__ faces,
+    int64_t nfaces, const int32_t* __restrict__ maps, EulerTabs T,
+    const double* __restrict__ tabs, double gamma, const double* __restrict__ u,
     double* __restrict__ G) {
   const int64_t stride = 4 * ncells;
-  const double* gx = gq;
-  const double* gw = gq + nq1;
-  const double nref[4][2] = {{0.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}, {-1.0, 0.0}};
+  const int nq1 = T.nq1;
+  const double* gx = tabs + T.off_fx;
+  const double* gw = tabs + T.off_fw;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nfaces * nq1;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t f = t / nq1;
@@ -165,26 +198,30 @@ __global__ __launch_bounds__(256) void k_euler_faces(
     const int32_t* Fc = faces + 5 * f;
     const int64_t cm = Fc[0], cp = Fc[2];
     const int sm = Fc[1], sp = Fc[3], flip = Fc[4];
+    const int ctm = ctype[cm], ctp = ctype[cp];
     double Xm[4][2];
-    load_cell(coords, c2n, cm, Xm);
-    const double ex = Xm[(sm + 1) & 3][0] - Xm[sm][0], ey = Xm[(sm + 1) & 3][1] - Xm[sm][1];
+    load_cell(coords, c2n, cm, ctm ? 3 : 4, Xm);
+    const int sm1 = sm + 1 == nsides_of(ctm) ? 0 : sm + 1;
+    const double ex = Xm[sm1][0] - Xm[sm][0], ey = Xm[sm1][1] - Xm[sm][1];
     const double dS = sqrt(ex * ex + ey * ey) / 2.0;
     double xm, em, xp, ep;
-    side_point(sm, 0, gx[q], xm, em);
-    side_point(sp, flip, gx[q], xp, ep);
-    double J[2][2], phm[4], php[4], dd[4][2];
-    quad4_jacobian(Xm, xm, em, J);
+    side_point(ctm, sm, 0, gx[q], xm, em);
+    side_point(ctp, sp, flip, gx[q], xp, ep);
+    double J[2][2], phm[4], php[4];
+    jacobian_at(ctm, Xm, xm, em, J);
     const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
     const double r = 1.0 / det;
     // J⁻ᵀ n_ref, normalised (mapping.jl:843-855)
-    const double a = nref[sm][0] * (J[1][1] * r) + nref[sm][1] * (-J[1][0] * r);
-    const double b = nref[sm][0] * (-J[0][1] * r) + nref[sm][1] * (J[0][0] * r);
+    double nrx, nry;
+    ref_normal(ctm, sm, nrx, nry);
+    const double a = nrx * (J[1][1] * r) + nry * (-J[1][0] * r);
+    const double b = nrx * (-J[0][1] * r) + nry * (J[0][0] * r);
     const double inv = 1.0 / sqrt(a * a + b * b);
-    q1_basis(xm, em, phm, dd);
-    q1_basis(xp, ep, php, dd);
+    basis_at(ctm, xm, em, phm);
+    basis_at(ctp, xp, ep, php);
     double wm[5], wp[5], F[5];
-    point_state(u, maps, stride, cm, phm, wm);
-    point_state(u, maps, stride, cp, php, wp);
+    point_state(u, maps, stride, cm, ctm ? 3 : 4, phm, wm);
+    point_state(u, maps, stride, cp, ctp ? 3 : 4, php, wp);
     roe_flux(wm, wp, a * inv, b * inv, gamma, F);
     const double wf = gw[q] * dS;
 #pragma unroll
@@ -192,55 +229,70 @@ __global__ __launch_bounds__(256) void k_euler_faces(
   }
 }
 
-// side table: [ncells][4][4] = (face, unused, role, flip); role 0: no face
-// term on this side, 1: this cell is the face's side⁻, 2: its side⁺
+// side table: [ncells][4][4] = (face, 0, role, flip); role 0: no face term on
+// this side, 1: this cell is the face's side⁻, 2: its side⁺.  Volume tables
+// per shape (T.off_*[shape]): weights, basis values [np][4], reference
+// gradients [np][4][2], geometric basis gradients [np][4][2].
 __global__ __launch_bounds__(256) void k_euler_dg(
-    const double* __restrict__ coords, const int32_t* __restrict__ c2n, int64_t ncells,
-    const int32_t* __restrict__ sides, const int32_t* __restrict__ maps,
-    const double* __restrict__ gq, int nq1, double gamma, int terms, double fcoef, int step,
-    double dt, const double* __restrict__ u, const double* __restrict__ G,
-    double* __restrict__ out) {
+    const double* __restrict__ coords, const int32_t* __restrict__ c2n,
+    const int8_t* __restrict__ ctype, int64_t ncells, const int32_t* __restrict__ sides,
+    const int32_t* __restrict__ maps, EulerTabs T, const double* __restrict__ tabs,
+    double gamma, int terms, double fcoef, int step, double dt, const double* __restrict__ u,
+    const double* __restrict__ G, double* __restrict__ out) {
   const int lc = threadIdx.x >> 3, g = threadIdx.x & 7;
   const int64_t c = (int64_t)blockIdx.x * kEulerCellsPerBlock + lc;
   const bool valid = c < ncells;
   const int64_t stride = 4 * ncells;
-  const double* gx = gq;
-  const double* gw = gq + nq1;
+  const int nq1 = T.nq1;
+  const double* gx = tabs + T.off_fx;
+  const int ct = valid ? ctype[c] : 0;
+  const int nl = ct ? 3 : 4;
   double acc[5][4];
 #pragma unroll
   for (int k = 0; k < 5; ++k)
 #pragma unroll
     for (int l = 0; l < 4; ++l) acc[k][l] = 0.0;
   if (valid && (terms & 1)) {
-    // volume points q = g, g + 8, ... of the nq1 x nq1 tensor rule
-    for (int q = g; q < nq1 * nq1; q += 8) {
+    // volume points q = g, g + 8, ... of the shape's rule
+    const double* w_ = tabs + T.off_w[ct];
+    const double* ph_ = tabs + T.off_phi[ct];
+    const double* dph_ = tabs + T.off_dphi[ct];
+    const double* dg_ = tabs + T.off_dg[ct];
+    for (int q = g; q < T.npts[ct]; q += 8) {
       double X[4][2];
-      load_cell(coords, c2n, c, X);
-      const int qi = q % nq1, qj = q / nq1;
-      const double xi = gx[qi], eta = gx[qj], w = gw[qi] * gw[qj];
-      double J[2][2], phi[4], dphi[4][2];
-      quad4_jacobian(X, xi, eta, J);
-      q1_basis(xi, eta, phi, dphi);
+      load_cell(coords, c2n, c, nl, X);
+      double J[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+      for (int k = 0; k < nl; ++k) {
+        const double d0 = dg_[(q * 4 + k) * 2], d1 = dg_[(q * 4 + k) * 2 + 1];
+        J[0][0] += X[k][0] * d0;
+        J[0][1] += X[k][0] * d1;
+        J[1][0] += X[k][1] * d0;
+        J[1][1] += X[k][1] * d1;
+      }
       const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
       const double r = 1.0 / det;
       const double Ji[2][2] = {{J[1][1] * r, -J[0][1] * r}, {-J[1][0] * r, J[0][0] * r}};
+      double phi[4];
+#pragma unroll
+      for (int l = 0; l < 4; ++l) phi[l] = ph_[q * 4 + l];
       double w5[5];
-      point_state(u, maps, stride, c, phi, w5);
+      point_state(u, maps, stride, c, nl, phi, w5);
       // _flux_Ω (covo.jl:44-58)
-      const double rho = w5[0], rux = w5[1], ruy = w5[2], rE = w5[3], ph = w5[4];
+      const double rho = w5[0], rux = w5[1], ruy = w5[2], rE = w5[3], pf = w5[4];
       const double vx = rux / rho, vy = ruy / rho;
       const double p = pressure(rho, rux, ruy, rE, gamma);
       const double F[5][2] = {{rux, ruy},
                               {rux * vx + p, rux * vy},
                               {ruy * vx, ruy * vy + p},
                               {(rE + p) * vx, (rE + p) * vy},
-                              {ph * vx, ph * vy}};
-      const double wd = w * fabs(det);
+                              {pf * vx, pf * vy}};
+      const double wd = w_[q] * fabs(det);
 #pragma unroll
       for (int l = 0; l < 4; ++l) {
         // ∇φ = J⁻ᵀ ∇̂φ
-        const double gxl = dphi[l][0] * Ji[0][0] + dphi[l][1] * Ji[1][0];
-        const double gyl = dphi[l][0] * Ji[0][1] + dphi[l][1] * Ji[1][1];
+        const double d0 = dph_[(q * 4 + l) * 2], d1 = dph_[(q * 4 + l) * 2 + 1];
+        const double gxl = d0 * Ji[0][0] + d1 * Ji[1][0];
+        const double gyl = d0 * Ji[0][1] + d1 * Ji[1][1];
 #pragma unroll
         for (int k = 0; k < 5; ++k) acc[k][l] += wd * (gxl * F[k][0] + gyl * F[k][1]);
       }
@@ -258,9 +310,9 @@ __global__ __launch_bounds__(256) void k_euler_dg(
       // fcoef ∫ F ⋅ jump(v), jump(v) = v⁻ - v⁺ (fcoef = -1 in covo.jl's l)
       const double sg = role == 1 ? fcoef : -fcoef;
       for (int q = g & 1; q < nq1; q += 2) {
-        double xi, eta, ph[4], dd[4][2];
-        side_point(s, flip, gx[q], xi, eta);
-        q1_basis(xi, eta, ph, dd);
+        double xi, eta, ph[4];
+        side_point(ct, s, flip, gx[q], xi, eta);
+        basis_at(ct, xi, eta, ph);
         const double* Gq = G + (f * nq1 + q) * 5;
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
@@ -286,32 +338,42 @@ __global__ __launch_bounds__(256) void k_euler_dg(
     for (int k = 0; k < 5; ++k)
 #pragma unroll
       for (int l = 0; l < 4; ++l)
-        if (((k * 4 + l) & 7) == g) out[maps[k * stride + 4 * c + l]] += acc[k][l];
+        if (((k * 4 + l) & 7) == g && l < nl) out[maps[k * stride + 4 * c + l]] += acc[k][l];
     return;
   }
+  if (g >= 5) return;
   // u + Δt M⁻¹ b, one component per lane
-  if (g < 5 && nq1 == 2) {
-    // 2x2 Gauss points, as many as basis functions: M = Φᵀ D Φ with Φ[q][l] =
-    // φ_l(ξ_q) square and D = diag(w_q |J(ξ_q)|), so M⁻¹ = Φ⁻¹ D⁻¹ Φ⁻ᵀ (Φ⁻¹
-    // row-major after the points and weights in gq)
-    const double* Pi = gq + 2 * nq1;
-    double X[4][2];
-    load_cell(coords, c2n, c, X);
-    double rhs[4], y[4];
+  double X[4][2];
+  load_cell(coords, c2n, c, nl, X);
+  double rhs[4];
 #pragma unroll
-    for (int k = 0; k < 5; ++k)
-      if (k == g)
+  for (int k = 0; k < 5; ++k)
+    if (k == g)
 #pragma unroll
-        for (int l = 0; l < 4; ++l) rhs[l] = acc[k][l];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int qi = q & 1, qj = q >> 1;
-      double J[2][2];
-      quad4_jacobian(X, gx[qi], gx[qj], J);
-      const double wd = gw[qi] * gw[qj] * fabs(J[0][0] * J[1][1] - J[0][1] * J[1][0]);
-      // (Φ⁻ᵀ r)_q = Σ_l Φ⁻¹[l][q] r_l
-      y[q] = (Pi[q] * rhs[0] + Pi[4 + q] * rhs[1] + Pi[8 + q] * rhs[2] + Pi[12 + q] * rhs[3]) / wd;
+      for (int l = 0; l < 4; ++l) rhs[l] = acc[k][l];
+  const double* w_ = tabs + T.off_w[ct];
+  const double* ph_ = tabs + T.off_phi[ct];
+  const double* dg_ = tabs + T.off_dg[ct];
+  auto wdet = [&](int q) {
+    double J[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int k = 0; k < nl; ++k) {
+      const double d0 = dg_[(q * 4 + k) * 2], d1 = dg_[(q * 4 + k) * 2 + 1];
+      J[0][0] += X[k][0] * d0;
+      J[0][1] += X[k][0] * d1;
+      J[1][0] += X[k][1] * d0;
+      J[1][1] += X[k][1] * d1;
     }
+    return w_[q] * fabs(J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+  };
+  if (ct == 0 && T.off_pinv >= 0) {
+    // as many points as basis functions (2x2 Gauss): M = Φᵀ D Φ with Φ[q][l]
+    // = φ_l(ξ_q) square and D = diag(w_q |J(ξ_q)|), so M⁻¹ = Φ⁻¹ D⁻¹ Φ⁻ᵀ
+    const double* Pi = tabs + T.off_pinv;
+    double y[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      y[q] = (Pi[q] * rhs[0] + Pi[4 + q] * rhs[1] + Pi[8 + q] * rhs[2] + Pi[12 + q] * rhs[3]) /
+             wdet(q);
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
       const double x = Pi[4 * l] * y[0] + Pi[4 * l + 1] * y[1] + Pi[4 * l + 2] * y[2] +
@@ -319,66 +381,55 @@ __global__ __launch_bounds__(256) void k_euler_dg(
       const int32_t id = maps[g * stride + 4 * c + l];
       out[id] = u[id] + dt * x;
     }
-  } else if (g < 5) {
-    // the cell's Q1 mass matrix (same rule), Gaussian elimination
-    double X[4][2];
-    load_cell(coords, c2n, c, X);
-    double M[4][4] = {}, rhs[4];
-    for (int q = 0; q < nq1 * nq1; ++q) {
-      const int qi = q % nq1, qj = q / nq1;
-      const double xi = gx[qi], eta = gx[qj], w = gw[qi] * gw[qj];
-      double J[2][2], phi[4], dphi[4][2];
-      quad4_jacobian(X, xi, eta, J);
-      q1_basis(xi, eta, phi, dphi);
-      const double wd = w * fabs(J[0][0] * J[1][1] - J[0][1] * J[1][0]);
-      for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) M[i][j] += wd * phi[i] * phi[j];
+    return;
+  }
+  // the cell's mass matrix (same rule), Gaussian elimination (SPD)
+  double M[4][4] = {};
+  for (int q = 0; q < T.npts[ct]; ++q) {
+    const double wd = wdet(q);
+    for (int i = 0; i < nl; ++i)
+      for (int j = 0; j < nl; ++j) M[i][j] += wd * ph_[q * 4 + i] * ph_[q * 4 + j];
+  }
+  for (int i = 0; i < nl; ++i) {
+    const double piv = 1.0 / M[i][i];
+    for (int r = i + 1; r < nl; ++r) {
+      const double fct = M[r][i] * piv;
+      for (int j = i; j < nl; ++j) M[r][j] -= fct * M[i][j];
+      rhs[r] -= fct * rhs[i];
     }
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-      if (k == g)
-#pragma unroll
-        for (int l = 0; l < 4; ++l) rhs[l] = acc[k][l];
-    // Gaussian elimination (M is symmetric positive definite)
-    for (int i = 0; i < 4; ++i) {
-      const double piv = 1.0 / M[i][i];
-      for (int r = i + 1; r < 4; ++r) {
-        const double f = M[r][i] * piv;
-        for (int j = i; j < 4; ++j) M[r][j] -= f * M[i][j];
-        rhs[r] -= f * rhs[i];
-      }
-    }
-    for (int i = 3; i >= 0; --i) {
-      double v = rhs[i];
-      for (int j = i + 1; j < 4; ++j) v -= M[i][j] * rhs[j];
-      rhs[i] = v / M[i][i];
-    }
-    for (int l = 0; l < 4; ++l) {
-      const int32_t id = maps[g * stride + 4 * c + l];
-      out[id] = u[id] + dt * rhs[l];
-    }
+  }
+  for (int i = nl - 1; i >= 0; --i) {
+    double v = rhs[i];
+    for (int j = i + 1; j < nl; ++j) v -= M[i][j] * rhs[j];
+    rhs[i] = v / M[i][i];
+  }
+  for (int l = 0; l < nl; ++l) {
+    const int32_t id = maps[g * stride + 4 * c + l];
+    out[id] = u[id] + dt * rhs[l];
   }
 }
 
 }  // namespace
 
 hipError_t launch_euler_dg(hipStream_t s, const double* coords, const int32_t* c2n,
-                           int64_t ncells, const int32_t* sides, const int32_t* faces,
-                           int64_t nfaces, const int32_t* maps, const double* gq, int nq1,
-                           double gamma, int terms, double fcoef, int step, double dt,
-                           const double* u, double* G, double* out) {
+                           const int8_t* ctype, int64_t ncells, const int32_t* sides,
+                           const int32_t* faces, int64_t nfaces, const int32_t* maps,
+                           const EulerTabs& T, const double* tabs, double gamma, int terms,
+                           double fcoef, int step, double dt, const double* u, double* G,
+                           double* out) {
   if (ncells <= 0) return hipSuccess;
-  if (nq1 < 1 || nq1 > kEulerMaxQ1) return hipErrorInvalidValue;
+  if (T.nq1 < 1 || T.nq1 > kEulerMaxQ1) return hipErrorInvalidValue;
   const int64_t nb = (ncells + kEulerCellsPerBlock - 1) / kEulerCellsPerBlock;
   if (nb > 0x7fffffff) return hipErrorInvalidValue;
   if ((terms & 2) && nfaces > 0) {
-    const int64_t n = nfaces * nq1;
+    const int64_t n = nfaces * T.nq1;
     const dim3 grid((unsigned)((n + 255) / 256 > 65536 ? 65536 : (n + 255) / 256));
-    hipLaunchKernelGGL(k_euler_faces, grid, dim3(256), 0, s, coords, c2n, ncells, faces, nfaces,
-                       maps, gq, nq1, gamma, u, G);
+    hipLaunchKernelGGL(k_euler_faces, grid, dim3(256), 0, s, coords, c2n, ctype, ncells, faces,
+                       nfaces, maps, T, tabs, gamma, u, G);
   }
   hipLaunchKernelGGL(k_euler_dg, dim3((unsigned)nb), dim3(8 * kEulerCellsPerBlock), 0, s, coords,
-                     c2n, ncells, sides, maps, gq, nq1, gamma, terms, fcoef, step, dt, u, G, out);
+                     c2n, ctype, ncells, sides, maps, T, tabs, gamma, terms, fcoef, step, dt, u, G,
+                     out);
   return hipGetLastError();
 }
 

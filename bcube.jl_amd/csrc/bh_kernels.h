@@ -119,18 +119,28 @@ hipError_t launch_dgb_bnd_linear(hipStream_t s, int nl, const double* coords, co
 hipError_t launch_dgb_points(hipStream_t s, const double* coords, const int32_t* c2n,
                              const int32_t* faces, int64_t nfaces, const double* tabs, int nl,
                              int nq, double* xq);
-// COVO Euler DG residual / explicit step on DG Q1 Quad4 cells (bh_euler.hip).
-// faces [nfaces][5] = (cell⁻, side⁻, cell⁺, side⁺, flip); sides
+// COVO Euler DG residual / explicit step on DG P1/Q1 Tri3/Quad4 cells
+// (bh_euler.hip).  c2n [ncells][4] (-1 pads triangles), ctype 0 Quad4 / 1
+// Tri3; faces [nfaces][5] = (cell⁻, side⁻, cell⁺, side⁺, flip); sides
 // [ncells][4][4] = (face, 0, role 0/1/2, flip); maps [5][ncells*4] global ids
-// of (component, cell, local dof); gq = nq1 Gauss points then weights; G
-// scratch of nfaces*nq1*5 doubles; terms bit 0 volume, bit 1 faces with
-// coefficient fcoef (-1 in covo.jl's residual); step: out = u + dt M⁻¹ b, else
-// out += b.
+// of (component, cell, local dof), -1 past a triangle's 3; tabs holds the face
+// rule (nq1 points, weights) and per shape the volume rule's weights, basis
+// [np][4], reference gradients [np][4][2], geometric gradients [np][4][2], and
+// Φ⁻¹ of a square Quad4 point-basis matrix (off_pinv < 0: none); G scratch of
+// nfaces*nq1*5 doubles; terms bit 0 volume, bit 1 faces with coefficient
+// fcoef (-1 in covo.jl's residual); step: out = u + dt M⁻¹ b, else out += b.
+struct EulerTabs {
+  int nq1 = 0, off_fx = 0, off_fw = 0;
+  int npts[2] = {0, 0}, off_w[2] = {0, 0}, off_phi[2] = {0, 0}, off_dphi[2] = {0, 0},
+      off_dg[2] = {0, 0};
+  int off_pinv = -1;
+};
 hipError_t launch_euler_dg(hipStream_t s, const double* coords, const int32_t* c2n,
-                           int64_t ncells, const int32_t* sides, const int32_t* faces,
-                           int64_t nfaces, const int32_t* maps, const double* gq, int nq1,
-                           double gamma, int terms, double fcoef, int step, double dt,
-                           const double* u, double* G, double* out);
+                           const int8_t* ctype, int64_t ncells, const int32_t* sides,
+                           const int32_t* faces, int64_t nfaces, const int32_t* maps,
+                           const EulerTabs& T, const double* tabs, double gamma, int terms,
+                           double fcoef, int step, double dt, const double* u, double* G,
+                           double* out);
 // out[map[g]] += v[g] for g < n (the targets of one field are distinct).
 hipError_t launch_add_field(hipStream_t s, double* out, const int32_t* map, const double* v,
                             int64_t n);

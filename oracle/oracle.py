@@ -571,17 +571,45 @@ def roe_flux(ui, uj, n, gamma):
     return np.array([flu11 + duv1, flu21 + duv2, flu31 + duv3, flu41 + duv4, fphi])
 
 
+_TRI_COORDS = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])              # shape.jl:207
+_TRI_F2N = [(0, 1), (1, 2), (2, 0)]                                        # shape.jl:253
+_TRI_NORMALS = np.array([[0.0, -1.0], [1.0, 1.0] / np.sqrt(2.0), [-1.0, 0.0]])  # shape.jl:277
+
+
+def _shape2d(mesh, c):
+    """(geometry name, shape name, corners, faces2nodes, reference normals) of cell c."""
+    if int(mesh.celltypes[c]) == CT["Tri3"]:
+        return "Tri3", "Triangle", _TRI_COORDS, _TRI_F2N, _TRI_NORMALS
+    return "Quad4", "Square", _SQ_COORDS, _SQ_F2N, _SQ_NORMALS
+
+
+def faces_2d(mesh: OMesh):
+    """_build_faces! (mesh.jl:328-374) on a Quad4/Tri3 mesh: [cell_n, side_n,
+    cell_p, side_p] in creation order, cell_p = -1 on the boundary."""
+    faces, index = [], {}
+    for c in range(len(mesh.celltypes)):
+        nodes = mesh.c2n[mesh.c2n_ptr[c]:mesh.c2n_ptr[c + 1]]
+        for s, (a, b) in enumerate(_shape2d(mesh, c)[3]):
+            key = tuple(sorted((int(nodes[a]), int(nodes[b]))))
+            if key in index:
+                faces[index[key]][2:] = [c, s]
+            else:
+                index[key] = len(faces)
+                faces.append([c, s, -1, -1])
+    return faces
+
+
 def tag_boundary_faces(mesh: OMesh, nodes_by_tag):
     """Boundary faces per tag, (cell, side) in face order: a face with one
     cell whose two nodes both lie in the tag's node set, first matching tag
     (_build_boundary_faces!, mesh.jl:383-423)."""
     out = {name: [] for name in nodes_by_tag}
     sets = {name: set(int(v) for v in nodes) for name, nodes in nodes_by_tag.items()}
-    for cn, sn, cp, _ in quad_faces(mesh):
+    for cn, sn, cp, _ in faces_2d(mesh):
         if cp >= 0:
             continue
         nodes = mesh.c2n[mesh.c2n_ptr[cn]:mesh.c2n_ptr[cn + 1]]
-        a, b = (int(nodes[k]) for k in _SQ_F2N[sn])
+        a, b = (int(nodes[k]) for k in _shape2d(mesh, cn)[3][sn])
         for name, s in sets.items():
             if a in s and b in s:
                 out[name].append((cn, sn))
@@ -603,7 +631,7 @@ def periodic_faces(mesh: OMesh, nodes_by_tag, shift, master, slave, tol=1e-9):
 
     def face_nodes(c, s):
         nodes = mesh.c2n[mesh.c2n_ptr[c]:mesh.c2n_ptr[c + 1]]
-        return [int(nodes[k]) for k in _SQ_F2N[s]]
+        return [int(nodes[k]) for k in _shape2d(mesh, c)[3][s]]
 
     out = []
     for ci, si in f2:
@@ -623,47 +651,66 @@ def periodic_faces(mesh: OMesh, nodes_by_tag, shift, master, slave, tol=1e-9):
     return out
 
 
+def _side_eval2(mesh, c, xi_cell, side):
+    """Basis values (padded to 4), unit normal of `side` and the physical point
+    of cell c (Quad4 or Tri3) at reference point xi_cell."""
+    geo, shp, _, _, nref = _shape2d(mesh, c)
+    nodes = mesh.c2n[mesh.c2n_ptr[c]:mesh.c2n_ptr[c + 1]]
+    X = mesh.coords.reshape(-1, 2)[nodes]
+    J, Ji, det, x = geometry(geo, X, xi_cell)
+    val, _ = shape_functions(shp, 1, xi_cell)
+    n = nref[side] @ Ji
+    return np.pad(val, (0, 4 - len(val))), n / np.linalg.norm(n), x
+
+
 def euler_dg_residual(mesh: OMesh, U, gamma, degree, periodic=(), terms=("vol", "face")):
     """b = ∫ flux_Ω(u, v) dΩ - Σ_domains ∫ flux_Γ(u, v, n) dΓ of covo.jl:14-33
-    for DG Q1 on Quad4 cells: U and the result are (5, ncells, 4).  Interior
-    faces come from _build_faces!, `periodic` lists [cell⁻, side⁻, cell⁺,
-    side⁺, flip] faces (periodic_faces)."""
+    for DG P1/Q1 on Tri3/Quad4 cells: U and the result are (5, ncells, 4)
+    (the 4th entry unused on triangles).  Interior faces come from
+    _build_faces!, `periodic` lists [cell⁻, side⁻, cell⁺, side⁺, flip] faces
+    (periodic_faces)."""
     nc = len(mesh.celltypes)
     X = mesh.coords.reshape(-1, 2)
     b = np.zeros((5, nc, 4))
     if "vol" in terms:
-        wv, xv = quadrature("Square", degree)
-        xv = xv.reshape(-1, 2)[:len(wv)]
         for c in range(nc):
+            geo, shp = _shape2d(mesh, c)[:2]
+            wv, xv = quadrature(shp, degree)
+            xv = xv.reshape(-1, 2)[:len(wv)]
             nodes = mesh.c2n[mesh.c2n_ptr[c]:mesh.c2n_ptr[c + 1]]
+            nl = 3 if geo == "Tri3" else 4
             for w, xi in zip(wv, xv):
-                J, Ji, det, _ = geometry("Quad4", X[nodes], xi)
-                val, gref = shape_functions("Square", 1, xi)
-                grad = gref.reshape(4, 2) @ Ji
-                wq = U[:, c, :] @ val
+                J, Ji, det, _ = geometry(geo, X[nodes], xi)
+                val, gref = shape_functions(shp, 1, xi)
+                grad = gref.reshape(nl, 2) @ Ji
+                wq = U[:, c, :nl] @ val
                 F = euler_volume_flux(wq, gamma)
-                b[:, c, :] += w * abs(det) * (F @ grad.T)
+                b[:, c, :nl] += w * abs(det) * (F @ grad.T)
     if "face" in terms:
         wq_, xq_ = quadrature("Line", degree)
         xq_ = xq_.reshape(-1)[:len(wq_)]
-        faces = [[cn, sn, cp, sp, None] for cn, sn, cp, sp in quad_faces(mesh) if cp >= 0]
+        faces = [[cn, sn, cp, sp, None] for cn, sn, cp, sp in faces_2d(mesh) if cp >= 0]
         faces += [list(f) for f in periodic]
         for cn, sn, cp, sp, flip in faces:
             nodes_n = mesh.c2n[mesh.c2n_ptr[cn]:mesh.c2n_ptr[cn + 1]]
-            fa, fb = _SQ_F2N[sn]
+            f2n_n, f2n_p = _shape2d(mesh, cn)[3], _shape2d(mesh, cp)[3]
+            fa, fb = f2n_n[sn]
             dS = np.linalg.norm(X[nodes_n[fb]] - X[nodes_n[fa]]) / 2
             if flip is None:   # interior face: indexin(f2n_p, f2n_n)
                 nodes_p = mesh.c2n[mesh.c2n_ptr[cp]:mesh.c2n_ptr[cp + 1]]
-                gn = [int(nodes_n[k]) for k in _SQ_F2N[sn]]
-                gp = [int(nodes_p[k]) for k in _SQ_F2N[sp]]
+                gn = [int(nodes_n[k]) for k in f2n_n[sn]]
+                gp = [int(nodes_p[k]) for k in f2n_p[sp]]
                 perm = [gn.index(v) for v in gp]
-                loc_p = tuple(_SQ_F2N[sp][k] for k in perm)
+                loc_p = tuple(f2n_p[sp][k] for k in perm)
             else:
-                a, bb = _SQ_F2N[sp]
+                a, bb = f2n_p[sp]
                 loc_p = (bb, a) if flip else (a, bb)
-            for w, xi in zip(wq_, xq_):
-                vn, _, nn, _ = _side_eval(mesh, cn, 1, _face_to_cell(_SQ_F2N[sn], xi), sn)
-                vp = shape_functions("Square", 1, _face_to_cell(loc_p, xi))[0]
+            Vn, Vp = _shape2d(mesh, cn)[2], _shape2d(mesh, cp)[2]
+            for w, t in zip(wq_, xq_):
+                xn = 0.5 * (1.0 - t) * Vn[f2n_n[sn][0]] + 0.5 * (1.0 + t) * Vn[f2n_n[sn][1]]
+                xp = 0.5 * (1.0 - t) * Vp[loc_p[0]] + 0.5 * (1.0 + t) * Vp[loc_p[1]]
+                vn, nn, _ = _side_eval2(mesh, cn, xn, sn)
+                vp, _, _ = _side_eval2(mesh, cp, xp, sp)
                 F = roe_flux(U[:, cn, :] @ vn, U[:, cp, :] @ vp, nn, gamma)
                 b[:, cn, :] -= w * dS * np.outer(F, vn)
                 b[:, cp, :] += w * dS * np.outer(F, vp)
@@ -671,19 +718,21 @@ def euler_dg_residual(mesh: OMesh, U, gamma, degree, periodic=(), terms=("vol", 
 
 
 def euler_dg_mass_inverse_apply(mesh: OMesh, R, degree):
-    """M⁻¹ R with M the DG Q1 mass matrix (block-diagonal, one 4x4 block per
+    """M⁻¹ R with M the DG P1/Q1 mass matrix (block-diagonal, one block per
     cell and component): the `cache.mass \\ rhs` of covo.jl:31."""
     nc = len(mesh.celltypes)
     X = mesh.coords.reshape(-1, 2)
-    wv, xv = quadrature("Square", degree)
-    xv = xv.reshape(-1, 2)[:len(wv)]
     out = np.zeros_like(R)
     for c in range(nc):
+        geo, shp = _shape2d(mesh, c)[:2]
+        nl = 3 if geo == "Tri3" else 4
+        wv, xv = quadrature(shp, degree)
+        xv = xv.reshape(-1, 2)[:len(wv)]
         nodes = mesh.c2n[mesh.c2n_ptr[c]:mesh.c2n_ptr[c + 1]]
-        M = np.zeros((4, 4))
+        M = np.zeros((nl, nl))
         for w, xi in zip(wv, xv):
-            _, _, det, _ = geometry("Quad4", X[nodes], xi)
-            val, _ = shape_functions("Square", 1, xi)
+            _, _, det, _ = geometry(geo, X[nodes], xi)
+            val, _ = shape_functions(shp, 1, xi)
             M += w * abs(det) * np.outer(val, val)
-        out[:, c, :] = np.linalg.solve(M, R[:, c, :].T).T
+        out[:, c, :nl] = np.linalg.solve(M, R[:, c, :nl].T).T
     return out

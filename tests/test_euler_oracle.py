@@ -77,7 +77,7 @@ def test_roe_flux_is_the_textbook_roe_flux():
         assert F[4] == pytest.approx(v12 * (uL[4] if v12 > 0 else uR[4]), rel=1e-14)
 
 
-def _periodic_mesh(n, jitter=0.0, seed=0, rotate=False, l=0.05):
+def _periodic_mesh(n, jitter=0.0, seed=0, rotate=False, l=0.05, split=False):
     t = np.linspace(-l, l, n + 1)
     X = np.array([[x, y] for y in t for x in t])
     if jitter:
@@ -89,12 +89,16 @@ def _periodic_mesh(n, jitter=0.0, seed=0, rotate=False, l=0.05):
         for i in range(n):
             a = j * (n + 1) + i
             c = [a, a + 1, a + n + 2, a + n + 1]
+            if split and (i + 2 * j) % 3 == 0:   # two triangles, counter-clockwise
+                cells += [[a, a + 1, a + n + 2], [a, a + n + 2, a + n + 1]]
+                continue
             if rotate and (i + j) % 2:
                 k = (i + 2 * j) % 3 + 1
                 c = c[k:] + c[:k]
             cells.append(c)
-    m = O.OMesh(2, X.reshape(-1), np.zeros(n * n, dtype=np.int32),
-                np.arange(n * n + 1, dtype=np.int64) * 4, np.array(cells, dtype=np.int64).reshape(-1))
+    ct = np.array([O.CT["Tri3"] if len(c) == 3 else O.CT["Quad4"] for c in cells], dtype=np.int32)
+    ptr = np.concatenate([[0], np.cumsum([len(c) for c in cells])]).astype(np.int64)
+    m = O.OMesh(2, X.reshape(-1), ct, ptr, np.concatenate([np.array(c) for c in cells]).astype(np.int64))
     gx, gy = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="xy")
     gx, gy = gx.ravel(), gy.ravel()
     tags = {"West": np.flatnonzero(gx == 0), "East": np.flatnonzero(gx == n),
@@ -104,13 +108,15 @@ def _periodic_mesh(n, jitter=0.0, seed=0, rotate=False, l=0.05):
     return m, per
 
 
-@pytest.mark.parametrize("jitter,rotate", [(0.0, False), (0.2, True)])
-def test_free_stream_leaves_zero_residual(jitter, rotate):
+@pytest.mark.parametrize("jitter,rotate,split", [(0.0, False, False), (0.2, True, False),
+                                                 (0.15, True, True)])
+def test_free_stream_leaves_zero_residual(jitter, rotate, split):
     n = 5
-    m, per = _periodic_mesh(n, jitter, seed=1, rotate=rotate)
+    m, per = _periodic_mesh(n, jitter, seed=1, rotate=rotate, split=split)
+    nc = len(m.celltypes)
     assert len(per) == 2 * n
     w = _state(np.random.default_rng(3))[:, 0]
-    U = np.broadcast_to(w[:, None, None], (5, n * n, 4)).copy()
+    U = np.broadcast_to(w[:, None, None], (5, nc, 4)).copy()
     vol = O.euler_dg_residual(m, U, GAMMA, 3, per, terms=("vol",))
     b = O.euler_dg_residual(m, U, GAMMA, 3, per)
     assert np.abs(vol).max() > 1e-3
@@ -120,11 +126,13 @@ def test_free_stream_leaves_zero_residual(jitter, rotate):
     assert np.abs(b2).max() > 1e-3 * np.abs(vol).max()
 
 
-def test_conservation_on_a_periodic_mesh():
+@pytest.mark.parametrize("split", [False, True])
+def test_conservation_on_a_periodic_mesh(split):
     n = 4
-    m, per = _periodic_mesh(n, 0.15, seed=2, rotate=True)
-    U = _state(np.random.default_rng(4), 5 * n * n * 4 // 5)  # (5, n*n*4)
-    U = U.reshape(5, n * n, 4)
+    m, per = _periodic_mesh(n, 0.15, seed=2, rotate=True, split=split)
+    nc = len(m.celltypes)
+    U = _state(np.random.default_rng(4), nc * 4).reshape(5, nc, 4)
+    U[:, m.celltypes == O.CT["Tri3"], 3] = 0.0   # the unused slot of triangles
     b = O.euler_dg_residual(m, U, GAMMA, 3, per)
     face = O.euler_dg_residual(m, U, GAMMA, 3, per, terms=("face",))
     sums = b.sum(axis=(1, 2))

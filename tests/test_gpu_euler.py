@@ -244,3 +244,98 @@ def test_euler_residual_soa_numbering():
     err = np.abs(b[maps].reshape(5, n * n, 4) - ref).max() / np.abs(ref).max()
     print(f"SoA numbering: rel err {err:.2e}")
     assert err <= RTOL
+
+
+def _mixed_meshes(n, jitter, seed):
+    """n x n cells of [-l, l]², every third cell split into two triangles."""
+    t = np.linspace(-L_HALF, L_HALF, n + 1)
+    X = np.array([[x, y] for y in t for x in t])
+    rng = np.random.default_rng(seed)
+    inner = np.all((np.abs(X + L_HALF) > 1e-12) & (np.abs(X - L_HALF) > 1e-12), axis=1)
+    X[inner] += rng.uniform(-jitter, jitter, (inner.sum(), 2)) * 2 * L_HALF / n
+    cells = []
+    for j in range(n):
+        for i in range(n):
+            a = j * (n + 1) + i
+            if (i + 2 * j) % 3 == 0:
+                cells += [[a, a + 1, a + n + 2], [a, a + n + 2, a + n + 1]]
+            else:
+                cells.append([a, a + 1, a + n + 2, a + n + 1])
+    types = ["Tri3" if len(c) == 3 else "Quad4" for c in cells]
+    m = B.Mesh(X, types, cells)
+    gx, gy = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="xy")
+    gx, gy = gx.ravel(), gy.ravel()
+    m.bc_nodes = {"West": np.flatnonzero(gx == 0), "East": np.flatnonzero(gx == n),
+                  "South": np.flatnonzero(gy == 0), "North": np.flatnonzero(gy == n)}
+    om = O.OMesh(2, X.reshape(-1), m.celltypes, m.c2n_ptr, m.c2n)
+    return m, om
+
+
+def _to_global_mixed(U, Uo, m):
+    """u[maps[k][dof(c, l)]] = Uo[k, c, l] for the l < 3 or 4 dofs of each cell."""
+    from bcubehip.assembler import Fields
+    f = Fields(U)
+    dofs = _dofs(f.scalar)
+    nl = dofs.size // m.ncells
+    u = np.zeros(f.ndofs)
+    idx = np.full((5, m.ncells, 4), -1, dtype=np.int64)
+    for c in range(m.ncells):
+        for l in range(3 if m.celltypes[c] == 2 else 4):
+            g = dofs[c * nl + l]
+            idx[:, c, l] = f.maps[:, g]
+            u[f.maps[:, g]] = Uo[:, c, l]
+    return u, idx
+
+
+def _dofs(space):
+    import ctypes as C
+    from bcubehip import _lib as L
+    nl, nd = C.c_int32(), C.c_int64()
+    L.check(L.lib().bh_space_info(space.handle, C.byref(nd), C.byref(nl)))
+    out = np.zeros(space.mesh.ncells * nl.value, dtype=np.int32)
+    L.check(L.lib().bh_space_dofs(space.handle, out.ctypes.data))
+    return out
+
+
+def test_euler_on_a_mixed_tri_quad_mesh():
+    """The residual and the step on Quad4 + Tri3 cells (COVO's MeshConfig
+    triquad): P1 triangles with the degree-3 triangle rule, against the
+    oracle; free stream on the same mesh."""
+    n = 6
+    m, om = _mixed_meshes(n, 0.15, seed=13)
+    assert (m.celltypes == 2).any() and (m.celltypes == 0).any()
+    U, V = _setup(m)
+    nc = m.ncells
+    Uo = _state(nc, seed=14)
+    Uo[:, m.celltypes == 2, 3] = 0.0
+    u, idx = _to_global_mixed(U, Uo, m)
+    forms, (dΓx, dΓy) = _forms(m, U, u)
+    b = B.assemble_linear(forms["l"], V)
+    per = [list(r) for r in np.concatenate([dΓx.domain.periodic_faces(),
+                                            dΓy.domain.periodic_faces()])]
+    ref_per = (O.periodic_faces(om, m.bc_nodes, np.array([-2 * L_HALF, 0.0]), ("East",), ("West",))
+               + O.periodic_faces(om, m.bc_nodes, np.array([0.0, 2 * L_HALF]), ("South",),
+                                  ("North",)))
+    assert per == [list(r) for r in ref_per]
+    ref = O.euler_dg_residual(om, Uo, GAMMA, 3, per)
+    mask = idx >= 0
+    got = np.zeros_like(ref)
+    got[mask] = b[idx[mask]]
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"mixed mesh residual: rel err {err:.2e}", flush=True)
+    assert err <= RTOL
+    plan = B.EulerDG(V, GAMMA, 3, (dΓx.domain, dΓy.domain))
+    dt = 1e-4
+    out = plan.step(u, dt)
+    refs = Uo + dt * O.euler_dg_mass_inverse_apply(om, ref, 3)
+    gs = np.zeros_like(refs)
+    gs[mask] = out[idx[mask]]
+    inc = np.abs((gs - Uo)[mask] - (refs - Uo)[mask]).max() / np.abs((refs - Uo)[mask]).max()
+    print(f"mixed mesh step: increment rel err {inc:.2e}")
+    assert inc <= 1e-11
+    w = np.array([1.1, 0.3, -0.2, 2.9, 0.4])
+    uf, _ = _to_global_mixed(U, np.broadcast_to(w[:, None, None], (5, nc, 4)).copy(), m)
+    ff, _ = _forms(m, U, uf)
+    bf = B.assemble_linear(ff["l"], V)
+    vol = B.assemble_linear(ff["l_vol"], V)
+    assert np.abs(bf).max() <= 1e-12 * np.abs(vol).max()

@@ -206,6 +206,7 @@ struct bh_euler_s {
   int32_t *d_sides = nullptr, *d_faces = nullptr, *d_maps = nullptr, *d_c2n = nullptr;
   double *d_gq = nullptr, *d_G = nullptr;
   int8_t* d_ctype = nullptr;
+  int shapes = 0;  // 0 all Quad4, 1 all Tri3, 2 mixed
   EulerTabs tabs;
   double *d_u = nullptr, *d_out = nullptr;  // staging of host-`where` calls
   ~bh_euler_s() {
@@ -1795,6 +1796,11 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
   e->ntot = ndofs_total;
   e->nfaces = (int64_t)faces.size() / 5;
   e->tabs = T;
+  {
+    bool anyq = false, anyt = false;
+    for (int8_t t : ctype) (t ? anyt : anyq) = true;
+    e->shapes = anyq && anyt ? 2 : (anyt ? 1 : 0);
+  }
   if (int r = set_device(c)) return r;
   auto up = [&](auto** dst, const auto& v) -> int {
     BH_HIP(c, dalloc(dst, std::max<size_t>(v.size(), 1)));
@@ -1836,9 +1842,9 @@ static int euler_run(bh_euler e, int terms, double fcoef, int step, double dt, c
     dout = e->d_out;
   }
   time_begin(c, 2);
-  BH_HIP(c, launch_euler_dg(c->stream, e->V->mesh->d_coords, e->d_c2n, e->d_ctype, e->ncells,
-                            e->d_sides, e->d_faces, e->nfaces, e->d_maps, e->tabs, e->d_gq,
-                            e->gamma, terms, fcoef, step, dt, du, e->d_G, dout));
+  BH_HIP(c, launch_euler_dg(c->stream, e->V->mesh->d_coords, e->d_c2n, e->d_ctype, e->shapes,
+                            e->ncells, e->d_sides, e->d_faces, e->nfaces, e->d_maps, e->tabs,
+                            e->d_gq, e->gamma, terms, fcoef, step, dt, du, e->d_G, dout));
   time_end(c);
   if (where == BH_HOST) {
     BH_HIP(c, hipMemcpyAsync(o, dout, bytes, hipMemcpyDeviceToHost, c->stream));

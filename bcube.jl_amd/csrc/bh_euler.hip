@@ -181,6 +181,9 @@ __device__ __forceinline__ void point_state(const double* __restrict__ u,
 
 // The Roe flux of every face point once: faces [nfaces][5] = (cell⁻, side⁻,
 // cell⁺, side⁺, flip), G[(f*nq1 + q)*5 + k] = w_q |face|/2 F_k.
+// SHAPES: 0 all Quad4, 1 all Tri3, 2 mixed (the cell types read per cell);
+// uniform meshes fold every shape branch at compile time
+template <int SHAPES>
 __global__ __launch_bounds__(256) void k_euler_faces(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
     const int8_t* __restrict__ ctype, int64_t ncells, const int32_t* __restrict__ faces,
@@ -198,7 +201,7 @@ __global__ __launch_bounds__(256) void k_euler_faces(
     const int32_t* Fc = faces + 5 * f;
     const int64_t cm = Fc[0], cp = Fc[2];
     const int sm = Fc[1], sp = Fc[3], flip = Fc[4];
-    const int ctm = ctype[cm], ctp = ctype[cp];
+    const int ctm = SHAPES == 2 ? ctype[cm] : SHAPES, ctp = SHAPES == 2 ? ctype[cp] : SHAPES;
     double Xm[4][2];
     load_cell(coords, c2n, cm, ctm ? 3 : 4, Xm);
     const int sm1 = sm + 1 == nsides_of(ctm) ? 0 : sm + 1;
@@ -233,6 +236,7 @@ __global__ __launch_bounds__(256) void k_euler_faces(
 // this side, 1: this cell is the face's side⁻, 2: its side⁺.  Volume tables
 // per shape (T.off_*[shape]): weights, basis values [np][4], reference
 // gradients [np][4][2], geometric basis gradients [np][4][2].
+template <int SHAPES>
 __global__ __launch_bounds__(256) void k_euler_dg(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
     const int8_t* __restrict__ ctype, int64_t ncells, const int32_t* __restrict__ sides,
@@ -245,7 +249,7 @@ __global__ __launch_bounds__(256) void k_euler_dg(
   const int64_t stride = 4 * ncells;
   const int nq1 = T.nq1;
   const double* gx = tabs + T.off_fx;
-  const int ct = valid ? ctype[c] : 0;
+  const int ct = SHAPES == 2 ? (valid ? ctype[c] : 0) : SHAPES;
   const int nl = ct ? 3 : 4;
   double acc[5][4];
 #pragma unroll
@@ -412,24 +416,35 @@ __global__ __launch_bounds__(256) void k_euler_dg(
 }  // namespace
 
 hipError_t launch_euler_dg(hipStream_t s, const double* coords, const int32_t* c2n,
-                           const int8_t* ctype, int64_t ncells, const int32_t* sides,
+                           const int8_t* ctype, int shapes, int64_t ncells, const int32_t* sides,
                            const int32_t* faces, int64_t nfaces, const int32_t* maps,
                            const EulerTabs& T, const double* tabs, double gamma, int terms,
                            double fcoef, int step, double dt, const double* u, double* G,
                            double* out) {
   if (ncells <= 0) return hipSuccess;
+  if (shapes < 0 || shapes > 2) return hipErrorInvalidValue;
   if (T.nq1 < 1 || T.nq1 > kEulerMaxQ1) return hipErrorInvalidValue;
   const int64_t nb = (ncells + kEulerCellsPerBlock - 1) / kEulerCellsPerBlock;
   if (nb > 0x7fffffff) return hipErrorInvalidValue;
   if ((terms & 2) && nfaces > 0) {
     const int64_t n = nfaces * T.nq1;
     const dim3 grid((unsigned)((n + 255) / 256 > 65536 ? 65536 : (n + 255) / 256));
-    hipLaunchKernelGGL(k_euler_faces, grid, dim3(256), 0, s, coords, c2n, ctype, ncells, faces,
-                       nfaces, maps, T, tabs, gamma, u, G);
+#define BH_EF(SH)                                                                               \
+  hipLaunchKernelGGL(k_euler_faces<SH>, grid, dim3(256), 0, s, coords, c2n, ctype, ncells, faces, \
+                     nfaces, maps, T, tabs, gamma, u, G)
+    if (shapes == 0) BH_EF(0);
+    else if (shapes == 1) BH_EF(1);
+    else BH_EF(2);
+#undef BH_EF
   }
-  hipLaunchKernelGGL(k_euler_dg, dim3((unsigned)nb), dim3(8 * kEulerCellsPerBlock), 0, s, coords,
-                     c2n, ctype, ncells, sides, maps, T, tabs, gamma, terms, fcoef, step, dt, u, G,
-                     out);
+#define BH_EC(SH)                                                                               \
+  hipLaunchKernelGGL(k_euler_dg<SH>, dim3((unsigned)nb), dim3(8 * kEulerCellsPerBlock), 0, s,      \
+                     coords, c2n, ctype, ncells, sides, maps, T, tabs, gamma, terms, fcoef, step, \
+                     dt, u, G, out)
+  if (shapes == 0) BH_EC(0);
+  else if (shapes == 1) BH_EC(1);
+  else BH_EC(2);
+#undef BH_EC
   return hipGetLastError();
 }
 

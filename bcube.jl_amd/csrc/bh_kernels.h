@@ -121,7 +121,7 @@ hipError_t launch_dgb_points(hipStream_t s, const double* coords, const int32_t*
                              int nq, double* xq);
 // COVO Euler DG residual / explicit step on DG P1/Q1 Tri3/Quad4 cells
 // (bh_euler.hip).  c2n [ncells][4] (-1 pads triangles), ctype 0 Quad4 / 1
-// Tri3; faces [nfaces][5] = (cell⁻, side⁻, cell⁺, side⁺, flip); sides
+// Tri3 per cell, shapes 0 (all Quad4), 1 (all Tri3) or 2 (mixed); faces [nfaces][5] = (cell⁻, side⁻, cell⁺, side⁺, flip); sides
 // [ncells][4][4] = (face, 0, role 0/1/2, flip); maps [5][ncells*4] global ids
 // of (component, cell, local dof), -1 past a triangle's 3; tabs holds the face
 // rule (nq1 points, weights) and per shape the volume rule's weights, basis
@@ -136,7 +136,7 @@ struct EulerTabs {
   int off_pinv = -1;
 };
 hipError_t launch_euler_dg(hipStream_t s, const double* coords, const int32_t* c2n,
-                           const int8_t* ctype, int64_t ncells, const int32_t* sides,
+                           const int8_t* ctype, int shapes, int64_t ncells, const int32_t* sides,
                            const int32_t* faces, int64_t nfaces, const int32_t* maps,
                            const EulerTabs& T, const double* tabs, double gamma, int terms,
                            double fcoef, int step, double dt, const double* u, double* G,

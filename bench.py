@@ -1034,6 +1034,54 @@ def bench_covo(args, torch, B):
     torch.cuda.synchronize()
     dev_ms["explicit_step_in_graph"] = e0.elapsed_time(e1) / (20 * 20)
     m.backend.set_stream(stream.cuda_stream)
+    # the same suite on the reference's second mesh (MeshConfig=triquad,
+    # covo.jl:235-245): its gmsh asset, quads and triangles
+    tri_quad = None
+    asset = os.path.join(ROOT, "tests", "golden",
+                         "rectangle-mesh-tri-quad-nx129-ny129-lx1e-1-ly1e-1.msh22.gz")
+    if os.path.exists(asset):
+        import gzip
+        import tempfile
+        with tempfile.NamedTemporaryFile(suffix=".msh22", delete=False) as tmp:
+            tmp.write(gzip.open(asset).read())
+        mt = B.read_mesh(tmp.name, warn=False)
+        os.unlink(tmp.name)
+        fs = B.FunctionSpace("Lagrange", 1)
+        Ut_s = B.TrialFESpace(fs, mt, "discontinuous", size=1)
+        Ut_v = B.TrialFESpace(fs, mt, "discontinuous", size=2)
+        Ut = B.MultiFESpace(Ut_s, Ut_v, Ut_s, Ut_s)
+        Vt = B.MultiFESpace(B.TestFESpace(Ut_s), B.TestFESpace(Ut_v), B.TestFESpace(Ut_s),
+                            B.TestFESpace(Ut_s))
+        pxt = B.PeriodicBCType(B.Translation([-2 * l, 0.0]), ("East",), ("West",))
+        pyt = B.PeriodicBCType(B.Translation([0.0, 2 * l]), ("South",), ("North",))
+        plant = B.EulerDG(Vt, gamma, degree, (B.BoundaryFaceDomain(mt, pxt),
+                                              B.BoundaryFaceDomain(mt, pyt)))
+        mt.backend.set_stream(stream.cuda_stream)
+        rng = np.random.default_rng(0)
+        nd = plant.ndofs
+        ut = np.zeros(nd)
+        f = plant.fields
+        ut[f.maps[0]] = rng.uniform(0.9, 1.1, f.maps.shape[1])
+        ut[f.maps[3]] = rng.uniform(2.4, 2.6, f.maps.shape[1])
+        ut[f.maps[4]] = rng.uniform(0.0, 1.0, f.maps.shape[1])
+        dut = torch.from_numpy(ut).to(dev)
+        dbt = torch.zeros_like(dut)
+        tq = {}
+        for name, run in (("integral_volume", lambda: plant.residual_device(dut.data_ptr(), dbt.data_ptr(), 1, -1.0)),
+                          ("integral_surface", lambda: plant.residual_device(dut.data_ptr(), dbt.data_ptr(), 2, 1.0)),
+                          ("explicit_step", lambda: plant.step_device(dut.data_ptr(), dbt.data_ptr(), dt))):
+            for _ in range(5):
+                run()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(200):
+                run()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            tq[name] = e0.elapsed_time(e1) / 200
+        tri_quad = {"ncells": mt.ncells, "triangles": int((mt.celltypes == 2).sum()),
+                    "ndofs": nd, "device_resident_ms": tq}
     # 2. throughput: explicit steps on n x n, state resident in HBM
     n = args.n if args.n != 64 else 1024
     mb, Ub, Vb, ub_vals, mapsb, _, perb = problem(n)
@@ -1096,6 +1144,7 @@ def bench_covo(args, torch, B):
                      "kernel": "k_euler_faces + k_euler_dg",
                      "avg_launch_ms": kms, "alg_bytes_per_launch": alg},
         "reference_suite_128_ms": {"host_api": suite, "device_resident": dev_ms},
+        "reference_suite_triquad_mesh": tri_quad,
     }
     if not args.no_cpu:
         from oracle import oracle as O

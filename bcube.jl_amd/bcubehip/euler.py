@@ -1,7 +1,8 @@
 """The Euler DG residual of the reference's COVO benchmark (benchmark/covo.jl,
 driver_bench_covo.jl) on the HIP backend.
 
-    u  = FEFunction(U), U = MultiFESpace(U_sca, U_vec, U_sca, U_sca)  (DG Q1)
+    u  = FEFunction(U), U = MultiFESpace(U_sca, U_vec, U_sca, U_sca)  (DG order 1
+         on Quad4, Tri3 or mixed cells)
     l(v) = ∫(flux_Ω(u, v))dΩ - ∫(flux_Γ(u, v, nΓ))dΓ
            - ∫(flux_Γ(u, v, nx))dΓ_perio_x - ∫(flux_Γ(u, v, ny))dΓ_perio_y
     flux_Ω(u, v)    = compose(EulerFlux(γ), u, tuple(map(grad, v)))

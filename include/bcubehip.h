@@ -343,11 +343,11 @@ int bh_dgb_destroy(bh_dgb d);
 /* ---- the COVO Euler DG residual ------------------------------------------- */
 /* assemble_linear(l, V) (assembler.jl:448-520) of the reference's COVO
  * benchmark (benchmark/covo.jl:14-150, driver_bench_covo.jl): V a
- * MultiFESpace (ρ, ρu, ρE, ϕ) of DG Q1 spaces on Quad4 cells, and
+ * MultiFESpace (ρ, ρu, ρE, ϕ) of discontinuous order-1 spaces on Quad4, Tri3 or mixed cells, and
  *   l(v) = ∫(flux_Ω(u, v))dΩ - ∫(flux_Γ(u, v, nΓ))dΓ - Σ ∫(flux_Γ(u, v, n))dΓ_perio
  * with flux_Ω the Euler volume flux against ∇v (_flux_Ω, covo.jl:44-58) and
  * flux_Γ = flux_roe ∘ (side⁻(u), side⁺(u), jump(v), side⁻(n)) (covo.jl:69-150).
- * `V` is the scalar DG Q1 space every field component copies; maps [5][ndofs
+ * `V` is the scalar DG order-1 space every field component copies; maps [5][ndofs
  * of V] are the global ids (0-based) of the components ρ, ρu_x, ρu_y, ρE, ϕ
  * (the MultiFESpace's get_mapping composed with the vector numbering).
  * Interior faces come from the mesh (_build_faces!); `periodic` lists

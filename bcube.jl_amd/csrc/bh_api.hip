@@ -1749,6 +1749,8 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
     if (Tb.nq <= 0) return fail(c, BH_ERR_UNSUPPORTED, "no volume rule of this degree");
     const int np = Tb.nq, nl = Tb.nl, ng = Tb.ngeo;
     T.npts[sh] = np;
+    T.off_xi[sh] = (int)tabs.size();
+    tabs.insert(tabs.end(), Tb.xi.begin(), Tb.xi.begin() + 2 * np);
     T.off_w[sh] = (int)tabs.size();
     tabs.insert(tabs.end(), Tb.w.begin(), Tb.w.end());
     T.off_phi[sh] = (int)tabs.size();

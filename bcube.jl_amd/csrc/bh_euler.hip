@@ -262,23 +262,41 @@ __global__ __launch_bounds__(256) void k_euler_dg(
     const double* ph_ = tabs + T.off_phi[ct];
     const double* dph_ = tabs + T.off_dphi[ct];
     const double* dg_ = tabs + T.off_dg[ct];
+    const double* xq_ = tabs + T.off_xi[ct];
     for (int q = g; q < T.npts[ct]; q += 8) {
       double X[4][2];
       load_cell(coords, c2n, c, nl, X);
       double J[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-      for (int k = 0; k < nl; ++k) {
-        const double d0 = dg_[(q * 4 + k) * 2], d1 = dg_[(q * 4 + k) * 2 + 1];
-        J[0][0] += X[k][0] * d0;
-        J[0][1] += X[k][0] * d1;
-        J[1][0] += X[k][1] * d0;
-        J[1][1] += X[k][1] * d1;
+      double phi[4], dq[4][2];
+      if constexpr (SHAPES == 0) {
+        // all Quad4: the Q1 basis and the bilinear map at the point, in closed form
+        const double xi = xq_[2 * q], eta = xq_[2 * q + 1];
+        jacobian_at(0, X, xi, eta, J);
+        basis_at(0, xi, eta, phi);
+        const double a0 = 0.5 * (1.0 - xi), a1 = 0.5 * (1.0 + xi);
+        const double b0 = 0.5 * (1.0 - eta), b1 = 0.5 * (1.0 + eta);
+        dq[0][0] = -0.5 * b0; dq[0][1] = -0.5 * a0;
+        dq[1][0] = 0.5 * b0;  dq[1][1] = -0.5 * a1;
+        dq[2][0] = -0.5 * b1; dq[2][1] = 0.5 * a0;
+        dq[3][0] = 0.5 * b1;  dq[3][1] = 0.5 * a1;
+      } else {
+        for (int k = 0; k < nl; ++k) {
+          const double d0 = dg_[(q * 4 + k) * 2], d1 = dg_[(q * 4 + k) * 2 + 1];
+          J[0][0] += X[k][0] * d0;
+          J[0][1] += X[k][0] * d1;
+          J[1][0] += X[k][1] * d0;
+          J[1][1] += X[k][1] * d1;
+        }
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+          phi[l] = ph_[q * 4 + l];
+          dq[l][0] = dph_[(q * 4 + l) * 2];
+          dq[l][1] = dph_[(q * 4 + l) * 2 + 1];
+        }
       }
       const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
       const double r = 1.0 / det;
       const double Ji[2][2] = {{J[1][1] * r, -J[0][1] * r}, {-J[1][0] * r, J[0][0] * r}};
-      double phi[4];
-#pragma unroll
-      for (int l = 0; l < 4; ++l) phi[l] = ph_[q * 4 + l];
       double w5[5];
       point_state(u, maps, stride, c, nl, phi, w5);
       // _flux_Ω (covo.jl:44-58)
@@ -294,7 +312,7 @@ __global__ __launch_bounds__(256) void k_euler_dg(
 #pragma unroll
       for (int l = 0; l < 4; ++l) {
         // ∇φ = J⁻ᵀ ∇̂φ
-        const double d0 = dph_[(q * 4 + l) * 2], d1 = dph_[(q * 4 + l) * 2 + 1];
+        const double d0 = dq[l][0], d1 = dq[l][1];
         const double gxl = d0 * Ji[0][0] + d1 * Ji[1][0];
         const double gyl = d0 * Ji[0][1] + d1 * Ji[1][1];
 #pragma unroll
@@ -358,8 +376,13 @@ __global__ __launch_bounds__(256) void k_euler_dg(
   const double* w_ = tabs + T.off_w[ct];
   const double* ph_ = tabs + T.off_phi[ct];
   const double* dg_ = tabs + T.off_dg[ct];
+  const double* xq_ = tabs + T.off_xi[ct];
   auto wdet = [&](int q) {
     double J[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    if constexpr (SHAPES == 0) {
+      jacobian_at(0, X, xq_[2 * q], xq_[2 * q + 1], J);
+      return w_[q] * fabs(J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+    }
     for (int k = 0; k < nl; ++k) {
       const double d0 = dg_[(q * 4 + k) * 2], d1 = dg_[(q * 4 + k) * 2 + 1];
       J[0][0] += X[k][0] * d0;

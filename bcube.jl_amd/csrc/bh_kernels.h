@@ -124,7 +124,7 @@ hipError_t launch_dgb_points(hipStream_t s, const double* coords, const int32_t*
 // Tri3 per cell, shapes 0 (all Quad4), 1 (all Tri3) or 2 (mixed); faces [nfaces][5] = (cell⁻, side⁻, cell⁺, side⁺, flip); sides
 // [ncells][4][4] = (face, 0, role 0/1/2, flip); maps [5][ncells*4] global ids
 // of (component, cell, local dof), -1 past a triangle's 3; tabs holds the face
-// rule (nq1 points, weights) and per shape the volume rule's weights, basis
+// rule (nq1 points, weights) and per shape the volume rule's points [np][2], weights, basis
 // [np][4], reference gradients [np][4][2], geometric gradients [np][4][2], and
 // Φ⁻¹ of a square Quad4 point-basis matrix (off_pinv < 0: none); G scratch of
 // nfaces*nq1*5 doubles; terms bit 0 volume, bit 1 faces with coefficient
@@ -132,7 +132,7 @@ hipError_t launch_dgb_points(hipStream_t s, const double* coords, const int32_t*
 struct EulerTabs {
   int nq1 = 0, off_fx = 0, off_fw = 0;
   int npts[2] = {0, 0}, off_w[2] = {0, 0}, off_phi[2] = {0, 0}, off_dphi[2] = {0, 0},
-      off_dg[2] = {0, 0};
+      off_dg[2] = {0, 0}, off_xi[2] = {0, 0};
   int off_pinv = -1;
 };
 hipError_t launch_euler_dg(hipStream_t s, const double* coords, const int32_t* c2n,

@@ -339,3 +339,36 @@ def test_euler_on_a_mixed_tri_quad_mesh():
     bf = B.assemble_linear(ff["l"], V)
     vol = B.assemble_linear(ff["l_vol"], V)
     assert np.abs(bf).max() <= 1e-12 * np.abs(vol).max()
+
+
+def test_euler_and_face_term_error_paths():
+    """Refusals, not faults: a continuous space, a 3D mesh for face terms,
+    b aliasing u, and a periodic face on a side that already has a face."""
+    import ctypes as C
+    from bcubehip import _lib as L
+    m, _ = _meshes(4)
+    Uc = B.TrialFESpace(B.FunctionSpace("Lagrange", 1), m)
+    maps = np.arange(5 * Uc.get_ndofs(), dtype=np.int64)
+    h = C.c_void_p()
+    rc = L.lib().bh_euler_create(Uc.handle, 3, 1.4, 0, None, maps.ctypes.data, maps.size,
+                                 C.byref(h))
+    assert rc == -2 and not h.value          # BH_ERR_UNSUPPORTED
+    Ud = B.TrialFESpace(B.FunctionSpace("Lagrange", 1), m, "discontinuous")
+    ok = np.array([[0, 0, 12, 2, 1]], dtype=np.int32)   # bottom of cell 0, top of cell 12
+    per = np.array([[0, 1, 5, 3, 0]], dtype=np.int32)   # side 1 of cell 0 is interior
+    maps = np.arange(5 * Ud.get_ndofs(), dtype=np.int64)
+    rc = L.lib().bh_euler_create(Ud.handle, 3, 1.4, 1, per.ctypes.data, maps.ctypes.data,
+                                 maps.size, C.byref(h))
+    assert rc == -1 and not h.value          # BH_ERR_ARG
+    rc = L.lib().bh_euler_create(Ud.handle, 3, 1.4, 1, ok.ctypes.data, maps.ctypes.data,
+                                 maps.size, C.byref(h))
+    assert rc == 0
+    u = np.ones(maps.size)
+    assert L.lib().bh_euler_residual(h, 3, -1.0, u.ctypes.data, u.ctypes.data, L.HOST) == -1
+    L.lib().bh_euler_destroy(h)
+    mh = B.hexa_mesh(3, 3, 3)
+    Uh = B.TrialFESpace(B.FunctionSpace("Lagrange", 1), mh, "discontinuous")
+    dΓ = B.Measure(B.InteriorFaceDomain(mh), 3)
+    with pytest.raises(B.BcubeHipError):
+        B.assemble_bilinear(lambda u, v: B.integrate(B.side_n(u) * B.side_n(v), dΓ), Uh,
+                            B.TestFESpace(Uh))

@@ -15,7 +15,7 @@ import scipy.sparse as sp
 
 from . import _lib as L
 from .faces import FaceBilinearPlan, assemble_linear_faces_, has_faces
-from .forms import (Placeholder, _split_sum, lower_bilinear, lower_dg_transport, lower_linear,
+from .forms import (CellDomain, Placeholder, _split_sum, lower_bilinear, lower_dg_transport, lower_linear,
                     lower_multi_bilinear, lower_multi_linear)
 
 _COEF = {"const": L.COEF_CONST, "cell": L.COEF_CELL, "qp": L.COEF_QP}
@@ -24,6 +24,14 @@ _FORM = {"stiffness": L.FORM_STIFFNESS, "mass": L.FORM_MASS, "source": L.FORM_SO
 
 def _integrals(form_value):
     return form_value.terms()
+
+
+def _cells_only(items, what):
+    """Vector and multi-field assembly integrates over cells only."""
+    for it in items:
+        if not isinstance(it.measure.domain, CellDomain):
+            raise NotImplementedError(
+                f"{what} over a vector space or MultiFESpace: only CellDomain measures")
 
 
 def _single_measure(items):
@@ -109,6 +117,7 @@ class MultiBilinearPlan:
         self.fields = fu
         sizes = fu.sizes
         items = _integrals(a(fu.placeholders(U, "trial"), fv.placeholders(V, "test")))
+        _cells_only(items, "bilinear forms")
         self.measure = _single_measure(items)
         mesh = fu.scalar.mesh
         if self.measure.domain.mesh is not mesh:
@@ -220,6 +229,7 @@ def _fields_blocks(V):
 def _assemble_linear_multi(b: np.ndarray, l, V, mode: str) -> None:
     f, h = _fields_blocks(V)
     items = _integrals(l(f.placeholders(V, "test")))
+    _cells_only(items, "linear forms")
     measure = _single_measure(items)
     mesh = f.scalar.mesh
     ctx = mesh.backend.ctx
@@ -432,7 +442,7 @@ def assemble_linear_(b: np.ndarray, l, V, mode: str = "fp64") -> None:
         from .fespace import MultiFESpace
         if isinstance(V, MultiFESpace):
             f, _ = _fields_blocks(V)
-            eu = lower_euler_dg(_integrals(l(f.placeholders(V, "test"))))
+            eu = lower_euler_dg(_integrals(l(f.placeholders(V, "test"))), V)
             if eu is not None:
                 # the COVO Euler DG residual (benchmark/covo.jl): one kernel
                 gamma, u, degree, periodic, terms, fcoef = eu

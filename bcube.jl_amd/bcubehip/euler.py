@@ -117,6 +117,8 @@ class EulerDG:
     def step(self, u: np.ndarray, dt: float) -> np.ndarray:
         """forward_euler(u, compute_residual, t, Δt) (covo.jl:14-33, 152)."""
         u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.size != self.ndofs:
+            raise ValueError("u has the wrong size")
         out = np.empty_like(u)
         L.check(L.lib().bh_euler_step(self.handle, float(dt), u.ctypes.data, out.ctypes.data,
                                       L.HOST), self.ctx, "bh_euler_step")
@@ -149,9 +151,11 @@ def _is_tests(t, wrap):
         and x.a.field == i for i, x in enumerate(t)))
 
 
-def lower_euler_dg(items):
+def lower_euler_dg(items, V=None):
     """Recognise the COVO linear form.  Returns (gamma, u, degree, periodic
-    domains, terms bitmask, face coefficient) or None when the form is not it."""
+    domains, terms bitmask, face coefficient) or None when the form is not it.
+    With the test space V given, the state's space must be numbered as V and
+    every measure must lie on V's mesh (the kernel reads u in V's numbering)."""
     gamma = u = degree = fsign = None
     terms = 0
     periodic = []
@@ -167,6 +171,8 @@ def lower_euler_dg(items):
             uu, grads = cp.args
             if not isinstance(uu, FEFunction) or not _is_tests(grads, Grad):
                 return None
+            if terms & 1:
+                raise NotImplementedError("the volume integral appears twice")
             terms |= 1
         elif isinstance(cp.f, RoeFlux):
             if sign not in (1.0, -1.0) or len(cp.args) != 4:
@@ -182,8 +188,12 @@ def lower_euler_dg(items):
                 return None
             uu = a0.a
             if isinstance(dom, InteriorFaceDomain):
+                if terms & 2:
+                    raise NotImplementedError("the interior-face integral appears twice")
                 terms |= 2
             elif isinstance(dom, BoundaryFaceDomain) and dom.bc is not None:
+                if any(d is dom for d in periodic):
+                    raise NotImplementedError("a periodic face integral appears twice")
                 periodic.append(dom)
             else:
                 return None
@@ -198,6 +208,14 @@ def lower_euler_dg(items):
         u, gamma, degree = uu, cp.f.gamma, it.measure.degree
     if u is None:
         return None
+    if V is not None:
+        from .assembler import Fields
+        fv, fu = Fields(V), Fields(u.space)
+        if fu.sizes != fv.sizes or not np.array_equal(fu.maps, fv.maps):
+            raise NotImplementedError("the state's space must be numbered as the test space")
+        for it in items:
+            if it.measure.domain.mesh is not fv.scalar.mesh:
+                raise ValueError("a measure's mesh differs from the space's mesh")
     if periodic and not terms & 2:
         raise NotImplementedError("periodic faces without the interior faces")
     return gamma, u, degree, periodic, terms, (fsign if fsign is not None else -1.0)

@@ -48,6 +48,11 @@ def _check_mesh(items, mesh):
     for it in items:
         if it.measure.domain.mesh is not mesh:
             raise ValueError("a measure's mesh differs from the space's mesh")
+        d = it.measure.domain
+        if isinstance(d, BoundaryFaceDomain) and d.bc is not None:
+            raise NotImplementedError(
+                "face terms over a periodic BoundaryFaceDomain (only the Euler residual "
+                "pairs periodic faces)")
 
 
 class FaceBilinearPlan:

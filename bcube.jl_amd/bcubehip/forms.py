@@ -307,6 +307,13 @@ class BoundaryFaceDomain:
     side⁺."""
 
     def __init__(self, mesh, bc=None):
+        # A label (the reference's BoundaryFaceDomain(mesh, label)) would select a
+        # subset of the boundary faces; the face kernels integrate over all of
+        # them, so labels are refused rather than silently widened.
+        if bc is not None and not isinstance(bc, PeriodicBCType):
+            raise NotImplementedError(
+                "BoundaryFaceDomain takes no boundary label here: only None (every "
+                "boundary face) or a PeriodicBCType")
         self.mesh = mesh
         self.bc = bc
         self._pairs = None

@@ -1672,6 +1672,10 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
   const int nq1 = gauss_points_per_axis(quad_degree);
   if (nq1 > 4) return fail(c, BH_ERR_UNSUPPORTED, "the Euler residual: at most 4 Gauss points per axis");
   if (ndofs_total > INT32_MAX) return fail(c, BH_ERR_UNSUPPORTED, "more than 2^31 dofs");
+  // the five component maps are one-to-one; they must also be onto, or a
+  // step would hand back unwritten entries of the result
+  if (ndofs_total != 5 * V->ndofs)
+    return fail(c, BH_ERR_ARG, "ndofs_total must be 5 x the scalar space's dofs");
   std::vector<int8_t> ctype(nc);
   std::vector<int32_t> c2n4((size_t)nc * 4, -1);
   for (int64_t cc = 0; cc < nc; ++cc) {

@@ -104,56 +104,64 @@ __device__ __forceinline__ double pressure(double rho, double rux, double ruy, d
   return (gamma - 1.0) * (rE - (rux * vx + ruy * vy) / 2.0);
 }
 
-// flux_roe (covo.jl:69-150), term by term
-__device__ void roe_flux(const double ui[5], const double uj[5], double nx, double ny,
+// Roe's approximate Riemann flux across a face of unit normal n from the
+// state uL (side⁻) to uR (side⁺), the flux of covo.jl:69-150 (no entropy
+// fix).  Written as the textbook upwind-from-the-right form
+//     F = F(uR)⋅n − Σ_k max(λ_k, 0) α_k K_k
+// of Toro, "Riemann Solvers and Numerical Methods for Fluid Dynamics"
+// (3rd ed., §11.2.1-11.2.2): Roe averages of (v_n, v_t, H), eigenvalues
+// λ = v_n − a, v_n (entropy and shear waves), v_n + a, right eigenvectors K_k
+// and wave strengths α_k of Δ = uR − uL, all in the face's frame (n, t) with
+// t = (−n_y, n_x); the momentum rows are rotated back to (x, y) at the end.
+// The sound speed is √|(γ−1)(H − |v|²/2)|, as the reference takes it.  The
+// passive scalar ϕ is upwinded by the mean of the two normal velocities.
+__device__ void roe_flux(const double uL[5], const double uR[5], double nx, double ny,
                          double gamma, double F[5]) {
-  const double rho1 = ui[0], rux1 = ui[1], ruy1 = ui[2], rE1 = ui[3], phi1 = ui[4];
-  const double rho2 = uj[0], rux2 = uj[1], ruy2 = uj[2], rE2 = uj[3], phi2 = uj[4];
-  const double u1 = rux1 / rho1, v1 = ruy1 / rho1;
-  const double u2 = rux2 / rho2, v2 = ruy2 / rho2;
-  const double p1 = pressure(rho1, rux1, ruy1, rE1, gamma);
-  const double p2 = pressure(rho2, rux2, ruy2, rE2, gamma);
-  const double H2 = (gamma / (gamma - 1.0)) * p2 / rho2 + (u2 * u2 + v2 * v2) / 2.0;
-  const double H1 = (gamma / (gamma - 1.0)) * p1 / rho1 + (u1 * u1 + v1 * v1) / 2.0;
-  const double R = sqrt(rho1 / rho2);
-  const double invR1 = 1.0 / (R + 1.0);
-  const double uAv = (R * u1 + u2) * invR1;
-  const double vAv = (R * v1 + v2) * invR1;
-  const double Hav = (R * H1 + H2) * invR1;
-  const double cAv = sqrt(fabs((gamma - 1.0) * (Hav - (uAv * uAv + vAv * vAv) / 2.0)));
-  const double ecAv = (uAv * uAv + vAv * vAv) / 2.0;
-  const double l1 = nx * uAv + ny * vAv;
-  const double l3 = l1 + cAv;
-  const double l4 = l1 - cAv;
-  const double d1 = rho1 - rho2;
-  const double d2 = rho1 * u1 - rho2 * u2;
-  const double d3 = rho1 * v1 - rho2 * v2;
-  const double d4 = rE1 - rE2;
-  const double flu11 = nx * rho2 * u2 + ny * rho2 * v2;
-  const double flu21 = nx * p2 + flu11 * u2;
-  const double flu31 = ny * p2 + flu11 * v2;
-  const double flu41 = H2 * flu11;
-  const double rc1 = (gamma - 1.0) / cAv;
-  const double rc2 = (gamma - 1.0) / cAv / cAv;
-  const double uq41 = ecAv / cAv + cAv / (gamma - 1.0);
-  const double uq42 = nx * uAv + ny * vAv;
-  const double fdc1 = fmax(l1, 0.0) * (d1 + rc2 * (-ecAv * d1 + uAv * d2 + vAv * d3 - d4));
-  const double fdc2 = fmax(l1, 0.0) * ((nx * vAv - ny * uAv) * d1 + ny * d2 - nx * d3);
-  const double fdc3 = fmax(l3, 0.0) * ((-uq42 * d1 + nx * d2 + ny * d3) / 2.0 +
-                                       rc1 * (ecAv * d1 - uAv * d2 - vAv * d3 + d4) / 2.0);
-  const double fdc4 = fmax(l4, 0.0) * ((uq42 * d1 - nx * d2 - ny * d3) / 2.0 +
-                                       rc1 * (ecAv * d1 - uAv * d2 - vAv * d3 + d4) / 2.0);
-  const double duv1 = fdc1 + (fdc3 + fdc4) / cAv;
-  const double duv2 = uAv * fdc1 + ny * fdc2 + (uAv / cAv + nx) * fdc3 + (uAv / cAv - nx) * fdc4;
-  const double duv3 = vAv * fdc1 - nx * fdc2 + (vAv / cAv + ny) * fdc3 + (vAv / cAv - ny) * fdc4;
-  const double duv4 = ecAv * fdc1 + (ny * uAv - nx * vAv) * fdc2 + (uq41 + uq42) * fdc3 +
-                      (uq41 - uq42) * fdc4;
-  const double v12 = 0.5 * ((u1 + u2) * nx + (v1 + v2) * ny);
-  F[0] = flu11 + duv1;
-  F[1] = flu21 + duv2;
-  F[2] = flu31 + duv3;
-  F[3] = flu41 + duv4;
-  F[4] = fmax(0.0, v12) * phi1 + fmin(0.0, v12) * phi2;
+  const double g1 = gamma - 1.0;
+  // velocities in the face frame, pressure and total enthalpy of each state
+  const double rL = uL[0], rR = uR[0];
+  const double vxL = uL[1] / rL, vyL = uL[2] / rL, vxR = uR[1] / rR, vyR = uR[2] / rR;
+  const double vnL = vxL * nx + vyL * ny, vtL = vyL * nx - vxL * ny;
+  const double vnR = vxR * nx + vyR * ny, vtR = vyR * nx - vxR * ny;
+  const double pL = pressure(rL, uL[1], uL[2], uL[3], gamma);
+  const double pR = pressure(rR, uR[1], uR[2], uR[3], gamma);
+  const double HL = (uL[3] + pL) / rL, HR = (uR[3] + pR) / rR;
+  // Roe averages: weights √ρ
+  const double sL = sqrt(rL), sR = sqrt(rR), inv = 1.0 / (sL + sR);
+  const double vn = (sL * vnL + sR * vnR) * inv;
+  const double vt = (sL * vtL + sR * vtR) * inv;
+  const double H = (sL * HL + sR * HR) * inv;
+  const double q2 = vn * vn + vt * vt;
+  const double a = sqrt(fabs(g1 * (H - 0.5 * q2)));
+  // jumps of the conserved variables (ρ, ρv_n, ρv_t, ρE)
+  const double dr = rR - rL;
+  const double dmn = rR * vnR - rL * vnL;
+  const double dmt = rR * vtR - rL * vtL;
+  const double dE = uR[3] - uL[3];
+  // wave strengths (shear, entropy, the two acoustic waves)
+  const double at = dmt - vt * dr;
+  const double a2 = g1 / (a * a) * (dr * (H - vn * vn) + vn * dmn - (dE - at * vt));
+  const double a1 = (dr * (vn + a) - dmn - a * a2) / (2.0 * a);
+  const double a5 = dr - a1 - a2;
+  const double w1 = fmax(vn - a, 0.0) * a1;  // λ⁺α of each wave
+  const double w2 = fmax(vn, 0.0) * a2;
+  const double wt = fmax(vn, 0.0) * at;
+  const double w5 = fmax(vn + a, 0.0) * a5;
+  // Σ λ⁺ α K in the face frame; K1 = (1, v_n − a, v_t, H − v_n a),
+  // K2 = (1, v_n, v_t, |v|²/2), Kt = (0, 0, 1, v_t), K5 = (1, v_n + a, v_t, H + v_n a)
+  const double s0 = w1 + w2 + w5;
+  const double sn = w1 * (vn - a) + w2 * vn + w5 * (vn + a);
+  const double st = (w1 + w2 + w5) * vt + wt;
+  const double sE = w1 * (H - vn * a) + w2 * (0.5 * q2) + wt * vt + w5 * (H + vn * a);
+  // the right state's physical flux, face frame
+  const double mR = rR * vnR;
+  const double fn = mR * vnR + pR - sn, ft = mR * vtR - st;
+  F[0] = mR - s0;
+  F[1] = fn * nx - ft * ny;
+  F[2] = fn * ny + ft * nx;
+  F[3] = mR * HR - sE;
+  const double vbar = 0.5 * (vnL + vnR);
+  F[4] = fmax(0.0, vbar) * uL[4] + fmin(0.0, vbar) * uR[4];
 }
 
 __device__ __forceinline__ void load_cell(const double* __restrict__ coords,

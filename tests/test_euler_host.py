@@ -63,3 +63,11 @@ def test_lower_recognises_the_covo_forms():
     assert lower_euler_dg(B.integrate(v[0] * 2.0, dΩ).terms()) is None
     with pytest.raises(NotImplementedError):   # periodic faces without the interior ones
         lower_euler_dg((-B.integrate(fΓ(u, v, npp), dΓp)).terms())
+    # an integral given twice is refused, not dropped
+    with pytest.raises(NotImplementedError):
+        lower_euler_dg((B.integrate(fΩ(u, v), dΩ) + B.integrate(fΩ(u, v), dΩ)).terms())
+    with pytest.raises(NotImplementedError):
+        lower_euler_dg((B.integrate(fΓ(u, v, n), dΓ) + B.integrate(fΓ(u, v, n), dΓ)).terms())
+    with pytest.raises(NotImplementedError):
+        lower_euler_dg((B.integrate(fΓ(u, v, n), dΓ) + B.integrate(fΓ(u, v, npp), dΓp)
+                        + B.integrate(fΓ(u, v, npp), dΓp)).terms())

@@ -361,11 +361,20 @@ def test_euler_and_face_term_error_paths():
                                  maps.size, C.byref(h))
     assert rc == -1 and not h.value          # BH_ERR_ARG
     rc = L.lib().bh_euler_create(Ud.handle, 3, 1.4, 1, ok.ctypes.data, maps.ctypes.data,
+                                 maps.size + 7, C.byref(h))
+    assert rc == -1 and not h.value          # maps not onto ndofs_total
+    rc = L.lib().bh_euler_create(Ud.handle, 3, 1.4, 1, ok.ctypes.data, maps.ctypes.data,
                                  maps.size, C.byref(h))
     assert rc == 0
     u = np.ones(maps.size)
     assert L.lib().bh_euler_residual(h, 3, -1.0, u.ctypes.data, u.ctypes.data, L.HOST) == -1
     L.lib().bh_euler_destroy(h)
+    Ue, Ve = _setup(m)
+    u0, _ = _to_global(Ue, _state(16, seed=1))
+    _, (dΓx, dΓy) = _forms(m, Ue, u0)
+    plan = B.EulerDG(Ve, GAMMA, 3, (dΓx.domain, dΓy.domain))
+    with pytest.raises(ValueError):          # a short state: refused before any copy
+        plan.step(u0[:-1], 1e-5)
     mh = B.hexa_mesh(3, 3, 3)
     Uh = B.TrialFESpace(B.FunctionSpace("Lagrange", 1), mh, "discontinuous")
     dΓ = B.Measure(B.InteriorFaceDomain(mh), 3)

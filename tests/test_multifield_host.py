@@ -178,6 +178,37 @@ def test_unsupported_couplings_raise():
                             B.TestFESpace(Uv))
 
 
+def test_face_measures_on_vector_spaces_raise():
+    """Vector and multi-field assembly integrates over cells only: a face
+    measure must be refused, not assembled as a second cell integral."""
+    m = _mesh("quad")
+    Uv = B.TrialFESpace(LAG(1), m, size=2)
+    Vv = B.TestFESpace(Uv)
+    dΩ = B.Measure(B.CellDomain(m), 2)
+    for dom in (B.BoundaryFaceDomain(m), B.InteriorFaceDomain(m)):
+        dΓ = B.Measure(dom, 2)
+        with pytest.raises(NotImplementedError):
+            B.MultiBilinearPlan(lambda u, v: B.integrate(B.dot(u, v), dΩ)
+                                + B.integrate(B.dot(u, v), dΓ), Uv, Vv)
+        g = B.PhysicalFunction(lambda x: np.ones((x.shape[0], 2)))
+        with pytest.raises(NotImplementedError):
+            B.assemble_linear(lambda v: B.integrate(B.dot(g, v), dΓ), Vv)
+
+
+def test_boundary_face_domain_refuses_labels():
+    """BoundaryFaceDomain(mesh, label) selects a subset of faces in the
+    reference; the face kernels take all of them, so a label raises."""
+    m = _mesh("quad")
+    with pytest.raises(NotImplementedError):
+        B.BoundaryFaceDomain(m, "xmin")
+    bc = B.PeriodicBCType(B.Translation([1.0, 0.0]), ("xmax",), ("xmin",))
+    dΓp = B.Measure(B.BoundaryFaceDomain(m, bc), 2)
+    U = B.TrialFESpace(LAG(1), m, "discontinuous")
+    with pytest.raises(NotImplementedError):  # periodic faces in a face-term form
+        B.assemble_bilinear(lambda u, v: B.integrate(B.side_n(u) * B.side_n(v), dΓp), U,
+                            B.TestFESpace(U))
+
+
 def test_dot_of_vector_gradients_is_the_frobenius_product():
     """bench_assemble.jl:10-28 assembles ∫(∇(u) ⋅ ∇(v))dΩ on a size-2 space:
     Julia's dot of two matrices contracts both indices, the form of ∇u ⊡ ∇v."""

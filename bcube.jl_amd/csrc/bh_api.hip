@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <set>
 #include <string>
 #include <tuple>
@@ -237,7 +238,36 @@ struct bh_dg_s {
 };
 
 // ---------------------------------------------------------------------------
+// Static device scratch of bh_assemble_bilinear_noalloc: the element-row
+// workspace of one assembly, for callers that cannot allocate.  Each device
+// holds its own copy for the life of the process.  In host code `scratch` is
+// only a stand-in for the symbol: its device address comes from
+// hipGetSymbolAddress, once per device (bh_ctx_create), and only that address
+// is ever handed to kernels or copies.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// With `device` current: look up this device's copy of `scratch` (first
+// call per device only).
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  g_scratch_ptr[device] = e == hipSuccess ? ptr : nullptr;
+  if (e != hipSuccess) (void)hipGetLastError();
+  g_scratch_done[device] = true;
+}
 
 thread_local std::string g_last_error;  // errors of calls made without a context
 
@@ -292,11 +322,17 @@ int rule_key(int geo, int quad_degree) {
   return gauss_points_per_axis(quad_degree);
 }
 
-int get_tables(bh_ctx c, int geo, int order, int quad_degree, double** out) {
+const char* kNotLoaded =
+    "the reference tables of this (cell type, order, rule) are not on the device yet: run "
+    "bh_assemble_bilinear once on this context first";
+
+int get_tables(bh_ctx c, int geo, int order, int quad_degree, double** out,
+               bool may_alloc = true) {
   int nq1 = rule_key(geo, quad_degree);
   auto key = std::make_tuple(geo, order, nq1);
   auto it = c->tables.find(key);
   if (it != c->tables.end()) { *out = it->second; return BH_OK; }
+  if (!may_alloc) return fail(c, BH_ERR_ARG, kNotLoaded);
   if (c->capturing)
     return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
   int size = packed_table_size(geo, order, nq1);
@@ -334,11 +370,13 @@ int get_tables(bh_ctx c, int geo, int order, int quad_degree, double** out) {
 // p over the pairs i <= j in row-major upper order.  Diagonal G entries
 // multiply ∂_aφ̂_i ∂_aφ̂_j, off-diagonal ones ∂_aφ̂_i ∂_bφ̂_j + ∂_bφ̂_i ∂_aφ̂_j, the
 // mass factor φ̂_i φ̂_j.  Products in FP64, rounded once to FP32.
-int get_gp_table(bh_ctx c, int geo, int order, int quad_degree, bool mass, float** out) {
+int get_gp_table(bh_ctx c, int geo, int order, int quad_degree, bool mass, float** out,
+                 bool may_alloc = true) {
   const int nq1 = rule_key(geo, quad_degree);
   auto key = std::make_tuple(geo, order, nq1, mass ? 1 : 0);
   auto it = c->gp_tables.find(key);
   if (it != c->gp_tables.end()) { *out = it->second; return BH_OK; }
+  if (!may_alloc) return fail(c, BH_ERR_ARG, kNotLoaded);
   if (c->capturing)
     return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
   Tables T = make_tables(shape_of(geo), order, quad_degree, geo);
@@ -453,6 +491,7 @@ int bh_ctx_create(int device, bh_ctx* out) {
     delete c;
     return BH_ERR_HIP;
   }
+  scratch_setup(device);
   c->stream = c->own;
   *out = c;
   return BH_OK;
@@ -1921,9 +1960,13 @@ int bh_pattern_destroy(bh_pattern p) {
 // The assembly itself: values to nzval (host or device by `where`; a NULL
 // device nzval means the pattern's own buffer), per-cell coefficient arrays
 // read from host or device memory by coef_where.
+// ke_ext: a caller-provided element-row workspace of ke_cap bytes (the
+// allocation-free entry point); then nothing is allocated, the tables must
+// already be on the device, and there is neither autotuning nor timing.
 static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms,
                                   int quad_degree, int mode, double* nzval, int where,
-                                  int coef_where) {
+                                  int coef_where, void* ke_ext = nullptr, size_t ke_cap = 0) {
+  const bool noalloc = ke_ext != nullptr;
   if (!p || nterms <= 0 || !terms) return BH_ERR_ARG;
   bh_space V = p->V;
   bh_mesh m = V->mesh;
@@ -1964,7 +2007,15 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
   int64_t nc = m->h.ncells;
   const HostPattern& h = p->h;
   if (h.max_row > 512) return fail(c, BH_ERR_UNSUPPORTED, "CSR rows longer than 512 entries");
-  if (!p->d_ke) {
+  if (noalloc) {
+    const size_t need = (size_t)(nc + 16) * V->nl * ke_stride(V->nl) *
+                        (mode == BH_FP64 ? sizeof(double) : sizeof(float));
+    if (need > ke_cap)
+      return fail(c, BH_ERR_RANGE,
+                  "bh_assemble_bilinear_noalloc: the element rows of this mesh need " +
+                      std::to_string(need) + " bytes, more than the static scratch's " +
+                      std::to_string(ke_cap));
+  } else if (!p->d_ke) {
     if (c->capturing)
       return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
     const int ks = ke_stride(V->nl);
@@ -1977,7 +2028,7 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
   A.mode = mode;
   A.stream = c->stream;
   A.coords = m->d_coords;
-  A.ke = p->d_ke;
+  A.ke = noalloc ? ke_ext : p->d_ke;
   A.ke_fp64 = mode == BH_FP64;
   A.erow = V->nl;
   if (stiff) {
@@ -1998,7 +2049,7 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
     for (const GeoBucket& gb : m->buckets) {
       const int key = rule_key(gb.geo, quad_degree);
       double* tab = nullptr;
-      if (int r = get_tables(c, gb.geo, V->order, quad_degree, &tab)) return r;
+      if (int r = get_tables(c, gb.geo, V->order, quad_degree, &tab, !noalloc)) return r;
       LaunchArgs e = A;
       e.kind = LaunchArgs::BILINEAR;
       e.c2n = gb.d_c2n;
@@ -2010,7 +2061,8 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
       const bool gp = mode == BH_FP32 && gp_supported(gb.geo, V->order, key);
       if (gp) {
         float* pt = nullptr;
-        if (int r = get_gp_table(c, gb.geo, V->order, quad_degree, A.mass, &pt)) return r;
+        if (int r = get_gp_table(c, gb.geo, V->order, quad_degree, A.mass, &pt, !noalloc))
+          return r;
         e.ptab = pt;
         int tg[2];
         any12 = any12 || gp_wave_targets(gb.geo, V->order, key, tg) == 2;
@@ -2046,7 +2098,7 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
     return BH_OK;
   };
   Tuning& tu = p->tune[mode == BH_FP64 ? 0 : (mode == BH_FP32 ? 1 : 2)];
-  if (c->autotune && !tu.done && !c->capturing) {
+  if (c->autotune && !tu.done && !c->capturing && !noalloc) {
     // time every candidate shape on this stream (one warm launch, then the
     // fastest of three); all of them give bitwise identical values
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2095,8 +2147,8 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
     best.done = true;
     tu = best;
   }
-  if (int r = elements(tu.elem_waves, true)) return r;
-  if (int r = gather(tu.gather_lanes, tu.gather_rows, true)) return r;
+  if (int r = elements(tu.elem_waves, !noalloc)) return r;
+  if (int r = gather(tu.gather_lanes, tu.gather_rows, !noalloc)) return r;
   if (where == BH_HOST) {
     BH_HIP(c, hipMemcpyAsync(nzval, p->d_vals, (size_t)h.nnz * sizeof(double),
                              hipMemcpyDeviceToHost, c->stream));
@@ -2110,6 +2162,23 @@ int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int qua
   if (p && terms && where == BH_DEVICE)
     if (int r = check_device_coefs(p->V->mesh->ctx, nterms, terms)) return r;
   return assemble_bilinear_impl(p, nterms, terms, quad_degree, mode, nzval, where, where);
+}
+
+int bh_assemble_bilinear_noalloc(bh_pattern p, int nterms, const bh_term* terms, int quad_degree,
+                                 int mode, double* nzval) {
+  if (!p || !terms) return BH_ERR_ARG;
+  bh_ctx c = p->V->mesh->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "assembly needs a mesh created with a context");
+  if (!nzval) return fail(c, BH_ERR_ARG, "bh_assemble_bilinear_noalloc: nzval is NULL");
+  if (int r = check_device_coefs(c, nterms, terms)) return r;
+  const int dev = c->device;
+  if (dev < 0 || dev >= kMaxDevices || !g_scratch_done[dev] || !g_scratch_ptr[dev])
+    return fail(c, BH_ERR_HIP,
+                std::string("no static scratch on this device: hipGetSymbolAddress: ") +
+                    hipGetErrorString(dev >= 0 && dev < kMaxDevices ? g_scratch_err[dev]
+                                                                    : hipErrorInvalidDevice));
+  return assemble_bilinear_impl(p, nterms, terms, quad_degree, mode, nzval, BH_DEVICE, BH_DEVICE,
+                                g_scratch_ptr[dev], kScratchBytes);
 }
 
 int bh_blocks_assemble_block(bh_blocks b, int k, int nterms, const bh_term* terms,

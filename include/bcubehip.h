@@ -183,6 +183,22 @@ int bh_pattern_destroy(bh_pattern p);
  * assemble into the pattern's own buffer (bh_pattern_values_device). */
 int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int quad_degree,
                          int mode, double* nzval, int where);
+/* bh_assemble_bilinear with device `where` that allocates nothing: no
+ * counterpart in the reference.  It is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  nzval (nnz FP64) and per-cell coefficients are device
+ * memory the caller owns.  The element rows go to a static device scratch of
+ * BH_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; a mesh whose
+ * rows need more, (ncells + 16) * ndofs_per_cell * row stride * (8 in FP64, 4
+ * otherwise) bytes, is refused with BH_ERR_RANGE.  The reference tables of the
+ * (cell type, order, rule) must already be on the context's device: run
+ * bh_assemble_bilinear once on the context first (BH_ERR_ARG otherwise).  No
+ * autotuning and no per-launch timing.  Every call on one GPU shares its
+ * scratch: calls that overlap on one GPU (two streams, two contexts, two host
+ * threads) race. */
+#define BH_NOALLOC_SCRATCH_BYTES (64u << 20)
+int bh_assemble_bilinear_noalloc(bh_pattern p, int nterms, const bh_term* terms, int quad_degree,
+                                 int mode, double* nzval);
 /* assemble_linear!(b, l, V) (assembler.jl:465-473): b += l(v) for every test
  * function, like __update_b! (assembler.jl:855-865); b has ndofs entries.
  * assemble_linear(l, V) (assembler.jl:448-458) is this call on a zeroed b. */

@@ -36,6 +36,8 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <chrono>
+#include <omp.h>
 
 typedef int64_t i64;
 
@@ -1024,83 +1026,303 @@ static void phys_grad(int d, int nl, const double* rgrad, const double* Jinv, do
     }
 }
 
-// One integral ∫ term dΩ on every cell, appended as COO in reference order
-// (__assemble_bilinear! assembler.jl:280-307, _append_bilinear! :632-648;
-// integrate_on_ref_element integration.jl:18-105).
+// Quadrature rule of Measure(Ω, degree) per reference shape.  The reference's
+// rules are compile-time constants (@generated, quadrature.jl:455-487), so they
+// are computed once per assembly here, outside the cell loop.
+struct ShapeRules {
+  int degree = 0;
+  bool have[3] = {false, false, false};
+  std::vector<double> w[3], x[3];
+  int nq[3] = {0, 0, 0};
+  const std::vector<double>& weights(Shape s) { fill(s); return w[s]; }
+  const std::vector<double>& points(Shape s) { fill(s); return x[s]; }
+  int count(Shape s) { fill(s); return nq[s]; }
+  void fill(Shape s) {
+    if (!have[s]) { nq[s] = tensor_rule(s, degree, Q_LEGENDRE, w[s], x[s]); have[s] = true; }
+  }
+};
+
+// Element block of one integral on cell c, value [j][i] = a(λu_j, λv_i)
+// (__assemble_bilinear! assembler.jl:280-307; integrate_on_ref_element
+// integration.jl:18-105): acc[j*nl + i], quadrature points summed left to right.
+static int element_block(const Mesh& m, int order, int term, const double* eta, ShapeRules& R,
+                         i64 c, std::vector<double>& acc) {
+  int d = m.dim;
+  int ct = m.ctype[c];
+  Shape s = shape_of(ct);
+  int nl = ndofs_lagrange(s, order);
+  const std::vector<double>& w = R.weights(s);
+  const std::vector<double>& xq = R.points(s);
+  int nq = R.count(s);
+  const i64* nodes = &m.c2n[m.c2n_ptr[c]];
+  const i64 nn = m.c2n_ptr[c + 1] - m.c2n_ptr[c];
+  double Xc[27 * 3];
+  for (i64 k = 0; k < nn; ++k)
+    for (int a = 0; a < d; ++a) Xc[k * d + a] = m.coords[nodes[k] * d + a];
+  acc.assign((size_t)nl * nl, 0.0);
+  std::vector<double> val(nl), rg(nl * d), pg(nl * d);
+  for (int q = 0; q < nq; ++q) {
+    Geo g;
+    geometry(ct, Xc, &xq[q * d], g);
+    tensor_basis(s, order, &xq[q * d], val.data(), rg.data());
+    phys_grad(d, nl, rg.data(), g.Jinv, pg.data());
+    for (int j = 0; j < nl; ++j)
+      for (int i = 0; i < nl; ++i) {
+        double gij;
+        if (term == T_STIFF) {
+          double e = eta ? eta[c] : 1.0;
+          double sacc = (e * pg[j * d + 0]) * pg[i * d + 0];
+          for (int a = 1; a < d; ++a) sacc = sacc + (e * pg[j * d + a]) * pg[i * d + a];
+          gij = sacc;
+        } else {
+          gij = val[j] * val[i];
+        }
+        double wu = w[q] * (g.detabs * gij);
+        size_t k = (size_t)j * nl + i;
+        acc[k] = (q == 0) ? wu : acc[k] + wu;
+      }
+  }
+  return nl;
+}
+
+// The COO of Σ terms (MultiIntegration, assembler.jl:220-238): one segment per
+// term in term order, cells in element order, entry k = i + nV*j of a cell
+// holds (Vdofs[i], Udofs[j], a(λu_j, λv_i)) (_append_bilinear! :632-648,
+// _cartesian_product :1357-1370).  With nthreads > 1 the cells of a segment
+// are computed in parallel, each written at its own precomputed position: the
+// COO is the same array as the serial one.
+static void build_coo(const Mesh& m, const std::vector<i64>& iglob, const std::vector<i64>& offset,
+                      int order, int nterms, const int* terms, const double* eta, int degree,
+                      int nthreads, std::vector<i64>& I, std::vector<i64>& J,
+                      std::vector<double>& X) {
+  std::vector<i64> cells = element_order(m);
+  const i64 ncl = (i64)cells.size();
+  std::vector<i64> start(ncl + 1, 0);
+  for (i64 k = 0; k < ncl; ++k) {
+    const int nl = ndofs_lagrange(shape_of(m.ctype[cells[k]]), order);
+    start[k + 1] = start[k] + (i64)nl * nl;
+  }
+  const i64 seg = start[ncl];
+  I.resize((size_t)seg * nterms);
+  J.resize((size_t)seg * nterms);
+  X.resize((size_t)seg * nterms);
+  ShapeRules R0;
+  R0.degree = degree;
+  for (int ct : m.ctype) R0.fill(shape_of(ct));  // shared read-only copies per thread
+  for (int t = 0; t < nterms; ++t) {
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+      ShapeRules R = R0;
+      std::vector<double> acc;
+#pragma omp for schedule(static)
+      for (i64 k = 0; k < ncl; ++k) {
+        const i64 c = cells[k];
+        const int nl = element_block(m, order, terms[t], eta, R, c, acc);
+        size_t p = (size_t)t * seg + start[k];
+        for (int j = 0; j < nl; ++j)
+          for (int i = 0; i < nl; ++i, ++p) {
+            I[p] = iglob[offset[c] + i];
+            J[p] = iglob[offset[c] + j];
+            X[p] = acc[(size_t)j * nl + i];
+          }
+      }
+    }
+  }
+}
+
+// One integral ∫ term dΩ on every cell, appended as COO in reference order.
 static void append_term(const Mesh& m, const std::vector<i64>& iglob, const std::vector<i64>& offset,
                         int order, int term, const double* eta, int degree,
                         std::vector<i64>& I, std::vector<i64>& J, std::vector<double>& X) {
-  std::vector<i64> cells = element_order(m);
-  int d = m.dim;
-  for (i64 c : cells) {
-    int ct = m.ctype[c];
-    Shape s = shape_of(ct);
-    int nl = ndofs_lagrange(s, order);
-    std::vector<double> w, xq;
-    int nq = tensor_rule(s, degree, Q_LEGENDRE, w, xq);
-    std::vector<i64> nodes = m.cell(c);
-    std::vector<double> Xc(nodes.size() * d);
-    for (size_t k = 0; k < nodes.size(); ++k)
-      for (int a = 0; a < d; ++a) Xc[k * d + a] = m.coords[nodes[k] * d + a];
-    std::vector<double> acc((size_t)nl * nl, 0.0), val(nl), rg(nl * d), pg(nl * d);
-    for (int q = 0; q < nq; ++q) {
-      Geo g;
-      geometry(ct, Xc.data(), &xq[q * d], g);
-      tensor_basis(s, order, &xq[q * d], val.data(), rg.data());
-      phys_grad(d, nl, rg.data(), g.Jinv, pg.data());
-      for (int j = 0; j < nl; ++j)
-        for (int i = 0; i < nl; ++i) {
-          double gij;
-          if (term == T_STIFF) {
-            double e = eta ? eta[c] : 1.0;
-            double sacc = (e * pg[j * d + 0]) * pg[i * d + 0];
-            for (int a = 1; a < d; ++a) sacc = sacc + (e * pg[j * d + a]) * pg[i * d + a];
-            gij = sacc;
-          } else {
-            gij = val[j] * val[i];
-          }
-          double wu = w[q] * (g.detabs * gij);
-          size_t k = (size_t)j * nl + i;
-          acc[k] = (q == 0) ? wu : acc[k] + wu;
-        }
+  std::vector<i64> Ia, Ja;
+  std::vector<double> Xa;
+  build_coo(m, iglob, offset, order, 1, &term, eta, degree, 1, Ia, Ja, Xa);
+  I.insert(I.end(), Ia.begin(), Ia.end());
+  J.insert(J.end(), Ja.begin(), Ja.end());
+  X.insert(X.end(), Xa.begin(), Xa.end());
+}
+
+// coo_to_csc on nthreads threads, with the same result bit for bit: each
+// thread takes a contiguous run of the COO (then of the rows), and per-thread
+// counts give every thread its own positions, so entries keep the serial order
+// inside each row and each column and duplicates are added in COO order.
+static void coo_to_csc_par(const std::vector<i64>& I, const std::vector<i64>& J,
+                           const std::vector<double>& V, i64 m, i64 n, CSC& A, int T) {
+  if (T <= 1) { coo_to_csc(I, J, V, m, n, A); return; }
+  const i64 nz = (i64)I.size();
+  auto chunk = [](i64 len, int T_, int t, i64& a, i64& b) {
+    a = len * t / T_;
+    b = len * (t + 1) / T_;
+  };
+  // 1. bucket by row, COO order kept inside a row
+  std::vector<i64> cnt((size_t)T * m, 0), rptr(m + 1, 0);
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num();
+    i64 a, b;
+    chunk(nz, T, t, a, b);
+    i64* ct = &cnt[(size_t)t * m];
+    for (i64 k = a; k < b; ++k) ct[I[k] - 1]++;
+  }
+#pragma omp parallel for num_threads(T) schedule(static)
+  for (i64 r = 0; r < m; ++r) {
+    i64 s = 0;
+    for (int t = 0; t < T; ++t) s += cnt[(size_t)t * m + r];
+    rptr[r + 1] = s;
+  }
+  for (i64 r = 0; r < m; ++r) rptr[r + 1] += rptr[r];
+#pragma omp parallel for num_threads(T) schedule(static)
+  for (i64 r = 0; r < m; ++r) {
+    i64 s = rptr[r];
+    for (int t = 0; t < T; ++t) {
+      const i64 v = cnt[(size_t)t * m + r];
+      cnt[(size_t)t * m + r] = s;
+      s += v;
     }
-    // COO entry k = i + nV*j : row Vdofs[i], col Udofs[j]  (_cartesian_product :1357-1370)
-    for (int j = 0; j < nl; ++j)
-      for (int i = 0; i < nl; ++i) {
-        I.push_back(iglob[offset[c] + i]);
-        J.push_back(iglob[offset[c] + j]);
-        X.push_back(acc[(size_t)j * nl + i]);
+  }
+  std::vector<i64> cj(nz);
+  std::vector<double> cv(nz);
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num();
+    i64 a, b;
+    chunk(nz, T, t, a, b);
+    i64* ct = &cnt[(size_t)t * m];
+    for (i64 k = a; k < b; ++k) {
+      const i64 p = ct[I[k] - 1]++;
+      cj[p] = J[k];
+      cv[p] = V[k];
+    }
+  }
+  // 2. duplicates of a row combined in order: count, then write
+  std::vector<i64> uptr(m + 1, 0);
+#pragma omp parallel num_threads(T)
+  {
+    std::vector<i64> mark(n + 1, -1);
+#pragma omp for schedule(static)
+    for (i64 r = 0; r < m; ++r) {
+      i64 u = 0;
+      for (i64 p = rptr[r]; p < rptr[r + 1]; ++p)
+        if (mark[cj[p]] != r) { mark[cj[p]] = r; ++u; }
+      uptr[r + 1] = u;
+    }
+  }
+  for (i64 r = 0; r < m; ++r) uptr[r + 1] += uptr[r];
+  const i64 nu = uptr[m];
+  std::vector<i64> uc(nu);
+  std::vector<double> uv(nu);
+#pragma omp parallel num_threads(T)
+  {
+    std::vector<i64> mark(n + 1, -1), slot(n + 1, 0);
+#pragma omp for schedule(static)
+    for (i64 r = 0; r < m; ++r) {
+      i64 o = uptr[r];
+      for (i64 p = rptr[r]; p < rptr[r + 1]; ++p) {
+        const i64 c = cj[p];
+        if (mark[c] == r) uv[slot[c]] = uv[slot[c]] + cv[p];
+        else { mark[c] = r; slot[c] = o; uc[o] = c; uv[o] = cv[p]; ++o; }
+      }
+    }
+  }
+  // 3. transpose to CSC: threads own runs of rows, rows stay sorted per column
+  std::vector<i64> ccnt((size_t)T * n, 0);
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num();
+    i64 a, b;
+    chunk(m, T, t, a, b);
+    i64* ct = &ccnt[(size_t)t * n];
+    for (i64 k = uptr[a]; k < uptr[b]; ++k) ct[uc[k] - 1]++;
+  }
+  A.m = m;
+  A.n = n;
+  A.colptr.assign(n + 1, 1);
+  std::vector<i64> tot(n, 0);
+#pragma omp parallel for num_threads(T) schedule(static)
+  for (i64 j = 0; j < n; ++j) {
+    i64 s = 0;
+    for (int t = 0; t < T; ++t) s += ccnt[(size_t)t * n + j];
+    tot[j] = s;
+  }
+  for (i64 j = 0; j < n; ++j) A.colptr[j + 1] = A.colptr[j] + tot[j];
+#pragma omp parallel for num_threads(T) schedule(static)
+  for (i64 j = 0; j < n; ++j) {
+    i64 s = A.colptr[j] - 1;
+    for (int t = 0; t < T; ++t) {
+      const i64 v = ccnt[(size_t)t * n + j];
+      ccnt[(size_t)t * n + j] = s;
+      s += v;
+    }
+  }
+  A.rowval.assign(nu, 0);
+  A.nzval.assign(nu, 0.0);
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num();
+    i64 a, b;
+    chunk(m, T, t, a, b);
+    i64* ct = &ccnt[(size_t)t * n];
+    for (i64 r = a; r < b; ++r)
+      for (i64 k = uptr[r]; k < uptr[r + 1]; ++k) {
+        const i64 p = ct[uc[k] - 1]++;
+        A.rowval[p] = r + 1;
+        A.nzval[p] = uv[k];
       }
   }
 }
 
-// assemble_linear with a source term ∫ f v dΩ (assembler.jl:448-533, 855-865).
+// assemble_linear with a source term ∫ f v dΩ (assembler.jl:448-533, 855-865):
+// b[dof] += Σ_q w|J| f φ_i, cells in element order.  With nthreads > 1 the
+// element vectors are computed in parallel and added in element order, so b is
+// the serial result bit for bit.
 static void assemble_source(const Mesh& m, const std::vector<i64>& iglob, const std::vector<i64>& offset,
-                            int order, int fsrc, double fconst, int degree, std::vector<double>& b) {
+                            int order, int fsrc, double fconst, int degree, std::vector<double>& b,
+                            int nthreads = 1) {
   std::vector<i64> cells = element_order(m);
+  const i64 ncl = (i64)cells.size();
   int d = m.dim;
-  for (i64 c : cells) {
-    int ct = m.ctype[c];
-    Shape s = shape_of(ct);
-    int nl = ndofs_lagrange(s, order);
-    std::vector<double> w, xq;
-    int nq = tensor_rule(s, degree, Q_LEGENDRE, w, xq);
-    std::vector<i64> nodes = m.cell(c);
-    std::vector<double> Xc(nodes.size() * d);
-    for (size_t k = 0; k < nodes.size(); ++k)
-      for (int a = 0; a < d; ++a) Xc[k * d + a] = m.coords[nodes[k] * d + a];
-    std::vector<double> acc(nl, 0.0), val(nl), rg(nl * d);
-    for (int q = 0; q < nq; ++q) {
-      Geo g;
-      geometry(ct, Xc.data(), &xq[q * d], g);
-      tensor_basis(s, order, &xq[q * d], val.data(), rg.data());
-      double fv = eval_source(fsrc, fconst, g.x);
-      for (int i = 0; i < nl; ++i) {
-        double wu = w[q] * (g.detabs * (fv * val[i]));
-        acc[i] = (q == 0) ? wu : acc[i] + wu;
+  std::vector<i64> start(ncl + 1, 0);
+  for (i64 k = 0; k < ncl; ++k)
+    start[k + 1] = start[k] + ndofs_lagrange(shape_of(m.ctype[cells[k]]), order);
+  std::vector<double> ev(start[ncl]);
+  ShapeRules R0;
+  R0.degree = degree;
+  for (int ct : m.ctype) R0.fill(shape_of(ct));
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+  {
+    ShapeRules R = R0;
+#pragma omp for schedule(static)
+    for (i64 k = 0; k < ncl; ++k) {
+      const i64 c = cells[k];
+      int ct = m.ctype[c];
+      Shape s = shape_of(ct);
+      int nl = ndofs_lagrange(s, order);
+      const std::vector<double>& w = R.weights(s);
+      const std::vector<double>& xq = R.points(s);
+      int nq = R.count(s);
+      const i64* nodes = &m.c2n[m.c2n_ptr[c]];
+      const i64 nn = m.c2n_ptr[c + 1] - m.c2n_ptr[c];
+      double Xc[27 * 3];
+      for (i64 q = 0; q < nn; ++q)
+        for (int a = 0; a < d; ++a) Xc[q * d + a] = m.coords[nodes[q] * d + a];
+      std::vector<double> val(nl), rg(nl * d);
+      double* acc = &ev[start[k]];
+      for (int q = 0; q < nq; ++q) {
+        Geo g;
+        geometry(ct, Xc, &xq[q * d], g);
+        tensor_basis(s, order, &xq[q * d], val.data(), rg.data());
+        double fv = eval_source(fsrc, fconst, g.x);
+        for (int i = 0; i < nl; ++i) {
+          double wu = w[q] * (g.detabs * (fv * val[i]));
+          acc[i] = (q == 0) ? wu : acc[i] + wu;
+        }
       }
     }
-    for (int i = 0; i < nl; ++i) b[iglob[offset[c] + i] - 1] += acc[i];
+  }
+  for (i64 k = 0; k < ncl; ++k) {
+    const i64 c = cells[k];
+    for (i64 i = 0; i < start[k + 1] - start[k]; ++i)
+      b[iglob[offset[c] + i] - 1] += ev[start[k] + i];
   }
 }
 
@@ -1290,13 +1512,39 @@ static Faces build_faces(const Mesh& m) {
 // on CellDomain and InteriorFaceDomain (assemble_linear assembler.jl:448-533,
 // face update :768-800, side⁻/side⁺ eval_point.jl:164-206, normal
 // mapping.jl:843-855, face metric :279-311).  u, r: ndofs x nfields, field fastest.
-static void dg_transport(const Mesh& m, int order, const double* cvec, int vol_degree,
-                         int face_degree, int nfields, const double* u, double* r) {
+// Numbering and faces of a DG transport residual, built once per mesh (the
+// reference builds them with the FE space and the mesh).
+struct DGSetup {
   std::vector<i64> iglob, offset;
+  i64 ndofs = 0;
+  Faces F;
+  std::vector<size_t> inner;  // inner_faces (mesh.jl:552), creation order
+};
+
+static void dg_setup(const Mesh& m, int order, DGSetup& S) {
   std::vector<int> nd;
-  i64 ndofs = dofhandler(m, order, false, iglob, offset, nd);
+  S.ndofs = dofhandler(m, order, false, S.iglob, S.offset, nd);
+  S.F = build_faces(m);
+  S.inner.clear();
+  for (size_t f = 0; f < S.F.f2n.size(); ++f)
+    if (S.F.f2c[f][1] >= 0) S.inner.push_back(f);
+}
+
+// l(v) = ∫((c*u)⋅∇(v))dΩ - ∫(upwind(c⋅n) u jump(v))dΓ for nfields fields
+// (u, r: [ndofs][nfields]).  Faces first (assemble_linear runs a
+// MultiIntegration last-first, assembler.jl:496-516), in face order, then the
+// cells.  With nthreads > 1 a run of faces is computed in parallel and added
+// in face order, and cells (disjoint DG dofs) in parallel: the serial result
+// bit for bit.
+static void dg_residual(const Mesh& m, const DGSetup& S, int order, const double* cvec,
+                        int vol_degree, int face_degree, int nfields, const double* u, double* r,
+                        int nthreads) {
+  const std::vector<i64>& iglob = S.iglob;
+  const std::vector<i64>& offset = S.offset;
+  const Faces& F = S.F;
+  const int T = nthreads > 0 ? nthreads : 1;
   int nl = ndofs_lagrange(CUBE, order);
-  std::fill(r, r + ndofs * nfields, 0.0);
+  std::fill(r, r + S.ndofs * nfields, 0.0);
   auto cellX = [&](i64 c) {
     std::vector<i64> nodes = m.cell(c);
     std::vector<double> X(nodes.size() * 3);
@@ -1304,76 +1552,93 @@ static void dg_transport(const Mesh& m, int order, const double* cvec, int vol_d
       for (int a = 0; a < 3; ++a) X[k * 3 + a] = m.coords[nodes[k] * 3 + a];
     return X;
   };
-  // faces first: assemble_linear runs a MultiIntegration last-first
-  Faces F = build_faces(m);
   std::vector<double> wf, xf;
   tensor_rule(SQUARE, face_degree, Q_LEGENDRE, wf, xf);
   int nqf = (int)wf.size();
-  std::vector<double> val_n(nl), val_p(nl), grd(nl * 3);
-  for (size_t f = 0; f < F.f2n.size(); ++f) {
-    if (F.f2c[f][1] < 0) continue;  // inner_faces (mesh.jl:552)
-    i64 cn = F.f2c[f][0], cp = F.f2c[f][1];
-    std::vector<i64> c2n_n = m.cell(cn), c2n_p = m.cell(cp);
-    int sn = cell_side(m.ctype[cn], c2n_n, F.f2n[f]);
-    int sp = cell_side(m.ctype[cp], c2n_p, F.f2n[f]);
-    // permutation of side⁺ (eval_point.jl:174-206) on the cube vertices
-    int perm[4];
-    for (int k = 0; k < 4; ++k) {
-      i64 gp = c2n_p[kCubeF[sp][k]];
-      perm[k] = -1;
-      for (int q = 0; q < 4; ++q)
-        if (c2n_n[kCubeF[sn][q]] == gp) perm[k] = q;
-    }
-    std::vector<double> Xn = cellX(cn), Xp = cellX(cp);
-    std::vector<double> Xf(F.f2n[f].size() * 3);
-    for (size_t k = 0; k < F.f2n[f].size(); ++k)
-      for (int a = 0; a < 3; ++a) Xf[k * 3 + a] = m.coords[F.f2n[f][k] * 3 + a];
-    std::vector<double> acc_n((size_t)nl * nfields, 0.0), acc_p((size_t)nl * nfields, 0.0);
-    for (int q = 0; q < nqf; ++q) {
-      double metric = F.ftype[f] == QUAD9 ? face_metric_quad9(Xf.data(), &xf[2 * q])
-                                          : face_metric_quad4(Xf.data(), &xf[2 * q]);
-      double xin[3], xip[3];
-      mapping_face_cube(sn, nullptr, &xf[2 * q], xin);
-      mapping_face_cube(sp, perm, &xf[2 * q], xip);
-      Geo gn;
-      geometry3(m.ctype[cn], Xn.data(), xin, gn);
-      // normal: normalize(Jinv^T n_ref)
-      double nv[3];
-      for (int a = 0; a < 3; ++a)
-        nv[a] = gn.Jinv[0 * 3 + a] * kCubeN[sn][0] + gn.Jinv[1 * 3 + a] * kCubeN[sn][1] +
-                gn.Jinv[2 * 3 + a] * kCubeN[sn][2];
-      double nn = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-      for (int a = 0; a < 3; ++a) nv[a] /= nn;
-      double cdn = (cvec[0] * nv[0] + cvec[1] * nv[1]) + cvec[2] * nv[2];
-      tensor_basis(CUBE, order, xin, val_n.data(), grd.data());
-      tensor_basis(CUBE, order, xip, val_p.data(), grd.data());
-      for (int fi = 0; fi < nfields; ++fi) {
-        double un = 0, up = 0;
-        for (int j = 0; j < nl; ++j) {
-          un += u[(iglob[offset[cn] + j] - 1) * nfields + fi] * val_n[j];
-          up += u[(iglob[offset[cp] + j] - 1) * nfields + fi] * val_p[j];
-        }
-        double flux = cdn > 0 ? cdn * un : cdn * up;
-        for (int i = 0; i < nl; ++i) {
-          // -∫ flux jump(v): n side -flux φ⁻, p side +flux φ⁺
-          double gn_i = -(flux * val_n[i]);
-          double gp_i = -(flux * (-val_p[i]));
-          acc_n[(size_t)i * nfields + fi] += wf[q] * (metric * gn_i);
-          acc_p[(size_t)i * nfields + fi] += wf[q] * (metric * gp_i);
+  const size_t blk = (size_t)nl * nfields;
+  const size_t nin = S.inner.size();
+  const size_t run = 4096;
+  std::vector<double> facc(std::min(run, nin) * 2 * blk);
+  for (size_t f0 = 0; f0 < nin; f0 += run) {
+    const size_t f1 = std::min(nin, f0 + run);
+#pragma omp parallel for num_threads(T) schedule(static)
+    for (size_t k = f0; k < f1; ++k) {
+      const size_t f = S.inner[k];
+      std::vector<double> val_n(nl), val_p(nl), grd(nl * 3);
+      i64 cn = F.f2c[f][0], cp = F.f2c[f][1];
+      std::vector<i64> c2n_n = m.cell(cn), c2n_p = m.cell(cp);
+      int sn = cell_side(m.ctype[cn], c2n_n, F.f2n[f]);
+      int sp = cell_side(m.ctype[cp], c2n_p, F.f2n[f]);
+      // permutation of side⁺ (eval_point.jl:174-206) on the cube vertices
+      int perm[4];
+      for (int a = 0; a < 4; ++a) {
+        i64 gp = c2n_p[kCubeF[sp][a]];
+        perm[a] = -1;
+        for (int q = 0; q < 4; ++q)
+          if (c2n_n[kCubeF[sn][q]] == gp) perm[a] = q;
+      }
+      std::vector<double> Xn = cellX(cn);
+      std::vector<double> Xf(F.f2n[f].size() * 3);
+      for (size_t a = 0; a < F.f2n[f].size(); ++a)
+        for (int b = 0; b < 3; ++b) Xf[a * 3 + b] = m.coords[F.f2n[f][a] * 3 + b];
+      double* acc_n = &facc[(k - f0) * 2 * blk];
+      double* acc_p = acc_n + blk;
+      std::fill(acc_n, acc_n + 2 * blk, 0.0);
+      for (int q = 0; q < nqf; ++q) {
+        double metric = F.ftype[f] == QUAD9 ? face_metric_quad9(Xf.data(), &xf[2 * q])
+                                            : face_metric_quad4(Xf.data(), &xf[2 * q]);
+        double xin[3], xip[3];
+        mapping_face_cube(sn, nullptr, &xf[2 * q], xin);
+        mapping_face_cube(sp, perm, &xf[2 * q], xip);
+        Geo gn;
+        geometry3(m.ctype[cn], Xn.data(), xin, gn);
+        // normal: normalize(Jinv^T n_ref)
+        double nv[3];
+        for (int a = 0; a < 3; ++a)
+          nv[a] = gn.Jinv[0 * 3 + a] * kCubeN[sn][0] + gn.Jinv[1 * 3 + a] * kCubeN[sn][1] +
+                  gn.Jinv[2 * 3 + a] * kCubeN[sn][2];
+        double nn = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+        for (int a = 0; a < 3; ++a) nv[a] /= nn;
+        double cdn = (cvec[0] * nv[0] + cvec[1] * nv[1]) + cvec[2] * nv[2];
+        tensor_basis(CUBE, order, xin, val_n.data(), grd.data());
+        tensor_basis(CUBE, order, xip, val_p.data(), grd.data());
+        for (int fi = 0; fi < nfields; ++fi) {
+          double un = 0, up = 0;
+          for (int j = 0; j < nl; ++j) {
+            un += u[(iglob[offset[cn] + j] - 1) * nfields + fi] * val_n[j];
+            up += u[(iglob[offset[cp] + j] - 1) * nfields + fi] * val_p[j];
+          }
+          double flux = cdn > 0 ? cdn * un : cdn * up;
+          for (int i = 0; i < nl; ++i) {
+            // -∫ flux jump(v): n side -flux φ⁻, p side +flux φ⁺
+            double gn_i = -(flux * val_n[i]);
+            double gp_i = -(flux * (-val_p[i]));
+            acc_n[(size_t)i * nfields + fi] += wf[q] * (metric * gn_i);
+            acc_p[(size_t)i * nfields + fi] += wf[q] * (metric * gp_i);
+          }
         }
       }
     }
-    for (int i = 0; i < nl; ++i)
-      for (int fi = 0; fi < nfields; ++fi) {
-        r[(iglob[offset[cn] + i] - 1) * nfields + fi] += acc_n[(size_t)i * nfields + fi];
-        r[(iglob[offset[cp] + i] - 1) * nfields + fi] += acc_p[(size_t)i * nfields + fi];
-      }
+    for (size_t k = f0; k < f1; ++k) {
+      const size_t f = S.inner[k];
+      const i64 cn = F.f2c[f][0], cp = F.f2c[f][1];
+      const double* acc_n = &facc[(k - f0) * 2 * blk];
+      const double* acc_p = acc_n + blk;
+      for (int i = 0; i < nl; ++i)
+        for (int fi = 0; fi < nfields; ++fi) {
+          r[(iglob[offset[cn] + i] - 1) * nfields + fi] += acc_n[(size_t)i * nfields + fi];
+          r[(iglob[offset[cp] + i] - 1) * nfields + fi] += acc_p[(size_t)i * nfields + fi];
+        }
+    }
   }
   // volume: ∫((c*u)⋅∇(v))dΩ
   std::vector<double> wv, xv;
   int nqv = tensor_rule(CUBE, vol_degree, Q_LEGENDRE, wv, xv);
-  std::vector<double> val(nl), rg(nl * 3), pg(nl * 3);
-  for (i64 c : element_order(m)) {
+  std::vector<i64> cells = element_order(m);
+#pragma omp parallel for num_threads(T) schedule(static)
+  for (size_t k = 0; k < cells.size(); ++k) {
+    const i64 c = cells[k];
+    std::vector<double> val(nl), rg(nl * 3), pg(nl * 3);
     std::vector<double> Xc = cellX(c);
     std::vector<double> acc((size_t)nl * nfields, 0.0);
     for (int q = 0; q < nqv; ++q) {
@@ -1395,6 +1660,13 @@ static void dg_transport(const Mesh& m, int order, const double* cvec, int vol_d
       for (int fi = 0; fi < nfields; ++fi)
         r[(iglob[offset[c] + i] - 1) * nfields + fi] += acc[(size_t)i * nfields + fi];
   }
+}
+
+static void dg_transport(const Mesh& m, int order, const double* cvec, int vol_degree,
+                         int face_degree, int nfields, const double* u, double* r) {
+  DGSetup S;
+  dg_setup(m, order, S);
+  dg_residual(m, S, order, cvec, vol_degree, face_degree, nfields, u, r, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1569,6 +1841,77 @@ int or_geometry27(const double* X, const double* xi, double* J, double* Jinv, do
   *detabs = g.detabs;
   std::copy(g.x, g.x + 3, x);
   return 0;
+}
+
+// ---- timing of the reference algorithm (the CPU baseline) -------------------
+// The numbering (and for DG the faces) is built once, untimed: the reference
+// builds it with the FE space.  Then reps + 1 runs on nthreads threads, the
+// first a warm-up, each timed with steady_clock; the result of the last run is
+// returned.  Every thread count gives the serial result bit for bit.
+static double seconds_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// t[2k]: COO build, t[2k+1]: sparse(), for k = 0 .. reps (k = 0 the warm-up).
+void* or_bilinear_timed(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                        const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n, int order,
+                        int continuous, int nterms, const int* terms, const double* eta,
+                        int degree, int nthreads, int reps, double* t) {
+  try {
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    Result* r = new Result();
+    std::vector<int> nd;
+    r->ndofs = dofhandler(m, order, continuous != 0, r->iglob, r->offset, nd);
+    for (int k = 0; k <= reps; ++k) {
+      std::vector<i64> I, J;
+      std::vector<double> X;
+      auto t0 = std::chrono::steady_clock::now();
+      build_coo(m, r->iglob, r->offset, order, nterms, terms, eta, degree, nthreads, I, J, X);
+      t[2 * k] = seconds_since(t0);
+      t0 = std::chrono::steady_clock::now();
+      coo_to_csc_par(I, J, X, r->ndofs, r->ndofs, r->A, nthreads);
+      t[2 * k + 1] = seconds_since(t0);
+    }
+    return r;
+  } catch (std::exception& e) { g_err = e.what(); return nullptr; }
+}
+
+// t[k]: one assemble_linear (b zeroed, then the source term), k = 0 .. reps.
+void* or_linear_timed(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                      const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n, int order,
+                      int continuous, int fsrc, double fconst, int degree, int nthreads, int reps,
+                      double* t) {
+  try {
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    Result* r = new Result();
+    std::vector<int> nd;
+    r->ndofs = dofhandler(m, order, continuous != 0, r->iglob, r->offset, nd);
+    for (int k = 0; k <= reps; ++k) {
+      auto t0 = std::chrono::steady_clock::now();
+      r->b.assign(r->ndofs, 0.0);
+      assemble_source(m, r->iglob, r->offset, order, fsrc, fconst, degree, r->b, nthreads);
+      t[k] = seconds_since(t0);
+    }
+    return r;
+  } catch (std::exception& e) { g_err = e.what(); return nullptr; }
+}
+
+// t[k]: one DG transport residual of nfields fields, k = 0 .. reps; r as or_dg_transport.
+int64_t or_dg_timed(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                    const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n, int order,
+                    const double* c, int vol_degree, int face_degree, int nfields, const double* u,
+                    double* r, int nthreads, int reps, double* t) {
+  try {
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    DGSetup S;
+    dg_setup(m, order, S);
+    for (int k = 0; k <= reps; ++k) {
+      auto t0 = std::chrono::steady_clock::now();
+      dg_residual(m, S, order, c, vol_degree, face_degree, nfields, u, r, nthreads);
+      t[k] = seconds_since(t0);
+    }
+    return S.ndofs;
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
 
 int64_t or_result_ndofs(void* h) { return ((Result*)h)->ndofs; }

@@ -75,6 +75,15 @@ def lib():
         L.or_dg_transport.restype = C.c_int64
         L.or_geometry27.argtypes = [_d, _d, _d, _d, _d, _d]
         L.or_geometry27.restype = C.c_int
+        L.or_bilinear_timed.argtypes = mesh_args + [C.c_int, C.c_int, C.c_int, _i, C.c_void_p,
+                                                    C.c_int, C.c_int, C.c_int, _d]
+        L.or_bilinear_timed.restype = C.c_void_p
+        L.or_linear_timed.argtypes = mesh_args + [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                                  C.c_int, C.c_int, _d]
+        L.or_linear_timed.restype = C.c_void_p
+        L.or_dg_timed.argtypes = mesh_args + [C.c_int, _d, C.c_int, C.c_int, C.c_int, _d, _d,
+                                              C.c_int, C.c_int, _d]
+        L.or_dg_timed.restype = C.c_int64
         _LIB = L
     return _LIB
 
@@ -333,6 +342,82 @@ def dg_transport(mesh: OMesh, order, c, u, vol_degree, face_degree=None):
     if n != u.shape[0]:
         raise ValueError(f"u has {u.shape[0]} rows, the space has {n} dofs")
     return r
+
+
+def host_threads() -> int:
+    """Host threads for the all-core CPU baseline: OMP_NUM_THREADS when set,
+    else the cores this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def _csc_of(h):
+    try:
+        n = lib().or_result_ndofs(h)
+        nnz = lib().or_result_nnz(h)
+        colptr = np.zeros(n + 1, dtype=np.int64)
+        rowval = np.zeros(nnz, dtype=np.int64)
+        nzval = np.zeros(nnz)
+        lib().or_result_csc(h, colptr, rowval, nzval)
+    finally:
+        lib().or_result_free(h)
+    return colptr, rowval, nzval, int(n)
+
+
+def assemble_bilinear_timed(mesh: OMesh, order, terms, eta=None, degree=2, continuous=True,
+                            threads=1, reps=5):
+    """assemble_bilinear timed: the numbering once (untimed), then reps + 1
+    assemblies on `threads` threads, the first a warm-up.  Returns the CSC of
+    the last run and the per-run seconds of the COO build and of sparse(),
+    shape (reps, 2), warm-up dropped.  Any thread count gives the same CSC."""
+    t = np.ascontiguousarray([TERM[x] for x in terms], dtype=np.int32)
+    eta_p = None
+    if eta is not None:
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        eta_p = eta.ctypes.data
+    times = np.zeros(2 * (reps + 1))
+    h = lib().or_bilinear_timed(*mesh.args(), order, int(continuous), len(t), t, eta_p, degree,
+                                int(threads), int(reps), times)
+    if not h:
+        raise RuntimeError(_err())
+    return _csc_of(h), times.reshape(reps + 1, 2)[1:]
+
+
+def assemble_linear_timed(mesh: OMesh, order, source="sinsin", const=0.0, degree=2,
+                          continuous=True, threads=1, reps=5):
+    """assemble_linear timed as assemble_bilinear_timed: (b, seconds per run)."""
+    times = np.zeros(reps + 1)
+    h = lib().or_linear_timed(*mesh.args(), order, int(continuous), SOURCE[source], float(const),
+                              degree, int(threads), int(reps), times)
+    if not h:
+        raise RuntimeError(_err())
+    try:
+        b = np.zeros(lib().or_result_ndofs(h))
+        lib().or_result_vector(h, b)
+    finally:
+        lib().or_result_free(h)
+    return b, times[1:]
+
+
+def dg_transport_timed(mesh: OMesh, order, c, u, vol_degree, face_degree=None, threads=1,
+                       reps=5):
+    """dg_transport timed (numbering and faces built once, untimed): (r, seconds per run)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    if u.ndim == 1:
+        u = u[:, None]
+    r = np.zeros_like(u)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    fd = vol_degree if face_degree is None else face_degree
+    times = np.zeros(reps + 1)
+    n = lib().or_dg_timed(*mesh.args(), order, c, vol_degree, fd, u.shape[1], u, r, int(threads),
+                          int(reps), times)
+    if n < 0:
+        raise RuntimeError(_err())
+    if n != u.shape[0]:
+        raise ValueError(f"u has {u.shape[0]} rows, the space has {n} dofs")
+    return r, times[1:]
 
 
 def geometry27(cnodes, xi):

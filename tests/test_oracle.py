@@ -238,3 +238,42 @@ def test_linear_source_exact_p1_quad():
     mesh = O.rectangle_mesh(3, 3)
     b = O.assemble_linear(mesh, 1, "one", degree=2)
     np.testing.assert_allclose(sorted(b), sorted([1 / 16] * 4 + [1 / 8] * 4 + [1 / 4]), atol=1e-16)
+
+
+def test_threaded_oracle_equals_serial():
+    """The all-core CPU baseline (OpenMP COO build and sparse(), parallel
+    element vectors of the linear form, parallel DG faces and cells) returns
+    the serial oracle's results bit for bit, for any thread count."""
+    import bcubehip as bh
+    host = bh.BcubeBackendHostOnly()
+    n = 5
+    o = O.hexa_mesh(n + 1, n + 1, n + 1)
+    X = o.coords.reshape(-1, 3)
+    inner = np.all((X > 1e-12) & (X < 1 - 1e-12), axis=1)
+    X[inner] += np.random.default_rng(0).uniform(-0.15, 0.15, X.shape)[inner] / n
+    eta = np.random.default_rng(1).uniform(0.5, 2.0, o.ncells)
+    ref = O.assemble_bilinear(o, 2, ["stiffness", "mass"], eta=eta, degree=5)
+    m, _ = bh.mixed_tri_quad_mesh(12, block=4, p_split=0.5, seed=3, backend=host)
+    om = O.OMesh(2, m.coords, m.celltypes, m.c2n_ptr, m.c2n)
+    ref_mixed = O.assemble_bilinear(om, 3, ["stiffness", "mass"], degree=6)
+    o2 = O.rectangle_mesh(17, 17)
+    bref = O.assemble_linear(o2, 1, "sinsin", degree=2)
+    h27 = bh.hexa27_mesh(4, 4, 4, backend=host)
+    bh.sinusoidal_map(h27, 0.05, seed=0)
+    o27 = O.OMesh(h27.dim, h27.coords, h27.celltypes, h27.c2n_ptr, h27.c2n)
+    u = np.random.default_rng(2).normal(size=(27 * 27, 3))
+    rref = O.dg_transport(o27, 2, [1.0, 0.5, 0.25], u, 5)
+    for threads in (1, 3, 4):
+        got, t = O.assemble_bilinear_timed(o, 2, ["stiffness", "mass"], eta=eta, degree=5,
+                                           threads=threads, reps=1)
+        assert t.shape == (1, 2) and (t > 0).all()
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b)
+        got, _ = O.assemble_bilinear_timed(om, 3, ["stiffness", "mass"], degree=6,
+                                           threads=threads, reps=1)
+        for a, b in zip(got, ref_mixed):
+            assert np.array_equal(a, b)
+        b, _ = O.assemble_linear_timed(o2, 1, "sinsin", degree=2, threads=threads, reps=1)
+        assert np.array_equal(b, bref)
+        r, _ = O.dg_transport_timed(o27, 2, [1.0, 0.5, 0.25], u, 5, threads=threads, reps=1)
+        assert np.array_equal(r, rref)

@@ -2,13 +2,18 @@
 
 One step = one full assemble_bilinear of ∫η∇u⋅∇v dΩ + ∫u v dΩ into the
 device-resident CSR values (135,005,697 nnz), inputs already in HBM.  Prints
-one JSON line (see the field list in main()).
+one JSON line (see the field list in bench_c2()).
 
-    python bench.py [--steps K] [--warmup W] [--mode all|fp64|fp32|bf16x3] [--n 64] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--mode all|fp64|fp32|bf16x3] [--n 64]
+                    [--config all|c1|c2|c3|c4|c5|...] [--no-cpu] [--no-check]
 
 With --mode all (default) the FP64, FP32-MFMA and BF16x3 forms are timed in
 turn; `value` is the FP32-MFMA form (the headline), every mode is listed under
-"modes" with its error against the FP64 GPU result.
+"modes" with its error against the FP64 GPU result.  The default --config all
+adds the other workloads (C1 2D Poisson, C3 mixed tri/quad, C4 DG transport,
+C5 many-instance batch) under "configs", each with its dominant kernel's
+roofline, the CPU baseline (the oracle, 1 thread and all cores) and its matrix
+and solution errors against the oracle.
 """
 import argparse
 import json
@@ -68,52 +73,159 @@ def kernel_rooflines(mode, launches, kinds, steps, f_alg, nnz):
     return dom, [el, ga]
 
 
-def cpu_baseline(n_sample=32):
-    """Oracle (C++ restatement of the reference's serial CPU assembly), 1 thread."""
+# ---------------------------------------------------------------------------
+# CPU baseline and parity checks.  The CPU baseline is the oracle (a C++
+# restatement of the reference's serial assembly) on the host cores, on a
+# bounded sample of each workload: the numbering once (untimed, the reference
+# builds it with the FE space), then one warm-up and `reps` timed runs, median,
+# on 1 thread (the reference's BcubeBackendCPUSerial) and on every host core
+# (OpenMP, the same result bit for bit).  The sample's oracle result is also
+# the reference the GPU's values on the same sample are checked against.
+# ---------------------------------------------------------------------------
+def host_mesh(B, m):
+    from oracle import oracle as O
+    return O.OMesh(m.dim, m.coords, m.celltypes, m.c2n_ptr, m.c2n)
+
+
+def cpu_entry(run, work, unit, sample, reps=5):
+    """run(threads, reps) -> (seconds per run, {phase: seconds per run}).
+    `work` units per run; value = work / median seconds / 1e9."""
     from oracle import oracle as O
 
-    o = O.hexa_mesh(n_sample + 1, n_sample + 1, n_sample + 1)
-    rng = np.random.default_rng(0)
-    X = o.coords.reshape(-1, 3)
-    interior = np.all((X > 1e-12) & (X < 1 - 1e-12), axis=1)
-    X[interior] += rng.uniform(-0.15, 0.15, X.shape)[interior] / n_sample
-    eta = np.random.default_rng(1).uniform(0.5, 2.0, o.ncells)
-    t0 = time.perf_counter()
-    colptr, rowval, nzval, nd = O.assemble_bilinear(o, 2, ["stiffness", "mass"], eta=eta, degree=5)
-    dt = time.perf_counter() - t0
-    return {"value": nzval.size / dt / 1e9, "unit": "GNNZ/s", "cores": 1, "kind": "port",
-            "sample": f"hexa {n_sample}^3 Q2, stiffness+mass, COO + sparse(), "
-                      f"{nzval.size} nnz in {dt:.2f} s"}
+    nth = O.host_threads()
+    res = {}
+    for label, th in (("single_thread", 1), ("all_core", nth)):
+        if label == "all_core" and nth == 1:
+            res[label] = res["single_thread"]
+            continue
+        secs, phases = run(th, reps)
+        med = float(np.median(secs))
+        d = {"value": work / med / 1e9, "cores": th, "median_s": med, "runs": int(len(secs)),
+             "warmup_runs": 1}
+        d.update({k: float(np.median(v)) for k, v in phases.items()})
+        res[label] = d
+    a = res["all_core"]
+    out = {"value": a["value"], "unit": unit, "cores": a["cores"], "kind": "port",
+           "sample": sample}
+    out.update({k: v for k, v in a.items() if k not in ("value", "cores")})
+    out["single_thread"] = res["single_thread"]
+    out["host"] = {"cpu_count": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)),
+                   "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    return out
 
 
-def downstream_check(B, n=10):
-    """Error on the solution of the assembled system: the C2 workload (Q2,
-    jitter, η, ∫η∇u⋅∇v + ∫uv) at n^3 with b = ∫x v, solved by a direct sparse
-    solve with the oracle's matrix and with the GPU's FP64 and FP32-MFMA
-    matrices; relative 2-norm errors of the GPU solutions against the oracle's."""
+def bilinear_runs(parts, reps_out):
+    """run() for cpu_entry over one or more oracle problems
+    [(omesh, order, terms, eta, degree)]: the per-run seconds summed over the
+    parts; the last run's CSC of every part goes to reps_out."""
+    from oracle import oracle as O
+
+    def run(th, reps):
+        tot, coo, csc = 0.0, 0.0, 0.0
+        reps_out.clear()
+        for om, order, terms, eta, degree in parts:
+            r, t = O.assemble_bilinear_timed(om, order, terms, eta=eta, degree=degree,
+                                             threads=th, reps=reps)
+            reps_out.append(r)
+            coo = coo + t[:, 0]
+            csc = csc + t[:, 1]
+            tot = tot + t.sum(axis=1)
+        return tot, {"coo_build_s": coo, "sparse_s": csc}
+    return run
+
+
+def rel_err(got, ref):
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    return float(np.abs(got - ref).max() / np.abs(ref).max())
+
+
+def csc_err(plan, ref, mode):
+    """Max entry error of the plan's values in `mode` against an oracle CSC,
+    relative to max|A| (the structures must be identical)."""
+    cp, rv, nz, nd = ref
+    colptr, rowval = plan.csc_structure()
+    assert np.array_equal(colptr, cp - 1) and np.array_equal(rowval, rv - 1), "pattern differs"
+    return rel_err(plan.assemble_host(mode=mode), nz)
+
+
+def solve_err(plan, ref, b, modes=("fp64", "fp32")):
+    """Relative 2-norm error of the solutions of A x = b with the plan's
+    matrices against the solution with the oracle's matrix (direct solves)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
 
-    from oracle import oracle as O
+    cp, rv, nz, nd = ref
+    x_ref = spla.spsolve(sp.csc_matrix((nz, rv - 1, cp - 1), shape=(nd, nd)), b)
+    out = {}
+    for mode in modes:
+        A = sp.csc_matrix((plan.assemble_host(mode=mode), rv - 1, cp - 1), shape=(nd, nd))
+        x = spla.spsolve(A, b)
+        out[mode] = float(np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref))
+    return out
 
-    m = B.hexa_mesh(n + 1, n + 1, n + 1)
+
+def traffic_of(workload, kernel):
+    """Measured HBM bytes per dispatch of `kernel` on `workload` (rocprofv3
+    FETCH_SIZE/WRITE_SIZE, calibrated), from profiles/traffic.json, or None."""
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(tf):
+        return None
+    w = json.load(open(tf)).get("workloads", {}).get(workload, {})
+    k = {name.replace(" ", ""): v for name, v in w.get("kernels", {}).items()}
+    return k.get(kernel.replace(" ", ""))
+
+
+def c2_problem(B, n, backend=None):
+    """The C2 workload at n^3: hexa Q2, interior nodes jittered ±0.15h (seed 0),
+    η ~ U[0.5, 2] per cell (seed 1)."""
+    m = B.hexa_mesh(n + 1, n + 1, n + 1, backend=backend) if backend else \
+        B.hexa_mesh(n + 1, n + 1, n + 1)
     B.jitter_nodes(m, 0.15, 1.0 / n, seed=0)
     eta = np.random.default_rng(1).uniform(0.5, 2.0, m.ncells)
+    return m, eta
+
+
+def c2_plan(B, m, eta, build="host"):
     U = B.TrialFESpace(B.FunctionSpace("Lagrange", 2), m)
     V = B.TestFESpace(U)
     dΩ = B.Measure(B.CellDomain(m), 5)
     ηc = B.MeshCellData(eta)
-    plan = B.BilinearPlan(lambda u, v: B.integrate(ηc * B.dot(B.grad(u), B.grad(v)), dΩ)
-                          + B.integrate(u * v, dΩ), U, V)
-    b = B.assemble_linear(lambda v: B.integrate(B.PhysicalFunction(lambda x: x[:, 0]) * v, dΩ), V)
-    om = O.OMesh(m.dim, m.coords, m.celltypes, m.c2n_ptr, m.c2n)
-    cp, rv, nz, nd = O.assemble_bilinear(om, 2, ["stiffness", "mass"], eta=eta, degree=5)
-    x_ref = spla.spsolve(sp.csc_matrix((nz, rv - 1, cp - 1), shape=(nd, nd)), b)
-    out = {"workload": f"C2 at {n}^3 ({nd} dofs), b = ∫x v, direct solve"}
-    for mode in ("fp64", "fp32"):
-        x = spla.spsolve(plan.to_csc(plan.assemble_host(mode=mode)).tocsc(), b)
-        out[mode] = float(np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref))
-    return out
+    return B.BilinearPlan(lambda u, v: B.integrate(ηc * B.dot(B.grad(u), B.grad(v)), dΩ)
+                          + B.integrate(u * v, dΩ), U, V, build=build), U, V, dΩ
+
+
+def c2_cpu_and_checks(B, modes, timed=True, n_sample=32, n_solve=10):
+    """CPU baseline of C2 on an n_sample^3 sample (None unless timed); the
+    GPU's matrices on that sample against the oracle's; the solution of the
+    assembled system at n_solve^3 (b = ∫x v, direct solve) against the
+    oracle's solution."""
+    from oracle import oracle as O
+
+    m, eta = c2_problem(B, n_sample)
+    om = host_mesh(B, m)
+    nnz = workload(n_sample)[2]
+    refs = []
+    cpu = None
+    if timed:
+        cpu = cpu_entry(bilinear_runs([(om, 2, ["stiffness", "mass"], eta, 5)], refs), nnz,
+                        "GNNZ/s", f"hexa {n_sample}^3 Q2 (the same jitter and η seeds), "
+                                  f"∫η∇u⋅∇v + ∫uv: COO build + sparse() of {nnz} nnz, "
+                                  f"numbering excluded")
+    else:
+        refs.append(O.assemble_bilinear(om, 2, ["stiffness", "mass"], eta=eta, degree=5))
+    plan, _, _, _ = c2_plan(B, m, eta)
+    mat = {mode: csc_err(plan, refs[0], mode) for mode in modes}
+    mat["sample"] = f"hexa {n_sample}^3"
+    del plan
+    ms, eta_s = c2_problem(B, n_solve)
+    plan_s, _, _, _ = c2_plan(B, ms, eta_s)
+    oms = host_mesh(B, ms)
+    ref_s = O.assemble_bilinear(oms, 2, ["stiffness", "mass"], eta=eta_s, degree=5)
+    b = O.assemble_linear(oms, 2, "x", degree=5)
+    sol = solve_err(plan_s, ref_s, b, modes)
+    sol["workload"] = f"C2 at {n_solve}^3 ({ref_s[3]} dofs), b = ∫x v, direct solve"
+    return cpu, mat, sol
 
 
 def bench_c5(args, torch, B):
@@ -144,10 +256,12 @@ def bench_c5(args, torch, B):
         plan = B.BilinearPlan(form, U, B.TestFESpace(U))
         kd = torch.from_numpy(np.concatenate(kap)).to(dev)
         buckets[order] = (plan, kd, inst, kap)
-    out = {}
     res = {}
+    kern = {}
     for mode in ("fp64", "fp32"):
         tot_nnz, tot_ms = 0, 0.0
+        k_ms = {0: 0.0, 1: 0.0}
+        f_alg = 0
         for order, (plan, kd, _, _) in buckets.items():
             buf = torch.empty(plan.nnz, dtype=torch.float64, device=dev)
             for _ in range(args.warmup):
@@ -160,14 +274,39 @@ def bench_c5(args, torch, B):
             e1.record(stream)
             torch.cuda.synchronize()
             ms = e0.elapsed_time(e1) / args.steps
+            # per-kernel times in a separate pass (the events bracket each launch)
+            backend.set_timing(True)
+            for _ in range(args.steps):
+                plan.assemble_device(buf.data_ptr(), [kd.data_ptr(), None], mode)
+            torch.cuda.synchronize()
+            lt, lk = backend.timings_ex(cap=100000)
+            backend.set_timing(False)
+            lt = np.asarray(lt, dtype=np.float64)
+            for kind in (0, 1):
+                k_ms[kind] += float(lt[lk == kind].sum() / args.steps)
+            nl, nq = (order + 1) ** 2, (order + 1) ** 2
+            f_alg += int(plan.V.mesh.ncells) * 2 * nl * nl * 3 * nq
             tot_nnz += plan.nnz
             tot_ms += ms
             res[f"{mode}_Q{order}"] = {"nnz": plan.nnz, "ms": ms, "cells": int(plan.V.mesh.ncells),
                                        "tuning": plan.tuning(mode)}
+            del buf
         res[f"{mode}_aggregate_gnnz_per_s"] = tot_nnz / (tot_ms * 1e-3) / 1e9
+        el_ach = f_alg / (k_ms[0] * 1e-3) / 1e12
+        ga_ach = 8.0 * tot_nnz / (k_ms[1] * 1e-3) / 1e9
+        kern[mode] = [
+            {"bound": "mfma", "achieved": el_ach, "peak": PEAK_TFLOPS[mode], "unit": "TFLOP/s",
+             "frac": el_ach / PEAK_TFLOPS[mode], "kernel": "element kernels (Q1 and Q2 buckets)",
+             "avg_launch_ms": k_ms[0] / 2, "launches_per_step": 2, "flops_per_step": f_alg},
+            {"bound": "hbm", "achieved": ga_ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+             "frac": ga_ach / PEAK_HBM_GBS, "kernel": "row gathers (Q1 and Q2 buckets)",
+             "avg_launch_ms": k_ms[1] / 2, "launches_per_step": 2,
+             "alg_bytes_per_step": 8.0 * tot_nnz}]
     # per-instance latency (each instance its own plan), FP32 mode
     lat = []
     for order, (_, _, inst, kap) in buckets.items():
+        if args.no_check:  # profiling runs: only the batched launches
+            break
         for m, kk in zip(inst, kap):
             U = B.TrialFESpace(B.FunctionSpace("Lagrange", order), m)
             dΩ = B.Measure(B.CellDomain(m), 2 * order)
@@ -191,11 +330,65 @@ def bench_c5(args, torch, B):
            "config": {"workload": "C5: 2048 unit-square quad meshes, side~U{8..96} seed 3, "
                                   "Q1/Q2 alternating, κ~LogNormal(0,0.5), ∫κ∇u⋅∇v+∫uv"},
            "buckets": res,
-           "per_instance_latency_ms": {"p50": float(np.percentile(lat, 50)),
-                                       "p99": float(np.percentile(lat, 99)),
-                                       "max": float(lat.max()), "n": int(lat.size),
-                                       "mode": "fp32"}}
+           "per_instance_latency_ms": None if lat.size == 0 else
+           {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+            "max": float(lat.max()), "n": int(lat.size), "mode": "fp32"}}
+    k32 = kern["fp32"]
+    roof = dict(max(k32, key=lambda k: k["avg_launch_ms"]))
+    roof["traffic"] = traffic_of("c5_2048", roof["kernel"])
+    out["roofline"] = roof
+    out["kernels"] = k32
+    out["ms_per_step"] = res["fp32_Q1"]["ms"] + res["fp32_Q2"]["ms"]
+    out["fp32_speedup_vs_fp64"] = ((res["fp64_Q1"]["ms"] + res["fp64_Q2"]["ms"])
+                                   / out["ms_per_step"])
+    cpu, mat, sol = (None, None, None) if args.no_check else \
+        c5_cpu_and_checks(B, {o: (bk[2], bk[3]) for o, bk in buckets.items()},
+                          timed=not args.no_cpu)
+    if mat is not None:
+        out["matrix_rel_err_vs_oracle"] = mat
+        out["solution_rel_err_vs_oracle"] = sol
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    backend.set_stream(None)
     return out
+
+
+def c5_cpu_and_checks(B, inst_by_order, timed=True, n_sample=64):
+    """C5 on a sample: the first n_sample instances of each order (the same
+    sides and κ), batched per order as the GPU batches them.  The oracle's
+    assembly of both batches timed (summed); the GPU's batch matrices against
+    the oracle's; per order the solution of (K + M) x = ∫v (block diagonal:
+    every instance at once) against the oracle's."""
+    from oracle import oracle as O
+
+    parts, plans = [], []
+    for order, (inst, kap) in inst_by_order.items():
+        big, _, _ = B.batch_meshes(inst[:n_sample])
+        kc = np.concatenate(kap[:n_sample])
+        U = B.TrialFESpace(B.FunctionSpace("Lagrange", order), big)
+        dΩ = B.Measure(B.CellDomain(big), 2 * order)
+        kcd = B.MeshCellData(kc)
+        plans.append((order, big, B.BilinearPlan(
+            lambda u, v, kcd=kcd, dΩ=dΩ: B.integrate(kcd * B.dot(B.grad(u), B.grad(v)), dΩ)
+            + B.integrate(u * v, dΩ), U, B.TestFESpace(U))))
+        parts.append((host_mesh(B, big), order, ["stiffness", "mass"], kc, 2 * order))
+    nnz = sum(pl.nnz for _, _, pl in plans)
+    refs = []
+    cpu = None
+    if timed:
+        cpu = cpu_entry(bilinear_runs(parts, refs), nnz, "GNNZ/s",
+                        f"the first {n_sample} instances of each order (Q1 and Q2 batches, "
+                        f"{sum(pl.V.mesh.ncells for _, _, pl in plans)} cells): COO build + "
+                        f"sparse() of {nnz} nnz, numbering excluded")
+    else:
+        refs = [O.assemble_bilinear(om, o, t, eta=e, degree=d) for om, o, t, e, d in parts]
+    mat = {"sample": f"{n_sample} instances per order"}
+    sol = {"workload": f"{n_sample} instances per order, (K + M) x = ∫v, direct solve"}
+    for (order, big, plan), ref, part in zip(plans, refs, parts):
+        mat[f"Q{order}"] = {mode: csc_err(plan, ref, mode) for mode in ("fp64", "fp32")}
+        b = O.assemble_linear(part[0], order, "one", degree=2 * order)
+        sol[f"Q{order}"] = solve_err(plan, ref, b)
+    return cpu, mat, sol
 
 
 def c3_mesh(B, n=866, seed=0, backend=None):
@@ -216,25 +409,58 @@ def c1_problem(B, backend=None):
     return m
 
 
-def cpu_baseline_c1(reps=200):
-    """Oracle (serial restatement), 1 thread, on the whole C1 workload: the
-    stiffness matrix (COO + sparse()) and the right-hand side, repeated."""
-    import bcubehip as B
+def c1_cpu_and_checks(B, plan, U, m, timed=True, reps=21):
+    """C1 on the host: the oracle's whole step (stiffness COO + sparse() and
+    the sin πx sin πy right-hand side) on the full 64^2 workload, timed
+    (median of `reps` after a warm-up, 1 thread and all cores); the GPU's
+    matrix and right-hand side against it; and the solution of the Poisson
+    problem with u = 0 on the boundary (the system restricted to interior
+    dofs) with the GPU's system against the solution with the oracle's."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
     from oracle import oracle as O
 
-    m = c1_problem(B, backend=B.BcubeBackendHostOnly())
-    o = O.rectangle_mesh(65, 65)
-    o.coords = m.coords.copy()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        r = O.assemble_bilinear(o, 1, ["stiffness"], degree=2)
-        O.assemble_linear(o, 1, "sinsin", degree=2)
-    dt = (time.perf_counter() - t0) / reps
-    return {"value": r[2].size / dt / 1e9, "unit": "GNNZ/s", "cores": 1, "kind": "port",
-            "sample": f"the whole C1 step (64^2 Q1 stiffness, COO + sparse(), and the sin*sin "
-                      f"right-hand side; each oracle call also numbers the dofs), {reps} "
-                      f"repetitions, {dt * 1e3:.3f} ms each",
-            "ms_per_step": dt * 1e3}
+    om = host_mesh(B, m)
+    out = {}
+
+    def run(th, rp):
+        (A, tb) = O.assemble_bilinear_timed(om, 1, ["stiffness"], degree=2, threads=th, reps=rp)
+        b, tl = O.assemble_linear_timed(om, 1, "sinsin", degree=2, threads=th, reps=rp)
+        out["A"], out["b"] = A, b
+        return tb.sum(axis=1) + tl, {"coo_build_s": tb[:, 0], "sparse_s": tb[:, 1],
+                                     "rhs_s": tl}
+
+    cpu = None
+    if timed:
+        cpu = cpu_entry(run, plan.nnz, "GNNZ/s",
+                        "the whole C1 step at full size (64^2 jittered Q1: stiffness COO build "
+                        "+ sparse() and the sin πx sin πy right-hand side), numbering excluded",
+                        reps=reps)
+    else:
+        run(1, 1)
+    A_ref, b_ref = out["A"], out["b"]
+    mat = {mode: csc_err(plan, A_ref, mode) for mode in ("fp64", "fp32")}
+    dΩ = B.Measure(B.CellDomain(m), 2)
+    f = B.PhysicalFunction(lambda x: np.sin(np.pi * x[:, 0]) * np.sin(np.pi * x[:, 1]))
+    b = B.assemble_linear(lambda v: B.integrate(f * v, dΩ), B.TestFESpace(U))
+    mat["rhs"] = rel_err(b, b_ref)
+    X = B.dof_coordinates(U)
+    inner = np.flatnonzero(np.all((X > 1e-12) & (X < 1 - 1e-12), axis=1))
+    cp, rv, nz, nd = A_ref
+
+    def solve(vals, rhs):
+        K = sp.csc_matrix((vals, rv - 1, cp - 1), shape=(nd, nd))[inner][:, inner]
+        return spla.spsolve(K.tocsc(), rhs[inner])
+
+    x_ref = solve(nz, b_ref)
+    sol = {"workload": f"-Δu = sin πx sin πy, u = 0 on the boundary ({inner.size} interior dofs), "
+                       f"direct solve"}
+    for mode in ("fp64", "fp32"):
+        x = solve(plan.assemble_host(mode=mode), b)
+        sol[mode] = float(np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref))
+    exact = np.sin(np.pi * X[inner, 0]) * np.sin(np.pi * X[inner, 1]) / (2 * np.pi ** 2)
+    sol["oracle_vs_exact_solution"] = float(np.abs(x_ref - exact).max() / np.abs(exact).max())
+    return cpu, mat, sol
 
 
 def bench_c1(args, torch, B):
@@ -289,6 +515,35 @@ def bench_c1(args, torch, B):
     e1.record(stream)
     torch.cuda.synchronize()
     ms_graph = e0.elapsed_time(e1) / args.steps
+    # per-kernel times of one eager step: element kernel, row gather, the rest
+    backend.set_timing(True)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    lt, lk = backend.timings_ex(cap=100000)
+    backend.set_timing(False)
+    lt = np.asarray(lt, dtype=np.float64)
+    k_ms = {kind: float(lt[lk == kind].sum() / args.steps) for kind in (0, 1, 2)}
+    f_alg = m.ncells * 2 * 16 * 2 * 4  # dense ∇̂φᵀ G ∇̂φ per cell: nl=4, d=2, nq=4
+    el_ach = f_alg / (k_ms[0] * 1e-3) / 1e12
+    ga_ach = 8.0 * plan.nnz / (k_ms[1] * 1e-3) / 1e9
+    src_bytes = 8.0 * (U.get_ndofs() + m.ncells * 4)  # b written, f read per point
+    src_ach = src_bytes / (k_ms[2] * 1e-3) / 1e9
+    kernels = [
+        {"bound": "mfma", "achieved": el_ach, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+         "frac": el_ach / PEAK_TFLOPS["fp64"], "kernel": "element kernel (FP64)",
+         "avg_launch_ms": k_ms[0], "flops_per_launch": f_alg},
+        {"bound": "hbm", "achieved": ga_ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+         "frac": ga_ach / PEAK_HBM_GBS, "kernel": "k_gather_padded (row gather)",
+         "avg_launch_ms": k_ms[1], "alg_bytes_per_launch": 8.0 * plan.nnz},
+        {"bound": "hbm", "achieved": src_ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+         "frac": src_ach / PEAK_HBM_GBS, "kernel": "vector fill + coloured k_source_f64",
+         "avg_launch_ms": k_ms[2], "alg_bytes_per_launch": src_bytes}]
+    roof = dict(max(kernels, key=lambda k: k["avg_launch_ms"]))
+    roof["traffic"] = None
+    roof["note"] = "launch-latency bound: every kernel here runs a few microseconds"
+    cpu, mat, sol = (None, None, None) if args.no_check else \
+        c1_cpu_and_checks(B, plan, U, m, timed=not args.no_cpu)
     line = {"metric": "C1 2D Poisson Q1 64^2 assembly (matrix + right-hand side), GNNZ/s, FP64",
             "value": plan.nnz / (ms * 1e-3) / 1e9, "unit": "GNNZ/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -301,26 +556,56 @@ def bench_c1(args, torch, B):
             "note": "launch-latency bound (4,096 cells). The graph figure replays one capture "
                     "of the matrix and right-hand-side launches (the vector then accumulates "
                     "over replays, as assemble_linear! does); the eager step also zeroes it."}
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline_c1()
+    line["roofline"] = roof
+    line["kernels"] = kernels
+    if mat is not None:
+        line["matrix_rel_err_vs_oracle"] = mat
+        line["solution_rel_err_vs_oracle"] = sol
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     backend.set_stream(None)
     return line
 
 
-def cpu_baseline_c3(n_sample=400):
-    """Oracle (serial restatement), 1 thread: a bounded sample of the C3 workload,
-    the P2/Q2 space of an n_sample^2 mixed mesh with the same block layout."""
-    import bcubehip as B
+def c3_cpu_and_checks(B, timed=True, n_sample=192):
+    """C3 on an n_sample^2 sample with the same block layout and seeds: the
+    oracle's assembly of every order bucket (continuous P_p/Q_p on the union of
+    the blocks of order p, ∫∇u⋅∇v + ∫uv, degree max(2p, 4)) timed, summed over
+    the buckets; the GPU's bucket matrices on the sample against the oracle's;
+    and per bucket the solution of (K + M) x = ∫x v against the oracle's."""
     from oracle import oracle as O
 
-    m, _ = c3_mesh(B, n_sample, backend=B.BcubeBackendHostOnly())
-    o = O.OMesh(2, m.coords, m.celltypes, m.c2n_ptr, m.c2n)
-    t0 = time.perf_counter()
-    r = O.assemble_bilinear(o, 2, ["stiffness", "mass"], degree=4)
-    dt = time.perf_counter() - t0
-    return {"value": r[2].size / dt / 1e9, "unit": "GNNZ/s", "cores": 1, "kind": "port",
-            "sample": f"{n_sample}^2 mixed tri/quad mesh ({m.ncells} cells), P2/Q2, stiffness+mass, "
-                      f"COO + sparse(): {r[2].size} nnz in {dt:.2f} s"}
+    m, order = c3_mesh(B, n_sample)
+    parts, plans = [], []
+    for p in (1, 2, 3, 4):
+        cells = np.flatnonzero(order == p)
+        if cells.size == 0:
+            continue
+        sub = B.sub_mesh(m, cells)
+        deg = max(2 * p, 4)
+        U = B.TrialFESpace(B.FunctionSpace("Lagrange", p), sub)
+        dΩ = B.Measure(B.CellDomain(sub), deg)
+        plans.append((p, deg, sub, B.BilinearPlan(
+            lambda u, v, dΩ=dΩ: B.integrate(B.dot(B.grad(u), B.grad(v)), dΩ)
+            + B.integrate(u * v, dΩ), U, B.TestFESpace(U))))
+        parts.append((host_mesh(B, sub), p, ["stiffness", "mass"], None, deg))
+    refs = []
+    nnz = sum(pl.nnz for _, _, _, pl in plans)
+    cpu = None
+    if timed:
+        cpu = cpu_entry(bilinear_runs(parts, refs), nnz, "GNNZ/s",
+                        f"{n_sample}^2 mixed tri/quad mesh ({m.ncells} cells, same block layout "
+                        f"and order seeds), every order bucket: COO build + sparse() of {nnz} "
+                        f"nnz, numbering excluded")
+    else:
+        refs = [O.assemble_bilinear(om, p, t, degree=d) for om, p, t, _, d in parts]
+    mat, sol = {"sample": f"{n_sample}^2"}, {"workload": f"{n_sample}^2 sample, per bucket "
+                                                          f"(K + M) x = ∫x v, direct solve"}
+    for (p, deg, sub, plan), ref, part in zip(plans, refs, parts):
+        mat[f"P{p}"] = {mode: csc_err(plan, ref, mode) for mode in ("fp64", "fp32")}
+        b = O.assemble_linear(part[0], p, "x", degree=deg)
+        sol[f"P{p}"] = solve_err(plan, ref, b)
+    return cpu, mat, sol
 
 
 def bench_c3(args, torch, B):
@@ -406,7 +691,7 @@ def bench_c3(args, torch, B):
     db = r["buckets"][dom_p]
     roof = dict(db["gather"] if db["gather"]["ms"] >= db["element_kernels"]["ms"] else db["element_kernels"])
     roof["kernel"] = f"{'row gather' if roof['bound'] == 'hbm' else 'element kernels'} of bucket {dom_p}"
-    roof["traffic"] = None
+    roof["traffic"] = traffic_of(f"c3_{n}", roof["kernel"])
     line = {
         "metric": "mixed tri/quad assembly (C3), aggregate GNNZ/s over order buckets, FP32-MFMA",
         "value": r["gnnz_per_s"], "unit": "GNNZ/s", "n_gpus": 1, "steps": args.steps,
@@ -421,28 +706,87 @@ def bench_c3(args, torch, B):
         "fp32_speedup_vs_fp64": res["fp64"]["ms_per_step"] / r["ms_per_step"],
         "modes": res,
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline_c3()
+    cpu, mat, sol = (None, None, None) if args.no_check else \
+        c3_cpu_and_checks(B, timed=not args.no_cpu)
+    if mat is not None:
+        line["matrix_rel_err_vs_oracle"] = mat
+        line["solution_rel_err_vs_oracle"] = sol
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     backend.set_stream(None)
     return line
 
 
-def cpu_baseline_c4(n_sample=26):
-    """Oracle DG transport residual (serial C++ restatement), 1 thread, on a
-    bounded sample of the C4 workload: n_sample^3 curved Hexa27 cells, Q3, 8 fields."""
-    import bcubehip as B
+def c4_cpu_and_checks(B, torch, timed=True, n_sample=16, n_rk=4, rk_steps=3):
+    """C4 on an n_sample^3 sample (curved Hexa27, DG Q3, 8 N(0,1) fields, the
+    same seeds): the oracle's residual timed; the GPU's FP64 and FP32
+    residuals on the sample against it; and `rk_steps` RK4 steps of
+    M du/dt = l(u) on the GPU against the same steps on the CPU with the
+    oracle's residual and a sparse solve with the oracle's DG mass matrix
+    (n_rk^3 cells, 2 fields)."""
+    import scipy.sparse.linalg as spla
     from oracle import oracle as O
 
-    m = B.hexa27_mesh(n_sample + 1, n_sample + 1, n_sample + 1, backend=B.BcubeBackendHostOnly())
+    c = np.array([1.0, 0.5, 0.25])
+    m = B.hexa27_mesh(n_sample + 1, n_sample + 1, n_sample + 1)
     B.sinusoidal_map(m, 0.05, seed=0)
-    om = O.OMesh(3, m.coords, m.celltypes, m.c2n_ptr, m.c2n)
-    u = np.random.default_rng(2).standard_normal((m.ncells * 64, 8))
-    t0 = time.perf_counter()
-    O.dg_transport(om, 3, np.array([1.0, 0.5, 0.25]), u, 7)
-    dt = time.perf_counter() - t0
-    return {"value": u.size / dt / 1e9, "unit": "Gentries/s", "cores": 1, "kind": "port",
-            "sample": f"{n_sample}^3 curved Hexa27, DG Q3, 8 fields, vol+upwind faces: "
-                      f"{u.size} residual entries in {dt:.2f} s"}
+    om = host_mesh(B, m)
+    V = B.TrialFESpace(B.FunctionSpace("Lagrange", 3), m, "discontinuous")
+    u = np.random.default_rng(2).standard_normal((8, V.get_ndofs()))
+    out = {}
+
+    def run(th, rp):
+        r, t = O.dg_transport_timed(om, 3, c, u.T, 7, threads=th, reps=rp)
+        out["r"] = r.T
+        return t, {}
+
+    cpu = None
+    if timed:
+        cpu = cpu_entry(run, u.size, "Gentries/s",
+                        f"{n_sample}^3 curved Hexa27 ({m.ncells} cells), DG Q3, 8 fields, "
+                        f"volume + upwind interior faces: one residual of {u.size} entries, "
+                        f"numbering and faces excluded")
+    else:
+        run(1, 1)
+    ref = out["r"]
+    mat = {"sample": f"{n_sample}^3, 8 fields, residual vector"}
+    for mode in ("fp64", "fp32"):
+        mat[mode] = rel_err(B.DGTransport(V, c, 7, mode).residual(u), ref)
+    # the downstream solution: RK4 steps on the GPU and on the CPU
+    mr = B.hexa27_mesh(n_rk + 1, n_rk + 1, n_rk + 1)
+    B.sinusoidal_map(mr, 0.05, seed=0)
+    omr = host_mesh(B, mr)
+    Vr = B.TrialFESpace(B.FunctionSpace("Lagrange", 3), mr, "discontinuous")
+    x = B.dg_dof_coordinates(Vr)
+    u0 = np.stack([np.exp(-30 * ((x - 0.4) ** 2).sum(1)), np.sin(2 * np.pi * x[:, 0])])
+    cp, rv, nz, nd = O.assemble_bilinear(omr, 3, ["mass"], degree=7, continuous=False)
+    lu = spla.splu(O.csc_to_scipy(cp, rv, nz, nd).tocsc())
+
+    def L(w):
+        r = O.dg_transport(omr, 3, c, w.T, 7).T
+        return np.stack([lu.solve(r[f]) for f in range(r.shape[0])])
+
+    dt = 0.1 / (4 * n_rk)
+    xref = u0.copy()
+    for _ in range(rk_steps):
+        k1 = L(xref)
+        k2 = L(xref + dt / 2 * k1)
+        k3 = L(xref + dt / 2 * k2)
+        k4 = L(xref + dt * k3)
+        xref = xref + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    sol = {"workload": f"{rk_steps} RK4 steps of M du/dt = l(u), {n_rk}^3 curved Hexa27, DG Q3, "
+                       f"2 fields, against CPU RK4 on the oracle's residual and mass matrix"}
+    for mode in ("fp64", "fp32"):
+        ud = torch.from_numpy(u0).cuda()
+        if mode == "fp32":
+            ud = ud.float()
+        plan = B.DGTransport(Vr, c, 7, mode)
+        work = None
+        for _ in range(rk_steps):
+            work = plan.rk4_step(ud, dt, work)
+        torch.cuda.synchronize()
+        sol[mode] = rel_err(ud.double().cpu().numpy(), xref)
+    return cpu, mat, sol
 
 
 def bench_c4(args, torch, B):
@@ -503,12 +847,8 @@ def bench_c4(args, torch, B):
                                   "avg_launch_ms": k_ms, "alg_bytes_per_launch": b_alg,
                                   "factor_bytes_per_launch":
                                       (m.ncells * 4 * 64 + nfaces * 16) * es}}
-    tf = os.path.join(ROOT, "profiles", "traffic.json")
-    measured = {}
-    if os.path.exists(tf) and n == 32:
-        measured = {k.replace(" ", ""): v for k, v in json.load(open(tf))["kernels"].items()}
     for mode_r in res.values():
-        mode_r["roofline"]["traffic"] = measured.get(mode_r["roofline"]["kernel"].replace(" ", ""))
+        mode_r["roofline"]["traffic"] = (traffic_of(f"c4_{n}", mode_r["roofline"]["kernel"]))
     err = float((outs["fp32"].double() - outs["fp64"]).abs().max() / outs["fp64"].abs().max())
     # 50 RK4 steps of M du/dt = l(u): eager launches vs one graph of one step replayed
     rk = {}
@@ -556,8 +896,13 @@ def bench_c4(args, torch, B):
         "modes": res,
         "rk4": rk,
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline_c4()
+    cpu, mat, sol = (None, None, None) if args.no_check else \
+        c4_cpu_and_checks(B, torch, timed=not args.no_cpu)
+    if mat is not None:
+        line["matrix_rel_err_vs_oracle"] = mat
+        line["solution_rel_err_vs_oracle"] = sol
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     backend.set_stream(None)
     return line
 
@@ -1262,11 +1607,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", default="all", choices=["all", "fp64", "fp32", "bf16x3"])
     ap.add_argument("--n", type=int, default=64)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="c2",
-                    choices=["c1", "c2", "c3", "c4", "c5", "vec", "build", "basic", "sipg",
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline timing")
+    ap.add_argument("--no-check", action="store_true",
+                    help="skip the CPU baseline, the parity checks against the oracle and C5's "
+                         "per-instance latencies (for profiling runs: those launch the same "
+                         "kernels on other sizes)")
+    ap.add_argument("--config", default="all",
+                    choices=["all", "c1", "c2", "c3", "c4", "c5", "vec", "build", "basic", "sipg",
                              "covo"],
-                    help="c2: the headline 64^3 Q2 hexa workload; c1: the 64^2 Q1 Poisson "
+                    help="all (default): the headline C2 line with the C1, C3, C4 and C5 "
+                         "entries under `configs`; c2: the headline 64^3 Q2 hexa workload only; "
+                         "c1: the 64^2 Q1 Poisson "
                          "matrix + right-hand side; c3: mixed tri/quad order buckets; c4: the DG "
                          "transport residual; c5: the many-instance batch; vec: a 3-component "
                          "Q2 hexa space (48^3 unless --n); build: the device numbering and "
@@ -1313,18 +1664,32 @@ def main():
     if args.config == "c3":
         print(json.dumps(bench_c3(args, torch, B)), flush=True)
         return
+    line = bench_c2(args, torch, B)
+    if args.config == "all":
+        configs = {}
+        for name, fn in (("c1", bench_c1), ("c3", bench_c3), ("c4", bench_c4), ("c5", bench_c5)):
+            print(f"config {name}", file=sys.stderr, flush=True)
+            configs[name] = config_entry(fn(args, torch, B))
+        line["configs"] = configs
+    print(json.dumps(line), flush=True)
+
+
+def config_entry(line):
+    """The per-config summary carried in the default bench line."""
+    keep = ("metric", "value", "unit", "ms_per_step", "dtype", "roofline", "cpu_baseline",
+            "matrix_rel_err_vs_oracle", "solution_rel_err_vs_oracle", "fp32_speedup_vs_fp64",
+            "per_instance_latency_ms", "rk4", "elements_per_s", "kernels")
+    e = {k: line[k] for k in keep if k in line}
+    e["workload"] = line["config"]["workload"]
+    return e
+
+
+def bench_c2(args, torch, B):
+    """The headline: C2 at --n (64) in FP64, FP32-MFMA and BF16x3."""
     n = args.n
     t_setup = time.perf_counter()
-    m = B.hexa_mesh(n + 1, n + 1, n + 1)
-    B.jitter_nodes(m, 0.15, 1.0 / n, seed=0)
-    eta = np.random.default_rng(1).uniform(0.5, 2.0, m.ncells)
-    U = B.TrialFESpace(B.FunctionSpace("Lagrange", 2), m)
-    V = B.TestFESpace(U)
-    dΩ = B.Measure(B.CellDomain(m), 5)
-    ηc = B.MeshCellData(eta)
-    plan_form = (lambda u, v: B.integrate(ηc * B.dot(B.grad(u), B.grad(v)), dΩ)
-                 + B.integrate(u * v, dΩ))
-    plan = B.BilinearPlan(plan_form, U, V)
+    m, eta = c2_problem(B, n)
+    plan, U, V, dΩ = c2_plan(B, m, eta, build="device")
     t_setup = time.perf_counter() - t_setup
     ncells, ndofs, nnz, b_alg, f_alg = workload(n)
     assert plan.nnz == nnz, (plan.nnz, nnz)
@@ -1378,15 +1743,10 @@ def main():
     head = "fp32" if "fp32" in results else modes[0]
     r = results[head]
     roof = dict(r["roofline"])
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tf) and n == 64:
-        measured = {k.replace(" ", ""): v for k, v in json.load(open(tf))["kernels"].items()}
-        traffic = measured.get(roof["kernel"].replace(" ", ""))
-        for mode_r in results.values():
-            for kr in mode_r["kernels"]:
-                kr["traffic"] = measured.get(kr["kernel"].replace(" ", ""))
-    roof["traffic"] = traffic
+    for mode_r in results.values():
+        for kr in mode_r["kernels"]:
+            kr["traffic"] = traffic_of(f"c2_{n}", kr["kernel"])
+    roof["traffic"] = traffic_of(f"c2_{n}", roof["kernel"])
     speed = None
     if "fp64" in results and "fp32" in results:
         speed = results["fp64"]["ms_per_step"] / results["fp32"]["ms_per_step"]
@@ -1413,10 +1773,17 @@ def main():
         "rel_err_vs_fp64_gpu": r["rel_err_vs_fp64_gpu"],
         "modes": results,
     }
-    line["solution_rel_err_vs_oracle"] = downstream_check(B)
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline()
-    print(json.dumps(line), flush=True)
+    cpu, mat, sol = (None, None, None) if args.no_check else \
+        c2_cpu_and_checks(B, [md for md in ("fp64", "fp32", "bf16x3") if md in modes],
+                          timed=not args.no_cpu)
+    if mat is not None:
+        line["matrix_rel_err_vs_oracle"] = mat
+        line["solution_rel_err_vs_oracle"] = sol
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
+    backend.set_stream(None)
+    return line
+
 
 
 if __name__ == "__main__":

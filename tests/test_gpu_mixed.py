@@ -142,3 +142,31 @@ def test_mixed_mesh_linear_form_vs_oracle(p, deg, src):
     err = np.abs(b - ref).max() / np.abs(ref).max()
     print(f"mixed P{p} deg {deg} f={src}: rel err {err:.2e}")
     assert err <= 1e-12
+
+
+def test_c3_reduced_mesh_every_bucket_vs_oracle():
+    """The C3 workload reduced to 96^2 quads: 16x16 blocks split into
+    triangles with probability 1/3 (seed 0), block orders p in {1,2,3,4} with
+    probabilities {0.4,0.3,0.2,0.1} (seed 1); each order bucket (continuous
+    P_p/Q_p on the union of its blocks, ∫∇u⋅∇v + ∫uv, degree max(2p, 4))
+    against the oracle, FP64 and FP32."""
+    n = 96
+    m, blocks = B.mixed_tri_quad_mesh(n, block=16, p_split=1.0 / 3.0, seed=0)
+    nb = int(blocks.max()) + 1
+    border = np.random.default_rng(1).choice([1, 2, 3, 4], size=nb, p=[0.4, 0.3, 0.2, 0.1])
+    order = border[blocks]
+    assert (m.celltypes == 2).any() and (m.celltypes == 0).any()
+    seen = 0
+    for p in (1, 2, 3, 4):
+        cells = np.flatnonzero(order == p)
+        if cells.size == 0:
+            continue
+        seen += 1
+        sub = B.sub_mesh(m, cells)
+        deg = max(2 * p, 4)
+        plan = _plan(sub, p, deg)
+        ref = _ref(sub, p, deg)
+        e64 = _check(plan, ref, RTOL)
+        e32 = _check(plan, ref, RTOL_FP32, mode="fp32")
+        print(f"C3 96^2 bucket P{p}: {cells.size} cells, fp64 {e64:.2e}, fp32 {e32:.2e}")
+    assert seen >= 3

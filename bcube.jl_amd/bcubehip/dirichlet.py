@@ -24,9 +24,10 @@ import numpy as np
 
 from . import _lib as L
 from .forms import FEFunction, PhysicalFunction
-from .mesh import _H8_VERTS
+from .mesh import _H8_VERTS, _H27_REF
 
 _QUAD4, _TRI3, _HEXA8 = L.CELLTYPE["Quad4"], L.CELLTYPE["Tri3"], L.CELLTYPE["Hexa8"]
+_QUAD9, _HEXA27 = L.CELLTYPE["Quad9"], L.CELLTYPE["Hexa27"]
 
 # sides of each cell type as local vertex lists, and the reference vertices
 _SIDES = {
@@ -39,6 +40,20 @@ _VREF = {
     _TRI3: np.array([[0, 0], [1, 0], [0, 1]], dtype=np.float64),
     _HEXA8: _H8_VERTS.astype(np.float64),
 }
+# curved cells: sides and vertices are those of the straight cell of the same
+# shape (the first nodes in CGNS order); the geometry uses every node
+_SIDES[_QUAD9] = _SIDES[_QUAD4]
+_SIDES[_HEXA27] = _SIDES[_HEXA8]
+_VREF[_QUAD9] = _VREF[_QUAD4]
+_VREF[_HEXA27] = _VREF[_HEXA8]
+# reference coordinates of every geometric node of the curved cells, CGNS order
+# (entity.jl: Quad9 vertices, edge midpoints 12 23 34 41, centre; Hexa27
+# vertices, 12 edges, 6 face centres, centre)
+_GREF = {
+    _QUAD9: np.array([[-1, -1], [1, -1], [1, 1], [-1, 1], [0, -1], [1, 0], [0, 1], [-1, 0],
+                      [0, 0]], dtype=np.float64),
+}
+_GREF[_HEXA27] = np.asarray(_H27_REF, dtype=np.float64)
 # triangle Lagrange nodes as barycentric lattice points (i, j, k), i + j + k = p,
 # in the reference's dof order (triangle.jl:1-45, 120-145; P4 as the library)
 _TRI_LATTICE = {
@@ -59,18 +74,29 @@ def reference_nodes(celltype: int, p: int) -> np.ndarray:
         lat = np.array(_TRI_LATTICE[p], dtype=np.float64)
         return lat[:, 1:] / p
     x = -1.0 + 2.0 * np.arange(p + 1) / p
-    if celltype == _QUAD4:
+    if celltype in (_QUAD4, _QUAD9):
         return np.array([[x[i], x[j]] for j in range(p + 1) for i in range(p + 1)])
-    if celltype == _HEXA8:
+    if celltype in (_HEXA8, _HEXA27):
         return np.array([[x[i], x[j], x[k]] for k in range(p + 1) for j in range(p + 1)
                          for i in range(p + 1)])
-    raise NotImplementedError("Dirichlet conditions: Quad4, Tri3 and Hexa8 cells")
+    raise NotImplementedError("Dirichlet conditions: Quad4, Quad9, Tri3, Hexa8 and Hexa27 cells")
+
+
+def _quadratic_1d(s: np.ndarray, t: np.ndarray) -> np.ndarray:
+    """The 1D quadratic Lagrange basis on {-1, 0, 1}: the function of the node s at t."""
+    return np.where(s == -1, 0.5 * t * (t - 1.0), np.where(s == 0, 1.0 - t * t,
+                                                          0.5 * t * (t + 1.0)))
 
 
 def _geometric_weights(celltype: int, xi: np.ndarray) -> np.ndarray:
-    """(npts, nvertices) values of the affine/multilinear geometric basis."""
+    """(npts, ngeometric nodes) values of the geometric basis: affine /
+    multilinear on straight cells, products of 1D quadratics on Quad9 / Hexa27
+    (mapping.jl:544-570, 664-697)."""
     if celltype == _TRI3:
         return np.stack([1.0 - xi[:, 0] - xi[:, 1], xi[:, 0], xi[:, 1]], axis=1)
+    if celltype in _GREF:
+        g = _GREF[celltype]
+        return np.prod(_quadratic_1d(g[None, :, :], xi[:, None, :]), axis=2)
     v = _VREF[celltype]
     return np.prod((1.0 + xi[:, None, :] * v[None, :, :]) / 2.0, axis=2)
 
@@ -92,6 +118,12 @@ def _cell_vertices(mesh, icell: int) -> np.ndarray:
     return nodes[:len(_VREF[int(mesh.celltypes[icell])])]
 
 
+def _geometric_nodes(mesh, icell: int) -> np.ndarray:
+    """The nodes the geometric map of a cell uses: all of a curved cell's."""
+    ct = int(mesh.celltypes[icell])
+    return mesh.cell_nodes(icell) if ct in _GREF else _cell_vertices(mesh, icell)
+
+
 def boundary_faces(mesh) -> dict[str, list[tuple[int, int]]]:
     """Boundary faces of each tag as (cell, side), in face order (the creating
     cell, then its local side: `_build_faces!`, mesh.jl:328-374).  A face with
@@ -102,7 +134,8 @@ def boundary_faces(mesh) -> dict[str, list[tuple[int, int]]]:
     for c in range(mesh.ncells):
         ct = int(mesh.celltypes[c])
         if ct not in _SIDES:
-            raise NotImplementedError("Dirichlet conditions: Quad4, Tri3 and Hexa8 cells")
+            raise NotImplementedError("Dirichlet conditions: Quad4, Quad9, Tri3, Hexa8 and "
+                                      "Hexa27 cells")
         vn = _cell_vertices(mesh, c)
         for s, loc in enumerate(_SIDES[ct]):
             key = tuple(sorted(int(vn[k]) for k in loc))
@@ -137,7 +170,7 @@ def dof_coordinates(U) -> np.ndarray:
         xi = reference_nodes(ct, p)
         w = _geometric_weights(ct, xi)
         cells = np.flatnonzero(mesh.celltypes == ct)
-        vert = np.stack([_cell_vertices(mesh, c) for c in cells])          # (n, nv)
+        vert = np.stack([_geometric_nodes(mesh, c) for c in cells])        # (n, nv)
         x = np.einsum("qk,ckd->cqd", w, X[vert])                            # (n, nl, dim)
         off = U.dhl.offset[cells][:, None] + np.arange(xi.shape[0])[None, :]
         pts[U.dhl.iglob[off] - 1] = x

@@ -584,8 +584,8 @@ int bh_mesh_create(bh_ctx c, int dim, int64_t nnodes, const double* coords, int6
     if (v < 0 || v >= nnodes) return fail(c, BH_ERR_RANGE, "c2n: node index out of range");
   h.celltype = h.ctype[0];
   h.uniform = std::all_of(h.ctype.begin(), h.ctype.end(), [&](int t) { return t == h.celltype; });
-  if (h.uniform && (h.celltype == BH_QUAD4 || h.celltype == BH_HEXA8 || h.celltype == BH_HEXA27 ||
-                   h.celltype == BH_TRI3)) {
+  if (h.uniform && (h.celltype == BH_QUAD4 || h.celltype == BH_QUAD9 || h.celltype == BH_HEXA8 ||
+                   h.celltype == BH_HEXA27 || h.celltype == BH_TRI3)) {
     m->geo = h.celltype;
     m->npc = nnodes_of(h.celltype);
   } else if (!h.uniform && std::all_of(h.ctype.begin(), h.ctype.end(), [](int t) {
@@ -1979,9 +1979,10 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
   if (where == BH_HOST && !nzval) return fail(c, BH_ERR_ARG, "nzval is NULL");
   if ((where == BH_HOST || coef_where == BH_HOST) && c->capturing)
     return fail(c, BH_ERR_ARG, "host-`where` assembly cannot be captured in a graph");
-  if (m->geo == -1 || m->geo == BH_HEXA27 || m->buckets.empty())
+  if (m->geo == -1 || m->buckets.empty())
     return fail(c, BH_ERR_UNSUPPORTED,
-                "bilinear kernels need Quad4, Tri3 (or a mix of both) or Hexa8 cells");
+                "bilinear kernels need Quad4, Quad9, Tri3 (or a mix of Quad4 and Tri3), Hexa8 or "
+                "Hexa27 cells");
   const bh_term* stiff = nullptr;
   const bh_term* mass = nullptr;
   for (int k = 0; k < nterms; ++k) {
@@ -2203,9 +2204,10 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
   if (!b) return fail(c, BH_ERR_ARG, "b is NULL");
   if (where == BH_HOST && c->capturing)
     return fail(c, BH_ERR_ARG, "host-`where` assembly cannot be captured in a graph");
-  if (m->geo == -1 || m->geo == BH_HEXA27 || m->buckets.empty())
+  if (m->geo == -1 || m->buckets.empty())
     return fail(c, BH_ERR_UNSUPPORTED,
-                "linear-form kernels need Quad4, Tri3 (or a mix of both) or Hexa8 cells");
+                "linear-form kernels need Quad4, Quad9, Tri3 (or a mix of Quad4 and Tri3), Hexa8 "
+                "or Hexa27 cells");
   const bool mixed = m->geo == kMixed;
   const size_t nb = m->buckets.size();
   std::vector<int> bkey(nb);
@@ -2628,9 +2630,10 @@ int bh_quadrature_points(bh_mesh m, int quad_degree, double* xq, int where, int3
   if (!c) return fail(c, BH_ERR_ARG, "needs a mesh created with a context");
   if (c->capturing && where == BH_HOST)
     return fail(c, BH_ERR_ARG, "host-`where` call cannot be captured in a graph");
-  if (m->geo == -1 || m->geo == BH_HEXA27 || m->buckets.empty())
+  if (m->geo == -1 || m->buckets.empty())
     return fail(c, BH_ERR_UNSUPPORTED,
-                "quadrature points need Quad4, Tri3 (or a mix of both) or Hexa8 cells");
+                "quadrature points need Quad4, Quad9, Tri3 (or a mix of Quad4 and Tri3), Hexa8 "
+                "or Hexa27 cells");
   // per cell type: a kernel order carrying the geometric tables, the points;
   // every cell gets nq = the largest count (mixed meshes pad smaller rules)
   std::vector<int> border(m->buckets.size()), bkey(m->buckets.size());

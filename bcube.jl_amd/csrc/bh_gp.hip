@@ -366,11 +366,22 @@ struct GPEntry {
   {BH_TRI3, ORDER, NPTS, &launch_gp_t<2, ((ORDER) + 1) * ((ORDER) + 2) / 2, NPTS, 3>, \
    gp_has12<2, ((ORDER) + 1) * ((ORDER) + 2) / 2, NPTS, 3>()}
 
+// curved cells (Quad9, Hexa27 geometry): the elements whose block has a
+// thread for every prefetched geometric node
+#define BH_GQ9(ORDER, NQ1)                                                              \
+  {BH_QUAD9, ORDER, NQ1, &launch_gp_t<2, ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1), 9>, \
+   gp_has12<2, ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1), 9>()}
+#define BH_GH27(ORDER, NQ1)                                                                 \
+  {BH_HEXA27, ORDER, NQ1,                                                                   \
+   &launch_gp_t<3, ((ORDER) + 1) * ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1) * (NQ1), 27>, \
+   gp_has12<3, ((ORDER) + 1) * ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1) * (NQ1), 27>()}
+
 static const GPEntry kGP[] = {
     BH_GQ(1, 2), BH_GQ(1, 3), BH_GQ(2, 2), BH_GQ(2, 3), BH_GQ(3, 3), BH_GQ(3, 4), BH_GQ(4, 5),
     BH_GT(1, 1), BH_GT(1, 3), BH_GT(1, 4), BH_GT(1, 6),
     BH_GT(2, 3), BH_GT(2, 4), BH_GT(2, 6), BH_GT(2, 7), BH_GT(3, 12), BH_GT(4, 16),
     BH_GH(1, 2), BH_GH(2, 2), BH_GH(2, 3),
+    BH_GQ9(2, 2), BH_GQ9(2, 3), BH_GQ9(2, 4), BH_GH27(2, 2), BH_GH27(2, 3),
 };
 
 bool gp_supported(int geo, int order, int nq1) {

@@ -305,26 +305,57 @@ Tables make_tables(int shape, int order, int quad_degree, int geo_ct) {
       }
     }
   }
-  // geometric basis: bilinear / trilinear in CGNS vertex order (mapping.jl:460-465, 618-639)
-  static const int q4[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
-  static const int h8[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
-                               {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
-  T.ngeo = geo_ct == BH_HEXA8 ? 8 : (geo_ct == BH_QUAD4 ? 4 : 0);
+  // geometric basis in CGNS node order: bilinear / trilinear on Quad4 / Hexa8
+  // (mapping.jl:460-465, 618-639), products of the 1D quadratics on {-1, 0, 1}
+  // on Quad9 / Hexa27 (mapping.jl:544-570, 664-697); each node's reference
+  // coordinates are entity.jl's (Quad9: vertices, edge midpoints 12 23 34 41,
+  // centre; Hexa27: vertices, 12 edges, 6 face centres, centre)
+  static const int8_t q4[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+  static const int8_t h8[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                  {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  static const int8_t nq9[9][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}, {0, -1},
+                                   {1, 0},   {0, 1},  {-1, 0}, {0, 0}};
+  static const int8_t nh27[27][3] = {
+      {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1},
+      {-1, 1, 1},   {0, -1, -1}, {1, 0, -1}, {0, 1, -1},  {-1, 0, -1}, {-1, -1, 0}, {1, -1, 0},
+      {1, 1, 0},    {-1, 1, 0},  {0, -1, 1}, {1, 0, 1},   {0, 1, 1},   {-1, 0, 1},  {0, 0, -1},
+      {0, -1, 0},   {1, 0, 0},   {0, 1, 0},  {-1, 0, 0},  {0, 0, 1},   {0, 0, 0}};
+  const bool quadratic = geo_ct == BH_QUAD9 || geo_ct == BH_HEXA27;
+  T.ngeo = geo_ct == BH_HEXA27 ? 27 : geo_ct == BH_QUAD9 ? 9
+           : geo_ct == BH_HEXA8 ? 8 : (geo_ct == BH_QUAD4 ? 4 : 0);
   T.gphi.assign((size_t)T.nq * T.ngeo, 0.0);
   T.dgphi.assign((size_t)T.nq * T.ngeo * d, 0.0);
+  // 1D geometric basis at t for the node coordinate s in {-1, 0, 1}
+  auto basis1 = [&](int s, double t, double& v, double& dv) {
+    if (!quadratic) {
+      v = 0.5 * (1.0 + s * t);
+      dv = 0.5 * s;
+    } else if (s == -1) {
+      v = 0.5 * t * (t - 1.0);
+      dv = t - 0.5;
+    } else if (s == 0) {
+      v = 1.0 - t * t;
+      dv = -2.0 * t;
+    } else {
+      v = 0.5 * t * (t + 1.0);
+      dv = t + 0.5;
+    }
+  };
   for (int q = 0; q < T.nq; ++q) {
     const double* x = &T.xi[q * d];
     for (int k = 0; k < T.ngeo; ++k) {
-      double s[3], f[3];
+      double f[3], df[3];
       for (int a = 0; a < d; ++a) {
-        s[a] = d == 3 ? h8[k][a] : q4[k][a];
-        f[a] = 0.5 * (1.0 + s[a] * x[a]);
+        const int s = geo_ct == BH_HEXA27 ? nh27[k][a]
+                      : geo_ct == BH_QUAD9 ? nq9[k][a]
+                      : d == 3 ? h8[k][a] : q4[k][a];
+        basis1(s, x[a], f[a], df[a]);
       }
       double val = 1.0;
       for (int a = 0; a < d; ++a) val *= f[a];
       T.gphi[(size_t)q * T.ngeo + k] = val;
       for (int a = 0; a < d; ++a) {
-        double g = 0.5 * s[a];
+        double g = df[a];
         for (int b = 0; b < d; ++b)
           if (b != a) g *= f[b];
         T.dgphi[((size_t)q * T.ngeo + k) * d + a] = g;

@@ -509,6 +509,12 @@ struct Entry {
   {BH_QUAD4, ORDER, NQ1, &launch_all<2, BH_Q(ORDER, NQ1), 4>, &table_size<2, BH_Q(ORDER, NQ1), 4>}
 #define BH_ENTRY_H(ORDER, NQ1)                                                   \
   {BH_HEXA8, ORDER, NQ1, &launch_all<3, BH_H(ORDER, NQ1), 8>, &table_size<3, BH_H(ORDER, NQ1), 8>}
+// curved cells: Quad9 and Hexa27 geometry (9 and 27 geometric nodes)
+#define BH_ENTRY_Q9(ORDER, NQ1)                                                  \
+  {BH_QUAD9, ORDER, NQ1, &launch_all<2, BH_Q(ORDER, NQ1), 9>, &table_size<2, BH_Q(ORDER, NQ1), 9>}
+#define BH_ENTRY_H27(ORDER, NQ1)                                                  \
+  {BH_HEXA27, ORDER, NQ1, &launch_all<3, BH_H(ORDER, NQ1), 27>,                    \
+   &table_size<3, BH_H(ORDER, NQ1), 27>}
 // triangles: key = points of the triangle rule
 #define BH_ENTRY_T(ORDER, NPTS)                                                             \
   {BH_TRI3, ORDER, NPTS, &launch_all<2, ((ORDER) + 1) * ((ORDER) + 2) / 2, NPTS, 3>,          \
@@ -521,6 +527,8 @@ static const Entry kTable[] = {
     BH_ENTRY_T(1, 1), BH_ENTRY_T(1, 3), BH_ENTRY_T(1, 4), BH_ENTRY_T(1, 6),
     BH_ENTRY_T(2, 3), BH_ENTRY_T(2, 4), BH_ENTRY_T(2, 6), BH_ENTRY_T(2, 7), BH_ENTRY_T(3, 12),
     BH_ENTRY_T(4, 16),
+    BH_ENTRY_Q9(1, 2), BH_ENTRY_Q9(1, 3), BH_ENTRY_Q9(2, 2), BH_ENTRY_Q9(2, 3), BH_ENTRY_Q9(2, 4),
+    BH_ENTRY_H27(1, 2), BH_ENTRY_H27(1, 3), BH_ENTRY_H27(2, 2), BH_ENTRY_H27(2, 3),
 };
 
 bool kernel_supported(int geo, int order, int nq1) {

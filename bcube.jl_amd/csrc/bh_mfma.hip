@@ -325,6 +325,12 @@ struct MfmaEntry {
   {BH_HEXA8, ORDER, NQ1,                                                                      \
    &launch_mfma32<3, ((ORDER) + 1) * ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1) * (NQ1), 8>}
 
+#define BH_MQ9(ORDER, NQ1) \
+  {BH_QUAD9, ORDER, NQ1, &launch_mfma32<2, ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1), 9>}
+#define BH_MH27(ORDER, NQ1)                                                                   \
+  {BH_HEXA27, ORDER, NQ1,                                                                     \
+   &launch_mfma32<3, ((ORDER) + 1) * ((ORDER) + 1) * ((ORDER) + 1), (NQ1) * (NQ1) * (NQ1), 27>}
+
 #define BH_MT(ORDER, NPTS) \
   {BH_TRI3, ORDER, NPTS, &launch_mfma32<2, ((ORDER) + 1) * ((ORDER) + 2) / 2, NPTS, 3>}
 
@@ -333,6 +339,8 @@ static const MfmaEntry kMfma[] = {
     BH_MT(1, 1), BH_MT(1, 3), BH_MT(1, 4), BH_MT(1, 6),
     BH_MT(2, 3), BH_MT(2, 4), BH_MT(2, 6), BH_MT(2, 7), BH_MT(3, 12), BH_MT(4, 16),
     BH_MH(1, 2), BH_MH(2, 2), BH_MH(2, 3),
+    BH_MQ9(1, 2), BH_MQ9(1, 3), BH_MQ9(2, 2), BH_MQ9(2, 3), BH_MQ9(2, 4),
+    BH_MH27(1, 2), BH_MH27(1, 3), BH_MH27(2, 2), BH_MH27(2, 3),
 };
 
 bool mfma_supported(int geo, int order, int nq1) {

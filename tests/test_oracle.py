@@ -277,3 +277,145 @@ def test_threaded_oracle_equals_serial():
         assert np.array_equal(b, bref)
         r, _ = O.dg_transport_timed(o27, 2, [1.0, 0.5, 0.25], u, 5, threads=threads, reps=1)
         assert np.array_equal(r, rref)
+
+
+def _bubble_map(X, amp=0.04):
+    """x_d += amp sin(πx) sin(πy) (sin(πz)) (1 + d): a smooth map that moves no
+    node of the unit box's boundary, so the domain stays the box."""
+    Y = X.copy()
+    b = np.prod(np.sin(np.pi * X), axis=1)
+    for d in range(X.shape[1]):
+        Y[:, d] += amp * (1 + d) * b
+    return Y
+
+
+def _interior_patch_error(A, xd, u_lin):
+    """K u = 0 in the interior with u = u_lin on the boundary dofs: the
+    interior solution's max error against u_lin at the interior dofs."""
+    inner = np.all((xd > 1e-12) & (xd < 1 - 1e-12), axis=1)
+    K = A.tocsr()
+    KII = K[inner][:, inner].tocsc()
+    KIB = K[inner][:, ~inner]
+    u = u_lin(xd)
+    ui = spla.spsolve(KII, -KIB @ u[~inner])
+    return np.abs(ui - u[inner]).max() / np.abs(u).max()
+
+
+def _q2_dof_points(om, order, ndofs, iglob_offset):
+    """Physical points of the Q2 dofs of a curved mesh: the dof at reference
+    point ξ of a cell sits at the geometric node with that ξ (the Q2 dofs and
+    the Quad9/Hexa27 nodes share their reference points)."""
+    iglob, offset = iglob_offset
+    d = om.dim
+    X = om.coords.reshape(-1, d)
+    k = om.c2n_ptr[1] - om.c2n_ptr[0]
+    if d == 2:
+        ref = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1], [0, -1], [1, 0], [0, 1], [-1, 0],
+                        [0, 0]])
+    else:
+        import bcubehip.mesh as bm
+        ref = bm._H27_REF
+    lex = [tuple(r) for r in (np.stack(np.meshgrid(*([[-1, 0, 1]] * d), indexing="ij"), -1)
+                              .reshape(-1, d)[:, ::-1])]
+    node_of = [int(np.flatnonzero((ref == np.array(p)).all(1))[0]) for p in lex]
+    pts = np.zeros((ndofs, d))
+    for c in range(om.ncells):
+        nodes = om.c2n[c * k:(c + 1) * k]
+        for loc in range(len(lex)):
+            pts[iglob[offset[c] + loc] - 1] = X[nodes[node_of[loc]]]
+    return pts
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_curved_q2_patch_test(dim):
+    """The isoparametric Q2 stiffness on curved Quad9 / Hexa27 cells (a bubble
+    map of the unit box) reproduces a linear field: with linear Dirichlet data
+    the interior solution is that field, to rounding.  A wrong Jacobian, a
+    transposed J⁻¹ or a wrong node order in the quadratic map fails it."""
+    import bcubehip as bh
+    host = bh.BcubeBackendHostOnly()
+    n = 4
+    if dim == 2:
+        m = bh.rectangle_mesh(n + 1, n + 1, order=2, backend=host)
+    else:
+        m = bh.hexa27_mesh(n + 1, n + 1, n + 1, backend=host)
+    m.set_nodes(_bubble_map(m.nodes()))
+    om = O.OMesh(m.dim, m.coords, m.celltypes, m.c2n_ptr, m.c2n)
+    cp, rv, nz, nd = O.assemble_bilinear(om, 2, ["stiffness"], degree=5 if dim == 3 else 6)
+    A = O.csc_to_scipy(cp, rv, nz, nd)
+    iglob, offset, n_d = O.dofhandler(om, 2)
+    assert n_d == nd
+    xd = _q2_dof_points(om, 2, nd, (iglob, offset))
+    u_lin = (lambda x: 1.0 + 2.0 * x[:, 0] - 3.0 * x[:, 1]) if dim == 2 else \
+        (lambda x: 1.0 + 2.0 * x[:, 0] - 3.0 * x[:, 1] + 0.5 * x[:, 2])
+    err = _interior_patch_error(A, xd, u_lin)
+    assert err < 1e-11, err
+    # the cells are really curved: first-edge midpoint nodes (CGNS node 4 of a
+    # Quad9, 8 of a Hexa27) lie off the chord of their two vertices
+    Xn = om.coords.reshape(-1, dim)
+    k = om.c2n_ptr[1] - om.c2n_ptr[0]
+    c2n = om.c2n.reshape(-1, k)
+    mid = c2n[:, 4 if dim == 2 else 8]
+    off = np.linalg.norm(Xn[mid] - 0.5 * (Xn[c2n[:, 0]] + Xn[c2n[:, 1]]), axis=1)
+    assert off.max() > 1e-3
+
+
+def test_jittered_hexa8_q2_patch_test():
+    """The jittered Hexa8 Q2 stiffness of the C2 workload reproduces a linear
+    field (the trilinear map is affine on no cell here): pins the oracle's
+    J⁻¹ orientation on non-affine cells directly."""
+    n = 4
+    o = O.hexa_mesh(n + 1, n + 1, n + 1)
+    X = o.coords.reshape(-1, 3)
+    inner = np.all((X > 1e-12) & (X < 1 - 1e-12), axis=1)
+    X[inner] += np.random.default_rng(0).uniform(-0.15, 0.15, X.shape)[inner] / n
+    cp, rv, nz, nd = O.assemble_bilinear(o, 2, ["stiffness"], degree=5)
+    A = O.csc_to_scipy(cp, rv, nz, nd)
+    # Q2 dof points on trilinear cells: the trilinear map at the lattice points
+    nl = 27
+    iglob, offset, _ = O.dofhandler(o, 2)
+    v8 = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1],
+                   [1, 1, 1], [-1, 1, 1]], dtype=np.float64)
+    lex = np.stack(np.meshgrid([-1.0, 0.0, 1.0], [-1.0, 0.0, 1.0], [-1.0, 0.0, 1.0],
+                               indexing="ij"), -1).reshape(-1, 3)[:, ::-1]
+    w = np.prod((1.0 + lex[:, None, :] * v8[None, :, :]) / 2.0, axis=2)
+    xd = np.zeros((nd, 3))
+    for c in range(o.ncells):
+        xd[iglob[offset[c]:offset[c] + nl] - 1] = w @ X[o.c2n[8 * c:8 * c + 8]]
+    err = _interior_patch_error(A, xd, lambda x: 0.5 - x[:, 0] + 2.0 * x[:, 1] + 3.0 * x[:, 2])
+    assert err < 1e-11, err
+
+
+def test_quad9_on_straight_cells_equals_quad4():
+    """Quad9 cells whose edge and centre nodes sit at the bilinear map's points
+    describe the Quad4 cells: same matrices (pins the quadratic map's node
+    order against the bilinear one), on a jittered mesh."""
+    import bcubehip as bh
+    host = bh.BcubeBackendHostOnly()
+    m4 = bh.rectangle_mesh(6, 5, backend=host)
+    bh.jitter_nodes(m4, 0.2, 0.2, seed=3)
+    X = m4.nodes()
+    c4 = m4.c2n.reshape(-1, 4)
+    mids = {}
+    nodes = [tuple(p) for p in X]
+    c9 = []
+    for cell in c4:
+        P = X[cell]
+        row = list(cell)
+        for a, b in ((0, 1), (1, 2), (2, 3), (3, 0)):
+            key = (min(cell[a], cell[b]), max(cell[a], cell[b]))
+            if key not in mids:
+                mids[key] = len(nodes)
+                nodes.append(tuple((P[a] + P[b]) / 2))
+            row.append(mids[key])
+        row.append(len(nodes))
+        nodes.append(tuple(P.mean(0)))
+        c9.append(row)
+    o4 = O.OMesh(2, X, np.zeros(len(c4), np.int32), np.arange(len(c4) + 1) * 4, c4.reshape(-1))
+    o9 = O.OMesh(2, np.array(nodes), np.full(len(c9), 1, np.int32),
+                 np.arange(len(c9) + 1) * 9, np.array(c9).reshape(-1))
+    for order in (1, 2):
+        a = O.assemble_bilinear(o4, order, ["stiffness", "mass"], degree=4)
+        b = O.assemble_bilinear(o9, order, ["stiffness", "mass"], degree=4)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert np.abs(a[2] - b[2]).max() <= 1e-14 * np.abs(a[2]).max()

@@ -29,7 +29,7 @@ struct bh_ctx_s {
   std::string err;
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
-  std::vector<int32_t> kinds;  // per timed launch: 0 element, 1 row gather, 2 other
+  std::vector<int32_t> kinds;  // per timed launch: 0 element, 1 row gather, 2 other, 3 fused
   size_t nevents = 0;
   std::map<std::tuple<int, int, int>, double*> tables;  // (geo, order, nq1) -> device table
   std::map<std::tuple<int, int, int, int>, float*> gp_tables;  // (geo, order, nq1, mass) -> P
@@ -99,9 +99,24 @@ struct Tuning {
   float elem_ms = 0.f, gather_ms = 0.f;
 };
 
+// The fused kernel's schedule on the device (build_fused_schedule), built on
+// the first FP32 assembly of an eligible pattern.
+struct FusedDev {
+  bool tried = false, ok = false;
+  int nblocks = 0;
+  int64_t nspill = 0, local_rows = 0;
+  int32_t* d_bstep = nullptr;
+  int32_t* d_cells = nullptr;
+  int64_t* d_prog_off = nullptr;
+  uint8_t* d_prog = nullptr;
+  uint32_t* d_smask = nullptr;
+  int32_t* d_spill = nullptr;
+};
+
 struct bh_pattern_s {
   bh_space U = nullptr, V = nullptr;
   Tuning tune[3];  // BH_FP64, BH_FP32, BH_BF16X3
+  FusedDev fused;
   int dev = -1;
   HostPattern h;
   int64_t* d_rowptr = nullptr;
@@ -864,8 +879,8 @@ int bh_pattern_tuning(bh_pattern p, int mode, int32_t* shape, float* ms) {
 int bh_pattern_set_tuning(bh_pattern p, int mode, const int32_t* shape) {
   if (!p || !shape || (mode != BH_FP64 && mode != BH_FP32 && mode != BH_BF16X3)) return BH_ERR_ARG;
   bh_ctx c = p->V->mesh->ctx;
-  if (shape[0] != 0 && shape[0] != 8 && shape[0] != 12)
-    return fail(c, BH_ERR_ARG, "element-kernel waves per block: 0, 8 or 12");
+  if (shape[0] != 0 && shape[0] != 8 && shape[0] != 12 && shape[0] != -1)
+    return fail(c, BH_ERR_ARG, "element-kernel shape: 0, 8 or 12 waves, or -1 (fused)");
   const bool dflt = shape[1] == 0 && shape[2] == 0;
   bool ok = dflt;
   if (!dflt && p->h.pad > 0) {
@@ -1951,12 +1966,86 @@ int bh_pattern_destroy(bh_pattern p) {
     (void)hipFree(p->d_ke);
     (void)hipFree(p->d_vals);
     (void)hipFree(p->d_coef);
+    FusedDev& f = p->fused;
+    (void)hipFree(f.d_bstep);
+    (void)hipFree(f.d_cells);
+    (void)hipFree(f.d_prog_off);
+    (void)hipFree(f.d_prog);
+    (void)hipFree(f.d_smask);
+    (void)hipFree(f.d_spill);
   }
   delete p;
   return BH_OK;
 }
 
 // ---- assembly ----------------------------------------------------------------------
+// Builds the fused kernel's schedule for pattern p (once): the element-row
+// lists and column offsets come back from the device, the pencil order and
+// step programs are built on the host (bh_host.cpp), the result goes to the
+// device.  A pattern the fused kernel cannot take, or whose rows are mostly
+// spilled, keeps the two-kernel path (fused.ok false).
+static int ensure_fused(bh_pattern p, bh_ctx c) {
+  FusedDev& f = p->fused;
+  if (f.tried) return BH_OK;
+  f.tried = true;
+  const HostPattern& h = p->h;
+  bh_space V = p->V;
+  bh_mesh m = V->mesh;
+  if (m->geo < 0 || m->buckets.size() != 1 || V->nl > 32 || h.offset_bytes != 1 ||
+      h.max_row > 128 || p->U != p->V)
+    return BH_OK;
+  const int64_t nc = m->h.ncells, nl = V->nl, nrows = h.nrows;
+  if ((int64_t)h.pptr.size() != nrows + 1) return BH_OK;
+  const int64_t npairs = h.pptr[nrows];
+  std::vector<int32_t> pairs((size_t)npairs);
+  if (h.pad > 0) {
+    std::vector<int32_t> padded((size_t)nrows * h.pad);
+    BH_HIP(c, hipMemcpy(padded.data(), p->d_pairs, padded.size() * sizeof(int32_t),
+                        hipMemcpyDeviceToHost));
+    for (int64_t r = 0; r < nrows; ++r)
+      for (int32_t q = h.pptr[r]; q < h.pptr[r + 1]; ++q)
+        pairs[q] = padded[(size_t)r * h.pad + (q - h.pptr[r])];
+  } else {
+    BH_HIP(c, hipMemcpy(pairs.data(), p->d_pairs, pairs.size() * sizeof(int32_t),
+                        hipMemcpyDeviceToHost));
+  }
+  std::vector<uint8_t> goff((size_t)nc * nl * nl);
+  BH_HIP(c, hipMemcpy(goff.data(), p->d_goff, goff.size(), hipMemcpyDeviceToHost));
+  FusedSchedule fs;
+  std::string err;
+  int ncu = 0;
+  BH_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+  if (!build_fused_schedule(m->h, (int)nl, nrows, h.rowptr, h.pptr, pairs, goff, kFusedProgCap,
+                            std::max(ncu, 1), fs, err))
+    return BH_OK;
+  if (fs.local_rows * 2 < nrows || fs.max_prog > kFusedProgCap) return BH_OK;
+  BH_HIP(c, dalloc(&f.d_bstep, fs.bstep.size()));
+  BH_HIP(c, hipMemcpy(f.d_bstep, fs.bstep.data(), fs.bstep.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice));
+  BH_HIP(c, dalloc(&f.d_cells, fs.cells.size()));
+  BH_HIP(c, hipMemcpy(f.d_cells, fs.cells.data(), fs.cells.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice));
+  BH_HIP(c, dalloc(&f.d_prog_off, fs.prog_off.size()));
+  BH_HIP(c, hipMemcpy(f.d_prog_off, fs.prog_off.data(), fs.prog_off.size() * sizeof(int64_t),
+                      hipMemcpyHostToDevice));
+  // 16 spare bytes past the last program: whole 16-byte loads never leave the buffer
+  BH_HIP(c, dalloc(&f.d_prog, fs.prog.size() + 16));
+  if (!fs.prog.empty())
+    BH_HIP(c, hipMemcpy(f.d_prog, fs.prog.data(), fs.prog.size(), hipMemcpyHostToDevice));
+  BH_HIP(c, dalloc(&f.d_smask, fs.spill_mask.size()));
+  BH_HIP(c, hipMemcpy(f.d_smask, fs.spill_mask.data(), fs.spill_mask.size() * sizeof(uint32_t),
+                      hipMemcpyHostToDevice));
+  BH_HIP(c, dalloc(&f.d_spill, fs.spill_rows.size()));
+  if (!fs.spill_rows.empty())
+    BH_HIP(c, hipMemcpy(f.d_spill, fs.spill_rows.data(), fs.spill_rows.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
+  f.nblocks = fs.nblocks;
+  f.nspill = (int64_t)fs.spill_rows.size();
+  f.local_rows = fs.local_rows;
+  f.ok = true;
+  return BH_OK;
+}
+
 // The assembly itself: values to nzval (host or device by `where`; a NULL
 // device nzval means the pattern's own buffer), per-cell coefficient arrays
 // read from host or device memory by coef_where.
@@ -2098,7 +2187,60 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
     if (timed) time_end(c);
     return BH_OK;
   };
+  // 3. the fused kernel: FP32 on a uniform mesh whose element has a fused
+  //    instantiation; local rows written by the element kernel, the spill rows
+  //    by the row gather over the spill list
   Tuning& tu = p->tune[mode == BH_FP64 ? 0 : (mode == BH_FP32 ? 1 : 2)];
+  const int key0 = rule_key(m->buckets[0].geo, quad_degree);
+  bool can_fuse = mode == BH_FP32 && m->geo >= 0 && m->buckets.size() == 1 &&
+                  gp_fused_supported(m->geo, V->order, key0) && tu.elem_waves != 8 &&
+                  tu.elem_waves != 12;
+  // its schedule is built when the autotuner is about to time it or a caller
+  // fixed it (tuning shape -1)
+  if (can_fuse && !noalloc && !c->capturing && (tu.elem_waves == -1 || (c->autotune && !tu.done)))
+    if (int r = ensure_fused(p, c)) return r;
+  can_fuse = can_fuse && p->fused.ok;
+  auto fused = [&](int lanes, int rows, bool timed) -> int {
+    const GeoBucket& gb = m->buckets[0];
+    double* tab = nullptr;
+    if (int r = get_tables(c, gb.geo, V->order, quad_degree, &tab, !noalloc)) return r;
+    float* pt = nullptr;
+    if (int r = get_gp_table(c, gb.geo, V->order, quad_degree, A.mass, &pt, !noalloc)) return r;
+    FusedArgs F;
+    F.stream = c->stream;
+    F.coords = m->d_coords;
+    F.c2n = gb.d_c2n;
+    F.tab = tab;
+    F.ptab = pt;
+    F.nblocks = p->fused.nblocks;
+    F.bstep = p->fused.d_bstep;
+    F.cells = p->fused.d_cells;
+    F.prog_off = p->fused.d_prog_off;
+    F.prog = p->fused.d_prog;
+    F.smask = p->fused.d_smask;
+    F.erow = V->nl;
+    F.eta = A.eta;
+    F.eta_scale = A.eta_scale;
+    F.mcoef = A.mcoef;
+    F.mass_scale = A.mass_scale;
+    F.mass = A.mass;
+    F.ke = A.ke;
+    F.vals = G.vals;
+    if (timed) time_begin(c, 3);
+    BH_HIP(c, launch_gp_fused(gb.geo, V->order, key0, F));
+    if (timed) time_end(c);
+    if (p->fused.nspill > 0) {
+      LaunchArgs g = G;
+      g.nrows = p->fused.nspill;
+      g.rlist = p->fused.d_spill;
+      g.gather_lanes = lanes;
+      g.gather_rows = rows;
+      if (timed) time_begin(c, 1);
+      BH_HIP(c, launch_gather(V->nl, g));
+      if (timed) time_end(c);
+    }
+    return BH_OK;
+  };
   if (c->autotune && !tu.done && !c->capturing && !noalloc) {
     // time every candidate shape on this stream (one warm launch, then the
     // fastest of three); all of them give bitwise identical values
@@ -2142,14 +2284,30 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
         }
       }
     }
+    if (!r && can_fuse) {
+      // the fused kernel with the chosen gather shape for its spill rows,
+      // against the best two-kernel pair (both give bitwise identical values)
+      float ms = 0.f;
+      r = timed_ms([&] { return fused(best.gather_lanes, best.gather_rows, false); }, &ms);
+      if (!r && ms < best.elem_ms + best.gather_ms) {
+        best.elem_waves = -1;
+        best.elem_ms = ms;
+        best.gather_ms = 0.f;
+      }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (r) return r;
     best.done = true;
     tu = best;
   }
-  if (int r = elements(tu.elem_waves, !noalloc)) return r;
-  if (int r = gather(tu.gather_lanes, tu.gather_rows, !noalloc)) return r;
+  // the fused kernel when the autotuner chose it or a caller fixed it
+  if (can_fuse && tu.elem_waves == -1) {
+    if (int r = fused(tu.gather_lanes, tu.gather_rows, !noalloc)) return r;
+  } else {
+    if (int r = elements(tu.elem_waves > 0 ? tu.elem_waves : 0, !noalloc)) return r;
+    if (int r = gather(tu.gather_lanes, tu.gather_rows, !noalloc)) return r;
+  }
   if (where == BH_HOST) {
     BH_HIP(c, hipMemcpyAsync(nzval, p->d_vals, (size_t)h.nnz * sizeof(double),
                              hipMemcpyDeviceToHost, c->stream));

@@ -284,6 +284,398 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS, KSR, WT>::NTH)) void 
 }
 
 // ---------------------------------------------------------------------------
+// Fused element matrices and row assembly (schedule: build_fused_schedule).
+// One workgroup walks one pencil of cells, 16 per step.  Each step's element
+// matrices are computed exactly as k_element_gp computes them (same factors,
+// same products, same order) and kept in LDS as FP32 upper triangles for two
+// steps.  The CSR rows whose element rows all lie in that window are summed
+// there in FP64, in the row gather's order (element rows by increasing cell,
+// column offsets from the step program), and written once; for the other rows
+// the kernel writes the element rows they read (the cells' spill masks) to ke
+// for the row gather.  Step t: [element rows of step t-1 for the gather,
+// program of step t-1 loaded, nodes of t+1 staged, step t contracted] ->
+// [program to LDS] -> [rows of step t-1 summed from steps t-1 and t-2] ->
+// [step t's triangles over step t-2's, factors of t+1].
+// ---------------------------------------------------------------------------
+constexpr int kProgCap = kFusedProgCap;
+
+template <int D, int NL, int NQ, int NG, bool MASS, int KSR, int WT>
+struct FusedShape {
+  using S = GPShape<D, NL, NQ, NG, MASS, KSR, WT>;
+  static constexpr int NTH = S::NTH;
+  static constexpr int NP = S::NP;
+  static constexpr int TS = ((NP + 3) / 4) * 4 + (((NP + 3) / 4) * 4 % 32 == 0 ? 4 : 0);
+  static constexpr int NGRP = NTH / 32;                           // lane groups of 32
+  static constexpr int ROWCAP = 128;
+  static_assert(NTH % 64 == 0 && NTH >= 256, "fused kernel: at least four waves");
+  static_assert(NL <= 32, "one lane per element-row entry in a 32-lane group");
+  static_assert(S::CPB * NL <= NTH, "one thread per element row of a step");
+};
+
+template <int NL>
+__device__ __forceinline__ int tri_index(int i, int j) {
+  const int a = i < j ? i : j, b = i < j ? j : i;
+  return a * NL - (a * (a - 1)) / 2 + (b - a);
+}
+
+template <int D, int NL, int NQ, int NG, bool MASS, int KSR, int WT>
+__global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS, KSR, WT>::NTH)) void k_element_gp_fused(
+    const double* __restrict__ coords, const int32_t* __restrict__ c2n,
+    const double* __restrict__ tab, const float* __restrict__ ptab,
+    const int32_t* __restrict__ bstep, const int32_t* __restrict__ scells,
+    const int64_t* __restrict__ prog_off, const uint8_t* __restrict__ prog,
+    const uint32_t* __restrict__ smask, int erow, const double* __restrict__ eta,
+    double eta_scale, const double* __restrict__ mcoef, double mass_scale,
+    float* __restrict__ ke, double* __restrict__ vals) {
+  using S = GPShape<D, NL, NQ, NG, MASS, KSR, WT>;
+  using F = FusedShape<D, NL, NQ, NG, MASS, KSR, WT>;
+  using L = TabLayout<D, NL, NQ, NG>;
+  constexpr int CPB = S::CPB, KG = S::KG, TPW = S::TPW, NTH = S::NTH, GS = S::GS;
+  constexpr int NC = S::NC, NS = S::NS, NT = S::NT, NP = S::NP, KS = S::KS;
+  constexpr int TS = F::TS, NGRP = F::NGRP, ROWCAP = F::ROWCAP;
+
+  __shared__ double s_dg[NQ * NG * D];
+  __shared__ double s_w[NQ];
+  __shared__ double s_x[CPB * NG + 1][D];
+  __shared__ double s_c[CPB + 1][2];
+  __shared__ int s_gc[4][CPB + 1];        // global cell ids of the steps in flight (-1: none)
+  __shared__ uint32_t s_sm[4][CPB + 1];   // their spill masks
+  __shared__ __attribute__((aligned(16))) float s_G[2][CPB * GS];
+  __shared__ __attribute__((aligned(16))) float s_T[3][CPB * TS];
+  __shared__ __attribute__((aligned(16))) uint8_t s_P[2][kProgCap];  // programs of steps t-1, t
+  __shared__ double s_B[NGRP][ROWCAP + 32];  // + a slot per lane for inactive lanes' adds
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int ln = lane & 15;
+  const int lg = lane >> 4;
+  for (int k = tid; k < NQ * NG * D; k += NTH) s_dg[k] = tab[L::DG + k];
+  for (int k = tid; k < NQ; k += NTH) s_w[k] = tab[L::W + k];
+  for (int k = tid; k < 2 * CPB * GS; k += NTH) (&s_G[0][0])[k] = 0.f;
+
+  float bp[TPW][KG];
+  int pcol[TPW];
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int t = wave * TPW + tt;
+    const int col = t * 16 + ln;
+    const int tc = t < NT ? t : NT - 1;
+#pragma unroll
+    for (int s = 0; s < KG; ++s) {
+      const float v = ptab[(size_t)(lg * KG + s) * (NT * 16) + tc * 16 + ln];
+      bp[tt][s] = t < NT ? v : 0.f;
+    }
+    pcol[tt] = (t < NT && col < NP) ? col : -1;
+  }
+
+  const int s0 = bstep[blockIdx.x];
+  const int T = bstep[blockIdx.x + 1] - s0;
+  if (T <= 0) return;
+  const int pc = tid / NG, pn = tid - (tid / NG) * NG;
+  const bool loader = tid < CPB * NG;
+  auto cell_of = [&](int t, int l) -> int {
+    return (t >= 0 && t < T) ? scells[(int64_t)(s0 + t) * CPB + l] : -1;
+  };
+  auto node_of = [&](int t) -> int {
+    const int cell = loader ? cell_of(t, pc) : -1;
+    return cell >= 0 ? c2n[(int64_t)cell * NG + pn] : -1;
+  };
+  auto load_x = [&](int node, double (&x)[D]) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = node >= 0 ? coords[(int64_t)node * D + a] : 0.0;
+  };
+  auto gid_of = [&](int t) -> int { return tid < CPB ? cell_of(t, tid) : -1; };
+  auto load_c = [&](int gid, double& e, double& mc, uint32_t& msk) {
+    e = (gid >= 0 && eta) ? eta[gid] * eta_scale : eta_scale;
+    mc = (gid >= 0 && mcoef) ? mcoef[gid] * mass_scale : mass_scale;
+    msk = gid >= 0 ? smask[gid] : 0u;
+  };
+  const int xslot = loader ? tid : CPB * NG;
+  const int cslot = tid < CPB ? tid : CPB;
+  auto stage_nodes = [&](int t, const double (&x)[D], double e, double mc, int gid, uint32_t msk) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) s_x[xslot][a] = x[a];
+    s_c[cslot][0] = e;
+    s_c[cslot][1] = mc;
+    s_gc[t & 3][cslot] = gid;
+    s_sm[t & 3][cslot] = msk;
+  };
+  // the geometry of k_element_gp, operation for operation
+  auto geometry = [&](float* __restrict__ gbuf) {
+#pragma unroll
+    for (int it = 0; it < S::NGEO; ++it) {
+      const int k = min(tid + it * NTH, CPB * NQ - 1);
+      const int c = k / NQ, q = k - (k / NQ) * NQ;
+      double J[D][D];
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) J[a][b] = 0.0;
+#pragma unroll
+      for (int n = 0; n < NG; ++n) {
+        double xn[D], dn[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) xn[a] = s_x[c * NG + n][a];
+#pragma unroll
+        for (int b = 0; b < D; ++b) dn[b] = s_dg[(q * NG + n) * D + b];
+#pragma unroll
+        for (int b = 0; b < D; ++b)
+#pragma unroll
+          for (int a = 0; a < D; ++a) J[a][b] += xn[a] * dn[b];
+      }
+      double Ji[D][D];
+      const double det = invert<D>(J, Ji);
+      const double wdet = s_w[q] * fabs(det);
+      const double ge = wdet * s_c[c][0];
+      float* g = gbuf + c * GS + q * NC;
+      int f = 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = a; b < D; ++b) {
+          double v = 0.0;
+#pragma unroll
+          for (int t = 0; t < D; ++t) v += Ji[a][t] * Ji[b][t];
+          g[f++] = (float)(ge * v);
+        }
+      if constexpr (MASS) g[NS] = (float)(wdet * s_c[c][1]);
+    }
+  };
+
+  // prologue: factors of step 0; coordinates of step 1 in registers
+  double xr[D], er, mr;
+  uint32_t sr;
+  int gid = gid_of(0);
+  load_x(node_of(0), xr);
+  load_c(gid, er, mr, sr);
+  stage_nodes(0, xr, er, mr, gid, sr);
+  load_x(node_of(1), xr);
+  gid = gid_of(1);
+  load_c(gid, er, mr, sr);
+  int nid = node_of(2);
+  int gidn = gid_of(2);
+  __syncthreads();
+  geometry(s_G[0]);
+  __syncthreads();
+
+  // Step t: the program of step t to registers; then, per wave, the
+  // contraction of step t (its triangles straight to LDS slot t mod 3) and its
+  // share of the rows completed by step t-1 (and of step t-1's element rows
+  // for the gather), in opposite orders in the two halves of the block, so
+  // that each SIMD runs one wave's matrix-core work beside the other wave's
+  // LDS and store work; then the program to LDS and the factors of t+1.
+  const bool mfma_first = wave < (NTH / 64) / 2;  // waves 0 .. W/2-1; the others load programs
+  // the lambdas take the thread index through gp_opaque each step, so that
+  // the compiler recomputes per-thread addresses instead of holding (and
+  // spilling) them across the loop next to the P operand
+  auto contract = [&](int t) {
+    if (t >= T) return;
+    floatx4 acc[TPW];
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt) acc[tt] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const float* ga = &s_G[t & 1][ln * GS + lg * KG];
+#pragma unroll
+    for (int s = 0; s < KG; s += 4) {
+      const floatx4 a = *(const floatx4*)(ga + s);
+#pragma unroll
+      for (int tt = 0; tt < TPW; ++tt) {
+        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], bp[tt][s + 0], acc[tt], 0, 0, 0);
+        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], bp[tt][s + 1], acc[tt], 0, 0, 0);
+        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], bp[tt][s + 2], acc[tt], 0, 0, 0);
+        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], bp[tt][s + 3], acc[tt], 0, 0, 0);
+      }
+    }
+    float* tri = s_T[t % 3];
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt) {
+      if (pcol[tt] < 0) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) tri[(4 * lg + v) * TS + pcol[tt]] = acc[tt][v];
+    }
+  };
+  // rows completed by step t-1, from the window (steps t-1 and t-2), and the
+  // element rows of step t-1 that spill rows read, to ke
+  auto assemble = [&](int t, int tl) {
+    if (t < 1) return;
+    const int g = tl >> 5, gl = tl & 31;
+    const bool jl = gl < NL;
+    const float* tri0 = s_T[(t + 2) % 3];  // step t-1
+    const float* tri1 = s_T[(t + 1) % 3];  // step t-2
+    {
+      const int sl = (t + 3) & 3;  // ring slot of step t-1
+      const int c = tl / NL, i = tl - (tl / NL) * NL;
+      const int gc = tl < CPB * NL ? s_gc[sl][c] : -1;
+      if (gc >= 0 && ((s_sm[sl][c] >> i) & 1u)) {
+#pragma unroll
+        for (int j4 = 0; j4 < KS; j4 += 4) {
+          floatx4 v;
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            v[e] = j4 + e < NL ? tri0[c * TS + tri_index<NL>(i, j4 + e)] : 0.f;
+          *(floatx4*)&ke[((int64_t)gc * erow + i) * KS + j4] = v;
+        }
+      }
+    }
+    const uint8_t* Pp = s_P[(t + 1) & 1];  // the program of step t-1
+    const uint32_t nrows = *(const uint32_t*)&Pp[0];
+    const uint8_t* R = &Pp[16];
+    const uint8_t* go = &Pp[16 + 32 * nrows];
+    double* buf = s_B[g];
+    const int junk = ROWCAP + gl;  // inactive lanes add zeros to their own slot
+    // row records (32 bytes: first value, length, element-row count, first
+    // column-offset block, up to 8 element-row codes) are read one row ahead
+    uint32_t r = g;
+    uint4 na = {0u, 0u, 0u, 0u}, nb = {0u, 0u, 0u, 0u};
+    if (r < nrows) {
+      na = *(const uint4*)(R + 32 * r);
+      nb = *(const uint4*)(R + 32 * r + 16);
+    }
+#pragma unroll 1
+    for (; r < nrows; r += NGRP) {
+      const uint4 ca = na, cb = nb;
+      if (r + NGRP < nrows) {
+        na = *(const uint4*)(R + 32 * (r + NGRP));
+        nb = *(const uint4*)(R + 32 * (r + NGRP) + 16);
+      }
+      const int64_t vb = (int64_t)((uint64_t)ca.x | ((uint64_t)ca.y << 32));
+      const int len = (int)(ca.z & 0xFFFFu);
+      const int cnt = (int)(ca.z >> 16);
+      const uint32_t q0 = ca.w;
+      const uint32_t cw[4] = {cb.x, cb.y, cb.z, cb.w};
+#pragma unroll
+      for (int k = 0; k < ROWCAP / 32; ++k)
+        if (gl + 32 * k < len) buf[gl + 32 * k] = 0.0;
+      float v[8];
+      int o[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const bool on = u < cnt && jl;
+        const int cd = on ? (int)((cw[u >> 1] >> (16 * (u & 1))) & 0xFFFFu) : 0;
+        const int i = cd % NL;
+        const int cl = cd / NL;  // rel * 16 + lane
+        const float* tri = cl >= CPB ? tri1 : tri0;
+        const int c = cl >= CPB ? cl - CPB : cl;
+        v[u] = on ? tri[c * TS + tri_index<NL>(i, jl ? gl : 0)] : 0.f;
+        o[u] = on ? (int)go[32 * (q0 + u) + gl] : junk;
+      }
+      // every value converted before the first add: the adds then issue back
+      // to back instead of each waiting for the LDS queue to drain
+      double dv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) dv[u] = (double)v[u];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (u < cnt)
+          __hip_atomic_fetch_add(&buf[o[u]], dv[u], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+      __builtin_amdgcn_sched_barrier(0);
+      double out[ROWCAP / 32];
+#pragma unroll
+      for (int k = 0; k < ROWCAP / 32; ++k) out[k] = buf[gl + 32 * k];
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < ROWCAP / 32; ++k)
+        if (gl + 32 * k < len) vals[vb + gl + 32 * k] = out[k];
+    }
+  };
+
+  // the program of step t into s_P[t & 1], by the waves that assemble first
+  // (while the others run their matrix-core work)
+  auto load_program = [&](int t, int tl) {
+    if (t >= T) return;
+    // the loaders are the waves that assemble first: threads LD0 .. NTH-1
+    constexpr int LD0 = ((NTH / 64) / 2) * 64;
+    constexpr int HALF = NTH - LD0;
+    const int ht = tl - LD0;
+    if (ht < 0) return;
+    const int64_t po = prog_off[s0 + t];
+    const int64_t pbytes = prog_off[s0 + t + 1] - po;
+    uint8_t* dst = s_P[t & 1];
+    constexpr int NL2 = (kProgCap / 16 + HALF - 1) / HALF;
+    floatx4 pv[NL2];
+#pragma unroll
+    for (int r = 0; r < NL2; ++r) {
+      const int64_t b = (int64_t)(ht + r * HALF) * 16;
+      pv[r] = b < pbytes ? *(const floatx4*)(prog + po + b) : floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int r = 0; r < NL2; ++r) {
+      const int64_t b = (int64_t)(ht + r * HALF) * 16;
+      if (b < pbytes) *(floatx4*)&dst[b] = pv[r];
+    }
+    if (pbytes == 0 && ht == 0) *(uint32_t*)&dst[0] = 0u;  // a step without local rows
+  };
+
+  for (int t = 0; t <= T; ++t) {
+    stage_nodes(t + 1, xr, er, mr, gid, sr);
+#pragma unroll 1
+    for (int ph = 0; ph < 2; ++ph) {
+      if ((ph == 0) == mfma_first) {
+        contract(t);
+      } else {
+        if (!mfma_first) load_program(t, gp_opaque(tid));
+        assemble(t, gp_opaque(tid));
+      }
+    }
+    __syncthreads();
+    load_x(nid, xr);
+    gid = gidn;
+    load_c(gid, er, mr, sr);
+    nid = node_of(t + 3);
+    gidn = gid_of(t + 3);
+    geometry(s_G[(t + 1) & 1]);
+    __syncthreads();
+  }
+}
+
+template <int D, int NL, int NQ, int NG>
+static hipError_t launch_gp_fused_t(const FusedArgs& A) {
+  constexpr int KSR = KeStride<NL>::value;
+  constexpr int NTH1 = GPShape<D, NL, NQ, NG, true, KSR, 8>::NTH;
+  constexpr int NTH0 = GPShape<D, NL, NQ, NG, false, KSR, 8>::NTH;
+  if (A.erow != NL || A.nblocks <= 0) return hipErrorInvalidValue;
+  dim3 grid(A.nblocks);
+  if (A.mass)
+    hipLaunchKernelGGL((k_element_gp_fused<D, NL, NQ, NG, true, KSR, 8>), grid, dim3(NTH1), 0,
+                       A.stream, A.coords, A.c2n, A.tab, A.ptab, A.bstep, A.cells, A.prog_off,
+                       A.prog, A.smask, A.erow, A.eta, A.eta_scale, A.mcoef, A.mass_scale,
+                       (float*)A.ke, A.vals);
+  else
+    hipLaunchKernelGGL((k_element_gp_fused<D, NL, NQ, NG, false, KSR, 8>), grid, dim3(NTH0), 0,
+                       A.stream, A.coords, A.c2n, A.tab, A.ptab, A.bstep, A.cells, A.prog_off,
+                       A.prog, A.smask, A.erow, A.eta, A.eta_scale, A.mcoef, A.mass_scale,
+                       (float*)A.ke, A.vals);
+  return hipGetLastError();
+}
+
+struct FusedEntry {
+  int geo, order, nq1;
+  hipError_t (*fn)(const FusedArgs&);
+};
+
+// elements whose step keeps at least four waves busy with matrix-core work
+static const FusedEntry kFused[] = {
+    {BH_HEXA8, 2, 2, &launch_gp_fused_t<3, 27, 8, 8>},
+    {BH_HEXA8, 2, 3, &launch_gp_fused_t<3, 27, 27, 8>},
+    {BH_QUAD4, 3, 4, &launch_gp_fused_t<2, 16, 16, 4>},
+    {BH_QUAD4, 4, 5, &launch_gp_fused_t<2, 25, 25, 4>},
+};
+
+bool gp_fused_supported(int geo, int order, int nq1) {
+  for (const FusedEntry& e : kFused)
+    if (e.geo == geo && e.order == order && e.nq1 == nq1) return true;
+  return false;
+}
+
+hipError_t launch_gp_fused(int geo, int order, int nq1, const FusedArgs& A) {
+  for (const FusedEntry& e : kFused)
+    if (e.geo == geo && e.order == order && e.nq1 == nq1) return e.fn(A);
+  return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
 template <int D, int NL, int NQ, int NG, int KSR, int WT>
 static hipError_t launch_gp_kw(const LaunchArgs& A) {
   constexpr int CPB = 16;

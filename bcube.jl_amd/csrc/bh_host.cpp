@@ -934,3 +934,213 @@ bool build_dg_faces(const HostMesh& m, HostDGFaces& F, std::string& err) {
 }
 
 }  // namespace bh
+
+// ---------------------------------------------------------------------------
+// Schedule of the fused element + assembly kernel (see bh_internal.h).
+// ---------------------------------------------------------------------------
+namespace bh {
+
+bool build_fused_schedule(const HostMesh& m, int nl, int64_t nrows,
+                          const std::vector<int64_t>& rowptr, const std::vector<int32_t>& pptr,
+                          const std::vector<int32_t>& pairs, const std::vector<uint8_t>& goff,
+                          int prog_cap, int min_blocks, FusedSchedule& s, std::string& err) {
+  const int d = m.dim;
+  const int64_t nc = m.ncells;
+  if (nc <= 0 || nl <= 0 || nl > 32 || (d != 2 && d != 3)) {
+    err = "fused schedule: unsupported mesh";
+    return false;
+  }
+  if ((int64_t)pptr.size() != nrows + 1 || (int64_t)rowptr.size() != nrows + 1 ||
+      (int64_t)goff.size() < nc * nl * nl) {
+    err = "fused schedule: inconsistent pattern arrays";
+    return false;
+  }
+  // 1. cell centroids and a cell size estimate from the bounding box
+  std::vector<double> cen((size_t)nc * d, 0.0);
+  parallel_for(nc, [&](int64_t b, int64_t e) {
+    for (int64_t c = b; c < e; ++c) {
+      const int64_t k0 = m.c2n_ptr[c], k1 = m.c2n_ptr[c + 1];
+      for (int64_t k = k0; k < k1; ++k)
+        for (int a = 0; a < d; ++a) cen[c * d + a] += m.coords[m.c2n[k] * d + a];
+      for (int a = 0; a < d; ++a) cen[c * d + a] /= (double)(k1 - k0);
+    }
+  });
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t c = 0; c < nc; ++c)
+    for (int a = 0; a < d; ++a) {
+      lo[a] = std::min(lo[a], cen[c * d + a]);
+      hi[a] = std::max(hi[a], cen[c * d + a]);
+    }
+  double vol = 1.0;
+  double ext[3];
+  for (int a = 0; a < d; ++a) {
+    ext[a] = std::max(hi[a] - lo[a], 1e-300);
+    vol *= ext[a];
+  }
+  // centroids span (n - 1) cells per axis on a structured box
+  const double h = std::pow(vol / (double)nc, 1.0 / d);
+  // 2. pencils: tiles of the first d-1 axes, layers along the last axis
+  const double tw = d == 3 ? 4.0 : 16.0;  // tile width in cells
+  int nt[2] = {1, 1};
+  for (int a = 0; a < d - 1; ++a)
+    nt[a] = std::max(1, (int)std::floor((ext[a] + h) / (tw * h) + 0.5));
+  auto tile_of = [&](int64_t c, int a) {
+    const double u = (cen[c * d + a] - lo[a] + 0.5 * h) / (ext[a] + h);
+    return std::min(nt[a] - 1, std::max(0, (int)std::floor(u * nt[a])));
+  };
+  const int la = d - 1;  // layer axis
+  auto layer_of = [&](int64_t c) { return (int64_t)std::floor((cen[c * d + la] - lo[la]) / h + 0.5); };
+  std::vector<int64_t> order(nc);
+  std::vector<int64_t> pen(nc), lay(nc);
+  for (int64_t c = 0; c < nc; ++c) {
+    order[c] = c;
+    pen[c] = d == 3 ? (int64_t)tile_of(c, 1) * nt[0] + tile_of(c, 0) : tile_of(c, 0);
+    lay[c] = layer_of(c);
+  }
+  std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+    if (pen[x] != pen[y]) return pen[x] < pen[y];
+    if (lay[x] != lay[y]) return lay[x] < lay[y];
+    for (int a = d - 2; a >= 0; --a)
+      if (cen[x * d + a] != cen[y * d + a]) return cen[x * d + a] < cen[y * d + a];
+    return x < y;
+  });
+  // 3. steps of 16 cells per pencil; with fewer pencils than min_blocks each
+  //    pencil is cut into runs of at least 8 steps, one workgroup per run
+  //    (rows across a cut are spill rows)
+  s = FusedSchedule();
+  s.nl = nl;
+  std::vector<int64_t> pstart;
+  for (int64_t k = 0; k < nc; ++k)
+    if (k == 0 || pen[order[k]] != pen[order[k - 1]]) pstart.push_back(k);
+  pstart.push_back(nc);
+  const int64_t npen = (int64_t)pstart.size() - 1;
+  const int64_t cuts = std::max<int64_t>(1, (min_blocks + npen - 1) / std::max<int64_t>(npen, 1));
+  std::vector<int32_t> cstep(nc), clane(nc), cblk(nc);
+  s.bstep.push_back(0);
+  int64_t nsteps = 0;
+  for (int64_t pp = 0; pp < npen; ++pp) {
+    const int64_t k = pstart[pp], e = pstart[pp + 1];
+    const int64_t psteps = (e - k + 15) / 16;
+    const int64_t nrun = std::max<int64_t>(1, std::min(cuts, psteps / 8));  // runs of >= 8 steps
+    for (int64_t ru = 0; ru < nrun; ++ru) {
+      const int64_t q0 = k + 16 * (psteps * ru / nrun), q1 = std::min(e, k + 16 * (psteps * (ru + 1) / nrun));
+      for (int64_t q = q0; q < q1; q += 16) {
+        for (int l = 0; l < 16; ++l) {
+          if (q + l < q1) {
+            const int64_t c = order[q + l];
+            cstep[c] = (int32_t)nsteps;
+            clane[c] = l;
+            cblk[c] = s.nblocks;
+            s.cells.push_back((int32_t)c);
+          } else {
+            s.cells.push_back(-1);
+          }
+        }
+        ++nsteps;
+      }
+      ++s.nblocks;
+      s.bstep.push_back((int32_t)nsteps);
+    }
+  }
+  // 4. local rows: all element rows in one block, within two consecutive steps
+  std::vector<int32_t> row_step(nrows, -1);
+  parallel_for(nrows, [&](int64_t b, int64_t e) {
+    for (int64_t r = b; r < e; ++r) {
+      const int32_t p0 = pptr[r], p1 = pptr[r + 1];
+      if (p1 <= p0 || rowptr[r + 1] - rowptr[r] > 128) continue;
+      const int64_t c0 = pairs[p0] / nl;
+      int32_t smin = cstep[c0], smax = cstep[c0];
+      bool ok = true;
+      for (int32_t q = p0; q < p1 && ok; ++q) {
+        const int64_t c = pairs[q] / nl;
+        ok = cblk[c] == cblk[c0];
+        smin = std::min(smin, cstep[c]);
+        smax = std::max(smax, cstep[c]);
+      }
+      if (ok && smax - smin <= 1) row_step[r] = smax;
+    }
+  });
+  // 5. step programs (rows ascending); rows that do not fit spill.  Layout:
+  //    header {u32 rows, u32 pairs, 8 bytes 0}, then per row 32 bytes
+  //    {i64 first value, u16 length, u16 element rows, u32 first pair, u16
+  //    codes[8]}, then per pair 32 column offsets (0xFF: none); code =
+  //    (step - cell's step) * 16 + cell's lane) * nl + element row
+  std::vector<std::vector<int32_t>> rows_of(nsteps);
+  for (int64_t r = 0; r < nrows; ++r)
+    if (row_step[r] >= 0) {
+      if (pptr[r + 1] - pptr[r] > kFusedRowCodes) row_step[r] = -1;
+      else rows_of[row_step[r]].push_back((int32_t)r);
+    }
+  s.prog_off.assign(nsteps + 1, 0);
+  std::vector<int64_t> psize(nsteps, 0);
+  for (int64_t st = 0; st < nsteps; ++st) {
+    int64_t nr = 0, np = 0;
+    for (int32_t r : rows_of[st]) {
+      const int64_t np2 = np + (pptr[r + 1] - pptr[r]);
+      const int64_t sz = 16 + 32 * (nr + 1) + 32 * np2;
+      if (sz > prog_cap) {
+        row_step[r] = -1;  // spills
+        continue;
+      }
+      ++nr;
+      np = np2;
+    }
+    psize[st] = nr ? 16 + 32 * nr + 32 * np : 0;
+    s.prog_off[st + 1] = s.prog_off[st] + psize[st];
+  }
+  s.prog.assign((size_t)s.prog_off[nsteps], 0);
+  parallel_for(nsteps, [&](int64_t b, int64_t e) {
+    for (int64_t st = b; st < e; ++st) {
+      if (!psize[st]) continue;
+      uint8_t* P = s.prog.data() + s.prog_off[st];
+      uint32_t nr = 0, np = 0;
+      for (int32_t r : rows_of[st])
+        if (row_step[r] == st) { ++nr; np += (uint32_t)(pptr[r + 1] - pptr[r]); }
+      std::memcpy(P, &nr, 4);
+      std::memcpy(P + 4, &np, 4);
+      uint8_t* R = P + 16;
+      uint8_t* G = P + 16 + 32 * (int64_t)nr;
+      std::memset(G, 0xFF, (size_t)32 * np);
+      uint32_t k = 0, q = 0;
+      for (int32_t r : rows_of[st]) {
+        if (row_step[r] != st) continue;
+        const int64_t vb = rowptr[r];
+        const uint16_t len = (uint16_t)(rowptr[r + 1] - rowptr[r]);
+        const uint16_t cnt = (uint16_t)(pptr[r + 1] - pptr[r]);
+        uint8_t* rec = R + 32 * (int64_t)k;
+        std::memcpy(rec, &vb, 8);
+        std::memcpy(rec + 8, &len, 2);
+        std::memcpy(rec + 10, &cnt, 2);
+        std::memcpy(rec + 12, &q, 4);
+        int u = 0;
+        for (int32_t pq = pptr[r]; pq < pptr[r + 1]; ++pq, ++q, ++u) {
+          const int64_t c = pairs[pq] / nl;
+          const int i = (int)(pairs[pq] - c * nl);
+          const int rel = (int)(st - cstep[c]);
+          const uint16_t code = (uint16_t)((rel * 16 + clane[c]) * nl + i);
+          std::memcpy(rec + 16 + 2 * u, &code, 2);
+          std::memcpy(G + 32 * (int64_t)q, &goff[(size_t)pairs[pq] * nl], (size_t)nl);
+        }
+        ++k;
+      }
+    }
+  });
+  for (int64_t st = 0; st < nsteps; ++st) s.max_prog = std::max<int64_t>(s.max_prog, psize[st]);
+  // 6. spill rows and the element rows they read
+  s.spill_mask.assign(nc, 0u);
+  for (int64_t r = 0; r < nrows; ++r) {
+    if (row_step[r] >= 0) {
+      ++s.local_rows;
+      s.local_pairs += pptr[r + 1] - pptr[r];
+      continue;
+    }
+    s.spill_rows.push_back((int32_t)r);
+    for (int32_t q = pptr[r]; q < pptr[r + 1]; ++q) {
+      const int64_t c = pairs[q] / nl;
+      s.spill_mask[c] |= 1u << (pairs[q] - c * nl);
+    }
+  }
+  return true;
+}
+
+}  // namespace bh

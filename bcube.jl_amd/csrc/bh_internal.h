@@ -147,6 +147,48 @@ bool pattern_device(hipStream_t stream, const int32_t* d_dofs, int nl, int64_t n
                     int64_t ncells, HostPattern& h, int64_t** d_rowptr, int32_t** d_pptr,
                     int32_t** d_pairs, void** d_goff, std::string& err);
 
+// Schedule of the fused element + assembly kernel (bh_fused.hip) on a
+// uniform mesh.  The cells are ordered into "pencils" (3D: a 4 x 4-cell tile
+// in x, y marching in z; 2D: a 16-cell strip in x marching in y, both by cell
+// centroids), each cut into steps of 16 cells, the kernel's MFMA group; one
+// workgroup walks one pencil.  After step s the element matrices of steps s and
+// s-1 of its pencil are in LDS: every CSR row whose element rows all lie in
+// that window ("local" row) is summed there, in the row gather's order, and
+// written once; the others ("spill" rows) are summed by the row gather from
+// the element rows the kernel writes for them.  Step programs, 16-byte
+// aligned:
+//   u32 nrows, u32 npairs, 8 bytes pad;
+//   nrows x {i64 value offset, u32 length, u16 first pair, u16 pairs} (16 B);
+//   npairs x u16 window code (rel*16 + lane)*nl + i, rel = 0 for this step's
+//   group and 1 for the previous one (padded to 16 B);
+//   npairs x 32 bytes: the element row's column offsets inside the CSR row
+//   (entry j < nl; 0xFF past nl).
+// Rows keep their element rows in increasing cell order, so the sums equal
+// the row gather's bit for bit.
+constexpr int kFusedRowCodes = 8;  // element rows of a row summed in LDS
+struct FusedSchedule {
+  int nl = 0;
+  int nblocks = 0;
+  std::vector<int32_t> bstep;       // nblocks+1: each block's steps in the step list
+  std::vector<int32_t> cells;       // nsteps * 16 global cell ids, -1 = empty slot
+  std::vector<int64_t> prog_off;    // nsteps + 1 byte offsets into prog
+  std::vector<uint8_t> prog;
+  int max_prog = 0;                 // bytes of the largest step program
+  std::vector<int32_t> spill_rows;  // ascending
+  std::vector<uint32_t> spill_mask; // per cell: bit i = element row i is read by a spill row
+  int64_t local_rows = 0, local_pairs = 0;
+};
+// pptr/pairs: each CSR row's element rows (cell*nl + i), increasing cell
+// order; goff: uint8 column offsets per element-row entry.  Rows longer than
+// 128 entries, with more than kFusedRowCodes element rows, or beyond prog_cap
+// bytes of a step program, are spilled.
+// With fewer pencils than min_blocks, pencils are cut into runs of steps
+// (at least 8 steps each) toward min_blocks workgroups.
+bool build_fused_schedule(const HostMesh& m, int nl, int64_t nrows,
+                          const std::vector<int64_t>& rowptr, const std::vector<int32_t>& pptr,
+                          const std::vector<int32_t>& pairs, const std::vector<uint8_t>& goff,
+                          int prog_cap, int min_blocks, FusedSchedule& s, std::string& err);
+
 // Simple host thread pool loop.
 void parallel_for(i64 n, const std::function<void(i64, i64)>& body);
 

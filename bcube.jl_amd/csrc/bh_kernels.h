@@ -41,7 +41,8 @@ struct LaunchArgs {
   bool padded_pairs = false;
   int max_adj = 0;
   const void* goff = nullptr;
-  int64_t nrows = 0;
+  int64_t nrows = 0;              // rows to sum: all rows, or the length of rlist
+  const int32_t* rlist = nullptr;  // row gather over these rows only (NULL: 0 .. nrows-1)
   int max_row = 0;
   const double* eta = nullptr;
   double eta_scale = 0.0;
@@ -170,5 +171,35 @@ bool gp_supported(int geo_celltype, int order, int nq1);
 // block shape for this element (1 or 2 entries: 8, then 12).
 int gp_wave_targets(int geo_celltype, int order, int nq1, int targets[2]);
 hipError_t launch_gp(int geo_celltype, int order, int nq1, const LaunchArgs& a);
+
+// Fused element matrices + row assembly (bh_gp.hip, FP32 GEMM form) on a
+// uniform mesh, driven by a FusedSchedule (bh_internal.h) on the device:
+// one workgroup per pencil (bstep), the step cells, step programs, spill
+// masks.  Local CSR rows are written into vals; the element rows of spill
+// rows go to ke for launch_gather over the spill rows.
+struct FusedArgs {
+  hipStream_t stream = nullptr;
+  const double* coords = nullptr;
+  const int32_t* c2n = nullptr;
+  const double* tab = nullptr;
+  const float* ptab = nullptr;
+  int nblocks = 0;
+  const int32_t* bstep = nullptr;
+  const int32_t* cells = nullptr;
+  const int64_t* prog_off = nullptr;
+  const uint8_t* prog = nullptr;
+  const uint32_t* smask = nullptr;
+  int erow = 0;
+  const double* eta = nullptr;
+  double eta_scale = 0.0;
+  const double* mcoef = nullptr;
+  double mass_scale = 0.0;
+  bool mass = false;
+  void* ke = nullptr;
+  double* vals = nullptr;
+};
+constexpr int kFusedProgCap = 16384;  // bytes of one step program (the kernel's LDS buffer)
+bool gp_fused_supported(int geo_celltype, int order, int nq1);
+hipError_t launch_gp_fused(int geo_celltype, int order, int nq1, const FusedArgs& a);
 
 }  // namespace bh

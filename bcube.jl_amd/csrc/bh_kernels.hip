@@ -161,7 +161,9 @@ __global__ __launch_bounds__(kBlock) void k_element_f64(
 template <typename T, typename OFF, int NL, int ROWCAP, int MAXP, int RPH, int SG>
 __global__ __launch_bounds__(kBlock) void k_gather_padded(
     const T* __restrict__ ke, const int32_t* __restrict__ pairs, const OFF* __restrict__ goff,
-    const int64_t* __restrict__ rowptr, int64_t nrows, double* __restrict__ vals) {
+    const int64_t* __restrict__ rowptr, int64_t nrows, const int32_t* __restrict__ rlist,
+    double* __restrict__ vals) {
+  // rlist: the rows to sum (nrows of them), or NULL for rows 0 .. nrows-1
   constexpr int KS = KeStride<NL>::value;
   constexpr int HW = kBlock / SG;        // lane groups per block
   constexpr int NJ = (NL + SG - 1) / SG;
@@ -176,8 +178,9 @@ __global__ __launch_bounds__(kBlock) void k_gather_padded(
     int32_t pr[RPH][MAXP];
 #pragma unroll
     for (int r = 0; r < RPH; ++r) {
-      const int64_t row = row0 + r;
-      const bool ok = row < nrows;
+      const int64_t ri = row0 + r;
+      const bool ok = ri < nrows;
+      const int64_t row = ok ? (rlist ? (int64_t)rlist[ri] : ri) : 0;
       rs[r] = ok ? rowptr[row] : 0;
       len[r] = ok ? (int)(rowptr[row + 1] - rs[r]) : 0;
 #pragma unroll
@@ -221,7 +224,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(const T* __restrict__ ke,
                                                    const int32_t* __restrict__ pairs,
                                                    const OFF* __restrict__ goff,
                                                    const int64_t* __restrict__ rowptr,
-                                                   int64_t nrows, double* __restrict__ vals) {
+                                                   int64_t nrows,
+                                                   const int32_t* __restrict__ rlist,
+                                                   double* __restrict__ vals) {
   constexpr int KS = KeStride<NL>::value;
   constexpr int HW = kBlock / 32;
   constexpr int PB = 8;                 // element rows loaded before they are added
@@ -230,7 +235,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(const T* __restrict__ ke,
   __shared__ double buf[HW][ROWCAP];
   const int h = threadIdx.x >> 5;
   const int l = threadIdx.x & 31;
-  for (int64_t row = (int64_t)blockIdx.x * HW + h; row < nrows; row += (int64_t)gridDim.x * HW) {
+  for (int64_t ri = (int64_t)blockIdx.x * HW + h; ri < nrows; ri += (int64_t)gridDim.x * HW) {
+    const int64_t row = rlist ? (int64_t)rlist[ri] : ri;
     const int64_t rs = rowptr[row];
     const int len = (int)(rowptr[row + 1] - rs);
     for (int k = l; k < len; k += 32) buf[h][k] = 0.0;
@@ -382,7 +388,7 @@ static hipError_t gather_padded_go(const LaunchArgs& A) {
 #define BH_GP(MP)                                                                              \
   hipLaunchKernelGGL((k_gather_padded<T, OFFT, NL, CAP, MP, RPH, SG>), grid, dim3(kBlock), 0,   \
                      A.stream, (const T*)A.ke, A.pairs, (const OFFT*)A.goff, A.rowptr, A.nrows, \
-                     A.vals)
+                     A.rlist, A.vals)
   if (A.max_adj <= 4) BH_GP(4);
   else if (A.max_adj <= 8) BH_GP(8);
   else BH_GP(16);
@@ -443,7 +449,7 @@ static hipError_t gather_nl(const LaunchArgs& A) {
 #define BH_G(T, OFFT, CAP)                                                                  \
   hipLaunchKernelGGL((k_gather<T, OFFT, NL, CAP>), grid, dim3(kBlock), 0, A.stream,         \
                    (const T*)A.ke, A.pptr, A.pairs, (const OFFT*)A.goff, A.rowptr, A.nrows, \
-                   A.vals)
+                   A.rlist, A.vals)
     if (A.max_row <= 128 && A.offset_bytes == 1) {
       if (f64) BH_G(double, uint8_t, 128); else BH_G(float, uint8_t, 128);
     } else if (A.max_row <= 256 && A.offset_bytes == 1) {

@@ -37,6 +37,7 @@ KERNEL = {"fp64": "k_element_mfma<3,27,27,8,true,2>",
 GATHER = {"fp64": "k_gather_padded<double,unsigned char,27,128,8,2,32>",
           "fp32": "k_gather_padded<float,unsigned char,27,128,8,2,32>",
           "bf16x3": "k_gather_padded<float,unsigned char,27,128,8,2,32>"}
+FUSED_KERNEL = "k_element_gp_fused<3,27,27,8,true,32,8> (element matrices + LDS row sums)"
 
 
 def workload(n):
@@ -57,15 +58,18 @@ def kernel_rooflines(mode, launches, kinds, steps, f_alg, nnz):
     launch times are summed per kernel kind.  Returns (dominant, [element, gather])."""
     t = np.asarray(launches, dtype=np.float64)
     k = np.asarray(kinds)
-    el_ms = float(t[k == 0].sum() / steps)
-    ga_ms = float(t[k == 1].sum() / steps)
+    fused = bool((k == 3).any())  # the fused element + row-assembly kernel (kind 3)
+    el_mask = (k == 0) | (k == 3)
+    el_ms = max(float(t[el_mask].sum() / steps), 1e-9)
+    ga_ms = max(float(t[k == 1].sum() / steps), 1e-9)
     peak_f = PEAK_TFLOPS[mode]
     el_ach = f_alg / (el_ms * 1e-3) / 1e12
     ga_bytes = 8.0 * nnz
     ga_ach = ga_bytes / (ga_ms * 1e-3) / 1e9
     el = {"bound": "mfma", "achieved": el_ach, "peak": peak_f, "unit": "TFLOP/s",
-          "frac": el_ach / peak_f, "kernel": KERNEL[mode], "avg_launch_ms": el_ms,
-          "launches_per_step": int((k == 0).sum() // steps), "flops_per_launch": f_alg}
+          "frac": el_ach / peak_f, "kernel": FUSED_KERNEL if fused else KERNEL[mode],
+          "avg_launch_ms": el_ms, "launches_per_step": int(el_mask.sum() // steps),
+          "flops_per_launch": f_alg}
     ga = {"bound": "hbm", "achieved": ga_ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
           "frac": ga_ach / PEAK_HBM_GBS, "kernel": GATHER[mode], "avg_launch_ms": ga_ms,
           "launches_per_step": int((k == 1).sum() // steps), "alg_bytes_per_launch": ga_bytes}

@@ -94,7 +94,8 @@ int bh_sync(bh_ctx ctx);
 int bh_ctx_set_timing(bh_ctx ctx, int enable);
 int bh_ctx_timings(bh_ctx ctx, float* ms, int cap, int* count);
 /* Same, with the kind of each timed launch: 0 element kernel, 1 row gather,
- * 2 other (DG, linear forms, quadrature points). */
+ * 2 other (DG, linear forms, quadrature points), 3 fused element kernel and
+ * row assembly (FP32; its spill rows then come as a row gather). */
 int bh_ctx_timings_ex(bh_ctx ctx, float* ms, int32_t* kinds, int cap, int* count);
 /* Autotuning of launch shapes (off by default).  When enabled, the first
  * bh_assemble_bilinear of a pattern in each mode outside graph capture times
@@ -156,9 +157,14 @@ int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out);
 int bh_pattern_build_device(bh_space U, bh_space V, bh_pattern* out);
 /* max_row: the longest CSR row (entries). */
 /* Launch shapes the autotuner chose for `mode`: shape[0] element-kernel waves
- * per block target, shape[1] gather lanes per row, shape[2] gather rows per
- * lane group (0: the default); ms[0], ms[1] their measured times.  Returns 1
- * once tuned, 0 before, negative on error.  No counterpart in the reference. */
+ * per block target (8 or 12), or -1 for the fused element + row-assembly
+ * kernel (FP32 on uniform meshes of the elements that have one: one workgroup
+ * per pencil of cells, the CSR rows complete within two steps of a pencil
+ * summed in LDS and written once, the others by the row gather), shape[1]
+ * gather lanes per row, shape[2] gather rows per lane group (0: the default;
+ * the fused kernel runs only when the autotuner measured it faster or a caller
+ * set -1); ms[0], ms[1] their measured times.  Every shape gives bitwise identical values.  Returns 1 once tuned,
+ * 0 before, negative on error.  No counterpart in the reference. */
 int bh_pattern_tuning(bh_pattern p, int mode, int32_t* shape, float* ms);
 /* Fix the launch shapes for `mode` (same layout as bh_pattern_tuning's shape;
  * zeros: the defaults), e.g. from an earlier tuning; the autotuner then leaves

@@ -7,9 +7,10 @@ one JSON line (see the field list in bench_c2()).
     python bench.py [--steps K] [--warmup W] [--mode all|fp64|fp32|bf16x3] [--n 64]
                     [--config all|c1|c2|c3|c4|c5|...] [--no-cpu] [--no-check]
 
-With --mode all (default) the FP64, FP32-MFMA and BF16x3 forms are timed in
-turn; `value` is the FP32-MFMA form (the headline), every mode is listed under
-"modes" with its error against the FP64 GPU result.  The default --config all
+With --mode all (default) the FP64 and FP32-MFMA forms are timed in turn
+(BF16x3 only with --mode bf16x3); `value` is the FP32-MFMA form (the
+headline), every mode is listed under "modes" with its error against the FP64
+GPU result.  The default --config all
 adds the other workloads (C1 2D Poisson, C3 mixed tri/quad, C4 DG transport,
 C5 many-instance batch) under "configs", each with its dominant kernel's
 roofline, the CPU baseline (the oracle, 1 thread and all cores) and its matrix
@@ -544,7 +545,7 @@ def bench_c1(args, torch, B):
          "frac": src_ach / PEAK_HBM_GBS, "kernel": "vector fill + coloured k_source_f64",
          "avg_launch_ms": k_ms[2], "alg_bytes_per_launch": src_bytes}]
     roof = dict(max(kernels, key=lambda k: k["avg_launch_ms"]))
-    roof["traffic"] = None
+    roof["traffic"] = traffic_of("c1_64", roof["kernel"])
     roof["note"] = "launch-latency bound: every kernel here runs a few microseconds"
     cpu, mat, sol = (None, None, None) if args.no_check else \
         c1_cpu_and_checks(B, plan, U, m, timed=not args.no_cpu)
@@ -1712,7 +1713,9 @@ def bench_c2(args, torch, B):
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    modes = ["fp64", "fp32", "bf16x3"] if args.mode == "all" else [args.mode]
+    # BF16x3 runs only when asked for: its two-term split keeps about 16
+    # mantissa bits per operand, which cannot meet the 1e-6 bound at any speed
+    modes = ["fp64", "fp32"] if args.mode == "all" else [args.mode]
     results = {}
     ref = None
     for mode in modes:

@@ -90,6 +90,11 @@ struct bh_space_s {
   double* d_b = nullptr;      // scratch output for host-side linear forms
   double* d_coef = nullptr;   // staged coefficients
   size_t coef_cap = 0;
+  // FP32 linear forms: element vectors (ncells * nl floats) and, per dof, its
+  // element-vector entries cell*nl + i in increasing cell order
+  float* d_be = nullptr;
+  int32_t* d_vptr = nullptr;
+  int32_t* d_vpair = nullptr;
 };
 
 // Launch shapes chosen by the autotuner for one pattern and mode (0: default).
@@ -424,6 +429,33 @@ int get_gp_table(bh_ctx c, int geo, int order, int quad_degree, bool mass, float
   BH_HIP(c, hipMemcpy(dp, P.data(), P.size() * sizeof(float), hipMemcpyHostToDevice));
   c->gp_tables[key] = dp;
   *out = dp;
+  return BH_OK;
+}
+
+// FP32 linear forms: per dof, its element-vector entries cell*nl + i in
+// increasing cell order (the order of the reference's cell loop,
+// assembler.jl:855-865), and the element-vector workspace.
+int build_vector_gather(bh_ctx c, bh_space V) {
+  const int64_t nc = V->mesh->h.ncells, nl = V->nl, nd = V->ndofs;
+  if (nc * nl >= (int64_t)INT32_MAX)
+    return fail(c, BH_ERR_RANGE, "FP32 linear forms: more than 2^31 element-vector entries");
+  if ((int64_t)V->dofs.size() != nc * nl) return fail(c, BH_ERR_ARG, "space has no dof table");
+  std::vector<int32_t> ptr((size_t)nd + 1, 0);
+  for (int64_t k = 0; k < nc * nl; ++k)
+    if (V->dofs[k] >= 0) ++ptr[(size_t)V->dofs[k] + 1];
+  for (int64_t r = 0; r < nd; ++r) ptr[r + 1] += ptr[r];
+  std::vector<int32_t> pair((size_t)ptr[nd]);
+  std::vector<int32_t> next(ptr.begin(), ptr.end() - 1);
+  for (int64_t k = 0; k < nc * nl; ++k)
+    if (V->dofs[k] >= 0) pair[next[V->dofs[k]]++] = (int32_t)k;
+  BH_HIP(c, dalloc(&V->d_vptr, (size_t)nd + 1));
+  BH_HIP(c, dalloc(&V->d_vpair, pair.size()));
+  BH_HIP(c, dalloc(&V->d_be, (size_t)(nc * nl)));
+  BH_HIP(c, hipMemcpyAsync(V->d_vptr, ptr.data(), ptr.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice, c->stream));
+  BH_HIP(c, hipMemcpyAsync(V->d_vpair, pair.data(), pair.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice, c->stream));
+  BH_HIP(c, hipStreamSynchronize(c->stream));  // the host tables go out of scope
   return BH_OK;
 }
 
@@ -792,6 +824,9 @@ int bh_space_destroy(bh_space s) {
     (void)hipFree(s->col.d_lcells);
     (void)hipFree(s->d_b);
     (void)hipFree(s->d_coef);
+    (void)hipFree(s->d_be);
+    (void)hipFree(s->d_vptr);
+    (void)hipFree(s->d_vpair);
   }
   delete s;
   return BH_OK;
@@ -2358,7 +2393,8 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
   bh_mesh m = V->mesh;
   bh_ctx c = m->ctx;
   if (!c) return fail(c, BH_ERR_ARG, "assembly needs a mesh created with a context");
-  if (mode != BH_FP64) return fail(c, BH_ERR_UNSUPPORTED, "only BH_FP64 is implemented");
+  if (mode != BH_FP64 && mode != BH_FP32)
+    return fail(c, BH_ERR_UNSUPPORTED, "linear forms run in BH_FP64 or BH_FP32");
   if (!b) return fail(c, BH_ERR_ARG, "b is NULL");
   if (where == BH_HOST && c->capturing)
     return fail(c, BH_ERR_ARG, "host-`where` assembly cannot be captured in a graph");
@@ -2375,6 +2411,9 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
     bkey[bk] = rule_key(geo, quad_degree);
     if (!kernel_supported(geo, V->order, bkey[bk]))
       return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this (cell type, order, quadrature)");
+    if (mode == BH_FP32 && !source_gp_supported(geo, V->order, bkey[bk]))
+      return fail(c, BH_ERR_UNSUPPORTED,
+                  "no FP32 linear-form kernel for this (cell type, order, quadrature)");
     nq = std::max(nq, geo == BH_TRI3 ? bkey[bk]
                                      : bkey[bk] * bkey[bk] * (m->h.dim == 3 ? bkey[bk] : 1));
   }
@@ -2387,7 +2426,12 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
   std::vector<double*> btab(nb, nullptr);
   for (size_t bk = 0; bk < nb; ++bk)
     if (int r = get_tables(c, m->buckets[bk].geo, V->order, quad_degree, &btab[bk])) return r;
-  if (V->col.ncolors == 0) {
+  if (mode == BH_FP32 && !V->d_vptr) {
+    if (c->capturing)
+      return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
+    if (int r = build_vector_gather(c, V)) return r;
+  }
+  if (mode == BH_FP64 && V->col.ncolors == 0) {
     if (c->capturing)
       return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
     if (int r = build_colouring(c, V->dofs, V->nl, V->ndofs, m->h.ncells, V->col,
@@ -2430,6 +2474,25 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
       } else {
         A.f = t.coef;
       }
+    }
+    if (mode == BH_FP32) {
+      // element vectors of every cell type on the matrix cores, then one sum
+      // per dof in cell order
+      A.ke = V->d_be;
+      for (size_t bk = 0; bk < nb; ++bk) {
+        const GeoBucket& gb = m->buckets[bk];
+        A.c2n = gb.d_c2n;
+        A.cells = gb.d_cells;
+        A.ncells_total = gb.n;
+        A.tab = btab[bk];
+        time_begin(c);
+        BH_HIP(c, launch_source_gp(gb.geo, V->order, bkey[bk], A));
+        time_end(c);
+      }
+      time_begin(c);
+      BH_HIP(c, launch_vgather(c->stream, V->d_vptr, V->d_vpair, V->d_be, V->ndofs, db));
+      time_end(c);
+      continue;
     }
     // one colour at a time (its cells share no dof row), on a mixed mesh one
     // launch per cell type inside it

@@ -167,6 +167,13 @@ constexpr int gp_kgroup(int dim, int nq, bool mass) {
 }
 constexpr int gp_ntiles(int nl) { return (nl * (nl + 1) / 2 + 15) / 16; }
 bool gp_supported(int geo_celltype, int order, int nq1);
+// FP32 linear forms (bh_source.hip): element vectors of one cell bucket on the
+// matrix cores into a.ke (float, cell*erow + i), and their sum into b per dof
+// in cell order.
+bool source_gp_supported(int geo_celltype, int order, int nq1);
+hipError_t launch_source_gp(int geo_celltype, int order, int nq1, const LaunchArgs& a);
+hipError_t launch_vgather(hipStream_t stream, const int32_t* vptr, const int32_t* vpair,
+                          const float* be, int64_t ndofs, double* b);
 // Waves-per-block targets of the GEMM element kernel that give a different
 // block shape for this element (1 or 2 entries: 8, then 12).
 int gp_wave_targets(int geo_celltype, int order, int nq1, int targets[2]);

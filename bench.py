@@ -447,8 +447,10 @@ def c1_cpu_and_checks(B, plan, U, m, timed=True, reps=21):
     mat = {mode: csc_err(plan, A_ref, mode) for mode in ("fp64", "fp32")}
     dΩ = B.Measure(B.CellDomain(m), 2)
     f = B.PhysicalFunction(lambda x: np.sin(np.pi * x[:, 0]) * np.sin(np.pi * x[:, 1]))
-    b = B.assemble_linear(lambda v: B.integrate(f * v, dΩ), B.TestFESpace(U))
-    mat["rhs"] = rel_err(b, b_ref)
+    rhs = {mode: B.assemble_linear(lambda v: B.integrate(f * v, dΩ), B.TestFESpace(U), mode=mode)
+           for mode in ("fp64", "fp32")}
+    mat["rhs"] = rel_err(rhs["fp64"], b_ref)
+    mat["rhs_fp32"] = rel_err(rhs["fp32"], b_ref)
     X = B.dof_coordinates(U)
     inner = np.flatnonzero(np.all((X > 1e-12) & (X < 1 - 1e-12), axis=1))
     cp, rv, nz, nd = A_ref
@@ -460,8 +462,8 @@ def c1_cpu_and_checks(B, plan, U, m, timed=True, reps=21):
     x_ref = solve(nz, b_ref)
     sol = {"workload": f"-Δu = sin πx sin πy, u = 0 on the boundary ({inner.size} interior dofs), "
                        f"direct solve"}
-    for mode in ("fp64", "fp32"):
-        x = solve(plan.assemble_host(mode=mode), b)
+    for mode in ("fp64", "fp32"):  # matrix and right-hand side both in `mode`
+        x = solve(plan.assemble_host(mode=mode), rhs[mode])
         sol[mode] = float(np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref))
     exact = np.sin(np.pi * X[inner, 0]) * np.sin(np.pi * X[inner, 1]) / (2 * np.pi ** 2)
     sol["oracle_vs_exact_solution"] = float(np.abs(x_ref - exact).max() / np.abs(exact).max())
@@ -491,22 +493,25 @@ def bench_c1(args, torch, B):
     vals = torch.empty(plan.nnz, dtype=torch.float64, device=dev)
     term = (L.BhTerm * 1)(L.BhTerm(L.FORM_SOURCE, L.COEF_QP, fq.data_ptr(), 1.0))
 
-    def step():
-        plan.assemble_device(vals.data_ptr(), None, "fp64")
+    def step(mode="fp64"):
+        plan.assemble_device(vals.data_ptr(), None, mode)
         b.zero_()
-        L.check(L.lib().bh_assemble_linear(U.handle, 1, term, 2, L.MODE["fp64"], b.data_ptr(),
+        L.check(L.lib().bh_assemble_linear(U.handle, 1, term, 2, L.MODE[mode], b.data_ptr(),
                                            L.DEVICE), backend.ctx, "bh_assemble_linear")
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
+    ms_mode = {}
+    for mode in ("fp32", "fp64"):
+        for _ in range(args.warmup):
+            step(mode)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            step(mode)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_mode[mode] = e0.elapsed_time(e1) / args.steps
+    ms = ms_mode["fp64"]
     # one step captured in a HIP graph and replayed (launch-bound size)
     with backend.capture() as g:
         plan.assemble_device(vals.data_ptr(), None, "fp64")
@@ -558,6 +563,7 @@ def bench_c1(args, torch, B):
                                    "±0.2h (seed 0), Gauss degree 2, ∫∇u⋅∇v + ∫ sin(πx)sin(πy) v",
                        "ndofs": int(U.get_ndofs()), "nnz": int(plan.nnz), "cells": int(m.ncells)},
             "ms_per_step_graph_replay": ms_graph,
+            "ms_per_step_fp32": ms_mode["fp32"],
             "note": "launch-latency bound (4,096 cells). The graph figure replays one capture "
                     "of the matrix and right-hand-side launches (the vector then accumulates "
                     "over replays, as assemble_linear! does); the eager step also zeroes it."}

@@ -341,6 +341,17 @@ class BilinearPlan:
                 "bh_pattern_values_device")
         return p.value
 
+    def onepass(self):
+        """The one-pass structured Q2 kernel's analysis of this pattern
+        (bh_pattern_onepass): {"ok", "nx", "ny", "nz", "templates",
+        "layers_per_group", "why"}."""
+        info = np.zeros(6, dtype=np.int32)
+        L.check(L.lib().bh_pattern_onepass(self.handle, info.ctypes.data), self.ctx,
+                "bh_pattern_onepass")
+        why = "" if info[0] else L.lib().bh_last_error(self.ctx).decode()
+        return {"ok": bool(info[0]), "nx": int(info[1]), "ny": int(info[2]), "nz": int(info[3]),
+                "templates": int(info[4]), "layers_per_group": int(info[5]), "why": why}
+
     def to_csc(self, nzval: np.ndarray) -> sp.csc_matrix:
         colptr, rowval = self.csc_structure()
         A = sp.csc_matrix((nzval, rowval, colptr), shape=(self.nrows, self.ncols))

@@ -118,10 +118,24 @@ struct FusedDev {
   int32_t* d_spill = nullptr;
 };
 
+// The one-pass structured Q2 kernel's plan (bh_sf.hip): lattice, row
+// templates, tiling; built on the first FP64 assembly of an eligible pattern.
+struct SfPlan {
+  bool tried = false, ok = false;
+  int nx = 0, ny = 0, nz = 0, zlen = 0, ntpl = 0;
+  double B[3][3] = {}, Dd[3][3] = {};
+  uint64_t* d_meta = nullptr;  // tile-major row records (SfArgs::meta)
+  uint8_t* d_tpl = nullptr;
+  std::string why;  // why the pattern is not eligible (empty when ok)
+};
+
+static int ensure_sf(bh_pattern p, bh_ctx c, bool upload = true);
+
 struct bh_pattern_s {
   bh_space U = nullptr, V = nullptr;
   Tuning tune[3];  // BH_FP64, BH_FP32, BH_BF16X3
   FusedDev fused;
+  SfPlan sf;
   int dev = -1;
   HostPattern h;
   int64_t* d_rowptr = nullptr;
@@ -914,8 +928,9 @@ int bh_pattern_tuning(bh_pattern p, int mode, int32_t* shape, float* ms) {
 int bh_pattern_set_tuning(bh_pattern p, int mode, const int32_t* shape) {
   if (!p || !shape || (mode != BH_FP64 && mode != BH_FP32 && mode != BH_BF16X3)) return BH_ERR_ARG;
   bh_ctx c = p->V->mesh->ctx;
-  if (shape[0] != 0 && shape[0] != 8 && shape[0] != 12 && shape[0] != -1)
-    return fail(c, BH_ERR_ARG, "element-kernel shape: 0, 8 or 12 waves, or -1 (fused)");
+  if (shape[0] != 0 && shape[0] != 8 && shape[0] != 12 && shape[0] != -1 && shape[0] != -2)
+    return fail(c, BH_ERR_ARG,
+                "element-kernel shape: 0, 8 or 12 waves, -1 (fused FP32), -2 (one-pass FP64)");
   const bool dflt = shape[1] == 0 && shape[2] == 0;
   bool ok = dflt;
   if (!dflt && p->h.pad > 0) {
@@ -930,6 +945,31 @@ int bh_pattern_set_tuning(bh_pattern p, int mode, const int32_t* shape) {
   tu.elem_waves = shape[0];
   tu.gather_lanes = shape[1];
   tu.gather_rows = shape[2];
+  return BH_OK;
+}
+
+int bh_pattern_onepass(bh_pattern p, int32_t* info) {
+  if (!p || !info) return BH_ERR_ARG;
+  bh_ctx c = p->V->mesh->ctx;
+  const bool dev = c && p->dev >= 0;
+  if (dev)
+    if (int r = set_device(c)) return r;
+  const bool was = p->sf.tried;
+  if (int r = ensure_sf(p, c, dev)) return r;
+  SfPlan& s = p->sf;
+  // a host-only analysis marks nothing usable; the structure is what it found
+  const bool ok = dev ? s.ok : (s.why.empty() && s.nx > 0);
+  if (!dev && !was) s.tried = true;
+  info[0] = ok ? 1 : 0;
+  info[1] = s.nx;
+  info[2] = s.ny;
+  info[3] = s.nz;
+  info[4] = s.ntpl;
+  info[5] = s.zlen;
+  if (!ok) {
+    if (c) c->err = s.why;
+    else g_last_error = s.why;
+  }
   return BH_OK;
 }
 
@@ -2008,6 +2048,8 @@ int bh_pattern_destroy(bh_pattern p) {
     (void)hipFree(f.d_prog);
     (void)hipFree(f.d_smask);
     (void)hipFree(f.d_spill);
+    (void)hipFree(p->sf.d_meta);
+    (void)hipFree(p->sf.d_tpl);
   }
   delete p;
   return BH_OK;
@@ -2087,6 +2129,83 @@ static int ensure_fused(bh_pattern p, bh_ctx c) {
 // ke_ext: a caller-provided element-row workspace of ke_cap bytes (the
 // allocation-free entry point); then nothing is allocated, the tables must
 // already be on the device, and there is neither autotuning nor timing.
+// Builds the one-pass structured Q2 plan of pattern p (once): the lattice and
+// row templates on the host (build_sf_structure), the 1D Lagrange tables at
+// the 3 Gauss points, and a z split that gives about one workgroup per
+// compute unit.  A pattern that is not a structured Q2 grid keeps the
+// element + row-gather path (sf.ok false, sf.why says why).
+static int ensure_sf(bh_pattern p, bh_ctx c, bool upload) {
+  SfPlan& s = p->sf;
+  if (s.tried) return BH_OK;
+  s.tried = true;
+  bh_space V = p->V;
+  bh_mesh m = V->mesh;
+  if (p->U != p->V || !V->continuous || V->order != 2 || V->nl != 27 || m->geo < 0 ||
+      (upload && m->buckets.size() != 1) || !sf_supported(m->geo, 2, 3)) {
+    s.why = "not a continuous Q2 space on a uniform Hexa8 / Hexa27 mesh";
+    return BH_OK;
+  }
+  SfStructure S;
+  if (!build_sf_structure(V->dofs, m->h.ncells, V->ndofs, p->h.rowptr, p->h.colval, S, s.why))
+    return BH_OK;
+  std::vector<double> gx, gw;
+  gauss_legendre(3, gx, gw);
+  for (int q = 0; q < 3; ++q) {
+    double v[3], dv[3];
+    lagrange_1d(2, gx[q], v, dv);
+    for (int n = 0; n < 3; ++n) {
+      s.B[n][q] = v[n];
+      s.Dd[n][q] = dv[n];
+    }
+  }
+  int ncu = 256;
+  if (upload)
+    BH_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+  const int64_t ntiles = (int64_t)((S.nx + kSfTileCells - 1) / kSfTileCells) *
+                         ((S.ny + kSfTileCells - 1) / kSfTileCells);
+  const int64_t nzs = std::min<int64_t>(S.nz, std::max<int64_t>(1, (std::max(ncu, 1) + ntiles - 1) / ntiles));
+  s.zlen = (int)((S.nz + nzs - 1) / nzs);
+  s.nx = S.nx;
+  s.ny = S.ny;
+  s.nz = S.nz;
+  s.ntpl = S.ntpl;
+  if (!upload) return BH_OK;
+  // the row table, tile-major: each tile's rows of a lattice level together,
+  // so a workgroup reads one level's records in one coalesced load
+  const int64_t ntx = (S.nx + kSfTileCells - 1) / kSfTileCells;
+  const int64_t nty = (S.ny + kSfTileCells - 1) / kSfTileCells;
+  const int64_t LX = 2 * (int64_t)S.nx + 1, LY = 2 * (int64_t)S.ny + 1, LZ = 2 * (int64_t)S.nz + 1;
+  const std::vector<int64_t>& rp = p->h.rowptr;
+  std::vector<uint64_t> meta((size_t)(ntx * nty * LZ * 81), 0);
+  parallel_for(ntx * nty, [&](int64_t b, int64_t e) {
+    for (int64_t t = b; t < e; ++t) {
+      const int64_t tx = t % ntx, ty = t / ntx;
+      const int64_t x0 = tx * kSfTileCells, x1 = std::min<int64_t>(x0 + kSfTileCells, S.nx);
+      const int64_t y0 = ty * kSfTileCells, y1 = std::min<int64_t>(y0 + kSfTileCells, S.ny);
+      const int64_t npx = 2 * (x1 - x0) + (x1 == S.nx), npy = 2 * (y1 - y0) + (y1 == S.ny);
+      for (int64_t Z = 0; Z < LZ; ++Z)
+        for (int64_t r = 0; r < npx * npy; ++r) {
+          const int64_t X = 2 * x0 + r % npx, Y = 2 * y0 + r / npx;
+          const int32_t d = S.lat2dof[(size_t)((Z * LY + Y) * LX + X)];
+          const uint64_t start = (uint64_t)rp[d], len = (uint64_t)(rp[d + 1] - rp[d]);
+          meta[(size_t)((t * LZ + Z) * 81 + r)] =
+              start | (len << 40) | ((uint64_t)S.tplid[d] << 48);
+        }
+    }
+  });
+  if (p->h.nnz >= (int64_t)1 << 40) {
+    s.why = "more than 2^40 entries";
+    return BH_OK;
+  }
+  BH_HIP(c, dalloc(&s.d_meta, meta.size()));
+  BH_HIP(c, hipMemcpy(s.d_meta, meta.data(), meta.size() * sizeof(uint64_t),
+                      hipMemcpyHostToDevice));
+  BH_HIP(c, dalloc(&s.d_tpl, S.tpl.size()));
+  BH_HIP(c, hipMemcpy(s.d_tpl, S.tpl.data(), S.tpl.size(), hipMemcpyHostToDevice));
+  s.ok = true;
+  return BH_OK;
+}
+
 static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms,
                                   int quad_degree, int mode, double* nzval, int where,
                                   int coef_where, void* ke_ext = nullptr, size_t ke_cap = 0) {
@@ -2132,6 +2251,81 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
   int64_t nc = m->h.ncells;
   const HostPattern& h = p->h;
   if (h.max_row > 512) return fail(c, BH_ERR_UNSUPPORTED, "CSR rows longer than 512 entries");
+  // 0. the one-pass structured Q2 kernel (bh_sf.hip): FP64, no element rows,
+  //    the default for an eligible pattern unless a caller fixed another
+  //    element-kernel shape
+  {
+    Tuning& tu0 = p->tune[0];
+    const int key = rule_key(m->buckets[0].geo, quad_degree);
+    const bool want = mode == BH_FP64 && m->buckets.size() == 1 &&
+                      sf_supported(m->geo, V->order, key) &&
+                      (tu0.elem_waves == 0 || tu0.elem_waves == -2);
+    if (want && !noalloc && !c->capturing)
+      if (int r = ensure_sf(p, c)) return r;
+    if (want && p->sf.ok) {
+      const SfPlan& sp = p->sf;
+      double* tab = nullptr;
+      if (int r = get_tables(c, m->geo, V->order, quad_degree, &tab, !noalloc)) return r;
+      SfArgs S;
+      S.stream = c->stream;
+      S.coords = m->d_coords;
+      S.c2n = m->buckets[0].d_c2n;
+      S.geo = m->geo;
+      S.tab = tab;
+      for (int n = 0; n < 3; ++n)
+        for (int q = 0; q < 3; ++q) {
+          S.B[n][q] = sp.B[n][q];
+          S.Dd[n][q] = sp.Dd[n][q];
+        }
+      S.nx = sp.nx;
+      S.ny = sp.ny;
+      S.nz = sp.nz;
+      S.zlen = sp.zlen;
+      S.meta = sp.d_meta;
+      S.tpl = sp.d_tpl;
+      if (stiff) {
+        S.eta = coef_ptr(*stiff, coef_where, p->d_coef, nc, c->stream);
+        S.eta_scale = stiff->scale;
+      }
+      if (mass) {
+        S.mass = true;
+        S.mcoef = coef_ptr(*mass, coef_where, p->d_coef + nc, nc, c->stream);
+        S.mass_scale = mass->scale;
+      }
+      S.vals = (where == BH_DEVICE && nzval) ? nzval : p->d_vals;
+      if (c->autotune && !tu0.done && !c->capturing && !noalloc) {
+        // nothing to choose: record the kernel's time for bh_pattern_tuning
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        BH_HIP(c, hipEventCreate(&e0));
+        BH_HIP(c, hipEventCreate(&e1));
+        float best = 1e30f;
+        for (int rep = 0; rep < 4; ++rep) {
+          BH_HIP(c, hipEventRecord(e0, c->stream));
+          BH_HIP(c, launch_sf(S));
+          BH_HIP(c, hipEventRecord(e1, c->stream));
+          BH_HIP(c, hipEventSynchronize(e1));
+          float t = 0.f;
+          BH_HIP(c, hipEventElapsedTime(&t, e0, e1));
+          if (rep > 0) best = std::min(best, t);
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        tu0 = Tuning();
+        tu0.done = true;
+        tu0.elem_waves = -2;
+        tu0.elem_ms = best;
+      }
+      if (!noalloc) time_begin(c, 3);
+      BH_HIP(c, launch_sf(S));
+      if (!noalloc) time_end(c);
+      if (where == BH_HOST) {
+        BH_HIP(c, hipMemcpyAsync(nzval, p->d_vals, (size_t)h.nnz * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+        BH_HIP(c, hipStreamSynchronize(c->stream));
+      }
+      return BH_OK;
+    }
+  }
   if (noalloc) {
     const size_t need = (size_t)(nc + 16) * V->nl * ke_stride(V->nl) *
                         (mode == BH_FP64 ? sizeof(double) : sizeof(float));

@@ -7,6 +7,8 @@
 // Bcube.jl source tree.
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -1140,6 +1142,151 @@ bool build_fused_schedule(const HostMesh& m, int nl, int64_t nrows,
       s.spill_mask[c] |= 1u << (pairs[q] - c * nl);
     }
   }
+  return true;
+}
+
+
+bool build_sf_structure(const std::vector<int32_t>& dofs, i64 ncells, i64 ndofs,
+                        const std::vector<i64>& rowptr, const std::vector<int32_t>& colval,
+                        SfStructure& S, std::string& why) {
+  S = SfStructure();
+  if (ncells < 1 || (i64)dofs.size() != ncells * 27 || (i64)rowptr.size() != ndofs + 1) {
+    why = "not a Q2 hexahedral space";
+    return false;
+  }
+  auto f = [&](i64 c, int i1, int i2, int i3) { return dofs[c * 27 + i1 + 3 * i2 + 9 * i3]; };
+  auto xadj = [&](i64 a, i64 b) {
+    for (int i3 = 0; i3 < 3; ++i3)
+      for (int i2 = 0; i2 < 3; ++i2)
+        if (f(a, 2, i2, i3) != f(b, 0, i2, i3)) return false;
+    return true;
+  };
+  auto yadj = [&](i64 a, i64 b) {
+    for (int i3 = 0; i3 < 3; ++i3)
+      for (int i1 = 0; i1 < 3; ++i1)
+        if (f(a, i1, 2, i3) != f(b, i1, 0, i3)) return false;
+    return true;
+  };
+  i64 nx = 1;
+  while (nx < ncells && xadj(nx - 1, nx)) ++nx;
+  i64 ny = 1;
+  while ((ny + 1) * nx <= ncells && yadj((ny - 1) * nx, ny * nx)) ++ny;
+  if (ncells % (nx * ny) != 0) {
+    why = "cells do not form an nx x ny x nz grid";
+    return false;
+  }
+  const i64 nz = ncells / (nx * ny);
+  const i64 LX = 2 * nx + 1, LY = 2 * ny + 1, LZ = 2 * nz + 1;
+  if (nx > 32767 || ny > 32767 || nz > 32767 || LX * LY * LZ != ndofs ||
+      LX * LY * LZ > (i64)INT32_MAX) {
+    why = "the dofs are not the grid's Q2 lattice";
+    return false;
+  }
+  S.lat2dof.assign((size_t)ndofs, -1);
+  std::vector<i64> dof2lat((size_t)ndofs, -1);
+  for (i64 cz = 0, c = 0; cz < nz; ++cz)
+    for (i64 cy = 0; cy < ny; ++cy)
+      for (i64 cx = 0; cx < nx; ++cx, ++c)
+        for (int i3 = 0; i3 < 3; ++i3)
+          for (int i2 = 0; i2 < 3; ++i2)
+            for (int i1 = 0; i1 < 3; ++i1) {
+              const i64 L = ((2 * cz + i3) * LY + 2 * cy + i2) * LX + 2 * cx + i1;
+              const int32_t d = f(c, i1, i2, i3);
+              if (d < 0 || d >= ndofs) {
+                why = "dof id out of range";
+                return false;
+              }
+              if (S.lat2dof[L] < 0) {
+                if (dof2lat[d] >= 0) {
+                  why = "a dof sits on two lattice points";
+                  return false;
+                }
+                S.lat2dof[L] = d;
+                dof2lat[d] = L;
+              } else if (S.lat2dof[L] != d) {
+                why = "neighbouring cells disagree on a shared dof";
+                return false;
+              }
+            }
+  // row templates: per row, the box entry of each CSR position; rows hashed in
+  // parallel, then one id per distinct template, then every row checked
+  // against its template's bytes (a hash collision disqualifies the space)
+  auto row_key = [&](i64 r, uint8_t* key, int* len) -> bool {
+    const i64 L = dof2lat[r];
+    const i64 X = L % LX, Y = (L / LX) % LY, Z = L / (LX * LY);
+    const int ex = (X & 1) ? 3 : 5, ey = (Y & 1) ? 3 : 5, ez = (Z & 1) ? 3 : 5;
+    const int hx = ex / 2, hy = ey / 2, hz = ez / 2;
+    auto span = [](i64 p, int h, i64 n) {
+      return (i64)(std::min(p + h, n - 1) - std::max<i64>(p - h, 0) + 1);
+    };
+    const i64 n = rowptr[r + 1] - rowptr[r];
+    if (n != span(X, hx, LX) * span(Y, hy, LY) * span(Z, hz, LZ) || n > 125) return false;
+    for (i64 k = 0; k < n; ++k) {
+      const i64 Lc = dof2lat[colval[rowptr[r] + k]];
+      const i64 dx = Lc % LX - X, dy = (Lc / LX) % LY - Y, dz = Lc / (LX * LY) - Z;
+      if (dx < -hx || dx > hx || dy < -hy || dy > hy || dz < -hz || dz > hz) return false;
+      key[k] = (uint8_t)(((dz + hz) * ey + dy + hy) * ex + dx + hx);
+    }
+    *len = (int)n;
+    return true;
+  };
+  auto hash = [](const uint8_t* key, int len) {
+    uint64_t h = 1469598103934665603ull ^ (uint64_t)len;
+    for (int k = 0; k < len; ++k) h = (h ^ key[k]) * 1099511628211ull;
+    return h;
+  };
+  std::vector<uint64_t> rh((size_t)ndofs);
+  std::atomic<bool> fail_rows{false};
+  parallel_for(ndofs, [&](i64 b, i64 e) {
+    uint8_t key[128];
+    for (i64 r = b; r < e; ++r) {
+      int len = 0;
+      if (!row_key(r, key, &len)) {
+        fail_rows = true;
+        return;
+      }
+      rh[r] = hash(key, len);
+    }
+  });
+  if (fail_rows) {
+    why = "a CSR row is not the box of its lattice point";
+    return false;
+  }
+  std::unordered_map<uint64_t, int> ids;
+  S.tplid.resize((size_t)ndofs);
+  for (i64 r = 0; r < ndofs; ++r) {
+    auto it = ids.find(rh[r]);
+    if (it == ids.end()) {
+      if (S.ntpl >= 65535) {
+        why = "too many row shapes";
+        return false;
+      }
+      uint8_t key[128] = {0};
+      int len = 0;
+      row_key(r, key, &len);
+      S.tpl.insert(S.tpl.end(), key, key + 128);
+      it = ids.emplace(rh[r], S.ntpl++).first;
+    }
+    S.tplid[r] = (uint16_t)it->second;
+  }
+  parallel_for(ndofs, [&](i64 b, i64 e) {
+    uint8_t key[128];
+    for (i64 r = b; r < e; ++r) {
+      int len = 0;
+      row_key(r, key, &len);
+      if (std::memcmp(key, &S.tpl[(size_t)S.tplid[r] * 128], (size_t)len) != 0) {
+        fail_rows = true;
+        return;
+      }
+    }
+  });
+  if (fail_rows) {
+    why = "row template hash collision";
+    return false;
+  }
+  S.nx = (int)nx;
+  S.ny = (int)ny;
+  S.nz = (int)nz;
   return true;
 }
 

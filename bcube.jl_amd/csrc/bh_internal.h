@@ -192,4 +192,25 @@ bool build_fused_schedule(const HostMesh& m, int nl, int64_t nrows,
 // Simple host thread pool loop.
 void parallel_for(i64 n, const std::function<void(i64, i64)>& body);
 
+// Lattice structure of a continuous Q2 space on a structured hexahedral grid,
+// for the one-pass kernel (bh_sf.hip).  The grid size is inferred from the
+// dof map alone: cells c and c+1 are x-neighbours while their shared face
+// dofs agree (nx), rows of nx cells are y-neighbours likewise (ny), and every
+// cell (cx, cy, cz) = c must then put local dof i1 + 3 i2 + 9 i3 on lattice
+// point (2cx + i1, 2cy + i2, 2cz + i3) with one dof per point.  Each CSR row
+// must hold exactly the points of its box (5 or 3 positions per axis around
+// the row's point, clipped at the boundary); its template lists, per CSR
+// position, the box entry ((dz + hz) ey + dy + hy) ex + dx + hx.  Returns
+// false with `why` when the space is not of this form.
+struct SfStructure {
+  int nx = 0, ny = 0, nz = 0;
+  std::vector<int32_t> lat2dof;  // (Z (2ny+1) + Y)(2nx+1) + X
+  std::vector<uint16_t> tplid;   // per dof
+  std::vector<uint8_t> tpl;      // ntpl * 128
+  int ntpl = 0;
+};
+bool build_sf_structure(const std::vector<int32_t>& dofs, i64 ncells, i64 ndofs,
+                        const std::vector<i64>& rowptr, const std::vector<int32_t>& colval,
+                        SfStructure& s, std::string& why);
+
 }  // namespace bh

@@ -209,4 +209,35 @@ constexpr int kFusedProgCap = 16384;  // bytes of one step program (the kernel's
 bool gp_fused_supported(int geo_celltype, int order, int nq1);
 hipError_t launch_gp_fused(int geo_celltype, int order, int nq1, const FusedArgs& a);
 
+// Continuous Q2 stiffness + mass on a structured nx × ny × nz hexahedral grid
+// (Hexa8 or Hexa27 cells, 3 Gauss points per axis) in one pass (bh_sf.hip):
+// FP64 element matrices by sum factorisation, summed in LDS per tile of at
+// most tw × tw cell columns and zlen layers, every CSR row written once.
+// meta is the tile-major row table: for tile tx + ntx ty, lattice level Z and
+// tile row r = y npx + x (the tile's owned positions), entry
+// ((tile (2nz+1) + Z) 81 + r) = CSR start | length << 40 | template << 48;
+// tpl holds 128-byte templates, entry k = the box entry of a row's k-th CSR
+// position (box: 5 or 3 positions per axis, x fastest).
+constexpr int kSfTileCells = 4;
+struct SfArgs {
+  hipStream_t stream = nullptr;
+  const double* coords = nullptr;
+  const int32_t* c2n = nullptr;
+  int geo = BH_HEXA8;
+  const double* tab = nullptr;  // packed tables of (geo, Q2, 3 points per axis)
+  double B[3][3] = {}, Dd[3][3] = {};  // 1D Lagrange values, derivatives [node][point]
+  int nx = 0, ny = 0, nz = 0;
+  int tw = kSfTileCells, zlen = 0;
+  const uint64_t* meta = nullptr;
+  const uint8_t* tpl = nullptr;
+  const double* eta = nullptr;
+  double eta_scale = 0.0;
+  const double* mcoef = nullptr;
+  double mass_scale = 0.0;
+  bool mass = false;
+  double* vals = nullptr;
+};
+bool sf_supported(int geo_celltype, int order, int nq1);
+hipError_t launch_sf(const SfArgs& a);
+
 }  // namespace bh

@@ -39,6 +39,7 @@ GATHER = {"fp64": "k_gather_padded<double,unsigned char,27,128,8,2,32>",
           "fp32": "k_gather_padded<float,unsigned char,27,128,8,2,32>",
           "bf16x3": "k_gather_padded<float,unsigned char,27,128,8,2,32>"}
 FUSED_KERNEL = "k_element_gp_fused<3,27,27,8,true,32,8> (element matrices + LDS row sums)"
+ONEPASS_KERNEL = "k_sf_q2<8,true>"
 
 
 def workload(n):
@@ -52,14 +53,26 @@ def workload(n):
     return ncells, ndofs, nnz, b_alg, f_alg
 
 
-def kernel_rooflines(mode, launches, kinds, steps, f_alg, nnz):
+def kernel_rooflines(mode, launches, kinds, steps, f_alg, nnz, b_alg=None):
     """Per-kernel rooflines.  Each step is the element kernel (algorithmic
     flops F_alg, bound by the matrix/vector FP peak of the mode) and the row
     gather (algorithmic bytes: the 8*nnz CSR values it writes, bound by HBM);
-    launch times are summed per kernel kind.  Returns (dominant, [element, gather])."""
+    launch times are summed per kernel kind.  In FP64 on a structured Q2 grid
+    the step is the one-pass kernel alone (kind 3): bound by HBM, algorithmic
+    bytes B_alg (the values written and the mesh and coefficients read).
+    Returns (dominant, [kernels])."""
     t = np.asarray(launches, dtype=np.float64)
     k = np.asarray(kinds)
     fused = bool((k == 3).any())  # the fused element + row-assembly kernel (kind 3)
+    if fused and mode == "fp64" and b_alg is not None:
+        ms = max(float(t[k == 3].sum() / steps), 1e-9)
+        ach = b_alg / (ms * 1e-3) / 1e9
+        one = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+               "frac": ach / PEAK_HBM_GBS, "kernel": ONEPASS_KERNEL, "avg_launch_ms": ms,
+               "launches_per_step": int((k == 3).sum() // steps), "alg_bytes_per_launch": b_alg,
+               "what": "one pass: FP64 element matrices by sum factorisation, LDS row sums, "
+                       "every CSR row written once (no element-row workspace, no row gather)"}
+        return one, [one]
     el_mask = (k == 0) | (k == 3)
     el_ms = max(float(t[el_mask].sum() / steps), 1e-9)
     ga_ms = max(float(t[k == 1].sum() / steps), 1e-9)
@@ -1745,12 +1758,38 @@ def bench_c2(args, torch, B):
         err = None
         if ref is not None and mode != "fp64":
             err = float((out - ref).abs().max() / ref.abs().max())
-        dom, kernels = kernel_rooflines(mode, launches, kinds, args.steps, f_alg, nnz)
+        dom, kernels = kernel_rooflines(mode, launches, kinds, args.steps, f_alg, nnz, b_alg)
         results[mode] = {"ms_per_step": ms_step, "gnnz_per_s": nnz / (ms_step * 1e-3) / 1e9,
                          "elements_per_s": ncells / (ms_step * 1e-3),
                          "launches_per_step": len(launches) / args.steps,
                          "rel_err_vs_fp64_gpu": err, "roofline": dom, "kernels": kernels,
                          "tuning": plan.tuning(mode)}
+        if mode == "fp64":
+            results[mode]["onepass"] = plan.onepass()
+            if results[mode]["onepass"]["ok"]:
+                # the FP64 element + row-gather kernels on the same pattern, for
+                # comparison (the path every non-grid mesh takes)
+                out2 = torch.empty(nnz, dtype=torch.float64, device=dev)
+                plan.set_tuning(8, mode="fp64")
+                for _ in range(args.warmup):
+                    plan.assemble_device(out2.data_ptr(), coef, mode)
+                torch.cuda.synchronize()
+                backend.set_timing(True)
+                e0.record(stream)
+                for _ in range(args.steps):
+                    plan.assemble_device(out2.data_ptr(), coef, mode)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms2 = e0.elapsed_time(e1) / args.steps
+                l2, k2 = backend.timings_ex(cap=100000)
+                backend.set_timing(False)
+                d2, kern2 = kernel_rooflines(mode, l2, k2, args.steps, f_alg, nnz)
+                results["fp64_element_gather"] = {
+                    "ms_per_step": ms2, "gnnz_per_s": nnz / (ms2 * 1e-3) / 1e9,
+                    "rel_err_vs_fp64_gpu": float((out2 - out).abs().max() / out.abs().max()),
+                    "roofline": d2, "kernels": kern2}
+                plan.set_tuning(0, mode="fp64")
+                del out2
         if mode != "fp64":
             del out
     head = "fp32" if "fp32" in results else modes[0]

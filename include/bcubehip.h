@@ -160,12 +160,27 @@ int bh_pattern_build_device(bh_space U, bh_space V, bh_pattern* out);
  * per block target (8 or 12), or -1 for the fused element + row-assembly
  * kernel (FP32 on uniform meshes of the elements that have one: one workgroup
  * per pencil of cells, the CSR rows complete within two steps of a pencil
- * summed in LDS and written once, the others by the row gather), shape[1]
- * gather lanes per row, shape[2] gather rows per lane group (0: the default;
- * the fused kernel runs only when the autotuner measured it faster or a caller
- * set -1); ms[0], ms[1] their measured times.  Every shape gives bitwise identical values.  Returns 1 once tuned,
- * 0 before, negative on error.  No counterpart in the reference. */
+ * summed in LDS and written once, the others by the row gather), or -2 for the
+ * one-pass kernel (bh_pattern_onepass, BH_FP64), shape[1] gather lanes per row,
+ * shape[2] gather rows per lane group (0: the default; the fused kernel runs
+ * only when the autotuner measured it faster or a caller set -1); ms[0], ms[1]
+ * their measured times.  Shapes 0, 8, 12 and -1 give bitwise identical values;
+ * -2 sums each entry in another fixed order.  Returns 1 once tuned, 0 before,
+ * negative on error.  No counterpart in the reference. */
 int bh_pattern_tuning(bh_pattern p, int mode, int32_t* shape, float* ms);
+/* The one-pass kernel for continuous Q2 spaces on structured hexahedral grids
+ * (Hexa8 or Hexa27 cells, 3 Gauss points per axis, BH_FP64): FP64 element
+ * matrices by sum factorisation, each CSR row summed in LDS by the workgroup
+ * that owns its lattice point and written once, with no element-row workspace.
+ * No counterpart in the reference, whose assemble_bilinear
+ * (assembler.jl:172-186) loops over cells into a COO.  bh_assemble_bilinear
+ * takes it by default for eligible patterns in BH_FP64 (bh_pattern_set_tuning
+ * shape[0] 8 or 12 selects the element + row-gather kernels).  The grid is
+ * inferred from the dof map; this call analyses p on first use (host-only
+ * patterns without device buffers) and reports: info[0] 1 eligible, 0 not
+ * (bh_last_error gives the reason), info[1..3] nx, ny, nz, info[4] distinct
+ * row templates, info[5] layers per workgroup. */
+int bh_pattern_onepass(bh_pattern p, int32_t* info);
 /* Fix the launch shapes for `mode` (same layout as bh_pattern_tuning's shape;
  * zeros: the defaults), e.g. from an earlier tuning; the autotuner then leaves
  * them.  A gather shape must be one of the pattern's candidates (BH_ERR_ARG

@@ -104,20 +104,47 @@ class DGTransport:
                                         x2_ptr, float(a2), out2_ptr), self._ctx,
                 "bh_dg_minv_axpy")
 
-    def rk4_step(self, u, dt: float, work=None):
-        """One classical RK4 step of M du/dt = l(u) in place on a GPU tensor u
-        (nfields, ndofs): four residual assemblies and four M⁻¹ updates.
-        `work` = (r, acc, stage) tensors shaped like u (allocated if None) --
-        pass them to keep the call allocation-free (and graph-capturable)."""
+    def stage(self, nfields: int, u_ptr: int, x1_ptr: int, a1: float, out1_ptr: int,
+              x2_ptr: int | None = None, a2: float = 0.0, out2_ptr: int | None = None,
+              work_ptr: int | None = None):
+        """out1 = x1 + a1 M⁻¹ l(u) (and out2 = x2 + a2 M⁻¹ l(u)) in one pass
+        (bh_dg_transport_stage); u must not alias an output."""
+        L.check(L.lib().bh_dg_transport_stage(self._h, nfields, u_ptr, x1_ptr, float(a1),
+                                              out1_ptr, x2_ptr, float(a2), out2_ptr, work_ptr),
+                self._ctx, "bh_dg_transport_stage")
+
+    def _rk4_args(self, u, work):
         t = _torch()
         if t is None or not isinstance(u, t.Tensor) or not u.is_cuda:
             raise ValueError("rk4_step works on a GPU tensor")
         if work is None:
             work = tuple(t.empty_like(u) for _ in range(3))
-        r, acc, us = work
         nf = 1 if u.dim() == 1 else u.shape[0]
-        p = (u.data_ptr(), r.data_ptr(), acc.data_ptr(), us.data_ptr())
-        U, R, A, S = p
+        return work, nf
+
+    def rk4_step(self, u, dt: float, work=None):
+        """One classical RK4 step of M du/dt = l(u) in place on a GPU tensor u
+        (nfields, ndofs): four stages, each a residual assembly and its M⁻¹
+        updates in one pass (bh_dg_transport_stage), the stage inputs in two
+        alternating buffers.  `work` = three tensors shaped like u (allocated
+        if None) -- pass them to keep the call allocation-free (and
+        graph-capturable).  Bitwise equal to rk4_step_two_pass."""
+        work, nf = self._rk4_args(u, work)
+        r, acc, us = work
+        U, S2, A, S1 = u.data_ptr(), r.data_ptr(), acc.data_ptr(), us.data_ptr()
+        # the work array of the two-launch form (FP64 Q3) is each stage's
+        # second output, or the free stage buffer in the last stage
+        self.stage(nf, U, U, dt / 6, A, U, dt / 2, S1, S1)
+        self.stage(nf, S1, A, dt / 3, A, U, dt / 2, S2, S2)
+        self.stage(nf, S2, A, dt / 3, A, U, dt, S1, S1)
+        self.stage(nf, S1, A, dt / 6, U, None, 0.0, None, S2)
+        return work
+
+    def rk4_step_two_pass(self, u, dt: float, work=None):
+        """The same RK4 step as four residual assemblies and four M⁻¹ updates."""
+        work, nf = self._rk4_args(u, work)
+        r, acc, us = work
+        U, R, A, S = u.data_ptr(), r.data_ptr(), acc.data_ptr(), us.data_ptr()
         self.residual_device(nf, U, R)
         self.minv_axpy(nf, R, U, dt / 6, A, U, dt / 2, S)
         self.residual_device(nf, S, R)

@@ -2985,6 +2985,47 @@ int bh_dg_minv_axpy(bh_dg g, int32_t nfields, const void* r, const void* x1, dou
   return BH_OK;
 }
 
+int bh_dg_transport_stage(bh_dg g, int32_t nfields, const void* u, const void* x1, double a1,
+                          void* out1, const void* x2, double a2, void* out2, void* work) {
+  if (!g) return BH_ERR_ARG;
+  bh_ctx c = g->V->mesh->ctx;
+  if (nfields <= 0 || !u || !x1 || !out1 || (out2 && !x2))
+    return fail(c, BH_ERR_ARG, "bh_dg_transport_stage: bad arguments");
+  if (u == out1 || (out2 && u == out2))
+    return fail(c, BH_ERR_ARG,
+                "bh_dg_transport_stage: u is read by neighbouring cells during the stage, so "
+                "no output may alias it");
+  const bool fused = dg_stage_fused(g->mode == BH_FP32, g->n);
+  if (!fused && !work)
+    return fail(c, BH_ERR_ARG,
+                "bh_dg_transport_stage: this mode and order need a work array of nfields*ndofs "
+                "values");
+  if (int e = set_device(c)) return e;
+  DGArgs a;
+  a.stream = c->stream;
+  a.n = g->n;
+  a.fp32 = g->mode == BH_FP32;
+  a.ndofs = g->ndofs;
+  a.ncells = g->ncells;
+  a.nfields = nfields;
+  a.vfac = g->d_vfac;
+  a.ffac = g->d_ffac;
+  a.sfac = g->d_sfac;
+  a.cs = g->d_cs;
+  a.u = u;
+  a.r = fused ? nullptr : work;
+  a.x1 = x1;
+  a.a1 = a1;
+  a.out1 = out1;
+  a.x2 = x2;
+  a.a2 = a2;
+  a.out2 = out2;
+  time_begin(c, 0);
+  BH_HIP(c, launch_dg_stage(a, g->tab));
+  time_end(c);
+  return BH_OK;
+}
+
 int bh_dg_destroy(bh_dg g) {
   if (!g) return BH_ERR_ARG;
   dg_free(g);

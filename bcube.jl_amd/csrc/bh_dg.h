@@ -61,5 +61,10 @@ struct DGArgs {
 hipError_t launch_dg_geometry(const DGGeomArgs& a, const DGTables& t);
 hipError_t launch_dg_residual(const DGArgs& a, const DGTables& t);
 hipError_t launch_dg_minv_axpy(const DGArgs& a, const DGTables& t);
+// One explicit stage, out1 = x1 + a1 M⁻¹ l(u) and out2 = x2 + a2 M⁻¹ l(u):
+// one launch where the residual has a register form (dg_stage_fused), else the
+// residual into a.r then the update.
+hipError_t launch_dg_stage(const DGArgs& a, const DGTables& t);
+bool dg_stage_fused(bool fp32, int n);
 
 }  // namespace bh

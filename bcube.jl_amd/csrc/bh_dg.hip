@@ -562,7 +562,7 @@ __device__ __forceinline__ void load_side(const T* __restrict__ blk, int s, T (&
 // ---------------------------------------------------------------------------
 template <typename T>
 struct Ops2 {
-  T phi[16], dg[16], em[4], ep[4];
+  T phi[16], dg[16], em[4], ep[4], pinv[16];
 };
 
 template <int N>
@@ -610,11 +610,25 @@ __device__ __forceinline__ void wave_blocks(T* __restrict__ g, T* W, int64_t g0,
   }
 }
 
-template <typename T, int N>
+// Explicit-stage epilogue of the register form (EPI): the residual r of the
+// lane's (cell, field), still in registers, becomes k = M_K⁻¹ r by the same
+// five axis passes and the same operations as k_dg_minv_axpy (so the results
+// are bitwise those of the residual + update pair), then out1 = x1 + a1 k and
+// out2 = x2 + a2 k go through the wave's LDS blocks like the input.
+struct StageOut {
+  const void* x1 = nullptr;
+  double a1 = 0.0;
+  void* out1 = nullptr;
+  const void* x2 = nullptr;
+  double a2 = 0.0;
+  void* out2 = nullptr;
+};
+
+template <typename T, int N, bool EPI>
 __global__ __launch_bounds__(kBlock) void k_dg_residual_reg(
     const T* __restrict__ u, T* __restrict__ r, int64_t ndofs, int nfields, int64_t ncells,
     const T* __restrict__ vfac, const T* __restrict__ sfac, const int4* __restrict__ cs,
-    Ops2<T> op) {
+    Ops2<T> op, StageOut so) {
   constexpr int N2 = N * N, N3 = N * N * N, LST = SlotLds<N3>::stride;
   __shared__ T lds[kBlock / 64][64 * LST];
   const int lane = threadIdx.x & 63;
@@ -764,10 +778,114 @@ __global__ __launch_bounds__(kBlock) void k_dg_residual_reg(
       }
     }
   }
-  if (!valid) return;
-  T* rc = r + (int64_t)f * ndofs + cell * N3;
+  if constexpr (!EPI) {
+    if (!valid) return;
+    T* rc = r + (int64_t)f * ndofs + cell * N3;
 #pragma unroll
-  for (int i = 0; i < N3; ++i) rc[i] = t[i];
+    for (int i = 0; i < N3; ++i) rc[i] = t[i];
+  } else {
+    // k = M_K⁻¹ r = Φ₃⁻¹ diag(1/W_q) Φ₃⁻ᵀ r, axis by axis as k_dg_minv_axpy
+    const T* Wq = vfac + cell * 4 * N3 + 3 * N3;
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {  // axis 0: Φ⁻ᵀ
+        T v[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = t[i + N * j + N2 * k];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+          T s = 0;
+#pragma unroll
+          for (int i = 0; i < N; ++i) s = madd(op.pinv[i * 4 + q], v[i], s);
+          t[q + N * j + N2 * k] = s;
+        }
+      }
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {  // axis 1: Φ⁻ᵀ
+        T v[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = t[j + N * i + N2 * k];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+          T s = 0;
+#pragma unroll
+          for (int i = 0; i < N; ++i) s = madd(op.pinv[i * 4 + q], v[i], s);
+          t[j + N * q + N2 * k] = s;
+        }
+      }
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {  // axis 2: Φ⁻ᵀ, 1/W_q, then Φ⁻¹
+        T v[N], y[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = t[j + N * k + N2 * i];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+          T s = 0;
+#pragma unroll
+          for (int i = 0; i < N; ++i) s = madd(op.pinv[i * 4 + q], v[i], s);
+          y[q] = s / Wq[j + N * k + N2 * q];
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          T s = 0;
+#pragma unroll
+          for (int q = 0; q < N; ++q) s = madd(op.pinv[i * 4 + q], y[q], s);
+          t[j + N * k + N2 * i] = s;
+        }
+      }
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {  // axis 1: Φ⁻¹
+        T v[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) v[q] = t[j + N * q + N2 * k];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          T s = 0;
+#pragma unroll
+          for (int q = 0; q < N; ++q) s = madd(op.pinv[i * 4 + q], v[q], s);
+          t[j + N * i + N2 * k] = s;
+        }
+      }
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {  // axis 0: Φ⁻¹
+        T v[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) v[q] = t[q + N * j + N2 * k];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          T s = 0;
+#pragma unroll
+          for (int q = 0; q < N; ++q) s = madd(op.pinv[i * 4 + q], v[q], s);
+          t[i + N * j + N2 * k] = s;
+        }
+      }
+    // out = x + a k through the wave's LDS blocks (the staged input, which
+    // the neighbour reads above used, is no longer needed)
+    for (int o = 0; o < 2; ++o) {
+      const T* x = (const T*)(o == 0 ? so.x1 : so.x2);
+      T* out = (T*)(o == 0 ? so.out1 : so.out2);
+      if (!out) break;
+      const T a = (T)(o == 0 ? so.a1 : so.a2);
+      wave_sync();
+      wave_blocks<T, N, false>(const_cast<T*>(x), W, g0, total, nfields, ndofs, lane);
+      wave_sync();
+      if (valid) {
+#pragma unroll
+        for (int i = 0; i < N3; ++i) W[lane * LST + i] = W[lane * LST + i] + a * t[i];
+      }
+      wave_sync();
+      wave_blocks<T, N, true>(out, W, g0, total, nfields, ndofs, lane);
+    }
+  }
 }
 
 // out1 = x1 + a1 M⁻¹ r, out2 = x2 + a2 M⁻¹ r for the block-diagonal DG mass
@@ -896,6 +1014,7 @@ Ops2<T> ops2_of(const DGTables& t) {
     o.em[i] = (T)t.em[i];
     o.ep[i] = (T)t.ep[i];
   }
+  for (int i = 0; i < 16; ++i) o.pinv[i] = (T)t.pinv[i];
   return o;
 }
 
@@ -939,9 +1058,10 @@ hipError_t residual_impl(const DGArgs& a, const DGTables& t) {
   const int64_t nslots = a.ncells * a.nfields;
   if (nslots == 0) return hipSuccess;
   if constexpr (kResidualInRegisters<T, N>) {
-    hipLaunchKernelGGL((k_dg_residual_reg<T, N>), dim3(blocks_for(nslots, kBlock)), dim3(kBlock),
-                       0, a.stream, (const T*)a.u, (T*)a.r, a.ndofs, a.nfields, a.ncells,
-                       (const T*)a.vfac, (const T*)a.sfac, (const int4*)a.cs, ops2_of<T>(t));
+    hipLaunchKernelGGL((k_dg_residual_reg<T, N, false>), dim3(blocks_for(nslots, kBlock)),
+                       dim3(kBlock), 0, a.stream, (const T*)a.u, (T*)a.r, a.ndofs, a.nfields,
+                       a.ncells, (const T*)a.vfac, (const T*)a.sfac, (const int4*)a.cs,
+                       ops2_of<T>(t), StageOut());
   } else {
     hipLaunchKernelGGL((k_dg_residual<T, N>), dim3(blocks_for(nslots, kSlots)), dim3(kBlock), 0,
                        a.stream, (const T*)a.u, (T*)a.r, a.ndofs, a.nfields, a.ncells,
@@ -971,6 +1091,43 @@ hipError_t residual_t(const DGArgs& a, const DGTables& t) {
   return hipErrorInvalidValue;
 }
 
+// One explicit stage: out1 = x1 + a1 M⁻¹ l(u), out2 = x2 + a2 M⁻¹ l(u).  The
+// register form does it in one launch; the other form runs the residual into
+// a.r and then the update.
+template <typename T, int N>
+hipError_t stage_impl(const DGArgs& a, const DGTables& t) {
+  const int64_t nslots = a.ncells * a.nfields;
+  if (nslots == 0) return hipSuccess;
+  if constexpr (kResidualInRegisters<T, N>) {
+    StageOut so;
+    so.x1 = a.x1;
+    so.a1 = a.a1;
+    so.out1 = a.out1;
+    so.x2 = a.x2;
+    so.a2 = a.a2;
+    so.out2 = a.out2;
+    hipLaunchKernelGGL((k_dg_residual_reg<T, N, true>), dim3(blocks_for(nslots, kBlock)),
+                       dim3(kBlock), 0, a.stream, (const T*)a.u, (T*)nullptr, a.ndofs, a.nfields,
+                       a.ncells, (const T*)a.vfac, (const T*)a.sfac, (const int4*)a.cs,
+                       ops2_of<T>(t), so);
+    return hipGetLastError();
+  } else {
+    if (!a.r) return hipErrorInvalidValue;
+    if (hipError_t e = residual_impl<T, N>(a, t)) return e;
+    return minv_impl<T, N>(a, t);
+  }
+}
+
+template <typename T>
+hipError_t stage_t(const DGArgs& a, const DGTables& t) {
+  switch (a.n) {
+    case 2: return stage_impl<T, 2>(a, t);
+    case 3: return stage_impl<T, 3>(a, t);
+    case 4: return stage_impl<T, 4>(a, t);
+  }
+  return hipErrorInvalidValue;
+}
+
 template <typename T>
 hipError_t minv_t(const DGArgs& a, const DGTables& t) {
   switch (a.n) {
@@ -991,6 +1148,12 @@ hipError_t launch_dg_residual(const DGArgs& a, const DGTables& t) {
 }
 hipError_t launch_dg_minv_axpy(const DGArgs& a, const DGTables& t) {
   return a.fp32 ? minv_t<float>(a, t) : minv_t<double>(a, t);
+}
+hipError_t launch_dg_stage(const DGArgs& a, const DGTables& t) {
+  return a.fp32 ? stage_t<float>(a, t) : stage_t<double>(a, t);
+}
+bool dg_stage_fused(bool fp32, int n) {
+  return fp32 || n <= 3;
 }
 
 }  // namespace bh

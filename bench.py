@@ -899,8 +899,29 @@ def bench_c4(args, torch, B):
         e1.record(stream)
         torch.cuda.synchronize()
         graph_ms = e0.elapsed_time(e1)
+        # the stage kernel's own time (4 launches per step)
+        backend.set_timing(True)
+        plan.rk4_step(u, dt, work)
+        torch.cuda.synchronize()
+        st, _ = backend.timings_ex()
+        backend.set_timing(False)
+        # the same loop as four residuals + four M⁻¹ updates per step, for comparison
+        with backend.capture() as g2:
+            plan.rk4_step_two_pass(u, dt, work)
+        g2.launch()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(50):
+            g2.launch()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        two_ms = e0.elapsed_time(e1)
         rk[mode] = {"steps": 50, "eager_ms": eager_ms, "graph_ms": graph_ms,
                     "residuals_per_s_graph": 200 / (graph_ms * 1e-3),
+                    "stage_kernel_ms": float(np.mean(st)) if len(st) else None,
+                    "stage_kernel": "k_dg_residual_reg<float,4,true> (residual, M⁻¹ and both "
+                                    "updates in one pass)",
+                    "two_pass_graph_ms": two_ms,
                     "finite": bool(torch.isfinite(u).all().item())}
     r = res["fp32"]
     line = {

@@ -345,6 +345,19 @@ int bh_dg_transport_residual(bh_dg dg, int32_t nfields, const void* u, void* r, 
  * the update step of explicit time loops that re-assemble l(v) every stage. */
 int bh_dg_minv_axpy(bh_dg dg, int32_t nfields, const void* r, const void* x1, double a1,
                     void* out1, const void* x2, double a2, void* out2);
+/* One stage of an explicit time loop in one pass: out1 = x1 + a1 M⁻¹ l(u) and,
+ * if out2 is not NULL, out2 = x2 + a2 M⁻¹ l(u), with l the transport residual
+ * of bh_dg_transport_residual and M the mass matrix of bh_dg_minv_axpy; the
+ * values are bitwise those of that residual + update pair.  Where the residual
+ * runs in registers (BH_FP32, or N <= 3) the M⁻¹ and both updates happen in
+ * the residual kernel and l(u) never goes to memory; otherwise `work`
+ * (nfields*ndofs values, may be out2) holds it between two launches.  u must not alias
+ * out1 or out2 (neighbouring cells read it during the stage); out1 may be x1
+ * and out2 may be x2.  Device pointers only.  No counterpart in the
+ * reference's assembly API (the update of explicit loops such as
+ * benchmark/covo.jl:14-33). */
+int bh_dg_transport_stage(bh_dg dg, int32_t nfields, const void* u, const void* x1, double a1,
+                          void* out1, const void* x2, double a2, void* out2, void* work);
 int bh_dg_destroy(bh_dg dg);
 
 /* ---- face terms of bilinear and linear forms (Quad4 meshes) --------------- */

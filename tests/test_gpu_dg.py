@@ -185,6 +185,29 @@ def test_rk4_matches_cpu_rk4_and_conserves_mass():
     assert torch.equal(ug, ud)
 
 
+@pytest.mark.parametrize("p,mode", [(2, "fp64"), (3, "fp32"), (3, "fp64"), (1, "fp32")])
+def test_fused_rk4_stage_bitwise_equals_two_pass(p, mode):
+    """bh_dg_transport_stage (the residual kernel applying M⁻¹ and both updates,
+    or residual + update through the work array for FP64 Q3) gives the RK4
+    step of four residuals and four M⁻¹ updates bit for bit; an output that
+    aliases the stage input is refused."""
+    import torch
+    m = _curved(3, seed=8)
+    V = _space(m, p)
+    plan = B.DGTransport(V, C, 2 * p + 1, mode)
+    dt = torch.float32 if mode == "fp32" else torch.float64
+    u0 = torch.from_numpy(_fields(V.get_ndofs(), 3, seed=4)).to("cuda", dt)
+    a, b = u0.clone(), u0.clone()
+    wa, wb = None, None
+    for _ in range(2):
+        wa = plan.rk4_step(a, 0.01, wa)
+        wb = plan.rk4_step_two_pass(b, 0.01, wb)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b) and not torch.equal(a, u0)
+    with pytest.raises(B.BcubeHipError):
+        plan.stage(3, a.data_ptr(), wa[1].data_ptr(), 0.1, a.data_ptr())
+
+
 def test_full_size_invariants_fp32():
     """The C4 workload: 32^3 curved Hexa27 cells, DG Q3, 8 fields, FP32."""
     import torch

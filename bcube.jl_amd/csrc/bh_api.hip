@@ -122,7 +122,9 @@ struct FusedDev {
 // templates, tiling; built on the first FP64 assembly of an eligible pattern.
 struct SfPlan {
   bool tried = false, ok = false;
-  int nx = 0, ny = 0, nz = 0, zlen = 0, ntpl = 0;
+  int nx = 0, ny = 0, nz = 0, ntpl = 0;
+  int zlen = 0;    // layers per workgroup: about one workgroup per CU (FP64)
+  int zlen32 = 0;  // about two per CU (the FP32 kernel fits two per CU)
   double B[3][3] = {}, Dd[3][3] = {};
   uint64_t* d_meta = nullptr;  // tile-major row records (SfArgs::meta)
   uint8_t* d_tpl = nullptr;
@@ -2163,8 +2165,12 @@ static int ensure_sf(bh_pattern p, bh_ctx c, bool upload) {
     BH_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
   const int64_t ntiles = (int64_t)((S.nx + kSfTileCells - 1) / kSfTileCells) *
                          ((S.ny + kSfTileCells - 1) / kSfTileCells);
-  const int64_t nzs = std::min<int64_t>(S.nz, std::max<int64_t>(1, (std::max(ncu, 1) + ntiles - 1) / ntiles));
-  s.zlen = (int)((S.nz + nzs - 1) / nzs);
+  auto zlen_for = [&](int64_t groups) {
+    const int64_t nzs = std::min<int64_t>(S.nz, std::max<int64_t>(1, (groups + ntiles - 1) / ntiles));
+    return (int)((S.nz + nzs - 1) / nzs);
+  };
+  s.zlen = zlen_for(std::max(ncu, 1));
+  s.zlen32 = zlen_for(2 * (int64_t)std::max(ncu, 1));
   s.nx = S.nx;
   s.ny = S.ny;
   s.nz = S.nz;
@@ -2251,15 +2257,17 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
   int64_t nc = m->h.ncells;
   const HostPattern& h = p->h;
   if (h.max_row > 512) return fail(c, BH_ERR_UNSUPPORTED, "CSR rows longer than 512 entries");
-  // 0. the one-pass structured Q2 kernel (bh_sf.hip): FP64, no element rows,
-  //    the default for an eligible pattern unless a caller fixed another
-  //    element-kernel shape
+  // 0. the one-pass structured Q2 kernel (bh_sf.hip): FP64 or FP32 element
+  //    matrices and row sums (FP64 geometry, FP64 values), no element rows.
+  //    FP64: the default for an eligible pattern unless a caller fixed another
+  //    element-kernel shape; FP32: when a caller fixed shape -2 (the matrix-core
+  //    element kernel + row gather stays the FP32 default: faster at 64^3)
   {
-    Tuning& tu0 = p->tune[0];
+    Tuning& tu0 = p->tune[mode == BH_FP64 ? 0 : (mode == BH_FP32 ? 1 : 2)];
     const int key = rule_key(m->buckets[0].geo, quad_degree);
-    const bool want = mode == BH_FP64 && m->buckets.size() == 1 &&
-                      sf_supported(m->geo, V->order, key) &&
-                      (tu0.elem_waves == 0 || tu0.elem_waves == -2);
+    const bool want = m->buckets.size() == 1 && sf_supported(m->geo, V->order, key) &&
+                      ((mode == BH_FP64 && (tu0.elem_waves == 0 || tu0.elem_waves == -2)) ||
+                       (mode == BH_FP32 && tu0.elem_waves == -2));
     if (want && !noalloc && !c->capturing)
       if (int r = ensure_sf(p, c)) return r;
     if (want && p->sf.ok) {
@@ -2280,7 +2288,8 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
       S.nx = sp.nx;
       S.ny = sp.ny;
       S.nz = sp.nz;
-      S.zlen = sp.zlen;
+      S.fp32 = mode == BH_FP32;
+      S.zlen = S.fp32 ? sp.zlen32 : sp.zlen;
       S.meta = sp.d_meta;
       S.tpl = sp.d_tpl;
       if (stiff) {

@@ -228,6 +228,7 @@ struct SfArgs {
   double B[3][3] = {}, Dd[3][3] = {};  // 1D Lagrange values, derivatives [node][point]
   int nx = 0, ny = 0, nz = 0;
   int tw = kSfTileCells, zlen = 0;
+  bool fp32 = false;  // element matrices and row sums in FP32 (geometry FP64)
   const uint64_t* meta = nullptr;
   const uint8_t* tpl = nullptr;
   const double* eta = nullptr;

@@ -49,11 +49,12 @@ constexpr int kSfThreads = 64 * kSfWaves;
 constexpr int kSfRow = 37;
 constexpr int kSfEven = 5 * kSfRow * kSfRow;  // accumulators of an even level
 constexpr int kSfOdd = 3 * kSfRow * kSfRow;   // accumulators of an odd level
-constexpr int kSfAcc = 2 * kSfEven + kSfOdd;  // 17797 doubles, 142 KB
+constexpr int kSfAcc = 2 * kSfEven + kSfOdd;  // 17797 values: 142 KB in FP64, 71 KB in FP32
 
 // Numeric parameters of one launch (the arrays travel as kernel arguments).
 struct SfP {
   double B[3][3], Dd[3][3];  // 1D Lagrange values / derivatives [node][Gauss point]
+  float Bf[3][3], Ddf[3][3];  // the same, rounded to FP32
   int nx, ny, nz, tw, zlen, ntx, nty;
   double eta_scale, mass_scale;
 };
@@ -95,15 +96,18 @@ __device__ __forceinline__ int sf_entry(int dx, int dy, int dz, int ex, int ey, 
 // t1[0] = B, t1[1] = D, [node][point]: an indexed read of the kernel-argument
 // copy would move the whole argument block to scratch memory; the factors of
 // the uniform nodes (j1, j2, i1) come from the kernel arguments.
-template <bool MASS>
-__device__ __forceinline__ void sf_element(const double* __restrict__ cf,
-                                           const double (*__restrict__ t1)[3][4], int i2, int i3,
-                                           int j3, const SfP& A, double (&K)[3][3][3]) {
-  double x00[3], x11[3], x01[3], x10[3], bi2[3], di2[3];
+template <typename T, bool MASS>
+__device__ __forceinline__ void sf_element(const T* __restrict__ cf,
+                                           const T (*__restrict__ t1)[3][4], int i2, int i3,
+                                           int j3, const SfP& A, T (&K)[3][3][3]) {
+  constexpr bool F = sizeof(T) == 4;
+  auto Bv = [&](int n, int q) -> T { return F ? (T)A.Bf[n][q] : (T)A.B[n][q]; };
+  auto Dv = [&](int n, int q) -> T { return F ? (T)A.Ddf[n][q] : (T)A.Dd[n][q]; };
+  T x00[3], x11[3], x01[3], x10[3], bi2[3], di2[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const double bi = t1[0][i3][q], di = t1[1][i3][q];
-    const double bj = t1[0][j3][q], dj = t1[1][j3][q];
+    const T bi = t1[0][i3][q], di = t1[1][i3][q];
+    const T bj = t1[0][j3][q], dj = t1[1][j3][q];
     x00[q] = bi * bj;
     x11[q] = di * dj;
     x01[q] = bi * dj;
@@ -116,41 +120,54 @@ __device__ __forceinline__ void sf_element(const double* __restrict__ cf,
 #pragma unroll
     for (int b = 0; b < 3; ++b)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) K[a][b][c] = 0.0;
+      for (int c = 0; c < 3; ++c) K[a][b][c] = (T)0;
 #pragma unroll
   for (int q1 = 0; q1 < 3; ++q1) {
-    double t00[3] = {0.0, 0.0, 0.0}, t11[3] = {0.0, 0.0, 0.0};
-    double t10[3] = {0.0, 0.0, 0.0}, t01[3] = {0.0, 0.0, 0.0};
+    T t00[3] = {0, 0, 0}, t11[3] = {0, 0, 0};
+    T t10[3] = {0, 0, 0}, t01[3] = {0, 0, 0};
 #pragma unroll
     for (int q2 = 0; q2 < 3; ++q2) {
       // z contraction: A (G33 DD + m BB), B (G11), C (G22), D (G12), F, G (G13
       // with BD / DB along z), H, I (G23 with BD / DB along z)
-      double tA = 0.0, tB = 0.0, tC = 0.0, tD = 0.0, tF = 0.0, tG = 0.0, tH = 0.0, tI = 0.0;
+      T tA = 0, tB = 0, tC = 0, tD = 0, tF = 0, tG = 0, tH = 0, tI = 0;
 #pragma unroll
       for (int q3 = 0; q3 < 3; ++q3) {
-        const double2* c = (const double2*)(cf + (q1 + 3 * q2 + 9 * q3) * 8);
-        const double2 g01 = c[0], g23 = c[1], g45 = c[2];
-        tA += g23.x * x11[q3];
-        if constexpr (MASS) tA += c[3].x * x00[q3];
-        tB += g01.x * x00[q3];
-        tC += g01.y * x00[q3];
-        tD += g23.y * x00[q3];
-        tF += g45.x * x01[q3];
-        tG += g45.x * x10[q3];
-        tH += g45.y * x01[q3];
-        tI += g45.y * x10[q3];
+        // (G11, G22, G33, G12, G13, G23, m) of point q1 + 3 q2 + 9 q3
+        T g[8];
+        const T* c = cf + (q1 + 3 * q2 + 9 * q3) * 8;
+        if constexpr (F) {
+          typedef float f4 __attribute__((ext_vector_type(4)));
+          const f4 u0 = *(const f4*)c, u1 = *(const f4*)(c + 4);
+          g[0] = u0.x; g[1] = u0.y; g[2] = u0.z; g[3] = u0.w;
+          g[4] = u1.x; g[5] = u1.y; g[6] = u1.z; g[7] = u1.w;
+        } else {
+          const double2 u0 = ((const double2*)c)[0], u1 = ((const double2*)c)[1],
+                        u2 = ((const double2*)c)[2];
+          g[0] = u0.x; g[1] = u0.y; g[2] = u1.x; g[3] = u1.y; g[4] = u2.x; g[5] = u2.y;
+          g[6] = MASS ? c[6] : (T)0;
+          g[7] = 0;
+        }
+        tA += g[2] * x11[q3];
+        if constexpr (MASS) tA += g[6] * x00[q3];
+        tB += g[0] * x00[q3];
+        tC += g[1] * x00[q3];
+        tD += g[3] * x00[q3];
+        tF += g[4] * x01[q3];
+        tG += g[4] * x10[q3];
+        tH += g[5] * x01[q3];
+        tI += g[5] * x10[q3];
       }
       // y contraction, grouped by the x factors: t00 (BB: A, C, H, I),
       // t11 (DD: B), t10 (DB: D, F), t01 (BD: the (2,1) term, G)
-      const double a = bi2[q2] * tA + di2[q2] * tH;
-      const double b = di2[q2] * tC + bi2[q2] * tI;
-      const double cB = bi2[q2] * tB;
-      const double d = bi2[q2] * tD;
-      const double f = bi2[q2] * tF;
-      const double e = di2[q2] * tD + bi2[q2] * tG;
+      const T a = bi2[q2] * tA + di2[q2] * tH;
+      const T b = di2[q2] * tC + bi2[q2] * tI;
+      const T cB = bi2[q2] * tB;
+      const T d = bi2[q2] * tD;
+      const T f = bi2[q2] * tF;
+      const T e = di2[q2] * tD + bi2[q2] * tG;
 #pragma unroll
       for (int j2 = 0; j2 < 3; ++j2) {
-        const double bj = A.B[j2][q2], dj = A.Dd[j2][q2];
+        const T bj = Bv(j2, q2), dj = Dv(j2, q2);
         t00[j2] += bj * a + dj * b;
         t11[j2] += bj * cB;
         t10[j2] += dj * d + bj * f;
@@ -162,16 +179,19 @@ __device__ __forceinline__ void sf_element(const double* __restrict__ cf,
     for (int j2 = 0; j2 < 3; ++j2)
 #pragma unroll
       for (int j1 = 0; j1 < 3; ++j1) {
-        const double u = A.B[j1][q1] * t00[j2] + A.Dd[j1][q1] * t01[j2];
-        const double v = A.Dd[j1][q1] * t11[j2] + A.B[j1][q1] * t10[j2];
+        const T u = Bv(j1, q1) * t00[j2] + Dv(j1, q1) * t01[j2];
+        const T v = Dv(j1, q1) * t11[j2] + Bv(j1, q1) * t10[j2];
 #pragma unroll
-        for (int i1 = 0; i1 < 3; ++i1) K[i1][j1][j2] += A.B[i1][q1] * u + A.Dd[i1][q1] * v;
+        for (int i1 = 0; i1 < 3; ++i1) K[i1][j1][j2] += Bv(i1, q1) * u + Dv(i1, q1) * v;
       }
   }
 }
 
-template <int NG, bool MASS>
-__global__ __launch_bounds__(kSfThreads) void k_sf_q2(
+// T: the arithmetic of the element matrices and of the LDS row sums (FP64, or
+// FP32 with the geometry still in FP64: 71 KB of LDS and at most 128 VGPRs,
+// two workgroups and four waves per SIMD).
+template <typename T, int NG, bool MASS>
+__global__ __launch_bounds__(kSfThreads, sizeof(T) == 4 ? 4 : 2) void k_sf_q2(
     const SfP P, const double* __restrict__ coords, const int32_t* __restrict__ c2n,
     const double* __restrict__ tab, const uint64_t* __restrict__ meta,
     const uint8_t* __restrict__ tpl, const double* __restrict__ eta,
@@ -179,16 +199,18 @@ __global__ __launch_bounds__(kSfThreads) void k_sf_q2(
   constexpr int kSlots = 25;                                       // 5 x 5 footprint cells
   constexpr int kNodeIt = (kSlots * NG + kSfThreads - 1) / kSfThreads;  // node loads per thread
   constexpr int kOutIt = (81 + kSfWaves - 1) / kSfWaves;                 // rows per wave and level
-  __shared__ double s_acc[kSfAcc];
+  __shared__ T s_acc[kSfAcc];
   __shared__ double s_xn[kSlots * NG * 3];  // node coordinates of the current layer
   __shared__ uint64_t s_meta[3][81];        // row records of the current layer's levels
-  __shared__ double s_t1[2][3][4];           // 1D B, D tables [node][point]
+  __shared__ T s_t1[2][3][4];                // 1D B, D tables [node][point]
   __shared__ int s_roff[2][81];              // box offsets of the tile's rows, even / odd levels
-  __shared__ double s_trash[16];             // target of the entries of rows other tiles own
+  __shared__ T s_trash[16];                  // target of the entries of rows other tiles own
   __shared__ double s_cc[kSlots][2];         // the layer's cell coefficients (η, mass)
-  // Hexa8: the geometric basis gradients at the 27 points (Hexa27 reads them
-  // from the table in memory: 17 KB more would not fit)
-  constexpr int kDg = NG == 8 ? 27 * 8 * 3 : 1;
+  // FP64 Hexa8: the geometric basis gradients at the 27 points (Hexa27 and the
+  // FP32 kernel, which must fit two workgroups per CU, read them from the
+  // table in memory)
+  constexpr bool kDgLds = NG == 8 && sizeof(T) == 8;
+  constexpr int kDg = kDgLds ? 27 * 8 * 3 : 1;
   __shared__ double s_dg[kDg];
   const int tid = threadIdx.x;
   int bid = blockIdx.x;
@@ -294,7 +316,7 @@ __global__ __launch_bounds__(kSfThreads) void k_sf_q2(
   // w + 8i, each lane the CSR positions k, k + 1 (k = 2 lane) in one 16-byte
   // store; the template byte pairs of all the levels written are loaded
   // before the first store, so their latency is paid once
-  auto write_levels = [&](const double* const* lb, int lv0, int nlv) {
+  auto write_levels = [&](const T* const* lb, int lv0, int nlv) {
     const int k = 2 * (tid & 63), r0 = tid >> 6;
     uint32_t e[3][kOutIt];
 #pragma unroll
@@ -316,13 +338,13 @@ __global__ __launch_bounds__(kSfThreads) void k_sf_q2(
         const int r = r0 + 8 * i;
         if (e[l][i] != 0xffffffffu) {
           const uint64_t rec = s_meta[lv0 + l][r];
-          const double* rb = lb[l] + s_roff[(lv0 + l) & 1][r];
+          const T* rb = lb[l] + s_roff[(lv0 + l) & 1][r];
           double* out = vals + rec_start(rec) + k;
-          const double v0 = rb[e[l][i] & 0xff];
+          const double v0 = (double)rb[e[l][i] & 0xff];
           if (k + 1 < rec_len(rec)) {
             double2 v;
             v.x = v0;
-            v.y = rb[e[l][i] >> 8];
+            v.y = (double)rb[e[l][i] >> 8];
             *(double2*)out = v;
           } else {
             *out = v0;
@@ -331,22 +353,22 @@ __global__ __launch_bounds__(kSfThreads) void k_sf_q2(
       }
   };
 
-  double* const E0 = s_acc;
-  double* const E1 = s_acc + kSfEven;
-  double* const O = s_acc + 2 * kSfEven;
-  for (int k = tid; k < kSfAcc; k += kSfThreads) s_acc[k] = 0.0;
+  T* const E0 = s_acc;
+  T* const E1 = s_acc + kSfEven;
+  T* const O = s_acc + 2 * kSfEven;
+  for (int k = tid; k < kSfAcc; k += kSfThreads) s_acc[k] = (T)0;
   if (tid < 18) {
     const int n = (tid % 9) / 3, q = tid % 3;
-    s_t1[tid / 9][n][q] = tid < 9 ? P.B[n][q] : P.Dd[n][q];
+    s_t1[tid / 9][n][q] = (T)(tid < 9 ? P.B[n][q] : P.Dd[n][q]);
   }
   if (tid < 162) {
     const int lv = tid / 81, r = tid % 81;
     s_roff[lv][r] = r < nrows ? sf_row_off(r % npx, r / npx, lv ? 3 : 5) : 0;
   }
   using L = TabLayout<3, 27, 27, NG>;
-  if constexpr (NG == 8)
+  if constexpr (kDgLds)
     for (int k = tid; k < kDg; k += kSfThreads) s_dg[k] = tab[L::DG + k];
-  const double* const dgt = NG == 8 ? s_dg : tab + L::DG;
+  const double* const dgt = kDgLds ? s_dg : tab + L::DG;
   // a segment that starts above the bottom first runs the layer below it for
   // the contributions of its upper rows to the segment's first level
   const int cz0 = z0 > 0 ? z0 - 1 : 0;
@@ -360,9 +382,9 @@ __global__ __launch_bounds__(kSfThreads) void k_sf_q2(
 
   int ein = 0;
   for (int cz = cz0; cz < z1; ++cz) {
-    double* const Ein = ein ? E1 : E0;   // level 2cz: partial sums of layer cz-1
-    double* const Eout = ein ? E0 : E1;  // level 2cz+2: free until the pushes
-    double* const cf = Eout;             // point factors [slot][27][8]
+    T* const Ein = ein ? E1 : E0;   // level 2cz: partial sums of layer cz-1
+    T* const Eout = ein ? E0 : E1;  // level 2cz+2: free until the pushes
+    T* const cf = Eout;             // point factors [slot][27][8]
     const bool more = cz + 1 < z1;
     for (int k = tid; k < nslots * 27; k += kSfThreads) {
       const int s = k / 27, q = k - s * 27;
@@ -385,18 +407,18 @@ __global__ __launch_bounds__(kSfThreads) void k_sf_q2(
       const double det = invert<3>(J, Ji);
       const double wdet = tab[L::W + q] * fabs(det);
       const double ge = wdet * s_cc[s][0];
-      double* o = cf + k * 8;
+      T* o = cf + k * 8;
       auto g = [&](int a, int b) {
         return ge * (Ji[a][0] * Ji[b][0] + Ji[a][1] * Ji[b][1] + Ji[a][2] * Ji[b][2]);
       };
-      o[0] = g(0, 0);
-      o[1] = g(1, 1);
-      o[2] = g(2, 2);
-      o[3] = g(0, 1);
-      o[4] = g(0, 2);
-      o[5] = g(1, 2);
-      o[6] = MASS ? wdet * s_cc[s][1] : 0.0;
-      o[7] = 0.0;
+      o[0] = (T)g(0, 0);
+      o[1] = (T)g(1, 1);
+      o[2] = (T)g(2, 2);
+      o[3] = (T)g(0, 1);
+      o[4] = (T)g(0, 2);
+      o[5] = (T)g(1, 2);
+      o[6] = MASS ? (T)(wdet * s_cc[s][1]) : (T)0;
+      o[7] = (T)0;
     }
     __syncthreads();
     // the next layer's node ids and row records load under the element stage
@@ -406,11 +428,11 @@ __global__ __launch_bounds__(kSfThreads) void k_sf_q2(
       load_ids(cz + 1);
       load_cc(cz + 1);
     }
-    double K[3][3][3];
-    if (my_slot >= 0) sf_element<MASS>(cf + my_slot * 27 * 8, s_t1, i2, i3, j3, P, K);
+    T K[3][3][3];
+    if (my_slot >= 0) sf_element<T, MASS>(cf + my_slot * 27 * 8, s_t1, i2, i3, j3, P, K);
     if (more) load_coords();
     __syncthreads();
-    for (int k = tid; k < kSfEven; k += kSfThreads) Eout[k] = 0.0;
+    for (int k = tid; k < kSfEven; k += kSfThreads) Eout[k] = (T)0;
     __syncthreads();
     // add the entries into the rows this tile owns, one colour at a time; the
     // targets of one colour phase are distinct, so each lane reads all its
@@ -419,9 +441,9 @@ __global__ __launch_bounds__(kSfThreads) void k_sf_q2(
     for (int c = 0; c < 4; ++c) {
       if (my_slot >= 0 && mcol == c) {
         // rows (i1, i2, i3), columns (j1, j2, j3): entry j2 ex + j1 past pd[i1]
-        double* pd[3];
+        T* pd[3];
         {
-          double* const lv = i3 == 0 ? Ein : (i3 == 1 ? O : Eout);
+          T* const lv = i3 == 0 ? Ein : (i3 == 1 ? O : Eout);
           const int ez = (i3 & 1) ? 3 : 5, ey = (i2 & 1) ? 3 : 5;
           const int y = yb + i2;
 #pragma unroll
@@ -436,9 +458,9 @@ __global__ __launch_bounds__(kSfThreads) void k_sf_q2(
         // i1 past pt[j1][j2]; entries of rows the tile does not own (and the
         // transposed set of the lanes with i3 = j3) go to s_trash, whose
         // contents no one reads
-        double* pt[3][3];
+        T* pt[3][3];
         {
-          double* const lv = j3 == 1 ? O : Eout;
+          T* const lv = j3 == 1 ? O : Eout;
           const int ez = (j3 & 1) ? 3 : 5;
 #pragma unroll
           for (int j2 = 0; j2 < 3; ++j2)
@@ -451,7 +473,7 @@ __global__ __launch_bounds__(kSfThreads) void k_sf_q2(
                                : s_trash;
             }
         }
-        double vd[3][3][3], vt[3][3][3];
+        T vd[3][3][3], vt[3][3][3];
 #pragma unroll
         for (int i1 = 0; i1 < 3; ++i1)
 #pragma unroll
@@ -478,12 +500,12 @@ __global__ __launch_bounds__(kSfThreads) void k_sf_q2(
     if (cz >= z0) {
       if (tid < 243) s_meta[tid / 81][tid % 81] = mrec;
       __syncthreads();
-      const double* lb[3] = {Ein, O, Eout};
+      const T* lb[3] = {Ein, O, Eout};
       write_levels(lb, 0, cz == P.nz - 1 ? 3 : 2);
     }
     __syncthreads();
     // the odd level starts the next layer empty; the next layer's nodes
-    for (int k = tid; k < kSfOdd; k += kSfThreads) O[k] = 0.0;
+    for (int k = tid; k < kSfOdd; k += kSfThreads) O[k] = (T)0;
     if (more) {
       store_coords();
       store_cc();
@@ -509,6 +531,8 @@ hipError_t launch_sf(const SfArgs& a) {
     for (int q = 0; q < 3; ++q) {
       k.B[n][q] = a.B[n][q];
       k.Dd[n][q] = a.Dd[n][q];
+      k.Bf[n][q] = (float)a.B[n][q];
+      k.Ddf[n][q] = (float)a.Dd[n][q];
     }
   k.nx = a.nx;
   k.ny = a.ny;
@@ -521,16 +545,23 @@ hipError_t launch_sf(const SfArgs& a) {
   k.mass_scale = a.mass_scale;
   const int nzs = (a.nz + a.zlen - 1) / a.zlen;
   dim3 grid((unsigned)(k.ntx * k.nty * nzs));
-#define BH_SF(NG, MASS)                                                                        \
-  hipLaunchKernelGGL((k_sf_q2<NG, MASS>), grid, dim3(kSfThreads), 0, a.stream, k, a.coords,   \
+#define BH_SF(T, NG, MASS)                                                                      \
+  hipLaunchKernelGGL((k_sf_q2<T, NG, MASS>), grid, dim3(kSfThreads), 0, a.stream, k, a.coords, \
                      a.c2n, a.tab, a.meta, a.tpl, a.eta, a.mcoef, a.vals)
-  if (a.geo == BH_HEXA8) {
-    if (a.mass) BH_SF(8, true);
-    else BH_SF(8, false);
-  } else {
-    if (a.mass) BH_SF(27, true);
-    else BH_SF(27, false);
+#define BH_SF_T(T)                  \
+  if (a.geo == BH_HEXA8) {          \
+    if (a.mass) BH_SF(T, 8, true);  \
+    else BH_SF(T, 8, false);        \
+  } else {                          \
+    if (a.mass) BH_SF(T, 27, true); \
+    else BH_SF(T, 27, false);       \
   }
+  if (a.fp32) {
+    BH_SF_T(float)
+  } else {
+    BH_SF_T(double)
+  }
+#undef BH_SF_T
 #undef BH_SF
   return hipGetLastError();
 }

@@ -39,7 +39,7 @@ GATHER = {"fp64": "k_gather_padded<double,unsigned char,27,128,8,2,32>",
           "fp32": "k_gather_padded<float,unsigned char,27,128,8,2,32>",
           "bf16x3": "k_gather_padded<float,unsigned char,27,128,8,2,32>"}
 FUSED_KERNEL = "k_element_gp_fused<3,27,27,8,true,32,8> (element matrices + LDS row sums)"
-ONEPASS_KERNEL = "k_sf_q2<8,true>"
+ONEPASS_KERNEL = "k_sf_q2<double,8,true>"
 
 
 def workload(n):
@@ -1813,6 +1813,38 @@ def bench_c2(args, torch, B):
                 del out2
         if mode != "fp64":
             del out
+    if ref is not None and "fp32" in results and results["fp64"].get("onepass", {}).get("ok"):
+        # the one-pass kernel in FP32 (FP32 element matrices and row sums from
+        # FP64 geometry; FP32 takes the matrix-core kernels by default), then
+        # the autotuned FP32 shapes again
+        tuned = plan.tuning("fp32")
+        out = ref
+        out2 = torch.empty(nnz, dtype=torch.float64, device=dev)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        plan.set_tuning(-2, mode="fp32")
+        for _ in range(args.warmup):
+            plan.assemble_device(out2.data_ptr(), coef, "fp32")
+        torch.cuda.synchronize()
+        backend.set_timing(True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            plan.assemble_device(out2.data_ptr(), coef, "fp32")
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms3 = e0.elapsed_time(e1) / args.steps
+        l3, k3 = backend.timings_ex(cap=100000)
+        backend.set_timing(False)
+        d3, kern3 = kernel_rooflines("fp64", l3, k3, args.steps, f_alg, nnz, b_alg)
+        d3["kernel"] = "k_sf_q2<float,8,true>"
+        results["fp32_onepass"] = {
+            "ms_per_step": ms3, "gnnz_per_s": nnz / (ms3 * 1e-3) / 1e9,
+            "rel_err_vs_fp64_gpu": float((out2 - out).abs().max() / out.abs().max()),
+            "roofline": d3, "kernels": [d3]}
+        if tuned:
+            plan.set_tuning(tuned["elem_waves"], tuned["gather_lanes"], tuned["gather_rows"],
+                            mode="fp32")
+        del out2
     head = "fp32" if "fp32" in results else modes[0]
     r = results[head]
     roof = dict(r["roofline"])

@@ -169,13 +169,15 @@ int bh_pattern_build_device(bh_space U, bh_space V, bh_pattern* out);
  * negative on error.  No counterpart in the reference. */
 int bh_pattern_tuning(bh_pattern p, int mode, int32_t* shape, float* ms);
 /* The one-pass kernel for continuous Q2 spaces on structured hexahedral grids
- * (Hexa8 or Hexa27 cells, 3 Gauss points per axis, BH_FP64): FP64 element
- * matrices by sum factorisation, each CSR row summed in LDS by the workgroup
- * that owns its lattice point and written once, with no element-row workspace.
- * No counterpart in the reference, whose assemble_bilinear
- * (assembler.jl:172-186) loops over cells into a COO.  bh_assemble_bilinear
- * takes it by default for eligible patterns in BH_FP64 (bh_pattern_set_tuning
- * shape[0] 8 or 12 selects the element + row-gather kernels).  The grid is
+ * (Hexa8 or Hexa27 cells, 3 Gauss points per axis): element matrices by sum
+ * factorisation, each CSR row summed in LDS by the workgroup that owns its
+ * lattice point and written once, with no element-row workspace (BH_FP64:
+ * FP64 throughout; BH_FP32: FP32 element matrices and row sums from FP64
+ * geometry, FP64 values).  No counterpart in the reference, whose
+ * assemble_bilinear (assembler.jl:172-186) loops over cells into a COO.
+ * bh_assemble_bilinear takes it by default for eligible patterns in BH_FP64
+ * (bh_pattern_set_tuning shape[0] 8 or 12 selects the element + row-gather
+ * kernels) and in BH_FP32 when shape[0] is -2.  The grid is
  * inferred from the dof map; this call analyses p on first use (host-only
  * patterns without device buffers) and reports: info[0] 1 eligible, 0 not
  * (bh_last_error gives the reason), info[1..3] nx, ny, nz, info[4] distinct

@@ -100,6 +100,29 @@ def test_onepass_vs_oracle(dims, terms):
     assert err <= RTOL
 
 
+@pytest.mark.parametrize("dims,curved", [((5, 6, 3), False), ((9, 5, 4), False), ((4, 4, 4), True)])
+def test_onepass_fp32_vs_oracle(dims, curved):
+    """The FP32 variant (shape -2 in BH_FP32): FP32 element matrices and row sums
+    from FP64 geometry, within 1e-6 of max|A| of the FP64 oracle."""
+    m = _grid(*dims, seed=7, curved=curved)
+    eta = np.random.default_rng(4).uniform(0.5, 2.0, m.ncells)
+    plan = _plan(m, ("stiffness", "mass"), eta)
+    plan.set_tuning(-2, mode="fp32")
+    backend = m.backend
+    backend.set_timing(True)
+    try:
+        vals = plan.assemble_host(mode="fp32")
+        _, kinds = backend.timings_ex()
+    finally:
+        backend.set_timing(False)
+    assert list(kinds) == [3]
+    colptr, rowval, ref = _oracle(m, ("stiffness", "mass"), eta)
+    err = np.abs(vals - ref).max() / np.abs(ref).max()
+    print(f"fp32 one pass {dims} curved={curved}: rel err {err:.2e}")
+    assert 0 < err <= 1e-6
+    assert np.array_equal(vals, plan.assemble_host(mode="fp32"))
+
+
 def test_onepass_constant_coefficients():
     m = _grid(6, 5, 7, seed=4)
     plan = _plan(m, ("stiffness", "mass"), None, c=0.3)
@@ -200,3 +223,9 @@ def test_onepass_full_size_matches_gather_path():
     err = (a - b).abs().max().item() / scale
     print(f"64^3 one-pass vs element + row gather: {err:.2e}")
     assert err <= 1e-13
+    plan.set_tuning(-2, mode="fp32")
+    plan.assemble_device(b.data_ptr(), [eta_d.data_ptr(), None], "fp32")
+    m.backend.sync()
+    e32 = (a - b).abs().max().item() / scale
+    print(f"64^3 FP32 one pass vs FP64: {e32:.2e}")
+    assert 0 < e32 <= 1e-6

@@ -1327,6 +1327,247 @@ static void assemble_source(const Mesh& m, const std::vector<i64>& iglob, const 
 }
 
 // ---------------------------------------------------------------------------
+// Continuous Lagrange spaces of per-cell order on 2D Quad4 / Tri3 meshes.
+// No reference counterpart: the reference's DofHandler takes one
+// FunctionSpace (dofhandler.jl:37-42).  The space extends its rules:
+//  * cell c has order p_c; an edge has the lowest order p_e of its cells;
+//  * V = { v continuous : v|c in P_{p_c} (Q_{p_c} on quads), v|e in P_{p_e} on
+//    every edge e } -- continuity forces the trace on an edge into the lower
+//    order's space, so this is the largest conforming space of these orders;
+//  * dofs: one per vertex, p_e - 1 per edge (the P_{p_e} Lagrange nodes of the
+//    edge, in the direction of the first cell that has it, as the reference
+//    orients edge dofs by their first cell, dofhandler.jl:239-290), and each
+//    cell's interior nodes;
+//  * numbering: first appearance over cells in mesh order and each cell's
+//    local dofs in Lagrange order.  With one order everywhere this is the
+//    reference's DofHandler + densify;
+//  * a cell's own nodes on an edge of lower order (p_e < p_c) are not dofs:
+//    the value at its k-th edge node (parameter k/p_c along the cell's local
+//    edge direction) is the edge's interpolant Σ_j ℓ_j(k/p_c) u_j, ℓ_j the
+//    P_{p_e} Lagrange basis on the nodes j/p_e (j = 0: the edge's first vertex,
+//    j = p_e: its second).  The cell keeps its p_c - 1 slots for that edge: slot
+//    k <= p_e - 1 holds the edge's dof j = k (counted in the cell's direction),
+//    the others are unused (-1).
+// The element matrix in the slots is Cᵀ K C with K the cell's Lagrange element
+// matrix and C the slot-to-node interpolation above (identity on free nodes).
+// Parity pinned by mathematics (tests/test_orders_oracle.py): the numbering
+// and the matrix with one order everywhere equal the reference restatement's
+// bit for bit; every basis function's traces from the two cells of each edge
+// agree; the matrix equals Pᵀ A P with A the uniform highest-order matrix of the
+// same mesh and rule and P the interpolation of the space into it; linear
+// fields are in the kernel of the stiffness matrix and 1ᵀM1 is the area.
+// ---------------------------------------------------------------------------
+struct OrderSpace {
+  i64 ndofs = 0;
+  int nl = 0;              // slots per cell: the largest cell's Lagrange dof count
+  std::vector<i64> dofs;   // ncells * nl, 0-based; -1: unused slot
+  std::vector<int> eord;   // ncells * 4: order of a local edge of lower order, 0 if free
+};
+
+static std::pair<i64, i64> edge_key(i64 a, i64 b) { return a < b ? std::make_pair(a, b) : std::make_pair(b, a); }
+
+static OrderSpace order_space(const Mesh& m, const int* order) {
+  OrderSpace S;
+  const i64 nc = m.ncells();
+  for (i64 c = 0; c < nc; ++c) {
+    if (m.dim != 2 || (shape_of(m.ctype[c]) != SQUARE && shape_of(m.ctype[c]) != TRIANGLE))
+      throw std::runtime_error("oracle: per-cell orders need 2D Quad4 / Tri3 cells");
+    if (order[c] < 1 || order[c] > 4) throw std::runtime_error("oracle: cell orders are 1..4");
+    S.nl = std::max(S.nl, ndofs_lagrange(shape_of(m.ctype[c]), order[c]));
+  }
+  std::map<std::pair<i64, i64>, int> pe;
+  for (i64 c = 0; c < nc; ++c) {
+    std::vector<i64> nodes = m.cell(c);
+    for (const auto& e : edges2nodes(m.ctype[c])) {
+      auto key = edge_key(nodes[e[0]], nodes[e[1]]);
+      auto it = pe.find(key);
+      if (it == pe.end()) pe[key] = order[c];
+      else it->second = std::min(it->second, order[c]);
+    }
+  }
+  S.dofs.assign((size_t)nc * S.nl, -1);
+  S.eord.assign((size_t)nc * 4, 0);
+  std::map<i64, i64> vid;
+  std::map<std::pair<i64, i64>, std::pair<i64, std::vector<i64>>> eid;  // first node, ids
+  i64 next = 0;
+  for (i64 c = 0; c < nc; ++c) {
+    const int p = order[c];
+    const Shape s = shape_of(m.ctype[c]);
+    const int nlc = ndofs_lagrange(s, p);
+    std::vector<i64> nodes = m.cell(c);
+    Lists vd = idof_by_vertex(s, p), ed = idof_by_edge(s, p), e2n = edges2nodes(m.ctype[c]);
+    // role of each local dof: vertex v, edge (e, k), or interior
+    std::vector<int> rv(nlc, -1), re(nlc, -1), rk(nlc, 0);
+    for (size_t v = 0; v < vd.size(); ++v) rv[vd[v][0]] = (int)v;
+    for (size_t e = 0; e < ed.size(); ++e)
+      for (size_t k = 0; k < ed[e].size(); ++k) { re[ed[e][k]] = (int)e; rk[ed[e][k]] = (int)k + 1; }
+    for (size_t e = 0; e < e2n.size(); ++e) {
+      const int q = pe[edge_key(nodes[e2n[e][0]], nodes[e2n[e][1]])];
+      if (q < p) S.eord[c * 4 + e] = q;
+    }
+    i64* slot = &S.dofs[(size_t)c * S.nl];
+    for (int i = 0; i < nlc; ++i) {
+      if (rv[i] >= 0) {
+        const i64 node = nodes[rv[i]];
+        auto it = vid.find(node);
+        if (it == vid.end()) it = vid.emplace(node, next++).first;
+        slot[i] = it->second;
+      } else if (re[i] >= 0) {
+        const int e = re[i], k = rk[i];
+        const i64 n0 = nodes[e2n[e][0]], n1 = nodes[e2n[e][1]];
+        const auto key = edge_key(n0, n1);
+        const int q = pe[key];
+        if (k > q - 1) continue;  // a node of the cell's higher order: no dof
+        auto it = eid.find(key);
+        if (it == eid.end()) it = eid.emplace(key, std::make_pair(n0, std::vector<i64>(q - 1, -1))).first;
+        const int j = it->second.first == n0 ? k : q - k;
+        i64& id = it->second.second[j - 1];
+        if (id < 0) id = next++;
+        slot[i] = id;
+      } else {
+        slot[i] = next++;
+      }
+    }
+  }
+  S.ndofs = next;
+  return S;
+}
+
+// ℓ_j(s) of the order-q Lagrange basis on the nodes 0, 1/q, ..., 1.
+static double lagrange_unit(int q, int j, double s) {
+  double v = 1.0;
+  for (int i = 0; i <= q; ++i)
+    if (i != j) v *= (q * s - i) / (double)(j - i);
+  return v;
+}
+
+// C[a * nlc + m]: value at cell node a of the slot function m (see above).
+static void cell_constraint(int ct, int p, const int* eord, std::vector<double>& C) {
+  const Shape s = shape_of(ct);
+  const int nlc = ndofs_lagrange(s, p);
+  C.assign((size_t)nlc * nlc, 0.0);
+  for (int a = 0; a < nlc; ++a) C[(size_t)a * nlc + a] = 1.0;
+  Lists vd = idof_by_vertex(s, p), ed = idof_by_edge(s, p), e2n = edges2nodes(ct);
+  for (size_t e = 0; e < e2n.size(); ++e) {
+    const int q = eord[e];
+    if (q == 0) continue;
+    const int va = vd[e2n[e][0]][0], vb = vd[e2n[e][1]][0];
+    for (int k = 1; k <= p - 1; ++k) {
+      const int a = ed[e][k - 1];
+      const double t = (double)k / p;
+      double* row = &C[(size_t)a * nlc];
+      for (int m2 = 0; m2 < nlc; ++m2) row[m2] = 0.0;
+      row[va] = lagrange_unit(q, 0, t);
+      row[vb] = lagrange_unit(q, q, t);
+      for (int j = 1; j <= q - 1; ++j) row[ed[e][j - 1]] = lagrange_unit(q, j, t);
+    }
+  }
+}
+
+static bool constrained(const int* eord) { return eord[0] || eord[1] || eord[2] || eord[3]; }
+
+// Σ terms of the bilinear form on a per-cell-order space: COO in the
+// reference's order (terms, then cells in element order, entry i + nl*j of
+// the slots holding dofs) with slot blocks Cᵀ K C, then sparse().  degree[p]:
+// the Measure degree of the cells of order p.
+static void assemble_orders(const Mesh& m, const OrderSpace& S, const int* order, int nterms,
+                            const int* terms, const double* eta, const int* degree, CSC& A) {
+  std::vector<i64> cells = element_order(m);
+  std::vector<i64> I, J;
+  std::vector<double> X;
+  std::map<int, ShapeRules> rules;
+  std::vector<double> acc, C, T, Kp;
+  for (int t = 0; t < nterms; ++t)
+    for (i64 c : cells) {
+      const int p = order[c];
+      ShapeRules& R = rules[degree[p]];
+      R.degree = degree[p];
+      const int nlc = element_block(m, p, terms[t], eta, R, c, acc);
+      const int* eo = &S.eord[c * 4];
+      const std::vector<double>* K = &acc;
+      if (constrained(eo)) {
+        cell_constraint(m.ctype[c], p, eo, C);
+        // acc[b*nl + a] = a(φ_b, φ_a); Kp[n*nl + m] = Σ_b Σ_a C[b][n] C[a][m] acc[b*nl + a]
+        T.assign((size_t)nlc * nlc, 0.0);
+        for (int b = 0; b < nlc; ++b)
+          for (int mm = 0; mm < nlc; ++mm) {
+            double v = 0.0;
+            for (int a = 0; a < nlc; ++a) v += acc[(size_t)b * nlc + a] * C[(size_t)a * nlc + mm];
+            T[(size_t)b * nlc + mm] = v;
+          }
+        Kp.assign((size_t)nlc * nlc, 0.0);
+        for (int n = 0; n < nlc; ++n)
+          for (int mm = 0; mm < nlc; ++mm) {
+            double v = 0.0;
+            for (int b = 0; b < nlc; ++b) v += C[(size_t)b * nlc + n] * T[(size_t)b * nlc + mm];
+            Kp[(size_t)n * nlc + mm] = v;
+          }
+        K = &Kp;
+      }
+      const i64* sl = &S.dofs[(size_t)c * S.nl];
+      for (int j = 0; j < nlc; ++j) {
+        if (sl[j] < 0) continue;
+        for (int i = 0; i < nlc; ++i) {
+          if (sl[i] < 0) continue;
+          I.push_back(sl[i] + 1);
+          J.push_back(sl[j] + 1);
+          X.push_back((*K)[(size_t)j * nlc + i]);
+        }
+      }
+    }
+  coo_to_csc(I, J, X, S.ndofs, S.ndofs, A);
+}
+
+// ∫ f v dΩ on a per-cell-order space: element vectors Cᵀ b_c added in element order.
+static void assemble_source_orders(const Mesh& m, const OrderSpace& S, const int* order, int fsrc,
+                                   double fconst, const int* degree, std::vector<double>& b) {
+  b.assign(S.ndofs, 0.0);
+  std::vector<i64> cells = element_order(m);
+  std::map<int, ShapeRules> rules;
+  std::vector<double> C;
+  const int d = m.dim;
+  for (i64 c : cells) {
+    const int p = order[c];
+    ShapeRules& R = rules[degree[p]];
+    R.degree = degree[p];
+    const int ct = m.ctype[c];
+    const Shape s = shape_of(ct);
+    const int nlc = ndofs_lagrange(s, p);
+    const std::vector<double>& w = R.weights(s);
+    const std::vector<double>& xq = R.points(s);
+    const int nq = R.count(s);
+    std::vector<i64> nodes = m.cell(c);
+    double Xc[27 * 3];
+    for (size_t q = 0; q < nodes.size(); ++q)
+      for (int a = 0; a < d; ++a) Xc[q * d + a] = m.coords[nodes[q] * d + a];
+    std::vector<double> val(nlc), rg(nlc * d), acc(nlc, 0.0);
+    for (int q = 0; q < nq; ++q) {
+      Geo g;
+      geometry(ct, Xc, &xq[q * d], g);
+      tensor_basis(s, p, &xq[q * d], val.data(), rg.data());
+      const double fv = eval_source(fsrc, fconst, g.x);
+      for (int i = 0; i < nlc; ++i) {
+        const double wu = w[q] * (g.detabs * (fv * val[i]));
+        acc[i] = (q == 0) ? wu : acc[i] + wu;
+      }
+    }
+    const int* eo = &S.eord[c * 4];
+    std::vector<double> be = acc;
+    if (constrained(eo)) {
+      cell_constraint(ct, p, eo, C);
+      for (int mm = 0; mm < nlc; ++mm) {
+        double v = 0.0;
+        for (int a = 0; a < nlc; ++a) v += C[(size_t)a * nlc + mm] * acc[a];
+        be[mm] = v;
+      }
+    }
+    const i64* sl = &S.dofs[(size_t)c * S.nl];
+    for (int i = 0; i < nlc; ++i)
+      if (sl[i] >= 0) b[sl[i]] += be[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Hexa27 geometry (mapping.jl:664-697).  mapping_jacobian for Hexa27 is the
 // ForwardDiff fallback (mapping.jl:166-168): restated with Dual<3>.
 // ---------------------------------------------------------------------------
@@ -1765,6 +2006,51 @@ int64_t or_dofhandler(int dim, int64_t nnodes, const double* coords, int64_t nce
     std::copy(off.begin(), off.end(), offset);
     return tot;
   } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Per-cell-order space (order_space): returns ndofs or -1; dofs ncells * nl
+// (0-based, -1 unused slots, nl = the largest cell's Lagrange dof count, which
+// the caller passes), eord ncells * 4.
+int64_t or_order_space(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                       const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n,
+                       const int* order, int nl, int64_t* dofs, int* eord) {
+  try {
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    OrderSpace S = order_space(m, order);
+    if (S.nl != nl) throw std::runtime_error("oracle: nl is not the largest cell's dof count");
+    std::copy(S.dofs.begin(), S.dofs.end(), dofs);
+    std::copy(S.eord.begin(), S.eord.end(), eord);
+    return S.ndofs;
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// assemble_bilinear / assemble_linear (source) on a per-cell-order space;
+// degree[p], p = 0..4, is the Measure degree of the cells of order p.
+void* or_assemble_bilinear_orders(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                                  const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n,
+                                  const int* order, int nterms, const int* terms,
+                                  const double* eta, const int* degree) {
+  try {
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    OrderSpace S = order_space(m, order);
+    Result* r = new Result();
+    r->ndofs = S.ndofs;
+    assemble_orders(m, S, order, nterms, terms, eta, degree, r->A);
+    return r;
+  } catch (std::exception& e) { g_err = e.what(); return nullptr; }
+}
+
+void* or_assemble_linear_orders(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                                const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n,
+                                const int* order, int fsrc, double fconst, const int* degree) {
+  try {
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    OrderSpace S = order_space(m, order);
+    Result* r = new Result();
+    r->ndofs = S.ndofs;
+    assemble_source_orders(m, S, order, fsrc, fconst, degree, r->b);
+    return r;
+  } catch (std::exception& e) { g_err = e.what(); return nullptr; }
 }
 
 // DofHandler(mesh, fs, ncomp, continuous) of a vector space (size = ncomp):

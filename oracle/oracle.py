@@ -84,6 +84,12 @@ def lib():
         L.or_dg_timed.argtypes = mesh_args + [C.c_int, _d, C.c_int, C.c_int, C.c_int, _d, _d,
                                               C.c_int, C.c_int, _d]
         L.or_dg_timed.restype = C.c_int64
+        L.or_order_space.argtypes = mesh_args + [_i, C.c_int, _l, _i]
+        L.or_order_space.restype = C.c_int64
+        L.or_assemble_bilinear_orders.argtypes = mesh_args + [_i, C.c_int, _i, C.c_void_p, _i]
+        L.or_assemble_bilinear_orders.restype = C.c_void_p
+        L.or_assemble_linear_orders.argtypes = mesh_args + [_i, C.c_int, C.c_double, _i]
+        L.or_assemble_linear_orders.restype = C.c_void_p
         _LIB = L
     return _LIB
 
@@ -313,6 +319,76 @@ def assemble_bilinear(mesh: OMesh, order, terms, eta=None, degree=2, continuous=
 def assemble_linear(mesh: OMesh, order, source="sinsin", const=0.0, degree=2, continuous=True):
     h = lib().or_assemble_linear(*mesh.args(), order, int(continuous), SOURCE[source],
                                  float(const), degree)
+    if not h:
+        raise RuntimeError(_err())
+    try:
+        n = lib().or_result_ndofs(h)
+        b = np.zeros(n)
+        lib().or_result_vector(h, b)
+    finally:
+        lib().or_result_free(h)
+    return b
+
+
+def _orders_args(mesh: OMesh, order, degree):
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    assert order.size == mesh.ncells
+    deg = np.zeros(5, dtype=np.int32)
+    for p in range(1, 5):
+        deg[p] = degree(p) if callable(degree) else int(degree)
+    return order, deg
+
+
+def order_slots(mesh: OMesh, order):
+    """Slots per cell of a per-cell-order space: the largest cell's Lagrange
+    dof count."""
+    nd = {1: lambda p: (p + 1) ** 2, 2: lambda p: (p + 1) * (p + 2) // 2}
+    return max(nd[1 if int(t) in (0, 1) else 2](int(p)) for t, p in zip(mesh.celltypes, order))
+
+
+def order_space(mesh: OMesh, order):
+    """Continuous Lagrange space of per-cell order on a 2D Quad4/Tri3 mesh
+    (edges take their cells' lowest order; see bcube_oracle.cpp).  Returns
+    (dofs (ncells, nl) 0-based with -1 on unused slots, eord (ncells, 4): the
+    order of each local edge of lower order than its cell, 0 if free, ndofs)."""
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    nl = order_slots(mesh, order)
+    dofs = np.zeros(mesh.ncells * nl, dtype=np.int64)
+    eord = np.zeros(mesh.ncells * 4, dtype=np.int32)
+    n = lib().or_order_space(*mesh.args(), order, nl, dofs, eord)
+    if n < 0:
+        raise RuntimeError(_err())
+    return dofs.reshape(-1, nl), eord.reshape(-1, 4), int(n)
+
+
+def assemble_bilinear_orders(mesh: OMesh, order, terms, eta=None, degree=lambda p: max(2 * p, 4)):
+    """CSC (1-based) of Σ terms on the per-cell-order space; degree: the
+    Measure degree of the cells of order p (a function of p, or one int)."""
+    order, deg = _orders_args(mesh, order, degree)
+    t = np.ascontiguousarray([TERM[x] for x in terms], dtype=np.int32)
+    eta_p = None
+    if eta is not None:
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        eta_p = eta.ctypes.data
+    h = lib().or_assemble_bilinear_orders(*mesh.args(), order, len(t), t, eta_p, deg)
+    if not h:
+        raise RuntimeError(_err())
+    try:
+        n = lib().or_result_ndofs(h)
+        nnz = lib().or_result_nnz(h)
+        colptr = np.zeros(n + 1, dtype=np.int64)
+        rowval = np.zeros(nnz, dtype=np.int64)
+        nzval = np.zeros(nnz)
+        lib().or_result_csc(h, colptr, rowval, nzval)
+    finally:
+        lib().or_result_free(h)
+    return colptr, rowval, nzval, int(n)
+
+
+def assemble_linear_orders(mesh: OMesh, order, source="one", const=0.0,
+                           degree=lambda p: max(2 * p, 4)):
+    order, deg = _orders_args(mesh, order, degree)
+    h = lib().or_assemble_linear_orders(*mesh.args(), order, SOURCE[source], float(const), deg)
     if not h:
         raise RuntimeError(_err())
     try:

@@ -12,7 +12,7 @@ from .mesh import (AbstractBcubeBackend, BcubeBackendHIP, BcubeBackendHostOnly, 
 from .fespace import (DofHandler, FunctionSpace, MultiFESpace, TestFESpace, TrialFESpace,
                       component_numbering)
 from .forms import (BoundaryFaceDomain, CellDomain, EulerFlux, FEFunction, InteriorFaceDomain,
-                    Measure, MeshCellData, PeriodicBCType, PhysicalFunction, RoeFlux,
+                    Measure, MeshCellData, OrderDegree, PeriodicBCType, PhysicalFunction, RoeFlux,
                     Translation, UpwindFlux, avg, compose, ddot, dot, get_face_normals, grad,
                     integrate, jump, nabla, side_minus, side_n, side_p, side_plus)
 from .euler import EulerDG, periodic_face_pairs
@@ -46,5 +46,5 @@ __all__ = [
     "nodes_gmsh2cgns", "read_mesh", "projection_l2_", "MultiFESpace", "component_numbering",
     "ddot", "MultiBilinearPlan", "BoundaryFaceDomain", "avg", "BoundaryFaces",
     "FaceBilinearPlan", "EulerFlux", "RoeFlux", "PeriodicBCType", "Translation", "EulerDG",
-    "periodic_face_pairs", "AffineFESystem",
+    "periodic_face_pairs", "AffineFESystem", "OrderDegree",
 ]

@@ -16,17 +16,33 @@ from .mesh import Mesh
 
 
 class FunctionSpace:
-    """FunctionSpace(:Lagrange, degree) -- uniform-node Lagrange (Lagrange{:Uniform})."""
+    """FunctionSpace(:Lagrange, degree) -- uniform-node Lagrange (Lagrange{:Uniform}).
 
-    def __init__(self, kind: str, degree: int):
+    `degree` may also be one order (1..4) per cell of a 2D Quad4/Tri3 mesh:
+    the continuous space of per-cell order whose edges take their cells'
+    lowest order (bh_space_create_orders; no counterpart in the reference,
+    whose FunctionSpace has one degree).  `degree` is then the highest order
+    and `cell_degrees` the array."""
+
+    def __init__(self, kind: str, degree):
         if kind.lower() != "lagrange":
             raise NotImplementedError(f"function space {kind!r}: only Lagrange is supported")
+        self.kind = "Lagrange"
+        self.cell_degrees = None
+        if np.ndim(degree) > 0:
+            d = np.ascontiguousarray(degree, dtype=np.int32).reshape(-1)
+            if d.size == 0 or d.min() < 1 or d.max() > 4:
+                raise ValueError("per-cell degrees must be 1..4")
+            self.cell_degrees = d
+            self.degree = int(d.max())
+            return
         if degree < 0:
             raise ValueError("degree must be >= 0")
-        self.kind = "Lagrange"
         self.degree = int(degree)
 
     def __repr__(self):
+        if self.cell_degrees is not None:
+            return f"FunctionSpace(:{self.kind}, per-cell degrees {np.bincount(self.cell_degrees)[1:]})"
         return f"FunctionSpace(:{self.kind}, {self.degree})"
 
 
@@ -145,17 +161,66 @@ class SingleFESpace:
         self.mesh = mesh
         self.continuous = bool(isContinuous)
         self.size = int(size)
+        self._handle = None
+        if fs.cell_degrees is not None:
+            # continuous space of per-cell order: numbered by the library
+            if not self.continuous or self.size != 1:
+                raise NotImplementedError("per-cell degrees: continuous scalar spaces only")
+            if fs.cell_degrees.size != mesh.ncells:
+                raise ValueError("one degree per cell")
+            h = C.c_void_p()
+            L.check(L.lib().bh_space_create_orders(mesh.handle, fs.cell_degrees.ctypes.data,
+                                                   C.byref(h)), mesh.backend.ctx,
+                    "bh_space_create_orders")
+            self._handle = h
+            n, nl = C.c_int64(), C.c_int32()
+            L.check(L.lib().bh_space_info(h, C.byref(n), C.byref(nl)), mesh.backend.ctx,
+                    "bh_space_info")
+            self._ndofs, self._nl = n.value, nl.value
+            self.dhl = self._scalar = None
+            return
         self.dhl = DofHandler(mesh, fs, self.size, self.continuous)
         self._scalar = DofHandler(mesh, fs, 1, self.continuous) if self.size > 1 else self.dhl
-        self._handle = None
 
     def get_ndofs(self) -> int:
+        if self.dhl is None:
+            return self._ndofs
         return self.dhl.ndofs_tot
 
     def get_scalar_ndofs(self) -> int:
+        if self._scalar is None:
+            return self._ndofs
         return self._scalar.ndofs_tot
 
+    def cell_slots(self) -> np.ndarray:
+        """(ncells, nl) 0-based dof of each cell's Lagrange slot, -1 where the
+        slot holds no dof (the library's per-cell dof map)."""
+        out = np.empty(self.mesh.ncells * self.nl_slots, dtype=np.int32)
+        L.check(L.lib().bh_space_dofs(self.handle, out.ctypes.data), self.mesh.backend.ctx,
+                "bh_space_dofs")
+        return out.reshape(self.mesh.ncells, -1)
+
+    @property
+    def nl_slots(self) -> int:
+        n, nl = C.c_int64(), C.c_int32()
+        L.check(L.lib().bh_space_info(self.handle, C.byref(n), C.byref(nl)),
+                self.mesh.backend.ctx, "bh_space_info")
+        return nl.value
+
+    def edge_orders(self):
+        """Space of per-cell order: (ncells, 4) order of each local edge of
+        lower order than its cell (0 where the edge has the cell's order), and
+        the number of cells with such an edge."""
+        eo = np.zeros(self.mesh.ncells * 4, dtype=np.uint8)
+        nt = C.c_int64()
+        L.check(L.lib().bh_space_edge_orders(self.handle, eo.ctypes.data, C.byref(nt)),
+                self.mesh.backend.ctx, "bh_space_edge_orders")
+        return eo.reshape(-1, 4), nt.value
+
     def get_dofs(self, icell: int) -> np.ndarray:
+        if self.dhl is None:
+            d = self.cell_slots()[icell]
+            return d[d >= 0].astype(np.int64) + 1
         return self.dhl.get_dof(icell)
 
     @property

@@ -328,15 +328,27 @@ class BoundaryFaceDomain:
         return self._pairs
 
 
-class Measure:
-    """Measure(domain, degree): Gauss-Legendre of that degree (measure.jl:30-55)."""
+class OrderDegree:
+    """The Measure degree of a space of per-cell order: 2p on the cells of
+    order p, at least `minimum` (no counterpart in the reference).  The
+    library receives it as the degree -minimum."""
 
-    def __init__(self, domain, degree: int):
+    def __init__(self, minimum: int = 1):
+        if int(minimum) < 1:
+            raise ValueError("minimum degree must be >= 1")
+        self.minimum = int(minimum)
+
+
+class Measure:
+    """Measure(domain, degree): Gauss-Legendre of that degree (measure.jl:30-55).
+    `degree` may be an OrderDegree on a space of per-cell order."""
+
+    def __init__(self, domain, degree):
         if not isinstance(domain, (CellDomain, InteriorFaceDomain, BoundaryFaceDomain)):
             raise NotImplementedError(
                 "only CellDomain, InteriorFaceDomain and BoundaryFaceDomain measures are supported")
         self.domain = domain
-        self.degree = int(degree)
+        self.degree = -degree.minimum if isinstance(degree, OrderDegree) else int(degree)
 
 
 class Integral:

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "bcubehip", "libbcubehip.so")
 BUILD = os.path.join(HERE, "_build")
 SOURCES = ["bh_api.hip", "bh_kernels.hip", "bh_mfma.hip", "bh_gp.hip", "bh_dg.hip", "bh_build.hip", "bh_sf.hip",
-           "bh_dgb.hip", "bh_euler.hip", "bh_source.hip", "bh_host.cpp"]
+           "bh_dgb.hip", "bh_euler.hip", "bh_source.hip", "bh_orders.hip", "bh_host.cpp"]
 HEADERS = ["bh_internal.h", "bh_kernels.h", "bh_device.h", "bh_dg.h", os.path.join("..", "..", "include", "bcubehip.h")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BH_OFFLOAD_ARCH", "gfx950")

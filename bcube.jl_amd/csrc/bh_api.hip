@@ -95,6 +95,17 @@ struct bh_space_s {
   float* d_be = nullptr;
   int32_t* d_vptr = nullptr;
   int32_t* d_vpair = nullptr;
+  // spaces of per-cell order (bh_space_create_orders): each cell's order, the
+  // order of its local edges of lower order (eord[c*4 + e], 0 free), the cells
+  // by (cell type, order) on the device, and the cells with an edge of lower
+  // order with their constraint descriptors (launch_orders_constrain)
+  std::vector<int32_t> orders;
+  std::vector<uint8_t> eord;
+  std::vector<GeoBucket> obuckets;
+  std::vector<int> obucket_order;
+  int64_t ntrans = 0;
+  int32_t* d_tcells = nullptr;
+  uint32_t* d_tdesc = nullptr;
 };
 
 // Launch shapes chosen by the autotuner for one pattern and mode (0: default).
@@ -818,6 +829,97 @@ int bh_space_create_from_dofs(bh_mesh m, int order, int continuous, int64_t ndof
   return make_space(m, order, continuous != 0, ndofs, ig, off, out);
 }
 
+int bh_space_create_orders(bh_mesh m, const int32_t* cell_order, bh_space* out) {
+  if (!m || !cell_order || !out) return BH_ERR_ARG;
+  *out = nullptr;
+  bh_ctx c = m->ctx;
+  const int64_t nc = m->h.ncells;
+  int nl = 0, pmax = 0;
+  for (int64_t k = 0; k < nc; ++k) {
+    const int ct = m->h.ctype[k], p = cell_order[k];
+    if (ct != BH_QUAD4 && ct != BH_TRI3)
+      return fail(c, BH_ERR_UNSUPPORTED, "per-cell orders need Quad4 / Tri3 cells");
+    if (p < 1 || p > 4) return fail(c, BH_ERR_ARG, "cell orders must be 1..4");
+    nl = std::max(nl, ndofs_lagrange(shape_of(ct), p));
+    pmax = std::max(pmax, p);
+  }
+  auto s = std::make_unique<bh_space_s>();
+  std::string err;
+  const int64_t n = dofhandler_orders(m->h, cell_order, nl, s->dofs, s->eord, err);
+  if (n < 0) return fail(c, BH_ERR_UNSUPPORTED, err);
+  s->mesh = m;
+  s->dev = m->dev;
+  s->order = pmax;
+  s->continuous = true;
+  s->nl = nl;
+  s->ndofs = n;
+  s->orders.assign(cell_order, cell_order + nc);
+  if (c) {
+    if (int r = set_device(c)) return r;
+    BH_HIP(c, dalloc(&s->d_dofs, s->dofs.size()));
+    BH_HIP(c, hipMemcpy(s->d_dofs, s->dofs.data(), s->dofs.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice));
+    // buckets of (cell type, order), cells ascending in each
+    for (int ct : {BH_QUAD4, BH_TRI3})
+      for (int p = 1; p <= 4; ++p) {
+        std::vector<int32_t> cells, c2n;
+        const int npc = ct == BH_TRI3 ? 3 : 4;
+        for (int64_t k = 0; k < nc; ++k)
+          if (m->h.ctype[k] == ct && cell_order[k] == p) {
+            cells.push_back((int32_t)k);
+            for (int v = 0; v < npc; ++v) c2n.push_back((int32_t)m->h.c2n[m->h.c2n_ptr[k] + v]);
+          }
+        if (cells.empty()) continue;
+        GeoBucket gb;
+        gb.geo = ct;
+        gb.n = (int64_t)cells.size();
+        s->obuckets.push_back(gb);
+        s->obucket_order.push_back(p);
+        GeoBucket& b = s->obuckets.back();
+        BH_HIP(c, dalloc(&b.d_cells, cells.size()));
+        BH_HIP(c, hipMemcpy(b.d_cells, cells.data(), cells.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice));
+        BH_HIP(c, dalloc(&b.d_c2n, c2n.size()));
+        BH_HIP(c, hipMemcpy(b.d_c2n, c2n.data(), c2n.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice));
+      }
+    std::vector<int32_t> tc;
+    std::vector<uint32_t> td;
+    for (int64_t k = 0; k < nc; ++k) {
+      const uint8_t* eo = &s->eord[k * 4];
+      if (!(eo[0] | eo[1] | eo[2] | eo[3])) continue;
+      uint32_t d = (m->h.ctype[k] == BH_TRI3 ? 1u : 0u) | ((uint32_t)cell_order[k] << 1);
+      for (int e = 0; e < 4; ++e) d |= (uint32_t)eo[e] << (4 + 3 * e);
+      tc.push_back((int32_t)k);
+      td.push_back(d);
+    }
+    s->ntrans = (int64_t)tc.size();
+    if (!tc.empty()) {
+      BH_HIP(c, dalloc(&s->d_tcells, tc.size()));
+      BH_HIP(c, hipMemcpy(s->d_tcells, tc.data(), tc.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice));
+      BH_HIP(c, dalloc(&s->d_tdesc, td.size()));
+      BH_HIP(c, hipMemcpy(s->d_tdesc, td.data(), td.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice));
+    }
+  }
+  *out = s.release();
+  return BH_OK;
+}
+
+int bh_space_edge_orders(bh_space s, uint8_t* eord, int64_t* ntransition) {
+  if (!s) return BH_ERR_ARG;
+  if (s->orders.empty()) return fail(s->mesh->ctx, BH_ERR_ARG, "not a space of per-cell order");
+  if (eord) std::copy(s->eord.begin(), s->eord.end(), eord);
+  if (ntransition) {
+    int64_t n = 0;
+    for (size_t k = 0; k < s->eord.size(); k += 4)
+      n += (s->eord[k] | s->eord[k + 1] | s->eord[k + 2] | s->eord[k + 3]) ? 1 : 0;
+    *ntransition = n;
+  }
+  return BH_OK;
+}
+
 int bh_space_info(bh_space s, int64_t* ndofs, int32_t* nl) {
   if (!s) return BH_ERR_ARG;
   if (ndofs) *ndofs = s->ndofs;
@@ -843,6 +945,12 @@ int bh_space_destroy(bh_space s) {
     (void)hipFree(s->d_be);
     (void)hipFree(s->d_vptr);
     (void)hipFree(s->d_vpair);
+    for (GeoBucket& b : s->obuckets) {
+      (void)hipFree(b.d_cells);
+      (void)hipFree(b.d_c2n);
+    }
+    (void)hipFree(s->d_tcells);
+    (void)hipFree(s->d_tdesc);
   }
   delete s;
   return BH_OK;
@@ -1334,6 +1442,8 @@ int bh_dgb_create(bh_space V, int quad_degree, int nvol, const bh_term* vol, int
   *out = nullptr;
   bh_mesh m = V->mesh;
   bh_ctx c = m->ctx;
+  if (!V->orders.empty())
+    return fail(c, BH_ERR_UNSUPPORTED, "face forms: not on a space of per-cell order");
   if (!c) return fail(c, BH_ERR_ARG, "face terms need a mesh created with a context");
   if (m->geo != BH_QUAD4) return fail(c, BH_ERR_UNSUPPORTED, "face terms need a uniform Quad4 mesh");
   const int nl = V->nl;
@@ -1791,6 +1901,8 @@ int bh_euler_create(bh_space V, int quad_degree, double gamma, int64_t nperiodic
   *out = nullptr;
   bh_mesh m = V->mesh;
   bh_ctx c = m->ctx;
+  if (!V->orders.empty())
+    return fail(c, BH_ERR_UNSUPPORTED, "the Euler DG residual: not on a space of per-cell order");
   if (!c) return fail(c, BH_ERR_ARG, "the Euler residual needs a mesh created with a context");
   const HostMesh& hm = m->h;
   const int64_t nc = hm.ncells, ns = V->ndofs, nlV = V->nl;
@@ -2232,6 +2344,26 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
     return fail(c, BH_ERR_UNSUPPORTED,
                 "bilinear kernels need Quad4, Quad9, Tri3 (or a mix of Quad4 and Tri3), Hexa8 or "
                 "Hexa27 cells");
+  // element buckets: the mesh's cell types at the space's order, or on a space
+  // of per-cell order its (cell type, order) buckets, each with its Measure
+  // degree (quad_degree > 0: that degree; < 0: max(2p, -quad_degree))
+  const bool orders = !V->orders.empty();
+  struct EBucket {
+    const GeoBucket* g;
+    int order, degree;
+  };
+  std::vector<EBucket> ebs;
+  if (orders) {
+    if (quad_degree == 0) return fail(c, BH_ERR_ARG, "quadrature degree 0");
+    for (size_t k = 0; k < V->obuckets.size(); ++k) {
+      const int p = V->obucket_order[k];
+      ebs.push_back({&V->obuckets[k], p, quad_degree > 0 ? quad_degree : std::max(2 * p, -quad_degree)});
+    }
+  } else {
+    if (quad_degree < 0)
+      return fail(c, BH_ERR_ARG, "a per-order quadrature degree needs a space of per-cell order");
+    for (const GeoBucket& gb : m->buckets) ebs.push_back({&gb, V->order, quad_degree});
+  }
   const bh_term* stiff = nullptr;
   const bh_term* mass = nullptr;
   for (int k = 0; k < nterms; ++k) {
@@ -2243,11 +2375,11 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
     else if (t.form == BH_FORM_MASS && !mass) mass = &t;
     else return fail(c, BH_ERR_UNSUPPORTED, "unsupported or repeated bilinear term");
   }
-  for (const GeoBucket& gb : m->buckets) {
-    const int key = rule_key(gb.geo, quad_degree);
-    if (!kernel_supported(gb.geo, V->order, key))
+  for (const EBucket& eb : ebs) {
+    const int key = rule_key(eb.g->geo, eb.degree);
+    if (!kernel_supported(eb.g->geo, eb.order, key))
       return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this (cell type, order, quadrature)");
-    if (mode != BH_FP64 && !mfma_supported(gb.geo, V->order, key))
+    if (mode != BH_FP64 && !mfma_supported(eb.g->geo, eb.order, key))
       return fail(c, BH_ERR_UNSUPPORTED,
                   "no FP32 matrix-core kernel for this (cell type, order, quadrature)");
   }
@@ -2265,7 +2397,7 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
   {
     Tuning& tu0 = p->tune[mode == BH_FP64 ? 0 : (mode == BH_FP32 ? 1 : 2)];
     const int key = rule_key(m->buckets[0].geo, quad_degree);
-    const bool want = m->buckets.size() == 1 && sf_supported(m->geo, V->order, key) &&
+    const bool want = !orders && m->buckets.size() == 1 && sf_supported(m->geo, V->order, key) &&
                       ((mode == BH_FP64 && (tu0.elem_waves == 0 || tu0.elem_waves == -2)) ||
                        (mode == BH_FP32 && tu0.elem_waves == -2));
     if (want && !noalloc && !c->capturing)
@@ -2374,10 +2506,12 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
   //    ke[(cell * nl_max + i) * ke_stride(nl_max) ...].
   bool any12 = false;
   auto elements = [&](int waves, bool timed) -> int {
-    for (const GeoBucket& gb : m->buckets) {
-      const int key = rule_key(gb.geo, quad_degree);
+    for (const EBucket& eb : ebs) {
+      const GeoBucket& gb = *eb.g;
+      const int ord = eb.order, deg = eb.degree;
+      const int key = rule_key(gb.geo, deg);
       double* tab = nullptr;
-      if (int r = get_tables(c, gb.geo, V->order, quad_degree, &tab, !noalloc)) return r;
+      if (int r = get_tables(c, gb.geo, ord, deg, &tab, !noalloc)) return r;
       LaunchArgs e = A;
       e.kind = LaunchArgs::BILINEAR;
       e.c2n = gb.d_c2n;
@@ -2386,19 +2520,25 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
       e.ncells_total = gb.n;
       e.spare_cell = nc;
       e.elem_waves = waves;
-      const bool gp = mode == BH_FP32 && gp_supported(gb.geo, V->order, key);
+      const bool gp = mode == BH_FP32 && gp_supported(gb.geo, ord, key);
       if (gp) {
         float* pt = nullptr;
-        if (int r = get_gp_table(c, gb.geo, V->order, quad_degree, A.mass, &pt, !noalloc))
-          return r;
+        if (int r = get_gp_table(c, gb.geo, ord, deg, A.mass, &pt, !noalloc)) return r;
         e.ptab = pt;
         int tg[2];
-        any12 = any12 || gp_wave_targets(gb.geo, V->order, key, tg) == 2;
+        any12 = any12 || gp_wave_targets(gb.geo, ord, key, tg) == 2;
       }
       if (timed) time_begin(c, 0);
-      if (gp) BH_HIP(c, launch_gp(gb.geo, V->order, key, e));
-      else if (mfma_supported(gb.geo, V->order, key)) BH_HIP(c, launch_mfma(gb.geo, V->order, key, e));
-      else BH_HIP(c, launch(gb.geo, V->order, key, e));
+      if (gp) BH_HIP(c, launch_gp(gb.geo, ord, key, e));
+      else if (mfma_supported(gb.geo, ord, key)) BH_HIP(c, launch_mfma(gb.geo, ord, key, e));
+      else BH_HIP(c, launch(gb.geo, ord, key, e));
+      if (timed) time_end(c);
+    }
+    // cells with an edge of lower order: slot matrices Cᵀ K C in place
+    if (orders && V->ntrans > 0) {
+      if (timed) time_begin(c, 4);
+      BH_HIP(c, launch_orders_constrain(c->stream, mode == BH_FP64, V->d_tcells, V->d_tdesc,
+                                        V->ntrans, V->nl, A.ke));
       if (timed) time_end(c);
     }
     return BH_OK;
@@ -2430,7 +2570,7 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
   //    by the row gather over the spill list
   Tuning& tu = p->tune[mode == BH_FP64 ? 0 : (mode == BH_FP32 ? 1 : 2)];
   const int key0 = rule_key(m->buckets[0].geo, quad_degree);
-  bool can_fuse = mode == BH_FP32 && m->geo >= 0 && m->buckets.size() == 1 &&
+  bool can_fuse = mode == BH_FP32 && !orders && m->geo >= 0 && m->buckets.size() == 1 &&
                   gp_fused_supported(m->geo, V->order, key0) && tu.elem_waves != 8 &&
                   tu.elem_waves != 12;
   // its schedule is built when the autotuner is about to time it or a caller
@@ -2595,6 +2735,8 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
   if (!V || nterms <= 0 || !terms) return BH_ERR_ARG;
   bh_mesh m = V->mesh;
   bh_ctx c = m->ctx;
+  if (!V->orders.empty())
+    return fail(c, BH_ERR_UNSUPPORTED, "linear forms: not on a space of per-cell order");
   if (!c) return fail(c, BH_ERR_ARG, "assembly needs a mesh created with a context");
   if (mode != BH_FP64 && mode != BH_FP32)
     return fail(c, BH_ERR_UNSUPPORTED, "linear forms run in BH_FP64 or BH_FP32");
@@ -2842,6 +2984,8 @@ int bh_dg_create(bh_space V, int quad_degree, const double* velocity, int mode, 
   *out = nullptr;
   bh_mesh m = V->mesh;
   bh_ctx c = m->ctx;
+  if (!V->orders.empty())
+    return fail(c, BH_ERR_UNSUPPORTED, "the DG transport residual: not on a space of per-cell order");
   if (!c) return fail(c, BH_ERR_ARG, "DG transport needs a mesh created with a context");
   if (c->capturing) return fail(c, BH_ERR_ARG, "bh_dg_create cannot be captured in a graph");
   if (mode != BH_FP64 && mode != BH_FP32)

@@ -555,6 +555,119 @@ bool dof_table(int ct, int order, DofTab& t) {
   return true;
 }
 
+// Continuous Lagrange space of per-cell order on a 2D Quad4/Tri3 mesh (no
+// reference counterpart: the reference's DofHandler takes one FunctionSpace,
+// dofhandler.jl:37-42).  An edge carries the lowest order p_e of its cells and
+// p_e - 1 dofs, oriented like the lowest-numbered cell that has it (the
+// reference orients edge dofs by their first cell, dofhandler.jl:239-290).
+// Dofs are numbered by first appearance over cells in mesh order and each
+// cell's local dofs in Lagrange order, which with one order everywhere is the
+// reference's DofHandler + densify.  A cell's own nodes on an edge of lower
+// order are not dofs: of its p_c - 1 slots on that edge, slot k <= p_e - 1
+// holds the edge's dof j = k counted in the cell's direction and the others
+// are -1; eord[c*4 + e] = p_e for such an edge (0 when the edge has the cell's
+// order).  Edges are found by sorting (lower node, higher node) keys, so the
+// cost is one sort of the cell-edge list and two passes over the cells.
+i64 dofhandler_orders(const HostMesh& m, const int32_t* order, int nl, std::vector<int32_t>& dofs,
+                      std::vector<uint8_t>& eord, std::string& err) {
+  const i64 nc = m.ncells;
+  if (m.dim != 2) { err = "per-cell orders need a 2D mesh"; return -1; }
+  if (m.nnodes >= ((i64)1 << 31)) { err = "too many nodes"; return -1; }
+  // local roles per (cell type, order): kind 0 vertex / 1 edge / 2 interior
+  struct Role { int8_t kind[25], ent[25], k[25]; int nd; DofTab t; };
+  std::vector<Role> roles(10);  // [Quad4: 0, Tri3: 1][order]
+  auto role_of = [&](int ct, int p) -> Role& { return roles[(ct == BH_TRI3 ? 5 : 0) + p]; };
+  for (int ct : {BH_QUAD4, BH_TRI3})
+    for (int p = 1; p <= 4; ++p) {
+      Role& r = role_of(ct, p);
+      if (!dof_table(ct, p, r.t)) { err = "no dof table"; return -1; }
+      r.nd = r.t.nd;
+      for (int i = 0; i < r.nd; ++i) { r.kind[i] = 2; r.ent[i] = -1; r.k[i] = 0; }
+      for (int v = 0; v < r.t.nv; ++v) { r.kind[r.t.vdof[v]] = 0; r.ent[r.t.vdof[v]] = (int8_t)v; }
+      for (int e = 0; e < r.t.ne; ++e)
+        for (int k = 0; k < r.t.ned; ++k) {
+          const int i = r.t.edof[e][k];
+          r.kind[i] = 1;
+          r.ent[i] = (int8_t)e;
+          r.k[i] = (int8_t)(k + 1);
+        }
+    }
+  int need = 0;
+  for (i64 c = 0; c < nc; ++c) {
+    const int ct = m.ctype[c], p = order[c];
+    if (ct != BH_QUAD4 && ct != BH_TRI3) { err = "per-cell orders need Quad4 / Tri3 cells"; return -1; }
+    if (p < 1 || p > 4) { err = "cell orders must be 1..4"; return -1; }
+    need = std::max(need, role_of(ct, p).nd);
+  }
+  if (need != nl) { err = "slot count is not the largest cell's dof count"; return -1; }
+  // edges: sorted (key, cell*4 + local edge)
+  std::vector<std::pair<uint64_t, int64_t>> er;
+  er.reserve((size_t)nc * 4);
+  for (i64 c = 0; c < nc; ++c) {
+    const Role& r = role_of(m.ctype[c], order[c]);
+    const i64* nodes = &m.c2n[m.c2n_ptr[c]];
+    for (int e = 0; e < r.t.ne; ++e) {
+      const uint64_t a = (uint64_t)nodes[r.t.enode[e][0]], b = (uint64_t)nodes[r.t.enode[e][1]];
+      er.push_back({a < b ? (a << 32) | b : (b << 32) | a, c * 4 + e});
+    }
+  }
+  std::sort(er.begin(), er.end());
+  std::vector<int32_t> eid((size_t)nc * 4, -1);
+  std::vector<int8_t> pe;
+  std::vector<int32_t> efirst;  // first node of the edge's dof direction
+  for (size_t a = 0; a < er.size();) {
+    size_t b = a;
+    int q = 5;
+    while (b < er.size() && er[b].first == er[a].first) {
+      q = std::min(q, (int)order[er[b].second / 4]);
+      ++b;
+    }
+    const i64 c0 = er[a].second / 4;
+    const int e0 = (int)(er[a].second % 4);
+    const Role& r0 = role_of(m.ctype[c0], order[c0]);
+    efirst.push_back((int32_t)m.c2n[m.c2n_ptr[c0] + r0.t.enode[e0][0]]);
+    for (size_t k = a; k < b; ++k) eid[er[k].second] = (int32_t)pe.size();
+    pe.push_back((int8_t)q);
+    a = b;
+  }
+  std::vector<i64> eoff(pe.size() + 1, 0);
+  for (size_t E = 0; E < pe.size(); ++E) eoff[E + 1] = eoff[E] + pe[E] - 1;
+  std::vector<int32_t> pool(eoff.back(), -1), vid(m.nnodes, -1);
+  dofs.assign((size_t)nc * nl, -1);
+  eord.assign((size_t)nc * 4, 0);
+  i64 next = 0;
+  for (i64 c = 0; c < nc; ++c) {
+    const int p = order[c];
+    const Role& r = role_of(m.ctype[c], p);
+    const i64* nodes = &m.c2n[m.c2n_ptr[c]];
+    for (int e = 0; e < r.t.ne; ++e) {
+      const int q = pe[eid[c * 4 + e]];
+      if (q < p) eord[c * 4 + e] = (uint8_t)q;
+    }
+    int32_t* slot = &dofs[(size_t)c * nl];
+    for (int i = 0; i < r.nd; ++i) {
+      if (r.kind[i] == 0) {
+        int32_t& id = vid[nodes[r.ent[i]]];
+        if (id < 0) id = (int32_t)next++;
+        slot[i] = id;
+      } else if (r.kind[i] == 1) {
+        const int e = r.ent[i], k = r.k[i];
+        const int32_t E = eid[c * 4 + e];
+        const int q = pe[E];
+        if (k > q - 1) continue;
+        const int j = efirst[E] == nodes[r.t.enode[e][0]] ? k : q - k;
+        int32_t& id = pool[eoff[E] + j - 1];
+        if (id < 0) id = (int32_t)next++;
+        slot[i] = id;
+      } else {
+        slot[i] = (int32_t)next++;
+      }
+      if (next >= (i64)INT32_MAX) { err = "too many dofs"; return -1; }
+    }
+  }
+  return next;
+}
+
 i64 dofhandler(const HostMesh& m, int order, bool continuous, std::vector<i64>& iglob,
                std::vector<i64>& offset, std::string& err) {
   i64 nc = m.ncells;

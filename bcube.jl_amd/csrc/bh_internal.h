@@ -53,6 +53,14 @@ struct HostMesh {
   int celltype = -1;
 };
 
+// Continuous Lagrange space of per-cell order (1..4) on a 2D Quad4/Tri3 mesh,
+// conforming by the minimum rule on edges: dofs ncells * nl (nl = the largest
+// cell's Lagrange dof count; 0-based, -1 on unused slots), eord ncells * 4 (the
+// order of each local edge of lower order than its cell, else 0).  Returns
+// ndofs or -1 with err set.
+i64 dofhandler_orders(const HostMesh& m, const int32_t* order, int nl, std::vector<int32_t>& dofs,
+                      std::vector<uint8_t>& eord, std::string& err);
+
 // DofHandler numbering (dofhandler.jl:37-156, densify utils.jl:7-16).
 // iglob is 1-based like the reference, offsets per cell.
 i64 dofhandler(const HostMesh& m, int order, bool continuous, std::vector<i64>& iglob,

@@ -241,4 +241,11 @@ struct SfArgs {
 bool sf_supported(int geo_celltype, int order, int nq1);
 hipError_t launch_sf(const SfArgs& a);
 
+// Continuous spaces of per-cell order (bh_orders.hip): the element matrices of
+// the n listed cells (global ids; desc: bit 0 triangle, bits 1-3 order p,
+// bits 4+3e.. the order of each local edge of lower order, 0 free) in ke rows
+// of erow (<= 25) entries become their slot matrices Cᵀ K C, in place.
+hipError_t launch_orders_constrain(hipStream_t s, bool fp64, const int32_t* cells,
+                                   const uint32_t* desc, int64_t n, int erow, void* ke);
+
 }  // namespace bh

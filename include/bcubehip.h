@@ -136,6 +136,25 @@ int bh_space_create(bh_mesh mesh, int order, int continuous, bh_space* out);
  * get_dofs at dofhandler.jl:496-498): iglob 1-based, offset per cell. */
 int bh_space_create_from_dofs(bh_mesh mesh, int order, int continuous, int64_t ndofs,
                               const int64_t* iglob, const int64_t* offset, bh_space* out);
+/* Continuous Lagrange space whose order (1..4) is constant per cell, on a 2D
+ * Quad4 / Tri3 mesh (no counterpart in the reference, whose DofHandler takes
+ * one FunctionSpace, src/fespace/dofhandler.jl:37-42; it extends that
+ * numbering).  Conforming by the minimum rule: an edge carries the lowest
+ * order p_e of its cells and p_e - 1 dofs, oriented like its lowest-numbered
+ * cell; a cell's own nodes on an edge of lower order are not dofs but the
+ * edge's order-p_e interpolant of its dofs.  Numbering: first appearance over
+ * cells in mesh order and each cell's local dofs in Lagrange order (with one
+ * order everywhere: bh_dofhandler's numbering).  The per-cell dof map keeps
+ * each cell's Lagrange slots, ndofs_per_cell = the largest cell's count: -1
+ * on the unused slots of a lower-order edge and past a cell's own count.
+ * bh_assemble_bilinear takes it (cell forms: stiffness and mass), with
+ * quad_degree > 0 one Measure degree for every cell or quad_degree < 0 the
+ * degree max(2p, -quad_degree) for the cells of order p. */
+int bh_space_create_orders(bh_mesh mesh, const int32_t* cell_order, bh_space* out);
+/* Of a space of per-cell order: eord (ncells*4, may be NULL) the order of each
+ * local edge of lower order than its cell, 0 where the edge has the cell's
+ * order; ntransition (may be NULL) the number of cells with such an edge. */
+int bh_space_edge_orders(bh_space space, uint8_t* eord, int64_t* ntransition);
 int bh_space_info(bh_space space, int64_t* ndofs, int32_t* ndofs_per_cell);
 /* Per-cell dof map, 0-based, ncells*ndofs_per_cell int32 (host copy). */
 int bh_space_dofs(bh_space space, int32_t* dofs);

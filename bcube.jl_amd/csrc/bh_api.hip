@@ -2505,7 +2505,43 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
   //    exists (FP64, BF16x3), else the FP64 vector-ALU kernel.  Rows go to
   //    ke[(cell * nl_max + i) * ke_stride(nl_max) ...].
   bool any12 = false;
+  // FP32 on a 2D space of per-cell order whose buckets all have a slot of the
+  // multi-bucket kernel (Measure degrees max(2p, 4), rows of 32 floats): every
+  // bucket's element matrices in one launch.  Shape 12 keeps one launch per
+  // bucket (the same values bit for bit).
+  bool multi = orders && mode == BH_FP32 && ke_stride(V->nl) == 32;
+  for (const EBucket& eb : ebs)
+    multi = multi && gp_orders_slot(eb.g->geo, eb.order, rule_key(eb.g->geo, eb.degree)) >= 0 &&
+            gp_supported(eb.g->geo, eb.order, rule_key(eb.g->geo, eb.degree));
   auto elements = [&](int waves, bool timed) -> int {
+    if (multi && waves != 12) {
+      GPOrdersArgs O;
+      O.coords = m->d_coords;
+      O.spare = (int)nc;
+      O.erow = V->nl;
+      O.eta = A.eta;
+      O.eta_scale = A.eta_scale;
+      O.mcoef = A.mcoef;
+      O.mass_scale = A.mass_scale;
+      O.ke = (float*)A.ke;
+      for (const EBucket& eb : ebs) {
+        const int key = rule_key(eb.g->geo, eb.degree);
+        const int sl = gp_orders_slot(eb.g->geo, eb.order, key);
+        double* tab = nullptr;
+        if (int r = get_tables(c, eb.g->geo, eb.order, eb.degree, &tab, !noalloc)) return r;
+        float* pt = nullptr;
+        if (int r = get_gp_table(c, eb.g->geo, eb.order, eb.degree, A.mass, &pt, !noalloc))
+          return r;
+        O.c2n[sl] = eb.g->d_c2n;
+        O.cells[sl] = eb.g->d_cells;
+        O.ncells[sl] = (int)eb.g->n;
+        O.tab[sl] = tab;
+        O.ptab[sl] = pt;
+      }
+      if (timed) time_begin(c, 0);
+      BH_HIP(c, launch_gp_orders(O, A.mass, c->stream));
+      if (timed) time_end(c);
+    } else {
     for (const EBucket& eb : ebs) {
       const GeoBucket& gb = *eb.g;
       const int ord = eb.order, deg = eb.degree;
@@ -2533,6 +2569,7 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
       else if (mfma_supported(gb.geo, ord, key)) BH_HIP(c, launch_mfma(gb.geo, ord, key, e));
       else BH_HIP(c, launch(gb.geo, ord, key, e));
       if (timed) time_end(c);
+    }
     }
     // cells with an edge of lower order: slot matrices Cᵀ K C in place
     if (orders && V->ntrans > 0) {

@@ -31,7 +31,7 @@ namespace bh {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-template <int D, int NL, int NQ, int NG, bool MASS, int KSR, int WT = 8>
+template <int D, int NL, int NQ, int NG, bool MASS, int KSR, int WT = 8, int NTHX = 0>
 struct GPShape {
   static constexpr int CPB = 16;                        // cells per tile (MFMA M)
   static constexpr int NS = D * (D + 1) / 2;            // entries of G
@@ -42,7 +42,8 @@ struct GPShape {
   static constexpr int NT = gp_ntiles(NL);              // 16-wide output tiles
   static constexpr int TPW = (NT + WT - 1) / WT;        // tiles per wave (about WT waves)
   static constexpr int W = (NT + TPW - 1) / TPW;        // waves per block
-  static constexpr int NTH = W * 64;
+  // NTHX: a larger block (the multi-bucket kernel); waves past W hold no tiles
+  static constexpr int NTH = NTHX > 0 ? NTHX : W * 64;
   static constexpr int GS = 4 * KG + 4;                 // LDS row stride of Gf (floats)
   static constexpr int KS = KSR;                        // ke row stride (floats)
   static constexpr int KSL = 32;                        // staged row stride in LDS (floats)
@@ -50,6 +51,7 @@ struct GPShape {
   static constexpr int NOUT = (CPB * NL * (KS / 4) + NTH - 1) / NTH;  // float4 row stores per thread
   static_assert(KT <= 4 * KG, "k groups cover the contraction");
   static_assert(CPB * NG <= NTH, "one thread per prefetched node");
+  static_assert(NTHX == 0 || NTHX >= W * 64, "the block holds the waves of every tile");
   static_assert(KS % 4 == 0 && KS <= KSL, "float4 rows inside the swizzled 32-float LDS rows");
 };
 
@@ -65,32 +67,47 @@ __device__ __forceinline__ int gp_opaque(int v) {
 // fall into different LDS banks while rows are still read as float4.
 __device__ __forceinline__ int gp_swz(int row, int j) { return j ^ ((row & 7) << 2); }
 
-template <int D, int NL, int NQ, int NG, bool MASS, int KSR, bool IND, int WT>
-__global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS, KSR, WT>::NTH)) void k_element_gp(
+// LDS of one block of the GEMM element kernel.
+template <int D, int NL, int NQ, int NG, bool MASS, int KSR, int WT, int NTHX>
+struct GPSmem {
+  using S = GPShape<D, NL, NQ, NG, MASS, KSR, WT, NTHX>;
+  double dg[NQ * NG * D];
+  double w[NQ];
+  // nodes and per-cell coefficients; the last slot takes the stores of
+  // threads with nothing to stage, so staging needs no branch
+  double x[S::CPB * NG + 1][D];
+  double c[S::CPB + 1][2];  // η and mass coefficient of each cell
+  int gc[4][S::CPB + 1];    // global cell ids of the groups in flight (-1: none)
+  __attribute__((aligned(16))) float G[2][S::CPB * S::GS];
+  __attribute__((aligned(16))) float K[S::CPB * NL * S::KSL];
+};
+
+// The element kernel's work for block `bid` of `nblk` blocks on one cell
+// bucket (k_element_gp: all blocks of the launch; k_element_gp_orders: the
+// blocks of its bucket).  cells[k]: global id of the k-th cell of the bucket
+// (IND; else k itself), c2n in bucket order; coefficients are per global cell
+// and element rows go to ke[(gcell*erow + i)*KSR + j] for i < NL, whole
+// KSR-float rows.  The missing cells of a last, partial group write rows of
+// the spare cells spare .. spare+15, which ke must hold.
+template <int D, int NL, int NQ, int NG, bool MASS, int KSR, bool IND, int WT, int NTHX>
+__device__ __forceinline__ void gp_body(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
     const double* __restrict__ tab, const float* __restrict__ ptab, int ncells,
     const int32_t* __restrict__ cells, int spare, int erow, const double* __restrict__ eta,
     double eta_scale, const double* __restrict__ mcoef, double mass_scale,
-    float* __restrict__ ke) {
-  // cells[k]: global id of the k-th cell of this launch (IND; else k itself),
-  // c2n in launch order; coefficients are per global cell and element rows go to
-  // ke[(gcell*erow + i)*KSR + j] for i < NL, whole KSR-float rows.  The
-  // missing cells of a last, partial group write rows of the spare cells
-  // spare .. spare+15, which ke must hold.
-  using S = GPShape<D, NL, NQ, NG, MASS, KSR, WT>;
+    float* __restrict__ ke, GPSmem<D, NL, NQ, NG, MASS, KSR, WT, NTHX>& sm, int bid, int nblk) {
+  using S = GPShape<D, NL, NQ, NG, MASS, KSR, WT, NTHX>;
   using L = TabLayout<D, NL, NQ, NG>;
   constexpr int CPB = S::CPB, KG = S::KG, TPW = S::TPW, NTH = S::NTH, GS = S::GS;
   constexpr int NC = S::NC, NS = S::NS, NT = S::NT, NP = S::NP, KS = S::KS, KSL = S::KSL;
 
-  __shared__ double s_dg[NQ * NG * D];
-  __shared__ double s_w[NQ];
-  // nodes and per-cell coefficients; the last slot takes the stores of
-  // threads with nothing to stage, so staging needs no branch
-  __shared__ double s_x[CPB * NG + 1][D];
-  __shared__ double s_c[CPB + 1][2];  // η and mass coefficient of each cell
-  __shared__ int s_gc[4][CPB + 1];    // global cell ids of the groups in flight (-1: none)
-  __shared__ __attribute__((aligned(16))) float s_G[2][CPB * GS];
-  __shared__ __attribute__((aligned(16))) float s_K[CPB * NL * KSL];
+  double* __restrict__ s_dg = sm.dg;
+  double* __restrict__ s_w = sm.w;
+  double(*__restrict__ s_x)[D] = sm.x;
+  double(*__restrict__ s_c)[2] = sm.c;
+  int(*__restrict__ s_gc)[CPB + 1] = sm.gc;
+  float(*__restrict__ s_G)[CPB * GS] = sm.G;
+  float* __restrict__ s_K = sm.K;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -128,12 +145,13 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS, KSR, WT>::NTH)) void 
   // Then it mirrors group t into s_K and forms the factors of group t+1 in
   // s_G[(t+1)&1].  Node coordinates are loaded two groups ahead, ids three.
   const int ngroups = (ncells + CPB - 1) / CPB;
-  const int G = (int)gridDim.x;
-  const int T = ((int)blockIdx.x < ngroups) ? (ngroups - 1 - (int)blockIdx.x) / G + 1 : 0;
+  const int G = nblk;
+  const int T = (bid < ngroups) ? (ngroups - 1 - bid) / G + 1 : 0;
   const int pc = tid / NG, pn = tid - (tid / NG) * NG;
   const bool loader = tid < CPB * NG;
+  const bool mma = wave * TPW < NT;  // waves past the last tile skip the contraction
   if (T == 0) return;
-  auto group = [&](int t) { return (int)blockIdx.x + t * G; };
+  auto group = [&](int t) { return bid + t * G; };
   auto node_of = [&](int t) -> int {
     const int cell = group(t) * CPB + pc;
     return (loader && t < T && cell < ncells) ? c2n[(int64_t)cell * NG + pn] : -1;
@@ -248,15 +266,17 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS, KSR, WT>::NTH)) void 
 #pragma unroll
     for (int tt = 0; tt < TPW; ++tt) acc[tt] = floatx4{0.f, 0.f, 0.f, 0.f};
     const float* ga = &s_G[t & 1][ln * GS + lg * KG];
+    if (mma) {
 #pragma unroll
-    for (int s = 0; s < KG; s += 4) {
-      const floatx4 a = *(const floatx4*)(ga + s);
+      for (int s = 0; s < KG; s += 4) {
+        const floatx4 a = *(const floatx4*)(ga + s);
 #pragma unroll
-      for (int tt = 0; tt < TPW; ++tt) {
-        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], bp[tt][s + 0], acc[tt], 0, 0, 0);
-        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], bp[tt][s + 1], acc[tt], 0, 0, 0);
-        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], bp[tt][s + 2], acc[tt], 0, 0, 0);
-        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], bp[tt][s + 3], acc[tt], 0, 0, 0);
+        for (int tt = 0; tt < TPW; ++tt) {
+          acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], bp[tt][s + 0], acc[tt], 0, 0, 0);
+          acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], bp[tt][s + 1], acc[tt], 0, 0, 0);
+          acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], bp[tt][s + 2], acc[tt], 0, 0, 0);
+          acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], bp[tt][s + 3], acc[tt], 0, 0, 0);
+        }
       }
     }
     __syncthreads();
@@ -281,6 +301,103 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS, KSR, WT>::NTH)) void 
     geometry(s_G[(t + 1) & 1]);
     __syncthreads();
   }
+}
+
+template <int D, int NL, int NQ, int NG, bool MASS, int KSR, bool IND, int WT>
+__global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS, KSR, WT>::NTH)) void k_element_gp(
+    const double* __restrict__ coords, const int32_t* __restrict__ c2n,
+    const double* __restrict__ tab, const float* __restrict__ ptab, int ncells,
+    const int32_t* __restrict__ cells, int spare, int erow, const double* __restrict__ eta,
+    double eta_scale, const double* __restrict__ mcoef, double mass_scale,
+    float* __restrict__ ke) {
+  __shared__ GPSmem<D, NL, NQ, NG, MASS, KSR, WT, 0> sm;
+  gp_body<D, NL, NQ, NG, MASS, KSR, IND, WT, 0>(coords, c2n, tab, ptab, ncells, cells, spare,
+                                                erow, eta, eta_scale, mcoef, mass_scale, ke, sm,
+                                                (int)blockIdx.x, (int)gridDim.x);
+}
+
+// ---------------------------------------------------------------------------
+// All (cell type, order) buckets of a 2D space of per-cell order in one
+// persistent launch: the eight Quad4 / Tri3 Q1..Q4 / P1..P4 elements with the
+// Measure degrees max(2p, 4) (Gauss 3, 3, 4, 5 points per axis on quads;
+// triangle rules of 6, 6, 12, 16 points), rows of 32 floats (a space with Q4
+// cells).  Blocks [bstart[b], bstart[b+1]) run bucket b as k_element_gp runs
+// it (the same products in the same order: bitwise the per-bucket launches);
+// the block takes the largest element's thread count and LDS, and waves past
+// an element's tiles skip the contraction.
+// ---------------------------------------------------------------------------
+constexpr int kOrdSlots = 8;
+constexpr int kOrdKS = 32;
+template <int S, bool MASS>
+struct OrdSlot;  // slot -> element: Quad4 p = S + 1 (S < 4), Tri3 p = S - 3
+#define BH_OS(S, NL, NQ, NG)                                              \
+  template <bool MASS>                                                   \
+  struct OrdSlot<S, MASS> {                                              \
+    static constexpr int nl = NL, nq = NQ, ng = NG;                     \
+    static constexpr int nth = GPShape<2, NL, NQ, NG, MASS, kOrdKS, 8>::W * 64; \
+  };
+BH_OS(0, 4, 9, 4)
+BH_OS(1, 9, 9, 4)
+BH_OS(2, 16, 16, 4)
+BH_OS(3, 25, 25, 4)
+BH_OS(4, 3, 6, 3)
+BH_OS(5, 6, 6, 3)
+BH_OS(6, 10, 12, 3)
+BH_OS(7, 15, 16, 3)
+#undef BH_OS
+
+template <bool MASS>
+constexpr int ord_nth() {
+  int m = 0;
+  m = OrdSlot<0, MASS>::nth > m ? OrdSlot<0, MASS>::nth : m;
+  m = OrdSlot<1, MASS>::nth > m ? OrdSlot<1, MASS>::nth : m;
+  m = OrdSlot<2, MASS>::nth > m ? OrdSlot<2, MASS>::nth : m;
+  m = OrdSlot<3, MASS>::nth > m ? OrdSlot<3, MASS>::nth : m;
+  m = OrdSlot<4, MASS>::nth > m ? OrdSlot<4, MASS>::nth : m;
+  m = OrdSlot<5, MASS>::nth > m ? OrdSlot<5, MASS>::nth : m;
+  m = OrdSlot<6, MASS>::nth > m ? OrdSlot<6, MASS>::nth : m;
+  m = OrdSlot<7, MASS>::nth > m ? OrdSlot<7, MASS>::nth : m;
+  return m;
+}
+template <int S, bool MASS>
+using OrdSmem = GPSmem<2, OrdSlot<S, MASS>::nl, OrdSlot<S, MASS>::nq, OrdSlot<S, MASS>::ng, MASS,
+                       kOrdKS, 8, ord_nth<MASS>()>;
+template <bool MASS>
+constexpr size_t ord_smem() {
+  size_t m = 0;
+  m = sizeof(OrdSmem<0, MASS>) > m ? sizeof(OrdSmem<0, MASS>) : m;
+  m = sizeof(OrdSmem<1, MASS>) > m ? sizeof(OrdSmem<1, MASS>) : m;
+  m = sizeof(OrdSmem<2, MASS>) > m ? sizeof(OrdSmem<2, MASS>) : m;
+  m = sizeof(OrdSmem<3, MASS>) > m ? sizeof(OrdSmem<3, MASS>) : m;
+  m = sizeof(OrdSmem<4, MASS>) > m ? sizeof(OrdSmem<4, MASS>) : m;
+  m = sizeof(OrdSmem<5, MASS>) > m ? sizeof(OrdSmem<5, MASS>) : m;
+  m = sizeof(OrdSmem<6, MASS>) > m ? sizeof(OrdSmem<6, MASS>) : m;
+  m = sizeof(OrdSmem<7, MASS>) > m ? sizeof(OrdSmem<7, MASS>) : m;
+  return m;
+}
+
+template <int S, bool MASS>
+__device__ __forceinline__ void ord_run(const GPOrdersArgs& A, unsigned char* buf, int bid) {
+  using O = OrdSlot<S, MASS>;
+  gp_body<2, O::nl, O::nq, O::ng, MASS, kOrdKS, true, 8, ord_nth<MASS>()>(
+      A.coords, A.c2n[S], A.tab[S], A.ptab[S], A.ncells[S], A.cells[S], A.spare, A.erow, A.eta,
+      A.eta_scale, A.mcoef, A.mass_scale, A.ke, *reinterpret_cast<OrdSmem<S, MASS>*>(buf),
+      bid - A.bstart[S], A.bstart[S + 1] - A.bstart[S]);
+}
+
+template <bool MASS>
+__global__ __launch_bounds__(ord_nth<MASS>()) void k_element_gp_orders(const GPOrdersArgs A) {
+  __shared__ __attribute__((aligned(16))) unsigned char buf[ord_smem<MASS>()];
+  const int bid = (int)blockIdx.x;
+  // the bucket of this block (a uniform branch: every wave takes the same one)
+  if (bid < A.bstart[1]) ord_run<0, MASS>(A, buf, bid);
+  else if (bid < A.bstart[2]) ord_run<1, MASS>(A, buf, bid);
+  else if (bid < A.bstart[3]) ord_run<2, MASS>(A, buf, bid);
+  else if (bid < A.bstart[4]) ord_run<3, MASS>(A, buf, bid);
+  else if (bid < A.bstart[5]) ord_run<4, MASS>(A, buf, bid);
+  else if (bid < A.bstart[6]) ord_run<5, MASS>(A, buf, bid);
+  else if (bid < A.bstart[7]) ord_run<6, MASS>(A, buf, bid);
+  else if (bid < A.bstart[8]) ord_run<7, MASS>(A, buf, bid);
 }
 
 // ---------------------------------------------------------------------------
@@ -796,6 +913,61 @@ hipError_t launch_gp(int geo, int order, int nq1, const LaunchArgs& A) {
   for (const GPEntry& e : kGP)
     if (e.geo == geo && e.order == order && e.nq1 == nq1) return e.fn(A);
   return hipErrorInvalidValue;
+}
+
+int gp_orders_slot(int geo, int order, int nq1) {
+  if (order < 1 || order > 4) return -1;
+  static const int kq[5] = {0, 3, 3, 4, 5}, kt[5] = {0, 6, 6, 12, 16};
+  if (geo == BH_QUAD4 && nq1 == kq[order]) return order - 1;
+  if (geo == BH_TRI3 && nq1 == kt[order]) return 3 + order;
+  return -1;
+}
+
+hipError_t launch_gp_orders(GPOrdersArgs A, bool mass, hipStream_t stream) {
+  if (ke_stride(A.erow) != kOrdKS) return hipErrorInvalidValue;
+  // one block per CU slot the kernel can hold, split over the buckets in
+  // proportion to their work (16-cell groups x element size), at least one
+  // block per bucket and at most one per group
+  static int resident[2] = {0, 0};
+  int& res = resident[mass ? 1 : 0];
+  if (res == 0) {
+    int dev = 0, ncu = 0, per = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+      e = mass ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_element_gp_orders<true>,
+                                                               ord_nth<true>(), 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_element_gp_orders<false>,
+                                                               ord_nth<false>(), 0);
+    if (e != hipSuccess) return e;
+    res = ncu * (per > 0 ? per : 1);
+  }
+  static const int nl[kOrdSlots] = {4, 9, 16, 25, 3, 6, 10, 15};
+  static const int nq[kOrdSlots] = {9, 9, 16, 25, 6, 6, 12, 16};
+  double w[kOrdSlots], tot = 0.0;
+  int64_t groups[kOrdSlots];
+  for (int b = 0; b < kOrdSlots; ++b) {
+    groups[b] = (A.ncells[b] + 15) / 16;
+    w[b] = (double)groups[b] * nl[b] * (nl[b] + nq[b]);
+    tot += w[b];
+  }
+  if (tot <= 0.0) return hipSuccess;
+  A.bstart[0] = 0;
+  for (int b = 0; b < kOrdSlots; ++b) {
+    int64_t nb = 0;
+    if (groups[b] > 0) {
+      nb = (int64_t)(res * w[b] / tot + 0.5);
+      nb = nb < 1 ? 1 : nb;
+      nb = nb > groups[b] ? groups[b] : nb;
+    }
+    A.bstart[b + 1] = A.bstart[b] + (int)nb;
+  }
+  dim3 grid((unsigned)A.bstart[kOrdSlots]);
+  if (mass)
+    hipLaunchKernelGGL(k_element_gp_orders<true>, grid, dim3(ord_nth<true>()), 0, stream, A);
+  else
+    hipLaunchKernelGGL(k_element_gp_orders<false>, grid, dim3(ord_nth<false>()), 0, stream, A);
+  return hipGetLastError();
 }
 
 }  // namespace bh

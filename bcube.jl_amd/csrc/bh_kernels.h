@@ -167,6 +167,29 @@ constexpr int gp_kgroup(int dim, int nq, bool mass) {
 }
 constexpr int gp_ntiles(int nl) { return (nl * (nl + 1) / 2 + 15) / 16; }
 bool gp_supported(int geo_celltype, int order, int nq1);
+// All (cell type, order) buckets of a 2D space of per-cell order in one launch
+// of the FP32 GEMM element kernel (bh_gp.hip): slots 0..3 Quad4 Q1..Q4 with
+// 3, 3, 4, 5 Gauss points per axis, 4..7 Tri3 P1..P4 with the 6, 6, 12, 16
+// point rules (Measure degrees max(2p, 4)); rows of ke_stride(erow) = 32
+// floats.  gp_orders_slot: the slot of an element, -1 if it has none.  The
+// launcher fills bstart (blocks per bucket) itself.
+struct GPOrdersArgs {
+  const double* coords = nullptr;
+  const int32_t* c2n[8] = {};
+  const int32_t* cells[8] = {};
+  int ncells[8] = {};
+  const double* tab[8] = {};
+  const float* ptab[8] = {};
+  int bstart[9] = {};
+  int spare = 0, erow = 0;
+  const double* eta = nullptr;
+  double eta_scale = 0.0;
+  const double* mcoef = nullptr;
+  double mass_scale = 0.0;
+  float* ke = nullptr;
+};
+int gp_orders_slot(int geo_celltype, int order, int nq1);
+hipError_t launch_gp_orders(GPOrdersArgs a, bool mass, hipStream_t stream);
 // FP32 linear forms (bh_source.hip): element vectors of one cell bucket on the
 // matrix cores into a.ke (float, cell*erow + i), and their sum into b per dof
 // in cell order.

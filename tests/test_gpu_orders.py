@@ -2,6 +2,9 @@
 Quad4/Tri3 meshes, orders 1..4 per block of cells, edges at their cells'
 lowest order): ∫η∇u⋅∇v + ∫uv with Measure degree max(2p, 4) per cell.
 
+FP32 on a space with Q4 cells runs every (cell type, order) bucket's element
+matrices in one launch, bitwise equal to one launch per bucket.
+
 Tolerances: FP64 every entry within 1e-12 * max|A_oracle| of the oracle's
 assembly of the same space, the CSR pattern bit-exact; FP32 (FP64 geometry,
 FP32 element matrices, FP64 sums) within 1e-6 * max|A_oracle|.  The solution
@@ -75,6 +78,14 @@ def test_fp64_and_fp32_vs_oracle(n, seed, block):
     assert (4 in k64) == (ntrans > 0) and (4 in k32) == (ntrans > 0)
     assert np.array_equal(v32, plan.assemble_host(mode="fp32"))
     assert np.array_equal(v64, plan.assemble_host(mode="fp64"))
+    # FP32 with Q4 cells (rows of 32 floats): every bucket's element matrices
+    # in one launch, bitwise the per-bucket launches (tuning shape 12)
+    q4 = bool(np.any((order == 4) & (m.celltypes == 0)))
+    assert (k32.count(0) == 1) == q4, k32
+    plan.set_tuning(12, mode="fp32")
+    v32b, k32b = _kinds(plan, "fp32")
+    assert np.array_equal(v32b, v32)
+    assert k32b.count(0) == len(np.unique(np.stack([order, m.celltypes]), axis=1).T)
 
 
 def test_solution_and_device_call():

@@ -106,6 +106,7 @@ struct bh_space_s {
   int64_t ntrans = 0;
   int32_t* d_tcells = nullptr;
   uint32_t* d_tdesc = nullptr;
+  uint32_t* d_cdesc = nullptr;  // every cell's descriptor (0: no edge of lower order)
 };
 
 // Launch shapes chosen by the autotuner for one pattern and mode (0: default).
@@ -894,6 +895,10 @@ int bh_space_create_orders(bh_mesh m, const int32_t* cell_order, bh_space* out) 
       td.push_back(d);
     }
     s->ntrans = (int64_t)tc.size();
+    std::vector<uint32_t> cd((size_t)nc, 0u);
+    for (size_t k = 0; k < tc.size(); ++k) cd[tc[k]] = td[k];
+    BH_HIP(c, dalloc(&s->d_cdesc, cd.size()));
+    BH_HIP(c, hipMemcpy(s->d_cdesc, cd.data(), cd.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     if (!tc.empty()) {
       BH_HIP(c, dalloc(&s->d_tcells, tc.size()));
       BH_HIP(c, hipMemcpy(s->d_tcells, tc.data(), tc.size() * sizeof(int32_t),
@@ -951,6 +956,7 @@ int bh_space_destroy(bh_space s) {
     }
     (void)hipFree(s->d_tcells);
     (void)hipFree(s->d_tdesc);
+    (void)hipFree(s->d_cdesc);
   }
   delete s;
   return BH_OK;
@@ -2510,6 +2516,9 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
   // bucket's element matrices in one launch.  Shape 12 keeps one launch per
   // bucket (the same values bit for bit).
   bool multi = orders && mode == BH_FP32 && ke_stride(V->nl) == 32;
+  bool gp_all = mode == BH_FP32;
+  for (const EBucket& eb : ebs)
+    gp_all = gp_all && gp_supported(eb.g->geo, eb.order, rule_key(eb.g->geo, eb.degree));
   for (const EBucket& eb : ebs)
     multi = multi && gp_orders_slot(eb.g->geo, eb.order, rule_key(eb.g->geo, eb.degree)) >= 0 &&
             gp_supported(eb.g->geo, eb.order, rule_key(eb.g->geo, eb.degree));
@@ -2523,6 +2532,7 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
       O.eta_scale = A.eta_scale;
       O.mcoef = A.mcoef;
       O.mass_scale = A.mass_scale;
+      O.cdesc = V->d_cdesc;
       O.ke = (float*)A.ke;
       for (const EBucket& eb : ebs) {
         const int key = rule_key(eb.g->geo, eb.degree);
@@ -2561,6 +2571,7 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
         float* pt = nullptr;
         if (int r = get_gp_table(c, gb.geo, ord, deg, A.mass, &pt, !noalloc)) return r;
         e.ptab = pt;
+        e.cdesc = orders ? V->d_cdesc : nullptr;
         int tg[2];
         any12 = any12 || gp_wave_targets(gb.geo, ord, key, tg) == 2;
       }
@@ -2571,8 +2582,9 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
       if (timed) time_end(c);
     }
     }
-    // cells with an edge of lower order: slot matrices Cᵀ K C in place
-    if (orders && V->ntrans > 0) {
+    // cells with an edge of lower order: slot matrices Cᵀ K C in place (the
+    // FP32 GEMM element kernel forms them itself)
+    if (orders && V->ntrans > 0 && !gp_all) {
       if (timed) time_begin(c, 4);
       BH_HIP(c, launch_orders_constrain(c->stream, mode == BH_FP64, V->d_tcells, V->d_tdesc,
                                         V->ntrans, V->nl, A.ke));

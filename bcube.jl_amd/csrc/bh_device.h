@@ -64,4 +64,47 @@ struct SymN {
   static constexpr int N = D * (D + 1) / 2;
 };
 
+// Spaces of per-cell order (2D Quad4 / Tri3, orders 1..4).  A cell's
+// constraint descriptor: bit 0 triangle, bits 1-3 order p, bits 4+3e .. 6+3e
+// the order q < p of local edge e where the edge has a lower order (0: free).
+// Local slot of vertex v / of the k-th node (k = 1..p-1, from the edge's first
+// vertex) of edge e, in the Lagrange dof order of the element kernels (tensor
+// order on quads, vertices then edges from first vertex to second on
+// triangles); edge e runs from vertex e to vertex e + 1 (mod the vertex count).
+__device__ __forceinline__ int ord_vslot(bool tri, int p, int v) {
+  if (tri) return v;
+  const int n = p + 1;
+  return v == 0 ? 0 : v == 1 ? p : v == 2 ? p + n * p : n * p;
+}
+__device__ __forceinline__ int ord_eslot(bool tri, int p, int e, int k) {
+  if (tri) return 3 + e * (p - 1) + (k - 1);
+  const int n = p + 1;
+  switch (e) {
+    case 0: return k;
+    case 1: return p + n * k;
+    case 2: return (p - k) + n * p;
+    default: return n * (p - k);
+  }
+}
+// ℓ_j(k/p) of the order-q Lagrange basis on the nodes 0, 1/q, .., 1: the
+// weight of the edge's dof j in the value at the cell's k-th edge node.
+constexpr double ord_lagrange(int q, int j, int k, int p) {
+  double v = 1.0;
+  for (int i = 0; i <= q; ++i)
+    if (i != j) v *= (q * ((double)k / p) - i) / (double)(j - i);
+  return v;
+}
+struct OrdWeights {
+  float w[5][4][4][4];  // [p][q][k][j], zero outside 1 <= q < p, 1 <= k < p, j <= q
+};
+constexpr OrdWeights make_ord_weights() {
+  OrdWeights r{};
+  for (int p = 2; p <= 4; ++p)
+    for (int q = 1; q < p; ++q)
+      for (int k = 1; k < p; ++k)
+        for (int j = 0; j <= q; ++j) r.w[p][q][k][j] = (float)ord_lagrange(q, j, k, p);
+  return r;
+}
+static __constant__ OrdWeights kOrdWeights = make_ord_weights();
+
 }  // namespace bh

@@ -78,24 +78,36 @@ struct GPSmem {
   double x[S::CPB * NG + 1][D];
   double c[S::CPB + 1][2];  // η and mass coefficient of each cell
   int gc[4][S::CPB + 1];    // global cell ids of the groups in flight (-1: none)
+  uint32_t cd[4][S::CPB + 1];  // their constraint descriptors (spaces of per-cell order)
   __attribute__((aligned(16))) float G[2][S::CPB * S::GS];
   __attribute__((aligned(16))) float K[S::CPB * NL * S::KSL];
 };
 
 // The element kernel's work for block `bid` of `nblk` blocks on one cell
-// bucket (k_element_gp: all blocks of the launch; k_element_gp_orders: the
-// blocks of its bucket).  cells[k]: global id of the k-th cell of the bucket
+// bucket, run by NTH threads (tid: the thread's index among them) --
+// k_element_gp: all blocks of the launch; k_element_gp_orders: the sub-blocks
+// of its bucket.  cells[k]: global id of the k-th cell of the bucket
 // (IND; else k itself), c2n in bucket order; coefficients are per global cell
 // and element rows go to ke[(gcell*erow + i)*KSR + j] for i < NL, whole
 // KSR-float rows.  The missing cells of a last, partial group write rows of
-// the spare cells spare .. spare+15, which ke must hold.
+// the spare cells spare .. spare+15, which ke must hold.  cdesc (a space of
+// per-cell order, IND; else NULL): per global cell its constraint descriptor
+// (ord_vslot in bh_device.h); a cell with an edge of lower order has its
+// element matrix K replaced by Cᵀ K C in LDS before it is written (the slot
+// matrix bh_orders.hip describes), in FP32, rows then columns.  cflag[4]
+// (block-wide LDS, shared by the sub-blocks of a block): per group slot,
+// whether any cell staged there by any sub-block has a constraint, so the
+// block takes the constraint step's barriers only when some group needs it;
+// cw (block-wide LDS, 320 floats): the constraint weights ℓ_j(k/p).
 template <int D, int NL, int NQ, int NG, bool MASS, int KSR, bool IND, int WT, int NTHX>
 __device__ __forceinline__ void gp_body(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
     const double* __restrict__ tab, const float* __restrict__ ptab, int ncells,
     const int32_t* __restrict__ cells, int spare, int erow, const double* __restrict__ eta,
     double eta_scale, const double* __restrict__ mcoef, double mass_scale,
-    float* __restrict__ ke, GPSmem<D, NL, NQ, NG, MASS, KSR, WT, NTHX>& sm, int bid, int nblk) {
+    const uint32_t* __restrict__ cdesc, float* __restrict__ ke,
+    GPSmem<D, NL, NQ, NG, MASS, KSR, WT, NTHX>& sm, int tid, int bid, int nblk, int tfix,
+    uint32_t* cflag, float* cw) {
   using S = GPShape<D, NL, NQ, NG, MASS, KSR, WT, NTHX>;
   using L = TabLayout<D, NL, NQ, NG>;
   constexpr int CPB = S::CPB, KG = S::KG, TPW = S::TPW, NTH = S::NTH, GS = S::GS;
@@ -106,10 +118,10 @@ __device__ __forceinline__ void gp_body(
   double(*__restrict__ s_x)[D] = sm.x;
   double(*__restrict__ s_c)[2] = sm.c;
   int(*__restrict__ s_gc)[CPB + 1] = sm.gc;
+  uint32_t(*__restrict__ s_cd)[CPB + 1] = sm.cd;
   float(*__restrict__ s_G)[CPB * GS] = sm.G;
   float* __restrict__ s_K = sm.K;
 
-  const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int ln = lane & 15;  // MFMA row (A) / column (B, C) of this lane
@@ -146,7 +158,10 @@ __device__ __forceinline__ void gp_body(
   // s_G[(t+1)&1].  Node coordinates are loaded two groups ahead, ids three.
   const int ngroups = (ncells + CPB - 1) / CPB;
   const int G = nblk;
-  const int T = (bid < ngroups) ? (ngroups - 1 - bid) / G + 1 : 0;
+  // tfix > 0: this many steps whatever the block's own count (the sub-blocks
+  // of one multi-bucket block keep to the same barriers; steps past a
+  // sub-block's last group stage no cells and write spare rows)
+  const int T = tfix > 0 ? tfix : (bid < ngroups) ? (ngroups - 1 - bid) / G + 1 : 0;
   const int pc = tid / NG, pn = tid - (tid / NG) * NG;
   const bool loader = tid < CPB * NG;
   const bool mma = wave * TPW < NT;  // waves past the last tile skip the contraction
@@ -166,18 +181,67 @@ __device__ __forceinline__ void gp_body(
     if constexpr (IND) return cells[cell];
     return cell;
   };
-  auto load_c = [&](int gid, double& e, double& mc) {
+  auto load_c = [&](int gid, double& e, double& mc, uint32_t& cd) {
     e = (gid >= 0 && eta) ? eta[gid] * eta_scale : eta_scale;
     mc = (gid >= 0 && mcoef) ? mcoef[gid] * mass_scale : mass_scale;
+    cd = (gid >= 0 && cdesc) ? cdesc[gid] : 0u;
   };
   const int xslot = loader ? tid : CPB * NG;
   const int cslot = tid < CPB ? tid : CPB;
-  auto stage_nodes = [&](int t, const double (&x)[D], double e, double mc, int gid) {
+  auto stage_nodes = [&](int t, const double (&x)[D], double e, double mc, int gid,
+                         uint32_t cd) {
 #pragma unroll
     for (int a = 0; a < D; ++a) s_x[xslot][a] = x[a];
     s_c[cslot][0] = e;
     s_c[cslot][1] = mc;
-    if constexpr (IND) s_gc[t & 3][cslot] = gid;
+    if constexpr (IND) {
+      s_gc[t & 3][cslot] = gid;
+      s_cd[t & 3][cslot] = cd;
+      if (cd != 0u) atomicOr(&cflag[t & 3], 1u);
+    }
+  };
+  // Cᵀ K C on the staged element matrices of the cells of one group with an
+  // edge of lower order.  A wave takes two cells at a time (lanes 0-31 and
+  // 32-63) and applies the rows pass (lane = column x), then, after a
+  // wave-level LDS fence, the columns pass (lane = row x); no other wave
+  // touches those cells' rows.  For an edge of order q: t_j = Σ_k ℓ_j(k/p)
+  // K[s_k], then K[va] += t_0, K[vb] += t_q, K[s_j] = t_j (j < q), K[s_k] = 0.
+  auto constrain = [&](int slot) {
+    constexpr int NW = NTH / 64;
+    for (int c2 = 2 * wave; c2 < CPB; c2 += 2 * NW) {
+      const int c = c2 + (lane >> 5), x = lane & 31;
+      const uint32_t d = s_cd[slot][c];
+      const bool on = d != 0u && x < NL;
+      const bool tri = d & 1u;
+      const int p = (int)((d >> 1) & 7u), ne = tri ? 3 : 4;
+      for (int pass = 0; pass < 2; ++pass) {
+        auto at = [&](int a) -> float& {
+          const int r = c * NL + (pass == 0 ? a : x);
+          return s_K[r * KSL + gp_swz(r, pass == 0 ? x : a)];
+        };
+        for (int e = 0; e < ne && on; ++e) {
+          const int q = (int)((d >> (4 + 3 * e)) & 7u);
+          if (q == 0) continue;
+          const float(*W)[4] = reinterpret_cast<const float(*)[4]>(cw + (p * 4 + q) * 16);
+          float t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
+          for (int k = 1; k < p; ++k) {
+            const float v = at(ord_eslot(tri, p, e, k));
+            t0 += W[k][0] * v;
+            t1 += W[k][1] * v;
+            t2 += W[k][2] * v;
+            t3 += W[k][3] * v;
+          }
+          at(ord_vslot(tri, p, e)) += t0;
+          at(ord_vslot(tri, p, (e + 1) % ne)) += q == 1 ? t1 : q == 2 ? t2 : t3;
+          for (int k = 1; k < p; ++k)
+            at(ord_eslot(tri, p, e, k)) = k < q ? (k == 1 ? t1 : t2) : 0.f;
+        }
+        // the wave's LDS writes of this pass before its reads of the next
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
   };
   // FP64 geometry of the staged group -> FP32 factors Gf[c][q*NC + f] in g.
   // Threads past CPB*NQ repeat the last point (same values, same address).
@@ -223,14 +287,20 @@ __device__ __forceinline__ void gp_body(
   };
 
   // prologue: factors of group 0; coordinates of group 1 in registers
+  if (cdesc) {
+    if (threadIdx.x < 4) cflag[threadIdx.x] = 0u;
+    for (int k = threadIdx.x; k < 320; k += blockDim.x) cw[k] = (&kOrdWeights.w[0][0][0][0])[k];
+    __syncthreads();
+  }
   double xr[D], er, mr;
+  uint32_t dr;
   int gid = gid_of(0);
   load_x(node_of(0), xr);
-  load_c(gid, er, mr);
-  stage_nodes(0, xr, er, mr, gid);
+  load_c(gid, er, mr, dr);
+  stage_nodes(0, xr, er, mr, gid, dr);
   load_x(node_of(1), xr);
   gid = gid_of(1);
-  load_c(gid, er, mr);
+  load_c(gid, er, mr, dr);
   int nid = node_of(2);
   int gidn = gid_of(2);
   __syncthreads();
@@ -261,7 +331,7 @@ __device__ __forceinline__ void gp_body(
         }
       }
     }
-    stage_nodes(t + 1, xr, er, mr, gid);
+    stage_nodes(t + 1, xr, er, mr, gid, dr);
     floatx4 acc[TPW];
 #pragma unroll
     for (int tt = 0; tt < TPW; ++tt) acc[tt] = floatx4{0.f, 0.f, 0.f, 0.f};
@@ -295,11 +365,17 @@ __device__ __forceinline__ void gp_body(
     }
     load_x(nid, xr);
     gid = gidn;
-    load_c(gid, er, mr);
+    load_c(gid, er, mr, dr);
     nid = node_of(t + 3);
     gidn = gid_of(t + 3);
     geometry(s_G[(t + 1) & 1]);
+    // the flag of the slot that group t+2 takes next step (last read two steps ago)
+    if (cdesc && threadIdx.x == 0) cflag[(t + 2) & 3] = 0u;
     __syncthreads();
+    if (cdesc && cflag[t & 3] != 0u) {  // the same value in every sub-block of the block
+      constrain(t & 3);
+      __syncthreads();
+    }
   }
 }
 
@@ -309,95 +385,100 @@ __global__ __launch_bounds__((GPShape<D, NL, NQ, NG, MASS, KSR, WT>::NTH)) void 
     const double* __restrict__ tab, const float* __restrict__ ptab, int ncells,
     const int32_t* __restrict__ cells, int spare, int erow, const double* __restrict__ eta,
     double eta_scale, const double* __restrict__ mcoef, double mass_scale,
-    float* __restrict__ ke) {
+    const uint32_t* __restrict__ cdesc, float* __restrict__ ke) {
   __shared__ GPSmem<D, NL, NQ, NG, MASS, KSR, WT, 0> sm;
+  __shared__ uint32_t cflag[4];
+  __shared__ float cw[IND ? 320 : 1];
   gp_body<D, NL, NQ, NG, MASS, KSR, IND, WT, 0>(coords, c2n, tab, ptab, ncells, cells, spare,
-                                                erow, eta, eta_scale, mcoef, mass_scale, ke, sm,
-                                                (int)blockIdx.x, (int)gridDim.x);
+                                                erow, eta, eta_scale, mcoef, mass_scale,
+                                                IND ? cdesc : nullptr, ke, sm, (int)threadIdx.x,
+                                                (int)blockIdx.x, (int)gridDim.x, 0, cflag, cw);
 }
 
 // ---------------------------------------------------------------------------
 // All (cell type, order) buckets of a 2D space of per-cell order in one
-// persistent launch: the eight Quad4 / Tri3 Q1..Q4 / P1..P4 elements with the
-// Measure degrees max(2p, 4) (Gauss 3, 3, 4, 5 points per axis on quads;
-// triangle rules of 6, 6, 12, 16 points), rows of 32 floats (a space with Q4
-// cells).  Blocks [bstart[b], bstart[b+1]) run bucket b as k_element_gp runs
-// it (the same products in the same order: bitwise the per-bucket launches);
-// the block takes the largest element's thread count and LDS, and waves past
-// an element's tiles skip the contraction.
+// launch: the eight Quad4 / Tri3 Q1..Q4 / P1..P4 elements with the Measure
+// degrees max(2p, 4) (Gauss 3, 3, 4, 5 points per axis on quads; triangle
+// rules of 6, 6, 12, 16 points), rows of 32 floats (a space with Q4 cells).
+// A 512-thread block of bucket b splits into 512 / NTH_b sub-blocks of the
+// element's own size (8 one-wave sub-blocks for Q1 and P1, one for Q4), each
+// running the element kernel's steps on its own groups of 16 cells with its own
+// LDS, so small elements keep as many groups in flight as their own launches
+// would.  The sub-blocks of a block take the same number of steps (the
+// barriers are the block's).  Each sub-block computes exactly what
+// k_element_gp computes for its groups: bitwise the per-bucket launches.
+// Buckets go heaviest first, and the grid holds several blocks per CU so the
+// dispatcher balances them.
 // ---------------------------------------------------------------------------
 constexpr int kOrdSlots = 8;
 constexpr int kOrdKS = 32;
+constexpr int kOrdThreads = 512;
 template <int S, bool MASS>
-struct OrdSlot;  // slot -> element: Quad4 p = S + 1 (S < 4), Tri3 p = S - 3
-#define BH_OS(S, NL, NQ, NG)                                              \
-  template <bool MASS>                                                   \
-  struct OrdSlot<S, MASS> {                                              \
-    static constexpr int nl = NL, nq = NQ, ng = NG;                     \
-    static constexpr int nth = GPShape<2, NL, NQ, NG, MASS, kOrdKS, 8>::W * 64; \
+struct OrdSlot;  // slots heaviest first: Q4, P4, Q3, P3, Q2, P2, Q1, P1
+#define BH_OS(S, NL, NQ, NG, WT)                                                        \
+  template <bool MASS>                                                                 \
+  struct OrdSlot<S, MASS> {                                                            \
+    static constexpr int nl = NL, nq = NQ, ng = NG, wt = WT;                          \
+    static constexpr int w = GPShape<2, NL, NQ, NG, MASS, kOrdKS, WT>::W;             \
+    static constexpr int nth = w <= 1 ? 64 : w <= 2 ? 128 : w <= 4 ? 256 : 512;       \
+    static constexpr int subs = kOrdThreads / nth;                                    \
+    using Smem = GPSmem<2, NL, NQ, NG, MASS, kOrdKS, WT, nth>;                        \
   };
-BH_OS(0, 4, 9, 4)
-BH_OS(1, 9, 9, 4)
-BH_OS(2, 16, 16, 4)
-BH_OS(3, 25, 25, 4)
-BH_OS(4, 3, 6, 3)
-BH_OS(5, 6, 6, 3)
-BH_OS(6, 10, 12, 3)
-BH_OS(7, 15, 16, 3)
+BH_OS(0, 25, 25, 4, 8)  // Quad4 Q4: 21 tiles, 3 per wave, 7 waves
+BH_OS(1, 15, 16, 3, 8)  // Tri3 P4: 8 tiles, 8 waves
+BH_OS(2, 16, 16, 4, 3)  // Quad4 Q3: 9 tiles, 3 per wave, 3 waves
+BH_OS(3, 10, 12, 3, 8)  // Tri3 P3: 4 tiles, 4 waves
+BH_OS(4, 9, 9, 4, 2)    // Quad4 Q2: 3 tiles, 2 per wave, 2 waves
+BH_OS(5, 6, 6, 3, 8)    // Tri3 P2: 2 tiles, 2 waves
+BH_OS(6, 4, 9, 4, 8)    // Quad4 Q1: 1 tile, 1 wave
+BH_OS(7, 3, 6, 3, 8)    // Tri3 P1: 1 tile, 1 wave
 #undef BH_OS
 
 template <bool MASS>
-constexpr int ord_nth() {
-  int m = 0;
-  m = OrdSlot<0, MASS>::nth > m ? OrdSlot<0, MASS>::nth : m;
-  m = OrdSlot<1, MASS>::nth > m ? OrdSlot<1, MASS>::nth : m;
-  m = OrdSlot<2, MASS>::nth > m ? OrdSlot<2, MASS>::nth : m;
-  m = OrdSlot<3, MASS>::nth > m ? OrdSlot<3, MASS>::nth : m;
-  m = OrdSlot<4, MASS>::nth > m ? OrdSlot<4, MASS>::nth : m;
-  m = OrdSlot<5, MASS>::nth > m ? OrdSlot<5, MASS>::nth : m;
-  m = OrdSlot<6, MASS>::nth > m ? OrdSlot<6, MASS>::nth : m;
-  m = OrdSlot<7, MASS>::nth > m ? OrdSlot<7, MASS>::nth : m;
-  return m;
-}
-template <int S, bool MASS>
-using OrdSmem = GPSmem<2, OrdSlot<S, MASS>::nl, OrdSlot<S, MASS>::nq, OrdSlot<S, MASS>::ng, MASS,
-                       kOrdKS, 8, ord_nth<MASS>()>;
-template <bool MASS>
 constexpr size_t ord_smem() {
   size_t m = 0;
-  m = sizeof(OrdSmem<0, MASS>) > m ? sizeof(OrdSmem<0, MASS>) : m;
-  m = sizeof(OrdSmem<1, MASS>) > m ? sizeof(OrdSmem<1, MASS>) : m;
-  m = sizeof(OrdSmem<2, MASS>) > m ? sizeof(OrdSmem<2, MASS>) : m;
-  m = sizeof(OrdSmem<3, MASS>) > m ? sizeof(OrdSmem<3, MASS>) : m;
-  m = sizeof(OrdSmem<4, MASS>) > m ? sizeof(OrdSmem<4, MASS>) : m;
-  m = sizeof(OrdSmem<5, MASS>) > m ? sizeof(OrdSmem<5, MASS>) : m;
-  m = sizeof(OrdSmem<6, MASS>) > m ? sizeof(OrdSmem<6, MASS>) : m;
-  m = sizeof(OrdSmem<7, MASS>) > m ? sizeof(OrdSmem<7, MASS>) : m;
+#define BH_OM(S)                                                                        \
+  m = OrdSlot<S, MASS>::subs * sizeof(typename OrdSlot<S, MASS>::Smem) > m             \
+          ? OrdSlot<S, MASS>::subs * sizeof(typename OrdSlot<S, MASS>::Smem)           \
+          : m;
+  BH_OM(0) BH_OM(1) BH_OM(2) BH_OM(3) BH_OM(4) BH_OM(5) BH_OM(6) BH_OM(7)
+#undef BH_OM
   return m;
 }
 
 template <int S, bool MASS>
-__device__ __forceinline__ void ord_run(const GPOrdersArgs& A, unsigned char* buf, int bid) {
+__device__ __forceinline__ void ord_run(const GPOrdersArgs& A, unsigned char* buf,
+                                        uint32_t* cflag, float* cw, int bid) {
   using O = OrdSlot<S, MASS>;
-  gp_body<2, O::nl, O::nq, O::ng, MASS, kOrdKS, true, 8, ord_nth<MASS>()>(
+  const int sub = (int)threadIdx.x / O::nth;
+  const int nblk = (A.bstart[S + 1] - A.bstart[S]) * O::subs;
+  const int b0 = (bid - A.bstart[S]) * O::subs;
+  const int ngroups = (A.ncells[S] + 15) / 16;
+  // the block's step count: that of its first sub-block, which has the most
+  const int tfix = b0 < ngroups ? (ngroups - 1 - b0) / nblk + 1 : 0;
+  if (tfix == 0) return;  // the whole block: every sub-block has no group
+  auto* sm = reinterpret_cast<typename O::Smem*>(buf) + sub;
+  gp_body<2, O::nl, O::nq, O::ng, MASS, kOrdKS, true, O::wt, O::nth>(
       A.coords, A.c2n[S], A.tab[S], A.ptab[S], A.ncells[S], A.cells[S], A.spare, A.erow, A.eta,
-      A.eta_scale, A.mcoef, A.mass_scale, A.ke, *reinterpret_cast<OrdSmem<S, MASS>*>(buf),
-      bid - A.bstart[S], A.bstart[S + 1] - A.bstart[S]);
+      A.eta_scale, A.mcoef, A.mass_scale, A.cdesc, A.ke, *sm, (int)threadIdx.x - sub * O::nth,
+      b0 + sub, nblk, tfix, cflag, cw);
 }
 
 template <bool MASS>
-__global__ __launch_bounds__(ord_nth<MASS>()) void k_element_gp_orders(const GPOrdersArgs A) {
+__global__ __launch_bounds__(kOrdThreads) void k_element_gp_orders(const GPOrdersArgs A) {
   __shared__ __attribute__((aligned(16))) unsigned char buf[ord_smem<MASS>()];
+  __shared__ uint32_t cflag[4];
+  __shared__ float cw[320];
   const int bid = (int)blockIdx.x;
   // the bucket of this block (a uniform branch: every wave takes the same one)
-  if (bid < A.bstart[1]) ord_run<0, MASS>(A, buf, bid);
-  else if (bid < A.bstart[2]) ord_run<1, MASS>(A, buf, bid);
-  else if (bid < A.bstart[3]) ord_run<2, MASS>(A, buf, bid);
-  else if (bid < A.bstart[4]) ord_run<3, MASS>(A, buf, bid);
-  else if (bid < A.bstart[5]) ord_run<4, MASS>(A, buf, bid);
-  else if (bid < A.bstart[6]) ord_run<5, MASS>(A, buf, bid);
-  else if (bid < A.bstart[7]) ord_run<6, MASS>(A, buf, bid);
-  else if (bid < A.bstart[8]) ord_run<7, MASS>(A, buf, bid);
+  if (bid < A.bstart[1]) ord_run<0, MASS>(A, buf, cflag, cw, bid);
+  else if (bid < A.bstart[2]) ord_run<1, MASS>(A, buf, cflag, cw, bid);
+  else if (bid < A.bstart[3]) ord_run<2, MASS>(A, buf, cflag, cw, bid);
+  else if (bid < A.bstart[4]) ord_run<3, MASS>(A, buf, cflag, cw, bid);
+  else if (bid < A.bstart[5]) ord_run<4, MASS>(A, buf, cflag, cw, bid);
+  else if (bid < A.bstart[6]) ord_run<5, MASS>(A, buf, cflag, cw, bid);
+  else if (bid < A.bstart[7]) ord_run<6, MASS>(A, buf, cflag, cw, bid);
+  else if (bid < A.bstart[8]) ord_run<7, MASS>(A, buf, cflag, cw, bid);
 }
 
 // ---------------------------------------------------------------------------
@@ -811,7 +892,7 @@ static hipError_t launch_gp_kw(const LaunchArgs& A) {
     dim3 grid(ng < resident ? ng : resident);
     hipLaunchKernelGGL(kern, grid, dim3(nth), 0, A.stream, A.coords, A.c2n, A.tab, A.ptab,
                        (int)A.ncells_total, A.cells, (int)A.spare_cell, A.erow > 0 ? A.erow : NL,
-                       A.eta, A.eta_scale, A.mcoef, A.mass_scale, (float*)A.ke);
+                       A.eta, A.eta_scale, A.mcoef, A.mass_scale, A.cdesc, (float*)A.ke);
     return hipGetLastError();
   };
   constexpr int NT1 = GPShape<D, NL, NQ, NG, true, KSR, WT>::NTH;
@@ -918,55 +999,52 @@ hipError_t launch_gp(int geo, int order, int nq1, const LaunchArgs& A) {
 int gp_orders_slot(int geo, int order, int nq1) {
   if (order < 1 || order > 4) return -1;
   static const int kq[5] = {0, 3, 3, 4, 5}, kt[5] = {0, 6, 6, 12, 16};
-  if (geo == BH_QUAD4 && nq1 == kq[order]) return order - 1;
-  if (geo == BH_TRI3 && nq1 == kt[order]) return 3 + order;
+  static const int quad_slot[5] = {-1, 6, 4, 2, 0}, tri_slot[5] = {-1, 7, 5, 3, 1};
+  if (geo == BH_QUAD4 && nq1 == kq[order]) return quad_slot[order];
+  if (geo == BH_TRI3 && nq1 == kt[order]) return tri_slot[order];
   return -1;
 }
 
+template <int S>
+constexpr int ord_subs() { return OrdSlot<S, true>::subs; }
+
 hipError_t launch_gp_orders(GPOrdersArgs A, bool mass, hipStream_t stream) {
   if (ke_stride(A.erow) != kOrdKS) return hipErrorInvalidValue;
-  // one block per CU slot the kernel can hold, split over the buckets in
-  // proportion to their work (16-cell groups x element size), at least one
-  // block per bucket and at most one per group
-  static int resident[2] = {0, 0};
-  int& res = resident[mass ? 1 : 0];
-  if (res == 0) {
+  static_assert(OrdSlot<0, true>::subs == OrdSlot<0, false>::subs &&
+                    OrdSlot<7, true>::subs == OrdSlot<7, false>::subs,
+                "sub-block counts do not depend on the mass term");
+  static const int subs[kOrdSlots] = {ord_subs<0>(), ord_subs<1>(), ord_subs<2>(), ord_subs<3>(),
+                                      ord_subs<4>(), ord_subs<5>(), ord_subs<6>(), ord_subs<7>()};
+  static int resident = 0;  // blocks the device holds at once
+  if (resident == 0) {
     int dev = 0, ncu = 0, per = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess)
-      e = mass ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_element_gp_orders<true>,
-                                                               ord_nth<true>(), 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_element_gp_orders<false>,
-                                                               ord_nth<false>(), 0);
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_element_gp_orders<true>,
+                                                       kOrdThreads, 0);
     if (e != hipSuccess) return e;
-    res = ncu * (per > 0 ? per : 1);
+    resident = ncu * (per > 0 ? per : 1);
   }
-  static const int nl[kOrdSlots] = {4, 9, 16, 25, 3, 6, 10, 15};
-  static const int nq[kOrdSlots] = {9, 9, 16, 25, 6, 6, 12, 16};
-  double w[kOrdSlots], tot = 0.0;
-  int64_t groups[kOrdSlots];
+  // block-steps (one step of every sub-block of a block) per bucket, then
+  // about four blocks per resident slot, each taking `steps` block-steps
+  int64_t bsteps[kOrdSlots], tot = 0;
   for (int b = 0; b < kOrdSlots; ++b) {
-    groups[b] = (A.ncells[b] + 15) / 16;
-    w[b] = (double)groups[b] * nl[b] * (nl[b] + nq[b]);
-    tot += w[b];
+    const int64_t groups = (A.ncells[b] + 15) / 16;
+    bsteps[b] = (groups + subs[b] - 1) / subs[b];
+    tot += bsteps[b];
   }
-  if (tot <= 0.0) return hipSuccess;
+  if (tot == 0) return hipSuccess;
+  const int64_t target = 4 * (int64_t)resident;
+  const int64_t steps = (tot + target - 1) / target;
   A.bstart[0] = 0;
-  for (int b = 0; b < kOrdSlots; ++b) {
-    int64_t nb = 0;
-    if (groups[b] > 0) {
-      nb = (int64_t)(res * w[b] / tot + 0.5);
-      nb = nb < 1 ? 1 : nb;
-      nb = nb > groups[b] ? groups[b] : nb;
-    }
-    A.bstart[b + 1] = A.bstart[b] + (int)nb;
-  }
+  for (int b = 0; b < kOrdSlots; ++b)
+    A.bstart[b + 1] = A.bstart[b] + (int)((bsteps[b] + steps - 1) / steps);
   dim3 grid((unsigned)A.bstart[kOrdSlots]);
   if (mass)
-    hipLaunchKernelGGL(k_element_gp_orders<true>, grid, dim3(ord_nth<true>()), 0, stream, A);
+    hipLaunchKernelGGL(k_element_gp_orders<true>, grid, dim3(kOrdThreads), 0, stream, A);
   else
-    hipLaunchKernelGGL(k_element_gp_orders<false>, grid, dim3(ord_nth<false>()), 0, stream, A);
+    hipLaunchKernelGGL(k_element_gp_orders<false>, grid, dim3(kOrdThreads), 0, stream, A);
   return hipGetLastError();
 }
 

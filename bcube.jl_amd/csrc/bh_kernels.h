@@ -21,6 +21,9 @@ struct LaunchArgs {
   const float* ptab = nullptr;     // reference products P of the GEMM element kernel (bh_gp.hip)
   const int32_t* cells = nullptr;  // cells of this colour (source) / of this cell type (bilinear)
   const int32_t* lcells = nullptr; // source on a mixed mesh: the cells' rows in this type's c2n
+  // FP32 GEMM element kernel on a space of per-cell order: per global cell its
+  // constraint descriptor (0: none); the kernel writes Cᵀ K C (bh_gp.hip)
+  const uint32_t* cdesc = nullptr;
   int fstride = 0;                 // source / points: quadrature points per cell in f / xq (0: NQ)
   int ncells_color = 0;
   int slot0 = 0;                   // slot of cells[0] in the scatter map
@@ -186,6 +189,7 @@ struct GPOrdersArgs {
   double eta_scale = 0.0;
   const double* mcoef = nullptr;
   double mass_scale = 0.0;
+  const uint32_t* cdesc = nullptr;  // per-cell constraint descriptors (Cᵀ K C in LDS)
   float* ke = nullptr;
 };
 int gp_orders_slot(int geo_celltype, int order, int nq1);

@@ -13,11 +13,13 @@
 //     K[va][:] += t_0, K[vb][:] += t_q, K[s_j][:] = t_j (j < q), K[s_k][:] = 0 (k >= q)
 // and the same on columns.  Edges are handled in local order, so every entry
 // is a fixed sequence of FP64 operations (bitwise repeatable), rounded once
-// to the workspace type.  One wave per cell, the matrix in LDS.
+// to the workspace type.  One wave per cell; only the rows and columns of the
+// touched slots move between the workspace and LDS (the rest of Cᵀ K C is K).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
+#include "bh_device.h"
 #include "bh_kernels.h"
 
 namespace bh {
@@ -26,33 +28,6 @@ namespace {
 constexpr int kOrdWaves = 4;
 constexpr int kOrdLd = 26;  // LDS row stride (doubles) of a 25 x 25 matrix
 
-// Local slot of vertex v / of the k-th node (k = 1..p-1, from the edge's first
-// vertex) of edge e, in the Lagrange dof order the element kernels use
-// (tensor order on quads, vertices then edges on triangles).
-__device__ __forceinline__ int ord_vslot(bool tri, int p, int v) {
-  if (tri) return v;
-  const int n = p + 1;
-  return v == 0 ? 0 : v == 1 ? p : v == 2 ? p + n * p : n * p;
-}
-__device__ __forceinline__ int ord_eslot(bool tri, int p, int e, int k) {
-  if (tri) return 3 + e * (p - 1) + (k - 1);
-  const int n = p + 1;
-  switch (e) {
-    case 0: return k;
-    case 1: return p + n * k;
-    case 2: return (p - k) + n * p;
-    default: return n * (p - k);
-  }
-}
-// ℓ_j of the order-q Lagrange basis on the nodes 0, 1/q, .., 1, at t = k/p.
-__device__ __forceinline__ double ord_weight(int q, int j, int k, int p) {
-  const double t = (double)k / p;
-  double v = 1.0;
-  for (int i = 0; i <= q; ++i)
-    if (i != j) v *= (q * t - i) / (double)(j - i);
-  return v;
-}
-
 // desc: bit 0 triangle, bits 1-3 order p, bits 4+3e .. 6+3e order q of local
 // edge e (0: free).  Rows first (lane = column), then columns (lane = row).
 template <typename T>
@@ -60,6 +35,7 @@ __global__ __launch_bounds__(64 * kOrdWaves) void k_orders_constrain(
     const int32_t* __restrict__ cells, const uint32_t* __restrict__ desc, int64_t n, int erow,
     int ks, T* __restrict__ ke) {
   __shared__ double s_a[kOrdWaves][25 * kOrdLd];
+  __shared__ double s_wt[kOrdWaves][4 * 3 * 4];  // ℓ_j(k/p) per edge: [e][k - 1][j]
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t w = (int64_t)blockIdx.x * kOrdWaves + wave;
   // waves past the last cell keep to the barriers with an empty matrix
@@ -70,9 +46,35 @@ __global__ __launch_bounds__(64 * kOrdWaves) void k_orders_constrain(
   const int ne = tri ? 3 : 4;
   const int64_t base = nl > 0 ? (int64_t)cells[w] * erow : 0;
   double* A = s_a[wave];
-  for (int k = lane; k < nl * nl; k += 64) {
-    const int i = k / nl, j = k - (k / nl) * nl;
-    A[i * kOrdLd + j] = (double)ke[(base + i) * ks + j];
+  double* Wt = s_wt[wave];
+  if (lane < 48) {  // one weight per lane
+    const int e = lane / 12, k = (lane % 12) / 4 + 1, j = lane % 4;
+    const int q = nl > 0 ? (int)((d >> (4 + 3 * e)) & 7) : 0;
+    Wt[lane] = (q > 0 && k <= p - 1 && j <= q) ? ord_lagrange(q, j, k, p) : 0.0;
+  }
+  // slots whose row or column changes: the constrained edges' slots and
+  // their vertices; only entries in those rows or columns are read and written
+  uint32_t aff = 0u;
+  for (int e = 0; e < ne && nl > 0; ++e) {
+    if (((d >> (4 + 3 * e)) & 7) == 0) continue;
+    aff |= 1u << ord_vslot(tri, p, e);
+    aff |= 1u << ord_vslot(tri, p, (e + 1) % ne);
+    for (int k = 1; k <= p - 1; ++k) aff |= 1u << ord_eslot(tri, p, e, k);
+  }
+  // two rows per pass: lanes 0-31 row i, lanes 32-63 row i + 1; every load is
+  // issued before the first LDS store, so their latencies overlap
+  const int jc = lane & 31, half = lane >> 5;
+  auto moves = [&](int i) { return i < nl && jc < nl && (((aff >> i) | (aff >> jc)) & 1u); };
+  T v[13];
+#pragma unroll
+  for (int it = 0; it < 13; ++it) {
+    const int i = half + 2 * it;
+    v[it] = moves(i) ? ke[(base + i) * ks + jc] : (T)0;
+  }
+#pragma unroll
+  for (int it = 0; it < 13; ++it) {
+    const int i = half + 2 * it;
+    if (moves(i)) A[i * kOrdLd + jc] = (double)v[it];
   }
   __syncthreads();
   for (int pass = 0; pass < 2; ++pass) {
@@ -88,7 +90,7 @@ __global__ __launch_bounds__(64 * kOrdWaves) void k_orders_constrain(
         double t[4] = {0.0, 0.0, 0.0, 0.0};
         for (int k = 1; k <= p - 1; ++k) {
           const double v = at(ord_eslot(tri, p, e, k));
-          for (int j = 0; j <= q; ++j) t[j] += ord_weight(q, j, k, p) * v;
+          for (int j = 0; j <= q; ++j) t[j] += Wt[e * 12 + (k - 1) * 4 + j] * v;
         }
         const int va = ord_vslot(tri, p, e), vb = ord_vslot(tri, p, (e + 1) % ne);
         at(va) += t[0];
@@ -98,9 +100,10 @@ __global__ __launch_bounds__(64 * kOrdWaves) void k_orders_constrain(
     }
     __syncthreads();
   }
-  for (int k = lane; k < nl * nl; k += 64) {
-    const int i = k / nl, j = k - (k / nl) * nl;
-    ke[(base + i) * ks + j] = (T)A[i * kOrdLd + j];
+#pragma unroll
+  for (int it = 0; it < 13; ++it) {
+    const int i = half + 2 * it;
+    if (moves(i)) ke[(base + i) * ks + jc] = (T)A[i * kOrdLd + jc];
   }
 }
 

@@ -591,142 +591,139 @@ def bench_c1(args, torch, B):
     return line
 
 
+def c3_problem(B, n, backend=None):
+    """The C3 space: c3_mesh at n^2, the continuous Lagrange space of per-cell
+    order (edges at their cells' lowest order), ∫∇u⋅∇v + ∫uv with Measure
+    degree max(2p, 4) on the cells of order p."""
+    m, order = c3_mesh(B, n, backend=backend)
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", order), m)
+    dΩ = B.Measure(B.CellDomain(m), B.OrderDegree(4))
+    plan = B.BilinearPlan(lambda u, v: B.integrate(B.dot(B.grad(u), B.grad(v)), dΩ)
+                          + B.integrate(u * v, dΩ), U, B.TestFESpace(U))
+    return m, order, U, plan
+
+
 def c3_cpu_and_checks(B, timed=True, n_sample=192):
     """C3 on an n_sample^2 sample with the same block layout and seeds: the
-    oracle's assembly of every order bucket (continuous P_p/Q_p on the union of
-    the blocks of order p, ∫∇u⋅∇v + ∫uv, degree max(2p, 4)) timed, summed over
-    the buckets; the GPU's bucket matrices on the sample against the oracle's;
-    and per bucket the solution of (K + M) x = ∫x v against the oracle's."""
+    oracle's assembly of the space of per-cell order (COO build + sparse(),
+    numbering excluded) timed; the GPU's matrix on the sample against it; and
+    the solution of (K + M) x = ∫x v against the oracle's."""
     from oracle import oracle as O
 
-    m, order = c3_mesh(B, n_sample)
-    parts, plans = [], []
-    for p in (1, 2, 3, 4):
-        cells = np.flatnonzero(order == p)
-        if cells.size == 0:
-            continue
-        sub = B.sub_mesh(m, cells)
-        deg = max(2 * p, 4)
-        U = B.TrialFESpace(B.FunctionSpace("Lagrange", p), sub)
-        dΩ = B.Measure(B.CellDomain(sub), deg)
-        plans.append((p, deg, sub, B.BilinearPlan(
-            lambda u, v, dΩ=dΩ: B.integrate(B.dot(B.grad(u), B.grad(v)), dΩ)
-            + B.integrate(u * v, dΩ), U, B.TestFESpace(U))))
-        parts.append((host_mesh(B, sub), p, ["stiffness", "mass"], None, deg))
-    refs = []
-    nnz = sum(pl.nnz for _, _, _, pl in plans)
+    m, order, U, plan = c3_problem(B, n_sample)
+    om = host_mesh(B, m)
+    out = {}
+
+    def run(th, rp):
+        r, t = O.assemble_bilinear_orders_timed(om, order, ["stiffness", "mass"], threads=th,
+                                                reps=rp)
+        out["A"] = r
+        return t.sum(axis=1), {"coo_build_s": t[:, 0], "sparse_s": t[:, 1]}
+
     cpu = None
     if timed:
-        cpu = cpu_entry(bilinear_runs(parts, refs), nnz, "GNNZ/s",
+        cpu = cpu_entry(run, plan.nnz, "GNNZ/s",
                         f"{n_sample}^2 mixed tri/quad mesh ({m.ncells} cells, same block layout "
-                        f"and order seeds), every order bucket: COO build + sparse() of {nnz} "
-                        f"nnz, numbering excluded")
+                        f"and order seeds), the space of per-cell order: COO build + sparse() of "
+                        f"{plan.nnz} nnz, numbering excluded")
     else:
-        refs = [O.assemble_bilinear(om, p, t, degree=d) for om, p, t, _, d in parts]
-    mat, sol = {"sample": f"{n_sample}^2"}, {"workload": f"{n_sample}^2 sample, per bucket "
-                                                          f"(K + M) x = ∫x v, direct solve"}
-    for (p, deg, sub, plan), ref, part in zip(plans, refs, parts):
-        mat[f"P{p}"] = {mode: csc_err(plan, ref, mode) for mode in ("fp64", "fp32")}
-        b = O.assemble_linear(part[0], p, "x", degree=deg)
-        sol[f"P{p}"] = solve_err(plan, ref, b)
+        run(1, 1)
+    ref = out["A"]
+    mat = {"sample": f"{n_sample}^2"}
+    mat.update({mode: csc_err(plan, ref, mode) for mode in ("fp64", "fp32")})
+    b = O.assemble_linear_orders(om, order, "x")
+    sol = {"workload": f"{n_sample}^2 sample, (K + M) x = ∫x v, direct solve"}
+    sol.update(solve_err(plan, ref, b))
     return cpu, mat, sol
 
 
 def bench_c3(args, torch, B):
-    """Mixed triangle/quad assembly per order bucket: the cells of all blocks of
-    order p form one mesh (triangles and quads), continuous Lagrange P_p/Q_p,
-    ∫∇u⋅∇v + ∫uv with Measure degree max(2p, 4) (the reference's inline triangle
-    rules).  One step = the assembly of every bucket (one element launch per
-    cell type per bucket + one row gather per bucket) into device CSR values."""
+    """C3: one continuous space of per-cell order (blocks of orders 1..4 on a
+    mixed triangle/quad mesh, edges at their cells' lowest order), ∫∇u⋅∇v +
+    ∫uv with Measure degree max(2p, 4).  One step = the assembly into device
+    CSR values: FP32, one launch of the element kernels of every (cell type,
+    order) bucket, the constraint kernel on the cells with an edge of lower
+    order, and the row gather; FP64, one element launch per bucket."""
     n = args.n if args.n != 64 else 866
     t_setup = time.perf_counter()
-    m, order = c3_mesh(B, n)
     dev = torch.device("cuda:0")
+    m, order, U, plan = c3_problem(B, n)
     stream = torch.cuda.current_stream(dev)
     backend = m.backend
     backend.set_stream(stream.cuda_stream)
-    backend.set_autotune(True)  # the first (warm-up) call of each plan tunes its launch shapes
+    backend.set_autotune(True)  # the first (warm-up) call tunes the gather shape
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    ntri = m.celltypes == 2
+    f_alg = 0
     buckets = {}
     for p in (1, 2, 3, 4):
-        cells = np.flatnonzero(order == p)
-        if cells.size == 0:
-            continue
-        sub = B.sub_mesh(m, cells)
-        U = B.TrialFESpace(B.FunctionSpace("Lagrange", p), sub)
         deg = max(2 * p, 4)
-        dΩ = B.Measure(B.CellDomain(sub), deg)
-        form = lambda u, v, dΩ=dΩ: B.integrate(B.dot(B.grad(u), B.grad(v)), dΩ) + B.integrate(u * v, dΩ)
-        plan = B.BilinearPlan(form, U, B.TestFESpace(U))
-        ntri = int(np.sum(sub.celltypes == 2))
         nq_t = {4: 6, 6: 12, 8: 16}[deg]
         nq_q = ((deg + 2) // 2) ** 2
         nl_t, nl_q = (p + 1) * (p + 2) // 2, (p + 1) ** 2
-        f_alg = (ntri * 2 * nl_t * nl_t * 3 * nq_t + (cells.size - ntri) * 2 * nl_q * nl_q * 3 * nq_q)
-        buckets[p] = {"plan": plan, "cells": int(cells.size), "triangles": ntri, "degree": deg,
-                      "f_alg": f_alg, "ndofs": plan.nrows}
-    torch.cuda.synchronize()
-    t_setup = time.perf_counter() - t_setup
+        nt = int(np.sum((order == p) & ntri))
+        nqd = int(np.sum((order == p) & ~ntri))
+        f_alg += nt * 2 * nl_t * nl_t * 3 * nq_t + nqd * 2 * nl_q * nl_q * 3 * nq_q
+        buckets[f"P{p}"] = {"triangles": nt, "quads": nqd, "degree": deg}
+    _, ntrans = U.edge_orders()
     res = {}
     for mode in ("fp64", "fp32"):
-        tot_ms, tot_nnz, tot_cells = 0.0, 0, 0
-        per = {}
-        for p, bk in buckets.items():
-            plan = bk["plan"]
-            out = torch.empty(plan.nnz, dtype=torch.float64, device=dev)
-            for _ in range(args.warmup):
-                plan.assemble_device(out.data_ptr(), None, mode)
-            torch.cuda.synchronize()
-            backend.set_timing(True)
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            for _ in range(args.steps):
-                plan.assemble_device(out.data_ptr(), None, mode)
-            e1.record(stream)
-            torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1) / args.steps
-            lt, lk = backend.timings_ex(cap=100000)
-            backend.set_timing(False)
-            lt = np.asarray(lt, dtype=np.float64)
-            el_ms = float(lt[lk == 0].sum() / args.steps)
-            ga_ms = float(lt[lk == 1].sum() / args.steps)
-            peak = PEAK_TFLOPS[mode]
-            el_ach = bk["f_alg"] / (el_ms * 1e-3) / 1e12
-            ga_ach = 8.0 * plan.nnz / (ga_ms * 1e-3) / 1e9
-            per[f"P{p}"] = {"cells": bk["cells"], "triangles": bk["triangles"], "degree": bk["degree"],
-                           "ndofs": bk["ndofs"], "nnz": plan.nnz, "ms": ms,
-                           "tuning": plan.tuning(mode),
-                           "gnnz_per_s": plan.nnz / (ms * 1e-3) / 1e9,
-                           "elements_per_s": bk["cells"] / (ms * 1e-3),
-                           "element_kernels": {"bound": "mfma", "achieved": el_ach, "peak": peak,
-                                               "unit": "TFLOP/s", "frac": el_ach / peak,
-                                               "ms": el_ms, "flops_per_step": bk["f_alg"]},
-                           "gather": {"bound": "hbm", "achieved": ga_ach, "peak": PEAK_HBM_GBS,
-                                      "unit": "GB/s", "frac": ga_ach / PEAK_HBM_GBS, "ms": ga_ms,
-                                      "alg_bytes": 8.0 * plan.nnz}}
-            tot_ms += ms
-            tot_nnz += plan.nnz
-            tot_cells += bk["cells"]
-            del out
-        res[mode] = {"buckets": per, "ms_per_step": tot_ms,
-                     "gnnz_per_s": tot_nnz / (tot_ms * 1e-3) / 1e9,
-                     "elements_per_s": tot_cells / (tot_ms * 1e-3)}
+        out = torch.empty(plan.nnz, dtype=torch.float64, device=dev)
+        for _ in range(args.warmup):
+            plan.assemble_device(out.data_ptr(), None, mode)
+        torch.cuda.synchronize()
+        backend.set_timing(True)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            plan.assemble_device(out.data_ptr(), None, mode)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        lt, lk = backend.timings_ex(cap=100000)
+        backend.set_timing(False)
+        lt, lk = np.asarray(lt, dtype=np.float64), np.asarray(lk)
+        el_ms = float(lt[lk == 0].sum() / args.steps)
+        cn_ms = float(lt[lk == 4].sum() / args.steps)
+        ga_ms = float(lt[lk == 1].sum() / args.steps)
+        peak = PEAK_TFLOPS[mode]
+        el_ach = f_alg / (el_ms * 1e-3) / 1e12
+        ga_ach = 8.0 * plan.nnz / (ga_ms * 1e-3) / 1e9
+        kern = "k_element_gp_orders<true>" if mode == "fp32" else "element kernels (one per bucket)"
+        res[mode] = {
+            "ms_per_step": ms, "gnnz_per_s": plan.nnz / (ms * 1e-3) / 1e9,
+            "elements_per_s": m.ncells / (ms * 1e-3),
+            "element_launches_per_step": int((lk == 0).sum() // args.steps),
+            "launches_per_step": int(len(lk) // args.steps), "tuning": plan.tuning(mode),
+            "element_kernels": {"bound": "mfma", "achieved": el_ach, "peak": peak,
+                                "unit": "TFLOP/s", "frac": el_ach / peak, "kernel": kern,
+                                "avg_launch_ms": el_ms, "flops_per_step": f_alg},
+            "constraint": {"kernel": "k_orders_constrain", "avg_launch_ms": cn_ms,
+                           "cells": int(ntrans)},
+            "gather": {"bound": "hbm", "achieved": ga_ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                       "frac": ga_ach / PEAK_HBM_GBS, "avg_launch_ms": ga_ms,
+                       "kernel": "k_gather_padded<float,unsigned char,25,...>",
+                       "alg_bytes_per_launch": 8.0 * plan.nnz}}
+        del out
     r = res["fp32"]
-    dom_p = max(r["buckets"], key=lambda k: r["buckets"][k]["ms"])
-    db = r["buckets"][dom_p]
-    roof = dict(db["gather"] if db["gather"]["ms"] >= db["element_kernels"]["ms"] else db["element_kernels"])
-    roof["kernel"] = f"{'row gather' if roof['bound'] == 'hbm' else 'element kernels'} of bucket {dom_p}"
+    roof = dict(r["gather"] if r["gather"]["avg_launch_ms"] >= r["element_kernels"]["avg_launch_ms"]
+                else r["element_kernels"])
     roof["traffic"] = traffic_of(f"c3_{n}", roof["kernel"])
     line = {
-        "metric": "mixed tri/quad assembly (C3), aggregate GNNZ/s over order buckets, FP32-MFMA",
+        "metric": "mixed tri/quad assembly (C3), space of per-cell order 1..4, GNNZ/s, FP32-MFMA",
         "value": r["gnnz_per_s"], "unit": "GNNZ/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"C3: {n}^2 quads, 16x16 blocks split into triangles w.p. 1/3 "
-                               f"(seed 0), block orders p~{{1:.4,2:.3,3:.2,4:.1}} (seed 1), "
-                               f"per-bucket continuous P_p/Q_p, ∫∇u⋅∇v+∫uv, degree max(2p,4)",
-                   "cells": int(m.ncells), "triangles": int(np.sum(m.celltypes == 2)),
+                               f"(seed 0), block orders p~{{1:.4,2:.3,3:.2,4:.1}} (seed 1), one "
+                               f"continuous space of per-cell order (edges at the lower order), "
+                               f"∫∇u⋅∇v+∫uv, degree max(2p,4)",
+                   "cells": int(m.ncells), "triangles": int(ntri.sum()), "ndofs": plan.nrows,
+                   "nnz": plan.nnz, "transition_cells": int(ntrans), "buckets": buckets,
                    "setup_s": round(t_setup, 2)},
         "roofline": roof,
+        "kernels": [r["element_kernels"], r["constraint"], r["gather"]],
         "fp32_speedup_vs_fp64": res["fp64"]["ms_per_step"] / r["ms_per_step"],
         "modes": res,
     }

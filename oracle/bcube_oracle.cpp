@@ -1468,53 +1468,86 @@ static bool constrained(const int* eord) { return eord[0] || eord[1] || eord[2] 
 
 // Σ terms of the bilinear form on a per-cell-order space: COO in the
 // reference's order (terms, then cells in element order, entry i + nl*j of
-// the slots holding dofs) with slot blocks Cᵀ K C, then sparse().  degree[p]:
-// the Measure degree of the cells of order p.
-static void assemble_orders(const Mesh& m, const OrderSpace& S, const int* order, int nterms,
-                            const int* terms, const double* eta, const int* degree, CSC& A) {
+// the slots holding dofs) with slot blocks Cᵀ K C.  degree[p]: the Measure
+// degree of the cells of order p.  With nthreads > 1 the cells are computed in
+// parallel, each written at its own precomputed position: the same COO.
+static void build_coo_orders(const Mesh& m, const OrderSpace& S, const int* order, int nterms,
+                             const int* terms, const double* eta, const int* degree,
+                             int nthreads, std::vector<i64>& I, std::vector<i64>& J,
+                             std::vector<double>& X) {
   std::vector<i64> cells = element_order(m);
-  std::vector<i64> I, J;
-  std::vector<double> X;
-  std::map<int, ShapeRules> rules;
-  std::vector<double> acc, C, T, Kp;
-  for (int t = 0; t < nterms; ++t)
-    for (i64 c : cells) {
-      const int p = order[c];
-      ShapeRules& R = rules[degree[p]];
-      R.degree = degree[p];
-      const int nlc = element_block(m, p, terms[t], eta, R, c, acc);
-      const int* eo = &S.eord[c * 4];
-      const std::vector<double>* K = &acc;
-      if (constrained(eo)) {
-        cell_constraint(m.ctype[c], p, eo, C);
-        // acc[b*nl + a] = a(φ_b, φ_a); Kp[n*nl + m] = Σ_b Σ_a C[b][n] C[a][m] acc[b*nl + a]
-        T.assign((size_t)nlc * nlc, 0.0);
-        for (int b = 0; b < nlc; ++b)
-          for (int mm = 0; mm < nlc; ++mm) {
-            double v = 0.0;
-            for (int a = 0; a < nlc; ++a) v += acc[(size_t)b * nlc + a] * C[(size_t)a * nlc + mm];
-            T[(size_t)b * nlc + mm] = v;
+  const i64 ncl = (i64)cells.size();
+  std::vector<i64> start(ncl + 1, 0);
+  for (i64 k = 0; k < ncl; ++k) {
+    const i64* sl = &S.dofs[(size_t)cells[k] * S.nl];
+    i64 nv = 0;
+    for (int i = 0; i < S.nl; ++i) nv += sl[i] >= 0;
+    start[k + 1] = start[k] + nv * nv;
+  }
+  const i64 seg = start[ncl];
+  I.resize((size_t)seg * nterms);
+  J.resize((size_t)seg * nterms);
+  X.resize((size_t)seg * nterms);
+  std::map<int, ShapeRules> rules0;  // filled here, copied per thread (read-only there)
+  for (i64 c = 0; c < m.ncells(); ++c) {
+    ShapeRules& R = rules0[degree[order[c]]];
+    R.degree = degree[order[c]];
+    R.fill(shape_of(m.ctype[c]));
+  }
+  for (int t = 0; t < nterms; ++t) {
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+    {
+      std::map<int, ShapeRules> rules = rules0;
+      std::vector<double> acc, C, T, Kp;
+#pragma omp for schedule(static)
+      for (i64 k = 0; k < ncl; ++k) {
+        const i64 c = cells[k];
+        const int p = order[c];
+        ShapeRules& R = rules[degree[p]];
+        const int nlc = element_block(m, p, terms[t], eta, R, c, acc);
+        const int* eo = &S.eord[c * 4];
+        const std::vector<double>* K = &acc;
+        if (constrained(eo)) {
+          cell_constraint(m.ctype[c], p, eo, C);
+          // acc[b*nl + a] = a(φ_b, φ_a); Kp[n*nl + m] = Σ_b Σ_a C[b][n] C[a][m] acc[b*nl + a]
+          T.assign((size_t)nlc * nlc, 0.0);
+          for (int b = 0; b < nlc; ++b)
+            for (int mm = 0; mm < nlc; ++mm) {
+              double v = 0.0;
+              for (int a = 0; a < nlc; ++a) v += acc[(size_t)b * nlc + a] * C[(size_t)a * nlc + mm];
+              T[(size_t)b * nlc + mm] = v;
+            }
+          Kp.assign((size_t)nlc * nlc, 0.0);
+          for (int n = 0; n < nlc; ++n)
+            for (int mm = 0; mm < nlc; ++mm) {
+              double v = 0.0;
+              for (int b = 0; b < nlc; ++b) v += C[(size_t)b * nlc + n] * T[(size_t)b * nlc + mm];
+              Kp[(size_t)n * nlc + mm] = v;
+            }
+          K = &Kp;
+        }
+        const i64* sl = &S.dofs[(size_t)c * S.nl];
+        size_t q = (size_t)t * seg + start[k];
+        for (int j = 0; j < nlc; ++j) {
+          if (sl[j] < 0) continue;
+          for (int i = 0; i < nlc; ++i) {
+            if (sl[i] < 0) continue;
+            I[q] = sl[i] + 1;
+            J[q] = sl[j] + 1;
+            X[q] = (*K)[(size_t)j * nlc + i];
+            ++q;
           }
-        Kp.assign((size_t)nlc * nlc, 0.0);
-        for (int n = 0; n < nlc; ++n)
-          for (int mm = 0; mm < nlc; ++mm) {
-            double v = 0.0;
-            for (int b = 0; b < nlc; ++b) v += C[(size_t)b * nlc + n] * T[(size_t)b * nlc + mm];
-            Kp[(size_t)n * nlc + mm] = v;
-          }
-        K = &Kp;
-      }
-      const i64* sl = &S.dofs[(size_t)c * S.nl];
-      for (int j = 0; j < nlc; ++j) {
-        if (sl[j] < 0) continue;
-        for (int i = 0; i < nlc; ++i) {
-          if (sl[i] < 0) continue;
-          I.push_back(sl[i] + 1);
-          J.push_back(sl[j] + 1);
-          X.push_back((*K)[(size_t)j * nlc + i]);
         }
       }
     }
+  }
+}
+
+static void assemble_orders(const Mesh& m, const OrderSpace& S, const int* order, int nterms,
+                            const int* terms, const double* eta, const int* degree, CSC& A) {
+  std::vector<i64> I, J;
+  std::vector<double> X;
+  build_coo_orders(m, S, order, nterms, terms, eta, degree, 1, I, J, X);
   coo_to_csc(I, J, X, S.ndofs, S.ndofs, A);
 }
 
@@ -2156,6 +2189,31 @@ void* or_bilinear_timed(int dim, int64_t nnodes, const double* coords, int64_t n
       t[2 * k] = seconds_since(t0);
       t0 = std::chrono::steady_clock::now();
       coo_to_csc_par(I, J, X, r->ndofs, r->ndofs, r->A, nthreads);
+      t[2 * k + 1] = seconds_since(t0);
+    }
+    return r;
+  } catch (std::exception& e) { g_err = e.what(); return nullptr; }
+}
+
+// As or_bilinear_timed on a per-cell-order space (the numbering once,
+// untimed): t[2k] COO build, t[2k+1] sparse(), k = 0 .. reps.
+void* or_bilinear_orders_timed(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                               const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n,
+                               const int* order, int nterms, const int* terms, const double* eta,
+                               const int* degree, int nthreads, int reps, double* t) {
+  try {
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    OrderSpace S = order_space(m, order);
+    Result* r = new Result();
+    r->ndofs = S.ndofs;
+    for (int k = 0; k <= reps; ++k) {
+      std::vector<i64> I, J;
+      std::vector<double> X;
+      auto t0 = std::chrono::steady_clock::now();
+      build_coo_orders(m, S, order, nterms, terms, eta, degree, nthreads, I, J, X);
+      t[2 * k] = seconds_since(t0);
+      t0 = std::chrono::steady_clock::now();
+      coo_to_csc_par(I, J, X, S.ndofs, S.ndofs, r->A, nthreads);
       t[2 * k + 1] = seconds_since(t0);
     }
     return r;

@@ -90,6 +90,9 @@ def lib():
         L.or_assemble_bilinear_orders.restype = C.c_void_p
         L.or_assemble_linear_orders.argtypes = mesh_args + [_i, C.c_int, C.c_double, _i]
         L.or_assemble_linear_orders.restype = C.c_void_p
+        L.or_bilinear_orders_timed.argtypes = mesh_args + [_i, C.c_int, _i, C.c_void_p, _i,
+                                                           C.c_int, C.c_int, _d]
+        L.or_bilinear_orders_timed.restype = C.c_void_p
         _LIB = L
     return _LIB
 
@@ -383,6 +386,35 @@ def assemble_bilinear_orders(mesh: OMesh, order, terms, eta=None, degree=lambda 
     finally:
         lib().or_result_free(h)
     return colptr, rowval, nzval, int(n)
+
+
+def assemble_bilinear_orders_timed(mesh: OMesh, order, terms, eta=None,
+                                   degree=lambda p: max(2 * p, 4), threads=1, reps=5):
+    """assemble_bilinear_orders timed: the numbering once (untimed), then
+    reps + 1 runs on `threads` threads (the first a warm-up).  Returns
+    ((colptr, rowval, nzval, n) of the last run, t (reps, 2): COO build and
+    sparse() seconds per timed run)."""
+    order, deg = _orders_args(mesh, order, degree)
+    tt = np.ascontiguousarray([TERM[x] for x in terms], dtype=np.int32)
+    eta_p = None
+    if eta is not None:
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        eta_p = eta.ctypes.data
+    t = np.zeros(2 * (reps + 1))
+    h = lib().or_bilinear_orders_timed(*mesh.args(), order, len(tt), tt, eta_p, deg, int(threads),
+                                       int(reps), t)
+    if not h:
+        raise RuntimeError(_err())
+    try:
+        n = lib().or_result_ndofs(h)
+        nnz = lib().or_result_nnz(h)
+        colptr = np.zeros(n + 1, dtype=np.int64)
+        rowval = np.zeros(nnz, dtype=np.int64)
+        nzval = np.zeros(nnz)
+        lib().or_result_csc(h, colptr, rowval, nzval)
+    finally:
+        lib().or_result_free(h)
+    return (colptr, rowval, nzval, int(n)), t.reshape(-1, 2)[1:]
 
 
 def assemble_linear_orders(mesh: OMesh, order, source="one", const=0.0,

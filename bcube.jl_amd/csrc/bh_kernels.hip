@@ -50,12 +50,14 @@ __global__ __launch_bounds__(kBlock) void k_element_f64(
   constexpr int NS = SymN<D>::N;
   const int ks = ke_stride(erow);
 
-  __shared__ double s_tab[L::SIZE];
+  // the weights, the basis and the geometric gradients: the table up to L::G
+  // (Q3 hexahedra at 4^3 points: 141 KB of LDS)
+  __shared__ double s_tab[L::G];
   __shared__ double s_x[CPB][NG][D];
   __shared__ double s_g[CPB][NQ][NS + 1];
 
   const int tid = threadIdx.x;
-  for (int k = tid; k < L::SIZE; k += kBlock) s_tab[k] = tab[k];
+  for (int k = tid; k < L::G; k += kBlock) s_tab[k] = tab[k];
 
   const int ngroups = (ncells + CPB - 1) / CPB;
   const int cs = tid / NL;  // cell slot of this lane in phase 3/4
@@ -530,6 +532,7 @@ static const Entry kTable[] = {
     BH_ENTRY_Q(1, 1), BH_ENTRY_Q(1, 2), BH_ENTRY_Q(1, 3), BH_ENTRY_Q(2, 2), BH_ENTRY_Q(2, 3),
     BH_ENTRY_Q(2, 4), BH_ENTRY_Q(3, 3), BH_ENTRY_Q(3, 4), BH_ENTRY_Q(4, 5),
     BH_ENTRY_H(1, 1), BH_ENTRY_H(1, 2), BH_ENTRY_H(1, 3), BH_ENTRY_H(2, 2), BH_ENTRY_H(2, 3),
+    BH_ENTRY_H(3, 4),
     BH_ENTRY_T(1, 1), BH_ENTRY_T(1, 3), BH_ENTRY_T(1, 4), BH_ENTRY_T(1, 6),
     BH_ENTRY_T(2, 3), BH_ENTRY_T(2, 4), BH_ENTRY_T(2, 6), BH_ENTRY_T(2, 7), BH_ENTRY_T(3, 12),
     BH_ENTRY_T(4, 16),
@@ -556,7 +559,7 @@ int packed_table_size(int geo, int order, int nq1) {
 }
 
 // Element-row lengths with a gather instantiation: quads Q1..Q4, hexahedra
-// Q1/Q2, triangles P1..P4 (mixed Quad4/Tri3 meshes use the quad length).
+// Q1..Q3, triangles P1..P4 (mixed Quad4/Tri3 meshes use the quad length).
 int gather_shapes(int nl, int max_row, int offset_bytes, int lanes[4], int rows[4]) {
   const int sgs = nl <= 4 ? 4 : nl <= 8 ? 8 : nl <= 16 ? 16 : 32;
   // the default lanes per row for these rows (as gather_padded_t picks them)
@@ -574,7 +577,7 @@ int gather_shapes(int nl, int max_row, int offset_bytes, int lanes[4], int rows[
 
 bool gather_supported(int nl) {
   switch (nl) {
-    case 3: case 4: case 6: case 8: case 9: case 10: case 15: case 16: case 25: case 27:
+    case 3: case 4: case 6: case 8: case 9: case 10: case 15: case 16: case 25: case 27: case 64:
       return true;
   }
   return false;
@@ -592,6 +595,7 @@ hipError_t launch_gather(int nl, const LaunchArgs& A) {
     case 16: return gather_nl<16>(A);
     case 25: return gather_nl<25>(A);
     case 27: return gather_nl<27>(A);
+    case 64: return gather_nl<64>(A);
   }
   return hipErrorInvalidValue;
 }

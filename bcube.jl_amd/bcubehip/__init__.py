@@ -7,8 +7,8 @@ mirror of Bcube.jl's assembly API (src/assembler/assembler.jl).
 from ._lib import BcubeHipError, LIB_PATH, lib
 from .mesh import (AbstractBcubeBackend, BcubeBackendHIP, BcubeBackendHostOnly, Mesh,
                    batch_meshes, get_bcube_backend, hexa27_mesh, hexa8_to_hexa27, hexa_mesh,
-                   jitter_nodes, mixed_tri_quad_mesh, rectangle_mesh, sinusoidal_map,
-                   sub_mesh)
+                   jitter_nodes, mixed_tri_quad_mesh, one_cell_mesh, rectangle_mesh,
+                   sinusoidal_map, sub_mesh)
 from .fespace import (DofHandler, FunctionSpace, MultiFESpace, TestFESpace, TrialFESpace,
                       component_numbering)
 from .forms import (BoundaryFaceDomain, CellDomain, EulerFlux, FEFunction, InteriorFaceDomain,
@@ -20,7 +20,8 @@ from .faces import BoundaryFaces, FaceBilinearPlan
 from .dg import DGTransport, dg_dof_coordinates
 from .assembler import (BilinearPlan, MultiBilinearPlan, Pattern, assemble_bilinear,
                         assemble_linear,
-                        assemble_linear_, pattern_of, quadrature_points)
+                        assemble_linear_, fe_values_at_points, pattern_of,
+                        quadrature_points)
 from .dirichlet import (DirichletPlan, apply_dirichlet_to_matrix_, apply_dirichlet_to_vector_,
                         apply_homogeneous_dirichlet_, apply_homogeneous_dirichlet_to_matrix_,
                         apply_homogeneous_dirichlet_to_vector_, assemble_dirichlet_vector,
@@ -46,5 +47,6 @@ __all__ = [
     "nodes_gmsh2cgns", "read_mesh", "projection_l2_", "MultiFESpace", "component_numbering",
     "ddot", "MultiBilinearPlan", "BoundaryFaceDomain", "avg", "BoundaryFaces",
     "FaceBilinearPlan", "EulerFlux", "RoeFlux", "PeriodicBCType", "Translation", "EulerDG",
-    "periodic_face_pairs", "AffineFESystem", "OrderDegree",
+    "periodic_face_pairs", "AffineFESystem", "OrderDegree", "one_cell_mesh",
+    "fe_values_at_points",
 ]

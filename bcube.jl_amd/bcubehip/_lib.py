@@ -30,7 +30,7 @@ EXPORTED = [
     "bh_blocks_create", "bh_blocks_info", "bh_blocks_export_csc", "bh_blocks_block_values",
     "bh_blocks_assemble", "bh_blocks_assemble_block", "bh_blocks_add_field", "bh_blocks_destroy",
     "bh_pattern_export_csc", "bh_pattern_values_device", "bh_pattern_destroy",
-    "bh_assemble_bilinear", "bh_assemble_bilinear_noalloc", "bh_assemble_linear", "bh_quadrature_points", "bh_dg_create",
+    "bh_assemble_bilinear", "bh_assemble_bilinear_noalloc", "bh_assemble_linear", "bh_quadrature_points", "bh_fe_eval_qp", "bh_dg_create",
     "bh_dg_info", "bh_dg_update_geometry", "bh_dg_transport_residual", "bh_dg_minv_axpy", "bh_dg_transport_stage",
     "bh_dg_destroy", "bh_dgb_create", "bh_dgb_info", "bh_dgb_export_csc", "bh_dgb_assemble",
     "bh_dgb_boundary_points", "bh_dgb_boundary_linear", "bh_dgb_destroy", "bh_euler_create",
@@ -120,6 +120,7 @@ def lib():
         "bh_assemble_bilinear_noalloc": (C.c_int, [vp, C.c_int, P(BhTerm), C.c_int, C.c_int, vp]),
         "bh_assemble_linear": (C.c_int, [vp, C.c_int, P(BhTerm), C.c_int, C.c_int, vp, C.c_int]),
         "bh_quadrature_points": (C.c_int, [vp, C.c_int, vp, C.c_int, P(i32)]),
+        "bh_fe_eval_qp": (C.c_int, [vp, C.c_int, vp, vp, C.c_int]),
         "bh_dg_create": (C.c_int, [vp, C.c_int, vp, C.c_int, P(vp)]),
         "bh_dg_info": (C.c_int, [vp, P(i64), P(i64), P(i32)]),
         "bh_dg_update_geometry": (C.c_int, [vp]),

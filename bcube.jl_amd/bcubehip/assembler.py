@@ -18,7 +18,7 @@ from .faces import FaceBilinearPlan, assemble_linear_faces_, has_faces
 from .forms import (CellDomain, Placeholder, _split_sum, lower_bilinear, lower_dg_transport, lower_linear,
                     lower_multi_bilinear, lower_multi_linear)
 
-_COEF = {"const": L.COEF_CONST, "cell": L.COEF_CELL, "qp": L.COEF_QP}
+_COEF = {"const": L.COEF_CONST, "cell": L.COEF_CELL, "qp": L.COEF_QP, "fe": L.COEF_QP}
 _FORM = {"stiffness": L.FORM_STIFFNESS, "mass": L.FORM_MASS, "source": L.FORM_SOURCE}
 
 
@@ -255,7 +255,14 @@ class BilinearPlan:
         U = getattr(U, "U", U)
         V = getattr(V, "U", V)
         if U is not V:
-            raise NotImplementedError("assembly with distinct trial and test spaces")
+            # distinct trial and test spaces: one mesh, scalar, FP64
+            # (an ndofs(V) x ndofs(U) matrix)
+            if U.mesh is not V.mesh:
+                raise NotImplementedError("trial and test spaces on different meshes")
+            if getattr(U, "size", 1) != 1 or getattr(V, "size", 1) != 1:
+                raise NotImplementedError("distinct trial and test spaces: scalar spaces only")
+            if build != "host":
+                raise NotImplementedError("distinct trial and test spaces: host pattern build")
         self.U, self.V = U, V
         self.mode = L.MODE[mode]
         items = _integrals(a(Placeholder("trial", U), Placeholder("test", V)))
@@ -438,6 +445,12 @@ def _linear_bh_terms(lowered, measure, mesh):
                                                    dtype=np.float64).reshape(-1))
             keep.append(vals)
             t.coef = vals.ctypes.data
+        elif kind == "fe":
+            if data.space.mesh is not mesh:
+                raise NotImplementedError("an FE function of another mesh")
+            vals = fe_values_at_points(data, measure.degree).reshape(-1)
+            keep.append(vals)
+            t.coef = vals.ctypes.data
         terms.append(t)
     return keep, (L.BhTerm * len(terms))(*terms), len(terms)
 
@@ -503,6 +516,24 @@ def assemble_linear(l, V, T=np.float64, mode: str = "fp64") -> np.ndarray:
     b = np.zeros(V.get_ndofs(), dtype=np.float64)
     assemble_linear_(b, l, V, mode)
     return b
+
+
+def fe_values_at_points(u, degree: int) -> np.ndarray:
+    """Values of the FEFunction u at the quadrature points of Measure(Ω, degree),
+    (ncells, nq), computed on the GPU (bh_fe_eval_qp)."""
+    U = u.space
+    mesh = U.mesh
+    ctx = mesh.backend.ctx
+    nq = C.c_int32()
+    L.check(L.lib().bh_quadrature_points(mesh.handle, degree, None, L.HOST, C.byref(nq)), ctx,
+            "bh_quadrature_points")
+    vals = np.ascontiguousarray(u.values, dtype=np.float64)
+    if vals.shape != (U.get_ndofs(),):
+        raise ValueError("FE function values do not match its space")
+    out = np.zeros(mesh.ncells * nq.value, dtype=np.float64)
+    L.check(L.lib().bh_fe_eval_qp(U.handle, degree, vals.ctypes.data, out.ctypes.data, L.HOST),
+            ctx, "bh_fe_eval_qp")
+    return out.reshape(mesh.ncells, nq.value)
 
 
 def quadrature_points(mesh, degree: int) -> np.ndarray:

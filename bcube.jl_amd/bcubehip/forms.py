@@ -425,6 +425,13 @@ def _coef(factors):
             if data is not None:
                 raise NotImplementedError("product of two non-constant coefficients")
             data, kind = f.f, "qp"
+        elif isinstance(f, FEFunction):
+            # materialised at the quadrature points on the device (bh_fe_eval_qp)
+            if data is not None:
+                raise NotImplementedError("product of two non-constant coefficients")
+            if np.ndim(f.values) != 1:
+                raise NotImplementedError("an FE function of several fields as a coefficient")
+            data, kind = f, "fe"
         else:
             raise NotImplementedError(f"unsupported factor {type(f).__name__}")
     return kind, data, scale
@@ -450,7 +457,7 @@ def lower_bilinear(expr):
         coefs = [f for f in inner if f not in gu and f not in gv] + rest
         if len(gu) == 1 and len(gv) == 1 and (gu[0] in left) != (gv[0] in left):
             kind, data, scale = _coef(coefs)
-            if kind == "qp":
+            if kind in ("qp", "fe"):
                 raise NotImplementedError("stiffness coefficient must be constant or per cell")
             return "stiffness", kind, data, scale
     elif not dots:
@@ -458,7 +465,7 @@ def lower_bilinear(expr):
         vs = [f for f in rest if _is(f, Placeholder, "test")]
         if len(us) == 1 and len(vs) == 1:
             kind, data, scale = _coef([f for f in rest if f is not us[0] and f is not vs[0]])
-            if kind == "qp":
+            if kind in ("qp", "fe"):
                 raise NotImplementedError("mass coefficient must be constant or per cell")
             return "mass", kind, data, scale
     raise NotImplementedError("bilinear integrand is not one of: η ∇u⋅∇v, c u v")

@@ -210,6 +210,28 @@ class Mesh:
             pass
 
 
+def one_cell_mesh(kind: str, xmin=-1.0, xmax=1.0, ymin=-1.0, ymax=1.0, zmin=-1.0, zmax=1.0,
+                  backend: AbstractBcubeBackend | None = None) -> Mesh:
+    """one_cell_mesh(type; xmin, xmax, ...) (mesh_generator.jl:38-65) for one
+    first-order cell: :quad/:square (Quad4, one_quad_mesh :600-608), :tri/
+    :triangle (Tri3, one_tri_mesh :576-581), :hexa/:cube (Hexa8, one_hexa_mesh
+    :681-695), nodes in those functions' order."""
+    kind = kind.lstrip(":")
+    if kind in ("quad", "square"):
+        nodes = [[xmin, ymin], [xmax, ymin], [xmax, ymax], [xmin, ymax]]
+        m = Mesh(nodes, ["Quad4"], [[0, 1, 2, 3]], backend)
+        m.bc_nodes = {"xmin": np.array([0, 3]), "xmax": np.array([1, 2]),
+                      "ymin": np.array([0, 1]), "ymax": np.array([2, 3])}
+        return m
+    if kind in ("tri", "triangle"):
+        return Mesh([[xmin, ymin], [xmax, ymin], [xmin, ymax]], ["Tri3"], [[0, 1, 2]], backend)
+    if kind in ("hexa", "cube"):
+        nodes = [[xmin, ymin, zmin], [xmax, ymin, zmin], [xmax, ymax, zmin], [xmin, ymax, zmin],
+                 [xmin, ymin, zmax], [xmax, ymin, zmax], [xmax, ymax, zmax], [xmin, ymax, zmax]]
+        return Mesh(nodes, ["Hexa8"], [list(range(8))], backend)
+    raise NotImplementedError(f"one_cell_mesh: {kind}")
+
+
 def rectangle_mesh(nx, ny, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, order=1,
                    backend: AbstractBcubeBackend | None = None,
                    bnd_names=("xmin", "xmax", "ymin", "ymax")) -> Mesh:

@@ -973,7 +973,9 @@ int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out) {
   p->V = V;
   p->dev = V->dev;
   std::string err;
-  if (!build_pattern(V->dofs, V->nl, V->ndofs, U->dofs, U->nl, U->ndofs, V->mesh->h.ncells, p->h, err))
+  // internal rows: U's dofs (the CSC columns), their entries V's dofs (the
+  // CSC row indices); for U == V the CSR and CSC arrays coincide
+  if (!build_pattern(U->dofs, U->nl, U->ndofs, V->dofs, V->nl, V->ndofs, V->mesh->h.ncells, p->h, err))
     return fail(c, BH_ERR_ARG, err);
   HostPattern& h = p->h;
   if (!c) {  // host-only pattern: structure queries only
@@ -2127,8 +2129,9 @@ int bh_euler_destroy(bh_euler e) {
 
 int bh_pattern_info(bh_pattern p, int64_t* nrows, int64_t* ncols, int64_t* nnz, int32_t* max_row) {
   if (!p) return BH_ERR_ARG;
-  if (nrows) *nrows = p->h.nrows;
-  if (ncols) *ncols = p->h.ncols;
+  // the matrix is ndofs(V) x ndofs(U); the pattern's internal rows are U's dofs
+  if (nrows) *nrows = p->h.ncols;
+  if (ncols) *ncols = p->h.nrows;
   if (nnz) *nnz = p->h.nnz;
   if (max_row) *max_row = p->h.max_row;
   return BH_OK;
@@ -2136,8 +2139,6 @@ int bh_pattern_info(bh_pattern p, int64_t* nrows, int64_t* ncols, int64_t* nnz, 
 
 int bh_pattern_export_csc(bh_pattern p, int64_t* colptr, int64_t* rowval, int base) {
   if (!p || !colptr || !rowval) return BH_ERR_ARG;
-  if (p->U != p->V)
-    return fail(p->V->mesh->ctx, BH_ERR_UNSUPPORTED, "CSC export of a rectangular pattern");
   const HostPattern& h = p->h;
   for (size_t k = 0; k < h.rowptr.size(); ++k) colptr[k] = h.rowptr[k] + base;
   for (int64_t k = 0; k < h.nnz; ++k) rowval[k] = (int64_t)h.colval[k] + base;
@@ -2330,6 +2331,100 @@ static int ensure_sf(bh_pattern p, bh_ctx c, bool upload) {
   return BH_OK;
 }
 
+// assemble_bilinear(a, U, V) with U != V (assembler.jl:172-186: an
+// ndofs(V) x ndofs(U) SparseMatrixCSC): FP64 element rows a(φ_j, ψ_i) per
+// (cell, trial dof j) by k_element_rect_f64, then the row gather over the CSC
+// columns (the pattern's internal rows are U's dofs, their entries V's).
+// Uniform meshes whose (cell type, trial order, test order, rule) has an
+// instantiation.
+static int assemble_rect(bh_pattern p, int nterms, const bh_term* terms, int quad_degree,
+                         int mode, double* nzval, int where, int coef_where, bool noalloc) {
+  bh_space U = p->U, V = p->V;
+  bh_mesh m = V->mesh;
+  bh_ctx c = m->ctx;
+  if (noalloc) return fail(c, BH_ERR_UNSUPPORTED, "the no-allocation call needs U == V");
+  if (mode != BH_FP64) return fail(c, BH_ERR_UNSUPPORTED, "assembly with U != V is FP64 only");
+  if (!U->orders.empty() || !V->orders.empty())
+    return fail(c, BH_ERR_UNSUPPORTED, "U != V: scalar spaces of one order each");
+  if (m->geo < 0 || m->buckets.size() != 1)
+    return fail(c, BH_ERR_UNSUPPORTED, "U != V: uniform meshes only");
+  if (quad_degree <= 0) return fail(c, BH_ERR_ARG, "bad quadrature degree");
+  const bh_term* stiff = nullptr;
+  const bh_term* mass = nullptr;
+  for (int k = 0; k < nterms; ++k) {
+    const bh_term& t = terms[k];
+    if (t.coef_kind != BH_COEF_CONST && t.coef_kind != BH_COEF_CELL)
+      return fail(c, BH_ERR_UNSUPPORTED, "bilinear coefficients must be constant or per cell");
+    if (t.coef_kind == BH_COEF_CELL && !t.coef) return fail(c, BH_ERR_ARG, "per-cell coefficient is NULL");
+    if (t.form == BH_FORM_STIFFNESS && !stiff) stiff = &t;
+    else if (t.form == BH_FORM_MASS && !mass) mass = &t;
+    else return fail(c, BH_ERR_UNSUPPORTED, "unsupported or repeated bilinear term");
+  }
+  const GeoBucket& gb = m->buckets[0];
+  const int key = rule_key(gb.geo, quad_degree);
+  if (!rect_supported(gb.geo, U->order, V->order, key))
+    return fail(c, BH_ERR_UNSUPPORTED,
+                "no kernel for this (cell type, trial order, test order, quadrature)");
+  if (!gather_supported(V->nl))
+    return fail(c, BH_ERR_UNSUPPORTED, "no row-gather kernel for this element size");
+  const HostPattern& h = p->h;
+  if (h.max_row > 512) return fail(c, BH_ERR_UNSUPPORTED, "CSC columns longer than 512 entries");
+  if (int r = set_device(c)) return r;
+  const int64_t nc = m->h.ncells;
+  double *tab_u = nullptr, *tab_v = nullptr;
+  if (int r = get_tables(c, gb.geo, U->order, quad_degree, &tab_u)) return r;
+  if (int r = get_tables(c, gb.geo, V->order, quad_degree, &tab_v)) return r;
+  if (!p->d_ke) {
+    if (c->capturing)
+      return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
+    BH_HIP(c, dalloc((double**)&p->d_ke, (size_t)nc * U->nl * ke_stride(V->nl)));
+  }
+  LaunchArgs A;
+  A.kind = LaunchArgs::BILINEAR;
+  A.mode = BH_FP64;
+  A.stream = c->stream;
+  A.coords = m->d_coords;
+  A.c2n = gb.d_c2n;
+  A.tab = tab_v;
+  A.tab_trial = tab_u;
+  A.ncells_total = nc;
+  A.ke = p->d_ke;
+  A.ke_fp64 = true;
+  if (stiff) {
+    A.eta = coef_ptr(*stiff, coef_where, p->d_coef, nc, c->stream);
+    A.eta_scale = stiff->scale;
+  }
+  if (mass) {
+    A.mass = true;
+    A.mcoef = coef_ptr(*mass, coef_where, p->d_coef + nc, nc, c->stream);
+    A.mass_scale = mass->scale;
+  }
+  time_begin(c, 0);
+  BH_HIP(c, launch_rect(gb.geo, U->order, V->order, key, A));
+  time_end(c);
+  LaunchArgs G = A;
+  G.kind = LaunchArgs::GATHER;
+  G.pptr = p->d_pptr;
+  G.pairs = p->d_pairs;
+  G.padded_pairs = h.pad > 0;
+  G.max_adj = h.pad;
+  G.goff = p->d_goff;
+  G.offset_bytes = h.offset_bytes;
+  G.rowptr = p->d_rowptr;
+  G.nrows = h.nrows;
+  G.max_row = h.max_row;
+  G.vals = (where == BH_DEVICE && nzval) ? nzval : p->d_vals;
+  time_begin(c, 1);
+  BH_HIP(c, launch_gather(V->nl, G));
+  time_end(c);
+  if (where == BH_HOST) {
+    BH_HIP(c, hipMemcpyAsync(nzval, p->d_vals, (size_t)h.nnz * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    BH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return BH_OK;
+}
+
 static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms,
                                   int quad_degree, int mode, double* nzval, int where,
                                   int coef_where, void* ke_ext = nullptr, size_t ke_cap = 0) {
@@ -2339,7 +2434,12 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
   bh_mesh m = V->mesh;
   bh_ctx c = m->ctx;
   if (!c) return fail(c, BH_ERR_ARG, "assembly needs a mesh created with a context");
-  if (p->U != p->V) return fail(c, BH_ERR_UNSUPPORTED, "assembly with U != V is not supported yet");
+  if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
+  if (where == BH_HOST && !nzval) return fail(c, BH_ERR_ARG, "nzval is NULL");
+  if ((where == BH_HOST || coef_where == BH_HOST) && c->capturing)
+    return fail(c, BH_ERR_ARG, "host-`where` assembly cannot be captured in a graph");
+  if (p->U != p->V)
+    return assemble_rect(p, nterms, terms, quad_degree, mode, nzval, where, coef_where, noalloc);
   if (mode != BH_FP64 && mode != BH_FP32 && mode != BH_BF16X3)
     return fail(c, BH_ERR_UNSUPPORTED, "bilinear modes: BH_FP64, BH_FP32, BH_BF16X3");
   if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
@@ -3279,6 +3379,66 @@ int bh_graph_destroy(bh_graph g) {
   (void)hipGraphExecDestroy(g->exec);
   (void)hipGraphDestroy(g->graph);
   delete g;
+  return BH_OK;
+}
+
+int bh_fe_eval_qp(bh_space U, int quad_degree, const double* u, double* uq, int where) {
+  if (!U || !u || !uq) return BH_ERR_ARG;
+  bh_mesh m = U->mesh;
+  bh_ctx c = m->ctx;
+  if (!c) return fail(c, BH_ERR_ARG, "needs a mesh created with a context");
+  if (where != BH_HOST && where != BH_DEVICE) return fail(c, BH_ERR_ARG, "bad `where`");
+  if (c->capturing && where == BH_HOST)
+    return fail(c, BH_ERR_ARG, "host-`where` call cannot be captured in a graph");
+  if (!U->orders.empty())
+    return fail(c, BH_ERR_UNSUPPORTED, "FE functions of a space of per-cell order");
+  if (m->geo == -1 || m->buckets.empty())
+    return fail(c, BH_ERR_UNSUPPORTED, "FE function values need the cell types of assembly");
+  int32_t nq = 0;
+  if (int r = bh_quadrature_points(m, quad_degree, nullptr, BH_HOST, &nq)) return r;
+  std::vector<int> bkey(m->buckets.size());
+  for (size_t b = 0; b < m->buckets.size(); ++b) {
+    bkey[b] = rule_key(m->buckets[b].geo, quad_degree);
+    if (!kernel_supported(m->buckets[b].geo, U->order, bkey[b]))
+      return fail(c, BH_ERR_UNSUPPORTED, "no tables for this (cell type, order, quadrature)");
+  }
+  if (int r = set_device(c)) return r;
+  std::vector<double*> btab(m->buckets.size(), nullptr);
+  for (size_t b = 0; b < m->buckets.size(); ++b)
+    if (int r = get_tables(c, m->buckets[b].geo, U->order, quad_degree, &btab[b])) return r;
+  const int64_t n = m->h.ncells * nq;
+  const double* du = u;
+  double* dq = uq;
+  double* stage = nullptr;
+  if (where == BH_HOST) {
+    BH_HIP(c, dalloc(&stage, (size_t)(U->ndofs + n)));
+    BH_HIP(c, hipMemcpyAsync(stage, u, U->ndofs * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    du = stage;
+    dq = stage + U->ndofs;
+  }
+  hipError_t e = hipSuccess;
+  for (size_t b = 0; b < m->buckets.size() && e == hipSuccess; ++b) {
+    const GeoBucket& gb = m->buckets[b];
+    LaunchArgs A;
+    A.kind = LaunchArgs::FE_EVAL;
+    A.stream = c->stream;
+    A.rdofs = U->d_dofs;
+    A.erow = U->nl;
+    A.tab = btab[b];
+    A.cells = gb.d_cells;  // NULL on a uniform mesh
+    A.ncells_total = gb.n;
+    A.fstride = nq;
+    A.f = du;
+    A.vals = dq;
+    e = launch(gb.geo, U->order, bkey[b], A);
+  }
+  if (e == hipSuccess && where == BH_HOST) {
+    e = hipMemcpyAsync(uq, dq, n * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  if (stage) (void)hipFree(stage);
+  if (e != hipSuccess) return fail(c, BH_ERR_HIP, hipGetErrorString(e));
   return BH_OK;
 }
 

@@ -10,7 +10,7 @@
 namespace bh {
 
 struct LaunchArgs {
-  enum Kind { BILINEAR = 0, SOURCE = 1, QPOINTS = 2, GATHER = 3 };
+  enum Kind { BILINEAR = 0, SOURCE = 1, QPOINTS = 2, GATHER = 3, FE_EVAL = 4 };
   int kind = BILINEAR;
   int mode = BH_FP64;
   hipStream_t stream = nullptr;
@@ -18,6 +18,7 @@ struct LaunchArgs {
   const int32_t* c2n = nullptr;    // ncells * nodes-per-cell (geometric nodes)
   const int32_t* rdofs = nullptr;  // ncells * nl
   const double* tab = nullptr;     // packed reference tables
+  const double* tab_trial = nullptr;  // rectangular element kernel: the trial space's tables
   const float* ptab = nullptr;     // reference products P of the GEMM element kernel (bh_gp.hip)
   const int32_t* cells = nullptr;  // cells of this colour (source) / of this cell type (bilinear)
   const int32_t* lcells = nullptr; // source on a mixed mesh: the cells' rows in this type's c2n
@@ -54,7 +55,7 @@ struct LaunchArgs {
   bool mass = false;
   const int64_t* rowptr = nullptr;
   int offset_bytes = 1;
-  // source
+  // source (FE_EVAL: f holds the FE function's dof values)
   int fkind = BH_COEF_CONST;
   const double* f = nullptr;
   double fscale = 0.0;
@@ -74,6 +75,10 @@ constexpr int ke_stride(int nl) {
 bool kernel_supported(int geo_celltype, int order, int nq1);
 int packed_table_size(int geo_celltype, int order, int nq1);
 hipError_t launch(int geo_celltype, int order, int nq1, const LaunchArgs& a);
+// U != V: element rows a(φ_j, ψ_i) per (cell, trial dof j), FP64, on uniform
+// meshes of the cell types and order pairs with an instantiation.
+bool rect_supported(int geo_celltype, int order_u, int order_v, int nq1);
+hipError_t launch_rect(int geo_celltype, int order_u, int order_v, int nq1, const LaunchArgs& a);
 hipError_t launch_fill(hipStream_t s, double* p, int64_t n, double v);
 hipError_t launch_fill_i32(hipStream_t s, int32_t* p, int64_t n, int32_t v);
 hipError_t launch_fill_u64(hipStream_t s, uint64_t* p, int64_t n, uint64_t v);

@@ -152,6 +152,146 @@ __global__ __launch_bounds__(kBlock) void k_element_f64(
 }
 
 // -------------------------------------------------------------------------
+// Element matrices of a(u, v) with a trial space U of NLU local dofs and a
+// test space V of NLV (assemble_bilinear(a, U, V) with U != V,
+// assembler.jl:172-186, whose block value [j][i] is a(λu_j, λv_i),
+// assembler.jl:1185-1220), FP64.  One lane per (cell, trial dof j) writes
+//   ke[(gcell*NLU + j)*KS + i] = Σ_q ∇̂φ_qj·G_q ∇̂ψ_qi + m_q φ_qj ψ_qi
+// for every test dof i: the element rows of the row gather over the CSC
+// columns (trial dofs), whose entries are test dofs.  Geometry and weights
+// come from V's table (both tables share the rule and the geometric nodes).
+// -------------------------------------------------------------------------
+template <int D, int NLU, int NLV, int NQ, int NG, bool MASS>
+__global__ __launch_bounds__(kBlock) void k_element_rect_f64(
+    const double* __restrict__ coords, const int32_t* __restrict__ c2n,
+    const double* __restrict__ tab_u, const double* __restrict__ tab_v, int ncells,
+    const double* __restrict__ eta, double eta_scale, const double* __restrict__ mcoef,
+    double mass_scale, double* __restrict__ ke) {
+  using LU = TabLayout<D, NLU, NQ, NG>;
+  using LV = TabLayout<D, NLV, NQ, NG>;
+  constexpr int CPB = kBlock / NLU;
+  constexpr int NS = SymN<D>::N;
+  constexpr int KS = KeStride<NLV>::value;
+
+  __shared__ double s_ru[NQ * NLU * 4];  // trial basis: ∇̂φ, φ per (q, j)
+  __shared__ double s_tab[LV::G];        // weights, test basis, geometric gradients
+  __shared__ double s_x[CPB][NG][D];
+  __shared__ double s_g[CPB][NQ][NS + 1];
+
+  const int tid = threadIdx.x;
+  for (int k = tid; k < NQ * NLU * 4; k += kBlock) s_ru[k] = tab_u[LU::R + k];
+  for (int k = tid; k < LV::G; k += kBlock) s_tab[k] = tab_v[k];
+
+  const int ngroups = (ncells + CPB - 1) / CPB;
+  const int cs = tid / NLU;
+  const int lj = tid - cs * NLU;
+
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int base = grp * CPB;
+    __syncthreads();
+    for (int k = tid; k < CPB * NG; k += kBlock) {
+      int c = k / NG, n = k - c * NG;
+      if (base + c < ncells) {
+        int node = c2n[(int64_t)(base + c) * NG + n];
+#pragma unroll
+        for (int a = 0; a < D; ++a) s_x[c][n][a] = coords[(int64_t)node * D + a];
+      }
+    }
+    __syncthreads();
+    for (int k = tid; k < CPB * NQ; k += kBlock) {
+      int c = k / NQ, q = k - c * NQ;
+      if (base + c >= ncells) continue;
+      const int cell = base + c;
+      double J[D][D];
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = 0; b < D; ++b) J[a][b] = 0.0;
+#pragma unroll
+      for (int n = 0; n < NG; ++n)
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+          double dn = s_tab[LV::DG + (q * NG + n) * D + b];
+#pragma unroll
+          for (int a = 0; a < D; ++a) J[a][b] += s_x[c][n][a] * dn;
+        }
+      double Ji[D][D];
+      double det = invert<D>(J, Ji);
+      double wdet = s_tab[LV::W + q] * fabs(det);
+      double ge = wdet * (eta ? eta[cell] * eta_scale : eta_scale);
+      int s = 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = a; b < D; ++b) {
+          double v = 0.0;
+#pragma unroll
+          for (int t = 0; t < D; ++t) v += Ji[a][t] * Ji[b][t];
+          s_g[c][q][s++] = ge * v;
+        }
+      s_g[c][q][NS] = MASS ? wdet * (mcoef ? mcoef[cell] * mass_scale : mass_scale) : 0.0;
+    }
+    __syncthreads();
+    if (cs >= CPB || base + cs >= ncells) continue;
+    double acc[NLV];
+#pragma unroll
+    for (int i = 0; i < NLV; ++i) acc[i] = 0.0;
+#pragma unroll 1
+    for (int q = 0; q < NQ; ++q) {
+      const double* g = s_g[cs][q];
+      const double* rj = &s_ru[(q * NLU + lj) * 4];
+      double t[D];
+      if constexpr (D == 2) {
+        t[0] = g[0] * rj[0] + g[1] * rj[1];
+        t[1] = g[1] * rj[0] + g[2] * rj[1];
+      } else {
+        t[0] = g[0] * rj[0] + g[1] * rj[1] + g[2] * rj[2];
+        t[1] = g[1] * rj[0] + g[3] * rj[1] + g[4] * rj[2];
+        t[2] = g[2] * rj[0] + g[4] * rj[1] + g[5] * rj[2];
+      }
+      const double mp = MASS ? g[NS] * rj[D] : 0.0;
+      const double* rq = &s_tab[LV::R + q * NLV * 4];
+#pragma unroll
+      for (int i = 0; i < NLV; ++i) {
+        double v = acc[i];
+#pragma unroll
+        for (int a = 0; a < D; ++a) v = fma(rq[i * 4 + a], t[a], v);
+        if (MASS) v = fma(rq[i * 4 + D], mp, v);
+        acc[i] = v;
+      }
+    }
+    double* out = ke + ((int64_t)(base + cs) * NLU + lj) * KS;
+#pragma unroll
+    for (int i = 0; i < NLV; ++i) out[i] = acc[i];
+  }
+}
+
+// -------------------------------------------------------------------------
+// Values of an FE function at the quadrature points, u_h(ξ_q) = Σ_k u[dof_k] φ_qk
+// (the materialised FEFunction of a CellPoint, fefunction.jl,
+// eval_point.jl:107-111), in the layout of BH_COEF_QP coefficients: qstride
+// points per cell, cells of this launch to rows gcells[k] (NULL: k), the
+// slots q >= NQ repeating the first point.
+// -------------------------------------------------------------------------
+template <int D, int NL, int NQ, int NG>
+__global__ __launch_bounds__(kBlock) void k_fe_eval(const int32_t* __restrict__ rdofs, int rstride,
+                                                    const double* __restrict__ tab, int64_t ncells,
+                                                    const int32_t* __restrict__ gcells, int qstride,
+                                                    const double* __restrict__ u,
+                                                    double* __restrict__ uq) {
+  using L = TabLayout<D, NL, NQ, NG>;
+  int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= ncells * qstride) return;
+  int64_t c = k / qstride;
+  int q = (int)(k - c * qstride);
+  const int qe = q < NQ ? q : 0;
+  const int64_t cell = gcells ? (int64_t)gcells[c] : c;
+  double v = 0.0;
+  for (int l = 0; l < NL; ++l) v = fma(u[rdofs[cell * rstride + l]], tab[L::R + (qe * NL + l) * 4 + D], v);
+  uq[cell * qstride + q] = v;
+}
+
+// -------------------------------------------------------------------------
 // Row gather: vals[row] = Σ over (cell, i) touching row, in cell order, of
 // element row ke[cell][i][:] placed by goff.
 //   k_gather_padded: pair lists padded to MAXP slots per row (-1 = empty);
@@ -490,6 +630,12 @@ static hipError_t launch_all(const LaunchArgs& A) {
                        A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.lcells,
                        A.ncells_color, A.erow > 0 ? A.erow : NL, A.fstride > 0 ? A.fstride : NQ,
                        A.fkind, A.f, A.fscale, A.vals);
+  } else if (A.kind == LaunchArgs::FE_EVAL) {
+    const int qs = A.fstride > 0 ? A.fstride : NQ;
+    int64_t n = A.ncells_total * qs;
+    hipLaunchKernelGGL((k_fe_eval<D, NL, NQ, NG>), dim3((unsigned)((n + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, A.stream, A.rdofs, A.erow > 0 ? A.erow : NL, A.tab,
+                       A.ncells_total, A.cells, qs, A.f, A.vals);
   } else {
     const int qs = A.fstride > 0 ? A.fstride : NQ;
     int64_t n = A.ncells_total * qs;
@@ -549,6 +695,55 @@ bool kernel_supported(int geo, int order, int nq1) {
 hipError_t launch(int geo, int order, int nq1, const LaunchArgs& A) {
   for (const Entry& e : kTable)
     if (e.geo == geo && e.order == order && e.nq1 == nq1) return e.fn(A);
+  return hipErrorInvalidValue;
+}
+
+// Rectangular element kernels (U != V): the trial and test orders of one
+// cell type on one rule.
+template <int D, int NLU, int NLV, int NQ, int NG>
+static hipError_t launch_rect_t(const LaunchArgs& A) {
+  constexpr int CPB = kBlock / NLU;
+  int ng = (int)((A.ncells_total + CPB - 1) / CPB);
+  dim3 grid(grid_for(ng));
+#define BH_R(M)                                                                                 \
+  hipLaunchKernelGGL((k_element_rect_f64<D, NLU, NLV, NQ, NG, M>), grid, dim3(kBlock), 0,       \
+                     A.stream, A.coords, A.c2n, A.tab_trial, A.tab, (int)A.ncells_total, A.eta, \
+                     A.eta_scale, A.mcoef, A.mass_scale, (double*)A.ke)
+  if (A.mass) BH_R(true);
+  else BH_R(false);
+#undef BH_R
+  return hipGetLastError();
+}
+
+struct RectEntry {
+  int geo, order_u, order_v, nq1;
+  hipError_t (*fn)(const LaunchArgs&);
+};
+#define BH_RQ(PU, PV, NQ1) \
+  {BH_QUAD4, PU, PV, NQ1, &launch_rect_t<2, (PU + 1) * (PU + 1), (PV + 1) * (PV + 1), NQ1 * NQ1, 4>}
+#define BH_RT(PU, PV, NPTS)                                                                    \
+  {BH_TRI3, PU, PV, NPTS,                                                                      \
+   &launch_rect_t<2, (PU + 1) * (PU + 2) / 2, (PV + 1) * (PV + 2) / 2, NPTS, 3>}
+#define BH_RH(PU, PV, NQ1)                                                                     \
+  {BH_HEXA8, PU, PV, NQ1,                                                                      \
+   &launch_rect_t<3, (PU + 1) * (PU + 1) * (PU + 1), (PV + 1) * (PV + 1) * (PV + 1),            \
+                  NQ1 * NQ1 * NQ1, 8>}
+static const RectEntry kRectTable[] = {
+    BH_RQ(1, 2, 2), BH_RQ(2, 1, 2), BH_RQ(1, 2, 3), BH_RQ(2, 1, 3),
+    BH_RT(1, 2, 3), BH_RT(2, 1, 3), BH_RT(1, 2, 4), BH_RT(2, 1, 4), BH_RT(1, 2, 6), BH_RT(2, 1, 6),
+    BH_RH(1, 2, 2), BH_RH(2, 1, 2), BH_RH(1, 2, 3), BH_RH(2, 1, 3),
+};
+
+bool rect_supported(int geo, int order_u, int order_v, int nq1) {
+  for (const RectEntry& e : kRectTable)
+    if (e.geo == geo && e.order_u == order_u && e.order_v == order_v && e.nq1 == nq1) return true;
+  return false;
+}
+
+hipError_t launch_rect(int geo, int order_u, int order_v, int nq1, const LaunchArgs& A) {
+  for (const RectEntry& e : kRectTable)
+    if (e.geo == geo && e.order_u == order_u && e.order_v == order_v && e.nq1 == nq1)
+      return e.fn(A);
   return hipErrorInvalidValue;
 }
 

@@ -163,7 +163,9 @@ int bh_space_destroy(bh_space space);
 /* ---- sparsity pattern --------------------------------------------------- */
 /* The CSC structure sparse(I,J,X,m,n) produces (assembler.jl:185), built once
  * per (U, V) pair: union of element dof-pair graphs, rows sorted per column,
- * explicit zeros kept.  Also builds, per row, the list of element rows that
+ * explicit zeros kept.  U and V live on one mesh; with U != V (different
+ * orders) the matrix is ndofs(V) x ndofs(U) (bh_pattern_info's nrows x
+ * ncols) and the gather runs over its columns.  Also builds, per row, the list of element rows that
  * touch it (in cell order) and their column positions: the deterministic
  * segmented reduction of the assembly. */
 int bh_pattern_build(bh_space U, bh_space V, bh_pattern* out);
@@ -221,7 +223,10 @@ int bh_pattern_destroy(bh_pattern p);
 /* ---- assembly ------------------------------------------------------------ */
 /* assemble_bilinear(a, U, V) (assembler.jl:172-186) for a = Σ terms on
  * CellDomain with Measure(Ω, quad_degree).  nzval (nnz) receives the CSC
- * values in export order.  With where == BH_DEVICE, nzval may be NULL to
+ * values in export order.  U != V (a pattern of two spaces): BH_FP64 on
+ * uniform Quad4, Tri3 and Hexa8 meshes, trial/test orders 1/2 and 2/1, the
+ * value of column j, row i being a(φ_j, ψ_i) as in the reference's element
+ * block (assembler.jl:1185-1220).  With where == BH_DEVICE, nzval may be NULL to
  * assemble into the pattern's own buffer (bh_pattern_values_device). */
 int bh_assemble_bilinear(bh_pattern p, int nterms, const bh_term* terms, int quad_degree,
                          int mode, double* nzval, int where);
@@ -258,6 +263,15 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
  * every cell gets the larger rule's count, its unused slots repeating its
  * first point; BH_COEF_QP coefficients use the same layout. */
 int bh_quadrature_points(bh_mesh mesh, int quad_degree, double* xq, int where, int32_t* nq);
+/* Values at every quadrature point of Measure(Ω, quad_degree) of the FE
+ * function with dof values u (ndofs(U) of them): u_h(x_q) = Σ_k u[dof_k]
+ * φ_k(ξ_q), the FEFunction the reference evaluates at each CellPoint of an
+ * integrand (fefunction.jl, eval_point.jl:107-111), e.g. ∫(u*v)dΩ with u on
+ * another space than v (test_assembler.jl:241-269).  uq receives ncells*nq
+ * FP64 values in the layout of BH_COEF_QP coefficients (nq as
+ * bh_quadrature_points reports), so it feeds bh_assemble_linear as is.  u and
+ * uq are host arrays (where = BH_HOST) or device memory (BH_DEVICE). */
+int bh_fe_eval_qp(bh_space U, int quad_degree, const double* u, double* uq, int where);
 
 /* ---- Dirichlet conditions ------------------------------------------------ */
 /* The device side of apply_dirichlet_to_matrix! / apply_dirichlet_to_vector! /

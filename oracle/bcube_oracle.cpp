@@ -1130,6 +1130,104 @@ static void build_coo(const Mesh& m, const std::vector<i64>& iglob, const std::v
   }
 }
 
+// Element block with distinct trial order pu and test order pv on cell c
+// (the same loop as element_block, assembler.jl:280-307, with λu from U's
+// basis and λv from V's): acc[j*nv + i] = a(λu_j, λv_i).
+static void element_block_rect(const Mesh& m, int pu, int pv, int term, const double* eta,
+                               ShapeRules& R, i64 c, int& nu, int& nv, std::vector<double>& acc) {
+  int d = m.dim;
+  int ct = m.ctype[c];
+  Shape s = shape_of(ct);
+  nu = ndofs_lagrange(s, pu);
+  nv = ndofs_lagrange(s, pv);
+  const std::vector<double>& w = R.weights(s);
+  const std::vector<double>& xq = R.points(s);
+  int nq = R.count(s);
+  const i64* nodes = &m.c2n[m.c2n_ptr[c]];
+  const i64 nn = m.c2n_ptr[c + 1] - m.c2n_ptr[c];
+  double Xc[27 * 3];
+  for (i64 k = 0; k < nn; ++k)
+    for (int a = 0; a < d; ++a) Xc[k * d + a] = m.coords[nodes[k] * d + a];
+  acc.assign((size_t)nu * nv, 0.0);
+  std::vector<double> vu(nu), ru(nu * d), gu(nu * d), vv(nv), rv(nv * d), gv(nv * d);
+  for (int q = 0; q < nq; ++q) {
+    Geo g;
+    geometry(ct, Xc, &xq[q * d], g);
+    tensor_basis(s, pu, &xq[q * d], vu.data(), ru.data());
+    tensor_basis(s, pv, &xq[q * d], vv.data(), rv.data());
+    phys_grad(d, nu, ru.data(), g.Jinv, gu.data());
+    phys_grad(d, nv, rv.data(), g.Jinv, gv.data());
+    for (int j = 0; j < nu; ++j)
+      for (int i = 0; i < nv; ++i) {
+        double gij;
+        if (term == T_STIFF) {
+          double e = eta ? eta[c] : 1.0;
+          double sacc = (e * gu[j * d + 0]) * gv[i * d + 0];
+          for (int a = 1; a < d; ++a) sacc = sacc + (e * gu[j * d + a]) * gv[i * d + a];
+          gij = sacc;
+        } else {
+          gij = vu[j] * vv[i];
+        }
+        double wu = w[q] * (g.detabs * gij);
+        size_t k = (size_t)j * nv + i;
+        acc[k] = (q == 0) ? wu : acc[k] + wu;
+      }
+  }
+}
+
+// Linear forms with an FE function u of order pu on the same mesh (its dof
+// values ug in U's numbering), tested against V of order pv
+// (__assemble_linear! assembler.jl:523-533 with the FEFunction materialised at
+// each CellPoint, fefunction.jl, eval_point.jl:107-111):
+//   kind 0: ∫(u v)dΩ             b_i += Σ_q w|J| u_h(x_q) λv_i(x_q)
+//   kind 1: ∫((c u)⋅∇v)dΩ        b_i += Σ_q w|J| u_h(x_q) (c⋅∇λv_i(x_q))
+// (test_assembler.jl:40-68, 241-269); cells in element order.
+static void assemble_linear_fe(const Mesh& m, const std::vector<i64>& igu,
+                               const std::vector<i64>& offu, int pu,
+                               const std::vector<i64>& igv, const std::vector<i64>& offv, int pv,
+                               const double* ug, int kind, const double* cvec, int degree,
+                               std::vector<double>& b) {
+  const int d = m.dim;
+  ShapeRules R;
+  R.degree = degree;
+  for (i64 c : element_order(m)) {
+    const int ct = m.ctype[c];
+    const Shape s = shape_of(ct);
+    R.fill(s);
+    const int nu = ndofs_lagrange(s, pu), nv = ndofs_lagrange(s, pv);
+    const std::vector<double>& w = R.weights(s);
+    const std::vector<double>& xq = R.points(s);
+    const i64* nodes = &m.c2n[m.c2n_ptr[c]];
+    const i64 nn = m.c2n_ptr[c + 1] - m.c2n_ptr[c];
+    double Xc[27 * 3];
+    for (i64 k = 0; k < nn; ++k)
+      for (int a = 0; a < d; ++a) Xc[k * d + a] = m.coords[nodes[k] * d + a];
+    std::vector<double> vu(nu), ru(nu * d), vv(nv), rv(nv * d), gv(nv * d), acc(nv, 0.0);
+    for (int q = 0; q < R.count(s); ++q) {
+      Geo g;
+      geometry(ct, Xc, &xq[q * d], g);
+      tensor_basis(s, pu, &xq[q * d], vu.data(), ru.data());
+      tensor_basis(s, pv, &xq[q * d], vv.data(), rv.data());
+      phys_grad(d, nv, rv.data(), g.Jinv, gv.data());
+      double uh = 0.0;
+      for (int k = 0; k < nu; ++k) uh += ug[igu[offu[c] + k] - 1] * vu[k];
+      for (int i = 0; i < nv; ++i) {
+        double f;
+        if (kind == 0) {
+          f = uh * vv[i];
+        } else {
+          double cg = cvec[0] * gv[i * d + 0];
+          for (int a = 1; a < d; ++a) cg += cvec[a] * gv[i * d + a];
+          f = uh * cg;
+        }
+        const double wu = w[q] * (g.detabs * f);
+        acc[i] = (q == 0) ? wu : acc[i] + wu;
+      }
+    }
+    for (int i = 0; i < nv; ++i) b[igv[offv[c] + i] - 1] += acc[i];
+  }
+}
+
 // One integral ∫ term dΩ on every cell, appended as COO in reference order.
 static void append_term(const Mesh& m, const std::vector<i64>& iglob, const std::vector<i64>& offset,
                         int order, int term, const double* eta, int degree,
@@ -2123,6 +2221,61 @@ void* or_assemble_bilinear(int dim, int64_t nnodes, const double* coords, int64_
   } catch (std::exception& e) { g_err = e.what(); return nullptr; }
 }
 
+// assemble_bilinear(a, U, V) with U of order pu and V of order pv on one mesh:
+// the ndofs(V) x ndofs(U) CSC (or_result_nrows x or_result_ndofs).
+void* or_assemble_bilinear_rect(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                                const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n,
+                                int pu, int pv, int continuous, int nterms, const int* terms,
+                                const double* eta, int degree) {
+  try {
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    Result* r = new Result();
+    std::vector<int> nd;
+    std::vector<i64> igv, offv;
+    r->ndofs = dofhandler(m, pu, continuous != 0, r->iglob, r->offset, nd);
+    const i64 nrows = dofhandler(m, pv, continuous != 0, igv, offv, nd);
+    ShapeRules R;
+    R.degree = degree;
+    for (int ct : m.ctype) R.fill(shape_of(ct));
+    std::vector<i64> I, J;
+    std::vector<double> X, acc;
+    // one COO segment per term, cells in element order, entry i + nv*j
+    // (_append_bilinear! assembler.jl:632-648, _cartesian_product :1357-1370)
+    for (int t = 0; t < nterms; ++t)
+      for (i64 c : element_order(m)) {
+        int nu = 0, nv = 0;
+        element_block_rect(m, pu, pv, terms[t], eta, R, c, nu, nv, acc);
+        for (int j = 0; j < nu; ++j)
+          for (int i = 0; i < nv; ++i) {
+            I.push_back(igv[offv[c] + i]);
+            J.push_back(r->iglob[r->offset[c] + j]);
+            X.push_back(acc[(size_t)j * nv + i]);
+          }
+      }
+    coo_to_csc(I, J, X, nrows, r->ndofs, r->A);
+    return r;
+  } catch (std::exception& e) { g_err = e.what(); return nullptr; }
+}
+
+// Linear forms with an FE function (assemble_linear_fe): u holds ndofs(U)
+// values of the order-pu space; b (ndofs of the order-pv space) is returned.
+void* or_assemble_linear_fe(int dim, int64_t nnodes, const double* coords, int64_t ncells,
+                            const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n, int pu,
+                            int pv, int continuous, const double* u, int kind, const double* cvec,
+                            int degree) {
+  try {
+    Mesh m = make_mesh(dim, nnodes, coords, ncells, ctypes, c2n_ptr, c2n);
+    Result* r = new Result();
+    std::vector<int> nd;
+    std::vector<i64> igu, offu;
+    dofhandler(m, pu, continuous != 0, igu, offu, nd);
+    r->ndofs = dofhandler(m, pv, continuous != 0, r->iglob, r->offset, nd);
+    r->b.assign(r->ndofs, 0.0);
+    assemble_linear_fe(m, igu, offu, pu, r->iglob, r->offset, pv, u, kind, cvec, degree, r->b);
+    return r;
+  } catch (std::exception& e) { g_err = e.what(); return nullptr; }
+}
+
 void* or_assemble_linear(int dim, int64_t nnodes, const double* coords, int64_t ncells,
                          const int* ctypes, const int64_t* c2n_ptr, const int64_t* c2n, int order,
                          int continuous, int fsrc, double fconst, int degree) {
@@ -2259,6 +2412,7 @@ int64_t or_dg_timed(int dim, int64_t nnodes, const double* coords, int64_t ncell
 }
 
 int64_t or_result_ndofs(void* h) { return ((Result*)h)->ndofs; }
+int64_t or_result_nrows(void* h) { return ((Result*)h)->A.m; }
 int64_t or_result_nnz(void* h) { return (int64_t)((Result*)h)->A.nzval.size(); }
 void or_result_csc(void* h, int64_t* colptr, int64_t* rowval, double* nzval) {
   Result* r = (Result*)h;

@@ -64,7 +64,13 @@ def lib():
         L.or_assemble_linear.argtypes = mesh_args + [C.c_int, C.c_int, C.c_int, C.c_double,
                                                      C.c_int]
         L.or_assemble_linear.restype = C.c_void_p
-        for n in ("or_result_ndofs", "or_result_nnz"):
+        L.or_assemble_bilinear_rect.argtypes = mesh_args + [C.c_int, C.c_int, C.c_int, C.c_int,
+                                                            _i, C.c_void_p, C.c_int]
+        L.or_assemble_bilinear_rect.restype = C.c_void_p
+        L.or_assemble_linear_fe.argtypes = mesh_args + [C.c_int, C.c_int, C.c_int, _d, C.c_int,
+                                                        _d, C.c_int]
+        L.or_assemble_linear_fe.restype = C.c_void_p
+        for n in ("or_result_ndofs", "or_result_nnz", "or_result_nrows"):
             getattr(L, n).argtypes = [C.c_void_p]
             getattr(L, n).restype = C.c_int64
         L.or_result_csc.argtypes = [C.c_void_p, _l, _l, _d]
@@ -317,6 +323,50 @@ def assemble_bilinear(mesh: OMesh, order, terms, eta=None, degree=2, continuous=
     finally:
         lib().or_result_free(h)
     return colptr, rowval, nzval, int(n)
+
+
+def assemble_bilinear_rect(mesh: OMesh, order_u, order_v, terms, eta=None, degree=2,
+                           continuous=True):
+    """assemble_bilinear(a, U, V) with U of order_u and V of order_v on one mesh:
+    CSC (1-based colptr over U's dofs, rowval over V's), nzval, (nrows, ncols)."""
+    t = np.ascontiguousarray([TERM[x] for x in terms], dtype=np.int32)
+    eta_p = None
+    if eta is not None:
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        eta_p = eta.ctypes.data
+    h = lib().or_assemble_bilinear_rect(*mesh.args(), order_u, order_v, int(continuous), len(t),
+                                        t, eta_p, degree)
+    if not h:
+        raise RuntimeError(_err())
+    try:
+        n = lib().or_result_ndofs(h)
+        m = lib().or_result_nrows(h)
+        nnz = lib().or_result_nnz(h)
+        colptr = np.zeros(n + 1, dtype=np.int64)
+        rowval = np.zeros(nnz, dtype=np.int64)
+        nzval = np.zeros(nnz)
+        lib().or_result_csc(h, colptr, rowval, nzval)
+    finally:
+        lib().or_result_free(h)
+    return colptr, rowval, nzval, (int(m), int(n))
+
+
+def assemble_linear_fe(mesh: OMesh, order_u, u, order_v, kind="uv", c=None, degree=2,
+                       continuous=True):
+    """Linear forms with an FE function u (dof values of the order_u space),
+    tested on the order_v space: kind "uv" ∫(u v)dΩ, "cu_grad" ∫((c u)⋅∇v)dΩ."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    cv = np.ascontiguousarray(np.zeros(3) if c is None else np.resize(np.asarray(c, float), 3))
+    h = lib().or_assemble_linear_fe(*mesh.args(), order_u, order_v, int(continuous), u,
+                                    {"uv": 0, "cu_grad": 1}[kind], cv, degree)
+    if not h:
+        raise RuntimeError(_err())
+    try:
+        b = np.zeros(lib().or_result_ndofs(h))
+        lib().or_result_vector(h, b)
+    finally:
+        lib().or_result_free(h)
+    return b
 
 
 def assemble_linear(mesh: OMesh, order, source="sinsin", const=0.0, degree=2, continuous=True):

@@ -16,6 +16,7 @@ import scipy.sparse as sp
 from . import _lib as L
 from .faces import FaceBilinearPlan, assemble_linear_faces_, has_faces
 from .forms import (CellDomain, Placeholder, _split_sum, lower_bilinear, lower_dg_transport, lower_linear,
+                    lower_linear_terms,
                     lower_multi_bilinear, lower_multi_linear)
 
 _COEF = {"const": L.COEF_CONST, "cell": L.COEF_CELL, "qp": L.COEF_QP, "fe": L.COEF_QP}
@@ -425,7 +426,8 @@ def _linear_terms(l, V, items=None):
     if items is None:
         items = _integrals(l(Placeholder("test", V)))
     measure = _single_measure(items)
-    return (measure,) + _linear_bh_terms([lower_linear(it.expr) for it in items], measure,
+    return (measure,) + _linear_bh_terms([t for it in items for t in lower_linear_terms(it.expr)],
+                                         measure,
                                          V.mesh)
 
 
@@ -433,7 +435,14 @@ def _linear_bh_terms(lowered, measure, mesh):
     keep, terms = [], []
     xq = None
     for form, kind, data, scale in lowered:
-        t = L.BhTerm(_FORM[form], _COEF[kind], None, scale)
+        if isinstance(form, tuple):  # ("grad", a): ∫ f ∂v/∂x_a
+            if form[1] >= mesh.dim:
+                raise NotImplementedError("the vector in (f c)⋅∇v has more entries than the mesh "
+                                          "has dimensions")
+            code = L.FORM_GRAD_SOURCE[form[1]]
+        else:
+            code = _FORM[form]
+        t = L.BhTerm(code, _COEF[kind], None, scale)
         if kind == "cell":
             arr = np.ascontiguousarray(data, dtype=np.float64)
             keep.append(arr)

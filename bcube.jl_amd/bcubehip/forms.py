@@ -582,6 +582,27 @@ def lower_linear(expr):
     raise NotImplementedError("linear integrand is not of the form f v")
 
 
+def lower_linear_terms(expr):
+    """lower_linear, plus ((f c)⋅∇v) with c a constant vector: one term
+    ∫ f ∂v/∂x_a per axis, scale c_a (form ("grad", a)).  Returns a list."""
+    fs = _factors(expr, [])
+    dots = [f for f in fs if isinstance(f, Dot)]
+    if len(dots) != 1:
+        return [lower_linear(expr)]
+    d = dots[0]
+    outer = [f for f in fs if f is not d]
+    for g, w in ((_factors(d.a, []), _factors(d.b, [])), (_factors(d.b, []), _factors(d.a, []))):
+        if len(g) == 1 and isinstance(g[0], Grad) and _is(g[0].a, Placeholder, "test"):
+            cs = [f for f in w if isinstance(f, VecConst)]
+            rest = [f for f in w if not isinstance(f, VecConst)] + outer
+            if len(cs) == 1 and not any(isinstance(f, (Dot, Grad, Add, Placeholder, VecConst))
+                                        for f in rest):
+                kind, data, scale = _coef(rest)
+                return [(("grad", a), kind, data, scale * float(c))
+                        for a, c in enumerate(cs[0].v)]
+    raise NotImplementedError("linear integrand is not of the form f v or (f c)⋅∇v")
+
+
 def lower_dg_transport(items):
     """Recognise the upwind DG transport form
 

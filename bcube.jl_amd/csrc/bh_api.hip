@@ -2912,7 +2912,12 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
                                      : bkey[bk] * bkey[bk] * (m->h.dim == 3 ? bkey[bk] : 1));
   }
   for (int k = 0; k < nterms; ++k) {
-    if (terms[k].form != BH_FORM_SOURCE) return fail(c, BH_ERR_UNSUPPORTED, "unsupported linear term");
+    const int fm = terms[k].form;
+    const bool grad = fm >= BH_FORM_GRAD_SOURCE_X && fm < BH_FORM_GRAD_SOURCE_X + m->h.dim;
+    if (fm != BH_FORM_SOURCE && !grad)
+      return fail(c, BH_ERR_UNSUPPORTED, "unsupported linear term");
+    if (grad && mode != BH_FP64)
+      return fail(c, BH_ERR_UNSUPPORTED, "∫ f ∂v/∂x_a terms are assembled in BH_FP64");
     if (terms[k].coef_kind != BH_COEF_CONST && !terms[k].coef)
       return fail(c, BH_ERR_ARG, "coefficient is NULL");
   }
@@ -2953,6 +2958,7 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
     A.fstride = nq;
     A.fkind = t.coef_kind;
     A.fscale = t.scale;
+    A.deriv = t.form == BH_FORM_SOURCE ? -1 : t.form - BH_FORM_GRAD_SOURCE_X;
     A.vals = db;
     if (t.coef_kind != BH_COEF_CONST) {
       int64_t n = t.coef_kind == BH_COEF_CELL ? nc : nc * nq;

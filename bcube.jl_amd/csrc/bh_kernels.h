@@ -55,7 +55,9 @@ struct LaunchArgs {
   bool mass = false;
   const int64_t* rowptr = nullptr;
   int offset_bytes = 1;
-  // source (FE_EVAL: f holds the FE function's dof values)
+  // source (FE_EVAL: f holds the FE function's dof values); deriv >= 0:
+  // ∫ f ∂v/∂x_deriv instead of ∫ f v
+  int deriv = -1;
   int fkind = BH_COEF_CONST;
   const double* f = nullptr;
   double fscale = 0.0;

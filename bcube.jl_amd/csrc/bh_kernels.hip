@@ -416,14 +416,15 @@ __global__ __launch_bounds__(kBlock) void k_gather(const T* __restrict__ ke,
 }
 
 // -------------------------------------------------------------------------
-// Linear form ∫ f v dΩ (FP64): b_i += Σ_q w_q |J_q| f_q φ_qi, colour-ordered.
+// Linear form ∫ f v dΩ (FP64): b_i += Σ_q w_q |J_q| f_q φ_qi, colour-ordered;
+// with deriv = a >= 0, ∫ f ∂v/∂x_a dΩ: b_i += Σ_q w_q |J_q| f_q (J_q^-T ∇̂φ_qi)_a.
 // -------------------------------------------------------------------------
 template <int D, int NL, int NQ, int NG>
 __global__ __launch_bounds__(kBlock) void k_source_f64(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
     const int32_t* __restrict__ rdofs, const double* __restrict__ tab,
     const int32_t* __restrict__ cells, const int32_t* __restrict__ lcells, int ncells_color,
-    int rstride, int fstride, int fkind, const double* __restrict__ f, double fscale,
+    int rstride, int fstride, int fkind, const double* __restrict__ f, double fscale, int deriv,
     double* __restrict__ b) {
   // cells: global ids (dof rows rdofs[cell*rstride], coefficients f[cell] or
   // f[cell*fstride + q]); lcells: the same cells' rows in c2n (a mixed mesh's
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_source_f64(
   constexpr int CPB = kBlock / NL;
   __shared__ double s_tab[L::SIZE];
   __shared__ double s_x[CPB][NG][D];
-  __shared__ double s_m[CPB][NQ];
+  __shared__ double s_m[CPB][NQ][D];
   const int tid = threadIdx.x;
   for (int k = tid; k < L::SIZE; k += kBlock) s_tab[k] = tab[k];
   const int ngroups = (ncells_color + CPB - 1) / CPB;
@@ -466,12 +467,25 @@ __global__ __launch_bounds__(kBlock) void k_source_f64(
       double fv = fkind == BH_COEF_CONST ? fscale
                   : fkind == BH_COEF_CELL ? f[cell] * fscale
                                           : f[(int64_t)cell * fstride + q] * fscale;
-      s_m[c][q] = s_tab[L::W + q] * fabs(det) * fv;
+      const double m = s_tab[L::W + q] * fabs(det) * fv;
+      if (deriv < 0) {
+        s_m[c][q][0] = m;
+      } else {
+        // (∇φ)_deriv = Σ_a (J^-1)_{a,deriv} ∂̂_a φ
+        for (int a = 0; a < D; ++a) s_m[c][q][a] = m * Ji[a][deriv];
+      }
     }
     __syncthreads();
     if (cs >= CPB || base + cs >= ncells_color) continue;
     double acc = 0.0;
-    for (int q = 0; q < NQ; ++q) acc = fma(s_m[cs][q], s_tab[L::R + (q * NL + li) * 4 + D], acc);
+    if (deriv < 0) {
+      for (int q = 0; q < NQ; ++q)
+        acc = fma(s_m[cs][q][0], s_tab[L::R + (q * NL + li) * 4 + D], acc);
+    } else {
+      for (int q = 0; q < NQ; ++q)
+        for (int a = 0; a < D; ++a)
+          acc = fma(s_m[cs][q][a], s_tab[L::R + (q * NL + li) * 4 + a], acc);
+    }
     const int cell = cells[base + cs];
     b[rdofs[(int64_t)cell * rstride + li]] += acc;
   }
@@ -629,7 +643,7 @@ static hipError_t launch_all(const LaunchArgs& A) {
     hipLaunchKernelGGL((k_source_f64<D, NL, NQ, NG>), dim3(grid_for(ng)), dim3(kBlock), 0,
                        A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.lcells,
                        A.ncells_color, A.erow > 0 ? A.erow : NL, A.fstride > 0 ? A.fstride : NQ,
-                       A.fkind, A.f, A.fscale, A.vals);
+                       A.fkind, A.f, A.fscale, A.deriv, A.vals);
   } else if (A.kind == LaunchArgs::FE_EVAL) {
     const int qs = A.fstride > 0 ? A.fstride : NQ;
     int64_t n = A.ncells_total * qs;

@@ -54,7 +54,13 @@ enum {
 };
 /* Linear integrands (assembler.jl:448 lowers `l(v)`). */
 enum {
-  BH_FORM_SOURCE = 10 /* ∫ f v dΩ  (f: constant, per cell or per quadrature point) */
+  BH_FORM_SOURCE = 10, /* ∫ f v dΩ  (f: constant, per cell or per quadrature point) */
+  /* ∫ f ∂v/∂x_a dΩ for a = 0, 1, 2 (FP64): the components of ∫ (f c)⋅∇v dΩ,
+   * e.g. the transport volume term ∫((c u)⋅∇v)dΩ of test_assembler.jl:40-68
+   * as one term per axis with scale c_a */
+  BH_FORM_GRAD_SOURCE_X = 11,
+  BH_FORM_GRAD_SOURCE_Y = 12,
+  BH_FORM_GRAD_SOURCE_Z = 13
 };
 /* How a coefficient is given. */
 enum {

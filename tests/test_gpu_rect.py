@@ -97,3 +97,40 @@ def test_fe_function_linear_form_vs_oracle(kind, pu, pv, deg):
     err = np.abs(b - ref).max() / np.abs(ref).max()
     print(f"{kind} ∫u v, u P{pu}, v P{pv}: rel err {err:.2e}", flush=True)
     assert err <= RTOL
+
+
+def test_q1_transport_known_answer():
+    """test_assembler.jl:40-68 through the product: one quad [-2,2] x [-3,3],
+    discontinuous Q1, u = α, ∫((c u)⋅∇v)dΩ."""
+    alpha, c, dx, dy = 10.0, np.array([0.5, 1.0]), 4.0, 6.0
+    m = B.one_cell_mesh(":quad")
+    m.set_nodes(m.nodes() * [2.0, 3.0])
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", 1), m, isContinuous=False)
+    V = B.TestFESpace(U)
+    dΩ = B.Measure(B.CellDomain(m), 3)
+    u = B.FEFunction(U, alpha * np.ones(V.get_ndofs()))
+    b = B.assemble_linear(lambda v: B.integrate(B.dot(c * u, B.grad(v)), dΩ), V)
+    ref = [-alpha / 2 * (c[0] * dy + c[1] * dx), alpha / 2 * (c[0] * dy - c[1] * dx),
+           alpha / 2 * (-c[0] * dy + c[1] * dx), alpha / 2 * (c[0] * dy + c[1] * dx)]
+    print("Q1 transport:", b, flush=True)
+    assert np.allclose(b, ref, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("kind,p,cont", [("Quad4", 1, False), ("Quad4", 2, True),
+                                         ("Tri3", 2, True), ("Hexa8", 1, True)])
+def test_transport_linear_form_vs_oracle(kind, p, cont):
+    m = _mesh(kind, 3, seed=8)
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", p), m, isContinuous=cont)
+    V = B.TestFESpace(U)
+    c = np.array([1.0, 0.5, 0.25])[:m.dim]
+    vals = np.random.default_rng(4).normal(size=U.get_ndofs())
+    u = B.FEFunction(U, vals)
+    dΩ = B.Measure(B.CellDomain(m), 2 * p + 1)
+    b = B.assemble_linear(lambda v: B.integrate(B.dot(c * u, B.grad(v)), dΩ), V)
+    ref = O.assemble_linear_fe(_om(m), p, vals, p, "cu_grad", c=c, degree=2 * p + 1,
+                               continuous=cont)
+    err = np.abs(b - ref).max() / np.abs(ref).max()
+    print(f"{kind} P{p} continuous={cont} ∫(c u)⋅∇v: rel err {err:.2e}", flush=True)
+    assert err <= RTOL
+    with pytest.raises(BcubeHipError, match="FP64"):
+        B.assemble_linear(lambda v: B.integrate(B.dot(c * u, B.grad(v)), dΩ), V, mode="fp32")

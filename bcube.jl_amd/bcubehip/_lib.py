@@ -23,7 +23,7 @@ COEF_CONST, COEF_CELL, COEF_QP = 0, 1, 2
 
 EXPORTED = [
     "bh_ctx_create", "bh_ctx_destroy", "bh_last_error", "bh_ctx_set_stream", "bh_sync",
-    "bh_ctx_set_timing", "bh_ctx_timings", "bh_ctx_timings_ex", "bh_ctx_set_autotune", "bh_mesh_create", "bh_mesh_set_nodes",
+    "bh_ctx_set_timing", "bh_ctx_timings", "bh_ctx_timings_ex", "bh_ctx_set_autotune", "bh_ctx_set_tuning_cache", "bh_mesh_create", "bh_mesh_set_nodes",
     "bh_mesh_destroy", "bh_dofhandler", "bh_dofhandler_device", "bh_space_create", "bh_space_create_from_dofs", "bh_space_create_orders", "bh_space_edge_orders",
     "bh_space_info", "bh_space_dofs", "bh_space_destroy", "bh_pattern_build", "bh_pattern_build_device", "bh_pattern_info", "bh_pattern_tuning", "bh_pattern_set_tuning", "bh_pattern_onepass",
     "bh_dirichlet_create", "bh_dirichlet_info", "bh_dirichlet_positions", "bh_dirichlet_apply_matrix",
@@ -81,6 +81,7 @@ def lib():
         "bh_ctx_timings": (C.c_int, [vp, vp, C.c_int, P(C.c_int)]),
         "bh_ctx_timings_ex": (C.c_int, [vp, vp, vp, C.c_int, P(C.c_int)]),
         "bh_ctx_set_autotune": (C.c_int, [vp, C.c_int]),
+        "bh_ctx_set_tuning_cache": (C.c_int, [vp, C.c_char_p]),
         "bh_pattern_tuning": (C.c_int, [vp, C.c_int, vp, vp]),
         "bh_pattern_set_tuning": (C.c_int, [vp, C.c_int, vp]),
         "bh_pattern_onepass": (C.c_int, [vp, vp]),

@@ -8,6 +8,7 @@ node jitter is this package's synthetic-workload generator.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -47,6 +48,14 @@ class BcubeBackendHIP(AbstractBcubeBackend):
         """Time the candidate launch shapes on the first assembly of each plan
         and mode, and keep the fastest (bitwise identical values)."""
         L.check(L.lib().bh_ctx_set_autotune(self.ctx, int(enable)), self.ctx, "bh_ctx_set_autotune")
+
+    def set_tuning_cache(self, path: str | None):
+        """Keep the autotuner's choices in `path` across processes: patterns of a
+        shape already in the file take its launch shapes without timing
+        (bh_ctx_set_tuning_cache).  None turns the cache off."""
+        L.check(L.lib().bh_ctx_set_tuning_cache(self.ctx, None if path is None
+                                                 else os.fsencode(path)),
+                self.ctx, "bh_ctx_set_tuning_cache")
 
     def timings(self, cap: int = 4096) -> np.ndarray:
         return self.timings_ex(cap)[0]

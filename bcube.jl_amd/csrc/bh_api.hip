@@ -3,17 +3,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <set>
+#include <sstream>
 #include <string>
 #include <tuple>
 #include <unordered_map>
-#include <array>
 #include <vector>
 
 #include "bh_dg.h"
@@ -35,6 +38,11 @@ struct bh_ctx_s {
   std::map<std::tuple<int, int, int, int>, float*> gp_tables;  // (geo, order, nq1, mass) -> P
   bool capturing = false;
   bool autotune = false;
+  // the autotuner's persistent cache (bh_ctx_set_tuning_cache): file path and
+  // its entries by key (tune_key), and the device's architecture name
+  std::string tune_path;
+  std::map<std::string, std::array<float, 5>> tune_cache;
+  std::string arch;
 };
 
 struct bh_graph_s {
@@ -606,6 +614,23 @@ int bh_ctx_set_timing(bh_ctx c, int enable) {
 int bh_ctx_set_autotune(bh_ctx c, int enable) {
   if (!c) return BH_ERR_ARG;
   c->autotune = enable != 0;
+  return BH_OK;
+}
+
+int bh_ctx_set_tuning_cache(bh_ctx c, const char* path) {
+  if (!c) return BH_ERR_ARG;
+  c->tune_cache.clear();
+  c->tune_path = path ? path : "";
+  if (c->tune_path.empty()) return BH_OK;
+  std::ifstream in(c->tune_path);
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream ls(line);
+    std::string key;
+    std::array<float, 5> v{};
+    if (!(ls >> key >> v[0] >> v[1] >> v[2] >> v[3] >> v[4])) continue;  // not an entry
+    c->tune_cache[key] = v;
+  }
   return BH_OK;
 }
 
@@ -2331,6 +2356,39 @@ static int ensure_sf(bh_pattern p, bh_ctx c, bool upload) {
   return BH_OK;
 }
 
+// Key of a pattern and mode in the autotuner's cache: the device architecture,
+// the cell types and order(s), the rule, the mode and the pattern's shape
+// (rows, entries, longest row, pair padding, offset width).  Shapes that give
+// bitwise identical values are all that a cached entry selects.
+static std::string tune_key(bh_ctx c, bh_pattern p, int mode, int quad_degree) {
+  if (c->arch.empty()) {
+    hipDeviceProp_t prop;
+    c->arch = hipGetDeviceProperties(&prop, c->device) == hipSuccess ? prop.gcnArchName : "gpu";
+    for (char& ch : c->arch)
+      if (ch == ' ') ch = '_';
+  }
+  const bh_space V = p->V;
+  std::ostringstream k;
+  k << c->arch << "|geo" << V->mesh->geo << "|b" << V->mesh->buckets.size() << "|p"
+    << (V->orders.empty() ? V->order : -1) << "|q" << quad_degree << "|m" << mode << "|r"
+    << p->h.nrows << "|z" << p->h.nnz << "|w" << p->h.max_row << "|a" << p->h.pad << "|o"
+    << p->h.offset_bytes;
+  return k.str();
+}
+
+static void save_tune_cache(bh_ctx c) {
+  const std::string tmp = c->tune_path + ".tmp";
+  {
+    std::ofstream out(tmp);
+    if (!out) return;
+    out << "# bcubehip autotuner cache: key elem_waves gather_lanes gather_rows elem_ms gather_ms\n";
+    for (const auto& kv : c->tune_cache)
+      out << kv.first << ' ' << (int)kv.second[0] << ' ' << (int)kv.second[1] << ' '
+          << (int)kv.second[2] << ' ' << kv.second[3] << ' ' << kv.second[4] << '\n';
+  }
+  (void)std::rename(tmp.c_str(), c->tune_path.c_str());
+}
+
 // assemble_bilinear(a, U, V) with U != V (assembler.jl:172-186: an
 // ndofs(V) x ndofs(U) SparseMatrixCSC): FP64 element rows a(φ_j, ψ_i) per
 // (cell, trial dof j) by k_element_rect_f64, then the row gather over the CSC
@@ -2768,6 +2826,42 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
     }
     return BH_OK;
   };
+  // a cached choice for this pattern shape (bh_ctx_set_tuning_cache), when it
+  // is still a candidate here
+  std::string tkey;
+  if (c->autotune && !tu.done && !c->capturing && !noalloc && !c->tune_path.empty()) {
+    tkey = tune_key(c, p, mode, quad_degree);
+    auto it = c->tune_cache.find(tkey);
+    if (it != c->tune_cache.end()) {
+      const int w = (int)it->second[0], ln = (int)it->second[1], rw = (int)it->second[2];
+      bool can12 = false;  // a 12-wave shape differs from 8 on one of the buckets
+      for (const EBucket& eb : ebs) {
+        const int k = rule_key(eb.g->geo, eb.degree);
+        int tg[2];
+        if (mode == BH_FP32 && gp_supported(eb.g->geo, eb.order, k))
+          can12 = can12 || gp_wave_targets(eb.g->geo, eb.order, k, tg) == 2;
+      }
+      bool ok = w == 0 || w == 8 || (w == 12 && can12) || (w == -1 && can_fuse);
+      if (ln != 0 || rw != 0) {
+        bool g = false;
+        if (h.pad > 0) {
+          int lanes[4], rows[4];
+          const int n = gather_shapes(V->nl, h.max_row, h.offset_bytes, lanes, rows);
+          for (int k = 0; k < n; ++k) g = g || (lanes[k] == ln && rows[k] == rw);
+        }
+        ok = ok && g;
+      }
+      if (ok) {
+        tu = Tuning();
+        tu.done = true;
+        tu.elem_waves = w;
+        tu.gather_lanes = ln;
+        tu.gather_rows = rw;
+        tu.elem_ms = it->second[3];
+        tu.gather_ms = it->second[4];
+      }
+    }
+  }
   if (c->autotune && !tu.done && !c->capturing && !noalloc) {
     // time every candidate shape on this stream (one warm launch, then the
     // fastest of three); all of them give bitwise identical values
@@ -2827,6 +2921,12 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
     if (r) return r;
     best.done = true;
     tu = best;
+    if (!c->tune_path.empty()) {
+      if (tkey.empty()) tkey = tune_key(c, p, mode, quad_degree);
+      c->tune_cache[tkey] = {(float)tu.elem_waves, (float)tu.gather_lanes, (float)tu.gather_rows,
+                             tu.elem_ms, tu.gather_ms};
+      save_tune_cache(c);
+    }
   }
   // the fused kernel when the autotuner chose it or a caller fixed it
   if (can_fuse && tu.elem_waves == -1) {

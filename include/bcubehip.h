@@ -110,6 +110,14 @@ int bh_ctx_timings_ex(bh_ctx ctx, float* ms, int32_t* kinds, int cap, int* count
  * of the row gather) and keeps the fastest for that pattern and mode.  All
  * candidates give bitwise identical values.  No counterpart in the reference. */
 int bh_ctx_set_autotune(bh_ctx ctx, int enable);
+/* A file that keeps the autotuner's choices across processes (no counterpart
+ * in the reference): read now (missing file: empty), rewritten whenever the
+ * autotuner times a new pattern shape.  With autotuning on, a pattern whose
+ * key (device architecture, cell types, order, rule, mode, rows, entries,
+ * longest row, pair padding, offset width) is in the file takes the cached
+ * launch shapes without timing them, when they are still candidates for it.
+ * path NULL or "" turns the cache off. */
+int bh_ctx_set_tuning_cache(bh_ctx ctx, const char* path);
 
 /* ---- mesh --------------------------------------------------------------- */
 /* Mesh(nodes, celltypes, Connectivity) (src/mesh/mesh.jl:221-262).

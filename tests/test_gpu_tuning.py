@@ -73,3 +73,47 @@ def test_autotuned_shapes_bitwise_equal(name):
         assert np.array_equal(plan.assemble_host(), ref)
     finally:
         m.backend.set_autotune(False)
+
+
+@pytest.mark.parametrize("name", ["mixed_p2", "quad_q1"])
+def test_tuning_cache_persists_choice(name, tmp_path):
+    """The autotuner's choice is written to the cache file and taken from it by
+    a new context without timing (bh_ctx_set_tuning_cache); values bitwise
+    equal."""
+    path = str(tmp_path / "tune.txt")
+    m, U, form = _case(name)
+    be = m.backend
+    be.set_autotune(True)
+    be.set_tuning_cache(path)
+    try:
+        plan = B.BilinearPlan(form, U, B.TestFESpace(U), "fp32")
+        v1 = plan.assemble_host().copy()
+        t1 = plan.tuning()
+    finally:  # the default backend is shared with the other tests
+        be.set_autotune(False)
+        be.set_tuning_cache(None)
+    lines = [l.split() for l in open(path) if not l.startswith("#")]
+    assert len(lines) == 1 and [int(x) for x in lines[0][1:4]] == [
+        t1["elem_waves"], t1["gather_lanes"], t1["gather_rows"]]
+    # a fresh context and mesh reading the same file: the cached shape, its
+    # recorded times, no new entry
+    be2 = B.BcubeBackendHIP()
+    be2.set_autotune(True)
+    be2.set_tuning_cache(path)
+    m2, _, _ = _case(name)
+    m2 = B.Mesh(m2.nodes(), m2.celltypes, [m2.cell_nodes(c) for c in range(m2.ncells)], be2)
+    kappa = np.random.default_rng(3).uniform(0.5, 2.0, m2.ncells)
+    p, deg = (2, 4) if name == "mixed_p2" else (1, 2)
+    U2 = B.TrialFESpace(B.FunctionSpace("Lagrange", p), m2)
+    dΩ = B.Measure(B.CellDomain(m2), deg)
+    kc = B.MeshCellData(kappa)
+    plan2 = B.BilinearPlan(lambda u, v: B.integrate(kc * B.dot(B.grad(u), B.grad(v)), dΩ)
+                           + B.integrate(u * v, dΩ), U2, B.TestFESpace(U2), "fp32")
+    v2 = plan2.assemble_host()
+    t2 = plan2.tuning()
+    assert {k: t2[k] for k in ("elem_waves", "gather_lanes", "gather_rows")} == {
+        k: t1[k] for k in ("elem_waves", "gather_lanes", "gather_rows")}
+    assert abs(t2["elem_ms"] - t1["elem_ms"]) <= 1e-3 * max(1.0, t1["elem_ms"])
+    assert np.array_equal(v1, v2)
+    assert len([l for l in open(path) if not l.startswith("#")]) == 1
+    be2.set_tuning_cache(None)

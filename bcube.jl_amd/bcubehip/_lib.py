@@ -25,7 +25,7 @@ EXPORTED = [
     "bh_ctx_create", "bh_ctx_destroy", "bh_last_error", "bh_ctx_set_stream", "bh_sync",
     "bh_ctx_set_timing", "bh_ctx_timings", "bh_ctx_timings_ex", "bh_ctx_set_autotune", "bh_ctx_set_tuning_cache", "bh_mesh_create", "bh_mesh_set_nodes",
     "bh_mesh_destroy", "bh_dofhandler", "bh_dofhandler_device", "bh_space_create", "bh_space_create_from_dofs", "bh_space_create_orders", "bh_space_edge_orders",
-    "bh_space_info", "bh_space_dofs", "bh_space_destroy", "bh_pattern_build", "bh_pattern_build_device", "bh_pattern_info", "bh_pattern_tuning", "bh_pattern_set_tuning", "bh_pattern_onepass",
+    "bh_space_info", "bh_space_dofs", "bh_space_destroy", "bh_pattern_build", "bh_pattern_build_device", "bh_pattern_info", "bh_pattern_tuning", "bh_pattern_set_tuning", "bh_pattern_onepass", "bh_pattern_set_offset_templates", "bh_pattern_offset_templates",
     "bh_dirichlet_create", "bh_dirichlet_info", "bh_dirichlet_positions", "bh_dirichlet_apply_matrix",
     "bh_dirichlet_apply_vector", "bh_dirichlet_destroy",
     "bh_blocks_create", "bh_blocks_info", "bh_blocks_export_csc", "bh_blocks_block_values",
@@ -85,6 +85,8 @@ def lib():
         "bh_pattern_tuning": (C.c_int, [vp, C.c_int, vp, vp]),
         "bh_pattern_set_tuning": (C.c_int, [vp, C.c_int, vp]),
         "bh_pattern_onepass": (C.c_int, [vp, vp]),
+        "bh_pattern_set_offset_templates": (C.c_int, [vp, C.c_int]),
+        "bh_pattern_offset_templates": (C.c_int, [vp]),
         "bh_dirichlet_create": (C.c_int, [vp, i64, vp, P(vp)]),
         "bh_dirichlet_info": (C.c_int, [vp, P(i64), P(i64)]),
         "bh_dirichlet_positions": (C.c_int, [vp, vp, vp, vp, vp, vp]),

@@ -319,6 +319,19 @@ class BilinearPlan:
                                              md, nzval_ptr, L.DEVICE), self.ctx,
                 "bh_assemble_bilinear")
 
+    def set_offset_templates(self, enable: bool):
+        """Row gather offsets from the pattern's templates (default, where it has
+        them) or per entry (bh_pattern_set_offset_templates); same values."""
+        L.check(L.lib().bh_pattern_set_offset_templates(self.handle, int(enable)), self.ctx,
+                "bh_pattern_set_offset_templates")
+
+    def offset_templates(self) -> int:
+        """Number of offset templates the row gather uses (0: per-entry offsets)."""
+        n = L.lib().bh_pattern_offset_templates(self.handle)
+        if n < 0:
+            L.check(n, self.ctx, "bh_pattern_offset_templates")
+        return n
+
     def tuning(self, mode: str | None = None):
         """Launch shapes the autotuner chose for this plan and mode, or None
         before tuning: element-kernel waves per block target, gather lanes per

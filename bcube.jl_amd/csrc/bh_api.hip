@@ -164,6 +164,13 @@ struct bh_pattern_s {
   int32_t* d_pptr = nullptr;
   int32_t* d_pairs = nullptr;
   void* d_goff = nullptr;
+  // offset templates of the row gather (offset_templates): built on the first
+  // assembly; ntpl 0 when the pattern has none (more than 255 distinct rows)
+  bool tpl_tried = false;
+  bool tpl_off = false;  // bh_pattern_set_offset_templates(p, 0)
+  int ntpl = 0;
+  uint32_t* d_tpairs = nullptr;
+  uint8_t* d_tpl = nullptr;
   void* d_ke = nullptr;      // element rows, (ncells+16)*nl*ke_stride(nl) FP64 (FP32 uses half)
   double* d_vals = nullptr;
   double* d_coef = nullptr;  // staged host coefficients (2 * ncells)
@@ -1089,6 +1096,17 @@ int bh_pattern_set_tuning(bh_pattern p, int mode, const int32_t* shape) {
   tu.gather_lanes = shape[1];
   tu.gather_rows = shape[2];
   return BH_OK;
+}
+
+int bh_pattern_set_offset_templates(bh_pattern p, int enable) {
+  if (!p) return BH_ERR_ARG;
+  p->tpl_off = enable == 0;
+  return BH_OK;
+}
+
+int bh_pattern_offset_templates(bh_pattern p) {
+  if (!p) return BH_ERR_ARG;
+  return p->tpl_off ? 0 : p->ntpl;
 }
 
 int bh_pattern_onepass(bh_pattern p, int32_t* info) {
@@ -2184,6 +2202,8 @@ int bh_pattern_destroy(bh_pattern p) {
     (void)hipFree(p->d_pptr);
     (void)hipFree(p->d_pairs);
     (void)hipFree(p->d_goff);
+    (void)hipFree(p->d_tpairs);
+    (void)hipFree(p->d_tpl);
     (void)hipFree(p->d_ke);
     (void)hipFree(p->d_vals);
     (void)hipFree(p->d_coef);
@@ -2353,6 +2373,38 @@ static int ensure_sf(bh_pattern p, bh_ctx c, bool upload) {
   BH_HIP(c, dalloc(&s.d_tpl, S.tpl.size()));
   BH_HIP(c, hipMemcpy(s.d_tpl, S.tpl.data(), S.tpl.size(), hipMemcpyHostToDevice));
   s.ok = true;
+  return BH_OK;
+}
+
+// Offset templates of the row gather, once per pattern: the distinct
+// element-row offset vectors (122 on any structured Q2 hexahedral grid) in a
+// table the gather stages in LDS, and each padded pair slot's template id
+// beside its element-row id, so the gather reads no goff stream.  The offsets
+// are the same, so are the sums.  Patterns with more than 255 distinct
+// vectors, 16-bit offsets or unpadded pair lists keep goff.
+static int ensure_templates(bh_pattern p, bh_ctx c) {
+  if (p->tpl_tried) return BH_OK;
+  p->tpl_tried = true;
+  const HostPattern& h = p->h;
+  const int nl = h.nl_col;
+  if (h.offset_bytes != 1 || h.pad == 0 || nl > 32 || !p->d_goff || !p->d_pairs) return BH_OK;
+  const int64_t nrow_el = p->U->mesh->h.ncells * (int64_t)h.nl_row;
+  const int64_t nslots = h.nrows * (int64_t)h.pad;
+  std::vector<uint8_t> goff((size_t)(nrow_el * nl));
+  std::vector<int32_t> pairs((size_t)nslots);
+  BH_HIP(c, hipMemcpy(goff.data(), p->d_goff, goff.size(), hipMemcpyDeviceToHost));
+  BH_HIP(c, hipMemcpy(pairs.data(), p->d_pairs, pairs.size() * sizeof(int32_t),
+                      hipMemcpyDeviceToHost));
+  std::vector<uint8_t> tpl;
+  std::vector<uint32_t> words;
+  const int n = offset_templates(goff.data(), nl, nrow_el, pairs.data(), nslots, tpl, words);
+  if (n <= 0) return BH_OK;
+  BH_HIP(c, dalloc(&p->d_tpairs, words.size()));
+  BH_HIP(c, hipMemcpy(p->d_tpairs, words.data(), words.size() * sizeof(uint32_t),
+                      hipMemcpyHostToDevice));
+  BH_HIP(c, dalloc(&p->d_tpl, tpl.size()));
+  BH_HIP(c, hipMemcpy(p->d_tpl, tpl.data(), tpl.size(), hipMemcpyHostToDevice));
+  p->ntpl = n;
   return BH_OK;
 }
 
@@ -2751,6 +2803,8 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
     return BH_OK;
   };
   // 2. deterministic row reduction into the CSR values
+  if (!noalloc && !c->capturing)
+    if (int r = ensure_templates(p, c)) return r;
   LaunchArgs G = A;
   G.kind = LaunchArgs::GATHER;
   G.pptr = p->d_pptr;
@@ -2758,6 +2812,11 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
   G.padded_pairs = h.pad > 0;
   G.max_adj = h.pad;
   G.goff = p->d_goff;
+  if (p->ntpl > 0 && !p->tpl_off) {
+    G.tpairs = p->d_tpairs;
+    G.tpl = p->d_tpl;
+    G.ntpl = p->ntpl;
+  }
   G.offset_bytes = h.offset_bytes;
   G.rowptr = p->d_rowptr;
   G.nrows = h.nrows;

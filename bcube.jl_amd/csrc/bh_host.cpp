@@ -1403,4 +1403,72 @@ bool build_sf_structure(const std::vector<int32_t>& dofs, i64 ncells, i64 ndofs,
   return true;
 }
 
+
+// ---------------------------------------------------------------------------
+// Offset templates of the row gather (see bh_internal.h).  Each thread keeps
+// the distinct vectors of its run of element rows (by a 64-bit FNV-1a hash,
+// bytes compared on a hit), the runs are merged in thread order, then every
+// element row gets its template id.
+int offset_templates(const uint8_t* goff, int nl, i64 nrow_el, const int32_t* pairs_padded,
+                     i64 nslots, std::vector<uint8_t>& tpl, std::vector<uint32_t>& words) {
+  if (nrow_el >= ((i64)1 << 24) || nl <= 0) return -1;
+  auto hash = [nl](const uint8_t* k) {
+    uint64_t h = 1469598103934665603ull;
+    for (int j = 0; j < nl; ++j) h = (h ^ k[j]) * 1099511628211ull;
+    return h;
+  };
+  constexpr int kMax = 255;
+  const int T = std::max(1, std::min(64, (int)(nrow_el / 65536) + 1));
+  std::vector<std::vector<i64>> firsts(T);  // per run: element rows that first showed a vector
+  std::vector<char> over(T, 0);
+  parallel_for(T, [&](i64 b, i64 e) {
+    for (i64 t = b; t < e; ++t) {
+      const i64 r0 = nrow_el * t / T, r1 = nrow_el * (t + 1) / T;
+      std::unordered_map<uint64_t, i64> seen;
+      for (i64 r = r0; r < r1 && !over[t]; ++r) {
+        const uint8_t* k = goff + r * nl;
+        auto it = seen.find(hash(k));
+        if (it == seen.end()) {
+          seen.emplace(hash(k), r);
+          firsts[t].push_back(r);
+          if ((int)firsts[t].size() > kMax) over[t] = 1;
+        } else if (std::memcmp(goff + it->second * nl, k, nl) != 0) {
+          over[t] = 1;  // a hash collision: no templates
+        }
+      }
+    }
+  });
+  for (int t = 0; t < T; ++t)
+    if (over[t]) return -1;
+  std::unordered_map<uint64_t, int> id;
+  tpl.clear();
+  for (int t = 0; t < T; ++t)
+    for (i64 r : firsts[t]) {
+      const uint8_t* k = goff + r * nl;
+      auto it = id.find(hash(k));
+      if (it != id.end()) {
+        if (std::memcmp(&tpl[(size_t)it->second * nl], k, nl) != 0) return -1;
+        continue;
+      }
+      if ((int)id.size() == kMax) return -1;
+      id.emplace(hash(k), (int)id.size());
+      tpl.insert(tpl.end(), k, k + nl);
+    }
+  std::vector<uint8_t> rid((size_t)nrow_el);
+  parallel_for(nrow_el, [&](i64 b, i64 e) {
+    for (i64 r = b; r < e; ++r) rid[r] = (uint8_t)id.find(hash(goff + r * nl))->second;
+  });
+  words.assign((size_t)nslots, 0xFFFFFFFFu);
+  std::atomic<bool> bad{false};
+  parallel_for(nslots, [&](i64 b, i64 e) {
+    for (i64 q = b; q < e; ++q) {
+      const int32_t v = pairs_padded[q];
+      if (v < 0) continue;
+      if ((i64)v >= nrow_el) { bad.store(true); continue; }
+      words[q] = (uint32_t)v | ((uint32_t)rid[v] << 24);
+    }
+  });
+  return bad.load() ? -1 : (int)id.size();
+}
+
 }  // namespace bh

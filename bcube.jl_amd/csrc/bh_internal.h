@@ -93,6 +93,14 @@ bool build_pattern(const std::vector<int32_t>& rdofs, int nl_row, i64 nrows,
                    const std::vector<int32_t>& cdofs, int nl_col, i64 ncols, i64 ncells,
                    HostPattern& p, std::string& err);
 
+// Offset templates of the row gather: the distinct element-row offset vectors
+// (nl bytes each, uint8 offsets) among the nrow_el element rows of goff, and
+// per padded pair slot the word pair | template << 24 (0xFFFFFFFF: empty).
+// Returns the number of templates, or -1 when there are more than 255 of them
+// or an element-row id needs more than 24 bits.
+int offset_templates(const uint8_t* goff, int nl, i64 nrow_el, const int32_t* pairs_padded,
+                     i64 nslots, std::vector<uint8_t>& tpl, std::vector<uint32_t>& words);
+
 // Colouring of cells so that no two cells of one colour share a row dof.
 int colour_cells(const std::vector<int32_t>& rdofs, int nl, i64 nrows, i64 ncells,
                  std::vector<int32_t>& color_ptr, std::vector<int32_t>& cells_by_color);

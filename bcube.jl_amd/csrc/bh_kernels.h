@@ -45,6 +45,11 @@ struct LaunchArgs {
   bool padded_pairs = false;
   int max_adj = 0;
   const void* goff = nullptr;
+  // offset templates (uint8 offsets): pair slots pair | template << 24 and
+  // the ntpl template rows; NULL: goff
+  const uint32_t* tpairs = nullptr;
+  const uint8_t* tpl = nullptr;
+  int ntpl = 0;
   int64_t nrows = 0;              // rows to sum: all rows, or the length of rlist
   const int32_t* rlist = nullptr;  // row gather over these rows only (NULL: 0 .. nrows-1)
   int max_row = 0;

@@ -300,17 +300,27 @@ __global__ __launch_bounds__(kBlock) void k_fe_eval(const int32_t* __restrict__ 
 //     and RPH*MAXP loads per lane are in flight before the LDS adds.
 //   k_gather: variable-length lists (rows touched by more than 16 cells).
 // -------------------------------------------------------------------------
-template <typename T, typename OFF, int NL, int ROWCAP, int MAXP, int RPH, int SG>
+template <typename T, typename OFF, int NL, int ROWCAP, int MAXP, int RPH, int SG, bool TPL>
 __global__ __launch_bounds__(kBlock) void k_gather_padded(
     const T* __restrict__ ke, const int32_t* __restrict__ pairs, const OFF* __restrict__ goff,
+    const uint32_t* __restrict__ tpairs, const uint8_t* __restrict__ tpl, int ntpl,
     const int64_t* __restrict__ rowptr, int64_t nrows, const int32_t* __restrict__ rlist,
     double* __restrict__ vals) {
-  // rlist: the rows to sum (nrows of them), or NULL for rows 0 .. nrows-1
+  // rlist: the rows to sum (nrows of them), or NULL for rows 0 .. nrows-1.
+  // TPL: the pair slots hold pair | template << 24 (tpairs, 0xFFFFFFFF empty)
+  // and the element row's offsets are template row tpl[template] (ntpl of
+  // them, staged in LDS) instead of goff[pair]: the same offsets, no goff
+  // stream.
   constexpr int KS = KeStride<NL>::value;
   constexpr int HW = kBlock / SG;        // lane groups per block
   constexpr int NJ = (NL + SG - 1) / SG;
   constexpr int kSent = (int)(OFF)~(OFF)0;  // padding column (mixed meshes)
   __shared__ double buf[HW][RPH][ROWCAP];
+  __shared__ uint8_t s_tpl[TPL ? 255 * NL : 1];
+  if constexpr (TPL) {
+    for (int k = threadIdx.x; k < ntpl * NL; k += kBlock) s_tpl[k] = tpl[k];
+    __syncthreads();
+  }
   const int h = threadIdx.x / SG;
   const int l = threadIdx.x % SG;
   const int64_t step = (int64_t)gridDim.x * HW * RPH;
@@ -318,6 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_padded(
     int64_t rs[RPH];
     int len[RPH];
     int32_t pr[RPH][MAXP];
+    int tt[RPH][MAXP];
 #pragma unroll
     for (int r = 0; r < RPH; ++r) {
       const int64_t ri = row0 + r;
@@ -326,7 +337,16 @@ __global__ __launch_bounds__(kBlock) void k_gather_padded(
       rs[r] = ok ? rowptr[row] : 0;
       len[r] = ok ? (int)(rowptr[row + 1] - rs[r]) : 0;
 #pragma unroll
-      for (int k = 0; k < MAXP; ++k) pr[r][k] = ok ? pairs[row * MAXP + k] : -1;
+      for (int k = 0; k < MAXP; ++k) {
+        if constexpr (TPL) {
+          const uint32_t w = ok ? tpairs[row * MAXP + k] : 0xFFFFFFFFu;
+          pr[r][k] = w == 0xFFFFFFFFu ? -1 : (int32_t)(w & 0xFFFFFFu);
+          tt[r][k] = (int)(w >> 24);
+        } else {
+          pr[r][k] = ok ? pairs[row * MAXP + k] : -1;
+          tt[r][k] = 0;
+        }
+      }
     }
     T v[RPH][MAXP][NJ];
     int o[RPH][MAXP][NJ];
@@ -340,7 +360,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_padded(
           o[r][k][t] = kSent;
           if (pr[r][k] >= 0 && j < NL) {
             v[r][k][t] = ke[(int64_t)pr[r][k] * KS + j];
-            o[r][k][t] = goff[(int64_t)pr[r][k] * NL + j];
+            if constexpr (TPL) o[r][k][t] = s_tpl[tt[r][k] * NL + j];
+            else o[r][k][t] = goff[(int64_t)pr[r][k] * NL + j];
           }
         }
 #pragma unroll
@@ -541,13 +562,24 @@ static hipError_t gather_padded_go(const LaunchArgs& A) {
   static_assert(HW * RPH * CAP * 8 <= 65536, "row buffers fit in 64 KB of LDS");
   int64_t blocks = (A.nrows + HW * RPH - 1) / (HW * RPH);
   dim3 grid((unsigned)(blocks < 16384 ? (blocks < 1 ? 1 : blocks) : 16384));
-#define BH_GP(MP)                                                                              \
-  hipLaunchKernelGGL((k_gather_padded<T, OFFT, NL, CAP, MP, RPH, SG>), grid, dim3(kBlock), 0,   \
-                     A.stream, (const T*)A.ke, A.pairs, (const OFFT*)A.goff, A.rowptr, A.nrows, \
-                     A.rlist, A.vals)
-  if (A.max_adj <= 4) BH_GP(4);
-  else if (A.max_adj <= 8) BH_GP(8);
-  else BH_GP(16);
+#define BH_GP(MP, TP)                                                                          \
+  hipLaunchKernelGGL((k_gather_padded<T, OFFT, NL, CAP, MP, RPH, SG, TP>), grid, dim3(kBlock),  \
+                     0, A.stream, (const T*)A.ke, A.pairs, (const OFFT*)A.goff, A.tpairs, A.tpl, \
+                     A.ntpl, A.rowptr, A.nrows, A.rlist, A.vals)
+  // offset templates (uint8 offsets, rows of up to 32 entries)
+  constexpr bool kTpl = std::is_same<OFFT, uint8_t>::value && NL <= 32;
+  const bool tp = kTpl && A.tpairs && A.ntpl > 0 && A.ntpl <= 255;
+  if constexpr (kTpl) {
+    if (tp) {
+      if (A.max_adj <= 4) BH_GP(4, true);
+      else if (A.max_adj <= 8) BH_GP(8, true);
+      else BH_GP(16, true);
+      return hipGetLastError();
+    }
+  }
+  if (A.max_adj <= 4) BH_GP(4, false);
+  else if (A.max_adj <= 8) BH_GP(8, false);
+  else BH_GP(16, false);
 #undef BH_GP
   return hipGetLastError();
 }

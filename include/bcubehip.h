@@ -223,6 +223,17 @@ int bh_pattern_onepass(bh_pattern p, int32_t* info);
  * them.  A gather shape must be one of the pattern's candidates (BH_ERR_ARG
  * otherwise).  No counterpart in the reference. */
 int bh_pattern_set_tuning(bh_pattern p, int mode, const int32_t* shape);
+/* Offset templates of the row gather (no counterpart in the reference): on
+ * its first assembly a pattern whose element rows carry at most 255 distinct
+ * vectors of uint8 column offsets (122 on any structured Q2 hexahedral grid)
+ * keeps those vectors in a table the gather stages in LDS, and each pair slot
+ * carries its vector's id beside the element row's, so the gather reads no
+ * per-entry offsets.  The offsets and sums are the same.  enable 0 makes the
+ * gather read the per-entry offsets (for comparisons), 1 (the default) the
+ * templates where the pattern has them.  bh_pattern_offset_templates returns
+ * the number of templates in use (0: per-entry offsets). */
+int bh_pattern_set_offset_templates(bh_pattern p, int enable);
+int bh_pattern_offset_templates(bh_pattern p);
 int bh_pattern_info(bh_pattern p, int64_t* nrows, int64_t* ncols, int64_t* nnz,
                     int32_t* max_row);
 /* Julia SparseMatrixCSC{Float64,Int64} structure: colptr (ncols+1), rowval

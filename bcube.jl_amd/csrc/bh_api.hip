@@ -169,6 +169,7 @@ struct bh_pattern_s {
   bool tpl_tried = false;
   bool tpl_off = false;  // bh_pattern_set_offset_templates(p, 0)
   int ntpl = 0;
+  int tpl_shift = 24;    // slot word: pair | template << tpl_shift
   uint32_t* d_tpairs = nullptr;
   uint8_t* d_tpl = nullptr;
   void* d_ke = nullptr;      // element rows, (ncells+16)*nl*ke_stride(nl) FP64 (FP32 uses half)
@@ -2378,8 +2379,8 @@ static int ensure_sf(bh_pattern p, bh_ctx c, bool upload) {
 
 // Offset templates of the row gather, once per pattern: the distinct
 // element-row offset vectors (122 on any structured Q2 hexahedral grid) in a
-// table the gather stages in LDS, and each padded pair slot's template id
-// beside its element-row id, so the gather reads no goff stream.  The offsets
+// table the gather stages in LDS, and each padded pair slot's template id in
+// the top bits beside its element-row id, so the gather reads no goff stream.  The offsets
 // are the same, so are the sums.  Patterns with more than 255 distinct
 // vectors, 16-bit offsets or unpadded pair lists keep goff.
 static int ensure_templates(bh_pattern p, bh_ctx c) {
@@ -2397,8 +2398,11 @@ static int ensure_templates(bh_pattern p, bh_ctx c) {
                       hipMemcpyDeviceToHost));
   std::vector<uint8_t> tpl;
   std::vector<uint32_t> words;
-  const int n = offset_templates(goff.data(), nl, nrow_el, pairs.data(), nslots, tpl, words);
+  int shift = 24;
+  const int n =
+      offset_templates(goff.data(), nl, nrow_el, pairs.data(), nslots, tpl, words, shift);
   if (n <= 0) return BH_OK;
+  p->tpl_shift = shift;
   BH_HIP(c, dalloc(&p->d_tpairs, words.size()));
   BH_HIP(c, hipMemcpy(p->d_tpairs, words.data(), words.size() * sizeof(uint32_t),
                       hipMemcpyHostToDevice));
@@ -2816,6 +2820,7 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
     G.tpairs = p->d_tpairs;
     G.tpl = p->d_tpl;
     G.ntpl = p->ntpl;
+    G.tpl_shift = p->tpl_shift;
   }
   G.offset_bytes = h.offset_bytes;
   G.rowptr = p->d_rowptr;

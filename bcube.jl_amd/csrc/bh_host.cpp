@@ -1410,8 +1410,9 @@ bool build_sf_structure(const std::vector<int32_t>& dofs, i64 ncells, i64 ndofs,
 // bytes compared on a hit), the runs are merged in thread order, then every
 // element row gets its template id.
 int offset_templates(const uint8_t* goff, int nl, i64 nrow_el, const int32_t* pairs_padded,
-                     i64 nslots, std::vector<uint8_t>& tpl, std::vector<uint32_t>& words) {
-  if (nrow_el >= ((i64)1 << 24) || nl <= 0) return -1;
+                     i64 nslots, std::vector<uint8_t>& tpl, std::vector<uint32_t>& words,
+                     int& shift) {
+  if (nrow_el >= ((i64)1 << 31) || nl <= 0) return -1;
   auto hash = [nl](const uint8_t* k) {
     uint64_t h = 1469598103934665603ull;
     for (int j = 0; j < nl; ++j) h = (h ^ k[j]) * 1099511628211ull;
@@ -1454,6 +1455,11 @@ int offset_templates(const uint8_t* goff, int nl, i64 nrow_el, const int32_t* pa
       id.emplace(hash(k), (int)id.size());
       tpl.insert(tpl.end(), k, k + nl);
     }
+  // the template id takes the top bits, as few as its count needs
+  int tbits = 1;
+  while ((1 << tbits) < (int)id.size()) ++tbits;
+  shift = 32 - tbits;
+  if (nrow_el >= ((i64)1 << shift)) return -1;
   std::vector<uint8_t> rid((size_t)nrow_el);
   parallel_for(nrow_el, [&](i64 b, i64 e) {
     for (i64 r = b; r < e; ++r) rid[r] = (uint8_t)id.find(hash(goff + r * nl))->second;
@@ -1465,7 +1471,7 @@ int offset_templates(const uint8_t* goff, int nl, i64 nrow_el, const int32_t* pa
       const int32_t v = pairs_padded[q];
       if (v < 0) continue;
       if ((i64)v >= nrow_el) { bad.store(true); continue; }
-      words[q] = (uint32_t)v | ((uint32_t)rid[v] << 24);
+      words[q] = (uint32_t)v | ((uint32_t)rid[v] << shift);
     }
   });
   return bad.load() ? -1 : (int)id.size();

@@ -95,11 +95,13 @@ bool build_pattern(const std::vector<int32_t>& rdofs, int nl_row, i64 nrows,
 
 // Offset templates of the row gather: the distinct element-row offset vectors
 // (nl bytes each, uint8 offsets) among the nrow_el element rows of goff, and
-// per padded pair slot the word pair | template << 24 (0xFFFFFFFF: empty).
-// Returns the number of templates, or -1 when there are more than 255 of them
-// or an element-row id needs more than 24 bits.
+// per padded pair slot the word pair | template << shift (0xFFFFFFFF: empty),
+// shift = 32 - the bits the template count needs.  Returns the number of
+// templates, or -1 when there are more than 255 of them or an element-row id
+// does not fit below the template bits.
 int offset_templates(const uint8_t* goff, int nl, i64 nrow_el, const int32_t* pairs_padded,
-                     i64 nslots, std::vector<uint8_t>& tpl, std::vector<uint32_t>& words);
+                     i64 nslots, std::vector<uint8_t>& tpl, std::vector<uint32_t>& words,
+                     int& shift);
 
 // Colouring of cells so that no two cells of one colour share a row dof.
 int colour_cells(const std::vector<int32_t>& rdofs, int nl, i64 nrows, i64 ncells,

@@ -45,11 +45,12 @@ struct LaunchArgs {
   bool padded_pairs = false;
   int max_adj = 0;
   const void* goff = nullptr;
-  // offset templates (uint8 offsets): pair slots pair | template << 24 and
-  // the ntpl template rows; NULL: goff
+  // offset templates (uint8 offsets): pair slots pair | template << tpl_shift
+  // and the ntpl template rows; NULL: goff
   const uint32_t* tpairs = nullptr;
   const uint8_t* tpl = nullptr;
   int ntpl = 0;
+  int tpl_shift = 24;  // slot word: pair | template << tpl_shift
   int64_t nrows = 0;              // rows to sum: all rows, or the length of rlist
   const int32_t* rlist = nullptr;  // row gather over these rows only (NULL: 0 .. nrows-1)
   int max_row = 0;

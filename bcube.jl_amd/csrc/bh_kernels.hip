@@ -303,11 +303,11 @@ __global__ __launch_bounds__(kBlock) void k_fe_eval(const int32_t* __restrict__ 
 template <typename T, typename OFF, int NL, int ROWCAP, int MAXP, int RPH, int SG, bool TPL>
 __global__ __launch_bounds__(kBlock) void k_gather_padded(
     const T* __restrict__ ke, const int32_t* __restrict__ pairs, const OFF* __restrict__ goff,
-    const uint32_t* __restrict__ tpairs, const uint8_t* __restrict__ tpl, int ntpl,
+    const uint32_t* __restrict__ tpairs, const uint8_t* __restrict__ tpl, int ntpl, int tshift,
     const int64_t* __restrict__ rowptr, int64_t nrows, const int32_t* __restrict__ rlist,
     double* __restrict__ vals) {
   // rlist: the rows to sum (nrows of them), or NULL for rows 0 .. nrows-1.
-  // TPL: the pair slots hold pair | template << 24 (tpairs, 0xFFFFFFFF empty)
+  // TPL: the pair slots hold pair | template << tshift (tpairs, 0xFFFFFFFF empty)
   // and the element row's offsets are template row tpl[template] (ntpl of
   // them, staged in LDS) instead of goff[pair]: the same offsets, no goff
   // stream.
@@ -340,8 +340,8 @@ __global__ __launch_bounds__(kBlock) void k_gather_padded(
       for (int k = 0; k < MAXP; ++k) {
         if constexpr (TPL) {
           const uint32_t w = ok ? tpairs[row * MAXP + k] : 0xFFFFFFFFu;
-          pr[r][k] = w == 0xFFFFFFFFu ? -1 : (int32_t)(w & 0xFFFFFFu);
-          tt[r][k] = (int)(w >> 24);
+          pr[r][k] = w == 0xFFFFFFFFu ? -1 : (int32_t)(w & ((1u << tshift) - 1u));
+          tt[r][k] = (int)(w >> tshift);
         } else {
           pr[r][k] = ok ? pairs[row * MAXP + k] : -1;
           tt[r][k] = 0;
@@ -565,7 +565,7 @@ static hipError_t gather_padded_go(const LaunchArgs& A) {
 #define BH_GP(MP, TP)                                                                          \
   hipLaunchKernelGGL((k_gather_padded<T, OFFT, NL, CAP, MP, RPH, SG, TP>), grid, dim3(kBlock),  \
                      0, A.stream, (const T*)A.ke, A.pairs, (const OFFT*)A.goff, A.tpairs, A.tpl, \
-                     A.ntpl, A.rowptr, A.nrows, A.rlist, A.vals)
+                     A.ntpl, A.tpl_shift, A.rowptr, A.nrows, A.rlist, A.vals)
   // offset templates (uint8 offsets, rows of up to 32 entries)
   constexpr bool kTpl = std::is_same<OFFT, uint8_t>::value && NL <= 32;
   const bool tp = kTpl && A.tpairs && A.ntpl > 0 && A.ntpl <= 255;

@@ -35,9 +35,11 @@ DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16"}
 KERNEL = {"fp64": "k_element_mfma<3,27,27,8,true,2>",
           "fp32": "k_element_gp<3,27,27,8,true,32,false>",
           "bf16x3": "k_element_mfma<3,27,27,8,true,1>"}
-GATHER = {"fp64": "k_gather_padded<double,unsigned char,27,128,8,2,32>",
-          "fp32": "k_gather_padded<float,unsigned char,27,128,8,2,32>",
-          "bf16x3": "k_gather_padded<float,unsigned char,27,128,8,2,32>"}
+# the last template argument: offsets from the pattern's templates (122 distinct
+# element-row offset vectors on the Q2 grid)
+GATHER = {"fp64": "k_gather_padded<double,unsigned char,27,128,8,2,32,true>",
+          "fp32": "k_gather_padded<float,unsigned char,27,128,8,2,32,true>",
+          "bf16x3": "k_gather_padded<float,unsigned char,27,128,8,2,32,true>"}
 # row gathers of the C3 space of per-cell order (25-slot rows), as tuned at 866^2
 GATHER_ORDERS = {"fp32": "k_gather_padded<float,unsigned char,25,128,8,2,32>",
                  "fp64": "k_gather_padded<double,unsigned char,25,128,8,1,32>"}
@@ -310,7 +312,8 @@ def bench_c5(args, torch, B):
             tot_nnz += plan.nnz
             tot_ms += ms
             res[f"{mode}_Q{order}"] = {"nnz": plan.nnz, "ms": ms, "cells": int(plan.V.mesh.ncells),
-                                       "tuning": plan.tuning(mode)}
+                                       "tuning": plan.tuning(mode),
+                                       "offset_templates": plan.offset_templates()}
             del buf
         res[f"{mode}_aggregate_gnnz_per_s"] = tot_nnz / (tot_ms * 1e-3) / 1e9
         el_ach = f_alg / (k_ms[0] * 1e-3) / 1e12
@@ -699,6 +702,7 @@ def bench_c3(args, torch, B):
             "elements_per_s": m.ncells / (ms * 1e-3),
             "element_launches_per_step": int((lk == 0).sum() // args.steps),
             "launches_per_step": int(len(lk) // args.steps), "tuning": plan.tuning(mode),
+            "offset_templates": plan.offset_templates(),
             "element_kernels": {"bound": "mfma", "achieved": el_ach, "peak": peak,
                                 "unit": "TFLOP/s", "frac": el_ach / peak, "kernel": kern,
                                 "avg_launch_ms": el_ms, "flops_per_step": f_alg},
@@ -1786,7 +1790,8 @@ def bench_c2(args, torch, B):
                          "elements_per_s": ncells / (ms_step * 1e-3),
                          "launches_per_step": len(launches) / args.steps,
                          "rel_err_vs_fp64_gpu": err, "roofline": dom, "kernels": kernels,
-                         "tuning": plan.tuning(mode)}
+                         "tuning": plan.tuning(mode),
+                         "offset_templates": plan.offset_templates()}
         if mode == "fp64":
             results[mode]["onepass"] = plan.onepass()
             if results[mode]["onepass"]["ok"]:

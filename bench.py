@@ -33,7 +33,7 @@ PEAK_HBM_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
 DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16"}
 KERNEL = {"fp64": "k_element_mfma<3,27,27,8,true,2>",
-          "fp32": "k_element_gp<3,27,27,8,true,32,false>",
+          "fp32": "k_element_gp<3,27,27,8,true,32,false,8>",
           "bf16x3": "k_element_mfma<3,27,27,8,true,1>"}
 # the last template argument: offsets from the pattern's templates (122 distinct
 # element-row offset vectors on the Q2 grid)
@@ -41,8 +41,9 @@ GATHER = {"fp64": "k_gather_padded<double,unsigned char,27,128,8,2,32,true>",
           "fp32": "k_gather_padded<float,unsigned char,27,128,8,2,32,true>",
           "bf16x3": "k_gather_padded<float,unsigned char,27,128,8,2,32,true>"}
 # row gathers of the C3 space of per-cell order (25-slot rows), as tuned at 866^2
-GATHER_ORDERS = {"fp32": "k_gather_padded<float,unsigned char,25,128,8,2,32>",
-                 "fp64": "k_gather_padded<double,unsigned char,25,128,8,1,32>"}
+# (per-entry offsets: the space has more than 255 distinct offset vectors)
+GATHER_ORDERS = {"fp32": "k_gather_padded<float,unsigned char,25,128,8,2,32,false>",
+                 "fp64": "k_gather_padded<double,unsigned char,25,128,8,1,32,false>"}
 FUSED_KERNEL = "k_element_gp_fused<3,27,27,8,true,32,8> (element matrices + LDS row sums)"
 ONEPASS_KERNEL = "k_sf_q2<double,8,true>"
 

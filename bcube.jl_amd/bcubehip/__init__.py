@@ -7,7 +7,7 @@ mirror of Bcube.jl's assembly API (src/assembler/assembler.jl).
 from ._lib import BcubeHipError, LIB_PATH, lib
 from .mesh import (AbstractBcubeBackend, BcubeBackendHIP, BcubeBackendHostOnly, Mesh,
                    batch_meshes, get_bcube_backend, hexa27_mesh, hexa8_to_hexa27, hexa_mesh,
-                   jitter_nodes, mixed_tri_quad_mesh, one_cell_mesh, rectangle_mesh,
+                   jitter_nodes, line_mesh, mixed_tri_quad_mesh, one_cell_mesh, rectangle_mesh,
                    sinusoidal_map, sub_mesh)
 from .fespace import (DofHandler, FunctionSpace, MultiFESpace, TestFESpace, TrialFESpace,
                       component_numbering)
@@ -47,6 +47,6 @@ __all__ = [
     "nodes_gmsh2cgns", "read_mesh", "projection_l2_", "MultiFESpace", "component_numbering",
     "ddot", "MultiBilinearPlan", "BoundaryFaceDomain", "avg", "BoundaryFaces",
     "FaceBilinearPlan", "EulerFlux", "RoeFlux", "PeriodicBCType", "Translation", "EulerDG",
-    "periodic_face_pairs", "AffineFESystem", "OrderDegree", "one_cell_mesh",
+    "periodic_face_pairs", "AffineFESystem", "OrderDegree", "one_cell_mesh", "line_mesh",
     "fe_values_at_points",
 ]

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libbcubehip.so")
 BH_OK = 0
 ERRORS = {-1: "BH_ERR_ARG", -2: "BH_ERR_UNSUPPORTED", -3: "BH_ERR_HIP", -4: "BH_ERR_OOM",
           -5: "BH_ERR_RANGE"}
-CELLTYPE = {"Quad4": 0, "Quad9": 1, "Tri3": 2, "Hexa8": 3, "Hexa27": 4}
+CELLTYPE = {"Quad4": 0, "Quad9": 1, "Tri3": 2, "Hexa8": 3, "Hexa27": 4, "Bar2": 5}
 MODE = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 HOST, DEVICE = 0, 1
 FORM_STIFFNESS, FORM_MASS, FORM_SOURCE = 0, 1, 10

@@ -89,7 +89,7 @@ class DofHandler:
         self.continuous = bool(isContinuous)
         nd = {0: (fs.degree + 1) ** 2, 1: (fs.degree + 1) ** 2,
               2: (fs.degree + 1) * (fs.degree + 2) // 2, 3: (fs.degree + 1) ** 3,
-              4: (fs.degree + 1) ** 3}
+              4: (fs.degree + 1) ** 3, 5: fs.degree + 1}
         total = int(sum(nd[int(t)] for t in mesh.celltypes))
         self.iglob = np.zeros(total, dtype=np.int64)
         self.offset = np.zeros(mesh.ncells, dtype=np.int64)

@@ -460,6 +460,16 @@ def lower_bilinear(expr):
             if kind in ("qp", "fe"):
                 raise NotImplementedError("stiffness coefficient must be constant or per cell")
             return "stiffness", kind, data, scale
+        # u ⋅ v of scalar fields (Julia's dot of two numbers): c u v
+        us = [f for f in inner if _is(f, Placeholder, "trial")]
+        vs = [f for f in inner if _is(f, Placeholder, "test")]
+        if (not gu and not gv and len(us) == 1 and len(vs) == 1 and (us[0] in left) != (vs[0] in left)
+                and not any(isinstance(f, (Dot, Grad, Add, VecConst)) for f in inner + rest
+                            if f is not us[0] and f is not vs[0])):
+            kind, data, scale = _coef([f for f in inner + rest if f is not us[0] and f is not vs[0]])
+            if kind in ("qp", "fe"):
+                raise NotImplementedError("mass coefficient must be constant or per cell")
+            return "mass", kind, data, scale
     elif not dots:
         us = [f for f in rest if _is(f, Placeholder, "trial")]
         vs = [f for f in rest if _is(f, Placeholder, "test")]

@@ -138,7 +138,7 @@ def get_bcube_backend() -> AbstractBcubeBackend:
 
 
 NNODES = {0: 4, 1: 9, 2: 3, 3: 8, 4: 27}
-DIM = {0: 2, 1: 2, 2: 2, 3: 3, 4: 3}
+DIM = {0: 2, 1: 2, 2: 2, 3: 3, 4: 3, 5: 1}
 
 
 class Mesh:
@@ -222,10 +222,14 @@ class Mesh:
 def one_cell_mesh(kind: str, xmin=-1.0, xmax=1.0, ymin=-1.0, ymax=1.0, zmin=-1.0, zmax=1.0,
                   backend: AbstractBcubeBackend | None = None) -> Mesh:
     """one_cell_mesh(type; xmin, xmax, ...) (mesh_generator.jl:38-65) for one
-    first-order cell: :quad/:square (Quad4, one_quad_mesh :600-608), :tri/
+    first-order cell: :line/:bar (Bar2), :quad/:square (Quad4, one_quad_mesh :600-608), :tri/
     :triangle (Tri3, one_tri_mesh :576-581), :hexa/:cube (Hexa8, one_hexa_mesh
     :681-695), nodes in those functions' order."""
     kind = kind.lstrip(":")
+    if kind in ("line", "bar"):  # one_line_mesh(Val(1)) :554-560
+        m = Mesh([[xmin], [xmax]], ["Bar2"], [[0, 1]], backend)
+        m.bc_nodes = {"xmin": np.array([0]), "xmax": np.array([1])}
+        return m
     if kind in ("quad", "square"):
         nodes = [[xmin, ymin], [xmax, ymin], [xmax, ymax], [xmin, ymax]]
         m = Mesh(nodes, ["Quad4"], [[0, 1, 2, 3]], backend)
@@ -239,6 +243,20 @@ def one_cell_mesh(kind: str, xmin=-1.0, xmax=1.0, ymin=-1.0, ymax=1.0, zmin=-1.0
                  [xmin, ymin, zmax], [xmax, ymin, zmax], [xmax, ymax, zmax], [xmin, ymax, zmax]]
         return Mesh(nodes, ["Hexa8"], [list(range(8))], backend)
     raise NotImplementedError(f"one_cell_mesh: {kind}")
+
+
+def line_mesh(n, xmin=0.0, xmax=1.0, names=("xmin", "xmax"),
+              backend: AbstractBcubeBackend | None = None) -> Mesh:
+    """line_mesh(n; xmin, xmax) (mesh_generator.jl:104-146), first order: n
+    nodes at xmin + (i-1)Δx, cell i joining nodes i and i+1, boundary tags on
+    the end nodes."""
+    if n < 2:
+        raise ValueError("a line mesh needs at least 2 vertices")
+    x = xmin + (xmax - xmin) / (n - 1) * np.arange(n, dtype=np.float64)
+    c2n = np.stack([np.arange(n - 1), np.arange(1, n)], axis=1)
+    m = Mesh(x[:, None], ["Bar2"] * (n - 1), c2n, backend)
+    m.bc_nodes = {names[0]: np.array([0]), names[1]: np.array([n - 1])}
+    return m
 
 
 def rectangle_mesh(nx, ny, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, order=1,

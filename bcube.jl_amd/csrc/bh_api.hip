@@ -666,7 +666,7 @@ int bh_mesh_create(bh_ctx c, int dim, int64_t nnodes, const double* coords, int6
                    const int32_t* celltypes, const int64_t* c2n_ptr, const int64_t* c2n,
                    bh_mesh* out) {
   // c == NULL creates a host-only mesh: numbering and pattern queries work, assembly does not.
-  if (!out || !coords || !celltypes || !c2n_ptr || !c2n || (dim != 2 && dim != 3) ||
+  if (!out || !coords || !celltypes || !c2n_ptr || !c2n || dim < 1 || dim > 3 ||
       nnodes <= 0 || ncells <= 0)
     return fail(c, BH_ERR_ARG, "bh_mesh_create: bad arguments");
   *out = nullptr;
@@ -693,7 +693,7 @@ int bh_mesh_create(bh_ctx c, int dim, int64_t nnodes, const double* coords, int6
   h.celltype = h.ctype[0];
   h.uniform = std::all_of(h.ctype.begin(), h.ctype.end(), [&](int t) { return t == h.celltype; });
   if (h.uniform && (h.celltype == BH_QUAD4 || h.celltype == BH_QUAD9 || h.celltype == BH_HEXA8 ||
-                   h.celltype == BH_HEXA27 || h.celltype == BH_TRI3)) {
+                   h.celltype == BH_HEXA27 || h.celltype == BH_TRI3 || h.celltype == BH_BAR2)) {
     m->geo = h.celltype;
     m->npc = nnodes_of(h.celltype);
   } else if (!h.uniform && std::all_of(h.ctype.begin(), h.ctype.end(), [](int t) {
@@ -3073,6 +3073,7 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
       return fail(c, BH_ERR_UNSUPPORTED,
                   "no FP32 linear-form kernel for this (cell type, order, quadrature)");
     nq = std::max(nq, geo == BH_TRI3 ? bkey[bk]
+                                     : m->h.dim == 1 ? bkey[bk]
                                      : bkey[bk] * bkey[bk] * (m->h.dim == 3 ? bkey[bk] : 1));
   }
   for (int k = 0; k < nterms; ++k) {
@@ -3633,7 +3634,9 @@ int bh_quadrature_points(bh_mesh m, int quad_degree, double* xq, int where, int3
     while (order <= 4 && !kernel_supported(geo, order, bkey[b])) ++order;
     if (order > 4) return fail(c, BH_ERR_UNSUPPORTED, "no kernel for this quadrature");
     border[b] = order;
-    const int nqb = geo == BH_TRI3 ? bkey[b] : bkey[b] * bkey[b] * (m->h.dim == 3 ? bkey[b] : 1);
+    const int nqb = geo == BH_TRI3 ? bkey[b]
+                    : m->h.dim == 1 ? bkey[b]
+                    : bkey[b] * bkey[b] * (m->h.dim == 3 ? bkey[b] : 1);
     nq = std::max(nq, nqb);
   }
   if (!xq) {  // size query

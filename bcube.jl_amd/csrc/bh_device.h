@@ -11,7 +11,10 @@ constexpr int kBlock = 256;
 
 template <int D>
 __device__ __forceinline__ double invert(const double (&J)[D][D], double (&Ji)[D][D]) {
-  if constexpr (D == 2) {
+  if constexpr (D == 1) {
+    Ji[0][0] = 1.0 / J[0][0];
+    return J[0][0];
+  } else if constexpr (D == 2) {
     double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
     double r = 1.0 / det;
     Ji[0][0] = J[1][1] * r;

@@ -42,14 +42,15 @@ int shape_of(int ct) {  // src/mesh/shape.jl:25-32
     case BH_QUAD4: case BH_QUAD9: return SQUARE;
     case BH_TRI3: return TRIANGLE;
     case BH_HEXA8: case BH_HEXA27: return CUBE;
+    case BH_BAR2: return LINE;
   }
   return -1;
 }
-int dim_of_shape(int s) { return s == CUBE ? 3 : 2; }
+int dim_of_shape(int s) { return s == CUBE ? 3 : (s == LINE ? 1 : 2); }
 int nnodes_of(int ct) {
   switch (ct) {
     case BH_QUAD4: return 4; case BH_QUAD9: return 9; case BH_TRI3: return 3;
-    case BH_HEXA8: return 8; case BH_HEXA27: return 27;
+    case BH_HEXA8: return 8; case BH_HEXA27: return 27; case BH_BAR2: return 2;
   }
   return 0;
 }
@@ -57,7 +58,7 @@ int ndofs_lagrange(int shape, int order) {  // hypercube.jl:122-133, triangle.jl
   if (order < 0) return 0;
   if (shape == TRIANGLE) return (order + 1) * (order + 2) / 2;
   int n = order + 1;
-  return shape == SQUARE ? n * n : n * n * n;
+  return shape == LINE ? n : (shape == SQUARE ? n * n : n * n * n);
 }
 
 // ---------------------------------------------------------------------------
@@ -273,15 +274,17 @@ Tables make_tables(int shape, int order, int quad_degree, int geo_ct) {
   int n1 = gauss_points_per_axis(quad_degree);
   std::vector<double> gx, gw;
   gauss_legendre(n1, gx, gw);
-  T.nq = d == 2 ? n1 * n1 : n1 * n1 * n1;
-  // tensor rule, first axis fastest (quadrature.jl:455-487)
+  T.nq = d == 1 ? n1 : (d == 2 ? n1 * n1 : n1 * n1 * n1);
+  // tensor rule, first axis fastest (quadrature.jl:455-487); the Line rule
+  // is the 1D Gauss-Legendre rule itself
   for (int q = 0; q < T.nq; ++q) {
     int i = q % n1, j = (q / n1) % n1, k = q / (n1 * n1);
-    double w = gw[i] * gw[j];
+    double w = gw[i];
+    if (d >= 2) w *= gw[j];
     if (d == 3) w *= gw[k];
     T.w.push_back(w);
     T.xi.push_back(gx[i]);
-    T.xi.push_back(gx[j]);
+    if (d >= 2) T.xi.push_back(gx[j]);
     if (d == 3) T.xi.push_back(gx[k]);
   }
   // tensor Lagrange basis, dof index first-axis fastest (hypercube.jl:56-92)
@@ -295,7 +298,10 @@ Tables make_tables(int shape, int order, int quad_degree, int geo_ct) {
     for (int l = 0; l < T.nl; ++l) {
       int i = l % n, j = (l / n) % n, k = l / (n * n);
       double* g = &T.dphi[((size_t)q * T.nl + l) * d];
-      if (d == 2) {
+      if (d == 1) {  // Line: the 1D Lagrange basis (hypercube.jl:51-54)
+        T.phi[(size_t)q * T.nl + l] = v[0][i];
+        g[0] = dv[0][i];
+      } else if (d == 2) {
         T.phi[(size_t)q * T.nl + l] = v[0][i] * v[1][j];
         g[0] = dv[0][i] * v[1][j];
         g[1] = v[0][i] * dv[1][j];
@@ -324,7 +330,9 @@ Tables make_tables(int shape, int order, int quad_degree, int geo_ct) {
       {0, -1, 0},   {1, 0, 0},   {0, 1, 0},  {-1, 0, 0},  {0, 0, 1},   {0, 0, 0}};
   const bool quadratic = geo_ct == BH_QUAD9 || geo_ct == BH_HEXA27;
   T.ngeo = geo_ct == BH_HEXA27 ? 27 : geo_ct == BH_QUAD9 ? 9
-           : geo_ct == BH_HEXA8 ? 8 : (geo_ct == BH_QUAD4 ? 4 : 0);
+           : geo_ct == BH_HEXA8 ? 8 : geo_ct == BH_QUAD4 ? 4 : (geo_ct == BH_BAR2 ? 2 : 0);
+  // Bar2: nodes at ξ = -1, 1 (mapping.jl's Bar2 map (1-ξ)/2 x1 + (1+ξ)/2 x2)
+  static const int8_t b2[2] = {-1, 1};
   T.gphi.assign((size_t)T.nq * T.ngeo, 0.0);
   T.dgphi.assign((size_t)T.nq * T.ngeo * d, 0.0);
   // 1D geometric basis at t for the node coordinate s in {-1, 0, 1}
@@ -350,7 +358,7 @@ Tables make_tables(int shape, int order, int quad_degree, int geo_ct) {
       for (int a = 0; a < d; ++a) {
         const int s = geo_ct == BH_HEXA27 ? nh27[k][a]
                       : geo_ct == BH_QUAD9 ? nq9[k][a]
-                      : d == 3 ? h8[k][a] : q4[k][a];
+                      : d == 1 ? b2[k] : d == 3 ? h8[k][a] : q4[k][a];
         basis1(s, x[a], f[a], df[a]);
       }
       double val = 1.0;
@@ -399,10 +407,11 @@ static Lists faces_of(int ct) {
 }
 
 static Lists vertex_dofs(int shape, int order) {
-  int nv = shape == CUBE ? 8 : (shape == SQUARE ? 4 : 3);
+  int nv = shape == CUBE ? 8 : (shape == SQUARE ? 4 : (shape == LINE ? 2 : 3));
   if (order == 0) return Lists(nv);
   if (shape == TRIANGLE) return {{0}, {1}, {2}};
   int n = order + 1, m = n - 1;
+  if (shape == LINE) return {{0}, {m}};  // _idof_by_vertex(::Line), hypercube.jl:136-141
   if (shape == SQUARE) return {{0}, {m}, {m + n * m}, {n * m}};
   int nn = n * n;
   return {{0}, {m}, {m + n * m}, {n * m},

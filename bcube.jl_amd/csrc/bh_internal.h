@@ -15,7 +15,7 @@ namespace bh {
 typedef int64_t i64;
 
 // Reference shapes (src/mesh/shape.jl) and their cell types.
-enum Shape { SQUARE = 0, TRIANGLE = 1, CUBE = 2 };
+enum Shape { SQUARE = 0, TRIANGLE = 1, CUBE = 2, LINE = 3 };
 int shape_of(int celltype);
 int dim_of_shape(int shape);
 int nnodes_of(int celltype);

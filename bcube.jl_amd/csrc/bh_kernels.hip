@@ -124,7 +124,9 @@ __global__ __launch_bounds__(kBlock) void k_element_f64(
       const double* g = s_g[cs][q];
       const double* ri = &s_tab[L::R + (q * NL + li) * 4];
       double t[D];
-      if constexpr (D == 2) {
+      if constexpr (D == 1) {
+        t[0] = g[0] * ri[0];
+      } else if constexpr (D == 2) {
         t[0] = g[0] * ri[0] + g[1] * ri[1];
         t[1] = g[1] * ri[0] + g[2] * ri[1];
       } else {
@@ -241,7 +243,9 @@ __global__ __launch_bounds__(kBlock) void k_element_rect_f64(
       const double* g = s_g[cs][q];
       const double* rj = &s_ru[(q * NLU + lj) * 4];
       double t[D];
-      if constexpr (D == 2) {
+      if constexpr (D == 1) {
+        t[0] = g[0] * rj[0];
+      } else if constexpr (D == 2) {
         t[0] = g[0] * rj[0] + g[1] * rj[1];
         t[1] = g[1] * rj[0] + g[2] * rj[1];
       } else {
@@ -715,6 +719,9 @@ struct Entry {
 #define BH_ENTRY_H27(ORDER, NQ1)                                                  \
   {BH_HEXA27, ORDER, NQ1, &launch_all<3, BH_H(ORDER, NQ1), 27>,                    \
    &table_size<3, BH_H(ORDER, NQ1), 27>}
+// Bar2 lines (1D): order 0..3, NQ1 Gauss points
+#define BH_ENTRY_L(ORDER, NQ1) \
+  {BH_BAR2, ORDER, NQ1, &launch_all<1, (ORDER) + 1, NQ1, 2>, &table_size<1, (ORDER) + 1, NQ1, 2>}
 // triangles: key = points of the triangle rule
 #define BH_ENTRY_T(ORDER, NPTS)                                                             \
   {BH_TRI3, ORDER, NPTS, &launch_all<2, ((ORDER) + 1) * ((ORDER) + 2) / 2, NPTS, 3>,          \
@@ -730,6 +737,8 @@ static const Entry kTable[] = {
     BH_ENTRY_T(4, 16),
     BH_ENTRY_Q9(1, 2), BH_ENTRY_Q9(1, 3), BH_ENTRY_Q9(2, 2), BH_ENTRY_Q9(2, 3), BH_ENTRY_Q9(2, 4),
     BH_ENTRY_H27(1, 2), BH_ENTRY_H27(1, 3), BH_ENTRY_H27(2, 2), BH_ENTRY_H27(2, 3),
+    BH_ENTRY_L(0, 1), BH_ENTRY_L(0, 2), BH_ENTRY_L(1, 1), BH_ENTRY_L(1, 2), BH_ENTRY_L(1, 3),
+    BH_ENTRY_L(2, 2), BH_ENTRY_L(2, 3), BH_ENTRY_L(3, 3), BH_ENTRY_L(3, 4),
 };
 
 bool kernel_supported(int geo, int order, int nq1) {
@@ -774,7 +783,9 @@ struct RectEntry {
   {BH_HEXA8, PU, PV, NQ1,                                                                      \
    &launch_rect_t<3, (PU + 1) * (PU + 1) * (PU + 1), (PV + 1) * (PV + 1) * (PV + 1),            \
                   NQ1 * NQ1 * NQ1, 8>}
+#define BH_RL(PU, PV, NQ1) {BH_BAR2, PU, PV, NQ1, &launch_rect_t<1, PU + 1, PV + 1, NQ1, 2>}
 static const RectEntry kRectTable[] = {
+    BH_RL(1, 2, 2), BH_RL(2, 1, 2), BH_RL(1, 2, 3), BH_RL(2, 1, 3),
     BH_RQ(1, 2, 2), BH_RQ(2, 1, 2), BH_RQ(1, 2, 3), BH_RQ(2, 1, 3),
     BH_RT(1, 2, 3), BH_RT(2, 1, 3), BH_RT(1, 2, 4), BH_RT(2, 1, 4), BH_RT(1, 2, 6), BH_RT(2, 1, 6),
     BH_RH(1, 2, 2), BH_RH(2, 1, 2), BH_RH(1, 2, 3), BH_RH(2, 1, 3),
@@ -818,7 +829,8 @@ int gather_shapes(int nl, int max_row, int offset_bytes, int lanes[4], int rows[
 
 bool gather_supported(int nl) {
   switch (nl) {
-    case 3: case 4: case 6: case 8: case 9: case 10: case 15: case 16: case 25: case 27: case 64:
+    case 1: case 2: case 3: case 4: case 6: case 8: case 9: case 10: case 15: case 16: case 25:
+    case 27: case 64:
       return true;
   }
   return false;
@@ -826,6 +838,8 @@ bool gather_supported(int nl) {
 
 hipError_t launch_gather(int nl, const LaunchArgs& A) {
   switch (nl) {
+    case 1: return gather_nl<1>(A);
+    case 2: return gather_nl<2>(A);
     case 3: return gather_nl<3>(A);
     case 4: return gather_nl<4>(A);
     case 6: return gather_nl<6>(A);

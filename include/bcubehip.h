@@ -38,7 +38,7 @@ extern "C" {
 #define BH_ERR_RANGE (-5)       /* dof or node index out of range             */
 
 /* Cell types, CGNS node order (src/mesh/entity.jl).  */
-enum { BH_QUAD4 = 0, BH_QUAD9 = 1, BH_TRI3 = 2, BH_HEXA8 = 3, BH_HEXA27 = 4 };
+enum { BH_QUAD4 = 0, BH_QUAD9 = 1, BH_TRI3 = 2, BH_HEXA8 = 3, BH_HEXA27 = 4, BH_BAR2 = 5 };
 
 /* Arithmetic of the element contractions.  Geometry (J, J^-1, |J|) and the
  * global scatter-sum are FP64 in every mode. */

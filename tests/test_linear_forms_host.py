@@ -45,3 +45,14 @@ def test_fe_function_coefficient_and_refusals():
         lower_bilinear(_integrals(ex)[0].expr)
     with pytest.raises(NotImplementedError):
         _lower(lambda v: B.integrate(B.dot(B.grad(v), B.grad(v)), dΩ), U)
+
+
+def test_scalar_dot_is_mass():
+    """u ⋅ v of scalar fields (test_assembler.jl:233-239) is the mass form."""
+    m, U = _space(p=2)
+    dΩ = B.Measure(B.CellDomain(m), 2)
+    ex = B.integrate(B.dot(Placeholder("trial", U), 3.0 * Placeholder("test", U)), dΩ)
+    assert lower_bilinear(_integrals(ex)[0].expr) == ("mass", "const", None, 3.0)
+    ex = B.integrate(B.dot(B.grad(Placeholder("trial", U)), Placeholder("test", U)), dΩ)
+    with pytest.raises(NotImplementedError):
+        lower_bilinear(_integrals(ex)[0].expr)

@@ -19,6 +19,7 @@ MODE = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 HOST, DEVICE = 0, 1
 FORM_STIFFNESS, FORM_MASS, FORM_SOURCE = 0, 1, 10
 FORM_GRAD_SOURCE = (11, 12, 13)  # ∫ f ∂v/∂x_a dΩ, a = 0, 1, 2
+FORM_ADVECTION = (2, 3, 4)  # ∫ η ∂u/∂x_a v dΩ, a = 0, 1, 2
 COEF_CONST, COEF_CELL, COEF_QP = 0, 1, 2
 
 EXPORTED = [

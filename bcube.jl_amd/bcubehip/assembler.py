@@ -16,7 +16,7 @@ import scipy.sparse as sp
 from . import _lib as L
 from .faces import FaceBilinearPlan, assemble_linear_faces_, has_faces
 from .forms import (CellDomain, Placeholder, _split_sum, lower_bilinear, lower_dg_transport, lower_linear,
-                    lower_linear_terms,
+                    lower_bilinear_terms, lower_linear_terms,
                     lower_multi_bilinear, lower_multi_linear)
 
 _COEF = {"const": L.COEF_CONST, "cell": L.COEF_CELL, "qp": L.COEF_QP, "fe": L.COEF_QP}
@@ -45,10 +45,20 @@ def _single_measure(items):
 
 def _bilinear_bh_terms(lowered, mesh):
     keep, terms = [], []
+    arrays = {}  # one array per coefficient object: the axis terms of (c⋅∇u) v share it
     for form, kind, data, scale in lowered:
-        t = L.BhTerm(_FORM[form], _COEF[kind], None, scale)
+        if isinstance(form, tuple):  # ("adv", a): ∫ η ∂u/∂x_a v
+            if form[1] >= mesh.dim:
+                raise NotImplementedError("the vector in (c⋅∇u) v has more entries than the "
+                                          "mesh has dimensions")
+            code = L.FORM_ADVECTION[form[1]]
+        else:
+            code = _FORM[form]
+        t = L.BhTerm(code, _COEF[kind], None, scale)
         if kind == "cell":
-            arr = np.ascontiguousarray(data, dtype=np.float64)
+            arr = arrays.get(id(data))
+            if arr is None:
+                arr = arrays[id(data)] = np.ascontiguousarray(data, dtype=np.float64)
             if arr.size != mesh.ncells:
                 raise ValueError("MeshCellData size differs from the number of cells")
             keep.append(arr)
@@ -270,7 +280,8 @@ class BilinearPlan:
         self.measure = _single_measure(items)
         if self.measure.domain.mesh is not V.mesh:
             raise ValueError("the measure's mesh differs from the space's mesh")
-        lowered = [lower_bilinear(t) for it in items for t in _split_sum(it.expr)]
+        lowered = [x for it in items for t in _split_sum(it.expr)
+                   for x in lower_bilinear_terms(t, V.mesh.dim)]
         self._keep, self.terms, self.nterms = _bilinear_bh_terms(lowered, V.mesh)
         ctx = V.mesh.backend.ctx
         h = C.c_void_p()

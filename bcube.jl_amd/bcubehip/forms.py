@@ -481,6 +481,40 @@ def lower_bilinear(expr):
     raise NotImplementedError("bilinear integrand is not one of: η ∇u⋅∇v, c u v")
 
 
+def lower_bilinear_terms(expr, dim):
+    """lower_bilinear, plus the advection form α (c⋅∇u) v (c a constant
+    vector; on a line also ∇(u)⋅v, c = 1) as one term per axis, form
+    ("adv", a) with scale α c_a.  Returns a list."""
+    fs = _factors(expr, [])
+    dots = [f for f in fs if isinstance(f, Dot)]
+    if len(dots) == 1:
+        d = dots[0]
+        left, right = _factors(d.a, []), _factors(d.b, [])
+        rest = [f for f in fs if f is not d]
+        for g, w in ((left, right), (right, left)):
+            gu = [f for f in g if isinstance(f, Grad) and _is(f.a, Placeholder, "trial")]
+            if len(gu) != 1:
+                continue
+            others = [f for f in g if f is not gu[0]] + w + rest
+            vs = [f for f in others if _is(f, Placeholder, "test")]
+            cs = [f for f in others if isinstance(f, VecConst)]
+            coefs = [f for f in others if f not in vs and f not in cs]
+            if len(vs) != 1 or len(cs) > 1 or any(isinstance(f, (Dot, Grad, Add, Placeholder))
+                                                  for f in coefs):
+                continue
+            if cs:
+                c = cs[0].v
+            elif dim == 1:
+                c = np.ones(1)
+            else:
+                continue
+            kind, data, scale = _coef(coefs)
+            if kind in ("qp", "fe"):
+                raise NotImplementedError("advection coefficient must be constant or per cell")
+            return [(("adv", a), kind, data, scale * float(ca)) for a, ca in enumerate(c)]
+    return [lower_bilinear(expr)]
+
+
 def _split_sum(e):
     """The product terms of an integrand: sums inside products distributed."""
     if isinstance(e, Add):

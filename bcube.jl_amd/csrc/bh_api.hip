@@ -175,6 +175,7 @@ struct bh_pattern_s {
   void* d_ke = nullptr;      // element rows, (ncells+16)*nl*ke_stride(nl) FP64 (FP32 uses half)
   double* d_vals = nullptr;
   double* d_coef = nullptr;  // staged host coefficients (2 * ncells)
+  double* d_acoef = nullptr; // staged host advection coefficient (ncells)
 };
 
 // Dirichlet dofs of a pattern and the CSC value positions they touch
@@ -2208,6 +2209,7 @@ int bh_pattern_destroy(bh_pattern p) {
     (void)hipFree(p->d_ke);
     (void)hipFree(p->d_vals);
     (void)hipFree(p->d_coef);
+    (void)hipFree(p->d_acoef);
     FusedDev& f = p->fused;
     (void)hipFree(f.d_bstep);
     (void)hipFree(f.d_cells);
@@ -2450,14 +2452,20 @@ static void save_tune_cache(bh_ctx c) {
 // (cell, trial dof j) by k_element_rect_f64, then the row gather over the CSC
 // columns (the pattern's internal rows are U's dofs, their entries V's).
 // Uniform meshes whose (cell type, trial order, test order, rule) has an
-// instantiation.
+// instantiation.  Forms that are not symmetric (∫ α (c⋅∇u) v, the
+// BH_FORM_ADVECTION_* terms) take this path also when U == V: its element rows
+// are trial-major, which is what the CSC columns need.
 static int assemble_rect(bh_pattern p, int nterms, const bh_term* terms, int quad_degree,
                          int mode, double* nzval, int where, int coef_where, bool noalloc) {
   bh_space U = p->U, V = p->V;
   bh_mesh m = V->mesh;
   bh_ctx c = m->ctx;
-  if (noalloc) return fail(c, BH_ERR_UNSUPPORTED, "the no-allocation call needs U == V");
-  if (mode != BH_FP64) return fail(c, BH_ERR_UNSUPPORTED, "assembly with U != V is FP64 only");
+  if (noalloc)
+    return fail(c, BH_ERR_UNSUPPORTED,
+                "the no-allocation call needs U == V and a symmetric form");
+  if (mode != BH_FP64)
+    return fail(c, BH_ERR_UNSUPPORTED,
+                "assembly with U != V or of advection terms is FP64 only");
   if (!U->orders.empty() || !V->orders.empty())
     return fail(c, BH_ERR_UNSUPPORTED, "U != V: scalar spaces of one order each");
   if (m->geo < 0 || m->buckets.size() != 1)
@@ -2465,14 +2473,26 @@ static int assemble_rect(bh_pattern p, int nterms, const bh_term* terms, int qua
   if (quad_degree <= 0) return fail(c, BH_ERR_ARG, "bad quadrature degree");
   const bh_term* stiff = nullptr;
   const bh_term* mass = nullptr;
+  const bh_term* adv = nullptr;  // the first advection term (its coefficient is every one's)
+  double avec[3] = {0.0, 0.0, 0.0};
+  bool seen[3] = {false, false, false};
   for (int k = 0; k < nterms; ++k) {
     const bh_term& t = terms[k];
     if (t.coef_kind != BH_COEF_CONST && t.coef_kind != BH_COEF_CELL)
       return fail(c, BH_ERR_UNSUPPORTED, "bilinear coefficients must be constant or per cell");
     if (t.coef_kind == BH_COEF_CELL && !t.coef) return fail(c, BH_ERR_ARG, "per-cell coefficient is NULL");
+    const int ax = t.form - BH_FORM_ADVECTION_X;
     if (t.form == BH_FORM_STIFFNESS && !stiff) stiff = &t;
     else if (t.form == BH_FORM_MASS && !mass) mass = &t;
-    else return fail(c, BH_ERR_UNSUPPORTED, "unsupported or repeated bilinear term");
+    else if (ax >= 0 && ax < m->h.dim && !seen[ax]) {
+      if (adv && (t.coef_kind != adv->coef_kind || t.coef != adv->coef))
+        return fail(c, BH_ERR_UNSUPPORTED, "advection terms with different coefficients");
+      if (!adv) adv = &t;
+      seen[ax] = true;
+      avec[ax] = t.scale;
+    } else {
+      return fail(c, BH_ERR_UNSUPPORTED, "unsupported or repeated bilinear term");
+    }
   }
   const GeoBucket& gb = m->buckets[0];
   const int key = rule_key(gb.geo, quad_degree);
@@ -2513,6 +2533,14 @@ static int assemble_rect(bh_pattern p, int nterms, const bh_term* terms, int qua
     A.mcoef = coef_ptr(*mass, coef_where, p->d_coef + nc, nc, c->stream);
     A.mass_scale = mass->scale;
   }
+  if (adv) {
+    A.adv = true;
+    if (adv->coef_kind == BH_COEF_CELL) {
+      if (coef_where == BH_HOST && !p->d_acoef) BH_HIP(c, dalloc(&p->d_acoef, (size_t)nc));
+      A.acoef = coef_ptr(*adv, coef_where, p->d_acoef, nc, c->stream);
+    }
+    for (int a = 0; a < 3; ++a) A.avec[a] = avec[a];
+  }
   time_begin(c, 0);
   BH_HIP(c, launch_rect(gb.geo, U->order, V->order, key, A));
   time_end(c);
@@ -2552,7 +2580,11 @@ static int assemble_bilinear_impl(bh_pattern p, int nterms, const bh_term* terms
   if (where == BH_HOST && !nzval) return fail(c, BH_ERR_ARG, "nzval is NULL");
   if ((where == BH_HOST || coef_where == BH_HOST) && c->capturing)
     return fail(c, BH_ERR_ARG, "host-`where` assembly cannot be captured in a graph");
-  if (p->U != p->V)
+  bool advection = false;
+  for (int k = 0; k < nterms; ++k)
+    advection = advection || (terms[k].form >= BH_FORM_ADVECTION_X &&
+                              terms[k].form <= BH_FORM_ADVECTION_Z);
+  if (p->U != p->V || advection)
     return assemble_rect(p, nterms, terms, quad_degree, mode, nzval, where, coef_where, noalloc);
   if (mode != BH_FP64 && mode != BH_FP32 && mode != BH_BF16X3)
     return fail(c, BH_ERR_UNSUPPORTED, "bilinear modes: BH_FP64, BH_FP32, BH_BF16X3");

@@ -59,6 +59,10 @@ struct LaunchArgs {
   const double* mcoef = nullptr;
   double mass_scale = 0.0;
   bool mass = false;
+  // rectangular element kernel: ∫ α (c⋅∇u) v with α = acoef[cell] (NULL: 1)
+  bool adv = false;
+  const double* acoef = nullptr;
+  double avec[3] = {0.0, 0.0, 0.0};
   const int64_t* rowptr = nullptr;
   int offset_bytes = 1;
   // source (FE_EVAL: f holds the FE function's dof values); deriv >= 0:

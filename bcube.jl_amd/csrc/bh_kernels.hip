@@ -168,17 +168,21 @@ __global__ __launch_bounds__(kBlock) void k_element_rect_f64(
     const double* __restrict__ coords, const int32_t* __restrict__ c2n,
     const double* __restrict__ tab_u, const double* __restrict__ tab_v, int ncells,
     const double* __restrict__ eta, double eta_scale, const double* __restrict__ mcoef,
-    double mass_scale, double* __restrict__ ke) {
+    double mass_scale, bool adv, const double* __restrict__ acoef, double cx, double cy,
+    double cz, double* __restrict__ ke) {
+  // adv: add ∫ α (c⋅∇u) v, α = acoef[cell] (NULL: 1), c = (cx, cy, cz): per
+  // point the vector d = w|J| α J^-1 c, so that c⋅∇φ_j = d⋅∇̂φ_j
   using LU = TabLayout<D, NLU, NQ, NG>;
   using LV = TabLayout<D, NLV, NQ, NG>;
   constexpr int CPB = kBlock / NLU;
   constexpr int NS = SymN<D>::N;
   constexpr int KS = KeStride<NLV>::value;
 
+  constexpr int NF = NS + 1 + D;          // per point: G, m, d
   __shared__ double s_ru[NQ * NLU * 4];  // trial basis: ∇̂φ, φ per (q, j)
   __shared__ double s_tab[LV::G];        // weights, test basis, geometric gradients
   __shared__ double s_x[CPB][NG][D];
-  __shared__ double s_g[CPB][NQ][NS + 1];
+  __shared__ double s_g[CPB][NQ][NF];
 
   const int tid = threadIdx.x;
   for (int k = tid; k < NQ * NLU * 4; k += kBlock) s_ru[k] = tab_u[LU::R + k];
@@ -232,6 +236,15 @@ __global__ __launch_bounds__(kBlock) void k_element_rect_f64(
           s_g[c][q][s++] = ge * v;
         }
       s_g[c][q][NS] = MASS ? wdet * (mcoef ? mcoef[cell] * mass_scale : mass_scale) : 0.0;
+      const double cv[3] = {cx, cy, cz};
+      const double ae = adv ? wdet * (acoef ? acoef[cell] : 1.0) : 0.0;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        double v = 0.0;
+#pragma unroll
+        for (int t = 0; t < D; ++t) v += Ji[a][t] * cv[t];
+        s_g[c][q][NS + 1 + a] = ae * v;
+      }
     }
     __syncthreads();
     if (cs >= CPB || base + cs >= ncells) continue;
@@ -253,14 +266,19 @@ __global__ __launch_bounds__(kBlock) void k_element_rect_f64(
         t[1] = g[1] * rj[0] + g[3] * rj[1] + g[4] * rj[2];
         t[2] = g[2] * rj[0] + g[4] * rj[1] + g[5] * rj[2];
       }
-      const double mp = MASS ? g[NS] * rj[D] : 0.0;
+      double mp = MASS ? g[NS] * rj[D] : 0.0;
+      if (adv) {  // (c⋅∇φ_j) times ψ_i: folded into the value factor
+#pragma unroll
+        for (int a = 0; a < D; ++a) mp = fma(g[NS + 1 + a], rj[a], mp);
+      }
+      const bool val = MASS || adv;
       const double* rq = &s_tab[LV::R + q * NLV * 4];
 #pragma unroll
       for (int i = 0; i < NLV; ++i) {
         double v = acc[i];
 #pragma unroll
         for (int a = 0; a < D; ++a) v = fma(rq[i * 4 + a], t[a], v);
-        if (MASS) v = fma(rq[i * 4 + D], mp, v);
+        if (val) v = fma(rq[i * 4 + D], mp, v);
         acc[i] = v;
       }
     }
@@ -763,7 +781,8 @@ static hipError_t launch_rect_t(const LaunchArgs& A) {
 #define BH_R(M)                                                                                 \
   hipLaunchKernelGGL((k_element_rect_f64<D, NLU, NLV, NQ, NG, M>), grid, dim3(kBlock), 0,       \
                      A.stream, A.coords, A.c2n, A.tab_trial, A.tab, (int)A.ncells_total, A.eta, \
-                     A.eta_scale, A.mcoef, A.mass_scale, (double*)A.ke)
+                     A.eta_scale, A.mcoef, A.mass_scale, A.adv, A.acoef, A.avec[0], A.avec[1],  \
+                     A.avec[2], (double*)A.ke)
   if (A.mass) BH_R(true);
   else BH_R(false);
 #undef BH_R
@@ -784,8 +803,12 @@ struct RectEntry {
    &launch_rect_t<3, (PU + 1) * (PU + 1) * (PU + 1), (PV + 1) * (PV + 1) * (PV + 1),            \
                   NQ1 * NQ1 * NQ1, 8>}
 #define BH_RL(PU, PV, NQ1) {BH_BAR2, PU, PV, NQ1, &launch_rect_t<1, PU + 1, PV + 1, NQ1, 2>}
+// pu == pv: non-symmetric forms (advection) on one space
 static const RectEntry kRectTable[] = {
     BH_RL(1, 2, 2), BH_RL(2, 1, 2), BH_RL(1, 2, 3), BH_RL(2, 1, 3),
+    BH_RL(1, 1, 2), BH_RL(2, 2, 2), BH_RL(2, 2, 3),
+    BH_RQ(1, 1, 2), BH_RQ(2, 2, 3), BH_RT(1, 1, 3), BH_RT(2, 2, 6), BH_RH(1, 1, 2),
+    BH_RH(2, 2, 3),
     BH_RQ(1, 2, 2), BH_RQ(2, 1, 2), BH_RQ(1, 2, 3), BH_RQ(2, 1, 3),
     BH_RT(1, 2, 3), BH_RT(2, 1, 3), BH_RT(1, 2, 4), BH_RT(2, 1, 4), BH_RT(1, 2, 6), BH_RT(2, 1, 6),
     BH_RH(1, 2, 2), BH_RH(2, 1, 2), BH_RH(1, 2, 3), BH_RH(2, 1, 3),

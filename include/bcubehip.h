@@ -50,7 +50,15 @@ enum { BH_HOST = 0, BH_DEVICE = 1 };
  * `a(u,v)` to a list of these terms). */
 enum {
   BH_FORM_STIFFNESS = 0, /* ∫ η ∇u⋅∇v dΩ   (η: constant or per cell) */
-  BH_FORM_MASS = 1       /* ∫ c u v dΩ     (c: constant or per cell) */
+  BH_FORM_MASS = 1,      /* ∫ c u v dΩ     (c: constant or per cell) */
+  /* ∫ η ∂u/∂x_a v dΩ for a = 0, 1, 2 (FP64, uniform meshes): the components of
+   * ∫ η (c⋅∇u) v dΩ as one term per axis with scale c_a, all with the same η
+   * (e.g. ∫(∇(u)⋅v)dΩ on a line, test_assembler.jl:214-231).  Not symmetric:
+   * assembled by the element kernel of distinct trial and test spaces, also
+   * when U == V */
+  BH_FORM_ADVECTION_X = 2,
+  BH_FORM_ADVECTION_Y = 3,
+  BH_FORM_ADVECTION_Z = 4
 };
 /* Linear integrands (assembler.jl:448 lowers `l(v)`). */
 enum {

@@ -136,3 +136,16 @@ def test_line_fe_function_linear_form():
     b = B.assemble_linear(lambda v: B.integrate(B.FEFunction(U, vals) * v, dΩ), V)
     M = B.assemble_bilinear(lambda u, v: B.integrate(u * v, dΩ), U, V)
     assert np.abs(b - M @ vals).max() <= 1e-14 * np.abs(b).max()
+
+
+def test_bilinear_rectangular_grad_p1_p2():
+    """test_assembler.jl:214-231: ∫(∇(u)⋅v) with U = P1, V = P2 on one line cell."""
+    m = B.one_cell_mesh(":line")
+    U = B.TrialFESpace(LAG(1), m)
+    V = B.TestFESpace(B.TrialFESpace(LAG(2), m))
+    dΩ = B.Measure(B.CellDomain(m), 2)
+    A = B.assemble_bilinear(lambda u, v: B.integrate(B.dot(B.grad(u), v), dΩ), U, V).toarray()
+    print(A, flush=True)
+    assert np.allclose(A[0], [-1 / 6, 1 / 6], rtol=0, atol=1e-15)
+    assert np.allclose(A[1], [-2 / 3, 2 / 3], rtol=0, atol=1e-15)
+    assert np.allclose(A[2], [-1 / 6, 1 / 6], rtol=0, atol=1e-15)

@@ -134,3 +134,38 @@ def test_transport_linear_form_vs_oracle(kind, p, cont):
     assert err <= RTOL
     with pytest.raises(BcubeHipError, match="FP64"):
         B.assemble_linear(lambda v: B.integrate(B.dot(c * u, B.grad(v)), dΩ), V, mode="fp32")
+
+
+@pytest.mark.parametrize("kind,pu,pv,deg", [("Quad4", 1, 1, 2), ("Quad4", 2, 2, 4), ("Quad4", 1, 2, 4),
+                                            ("Tri3", 2, 2, 4), ("Tri3", 2, 1, 4), ("Hexa8", 1, 1, 2),
+                                            ("Hexa8", 2, 2, 4)])
+def test_advection_matrix_vs_oracle(kind, pu, pv, deg):
+    """∫ α (c⋅∇u) v (not symmetric; U == V included): for w in V's space,
+    Aᵀ w is the oracle's ∫ w_h (α c)⋅∇λ_i over U's basis (test_assembler.jl:40-68's
+    form), with α per cell; A·1 = 0 (constants have no gradient)."""
+    m = _mesh(kind, 3, seed=11)
+    U = B.TrialFESpace(B.FunctionSpace("Lagrange", pu), m)
+    V = U if pu == pv else B.TrialFESpace(B.FunctionSpace("Lagrange", pv), m)
+    c = np.array([1.0, -0.5, 0.25])[:m.dim]
+    dΩ = B.Measure(B.CellDomain(m), deg)
+    A = B.assemble_bilinear(lambda u, v: B.integrate(B.dot(c, B.grad(u)) * v, dΩ), U,
+                            B.TestFESpace(V))
+    assert A.shape == (V.get_ndofs(), U.get_ndofs())
+    w = np.random.default_rng(12).normal(size=V.get_ndofs())
+    ref = O.assemble_linear_fe(_om(m), pv, w, pu, "cu_grad", c=c, degree=deg)
+    err = np.abs(A.T @ w - ref).max() / np.abs(ref).max()
+    one = np.abs(A @ np.ones(U.get_ndofs())).max() / abs(A).max()
+    print(f"{kind} U=P{pu} V=P{pv}: advection Aᵀw rel err {err:.2e}, A·1 {one:.1e}", flush=True)
+    assert err <= 1e-12 and one <= 1e-13
+    # a per-cell coefficient scales each cell's contribution
+    alpha = np.random.default_rng(13).uniform(0.5, 2.0, m.ncells)
+    ac = B.MeshCellData(alpha)
+    A2 = B.assemble_bilinear(lambda u, v: B.integrate(ac * B.dot(c, B.grad(u)) * v, dΩ), U,
+                             B.TestFESpace(V))
+    assert np.abs(A2 @ np.ones(U.get_ndofs())).max() <= 1e-13 * abs(A2).max()
+    if pu == pv:  # U == V: repeatable, and the plan refuses FP32
+        plan = B.BilinearPlan(lambda u, v: B.integrate(B.dot(c, B.grad(u)) * v, dΩ), U,
+                              B.TestFESpace(V))
+        assert np.array_equal(plan.assemble_host(), A.data)
+        with pytest.raises(BcubeHipError, match="FP64"):
+            plan.assemble_host(mode="fp32")

@@ -56,3 +56,21 @@ def test_scalar_dot_is_mass():
     ex = B.integrate(B.dot(B.grad(Placeholder("trial", U)), Placeholder("test", U)), dΩ)
     with pytest.raises(NotImplementedError):
         lower_bilinear(_integrals(ex)[0].expr)
+
+
+def test_advection_lowers_per_axis():
+    from bcubehip.forms import lower_bilinear_terms
+    m, U = _space(p=1)
+    dΩ = B.Measure(B.CellDomain(m), 2)
+    c = np.array([0.5, -2.0])
+    u, v = Placeholder("trial", U), Placeholder("test", U)
+    for ex in (B.integrate(B.dot(c, B.grad(u)) * v, dΩ),
+               B.integrate(v * B.dot(B.grad(u), c), dΩ)):
+        t = lower_bilinear_terms(_integrals(ex)[0].expr, 2)
+        assert [x[0] for x in t] == [("adv", 0), ("adv", 1)]
+        assert [x[3] for x in t] == [0.5, -2.0]
+    # on a line, ∇(u)⋅v is ∂u/∂x v (test_assembler.jl:214-231)
+    ex = B.integrate(B.dot(B.grad(u), v), dΩ)
+    assert lower_bilinear_terms(_integrals(ex)[0].expr, 1) == [(("adv", 0), "const", None, 1.0)]
+    with pytest.raises(NotImplementedError):
+        lower_bilinear_terms(_integrals(ex)[0].expr, 2)

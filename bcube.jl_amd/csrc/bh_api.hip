@@ -101,6 +101,7 @@ struct bh_space_s {
   // FP32 linear forms: element vectors (ncells * nl floats) and, per dof, its
   // element-vector entries cell*nl + i in increasing cell order
   float* d_be = nullptr;
+  double* d_be64 = nullptr;  // FP64 element vectors (ncells * nl)
   int32_t* d_vptr = nullptr;
   int32_t* d_vpair = nullptr;
   // spaces of per-cell order (bh_space_create_orders): each cell's order, the
@@ -982,6 +983,7 @@ int bh_space_destroy(bh_space s) {
     (void)hipFree(s->d_b);
     (void)hipFree(s->d_coef);
     (void)hipFree(s->d_be);
+    (void)hipFree(s->d_be64);
     (void)hipFree(s->d_vptr);
     (void)hipFree(s->d_vpair);
     for (GeoBucket& b : s->obuckets) {
@@ -3127,7 +3129,20 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
       return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
     if (int r = build_vector_gather(c, V)) return r;
   }
-  if (mode == BH_FP64 && V->col.ncolors == 0) {
+  // FP64: element vectors of every cell (one launch per cell type), then one
+  // sum per dof in increasing cell order, the reference's order
+  // (assembler.jl:855-865); colours of cells that share no dof only when the
+  // per-dof lists do not fit 32-bit ids
+  if (mode == BH_FP64 && !V->d_vptr && !c->capturing &&
+      m->h.ncells * (int64_t)V->nl < (int64_t)INT32_MAX)
+    if (int r = build_vector_gather(c, V)) return r;
+  if (mode == BH_FP64 && V->d_vptr && !V->d_be64) {
+    if (c->capturing)
+      return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
+    BH_HIP(c, dalloc(&V->d_be64, (size_t)(m->h.ncells * V->nl)));
+  }
+  const bool evec64 = mode == BH_FP64 && V->d_vptr && V->d_be64;
+  if (mode == BH_FP64 && !evec64 && V->col.ncolors == 0) {
     if (c->capturing)
       return fail(c, BH_ERR_ARG, "run this assembly once before capturing it in a graph");
     if (int r = build_colouring(c, V->dofs, V->nl, V->ndofs, m->h.ncells, V->col,
@@ -3188,6 +3203,25 @@ int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_de
       }
       time_begin(c);
       BH_HIP(c, launch_vgather(c->stream, V->d_vptr, V->d_vpair, V->d_be, V->ndofs, db));
+      time_end(c);
+      continue;
+    }
+    if (evec64) {
+      A.ke = V->d_be64;
+      A.evec = true;
+      for (size_t bk = 0; bk < nb; ++bk) {
+        const GeoBucket& gb = m->buckets[bk];
+        A.c2n = gb.d_c2n;
+        A.cells = gb.d_cells;  // NULL on a uniform mesh
+        A.lcells = nullptr;
+        A.ncells_color = (int)gb.n;
+        A.tab = btab[bk];
+        time_begin(c);
+        BH_HIP(c, launch(gb.geo, V->order, bkey[bk], A));
+        time_end(c);
+      }
+      time_begin(c);
+      BH_HIP(c, launch_vgather64(c->stream, V->d_vptr, V->d_vpair, V->d_be64, V->ndofs, db));
       time_end(c);
       continue;
     }

@@ -66,8 +66,11 @@ struct LaunchArgs {
   const int64_t* rowptr = nullptr;
   int offset_bytes = 1;
   // source (FE_EVAL: f holds the FE function's dof values); deriv >= 0:
-  // ∫ f ∂v/∂x_deriv instead of ∫ f v
+  // ∫ f ∂v/∂x_deriv instead of ∫ f v; evec: element vectors of the launch's
+  // cells (k-th cell: c2n row k, global id cells[k] or k) to ke (FP64,
+  // cell*erow + i) instead of adding into vals by colour
   int deriv = -1;
+  bool evec = false;
   int fkind = BH_COEF_CONST;
   const double* f = nullptr;
   double fscale = 0.0;
@@ -216,6 +219,9 @@ hipError_t launch_gp_orders(GPOrdersArgs a, bool mass, hipStream_t stream);
 // in cell order.
 bool source_gp_supported(int geo_celltype, int order, int nq1);
 hipError_t launch_source_gp(int geo_celltype, int order, int nq1, const LaunchArgs& a);
+// The same sum over FP64 element vectors (the FP64 linear forms).
+hipError_t launch_vgather64(hipStream_t stream, const int32_t* vptr, const int32_t* vpair,
+                            const double* be, int64_t ndofs, double* b);
 hipError_t launch_vgather(hipStream_t stream, const int32_t* vptr, const int32_t* vpair,
                           const float* be, int64_t ndofs, double* b);
 // Waves-per-block targets of the GEMM element kernel that give a different

@@ -468,10 +468,13 @@ __global__ __launch_bounds__(kBlock) void k_source_f64(
     const int32_t* __restrict__ rdofs, const double* __restrict__ tab,
     const int32_t* __restrict__ cells, const int32_t* __restrict__ lcells, int ncells_color,
     int rstride, int fstride, int fkind, const double* __restrict__ f, double fscale, int deriv,
-    double* __restrict__ b) {
+    double* __restrict__ be, double* __restrict__ b) {
   // cells: global ids (dof rows rdofs[cell*rstride], coefficients f[cell] or
   // f[cell*fstride + q]); lcells: the same cells' rows in c2n (a mixed mesh's
   // per-type connectivity), or NULL when c2n is indexed by global id.
+  // be != NULL (element vectors): the k-th cell of the launch has c2n row k
+  // and global id cells[k] (cells NULL: k); its vector goes to
+  // be[cell*rstride + i] for the per-dof gather instead of into b.
   using L = TabLayout<D, NL, NQ, NG>;
   constexpr int CPB = kBlock / NL;
   __shared__ double s_tab[L::SIZE];
@@ -487,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void k_source_f64(
     for (int k = tid; k < CPB * NG; k += kBlock) {
       int c = k / NG, n = k - c * NG;
       if (base + c < ncells_color) {
-        const int32_t crow = lcells ? lcells[base + c] : cells[base + c];
+        const int32_t crow = be ? base + c : (lcells ? lcells[base + c] : cells[base + c]);
         int node = c2n[(int64_t)crow * NG + n];
         for (int a = 0; a < D; ++a) s_x[c][n][a] = coords[(int64_t)node * D + a];
       }
@@ -496,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_source_f64(
     for (int k = tid; k < CPB * NQ; k += kBlock) {
       int c = k / NQ, q = k - c * NQ;
       if (base + c >= ncells_color) continue;
-      int cell = cells[base + c];
+      const int cell = (be && !cells) ? base + c : cells[base + c];
       double J[D][D];
       for (int a = 0; a < D; ++a)
         for (int bb = 0; bb < D; ++bb) J[a][bb] = 0.0;
@@ -529,8 +532,9 @@ __global__ __launch_bounds__(kBlock) void k_source_f64(
         for (int a = 0; a < D; ++a)
           acc = fma(s_m[cs][q][a], s_tab[L::R + (q * NL + li) * 4 + a], acc);
     }
-    const int cell = cells[base + cs];
-    b[rdofs[(int64_t)cell * rstride + li]] += acc;
+    const int cell = (be && !cells) ? base + cs : cells[base + cs];
+    if (be) be[(int64_t)cell * rstride + li] = acc;
+    else b[rdofs[(int64_t)cell * rstride + li]] += acc;
   }
 }
 
@@ -697,7 +701,8 @@ static hipError_t launch_all(const LaunchArgs& A) {
     hipLaunchKernelGGL((k_source_f64<D, NL, NQ, NG>), dim3(grid_for(ng)), dim3(kBlock), 0,
                        A.stream, A.coords, A.c2n, A.rdofs, A.tab, A.cells, A.lcells,
                        A.ncells_color, A.erow > 0 ? A.erow : NL, A.fstride > 0 ? A.fstride : NQ,
-                       A.fkind, A.f, A.fscale, A.deriv, A.vals);
+                       A.fkind, A.f, A.fscale, A.deriv, A.evec ? (double*)A.ke : nullptr,
+                       A.vals);
   } else if (A.kind == LaunchArgs::FE_EVAL) {
     const int qs = A.fstride > 0 ? A.fstride : NQ;
     int64_t n = A.ncells_total * qs;

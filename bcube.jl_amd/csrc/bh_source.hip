@@ -121,9 +121,10 @@ __global__ __launch_bounds__(64) void k_source_gp(
 
 // b[r] += Σ be[k] over the element-vector entries k = cell*erow + i of dof r,
 // in increasing cell order (vptr/vpair), summed in FP64.
+template <typename T>
 __global__ __launch_bounds__(256) void k_vgather(const int32_t* __restrict__ vptr,
                                                  const int32_t* __restrict__ vpair,
-                                                 const float* __restrict__ be, int64_t ndofs,
+                                                 const T* __restrict__ be, int64_t ndofs,
                                                  double* __restrict__ b) {
   for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < ndofs;
        r += (int64_t)gridDim.x * 256) {
@@ -191,8 +192,17 @@ hipError_t launch_vgather(hipStream_t stream, const int32_t* vptr, const int32_t
                           const float* be, int64_t ndofs, double* b) {
   if (ndofs <= 0) return hipSuccess;
   const int64_t nb = (ndofs + 255) / 256;
-  hipLaunchKernelGGL(k_vgather, dim3(nb < 16384 ? (unsigned)nb : 16384u), dim3(256), 0, stream,
-                     vptr, vpair, be, ndofs, b);
+  hipLaunchKernelGGL(k_vgather<float>, dim3(nb < 16384 ? (unsigned)nb : 16384u), dim3(256), 0,
+                     stream, vptr, vpair, be, ndofs, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_vgather64(hipStream_t stream, const int32_t* vptr, const int32_t* vpair,
+                            const double* be, int64_t ndofs, double* b) {
+  if (ndofs <= 0) return hipSuccess;
+  const int64_t nb = (ndofs + 255) / 256;
+  hipLaunchKernelGGL(k_vgather<double>, dim3(nb < 16384 ? (unsigned)nb : 16384u), dim3(256), 0,
+                     stream, vptr, vpair, be, ndofs, b);
   return hipGetLastError();
 }
 

@@ -567,7 +567,8 @@ def bench_c1(args, torch, B):
          "frac": ga_ach / PEAK_HBM_GBS, "kernel": "k_gather_padded (row gather)",
          "avg_launch_ms": k_ms[1], "alg_bytes_per_launch": 8.0 * plan.nnz},
         {"bound": "hbm", "achieved": src_ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-         "frac": src_ach / PEAK_HBM_GBS, "kernel": "vector fill + coloured k_source_f64",
+         "frac": src_ach / PEAK_HBM_GBS,
+         "kernel": "right-hand side: k_source_f64 element vectors + k_vgather per-dof sums",
          "avg_launch_ms": k_ms[2], "alg_bytes_per_launch": src_bytes}]
     roof = dict(max(kernels, key=lambda k: k["avg_launch_ms"]))
     roof["traffic"] = traffic_of("c1_64", roof["kernel"])

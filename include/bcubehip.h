@@ -282,11 +282,12 @@ int bh_assemble_bilinear_noalloc(bh_pattern p, int nterms, const bh_term* terms,
 /* assemble_linear!(b, l, V) (assembler.jl:465-473): b += l(v) for every test
  * function, like __update_b! (assembler.jl:855-865); b has ndofs entries.
  * assemble_linear(l, V) (assembler.jl:448-458) is this call on a zeroed b.
- * mode BH_FP64: element vectors in FP64 added into b one colour of cells at a
- * time; BH_FP32: element vectors Φᵀ(w|J|f) on the matrix cores (geometry and
- * coefficients in FP64, rounded once), added into b in FP64 per dof in
- * increasing cell order (the reference's order).  Repeated calls give
- * bitwise identical values in either mode. */
+ * mode BH_FP64: element vectors in FP64; BH_FP32: element vectors Φᵀ(w|J|f)
+ * on the matrix cores (geometry and coefficients in FP64, rounded once).  In
+ * both, the element vectors are added into b in FP64 per dof in increasing
+ * cell order (the reference's order; FP64 falls back to one colour of cells
+ * at a time past 2^31 element-vector entries).  Repeated calls give bitwise
+ * identical values in either mode. */
 int bh_assemble_linear(bh_space V, int nterms, const bh_term* terms, int quad_degree, int mode,
                        double* b, int where);
 /* Physical coordinates x = F(ξ_q) of every quadrature point

@@ -133,22 +133,26 @@ class MultiBilinearPlan:
         mesh = fu.scalar.mesh
         if self.measure.domain.mesh is not mesh:
             raise ValueError("the measure's mesh differs from the space's mesh")
-        by_pair = lower_multi_bilinear(items, sizes)
+        by_pair = lower_multi_bilinear(items, sizes, mesh.dim)
         self._keep, self.blocks = [], []
         pairs, blk = [], []
         seen = {}  # identical scalar forms share one block
-        for (i, j), lowered in by_pair.items():
-            key = tuple((f, k, id(d) if d is not None else None, sc) for f, k, d, sc in lowered)
-            if key not in seen:
-                keep, terms, n = _bilinear_bh_terms(lowered, mesh)
-                self._keep.append((keep, lowered))
-                seen[key] = len(self.blocks)
-                self.blocks.append((terms, n))
-            b = seen[key]
+        for (i, j), comp in by_pair.items():
             for ki in range(sizes[i]):
                 for kj in range(sizes[j]):
+                    lowered = comp.get((ki, kj))
+                    b = -1  # a component pair the form does not couple: explicit zeros
+                    if lowered is not None:
+                        key = tuple((f, k, id(d) if d is not None else None, sc)
+                                    for f, k, d, sc in lowered)
+                        if key not in seen:
+                            keep, terms, n = _bilinear_bh_terms(lowered, mesh)
+                            self._keep.append((keep, lowered))
+                            seen[key] = len(self.blocks)
+                            self.blocks.append((terms, n))
+                        b = seen[key]
                     pairs += [fu.field(i, ki), fu.field(j, kj)]
-                    blk.append(b if (sizes[i] == 1 or ki == kj) else -1)
+                    blk.append(b)
         if len(self.blocks) > 32:
             raise NotImplementedError("more than 32 coupled forms")
         self.block_of_pair = np.ascontiguousarray(blk, dtype=np.int8)

@@ -551,14 +551,36 @@ def _to_scalar_form(e, vector: bool):
     return e
 
 
-def lower_multi_bilinear(items, sizes):
+def _grad_scalar_dot_vector(term, ncomp, dim):
+    """α ∇(p)⋅v with p a scalar trial field and v a vector test field of dim
+    components: per component a the scalar advection term α ∂p/∂x_a v_a."""
+    fs = _factors(term, [])
+    dots = [f for f in fs if isinstance(f, Dot)]
+    if len(dots) == 1 and ncomp == dim:
+        d = dots[0]
+        rest = [f for f in fs if f is not d]
+        for g, v in ((d.a, d.b), (d.b, d.a)):
+            if (isinstance(g, Grad) and _is(g.a, Placeholder, "trial") and _is(v, Placeholder, "test")
+                    and not any(isinstance(f, (Dot, Grad, Add, Placeholder, VecConst)) for f in rest)):
+                kind, data, scale = _coef(rest)
+                if kind in ("qp", "fe"):
+                    raise NotImplementedError("coefficient must be constant or per cell")
+                return [[(("adv", a), kind, data, scale)] for a in range(ncomp)]
+    raise NotImplementedError("the only term coupling a scalar trial and a vector test space "
+                              "lowered is ∇(p)⋅v")
+
+
+def lower_multi_bilinear(items, sizes, dim=None):
     """Lower a bilinear form over several fields (assembler.jl:327-420).
 
-    sizes[i]: components of space i.  Returns {(test space i, trial space j):
-    [(form, kind, data, scale), ...]} for the pairs the form couples; a pair of
-    vector spaces gets the scalar form of each component (component-diagonal,
-    the only vector integrands lowered: c u⋅v and η ∇u ⊡ ∇v)."""
-    out: dict[tuple[int, int], list] = {}
+    sizes[i]: components of space i, dim: the mesh dimension.  Returns
+    {(test space i, trial space j): {(component ki, component kj): [(form,
+    kind, data, scale), ...]}} for the pairs the form couples.  A pair of vector
+    spaces gets the scalar form on each diagonal component pair (c u⋅v and
+    η ∇u ⊡ ∇v); ∇(p)⋅v couples a scalar trial field p to each component a of a
+    vector test field v by the advection term ∂p/∂x_a v_a.  Component pairs
+    absent from the dict hold explicit zeros in the pattern."""
+    out: dict[tuple[int, int], dict] = {}
     for it in items:
         for term in _split_sum(it.expr):
             ph = _placeholders(term, [])
@@ -567,10 +589,16 @@ def lower_multi_bilinear(items, sizes):
             if len(us) != 1 or len(vs) != 1:
                 raise NotImplementedError("each term must hold one trial and one test function")
             i, j = vs[0].field or 0, us[0].field or 0
+            comp = out.setdefault((i, j), {})
+            if sizes[i] > 1 and sizes[j] == 1:
+                for a, forms in enumerate(_grad_scalar_dot_vector(term, sizes[i], dim)):
+                    comp.setdefault((a, 0), []).extend(forms)
+                continue
             if (sizes[i] > 1) != (sizes[j] > 1) or (sizes[i] > 1 and sizes[i] != sizes[j]):
                 raise NotImplementedError("terms coupling a scalar and a vector space")
-            out.setdefault((i, j), []).append(
-                lower_bilinear(_to_scalar_form(term, sizes[i] > 1)))
+            lowered = lower_bilinear(_to_scalar_form(term, sizes[i] > 1))
+            for k in range(sizes[i]):
+                comp.setdefault((k, k), []).append(lowered)
     return out
 
 

@@ -812,8 +812,9 @@ struct RectEntry {
 static const RectEntry kRectTable[] = {
     BH_RL(1, 2, 2), BH_RL(2, 1, 2), BH_RL(1, 2, 3), BH_RL(2, 1, 3),
     BH_RL(1, 1, 2), BH_RL(2, 2, 2), BH_RL(2, 2, 3),
-    BH_RQ(1, 1, 2), BH_RQ(2, 2, 3), BH_RT(1, 1, 3), BH_RT(2, 2, 6), BH_RH(1, 1, 2),
-    BH_RH(2, 2, 3),
+    BH_RQ(1, 1, 2), BH_RQ(1, 1, 3), BH_RQ(2, 2, 3), BH_RT(1, 1, 3), BH_RT(1, 1, 4),
+    BH_RT(1, 1, 6), BH_RT(2, 2, 4), BH_RT(2, 2, 6), BH_RT(2, 2, 7), BH_RH(1, 1, 2),
+    BH_RH(1, 1, 3), BH_RH(2, 2, 3),
     BH_RQ(1, 2, 2), BH_RQ(2, 1, 2), BH_RQ(1, 2, 3), BH_RQ(2, 1, 3),
     BH_RT(1, 2, 3), BH_RT(2, 1, 3), BH_RT(1, 2, 4), BH_RT(2, 1, 4), BH_RT(1, 2, 6), BH_RT(2, 1, 6),
     BH_RH(1, 2, 2), BH_RH(2, 1, 2), BH_RH(1, 2, 3), BH_RH(2, 1, 3),

@@ -187,3 +187,91 @@ def test_reference_basic_bilinear_benchmark(form):
     err = np.abs(A.data - nzval).max() / np.abs(nzval).max()
     print(f"basic bilinear {form}: nnz {A.nnz}, rel err {err:.2e}")
     assert err <= 1e-12
+
+
+def test_linear_multi_vector_fe_known_answer():
+    """linear-multi-vector-fe (test_assembler.jl:180-212): on one_cell_mesh(:tri),
+    ∫(u_xy⋅v_xy + u_p⋅v_z + ∇(u_p)⋅v_xy) over MultiFESpace(U_p, U_xy, U_z),
+    A[12, 1] ≈ 1/6, A[12, 2] ≈ 1/6, A[12, 3] ≈ 1/3."""
+    m = B.one_cell_mesh(":tri")
+    U_xy = B.TrialFESpace(LAG(1), m, size=2)
+    U_z = B.TrialFESpace(LAG(1), m)
+    U_p = B.TrialFESpace(LAG(1), m)
+    U = B.MultiFESpace(U_p, U_xy, U_z)
+    V = B.MultiFESpace(B.TestFESpace(U_p), B.TestFESpace(U_xy), B.TestFESpace(U_z))
+    dΩ = B.Measure(B.CellDomain(m), 3)
+
+    def a1(u, v):
+        (u_p, u_xy, u_z), (v_p, v_xy, v_z) = u, v
+        return B.integrate(B.dot(u_xy, v_xy) + B.dot(u_p, v_z) + B.dot(B.grad(u_p), v_xy), dΩ)
+    A = B.assemble_bilinear(a1, U, V).toarray()
+    print(A[11, :3], flush=True)
+    assert A.shape == (12, 12)
+    assert np.allclose(A[11, :3], [1 / 6, 1 / 6, 1 / 3], rtol=1e-14, atol=0)
+    # the ∇(u_p)⋅v_xy block: ∂φ_j/∂x_a ∫ψ_i = ∂φ_j/∂x_a |T|/3 on P1 triangles
+    xy = U.get_mapping(1) - 1
+    cm = U_xy.component_map
+    grad = np.array([[-1.0, -1.0], [1.0, 0.0], [0.0, 1.0]]) / 2.0  # ∇φ_j on (-1,-1),(1,-1),(-1,1)
+    for a in range(2):
+        want = np.tile(grad[:, a], (3, 1)) * (2.0 / 3.0)
+        assert np.allclose(A[np.ix_(xy[cm[a]], [0, 1, 2])], want, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("kind,aos", [("quad", True), ("hexa", False)])
+def test_grad_scalar_dot_vector_vs_oracle(kind, aos):
+    """∫(u⋅v + α ∇(p)⋅v + p q) over MultiFESpace(vector u, scalar p): the
+    global pattern bit-exact with the oracle's block assembly, the diagonal
+    blocks within 1e-12, no values in the (q, u) blocks, coupling rows that sum
+    to zero (constants have no gradient); with a constant α the coupling block
+    of component a, transposed onto w, equals the oracle's ∫ w_h α e_a⋅∇λ_j
+    within 1e-12; FP32 is refused (advection terms are FP64 only)."""
+    if kind == "quad":
+        m = B.rectangle_mesh(7, 6)
+        B.jitter_nodes(m, 0.2, 1 / 6, seed=4)
+        p = 2
+    else:
+        m = B.hexa_mesh(4, 4, 3)
+        B.jitter_nodes(m, 0.15, 0.25, seed=5)
+        p = 1
+    d = m.dim
+    deg = 2 * p
+    Uv = B.TrialFESpace(LAG(p), m, size=d)
+    Up = B.TrialFESpace(LAG(p), m)
+    U = B.MultiFESpace(Uv, Up, arrayOfStruct=aos)
+    V = B.MultiFESpace(B.TestFESpace(Uv), B.TestFESpace(Up), arrayOfStruct=aos)
+    dΩ = B.Measure(B.CellDomain(m), deg)
+    ac = B.MeshCellData(np.random.default_rng(6).uniform(0.5, 2.0, m.ncells))
+
+    def a(u, v):
+        return B.integrate(B.dot(u[0], v[0]) + ac * B.dot(B.grad(u[1]), v[0]) + u[1] * v[1], dΩ)
+    plan = B.MultiBilinearPlan(a, U, V)
+    colptr, rowval, nzval, n = O.assemble_blocks(
+        _om(m), p, [d, 1], {(0, 0): (["mass"], None), (0, 1): (["mass"], None),
+                            (1, 1): (["mass"], None)}, aos=aos, degree=deg)
+    c, r = plan.csc_structure()
+    assert plan.nrows == n
+    assert np.array_equal(c + 1, colptr) and np.array_equal(r + 1, rowval)
+    A = plan.to_csc(plan.assemble_host()).toarray()
+    R = np.zeros((n, n))
+    R[rowval - 1, np.repeat(np.arange(n), np.diff(colptr))] = nzval
+    mv, mp = U.get_mapping(0) - 1, U.get_mapping(1) - 1
+    cm = Uv.component_map
+    vrows = np.concatenate([mv[cm[k]] for k in range(d)])
+    for blk in (np.ix_(vrows, vrows), np.ix_(mp, mp)):
+        assert np.abs(A[blk] - R[blk]).max() <= 1e-12 * np.abs(R).max()
+    assert np.abs(A[np.ix_(mp, vrows)]).max() == 0.0
+    Cp = A[np.ix_(vrows, mp)]
+    assert np.abs(Cp).max() > 0 and np.abs(Cp.sum(axis=1)).max() <= 1e-13 * np.abs(Cp).max()
+    with pytest.raises(B.BcubeHipError, match="FP64"):
+        plan.assemble_host(mode="fp32")
+    alpha = 1.5
+    plan1 = B.MultiBilinearPlan(
+        lambda u, v: B.integrate(alpha * B.dot(B.grad(u[1]), v[0]), dΩ), U, V)
+    A1 = plan1.to_csc(plan1.assemble_host()).toarray()
+    om = _om(m)
+    for k in range(d):
+        w = np.random.default_rng(20 + k).normal(size=Up.get_ndofs())
+        want = O.assemble_linear_fe(om, p, w, p, "cu_grad", c=alpha * np.eye(d)[k], degree=deg)
+        err = np.abs(A1[np.ix_(mv[cm[k]], mp)].T @ w - want).max() / np.abs(want).max()
+        print(f"{kind} P{p} aos={aos}: ∇(p)⋅v component {k} rel err {err:.2e}", flush=True)
+        assert err <= 1e-12

@@ -167,8 +167,8 @@ def test_unsupported_couplings_raise():
     U = B.MultiFESpace(*sp)
     V = B.MultiFESpace(*[B.TestFESpace(s) for s in sp])
     dΩ = B.Measure(B.CellDomain(m), 2)
-    with pytest.raises(NotImplementedError):  # ∇(u_p)⋅v_xy: a scalar and a vector space
-        B.MultiBilinearPlan(lambda u, v: B.integrate(B.dot(B.grad(u[0]), v[1]), dΩ), U, V)
+    with pytest.raises(NotImplementedError):  # u_xy⋅∇(v_p): a vector trial, a scalar test space
+        B.MultiBilinearPlan(lambda u, v: B.integrate(B.dot(u[1], B.grad(v[0])), dΩ), U, V)
     W = B.MultiFESpace(B.TrialFESpace(LAG(1), m), B.TrialFESpace(LAG(2), m))
     with pytest.raises(NotImplementedError):  # two orders: rectangular blocks
         B.MultiBilinearPlan(lambda u, v: B.integrate(u[0] * v[0], dΩ), W, W)
@@ -303,3 +303,43 @@ def _many_fields(m, nf):
         blocks[(i, i)] = (["mass"], None)
         blocks[(i, i - 1)] = (["mass"], None)
     return U, V, a, blocks
+
+
+def test_grad_scalar_dot_vector_lowering():
+    """∇(p)⋅v over (v vector of the mesh's dimension, p scalar) lowers to one
+    advection term per component of v (the form of test_assembler.jl:201-203);
+    the terms a scalar test space would need from a vector trial space (q ∇⋅u)
+    are refused."""
+    from bcubehip.assembler import Fields, _integrals
+    from bcubehip.forms import lower_multi_bilinear
+    m = B.one_cell_mesh(":tri", backend=HOST)
+    U_xy = B.TrialFESpace(LAG(1), m, size=2)
+    U_z, U_p = B.TrialFESpace(LAG(1), m), B.TrialFESpace(LAG(1), m)
+    U = B.MultiFESpace(U_p, U_xy, U_z)
+    V = B.MultiFESpace(B.TestFESpace(U_p), B.TestFESpace(U_xy), B.TestFESpace(U_z))
+    dΩ = B.Measure(B.CellDomain(m), 3)
+    fu = Fields(U)
+
+    def lower(a):
+        items = _integrals(a(fu.placeholders(U, "trial"), fu.placeholders(V, "test")))
+        return lower_multi_bilinear(items, fu.sizes, m.dim)
+    out = lower(lambda u, v: B.integrate(B.dot(u[1], v[1]) + B.dot(u[0], v[2])
+                                         + 2.0 * B.dot(B.grad(u[0]), v[1]), dΩ))
+    assert out[(1, 1)] == {(0, 0): [("mass", "const", None, 1.0)],
+                           (1, 1): [("mass", "const", None, 1.0)]}
+    assert out[(2, 0)] == {(0, 0): [("mass", "const", None, 1.0)]}
+    assert out[(1, 0)] == {(0, 0): [(("adv", 0), "const", None, 2.0)],
+                           (1, 0): [(("adv", 1), "const", None, 2.0)]}
+    with pytest.raises(NotImplementedError):
+        lower(lambda u, v: B.integrate(B.dot(u[1], B.grad(v[0])), dΩ))
+    with pytest.raises(NotImplementedError):
+        lower(lambda u, v: B.integrate(u[0] * v[1], dΩ))
+    # the global pattern of the coupled system: every (component of v, p) pair
+    # of each cell, bit-exact with the oracle's block assembly
+    plan = B.MultiBilinearPlan(lambda u, v: B.integrate(B.dot(u[1], v[1]) + B.dot(u[0], v[2])
+                                                        + B.dot(B.grad(u[0]), v[1]), dΩ), U, V)
+    ocp, orv, _, n = O.assemble_blocks(_om(m), 1, [1, 2, 1], {
+        (1, 1): (["mass"], None), (2, 0): (["mass"], None), (1, 0): (["mass"], None)}, degree=3)
+    cp, rv = plan.csc_structure()
+    assert plan.nrows == n == 12
+    assert np.array_equal(cp + 1, ocp) and np.array_equal(rv + 1, orv)
